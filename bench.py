@@ -1,0 +1,168 @@
+"""Benchmark of the column biogeophysics soil step on one MI355X.
+
+A "step" is one time step (dtime = 1800 s) of the soil physics over every
+column of the batch: SoilTemperature (snow + soil heat solve with phase
+change), SetSoilWaterFractions, and the moisture-form Richards solve, in ONE
+fused launch (the library's ctsm_soil_step_fused_dev).  The workload is the
+uneven-column batch: 65,536 columns, snow layers uniform 0..12, bedrock index
+uniform 5..20, one patch per column, FP64, synthetic seeded inputs resident
+in HBM.  The same steps are also timed in the plain form (one launch per
+reference routine, intermediates through HBM) for the fused/plain ratio.
+
+Prints ONE JSON line (rank 0) with the metric, the roofline of the fused
+kernel (algorithmic bytes per launch / measured launch time vs 8 TB/s) and a
+CPU baseline (the C oracle, single thread, on a bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(st):
+    """Compulsory HBM bytes of one fused soil step, summed over the columns.
+
+    Every array element the step must read is counted once and every element
+    it must write once (FP64 = 8 B, int32 = 4 B), over active extents only:
+      per column   : 14 FP64 + 4 int32 scalars read, 18 FP64 scalars written;
+      snow layer   : 6 FP64 read (t, liq, ice, dz, z, zi), 7 FP64 + 1 int32 written
+                     (t, liq, ice, thk, fact, bw + imelt) ;
+      all nlevsno snow slots: qflx_snomelt_lyr, qflx_snofrz_lyr written (2 FP64);
+      ground level : 12 FP64 read (t, liq, ice, dz, z, zi, watsat, tkmg, tkdry,
+                     csol, bsw, sucsat) + zi(0), 6 FP64 + 1 int32 written
+                     (t, liq, ice, thk, fact, eflx_fgr + imelt);
+      nlevsoi level: eff_porosity, icefrac written (2 FP64);
+      Richards layer (nbedrock): hksat, qflx_rootsoi read; smp_l, hk_l, qin,
+                     qout written (2 + 4 FP64);
+      patch        : 15 FP64 + 1 int32 read, sabg_lyr over snl+1..1 read,
+                     3 FP64 written.
+    """
+    S = -st.col["snl"].astype(np.int64)
+    L = st.col["nbedrock"].astype(np.int64)
+    P = (st.col["patchf"] - st.col["patchi"] + 1).astype(np.int64)
+    NS, NSOI, NG = st.dims.nlevsno, st.dims.nlevsoi, st.dims.nlevgrnd
+    per_col = (14 * 8 + 4 * 4 + 18 * 8
+               + S * (6 * 8 + 7 * 8 + 4)
+               + NS * 2 * 8
+               + NG * (12 * 8 + 6 * 8 + 4) + 8
+               + NSOI * 2 * 8
+               + L * 6 * 8
+               + P * (15 * 8 + 4 + 3 * 8) + P * (S + 1) * 8)
+    return int(per_col.sum())
+
+
+def cpu_baseline(st, seconds=15.0):
+    """Time the C oracle (single thread) on a bounded slice of the same workload."""
+    from oracle import oracle as O
+    ncol = st.dims.ncol
+    sample = min(ncol, 4096)
+    sub = st.subset(np.arange(sample))
+    t0 = time.perf_counter()
+    steps = 0
+    while True:
+        rc, fail = O.soil_step(sub)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or steps >= 48:
+            break
+    return sample * steps / el, f"{sample} columns x {steps} steps of the same workload ({el:.1f} s)"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=48)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--ncol", type=int, default=65536)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-sort", action="store_true", help="keep generation order (no binning)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
+                                   "(columns shard trivially by range, see DESIGN.md)"}))
+        sys.exit(2)
+
+    import torch
+    from ctsm_amd import lib, synthetic
+    from ctsm_amd.state import DeviceState
+
+    st = synthetic.workload("uneven", ncol=args.ncol)
+    ctx = lib.Context(device=0)
+    stream = torch.cuda.ExternalStream(ctx.stream)
+
+    filt = None
+    if not args.no_sort:
+        perm = np.lexsort((st.col["snl"], st.col["nbedrock"])) + 1
+        filt = torch.from_numpy(perm.astype(np.int32)).cuda()
+
+    def run(plain):
+        ds = DeviceState(st)
+        structs = ds.structs()
+        for _ in range(args.warmup):
+            ctx.step_dev(ds, filter_dev=filt, plain=plain, structs=structs)
+        ctx.sync()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            ev[k][0].record(stream)
+            ctx.step_dev(ds, filter_dev=filt, plain=plain, structs=structs)
+            ev[k][1].record(stream)
+        ctx.sync()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        per = [a.elapsed_time(b) for a, b in ev]
+        return wall, float(np.mean(per)), ds
+
+    wall, step_ms, ds = run(plain=False)
+    value = args.ncol * args.steps / wall
+    res = {"fused_step_ms": step_ms}
+    if not args.no_plain:
+        wall_p, step_ms_p, _ = run(plain=True)
+        res["plain_step_ms"] = step_ms_p
+        res["plain_column_steps_per_s"] = args.ncol * args.steps / wall_p
+        res["fused_over_plain"] = value / res["plain_column_steps_per_s"]
+
+    bytes_alg = algorithmic_bytes(st)
+    achieved = bytes_alg / (step_ms * 1e-3) / 1e9
+    out = {
+        "metric": "column-steps/sec (FP64, 1 GPU)",
+        "value": value,
+        "unit": "column-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generator, inputs resident in HBM)",
+        "config": {"workload": "uneven columns: 65,536 soil columns, snow 0-12 layers, bedrock 5-20, "
+                               "SoilTemperature + SoilWater, dtime 1800 s, 48-step day",
+                   "ncol": args.ncol, "binned": not args.no_sort},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_soil_step", "alg_bytes_per_launch": bytes_alg},
+        "detail": res,
+    }
+    if not args.no_cpu:
+        v, sample = cpu_baseline(st, args.cpu_seconds)
+        out["cpu_baseline"] = {"value": v, "unit": "column-steps/s", "cores": 1, "kind": "port",
+                               "sample": sample}
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,41 @@
+"""Builds libctsm_mi355x.so (the HIP kernels + C-ABI) in-tree with hipcc for gfx950."""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SRC = ["capi.hip", "soilwater.hip", "soiltemp.hip", "plain.hip"]
+LIB = os.path.join(HERE, "libctsm_mi355x.so")
+
+# -ffp-contract=off: no FMA contraction, every expression rounds as the
+#  reference Fortran writes it.
+# -packed-fp32-ops off: no packed FP32 instructions anywhere.
+FLAGS = ["-O3", "--offload-arch=gfx950", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-Wno-unused-command-line-argument"]
+
+
+def hipcc():
+    for p in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", "hipcc"):
+        if p and (os.path.isabs(p) and os.path.exists(p) or not os.path.isabs(p)):
+            return p
+    raise RuntimeError("hipcc not found")
+
+
+def build(verbose=False, extra=()):
+    srcs = [os.path.join(HERE, "csrc", s) for s in SRC]
+    cmd = [hipcc()] + FLAGS + list(extra) + ["-I", os.path.join(ROOT, "include"),
+                                             "-I", os.path.join(HERE, "csrc")] + srcs + ["-o", LIB + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed")
+    os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,494 @@
+// capi.hip -- the extern "C" boundary of libctsm_mi355x.so (include/ctsm_gpu.h).
+//
+// A context owns a HIP stream, the device error word, the heat-solve
+// workspace and, for the drop-in (host-array) entry points, device mirrors
+// of the fields a call touches.  Drop-in calls copy the filtered column range
+// of every field to the device (strided 2-D copies over the Fortran arrays),
+// launch, and copy the writable fields back before returning, which keeps the
+// reference's synchronous `intent(inout)` semantics at the call site.
+#include <algorithm>
+#include <climits>
+#include <cstddef>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "common.h"
+
+namespace ctsm {
+void launch_soilwater(hipStream_t, const View &, const Params &, const ctsm_col_state &, int, const int32_t *, double, ErrWord *);
+void launch_soilwater_fractions(hipStream_t, const View &, const ctsm_col_state &, int, const int32_t *);
+void launch_soiltemp(hipStream_t, const View &, const Params &, const ctsm_col_state &, const ctsm_patch_state &, int,
+                     const int32_t *, double, double *, long, ErrWord *);
+void launch_soil_step(hipStream_t, const View &, const Params &, const ctsm_col_state &, const ctsm_patch_state &, int,
+                      const int32_t *, double, double *, long, ErrWord *);
+void launch_soiltemp_plain(hipStream_t, const View &, const Params &, const ctsm_col_state &, const ctsm_patch_state &,
+                           int, const int32_t *, double, double *, long, ErrWord *);
+size_t soiltemp_workspace_doubles(int n);
+size_t soiltemp_plain_workspace_doubles(int n);
+size_t soilwater_plain_workspace_doubles(int n);
+void launch_soilwater_plain(hipStream_t, const View &, const Params &, const ctsm_col_state &, int, const int32_t *,
+                            double, double *, long, ErrWord *);
+}  // namespace ctsm
+
+using namespace ctsm;
+
+namespace {
+
+enum Kind { K_C, K_SNO, K_ZI, K_SOI, K_SNOW, K_P, K_LYR };
+
+struct FieldMeta {
+    size_t offset;   // offset of the pointer member in the state struct
+    Kind kind;
+    int elem;        // bytes per element
+    bool writable;   // copied back after a drop-in call
+};
+
+#define CF(name, kind, elem, wr) {offsetof(ctsm_col_state, name), kind, elem, wr}
+const FieldMeta kColFields[] = {
+    CF(snl, K_C, 4, false), CF(nbedrock, K_C, 4, false), CF(patchi, K_C, 4, false), CF(patchf, K_C, 4, false),
+    CF(dz, K_SNO, 8, false), CF(z, K_SNO, 8, false), CF(zi, K_ZI, 8, false),
+    CF(watsat, K_SOI, 8, false), CF(bsw, K_SOI, 8, false), CF(sucsat, K_SOI, 8, false), CF(hksat, K_SOI, 8, false),
+    CF(tkmg, K_SOI, 8, false), CF(tkdry, K_SOI, 8, false), CF(csol, K_SOI, 8, false),
+    CF(eff_porosity, K_SOI, 8, true), CF(smp_l, K_SOI, 8, true), CF(hk_l, K_SOI, 8, true), CF(thk, K_SNO, 8, true),
+    CF(icefrac, K_SOI, 8, true), CF(qcharge, K_C, 8, true), CF(num_substeps, K_C, 8, true),
+    CF(qin, K_SOI, 8, true), CF(qout, K_SOI, 8, true),
+    CF(h2osoi_liq, K_SNO, 8, true), CF(h2osoi_ice, K_SNO, 8, true), CF(h2osfc, K_C, 8, true),
+    CF(h2osno_no_layers, K_C, 8, true), CF(int_snow, K_C, 8, true),
+    CF(frac_sno_eff, K_C, 8, false), CF(frac_h2osfc, K_C, 8, false), CF(snow_depth, K_C, 8, true),
+    CF(bw, K_SNOW, 8, true), CF(snomelt_accum, K_C, 8, true),
+    CF(qflx_infl, K_C, 8, false), CF(qflx_rootsoi, K_SOI, 8, false), CF(qflx_snomelt, K_C, 8, true),
+    CF(qflx_snofrz, K_C, 8, true), CF(qflx_snow_drain, K_C, 8, true), CF(qflx_snomelt_lyr, K_SNOW, 8, true),
+    CF(qflx_snofrz_lyr, K_SNOW, 8, true), CF(qflx_h2osfc_to_ice, K_C, 8, true),
+    CF(t_soisno, K_SNO, 8, true), CF(t_h2osfc, K_C, 8, true), CF(t_grnd, K_C, 8, true), CF(fact, K_SNO, 8, true),
+    CF(xmf, K_C, 8, true), CF(xmf_h2osfc, K_C, 8, true), CF(c_h2osfc, K_C, 8, true), CF(imelt, K_SNO, 4, true),
+    CF(emg, K_C, 8, false), CF(htvp, K_C, 8, false), CF(eflx_bot, K_C, 8, false), CF(eflx_fgr12, K_C, 8, true),
+    CF(eflx_fgr, K_SOI, 8, true), CF(eflx_snomelt, K_C, 8, true), CF(eflx_h2osfc_to_snow, K_C, 8, true),
+    CF(forc_lwrad, K_C, 8, false),
+};
+#undef CF
+#define PF(name, kind, elem, wr) {offsetof(ctsm_patch_state, name), kind, elem, wr}
+const FieldMeta kPatchFields[] = {
+    PF(column, K_P, 4, false), PF(wtcol, K_P, 8, false), PF(frac_veg_nosno, K_P, 4, false),
+    PF(sabg, K_P, 8, false), PF(sabg_soil, K_P, 8, false), PF(sabg_snow, K_P, 8, false), PF(sabg_lyr, K_LYR, 8, false),
+    PF(dlrad, K_P, 8, false), PF(cgrnd, K_P, 8, false), PF(eflx_sh_grnd, K_P, 8, false),
+    PF(eflx_sh_snow, K_P, 8, false), PF(eflx_sh_soil, K_P, 8, false), PF(eflx_sh_h2osfc, K_P, 8, false),
+    PF(qflx_evap_soi, K_P, 8, false), PF(qflx_ev_snow, K_P, 8, false), PF(qflx_ev_soil, K_P, 8, false),
+    PF(qflx_ev_h2osfc, K_P, 8, false), PF(eflx_gnet, K_P, 8, true), PF(dgnetdT, K_P, 8, true),
+    PF(sabg_chk, K_P, 8, true),
+};
+#undef PF
+
+int nlev_of(Kind k, const ctsm_config &cfg)
+{
+    switch (k) {
+    case K_C: case K_P: return 1;
+    case K_SNO: return cfg.nlevsno + cfg.nlevgrnd;
+    case K_ZI: return cfg.nlevsno + cfg.nlevgrnd + 1;
+    case K_SOI: return cfg.nlevgrnd;
+    case K_SNOW: return cfg.nlevsno;
+    case K_LYR: return cfg.nlevsno + 1;
+    }
+    return 0;
+}
+
+// Device buffer that only grows.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t b)
+    {
+        if (b <= bytes) return hipSuccess;
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, b);
+        if (e == hipSuccess) bytes = b;
+        return e;
+    }
+    void release()
+    {
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+__global__ void k_reset_err(ErrWord *e)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        e->first_col = INT32_MAX;
+        e->code = 0;
+    }
+}
+
+}  // namespace
+
+struct ctsm_ctx {
+    int device = 0;
+    ctsm_config cfg{};
+    Params prm{};
+    hipStream_t stream = nullptr;
+    ErrWord *err = nullptr;            // device
+    DevBuf work;                       // heat-solve workspace
+    DevBuf filt;                       // filter (drop-in mode)
+    std::vector<DevBuf> colmir, pchmir; // drop-in mirrors
+    int32_t last_code = 0, last_index = 0;
+    std::mutex mu;
+};
+
+namespace {
+
+int hip_fail(ctsm_ctx *ctx, hipError_t e)
+{
+    if (ctx) { ctx->last_code = CTSM_E_HIP; ctx->last_index = (int32_t)e; }
+    return CTSM_E_HIP;
+}
+
+Params make_params(const ctsm_config &c)
+{
+    Params p{};
+    p.dtmin = c.dtmin;
+    p.verySmall = c.verySmall;
+    p.xTolerUpper = c.xTolerUpper;
+    p.xTolerLower = c.xTolerLower;
+    p.e_ice = c.e_ice;
+    p.lower_bc = c.lower_boundary_condition;
+    p.flux_calculation = c.flux_calculation;
+    p.snow_thermal_cond = c.snow_thermal_cond_method;
+    return p;
+}
+
+View make_view(const ctsm_ctx *ctx, const ctsm_layout &L)
+{
+    View v{};
+    v.begc = L.begc_alloc;
+    v.ldc = L.ldc;
+    v.begp = L.begp_alloc;
+    v.ldp = L.ldp;
+    v.nlevsno = ctx->cfg.nlevsno;
+    v.nlevsoi = ctx->cfg.nlevsoi;
+    v.nlevgrnd = ctx->cfg.nlevgrnd;
+    return v;
+}
+
+int check_layout(const ctsm_layout *L)
+{
+    if (!L || L->ldc <= 0 || L->ldp < 0) return CTSM_E_ARG;
+    return CTSM_OK;
+}
+
+int ensure_work(ctsm_ctx *ctx, size_t doubles)
+{
+    hipError_t e = ctx->work.ensure(std::max<size_t>(doubles, 1) * sizeof(double));
+    return e == hipSuccess ? CTSM_OK : hip_fail(ctx, e);
+}
+
+int finish_launch(ctsm_ctx *ctx)
+{
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? CTSM_OK : hip_fail(ctx, e);
+}
+
+int sync_and_check(ctsm_ctx *ctx)
+{
+    ErrWord h{};
+    hipError_t e = hipMemcpyAsync(&h, ctx->err, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e);
+    hipLaunchKernelGGL(k_reset_err, dim3(1), dim3(1), 0, ctx->stream, ctx->err);
+    e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e);
+    if (h.code != 0) {
+        ctx->last_code = h.code;
+        ctx->last_index = h.first_col;
+        return CTSM_E_SOLVER;
+    }
+    ctx->last_code = 0;
+    ctx->last_index = 0;
+    return CTSM_OK;
+}
+
+// ---- drop-in mirroring ------------------------------------------------------
+
+struct Range { int lo, hi; };
+
+Range filter_range(int n, const int32_t *f, int def_lo)
+{
+    if (!f) return {def_lo, def_lo + n - 1};
+    int lo = INT_MAX, hi = INT_MIN;
+    for (int i = 0; i < n; ++i) { lo = std::min(lo, (int)f[i]); hi = std::max(hi, (int)f[i]); }
+    return {lo, hi};
+}
+
+// Copy [lo, hi] of every level of each non-null field between host arrays
+// (layout ld / beg) and dense device mirrors (layout hi-lo+1 / lo).
+int mirror(ctsm_ctx *ctx, const FieldMeta *meta, int nmeta, const void *host_struct, void *dev_struct,
+           std::vector<DevBuf> &bufs, int beg_alloc, int ld, Range r, bool to_device)
+{
+    const int w = r.hi - r.lo + 1;
+    if (bufs.size() < (size_t)nmeta) bufs.resize(nmeta);
+    for (int f = 0; f < nmeta; ++f) {
+        const FieldMeta &m = meta[f];
+        char *hp = *(char *const *)((const char *)host_struct + m.offset);
+        char **dslot = (char **)((char *)dev_struct + m.offset);
+        if (!hp) { *dslot = nullptr; continue; }
+        if (!to_device && !m.writable) continue;
+        const int nl = nlev_of(m.kind, ctx->cfg);
+        if (to_device) {
+            hipError_t e = bufs[f].ensure((size_t)w * nl * m.elem);
+            if (e != hipSuccess) return hip_fail(ctx, e);
+            *dslot = (char *)bufs[f].p;
+        }
+        char *dp = (char *)bufs[f].p;
+        const char *hsrc = hp + (size_t)(r.lo - beg_alloc) * m.elem;
+        hipError_t e;
+        if (to_device)
+            e = hipMemcpy2DAsync(dp, (size_t)w * m.elem, hsrc, (size_t)ld * m.elem, (size_t)w * m.elem, nl,
+                                 hipMemcpyHostToDevice, ctx->stream);
+        else
+            e = hipMemcpy2DAsync((void *)hsrc, (size_t)ld * m.elem, dp, (size_t)w * m.elem, (size_t)w * m.elem, nl,
+                                 hipMemcpyDeviceToHost, ctx->stream);
+        if (e != hipSuccess) return hip_fail(ctx, e);
+    }
+    return CTSM_OK;
+}
+
+// Patch range of the filtered columns (inclusive).
+Range patch_range(const ctsm_layout *L, const ctsm_col_state *col, int n, const int32_t *f)
+{
+    int lo = INT_MAX, hi = INT_MIN;
+    for (int i = 0; i < n; ++i) {
+        const int c = f ? f[i] : L->begc_alloc + i;
+        const int pi = col->patchi[c - L->begc_alloc], pf = col->patchf[c - L->begc_alloc];
+        if (pf >= pi) { lo = std::min(lo, pi); hi = std::max(hi, pf); }
+    }
+    if (lo > hi) return {L->begp_alloc, L->begp_alloc};
+    return {lo, hi};
+}
+
+enum Routine { R_SOILWATER, R_SOILTEMP, R_FRACTIONS };
+
+int run_host(ctsm_ctx *ctx, Routine which, const ctsm_layout *L, const ctsm_col_state *col,
+             const ctsm_patch_state *pch, int32_t n, const int32_t *filter, double dtime)
+{
+    if (check_layout(L) || !col || n < 0 || (which == R_SOILTEMP && !pch)) return CTSM_E_ARG;
+    if (n == 0) return CTSM_OK;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipError_t he = hipSetDevice(ctx->device);
+    if (he != hipSuccess) return hip_fail(ctx, he);
+    const Range cr = filter_range(n, filter, L->begc_alloc);
+    if (cr.lo < L->begc_alloc || cr.hi >= L->begc_alloc + L->ldc) return CTSM_E_ARG;
+    ctsm_col_state dcol{};
+    ctsm_patch_state dpch{};
+    int rc = mirror(ctx, kColFields, (int)(sizeof(kColFields) / sizeof(kColFields[0])), col, &dcol, ctx->colmir,
+                    L->begc_alloc, L->ldc, cr, true);
+    if (rc) return rc;
+    Range pr{L->begp_alloc, L->begp_alloc};
+    if (which == R_SOILTEMP) {
+        pr = patch_range(L, col, n, filter);
+        if (pr.lo < L->begp_alloc || pr.hi >= L->begp_alloc + L->ldp) return CTSM_E_ARG;
+        rc = mirror(ctx, kPatchFields, (int)(sizeof(kPatchFields) / sizeof(kPatchFields[0])), pch, &dpch, ctx->pchmir,
+                    L->begp_alloc, L->ldp, pr, true);
+        if (rc) return rc;
+    }
+    const int32_t *dfilt = nullptr;
+    if (filter) {
+        he = ctx->filt.ensure((size_t)n * sizeof(int32_t));
+        if (he != hipSuccess) return hip_fail(ctx, he);
+        he = hipMemcpyAsync(ctx->filt.p, filter, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream);
+        if (he != hipSuccess) return hip_fail(ctx, he);
+        dfilt = (const int32_t *)ctx->filt.p;
+    }
+    ctsm_layout dl{cr.lo, cr.hi - cr.lo + 1, pr.lo, pr.hi - pr.lo + 1};
+    View V = make_view(ctx, dl);
+    switch (which) {
+    case R_SOILWATER:
+        launch_soilwater(ctx->stream, V, ctx->prm, dcol, n, dfilt, dtime, ctx->err);
+        break;
+    case R_FRACTIONS:
+        launch_soilwater_fractions(ctx->stream, V, dcol, n, dfilt);
+        break;
+    case R_SOILTEMP:
+        if ((rc = ensure_work(ctx, soiltemp_workspace_doubles(n)))) return rc;
+        launch_soiltemp(ctx->stream, V, ctx->prm, dcol, dpch, n, dfilt, dtime, (double *)ctx->work.p, n, ctx->err);
+        break;
+    }
+    if ((rc = finish_launch(ctx))) return rc;
+    rc = mirror(ctx, kColFields, (int)(sizeof(kColFields) / sizeof(kColFields[0])), col, &dcol, ctx->colmir,
+                L->begc_alloc, L->ldc, cr, false);
+    if (rc) return rc;
+    if (which == R_SOILTEMP) {
+        rc = mirror(ctx, kPatchFields, (int)(sizeof(kPatchFields) / sizeof(kPatchFields[0])), pch, &dpch, ctx->pchmir,
+                    L->begp_alloc, L->ldp, pr, false);
+        if (rc) return rc;
+    }
+    return sync_and_check(ctx);
+}
+
+}  // namespace
+
+extern "C" {
+
+void ctsm_config_default(ctsm_config *cfg)
+{
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->nlevsno = 12;
+    cfg->nlevsoi = 20;
+    cfg->nlevgrnd = 25;
+    cfg->soilwater_movement_method = 1;
+    cfg->upper_boundary_condition = CTSM_BC_FLUX;
+    cfg->lower_boundary_condition = CTSM_BC_ZERO_FLUX;
+    cfg->flux_calculation = 1;
+    cfg->snow_thermal_cond_method = CTSM_SNOWTHK_STURM1997;
+    cfg->itmax_canopy_fluxes = 40;
+    cfg->dtmin = 60.0;
+    cfg->verySmall = 1.e-8;
+    cfg->xTolerUpper = 1.e-1;
+    cfg->xTolerLower = 1.e-2;
+    cfg->e_ice = 6.0;
+}
+
+int ctsm_create(const ctsm_config *cfg, int device, ctsm_ctx **out)
+{
+    if (!cfg || !out) return CTSM_E_ARG;
+    *out = nullptr;
+    if (cfg->soilwater_movement_method != 1 || cfg->upper_boundary_condition != CTSM_BC_FLUX ||
+        (cfg->lower_boundary_condition != CTSM_BC_FLUX && cfg->lower_boundary_condition != CTSM_BC_ZERO_FLUX) ||
+        cfg->flux_calculation != 1 ||
+        (cfg->snow_thermal_cond_method != CTSM_SNOWTHK_JORDAN1991 &&
+         cfg->snow_thermal_cond_method != CTSM_SNOWTHK_STURM1997) ||
+        cfg->nlevsno < 1 || cfg->nlevsno > MAX_NLEVSNO || cfg->nlevsoi < 2 || cfg->nlevsoi > MAX_NLEVSOI ||
+        cfg->nlevgrnd < cfg->nlevsoi || cfg->nlevgrnd > MAX_NLEVGRND || cfg->nlevsno + 1 + cfg->nlevgrnd > MAX_ROWS)
+        return CTSM_E_CONFIG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return CTSM_E_NODEVICE;
+    if (device < 0 || device >= ndev) return CTSM_E_ARG;
+    if (hipSetDevice(device) != hipSuccess) return CTSM_E_HIP;
+    ctsm_ctx *ctx = new ctsm_ctx();
+    ctx->device = device;
+    ctx->cfg = *cfg;
+    ctx->prm = make_params(*cfg);
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc((void **)&ctx->err, sizeof(ErrWord)) != hipSuccess) {
+        ctsm_destroy(ctx);
+        return CTSM_E_HIP;
+    }
+    hipLaunchKernelGGL(k_reset_err, dim3(1), dim3(1), 0, ctx->stream, ctx->err);
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        ctsm_destroy(ctx);
+        return CTSM_E_HIP;
+    }
+    *out = ctx;
+    return CTSM_OK;
+}
+
+int ctsm_destroy(ctsm_ctx *ctx)
+{
+    if (!ctx) return CTSM_OK;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    ctx->work.release();
+    ctx->filt.release();
+    for (auto &b : ctx->colmir) b.release();
+    for (auto &b : ctx->pchmir) b.release();
+    if (ctx->err) hipFree(ctx->err);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return CTSM_OK;
+}
+
+int ctsm_last_error(const ctsm_ctx *ctx, int32_t *code, int32_t *index)
+{
+    if (!ctx) return CTSM_E_ARG;
+    if (code) *code = ctx->last_code;
+    if (index) *index = ctx->last_index;
+    return CTSM_OK;
+}
+
+int ctsm_soilwater(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col, int32_t n,
+                   const int32_t *filter, double dtime)
+{
+    if (!ctx) return CTSM_E_ARG;
+    return run_host(ctx, R_SOILWATER, lay, col, nullptr, n, filter, dtime);
+}
+
+int ctsm_soiltemperature(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                         const ctsm_patch_state *pch, int32_t n, const int32_t *filter, double dtime)
+{
+    if (!ctx) return CTSM_E_ARG;
+    return run_host(ctx, R_SOILTEMP, lay, col, pch, n, filter, dtime);
+}
+
+int ctsm_soilwater_fractions(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col, int32_t n,
+                             const int32_t *filter)
+{
+    if (!ctx) return CTSM_E_ARG;
+    return run_host(ctx, R_FRACTIONS, lay, col, nullptr, n, filter, 0.0);
+}
+
+int ctsm_soilwater_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col, int32_t n,
+                       const int32_t *filter_dev, double dtime, int32_t plain)
+{
+    if (!ctx || check_layout(lay) || !col || n < 0 || (!filter_dev && n > lay->ldc)) return CTSM_E_ARG;
+    if (n == 0) return CTSM_OK;
+    if (plain) {
+        int rc;
+        if ((rc = ensure_work(ctx, soilwater_plain_workspace_doubles(n)))) return rc;
+        launch_soilwater_plain(ctx->stream, make_view(ctx, *lay), ctx->prm, *col, n, filter_dev, dtime,
+                               (double *)ctx->work.p, n, ctx->err);
+    } else {
+        launch_soilwater(ctx->stream, make_view(ctx, *lay), ctx->prm, *col, n, filter_dev, dtime, ctx->err);
+    }
+    return finish_launch(ctx);
+}
+
+int ctsm_soiltemperature_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                             const ctsm_patch_state *pch, int32_t n, const int32_t *filter_dev, double dtime,
+                             int32_t plain)
+{
+    if (!ctx || check_layout(lay) || !col || !pch || n < 0 || (!filter_dev && n > lay->ldc)) return CTSM_E_ARG;
+    if (n == 0) return CTSM_OK;
+    int rc;
+    if (plain) {
+        if ((rc = ensure_work(ctx, soiltemp_plain_workspace_doubles(n)))) return rc;
+        launch_soiltemp_plain(ctx->stream, make_view(ctx, *lay), ctx->prm, *col, *pch, n, filter_dev, dtime,
+                              (double *)ctx->work.p, n, ctx->err);
+    } else {
+        if ((rc = ensure_work(ctx, soiltemp_workspace_doubles(n)))) return rc;
+        launch_soiltemp(ctx->stream, make_view(ctx, *lay), ctx->prm, *col, *pch, n, filter_dev, dtime,
+                        (double *)ctx->work.p, n, ctx->err);
+    }
+    return finish_launch(ctx);
+}
+
+int ctsm_soilwater_fractions_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col, int32_t n,
+                                 const int32_t *filter_dev)
+{
+    if (!ctx || check_layout(lay) || !col || n < 0 || (!filter_dev && n > lay->ldc)) return CTSM_E_ARG;
+    if (n == 0) return CTSM_OK;
+    launch_soilwater_fractions(ctx->stream, make_view(ctx, *lay), *col, n, filter_dev);
+    return finish_launch(ctx);
+}
+
+int ctsm_soil_step_fused_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                             const ctsm_patch_state *pch, int32_t n, const int32_t *filter_dev, double dtime)
+{
+    if (!ctx || check_layout(lay) || !col || !pch || n < 0 || (!filter_dev && n > lay->ldc)) return CTSM_E_ARG;
+    if (n == 0) return CTSM_OK;
+    int rc;
+    if ((rc = ensure_work(ctx, soiltemp_workspace_doubles(n)))) return rc;
+    launch_soil_step(ctx->stream, make_view(ctx, *lay), ctx->prm, *col, *pch, n, filter_dev, dtime,
+                     (double *)ctx->work.p, n, ctx->err);
+    return finish_launch(ctx);
+}
+
+int ctsm_sync(ctsm_ctx *ctx)
+{
+    if (!ctx) return CTSM_E_ARG;
+    return sync_and_check(ctx);
+}
+
+void *ctsm_stream(ctsm_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+}  // extern "C"

@@ -1,0 +1,72 @@
+// soilwater.hip -- moisture-form Richards solve on gfx950.
+//
+// Replaces soilwater_moisture_form (SoilWaterMovementMod.F90:976-1440).  One
+// column per lane: the column's <= 20 layers live in VGPRs/AGPRs for the
+// whole adaptive sub-step loop, so a trial touches no memory except the
+// column's inputs loaded once.  Each trial is two sweeps over the layers:
+//
+//   forward : Clapp-Hornberger hydraulic properties (:1444-1556, CH78
+//             soil_hk / soil_suction), Darcy fluxes and their derivatives
+//             (:1560-1866), the RHS / LHS rows (:1871-2012), and the
+//             elimination step of LAPACK dgtsv for the previous row, all
+//             streamed one layer behind the property lookahead;
+//   backward: dgtsv back substitution fused with the error estimate
+//             (:1307-1351);
+//
+// then the accept / halve / double decision of :1353-1397.  Every value is
+// computed by the reference's expression in the reference's order, so the
+// fusion changes no rounding.  Lanes of a wavefront retire independently;
+// the column order in the filter (sorted by layer count by the caller) is
+// what keeps a wavefront's lanes doing the same amount of work.
+#include "common.h"
+#include "soilwater_dev.h"
+
+namespace ctsm {
+
+__global__ void __launch_bounds__(64)
+k_soilwater(View V, Params P, ctsm_col_state S, int n, const int32_t *__restrict__ filter,
+            double dtime, ErrWord *err)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = filter ? filter[i] : V.begc + i;
+    if (!col_in_range(V, c)) { report_error(err, c, CTSM_E_ARG); return; }
+    const int rc = richards_column<MAX_NLEVSOI>(V, P, S, c, dtime);
+    if (rc) report_error(err, c, CTSM_E_SOLVER);
+}
+
+// SetSoilWaterFractions (SoilHydrologyMod.F90:240-252), excess_ice = 0.
+__global__ void __launch_bounds__(256)
+k_soilwater_fractions(View V, ctsm_col_state S, int n, const int32_t *__restrict__ filter)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = filter ? filter[i] : V.begc + i;
+    if (!col_in_range(V, c)) return;
+    for (int j = 1; j <= V.nlevsoi; ++j) {
+        const double ws = S.watsat[V.soi(c, j)];
+        const double dz_ext = S.dz[V.sno(c, j)] + 0.0 / DENICE;
+        const double vol_ice = fmin90(ws, (S.h2osoi_ice[V.sno(c, j)] + 0.0) / (dz_ext * DENICE));
+        S.eff_porosity[V.soi(c, j)] = fmax90(0.01, ws - vol_ice);
+        S.icefrac[V.soi(c, j)] = fmin90(1.0, vol_ice / ws);
+    }
+}
+
+}  // namespace ctsm
+
+namespace ctsm {
+void launch_soilwater(hipStream_t st, const View &V, const Params &P, const ctsm_col_state &S, int n,
+                      const int32_t *filter, double dtime, ErrWord *err)
+{
+    if (n <= 0) return;
+    const int bs = 64;
+    hipLaunchKernelGGL(k_soilwater, dim3((n + bs - 1) / bs), dim3(bs), 0, st, V, P, S, n, filter, dtime, err);
+}
+void launch_soilwater_fractions(hipStream_t st, const View &V, const ctsm_col_state &S, int n,
+                                const int32_t *filter)
+{
+    if (n <= 0) return;
+    const int bs = 256;
+    hipLaunchKernelGGL(k_soilwater_fractions, dim3((n + bs - 1) / bs), dim3(bs), 0, st, V, S, n, filter);
+}
+}  // namespace ctsm

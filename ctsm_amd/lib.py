@@ -1,0 +1,168 @@
+"""Loader of libctsm_mi355x.so and thin Python mirrors of its entry points.
+
+The Python side mirrors the reference's call sites: soilwater() stands for
+SoilWater (SoilWaterMovementMod.F90:240), soiltemperature() for
+SoilTemperature (SoilTemperatureMod.F90:92), both taking the state and a
+column filter and mutating the state in place, and raising CtsmError where
+the reference would call endrun.  There is no CPU fallback: if the HIP
+library is missing or no GPU is visible, every call raises.
+"""
+import ctypes as C
+import os
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libctsm_mi355x.so")
+_LIB = None
+
+CTSM_OK, CTSM_E_ARG, CTSM_E_CONFIG, CTSM_E_HIP, CTSM_E_SOLVER, CTSM_E_NODEVICE = range(6)
+_NAMES = {1: "CTSM_E_ARG", 2: "CTSM_E_CONFIG", 3: "CTSM_E_HIP", 4: "CTSM_E_SOLVER", 5: "CTSM_E_NODEVICE"}
+
+# every symbol declared in include/ctsm_gpu.h
+EXPORTS = [
+    "ctsm_config_default", "ctsm_create", "ctsm_destroy", "ctsm_last_error",
+    "ctsm_soilwater", "ctsm_soiltemperature", "ctsm_soilwater_fractions",
+    "ctsm_soilwater_dev", "ctsm_soiltemperature_dev", "ctsm_soilwater_fractions_dev",
+    "ctsm_soil_step_fused_dev", "ctsm_sync", "ctsm_stream",
+]
+
+
+class CtsmError(RuntimeError):
+    """Raised where the reference calls endrun (e.g. LAPACK info != 0)."""
+
+    def __init__(self, code, index=0, where=""):
+        self.code, self.index = code, index
+        super().__init__(f"{where}: {_NAMES.get(code, code)} (first failing column {index})")
+
+
+def load():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    L.ctsm_config_default.argtypes = [P(abi.Config)]
+    L.ctsm_config_default.restype = None
+    L.ctsm_create.argtypes = [P(abi.Config), C.c_int, P(C.c_void_p)]
+    L.ctsm_destroy.argtypes = [C.c_void_p]
+    L.ctsm_last_error.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_int32)]
+    col = [C.c_void_p, P(abi.Layout), P(abi.ColState)]
+    L.ctsm_soilwater.argtypes = col + [C.c_int32, abi.i32p, C.c_double]
+    L.ctsm_soilwater_fractions.argtypes = col + [C.c_int32, abi.i32p]
+    L.ctsm_soiltemperature.argtypes = col + [P(abi.PatchState), C.c_int32, abi.i32p, C.c_double]
+    L.ctsm_soilwater_dev.argtypes = col + [C.c_int32, C.c_void_p, C.c_double, C.c_int32]
+    L.ctsm_soilwater_fractions_dev.argtypes = col + [C.c_int32, C.c_void_p]
+    L.ctsm_soiltemperature_dev.argtypes = col + [P(abi.PatchState), C.c_int32, C.c_void_p, C.c_double, C.c_int32]
+    L.ctsm_soil_step_fused_dev.argtypes = col + [P(abi.PatchState), C.c_int32, C.c_void_p, C.c_double]
+    L.ctsm_sync.argtypes = [C.c_void_p]
+    L.ctsm_stream.argtypes = [C.c_void_p]
+    L.ctsm_stream.restype = C.c_void_p
+    _LIB = L
+    return L
+
+
+def default_config(**kw):
+    cfg = abi.Config()
+    load().ctsm_config_default(C.byref(cfg))
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+class Context:
+    """One library context (HIP stream + workspace) on one device."""
+
+    def __init__(self, cfg=None, device=0):
+        self.L = load()
+        self.cfg = cfg if cfg is not None else default_config()
+        h = C.c_void_p()
+        rc = self.L.ctsm_create(C.byref(self.cfg), device, C.byref(h))
+        if rc != CTSM_OK:
+            raise CtsmError(rc, 0, "ctsm_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ctsm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, where):
+        if rc != CTSM_OK:
+            code, idx = C.c_int32(), C.c_int32()
+            self.L.ctsm_last_error(self.h, C.byref(code), C.byref(idx))
+            raise CtsmError(rc, idx.value, where)
+
+    @property
+    def stream(self):
+        return self.L.ctsm_stream(self.h)
+
+    # ---- drop-in (host numpy state) ------------------------------------
+    def soilwater(self, st, filter_=None, dtime=1800.0):
+        cs, _ = st.structs()
+        f, fp, n = _filter(filter_, st.dims.ncol)
+        self._check(self.L.ctsm_soilwater(self.h, C.byref(st.layout()), C.byref(cs), n, fp, dtime), "SoilWater")
+
+    def soiltemperature(self, st, filter_=None, dtime=1800.0):
+        cs, ps = st.structs()
+        f, fp, n = _filter(filter_, st.dims.ncol)
+        self._check(self.L.ctsm_soiltemperature(self.h, C.byref(st.layout()), C.byref(cs), C.byref(ps), n, fp,
+                                                dtime), "SoilTemperature")
+
+    def soilwater_fractions(self, st, filter_=None):
+        cs, _ = st.structs()
+        f, fp, n = _filter(filter_, st.dims.ncol)
+        self._check(self.L.ctsm_soilwater_fractions(self.h, C.byref(st.layout()), C.byref(cs), n, fp),
+                    "SetSoilWaterFractions")
+
+    # ---- device-resident (torch tensors) -------------------------------
+    def step_dev(self, ds, filter_dev=None, n=None, dtime=1800.0, plain=False, structs=None):
+        """One soil step on a DeviceState: fused kernel, or the plain
+        one-launch-per-routine sequence.  Enqueues only."""
+        cs, ps = structs if structs is not None else ds.structs()
+        lay = ds.layout()
+        n = ds.dims.ncol if n is None else n
+        fp = C.c_void_p(filter_dev.data_ptr()) if filter_dev is not None else None
+        if plain:
+            self._check(self.L.ctsm_soiltemperature_dev(self.h, C.byref(lay), C.byref(cs), C.byref(ps), n, fp,
+                                                        dtime, 1), "SoilTemperature(plain)")
+            self._check(self.L.ctsm_soilwater_fractions_dev(self.h, C.byref(lay), C.byref(cs), n, fp),
+                        "SetSoilWaterFractions")
+            self._check(self.L.ctsm_soilwater_dev(self.h, C.byref(lay), C.byref(cs), n, fp, dtime, 1),
+                        "SoilWater(plain)")
+        else:
+            self._check(self.L.ctsm_soil_step_fused_dev(self.h, C.byref(lay), C.byref(cs), C.byref(ps), n, fp,
+                                                        dtime), "soil step")
+
+    def soilwater_dev(self, ds, filter_dev=None, n=None, dtime=1800.0, plain=False):
+        cs, _ = ds.structs()
+        n = ds.dims.ncol if n is None else n
+        fp = C.c_void_p(filter_dev.data_ptr()) if filter_dev is not None else None
+        self._check(self.L.ctsm_soilwater_dev(self.h, C.byref(ds.layout()), C.byref(cs), n, fp, dtime,
+                                              int(plain)), "SoilWater")
+
+    def soiltemperature_dev(self, ds, filter_dev=None, n=None, dtime=1800.0, plain=False):
+        cs, ps = ds.structs()
+        n = ds.dims.ncol if n is None else n
+        fp = C.c_void_p(filter_dev.data_ptr()) if filter_dev is not None else None
+        self._check(self.L.ctsm_soiltemperature_dev(self.h, C.byref(ds.layout()), C.byref(cs), C.byref(ps), n,
+                                                    fp, dtime, int(plain)), "SoilTemperature")
+
+    def sync(self):
+        self._check(self.L.ctsm_sync(self.h), "sync")
+
+
+def _filter(filter_, ncol):
+    if filter_ is None:
+        return None, None, ncol
+    import numpy as np
+    f = np.ascontiguousarray(filter_, dtype=np.int32)
+    return f, f.ctypes.data_as(abi.i32p), len(f)

@@ -1,0 +1,125 @@
+"""Column / patch state containers in the reference's memory layout.
+
+A state is a dict of arrays, one per field of ctsm_col_state /
+ctsm_patch_state, laid out exactly as the reference allocates them
+(ColumnType.F90:137-155 and friends): level-major, column-fastest, so array
+[k, c] is Fortran element (c, lo + k).  Host states hold numpy arrays; device
+states hold torch tensors on the GPU (PyTorch is only the allocator here).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+_NP = {"i32": np.int32, "f64": np.float64}
+
+
+class Dims:
+    def __init__(self, ncol, npatch, nlevsno=12, nlevsoi=20, nlevgrnd=25):
+        self.ncol, self.npatch = int(ncol), int(npatch)
+        self.nlevsno, self.nlevsoi, self.nlevgrnd = nlevsno, nlevsoi, nlevgrnd
+
+    def shape(self, kind):
+        n = self.npatch if kind in ("p", "lyr") else self.ncol
+        nl = abi.nlev_of(kind, self.nlevsno, self.nlevgrnd)
+        return (n,) if kind in ("c", "p") else (nl, n)
+
+
+class HostState:
+    """numpy arrays for every column and patch field."""
+
+    def __init__(self, dims, fill_nan=True):
+        self.dims = dims
+        self.col = {}
+        self.pch = {}
+        for name, t, kind in abi.COL_FIELDS:
+            a = np.zeros(dims.shape(kind), dtype=_NP[t])
+            if fill_nan and t == "f64":
+                a[...] = np.nan
+            self.col[name] = a
+        for name, t, kind in abi.PATCH_FIELDS:
+            a = np.zeros(dims.shape(kind), dtype=_NP[t])
+            if fill_nan and t == "f64":
+                a[...] = np.nan
+            self.pch[name] = a
+
+    def copy(self):
+        o = HostState.__new__(HostState)
+        o.dims = self.dims
+        o.col = {k: v.copy() for k, v in self.col.items()}
+        o.pch = {k: v.copy() for k, v in self.pch.items()}
+        return o
+
+    def layout(self):
+        return abi.Layout(1, self.dims.ncol, 1, self.dims.npatch)
+
+    def structs(self):
+        """ctypes ColState / PatchState pointing at these numpy buffers."""
+        cs = abi.ColState()
+        for name, t, _ in abi.COL_FIELDS:
+            a = self.col[name]
+            assert a.flags.c_contiguous
+            setattr(cs, name, a.ctypes.data_as(abi._CT[t]))
+        ps = abi.PatchState()
+        for name, t, _ in abi.PATCH_FIELDS:
+            a = self.pch[name]
+            assert a.flags.c_contiguous
+            setattr(ps, name, a.ctypes.data_as(abi._CT[t]))
+        return cs, ps
+
+    def subset(self, cols):
+        """State holding only columns `cols` (0-based, sorted) and their patches,
+        renumbered densely (used by the alone-vs-batched determinism tests)."""
+        cols = np.asarray(cols)
+        d = self.dims
+        pi = self.col["patchi"][cols] - 1
+        pf = self.col["patchf"][cols] - 1
+        pidx = np.concatenate([np.arange(a, b + 1) for a, b in zip(pi, pf)])
+        nd = Dims(len(cols), len(pidx), d.nlevsno, d.nlevsoi, d.nlevgrnd)
+        o = HostState(nd)
+        for name, t, kind in abi.COL_FIELDS:
+            src = self.col[name]
+            o.col[name] = np.ascontiguousarray(src[..., cols])
+        for name, t, kind in abi.PATCH_FIELDS:
+            src = self.pch[name]
+            o.pch[name] = np.ascontiguousarray(src[..., pidx])
+        npc = pf - pi + 1
+        start = np.concatenate([[0], np.cumsum(npc)[:-1]])
+        o.col["patchi"] = (start + 1).astype(np.int32)
+        o.col["patchf"] = (start + npc).astype(np.int32)
+        o.pch["column"] = np.repeat(np.arange(1, len(cols) + 1, dtype=np.int32), npc)
+        return o
+
+
+class DeviceState:
+    """torch tensors on a GPU mirroring a HostState (device-resident mode)."""
+
+    def __init__(self, host, device="cuda:0"):
+        import torch
+        self.dims = host.dims
+        self.col = {k: torch.from_numpy(v).to(device) for k, v in host.col.items()}
+        self.pch = {k: torch.from_numpy(v).to(device) for k, v in host.pch.items()}
+
+    def layout(self):
+        return abi.Layout(1, self.dims.ncol, 1, self.dims.npatch)
+
+    def structs(self):
+        cs = abi.ColState()
+        for name, t, _ in abi.COL_FIELDS:
+            setattr(cs, name, C.cast(C.c_void_p(self.col[name].data_ptr()), abi._CT[t]))
+        ps = abi.PatchState()
+        for name, t, _ in abi.PATCH_FIELDS:
+            setattr(ps, name, C.cast(C.c_void_p(self.pch[name].data_ptr()), abi._CT[t]))
+        return cs, ps
+
+    def to_host(self):
+        o = HostState.__new__(HostState)
+        o.dims = self.dims
+        o.col = {k: v.cpu().numpy().copy() for k, v in self.col.items()}
+        o.pch = {k: v.cpu().numpy().copy() for k, v in self.pch.items()}
+        return o
+
+    def nbytes(self):
+        return sum(v.numel() * v.element_size() for v in self.col.values()) + \
+            sum(v.numel() * v.element_size() for v in self.pch.values())

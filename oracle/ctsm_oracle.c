@@ -1,0 +1,1005 @@
+/*
+ * ctsm_oracle.c -- CPU ORACLE (test infrastructure only).
+ *
+ * A plain-C, one-column-at-a-time restatement of the reference Fortran for
+ * the soil part of CTSM's biogeophysics step.  It is the checker the HIP
+ * kernels are compared against; it is never linked into, loaded by or called
+ * from the product library.  Only tests/, the smoke() check and the CPU
+ * baseline leg of bench.py use it.
+ *
+ * Why a restatement: the reference routines cannot be compiled here.  Every
+ * one of them `use`s shr_kind_mod / shr_log_mod / abortutils / decompMod
+ * from the CESM `share` submodule and the CIME build, which are absent from
+ * the reference checkout, and building them would need stand-ins for those
+ * libraries.
+ *
+ * Floating point: compiled with -O2 -ffp-contract=off, binary64, operation
+ * order as written in the Fortran (left-to-right, `-a*b` = -(a*b), integer
+ * powers by repeated squaring).  Fortran default-REAL literals that the
+ * reference writes without the _r8 kind (e.g. 0.023 in the Sturm snow
+ * conductivity, 1.0e-3 in tk_h2osfc) are single-precision constants and are
+ * kept so here ((double)0.023f).
+ *
+ * Functions and the reference lines they follow:
+ *   oracle_dgtsv            LAPACK dgtsv, NRHS=1 (called at SoilWaterMovementMod.F90:1287)
+ *   oracle_dgbsv            LAPACK dgbsv = dgbtf2 + dgbtrs (called at BandDiagonalMod.F90:197)
+ *   oracle_soilwater        SoilWaterMovementMod.F90:976-1440 (+ :1444-2012, :2139-2160,
+ *                           SoilWaterRetentionCurveClappHornberg1978Mod.F90:75-119)
+ *   oracle_soilwater_fractions  SoilHydrologyMod.F90:202-256
+ *   oracle_soiltemperature  SoilTemperatureMod.F90:92-508 with SoilThermProp :602-901,
+ *                           PhaseChangeH2osfc :904-1130, Phasechange :1133-1540,
+ *                           ComputeGroundHeatFluxAndDeriv :1543-1796,
+ *                           ComputeHeatDiffFluxAndFactor :1799-1910, SetRHSVec* :1913-2353,
+ *                           SetMatrix* / AssembleMatrixFromSubmatrices :2356-2926,
+ *                           BandDiagonal BandDiagonalMod.F90:29-221
+ *
+ * Parity pins (tests/test_oracle_*.py): dgtsv/dgbsv against scipy's LAPACK,
+ * brute-force dense solves and residuals; CH78 closed forms; Richards mass
+ * conservation, stationarity of a hydrostatic column, sub-step sums; heat-solve
+ * energy conservation (the errsoi residual of SoilFluxesMod.F90:399-445),
+ * isothermal zero-flux invariance, an analytic Crank-Nicolson slab step;
+ * latent-heat accounting of the phase change.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "ctsm_gpu.h"
+
+/* ---- constants: shr_const_mod (tools/mksurfdata_esmf/src/shr_const_mod.F90:14-52)
+ *      and clm_varcon.F90:50-121 ------------------------------------------ */
+#define TFRZ        273.15
+#define DENH2O      1.000e3
+#define DENICE      0.917e3
+#define CPLIQ       4.188e3
+#define CPICE       2.11727e3
+#define HFUS        3.337e5
+#define GRAV        9.80616
+#define SB          5.67e-8
+#define TKAIR       0.023
+#define TKICE       2.290
+#define TKWAT       0.57
+#define THK_BEDROCK 3.0
+#define CSOL_BEDROCK 2.0e6
+#define CAPR        0.34
+#define CNFAC       0.5
+#define THIN_SFCLAYER 1.0e-6
+#define M_TO_MM     1.e3
+
+#define NLEVMAX_SOI 64   /* scratch bound for layers */
+#define NBANDMAX    (64 + 64)
+
+/* ---- array addressing (first element of the Fortran allocation) ---------- */
+typedef struct {
+    int begc, ldc, begp, ldp, nlevsno;
+} lay_t;
+#define C1(a, c)      ((a)[(c) - Lo.begc])
+#define SNO(a, c, j)  ((a)[((c) - Lo.begc) + (long)((j) + Lo.nlevsno - 1) * Lo.ldc])
+#define ZI(a, c, j)   ((a)[((c) - Lo.begc) + (long)((j) + Lo.nlevsno) * Lo.ldc])
+#define SOI(a, c, j)  ((a)[((c) - Lo.begc) + (long)((j) - 1) * Lo.ldc])
+#define P1(a, p)      ((a)[(p) - Lo.begp])
+#define LYR(a, p, j)  ((a)[((p) - Lo.begp) + (long)((j) + Lo.nlevsno - 1) * Lo.ldp])
+
+/* Fortran max/min: return the first argument when equal (only -0/+0 differ). */
+static double fmaxf90(double a, double b) { return (b > a) ? b : a; }
+static double fminf90(double a, double b) { return (b < a) ? b : a; }
+
+/* ========================================================================= */
+/* LAPACK dgtsv, NRHS = 1 (LAPACK 3.x reference algorithm: Gaussian
+ * elimination with partial pivoting on a tridiagonal system).  Arrays are
+ * 0-based: dl[0..n-2], d[0..n-1], du[0..n-2], b[0..n-1].  Returns INFO. */
+int oracle_dgtsv(int n, double *dl, double *d, double *du, double *b)
+{
+    int i;
+    double fact, temp;
+    if (n < 0) return -1;
+    if (n == 0) return 0;
+    for (i = 0; i < n - 2; ++i) {
+        if (fabs(d[i]) >= fabs(dl[i])) {
+            if (d[i] != 0.0) {
+                fact = dl[i] / d[i];
+                d[i + 1] = d[i + 1] - fact * du[i];
+                b[i + 1] = b[i + 1] - fact * b[i];
+            } else {
+                return i + 1;
+            }
+            dl[i] = 0.0;
+        } else {
+            fact = d[i] / dl[i];
+            d[i] = dl[i];
+            temp = d[i + 1];
+            d[i + 1] = du[i] - fact * temp;
+            dl[i] = du[i + 1];
+            du[i + 1] = -fact * dl[i];
+            du[i] = temp;
+            temp = b[i];
+            b[i] = b[i + 1];
+            b[i + 1] = temp - fact * b[i + 1];
+        }
+    }
+    if (n > 1) {
+        i = n - 2;
+        if (fabs(d[i]) >= fabs(dl[i])) {
+            if (d[i] != 0.0) {
+                fact = dl[i] / d[i];
+                d[i + 1] = d[i + 1] - fact * du[i];
+                b[i + 1] = b[i + 1] - fact * b[i];
+            } else {
+                return i + 1;
+            }
+        } else {
+            fact = d[i] / dl[i];
+            d[i] = dl[i];
+            temp = d[i + 1];
+            d[i + 1] = du[i] - fact * temp;
+            du[i] = temp;
+            temp = b[i];
+            b[i] = b[i + 1];
+            b[i + 1] = temp - fact * b[i + 1];
+        }
+    }
+    if (d[n - 1] == 0.0) return n;
+    /* back substitution */
+    b[n - 1] = b[n - 1] / d[n - 1];
+    if (n > 1) b[n - 2] = (b[n - 2] - du[n - 2] * b[n - 1]) / d[n - 2];
+    for (i = n - 3; i >= 0; --i)
+        b[i] = (b[i] - du[i] * b[i + 1] - dl[i] * b[i + 2]) / d[i];
+    return 0;
+}
+
+/* ========================================================================= */
+/* LAPACK dgbsv = dgbtf2 (unblocked band LU with partial pivoting; dgbtrf
+ * selects it when the block size exceeds KL) followed by dgbtrs (NOTRANS,
+ * NRHS = 1: row swaps + unit-lower solve, then dtbsv upper solve).
+ * ab is column-major (ldab, n), 1-based band row convention
+ * AB(KL+KU+1+i-j, j) = A(i,j) stored at ab[(KL+KU+i-j) + (j-1)*ldab] for
+ * 1-based i, j.  ipiv[] returned 1-based.  Returns INFO. */
+#define AB(r, c) ab[((r) - 1) + (long)((c) - 1) * ldab]   /* 1-based (row, col) */
+int oracle_dgbsv(int n, int kl, int ku, double *ab, int ldab, int *ipiv, double *b)
+{
+    int kv = ku + kl, info = 0, i, j, jp, ju, km, k;
+    if (n < 0 || kl < 0 || ku < 0 || ldab < 2 * kl + ku + 1) return -1;
+    if (n == 0) return 0;
+    /* dgbtf2 */
+    for (j = ku + 2; j <= (kv < n ? kv : n); ++j)
+        for (i = kv - j + 2; i <= kl; ++i) AB(i, j) = 0.0;
+    ju = 1;
+    for (j = 1; j <= n; ++j) {
+        if (j + kv <= n)
+            for (i = 1; i <= kl; ++i) AB(i, j + kv) = 0.0;
+        km = kl < n - j ? kl : n - j;
+        /* IDAMAX over AB(kv+1 .. kv+1+km, j): first index of max |.| */
+        jp = 1;
+        {
+            double dmx = fabs(AB(kv + 1, j));
+            for (i = 2; i <= km + 1; ++i) {
+                double v = fabs(AB(kv + i, j));
+                if (v > dmx) { dmx = v; jp = i; }
+            }
+        }
+        ipiv[j - 1] = jp + j - 1;
+        if (AB(kv + jp, j) != 0.0) {
+            int jlim = j + ku + jp - 1;
+            if (jlim > n) jlim = n;
+            if (jlim > ju) ju = jlim;
+            if (jp != 1) {
+                /* swap rows j and j+jp-1 over columns j..ju */
+                for (k = 0; k <= ju - j; ++k) {
+                    double t = AB(kv + jp - k, j + k);
+                    AB(kv + jp - k, j + k) = AB(kv + 1 - k, j + k);
+                    AB(kv + 1 - k, j + k) = t;
+                }
+            }
+            if (km > 0) {
+                double r = 1.0 / AB(kv + 1, j);
+                for (i = 1; i <= km; ++i) AB(kv + 1 + i, j) = AB(kv + 1 + i, j) * r;
+                /* dger: A(j+i, j+k) += x(i) * (-y(k)),  y(k) = A(j, j+k) */
+                for (k = 1; k <= ju - j; ++k) {
+                    double y = AB(kv + 1 - k, j + k);
+                    if (y != 0.0) {
+                        double temp = -1.0 * y;
+                        for (i = 1; i <= km; ++i)
+                            AB(kv + 1 + i - k, j + k) = AB(kv + 1 + i - k, j + k) + AB(kv + 1 + i, j) * temp;
+                    }
+                }
+            }
+        } else if (info == 0) {
+            info = j;
+        }
+    }
+    if (info != 0) return info;
+    /* dgbtrs: L solve */
+    for (j = 1; j <= n - 1; ++j) {
+        int lm = kl < n - j ? kl : n - j;
+        int l = ipiv[j - 1];
+        if (l != j) { double t = b[l - 1]; b[l - 1] = b[j - 1]; b[j - 1] = t; }
+        if (b[j - 1] != 0.0) {
+            double temp = -1.0 * b[j - 1];
+            for (i = 1; i <= lm; ++i) b[j - 1 + i] = b[j - 1 + i] + AB(kv + 1 + i, j) * temp;
+        }
+    }
+    /* dtbsv Upper, NoTrans, Non-unit, K = kl+ku, column oriented */
+    for (j = n; j >= 1; --j) {
+        if (b[j - 1] != 0.0) {
+            int lo = j - kv > 1 ? j - kv : 1;
+            double temp;
+            b[j - 1] = b[j - 1] / AB(kv + 1, j);
+            temp = b[j - 1];
+            for (i = j - 1; i >= lo; --i) b[i - 1] = b[i - 1] - temp * AB(kv + 1 + i - j, j);
+        }
+    }
+    return 0;
+}
+#undef AB
+
+/* ========================================================================= */
+/* Richards equation, moisture form: SoilWaterMovementMod.F90:976-1440.   */
+static int soilwater_col(const ctsm_config *cfg, lay_t Lo, const ctsm_col_state *S,
+                         int c, double dtime)
+{
+    const int nlayers = C1(S->nbedrock, c);
+    double vwc_liq[NLEVMAX_SOI + 1], dt_dz[NLEVMAX_SOI + 1], hk[NLEVMAX_SOI + 1];
+    double smp[NLEVMAX_SOI + 1], dhkdw[NLEVMAX_SOI + 1], dsmpdw[NLEVMAX_SOI + 1];
+    double imped[NLEVMAX_SOI + 1], qin[NLEVMAX_SOI + 1], qout[NLEVMAX_SOI + 1];
+    double dqidw0[NLEVMAX_SOI + 1], dqidw1[NLEVMAX_SOI + 1], dqodw1[NLEVMAX_SOI + 1];
+    double dqodw2[NLEVMAX_SOI + 1], amx[NLEVMAX_SOI + 1], bmx[NLEVMAX_SOI + 1];
+    double cmx[NLEVMAX_SOI + 1], rmx[NLEVMAX_SOI + 1], dwat[NLEVMAX_SOI + 1];
+    double dLow[NLEVMAX_SOI], diag[NLEVMAX_SOI], dUpp[NLEVMAX_SOI], rhs[NLEVMAX_SOI];
+    double s2[NLEVMAX_SOI + 1];
+    double fluxNet0[NLEVMAX_SOI + 1], fluxNet1[NLEVMAX_SOI + 1];
+    int nsubstep = 0, j;
+    double dtsub = dtime, dtdone = 0.0, errorMax, qcTemp;
+
+    if (nlayers < 2 || nlayers > cfg->nlevsoi || nlayers > NLEVMAX_SOI) return -1;
+    C1(S->qcharge, c) = 0.0;
+
+    for (;;) {
+        nsubstep = nsubstep + 1;
+        for (j = 1; j <= nlayers; ++j) {
+            vwc_liq[j] = fmaxf90(SNO(S->h2osoi_liq, c, j), 1.0e-6) / (SNO(S->dz, c, j) * DENH2O);
+            dt_dz[j] = dtsub / (M_TO_MM * SNO(S->dz, c, j));
+        }
+        /* compute_hydraulic_properties :1444-1556 */
+        for (j = 1; j <= nlayers; ++j) {
+            s2[j] = vwc_liq[j] / SOI(S->watsat, c, j);
+            s2[j] = fminf90(s2[j], 1.0);
+            s2[j] = fmaxf90(0.01, s2[j]);
+        }
+        for (j = 1; j <= nlayers; ++j) {
+            double s1, dhkds, dsmpds, bsw = SOI(S->bsw, c, j);
+            if (j == nlayers) {
+                s1 = s2[j];
+                imped[j] = pow(10.0, -(cfg->e_ice * SOI(S->icefrac, c, j)));
+            } else {
+                s1 = 0.5 * (s2[j] + s2[j + 1]);
+                imped[j] = pow(10.0, -(cfg->e_ice * (0.5 * (SOI(S->icefrac, c, j) + SOI(S->icefrac, c, j + 1)))));
+            }
+            s1 = fminf90(s1, 1.0);
+            s1 = fmaxf90(0.01, s1);
+            /* CH78 soil_hk / soil_suction */
+            hk[j] = imped[j] * SOI(S->hksat, c, j) * pow(s1, 2.0 * bsw + 3.0);
+            dhkds = (2.0 * bsw + 3.0) * hk[j] / s1;
+            smp[j] = -(SOI(S->sucsat, c, j) * pow(s2[j], -bsw));
+            dsmpds = -(bsw * smp[j]) / s2[j];
+            dhkdw[j] = dhkds;
+            dsmpdw[j] = dsmpds / SOI(S->watsat, c, j);
+            SOI(S->smp_l, c, j) = smp[j];
+            SOI(S->hk_l, c, j) = hk[j];
+        }
+        /* compute_moisture_fluxes_and_derivs :1560-1866 */
+        for (j = 1; j <= nlayers; ++j) {
+            qin[j] = qout[j] = dqidw0[j] = dqidw1[j] = dqodw1[j] = dqodw2[j] = 0.0;
+        }
+        qin[1] = C1(S->qflx_infl, c);   /* bc_flux */
+        dqidw1[1] = 0.0;
+        for (j = 1; j <= nlayers - 1; ++j) {
+            double dhkds1, dhkds2, num, den;
+            if (j >= 2) {
+                qin[j] = qout[j - 1];
+                dqidw0[j] = dqodw1[j - 1];
+                dqidw1[j] = dqodw2[j - 1];
+            }
+            dhkds1 = 0.5 * dhkdw[j] / SOI(S->watsat, c, j);
+            dhkds2 = 0.5 * dhkdw[j] / SOI(S->watsat, c, j + 1);
+            num = smp[j + 1] - smp[j];
+            den = M_TO_MM * (SNO(S->z, c, j + 1) - SNO(S->z, c, j));
+            qout[j] = -(hk[j] * num / den) + hk[j];
+            dqodw1[j] = (hk[j] * dsmpdw[j] - dhkds1 * num) / den + dhkds1;
+            dqodw2[j] = (-(hk[j] * dsmpdw[j + 1]) - dhkds2 * num) / den + dhkds2;
+        }
+        j = nlayers;
+        qin[j] = qout[j - 1];
+        dqidw0[j] = dqodw1[j - 1];
+        dqidw1[j] = dqodw2[j - 1];
+        if (cfg->lower_boundary_condition == CTSM_BC_FLUX) {
+            qout[j] = hk[j];
+            dqodw1[j] = dhkdw[j] / SOI(S->watsat, c, j);
+        } else { /* bc_zero_flux */
+            qout[j] = 0.0;
+            dqodw1[j] = 0.0;
+        }
+        /* compute_RHS_moisture_form :1871-1937 */
+        for (j = 1; j <= nlayers; ++j) {
+            double fluxNet = qin[j] - qout[j] - SOI(S->qflx_rootsoi, c, j);
+            rmx[j] = -(fluxNet * dt_dz[j]);
+        }
+        /* compute_LHS_moisture_form :1941-2012 */
+        j = 1;
+        amx[j] = 0.0;
+        bmx[j] = -1.0 - (-dqidw1[j] + dqodw1[j]) * dt_dz[j];
+        cmx[j] = -(dqodw2[j] * dt_dz[j]);
+        for (j = 2; j <= nlayers - 1; ++j) {
+            amx[j] = dqidw0[j] * dt_dz[j];
+            bmx[j] = -1.0 - (-dqidw1[j] + dqodw1[j]) * dt_dz[j];
+            cmx[j] = -(dqodw2[j] * dt_dz[j]);
+        }
+        j = nlayers;
+        amx[j] = dqidw0[j] * dt_dz[j];
+        bmx[j] = -1.0 - (-dqidw1[j] + dqodw1[j]) * dt_dz[j];
+        cmx[j] = 0.0;
+        /* dgtsv :1275-1299 */
+        for (j = 1; j <= nlayers - 1; ++j) {
+            dLow[j - 1] = amx[j + 1];
+            dUpp[j - 1] = cmx[j];
+        }
+        for (j = 1; j <= nlayers; ++j) { diag[j - 1] = bmx[j]; rhs[j - 1] = rmx[j]; }
+        if (oracle_dgtsv(nlayers, dLow, diag, dUpp, rhs) != 0) return -2;
+        for (j = 1; j <= nlayers; ++j) dwat[j] = rhs[j - 1];
+        /* error estimate :1307-1351 */
+        errorMax = 0.0;
+        for (j = 1; j <= nlayers; ++j) {
+            if (cfg->flux_calculation == 42) {
+                double qin_test, qout_test;
+                if (j == 1) qin_test = qin[j] + dqidw1[j] * dwat[j];
+                else        qin_test = qin[j] + dqidw0[j] * dwat[j - 1] + dqidw1[j] * dwat[j];
+                if (j == nlayers) qout_test = qout[j] + dqodw1[j] * dwat[j];
+                else              qout_test = qout[j] + dqodw1[j] * dwat[j] + dqodw2[j] * dwat[j + 1];
+                fluxNet0[j] = qin_test - qout_test - SOI(S->qflx_rootsoi, c, j);
+            } else {
+                fluxNet0[j] = dwat[j] / dt_dz[j];
+            }
+            fluxNet1[j] = qin[j] - qout[j] - SOI(S->qflx_rootsoi, c, j);
+        }
+        for (j = 1; j <= nlayers; ++j) {
+            double e = fabs(fluxNet1[j] - fluxNet0[j]) * dtsub * 0.5;
+            if (j == 1 || e > errorMax) errorMax = e;
+        }
+        if (errorMax > cfg->xTolerUpper && dtsub > cfg->dtmin) {
+            dtsub = fmaxf90(dtsub / 2.0, cfg->dtmin);
+            continue;
+        }
+        for (j = 1; j <= nlayers; ++j)
+            SNO(S->h2osoi_liq, c, j) = SNO(S->h2osoi_liq, c, j) + dwat[j] * (M_TO_MM * SNO(S->dz, c, j));
+        if (cfg->lower_boundary_condition == CTSM_BC_FLUX)
+            qcTemp = hk[nlayers] + dhkdw[nlayers] * dwat[nlayers];
+        else
+            qcTemp = 0.0;
+        C1(S->qcharge, c) = C1(S->qcharge, c) + qcTemp * (dtsub / dtime);
+        dtdone = dtdone + dtsub;
+        if (fabs(dtime - dtdone) < cfg->verySmall) break;
+        if (errorMax < cfg->xTolerLower) dtsub = dtsub * 2.0;
+        dtsub = fminf90(dtsub, dtime - dtdone);
+    }
+    C1(S->num_substeps, c) = (double)nsubstep;
+    /* over-saturation push-up :1405-1409 */
+    for (j = nlayers; j >= 2; --j) {
+        double cap = SOI(S->eff_porosity, c, j) * M_TO_MM * SNO(S->dz, c, j);
+        double over = fmaxf90(SNO(S->h2osoi_liq, c, j) - cap, 0.0);
+        SNO(S->h2osoi_liq, c, j) = fminf90(cap, SNO(S->h2osoi_liq, c, j));
+        SNO(S->h2osoi_liq, c, j - 1) = SNO(S->h2osoi_liq, c, j - 1) + over;
+    }
+    for (j = 1; j <= nlayers; ++j) {
+        SOI(S->qin, c, j) = qin[j];
+        SOI(S->qout, c, j) = qout[j];
+    }
+    return 0;
+}
+
+int oracle_soilwater(const ctsm_config *cfg, const ctsm_layout *L, const ctsm_col_state *S,
+                     int num_filter, const int *filter, double dtime, int *fail_col)
+{
+    lay_t Lo = { L->begc_alloc, L->ldc, L->begp_alloc, L->ldp, cfg->nlevsno };
+    int fc, rc;
+    *fail_col = 0;
+    for (fc = 0; fc < num_filter; ++fc) {
+        int c = filter ? filter[fc] : L->begc_alloc + fc;
+        rc = soilwater_col(cfg, Lo, S, c, dtime);
+        if (rc != 0) { *fail_col = c; return rc; }
+    }
+    return 0;
+}
+
+/* SetSoilWaterFractions, SoilHydrologyMod.F90:240-252 with excess_ice = 0 */
+int oracle_soilwater_fractions(const ctsm_config *cfg, const ctsm_layout *L, const ctsm_col_state *S,
+                               int num_filter, const int *filter)
+{
+    lay_t Lo = { L->begc_alloc, L->ldc, L->begp_alloc, L->ldp, cfg->nlevsno };
+    int fc, j;
+    for (j = 1; j <= cfg->nlevsoi; ++j) {
+        for (fc = 0; fc < num_filter; ++fc) {
+            int c = filter ? filter[fc] : L->begc_alloc + fc;
+            double dz_ext = SNO(S->dz, c, j) + 0.0 / DENICE;
+            double vol_ice = fminf90(SOI(S->watsat, c, j), (SNO(S->h2osoi_ice, c, j) + 0.0) / (dz_ext * DENICE));
+            SOI(S->eff_porosity, c, j) = fmaxf90(0.01, SOI(S->watsat, c, j) - vol_ice);
+            SOI(S->icefrac, c, j) = fminf90(1.0, vol_ice / SOI(S->watsat, c, j));
+        }
+    }
+    return 0;
+}
+
+/* ========================================================================= */
+/* Soil + snow temperature, SoilTemperatureMod.F90:92-508 (non-urban soil
+ * column, use_excess_ice = .false.).  Levels are stored with the reference's
+ * index in [-nlevsno+1, nlevgrnd]; lv(j) maps to the local arrays. */
+#define MAXLEV 128
+static int soiltemp_col(const ctsm_config *cfg, lay_t Lo, const ctsm_col_state *S,
+                        const ctsm_patch_state *P, int c, double dtime)
+{
+    const int nlevsno = cfg->nlevsno, nlevgrnd = cfg->nlevgrnd;
+    const int snl = C1(S->snl, c), nbedrock = C1(S->nbedrock, c);
+    const double frac_sno = C1(S->frac_sno_eff, c), frac_h2osfc = C1(S->frac_h2osfc, c);
+    const double emg = C1(S->emg, c), forc_lwrad = C1(S->forc_lwrad, c), htvp = C1(S->htvp, c);
+    /* local arrays indexed [j + OFF], j in -nlevsno .. nlevgrnd */
+    const int OFF = nlevsno;
+    double tk[MAXLEV], cv[MAXLEV], fn[MAXLEV], fn1[MAXLEV], thk[MAXLEV], fact[MAXLEV];
+    double sabg_lyr_col[MAXLEV];
+    double bm[6][MAXLEV], rv[MAXLEV], tv[MAXLEV];
+    double hs_soil, hs_h2osfc, hs_top, hs_top_snow, dhsdT, tk_h2osfc, c_h2osfc, dz_h2osfc;
+    double lwrad_emit, dlwrad_emit, lwrad_emit_snow, lwrad_emit_soil, lwrad_emit_h2osfc;
+    int j, p, jtop = snl, jbot = nlevgrnd;
+#define T(jj)    SNO(S->t_soisno, c, jj)
+#define LIQ(jj)  SNO(S->h2osoi_liq, c, jj)
+#define ICE(jj)  SNO(S->h2osoi_ice, c, jj)
+#define DZ(jj)   SNO(S->dz, c, jj)
+#define Z(jj)    SNO(S->z, c, jj)
+#define ZIv(jj)  ZI(S->zi, c, jj)
+#define A(arr, jj) arr[(jj) + OFF]
+
+    if (nlevsno + nlevgrnd + 2 > MAXLEV || snl < -nlevsno || snl > 0) return -1;
+    for (j = -nlevsno; j <= nlevgrnd; ++j) {
+        A(tk, j) = A(cv, j) = A(fn, j) = A(fn1, j) = A(thk, j) = A(fact, j) = NAN;
+        A(sabg_lyr_col, j) = 0.0;
+        for (p = 0; p < 6; ++p) bm[p][j + OFF] = 0.0;
+        A(rv, j) = A(tv, j) = NAN;
+    }
+
+    /* ---- SoilThermProp :602-901 ---- */
+    for (j = 1; j <= nlevgrnd; ++j) {
+        double satw, dke, fl, dksat, th;
+        satw = (LIQ(j) / DENH2O + ICE(j) / DENICE + 0.0 / DENICE) / (DZ(j) * SOI(S->watsat, c, j));
+        satw = fminf90(1.0, satw);
+        if (satw > .1e-6) {
+            if (T(j) >= TFRZ) dke = fmaxf90(0.0, log10(satw) + 1.0);
+            else              dke = satw;
+            fl = (LIQ(j) / (DENH2O * DZ(j))) /
+                 (LIQ(j) / (DENH2O * DZ(j)) + ICE(j) / (DENICE * DZ(j)) + 0.0 / (DENICE * DZ(j)));
+            dksat = SOI(S->tkmg, c, j) * pow(TKWAT, fl * SOI(S->watsat, c, j)) *
+                    pow(TKICE, (1.0 - fl) * SOI(S->watsat, c, j));
+            th = dke * dksat + (1.0 - dke) * SOI(S->tkdry, c, j);
+        } else {
+            th = SOI(S->tkdry, c, j);
+        }
+        if (j > nbedrock) th = THK_BEDROCK;
+        A(thk, j) = th;
+    }
+    if (snl + 1 < 1) {
+        for (j = snl + 1; j <= 0; ++j) {
+            double bw = (ICE(j) + LIQ(j)) / (frac_sno * DZ(j));
+            if (S->bw) SNO(S->bw, c, j) = bw;
+            if (cfg->snow_thermal_cond_method == CTSM_SNOWTHK_JORDAN1991) {
+                A(thk, j) = TKAIR + (7.75e-5 * bw + 1.105e-6 * bw * bw) * (TKICE - TKAIR);
+            } else {
+                if (bw <= 156.0)
+                    A(thk, j) = (double)0.023f + (double)0.234f * (bw / 1000.0);
+                else
+                    A(thk, j) = (double)0.138f - (double)1.01f * (bw / 1000.0) +
+                                ((double)3.233f * ((bw / 1000.0) * (bw / 1000.0)));
+            }
+        }
+    }
+    for (j = snl + 1; j <= nlevgrnd; ++j) {
+        if (j <= nlevgrnd - 1)
+            A(tk, j) = A(thk, j) * A(thk, j + 1) * (Z(j + 1) - Z(j)) /
+                       (A(thk, j) * (Z(j + 1) - ZIv(j)) + A(thk, j + 1) * (ZIv(j) - Z(j)));
+        else
+            A(tk, j) = 0.0;
+    }
+    {
+        double zh2osfc = (double)1.0e-3f * (0.5 * C1(S->h2osfc, c));
+        tk_h2osfc = TKWAT * A(thk, 1) * (Z(1) + zh2osfc) / (TKWAT * Z(1) + A(thk, 1) * zh2osfc);
+    }
+    for (j = 1; j <= nlevgrnd; ++j) {
+        A(cv, j) = SOI(S->csol, c, j) * (1.0 - SOI(S->watsat, c, j)) * DZ(j) +
+                   (ICE(j) * CPICE + LIQ(j) * CPLIQ) + 0.0 * CPICE;
+        if (j > nbedrock) A(cv, j) = CSOL_BEDROCK * DZ(j);
+    }
+    if (C1(S->h2osno_no_layers, c) > 0.0) A(cv, 1) = A(cv, 1) + CPICE * C1(S->h2osno_no_layers, c);
+    if (snl + 1 < 1) {
+        for (j = snl + 1; j <= 0; ++j) {
+            if (frac_sno > 0.0) A(cv, j) = fmaxf90(THIN_SFCLAYER, (CPLIQ * LIQ(j) + CPICE * ICE(j)) / frac_sno);
+            else                A(cv, j) = THIN_SFCLAYER;
+        }
+    }
+    for (j = snl + 1; j <= nlevgrnd; ++j) SNO(S->thk, c, j) = A(thk, j);
+
+    /* ---- ComputeGroundHeatFluxAndDeriv :1543-1796 (non-urban patches) ---- */
+    {
+        double tg = C1(S->t_grnd, c), ts = T(snl + 1), t1 = T(1), th = C1(S->t_h2osfc, c);
+        double tg2 = tg * tg, ts2 = ts * ts, t12 = t1 * t1, th2 = th * th;
+        lwrad_emit = emg * SB * (tg2 * tg2);
+        dlwrad_emit = 4.0 * emg * SB * (tg2 * tg);
+        lwrad_emit_snow = emg * SB * (ts2 * ts2);
+        lwrad_emit_soil = emg * SB * (t12 * t12);
+        lwrad_emit_h2osfc = emg * SB * (th2 * th2);
+    }
+    hs_soil = hs_h2osfc = dhsdT = 0.0;
+    hs_top = hs_top_snow = 0.0;
+    {
+        double hs = 0.0;
+        int pi = C1(S->patchi, c), pf = C1(S->patchf, c);
+        for (p = pi; p <= pf; ++p) {
+            double fveg = (double)P1(P->frac_veg_nosno, p), wt = P1(P->wtcol, p);
+            double lwin = (1.0 - fveg) * emg * forc_lwrad;
+            double eg, egsnow, egsoil, egh2o;
+            eg = P1(P->sabg, p) + P1(P->dlrad, p) + lwin - lwrad_emit -
+                 (P1(P->eflx_sh_grnd, p) + P1(P->qflx_evap_soi, p) * htvp);
+            P1(P->eflx_gnet, p) = eg;
+            P1(P->sabg_chk, p) = frac_sno * P1(P->sabg_snow, p) + (1.0 - frac_sno) * P1(P->sabg_soil, p);
+            egsnow = P1(P->sabg_snow, p) + P1(P->dlrad, p) + lwin - lwrad_emit_snow -
+                     (P1(P->eflx_sh_snow, p) + P1(P->qflx_ev_snow, p) * htvp);
+            egsoil = P1(P->sabg_soil, p) + P1(P->dlrad, p) + lwin - lwrad_emit_soil -
+                     (P1(P->eflx_sh_soil, p) + P1(P->qflx_ev_soil, p) * htvp);
+            egh2o = P1(P->sabg_soil, p) + P1(P->dlrad, p) + lwin - lwrad_emit_h2osfc -
+                    (P1(P->eflx_sh_h2osfc, p) + P1(P->qflx_ev_h2osfc, p) * htvp);
+            (void)egsnow;
+            P1(P->dgnetdT, p) = -P1(P->cgrnd, p) - dlwrad_emit;
+            hs = hs + eg * wt;
+            dhsdT = dhsdT + P1(P->dgnetdT, p) * wt;
+            hs_soil = hs_soil + egsoil * wt;
+            hs_h2osfc = hs_h2osfc + egh2o * wt;
+        }
+        (void)hs;
+        for (p = pi; p <= pf; ++p) {
+            double fveg = (double)P1(P->frac_veg_nosno, p), wt = P1(P->wtcol, p);
+            double lwin = (1.0 - fveg) * emg * forc_lwrad;
+            int lyr_top = snl + 1;
+            double sl = LYR(P->sabg_lyr, p, lyr_top);
+            double egtop = sl + P1(P->dlrad, p) + lwin - lwrad_emit -
+                           (P1(P->eflx_sh_grnd, p) + P1(P->qflx_evap_soi, p) * htvp);
+            double egsnow = sl + P1(P->dlrad, p) + lwin - lwrad_emit_snow -
+                            (P1(P->eflx_sh_snow, p) + P1(P->qflx_ev_snow, p) * htvp);
+            hs_top = hs_top + egtop * wt;
+            hs_top_snow = hs_top_snow + egsnow * wt;
+            for (j = lyr_top; j <= 1; ++j)
+                A(sabg_lyr_col, j) = A(sabg_lyr_col, j) + LYR(P->sabg_lyr, p, j) * wt;
+        }
+    }
+    (void)hs_top;
+
+    /* ---- ComputeHeatDiffFluxAndFactor :1799-1910 ---- */
+    for (j = snl + 1; j <= nlevgrnd; ++j) {
+        if (j == snl + 1) {
+            A(fact, j) = dtime / A(cv, j) * DZ(j) /
+                         (0.5 * (Z(j) - ZIv(j - 1) + CAPR * (Z(j + 1) - ZIv(j - 1))));
+            A(fn, j) = A(tk, j) * (T(j + 1) - T(j)) / (Z(j + 1) - Z(j));
+        } else if (j <= nlevgrnd - 1) {
+            A(fact, j) = dtime / A(cv, j);
+            A(fn, j) = A(tk, j) * (T(j + 1) - T(j)) / (Z(j + 1) - Z(j));
+        } else {
+            A(fact, j) = dtime / A(cv, j);
+            A(fn, j) = C1(S->eflx_bot, c);
+        }
+    }
+    /* h2osfc thermal properties :257-266 */
+    if (C1(S->h2osfc, c) > THIN_SFCLAYER && frac_h2osfc > THIN_SFCLAYER) {
+        c_h2osfc = fmaxf90(THIN_SFCLAYER, CPLIQ * C1(S->h2osfc, c) / frac_h2osfc);
+        dz_h2osfc = fmaxf90(THIN_SFCLAYER, (double)1.0e-3f * C1(S->h2osfc, c) / frac_h2osfc);
+    } else {
+        c_h2osfc = THIN_SFCLAYER;
+        dz_h2osfc = THIN_SFCLAYER;
+    }
+    C1(S->c_h2osfc, c) = c_h2osfc;
+
+    /* ---- SetRHSVec :1913-2353 ---- */
+    {
+        double dzm, fn_h2osfc;
+        for (j = snl + 1; j <= 0; ++j) {
+            if (j == snl + 1)
+                A(rv, j - 1) = T(j) + A(fact, j) * (hs_top_snow - dhsdT * T(j) + CNFAC * A(fn, j));
+            else {
+                A(rv, j - 1) = T(j) + CNFAC * A(fact, j) * (A(fn, j) - A(fn, j - 1));
+                A(rv, j - 1) = A(rv, j - 1) + A(fact, j) * A(sabg_lyr_col, j);
+            }
+        }
+        dzm = (0.5 * dz_h2osfc + Z(1));
+        fn_h2osfc = tk_h2osfc * (T(1) - C1(S->t_h2osfc, c)) / dzm;
+        A(rv, 0) = C1(S->t_h2osfc, c) + (dtime / c_h2osfc) *
+                   (hs_h2osfc - dhsdT * C1(S->t_h2osfc, c) + CNFAC * fn_h2osfc);
+        for (j = 1; j <= nlevgrnd; ++j) {
+            if (j == snl + 1)
+                A(rv, j) = T(j) + A(fact, j) * (hs_top_snow - dhsdT * T(j) + CNFAC * A(fn, j));
+            else if (j == 1) {
+                A(rv, j) = T(j) + A(fact, j) * ((1.0 - frac_sno) * (hs_soil - dhsdT * T(j)) +
+                                               CNFAC * (A(fn, j) - frac_sno * A(fn, j - 1)));
+                A(rv, j) = A(rv, j) + frac_sno * A(fact, j) * A(sabg_lyr_col, j);
+            } else if (j <= nlevgrnd - 1)
+                A(rv, j) = T(j) + CNFAC * A(fact, j) * (A(fn, j) - A(fn, j - 1));
+            else
+                A(rv, j) = T(j) - CNFAC * A(fact, j) * A(fn, j - 1) + A(fact, j) * A(fn, j);
+        }
+        if (frac_h2osfc != 0.0)
+            A(rv, 1) = A(rv, 1) - frac_h2osfc * A(fact, 1) *
+                       ((hs_soil - dhsdT * T(1)) + CNFAC * fn_h2osfc);
+    }
+
+    /* ---- SetMatrix :2356-2926 (bands 1..5 as the reference's bmatrix) ---- */
+    {
+        double dzm, dzp;
+        for (j = snl + 1; j <= 0; ++j) {
+            dzp = Z(j + 1) - Z(j);
+            if (j == snl + 1) {
+                bm[4][j - 1 + OFF] = 0.0;
+                bm[3][j - 1 + OFF] = 1.0 + (1.0 - CNFAC) * A(fact, j) * A(tk, j) / dzp - A(fact, j) * dhsdT;
+            } else {
+                dzm = (Z(j) - Z(j - 1));
+                bm[4][j - 1 + OFF] = -((1.0 - CNFAC) * A(fact, j) * A(tk, j - 1) / dzm);
+                bm[3][j - 1 + OFF] = 1.0 + (1.0 - CNFAC) * A(fact, j) * (A(tk, j) / dzp + A(tk, j - 1) / dzm);
+            }
+            if (j != 0) bm[2][j - 1 + OFF] = -((1.0 - CNFAC) * A(fact, j) * A(tk, j) / dzp);
+            else        bm[1][j - 1 + OFF] = -((1.0 - CNFAC) * A(fact, j) * A(tk, j) / dzp);
+        }
+        for (j = 1; j <= nlevgrnd; ++j) {
+            if (j == snl + 1) {
+                dzp = Z(j + 1) - Z(j);
+                if (j != 1) bm[4][j + OFF] = 0.0; else bm[5][j + OFF] = 0.0;
+                bm[3][j + OFF] = 1.0 + (1.0 - CNFAC) * A(fact, j) * A(tk, j) / dzp - A(fact, j) * dhsdT;
+                bm[2][j + OFF] = -((1.0 - CNFAC) * A(fact, j) * A(tk, j) / dzp);
+            } else if (j == 1) {
+                dzm = (Z(j) - Z(j - 1));
+                dzp = (Z(j + 1) - Z(j));
+                bm[2][j + OFF] = -((1.0 - CNFAC) * A(fact, j) * A(tk, j) / dzp);
+                bm[3][j + OFF] = 1.0 + (1.0 - CNFAC) * A(fact, j) * (A(tk, j) / dzp + frac_sno * A(tk, j - 1) / dzm) -
+                                 (1.0 - frac_sno) * A(fact, j) * dhsdT;
+                bm[5][j + OFF] = -(frac_sno * (1.0 - CNFAC) * A(fact, j) * A(tk, j - 1) / dzm);
+            } else if (j <= nlevgrnd - 1) {
+                dzm = (Z(j) - Z(j - 1));
+                dzp = (Z(j + 1) - Z(j));
+                bm[2][j + OFF] = -((1.0 - CNFAC) * A(fact, j) * A(tk, j) / dzp);
+                bm[3][j + OFF] = 1.0 + (1.0 - CNFAC) * A(fact, j) * (A(tk, j) / dzp + A(tk, j - 1) / dzm);
+                bm[4][j + OFF] = -((1.0 - CNFAC) * A(fact, j) * A(tk, j - 1) / dzm);
+            } else {
+                dzm = (Z(j) - Z(j - 1));
+                bm[2][j + OFF] = 0.0;
+                bm[3][j + OFF] = 1.0 + (1.0 - CNFAC) * A(fact, j) * A(tk, j - 1) / dzm;
+                bm[4][j + OFF] = -((1.0 - CNFAC) * A(fact, j) * A(tk, j - 1) / dzm);
+            }
+        }
+        dzm = (0.5 * dz_h2osfc + Z(1));
+        if (frac_h2osfc != 0.0)
+            bm[3][1 + OFF] = bm[3][1 + OFF] + frac_h2osfc *
+                             ((1.0 - CNFAC) * A(fact, 1) * tk_h2osfc / dzm + A(fact, 1) * dhsdT);
+        /* SetMatrix_StandingSurfaceWater :2861-2926 */
+        bm[3][0 + OFF] = 1.0 + (1.0 - CNFAC) * (dtime / c_h2osfc) * tk_h2osfc / dzm - (dtime / c_h2osfc) * dhsdT;
+        bm[2][0 + OFF] = -((1.0 - CNFAC) * (dtime / c_h2osfc) * tk_h2osfc / dzm);
+        bm[4][0 + OFF] = 0.0;  /* assembly copies only bands 2,3 of row 0 */
+        bm[4][1 + OFF] = (frac_h2osfc != 0.0)
+                         ? -(frac_h2osfc * (1.0 - CNFAC) * A(fact, 1) * tk_h2osfc / dzm) : 0.0;
+        /* AssembleMatrixFromSubmatrices :2440-2480: the bottom snow row keeps
+         * bands 3,4 (+ band 1 from the snow-soil block), band 2 of it is 0. */
+        if (snl < 0) bm[2][-1 + OFF] = 0.0;
+        /* soil row 1 band 5 exists only through the soil-snow block */
+        if (snl == 0) bm[5][1 + OFF] = 0.0;
+    }
+
+    /* tvector :305-318 */
+    for (j = snl + 1; j <= 0; ++j) A(tv, j - 1) = T(j);
+    A(tv, 0) = C1(S->t_h2osfc, c);
+    for (j = 1; j <= nlevgrnd; ++j) A(tv, j) = T(j);
+
+    /* ---- BandDiagonal BandDiagonalMod.F90:29-221 ---- */
+    {
+        const int kl = 2, ku = 2, m = 2 * kl + ku + 1, n = jbot - jtop + 1;
+        double ab[7 * MAXLEV], result[MAXLEV];
+        int ipiv[MAXLEV], k, info;
+        for (k = 0; k < m * n; ++k) ab[k] = 0.0;
+#define ABm(r, cc) ab[((r) - 1) + ((cc) - 1) * m]
+        for (k = 3; k <= n; ++k) ABm(kl + ku - 1, k) = bm[1][jtop + k - 3 + OFF];
+        for (k = 2; k <= n; ++k) ABm(kl + ku + 0, k) = bm[2][jtop + k - 2 + OFF];
+        for (k = 1; k <= n; ++k) ABm(kl + ku + 1, k) = bm[3][jtop + k - 1 + OFF];
+        for (k = 1; k <= n - 1; ++k) ABm(kl + ku + 2, k) = bm[4][jtop + k + OFF];
+        for (k = 1; k <= n - 2; ++k) ABm(kl + ku + 3, k) = bm[5][jtop + k + 1 + OFF];
+#undef ABm
+        for (k = 0; k < n; ++k) result[k] = A(rv, jtop + k);
+        info = oracle_dgbsv(n, kl, ku, ab, m, ipiv, result);
+        if (info != 0) return -2;
+        for (k = 0; k < n; ++k) A(tv, jtop + k) = result[k];
+    }
+    /* unpack :331-343 */
+    for (j = snl + 1; j <= 0; ++j) T(j) = A(tv, j - 1);
+    for (j = 1; j <= nlevgrnd; ++j) T(j) = A(tv, j);
+    if (frac_h2osfc == 0.0) C1(S->t_h2osfc, c) = T(1);
+    else                    C1(S->t_h2osfc, c) = A(tv, 0);
+
+    /* fn1 :347-392 (non-urban) */
+    for (j = snl + 1; j <= nlevgrnd; ++j) {
+        if (j <= nlevgrnd - 1) A(fn1, j) = A(tk, j) * (T(j + 1) - T(j)) / (Z(j + 1) - Z(j));
+        else                   A(fn1, j) = 0.0;
+    }
+
+    /* ---- PhaseChangeH2osfc :904-1130 ---- */
+    C1(S->xmf_h2osfc, c) = 0.0;
+    C1(S->qflx_h2osfc_to_ice, c) = 0.0;
+    C1(S->eflx_h2osfc_to_snow, c) = 0.0;
+    {
+        double h2osno_total = C1(S->h2osno_no_layers, c);
+        for (j = snl + 1; j <= 0; ++j) h2osno_total = h2osno_total + ICE(j) + LIQ(j);
+        if (frac_h2osfc > 0.0 && C1(S->t_h2osfc, c) <= TFRZ) {
+            double tinc = TFRZ - C1(S->t_h2osfc, c), hm, xm, temp1, z_avg, rho_avg, c1, c2;
+            C1(S->t_h2osfc, c) = TFRZ;
+            hm = frac_h2osfc * (dhsdT * tinc - tinc * c_h2osfc / dtime);
+            xm = hm * dtime / HFUS;
+            temp1 = C1(S->h2osfc, c) + xm;
+            z_avg = frac_sno * C1(S->snow_depth, c);
+            if (z_avg > 0.0) rho_avg = fminf90(800.0, h2osno_total / z_avg);
+            else             rho_avg = 200.0;
+            if (temp1 >= 0.0) {
+                C1(S->int_snow, c) = C1(S->int_snow, c) - xm;
+                if (snl == 0) C1(S->h2osno_no_layers, c) = C1(S->h2osno_no_layers, c) - xm;
+                else          ICE(0) = ICE(0) - xm;
+                h2osno_total = h2osno_total - xm;
+                C1(S->h2osfc, c) = C1(S->h2osfc, c) + xm;
+                C1(S->xmf_h2osfc, c) = hm;
+                C1(S->qflx_h2osfc_to_ice, c) = -xm / dtime;
+                if (frac_sno > 0 && snl < 0) C1(S->snow_depth, c) = h2osno_total / (rho_avg * frac_sno);
+                else                         C1(S->snow_depth, c) = h2osno_total / DENICE;
+                if (snl == 0) {
+                    T(0) = C1(S->t_h2osfc, c);
+                    C1(S->eflx_h2osfc_to_snow, c) = 0.0;
+                } else {
+                    if (snl == -1) c1 = frac_sno * (dtime / A(fact, 0) - dhsdT * dtime);
+                    else           c1 = frac_sno / A(fact, 0) * dtime;
+                    if (frac_h2osfc != 0.0) c2 = (-CPLIQ * xm - frac_h2osfc * dhsdT * dtime);
+                    else                    c2 = 0.0;
+                    T(0) = (c1 * T(0) + c2 * C1(S->t_h2osfc, c)) / (c1 + c2);
+                    C1(S->eflx_h2osfc_to_snow, c) = (C1(S->t_h2osfc, c) - T(0)) * c2 / dtime;
+                }
+            } else {
+                rho_avg = (h2osno_total * rho_avg + C1(S->h2osfc, c) * DENICE) / (h2osno_total + C1(S->h2osfc, c));
+                C1(S->int_snow, c) = C1(S->int_snow, c) + C1(S->h2osfc, c);
+                if (snl == 0) C1(S->h2osno_no_layers, c) = C1(S->h2osno_no_layers, c) + C1(S->h2osfc, c);
+                else          ICE(0) = ICE(0) + C1(S->h2osfc, c);
+                h2osno_total = h2osno_total + C1(S->h2osfc, c);
+                C1(S->qflx_h2osfc_to_ice, c) = C1(S->h2osfc, c) / dtime;
+                C1(S->t_h2osfc, c) = C1(S->t_h2osfc, c) - temp1 * HFUS / (dtime * dhsdT - c_h2osfc);
+                C1(S->xmf_h2osfc, c) = (hm - frac_h2osfc * temp1 * HFUS / dtime);
+                if (snl == 0) {
+                    T(0) = C1(S->t_h2osfc, c);
+                } else {
+                    if (snl == -1) c1 = frac_sno * (dtime / A(fact, 0) - dhsdT * dtime);
+                    else           c1 = frac_sno / A(fact, 0) * dtime;
+                    if (frac_h2osfc != 0.0) c2 = frac_h2osfc * (c_h2osfc - dtime * dhsdT);
+                    else                    c2 = 0.0;
+                    T(0) = (c1 * T(0) + c2 * C1(S->t_h2osfc, c)) / (c1 + c2);
+                    C1(S->t_h2osfc, c) = T(0);
+                }
+                C1(S->h2osfc, c) = 0.0;
+                if (frac_sno > 0 && snl < 0) C1(S->snow_depth, c) = h2osno_total / (rho_avg * frac_sno);
+                else                         C1(S->snow_depth, c) = h2osno_total / DENICE;
+            }
+        }
+    }
+
+    /* ---- Phasechange :1133-1540 (excess_ice = 0) ---- */
+    {
+        double hm[MAXLEV], xm[MAXLEV], wmass0[MAXLEV], wice0[MAXLEV], wliq0[MAXLEV];
+        double supercool[MAXLEV], tinc[MAXLEV];
+        double xmf = 0.0, qflx_snomelt = 0.0, qflx_snofrz = 0.0, qflx_snow_drain = 0.0;
+        int imelt[MAXLEV];
+        for (j = -nlevsno + 1; j <= nlevgrnd; ++j) {
+            A(imelt, j) = 0; A(hm, j) = 0.0; A(xm, j) = 0.0; A(tinc, j) = 0.0; A(supercool, j) = 0.0;
+            if (j >= snl + 1) {
+                A(wice0, j) = ICE(j);
+                A(wliq0, j) = LIQ(j);
+                A(wmass0, j) = ICE(j) + LIQ(j) + 0.0;
+            }
+            if (j <= 0) {
+                if (S->qflx_snomelt_lyr) SNO(S->qflx_snomelt_lyr, c, j) = 0.0;
+                if (S->qflx_snofrz_lyr) SNO(S->qflx_snofrz_lyr, c, j) = 0.0;
+            }
+        }
+        for (j = snl + 1; j <= 0; ++j) {
+            if (ICE(j) > 0.0 && T(j) > TFRZ) {
+                A(imelt, j) = 1; A(tinc, j) = TFRZ - T(j); T(j) = TFRZ;
+            }
+            if (LIQ(j) > 0.0 && T(j) < TFRZ) {
+                A(imelt, j) = 2; A(tinc, j) = TFRZ - T(j); T(j) = TFRZ;
+            }
+        }
+        for (j = 1; j <= nlevgrnd; ++j) {
+            A(supercool, j) = 0.0;
+            if (ICE(j) > 0.0 && T(j) > TFRZ) {
+                A(imelt, j) = 1; A(tinc, j) = TFRZ - T(j); T(j) = TFRZ;
+            }
+            A(supercool, j) = 0.0;
+            if (T(j) < TFRZ) {
+                double smp = HFUS * (TFRZ - T(j)) / (GRAV * T(j)) * 1000.0;
+                A(supercool, j) = SOI(S->watsat, c, j) * pow(smp / SOI(S->sucsat, c, j), -1.0 / SOI(S->bsw, c, j));
+                A(supercool, j) = A(supercool, j) * DZ(j) * 1000.0;
+            }
+            if (LIQ(j) > A(supercool, j) && T(j) < TFRZ) {
+                A(imelt, j) = 2; A(tinc, j) = TFRZ - T(j); T(j) = TFRZ;
+            }
+            if (C1(S->h2osno_no_layers, c) > 0.0 && j == 1) {
+                if (T(j) > TFRZ) {
+                    A(imelt, j) = 1; A(tinc, j) = TFRZ - T(j); T(j) = TFRZ;
+                }
+            }
+        }
+        for (j = snl + 1; j <= nlevgrnd; ++j) {
+            if (A(imelt, j) > 0) {
+                if (j == snl + 1) {
+                    if (j > 0) A(hm, j) = dhsdT * A(tinc, j) - A(tinc, j) / A(fact, j);
+                    else       A(hm, j) = frac_sno * (dhsdT * A(tinc, j) - A(tinc, j) / A(fact, j));
+                    if (j == 1 && frac_h2osfc != 0.0)
+                        A(hm, j) = A(hm, j) - frac_h2osfc * (dhsdT * A(tinc, j));
+                } else if (j == 1) {
+                    A(hm, j) = (1.0 - frac_sno - frac_h2osfc) * dhsdT * A(tinc, j) - A(tinc, j) / A(fact, j);
+                } else {
+                    if (j < 1) A(hm, j) = -(frac_sno * (A(tinc, j) / A(fact, j)));
+                    else       A(hm, j) = -(A(tinc, j) / A(fact, j));
+                }
+            }
+            if (A(imelt, j) == 1 && A(hm, j) < 0.0) { A(hm, j) = 0.0; A(imelt, j) = 0; }
+            if (A(imelt, j) == 2 && A(hm, j) > 0.0) { A(hm, j) = 0.0; A(imelt, j) = 0; }
+            if (A(imelt, j) > 0 && fabs(A(hm, j)) > 0.0) {
+                double heatr, temp1, propor;
+                A(xm, j) = A(hm, j) * dtime / HFUS;
+                if (j == 1) {
+                    if (C1(S->h2osno_no_layers, c) > 0.0 && A(xm, j) > 0.0) {
+                        temp1 = C1(S->h2osno_no_layers, c);
+                        C1(S->h2osno_no_layers, c) = fmaxf90(0.0, temp1 - A(xm, j));
+                        propor = C1(S->h2osno_no_layers, c) / temp1;
+                        C1(S->snow_depth, c) = propor * C1(S->snow_depth, c);
+                        heatr = A(hm, j) - HFUS * (temp1 - C1(S->h2osno_no_layers, c)) / dtime;
+                        if (heatr > 0.0) {
+                            A(xm, j) = heatr * dtime / HFUS;
+                            A(hm, j) = heatr;
+                        } else {
+                            A(xm, j) = 0.0;
+                            A(hm, j) = 0.0;
+                        }
+                        qflx_snomelt = fmaxf90(0.0, (temp1 - C1(S->h2osno_no_layers, c))) / dtime;
+                        xmf = HFUS * qflx_snomelt;
+                        qflx_snow_drain = qflx_snomelt;
+                    }
+                }
+                heatr = 0.0;
+                if (A(xm, j) > 0.0) {
+                    ICE(j) = fmaxf90(0.0, A(wice0, j) - A(xm, j));
+                    /* xm2 = xm - wice0 would melt excess ice; none here (excess_ice = 0) */
+                    if (j >= 1)
+                        heatr = A(hm, j) - HFUS * (0.0 - 0.0 + A(wice0, j) - ICE(j)) / dtime;
+                    else
+                        heatr = A(hm, j) - HFUS * (A(wice0, j) - ICE(j)) / dtime;
+                } else if (A(xm, j) < 0.0) {
+                    if (j <= 0) {
+                        ICE(j) = fminf90(A(wmass0, j), A(wice0, j) - A(xm, j));
+                    } else {
+                        if (A(wmass0, j) - 0.0 < A(supercool, j))
+                            ICE(j) = 0.0;
+                        else
+                            ICE(j) = fminf90(A(wmass0, j) - 0.0 - A(supercool, j), A(wice0, j) - A(xm, j));
+                    }
+                    heatr = A(hm, j) - HFUS * (A(wice0, j) - ICE(j)) / dtime;
+                }
+                LIQ(j) = fmaxf90(0.0, A(wmass0, j) - ICE(j) - 0.0);
+                if (fabs(heatr) > 0.0) {
+                    if (j == snl + 1) {
+                        if (j == 1)
+                            T(j) = T(j) + A(fact, j) * heatr / (1.0 - (1.0 - frac_h2osfc) * A(fact, j) * dhsdT);
+                        else
+                            T(j) = T(j) + (A(fact, j) / frac_sno) * heatr / (1.0 - A(fact, j) * dhsdT);
+                    } else if (j == 1) {
+                        T(j) = T(j) + A(fact, j) * heatr / (1.0 - (1.0 - frac_sno - frac_h2osfc) * A(fact, j) * dhsdT);
+                    } else {
+                        if (j > 0) T(j) = T(j) + A(fact, j) * heatr;
+                        else if (frac_sno > 0.0) T(j) = T(j) + (A(fact, j) / frac_sno) * heatr;
+                    }
+                    if (j <= 0) {
+                        if (LIQ(j) * ICE(j) > 0.0) T(j) = TFRZ;
+                    }
+                }
+                if (j >= 1)
+                    xmf = xmf + HFUS * (A(wice0, j) - ICE(j)) / dtime + HFUS * (0.0 - 0.0) / dtime;
+                else
+                    xmf = xmf + HFUS * (A(wice0, j) - ICE(j)) / dtime;
+                if (A(imelt, j) == 1 && j < 1) {
+                    double ml = fmaxf90(0.0, (A(wice0, j) - ICE(j))) / dtime;
+                    if (S->qflx_snomelt_lyr) SNO(S->qflx_snomelt_lyr, c, j) = ml;
+                    qflx_snomelt = qflx_snomelt + ml;
+                    if (S->snomelt_accum) C1(S->snomelt_accum, c) = C1(S->snomelt_accum, c) + ml * dtime * 1.e-3;
+                }
+                if (A(imelt, j) == 2 && j < 1) {
+                    double fz = fmaxf90(0.0, (ICE(j) - A(wice0, j))) / dtime;
+                    if (S->qflx_snofrz_lyr) SNO(S->qflx_snofrz_lyr, c, j) = fz;
+                    qflx_snofrz = qflx_snofrz + fz;
+                }
+            }
+        }
+        for (j = snl + 1; j <= nlevgrnd; ++j) SNO(S->imelt, c, j) = A(imelt, j);
+        C1(S->xmf, c) = xmf;
+        C1(S->qflx_snomelt, c) = qflx_snomelt;
+        C1(S->qflx_snofrz, c) = qflx_snofrz;
+        C1(S->qflx_snow_drain, c) = qflx_snow_drain;
+        C1(S->eflx_snomelt, c) = qflx_snomelt * HFUS;
+    }
+
+    /* t_grnd :457-477 */
+    if (snl < 0) {
+        if (frac_h2osfc != 0.0)
+            C1(S->t_grnd, c) = frac_sno * T(snl + 1) + (1.0 - frac_sno - frac_h2osfc) * T(1) +
+                               frac_h2osfc * C1(S->t_h2osfc, c);
+        else
+            C1(S->t_grnd, c) = frac_sno * T(snl + 1) + (1.0 - frac_sno) * T(1);
+    } else {
+        if (frac_h2osfc != 0.0)
+            C1(S->t_grnd, c) = (1.0 - frac_h2osfc) * T(1) + frac_h2osfc * C1(S->t_h2osfc, c);
+        else
+            C1(S->t_grnd, c) = T(1);
+    }
+    /* eflx_fgr12 / eflx_fgr :481-504 */
+    C1(S->eflx_fgr12, c) = -(CNFAC * A(fn, 1)) - (1.0 - CNFAC) * A(fn1, 1);
+    for (j = 1; j <= nlevgrnd; ++j) {
+        if (j < nlevgrnd) SOI(S->eflx_fgr, c, j) = -(CNFAC * A(fn, j)) - (1.0 - CNFAC) * A(fn1, j);
+        else              SOI(S->eflx_fgr, c, j) = 0.0;
+    }
+    for (j = snl + 1; j <= nlevgrnd; ++j) SNO(S->fact, c, j) = A(fact, j);
+    return 0;
+#undef T
+#undef LIQ
+#undef ICE
+#undef DZ
+#undef Z
+#undef ZIv
+#undef A
+}
+
+int oracle_soiltemperature(const ctsm_config *cfg, const ctsm_layout *L, const ctsm_col_state *S,
+                           const ctsm_patch_state *P, int num_filter, const int *filter,
+                           double dtime, int *fail_col)
+{
+    lay_t Lo = { L->begc_alloc, L->ldc, L->begp_alloc, L->ldp, cfg->nlevsno };
+    int fc, rc;
+    *fail_col = 0;
+    for (fc = 0; fc < num_filter; ++fc) {
+        int c = filter ? filter[fc] : L->begc_alloc + fc;
+        rc = soiltemp_col(cfg, Lo, S, P, c, dtime);
+        if (rc != 0) { *fail_col = c; return rc; }
+    }
+    return 0;
+}
+
+/* One soil step in driver order: SoilTemperature (clm_driver.F90:900), then
+ * SetSoilWaterFractions and SoilWater (HydrologyNoDrainageMod.F90:302-346). */
+int oracle_soil_step(const ctsm_config *cfg, const ctsm_layout *L, const ctsm_col_state *S,
+                     const ctsm_patch_state *P, int num_filter, const int *filter,
+                     double dtime, int *fail_col)
+{
+    int rc = oracle_soiltemperature(cfg, L, S, P, num_filter, filter, dtime, fail_col);
+    if (rc) return rc;
+    oracle_soilwater_fractions(cfg, L, S, num_filter, filter);
+    return oracle_soilwater(cfg, L, S, num_filter, filter, dtime, fail_col);
+}
+
+void oracle_config_default(ctsm_config *cfg)
+{
+    memset(cfg, 0, sizeof(*cfg));
+    cfg->nlevsno = 12; cfg->nlevsoi = 20; cfg->nlevgrnd = 25;
+    cfg->soilwater_movement_method = 1;
+    cfg->upper_boundary_condition = CTSM_BC_FLUX;
+    cfg->lower_boundary_condition = CTSM_BC_ZERO_FLUX;
+    cfg->flux_calculation = 1;
+    cfg->snow_thermal_cond_method = CTSM_SNOWTHK_STURM1997;
+    cfg->itmax_canopy_fluxes = 40;
+    cfg->dtmin = 60.0; cfg->verySmall = 1.e-8; cfg->xTolerUpper = 1.e-1; cfg->xTolerLower = 1.e-2;
+    cfg->e_ice = 6.0;
+}

@@ -1,0 +1,102 @@
+"""ctypes loader of the CPU oracle (oracle/liboracle.so) -- TEST INFRASTRUCTURE.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, and only as the checker.  The product library never calls it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from ctsm_amd import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(path)
+        L.oracle_dgtsv.argtypes = [C.c_int, abi.f64p, abi.f64p, abi.f64p, abi.f64p]
+        L.oracle_dgbsv.argtypes = [C.c_int, C.c_int, C.c_int, abi.f64p, C.c_int, abi.i32p, abi.f64p]
+        sig = [C.POINTER(abi.Config), C.POINTER(abi.Layout), C.POINTER(abi.ColState)]
+        L.oracle_soilwater.argtypes = sig + [C.c_int, abi.i32p, C.c_double, abi.i32p]
+        L.oracle_soilwater_fractions.argtypes = sig + [C.c_int, abi.i32p]
+        L.oracle_soiltemperature.argtypes = sig + [C.POINTER(abi.PatchState), C.c_int, abi.i32p,
+                                                   C.c_double, abi.i32p]
+        L.oracle_soil_step.argtypes = sig + [C.POINTER(abi.PatchState), C.c_int, abi.i32p,
+                                             C.c_double, abi.i32p]
+        L.oracle_config_default.argtypes = [C.POINTER(abi.Config)]
+        _LIB = L
+    return _LIB
+
+
+def config(**kw):
+    cfg = abi.Config()
+    lib().oracle_config_default(C.byref(cfg))
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+def _filt(filter_):
+    if filter_ is None:
+        return None, None
+    f = np.ascontiguousarray(filter_, dtype=np.int32)
+    return f, f.ctypes.data_as(abi.i32p)
+
+
+def dgtsv(dl, d, du, b):
+    dl, d, du, b = (np.array(x, dtype=np.float64) for x in (dl, d, du, b))
+    info = lib().oracle_dgtsv(len(d), dl.ctypes.data_as(abi.f64p), d.ctypes.data_as(abi.f64p),
+                              du.ctypes.data_as(abi.f64p), b.ctypes.data_as(abi.f64p))
+    return b, info
+
+
+def dgbsv(kl, ku, ab, b):
+    """ab: (2*kl+ku+1, n) LAPACK band storage (rows kl..2kl+ku hold A)."""
+    ab = np.asfortranarray(np.array(ab, dtype=np.float64))
+    b = np.array(b, dtype=np.float64)
+    n = ab.shape[1]
+    ipiv = np.zeros(n, dtype=np.int32)
+    info = lib().oracle_dgbsv(n, kl, ku, ab.ctypes.data_as(abi.f64p), ab.shape[0],
+                              ipiv.ctypes.data_as(abi.i32p), b.ctypes.data_as(abi.f64p))
+    return b, ipiv, info
+
+
+def _run(fn, st, cfg, dtime, filter_, with_patch):
+    cs, ps = st.structs()
+    lay = st.layout()
+    f, fp = _filt(filter_)
+    n = st.dims.ncol if f is None else len(f)
+    fail = C.c_int32(0)
+    args = [C.byref(cfg), C.byref(lay), C.byref(cs)]
+    if with_patch:
+        args.append(C.byref(ps))
+    args += [n, fp, C.c_double(dtime), C.byref(fail)]
+    rc = fn(*args)
+    return rc, fail.value
+
+
+def soilwater(st, cfg=None, dtime=1800.0, filter_=None):
+    return _run(lib().oracle_soilwater, st, cfg or config(), dtime, filter_, False)
+
+
+def soiltemperature(st, cfg=None, dtime=1800.0, filter_=None):
+    return _run(lib().oracle_soiltemperature, st, cfg or config(), dtime, filter_, True)
+
+
+def soil_step(st, cfg=None, dtime=1800.0, filter_=None):
+    return _run(lib().oracle_soil_step, st, cfg or config(), dtime, filter_, True)
+
+
+def soilwater_fractions(st, cfg=None, filter_=None):
+    cs, _ = st.structs()
+    lay = st.layout()
+    f, fp = _filt(filter_)
+    n = st.dims.ncol if f is None else len(f)
+    return lib().oracle_soilwater_fractions(C.byref(cfg or config()), C.byref(lay), C.byref(cs), n, fp)
