@@ -92,43 +92,44 @@ def main():
                                    "(columns shard trivially by range, see DESIGN.md)"}))
         sys.exit(2)
 
-    import torch
     from ctsm_amd import lib, synthetic
     from ctsm_amd.state import DeviceState
 
     st = synthetic.workload("uneven", ncol=args.ncol)
     ctx = lib.Context(device=0)
-    stream = torch.cuda.ExternalStream(ctx.stream)
 
     filt = None
     if not args.no_sort:
         perm = np.lexsort((st.col["snl"], st.col["nbedrock"])) + 1
-        filt = torch.from_numpy(perm.astype(np.int32)).cuda()
+        filt = ctx.device_array(perm.astype(np.int32))
 
     def run(plain):
-        ds = DeviceState(st)
+        ds = DeviceState(st, ctx)
         structs = ds.structs()
         for _ in range(args.warmup):
             ctx.step_dev(ds, filter_dev=filt, plain=plain, structs=structs)
         ctx.sync()
-        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-        torch.cuda.synchronize()
+        # events on the library's own stream, around each step's launch(es)
+        starts, stops = [], []
         t0 = time.perf_counter()
         for k in range(args.steps):
-            ev[k][0].record(stream)
+            starts.append(ctx.record())
             ctx.step_dev(ds, filter_dev=filt, plain=plain, structs=structs)
-            ev[k][1].record(stream)
+            stops.append(ctx.record())
         ctx.sync()
-        torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        per = [a.elapsed_time(b) for a, b in ev]
+        per = [ctx.elapsed_ms(a, b) for a, b in zip(starts, stops)]
+        for e in starts + stops:
+            ctx.destroy_event(e)
         return wall, float(np.mean(per)), ds
 
     wall, step_ms, ds = run(plain=False)
+    ds.free()
     value = args.ncol * args.steps / wall
     res = {"fused_step_ms": step_ms}
     if not args.no_plain:
-        wall_p, step_ms_p, _ = run(plain=True)
+        wall_p, step_ms_p, dsp = run(plain=True)
+        dsp.free()
         res["plain_step_ms"] = step_ms_p
         res["plain_column_steps_per_s"] = args.ncol * args.steps / wall_p
         res["fused_over_plain"] = value / res["plain_column_steps_per_s"]

@@ -483,6 +483,65 @@ int ctsm_soil_step_fused_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_c
     return finish_launch(ctx);
 }
 
+int ctsm_dev_alloc(ctsm_ctx *ctx, uint64_t bytes, void **out)
+{
+    if (!ctx || !out) return CTSM_E_ARG;
+    *out = nullptr;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = hipMalloc(out, std::max<uint64_t>(bytes, 1));
+    return e == hipSuccess ? CTSM_OK : hip_fail(ctx, e);
+}
+
+int ctsm_dev_free(ctsm_ctx *ctx, void *ptr)
+{
+    if (!ctx) return CTSM_E_ARG;
+    if (!ptr) return CTSM_OK;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = hipFree(ptr);
+    return e == hipSuccess ? CTSM_OK : hip_fail(ctx, e);
+}
+
+int ctsm_memcpy(ctsm_ctx *ctx, void *dst, const void *src, uint64_t bytes, int32_t kind)
+{
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return CTSM_E_ARG;
+    hipMemcpyKind k;
+    switch (kind) {
+    case 1: k = hipMemcpyHostToDevice; break;
+    case 2: k = hipMemcpyDeviceToHost; break;
+    case 3: k = hipMemcpyDeviceToDevice; break;
+    default: return CTSM_E_ARG;
+    }
+    if (bytes == 0) return CTSM_OK;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, k, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? CTSM_OK : hip_fail(ctx, e);
+}
+
+int ctsm_event_record(ctsm_ctx *ctx, void **event)
+{
+    if (!ctx || !event) return CTSM_E_ARG;
+    hipEvent_t ev = (hipEvent_t)*event;
+    hipError_t e = hipSuccess;
+    if (!ev) e = hipEventCreate(&ev);
+    if (e == hipSuccess) e = hipEventRecord(ev, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e);
+    *event = (void *)ev;
+    return CTSM_OK;
+}
+
+int ctsm_event_elapsed(void *ev_start, void *ev_stop, float *ms)
+{
+    if (!ev_start || !ev_stop || !ms) return CTSM_E_ARG;
+    if (hipEventSynchronize((hipEvent_t)ev_stop) != hipSuccess) return CTSM_E_HIP;
+    return hipEventElapsedTime(ms, (hipEvent_t)ev_start, (hipEvent_t)ev_stop) == hipSuccess ? CTSM_OK : CTSM_E_HIP;
+}
+
+int ctsm_event_destroy(void *event)
+{
+    if (!event) return CTSM_OK;
+    return hipEventDestroy((hipEvent_t)event) == hipSuccess ? CTSM_OK : CTSM_E_HIP;
+}
+
 int ctsm_sync(ctsm_ctx *ctx)
 {
     if (!ctx) return CTSM_E_ARG;

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "ctsm_gpu.h"
+#include "ctsm_math.h"
 
 namespace ctsm {
 

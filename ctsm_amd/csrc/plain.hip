@@ -430,7 +430,7 @@ __global__ void k_phasechange(View V, ctsm_col_state S, int n, const int32_t *fi
             if (ice > 0.0 && t > TFRZ) { imelt = 1; tinc = TFRZ - t; t = TFRZ; }
             if (t < TFRZ) {
                 const double smp = HFUS * (TFRZ - t) / (GRAV * t) * 1000.0;
-                supercool = S.watsat[V.soi(c, j)] * pow(smp / S.sucsat[V.soi(c, j)], -1.0 / S.bsw[V.soi(c, j)]);
+                supercool = S.watsat[V.soi(c, j)] * cm_pow(smp / S.sucsat[V.soi(c, j)], -1.0 / S.bsw[V.soi(c, j)]);
                 supercool = supercool * S.dz[V.sno(c, j)] * 1000.0;
             }
             if (liq > supercool && t < TFRZ) { imelt = 2; tinc = TFRZ - t; t = TFRZ; }
@@ -580,20 +580,20 @@ __global__ void k_soilwater_plain(View V, Params P, ctsm_col_state S, int n, con
             const double bsw = S.bsw[V.soi(c, j)];
             if (j == nl) {
                 s1 = A(R_SMP, j);
-                A(R_IMPED, j) = pow(10.0, -(P.e_ice * S.icefrac[V.soi(c, j)]));
+                A(R_IMPED, j) = cm_pow(10.0, -(P.e_ice * S.icefrac[V.soi(c, j)]));
             } else {
                 s1 = 0.5 * (A(R_SMP, j) + A(R_SMP, j + 1));
-                A(R_IMPED, j) = pow(10.0, -(P.e_ice * (0.5 * (S.icefrac[V.soi(c, j)] + S.icefrac[V.soi(c, j + 1)]))));
+                A(R_IMPED, j) = cm_pow(10.0, -(P.e_ice * (0.5 * (S.icefrac[V.soi(c, j)] + S.icefrac[V.soi(c, j + 1)]))));
             }
             s1 = fmin90(s1, 1.0);
             s1 = fmax90(0.01, s1);
-            const double hk = A(R_IMPED, j) * S.hksat[V.soi(c, j)] * pow(s1, 2.0 * bsw + 3.0);
+            const double hk = A(R_IMPED, j) * S.hksat[V.soi(c, j)] * cm_pow(s1, 2.0 * bsw + 3.0);
             A(R_HK, j) = hk;
             A(R_DHKDW, j) = (2.0 * bsw + 3.0) * hk / s1;
         }
         for (int j = 1; j <= nl; ++j) {
             const double s2 = A(R_SMP, j), bsw = S.bsw[V.soi(c, j)];
-            const double smp = -(S.sucsat[V.soi(c, j)] * pow(s2, -bsw));
+            const double smp = -(S.sucsat[V.soi(c, j)] * cm_pow(s2, -bsw));
             A(R_DSMPDW, j) = (-(bsw * smp) / s2) / WS(j);
             A(R_SMP, j) = smp;
             S.smp_l[V.soi(c, j)] = smp;

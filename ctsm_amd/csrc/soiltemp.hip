@@ -485,7 +485,7 @@ __device__ int soiltemp_column(const View &V, const Params &P, const ctsm_col_st
             if (ice > 0.0 && t > TFRZ) { imelt = 1; tinc = TFRZ - t; t = TFRZ; }
             if (t < TFRZ) {
                 const double smp = HFUS * (TFRZ - t) / (GRAV * t) * 1000.0;
-                supercool = S.watsat[V.soi(c, j)] * pow(smp / S.sucsat[V.soi(c, j)], -1.0 / S.bsw[V.soi(c, j)]);
+                supercool = S.watsat[V.soi(c, j)] * cm_pow(smp / S.sucsat[V.soi(c, j)], -1.0 / S.bsw[V.soi(c, j)]);
                 supercool = supercool * S.dz[V.sno(c, j)] * 1000.0;
             }
             if (liq > supercool && t < TFRZ) { imelt = 2; tinc = TFRZ - t; t = TFRZ; }

@@ -18,11 +18,11 @@ __device__ __forceinline__ double level_thk(const View &V, const Params &P, cons
         satw = fmin90(1.0, satw);
         if (satw > .1e-6) {
             double dke;
-            if (S.t_soisno[V.sno(c, j)] >= TFRZ) dke = fmax90(0.0, log10(satw) + 1.0);
+            if (S.t_soisno[V.sno(c, j)] >= TFRZ) dke = fmax90(0.0, cm_log10(satw) + 1.0);
             else dke = satw;
             const double fl = (liq / (DENH2O * dz)) /
                               (liq / (DENH2O * dz) + ice / (DENICE * dz) + 0.0 / (DENICE * dz));
-            const double dksat = S.tkmg[V.soi(c, j)] * pow(TKWAT, fl * ws) * pow(TKICE, (1.0 - fl) * ws);
+            const double dksat = S.tkmg[V.soi(c, j)] * cm_pow(TKWAT, fl * ws) * cm_pow(TKICE, (1.0 - fl) * ws);
             th = dke * dksat + (1.0 - dke) * S.tkdry[V.soi(c, j)];
         } else {
             th = S.tkdry[V.soi(c, j)];

@@ -49,7 +49,7 @@ __device__ __forceinline__ int richards_column(const View &V, const Params &P, c
                 icei = 0.5 * (ic + S.icefrac[V.soi(c, jj + 1)]);
                 den[j] = M_TO_MM * (S.z[V.sno(c, jj + 1)] - S.z[V.sno(c, jj)]);
             }
-            hkc[j] = pow(10.0, -(P.e_ice * icei)) * S.hksat[V.soi(c, jj)];
+            hkc[j] = cm_pow(10.0, -(P.e_ice * icei)) * S.hksat[V.soi(c, jj)];
         }
     }
 
@@ -72,7 +72,7 @@ __device__ __forceinline__ int richards_column(const View &V, const Params &P, c
         double vwc = fmax90(liq[0], 1.0e-6) / mdz[0];
         double s2 = fmin90(vwc / ws[0], 1.0);
         s2 = fmax90(0.01, s2);
-        double smp = -(suc[0] * pow(s2, -bsw[0]));
+        double smp = -(suc[0] * cm_pow(s2, -bsw[0]));
         double dsmpdw = (-(bsw[0] * smp) / s2) / ws[0];
         double dtdz = dtsub / mdz[0];
 
@@ -92,7 +92,7 @@ __device__ __forceinline__ int richards_column(const View &V, const Params &P, c
                     const double vwcn = fmax90(liq[j + 1], 1.0e-6) / mdz[j + 1];
                     s2n = fmin90(vwcn / ws[j + 1], 1.0);
                     s2n = fmax90(0.01, s2n);
-                    smpn = -(suc[j + 1] * pow(s2n, -bsw[j + 1]));
+                    smpn = -(suc[j + 1] * cm_pow(s2n, -bsw[j + 1]));
                     dsmpdwn = (-(bsw[j + 1] * smpn) / s2n) / ws[j + 1];
                     dtdzn = dtsub / mdz[j + 1];
                 }
@@ -101,7 +101,7 @@ __device__ __forceinline__ int richards_column(const View &V, const Params &P, c
                 s1 = fmin90(s1, 1.0);
                 s1 = fmax90(0.01, s1);
                 const double e = 2.0 * bsw[j] + 3.0;
-                const double hk = hkc[j] * pow(s1, e);
+                const double hk = hkc[j] * cm_pow(s1, e);
                 const double dhkdw = e * hk / s1;
                 if (candidate) {
                     S.smp_l[V.soi(c, j + 1)] = smp;

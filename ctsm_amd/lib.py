@@ -10,6 +10,8 @@ library is missing or no GPU is visible, every call raises.
 import ctypes as C
 import os
 
+import numpy as np
+
 from . import abi
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -24,7 +26,8 @@ EXPORTS = [
     "ctsm_config_default", "ctsm_create", "ctsm_destroy", "ctsm_last_error",
     "ctsm_soilwater", "ctsm_soiltemperature", "ctsm_soilwater_fractions",
     "ctsm_soilwater_dev", "ctsm_soiltemperature_dev", "ctsm_soilwater_fractions_dev",
-    "ctsm_soil_step_fused_dev", "ctsm_sync", "ctsm_stream",
+    "ctsm_soil_step_fused_dev", "ctsm_dev_alloc", "ctsm_dev_free", "ctsm_memcpy",
+    "ctsm_event_record", "ctsm_event_elapsed", "ctsm_event_destroy", "ctsm_sync", "ctsm_stream",
 ]
 
 
@@ -60,6 +63,12 @@ def load():
     L.ctsm_sync.argtypes = [C.c_void_p]
     L.ctsm_stream.argtypes = [C.c_void_p]
     L.ctsm_stream.restype = C.c_void_p
+    L.ctsm_dev_alloc.argtypes = [C.c_void_p, C.c_uint64, P(C.c_void_p)]
+    L.ctsm_dev_free.argtypes = [C.c_void_p, C.c_void_p]
+    L.ctsm_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_int32]
+    L.ctsm_event_record.argtypes = [C.c_void_p, P(C.c_void_p)]
+    L.ctsm_event_elapsed.argtypes = [C.c_void_p, C.c_void_p, P(C.c_float)]
+    L.ctsm_event_destroy.argtypes = [C.c_void_p]
     _LIB = L
     return L
 
@@ -123,14 +132,55 @@ class Context:
         self._check(self.L.ctsm_soilwater_fractions(self.h, C.byref(st.layout()), C.byref(cs), n, fp),
                     "SetSoilWaterFractions")
 
-    # ---- device-resident (torch tensors) -------------------------------
+    # ---- device memory / events ------------------------------------------
+    def alloc(self, nbytes):
+        p = C.c_void_p()
+        self._check(self.L.ctsm_dev_alloc(self.h, int(nbytes), C.byref(p)), "ctsm_dev_alloc")
+        return p.value
+
+    def free(self, ptr):
+        self._check(self.L.ctsm_dev_free(self.h, C.c_void_p(ptr)), "ctsm_dev_free")
+
+    def h2d(self, dptr, arr):
+        a = np.ascontiguousarray(arr)
+        self._check(self.L.ctsm_memcpy(self.h, C.c_void_p(dptr), a.ctypes.data_as(C.c_void_p), a.nbytes, 1),
+                    "ctsm_memcpy")
+
+    def d2h(self, arr, dptr):
+        assert arr.flags.c_contiguous
+        self._check(self.L.ctsm_memcpy(self.h, arr.ctypes.data_as(C.c_void_p), C.c_void_p(dptr), arr.nbytes, 2),
+                    "ctsm_memcpy")
+
+    def device_array(self, arr):
+        """Copy a numpy array to a new device buffer; returns the pointer."""
+        a = np.ascontiguousarray(arr)
+        p = self.alloc(a.nbytes)
+        self.h2d(p, a)
+        return p
+
+    def record(self, ev=None):
+        e = C.c_void_p(ev)
+        self._check(self.L.ctsm_event_record(self.h, C.byref(e)), "ctsm_event_record")
+        return e.value
+
+    def elapsed_ms(self, ev0, ev1):
+        ms = C.c_float()
+        if self.L.ctsm_event_elapsed(C.c_void_p(ev0), C.c_void_p(ev1), C.byref(ms)) != CTSM_OK:
+            raise CtsmError(CTSM_E_HIP, 0, "ctsm_event_elapsed")
+        return ms.value
+
+    def destroy_event(self, ev):
+        self.L.ctsm_event_destroy(C.c_void_p(ev))
+
+    # ---- device-resident state ------------------------------------------
     def step_dev(self, ds, filter_dev=None, n=None, dtime=1800.0, plain=False, structs=None):
         """One soil step on a DeviceState: fused kernel, or the plain
-        one-launch-per-routine sequence.  Enqueues only."""
+        one-launch-per-routine sequence.  `filter_dev` is a device pointer
+        (int) to int32 column indices or None.  Enqueues only."""
         cs, ps = structs if structs is not None else ds.structs()
         lay = ds.layout()
         n = ds.dims.ncol if n is None else n
-        fp = C.c_void_p(filter_dev.data_ptr()) if filter_dev is not None else None
+        fp = C.c_void_p(filter_dev) if filter_dev is not None else None
         if plain:
             self._check(self.L.ctsm_soiltemperature_dev(self.h, C.byref(lay), C.byref(cs), C.byref(ps), n, fp,
                                                         dtime, 1), "SoilTemperature(plain)")
@@ -145,14 +195,14 @@ class Context:
     def soilwater_dev(self, ds, filter_dev=None, n=None, dtime=1800.0, plain=False):
         cs, _ = ds.structs()
         n = ds.dims.ncol if n is None else n
-        fp = C.c_void_p(filter_dev.data_ptr()) if filter_dev is not None else None
+        fp = C.c_void_p(filter_dev) if filter_dev is not None else None
         self._check(self.L.ctsm_soilwater_dev(self.h, C.byref(ds.layout()), C.byref(cs), n, fp, dtime,
                                               int(plain)), "SoilWater")
 
     def soiltemperature_dev(self, ds, filter_dev=None, n=None, dtime=1800.0, plain=False):
         cs, ps = ds.structs()
         n = ds.dims.ncol if n is None else n
-        fp = C.c_void_p(filter_dev.data_ptr()) if filter_dev is not None else None
+        fp = C.c_void_p(filter_dev) if filter_dev is not None else None
         self._check(self.L.ctsm_soiltemperature_dev(self.h, C.byref(ds.layout()), C.byref(cs), C.byref(ps), n,
                                                     fp, dtime, int(plain)), "SoilTemperature")
 
@@ -163,6 +213,5 @@ class Context:
 def _filter(filter_, ncol):
     if filter_ is None:
         return None, None, ncol
-    import numpy as np
     f = np.ascontiguousarray(filter_, dtype=np.int32)
     return f, f.ctypes.data_as(abi.i32p), len(f)

@@ -4,7 +4,7 @@ A state is a dict of arrays, one per field of ctsm_col_state /
 ctsm_patch_state, laid out exactly as the reference allocates them
 (ColumnType.F90:137-155 and friends): level-major, column-fastest, so array
 [k, c] is Fortran element (c, lo + k).  Host states hold numpy arrays; device
-states hold torch tensors on the GPU (PyTorch is only the allocator here).
+states hold buffers allocated through the library's own C-ABI.
 """
 import ctypes as C
 
@@ -93,13 +93,21 @@ class HostState:
 
 
 class DeviceState:
-    """torch tensors on a GPU mirroring a HostState (device-resident mode)."""
+    """Device-resident mirror of a HostState, allocated through the library
+    (ctsm_dev_alloc / ctsm_memcpy), so no other GPU runtime is involved."""
 
-    def __init__(self, host, device="cuda:0"):
-        import torch
+    def __init__(self, host, ctx):
+        self.ctx = ctx
         self.dims = host.dims
-        self.col = {k: torch.from_numpy(v).to(device) for k, v in host.col.items()}
-        self.pch = {k: torch.from_numpy(v).to(device) for k, v in host.pch.items()}
+        self._shapes = {}
+        self.col, self.pch = {}, {}
+        for group, src, dst in (("c", host.col, self.col), ("p", host.pch, self.pch)):
+            for k, v in src.items():
+                a = np.ascontiguousarray(v)
+                ptr = ctx.alloc(a.nbytes)
+                ctx.h2d(ptr, a)
+                dst[k] = ptr
+                self._shapes[(group, k)] = (a.shape, a.dtype)
 
     def layout(self):
         return abi.Layout(1, self.dims.ncol, 1, self.dims.npatch)
@@ -107,19 +115,35 @@ class DeviceState:
     def structs(self):
         cs = abi.ColState()
         for name, t, _ in abi.COL_FIELDS:
-            setattr(cs, name, C.cast(C.c_void_p(self.col[name].data_ptr()), abi._CT[t]))
+            setattr(cs, name, C.cast(C.c_void_p(self.col[name]), abi._CT[t]))
         ps = abi.PatchState()
         for name, t, _ in abi.PATCH_FIELDS:
-            setattr(ps, name, C.cast(C.c_void_p(self.pch[name].data_ptr()), abi._CT[t]))
+            setattr(ps, name, C.cast(C.c_void_p(self.pch[name]), abi._CT[t]))
         return cs, ps
 
     def to_host(self):
         o = HostState.__new__(HostState)
         o.dims = self.dims
-        o.col = {k: v.cpu().numpy().copy() for k, v in self.col.items()}
-        o.pch = {k: v.cpu().numpy().copy() for k, v in self.pch.items()}
+        o.col, o.pch = {}, {}
+        for group, src, dst in (("c", self.col, o.col), ("p", self.pch, o.pch)):
+            for k, ptr in src.items():
+                shape, dt = self._shapes[(group, k)]
+                a = np.empty(shape, dtype=dt)
+                self.ctx.d2h(a, ptr)
+                dst[k] = a
         return o
 
+    def upload(self, host):
+        """Overwrite the device copy with a host state of the same dims."""
+        for group, src, dst in (("c", host.col, self.col), ("p", host.pch, self.pch)):
+            for k, v in src.items():
+                self.ctx.h2d(dst[k], np.ascontiguousarray(v))
+
     def nbytes(self):
-        return sum(v.numel() * v.element_size() for v in self.col.values()) + \
-            sum(v.numel() * v.element_size() for v in self.pch.values())
+        return sum(int(np.prod(s)) * np.dtype(d).itemsize for s, d in self._shapes.values())
+
+    def free(self):
+        for d in (self.col, self.pch):
+            for k, ptr in d.items():
+                self.ctx.free(ptr)
+            d.clear()

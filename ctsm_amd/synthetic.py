@@ -213,8 +213,9 @@ def soil_only(ncol, seed, snow=False, bedrock=False, col0=0, nlevsno=12, nlevsoi
     c["qflx_rootsoi"][...] = wts * tot[None, :]
 
     # ---- snow / surface-water scalars
-    c["frac_sno_eff"][:] = np.where(nsno > 0, 1.0, 0.0)
-    fh = np.zeros(ncol) if frac_h2osfc is None else frac_h2osfc
+    fh = np.zeros(ncol) if frac_h2osfc is None else np.asarray(frac_h2osfc, dtype=np.float64)
+    # snow and standing water share the ground: frac_sno_eff + frac_h2osfc <= 1
+    c["frac_sno_eff"][:] = np.where(nsno > 0, 1.0 - fh, 0.0)
     c["frac_h2osfc"][:] = fh
     c["h2osfc"][:] = np.where(fh > 0, 2.0, 0.0)
     c["h2osno_no_layers"][:] = 0.0
@@ -223,7 +224,7 @@ def soil_only(ncol, seed, snow=False, bedrock=False, col0=0, nlevsno=12, nlevsoi
     c["snomelt_accum"][:] = 0.0
     top_t = np.where(nsno > 0, t[NS - nsno, np.arange(ncol)], t[NS])
     c["t_h2osfc"][:] = t[NS]
-    c["t_grnd"][:] = top_t
+    set_t_grnd_consistent(st)
     c["emg"][:] = 0.96
     c["htvp"][:] = HVAP
     c["eflx_bot"][:] = 0.0
@@ -248,6 +249,20 @@ def soil_only(ncol, seed, snow=False, bedrock=False, col0=0, nlevsno=12, nlevsoi
               "qflx_evap_soi", "qflx_ev_snow", "qflx_ev_soil", "qflx_ev_h2osfc"):
         p[k][:] = 0.0
     return st
+
+
+def set_t_grnd_consistent(st):
+    """t_grnd as SoilTemperature leaves it (SoilTemperatureMod.F90:457-477):
+    the snow / soil / standing-water blend of the current temperatures."""
+    c = st.col
+    NS = st.dims.nlevsno
+    snl = c["snl"]
+    fs, fh = c["frac_sno_eff"], c["frac_h2osfc"]
+    ncol = st.dims.ncol
+    t1 = c["t_soisno"][NS]
+    ttop = c["t_soisno"][NS + snl, np.arange(ncol)]
+    th = c["t_h2osfc"]
+    c["t_grnd"][:] = np.where(snl < 0, fs * ttop + (1.0 - fs - fh) * t1 + fh * th, (1.0 - fh) * t1 + fh * th)
 
 
 def workload(name, ncol=None, seed=None):

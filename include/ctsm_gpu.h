@@ -263,6 +263,19 @@ int ctsm_soil_step_fused_dev(ctsm_ctx *ctx, const ctsm_layout *lay,
                              const ctsm_col_state *col, const ctsm_patch_state *pch,
                              int32_t num_filter, const int32_t *filter_dev, double dtime);
 
+/* ---- device memory and timing helpers --------------------------------------
+ * Hosts need no other GPU runtime: device-resident state is allocated, filled
+ * and read back through these (all on the context's device and stream). */
+int ctsm_dev_alloc(ctsm_ctx *ctx, uint64_t bytes, void **out);
+int ctsm_dev_free(ctsm_ctx *ctx, void *ptr);
+/* kind: 1 = host->device, 2 = device->host, 3 = device->device.  Synchronous. */
+int ctsm_memcpy(ctsm_ctx *ctx, void *dst, const void *src, uint64_t bytes, int32_t kind);
+/* Records an event on the context's stream (created on first use). */
+int ctsm_event_record(ctsm_ctx *ctx, void **event);
+/* Milliseconds between two recorded events (waits for the second). */
+int ctsm_event_elapsed(void *ev_start, void *ev_stop, float *ms);
+int ctsm_event_destroy(void *event);
+
 /* Waits for the context's stream; returns CTSM_E_SOLVER if a column failed. */
 int ctsm_sync(ctsm_ctx *ctx);
 

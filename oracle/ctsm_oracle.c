@@ -13,6 +13,13 @@
  * the reference checkout, and building them would need stand-ins for those
  * libraries.
  *
+ * Elementary functions: log10 and x**y use ctsm_amd/csrc/ctsm_math.h, the
+ * deterministic implementation the GPU kernels use, so GPU and oracle round
+ * identically (the reference's own libm is whatever its compiler ships and
+ * is not pinned; ctsm_math.h is checked against the host libm to 1 ulp in
+ * tests/test_math.py).  Building with -DORACLE_LIBM switches back to the
+ * host libm to measure how far last-bit libm differences propagate.
+ *
  * Floating point: compiled with -O2 -ffp-contract=off, binary64, operation
  * order as written in the Fortran (left-to-right, `-a*b` = -(a*b), integer
  * powers by repeated squaring).  Fortran default-REAL literals that the
@@ -44,6 +51,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include "ctsm_gpu.h"
+#include "ctsm_math.h"   /* the same elementary functions as the kernels */
 
 /* ---- constants: shr_const_mod (tools/mksurfdata_esmf/src/shr_const_mod.F90:14-52)
  *      and clm_varcon.F90:50-121 ------------------------------------------ */
@@ -64,6 +72,13 @@
 #define CNFAC       0.5
 #define THIN_SFCLAYER 1.0e-6
 #define M_TO_MM     1.e3
+
+#ifdef ORACLE_LIBM
+#undef cm_pow
+#undef cm_log10
+#define cm_pow pow
+#define cm_log10 log10
+#endif
 
 #define NLEVMAX_SOI 64   /* scratch bound for layers */
 #define NBANDMAX    (64 + 64)
@@ -233,18 +248,108 @@ int oracle_dgbsv(int n, int kl, int ku, double *ab, int ldab, int *ipiv, double 
 
 /* ========================================================================= */
 /* Richards equation, moisture form: SoilWaterMovementMod.F90:976-1440.   */
-static int soilwater_col(const ctsm_config *cfg, lay_t Lo, const ctsm_col_state *S,
-                         int c, double dtime)
-{
-    const int nlayers = C1(S->nbedrock, c);
+
+/* Per-trial system of one column (compute_hydraulic_properties :1444-1556,
+ * compute_moisture_fluxes_and_derivs :1560-1866, compute_RHS/LHS :1871-2012)
+ * for the liquid water liq[1..nlayers] and sub-step dtsub.  Arrays 1-based. */
+typedef struct {
     double vwc_liq[NLEVMAX_SOI + 1], dt_dz[NLEVMAX_SOI + 1], hk[NLEVMAX_SOI + 1];
     double smp[NLEVMAX_SOI + 1], dhkdw[NLEVMAX_SOI + 1], dsmpdw[NLEVMAX_SOI + 1];
     double imped[NLEVMAX_SOI + 1], qin[NLEVMAX_SOI + 1], qout[NLEVMAX_SOI + 1];
     double dqidw0[NLEVMAX_SOI + 1], dqidw1[NLEVMAX_SOI + 1], dqodw1[NLEVMAX_SOI + 1];
     double dqodw2[NLEVMAX_SOI + 1], amx[NLEVMAX_SOI + 1], bmx[NLEVMAX_SOI + 1];
-    double cmx[NLEVMAX_SOI + 1], rmx[NLEVMAX_SOI + 1], dwat[NLEVMAX_SOI + 1];
-    double dLow[NLEVMAX_SOI], diag[NLEVMAX_SOI], dUpp[NLEVMAX_SOI], rhs[NLEVMAX_SOI];
+    double cmx[NLEVMAX_SOI + 1], rmx[NLEVMAX_SOI + 1];
+} richards_sys;
+
+static void richards_assemble(const ctsm_config *cfg, lay_t Lo, const ctsm_col_state *S, int c,
+                              int nlayers, double dtsub, const double *liq, richards_sys *R)
+{
     double s2[NLEVMAX_SOI + 1];
+    int j;
+    for (j = 1; j <= nlayers; ++j) {
+        R->vwc_liq[j] = fmaxf90(liq[j], 1.0e-6) / (SNO(S->dz, c, j) * DENH2O);
+        R->dt_dz[j] = dtsub / (M_TO_MM * SNO(S->dz, c, j));
+    }
+    for (j = 1; j <= nlayers; ++j) {
+        s2[j] = R->vwc_liq[j] / SOI(S->watsat, c, j);
+        s2[j] = fminf90(s2[j], 1.0);
+        s2[j] = fmaxf90(0.01, s2[j]);
+    }
+    for (j = 1; j <= nlayers; ++j) {
+        double s1, dhkds, dsmpds, bsw = SOI(S->bsw, c, j);
+        if (j == nlayers) {
+            s1 = s2[j];
+            R->imped[j] = cm_pow(10.0, -(cfg->e_ice * SOI(S->icefrac, c, j)));
+        } else {
+            s1 = 0.5 * (s2[j] + s2[j + 1]);
+            R->imped[j] = cm_pow(10.0, -(cfg->e_ice * (0.5 * (SOI(S->icefrac, c, j) + SOI(S->icefrac, c, j + 1)))));
+        }
+        s1 = fminf90(s1, 1.0);
+        s1 = fmaxf90(0.01, s1);
+        /* CH78 soil_hk / soil_suction (SoilWaterRetentionCurveClappHornberg1978Mod.F90:75-119) */
+        R->hk[j] = R->imped[j] * SOI(S->hksat, c, j) * cm_pow(s1, 2.0 * bsw + 3.0);
+        dhkds = (2.0 * bsw + 3.0) * R->hk[j] / s1;
+        R->smp[j] = -(SOI(S->sucsat, c, j) * cm_pow(s2[j], -bsw));
+        dsmpds = -(bsw * R->smp[j]) / s2[j];
+        R->dhkdw[j] = dhkds;
+        R->dsmpdw[j] = dsmpds / SOI(S->watsat, c, j);
+    }
+    for (j = 1; j <= nlayers; ++j)
+        R->qin[j] = R->qout[j] = R->dqidw0[j] = R->dqidw1[j] = R->dqodw1[j] = R->dqodw2[j] = 0.0;
+    R->qin[1] = C1(S->qflx_infl, c);   /* bc_flux */
+    R->dqidw1[1] = 0.0;
+    for (j = 1; j <= nlayers - 1; ++j) {
+        double dhkds1, dhkds2, num, den;
+        if (j >= 2) {
+            R->qin[j] = R->qout[j - 1];
+            R->dqidw0[j] = R->dqodw1[j - 1];
+            R->dqidw1[j] = R->dqodw2[j - 1];
+        }
+        dhkds1 = 0.5 * R->dhkdw[j] / SOI(S->watsat, c, j);
+        dhkds2 = 0.5 * R->dhkdw[j] / SOI(S->watsat, c, j + 1);
+        num = R->smp[j + 1] - R->smp[j];
+        den = M_TO_MM * (SNO(S->z, c, j + 1) - SNO(S->z, c, j));
+        R->qout[j] = -(R->hk[j] * num / den) + R->hk[j];
+        R->dqodw1[j] = (R->hk[j] * R->dsmpdw[j] - dhkds1 * num) / den + dhkds1;
+        R->dqodw2[j] = (-(R->hk[j] * R->dsmpdw[j + 1]) - dhkds2 * num) / den + dhkds2;
+    }
+    j = nlayers;
+    R->qin[j] = R->qout[j - 1];
+    R->dqidw0[j] = R->dqodw1[j - 1];
+    R->dqidw1[j] = R->dqodw2[j - 1];
+    if (cfg->lower_boundary_condition == CTSM_BC_FLUX) {
+        R->qout[j] = R->hk[j];
+        R->dqodw1[j] = R->dhkdw[j] / SOI(S->watsat, c, j);
+    } else { /* bc_zero_flux */
+        R->qout[j] = 0.0;
+        R->dqodw1[j] = 0.0;
+    }
+    for (j = 1; j <= nlayers; ++j) {
+        double fluxNet = R->qin[j] - R->qout[j] - SOI(S->qflx_rootsoi, c, j);
+        R->rmx[j] = -(fluxNet * R->dt_dz[j]);
+    }
+    j = 1;
+    R->amx[j] = 0.0;
+    R->bmx[j] = -1.0 - (-R->dqidw1[j] + R->dqodw1[j]) * R->dt_dz[j];
+    R->cmx[j] = -(R->dqodw2[j] * R->dt_dz[j]);
+    for (j = 2; j <= nlayers - 1; ++j) {
+        R->amx[j] = R->dqidw0[j] * R->dt_dz[j];
+        R->bmx[j] = -1.0 - (-R->dqidw1[j] + R->dqodw1[j]) * R->dt_dz[j];
+        R->cmx[j] = -(R->dqodw2[j] * R->dt_dz[j]);
+    }
+    j = nlayers;
+    R->amx[j] = R->dqidw0[j] * R->dt_dz[j];
+    R->bmx[j] = -1.0 - (-R->dqidw1[j] + R->dqodw1[j]) * R->dt_dz[j];
+    R->cmx[j] = 0.0;
+}
+
+static int soilwater_col(const ctsm_config *cfg, lay_t Lo, const ctsm_col_state *S,
+                         int c, double dtime)
+{
+    const int nlayers = C1(S->nbedrock, c);
+    richards_sys R;
+    double liq[NLEVMAX_SOI + 1], dwat[NLEVMAX_SOI + 1];
+    double dLow[NLEVMAX_SOI], diag[NLEVMAX_SOI], dUpp[NLEVMAX_SOI], rhs[NLEVMAX_SOI];
     double fluxNet0[NLEVMAX_SOI + 1], fluxNet1[NLEVMAX_SOI + 1];
     int nsubstep = 0, j;
     double dtsub = dtime, dtdone = 0.0, errorMax, qcTemp;
@@ -254,94 +359,18 @@ static int soilwater_col(const ctsm_config *cfg, lay_t Lo, const ctsm_col_state 
 
     for (;;) {
         nsubstep = nsubstep + 1;
+        for (j = 1; j <= nlayers; ++j) liq[j] = SNO(S->h2osoi_liq, c, j);
+        richards_assemble(cfg, Lo, S, c, nlayers, dtsub, liq, &R);
         for (j = 1; j <= nlayers; ++j) {
-            vwc_liq[j] = fmaxf90(SNO(S->h2osoi_liq, c, j), 1.0e-6) / (SNO(S->dz, c, j) * DENH2O);
-            dt_dz[j] = dtsub / (M_TO_MM * SNO(S->dz, c, j));
+            SOI(S->smp_l, c, j) = R.smp[j];
+            SOI(S->hk_l, c, j) = R.hk[j];
         }
-        /* compute_hydraulic_properties :1444-1556 */
-        for (j = 1; j <= nlayers; ++j) {
-            s2[j] = vwc_liq[j] / SOI(S->watsat, c, j);
-            s2[j] = fminf90(s2[j], 1.0);
-            s2[j] = fmaxf90(0.01, s2[j]);
-        }
-        for (j = 1; j <= nlayers; ++j) {
-            double s1, dhkds, dsmpds, bsw = SOI(S->bsw, c, j);
-            if (j == nlayers) {
-                s1 = s2[j];
-                imped[j] = pow(10.0, -(cfg->e_ice * SOI(S->icefrac, c, j)));
-            } else {
-                s1 = 0.5 * (s2[j] + s2[j + 1]);
-                imped[j] = pow(10.0, -(cfg->e_ice * (0.5 * (SOI(S->icefrac, c, j) + SOI(S->icefrac, c, j + 1)))));
-            }
-            s1 = fminf90(s1, 1.0);
-            s1 = fmaxf90(0.01, s1);
-            /* CH78 soil_hk / soil_suction */
-            hk[j] = imped[j] * SOI(S->hksat, c, j) * pow(s1, 2.0 * bsw + 3.0);
-            dhkds = (2.0 * bsw + 3.0) * hk[j] / s1;
-            smp[j] = -(SOI(S->sucsat, c, j) * pow(s2[j], -bsw));
-            dsmpds = -(bsw * smp[j]) / s2[j];
-            dhkdw[j] = dhkds;
-            dsmpdw[j] = dsmpds / SOI(S->watsat, c, j);
-            SOI(S->smp_l, c, j) = smp[j];
-            SOI(S->hk_l, c, j) = hk[j];
-        }
-        /* compute_moisture_fluxes_and_derivs :1560-1866 */
-        for (j = 1; j <= nlayers; ++j) {
-            qin[j] = qout[j] = dqidw0[j] = dqidw1[j] = dqodw1[j] = dqodw2[j] = 0.0;
-        }
-        qin[1] = C1(S->qflx_infl, c);   /* bc_flux */
-        dqidw1[1] = 0.0;
-        for (j = 1; j <= nlayers - 1; ++j) {
-            double dhkds1, dhkds2, num, den;
-            if (j >= 2) {
-                qin[j] = qout[j - 1];
-                dqidw0[j] = dqodw1[j - 1];
-                dqidw1[j] = dqodw2[j - 1];
-            }
-            dhkds1 = 0.5 * dhkdw[j] / SOI(S->watsat, c, j);
-            dhkds2 = 0.5 * dhkdw[j] / SOI(S->watsat, c, j + 1);
-            num = smp[j + 1] - smp[j];
-            den = M_TO_MM * (SNO(S->z, c, j + 1) - SNO(S->z, c, j));
-            qout[j] = -(hk[j] * num / den) + hk[j];
-            dqodw1[j] = (hk[j] * dsmpdw[j] - dhkds1 * num) / den + dhkds1;
-            dqodw2[j] = (-(hk[j] * dsmpdw[j + 1]) - dhkds2 * num) / den + dhkds2;
-        }
-        j = nlayers;
-        qin[j] = qout[j - 1];
-        dqidw0[j] = dqodw1[j - 1];
-        dqidw1[j] = dqodw2[j - 1];
-        if (cfg->lower_boundary_condition == CTSM_BC_FLUX) {
-            qout[j] = hk[j];
-            dqodw1[j] = dhkdw[j] / SOI(S->watsat, c, j);
-        } else { /* bc_zero_flux */
-            qout[j] = 0.0;
-            dqodw1[j] = 0.0;
-        }
-        /* compute_RHS_moisture_form :1871-1937 */
-        for (j = 1; j <= nlayers; ++j) {
-            double fluxNet = qin[j] - qout[j] - SOI(S->qflx_rootsoi, c, j);
-            rmx[j] = -(fluxNet * dt_dz[j]);
-        }
-        /* compute_LHS_moisture_form :1941-2012 */
-        j = 1;
-        amx[j] = 0.0;
-        bmx[j] = -1.0 - (-dqidw1[j] + dqodw1[j]) * dt_dz[j];
-        cmx[j] = -(dqodw2[j] * dt_dz[j]);
-        for (j = 2; j <= nlayers - 1; ++j) {
-            amx[j] = dqidw0[j] * dt_dz[j];
-            bmx[j] = -1.0 - (-dqidw1[j] + dqodw1[j]) * dt_dz[j];
-            cmx[j] = -(dqodw2[j] * dt_dz[j]);
-        }
-        j = nlayers;
-        amx[j] = dqidw0[j] * dt_dz[j];
-        bmx[j] = -1.0 - (-dqidw1[j] + dqodw1[j]) * dt_dz[j];
-        cmx[j] = 0.0;
         /* dgtsv :1275-1299 */
         for (j = 1; j <= nlayers - 1; ++j) {
-            dLow[j - 1] = amx[j + 1];
-            dUpp[j - 1] = cmx[j];
+            dLow[j - 1] = R.amx[j + 1];
+            dUpp[j - 1] = R.cmx[j];
         }
-        for (j = 1; j <= nlayers; ++j) { diag[j - 1] = bmx[j]; rhs[j - 1] = rmx[j]; }
+        for (j = 1; j <= nlayers; ++j) { diag[j - 1] = R.bmx[j]; rhs[j - 1] = R.rmx[j]; }
         if (oracle_dgtsv(nlayers, dLow, diag, dUpp, rhs) != 0) return -2;
         for (j = 1; j <= nlayers; ++j) dwat[j] = rhs[j - 1];
         /* error estimate :1307-1351 */
@@ -349,15 +378,15 @@ static int soilwater_col(const ctsm_config *cfg, lay_t Lo, const ctsm_col_state 
         for (j = 1; j <= nlayers; ++j) {
             if (cfg->flux_calculation == 42) {
                 double qin_test, qout_test;
-                if (j == 1) qin_test = qin[j] + dqidw1[j] * dwat[j];
-                else        qin_test = qin[j] + dqidw0[j] * dwat[j - 1] + dqidw1[j] * dwat[j];
-                if (j == nlayers) qout_test = qout[j] + dqodw1[j] * dwat[j];
-                else              qout_test = qout[j] + dqodw1[j] * dwat[j] + dqodw2[j] * dwat[j + 1];
+                if (j == 1) qin_test = R.qin[j] + R.dqidw1[j] * dwat[j];
+                else        qin_test = R.qin[j] + R.dqidw0[j] * dwat[j - 1] + R.dqidw1[j] * dwat[j];
+                if (j == nlayers) qout_test = R.qout[j] + R.dqodw1[j] * dwat[j];
+                else              qout_test = R.qout[j] + R.dqodw1[j] * dwat[j] + R.dqodw2[j] * dwat[j + 1];
                 fluxNet0[j] = qin_test - qout_test - SOI(S->qflx_rootsoi, c, j);
             } else {
-                fluxNet0[j] = dwat[j] / dt_dz[j];
+                fluxNet0[j] = dwat[j] / R.dt_dz[j];
             }
-            fluxNet1[j] = qin[j] - qout[j] - SOI(S->qflx_rootsoi, c, j);
+            fluxNet1[j] = R.qin[j] - R.qout[j] - SOI(S->qflx_rootsoi, c, j);
         }
         for (j = 1; j <= nlayers; ++j) {
             double e = fabs(fluxNet1[j] - fluxNet0[j]) * dtsub * 0.5;
@@ -370,7 +399,7 @@ static int soilwater_col(const ctsm_config *cfg, lay_t Lo, const ctsm_col_state 
         for (j = 1; j <= nlayers; ++j)
             SNO(S->h2osoi_liq, c, j) = SNO(S->h2osoi_liq, c, j) + dwat[j] * (M_TO_MM * SNO(S->dz, c, j));
         if (cfg->lower_boundary_condition == CTSM_BC_FLUX)
-            qcTemp = hk[nlayers] + dhkdw[nlayers] * dwat[nlayers];
+            qcTemp = R.hk[nlayers] + R.dhkdw[nlayers] * dwat[nlayers];
         else
             qcTemp = 0.0;
         C1(S->qcharge, c) = C1(S->qcharge, c) + qcTemp * (dtsub / dtime);
@@ -388,10 +417,35 @@ static int soilwater_col(const ctsm_config *cfg, lay_t Lo, const ctsm_col_state 
         SNO(S->h2osoi_liq, c, j - 1) = SNO(S->h2osoi_liq, c, j - 1) + over;
     }
     for (j = 1; j <= nlayers; ++j) {
-        SOI(S->qin, c, j) = qin[j];
-        SOI(S->qout, c, j) = qout[j];
+        SOI(S->qin, c, j) = R.qin[j];
+        SOI(S->qout, c, j) = R.qout[j];
     }
     return 0;
+}
+
+/* The trial system of column c at the column's current h2osoi_liq and a
+ * sub-step dtsub.  out[k*nlayers + (j-1)] for k = 0..15 in the order
+ * hk, smp, dhkdw, dsmpdw, imped, qin, qout, dqidw0, dqidw1, dqodw1, dqodw2,
+ * amx, bmx, cmx, rmx, dt_dz.  Exposed so tests can pin the derivatives. */
+int oracle_richards_system(const ctsm_config *cfg, const ctsm_layout *L, const ctsm_col_state *S,
+                           int c, double dtsub, double *out)
+{
+    lay_t Lo = { L->begc_alloc, L->ldc, L->begp_alloc, L->ldp, cfg->nlevsno };
+    const int n = C1(S->nbedrock, c);
+    richards_sys R;
+    double liq[NLEVMAX_SOI + 1];
+    int j;
+    if (n < 2 || n > NLEVMAX_SOI) return -1;
+    for (j = 1; j <= n; ++j) liq[j] = SNO(S->h2osoi_liq, c, j);
+    richards_assemble(cfg, Lo, S, c, n, dtsub, liq, &R);
+    {
+        const double *src[16] = { R.hk, R.smp, R.dhkdw, R.dsmpdw, R.imped, R.qin, R.qout, R.dqidw0,
+                                  R.dqidw1, R.dqodw1, R.dqodw2, R.amx, R.bmx, R.cmx, R.rmx, R.dt_dz };
+        int k;
+        for (k = 0; k < 16; ++k)
+            for (j = 1; j <= n; ++j) out[k * n + (j - 1)] = src[k][j];
+    }
+    return n;
 }
 
 int oracle_soilwater(const ctsm_config *cfg, const ctsm_layout *L, const ctsm_col_state *S,
@@ -468,12 +522,12 @@ static int soiltemp_col(const ctsm_config *cfg, lay_t Lo, const ctsm_col_state *
         satw = (LIQ(j) / DENH2O + ICE(j) / DENICE + 0.0 / DENICE) / (DZ(j) * SOI(S->watsat, c, j));
         satw = fminf90(1.0, satw);
         if (satw > .1e-6) {
-            if (T(j) >= TFRZ) dke = fmaxf90(0.0, log10(satw) + 1.0);
+            if (T(j) >= TFRZ) dke = fmaxf90(0.0, cm_log10(satw) + 1.0);
             else              dke = satw;
             fl = (LIQ(j) / (DENH2O * DZ(j))) /
                  (LIQ(j) / (DENH2O * DZ(j)) + ICE(j) / (DENICE * DZ(j)) + 0.0 / (DENICE * DZ(j)));
-            dksat = SOI(S->tkmg, c, j) * pow(TKWAT, fl * SOI(S->watsat, c, j)) *
-                    pow(TKICE, (1.0 - fl) * SOI(S->watsat, c, j));
+            dksat = SOI(S->tkmg, c, j) * cm_pow(TKWAT, fl * SOI(S->watsat, c, j)) *
+                    cm_pow(TKICE, (1.0 - fl) * SOI(S->watsat, c, j));
             th = dke * dksat + (1.0 - dke) * SOI(S->tkdry, c, j);
         } else {
             th = SOI(S->tkdry, c, j);
@@ -822,7 +876,7 @@ static int soiltemp_col(const ctsm_config *cfg, lay_t Lo, const ctsm_col_state *
             A(supercool, j) = 0.0;
             if (T(j) < TFRZ) {
                 double smp = HFUS * (TFRZ - T(j)) / (GRAV * T(j)) * 1000.0;
-                A(supercool, j) = SOI(S->watsat, c, j) * pow(smp / SOI(S->sucsat, c, j), -1.0 / SOI(S->bsw, c, j));
+                A(supercool, j) = SOI(S->watsat, c, j) * cm_pow(smp / SOI(S->sucsat, c, j), -1.0 / SOI(S->bsw, c, j));
                 A(supercool, j) = A(supercool, j) * DZ(j) * 1000.0;
             }
             if (LIQ(j) > A(supercool, j) && T(j) < TFRZ) {
@@ -1003,3 +1057,56 @@ void oracle_config_default(ctsm_config *cfg)
     cfg->dtmin = 60.0; cfg->verySmall = 1.e-8; cfg->xTolerUpper = 1.e-1; cfg->xTolerLower = 1.e-2;
     cfg->e_ice = 6.0;
 }
+
+/* ========================================================================= */
+/* Soil energy balance error of SoilFluxes (SoilFluxesMod.F90:341-351 for
+ * eflx_soil_grnd of non-urban patches, :399-425 for errsoi_patch), the
+ * reference's own acceptance check of the heat solve (BalanceCheckMod.F90:
+ * 1103-1121 warns above 1e-5 W/m2 and aborts above 1e-4 W/m2).  `S` is the
+ * state after SoilTemperature; tssbef, t_h2osfc_bef and t_grnd0 are the
+ * values before it.  qflx_ev_* corrections of SoilFluxes are zero here (the
+ * callers' patches carry no evaporation adjustment).  errsoi[p - begp]. */
+int oracle_errsoi(const ctsm_config *cfg, const ctsm_layout *L, const ctsm_col_state *S,
+                  const ctsm_patch_state *P, const double *tssbef, const double *t_h2osfc_bef,
+                  const double *t_grnd0, int num_filter, const int *filter, double dtime, double *errsoi)
+{
+    lay_t Lo = { L->begc_alloc, L->ldc, L->begp_alloc, L->ldp, cfg->nlevsno };
+    int fc, p, j;
+    for (fc = 0; fc < num_filter; ++fc) {
+        int c = filter ? filter[fc] : L->begc_alloc + fc;
+        int snl = C1(S->snl, c);
+        double fs = C1(S->frac_sno_eff, c), fh = C1(S->frac_h2osfc, c), emg = C1(S->emg, c);
+        double tb_top = SNO(tssbef, c, snl + 1), tb1 = SNO(tssbef, c, 1), thb = C1(t_h2osfc_bef, c);
+        double tbt2 = tb_top * tb_top, tb12 = tb1 * tb1, thb2 = thb * thb;
+        double lw_grnd = (fs * (tbt2 * tbt2) + (1.0 - fs - fh) * (tb12 * tb12) + fh * (thb2 * thb2));
+        double tg0 = C1(t_grnd0, c), tinc = C1(S->t_grnd, c) - tg0;
+        double tg02 = tg0 * tg0;
+        for (p = C1(S->patchi, c); p <= C1(S->patchf, c); ++p) {
+            double e = ((1.0 - fs) * P1(P->sabg_soil, p) + fs * P1(P->sabg_snow, p)) + P1(P->dlrad, p)
+                       + (1 - P1(P->frac_veg_nosno, p)) * emg * C1(S->forc_lwrad, c)
+                       - emg * SB * lw_grnd - emg * SB * (tg02 * tg0) * (4.0 * tinc)
+                       - (P1(P->eflx_sh_grnd, p) + P1(P->qflx_evap_soi, p) * C1(S->htvp, c));
+            e = e - C1(S->xmf, c) - C1(S->xmf_h2osfc, c)
+                - fh * (C1(S->t_h2osfc, c) - thb) * (C1(S->c_h2osfc, c) / dtime);
+            e = e + C1(S->eflx_h2osfc_to_snow, c);
+            for (j = -cfg->nlevsno + 1; j <= cfg->nlevgrnd; ++j) {
+                if (j >= snl + 1 && j < 1)
+                    e = e - fs * (SNO(S->t_soisno, c, j) - SNO(tssbef, c, j)) / SNO(S->fact, c, j);
+                if (j >= 1)
+                    e = e - (SNO(S->t_soisno, c, j) - SNO(tssbef, c, j)) / SNO(S->fact, c, j);
+            }
+            errsoi[p - Lo.begp] = e;
+        }
+    }
+    return 0;
+}
+
+/* vectorised elementary functions for tests/test_math.py */
+void oracle_cm_pow(int n, const double *x, const double *y, double *out)
+{ int i; for (i = 0; i < n; ++i) out[i] = cm_pow(x[i], y[i]); }
+void oracle_cm_log(int n, const double *x, double *out)
+{ int i; for (i = 0; i < n; ++i) out[i] = cm_log(x[i]); }
+void oracle_cm_exp(int n, const double *x, double *out)
+{ int i; for (i = 0; i < n; ++i) out[i] = cm_exp(x[i]); }
+void oracle_cm_log10(int n, const double *x, double *out)
+{ int i; for (i = 0; i < n; ++i) out[i] = cm_log10(x[i]); }

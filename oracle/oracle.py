@@ -100,3 +100,41 @@ def soilwater_fractions(st, cfg=None, filter_=None):
     f, fp = _filt(filter_)
     n = st.dims.ncol if f is None else len(f)
     return lib().oracle_soilwater_fractions(C.byref(cfg or config()), C.byref(lay), C.byref(cs), n, fp)
+
+
+_SYS_NAMES = ["hk", "smp", "dhkdw", "dsmpdw", "imped", "qin", "qout", "dqidw0", "dqidw1", "dqodw1",
+              "dqodw2", "amx", "bmx", "cmx", "rmx", "dt_dz"]
+
+
+def richards_system(st, col, dtsub, cfg=None):
+    """The oracle's per-trial Richards system of 1-based column `col` at its
+    current water content: dict of 1-D arrays over the column's layers."""
+    L = lib()
+    L.oracle_richards_system.argtypes = [C.POINTER(abi.Config), C.POINTER(abi.Layout), C.POINTER(abi.ColState),
+                                         C.c_int, C.c_double, abi.f64p]
+    cs, _ = st.structs()
+    out = np.zeros(16 * 64)
+    n = L.oracle_richards_system(C.byref(cfg or config()), C.byref(st.layout()), C.byref(cs), col, dtsub,
+                                 out.ctypes.data_as(abi.f64p))
+    assert n > 0
+    return {k: out[i * n:(i + 1) * n].copy() for i, k in enumerate(_SYS_NAMES)}
+
+
+def errsoi(st, tssbef, t_h2osfc_bef, t_grnd0, cfg=None, dtime=1800.0, filter_=None):
+    """Per-patch soil energy balance error (SoilFluxesMod.F90:399-425) of a
+    state after SoilTemperature, given the before-step temperatures."""
+    L = lib()
+    L.oracle_errsoi.argtypes = [C.POINTER(abi.Config), C.POINTER(abi.Layout), C.POINTER(abi.ColState),
+                                C.POINTER(abi.PatchState), abi.f64p, abi.f64p, abi.f64p, C.c_int, abi.i32p,
+                                C.c_double, abi.f64p]
+    cs, ps = st.structs()
+    f, fp = _filt(filter_)
+    n = st.dims.ncol if f is None else len(f)
+    tss = np.ascontiguousarray(tssbef, dtype=np.float64)
+    thb = np.ascontiguousarray(t_h2osfc_bef, dtype=np.float64)
+    tg0 = np.ascontiguousarray(t_grnd0, dtype=np.float64)
+    err = np.zeros(st.dims.npatch)
+    L.oracle_errsoi(C.byref(cfg or config()), C.byref(st.layout()), C.byref(cs), C.byref(ps),
+                    tss.ctypes.data_as(abi.f64p), thb.ctypes.data_as(abi.f64p), tg0.ctypes.data_as(abi.f64p),
+                    n, fp, C.c_double(dtime), err.ctypes.data_as(abi.f64p))
+    return err

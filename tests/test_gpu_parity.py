@@ -62,17 +62,20 @@ def test_soiltemperature_1step_uneven(ctx):
 
 
 def _run_device_steps(ctx, st, nsteps, plain=False, filter_perm=None):
-    import torch
     from ctsm_amd.state import DeviceState
-    ds = DeviceState(st)
+    ds = DeviceState(st, ctx)
     f = None
     if filter_perm is not None:
-        f = torch.from_numpy(np.ascontiguousarray(filter_perm, dtype=np.int32)).cuda()
+        f = ctx.device_array(np.ascontiguousarray(filter_perm, dtype=np.int32))
     structs = ds.structs()
     for _ in range(nsteps):
         ctx.step_dev(ds, filter_dev=f, plain=plain, structs=structs)
     ctx.sync()
-    return ds.to_host()
+    out = ds.to_host()
+    ds.free()
+    if f is not None:
+        ctx.free(f)
+    return out
 
 
 @pytest.mark.parametrize("nsteps,tol", [(1, TOL1), (48, TOL48)])
@@ -138,6 +141,7 @@ def test_standing_water_freeze(ctx):
     """frac_h2osfc > 0 with surface water below freezing exercises PhaseChangeH2osfc."""
     st = synthetic.soil_only(512, 77, snow=True, bedrock=True, frac_h2osfc=np.full(512, 0.3))
     st.col["t_h2osfc"][:] = 265.0
+    synthetic.set_t_grnd_consistent(st)
     g, o = st.copy(), st.copy()
     print("case: standing water freezing, 512 columns", flush=True)
     ctx.soiltemperature(g)
