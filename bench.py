@@ -86,6 +86,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-sort", action="store_true", help="keep generation order (no binning)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--lanes", type=int, default=0, help="force lanes per column in the Richards solve")
+    ap.add_argument("--single-kernel", action="store_true", help="force the one-kernel fused step")
+    ap.add_argument("--order", choices=["bins", "substeps"], default="substeps",
+                    help="column order: (bedrock, snow) bins, or also by the sub-steps of the warm-up step")
+    ap.add_argument("--breakdown", action="store_true",
+                    help="also time SoilTemperature and SoilWater launched separately")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="run a streaming copy kernel of this size (MB) after the steps, "
+                         "to calibrate the HBM byte counters of a profiler run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
@@ -98,17 +108,32 @@ def main():
     st = synthetic.workload("uneven", ncol=args.ncol)
     ctx = lib.Context(device=0)
 
+    from ctsm_amd import autotune as AT
+
     filt = None
     if not args.no_sort:
-        perm = np.lexsort((st.col["snl"], st.col["nbedrock"])) + 1
-        filt = ctx.device_array(perm.astype(np.int32))
+        filt = ctx.device_array(AT.column_order(st.col["snl"], st.col["nbedrock"]))
+
+    tuning = None
+    if args.lanes or args.single_kernel:
+        ctx.set_tuning(richards_lanes=args.lanes or 1, fused_single_kernel=int(args.single_kernel))
+    elif not args.no_autotune:
+        ds0 = DeviceState(st, ctx)
+        tuning, _, table = AT.autotune(ctx, ds0, filter_dev=filt, steps=2)
+        ds0.free()
 
     def run(plain):
+        nonlocal filt
         ds = DeviceState(st, ctx)
         structs = ds.structs()
         for _ in range(args.warmup):
             ctx.step_dev(ds, filter_dev=filt, plain=plain, structs=structs)
         ctx.sync()
+        if args.order == "substeps" and not args.no_sort:
+            # order columns by the sub-steps they took in the last warm-up step
+            nsub = np.empty(st.dims.ncol)
+            ctx.d2h(nsub, ds.col["num_substeps"])
+            ctx.h2d(filt, AT.column_order(st.col["snl"], st.col["nbedrock"], nsub))
         # events on the library's own stream, around each step's launch(es)
         starts, stops = [], []
         t0 = time.perf_counter()
@@ -126,13 +151,41 @@ def main():
     wall, step_ms, ds = run(plain=False)
     ds.free()
     value = args.ncol * args.steps / wall
-    res = {"fused_step_ms": step_ms}
+    res = {"fused_step_ms": step_ms, "tuning": ctx.get_tuning()}
     if not args.no_plain:
         wall_p, step_ms_p, dsp = run(plain=True)
         dsp.free()
         res["plain_step_ms"] = step_ms_p
         res["plain_column_steps_per_s"] = args.ncol * args.steps / wall_p
         res["fused_over_plain"] = value / res["plain_column_steps_per_s"]
+
+    if args.breakdown:
+        ds = DeviceState(st, ctx)
+        for name, fn in (("soiltemperature_fused_kernel_ms", lambda: ctx.soiltemperature_dev(ds, filt)),
+                         ("soilwater_fused_kernel_ms", lambda: ctx.soilwater_dev(ds, filt))):
+            fn()
+            ctx.sync()
+            e0 = ctx.record()
+            for _ in range(3):
+                fn()
+            e1 = ctx.record()
+            ctx.sync()
+            res[name] = ctx.elapsed_ms(e0, e1) / 3
+            ctx.destroy_event(e0)
+            ctx.destroy_event(e1)
+        ds.free()
+    if args.calib_copy_mb > 0:
+        nb = args.calib_copy_mb << 20
+        a, b = ctx.alloc(nb), ctx.alloc(nb)
+        ctx.copy(b, a, nb)
+        e0 = ctx.record()
+        ctx.copy(b, a, nb)
+        e1 = ctx.record()
+        ctx.sync()
+        ms = ctx.elapsed_ms(e0, e1)
+        res["copy_calib"] = {"bytes_read": nb, "bytes_written": nb, "ms": ms, "GBps": 2 * nb / ms / 1e6}
+        ctx.free(a)
+        ctx.free(b)
 
     bytes_alg = algorithmic_bytes(st)
     achieved = bytes_alg / (step_ms * 1e-3) / 1e9

@@ -31,6 +31,15 @@ class Config(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [
+        ("richards_lanes", C.c_int32),
+        ("richards_block", C.c_int32),
+        ("soiltemp_block", C.c_int32),
+        ("fused_single_kernel", C.c_int32),
+    ]
+
+
 class Layout(C.Structure):
     _fields_ = [
         ("begc_alloc", C.c_int32),

@@ -1,0 +1,95 @@
+"""Launch-shape autotuner and column binning for the soil step.
+
+Two decisions shape the step on an MI355X, and neither changes a single bit
+of the results (columns are independent; tests/test_gpu_parity.py checks every
+shape against the oracle):
+
+  * column order: the Richards sub-step count of a column varies from 1 to
+    ~35 trials and a wavefront runs as long as its slowest lane, so columns
+    are ordered by (bedrock depth, sub-steps taken in the previous step,
+    snow layers) -- the previous step is a good predictor of the next;
+  * launch shape: lanes per column in the Richards solve (1, 2, 4, 8), block
+    sizes, and one fused kernel vs. heat-solve kernel + Richards kernel.
+
+autotune() times each candidate shape for a few steps on a snapshot of the
+resident state (restored between candidates with device copies) and keeps
+the fastest on the context.
+"""
+import numpy as np
+
+DEFAULT_CANDIDATES = [
+    dict(richards_lanes=8, richards_block=128, soiltemp_block=128, fused_single_kernel=0),
+    dict(richards_lanes=8, richards_block=256, soiltemp_block=64, fused_single_kernel=0),
+    dict(richards_lanes=4, richards_block=128, soiltemp_block=128, fused_single_kernel=0),
+    dict(richards_lanes=2, richards_block=128, soiltemp_block=128, fused_single_kernel=0),
+    dict(richards_lanes=1, richards_block=64, soiltemp_block=128, fused_single_kernel=0),
+    dict(richards_lanes=1, richards_block=64, soiltemp_block=64, fused_single_kernel=1),
+]
+
+
+def column_order(snl, nbedrock, prev_substeps=None):
+    """1-based column order that groups similar work into wavefronts."""
+    snl = np.asarray(snl)
+    nb = np.asarray(nbedrock)
+    if prev_substeps is None:
+        keys = (snl, nb)
+    else:
+        keys = (snl, np.asarray(prev_substeps), nb)
+    return (np.lexsort(keys) + 1).astype(np.int32)
+
+
+def snapshot(ctx, ds):
+    """Device copy of every field of a DeviceState (returns dict of pointers)."""
+    snap = {}
+    for group, d in (("c", ds.col), ("p", ds.pch)):
+        for k, ptr in d.items():
+            shape, dt = ds._shapes[(group, k)]
+            nb = int(np.prod(shape)) * np.dtype(dt).itemsize
+            q = ctx.alloc(nb)
+            ctx.copy(q, ptr, nb)
+            snap[(group, k)] = (q, nb)
+    ctx.sync()
+    return snap
+
+
+def restore(ctx, ds, snap):
+    for (group, k), (q, nb) in snap.items():
+        dst = ds.col[k] if group == "c" else ds.pch[k]
+        ctx.copy(dst, q, nb)
+
+
+def free_snapshot(ctx, snap):
+    ctx.sync()
+    for q, _ in snap.values():
+        ctx.free(q)
+
+
+def autotune(ctx, ds, filter_dev=None, n=None, steps=2, candidates=None, dtime=1800.0, verbose=False):
+    """Pick the fastest launch shape for this state.  Leaves `ds` unchanged."""
+    candidates = candidates or DEFAULT_CANDIDATES
+    snap = snapshot(ctx, ds)
+    structs = ds.structs()
+    best, best_ms, table = None, float("inf"), []
+    for cand in candidates:
+        ctx.set_tuning(**cand)
+        restore(ctx, ds, snap)
+        ctx.step_dev(ds, filter_dev=filter_dev, n=n, dtime=dtime, structs=structs)   # warm
+        ctx.sync()
+        restore(ctx, ds, snap)
+        e0 = ctx.record()
+        for _ in range(steps):
+            ctx.step_dev(ds, filter_dev=filter_dev, n=n, dtime=dtime, structs=structs)
+        e1 = ctx.record()
+        ctx.sync()
+        ms = ctx.elapsed_ms(e0, e1) / steps
+        ctx.destroy_event(e0)
+        ctx.destroy_event(e1)
+        table.append((dict(cand), ms))
+        if verbose:
+            print(f"autotune {cand} -> {ms:.3f} ms/step", flush=True)
+        if ms < best_ms:
+            best, best_ms = cand, ms
+    restore(ctx, ds, snap)
+    free_snapshot(ctx, snap)
+    ctx.set_tuning(**best)
+    return best, best_ms, table

@@ -16,7 +16,11 @@
 #include "common.h"
 
 namespace ctsm {
-void launch_soilwater(hipStream_t, const View &, const Params &, const ctsm_col_state &, int, const int32_t *, double, ErrWord *);
+void launch_soilwater(hipStream_t, const View &, const Params &, const ctsm_col_state &, int, const int32_t *, double, ErrWord *,
+                      int lanes, int block);
+void launch_soiltemp_fractions(hipStream_t, const View &, const Params &, const ctsm_col_state &,
+                               const ctsm_patch_state &, int, const int32_t *, double, double *, long, ErrWord *,
+                               int block);
 void launch_soilwater_fractions(hipStream_t, const View &, const ctsm_col_state &, int, const int32_t *);
 void launch_soiltemp(hipStream_t, const View &, const Params &, const ctsm_col_state &, const ctsm_patch_state &, int,
                      const int32_t *, double, double *, long, ErrWord *);
@@ -114,6 +118,18 @@ struct DevBuf {
     }
 };
 
+// Streaming copy: 16 B per lane per iteration, grid-stride.
+__global__ void __launch_bounds__(256) k_copy16(uint4 *__restrict__ dst, const uint4 *__restrict__ src, size_t n16)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+__global__ void k_copy1(char *__restrict__ dst, const char *__restrict__ src, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void k_reset_err(ErrWord *e)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -128,6 +144,7 @@ struct ctsm_ctx {
     int device = 0;
     ctsm_config cfg{};
     Params prm{};
+    ctsm_tuning tun{4, 128, 128, 0};
     hipStream_t stream = nullptr;
     ErrWord *err = nullptr;            // device
     DevBuf work;                       // heat-solve workspace
@@ -303,7 +320,8 @@ int run_host(ctsm_ctx *ctx, Routine which, const ctsm_layout *L, const ctsm_col_
     View V = make_view(ctx, dl);
     switch (which) {
     case R_SOILWATER:
-        launch_soilwater(ctx->stream, V, ctx->prm, dcol, n, dfilt, dtime, ctx->err);
+        launch_soilwater(ctx->stream, V, ctx->prm, dcol, n, dfilt, dtime, ctx->err, ctx->tun.richards_lanes,
+                         ctx->tun.richards_block);
         break;
     case R_FRACTIONS:
         launch_soilwater_fractions(ctx->stream, V, dcol, n, dfilt);
@@ -398,6 +416,24 @@ int ctsm_destroy(ctsm_ctx *ctx)
     return CTSM_OK;
 }
 
+int ctsm_set_tuning(ctsm_ctx *ctx, const ctsm_tuning *t)
+{
+    if (!ctx || !t) return CTSM_E_ARG;
+    if (t->richards_lanes != 1 && t->richards_lanes != 2 && t->richards_lanes != 4 && t->richards_lanes != 8)
+        return CTSM_E_ARG;
+    if (t->richards_block < 0 || t->richards_block > 256 || (t->richards_block % 64) != 0) return CTSM_E_ARG;
+    if (t->soiltemp_block < 0 || t->soiltemp_block > 128 || (t->soiltemp_block % 64) != 0) return CTSM_E_ARG;
+    ctx->tun = *t;
+    return CTSM_OK;
+}
+
+int ctsm_get_tuning(const ctsm_ctx *ctx, ctsm_tuning *t)
+{
+    if (!ctx || !t) return CTSM_E_ARG;
+    *t = ctx->tun;
+    return CTSM_OK;
+}
+
 int ctsm_last_error(const ctsm_ctx *ctx, int32_t *code, int32_t *index)
 {
     if (!ctx) return CTSM_E_ARG;
@@ -438,7 +474,8 @@ int ctsm_soilwater_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_sta
         launch_soilwater_plain(ctx->stream, make_view(ctx, *lay), ctx->prm, *col, n, filter_dev, dtime,
                                (double *)ctx->work.p, n, ctx->err);
     } else {
-        launch_soilwater(ctx->stream, make_view(ctx, *lay), ctx->prm, *col, n, filter_dev, dtime, ctx->err);
+        launch_soilwater(ctx->stream, make_view(ctx, *lay), ctx->prm, *col, n, filter_dev, dtime, ctx->err,
+                         ctx->tun.richards_lanes, ctx->tun.richards_block);
     }
     return finish_launch(ctx);
 }
@@ -478,8 +515,16 @@ int ctsm_soil_step_fused_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_c
     if (n == 0) return CTSM_OK;
     int rc;
     if ((rc = ensure_work(ctx, soiltemp_workspace_doubles(n)))) return rc;
-    launch_soil_step(ctx->stream, make_view(ctx, *lay), ctx->prm, *col, *pch, n, filter_dev, dtime,
-                     (double *)ctx->work.p, n, ctx->err);
+    const View V = make_view(ctx, *lay);
+    if (ctx->tun.fused_single_kernel) {
+        launch_soil_step(ctx->stream, V, ctx->prm, *col, *pch, n, filter_dev, dtime, (double *)ctx->work.p, n,
+                         ctx->err);
+    } else {
+        launch_soiltemp_fractions(ctx->stream, V, ctx->prm, *col, *pch, n, filter_dev, dtime, (double *)ctx->work.p,
+                                  n, ctx->err, ctx->tun.soiltemp_block);
+        launch_soilwater(ctx->stream, V, ctx->prm, *col, n, filter_dev, dtime, ctx->err, ctx->tun.richards_lanes,
+                         ctx->tun.richards_block);
+    }
     return finish_launch(ctx);
 }
 
@@ -515,6 +560,24 @@ int ctsm_memcpy(ctsm_ctx *ctx, void *dst, const void *src, uint64_t bytes, int32
     hipError_t e = hipMemcpyAsync(dst, src, bytes, k, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     return e == hipSuccess ? CTSM_OK : hip_fail(ctx, e);
+}
+
+int ctsm_dev_copy(ctsm_ctx *ctx, void *dst, const void *src, uint64_t bytes)
+{
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return CTSM_E_ARG;
+    if (bytes == 0) return CTSM_OK;
+    const bool aligned = (((uintptr_t)dst | (uintptr_t)src | (uintptr_t)bytes) & 15) == 0;
+    if (aligned) {
+        const size_t n16 = bytes / 16;
+        const size_t blocks = std::min<size_t>((n16 + 255) / 256, 256 * 8 * 4);
+        hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, (uint4 *)dst,
+                           (const uint4 *)src, n16);
+    } else {
+        const size_t blocks = std::min<size_t>((bytes + 255) / 256, 256 * 8 * 4);
+        hipLaunchKernelGGL(k_copy1, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, (char *)dst,
+                           (const char *)src, (size_t)bytes);
+    }
+    return finish_launch(ctx);
 }
 
 int ctsm_event_record(ctsm_ctx *ctx, void **event)

@@ -610,7 +610,23 @@ k_soiltemp(View V, Params P, ctsm_col_state S, ctsm_patch_state Pt, int n,
 }
 
 
-// The fused soil step: SoilTemperature -> SetSoilWaterFractions -> SoilWater.
+// SoilTemperature followed by SetSoilWaterFractions, one column per lane: the
+// first kernel of the two-kernel soil step (the Richards solve follows with G
+// lanes per column).
+__global__ void __launch_bounds__(128)
+k_soiltemp_fractions(View V, Params P, ctsm_col_state S, ctsm_patch_state Pt, int n,
+                     const int32_t *__restrict__ filter, double dtime, TempWork wk, ErrWord *err)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = filter ? filter[i] : V.begc + i;
+    if (!col_in_range(V, c)) { report_error(err, c, CTSM_E_ARG); return; }
+    int rc = soiltemp_column(V, P, S, Pt, c, dtime, wk, i);
+    if (rc == 0) rc = soilwater_fractions_column(V, S, c);
+    if (rc) report_error(err, c, CTSM_E_SOLVER);
+}
+
+// The fused soil step in one kernel: SoilTemperature -> SetSoilWaterFractions -> SoilWater.
 __global__ void __launch_bounds__(64)
 k_soil_step(View V, Params P, ctsm_col_state S, ctsm_patch_state Pt, int n,
             const int32_t *__restrict__ filter, double dtime, TempWork wk, ErrWord *err)
@@ -645,6 +661,17 @@ void launch_soil_step(hipStream_t st, const View &V, const Params &P, const ctsm
     const int bs = 64;
     TempWork wk{ws, ld};
     hipLaunchKernelGGL(k_soil_step, dim3((n + bs - 1) / bs), dim3(bs), 0, st, V, P, S, Pt, n, filter, dtime, wk, err);
+}
+
+void launch_soiltemp_fractions(hipStream_t st, const View &V, const Params &P, const ctsm_col_state &S,
+                               const ctsm_patch_state &Pt, int n, const int32_t *filter, double dtime,
+                               double *ws, long ld, ErrWord *err, int block)
+{
+    if (n <= 0) return;
+    if (block <= 0 || block > 128) block = 128;
+    TempWork wk{ws, ld};
+    hipLaunchKernelGGL(k_soiltemp_fractions, dim3((n + block - 1) / block), dim3(block), 0, st, V, P, S, Pt, n,
+                       filter, dtime, wk, err);
 }
 
 }  // namespace ctsm

@@ -20,6 +20,7 @@
 // what keeps a wavefront's lanes doing the same amount of work.
 #include "common.h"
 #include "soilwater_dev.h"
+#include "richards_group.h"
 
 namespace ctsm {
 
@@ -52,15 +53,38 @@ k_soilwater_fractions(View V, ctsm_col_state S, int n, const int32_t *__restrict
     }
 }
 
+// G lanes per column (richards_group.h).  Groups are aligned to G lanes, so a
+// column's lanes retire together.
+template <int G>
+__global__ void __launch_bounds__(256)
+k_soilwater_g(View V, Params P, ctsm_col_state S, int n, const int32_t *__restrict__ filter, double dtime,
+              ErrWord *err)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t / G, g = t % G;
+    if (i >= n) return;
+    const int c = filter ? filter[i] : V.begc + i;
+    if (!col_in_range(V, c)) { if (g == 0) report_error(err, c, CTSM_E_ARG); return; }
+    const int rc = richards_group<G>(V, P, S, c, dtime, g);
+    if (rc && g == 0) report_error(err, c, CTSM_E_SOLVER);
+}
+
 }  // namespace ctsm
 
 namespace ctsm {
 void launch_soilwater(hipStream_t st, const View &V, const Params &P, const ctsm_col_state &S, int n,
-                      const int32_t *filter, double dtime, ErrWord *err)
+                      const int32_t *filter, double dtime, ErrWord *err, int lanes, int block)
 {
     if (n <= 0) return;
-    const int bs = 64;
-    hipLaunchKernelGGL(k_soilwater, dim3((n + bs - 1) / bs), dim3(bs), 0, st, V, P, S, n, filter, dtime, err);
+    if (block <= 0 || block > 256) block = (lanes == 1) ? 64 : 128;
+    const long threads = (long)n * (lanes > 0 ? lanes : 1);
+    const dim3 grid((unsigned)((threads + block - 1) / block)), blk(block);
+    switch (lanes) {
+    case 2: hipLaunchKernelGGL(k_soilwater_g<2>, grid, blk, 0, st, V, P, S, n, filter, dtime, err); break;
+    case 4: hipLaunchKernelGGL(k_soilwater_g<4>, grid, blk, 0, st, V, P, S, n, filter, dtime, err); break;
+    case 8: hipLaunchKernelGGL(k_soilwater_g<8>, grid, blk, 0, st, V, P, S, n, filter, dtime, err); break;
+    default: hipLaunchKernelGGL(k_soilwater, grid, blk, 0, st, V, P, S, n, filter, dtime, err); break;
+    }
 }
 void launch_soilwater_fractions(hipStream_t st, const View &V, const ctsm_col_state &S, int n,
                                 const int32_t *filter)

@@ -23,10 +23,10 @@ _NAMES = {1: "CTSM_E_ARG", 2: "CTSM_E_CONFIG", 3: "CTSM_E_HIP", 4: "CTSM_E_SOLVE
 
 # every symbol declared in include/ctsm_gpu.h
 EXPORTS = [
-    "ctsm_config_default", "ctsm_create", "ctsm_destroy", "ctsm_last_error",
+    "ctsm_config_default", "ctsm_create", "ctsm_destroy", "ctsm_last_error", "ctsm_set_tuning", "ctsm_get_tuning",
     "ctsm_soilwater", "ctsm_soiltemperature", "ctsm_soilwater_fractions",
     "ctsm_soilwater_dev", "ctsm_soiltemperature_dev", "ctsm_soilwater_fractions_dev",
-    "ctsm_soil_step_fused_dev", "ctsm_dev_alloc", "ctsm_dev_free", "ctsm_memcpy",
+    "ctsm_soil_step_fused_dev", "ctsm_dev_alloc", "ctsm_dev_free", "ctsm_memcpy", "ctsm_dev_copy",
     "ctsm_event_record", "ctsm_event_elapsed", "ctsm_event_destroy", "ctsm_sync", "ctsm_stream",
 ]
 
@@ -52,6 +52,8 @@ def load():
     L.ctsm_create.argtypes = [P(abi.Config), C.c_int, P(C.c_void_p)]
     L.ctsm_destroy.argtypes = [C.c_void_p]
     L.ctsm_last_error.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_int32)]
+    L.ctsm_set_tuning.argtypes = [C.c_void_p, P(abi.Tuning)]
+    L.ctsm_get_tuning.argtypes = [C.c_void_p, P(abi.Tuning)]
     col = [C.c_void_p, P(abi.Layout), P(abi.ColState)]
     L.ctsm_soilwater.argtypes = col + [C.c_int32, abi.i32p, C.c_double]
     L.ctsm_soilwater_fractions.argtypes = col + [C.c_int32, abi.i32p]
@@ -66,6 +68,7 @@ def load():
     L.ctsm_dev_alloc.argtypes = [C.c_void_p, C.c_uint64, P(C.c_void_p)]
     L.ctsm_dev_free.argtypes = [C.c_void_p, C.c_void_p]
     L.ctsm_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_int32]
+    L.ctsm_dev_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]
     L.ctsm_event_record.argtypes = [C.c_void_p, P(C.c_void_p)]
     L.ctsm_event_elapsed.argtypes = [C.c_void_p, C.c_void_p, P(C.c_float)]
     L.ctsm_event_destroy.argtypes = [C.c_void_p]
@@ -110,6 +113,18 @@ class Context:
             self.L.ctsm_last_error(self.h, C.byref(code), C.byref(idx))
             raise CtsmError(rc, idx.value, where)
 
+    def set_tuning(self, **kw):
+        t = abi.Tuning()
+        self.L.ctsm_get_tuning(self.h, C.byref(t))
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        self._check(self.L.ctsm_set_tuning(self.h, C.byref(t)), "ctsm_set_tuning")
+
+    def get_tuning(self):
+        t = abi.Tuning()
+        self.L.ctsm_get_tuning(self.h, C.byref(t))
+        return {k: getattr(t, k) for k, _ in abi.Tuning._fields_}
+
     @property
     def stream(self):
         return self.L.ctsm_stream(self.h)
@@ -150,6 +165,10 @@ class Context:
         assert arr.flags.c_contiguous
         self._check(self.L.ctsm_memcpy(self.h, arr.ctypes.data_as(C.c_void_p), C.c_void_p(dptr), arr.nbytes, 2),
                     "ctsm_memcpy")
+
+    def copy(self, dst, src, nbytes):
+        """Kernel copy device -> device on the library stream (enqueues)."""
+        self._check(self.L.ctsm_dev_copy(self.h, C.c_void_p(dst), C.c_void_p(src), int(nbytes)), "ctsm_dev_copy")
 
     def device_array(self, arr):
         """Copy a numpy array to a new device buffer; returns the pointer."""

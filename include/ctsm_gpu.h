@@ -195,6 +195,18 @@ typedef struct ctsm_patch_state {
     double        *sabg_chk;      /* (p) out                                             */
 } ctsm_patch_state;
 
+/* ---- launch shapes (no counterpart in the reference: set by the autotuner) --
+ * richards_lanes: lanes cooperating on one column in the Richards solve
+ *   (1, 2, 4 or 8); the heat solve always runs one column per lane.
+ * fused_single_kernel: 1 = the whole soil step in one kernel (one lane per
+ *   column), 0 = heat solve kernel + Richards kernel (2 launches per step). */
+typedef struct ctsm_tuning {
+    int32_t richards_lanes;
+    int32_t richards_block;
+    int32_t soiltemp_block;
+    int32_t fused_single_kernel;
+} ctsm_tuning;
+
 typedef struct ctsm_ctx ctsm_ctx;
 
 /* ---- set-up --------------------------------------------------------------- */
@@ -210,6 +222,9 @@ void ctsm_config_default(ctsm_config *cfg);
  * flux / zero flux, upper BC other than flux). */
 int ctsm_create(const ctsm_config *cfg, int device, ctsm_ctx **out);
 int ctsm_destroy(ctsm_ctx *ctx);
+
+int ctsm_set_tuning(ctsm_ctx *ctx, const ctsm_tuning *t);
+int ctsm_get_tuning(const ctsm_ctx *ctx, ctsm_tuning *t);
 
 /* First failing column (reference 1-based index) and code of the last call. */
 int ctsm_last_error(const ctsm_ctx *ctx, int32_t *code, int32_t *index);
@@ -270,6 +285,9 @@ int ctsm_dev_alloc(ctsm_ctx *ctx, uint64_t bytes, void **out);
 int ctsm_dev_free(ctsm_ctx *ctx, void *ptr);
 /* kind: 1 = host->device, 2 = device->host, 3 = device->device.  Synchronous. */
 int ctsm_memcpy(ctsm_ctx *ctx, void *dst, const void *src, uint64_t bytes, int32_t kind);
+/* Device-to-device copy by a streaming kernel on the context's stream
+ * (enqueues only; used for state snapshots). */
+int ctsm_dev_copy(ctsm_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 /* Records an event on the context's stream (created on first use). */
 int ctsm_event_record(ctsm_ctx *ctx, void **event);
 /* Milliseconds between two recorded events (waits for the second). */

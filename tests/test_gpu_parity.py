@@ -173,3 +173,30 @@ def test_bad_column_reports_index(ctx):
     with pytest.raises(CtsmError) as ei:
         ctx.soilwater(st)
     assert ei.value.index == 42
+
+
+@pytest.mark.parametrize("shape", [
+    dict(richards_lanes=1, fused_single_kernel=1),
+    dict(richards_lanes=1, fused_single_kernel=0),
+    dict(richards_lanes=2, fused_single_kernel=0),
+    dict(richards_lanes=4, fused_single_kernel=0),
+    dict(richards_lanes=8, fused_single_kernel=0),
+    dict(richards_lanes=8, richards_block=256, soiltemp_block=64, fused_single_kernel=0),
+])
+def test_launch_shapes_match_oracle_bitwise(ctx, shape):
+    """Every launch shape the autotuner may pick gives the oracle's bits."""
+    st = synthetic.workload("uneven", ncol=1536)
+    print(f"case: launch shape {shape}, 1,536 uneven columns, 3 steps", flush=True)
+    ctx.set_tuning(**shape)
+    try:
+        perm = np.lexsort((st.col["snl"], st.col["nbedrock"])) + 1
+        g = _run_device_steps(ctx, st, 3, filter_perm=perm)
+    finally:
+        ctx.set_tuning(richards_lanes=8, richards_block=128, soiltemp_block=128, fused_single_kernel=0)
+    o = st.copy()
+    for _ in range(3):
+        assert O.soil_step(o)[0] == 0
+    compare_states(g, o, 0.0, ST_FIELDS + ["qcharge", "num_substeps"])
+    m = soil_mask(o)
+    for f in ("smp_l", "hk_l", "qin", "qout"):
+        assert _soi(g, o, f, m) == 0.0, f
