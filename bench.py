@@ -114,7 +114,7 @@ def main():
     if not args.no_sort:
         filt = ctx.device_array(AT.column_order(st.col["snl"], st.col["nbedrock"]))
 
-    tuning = None
+    tuning, table = None, []
     if args.lanes or args.single_kernel:
         ctx.set_tuning(richards_lanes=args.lanes or 1, fused_single_kernel=int(args.single_kernel))
     elif not args.no_autotune:
@@ -152,6 +152,8 @@ def main():
     ds.free()
     value = args.ncol * args.steps / wall
     res = {"fused_step_ms": step_ms, "tuning": ctx.get_tuning()}
+    if tuning is not None:
+        res["autotune_ms_per_step"] = [[c, round(ms, 4)] for c, ms in table]
     if not args.no_plain:
         wall_p, step_ms_p, dsp = run(plain=True)
         dsp.free()
