@@ -1,0 +1,112 @@
+"""The C-ABI boundary, checked without a GPU.
+
+  - libctsm_mi355x.so loads and exports every function include/ctsm_gpu.h
+    declares;
+  - the ctypes mirrors (ctsm_amd/abi.py) have the header's sizes and member
+    offsets (a probe compiled by gcc against the header prints them);
+  - the Fortran ISO_C_BINDING mirrors (ctsm_amd/fortran/ctsm_gpu_mod.F90)
+    compile with flang and have the same storage sizes;
+  - namelist combinations the library does not run are rejected before any
+    device is touched (CTSM_E_CONFIG), and with no GPU ctsm_create reports
+    CTSM_E_NODEVICE rather than falling back to anything.
+"""
+import ctypes as C
+import os
+import re
+import shutil
+import subprocess
+
+import pytest
+
+from ctsm_amd import abi, lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "ctsm_gpu.h")
+
+
+def _declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(ctsm_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = lib.load()
+    declared = _declared_functions()
+    assert len(declared) >= 15
+    missing = [f for f in declared if not hasattr(L, f)]
+    assert not missing, missing
+    assert set(declared) == set(lib.EXPORTS)
+
+
+def _probe(tmp_path):
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ctsm_gpu.h"', 'int main(void){']
+    for cname, py in (("ctsm_config", abi.Config), ("ctsm_layout", abi.Layout), ("ctsm_tuning", abi.Tuning),
+                      ("ctsm_col_state", abi.ColState), ("ctsm_patch_state", abi.PatchState)):
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0;}")
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    return subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+
+
+def test_struct_layouts_match_header(tmp_path):
+    out = _probe(tmp_path)
+    table = {}
+    for line in out.splitlines():
+        parts = line.split()
+        table[(parts[0], parts[1])] = int(parts[2])
+    for cname, py in (("ctsm_config", abi.Config), ("ctsm_layout", abi.Layout), ("ctsm_tuning", abi.Tuning),
+                      ("ctsm_col_state", abi.ColState), ("ctsm_patch_state", abi.PatchState)):
+        assert table[(cname, "size")] == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert table[(cname, f)] == getattr(py, f).offset, (cname, f)
+
+
+@pytest.mark.skipif(not (shutil.which("flang") or os.path.exists("/opt/rocm/lib/llvm/bin/flang")),
+                    reason="flang not available")
+def test_fortran_interface_module_matches(tmp_path):
+    flang = shutil.which("flang") or "/opt/rocm/lib/llvm/bin/flang"
+    prog = tmp_path / "sizes.F90"
+    prog.write_text("""
+program sizes
+  use iso_c_binding
+  use ctsm_gpu_mod
+  type(ctsm_config_t) :: a
+  type(ctsm_layout_t) :: b
+  type(ctsm_col_state_t) :: c
+  type(ctsm_patch_state_t) :: d
+  print '(4I8)', c_sizeof(a), c_sizeof(b), c_sizeof(c), c_sizeof(d)
+end program sizes
+""")
+    mod = os.path.join(ROOT, "ctsm_amd", "fortran", "ctsm_gpu_mod.F90")
+    exe = tmp_path / "sizes"
+    subprocess.run([flang, "-J", str(tmp_path), mod, str(prog), "-o", str(exe)], check=True, cwd=tmp_path)
+    sizes = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
+    assert sizes == [C.sizeof(abi.Config), C.sizeof(abi.Layout), C.sizeof(abi.ColState), C.sizeof(abi.PatchState)]
+
+
+def test_config_rejected_before_device():
+    L = lib.load()
+    h = C.c_void_p()
+    for bad in (dict(soilwater_movement_method=0), dict(lower_boundary_condition=4),
+                dict(upper_boundary_condition=0), dict(nlevsno=13), dict(flux_calculation=42)):
+        cfg = lib.default_config(**bad)
+        assert L.ctsm_create(C.byref(cfg), 0, C.byref(h)) == lib.CTSM_E_CONFIG, bad
+
+
+def test_no_silent_fallback_without_gpu():
+    """On a machine without a GPU the library refuses to create a context."""
+    import torch  # device count only; does not initialise the GPU runtime
+    if torch.cuda.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    L = lib.load()
+    h = C.c_void_p()
+    cfg = lib.default_config()
+    assert L.ctsm_create(C.byref(cfg), 0, C.byref(h)) == lib.CTSM_E_NODEVICE
+    with pytest.raises(lib.CtsmError):
+        lib.Context()
