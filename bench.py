@@ -116,7 +116,9 @@ def main():
 
     tuning, table = None, []
     if args.lanes or args.single_kernel:
-        ctx.set_tuning(richards_lanes=args.lanes or 1, fused_single_kernel=int(args.single_kernel))
+        lanes = args.lanes or 1
+        ctx.set_tuning(richards_lanes=lanes, richards_block=64 if lanes == 1 else 128,
+                       fused_single_kernel=int(args.single_kernel))
     elif not args.no_autotune:
         ds0 = DeviceState(st, ctx)
         tuning, _, table = AT.autotune(ctx, ds0, filter_dev=filt, steps=2)

@@ -1,5 +1,5 @@
 /*
- * ctsm_math.h -- deterministic binary64 log / exp / pow / log10.
+ * ctsm_math.h -- deterministic binary64 log / exp / pow / log10 / atan.
  *
  * One implementation, compiled for the gfx950 kernels and for host code, so
  * both produce the same bits for the same arguments: only IEEE operations
@@ -18,10 +18,13 @@
  *           |r| <= ln2/2, e^r - 1 by a degree-14 Taylor polynomial, scaled by
  *           2^k.  The low word of a double-double z is folded into r.
  *   pow(x,y) = exp(y * log x) with y*log x formed in double-double.
+ *   atan(x):  table of atan(k/8) plus the odd series of a reduced argument
+ *           (see cm_atan).
  * Results are within about 1 ulp of the true value (checked against the host
  * libm in tests/test_math.py); they are not claimed correctly rounded.
- * Constants are ln 2, 1/ln 2, 1/ln 10, 2/3, 1/n! and 2/(2k+1) to binary64
- * (computed with Python's decimal module at 60 digits).
+ * Constants are ln 2, 1/ln 2, 1/ln 10, 2/3, pi/2, atan(k/8), 1/n! and
+ * 1/(2k+1) to binary64 (computed at 60 digits with Python's decimal and
+ * mpmath modules).
  */
 #ifndef CTSM_MATH_H
 #define CTSM_MATH_H
@@ -180,6 +183,71 @@ CM_FN double cm_log10(double x)
     cm_log_dd(x, &h, &l);
     cm_dd_mul(h, l, CM_INV_LN10_H, CM_INV_LN10_L, &rh, &rl);
     return rh + rl;
+}
+
+/* arctangent.  |x| > 1 is folded to pi/2 - atan(1/|x|) with 1/|x| kept as a
+ * double-double; the reduced argument a is split as a = c + (a - c) with
+ * c = k/8 nearest, atan a = atan c + atan t, t = (a - c) / (1 + a c) formed in
+ * double-double, atan c from a table (60-digit values rounded to
+ * double-double) and atan t - t by its odd Taylor series (|t| <= 1/16). */
+CM_FN double cm_atan(double x)
+{
+    if (x != x || x == 0.0) return x;
+    const double a = fabs(x);
+    if (a == HUGE_VAL) return x < 0.0 ? -0x1.921fb54442d18p+0 : 0x1.921fb54442d18p+0;
+    double ih, il;
+    int inv = 0;
+    if (a > 1.0) {
+        ih = 1.0 / a;
+        il = fma(-ih, a, 1.0) / a;
+        inv = 1;
+    } else {
+        ih = a;
+        il = 0.0;
+    }
+    static const double AT_H[9] = {
+        0.0, 0x1.fd5ba9aac2f6ep-4, 0x1.f5b75f92c80ddp-3, 0x1.6f61941e4def1p-2, 0x1.dac670561bb4fp-2,
+        0x1.1e00babdefeb4p-1, 0x1.4978fa3269ee1p-1, 0x1.700a7c5784634p-1, 0x1.921fb54442d18p-1};
+    static const double AT_L[9] = {
+        0.0, -0x1.cd37686760c17p-59, 0x1.8ab6e3cf7afbdp-57, -0x1.c63aae6f6e918p-56, 0x1.a2b7f222f65e2p-56,
+        -0x1.928df287a668fp-58, 0x1.2419a87f2a458p-56, -0x1.8c34d25aadef6p-56, 0x1.1a62633145c07p-55};
+    const int k = (int)rint(ih * 8.0);       /* 0..8 */
+    const double c = (double)k * 0.125;
+    /* numerator (ih - c) + il, exact difference (Sterbenz) */
+    double nh, nl;
+    cm_two_sum(ih - c, il, &nh, &nl);
+    /* denominator 1 + (ih + il) c */
+    const double p = ih * c;
+    const double pe = fma(ih, c, -p) + il * c;
+    double dh, dl;
+    cm_two_sum(1.0, p, &dh, &dl);
+    dl = dl + pe;
+    const double q1 = nh / dh;
+    double r = fma(-q1, dh, nh);
+    r = (r + nl) - q1 * dl;
+    const double q2 = r / dh;
+    const double s = q1 * q1;
+    double poly = 0x1.e1e1e1e1e1e1ep-5;                /*  1/17 */
+    poly = -0x1.1111111111111p-4 + s * poly;           /* -1/15 */
+    poly = 0x1.3b13b13b13b14p-4 + s * poly;            /*  1/13 */
+    poly = -0x1.745d1745d1746p-4 + s * poly;           /* -1/11 */
+    poly = 0x1.c71c71c71c71cp-4 + s * poly;            /*  1/9  */
+    poly = -0x1.2492492492492p-3 + s * poly;           /* -1/7  */
+    poly = 0x1.999999999999ap-3 + s * poly;            /*  1/5  */
+    poly = -0x1.5555555555555p-2 + s * poly;           /* -1/3  */
+    const double tail = (q1 * s) * poly;
+    double s1, e1, rh, rl;
+    cm_two_sum(AT_H[k], q1, &s1, &e1);
+    cm_fast_two_sum(s1, e1 + ((AT_L[k] + q2) + tail), &rh, &rl);
+    double res;
+    if (inv) {
+        double h, l;
+        cm_dd_add(0x1.921fb54442d18p+0, 0x1.1a62633145c07p-54, -rh, -rl, &h, &l);   /* pi/2 - atan(1/a) */
+        res = h + l;
+    } else {
+        res = rh + rl;
+    }
+    return x < 0.0 ? -res : res;
 }
 
 /* x**y for x > 0 (the only case the physics uses); x == 0 and x < 0 follow C. */

@@ -77,6 +77,7 @@ void launch_soilwater(hipStream_t st, const View &V, const Params &P, const ctsm
 {
     if (n <= 0) return;
     if (block <= 0 || block > 256) block = (lanes == 1) ? 64 : 128;
+    if (lanes != 2 && lanes != 4 && lanes != 8) block = 64;     /* k_soilwater: __launch_bounds__(64) */
     const long threads = (long)n * (lanes > 0 ? lanes : 1);
     const dim3 grid((unsigned)((threads + block - 1) / block)), blk(block);
     switch (lanes) {

@@ -1110,3 +1110,5 @@ void oracle_cm_exp(int n, const double *x, double *out)
 { int i; for (i = 0; i < n; ++i) out[i] = cm_exp(x[i]); }
 void oracle_cm_log10(int n, const double *x, double *out)
 { int i; for (i = 0; i < n; ++i) out[i] = cm_log10(x[i]); }
+void oracle_cm_atan(int n, const double *x, double *out)
+{ int i; for (i = 0; i < n; ++i) out[i] = cm_atan(x[i]); }

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 from ctsm_amd import synthetic
+from ctsm_amd.lib import CtsmError
 from oracle import oracle as O
 from tests.helpers import NS, compare_states, max_relerr, soil_mask
 
@@ -175,9 +176,17 @@ def test_bad_column_reports_index(ctx):
     assert ei.value.index == 42
 
 
+def test_tuning_rejects_oversized_one_lane_block(ctx):
+    """The one-lane Richards kernel is built for 64-lane blocks."""
+    before = ctx.get_tuning()
+    with pytest.raises(CtsmError):
+        ctx.set_tuning(richards_lanes=1, richards_block=128)
+    assert ctx.get_tuning() == before
+
+
 @pytest.mark.parametrize("shape", [
-    dict(richards_lanes=1, fused_single_kernel=1),
-    dict(richards_lanes=1, fused_single_kernel=0),
+    dict(richards_lanes=1, richards_block=64, fused_single_kernel=1),
+    dict(richards_lanes=1, richards_block=64, fused_single_kernel=0),
     dict(richards_lanes=2, fused_single_kernel=0),
     dict(richards_lanes=4, fused_single_kernel=0),
     dict(richards_lanes=8, fused_single_kernel=0),

@@ -109,3 +109,23 @@ def test_pow_within_one_ulp_on_physics_ranges():
 ])
 def test_pow_special_cases(x, y, want):
     assert _vec("pow", np.array([x]), np.array([y]))[0] == want
+
+
+def test_atan_within_one_ulp():
+    mp = pytest.importorskip("mpmath")
+    mp.mp.dps = 40
+    r = _rng(4)
+    x = np.concatenate([r.uniform(-1, 1, 500), 10.0 ** r.uniform(-8, 8, 500), -(10.0 ** r.uniform(-3, 3, 200)),
+                        1.0 + r.uniform(0, 1e-3, 100), (np.arange(0, 17) / 16.0), np.array([1e300, -1e-300, np.inf])])
+    got = _vec("atan", x)
+    worst = 0.0
+    for xi, gi in zip(x, got):
+        ex = mp.atan(mp.mpf(xi))
+        e = float(ex)
+        if e == 0.0:
+            assert gi == 0.0
+            continue
+        worst = max(worst, float(abs(mp.mpf(gi) - ex) / mp.mpf(math.ulp(e))))
+    assert worst < 1.0, worst
+    assert np.all(np.abs(got - np.arctan(x)) <= np.spacing(np.abs(np.arctan(x))))
+    assert _vec("atan", np.array([1.0]))[0] == math.atan(1.0)
