@@ -28,6 +28,12 @@ class Config(C.Structure):
         ("xTolerUpper", C.c_double),
         ("xTolerLower", C.c_double),
         ("e_ice", C.c_double),
+        ("z0param_method", C.c_int32),
+        ("stomatalcond_method", C.c_int32),
+        ("soil_resis_method", C.c_int32),
+        ("use_biomass_heat_storage", C.c_int32),
+        ("use_undercanopy_stability", C.c_int32),
+        ("light_inhibit", C.c_int32),
     ]
 
 
@@ -46,6 +52,8 @@ class Layout(C.Structure):
         ("ldc", C.c_int32),
         ("begp_alloc", C.c_int32),
         ("ldp", C.c_int32),
+        ("begg_alloc", C.c_int32),
+        ("ldg", C.c_int32),
     ]
 
 
@@ -136,7 +144,212 @@ PATCH_FIELDS = [
     ("sabg_chk", "f64", "p"),
 ]
 
+# CanopyFluxes state; extra level kinds: g (gridcell), soip (patch x nlevgrnd, lo 1)
+CANOPY_FIELDS = [
+    ("itype", "i32", "p"),
+    ("column", "i32", "p"),
+    ("gridcell", "i32", "p"),
+    ("wtcol", "f64", "p"),
+    ("frac_veg_nosno", "i32", "p"),
+    ("active", "i32", "p"),
+    ("forc_u", "f64", "g"),
+    ("forc_v", "f64", "g"),
+    ("forc_pco2", "f64", "g"),
+    ("forc_po2", "f64", "g"),
+    ("forc_hgt_u", "f64", "g"),
+    ("forc_hgt_t", "f64", "g"),
+    ("forc_hgt_q", "f64", "g"),
+    ("dayl", "f64", "g"),
+    ("max_dayl", "f64", "g"),
+    ("near_local_noon", "i32", "g"),
+    ("forc_q", "f64", "c"),
+    ("forc_pbot", "f64", "c"),
+    ("forc_th", "f64", "c"),
+    ("forc_rho", "f64", "c"),
+    ("thv", "f64", "c"),
+    ("qg", "f64", "c"),
+    ("qg_snow", "f64", "c"),
+    ("qg_soil", "f64", "c"),
+    ("qg_h2osfc", "f64", "c"),
+    ("dqgdT", "f64", "c"),
+    ("z0mg", "f64", "c"),
+    ("soilbeta", "f64", "c"),
+    ("soilresis", "f64", "c"),
+    ("h2osoi_liqvol", "f64", "soi"),
+    ("qflx_tran_veg_col", "f64", "c"),
+    ("rootr_col", "f64", "soi"),
+    ("qflx_rootsoi_col", "f64", "soi"),
+    ("elai", "f64", "p"),
+    ("esai", "f64", "p"),
+    ("laisun", "f64", "p"),
+    ("laisha", "f64", "p"),
+    ("htop", "f64", "p"),
+    ("sabv", "f64", "p"),
+    ("parsun_z", "f64", "p"),
+    ("parsha_z", "f64", "p"),
+    ("laisun_z", "f64", "p"),
+    ("laisha_z", "f64", "p"),
+    ("vcmaxcintsun", "f64", "p"),
+    ("vcmaxcintsha", "f64", "p"),
+    ("nrad", "i32", "p"),
+    ("thm", "f64", "p"),
+    ("emv", "f64", "p"),
+    ("fwet", "f64", "p"),
+    ("fdry", "f64", "p"),
+    ("t_a10", "f64", "p"),
+    ("o3coefvsun", "f64", "p"),
+    ("o3coefvsha", "f64", "p"),
+    ("o3coefgsun", "f64", "p"),
+    ("o3coefgsha", "f64", "p"),
+    ("rootfr", "f64", "soip"),
+    ("displa", "f64", "p"),
+    ("z0mv", "f64", "p"),
+    ("t_veg", "f64", "p"),
+    ("t_stem", "f64", "p"),
+    ("snocan", "f64", "p"),
+    ("liqcan", "f64", "p"),
+    ("cgrnds", "f64", "p"),
+    ("cgrndl", "f64", "p"),
+    ("z0hv", "f64", "p"),
+    ("z0qv", "f64", "p"),
+    ("forc_hgt_u_patch", "f64", "p"),
+    ("forc_hgt_t_patch", "f64", "p"),
+    ("forc_hgt_q_patch", "f64", "p"),
+    ("ram1", "f64", "p"),
+    ("rb1", "f64", "p"),
+    ("rah1", "f64", "p"),
+    ("rah2", "f64", "p"),
+    ("raw1", "f64", "p"),
+    ("raw2", "f64", "p"),
+    ("ustar", "f64", "p"),
+    ("um", "f64", "p"),
+    ("uaf", "f64", "p"),
+    ("taf", "f64", "p"),
+    ("qaf", "f64", "p"),
+    ("obu", "f64", "p"),
+    ("zeta", "f64", "p"),
+    ("vpd", "f64", "p"),
+    ("num_iter", "f64", "p"),
+    ("u10", "f64", "p"),
+    ("u10_clm", "f64", "p"),
+    ("va", "f64", "p"),
+    ("fv", "f64", "p"),
+    ("vds", "f64", "p"),
+    ("btran", "f64", "p"),
+    ("rresis", "f64", "soip"),
+    ("rootr", "f64", "soip"),
+    ("eflx_sh_veg", "f64", "p"),
+    ("eflx_sh_stem", "f64", "p"),
+    ("eflx_sh_grnd", "f64", "p"),
+    ("eflx_sh_snow", "f64", "p"),
+    ("eflx_sh_soil", "f64", "p"),
+    ("eflx_sh_h2osfc", "f64", "p"),
+    ("taux", "f64", "p"),
+    ("tauy", "f64", "p"),
+    ("dlrad", "f64", "p"),
+    ("ulrad", "f64", "p"),
+    ("cgrnd", "f64", "p"),
+    ("dhsdt_canopy", "f64", "p"),
+    ("qflx_evap_soi", "f64", "p"),
+    ("qflx_ev_snow", "f64", "p"),
+    ("qflx_ev_soil", "f64", "p"),
+    ("qflx_ev_h2osfc", "f64", "p"),
+    ("qflx_tran_veg", "f64", "p"),
+    ("qflx_evap_veg", "f64", "p"),
+    ("rh_af", "f64", "p"),
+    ("q_ref2m", "f64", "p"),
+    ("rh_ref2m", "f64", "p"),
+    ("rh_ref2m_r", "f64", "p"),
+    ("vpd_ref2m", "f64", "p"),
+    ("iwue_ln", "f64", "p"),
+    ("t_ref2m", "f64", "p"),
+    ("t_ref2m_r", "f64", "p"),
+    ("t_skin", "f64", "p"),
+    ("stem_biomass", "f64", "p"),
+    ("leaf_biomass", "f64", "p"),
+    ("dleaf_patch", "f64", "p"),
+    ("rssun", "f64", "p"),
+    ("rssha", "f64", "p"),
+    ("psnsun", "f64", "p"),
+    ("psnsha", "f64", "p"),
+    ("psnsun_wc", "f64", "p"),
+    ("psnsun_wj", "f64", "p"),
+    ("psnsun_wp", "f64", "p"),
+    ("psnsha_wc", "f64", "p"),
+    ("psnsha_wj", "f64", "p"),
+    ("psnsha_wp", "f64", "p"),
+    ("fpsn", "f64", "p"),
+    ("fpsn_wc", "f64", "p"),
+    ("fpsn_wj", "f64", "p"),
+    ("fpsn_wp", "f64", "p"),
+    ("lmrsun", "f64", "p"),
+    ("lmrsha", "f64", "p"),
+    ("gs_mol_sun", "f64", "p"),
+    ("gs_mol_sha", "f64", "p"),
+    ("gs_mol_sun_ln", "f64", "p"),
+    ("gs_mol_sha_ln", "f64", "p"),
+    ("cisun_z", "f64", "p"),
+    ("cisha_z", "f64", "p"),
+    ("rssun_z", "f64", "p"),
+    ("rssha_z", "f64", "p"),
+    ("lmrsun_z", "f64", "p"),
+    ("lmrsha_z", "f64", "p"),
+    ("psnsun_z", "f64", "p"),
+    ("psnsha_z", "f64", "p"),
+    ("vcmax_z", "f64", "p"),
+    ("tpu_z", "f64", "p"),
+    ("kp_z", "f64", "p"),
+    ("ac", "f64", "p"),
+    ("aj", "f64", "p"),
+    ("ap", "f64", "p"),
+    ("ag", "f64", "p"),
+    ("an", "f64", "p"),
+    ("gb_mol", "f64", "p"),
+    ("gs_mol", "f64", "p"),
+    ("cp", "f64", "p"),
+    ("kc", "f64", "p"),
+    ("ko", "f64", "p"),
+    ("qe", "f64", "p"),
+    ("bbb", "f64", "p"),
+    ("mbb", "f64", "p"),
+    ("rh_leaf", "f64", "p"),
+    ("vpd_can", "f64", "p"),
+    ("lnca", "f64", "p"),
+    ("c3flag", "i32", "p"),
+    ("canopy_err", "f64", "p"),
+]
+
+CANOPY_PARAM_NAMES = [
+    "cv", "a_coef", "a_exp", "csoilc", "z_dl", "lai_dl", "wind_min", "zetamaxstable",
+    "kc25_coef", "ko25_coef", "cp25_yr2000", "kcha", "koha", "cpha",
+    "vcmaxha", "jmaxha", "tpuha", "lmrha", "vcmaxhd", "jmaxhd", "tpuhd", "lmrhd", "lmrse",
+    "fnr", "act25", "theta_psii", "theta_ip", "fnps", "tpu25ratio", "kp25ratio",
+    "vcmaxse_sf", "jmaxse_sf", "tpuse_sf", "jmax25top_sf", "leaf_mr_vcm",
+]
+
+
+class CanopyParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in CANOPY_PARAM_NAMES]
+
+
+PFT_FIELDS = [
+    ("dleaf", "f64"), ("c3psn", "f64"), ("slatop", "f64"), ("leafcn", "f64"), ("flnr", "f64"), ("fnitr", "f64"),
+    ("mbbopt", "f64"), ("medlynslope", "f64"), ("medlynintercept", "f64"),
+    ("smpso", "f64"), ("smpsc", "f64"), ("theta_cj", "f64"),
+    ("z0v_Cr", "f64"), ("z0v_Cs", "f64"), ("z0v_c", "f64"), ("z0v_cw", "f64"), ("z0v_LAImax", "f64"),
+    ("is_tree", "i32"), ("is_shrub", "i32"),
+    ("dbh", "f64"), ("fbw", "f64"), ("nstem", "f64"), ("rstem_per_dbh", "f64"), ("wood_density", "f64"),
+]
+
 _CT = {"i32": i32p, "f64": f64p}
+
+
+class PftParams(C.Structure):
+    _fields_ = [("npft", C.c_int32), ("pad", C.c_int32)] + [(n, _CT[t]) for n, t in PFT_FIELDS]
+
+
+class CanopyState(C.Structure):
+    _fields_ = [(n, _CT[t]) for n, t, _ in CANOPY_FIELDS]
 
 
 class ColState(C.Structure):
@@ -157,10 +370,12 @@ def nlev_of(kind, nlevsno, nlevgrnd):
         "soi": nlevgrnd,
         "snow": nlevsno,
         "lyr": nlevsno + 1,
+        "g": 1,
+        "soip": nlevgrnd,
     }[kind]
 
 
 def lo_of(kind, nlevsno):
     """Fortran lower level bound of a level kind."""
-    return {"c": 0, "p": 0, "sno": 1 - nlevsno, "zi": -nlevsno, "soi": 1,
+    return {"c": 0, "p": 0, "g": 0, "sno": 1 - nlevsno, "zi": -nlevsno, "soi": 1, "soip": 1,
             "snow": 1 - nlevsno, "lyr": 1 - nlevsno}[kind]

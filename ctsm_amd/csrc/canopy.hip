@@ -1,0 +1,1301 @@
+// canopy.hip -- CanopyFluxes (CanopyFluxesMod.F90:191-1765) for gfx950: one
+// lane per patch runs the whole time step (root stress, roughness, Monin-
+// Obukhov initialisation, the Newton / stability iteration with sun and shade
+// photosynthesis, the post-iteration fluxes) with its state in registers.
+//
+// Callees restated here: FrictionVelocity / StabilityFunc1/2 / MoninObukIni
+// (FrictionVelocityMod.F90:754-1209), QSat (QSatMod.F90:61-127),
+// Photosynthesis / hybrid / brent / ci_func (PhotosynthesisMod.F90:1243-2701),
+// quadratic (quadraticMod.F90:17-74), calc_effective_soilporosity /
+// calc_volumetric_h2oliq / calc_root_moist_stress_clm45default
+// (SoilMoistStressMod.F90:70-215, 312-514), PhotosynthesisTotal
+// (PhotosynthesisMod.F90:2065-2151), truncate_small_values (NumericsMod.F90:
+// 60-100); plus the transpiration p2c and sink (clm_driver.F90:1714-1716,
+// SoilWaterPlantSinkMod.F90:332-424) in k_root_sink.
+//
+// Patches are independent, so the reference's shrinking filter of
+// unconverged patches becomes a per-lane loop exit; the operation sequence of
+// each patch is the reference's.  Leaf-temperature factors of Photosynthesis
+// (ft, fth, 2**x) depend on t_veg only, so they are evaluated once per
+// iteration and shared by the sun and shade phases (the same values the
+// reference computes twice).
+#include "common.h"
+
+namespace ctsm {
+namespace canopy {
+
+constexpr double VKC = 0.4;
+constexpr double CPAIR = 1.00464e3;
+constexpr double HVAP = 2.501e6;
+constexpr double CWATER = 4.188e3;
+constexpr double RGAS = 6.02214e26 * 1.38065e-23;
+constexpr double PI = 3.14159265358979323846;
+constexpr double SPVAL = 1.e36;
+constexpr double TLSAI_CRIT = 2.0;
+constexpr double ALPHA_AERO = 1.0;
+constexpr double NU = 1.5e-5;
+constexpr double CD1 = 7.5;
+constexpr double CDRYBIO = 1400.0;
+constexpr double C_TO_B = 2.0;
+constexpr double MAX_CS = 1.e-06;
+constexpr int ENDRUN = CTSM_E_ENDRUN;
+// the Meier2022 fixed point u = a exp(u) converges or overflows to NaN
+// within a few hundred steps for finite input; the cap only bounds a wave
+constexpr int MAX_FIXED_POINT = 100000;
+
+// PFT table field indices (rows of the device table, CTSM_MAX_PFT each)
+enum {
+    F_DLEAF, F_C3PSN, F_SLATOP, F_LEAFCN, F_FLNR, F_FNITR, F_MBBOPT, F_MEDSLOPE, F_MEDINT, F_SMPSO, F_SMPSC,
+    F_THETA_CJ, F_Z0V_CR, F_Z0V_CS, F_Z0V_C, F_Z0V_CW, F_Z0V_LAIMAX, F_IS_TREE, F_IS_SHRUB, F_DBH, F_FBW,
+    F_NSTEM, F_RSTEM, F_WOOD, N_PFT_FIELDS
+};
+
+struct CView {
+    int begc, ldc, begp, ldp, begg, ldg, nlevsno, nlevsoi, nlevgrnd;
+    __device__ __forceinline__ long c1(int c) const { return (long)(c - begc); }
+    __device__ __forceinline__ long g1(int g) const { return (long)(g - begg); }
+    __device__ __forceinline__ long p1(int p) const { return (long)(p - begp); }
+    __device__ __forceinline__ long sno(int c, int j) const { return (long)(c - begc) + (long)(j + nlevsno - 1) * ldc; }
+    __device__ __forceinline__ long soi(int c, int j) const { return (long)(c - begc) + (long)(j - 1) * ldc; }
+    __device__ __forceinline__ long soip(int p, int j) const { return (long)(p - begp) + (long)(j - 1) * ldp; }
+};
+
+struct CSwitch {
+    int z0param, stomatal, soil_resis, biomass, undercanopy, light_inhibit, itmax;
+};
+
+__device__ __forceinline__ double p4(double x) { const double x2 = x * x; return x2 * x2; }
+__device__ __forceinline__ double p3(double x) { return (x * x) * x; }
+
+// QSatMod.F90:61-127
+__device__ __forceinline__ void qsat(double T, double p, double &qs, double &es, double &qsdT)
+{
+    const double td = fmin90(100.0, fmax90(-75.0, T - TFRZ));
+    double e, d;
+    if (td >= 0.0) {
+        e = 6.11213476 + td * (0.444007856 + td * (0.143064234e-01 + td * (0.264461437e-03 + td * (0.305903558e-05
+            + td * (0.196237241e-07 + td * (0.892344772e-10 + td * (-0.373208410e-12 + td * 0.209339997e-15)))))));
+        d = 0.444017302 + td * (0.286064092e-01 + td * (0.794683137e-03 + td * (0.121211669e-04 + td * (0.103354611e-06
+            + td * (0.404125005e-09 + td * (-0.788037859e-12 + td * (-0.114596802e-13 + td * 0.381294516e-16)))))));
+    } else {
+        e = 6.11123516 + td * (0.503109514 + td * (0.188369801e-01 + td * (0.420547422e-03 + td * (0.614396778e-05
+            + td * (0.602780717e-07 + td * (0.387940929e-09 + td * (0.149436277e-11 + td * 0.262655803e-14)))))));
+        d = 0.503277922 + td * (0.377289173e-01 + td * (0.126801703e-02 + td * (0.249468427e-04 + td * (0.313703411e-06
+            + td * (0.257180651e-08 + td * (0.133268878e-10 + td * (0.394116744e-13 + td * 0.498070196e-16)))))));
+    }
+    es = e * 100.0;
+    const double vp = 1.0 / (p - 0.378 * es);
+    const double vp1 = 0.622 * vp;
+    qs = es * vp1;
+    const double esdT = d * 100.0;
+    const double vp2 = vp1 * vp;
+    qsdT = esdT * vp2 * p;
+}
+
+__device__ __forceinline__ double qsat_only(double T, double p, double &es)
+{
+    const double td = fmin90(100.0, fmax90(-75.0, T - TFRZ));
+    double e;
+    if (td >= 0.0)
+        e = 6.11213476 + td * (0.444007856 + td * (0.143064234e-01 + td * (0.264461437e-03 + td * (0.305903558e-05
+            + td * (0.196237241e-07 + td * (0.892344772e-10 + td * (-0.373208410e-12 + td * 0.209339997e-15)))))));
+    else
+        e = 6.11123516 + td * (0.503109514 + td * (0.188369801e-01 + td * (0.420547422e-03 + td * (0.614396778e-05
+            + td * (0.602780717e-07 + td * (0.387940929e-09 + td * (0.149436277e-11 + td * 0.262655803e-14)))))));
+    es = e * 100.0;
+    const double vp = 1.0 / (p - 0.378 * es);
+    return es * (0.622 * vp);
+}
+
+// quadraticMod.F90:17-74; false where the reference calls endrun
+__device__ __forceinline__ bool quadratic(double a, double b, double c, double &r1, double &r2)
+{
+    if (a == 0.0) { r1 = r2 = 0.0; return false; }
+    double root = b * b - 4.0 * a * c;
+    bool ok = true;
+    if (root < 0.0) {
+        if (-root < 3.0 * 2.220446049250313e-16) root = 0.0;
+        else { ok = false; root = 0.0; }
+    }
+    double q;
+    if (b >= 0.0) q = -0.5 * (b + sqrt(root));
+    else q = -0.5 * (b - sqrt(root));
+    r1 = q / a;
+    r2 = (q != 0.0) ? c / q : 1.e36;
+    return ok;
+}
+
+__device__ __forceinline__ double stab1(double zeta)
+{
+    const double chik2 = sqrt(1.0 - 16.0 * zeta);
+    const double chik = sqrt(chik2);
+    return 2.0 * cm_log((1.0 + chik) * 0.5) + cm_log((1.0 + chik2) * 0.5) - 2.0 * cm_atan(chik) + PI * 0.5;
+}
+__device__ __forceinline__ double stab2(double zeta)
+{
+    const double chik2 = sqrt(1.0 - 16.0 * zeta);
+    return 2.0 * cm_log((1.0 + chik2) * 0.5);
+}
+
+// the temperature / humidity profile block repeated in FrictionVelocityMod.F90:916-1060
+__device__ double temp_profile(double zldis, double z0, double obu)
+{
+    const double zetat = 0.465;
+    const double zeta = zldis / obu;
+    if (zeta < -zetat)
+        return VKC / (cm_log(-zetat * obu / z0) - stab2(-zetat) + stab2(z0 / obu)
+                      + 0.8 * (cm_pow(zetat, -0.333) - cm_pow(-zeta, -0.333)));
+    if (zeta < 0.0) return VKC / (cm_log(zldis / z0) - stab2(zeta) + stab2(z0 / obu));
+    if (zeta <= 1.0) return VKC / (cm_log(zldis / z0) + 5.0 * zeta - 5.0 * z0 / obu);
+    return VKC / (cm_log(obu / z0) + 5.0 - 5.0 * z0 / obu + (5.0 * cm_log(zeta) + zeta - 1.0));
+}
+
+__device__ __forceinline__ double fm_profile(double z)
+{
+    // dust-model fm of FrictionVelocityMod.F90:1072-1095 for z < 0
+    const double tmp1 = cm_pow(1.0 - 16.0 * z, 0.25);
+    const double tmp2 = cm_log((1.0 + tmp1 * tmp1) / 2.0);
+    const double tmp3 = cm_log((1.0 + tmp1) / 2.0);
+    return 2.0 * tmp3 + tmp2 - 2.0 * cm_atan(tmp1) + 1.5707963;
+}
+
+struct FV {
+    double ustar, temp1, temp2, temp12m, temp22m, fm, u10, u10_clm, va, fv, vds;
+};
+
+// FrictionVelocityMod.F90:754-1117 (patch form)
+__device__ void friction_velocity(double hu, double ht, double hq, double displa, double z0m, double z0h,
+                                  double z0q, double obu, int iter, double ur, double um, FV &o)
+{
+    const double zetam = 1.574;
+    double zldis = hu - displa;
+    double zeta = zldis / obu;
+    if (zeta < -zetam)
+        o.ustar = VKC * um / (cm_log(-zetam * obu / z0m) - stab1(-zetam) + stab1(z0m / obu)
+                              + 1.14 * (cm_pow(-zeta, 0.333) - cm_pow(zetam, 0.333)));
+    else if (zeta < 0.0)
+        o.ustar = VKC * um / (cm_log(zldis / z0m) - stab1(zeta) + stab1(z0m / obu));
+    else if (zeta <= 1.0)
+        o.ustar = VKC * um / (cm_log(zldis / z0m) + 5.0 * zeta - 5.0 * z0m / obu);
+    else
+        o.ustar = VKC * um / (cm_log(obu / z0m) + 5.0 - 5.0 * z0m / obu + (5.0 * cm_log(zeta) + zeta - 1.0));
+    if (zeta < 0.0) o.vds = 2.e-3 * o.ustar * (1.0 + cm_pow(300.0 / (-obu), 0.666));
+    else o.vds = 2.e-3 * o.ustar;
+    if (zldis - z0m <= 10.0) {
+        o.u10_clm = um;
+    } else {
+        const double z10 = 10.0 + z0m;
+        if (zeta < -zetam)
+            o.u10_clm = um - (o.ustar / VKC * (cm_log(-zetam * obu / z10) - stab1(-zetam) + stab1(z10 / obu)
+                                               + 1.14 * (cm_pow(-zeta, 0.333) - cm_pow(zetam, 0.333))));
+        else if (zeta < 0.0)
+            o.u10_clm = um - (o.ustar / VKC * (cm_log(zldis / z10) - stab1(zeta) + stab1(z10 / obu)));
+        else if (zeta <= 1.0)
+            o.u10_clm = um - (o.ustar / VKC * (cm_log(zldis / z10) + 5.0 * zeta - 5.0 * z10 / obu));
+        else
+            o.u10_clm = um - (o.ustar / VKC * (cm_log(obu / z10) + 5.0 - 5.0 * z10 / obu
+                                               + (5.0 * cm_log(zeta) + zeta - 1.0)));
+    }
+    o.va = um;
+    o.temp1 = temp_profile(ht - displa, z0h, obu);
+    if (hq == ht && z0q == z0h) o.temp2 = o.temp1;
+    else o.temp2 = temp_profile(hq - displa, z0q, obu);
+    o.temp12m = temp_profile(2.0 + z0h, z0h, obu);
+    if (z0q == z0h) o.temp22m = o.temp12m;
+    else o.temp22m = temp_profile(2.0 + z0q, z0q, obu);
+    zeta = zldis / obu;   // zldis is hu - displa again
+    const double zm1 = fmin90(zeta, 1.0);
+    const double fmnew = (zm1 < 0.0) ? fm_profile(zm1) : -5.0 * zm1;
+    o.fm = (iter == 1) ? fmnew : 0.5 * (o.fm + fmnew);
+    double zeta10 = fmin90(10.0 / obu, 1.0);
+    if (zeta == 0.0) zeta10 = 0.0;
+    const double fm10 = (zeta10 < 0.0) ? fm_profile(zeta10) : -5.0 * zeta10;
+    const double tmp4 = cm_log(fmax90(1.0, hu / 10.0));
+    o.u10 = ur - o.ustar / VKC * (tmp4 - o.fm + fm10);
+    o.fv = o.ustar;
+}
+
+__device__ __forceinline__ double ft(double tl, double ha)
+{
+    return cm_exp(ha / (RGAS * 1.e-3 * (TFRZ + 25.0)) * (1.0 - (TFRZ + 25.0) / tl));
+}
+__device__ __forceinline__ double fth(double tl, double hd, double se, double scaleFactor)
+{
+    return scaleFactor / (1.0 + cm_exp((-hd + se * tl) / (RGAS * 1.e-3 * tl)));
+}
+__device__ __forceinline__ double fth25(double hd, double se)
+{
+    return 1.0 + cm_exp((-hd + se * (TFRZ + 25.0)) / (RGAS * 1.e-3 * (TFRZ + 25.0)));
+}
+
+// One ci solve: the constants of ci_func and the photosyns_inst values it writes.
+struct CiSolve {
+    bool c3, medlyn;
+    double forc_pbot, cair, oair, gb_mol, je, lmr_z, par_z, rh_can;
+    double vcmax_z, tpu_z, kp_z, cp, kc, ko, qe, bbb, mbb, theta_cj, theta_ip, mslope, mint;
+    double ac, aj, ap, ag, an, gs_mol;
+    bool bad;
+};
+
+// PhotosynthesisMod.F90:2564-2701
+__device__ double ci_func(double ci, CiSolve &s)
+{
+    double r1, r2;
+    if (s.c3) {
+        s.ac = s.vcmax_z * fmax90(ci - s.cp, 0.0) / (ci + s.kc * (1.0 + s.oair / s.ko));
+        s.aj = s.je * fmax90(ci - s.cp, 0.0) / (4.0 * ci + 8.0 * s.cp);
+        s.ap = 3.0 * s.tpu_z;
+    } else {
+        s.ac = s.vcmax_z;
+        s.aj = s.qe * s.par_z * 4.6;
+        s.ap = s.kp_z * fmax90(ci, 0.0) / s.forc_pbot;
+    }
+    if (!quadratic(s.theta_cj, -(s.ac + s.aj), s.ac * s.aj, r1, r2)) s.bad = true;
+    const double ai = fmin90(r1, r2);
+    if (!quadratic(s.theta_ip, -(ai + s.ap), ai * s.ap, r1, r2)) s.bad = true;
+    s.ag = fmax90(0.0, fmin90(r1, r2));
+    s.an = s.ag - s.lmr_z;
+    if (s.an < 0.0) return 0.0;
+    double cs = s.cair - 1.4 / s.gb_mol * s.an * s.forc_pbot;
+    cs = fmax90(cs, MAX_CS);
+    if (s.medlyn) {
+        const double term = 1.6 * s.an / (cs / s.forc_pbot * 1.e06);
+        const double bq = -(2.0 * (s.mint * 1.e-06 + term)
+                            + (s.mslope * term) * (s.mslope * term) / (s.gb_mol * 1.e-06 * s.rh_can));
+        const double cq = s.mint * s.mint * 1.e-12 + (2.0 * s.mint * 1.e-06 + term * (1.0 - s.mslope * s.mslope / s.rh_can)) * term;
+        if (!quadratic(1.0, bq, cq, r1, r2)) s.bad = true;
+        s.gs_mol = fmax90(r1, r2) * 1.e06;
+    } else {
+        const double bq = cs * (s.gb_mol - s.bbb) - s.mbb * s.an * s.forc_pbot;
+        const double cq = -s.gb_mol * (cs * s.bbb + s.mbb * s.an * s.forc_pbot * s.rh_can);
+        if (!quadratic(cs, bq, cq, r1, r2)) s.bad = true;
+        s.gs_mol = fmax90(r1, r2);
+    }
+    return ci - s.cair + s.an * s.forc_pbot * (1.4 * s.gs_mol + 1.6 * s.gb_mol) / (s.gb_mol * s.gs_mol);
+}
+
+// PhotosynthesisMod.F90:2371-2485
+__device__ double brent(double x1, double x2, double f1, double f2, double tol, CiSolve &s)
+{
+    const double eps = 1.e-2;
+    double a = x1, b = x2, fa = f1, fb = f2, c, fc, d = 0.0, e = 0.0;
+    if ((fa > 0.0 && fb > 0.0) || (fa < 0.0 && fb < 0.0)) {
+        s.bad = true;
+        return b;
+    }
+    c = b;
+    fc = fb;
+    for (int iter = 0; iter < 20; ++iter) {
+        if ((fb > 0.0 && fc > 0.0) || (fb < 0.0 && fc < 0.0)) {
+            c = a;
+            fc = fa;
+            d = b - a;
+            e = d;
+        }
+        if (fabs(fc) < fabs(fb)) {
+            a = b;
+            b = c;
+            c = a;
+            fa = fb;
+            fb = fc;
+            fc = fa;
+        }
+        const double tol1 = 2.0 * eps * fabs(b) + 0.5 * tol;
+        const double xm = 0.5 * (c - b);
+        if (fabs(xm) <= tol1 || fb == 0.0) return b;
+        if (fabs(e) >= tol1 && fabs(fa) > fabs(fb)) {
+            const double ss = fb / fa;
+            double p, q;
+            if (a == c) {
+                p = 2.0 * xm * ss;
+                q = 1.0 - ss;
+            } else {
+                q = fa / fc;
+                const double r = fb / fc;
+                p = ss * (2.0 * xm * q * (q - r) - (b - a) * (r - 1.0));
+                q = (q - 1.0) * (r - 1.0) * (ss - 1.0);
+            }
+            if (p > 0.0) q = -q;
+            p = fabs(p);
+            if (2.0 * p < fmin90(3.0 * xm * q - fabs(tol1 * q), fabs(e * q))) {
+                e = d;
+                d = p / q;
+            } else {
+                d = xm;
+                e = d;
+            }
+        } else {
+            d = xm;
+            e = d;
+        }
+        a = b;
+        fa = fb;
+        if (fabs(d) > tol1) b = b + d;
+        else b = b + copysign(fabs(tol1), xm);
+        fb = ci_func(b, s);
+        if (fb == 0.0) break;
+    }
+    return b;
+}
+
+// PhotosynthesisMod.F90:2251-2368 (only the ci_func side effects matter to the caller)
+__device__ void hybrid(double x0, CiSolve &s)
+{
+    const double eps = 1.e-2, eps1 = 1.e-4;
+    double f0 = ci_func(x0, s);
+    if (f0 == 0.0) return;
+    double minx = x0, minf = f0;
+    double x1 = x0 * 0.99;
+    double f1 = ci_func(x1, s);
+    if (f1 == 0.0) return;
+    if (f1 < minf) {
+        minx = x1;
+        minf = f1;
+    }
+    for (int iter = 1;; ++iter) {
+        const double dx = -f1 * (x1 - x0) / (f1 - f0);
+        const double x = x1 + dx;
+        const double tol = fabs(x) * eps;
+        if (fabs(dx) < tol) return;
+        x0 = x1;
+        f0 = f1;
+        x1 = x;
+        f1 = ci_func(x1, s);
+        if (f1 < minf) {
+            minx = x1;
+            minf = f1;
+        }
+        if (fabs(f1) <= eps1) return;
+        if (f1 * f0 < 0.0) {
+            brent(x0, x1, f0, f1, tol, s);
+            return;
+        }
+        if (iter > 40) {
+            ci_func(minx, s);
+            return;
+        }
+    }
+}
+
+// per-phase outputs of Photosynthesis
+struct Phase {
+    double rs, rs_z, ci_z, lmr, lmr_z, psn, psn_z, psn_wc, psn_wj, psn_wp, gs_ph, gs_ln;
+};
+// photosyns_inst values shared by the phases (last writer wins, as in the reference)
+struct Shared {
+    double vcmax_z, tpu_z, kp_z, gb_mol, ac, aj, ap, ag, an, gs_mol, rh_leaf, vpd_can;
+};
+// leaf-temperature factors of one iteration (each a value the reference
+// computes inside Photosynthesis; products keep the reference's order)
+struct TFac {
+    double kc, ko, cp;
+    double ftL, fthL, dL4;          // lmr: c3 ft*fth, c4 divisor
+    double ftV, fthV, dV1, dV2;     // vcmax: c3 ft*fth, c4 divisors
+    double ftJ, fthJ, ftT, fthT;    // jmax, tpu
+    double pow2;                    // 2**((t_veg - (tfrz+25))/10)
+};
+// constants of the step for one patch
+struct PsnConst {
+    bool c3;
+    double bbb, mbb, qe, lnc, vcmax25top, jmax25top, tpu25top, kp25top, lmr25top, lmrc;
+    double vcmaxse, jmaxse, tpuse, vcmaxc, jmaxc, tpuc, kc25, ko25, cp25;
+    double theta_cj, mslope, mint;
+};
+
+// PhotosynthesisMod.F90:1243-2062 for one patch and one phase.
+__device__ void photosynthesis(const CSwitch &sw, const ctsm_canopy_params &P, const PsnConst &K, const TFac &T,
+                               int nrad, double par_z, double lai_z, double vcmaxcint, double o3v, double o3g,
+                               bool noon, double forc_pbot, double tgcm, double esat_tv, double eair, double oair,
+                               double cair, double rb, double btran, Phase &ph, Shared &sh, bool &bad)
+{
+    const bool medlyn = sw.stomatal == CTSM_STOMATA_MEDLYN2011;
+    double jmax_z = 0.0;
+    if (nrad >= 1) {
+        const double nscaler = vcmaxcint;
+        const double lmr25 = K.lmr25top * nscaler;
+        if (K.c3) {
+            ph.lmr_z = lmr25 * T.ftL * T.fthL;
+        } else {
+            ph.lmr_z = lmr25 * T.pow2;
+            ph.lmr_z = ph.lmr_z / T.dL4;
+        }
+        if (par_z <= 0.0) {
+            sh.vcmax_z = 0.0;
+            jmax_z = 0.0;
+            sh.tpu_z = 0.0;
+            sh.kp_z = 0.0;
+        } else {
+            const double vcmax25 = K.vcmax25top * nscaler;
+            const double jmax25 = K.jmax25top * nscaler;
+            const double tpu25 = K.tpu25top * nscaler;
+            const double kp25 = K.kp25top * nscaler;
+            sh.vcmax_z = vcmax25 * T.ftV * T.fthV;
+            jmax_z = jmax25 * T.ftJ * T.fthJ;
+            sh.tpu_z = tpu25 * T.ftT * T.fthT;
+            if (!K.c3) {
+                sh.vcmax_z = vcmax25 * T.pow2;
+                sh.vcmax_z = sh.vcmax_z / T.dV1;
+                sh.vcmax_z = sh.vcmax_z / T.dV2;
+            }
+            sh.kp_z = kp25 * T.pow2;
+        }
+        sh.vcmax_z = sh.vcmax_z * btran;
+        ph.lmr_z = ph.lmr_z * btran;
+        if (sw.light_inhibit && par_z > 0.0) ph.lmr_z = ph.lmr_z * 0.67;
+    }
+    const double rsmax0 = 2.e4;
+    const double cf = forc_pbot / (RGAS * 1.e-3 * tgcm) * 1.e06;
+    const double gb = 1.0 / rb;
+    sh.gb_mol = gb * cf;
+    double psn_wc_z = 0.0, psn_wj_z = 0.0, psn_wp_z = 0.0;
+    if (nrad >= 1) {
+        if (par_z <= 0.0) {
+            sh.ac = 0.0;
+            sh.aj = 0.0;
+            sh.ap = 0.0;
+            sh.ag = 0.0;
+            sh.an = sh.ag - ph.lmr_z;
+            ph.psn_z = 0.0;
+            if (!medlyn) ph.rs_z = fmin90(rsmax0, 1.0 / K.bbb * cf);
+            else ph.rs_z = fmin90(rsmax0, 1.0 / K.mint * cf);
+            ph.ci_z = 0.0;
+            sh.rh_leaf = 0.0;
+            ph.gs_ph = cf / ph.rs_z;
+        } else {
+            const double ceair = fmin90(eair, esat_tv);
+            double rh_can;
+            if (!medlyn) {
+                rh_can = ceair / esat_tv;
+            } else {
+                rh_can = fmax90(esat_tv - ceair, 50.0) * 0.001;
+                sh.vpd_can = rh_can;
+            }
+            const double qabs = 0.5 * (1.0 - P.fnps) * par_z * 4.6;
+            double r1, r2;
+            if (!quadratic(P.theta_psii, -(qabs + jmax_z), qabs * jmax_z, r1, r2)) bad = true;
+            const double je = fmin90(r1, r2);
+            const double ci0 = K.c3 ? 0.7 * cair : 0.4 * cair;
+            CiSolve s;
+            s.c3 = K.c3;
+            s.medlyn = medlyn;
+            s.forc_pbot = forc_pbot;
+            s.cair = cair;
+            s.oair = oair;
+            s.gb_mol = sh.gb_mol;
+            s.je = je;
+            s.lmr_z = ph.lmr_z;
+            s.par_z = par_z;
+            s.rh_can = rh_can;
+            s.vcmax_z = sh.vcmax_z;
+            s.tpu_z = sh.tpu_z;
+            s.kp_z = sh.kp_z;
+            s.cp = T.cp;
+            s.kc = T.kc;
+            s.ko = T.ko;
+            s.qe = K.qe;
+            s.bbb = K.bbb;
+            s.mbb = K.mbb;
+            s.theta_cj = K.theta_cj;
+            s.theta_ip = P.theta_ip;
+            s.mslope = K.mslope;
+            s.mint = K.mint;
+            s.ac = sh.ac;
+            s.aj = sh.aj;
+            s.ap = sh.ap;
+            s.ag = sh.ag;
+            s.an = sh.an;
+            s.gs_mol = sh.gs_mol;
+            s.bad = false;
+            hybrid(ci0, s);
+            if (s.bad) bad = true;
+            sh.ac = s.ac;
+            sh.aj = s.aj;
+            sh.ap = s.ap;
+            sh.ag = s.ag;
+            sh.an = s.an;
+            sh.gs_mol = s.gs_mol;
+            if (sh.an < 0.0) sh.gs_mol = medlyn ? K.mint : K.bbb;
+            const double gs_mol = sh.gs_mol;
+            ph.gs_ph = gs_mol;
+            ph.gs_ln = noon ? gs_mol : SPVAL;
+            const double an = sh.an;
+            ph.ci_z = cair - an * forc_pbot * (1.4 * gs_mol + 1.6 * sh.gb_mol) / (sh.gb_mol * gs_mol);
+            ph.ci_z = fmax90(ph.ci_z, 1.e-06);
+            const double gs = gs_mol / cf;
+            ph.rs_z = fmin90(1.0 / gs, rsmax0);
+            ph.rs_z = ph.rs_z / o3g;
+            ph.psn_z = sh.ag;
+            ph.psn_z = ph.psn_z * o3v;
+            if (sh.ac <= sh.aj && sh.ac <= sh.ap) psn_wc_z = ph.psn_z;
+            else if (sh.aj < sh.ac && sh.aj <= sh.ap) psn_wj_z = ph.psn_z;
+            else if (sh.ap < sh.ac && sh.ap < sh.aj) psn_wp_z = ph.psn_z;
+            if (gs_mol < 0.0) bad = true;
+            if (!medlyn) sh.rh_leaf = (sh.gb_mol * ceair + gs_mol * esat_tv) / ((sh.gb_mol + gs_mol) * esat_tv);
+        }
+    }
+    double psncan = 0.0, psncan_wc = 0.0, psncan_wj = 0.0, psncan_wp = 0.0, lmrcan = 0.0, gscan = 0.0, laican = 0.0;
+    if (nrad >= 1) {
+        psncan = psncan + ph.psn_z * lai_z;
+        psncan_wc = psncan_wc + psn_wc_z * lai_z;
+        psncan_wj = psncan_wj + psn_wj_z * lai_z;
+        psncan_wp = psncan_wp + psn_wp_z * lai_z;
+        lmrcan = lmrcan + ph.lmr_z * lai_z;
+        gscan = gscan + lai_z / (rb + ph.rs_z);
+        laican = laican + lai_z;
+    }
+    if (laican > 0.0) {
+        ph.psn = psncan / laican;
+        ph.psn_wc = psncan_wc / laican;
+        ph.psn_wj = psncan_wj / laican;
+        ph.psn_wp = psncan_wp / laican;
+        ph.lmr = lmrcan / laican;
+        ph.rs = laican / gscan - rb;
+    } else {
+        ph.psn = 0.0;
+        ph.psn_wc = 0.0;
+        ph.psn_wj = 0.0;
+        ph.psn_wp = 0.0;
+        ph.lmr = 0.0;
+        ph.rs = 0.0;
+    }
+}
+
+}  // namespace canopy
+}  // namespace ctsm
+
+namespace ctsm {
+namespace canopy {
+
+// constants of the time step for one patch (PhotosynthesisMod.F90:1525-1722)
+__device__ void psn_constants(const CSwitch &sw, const ctsm_canopy_params &P, const double *__restrict__ pft, int ivt,
+                              double forc_pbot, double oair, double t10, double btran, double dayl_factor,
+                              PsnConst &K, bool &bad)
+{
+    const bool medlyn = sw.stomatal == CTSM_STOMATA_MEDLYN2011;
+    K.lmrc = fth25(P.lmrhd, P.lmrse);
+    K.c3 = ((int)rint(pft[F_C3PSN * CTSM_MAX_PFT + ivt]) == 1);
+    double bbbopt = 0.0;
+    if (K.c3) {
+        K.qe = 0.0;
+        if (!medlyn) bbbopt = 10000.0;
+    } else {
+        K.qe = 0.05;
+        if (!medlyn) bbbopt = 40000.0;
+    }
+    K.bbb = fmax90(bbbopt * btran, 1.0);
+    K.mbb = pft[F_MBBOPT * CTSM_MAX_PFT + ivt];
+    K.kc25 = P.kc25_coef * forc_pbot;
+    K.ko25 = P.ko25_coef * forc_pbot;
+    const double sco = 0.5 * 0.209 / P.cp25_yr2000;
+    K.cp25 = 0.5 * oair / sco;
+    const double slatop = pft[F_SLATOP * CTSM_MAX_PFT + ivt], leafcn = pft[F_LEAFCN * CTSM_MAX_PFT + ivt];
+    if (slatop * leafcn <= 0.0) bad = true;
+    K.lnc = 1.0 / (slatop * leafcn);
+    K.vcmax25top = K.lnc * pft[F_FLNR * CTSM_MAX_PFT + ivt] * P.fnr * P.act25 * dayl_factor;
+    K.vcmax25top = K.vcmax25top * pft[F_FNITR * CTSM_MAX_PFT + ivt];
+    const double t10c = fmin90(fmax90(t10 - TFRZ, 11.0), 35.0);
+    K.jmax25top = ((2.59 - 0.035 * t10c) * K.vcmax25top) * P.jmax25top_sf;
+    K.tpu25top = P.tpu25ratio * K.vcmax25top;
+    K.kp25top = P.kp25ratio * K.vcmax25top;
+    K.lmr25top = K.c3 ? K.vcmax25top * P.leaf_mr_vcm : K.vcmax25top * 0.025;
+    K.vcmaxse = (668.39 - 1.07 * t10c) * P.vcmaxse_sf;
+    K.jmaxse = (659.70 - 0.75 * t10c) * P.jmaxse_sf;
+    K.tpuse = (668.39 - 1.07 * t10c) * P.tpuse_sf;
+    K.vcmaxc = fth25(P.vcmaxhd, K.vcmaxse);
+    K.jmaxc = fth25(P.jmaxhd, K.jmaxse);
+    K.tpuc = fth25(P.tpuhd, K.tpuse);
+    K.theta_cj = pft[F_THETA_CJ * CTSM_MAX_PFT + ivt];
+    K.mslope = pft[F_MEDSLOPE * CTSM_MAX_PFT + ivt];
+    K.mint = pft[F_MEDINT * CTSM_MAX_PFT + ivt];
+}
+
+__device__ void temp_factors(const ctsm_canopy_params &P, const PsnConst &K, double tv, bool anyday, TFac &T)
+{
+    T.kc = K.kc25 * ft(tv, P.kcha);
+    T.ko = K.ko25 * ft(tv, P.koha);
+    T.cp = K.cp25 * ft(tv, P.cpha);
+    T.pow2 = cm_pow(2.0, (tv - (TFRZ + 25.0)) / 10.0);
+    if (K.c3) {
+        T.ftL = ft(tv, P.lmrha);
+        T.fthL = fth(tv, P.lmrhd, P.lmrse, K.lmrc);
+    } else {
+        T.dL4 = 1.0 + cm_exp(1.3 * (tv - (TFRZ + 55.0)));
+    }
+    if (anyday) {
+        if (K.c3) {
+            T.ftV = ft(tv, P.vcmaxha);
+            T.fthV = fth(tv, P.vcmaxhd, K.vcmaxse, K.vcmaxc);
+        } else {
+            T.dV1 = 1.0 + cm_exp(0.2 * ((TFRZ + 15.0) - tv));
+            T.dV2 = 1.0 + cm_exp(0.3 * (tv - (TFRZ + 40.0)));
+        }
+        T.ftJ = ft(tv, P.jmaxha);
+        T.fthJ = fth(tv, P.jmaxhd, K.jmaxse, K.jmaxc);
+        T.ftT = ft(tv, P.tpuha);
+        T.fthT = fth(tv, P.tpuhd, K.tpuse, K.tpuc);
+    }
+}
+
+#define PF(f) pft[(f) * CTSM_MAX_PFT + ivt]
+
+__global__ void __launch_bounds__(64)
+k_canopy(CView V, CSwitch sw, ctsm_canopy_params P, const double *__restrict__ pft, ctsm_col_state col,
+         ctsm_canopy_state S, int n, const int32_t *__restrict__ filter, double dtime, ErrWord *err)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int p = filter ? filter[i] : V.begp + i;
+    const long pp = V.p1(p);
+    const int c = S.column[pp];
+    const int g = S.gridcell[pp];
+    const int ivt = S.itype[pp];
+    // indices that address other arrays must lie inside their allocations
+    if (ivt < 0 || ivt >= CTSM_MAX_PFT || c < V.begc || c >= V.begc + V.ldc || g < V.begg || g >= V.begg + V.ldg) {
+        report_error(err, p, CTSM_E_ARG);
+        return;
+    }
+    const long cc = V.c1(c), gg = V.g1(g);
+    const double btran0 = 0.0, zii = 1000.0, beta = 1.0, delmax = 1.0, dlemin = 0.1, dtmin = 0.01;
+    const int itmin = 2;
+    const double ria = 0.5;
+    const double k_vert = 0.1, k_cyl_vol = 1.0, k_cyl_area = 1.0, k_internal = 0.0;
+    const double min_stem_diameter = 0.05, min_lai = 0.1;
+    bool bad = false;
+
+    const double elai = S.elai[pp], esai = S.esai[pp], htop = S.htop[pp];
+    const double emv = S.emv[pp], thm = S.thm[pp];
+    const double forc_pbot = S.forc_pbot[cc], forc_q = S.forc_q[cc], forc_th = S.forc_th[cc];
+    const double forc_rho = S.forc_rho[cc], thv = S.thv[cc], qg = S.qg[cc];
+    const double t_grnd = col.t_grnd[cc], emg = col.emg[cc];
+
+    // ---- biomass heat capacities (CanopyFluxesMod.F90:717-819)
+    double frac_rad_abs_by_stem, sa_leaf, sa_stem, sa_internal, cp_leaf, cp_stem, rstem;
+    double stem_biomass = 0.0, leaf_biomass = 0.0;
+    if (sw.biomass) {
+        frac_rad_abs_by_stem = (esai) / (elai + esai);
+        if (elai > 0.0) frac_rad_abs_by_stem = k_vert * frac_rad_abs_by_stem;
+        const double dbh = PF(F_DBH);
+        sa_leaf = elai;
+        sa_leaf = 2.0 * sa_leaf;
+        sa_stem = PF(F_NSTEM) * (htop * PI * dbh);
+        sa_stem = k_cyl_area * sa_stem;
+        if (!((int)PF(F_IS_TREE) || (int)PF(F_IS_SHRUB)) || dbh < min_stem_diameter) {
+            frac_rad_abs_by_stem = 0.0;
+            sa_stem = 0.0;
+            sa_leaf = sa_leaf + esai;
+        } else {
+            if (elai < min_lai) sa_leaf = sa_leaf + esai;
+        }
+        const double fbw = PF(F_FBW);
+        leaf_biomass = (1.e-3 * C_TO_B / PF(F_SLATOP)) * fmax90(0.01, 0.5 * sa_leaf) / (1.0 - fbw);
+        const double carea_stem = PI * ((dbh * 0.5) * (dbh * 0.5));
+        stem_biomass = carea_stem * htop * k_cyl_vol * PF(F_NSTEM) * PF(F_WOOD) / (1.0 - fbw);
+        sa_internal = fmin90(sa_leaf, sa_stem);
+        sa_internal = k_internal * sa_internal;
+        cp_leaf = leaf_biomass * (CDRYBIO * (1.0 - fbw) + (fbw) * CWATER);
+        cp_stem = stem_biomass * (CDRYBIO * (1.0 - fbw) + (fbw) * CWATER);
+        cp_stem = k_cyl_vol * cp_stem;
+        rstem = PF(F_RSTEM) * dbh;
+        S.leaf_biomass[pp] = leaf_biomass;
+        S.stem_biomass[pp] = stem_biomass;
+    } else {
+        sa_leaf = (elai + esai);
+        frac_rad_abs_by_stem = 0.0;
+        sa_stem = 0.0;
+        sa_internal = 0.0;
+        cp_leaf = 0.0;
+        cp_stem = 0.0;
+        rstem = 0.0;
+    }
+    const double dayl = S.dayl[gg], max_dayl = S.max_dayl[gg];
+    const double dayl_factor = fmin90(1.0, fmax90(0.01, (dayl * dayl) / (max_dayl * max_dayl)));
+
+    // ---- effective porosity, volumetric liquid water, root moisture stress
+    double btran = btran0;
+    double rr[MAX_NLEVGRND];
+    {
+        const double smpsc = PF(F_SMPSC), smpso = PF(F_SMPSO);
+#pragma unroll
+        for (int j = 1; j <= MAX_NLEVGRND; ++j) {
+            rr[j - 1] = 0.0;
+            if (j > V.nlevgrnd) continue;
+            const long cs = V.soi(c, j), cn = V.sno(c, j);
+            const double watsat = col.watsat[cs], dz = col.dz[cn];
+            const double vol_ice = fmin90(watsat, col.h2osoi_ice[cn] / (DENICE * dz));
+            const double eff = watsat - vol_ice;
+            col.eff_porosity[cs] = eff;
+            const double liqvol = fmin90(eff, col.h2osoi_liq[cn] / (dz * DENH2O));
+            S.h2osoi_liqvol[cs] = liqvol;
+            if (liqvol <= 0.0 || col.t_soisno[cn] <= TFRZ - 2.0) {
+                rr[j - 1] = 0.0;
+            } else {
+                const double s_node = fmax90(liqvol / eff, 0.01);
+                double smp_node = -col.sucsat[cs] * cm_pow(s_node, -col.bsw[cs]);
+                smp_node = fmax90(smpsc, smp_node);
+                const double rres = fmin90((eff / watsat) * (smp_node - smpsc) / (smpso - smpsc), 1.0);
+                S.rresis[V.soip(p, j)] = rres;
+                rr[j - 1] = S.rootfr[V.soip(p, j)] * rres;
+                btran = btran + fmax90(rr[j - 1], 0.0);
+            }
+        }
+#pragma unroll
+        for (int j = 1; j <= MAX_NLEVGRND; ++j) {
+            if (j > V.nlevgrnd) continue;
+            S.rootr[V.soip(p, j)] = (btran > btran0) ? rr[j - 1] / btran : 0.0;
+        }
+    }
+
+    // ---- aerodynamic parameters (CanopyFluxesMod.F90:897-943)
+    double displa = S.displa[pp], z0mv = S.z0mv[pp];
+    if (sw.z0param == CTSM_Z0_ZENGWANG2007) {
+        const double lt = fmin90(elai + esai, TLSAI_CRIT);
+        const double egvf = (1.0 - ALPHA_AERO * cm_exp(-lt)) / (1.0 - ALPHA_AERO * cm_exp(-TLSAI_CRIT));
+        displa = egvf * displa;
+        z0mv = cm_exp(egvf * cm_log(z0mv) + (1.0 - egvf) * cm_log(S.z0mg[cc]));
+    } else {
+        double lt = fmax90(1.e-5, elai + esai);
+        displa = htop * (1.0 - (1.0 - cm_exp(-cm_pow(CD1 * lt, 0.5))) / cm_pow(CD1 * lt, 0.5));
+        lt = fmin90(lt, PF(F_Z0V_LAIMAX));
+        double delt = 2.0;
+        const double zc = PF(F_Z0V_C);
+        const double U_ustar_ini = cm_pow(PF(F_Z0V_CS) + PF(F_Z0V_CR) * lt * 0.5, -0.5) * zc * lt * 0.25;
+        double U_ustar = U_ustar_ini;
+        int guard = 0;
+        while (delt > 1.e-4) {
+            const double prev = U_ustar;
+            U_ustar = U_ustar_ini * cm_exp(prev);
+            delt = fabs(U_ustar - prev);
+            if (++guard > MAX_FIXED_POINT) { bad = true; break; }
+        }
+        U_ustar = 4.0 * U_ustar / lt / zc;
+        const double zcw = PF(F_Z0V_CW);
+        z0mv = htop * (1.0 - displa / htop) * cm_exp(-VKC * U_ustar + cm_log(zcw) - 1.0 + 1.0 / zcw);
+    }
+    const double z0hv = z0mv, z0qv = z0mv;
+    const double hu = S.forc_hgt_u[gg] + z0mv + displa;
+    const double ht = S.forc_hgt_t[gg] + z0hv + displa;
+    const double hq = S.forc_hgt_q[gg] + z0qv + displa;
+    S.displa[pp] = displa;
+    S.z0mv[pp] = z0mv;
+    S.z0hv[pp] = z0hv;
+    S.z0qv[pp] = z0qv;
+    S.forc_hgt_u_patch[pp] = hu;
+    S.forc_hgt_t_patch[pp] = ht;
+    S.forc_hgt_q_patch[pp] = hq;
+
+    // ---- radiation terms, leaf humidity, initial profiles (:945-1002)
+    const double forc_lwrad = col.forc_lwrad[cc];
+    const double air = emv * (1.0 + (1.0 - emv) * (1.0 - emg)) * forc_lwrad;
+    const double bir = -(2.0 - emv * (1.0 - emg)) * emv * SB;
+    const double cir = emv * emg * SB;
+    double t_veg = S.t_veg[pp];
+    double qsatl, el, qsatldT;
+    qsat(t_veg, forc_pbot, qsatl, el, qsatldT);
+    const double co2 = S.forc_pco2[gg], o2 = S.forc_po2[gg];
+    int nmozsgn = 0;
+    double taf = (t_grnd + thm) / 2.0;
+    double qaf = (forc_q + qg) / 2.0;
+    const double fu = S.forc_u[gg], fvv = S.forc_v[gg];
+    const double ur = fmax90(P.wind_min, sqrt(fu * fu + fvv * fvv));
+    double dth = thm - taf;
+    double dqh = forc_q - qaf;
+    double delq = qg - qaf;
+    const double dthv = dth * (1.0 + 0.61 * forc_q) + 0.61 * forc_th * dqh;
+    const double zldis = hu - displa;
+    if (zldis < 0.0) {
+        report_error(err, p, ENDRUN);
+        return;
+    }
+    double um, obu;
+    {
+        // FrictionVelocityMod.F90:1158-1209
+        const double wc = 0.5;
+        if (dthv >= 0.0) um = fmax90(ur, 0.1);
+        else um = sqrt(ur * ur + wc * wc);
+        const double rib = GRAV * zldis * dthv / (thv * um * um);
+        double zeta;
+        if (rib >= 0.0) {
+            zeta = rib * cm_log(zldis / z0mv) / (1.0 - 5.0 * fmin90(rib, 0.19));
+            zeta = fmin90(P.zetamaxstable, fmax90(zeta, 0.01));
+        } else {
+            zeta = rib * cm_log(zldis / z0mv);
+            zeta = fmax90(-100.0, fmin90(zeta, -0.01));
+        }
+        obu = zldis / zeta;
+    }
+    double num_iter = 0.0;
+    const double tl_ini = t_veg;
+    double t_stem = S.t_stem[pp];
+    const double ts_ini = t_stem;
+
+    // per-step constants of Photosynthesis
+    const int nrad = S.nrad[pp];
+    const double parsun = S.parsun_z[pp], parsha = S.parsha_z[pp];
+    const bool noon = S.near_local_noon[gg] != 0;
+    const bool anyday = nrad >= 1 && (parsun > 0.0 || parsha > 0.0);
+    PsnConst K;
+    psn_constants(sw, P, pft, ivt, forc_pbot, o2, S.t_a10[pp], btran, dayl_factor, K, bad);
+    Phase sun, sha;
+    Shared sh;
+    sh.gs_mol = S.gs_mol[pp];
+    sh.rh_leaf = S.rh_leaf[pp];
+    sh.vpd_can = S.vpd_can[pp];
+    sun.gs_ln = S.gs_mol_sun_ln[pp];
+    sha.gs_ln = S.gs_mol_sha_ln[pp];
+    if (nrad < 1) {
+        // nothing below overwrites these: keep the stored values
+        sh.vcmax_z = S.vcmax_z[pp]; sh.tpu_z = S.tpu_z[pp]; sh.kp_z = S.kp_z[pp];
+        sh.ac = S.ac[pp]; sh.aj = S.aj[pp]; sh.ap = S.ap[pp]; sh.ag = S.ag[pp]; sh.an = S.an[pp];
+        sun.rs_z = S.rssun_z[pp]; sun.ci_z = S.cisun_z[pp]; sun.lmr_z = S.lmrsun_z[pp]; sun.psn_z = S.psnsun_z[pp];
+        sha.rs_z = S.rssha_z[pp]; sha.ci_z = S.cisha_z[pp]; sha.lmr_z = S.lmrsha_z[pp]; sha.psn_z = S.psnsha_z[pp];
+        sun.gs_ph = S.gs_mol_sun[pp];
+        sha.gs_ph = S.gs_mol_sha[pp];
+    }
+    TFac T;
+
+    // ---- iteration state
+    double del = 0.0, del2 = 0.0, efeb = 0.0, efe = 0.0, obuold = 0.0;
+    double wtlq0 = 0.0, wtalq = 0.0, wtgq = 0.0, wtaq0 = 0.0;
+    double tlbef = t_veg, dt_veg = 0.0, errp = 0.0;
+    double ustar = 0.0, temp1 = 0.0, temp2 = 0.0, temp12m = 0.0, temp22m = 0.0;
+    double ram1 = 0.0, rb = 0.0, rah1 = 0.0, rah2 = 0.0, raw1 = 0.0, raw2 = 0.0, uaf = 0.0;
+    double wtg = 0.0, wta0 = 0.0, wtl0 = 0.0, wtstem0 = 0.0, wtal = 0.0, wtga = 0.0;
+    double lw_stem = 0.0, lw_leaf = 0.0, eflx_sh_veg = 0.0, eflx_sh_stem = 0.0;
+    double qflx_tran_veg = S.qflx_tran_veg[pp], qflx_evap_veg = 0.0, zeta = 0.0, svpts = 0.0, eah = 0.0;
+    double rh_af = 0.0, vpd = 0.0;
+    FV fv;
+    fv.fm = 0.0;
+    const double laisun = S.laisun[pp], laisha = S.laisha[pp], fdry = S.fdry[pp], fwet = S.fwet[pp];
+    const double liqcan0 = S.liqcan[pp], snocan0 = S.snocan[pp];
+    const double h2ocan = liqcan0 + snocan0;
+    const double sabv = S.sabv[pp];
+    const double fvn = (double)S.frac_veg_nosno[pp];
+    const double z0mg = S.z0mg[cc];
+    const double dleaf = PF(F_DLEAF);
+    const int snl = col.snl[cc];
+    if (snl < -V.nlevsno || snl > 0) {
+        report_error(err, p, CTSM_E_ARG);
+        return;
+    }
+    const double frac_sno = col.frac_sno_eff[cc], frac_h2osfc = col.frac_h2osfc[cc];
+    const double t_top = col.t_soisno[V.sno(c, snl + 1)], t_soi1 = col.t_soisno[V.sno(c, 1)];
+    const double t_h2osfc = col.t_h2osfc[cc];
+    const double lw_grnd = (frac_sno * p4(t_top) + (1.0 - frac_sno - frac_h2osfc) * p4(t_soi1)
+                            + frac_h2osfc * p4(t_h2osfc));
+    const double lai_dl_eff = P.lai_dl * (1.0 - fmin90(col.snow_depth[cc] / P.z_dl, 1.0));
+
+    for (int itlef = 0; itlef <= sw.itmax;) {
+        friction_velocity(hu, ht, hq, displa, z0mv, z0hv, z0qv, obu, itlef + 1, ur, um, fv);
+        ustar = fv.ustar;
+        temp1 = fv.temp1;
+        temp2 = fv.temp2;
+        temp12m = fv.temp12m;
+        temp22m = fv.temp22m;
+        tlbef = t_veg;
+        del2 = del;
+        ram1 = 1.0 / (ustar * ustar / um);
+        rah1 = 1.0 / (temp1 * ustar);
+        raw1 = 1.0 / (temp2 * ustar);
+        uaf = um * sqrt(1.0 / (ram1 * um));
+        const double uuc = fmin90(0.4, (0.03 * um / ustar));
+        const double cf = P.cv / (sqrt(uaf) * sqrt(dleaf));
+        rb = 1.0 / (cf * uaf);
+        const double w = cm_exp(-(elai + esai));
+        const double csoilb = VKC / (P.a_coef * cm_pow(z0mg * uaf / NU, P.a_exp));
+        const double ri = (GRAV * htop * (taf - t_grnd)) / (taf * (uaf * uaf));
+        double csoilcn;
+        if (sw.undercanopy && (taf - t_grnd) > 0.0) {
+            const double ricsoilc = P.csoilc / (1.00 + ria * fmin90(ri, 10.0));
+            csoilcn = csoilb * w + ricsoilc * (1.0 - w);
+        } else {
+            csoilcn = csoilb * w + P.csoilc * (1.0 - w);
+        }
+        rah2 = sw.biomass ? 1.0 / (csoilcn * uuc) : 1.0 / (csoilcn * uaf);
+        raw2 = rah2;
+        svpts = el;
+        eah = forc_pbot * qaf / 0.622;
+        rh_af = eah / svpts;
+        vpd = fmax90((svpts - eah), 50.0) * 0.001;
+
+        // Photosynthesis, sun then shade
+        temp_factors(P, K, t_veg, anyday, T);
+        photosynthesis(sw, P, K, T, nrad, parsun, S.laisun_z[pp], S.vcmaxcintsun[pp], S.o3coefvsun[pp],
+                       S.o3coefgsun[pp], noon, forc_pbot, thm, svpts, eah, o2, co2, rb, btran, sun, sh, bad);
+        photosynthesis(sw, P, K, T, nrad, parsha, S.laisha_z[pp], S.vcmaxcintsha[pp], S.o3coefvsha[pp],
+                       S.o3coefgsha[pp], noon, forc_pbot, thm, svpts, eah, o2, co2, rb, btran, sha, sh, bad);
+
+        // conductances and the leaf-temperature Newton step (:1119-1331)
+        const double wta = 1.0 / rah1;
+        const double wtl = sa_leaf / rb;
+        wtg = 1.0 / rah2;
+        const double wtstem = sa_stem / (rstem + rb);
+        const double wtshi = 1.0 / (wta + wtl + wtstem + wtg);
+        wtl0 = wtl * wtshi;
+        const double wtg0 = wtg * wtshi;
+        wta0 = wta * wtshi;
+        wtstem0 = wtstem * wtshi;
+        wtga = wta0 + wtg0 + wtstem0;
+        wtal = wta0 + wtl0 + wtstem0;
+        lw_stem = sa_internal * emv * SB * p4(t_stem);
+        lw_leaf = sa_internal * emv * SB * p4(t_veg);
+        double rppdry;
+        if (fdry > 0.0) rppdry = fdry * rb * (laisun / (rb + sun.rs) + laisha / (rb + sha.rs)) / elai;
+        else rppdry = 0.0;
+        double efpot = forc_rho * ((elai + esai) / rb) * (qsatl - qaf);
+        double rpp;
+        if (efpot > 0.0) {
+            if (btran > btran0) {
+                qflx_tran_veg = efpot * rppdry;
+                rpp = rppdry + fwet;
+            } else {
+                rpp = fwet;
+                qflx_tran_veg = 0.0;
+            }
+            rpp = fmin90(rpp, (qflx_tran_veg + h2ocan / dtime) / efpot);
+        } else {
+            rpp = 1.0;
+            qflx_tran_veg = 0.0;
+        }
+        const double wtaq = fvn / raw1;
+        const double wtlq = fvn * (elai + esai) / rb * rpp;
+        const double rdl = (1.0 - cm_exp(-lai_dl_eff)) / (0.004 * uaf);
+        if (delq < 0.0) {
+            wtgq = fvn / (raw2 + rdl);
+        } else {
+            if (sw.soil_resis == CTSM_SOILRESIS_LEEPIELKE1992) wtgq = S.soilbeta[cc] * fvn / (raw2 + rdl);
+            if (sw.soil_resis == CTSM_SOILRESIS_SL14) wtgq = fvn / (raw2 + S.soilresis[cc]);
+        }
+        const double wtsqi = 1.0 / (wtaq + wtlq + wtgq);
+        const double wtgq0 = wtgq * wtsqi;
+        wtlq0 = wtlq * wtsqi;
+        wtaq0 = wtaq * wtsqi;
+        const double wtgaq = wtaq0 + wtgq0;
+        wtalq = wtaq0 + wtlq0;
+        const double dc1 = forc_rho * CPAIR * wtl;
+        const double dc2 = HVAP * forc_rho * wtlq;
+        const double efsh = dc1 * (wtga * t_veg - wtg0 * t_grnd - wta0 * thm - wtstem0 * t_stem);
+        eflx_sh_stem = forc_rho * CPAIR * wtstem * ((wta0 + wtg0 + wtl0) * t_stem - wtg0 * t_grnd - wta0 * thm - wtl0 * t_veg);
+        efe = dc2 * (wtgaq * qsatl - wtgq0 * qg - wtaq0 * forc_q);
+        double erre = 0.0;
+        if (efe * efeb < 0.0) {
+            const double efeold = efe;
+            efe = 0.1 * efeold;
+            erre = efe - efeold;
+        }
+        const double fr = frac_rad_abs_by_stem;
+        dt_veg = ((1.0 - fr) * (sabv + air + bir * p4(t_veg) + cir * lw_grnd)
+                  - efsh - efe - lw_leaf + lw_stem - (cp_leaf / dtime) * (t_veg - tl_ini))
+                 / ((1.0 - fr) * (-4.0 * bir * p3(t_veg)) + 4.0 * sa_internal * emv * SB * p3(t_veg)
+                    + dc1 * wtga + dc2 * wtgaq * qsatldT + cp_leaf / dtime);
+        t_veg = tlbef + dt_veg;
+        const double dels = dt_veg;
+        del = fabs(dels);
+        errp = 0.0;
+        if (del > delmax) {
+            dt_veg = delmax * dels / del;
+            t_veg = tlbef + dt_veg;
+            errp = (1.0 - fr) * (sabv + air + bir * p3(tlbef) * (tlbef + 4.0 * dt_veg) + cir * lw_grnd)
+                   - sa_internal * emv * SB * p3(tlbef) * (tlbef + 4.0 * dt_veg)
+                   + lw_stem
+                   - (efsh + dc1 * wtga * dt_veg) - (efe + dc2 * wtgaq * qsatldT * dt_veg)
+                   - (cp_leaf / dtime) * (t_veg - tl_ini);
+        }
+        efpot = forc_rho * ((elai + esai) / rb) * (wtgaq * (qsatl + qsatldT * dt_veg) - wtgq0 * qg - wtaq0 * forc_q);
+        qflx_evap_veg = rpp * efpot;
+        if (efpot > 0.0 && btran > btran0) qflx_tran_veg = efpot * rppdry;
+        else qflx_tran_veg = 0.0;
+        const double ecidif = fmax90(0.0, qflx_evap_veg - qflx_tran_veg - h2ocan / dtime);
+        qflx_evap_veg = fmin90(qflx_evap_veg, qflx_tran_veg + h2ocan / dtime);
+        eflx_sh_veg = efsh + dc1 * wtga * dt_veg + errp + erre + HVAP * ecidif;
+        eflx_sh_stem = eflx_sh_stem + forc_rho * CPAIR * wtstem * (-wtl0 * dt_veg);
+        lw_leaf = sa_internal * emv * SB * p3(tlbef) * (tlbef + 4.0 * dt_veg);
+        qsat(t_veg, forc_pbot, qsatl, el, qsatldT);
+        taf = wtg0 * t_grnd + wta0 * thm + wtl0 * t_veg + wtstem0 * t_stem;
+        qaf = wtlq0 * qsatl + wtgq0 * qg + forc_q * wtaq0;
+        dth = thm - taf;
+        dqh = forc_q - qaf;
+        delq = wtalq * qg - wtlq0 * qsatl - wtaq0 * forc_q;
+        const double tstar = temp1 * dth;
+        const double qstar = temp2 * dqh;
+        const double thvstar = tstar * (1.0 + 0.61 * forc_q) + 0.61 * forc_th * qstar;
+        zeta = zldis * VKC * GRAV * thvstar / ((ustar * ustar) * thv);
+        if (zeta >= 0.0) {
+            zeta = fmin90(P.zetamaxstable, fmax90(zeta, 0.01));
+            um = fmax90(ur, 0.1);
+        } else {
+            zeta = fmax90(-100.0, fmin90(zeta, -0.01));
+            double wc;
+            if (ustar * thvstar > 0.0) wc = 0.0;
+            else wc = beta * cm_pow(-GRAV * ustar * thvstar * zii / thv, 0.333);
+            um = sqrt(ur * ur + wc * wc);
+        }
+        obu = zldis / zeta;
+        if (obuold * obu < 0.0) nmozsgn = nmozsgn + 1;
+        if (nmozsgn >= 4) obu = zldis / (-0.01);
+        obuold = obu;
+
+        itlef = itlef + 1;
+        if (itlef > itmin) {
+            const double dele = fabs(efe - efeb);
+            efeb = efe;
+            const double det = fmax90(del, del2);
+            num_iter = (double)itlef;
+            if (det < dtmin && dele < dlemin) break;
+        }
+    }
+
+    // ---- fluxes after the iteration (:1467-1615)
+    const double fr = frac_rad_abs_by_stem;
+    errp = (1.0 - fr) * (sabv + air + bir * p3(tlbef) * (tlbef + 4.0 * dt_veg) + cir * lw_grnd)
+           - lw_leaf + lw_stem - eflx_sh_veg - HVAP * qflx_evap_veg - ((t_veg - tl_ini) * cp_leaf / dtime);
+    double dt_stem = 0.0;
+    double dhsdt_canopy = 0.0;
+    if (sw.biomass) {
+        if (stem_biomass > 0.0) {
+            dt_stem = (fr * (sabv + air + bir * p4(ts_ini) + cir * lw_grnd) - eflx_sh_stem + lw_leaf - lw_stem)
+                      / (cp_stem / dtime - fr * bir * 4.0 * p3(ts_ini));
+        } else {
+            dt_stem = 0.0;
+        }
+        dhsdt_canopy = dt_stem * cp_stem / dtime + (t_veg - tl_ini) * cp_leaf / dtime;
+        t_stem = t_stem + dt_stem;
+    }
+    const double delt = wtal * t_grnd - wtl0 * t_veg - wta0 * thm - wtstem0 * t_stem;
+    S.taux[pp] = -forc_rho * fu / ram1;
+    S.tauy[pp] = -forc_rho * fvv / ram1;
+    S.eflx_sh_grnd[pp] = CPAIR * forc_rho * wtg * delt;
+    const double delt_snow = wtal * t_top - wtl0 * t_veg - wta0 * thm - wtstem0 * t_stem;
+    const double delt_soil = wtal * t_soi1 - wtl0 * t_veg - wta0 * thm - wtstem0 * t_stem;
+    const double delt_h2osfc = wtal * t_h2osfc - wtl0 * t_veg - wta0 * thm - wtstem0 * t_stem;
+    S.eflx_sh_snow[pp] = CPAIR * forc_rho * wtg * delt_snow;
+    S.eflx_sh_soil[pp] = CPAIR * forc_rho * wtg * delt_soil;
+    S.eflx_sh_h2osfc[pp] = CPAIR * forc_rho * wtg * delt_h2osfc;
+    S.qflx_evap_soi[pp] = forc_rho * wtgq * delq;
+    const double delq_snow = wtalq * S.qg_snow[cc] - wtlq0 * qsatl - wtaq0 * forc_q;
+    S.qflx_ev_snow[pp] = forc_rho * wtgq * delq_snow;
+    const double delq_soil = wtalq * S.qg_soil[cc] - wtlq0 * qsatl - wtaq0 * forc_q;
+    S.qflx_ev_soil[pp] = forc_rho * wtgq * delq_soil;
+    const double delq_h2osfc = wtalq * S.qg_h2osfc[cc] - wtlq0 * qsatl - wtaq0 * forc_q;
+    S.qflx_ev_h2osfc[pp] = forc_rho * wtgq * delq_h2osfc;
+    const double t_ref2m = thm + temp1 * dth * (1.0 / temp12m - 1.0 / temp1);
+    S.t_ref2m[pp] = t_ref2m;
+    S.t_ref2m_r[pp] = t_ref2m;
+    const double q_ref2m = forc_q + temp2 * dqh * (1.0 / temp22m - 1.0 / temp2);
+    S.q_ref2m[pp] = q_ref2m;
+    double e_ref2m;
+    const double qsat_ref2m = qsat_only(t_ref2m, forc_pbot, e_ref2m);
+    const double rh_ref2m = fmin90(100.0, q_ref2m / qsat_ref2m * 100.0);
+    S.rh_ref2m[pp] = rh_ref2m;
+    S.rh_ref2m_r[pp] = rh_ref2m;
+    S.vpd_ref2m[pp] = e_ref2m * (1.0 - rh_ref2m / 100.0);
+    S.dlrad[pp] = (1.0 - emv) * emg * forc_lwrad
+                  + emv * emg * SB * p3(tlbef) * (tlbef + 4.0 * dt_veg) * (1.0 - fr)
+                  + emv * emg * SB * p3(ts_ini) * (ts_ini + 4.0 * dt_stem) * fr;
+    S.ulrad[pp] = ((1.0 - emg) * (1.0 - emv) * (1.0 - emv) * forc_lwrad
+                   + emv * (1.0 + (1.0 - emg) * (1.0 - emv)) * SB * p3(tlbef) * (tlbef + 4.0 * dt_veg) * (1.0 - fr)
+                   + emv * (1.0 + (1.0 - emg) * (1.0 - emv)) * SB * p3(ts_ini) * (ts_ini + 4.0 * dt_stem) * fr
+                   + emg * (1.0 - emv) * SB * lw_grnd);
+    S.t_skin[pp] = emv * t_veg + (1.0 - emv) * sqrt(sqrt(lw_grnd));
+    const double cgrnds = S.cgrnds[pp] + CPAIR * forc_rho * wtg * wtal;
+    const double cgrndl = S.cgrndl[pp] + forc_rho * wtgq * wtalq * S.dqgdT[cc];
+    S.cgrnds[pp] = cgrnds;
+    S.cgrndl[pp] = cgrndl;
+    S.cgrnd[pp] = cgrnds + cgrndl * col.htvp[cc];
+    double snocan = snocan0, liqcan = liqcan0;
+    const double snocan_baseline = snocan;
+    if (t_veg > TFRZ) {
+        if ((qflx_evap_veg - qflx_tran_veg) * dtime > liqcan)
+            snocan = fmax90(0.0, snocan + liqcan + (qflx_tran_veg - qflx_evap_veg) * dtime);
+        liqcan = fmax90(0.0, liqcan + (qflx_tran_veg - qflx_evap_veg) * dtime);
+    } else if (t_veg <= TFRZ) {
+        if ((qflx_evap_veg - qflx_tran_veg) * dtime > snocan)
+            liqcan = liqcan + snocan + (qflx_tran_veg - qflx_evap_veg) * dtime;
+        snocan = fmax90(0.0, snocan + (qflx_tran_veg - qflx_evap_veg) * dtime);
+    }
+    if (fabs(snocan) < 1.e-10 * fabs(snocan_baseline)) snocan = 0.0;
+    S.snocan[pp] = snocan;
+    S.liqcan[pp] = liqcan;
+
+    // ---- PhotosynthesisTotal and the noon water-use efficiency
+    const double fpsn = sun.psn * laisun + sha.psn * laisha;
+    S.fpsn[pp] = fpsn;
+    S.fpsn_wc[pp] = sun.psn_wc * laisun + sha.psn_wc * laisha;
+    S.fpsn_wj[pp] = sun.psn_wj * laisun + sha.psn_wj * laisha;
+    S.fpsn_wp[pp] = sun.psn_wp * laisun + sha.psn_wp * laisha;
+    double iwue = SPVAL;
+    if (noon && fpsn > 0.0) {
+        const double gs = 1.e-6 * (laisun * sun.gs_ph + laisha * sha.gs_ph);
+        iwue = (gs > 0.0) ? fpsn / gs : SPVAL;
+    }
+    S.iwue_ln[pp] = iwue;
+
+    // ---- remaining outputs
+    S.btran[pp] = btran;
+    S.dhsdt_canopy[pp] = dhsdt_canopy;
+    S.eflx_sh_stem[pp] = eflx_sh_stem;
+    S.eflx_sh_veg[pp] = eflx_sh_veg;
+    S.qflx_tran_veg[pp] = qflx_tran_veg;
+    S.qflx_evap_veg[pp] = qflx_evap_veg;
+    S.t_veg[pp] = t_veg;
+    S.t_stem[pp] = t_stem;
+    S.um[pp] = um;
+    S.obu[pp] = obu;
+    S.num_iter[pp] = num_iter;
+    S.ustar[pp] = ustar;
+    S.u10[pp] = fv.u10;
+    S.u10_clm[pp] = fv.u10_clm;
+    S.va[pp] = fv.va;
+    S.fv[pp] = fv.fv;
+    S.vds[pp] = fv.vds;
+    S.ram1[pp] = ram1;
+    S.rb1[pp] = rb;
+    S.rah1[pp] = rah1;
+    S.rah2[pp] = rah2;
+    S.raw1[pp] = raw1;
+    S.raw2[pp] = raw2;
+    S.uaf[pp] = uaf;
+    S.taf[pp] = taf;
+    S.qaf[pp] = qaf;
+    S.zeta[pp] = zeta;
+    S.vpd[pp] = vpd;
+    S.rh_af[pp] = rh_af;
+    S.dleaf_patch[pp] = dleaf;
+    S.canopy_err[pp] = errp;
+    // photosyns_inst
+    S.c3flag[pp] = K.c3 ? 1 : 0;
+    S.qe[pp] = K.qe;
+    if (sw.stomatal == CTSM_STOMATA_BB1987) {
+        S.bbb[pp] = K.bbb;
+        S.mbb[pp] = K.mbb;
+    }
+    S.kc[pp] = T.kc;
+    S.ko[pp] = T.ko;
+    S.cp[pp] = T.cp;
+    S.lnca[pp] = K.lnc;
+    S.vcmax_z[pp] = sh.vcmax_z;
+    S.tpu_z[pp] = sh.tpu_z;
+    S.kp_z[pp] = sh.kp_z;
+    S.gb_mol[pp] = sh.gb_mol;
+    S.ac[pp] = sh.ac;
+    S.aj[pp] = sh.aj;
+    S.ap[pp] = sh.ap;
+    S.ag[pp] = sh.ag;
+    S.an[pp] = sh.an;
+    S.gs_mol[pp] = sh.gs_mol;
+    S.rh_leaf[pp] = sh.rh_leaf;
+    S.vpd_can[pp] = sh.vpd_can;
+    S.rssun[pp] = sun.rs;
+    S.rssha[pp] = sha.rs;
+    S.rssun_z[pp] = sun.rs_z;
+    S.rssha_z[pp] = sha.rs_z;
+    S.cisun_z[pp] = sun.ci_z;
+    S.cisha_z[pp] = sha.ci_z;
+    S.lmrsun[pp] = sun.lmr;
+    S.lmrsha[pp] = sha.lmr;
+    S.lmrsun_z[pp] = sun.lmr_z;
+    S.lmrsha_z[pp] = sha.lmr_z;
+    S.psnsun[pp] = sun.psn;
+    S.psnsha[pp] = sha.psn;
+    S.psnsun_z[pp] = sun.psn_z;
+    S.psnsha_z[pp] = sha.psn_z;
+    S.psnsun_wc[pp] = sun.psn_wc;
+    S.psnsun_wj[pp] = sun.psn_wj;
+    S.psnsun_wp[pp] = sun.psn_wp;
+    S.psnsha_wc[pp] = sha.psn_wc;
+    S.psnsha_wj[pp] = sha.psn_wj;
+    S.psnsha_wp[pp] = sha.psn_wp;
+    S.gs_mol_sun[pp] = sun.gs_ph;
+    S.gs_mol_sha[pp] = sha.gs_ph;
+    S.gs_mol_sun_ln[pp] = sun.gs_ln;
+    S.gs_mol_sha_ln[pp] = sha.gs_ln;
+    if (bad) report_error(err, p, ENDRUN);
+}
+#undef PF
+
+// photosyns_inst%TimeStepInit (PhotosynthesisMod.F90:1143-1201) and
+// rb1(begp:endp) = 0 (CanopyFluxesMod.F90:830): every patch of the bounds.
+__global__ void __launch_bounds__(256) k_canopy_init(int ldp, ctsm_canopy_state S)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ldp) return;
+    S.psnsun[i] = 0.0; S.psnsun_wc[i] = 0.0; S.psnsun_wj[i] = 0.0; S.psnsun_wp[i] = 0.0;
+    S.psnsha[i] = 0.0; S.psnsha_wc[i] = 0.0; S.psnsha_wj[i] = 0.0; S.psnsha_wp[i] = 0.0;
+    S.fpsn[i] = 0.0; S.fpsn_wc[i] = 0.0; S.fpsn_wj[i] = 0.0; S.fpsn_wp[i] = 0.0;
+    S.rb1[i] = 0.0;
+}
+
+// cgrnd / cgrnds / cgrndl = 0 before CanopyFluxes (BiogeophysPreFluxCalcsMod.F90:390-392)
+__global__ void __launch_bounds__(256) k_zero_cgrnd(int n, const int32_t *__restrict__ filter, int begp,
+                                                    ctsm_canopy_state S)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long pp = (long)((filter ? filter[i] : begp + i) - begp);
+    S.cgrnd[pp] = 0.0;
+    S.cgrnds[pp] = 0.0;
+    S.cgrndl[pp] = 0.0;
+}
+
+// p2c_1d_filter of qflx_tran_veg (clm_driver.F90:1714-1716) and
+// Compute_EffecRootFrac_And_VertTranSink_Default (SoilWaterPlantSinkMod.F90:332-424)
+__global__ void __launch_bounds__(256) k_root_sink(CView V, ctsm_col_state col, ctsm_canopy_state S, int n,
+                                                   const int32_t *__restrict__ filter, ErrWord *err)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = filter ? filter[i] : V.begc + i;
+    const long cc = V.c1(c);
+    const int pi = col.patchi[cc], pf = col.patchf[cc];
+    if (pf >= pi && (pi < V.begp || pf >= V.begp + V.ldp)) {   // patches outside the allocation
+        report_error(err, c, CTSM_E_ARG);
+        return;
+    }
+    double tcol = 0.0;
+    for (int p = pi; p <= pf; ++p)
+        if (S.active[V.p1(p)]) tcol = tcol + S.qflx_tran_veg[V.p1(p)] * S.wtcol[V.p1(p)];
+    S.qflx_tran_veg_col[cc] = tcol;
+    double temp = 0.0;
+    for (int p = pi; p <= pf; ++p)
+        if (S.active[V.p1(p)]) temp = temp + S.qflx_tran_veg[V.p1(p)] * S.wtcol[V.p1(p)];
+    for (int j = 1; j <= V.nlevsoi; ++j) {
+        double r = 0.0;
+        for (int p = pi; p <= pf; ++p) {
+            const long q = V.p1(p);
+            if (S.active[q]) r = r + S.rootr[V.soip(p, j)] * S.qflx_tran_veg[q] * S.wtcol[q];
+        }
+        if (temp != 0.0) r = r / temp;
+        S.rootr_col[V.soi(c, j)] = r;
+        S.qflx_rootsoi_col[V.soi(c, j)] = r * tcol;
+    }
+}
+
+}  // namespace canopy
+
+void launch_canopy(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg, const ctsm_canopy_params &prm,
+                   const double *pft_dev, const ctsm_col_state &col, const ctsm_canopy_state &can, int n,
+                   const int32_t *filter, double dtime, ErrWord *err, int block, bool zero_cgrnd)
+{
+    using namespace canopy;
+    CView V{L.begc_alloc, L.ldc, L.begp_alloc, L.ldp, L.begg_alloc, L.ldg, cfg.nlevsno, cfg.nlevsoi, cfg.nlevgrnd};
+    CSwitch sw{cfg.z0param_method, cfg.stomatalcond_method, cfg.soil_resis_method, cfg.use_biomass_heat_storage,
+               cfg.use_undercanopy_stability, cfg.light_inhibit, cfg.itmax_canopy_fluxes};
+    if (L.ldp > 0)
+        hipLaunchKernelGGL(k_canopy_init, dim3((L.ldp + 255) / 256), dim3(256), 0, st, L.ldp, can);
+    if (n <= 0) return;
+    if (zero_cgrnd)
+        hipLaunchKernelGGL(k_zero_cgrnd, dim3((n + 255) / 256), dim3(256), 0, st, n, filter, L.begp_alloc, can);
+    (void)block;
+    hipLaunchKernelGGL(k_canopy, dim3((n + 63) / 64), dim3(64), 0, st, V, sw, prm, pft_dev, col, can, n, filter,
+                       dtime, err);
+}
+
+void launch_root_sink(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg, const ctsm_col_state &col,
+                      const ctsm_canopy_state &can, int n, const int32_t *filter, ErrWord *err)
+{
+    using namespace canopy;
+    if (n <= 0) return;
+    CView V{L.begc_alloc, L.ldc, L.begp_alloc, L.ldp, L.begg_alloc, L.ldg, cfg.nlevsno, cfg.nlevsoi, cfg.nlevgrnd};
+    hipLaunchKernelGGL(k_root_sink, dim3((n + 255) / 256), dim3(256), 0, st, V, col, can, n, filter, err);
+}
+
+}  // namespace ctsm

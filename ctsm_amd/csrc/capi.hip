@@ -33,13 +33,18 @@ size_t soiltemp_plain_workspace_doubles(int n);
 size_t soilwater_plain_workspace_doubles(int n);
 void launch_soilwater_plain(hipStream_t, const View &, const Params &, const ctsm_col_state &, int, const int32_t *,
                             double, double *, long, ErrWord *);
+void launch_canopy(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_canopy_params &, const double *,
+                   const ctsm_col_state &, const ctsm_canopy_state &, int, const int32_t *, double, ErrWord *, int,
+                   bool);
+void launch_root_sink(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_col_state &,
+                      const ctsm_canopy_state &, int, const int32_t *, ErrWord *);
 }  // namespace ctsm
 
 using namespace ctsm;
 
 namespace {
 
-enum Kind { K_C, K_SNO, K_ZI, K_SOI, K_SNOW, K_P, K_LYR };
+enum Kind { K_C, K_SNO, K_ZI, K_SOI, K_SNOW, K_P, K_LYR, K_G, K_SOIP };
 
 struct FieldMeta {
     size_t offset;   // offset of the pointer member in the state struct
@@ -82,11 +87,86 @@ const FieldMeta kPatchFields[] = {
     PF(sabg_chk, K_P, 8, true),
 };
 #undef PF
+// CanopyFluxes state, split by the index space each member lives in
+#define CN(name, kind, elem, wr) {offsetof(ctsm_canopy_state, name), kind, elem, wr}
+const FieldMeta kCanopyPatch[] = {
+    CN(itype, K_P, 4, false), CN(column, K_P, 4, false), CN(gridcell, K_P, 4, false),
+    CN(wtcol, K_P, 8, false), CN(frac_veg_nosno, K_P, 4, false), CN(active, K_P, 4, false),
+    CN(elai, K_P, 8, false), CN(esai, K_P, 8, false), CN(laisun, K_P, 8, false),
+    CN(laisha, K_P, 8, false), CN(htop, K_P, 8, false), CN(sabv, K_P, 8, false),
+    CN(parsun_z, K_P, 8, false), CN(parsha_z, K_P, 8, false), CN(laisun_z, K_P, 8, false),
+    CN(laisha_z, K_P, 8, false), CN(vcmaxcintsun, K_P, 8, false), CN(vcmaxcintsha, K_P, 8, false),
+    CN(nrad, K_P, 4, false), CN(thm, K_P, 8, false), CN(emv, K_P, 8, false),
+    CN(fwet, K_P, 8, false), CN(fdry, K_P, 8, false), CN(t_a10, K_P, 8, false),
+    CN(o3coefvsun, K_P, 8, false), CN(o3coefvsha, K_P, 8, false), CN(o3coefgsun, K_P, 8, false),
+    CN(o3coefgsha, K_P, 8, false), CN(rootfr, K_SOIP, 8, false), CN(displa, K_P, 8, true),
+    CN(z0mv, K_P, 8, true), CN(t_veg, K_P, 8, true), CN(t_stem, K_P, 8, true),
+    CN(snocan, K_P, 8, true), CN(liqcan, K_P, 8, true), CN(cgrnds, K_P, 8, true),
+    CN(cgrndl, K_P, 8, true), CN(z0hv, K_P, 8, true), CN(z0qv, K_P, 8, true),
+    CN(forc_hgt_u_patch, K_P, 8, true), CN(forc_hgt_t_patch, K_P, 8, true), CN(forc_hgt_q_patch, K_P, 8, true),
+    CN(ram1, K_P, 8, true), CN(rb1, K_P, 8, true), CN(rah1, K_P, 8, true),
+    CN(rah2, K_P, 8, true), CN(raw1, K_P, 8, true), CN(raw2, K_P, 8, true),
+    CN(ustar, K_P, 8, true), CN(um, K_P, 8, true), CN(uaf, K_P, 8, true),
+    CN(taf, K_P, 8, true), CN(qaf, K_P, 8, true), CN(obu, K_P, 8, true),
+    CN(zeta, K_P, 8, true), CN(vpd, K_P, 8, true), CN(num_iter, K_P, 8, true),
+    CN(u10, K_P, 8, true), CN(u10_clm, K_P, 8, true), CN(va, K_P, 8, true),
+    CN(fv, K_P, 8, true), CN(vds, K_P, 8, true), CN(btran, K_P, 8, true),
+    CN(rresis, K_SOIP, 8, true), CN(rootr, K_SOIP, 8, true), CN(eflx_sh_veg, K_P, 8, true),
+    CN(eflx_sh_stem, K_P, 8, true), CN(eflx_sh_grnd, K_P, 8, true), CN(eflx_sh_snow, K_P, 8, true),
+    CN(eflx_sh_soil, K_P, 8, true), CN(eflx_sh_h2osfc, K_P, 8, true), CN(taux, K_P, 8, true),
+    CN(tauy, K_P, 8, true), CN(dlrad, K_P, 8, true), CN(ulrad, K_P, 8, true),
+    CN(cgrnd, K_P, 8, true), CN(dhsdt_canopy, K_P, 8, true), CN(qflx_evap_soi, K_P, 8, true),
+    CN(qflx_ev_snow, K_P, 8, true), CN(qflx_ev_soil, K_P, 8, true), CN(qflx_ev_h2osfc, K_P, 8, true),
+    CN(qflx_tran_veg, K_P, 8, true), CN(qflx_evap_veg, K_P, 8, true), CN(rh_af, K_P, 8, true),
+    CN(q_ref2m, K_P, 8, true), CN(rh_ref2m, K_P, 8, true), CN(rh_ref2m_r, K_P, 8, true),
+    CN(vpd_ref2m, K_P, 8, true), CN(iwue_ln, K_P, 8, true), CN(t_ref2m, K_P, 8, true),
+    CN(t_ref2m_r, K_P, 8, true), CN(t_skin, K_P, 8, true), CN(stem_biomass, K_P, 8, true),
+    CN(leaf_biomass, K_P, 8, true), CN(dleaf_patch, K_P, 8, true), CN(rssun, K_P, 8, true),
+    CN(rssha, K_P, 8, true), CN(psnsun, K_P, 8, true), CN(psnsha, K_P, 8, true),
+    CN(psnsun_wc, K_P, 8, true), CN(psnsun_wj, K_P, 8, true), CN(psnsun_wp, K_P, 8, true),
+    CN(psnsha_wc, K_P, 8, true), CN(psnsha_wj, K_P, 8, true), CN(psnsha_wp, K_P, 8, true),
+    CN(fpsn, K_P, 8, true), CN(fpsn_wc, K_P, 8, true), CN(fpsn_wj, K_P, 8, true),
+    CN(fpsn_wp, K_P, 8, true), CN(lmrsun, K_P, 8, true), CN(lmrsha, K_P, 8, true),
+    CN(gs_mol_sun, K_P, 8, true), CN(gs_mol_sha, K_P, 8, true), CN(gs_mol_sun_ln, K_P, 8, true),
+    CN(gs_mol_sha_ln, K_P, 8, true), CN(cisun_z, K_P, 8, true), CN(cisha_z, K_P, 8, true),
+    CN(rssun_z, K_P, 8, true), CN(rssha_z, K_P, 8, true), CN(lmrsun_z, K_P, 8, true),
+    CN(lmrsha_z, K_P, 8, true), CN(psnsun_z, K_P, 8, true), CN(psnsha_z, K_P, 8, true),
+    CN(vcmax_z, K_P, 8, true), CN(tpu_z, K_P, 8, true), CN(kp_z, K_P, 8, true),
+    CN(ac, K_P, 8, true), CN(aj, K_P, 8, true), CN(ap, K_P, 8, true),
+    CN(ag, K_P, 8, true), CN(an, K_P, 8, true), CN(gb_mol, K_P, 8, true),
+    CN(gs_mol, K_P, 8, true), CN(cp, K_P, 8, true), CN(kc, K_P, 8, true),
+    CN(ko, K_P, 8, true), CN(qe, K_P, 8, true), CN(bbb, K_P, 8, true),
+    CN(mbb, K_P, 8, true), CN(rh_leaf, K_P, 8, true), CN(vpd_can, K_P, 8, true),
+    CN(lnca, K_P, 8, true), CN(c3flag, K_P, 4, true), CN(canopy_err, K_P, 8, true),
+};
+const FieldMeta kCanopyCol[] = {
+    CN(forc_q, K_C, 8, false), CN(forc_pbot, K_C, 8, false), CN(forc_th, K_C, 8, false),
+    CN(forc_rho, K_C, 8, false), CN(thv, K_C, 8, false), CN(qg, K_C, 8, false),
+    CN(qg_snow, K_C, 8, false), CN(qg_soil, K_C, 8, false), CN(qg_h2osfc, K_C, 8, false),
+    CN(dqgdT, K_C, 8, false), CN(z0mg, K_C, 8, false), CN(soilbeta, K_C, 8, false),
+    CN(soilresis, K_C, 8, false), CN(h2osoi_liqvol, K_SOI, 8, true), CN(qflx_tran_veg_col, K_C, 8, true),
+    CN(rootr_col, K_SOI, 8, true), CN(qflx_rootsoi_col, K_SOI, 8, true),
+};
+const FieldMeta kCanopyGrid[] = {
+    CN(forc_u, K_G, 8, false), CN(forc_v, K_G, 8, false), CN(forc_pco2, K_G, 8, false),
+    CN(forc_po2, K_G, 8, false), CN(forc_hgt_u, K_G, 8, false), CN(forc_hgt_t, K_G, 8, false),
+    CN(forc_hgt_q, K_G, 8, false), CN(dayl, K_G, 8, false), CN(max_dayl, K_G, 8, false),
+    CN(near_local_noon, K_G, 4, false),
+};
+#undef CN
+constexpr int kNCanP = sizeof(kCanopyPatch) / sizeof(kCanopyPatch[0]);
+constexpr int kNCanC = sizeof(kCanopyCol) / sizeof(kCanopyCol[0]);
+constexpr int kNCanG = sizeof(kCanopyGrid) / sizeof(kCanopyGrid[0]);
+static_assert((kNCanP + kNCanC + kNCanG) * sizeof(void *) == sizeof(ctsm_canopy_state),
+              "every ctsm_canopy_state member has a mirror entry");
+static_assert(sizeof(kColFields) / sizeof(kColFields[0]) * sizeof(void *) == sizeof(ctsm_col_state),
+              "every ctsm_col_state member has a mirror entry");
 
 int nlev_of(Kind k, const ctsm_config &cfg)
 {
     switch (k) {
-    case K_C: case K_P: return 1;
+    case K_C: case K_P: case K_G: return 1;
+    case K_SOIP: return cfg.nlevgrnd;
     case K_SNO: return cfg.nlevsno + cfg.nlevgrnd;
     case K_ZI: return cfg.nlevsno + cfg.nlevgrnd + 1;
     case K_SOI: return cfg.nlevgrnd;
@@ -150,6 +230,10 @@ struct ctsm_ctx {
     DevBuf work;                       // heat-solve workspace
     DevBuf filt;                       // filter (drop-in mode)
     std::vector<DevBuf> colmir, pchmir; // drop-in mirrors
+    std::vector<DevBuf> canp, canc, cang; // canopy drop-in mirrors
+    ctsm_canopy_params cprm{};         // canopy param-file scalars
+    DevBuf pft;                        // PFT table (N_PFT_FIELDS x CTSM_MAX_PFT doubles)
+    bool have_canopy = false;
     int32_t last_code = 0, last_index = 0;
     std::mutex mu;
 };
@@ -218,7 +302,7 @@ int sync_and_check(ctsm_ctx *ctx)
     if (h.code != 0) {
         ctx->last_code = h.code;
         ctx->last_index = h.first_col;
-        return CTSM_E_SOLVER;
+        return h.code;
     }
     ctx->last_code = 0;
     ctx->last_index = 0;
@@ -365,6 +449,12 @@ void ctsm_config_default(ctsm_config *cfg)
     cfg->xTolerUpper = 1.e-1;
     cfg->xTolerLower = 1.e-2;
     cfg->e_ice = 6.0;
+    cfg->z0param_method = CTSM_Z0_MEIER2022;
+    cfg->stomatalcond_method = CTSM_STOMATA_MEDLYN2011;
+    cfg->soil_resis_method = CTSM_SOILRESIS_SL14;
+    cfg->use_biomass_heat_storage = 1;
+    cfg->use_undercanopy_stability = 0;
+    cfg->light_inhibit = 1;
 }
 
 int ctsm_create(const ctsm_config *cfg, int device, ctsm_ctx **out)
@@ -378,6 +468,12 @@ int ctsm_create(const ctsm_config *cfg, int device, ctsm_ctx **out)
          cfg->snow_thermal_cond_method != CTSM_SNOWTHK_STURM1997) ||
         cfg->nlevsno < 1 || cfg->nlevsno > MAX_NLEVSNO || cfg->nlevsoi < 2 || cfg->nlevsoi > MAX_NLEVSOI ||
         cfg->nlevgrnd < cfg->nlevsoi || cfg->nlevgrnd > MAX_NLEVGRND || cfg->nlevsno + 1 + cfg->nlevgrnd > MAX_ROWS)
+        return CTSM_E_CONFIG;
+    if ((cfg->z0param_method != CTSM_Z0_ZENGWANG2007 && cfg->z0param_method != CTSM_Z0_MEIER2022) ||
+        (cfg->stomatalcond_method != CTSM_STOMATA_BB1987 && cfg->stomatalcond_method != CTSM_STOMATA_MEDLYN2011) ||
+        (cfg->soil_resis_method != CTSM_SOILRESIS_LEEPIELKE1992 && cfg->soil_resis_method != CTSM_SOILRESIS_SL14) ||
+        (unsigned)cfg->use_biomass_heat_storage > 1u || (unsigned)cfg->use_undercanopy_stability > 1u ||
+        (unsigned)cfg->light_inhibit > 1u || cfg->itmax_canopy_fluxes < 0 || cfg->itmax_canopy_fluxes > 1000)
         return CTSM_E_CONFIG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return CTSM_E_NODEVICE;
@@ -528,6 +624,169 @@ int ctsm_soil_step_fused_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_c
                          ctx->tun.richards_block);
     }
     return finish_launch(ctx);
+}
+
+// ---- canopy ----------------------------------------------------------------
+
+int ctsm_set_canopy_params(ctsm_ctx *ctx, const ctsm_canopy_params *prm, const ctsm_pft_params *pft)
+{
+    if (!ctx || !prm || !pft || pft->npft < 1 || pft->npft > CTSM_MAX_PFT) return CTSM_E_ARG;
+    const double *cols[] = {pft->dleaf, pft->c3psn, pft->slatop, pft->leafcn, pft->flnr, pft->fnitr, pft->mbbopt,
+                            pft->medlynslope, pft->medlynintercept, pft->smpso, pft->smpsc, pft->theta_cj,
+                            pft->z0v_Cr, pft->z0v_Cs, pft->z0v_c, pft->z0v_cw, pft->z0v_LAImax};
+    const int ncols = (int)(sizeof(cols) / sizeof(cols[0]));
+    const double *cols2[] = {pft->dbh, pft->fbw, pft->nstem, pft->rstem_per_dbh, pft->wood_density};
+    for (const double *a : cols) if (!a) return CTSM_E_ARG;
+    for (const double *a : cols2) if (!a) return CTSM_E_ARG;
+    if (!pft->is_tree || !pft->is_shrub) return CTSM_E_ARG;
+    // table rows in the order of the kernel's field enum
+    std::vector<double> tab((size_t)24 * CTSM_MAX_PFT, 0.0);
+    auto put = [&](int field, const double *a) {
+        for (int i = 0; i < pft->npft; ++i) tab[(size_t)field * CTSM_MAX_PFT + i] = a[i];
+    };
+    for (int f = 0; f < ncols; ++f) put(f, cols[f]);
+    for (int i = 0; i < pft->npft; ++i) {
+        tab[(size_t)17 * CTSM_MAX_PFT + i] = (double)pft->is_tree[i];
+        tab[(size_t)18 * CTSM_MAX_PFT + i] = (double)pft->is_shrub[i];
+    }
+    for (int f = 0; f < 5; ++f) put(19 + f, cols2[f]);
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e);
+    if ((e = ctx->pft.ensure(tab.size() * sizeof(double))) != hipSuccess) return hip_fail(ctx, e);
+    if ((e = hipMemcpyAsync(ctx->pft.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice,
+                            ctx->stream)) != hipSuccess)
+        return hip_fail(ctx, e);
+    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return hip_fail(ctx, e);
+    ctx->cprm = *prm;
+    ctx->have_canopy = true;
+    return CTSM_OK;
+}
+
+static int check_canopy_layout(const ctsm_ctx *ctx, const ctsm_layout *L)
+{
+    if (!ctx->have_canopy) return CTSM_E_ARG;
+    if (check_layout(L) || L->ldp <= 0 || L->ldg <= 0) return CTSM_E_ARG;
+    return CTSM_OK;
+}
+
+int ctsm_canopyfluxes_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                          const ctsm_canopy_state *can, int32_t n, const int32_t *filterp_dev, double dtime)
+{
+    if (!ctx || !col || !can || n < 0 || check_canopy_layout(ctx, lay) || (!filterp_dev && n > lay->ldp))
+        return CTSM_E_ARG;
+    launch_canopy(ctx->stream, *lay, ctx->cfg, ctx->cprm, (const double *)ctx->pft.p, *col, *can, n, filterp_dev,
+                  dtime, ctx->err, 64, false);
+    return finish_launch(ctx);
+}
+
+int ctsm_root_sink_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                       const ctsm_canopy_state *can, int32_t n, const int32_t *filterc_dev)
+{
+    if (!ctx || !col || !can || n < 0 || check_layout(lay) || (!filterc_dev && n > lay->ldc)) return CTSM_E_ARG;
+    launch_root_sink(ctx->stream, *lay, ctx->cfg, *col, *can, n, filterc_dev, ctx->err);
+    return finish_launch(ctx);
+}
+
+int ctsm_land_step_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                       const ctsm_patch_state *pch, const ctsm_canopy_state *can, int32_t nc,
+                       const int32_t *filterc_dev, int32_t np, const int32_t *filterp_dev, double dtime)
+{
+    if (!ctx || !col || !pch || !can || nc < 0 || np < 0 || check_canopy_layout(ctx, lay) ||
+        (!filterc_dev && nc > lay->ldc) || (!filterp_dev && np > lay->ldp))
+        return CTSM_E_ARG;
+    // the arrays CanopyFluxes writes and the soil solves read must coincide
+    if (pch->dlrad != can->dlrad || pch->cgrnd != can->cgrnd || pch->eflx_sh_grnd != can->eflx_sh_grnd ||
+        pch->eflx_sh_snow != can->eflx_sh_snow || pch->eflx_sh_soil != can->eflx_sh_soil ||
+        pch->eflx_sh_h2osfc != can->eflx_sh_h2osfc || pch->qflx_evap_soi != can->qflx_evap_soi ||
+        pch->qflx_ev_snow != can->qflx_ev_snow || pch->qflx_ev_soil != can->qflx_ev_soil ||
+        pch->qflx_ev_h2osfc != can->qflx_ev_h2osfc || col->qflx_rootsoi != can->qflx_rootsoi_col)
+        return CTSM_E_ARG;
+    launch_canopy(ctx->stream, *lay, ctx->cfg, ctx->cprm, (const double *)ctx->pft.p, *col, *can, np, filterp_dev,
+                  dtime, ctx->err, 64, true);
+    launch_root_sink(ctx->stream, *lay, ctx->cfg, *col, *can, nc, filterc_dev, ctx->err);
+    int rc = finish_launch(ctx);
+    if (rc) return rc;
+    return ctsm_soil_step_fused_dev(ctx, lay, col, pch, nc, filterc_dev, dtime);
+}
+
+static int canopy_mirror(ctsm_ctx *ctx, const ctsm_layout *L, const ctsm_canopy_state *can, ctsm_canopy_state *dcan,
+                         bool to_device)
+{
+    int rc = mirror(ctx, kCanopyPatch, kNCanP, can, dcan, ctx->canp, L->begp_alloc, L->ldp,
+                    Range{L->begp_alloc, L->begp_alloc + L->ldp - 1}, to_device);
+    if (rc) return rc;
+    rc = mirror(ctx, kCanopyCol, kNCanC, can, dcan, ctx->canc, L->begc_alloc, L->ldc,
+                Range{L->begc_alloc, L->begc_alloc + L->ldc - 1}, to_device);
+    if (rc) return rc;
+    return mirror(ctx, kCanopyGrid, kNCanG, can, dcan, ctx->cang, L->begg_alloc, L->ldg,
+                  Range{L->begg_alloc, L->begg_alloc + L->ldg - 1}, to_device);
+}
+
+static int upload_filter(ctsm_ctx *ctx, int32_t n, const int32_t *filter, const int32_t **dfilt)
+{
+    *dfilt = nullptr;
+    if (!filter || n == 0) return CTSM_OK;
+    hipError_t he = ctx->filt.ensure((size_t)n * sizeof(int32_t));
+    if (he != hipSuccess) return hip_fail(ctx, he);
+    he = hipMemcpyAsync(ctx->filt.p, filter, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream);
+    if (he != hipSuccess) return hip_fail(ctx, he);
+    *dfilt = (const int32_t *)ctx->filt.p;
+    return CTSM_OK;
+}
+
+int ctsm_canopyfluxes(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                      const ctsm_canopy_state *can, int32_t n, const int32_t *filterp, double dtime)
+{
+    if (!ctx || !col || !can || n < 0 || check_canopy_layout(ctx, lay) || (!filterp && n > lay->ldp))
+        return CTSM_E_ARG;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipError_t he = hipSetDevice(ctx->device);
+    if (he != hipSuccess) return hip_fail(ctx, he);
+    for (int i = 0; filterp && i < n; ++i)
+        if (filterp[i] < lay->begp_alloc || filterp[i] >= lay->begp_alloc + lay->ldp) return CTSM_E_ARG;
+    const Range cr{lay->begc_alloc, lay->begc_alloc + lay->ldc - 1};
+    ctsm_col_state dcol{};
+    ctsm_canopy_state dcan{};
+    int rc = mirror(ctx, kColFields, (int)(sizeof(kColFields) / sizeof(kColFields[0])), col, &dcol, ctx->colmir,
+                    lay->begc_alloc, lay->ldc, cr, true);
+    if (rc) return rc;
+    if ((rc = canopy_mirror(ctx, lay, can, &dcan, true))) return rc;
+    const int32_t *dfilt;
+    if ((rc = upload_filter(ctx, n, filterp, &dfilt))) return rc;
+    launch_canopy(ctx->stream, *lay, ctx->cfg, ctx->cprm, (const double *)ctx->pft.p, dcol, dcan, n, dfilt, dtime,
+                  ctx->err, 64, false);
+    if ((rc = finish_launch(ctx))) return rc;
+    rc = mirror(ctx, kColFields, (int)(sizeof(kColFields) / sizeof(kColFields[0])), col, &dcol, ctx->colmir,
+                lay->begc_alloc, lay->ldc, cr, false);
+    if (rc) return rc;
+    if ((rc = canopy_mirror(ctx, lay, can, &dcan, false))) return rc;
+    return sync_and_check(ctx);
+}
+
+int ctsm_root_sink(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col, const ctsm_canopy_state *can,
+                   int32_t n, const int32_t *filterc)
+{
+    if (!ctx || !col || !can || n < 0 || check_canopy_layout(ctx, lay) || (!filterc && n > lay->ldc))
+        return CTSM_E_ARG;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipError_t he = hipSetDevice(ctx->device);
+    if (he != hipSuccess) return hip_fail(ctx, he);
+    for (int i = 0; filterc && i < n; ++i)
+        if (filterc[i] < lay->begc_alloc || filterc[i] >= lay->begc_alloc + lay->ldc) return CTSM_E_ARG;
+    const Range cr{lay->begc_alloc, lay->begc_alloc + lay->ldc - 1};
+    ctsm_col_state dcol{};
+    ctsm_canopy_state dcan{};
+    int rc = mirror(ctx, kColFields, (int)(sizeof(kColFields) / sizeof(kColFields[0])), col, &dcol, ctx->colmir,
+                    lay->begc_alloc, lay->ldc, cr, true);
+    if (rc) return rc;
+    if ((rc = canopy_mirror(ctx, lay, can, &dcan, true))) return rc;
+    const int32_t *dfilt;
+    if ((rc = upload_filter(ctx, n, filterc, &dfilt))) return rc;
+    launch_root_sink(ctx->stream, *lay, ctx->cfg, dcol, dcan, n, dfilt, ctx->err);
+    if ((rc = finish_launch(ctx))) return rc;
+    if ((rc = canopy_mirror(ctx, lay, can, &dcan, false))) return rc;
+    return sync_and_check(ctx);
 }
 
 int ctsm_dev_alloc(ctsm_ctx *ctx, uint64_t bytes, void **out)

@@ -18,7 +18,10 @@ module ctsm_gpu_mod
   private
 
   integer(c_int), parameter, public :: CTSM_OK = 0, CTSM_E_ARG = 1, CTSM_E_CONFIG = 2, &
-       CTSM_E_HIP = 3, CTSM_E_SOLVER = 4, CTSM_E_NODEVICE = 5
+       CTSM_E_HIP = 3, CTSM_E_SOLVER = 4, CTSM_E_NODEVICE = 5, CTSM_E_ENDRUN = 6
+  integer(c_int), parameter, public :: CTSM_Z0_ZENGWANG2007 = 0, CTSM_Z0_MEIER2022 = 1
+  integer(c_int), parameter, public :: CTSM_STOMATA_BB1987 = 1, CTSM_STOMATA_MEDLYN2011 = 2
+  integer(c_int), parameter, public :: CTSM_SOILRESIS_LEEPIELKE1992 = 0, CTSM_SOILRESIS_SL14 = 1
   integer(c_int), parameter, public :: CTSM_BC_FLUX = 1, CTSM_BC_ZERO_FLUX = 2
   integer(c_int), parameter, public :: CTSM_SNOWTHK_JORDAN1991 = 0, CTSM_SNOWTHK_STURM1997 = 1
 
@@ -29,10 +32,13 @@ module ctsm_gpu_mod
      integer(c_int32_t) :: flux_calculation, snow_thermal_cond_method
      integer(c_int32_t) :: itmax_canopy_fluxes, pad0
      real(c_double)     :: dtmin, verySmall, xTolerUpper, xTolerLower, e_ice
+     integer(c_int32_t) :: z0param_method, stomatalcond_method, soil_resis_method
+     integer(c_int32_t) :: use_biomass_heat_storage, use_undercanopy_stability, light_inhibit
   end type ctsm_config_t
 
   type, bind(C), public :: ctsm_layout_t
      integer(c_int32_t) :: begc_alloc, ldc, begp_alloc, ldp
+     integer(c_int32_t) :: begg_alloc = 1, ldg = 0
   end type ctsm_layout_t
 
   type, bind(C), public :: ctsm_col_state_t
@@ -68,8 +74,76 @@ module ctsm_gpu_mod
      type(c_ptr) :: sabg_chk = c_null_ptr
   end type ctsm_patch_state_t
 
+  ! param-file scalars of CanopyFluxes / Photosynthesis / FrictionVelocity
+  type, bind(C), public :: ctsm_canopy_params_t
+     real(c_double) :: cv, a_coef, a_exp, csoilc, z_dl, lai_dl
+     real(c_double) :: wind_min, zetamaxstable, kc25_coef, ko25_coef, cp25_yr2000, kcha
+     real(c_double) :: koha, cpha, vcmaxha, jmaxha, tpuha, lmrha
+     real(c_double) :: vcmaxhd, jmaxhd, tpuhd, lmrhd, lmrse, fnr
+     real(c_double) :: act25, theta_psii, theta_ip, fnps, tpu25ratio, kp25ratio
+     real(c_double) :: vcmaxse_sf, jmaxse_sf, tpuse_sf, jmax25top_sf, leaf_mr_vcm
+  end type ctsm_canopy_params_t
+
+  ! pftcon arrays (c_loc(pftcon%dleaf) etc., 0:mxpft)
+  type, bind(C), public :: ctsm_pft_params_t
+     integer(c_int32_t) :: npft = 0, pad = 0
+     type(c_ptr) :: dleaf = c_null_ptr, c3psn = c_null_ptr, slatop = c_null_ptr, leafcn = c_null_ptr, flnr = c_null_ptr
+     type(c_ptr) :: fnitr = c_null_ptr, mbbopt = c_null_ptr, medlynslope = c_null_ptr, medlynintercept = c_null_ptr, smpso = c_null_ptr
+     type(c_ptr) :: smpsc = c_null_ptr, theta_cj = c_null_ptr, z0v_Cr = c_null_ptr, z0v_Cs = c_null_ptr, z0v_c = c_null_ptr
+     type(c_ptr) :: z0v_cw = c_null_ptr, z0v_LAImax = c_null_ptr, is_tree = c_null_ptr, is_shrub = c_null_ptr, dbh = c_null_ptr
+     type(c_ptr) :: fbw = c_null_ptr, nstem = c_null_ptr, rstem_per_dbh = c_null_ptr, wood_density = c_null_ptr
+  end type ctsm_pft_params_t
+
+  ! CanopyFluxes state (member order and meaning as ctsm_canopy_state)
+  type, bind(C), public :: ctsm_canopy_state_t
+     type(c_ptr) :: itype = c_null_ptr, column = c_null_ptr, gridcell = c_null_ptr, wtcol = c_null_ptr
+     type(c_ptr) :: frac_veg_nosno = c_null_ptr, active = c_null_ptr, forc_u = c_null_ptr, forc_v = c_null_ptr
+     type(c_ptr) :: forc_pco2 = c_null_ptr, forc_po2 = c_null_ptr, forc_hgt_u = c_null_ptr, forc_hgt_t = c_null_ptr
+     type(c_ptr) :: forc_hgt_q = c_null_ptr, dayl = c_null_ptr, max_dayl = c_null_ptr, near_local_noon = c_null_ptr
+     type(c_ptr) :: forc_q = c_null_ptr, forc_pbot = c_null_ptr, forc_th = c_null_ptr, forc_rho = c_null_ptr
+     type(c_ptr) :: thv = c_null_ptr, qg = c_null_ptr, qg_snow = c_null_ptr, qg_soil = c_null_ptr
+     type(c_ptr) :: qg_h2osfc = c_null_ptr, dqgdT = c_null_ptr, z0mg = c_null_ptr, soilbeta = c_null_ptr
+     type(c_ptr) :: soilresis = c_null_ptr, h2osoi_liqvol = c_null_ptr, qflx_tran_veg_col = c_null_ptr, rootr_col = c_null_ptr
+     type(c_ptr) :: qflx_rootsoi_col = c_null_ptr, elai = c_null_ptr, esai = c_null_ptr, laisun = c_null_ptr
+     type(c_ptr) :: laisha = c_null_ptr, htop = c_null_ptr, sabv = c_null_ptr, parsun_z = c_null_ptr
+     type(c_ptr) :: parsha_z = c_null_ptr, laisun_z = c_null_ptr, laisha_z = c_null_ptr, vcmaxcintsun = c_null_ptr
+     type(c_ptr) :: vcmaxcintsha = c_null_ptr, nrad = c_null_ptr, thm = c_null_ptr, emv = c_null_ptr
+     type(c_ptr) :: fwet = c_null_ptr, fdry = c_null_ptr, t_a10 = c_null_ptr, o3coefvsun = c_null_ptr
+     type(c_ptr) :: o3coefvsha = c_null_ptr, o3coefgsun = c_null_ptr, o3coefgsha = c_null_ptr, rootfr = c_null_ptr
+     type(c_ptr) :: displa = c_null_ptr, z0mv = c_null_ptr, t_veg = c_null_ptr, t_stem = c_null_ptr
+     type(c_ptr) :: snocan = c_null_ptr, liqcan = c_null_ptr, cgrnds = c_null_ptr, cgrndl = c_null_ptr
+     type(c_ptr) :: z0hv = c_null_ptr, z0qv = c_null_ptr, forc_hgt_u_patch = c_null_ptr, forc_hgt_t_patch = c_null_ptr
+     type(c_ptr) :: forc_hgt_q_patch = c_null_ptr, ram1 = c_null_ptr, rb1 = c_null_ptr, rah1 = c_null_ptr
+     type(c_ptr) :: rah2 = c_null_ptr, raw1 = c_null_ptr, raw2 = c_null_ptr, ustar = c_null_ptr
+     type(c_ptr) :: um = c_null_ptr, uaf = c_null_ptr, taf = c_null_ptr, qaf = c_null_ptr
+     type(c_ptr) :: obu = c_null_ptr, zeta = c_null_ptr, vpd = c_null_ptr, num_iter = c_null_ptr
+     type(c_ptr) :: u10 = c_null_ptr, u10_clm = c_null_ptr, va = c_null_ptr, fv = c_null_ptr
+     type(c_ptr) :: vds = c_null_ptr, btran = c_null_ptr, rresis = c_null_ptr, rootr = c_null_ptr
+     type(c_ptr) :: eflx_sh_veg = c_null_ptr, eflx_sh_stem = c_null_ptr, eflx_sh_grnd = c_null_ptr, eflx_sh_snow = c_null_ptr
+     type(c_ptr) :: eflx_sh_soil = c_null_ptr, eflx_sh_h2osfc = c_null_ptr, taux = c_null_ptr, tauy = c_null_ptr
+     type(c_ptr) :: dlrad = c_null_ptr, ulrad = c_null_ptr, cgrnd = c_null_ptr, dhsdt_canopy = c_null_ptr
+     type(c_ptr) :: qflx_evap_soi = c_null_ptr, qflx_ev_snow = c_null_ptr, qflx_ev_soil = c_null_ptr, qflx_ev_h2osfc = c_null_ptr
+     type(c_ptr) :: qflx_tran_veg = c_null_ptr, qflx_evap_veg = c_null_ptr, rh_af = c_null_ptr, q_ref2m = c_null_ptr
+     type(c_ptr) :: rh_ref2m = c_null_ptr, rh_ref2m_r = c_null_ptr, vpd_ref2m = c_null_ptr, iwue_ln = c_null_ptr
+     type(c_ptr) :: t_ref2m = c_null_ptr, t_ref2m_r = c_null_ptr, t_skin = c_null_ptr, stem_biomass = c_null_ptr
+     type(c_ptr) :: leaf_biomass = c_null_ptr, dleaf_patch = c_null_ptr, rssun = c_null_ptr, rssha = c_null_ptr
+     type(c_ptr) :: psnsun = c_null_ptr, psnsha = c_null_ptr, psnsun_wc = c_null_ptr, psnsun_wj = c_null_ptr
+     type(c_ptr) :: psnsun_wp = c_null_ptr, psnsha_wc = c_null_ptr, psnsha_wj = c_null_ptr, psnsha_wp = c_null_ptr
+     type(c_ptr) :: fpsn = c_null_ptr, fpsn_wc = c_null_ptr, fpsn_wj = c_null_ptr, fpsn_wp = c_null_ptr
+     type(c_ptr) :: lmrsun = c_null_ptr, lmrsha = c_null_ptr, gs_mol_sun = c_null_ptr, gs_mol_sha = c_null_ptr
+     type(c_ptr) :: gs_mol_sun_ln = c_null_ptr, gs_mol_sha_ln = c_null_ptr, cisun_z = c_null_ptr, cisha_z = c_null_ptr
+     type(c_ptr) :: rssun_z = c_null_ptr, rssha_z = c_null_ptr, lmrsun_z = c_null_ptr, lmrsha_z = c_null_ptr
+     type(c_ptr) :: psnsun_z = c_null_ptr, psnsha_z = c_null_ptr, vcmax_z = c_null_ptr, tpu_z = c_null_ptr
+     type(c_ptr) :: kp_z = c_null_ptr, ac = c_null_ptr, aj = c_null_ptr, ap = c_null_ptr
+     type(c_ptr) :: ag = c_null_ptr, an = c_null_ptr, gb_mol = c_null_ptr, gs_mol = c_null_ptr
+     type(c_ptr) :: cp = c_null_ptr, kc = c_null_ptr, ko = c_null_ptr, qe = c_null_ptr
+     type(c_ptr) :: bbb = c_null_ptr, mbb = c_null_ptr, rh_leaf = c_null_ptr, vpd_can = c_null_ptr
+     type(c_ptr) :: lnca = c_null_ptr, c3flag = c_null_ptr, canopy_err = c_null_ptr
+  end type ctsm_canopy_state_t
+
   public :: ctsm_config_default, ctsm_create, ctsm_destroy, ctsm_last_error
   public :: ctsm_soilwater, ctsm_soiltemperature, ctsm_soilwater_fractions
+  public :: ctsm_set_canopy_params, ctsm_canopyfluxes, ctsm_root_sink
 
   interface
      subroutine ctsm_config_default(cfg) bind(C, name='ctsm_config_default')
@@ -127,6 +201,35 @@ module ctsm_gpu_mod
        integer(c_int32_t), value :: num_hydrologyc
        integer(c_int32_t), intent(in) :: filter_hydrologyc(*)
      end function ctsm_soilwater_fractions
+
+     integer(c_int) function ctsm_set_canopy_params(ctx, prm, pft) bind(C, name='ctsm_set_canopy_params')
+       import :: c_int, c_ptr, ctsm_canopy_params_t, ctsm_pft_params_t
+       type(c_ptr), value :: ctx
+       type(ctsm_canopy_params_t), intent(in) :: prm
+       type(ctsm_pft_params_t), intent(in) :: pft
+     end function ctsm_set_canopy_params
+
+     integer(c_int) function ctsm_canopyfluxes(ctx, lay, col, can, num_exposedvegp, filter_exposedvegp, dtime) &
+          bind(C, name='ctsm_canopyfluxes')
+       import :: c_int, c_int32_t, c_double, c_ptr, ctsm_layout_t, ctsm_col_state_t, ctsm_canopy_state_t
+       type(c_ptr), value :: ctx
+       type(ctsm_layout_t), intent(in) :: lay
+       type(ctsm_col_state_t), intent(in) :: col
+       type(ctsm_canopy_state_t), intent(in) :: can
+       integer(c_int32_t), value :: num_exposedvegp
+       integer(c_int32_t), intent(in) :: filter_exposedvegp(*)
+       real(c_double), value :: dtime
+     end function ctsm_canopyfluxes
+
+     integer(c_int) function ctsm_root_sink(ctx, lay, col, can, num_filterc, filterc) bind(C, name='ctsm_root_sink')
+       import :: c_int, c_int32_t, c_ptr, ctsm_layout_t, ctsm_col_state_t, ctsm_canopy_state_t
+       type(c_ptr), value :: ctx
+       type(ctsm_layout_t), intent(in) :: lay
+       type(ctsm_col_state_t), intent(in) :: col
+       type(ctsm_canopy_state_t), intent(in) :: can
+       integer(c_int32_t), value :: num_filterc
+       integer(c_int32_t), intent(in) :: filterc(*)
+     end function ctsm_root_sink
   end interface
 
 end module ctsm_gpu_mod

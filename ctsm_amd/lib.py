@@ -18,8 +18,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libctsm_mi355x.so")
 _LIB = None
 
-CTSM_OK, CTSM_E_ARG, CTSM_E_CONFIG, CTSM_E_HIP, CTSM_E_SOLVER, CTSM_E_NODEVICE = range(6)
-_NAMES = {1: "CTSM_E_ARG", 2: "CTSM_E_CONFIG", 3: "CTSM_E_HIP", 4: "CTSM_E_SOLVER", 5: "CTSM_E_NODEVICE"}
+CTSM_OK, CTSM_E_ARG, CTSM_E_CONFIG, CTSM_E_HIP, CTSM_E_SOLVER, CTSM_E_NODEVICE, CTSM_E_ENDRUN = range(7)
+_NAMES = {1: "CTSM_E_ARG", 2: "CTSM_E_CONFIG", 3: "CTSM_E_HIP", 4: "CTSM_E_SOLVER", 5: "CTSM_E_NODEVICE",
+          6: "CTSM_E_ENDRUN"}
 
 # every symbol declared in include/ctsm_gpu.h
 EXPORTS = [
@@ -28,6 +29,8 @@ EXPORTS = [
     "ctsm_soilwater_dev", "ctsm_soiltemperature_dev", "ctsm_soilwater_fractions_dev",
     "ctsm_soil_step_fused_dev", "ctsm_dev_alloc", "ctsm_dev_free", "ctsm_memcpy", "ctsm_dev_copy",
     "ctsm_event_record", "ctsm_event_elapsed", "ctsm_event_destroy", "ctsm_sync", "ctsm_stream",
+    "ctsm_set_canopy_params", "ctsm_canopyfluxes", "ctsm_root_sink", "ctsm_canopyfluxes_dev",
+    "ctsm_root_sink_dev", "ctsm_land_step_dev",
 ]
 
 
@@ -36,7 +39,7 @@ class CtsmError(RuntimeError):
 
     def __init__(self, code, index=0, where=""):
         self.code, self.index = code, index
-        super().__init__(f"{where}: {_NAMES.get(code, code)} (first failing column {index})")
+        super().__init__(f"{where}: {_NAMES.get(code, code)} (first failing column / patch {index})")
 
 
 def load():
@@ -72,6 +75,14 @@ def load():
     L.ctsm_event_record.argtypes = [C.c_void_p, P(C.c_void_p)]
     L.ctsm_event_elapsed.argtypes = [C.c_void_p, C.c_void_p, P(C.c_float)]
     L.ctsm_event_destroy.argtypes = [C.c_void_p]
+    L.ctsm_set_canopy_params.argtypes = [C.c_void_p, P(abi.CanopyParams), P(abi.PftParams)]
+    can = col + [P(abi.CanopyState)]
+    L.ctsm_canopyfluxes.argtypes = can + [C.c_int32, abi.i32p, C.c_double]
+    L.ctsm_root_sink.argtypes = can + [C.c_int32, abi.i32p]
+    L.ctsm_canopyfluxes_dev.argtypes = can + [C.c_int32, C.c_void_p, C.c_double]
+    L.ctsm_root_sink_dev.argtypes = can + [C.c_int32, C.c_void_p]
+    L.ctsm_land_step_dev.argtypes = col + [P(abi.PatchState), P(abi.CanopyState), C.c_int32, C.c_void_p,
+                                           C.c_int32, C.c_void_p, C.c_double]
     _LIB = L
     return L
 
@@ -146,6 +157,57 @@ class Context:
         f, fp, n = _filter(filter_, st.dims.ncol)
         self._check(self.L.ctsm_soilwater_fractions(self.h, C.byref(st.layout()), C.byref(cs), n, fp),
                     "SetSoilWaterFractions")
+
+    # ---- canopy ------------------------------------------------------------
+    def set_canopy_params(self, params, pft_table):
+        """params: dict of abi.CANOPY_PARAM_NAMES; pft_table: dict of abi.PFT_FIELDS arrays."""
+        prm = abi.CanopyParams()
+        for k in abi.CANOPY_PARAM_NAMES:
+            setattr(prm, k, float(params[k]))
+        keep = []
+        pft = abi.PftParams()
+        for name, t in abi.PFT_FIELDS:
+            a = np.ascontiguousarray(pft_table[name], dtype=np.int32 if t == "i32" else np.float64)
+            keep.append(a)
+            setattr(pft, name, a.ctypes.data_as(abi._CT[t]))
+            pft.npft = len(a)
+        self._check(self.L.ctsm_set_canopy_params(self.h, C.byref(prm), C.byref(pft)), "ctsm_set_canopy_params")
+
+    def canopyfluxes(self, st, filter_p=None, dtime=1800.0):
+        cs, _ = st.structs()
+        can = st.canopy_struct()
+        f, fp, n = _filter(filter_p, st.dims.npatch)
+        self._check(self.L.ctsm_canopyfluxes(self.h, C.byref(st.layout()), C.byref(cs), C.byref(can), n, fp, dtime),
+                    "CanopyFluxes")
+
+    def root_sink(self, st, filter_c=None):
+        cs, _ = st.structs()
+        can = st.canopy_struct()
+        f, fp, n = _filter(filter_c, st.dims.ncol)
+        self._check(self.L.ctsm_root_sink(self.h, C.byref(st.layout()), C.byref(cs), C.byref(can), n, fp),
+                    "root sink")
+
+    def canopyfluxes_dev(self, ds, filterp_dev=None, n=None, dtime=1800.0):
+        cs, _ = ds.structs()
+        can = ds.canopy_struct()
+        n = ds.dims.npatch if n is None else n
+        fp = C.c_void_p(filterp_dev) if filterp_dev is not None else None
+        self._check(self.L.ctsm_canopyfluxes_dev(self.h, C.byref(ds.layout()), C.byref(cs), C.byref(can), n, fp,
+                                                 dtime), "CanopyFluxes")
+
+    def land_step_dev(self, ds, filterc_dev=None, nc=None, filterp_dev=None, np_=None, dtime=1800.0, structs=None):
+        """One land step with canopy on a DeviceState (enqueues only)."""
+        if structs is None:
+            cs, ps = ds.structs()
+            can = ds.canopy_struct()
+        else:
+            cs, ps, can = structs
+        nc = ds.dims.ncol if nc is None else nc
+        np_ = ds.dims.npatch if np_ is None else np_
+        fc = C.c_void_p(filterc_dev) if filterc_dev is not None else None
+        fp = C.c_void_p(filterp_dev) if filterp_dev is not None else None
+        self._check(self.L.ctsm_land_step_dev(self.h, C.byref(ds.layout()), C.byref(cs), C.byref(ps), C.byref(can),
+                                              nc, fc, np_, fp, dtime), "land step")
 
     # ---- device memory / events ------------------------------------------
     def alloc(self, nbytes):

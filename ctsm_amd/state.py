@@ -15,15 +15,26 @@ from . import abi
 _NP = {"i32": np.int32, "f64": np.float64}
 
 
+# Canopy fields that are the same arrays as a column or patch field.
+CANOPY_ALIAS = {
+    "column": ("p", "column"), "wtcol": ("p", "wtcol"), "frac_veg_nosno": ("p", "frac_veg_nosno"),
+    "dlrad": ("p", "dlrad"), "cgrnd": ("p", "cgrnd"), "eflx_sh_grnd": ("p", "eflx_sh_grnd"),
+    "eflx_sh_snow": ("p", "eflx_sh_snow"), "eflx_sh_soil": ("p", "eflx_sh_soil"),
+    "eflx_sh_h2osfc": ("p", "eflx_sh_h2osfc"), "qflx_evap_soi": ("p", "qflx_evap_soi"),
+    "qflx_ev_snow": ("p", "qflx_ev_snow"), "qflx_ev_soil": ("p", "qflx_ev_soil"),
+    "qflx_ev_h2osfc": ("p", "qflx_ev_h2osfc"), "qflx_rootsoi_col": ("c", "qflx_rootsoi"),
+}
+
+
 class Dims:
-    def __init__(self, ncol, npatch, nlevsno=12, nlevsoi=20, nlevgrnd=25):
-        self.ncol, self.npatch = int(ncol), int(npatch)
+    def __init__(self, ncol, npatch, nlevsno=12, nlevsoi=20, nlevgrnd=25, ngrid=0):
+        self.ncol, self.npatch, self.ngrid = int(ncol), int(npatch), int(ngrid)
         self.nlevsno, self.nlevsoi, self.nlevgrnd = nlevsno, nlevsoi, nlevgrnd
 
     def shape(self, kind):
-        n = self.npatch if kind in ("p", "lyr") else self.ncol
+        n = {"p": self.npatch, "lyr": self.npatch, "soip": self.npatch, "g": self.ngrid}.get(kind, self.ncol)
         nl = abi.nlev_of(kind, self.nlevsno, self.nlevgrnd)
-        return (n,) if kind in ("c", "p") else (nl, n)
+        return (n,) if kind in ("c", "p", "g") else (nl, n)
 
 
 class HostState:
@@ -43,16 +54,41 @@ class HostState:
             if fill_nan and t == "f64":
                 a[...] = np.nan
             self.pch[name] = a
+        self.can = {}
+        if dims.ngrid > 0:
+            for name, t, kind in abi.CANOPY_FIELDS:
+                if name in CANOPY_ALIAS:
+                    continue
+                a = np.zeros(dims.shape(kind), dtype=_NP[t])
+                if fill_nan and t == "f64":
+                    a[...] = np.nan
+                self.can[name] = a
 
     def copy(self):
         o = HostState.__new__(HostState)
         o.dims = self.dims
         o.col = {k: v.copy() for k, v in self.col.items()}
         o.pch = {k: v.copy() for k, v in self.pch.items()}
+        o.can = {k: v.copy() for k, v in self.can.items()}
         return o
 
+    def canopy_array(self, name):
+        """The array behind canopy field `name` (aliases resolve to col/pch)."""
+        if name in CANOPY_ALIAS:
+            g, k = CANOPY_ALIAS[name]
+            return self.pch[k] if g == "p" else self.col[k]
+        return self.can[name]
+
     def layout(self):
-        return abi.Layout(1, self.dims.ncol, 1, self.dims.npatch)
+        return abi.Layout(1, self.dims.ncol, 1, self.dims.npatch, 1, self.dims.ngrid)
+
+    def canopy_struct(self):
+        cs = abi.CanopyState()
+        for name, t, _ in abi.CANOPY_FIELDS:
+            a = self.canopy_array(name)
+            assert a.flags.c_contiguous
+            setattr(cs, name, a.ctypes.data_as(abi._CT[t]))
+        return cs
 
     def structs(self):
         """ctypes ColState / PatchState pointing at these numpy buffers."""
@@ -89,6 +125,17 @@ class HostState:
         o.col["patchi"] = (start + 1).astype(np.int32)
         o.col["patchf"] = (start + npc).astype(np.int32)
         o.pch["column"] = np.repeat(np.arange(1, len(cols) + 1, dtype=np.int32), npc)
+        o.can = {}
+        if d.ngrid > 0:
+            # synthetic states carry one gridcell per column
+            o.dims.ngrid = len(cols)
+            for name, t, kind in abi.CANOPY_FIELDS:
+                if name in CANOPY_ALIAS:
+                    continue
+                src = self.can[name]
+                idx = {"p": pidx, "soip": pidx, "g": cols}.get(kind, cols)
+                o.can[name] = np.ascontiguousarray(src[..., idx])
+            o.can["gridcell"] = o.pch["column"].copy()
         return o
 
 
@@ -100,17 +147,29 @@ class DeviceState:
         self.ctx = ctx
         self.dims = host.dims
         self._shapes = {}
-        self.col, self.pch = {}, {}
-        for group, src, dst in (("c", host.col, self.col), ("p", host.pch, self.pch)):
+        self.col, self.pch, self.can = {}, {}, {}
+        for group, src, dst in (("c", host.col, self.col), ("p", host.pch, self.pch),
+                                ("g", getattr(host, "can", {}), self.can)):
             for k, v in src.items():
                 a = np.ascontiguousarray(v)
-                ptr = ctx.alloc(a.nbytes)
+                ptr = ctx.alloc(max(a.nbytes, 8))
                 ctx.h2d(ptr, a)
                 dst[k] = ptr
                 self._shapes[(group, k)] = (a.shape, a.dtype)
 
     def layout(self):
-        return abi.Layout(1, self.dims.ncol, 1, self.dims.npatch)
+        return abi.Layout(1, self.dims.ncol, 1, self.dims.npatch, 1, self.dims.ngrid)
+
+    def canopy_struct(self):
+        cs = abi.CanopyState()
+        for name, t, _ in abi.CANOPY_FIELDS:
+            if name in CANOPY_ALIAS:
+                g, k = CANOPY_ALIAS[name]
+                ptr = self.pch[k] if g == "p" else self.col[k]
+            else:
+                ptr = self.can[name]
+            setattr(cs, name, C.cast(C.c_void_p(ptr), abi._CT[t]))
+        return cs
 
     def structs(self):
         cs = abi.ColState()
@@ -124,8 +183,8 @@ class DeviceState:
     def to_host(self):
         o = HostState.__new__(HostState)
         o.dims = self.dims
-        o.col, o.pch = {}, {}
-        for group, src, dst in (("c", self.col, o.col), ("p", self.pch, o.pch)):
+        o.col, o.pch, o.can = {}, {}, {}
+        for group, src, dst in (("c", self.col, o.col), ("p", self.pch, o.pch), ("g", self.can, o.can)):
             for k, ptr in src.items():
                 shape, dt = self._shapes[(group, k)]
                 a = np.empty(shape, dtype=dt)
@@ -135,7 +194,8 @@ class DeviceState:
 
     def upload(self, host):
         """Overwrite the device copy with a host state of the same dims."""
-        for group, src, dst in (("c", host.col, self.col), ("p", host.pch, self.pch)):
+        for group, src, dst in (("c", host.col, self.col), ("p", host.pch, self.pch),
+                                ("g", getattr(host, "can", {}), self.can)):
             for k, v in src.items():
                 self.ctx.h2d(dst[k], np.ascontiguousarray(v))
 
@@ -143,7 +203,7 @@ class DeviceState:
         return sum(int(np.prod(s)) * np.dtype(d).itemsize for s, d in self._shapes.values())
 
     def free(self):
-        for d in (self.col, self.pch):
+        for d in (self.col, self.pch, self.can):
             for k, ptr in d.items():
                 self.ctx.free(ptr)
             d.clear()

@@ -273,3 +273,239 @@ def workload(name, ncol=None, seed=None):
     if name == "uneven":
         return soil_only(ncol or 65536, 0xC75D0001 if seed is None else seed, snow=True, bedrock=True)
     raise KeyError(name)
+
+
+# ---------------------------------------------------------------------------
+# Canopy workloads (1-16 vegetated patches per column)
+
+S_NPATCH, S_PFT, S_LAI, S_SAI, S_HTOP, S_TV = 40, 41, 42, 43, 44, 45
+S_FT, S_RH, S_WIND, S_WDIR, S_SW, S_LW, S_PBOT = 50, 51, 52, 53, 54, 55, 56
+S_VCS, S_VCH, S_FWET, S_LIQCAN, S_T10, S_DAYL, S_NOON, S_BETA, S_RESIS = 60, 61, 62, 63, 64, 65, 66, 67, 68
+
+CPAIR, RAIR = 1.00464e3, 6.02214e26 * 1.38065e-23 / 28.966
+
+
+def canopy_params():
+    """Representative CLM5/6 param-file scalars (the param file itself is not
+    available offline; values follow the CLM5 technical note)."""
+    return dict(cv=0.01, a_coef=0.13, a_exp=0.45, csoilc=0.004, z_dl=0.05, lai_dl=0.5, wind_min=1.0,
+                zetamaxstable=2.0, kc25_coef=404.9e-6, ko25_coef=278.4e-3, cp25_yr2000=42.75e-6,
+                kcha=79430.0, koha=36380.0, cpha=37830.0, vcmaxha=72000.0, jmaxha=50000.0, tpuha=72000.0,
+                lmrha=46390.0, vcmaxhd=200000.0, jmaxhd=200000.0, tpuhd=200000.0, lmrhd=150650.0,
+                lmrse=490.0, fnr=7.16, act25=60.0, theta_psii=0.7, theta_ip=0.95, fnps=0.15,
+                tpu25ratio=0.167, kp25ratio=20000.0, vcmaxse_sf=1.0, jmaxse_sf=1.0, tpuse_sf=1.0,
+                jmax25top_sf=1.0, leaf_mr_vcm=0.015)
+
+
+def pft_table():
+    """Rows 0..16 (bare soil, 14 natural PFTs, two crops) with representative
+    pftcon values; row 14 is the C4 grass."""
+    n = 17
+    trees = np.zeros(n, np.int32)
+    trees[1:9] = 1
+    shrubs = np.zeros(n, np.int32)
+    shrubs[9:12] = 1
+    t = dict(
+        dleaf=np.full(n, 0.04),
+        c3psn=np.where(np.arange(n) == 14, 0.0, 1.0),
+        slatop=np.array([0.01, 0.010, 0.008, 0.024, 0.012, 0.012, 0.030, 0.030, 0.030, 0.012, 0.030, 0.030,
+                         0.030, 0.030, 0.030, 0.030, 0.030]),
+        leafcn=np.array([25.0, 58.0, 58.0, 25.8, 29.6, 29.6, 23.1, 23.1, 23.1, 29.6, 23.1, 23.1, 28.1, 28.1,
+                         35.4, 28.0, 28.0]),
+        flnr=np.array([0.1, 0.0509, 0.0466, 0.0546, 0.0461, 0.0515, 0.0716, 0.1007, 0.1007, 0.0517, 0.0943,
+                       0.0943, 0.1365, 0.1365, 0.0900, 0.1200, 0.1200]),
+        fnitr=np.full(n, 1.0),
+        mbbopt=np.where(np.arange(n) == 14, 4.0, 9.0),
+        medlynslope=np.array([4.0, 2.35, 2.35, 2.35, 4.12, 4.12, 4.45, 4.45, 4.45, 4.70, 4.70, 4.70, 2.22,
+                              5.25, 1.62, 5.79, 5.79]),
+        medlynintercept=np.full(n, 100.0),
+        smpso=np.array([-66000.0, -66000.0, -66000.0, -66000.0, -66000.0, -35000.0, -35000.0, -35000.0,
+                        -35000.0, -83000.0, -83000.0, -83000.0, -74000.0, -74000.0, -74000.0, -74000.0,
+                        -74000.0]),
+        smpsc=np.array([-255000.0, -255000.0, -255000.0, -255000.0, -255000.0, -224000.0, -224000.0,
+                        -224000.0, -224000.0, -428000.0, -428000.0, -428000.0, -275000.0, -275000.0,
+                        -275000.0, -275000.0, -275000.0]),
+        theta_cj=np.where(np.arange(n) == 14, 0.80, 0.98),
+        z0v_Cr=np.full(n, 0.35), z0v_Cs=np.full(n, 0.003), z0v_c=np.full(n, 0.37), z0v_cw=np.full(n, 2.0),
+        z0v_LAImax=np.full(n, 2.0),
+        is_tree=trees, is_shrub=shrubs,
+        dbh=np.where(trees == 1, 0.35, np.where(shrubs == 1, 0.06, 0.0)),
+        fbw=np.where(trees + shrubs > 0, 0.65, 0.7),
+        nstem=np.where(trees == 1, 0.1, np.where(shrubs == 1, 1.0, 0.0)),
+        rstem_per_dbh=np.full(n, 5.0e3),
+        wood_density=np.where(trees + shrubs > 0, 500.0, 0.0),
+    )
+    return t
+
+
+def _qsat_np(T, p):
+    """Flatau et al. (1992) saturation humidity (input generation only)."""
+    a = [6.11213476, 0.444007856, 0.143064234e-01, 0.264461437e-03, 0.305903558e-05, 0.196237241e-07,
+         0.892344772e-10, -0.373208410e-12, 0.209339997e-15]
+    c = [6.11123516, 0.503109514, 0.188369801e-01, 0.420547422e-03, 0.614396778e-05, 0.602780717e-07,
+         0.387940929e-09, 0.149436277e-11, 0.262655803e-14]
+    td = np.clip(T - TFRZ, -75.0, 100.0)
+    k = np.where(td[None, :] >= 0.0, np.array(a)[:, None], np.array(c)[:, None])
+    es = k[8]
+    for i in range(7, -1, -1):
+        es = k[i] + td * es
+    es = es * 100.0
+    return 0.622 * es / (p - 0.378 * es), es
+
+
+def with_canopy(soil, seed, pmax=16, pmin=1, col0=0):
+    """Expand a soil-only state to 1..pmax vegetated patches per column with the
+    canopy forcing and state of the full-column workloads (one gridcell per
+    column).  Patch draws are keyed by global patch id col*pmax + k so any
+    column regenerates alone."""
+    d0 = soil.dims
+    ncol = d0.ncol
+    gid = np.arange(col0, col0 + ncol, dtype=np.int64)
+    npc = pmin + np.floor(uniform(seed, S_NPATCH, gid) * (pmax - pmin + 1)).astype(np.int32)
+    npc = np.minimum(npc, pmax)
+    npatch = int(npc.sum())
+    d = Dims(ncol, npatch, d0.nlevsno, d0.nlevsoi, d0.nlevgrnd, ngrid=ncol)
+    st = HostState(d)
+    for k, v in soil.col.items():
+        st.col[k] = v.copy()
+    c, p, g = st.col, st.pch, st.can
+    NS = d.nlevsno
+    start = np.concatenate([[0], np.cumsum(npc)[:-1]]).astype(np.int64)
+    c["patchi"][:] = (start + 1).astype(np.int32)
+    c["patchf"][:] = (start + npc).astype(np.int32)
+    pcol = np.repeat(np.arange(ncol), npc)                 # 0-based column of each patch
+    kin = np.arange(npatch) - np.repeat(start, npc)       # patch rank within its column
+    pid = gid[pcol] * pmax + kin                          # global patch id
+    for k in g:
+        g[k][...] = 0.0 if g[k].dtype == np.float64 else 0
+
+    # ---- gridcell / column forcing
+    ft = 250.0 + 65.0 * uniform(seed, S_FT, gid)
+    pbot = 1000.0 * (60.0 + 43.0 * uniform(seed, S_PBOT, gid))
+    rh = 20.0 + 80.0 * uniform(seed, S_RH, gid)
+    qs, es = _qsat_np(ft, pbot)
+    q = rh / 100.0 * qs
+    wind = 0.5 + 14.5 * uniform(seed, S_WIND, gid)
+    wdir = 2.0 * np.pi * uniform(seed, S_WDIR, gid)
+    sw = 1000.0 * uniform(seed, S_SW, gid)
+    lw = 150.0 + 300.0 * uniform(seed, S_LW, gid)
+    th = ft * (1.0e5 / pbot) ** (RAIR / CPAIR)
+    e = q * pbot / (0.622 + 0.378 * q)
+    g["forc_u"][:] = wind * np.cos(wdir)
+    g["forc_v"][:] = wind * np.sin(wdir)
+    g["forc_pco2"][:] = 367.0e-6 * pbot
+    g["forc_po2"][:] = 0.209 * pbot
+    for k in ("forc_hgt_u", "forc_hgt_t", "forc_hgt_q"):
+        g[k][:] = 30.0
+    dayl = 3600.0 * (8.0 + 8.0 * uniform(seed, S_DAYL, gid))
+    g["dayl"][:] = dayl
+    g["max_dayl"][:] = 16.0 * 3600.0
+    g["near_local_noon"][:] = (uniform(seed, S_NOON, gid) < 0.15).astype(np.int32)
+    g["forc_q"][:] = q
+    g["forc_pbot"][:] = pbot
+    g["forc_th"][:] = th
+    g["forc_rho"][:] = (pbot - 0.378 * e) / (RAIR * ft)
+    g["thv"][:] = th * (1.0 + 0.61 * q)
+    c["forc_lwrad"][:] = lw
+    tg = c["t_grnd"]
+    t1 = c["t_soisno"][NS]
+    ttop = c["t_soisno"][NS + c["snl"], np.arange(ncol)]
+    qsoil, _ = _qsat_np(t1, pbot)
+    qsnow, _ = _qsat_np(ttop, pbot)
+    qh2o, _ = _qsat_np(c["t_h2osfc"], pbot)
+    beta = 0.2 + 0.8 * uniform(seed, S_BETA, gid)
+    g["qg_soil"][:] = qsoil * beta
+    g["qg_snow"][:] = qsnow
+    g["qg_h2osfc"][:] = qh2o
+    fs, fh = c["frac_sno_eff"], c["frac_h2osfc"]
+    g["qg"][:] = fs * qsnow + (1.0 - fs - fh) * g["qg_soil"] + fh * qh2o
+    qs1, _ = _qsat_np(tg + 0.5, pbot)
+    qs0, _ = _qsat_np(tg - 0.5, pbot)
+    g["dqgdT"][:] = (qs1 - qs0) * beta
+    g["z0mg"][:] = 0.01
+    g["soilbeta"][:] = beta
+    g["soilresis"][:] = 50.0 + 1950.0 * uniform(seed, S_RESIS, gid)
+    for k in ("h2osoi_liqvol", "qflx_tran_veg_col", "rootr_col"):
+        g[k][...] = 0.0
+    c["qflx_rootsoi"][...] = 0.0
+
+    # ---- patches
+    p["column"][:] = (pcol + 1).astype(np.int32)
+    p["wtcol"][:] = 1.0 / npc[pcol]
+    p["frac_veg_nosno"][:] = 1
+    g["itype"][:] = (1 + np.floor(uniform(seed, S_PFT, pid) * 14)).astype(np.int32)
+    g["gridcell"][:] = p["column"]
+    g["active"][:] = 1
+    elai = 8.0 * uniform(seed, S_LAI, pid)
+    esai = 2.0 * uniform(seed, S_SAI, pid)
+    elai = np.maximum(elai, 1.0e-3)
+    htop = 0.1 + 34.9 * uniform(seed, S_HTOP, pid)
+    kb = 0.5
+    fsun = (1.0 - np.exp(-kb * elai)) / (kb * elai)
+    g["elai"][:] = elai
+    g["esai"][:] = esai
+    g["laisun"][:] = elai * fsun
+    g["laisha"][:] = elai - elai * fsun
+    g["laisun_z"][:] = g["laisun"]
+    g["laisha_z"][:] = g["laisha"]
+    g["htop"][:] = htop
+    swp = sw[pcol]
+    lt = elai + esai
+    g["sabv"][:] = 0.85 * swp * (1.0 - np.exp(-0.5 * lt))
+    par = 0.5 * swp
+    g["parsun_z"][:] = 0.5 * par * kb
+    g["parsha_z"][:] = 0.1 * par * kb
+    g["vcmaxcintsun"][:] = 0.6 + 0.4 * uniform(seed, S_VCS, pid)
+    g["vcmaxcintsha"][:] = 0.3 + 0.4 * uniform(seed, S_VCH, pid)
+    g["nrad"][:] = 1
+    fhgt = 30.0 + 0.055 * htop + 0.67 * htop
+    g["thm"][:] = ft[pcol] + 0.0098 * fhgt
+    g["emv"][:] = 1.0 - np.exp(-lt / 1.0)
+    fwet = 0.3 * uniform(seed, S_FWET, pid)
+    g["fwet"][:] = fwet
+    g["fdry"][:] = (1.0 - fwet) * elai / lt
+    g["t_a10"][:] = ft[pcol] + 6.0 * (uniform(seed, S_T10, pid) - 0.5)
+    for k in ("o3coefvsun", "o3coefvsha", "o3coefgsun", "o3coefgsha"):
+        g[k][:] = 1.0
+    # rooting profile (Zeng 2001 form) over the column's soil layers above bedrock
+    zi = c["zi"][NS:, :]
+    nbed = c["nbedrock"]
+    ra = 6.0 + (g["itype"] % 5).astype(np.float64)
+    rb = 1.5 + 0.1 * (g["itype"] % 7).astype(np.float64)
+    rf = np.zeros((d.nlevgrnd, npatch))
+    for j in range(1, d.nlevsoi + 1):
+        z0, z1 = zi[j - 1, pcol], zi[j, pcol]
+        v = 0.5 * (np.exp(-ra * z0) + np.exp(-rb * z0) - np.exp(-ra * z1) - np.exp(-rb * z1))
+        rf[j - 1] = np.where(j <= nbed[pcol], v, 0.0)
+    rf /= rf.sum(axis=0, keepdims=True)
+    g["rootfr"][...] = rf
+    g["displa"][:] = 0.67 * htop
+    g["z0mv"][:] = 0.055 * htop
+    tv = ft[pcol] + 4.0 * (uniform(seed, S_TV, pid) - 0.5)
+    g["t_veg"][:] = tv
+    g["t_stem"][:] = tv
+    g["snocan"][:] = 0.0
+    g["liqcan"][:] = 0.1 * uniform(seed, S_LIQCAN, pid)
+    # ground heat inputs of SoilTemperature not produced by CanopyFluxes
+    sabg = 0.8 * swp * np.exp(-0.5 * lt)
+    for k in ("sabg", "sabg_soil", "sabg_snow"):
+        p[k][:] = sabg
+    p["sabg_lyr"][...] = 0.0
+    p["sabg_lyr"][NS + c["snl"][pcol], np.arange(npatch)] = sabg
+    for k in ("dlrad", "cgrnd", "eflx_sh_grnd", "eflx_sh_snow", "eflx_sh_soil", "eflx_sh_h2osfc",
+              "qflx_evap_soi", "qflx_ev_snow", "qflx_ev_soil", "qflx_ev_h2osfc", "eflx_gnet", "dgnetdT",
+              "sabg_chk"):
+        p[k][:] = 0.0
+    return st
+
+
+def canopy_workload(name, ncol=None, seed=None):
+    """'canopy' (no snow, 20 layers, 1-16 patches) and 'canopy_uneven' (snow and
+    bedrock mixes of the uneven workload, 1-16 patches)."""
+    if name == "canopy":
+        s = 0xC75D0002 if seed is None else seed
+        return with_canopy(soil_only(ncol or 262144, s), s)
+    if name == "canopy_uneven":
+        s = 0xC75D0003 if seed is None else seed
+        return with_canopy(soil_only(ncol or 1048576, s, snow=True, bedrock=True), s)
+    raise KeyError(name)

@@ -56,6 +56,8 @@ extern "C" {
 #define CTSM_E_HIP         3   /* HIP runtime error                              */
 #define CTSM_E_SOLVER      4   /* LAPACK-style info != 0 in a column solve        */
 #define CTSM_E_NODEVICE    5   /* no GPU visible                                  */
+#define CTSM_E_ENDRUN      6   /* a check where the reference calls endrun fails   */
+                               /* (canopy: quadratic, brent bracket, gs < 0, ...)  */
 
 /* soil water lower / upper boundary conditions (SoilWaterMovementMod.F90:55-59) */
 #define CTSM_BC_HEAD        0
@@ -67,6 +69,14 @@ extern "C" {
 /* snow thermal conductivity (SoilTemperatureMod.F90:739-760) */
 #define CTSM_SNOWTHK_JORDAN1991 0
 #define CTSM_SNOWTHK_STURM1997  1
+
+/* CanopyFluxes / Photosynthesis switches (values as in the reference) */
+#define CTSM_Z0_ZENGWANG2007      0   /* z0param_method = 'ZengWang2007' (CanopyFluxesMod.F90:905-911) */
+#define CTSM_Z0_MEIER2022         1   /* z0param_method = 'Meier2022'    (CanopyFluxesMod.F90:913-932) */
+#define CTSM_STOMATA_BB1987       1   /* stomatalcond_mtd_bb1987     (PhotosynthesisMod.F90:82) */
+#define CTSM_STOMATA_MEDLYN2011   2   /* stomatalcond_mtd_medlyn2011 (PhotosynthesisMod.F90:83) */
+#define CTSM_SOILRESIS_LEEPIELKE1992 0 /* soil_resis_method (SurfaceResistanceMod.F90:23-24)       */
+#define CTSM_SOILRESIS_SL14       1
 
 /* ---- namelist / parameter block -----------------------------------------
  * Replaces init_soilwater_movement (SoilWaterMovementMod.F90:110-216),
@@ -89,6 +99,15 @@ typedef struct ctsm_config {
     double  xTolerUpper;        /* 0.1                                              */
     double  xTolerLower;        /* 0.01                                             */
     double  e_ice;              /* 6 (ice impedance exponent)                       */
+    /* CanopyFluxes / Photosynthesis namelist switches.  use_fates, use_cn,
+     * use_hydrstress, use_luna, use_c13, lnc_opt and perchroot are .false. and
+     * vcmax_opt = 0, nlevcan = 1 on this path. */
+    int32_t z0param_method;     /* CTSM_Z0_*                        (xml:621-622)    */
+    int32_t stomatalcond_method;/* CTSM_STOMATA_*                   (xml:108)        */
+    int32_t soil_resis_method;  /* CTSM_SOILRESIS_*                 (xml:270)        */
+    int32_t use_biomass_heat_storage;  /* 0/1                       (xml:457)        */
+    int32_t use_undercanopy_stability; /* 0/1                       (xml:454)        */
+    int32_t light_inhibit;      /* 0/1                              (xml:95)         */
 } ctsm_config;
 
 /* ---- array layout of one call -------------------------------------------- */
@@ -97,6 +116,8 @@ typedef struct ctsm_layout {
     int32_t ldc;          /* column extent of the allocation                          */
     int32_t begp_alloc;   /* lower patch bound of the allocation                      */
     int32_t ldp;          /* patch extent of the allocation                           */
+    int32_t begg_alloc;   /* lower gridcell bound of the allocation (canopy path only) */
+    int32_t ldg;          /* gridcell extent of the allocation                        */
 } ctsm_layout;
 
 /* ---- column state touched by the three routines --------------------------
@@ -195,6 +216,155 @@ typedef struct ctsm_patch_state {
     double        *sabg_chk;      /* (p) out                                             */
 } ctsm_patch_state;
 
+/* ---- CanopyFluxes parameters ----------------------------------------------
+ * Param-file scalars read once by CanopyFluxesMod readParams
+ * (CanopyFluxesMod.F90:158-189), PhotosynthesisMod readParams
+ * (PhotosynthesisMod.F90:479-560) and FrictionVelocityMod
+ * (zetamaxstable, FrictionVelocityMod.F90:511-538). */
+typedef struct ctsm_canopy_params {
+    double cv;             /* canopy turbulent transfer coefficient (m/s^0.5)   */
+    double a_coef;         /* bare-soil transfer coefficient a (Sakaguchi-Zeng)  */
+    double a_exp;          /* bare-soil transfer coefficient exponent           */
+    double csoilc;         /* dense-canopy soil drag coefficient                */
+    double z_dl;           /* litter thickness (m)                              */
+    double lai_dl;         /* plant litter area index                           */
+    double wind_min;       /* minimum wind speed (m/s)                          */
+    double zetamaxstable;  /* max zeta under stable conditions                  */
+    double kc25_coef, ko25_coef, cp25_yr2000;    /* mol/mol                      */
+    double kcha, koha, cpha;                     /* activation energies (J/mol)  */
+    double vcmaxha, jmaxha, tpuha, lmrha;        /* activation energies (J/mol)  */
+    double vcmaxhd, jmaxhd, tpuhd, lmrhd;        /* deactivation energies        */
+    double lmrse;                                /* entropy term (J/mol/K)       */
+    double fnr, act25;                           /* Rubisco N ratio and activity */
+    double theta_psii, theta_ip, fnps;           /* curvatures, fraction of PAR  */
+    double tpu25ratio, kp25ratio;
+    double vcmaxse_sf, jmaxse_sf, tpuse_sf, jmax25top_sf;
+    double leaf_mr_vcm;    /* canopystate_inst%leaf_mr_vcm                       */
+} ctsm_canopy_params;
+
+/* Per-PFT parameters (pftconMod.F90), host arrays of npft rows indexed by
+ * patch%itype (0-based PFT number, as in the reference). */
+#define CTSM_MAX_PFT 80
+typedef struct ctsm_pft_params {
+    int32_t npft;
+    int32_t pad;
+    const double *dleaf, *c3psn, *slatop, *leafcn, *flnr, *fnitr;
+    const double *mbbopt, *medlynslope, *medlynintercept;
+    const double *smpso, *smpsc, *theta_cj;
+    const double *z0v_Cr, *z0v_Cs, *z0v_c, *z0v_cw, *z0v_LAImax;
+    const int32_t *is_tree, *is_shrub;
+    const double *dbh, *fbw, *nstem, *rstem_per_dbh, *wood_density;
+} ctsm_pft_params;
+
+/* ---- canopy state: patch (p), column (c) and gridcell (g) arrays read or
+ * written by CanopyFluxes (CanopyFluxesMod.F90:450-640 associate list), its
+ * callees and the root-sink routine.  [SOIP] = (begp:endp, 1:nlevgrnd),
+ * patch index fastest; [SOI] as in ctsm_col_state.  Patch outputs that
+ * SoilTemperature reads (dlrad, cgrnd, eflx_sh_*, qflx_ev_*, qflx_evap_soi)
+ * are the same arrays ctsm_patch_state names. */
+typedef struct ctsm_canopy_state {
+    /* PatchType */
+    const int32_t *itype;          /* (p) in  PFT number                               */
+    const int32_t *column;         /* (p) in  patch%column                             */
+    const int32_t *gridcell;       /* (p) in  patch%gridcell                           */
+    const double  *wtcol;          /* (p) in  patch%wtcol                              */
+    const int32_t *frac_veg_nosno; /* (p) in                                           */
+    const int32_t *active;         /* (p) in  patch%active (root sink, p2c)            */
+    /* atm2lnd / GridcellType (gridcell) */
+    const double  *forc_u;         /* (g) in                                           */
+    const double  *forc_v;         /* (g) in                                           */
+    const double  *forc_pco2;      /* (g) in  Pa                                       */
+    const double  *forc_po2;       /* (g) in  Pa                                       */
+    const double  *forc_hgt_u;     /* (g) in  m                                        */
+    const double  *forc_hgt_t;     /* (g) in                                           */
+    const double  *forc_hgt_q;     /* (g) in                                           */
+    const double  *dayl;           /* (g) in  s                                        */
+    const double  *max_dayl;       /* (g) in  s                                        */
+    const int32_t *near_local_noon;/* (g) in  is_near_local_noon(londeg, 3600)          */
+    /* column forcing and ground state */
+    const double  *forc_q;         /* (c) in                                           */
+    const double  *forc_pbot;      /* (c) in                                           */
+    const double  *forc_th;        /* (c) in                                           */
+    const double  *forc_rho;       /* (c) in                                           */
+    const double  *thv;            /* (c) in                                           */
+    const double  *qg;             /* (c) in                                           */
+    const double  *qg_snow;        /* (c) in                                           */
+    const double  *qg_soil;        /* (c) in                                           */
+    const double  *qg_h2osfc;      /* (c) in                                           */
+    const double  *dqgdT;          /* (c) in                                           */
+    const double  *z0mg;           /* (c) in                                           */
+    const double  *soilbeta;       /* (c) in                                           */
+    const double  *soilresis;      /* (c) in                                           */
+    double        *h2osoi_liqvol;  /* [SOI] out                                        */
+    double        *qflx_tran_veg_col; /* (c) out (p2c, clm_driver.F90:1715)             */
+    double        *rootr_col;      /* [SOI] out root sink weights                      */
+    double        *qflx_rootsoi_col; /* [SOI] out (same array ctsm_col_state reads)   */
+    /* canopy state inputs (p) */
+    const double  *elai;
+    const double  *esai;
+    const double  *laisun;
+    const double  *laisha;
+    const double  *htop;
+    const double  *sabv;
+    const double  *parsun_z;       /* (p) nlevcan = 1                                  */
+    const double  *parsha_z;
+    const double  *laisun_z;
+    const double  *laisha_z;
+    const double  *vcmaxcintsun;
+    const double  *vcmaxcintsha;
+    const int32_t *nrad;
+    const double  *thm;
+    const double  *emv;
+    const double  *fwet;
+    const double  *fdry;
+    const double  *t_a10;
+    const double  *o3coefvsun;
+    const double  *o3coefvsha;
+    const double  *o3coefgsun;
+    const double  *o3coefgsha;
+    const double  *rootfr;         /* [SOIP] in                                        */
+    /* inout (p) */
+    double        *displa;
+    double        *z0mv;
+    double        *t_veg;
+    double        *t_stem;
+    double        *snocan;
+    double        *liqcan;
+    double        *cgrnds;
+    double        *cgrndl;
+    /* outputs (p): frictionvel_inst */
+    double        *z0hv, *z0qv;
+    double        *forc_hgt_u_patch, *forc_hgt_t_patch, *forc_hgt_q_patch;
+    double        *ram1, *rb1, *rah1, *rah2, *raw1, *raw2;
+    double        *ustar, *um, *uaf, *taf, *qaf, *obu, *zeta, *vpd, *num_iter;
+    double        *u10, *u10_clm, *va, *fv, *vds;
+    /* energyflux_inst */
+    double        *btran;
+    double        *rresis;         /* [SOIP] out                                       */
+    double        *rootr;          /* [SOIP] out                                       */
+    double        *eflx_sh_veg, *eflx_sh_stem, *eflx_sh_grnd, *eflx_sh_snow, *eflx_sh_soil, *eflx_sh_h2osfc;
+    double        *taux, *tauy, *dlrad, *ulrad, *cgrnd, *dhsdt_canopy;
+    /* waterflux / waterdiagnostic */
+    double        *qflx_evap_soi, *qflx_ev_snow, *qflx_ev_soil, *qflx_ev_h2osfc;
+    double        *qflx_tran_veg, *qflx_evap_veg;
+    double        *rh_af, *q_ref2m, *rh_ref2m, *rh_ref2m_r, *vpd_ref2m, *iwue_ln;
+    /* temperature / canopystate */
+    double        *t_ref2m, *t_ref2m_r, *t_skin;
+    double        *stem_biomass, *leaf_biomass, *dleaf_patch;
+    /* photosyns_inst (nlevcan = 1: the _z arrays are (p)) */
+    double        *rssun, *rssha, *psnsun, *psnsha;
+    double        *psnsun_wc, *psnsun_wj, *psnsun_wp, *psnsha_wc, *psnsha_wj, *psnsha_wp;
+    double        *fpsn, *fpsn_wc, *fpsn_wj, *fpsn_wp;
+    double        *lmrsun, *lmrsha, *gs_mol_sun, *gs_mol_sha, *gs_mol_sun_ln, *gs_mol_sha_ln;
+    double        *cisun_z, *cisha_z, *rssun_z, *rssha_z, *lmrsun_z, *lmrsha_z, *psnsun_z, *psnsha_z;
+    double        *vcmax_z, *tpu_z, *kp_z, *ac, *aj, *ap, *ag, *an, *gb_mol, *gs_mol;
+    double        *cp, *kc, *ko, *qe, *bbb, *mbb, *rh_leaf, *vpd_can, *lnca;
+    int32_t       *c3flag;
+    /* diagnostic: the canopy energy balance residual err(p) that the
+     * reference checks against 0.1 W/m2 (CanopyFluxesMod.F90:1742-1756) */
+    double        *canopy_err;
+} ctsm_canopy_state;
+
 /* ---- launch shapes (no counterpart in the reference: set by the autotuner) --
  * richards_lanes: lanes cooperating on one column in the Richards solve
  *   (1, 2, 4 or 8); the heat solve always runs one column per lane.
@@ -229,6 +399,12 @@ int ctsm_get_tuning(const ctsm_ctx *ctx, ctsm_tuning *t);
 /* First failing column (reference 1-based index) and code of the last call. */
 int ctsm_last_error(const ctsm_ctx *ctx, int32_t *code, int32_t *index);
 
+/* Canopy parameters, copied to the device once (replaces the readParams of
+ * CanopyFluxesMod / PhotosynthesisMod and the pftcon arrays).  Must precede any
+ * canopy call; CTSM_E_ARG for npft outside 1..CTSM_MAX_PFT. */
+int ctsm_set_canopy_params(ctsm_ctx *ctx, const ctsm_canopy_params *prm,
+                           const ctsm_pft_params *pft);
+
 /* ---- drop-in entry points (host arrays, synchronous) ----------------------
  * Each copies the filtered columns' fields to the device, runs the kernel and
  * copies the outputs back before returning, so the Fortran caller reads its
@@ -254,6 +430,26 @@ int ctsm_soilwater_fractions(ctsm_ctx *ctx, const ctsm_layout *lay,
                              const ctsm_col_state *col,
                              int32_t num_hydrologyc, const int32_t *filter_hydrologyc);
 
+/* Replaces CanopyFluxes (CanopyFluxesMod.F90:191-1765) without FATES, CN,
+ * plant hydraulic stress, LUNA, C13/C14, methane conductances and human
+ * stress indices, for nlevcan = 1; Photosynthesis / hybrid / brent / ci_func
+ * (PhotosynthesisMod.F90:1243-2701), FrictionVelocity / MoninObukIni
+ * (FrictionVelocityMod.F90:754-1209), QSat (QSatMod.F90:61-127) and
+ * calc_root_moist_stress (SoilMoistStressMod.F90:312-514) run inside.  Patch
+ * arrays are mirrored over the whole allocation (begp_alloc .. +ldp-1),
+ * column and gridcell arrays likewise. */
+int ctsm_canopyfluxes(ctsm_ctx *ctx, const ctsm_layout *lay,
+                      const ctsm_col_state *col, const ctsm_canopy_state *can,
+                      int32_t num_exposedvegp, const int32_t *filter_exposedvegp,
+                      double dtime);
+
+/* Replaces the p2c of qflx_tran_veg (clm_driver.F90:1714-1716, subgridAveMod
+ * p2c_1d_filter) followed by Compute_EffecRootFrac_And_VertTranSink_Default
+ * (SoilWaterPlantSinkMod.F90:332-424): rootr_col, qflx_rootsoi_col. */
+int ctsm_root_sink(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                   const ctsm_canopy_state *can,
+                   int32_t num_filterc, const int32_t *filterc);
+
 /* ---- device-resident entry points -----------------------------------------
  * Same arguments, but every pointer (fields and filter) is device memory and
  * the call only enqueues work on the context's stream.  filter == NULL means
@@ -277,6 +473,25 @@ int ctsm_soilwater_fractions_dev(ctsm_ctx *ctx, const ctsm_layout *lay,
 int ctsm_soil_step_fused_dev(ctsm_ctx *ctx, const ctsm_layout *lay,
                              const ctsm_col_state *col, const ctsm_patch_state *pch,
                              int32_t num_filter, const int32_t *filter_dev, double dtime);
+
+int ctsm_canopyfluxes_dev(ctsm_ctx *ctx, const ctsm_layout *lay,
+                          const ctsm_col_state *col, const ctsm_canopy_state *can,
+                          int32_t num_filterp, const int32_t *filterp_dev, double dtime);
+int ctsm_root_sink_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                       const ctsm_canopy_state *can,
+                       int32_t num_filterc, const int32_t *filterc_dev);
+
+/* One land step with canopy: CanopyFluxes over the patch filter, then per
+ * column SoilTemperature, SetSoilWaterFractions, the transpiration sink and
+ * SoilWater (clm_driver.F90:766, 900, 936; HydrologyNoDrainageMod.F90:302-346).
+ * The patch arrays CanopyFluxes writes and SoilTemperature reads must be the
+ * same device arrays in `pch` and `can`, and col->qflx_rootsoi must be
+ * can->qflx_rootsoi_col. */
+int ctsm_land_step_dev(ctsm_ctx *ctx, const ctsm_layout *lay,
+                       const ctsm_col_state *col, const ctsm_patch_state *pch,
+                       const ctsm_canopy_state *can,
+                       int32_t num_filterc, const int32_t *filterc_dev,
+                       int32_t num_filterp, const int32_t *filterp_dev, double dtime);
 
 /* ---- device memory and timing helpers --------------------------------------
  * Hosts need no other GPU runtime: device-resident state is allocated, filled

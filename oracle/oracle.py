@@ -138,3 +138,128 @@ def errsoi(st, tssbef, t_h2osfc_bef, t_grnd0, cfg=None, dtime=1800.0, filter_=No
                     tss.ctypes.data_as(abi.f64p), thb.ctypes.data_as(abi.f64p), tg0.ctypes.data_as(abi.f64p),
                     n, fp, C.c_double(dtime), err.ctypes.data_as(abi.f64p))
     return err
+
+
+# ---- canopy path --------------------------------------------------------------
+
+def _canopy_lib():
+    L = lib()
+    if not getattr(L, "_canopy_ready", False):
+        L.oracle_canopyfluxes.argtypes = [C.POINTER(abi.Config), C.POINTER(abi.CanopyParams),
+                                          C.POINTER(abi.PftParams), C.POINTER(abi.Layout),
+                                          C.POINTER(abi.ColState), C.POINTER(abi.CanopyState), C.c_int, abi.i32p,
+                                          C.c_double, abi.i32p]
+        L.oracle_root_sink.argtypes = [C.POINTER(abi.Config), C.POINTER(abi.Layout), C.POINTER(abi.ColState),
+                                       C.POINTER(abi.CanopyState), C.c_int, abi.i32p]
+        L.oracle_qsat_vec.argtypes = [C.c_int, abi.f64p, abi.f64p, abi.f64p, abi.f64p, abi.f64p]
+        L.oracle_quadratic.argtypes = [C.c_double, C.c_double, C.c_double, abi.f64p, abi.f64p]
+        L.oracle_frictionvelocity_vec.argtypes = [C.c_int] + [abi.f64p] * 11
+        L.oracle_moninobukini.argtypes = [C.c_double] * 6 + [abi.f64p, abi.f64p]
+        L._canopy_ready = True
+    return L
+
+
+def canopy_config(**kw):
+    """Oracle defaults plus the CLM6 canopy switches the synthetic workloads use."""
+    base = dict(z0param_method=1, stomatalcond_method=2, soil_resis_method=1, use_biomass_heat_storage=1,
+                use_undercanopy_stability=0, light_inhibit=1)
+    base.update(kw)
+    return config(**base)
+
+
+def canopy_params_struct(d):
+    s = abi.CanopyParams()
+    for k in abi.CANOPY_PARAM_NAMES:
+        setattr(s, k, float(d[k]))
+    return s
+
+
+class PftStruct:
+    """abi.PftParams over numpy copies of a pft table (kept alive here)."""
+
+    def __init__(self, table):
+        self.arrays = {}
+        self.s = abi.PftParams()
+        n = None
+        for name, t in abi.PFT_FIELDS:
+            a = np.ascontiguousarray(table[name], dtype=np.int32 if t == "i32" else np.float64)
+            n = len(a)
+            self.arrays[name] = a
+            setattr(self.s, name, a.ctypes.data_as(abi._CT[t]))
+        self.s.npft = n
+
+
+def canopyfluxes(st, cfg, prm, pft, filter_p=None, dtime=1800.0):
+    L = _canopy_lib()
+    cs, _ = st.structs()
+    can = st.canopy_struct()
+    f, fp = _filt(filter_p if filter_p is not None else np.arange(1, st.dims.npatch + 1))
+    bad = C.c_int32(0)
+    rc = L.oracle_canopyfluxes(C.byref(cfg), C.byref(canopy_params_struct(prm)), C.byref(pft.s),
+                               C.byref(st.layout()), C.byref(cs), C.byref(can), len(f), fp, C.c_double(dtime),
+                               C.byref(bad))
+    return rc, bad.value
+
+
+def root_sink(st, cfg, filter_c=None):
+    L = _canopy_lib()
+    cs, _ = st.structs()
+    can = st.canopy_struct()
+    f, fp = _filt(filter_c if filter_c is not None else np.arange(1, st.dims.ncol + 1))
+    L.oracle_root_sink(C.byref(cfg), C.byref(st.layout()), C.byref(cs), C.byref(can), len(f), fp)
+
+
+def land_step(st, cfg, prm, pft, dtime=1800.0):
+    """One land step with canopy, in clm_drv order: zero the ground-flux
+    derivatives (BiogeophysPreFluxCalcsMod.F90:390-392), CanopyFluxes,
+    SoilTemperature, SetSoilWaterFractions, the transpiration sink, SoilWater."""
+    st.pch["cgrnd"][:] = 0.0
+    st.can["cgrnds"][:] = 0.0
+    st.can["cgrndl"][:] = 0.0
+    rc, bad = canopyfluxes(st, cfg, prm, pft, dtime=dtime)
+    if rc:
+        return rc, bad
+    rc, bad = soiltemperature(st, cfg, dtime)
+    if rc:
+        return rc, bad
+    soilwater_fractions(st, cfg)
+    root_sink(st, cfg)
+    return soilwater(st, cfg, dtime)
+
+
+def qsat(T, p):
+    L = _canopy_lib()
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    p = np.ascontiguousarray(np.broadcast_to(p, T.shape), dtype=np.float64)
+    qs, es, qsdT = (np.zeros_like(T) for _ in range(3))
+    L.oracle_qsat_vec(len(T), *(a.ctypes.data_as(abi.f64p) for a in (T, p, qs, es, qsdT)))
+    return qs, es, qsdT
+
+
+def quadratic(a, b, c):
+    L = _canopy_lib()
+    r1, r2 = np.zeros(1), np.zeros(1)
+    rc = L.oracle_quadratic(a, b, c, r1.ctypes.data_as(abi.f64p), r2.ctypes.data_as(abi.f64p))
+    return rc, r1[0], r2[0]
+
+
+_FV_NAMES = ["ustar", "temp1", "temp2", "temp12m", "temp22m", "fm", "u10", "u10_clm", "va", "fv", "vds"]
+
+
+def frictionvelocity(hu, ht, hq, displa, z0m, z0h, z0q, obu, ur, um):
+    L = _canopy_lib()
+    args = [np.ascontiguousarray(np.atleast_1d(x), dtype=np.float64) for x in
+            np.broadcast_arrays(hu, ht, hq, displa, z0m, z0h, z0q, obu, ur, um)]
+    n = len(args[0])
+    out = np.zeros(11 * n)
+    L.oracle_frictionvelocity_vec(n, *(a.ctypes.data_as(abi.f64p) for a in args), out.ctypes.data_as(abi.f64p))
+    out = out.reshape(n, 11)
+    return {k: out[:, i] for i, k in enumerate(_FV_NAMES)}
+
+
+def moninobukini(ur, thv, dthv, zldis, z0m, zetamax):
+    L = _canopy_lib()
+    um, obu = np.zeros(1), np.zeros(1)
+    L.oracle_moninobukini(ur, thv, dthv, zldis, z0m, zetamax, um.ctypes.data_as(abi.f64p),
+                          obu.ctypes.data_as(abi.f64p))
+    return um[0], obu[0]

@@ -39,10 +39,15 @@ def test_library_exports_every_declared_symbol():
     assert set(declared) == set(lib.EXPORTS)
 
 
+STRUCTS = (("ctsm_config", abi.Config), ("ctsm_layout", abi.Layout), ("ctsm_tuning", abi.Tuning),
+           ("ctsm_col_state", abi.ColState), ("ctsm_patch_state", abi.PatchState),
+           ("ctsm_canopy_params", abi.CanopyParams), ("ctsm_pft_params", abi.PftParams),
+           ("ctsm_canopy_state", abi.CanopyState))
+
+
 def _probe(tmp_path):
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ctsm_gpu.h"', 'int main(void){']
-    for cname, py in (("ctsm_config", abi.Config), ("ctsm_layout", abi.Layout), ("ctsm_tuning", abi.Tuning),
-                      ("ctsm_col_state", abi.ColState), ("ctsm_patch_state", abi.PatchState)):
+    for cname, py in STRUCTS:
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
         for f, _ in py._fields_:
             lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
@@ -60,8 +65,7 @@ def test_struct_layouts_match_header(tmp_path):
     for line in out.splitlines():
         parts = line.split()
         table[(parts[0], parts[1])] = int(parts[2])
-    for cname, py in (("ctsm_config", abi.Config), ("ctsm_layout", abi.Layout), ("ctsm_tuning", abi.Tuning),
-                      ("ctsm_col_state", abi.ColState), ("ctsm_patch_state", abi.PatchState)):
+    for cname, py in STRUCTS:
         assert table[(cname, "size")] == C.sizeof(py), cname
         for f, _ in py._fields_:
             assert table[(cname, f)] == getattr(py, f).offset, (cname, f)
@@ -80,21 +84,27 @@ program sizes
   type(ctsm_layout_t) :: b
   type(ctsm_col_state_t) :: c
   type(ctsm_patch_state_t) :: d
-  print '(4I8)', c_sizeof(a), c_sizeof(b), c_sizeof(c), c_sizeof(d)
+  type(ctsm_canopy_params_t) :: e
+  type(ctsm_pft_params_t) :: f
+  type(ctsm_canopy_state_t) :: g
+  print '(7I8)', c_sizeof(a), c_sizeof(b), c_sizeof(c), c_sizeof(d), c_sizeof(e), c_sizeof(f), c_sizeof(g)
 end program sizes
 """)
     mod = os.path.join(ROOT, "ctsm_amd", "fortran", "ctsm_gpu_mod.F90")
     exe = tmp_path / "sizes"
     subprocess.run([flang, "-J", str(tmp_path), mod, str(prog), "-o", str(exe)], check=True, cwd=tmp_path)
     sizes = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
-    assert sizes == [C.sizeof(abi.Config), C.sizeof(abi.Layout), C.sizeof(abi.ColState), C.sizeof(abi.PatchState)]
+    assert sizes == [C.sizeof(abi.Config), C.sizeof(abi.Layout), C.sizeof(abi.ColState), C.sizeof(abi.PatchState),
+                     C.sizeof(abi.CanopyParams), C.sizeof(abi.PftParams), C.sizeof(abi.CanopyState)]
 
 
 def test_config_rejected_before_device():
     L = lib.load()
     h = C.c_void_p()
     for bad in (dict(soilwater_movement_method=0), dict(lower_boundary_condition=4),
-                dict(upper_boundary_condition=0), dict(nlevsno=13), dict(flux_calculation=42)):
+                dict(upper_boundary_condition=0), dict(nlevsno=13), dict(flux_calculation=42),
+                dict(z0param_method=2), dict(stomatalcond_method=0), dict(soil_resis_method=3),
+                dict(use_biomass_heat_storage=2), dict(itmax_canopy_fluxes=-1)):
         cfg = lib.default_config(**bad)
         assert L.ctsm_create(C.byref(cfg), 0, C.byref(h)) == lib.CTSM_E_CONFIG, bad
 

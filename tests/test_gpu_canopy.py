@@ -1,0 +1,130 @@
+"""CanopyFluxes path on the GPU against the oracle (bit-for-bit).
+
+Each test prints its case before the GPU work starts.
+"""
+import numpy as np
+import pytest
+
+from ctsm_amd import synthetic as S
+from ctsm_amd.lib import CtsmError
+from ctsm_amd.state import DeviceState
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+CANOPY_CFG_KEYS = ("z0param_method", "stomatalcond_method", "soil_resis_method", "use_biomass_heat_storage",
+                   "use_undercanopy_stability", "light_inhibit")
+
+
+@pytest.fixture(scope="module")
+def cctx():
+    """A context per canopy configuration (the switches are fixed at creation)."""
+    from ctsm_amd.lib import Context, default_config
+    made = {}
+
+    def get(**kw):
+        key = tuple(sorted(kw.items()))
+        if key not in made:
+            cfg = default_config(**kw)
+            ctx = Context(cfg)
+            ctx.set_canopy_params(S.canopy_params(), S.pft_table())
+            made[key] = ctx
+        return made[key]
+
+    yield get
+    for c in made.values():
+        c.close()
+
+
+def _assert_same(g, o, groups=("col", "pch", "can")):
+    for grp in groups:
+        a, b = getattr(g, grp), getattr(o, grp)
+        for k in b:
+            np.testing.assert_array_equal(a[k], b[k], err_msg=f"{grp}.{k}")
+
+
+def _oracle_cfg(ctx):
+    return O.canopy_config(**{k: getattr(ctx.cfg, k) for k in CANOPY_CFG_KEYS})
+
+
+@pytest.mark.parametrize("kw", [
+    dict(),
+    dict(z0param_method=0),
+    dict(stomatalcond_method=1),
+    dict(use_biomass_heat_storage=0),
+    dict(use_undercanopy_stability=1, soil_resis_method=0, light_inhibit=0),
+])
+def test_canopyfluxes_dev_bitwise(cctx, kw):
+    ctx = cctx(**kw)
+    st = S.canopy_workload("canopy", ncol=512)
+    print(f"case: CanopyFluxes dev, {kw}, 512 columns / {st.dims.npatch} patches", flush=True)
+    ds = DeviceState(st, ctx)
+    try:
+        ctx.canopyfluxes_dev(ds)
+        ctx.sync()
+        g = ds.to_host()
+    finally:
+        ds.free()
+    o = st.copy()
+    rc, bad = O.canopyfluxes(o, _oracle_cfg(ctx), S.canopy_params(), O.PftStruct(S.pft_table()))
+    assert rc == 0, bad
+    _assert_same(g, o)
+
+
+def test_canopyfluxes_dropin_filter_subset(cctx):
+    ctx = cctx()
+    st = S.canopy_workload("canopy", ncol=256)
+    filt = np.arange(1, st.dims.npatch + 1, 3, dtype=np.int32)
+    print(f"case: CanopyFluxes drop-in, every third of {st.dims.npatch} patches", flush=True)
+    g = st.copy()
+    ctx.canopyfluxes(g, filter_p=filt)
+    o = st.copy()
+    rc, bad = O.canopyfluxes(o, _oracle_cfg(ctx), S.canopy_params(), O.PftStruct(S.pft_table()), filter_p=filt)
+    assert rc == 0, bad
+    _assert_same(g, o)
+
+
+def test_root_sink_dropin(cctx):
+    ctx = cctx()
+    st = S.canopy_workload("canopy", ncol=256)
+    rc, bad = O.canopyfluxes(st, _oracle_cfg(ctx), S.canopy_params(), O.PftStruct(S.pft_table()))
+    assert rc == 0
+    print("case: root sink drop-in, 256 columns", flush=True)
+    g = st.copy()
+    ctx.root_sink(g)
+    o = st.copy()
+    O.root_sink(o, _oracle_cfg(ctx))
+    _assert_same(g, o)
+
+
+@pytest.mark.parametrize("nsteps,workload", [(1, "canopy"), (3, "canopy"), (3, "canopy_uneven")])
+def test_land_step_bitwise(cctx, nsteps, workload):
+    ctx = cctx()
+    st = S.canopy_workload(workload, ncol=384)
+    print(f"case: land step x{nsteps}, {workload}, 384 columns / {st.dims.npatch} patches", flush=True)
+    ds = DeviceState(st, ctx)
+    try:
+        for _ in range(nsteps):
+            ctx.land_step_dev(ds)
+        ctx.sync()
+        g = ds.to_host()
+    finally:
+        ds.free()
+    o = st.copy()
+    cfg = _oracle_cfg(ctx)
+    pft = O.PftStruct(S.pft_table())
+    for _ in range(nsteps):
+        rc, bad = O.land_step(o, cfg, S.canopy_params(), pft)
+        assert rc == 0, bad
+    _assert_same(g, o)
+
+
+def test_canopy_error_reports_patch(cctx):
+    ctx = cctx(z0param_method=0)
+    st = S.canopy_workload("canopy", ncol=16)
+    st.can["forc_hgt_u"][5] = -50.0
+    bad_patches = np.nonzero(st.pch["column"] == 6)[0] + 1
+    print("case: CanopyFluxes forcing height below the canopy, 16 columns", flush=True)
+    with pytest.raises(CtsmError) as ei:
+        ctx.canopyfluxes(st)
+    assert ei.value.index == bad_patches[0]

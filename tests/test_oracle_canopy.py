@@ -1,0 +1,219 @@
+"""Pins of the CanopyFluxes oracle (oracle/canopy_oracle.c), CPU only.
+
+What fixes each part, independent of the oracle's own formulas:
+  quadratic         the reference's known-answer tests
+                    (src/utils/test/quadratic_test/test_quadratic.pf)
+  QSat              Murphy & Koop (2005) saturation vapour pressure, the
+                    textbook q(e, p) relation and a finite-difference dq/dT
+  FrictionVelocity  neutral log-law limits (ustar, temp1/2, 2 m and 10 m
+                    winds) and continuity across the four stability regimes
+  MoninObukIni      neutral, stable-clamp and convective-velocity cases
+  Photosynthesis    the stomatal models' defining identities at the solved
+                    point: Ball-Berry gs = m A hs p / cs + b (the check the
+                    reference itself logs, PhotosynthesisMod.F90:2018-2027) and
+                    Medlyn (gs - g0 - t)^2 = g1^2 t^2 (gb + gs) / (gb D)
+  CanopyFluxes      the canopy energy-balance residual the reference checks
+                    (|err| <= 0.1 W/m2, CanopyFluxesMod.F90:1742-1756)
+  root stress/sink  normalisation of rootr and water conservation of the
+                    transpiration sink (sum_j qflx_rootsoi = qflx_tran_veg_col)
+"""
+import math
+
+import numpy as np
+import pytest
+
+from ctsm_amd import synthetic as S
+from oracle import oracle as O
+
+EPS = np.finfo(float).eps
+TFRZ = 273.15
+
+
+# ---- quadratic: the reference's own known-answer tests ---------------------
+
+@pytest.mark.parametrize("a", [1000.0, 5.12345678, -31.415927465859])
+def test_quadratic_zero_root(a):
+    rc, r1, r2 = O.quadratic(a, 0.0, 0.0)
+    assert rc == 0
+    assert abs(r1) <= 1e-15 and r2 == 1e36
+    assert abs(a * r1 * r1) <= 1e-15
+
+
+@pytest.mark.parametrize("b,c,e1,e2", [(3.0, 2.0, -2.0, -1.0), (0.0, -4.0, -2.0, 2.0)])
+def test_quadratic_simple_roots(b, c, e1, e2):
+    rc, r1, r2 = O.quadratic(1.0, b, c)
+    assert rc == 0
+    assert abs(r1 - e1) <= 1e-15 and abs(r2 - e2) <= 1e-15
+    for r in (r1, r2):
+        assert abs(r * r + b * r + c) <= 1e-15
+
+
+@pytest.mark.parametrize("pert", [0.0, 0.5 * EPS, 2.0 * EPS])
+def test_quadratic_near_zero_sqrt(pert):
+    c = 4.0 + pert
+    rc, r1, r2 = O.quadratic(1.0, 4.0, c)
+    assert rc == 0
+    assert abs(r1 + 2.0) <= 1e-6 and abs(r1 - r2) <= 1e-6
+    for r in (r1, r2):
+        assert abs(r * r + 4.0 * r + c) <= 1e-15
+
+
+@pytest.mark.parametrize("a,b,c", [(0.0, 0.0, 0.0), (1.0, 2.0, 5.0), (1.0, 4.0, 4.0 + 100.0 * EPS)])
+def test_quadratic_error_conditions(a, b, c):
+    assert O.quadratic(a, b, c)[0] != 0
+
+
+# ---- QSat --------------------------------------------------------------------
+
+def _murphy_koop(T):
+    T = np.asarray(T, dtype=float)
+    water = np.exp(54.842763 - 6763.22 / T - 4.210 * np.log(T) + 0.000367 * T
+                   + np.tanh(0.0415 * (T - 218.8)) * (53.878 - 1331.22 / T - 9.44523 * np.log(T) + 0.014025 * T))
+    ice = np.exp(9.550426 - 5723.265 / T + 3.53068 * np.log(T) - 0.00728332 * T)
+    return np.where(T >= TFRZ, water, ice)
+
+
+def test_qsat_against_murphy_koop_and_q_of_e():
+    T = np.linspace(TFRZ - 40.0, TFRZ + 45.0, 171)
+    p = np.full_like(T, 90000.0)
+    qs, es, qsdT = O.qsat(T, p)
+    assert np.max(np.abs(es / _murphy_koop(T) - 1.0)) < 5e-3
+    assert np.allclose(qs, 0.622 * es / (p - 0.378 * es), rtol=1e-14, atol=0)
+
+
+def test_qsat_derivative_matches_finite_difference():
+    T = np.concatenate([np.linspace(TFRZ - 35.0, TFRZ - 1.0, 40), np.linspace(TFRZ + 1.0, TFRZ + 40.0, 40)])
+    h = 1e-3
+    qp = O.qsat(T + h, 101325.0)[0]
+    qm = O.qsat(T - h, 101325.0)[0]
+    fd = (qp - qm) / (2 * h)
+    qsdT = O.qsat(T, 101325.0)[2]
+    assert np.max(np.abs(qsdT / fd - 1.0)) < 1e-2
+
+
+# ---- FrictionVelocity / MoninObukIni -----------------------------------------
+
+def test_friction_velocity_neutral_log_law():
+    hu, z0, um = 30.0, 0.1, 5.0
+    for obu in (1e12, -1e12):
+        o = O.frictionvelocity(hu, hu, hu, 0.0, z0, z0, z0, obu, um, um)
+        assert o["ustar"][0] == pytest.approx(0.4 * um / math.log(hu / z0), rel=1e-9)
+        assert o["temp1"][0] == pytest.approx(0.4 / math.log(hu / z0), rel=1e-9)
+        assert o["temp2"][0] == o["temp1"][0]
+        assert o["temp12m"][0] == pytest.approx(0.4 / math.log((2.0 + z0) / z0), rel=1e-9)
+        # 10 m winds on the log profile
+        assert o["u10"][0] == pytest.approx(um * math.log(10.0 / z0) / math.log(hu / z0), rel=1e-8)
+        assert o["u10_clm"][0] == pytest.approx(um * math.log((10.0 + z0) / z0) / math.log(hu / z0), rel=1e-8)
+
+
+@pytest.mark.parametrize("zeta_b", [-1.574, -0.465, 0.0, 1.0])
+def test_friction_velocity_continuous_across_regimes(zeta_b):
+    hu, z0, um = 30.0, 0.05, 4.0
+    vals = []
+    for dz in (-1e-9, 1e-9):
+        z = zeta_b + dz
+        if z == 0.0:
+            z = dz
+        obu = hu / z
+        o = O.frictionvelocity(hu, hu, hu, 0.0, z0, z0, z0, obu, um, um)
+        vals.append((o["ustar"][0], o["temp1"][0]))
+    for a, b in zip(*vals):
+        assert abs(a / b - 1.0) < 1e-6
+
+
+def test_monin_obukhov_initialisation():
+    um, obu = O.moninobukini(3.0, 290.0, 0.0, 20.0, 0.1, 0.5)
+    assert um == 3.0 and obu == pytest.approx(20.0 / 0.01, rel=1e-15)
+    um, obu = O.moninobukini(0.05, 290.0, 5.0, 20.0, 0.1, 0.5)     # very stable: clamp at zetamax
+    assert um == 0.1 and obu == pytest.approx(20.0 / 0.5, rel=1e-15)
+    um, obu = O.moninobukini(2.0, 290.0, -3.0, 20.0, 0.1, 0.5)     # unstable: convective velocity 0.5 m/s
+    assert um == pytest.approx(math.sqrt(4.25), rel=1e-15)
+    rib = 9.80616 * 20.0 * -3.0 / (290.0 * um * um)
+    assert obu == pytest.approx(20.0 / max(-100.0, min(rib * math.log(200.0), -0.01)), rel=1e-12)
+
+
+# ---- CanopyFluxes on the synthetic canopy workload -----------------------------
+
+def _run(ncol=192, **cfgkw):
+    st = S.canopy_workload("canopy", ncol=ncol)
+    cfg = O.canopy_config(**cfgkw)
+    pft = O.PftStruct(S.pft_table())
+    rc, bad = O.canopyfluxes(st, cfg, S.canopy_params(), pft)
+    assert rc == 0, bad
+    return st, cfg
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(z0param_method=0), dict(stomatalcond_method=1),
+                                dict(use_biomass_heat_storage=0), dict(use_undercanopy_stability=1,
+                                                                       soil_resis_method=0)])
+def test_canopy_energy_balance_closes(kw):
+    st, _ = _run(**kw)
+    c = st.can
+    for k, v in c.items():
+        assert np.all(np.isfinite(v)), k
+    assert np.max(np.abs(c["canopy_err"])) <= 0.1
+    assert np.all(c["num_iter"] >= 3) and np.all(c["num_iter"] <= 41)
+
+
+def test_ball_berry_identity_at_solution():
+    st, _ = _run(stomatalcond_method=1)
+    c, p = st.can, st.pch
+    pc = p["column"] - 1
+    day = (c["parsha_z"] > 0) & (c["an"] >= 0)
+    assert day.sum() > 100
+    pbot = c["forc_pbot"][pc]
+    cair = c["forc_pco2"][pc]
+    an, gs, gb, hs = c["an"], c["gs_mol"], c["gb_mol"], c["rh_leaf"]
+    cs = np.maximum(cair - 1.4 / gb * an * pbot, 1e-6)
+    gs_bb = c["mbb"] * an * hs / cs * pbot + c["bbb"]
+    rel = np.abs(gs - gs_bb) / gs
+    assert np.max(rel[day]) < 1e-9
+
+
+def test_medlyn_identity_at_solution():
+    st, _ = _run(stomatalcond_method=2)
+    c, p = st.can, st.pch
+    pc = p["column"] - 1
+    tab = S.pft_table()
+    g1 = tab["medlynslope"][c["itype"]]
+    g0 = tab["medlynintercept"][c["itype"]] * 1e-6
+    pbot = c["forc_pbot"][pc]
+    cair = c["forc_pco2"][pc]
+    an, gs, gb, vpd = c["an"], c["gs_mol"] * 1e-6, c["gb_mol"] * 1e-6, c["vpd_can"]
+    day = (c["parsha_z"] > 0) & (c["an"] >= 0)
+    assert day.sum() > 100
+    cs = np.maximum(cair - 1.4 / c["gb_mol"] * an * pbot, 1e-6)
+    t = 1.6 * an / (cs / pbot * 1e6)
+    lhs = (gs - g0 - t) ** 2
+    rhs = g1 ** 2 * t ** 2 * (gb + gs) / (gb * vpd)
+    assert np.max(np.abs(lhs - rhs)[day] / np.maximum(rhs[day], 1e-30)) < 1e-6
+    assert np.all((gs >= g0 + t - 1e-12)[day])        # the larger root
+
+
+def test_root_stress_normalised():
+    st, _ = _run()
+    c = st.can
+    rootr = c["rootr"]
+    pos = c["btran"] > 0
+    assert np.allclose(rootr[:, pos].sum(axis=0), 1.0, rtol=1e-12)
+    assert np.all(rootr >= 0) and np.all(c["btran"] <= 1.0 + 1e-12)
+
+
+def test_root_sink_conserves_transpiration():
+    st, cfg = _run()
+    O.root_sink(st, cfg)
+    c = st.can
+    tot = st.col["qflx_rootsoi"].sum(axis=0)
+    assert np.allclose(tot, c["qflx_tran_veg_col"], rtol=1e-12, atol=1e-22)
+    # the column transpiration is the area-weighted patch sum
+    w = np.bincount(st.pch["column"] - 1, weights=c["qflx_tran_veg"] * st.pch["wtcol"], minlength=st.dims.ncol)
+    assert np.allclose(c["qflx_tran_veg_col"], w, rtol=1e-13, atol=0)
+
+
+def test_forcing_height_below_canopy_is_an_error():
+    st = S.canopy_workload("canopy", ncol=8)
+    st.can["forc_hgt_u"][3] = -50.0
+    cfg = O.canopy_config(z0param_method=0)
+    rc, bad = O.canopyfluxes(st, cfg, S.canopy_params(), O.PftStruct(S.pft_table()))
+    assert rc != 0
+    assert st.pch["column"][bad - 1] == 4
