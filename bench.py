@@ -1,17 +1,20 @@
-"""Benchmark of the column biogeophysics soil step on one MI355X.
+"""Benchmark of the column biogeophysics step on one MI355X.
 
 A "step" is one time step (dtime = 1800 s) of the soil physics over every
 column of the batch: SoilTemperature (snow + soil heat solve with phase
-change), SetSoilWaterFractions, and the moisture-form Richards solve, in ONE
-fused launch (the library's ctsm_soil_step_fused_dev).  The workload is the
-uneven-column batch: 65,536 columns, snow layers uniform 0..12, bedrock index
-uniform 5..20, one patch per column, FP64, synthetic seeded inputs resident
-in HBM.  The same steps are also timed in the plain form (one launch per
-reference routine, intermediates through HBM) for the fused/plain ratio.
+change) with SetSoilWaterFractions, then the moisture-form Richards solve
+(the library's ctsm_soil_step_fused_dev; launch shape chosen by the
+autotuner).  The headline workload is the uneven-column batch: 65,536
+columns, snow layers uniform 0..12, bedrock index uniform 5..20, one patch per
+column, FP64, synthetic seeded inputs resident in HBM.  The same steps are
+also timed in the plain form (one launch per reference routine,
+intermediates through HBM) for the fused/plain ratio.
 
-Prints ONE JSON line (rank 0) with the metric, the roofline of the fused
-kernel (algorithmic bytes per launch / measured launch time vs 8 TB/s) and a
-CPU baseline (the C oracle, single thread, on a bounded sample).
+Prints ONE JSON line with the metric, the roofline of the dominant kernel
+(algorithmic bytes per launch / its measured launch time vs 8 TB/s), a CPU
+baseline (the C oracle on a bounded sample, one column chunk per thread) and,
+in "detail", the full-column step with CanopyFluxes (262,144 columns, 1-16
+patches each).
 """
 import argparse
 import json
@@ -26,54 +29,176 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per
+# dispatch), each scaled by the same counter's reading on a streaming copy of
+# known size; absent until measured for the kernel.
+TRAFFIC = {}
 
-def algorithmic_bytes(st):
-    """Compulsory HBM bytes of one fused soil step, summed over the columns.
 
-    Every array element the step must read is counted once and every element
-    it must write once (FP64 = 8 B, int32 = 4 B), over active extents only:
+def richards_bytes(st):
+    """Compulsory HBM bytes of one Richards launch (k_soilwater*), summed over
+    the columns: per Richards layer (nbedrock of them) 10 FP64 read
+    (h2osoi_liq, dz, z, watsat, bsw, sucsat, hksat, icefrac, eff_porosity,
+    qflx_rootsoi) and 5 written (h2osoi_liq, smp_l, hk_l, qin, qout); per column
+    nbedrock + qflx_infl read and qcharge + num_substeps written:
+    B_W = 120 L + 28 bytes."""
+    L = st.col["nbedrock"].astype(np.int64)
+    return int((120 * L + 28).sum())
+
+
+def soiltemp_bytes(st):
+    """Compulsory HBM bytes of one heat-solve launch (SoilTemperature), summed
+    over columns:
       per column   : 14 FP64 + 4 int32 scalars read, 18 FP64 scalars written;
-      snow layer   : 6 FP64 read (t, liq, ice, dz, z, zi), 7 FP64 + 1 int32 written
-                     (t, liq, ice, thk, fact, bw + imelt) ;
+      snow layer   : 6 FP64 read (t, liq, ice, dz, z, zi), 7 FP64 + 1 int32
+                     written (t, liq, ice, thk, fact, bw + imelt);
       all nlevsno snow slots: qflx_snomelt_lyr, qflx_snofrz_lyr written (2 FP64);
       ground level : 12 FP64 read (t, liq, ice, dz, z, zi, watsat, tkmg, tkdry,
                      csol, bsw, sucsat) + zi(0), 6 FP64 + 1 int32 written
                      (t, liq, ice, thk, fact, eflx_fgr + imelt);
-      nlevsoi level: eff_porosity, icefrac written (2 FP64);
-      Richards layer (nbedrock): hksat, qflx_rootsoi read; smp_l, hk_l, qin,
-                     qout written (2 + 4 FP64);
       patch        : 15 FP64 + 1 int32 read, sabg_lyr over snl+1..1 read,
                      3 FP64 written.
     """
     S = -st.col["snl"].astype(np.int64)
-    L = st.col["nbedrock"].astype(np.int64)
     P = (st.col["patchf"] - st.col["patchi"] + 1).astype(np.int64)
-    NS, NSOI, NG = st.dims.nlevsno, st.dims.nlevsoi, st.dims.nlevgrnd
+    NS, NG = st.dims.nlevsno, st.dims.nlevgrnd
     per_col = (14 * 8 + 4 * 4 + 18 * 8
                + S * (6 * 8 + 7 * 8 + 4)
                + NS * 2 * 8
                + NG * (12 * 8 + 6 * 8 + 4) + 8
-               + NSOI * 2 * 8
-               + L * 6 * 8
                + P * (15 * 8 + 4 + 3 * 8) + P * (S + 1) * 8)
     return int(per_col.sum())
 
 
-def cpu_baseline(st, seconds=15.0):
-    """Time the C oracle (single thread) on a bounded slice of the same workload."""
+_CANOPY_INOUT = {"displa", "z0mv", "t_veg", "t_stem", "snocan", "liqcan", "cgrnds", "cgrndl"}
+
+
+def _canopy_const_fields():
+    import re
+    with open(os.path.join(ROOT, "include", "ctsm_gpu.h")) as f:
+        s = f.read()
+    body = s[s.index("typedef struct ctsm_canopy_state {"):s.index("} ctsm_canopy_state;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    names = set()
+    for line in body.splitlines():
+        line = line.strip()
+        if line.startswith("const"):
+            for nm in line.split(None, 2)[2].rstrip(";").split(","):
+                names.add(nm.strip().lstrip("*").strip())
+    return names
+
+
+def canopy_bytes(st):
+    """Compulsory HBM bytes of one CanopyFluxes launch (k_canopy): every patch
+    field of ctsm_canopy_state counted once if read and once if written
+    (inout both), FP64 8 B / int32 4 B, the three nlevgrnd patch profiles
+    (rootfr read, rresis and rootr written) included; per column its ground
+    state (23 FP64 + snl, 7 nlevgrnd profiles read, eff_porosity and
+    h2osoi_liqvol written) and per gridcell the forcing (9 FP64 + 1 int32)."""
+    from ctsm_amd import abi
+    const = _canopy_const_fields()
+    ins = outs = 0
+    for name, t, kind in abi.CANOPY_FIELDS:
+        if kind not in ("p", "soip"):
+            continue
+        b = (4 if t == "i32" else 8) * (st.dims.nlevgrnd if kind == "soip" else 1)
+        if name in const:
+            ins += b
+        elif name in _CANOPY_INOUT:
+            ins += b
+            outs += b
+        else:
+            outs += b
+    NG = st.dims.nlevgrnd
+    per_col = 23 * 8 + 4 + NG * 7 * 8 + NG * 2 * 8
+    per_grid = 9 * 8 + 4
+    return int(st.dims.npatch * (ins + outs) + st.dims.ncol * (per_col + per_grid))
+
+
+def cpu_baseline(st, seconds=15.0, threads=None):
+    """Time the C oracle on a bounded slice of the same workload, one column
+    chunk per thread (the oracle runs outside the interpreter lock)."""
+    from concurrent.futures import ThreadPoolExecutor
     from oracle import oracle as O
+    threads = threads or min(16, os.cpu_count() or 1)
+    per = 1024
     ncol = st.dims.ncol
-    sample = min(ncol, 4096)
-    sub = st.subset(np.arange(sample))
+    subs = [st.subset((np.arange(per) + i * per) % ncol) for i in range(threads)]
+
+    def work(sub):
+        n = 0
+        t = time.perf_counter()
+        while True:
+            O.soil_step(sub)
+            n += 1
+            if time.perf_counter() - t >= seconds or n >= 48:
+                return n
+
     t0 = time.perf_counter()
-    steps = 0
-    while True:
-        rc, fail = O.soil_step(sub)
-        steps += 1
-        el = time.perf_counter() - t0
-        if el >= seconds or steps >= 48:
-            break
-    return sample * steps / el, f"{sample} columns x {steps} steps of the same workload ({el:.1f} s)"
+    with ThreadPoolExecutor(threads) as ex:
+        steps = list(ex.map(work, subs))
+    el = time.perf_counter() - t0
+    total = per * sum(steps)
+    return total / el, threads, (f"{threads} threads x {per} columns x {min(steps)}-{max(steps)} steps "
+                                 f"of the same workload ({el:.1f} s)")
+
+
+def time_events(ctx, fns, reps):
+    """Mean ms of each callable launched `reps` times, events on the library stream."""
+    acc = [0.0] * len(fns)
+    for _ in range(reps):
+        evs = [ctx.record()]
+        for fn in fns:
+            fn()
+            evs.append(ctx.record())
+        ctx.sync()
+        for i in range(len(fns)):
+            acc[i] += ctx.elapsed_ms(evs[i], evs[i + 1])
+        for e in evs:
+            ctx.destroy_event(e)
+    return [a / reps for a in acc]
+
+
+def full_column(ctx, args):
+    """Full column step with CanopyFluxes (1-16 patches per column): land
+    step throughput and the canopy kernel's own launch time."""
+    from ctsm_amd import autotune as AT, synthetic
+    from ctsm_amd.state import DeviceState
+    st = synthetic.canopy_workload("canopy", ncol=args.canopy_ncol)
+    ctx.set_canopy_params(synthetic.canopy_params(), synthetic.pft_table())
+    ds = DeviceState(st, ctx)
+    filt = ctx.device_array(AT.column_order(st.col["snl"], st.col["nbedrock"]))
+    structs = (*ds.structs(), ds.canopy_struct())
+    for _ in range(args.warmup):
+        ctx.land_step_dev(ds, filterc_dev=filt, structs=structs)
+    ctx.sync()
+    e0 = ctx.record()
+    t0 = time.perf_counter()
+    for _ in range(args.canopy_steps):
+        ctx.land_step_dev(ds, filterc_dev=filt, structs=structs)
+    e1 = ctx.record()
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    step_ms = ctx.elapsed_ms(e0, e1) / args.canopy_steps
+    ctx.destroy_event(e0)
+    ctx.destroy_event(e1)
+    can_ms, = time_events(ctx, [lambda: ctx.canopyfluxes_dev(ds)], 3)
+    b = canopy_bytes(st)
+    nit = np.empty(st.dims.npatch)
+    ctx.d2h(nit, ds.can["num_iter"])
+    out = {"workload": f"{st.dims.ncol:,} columns, {st.dims.npatch:,} patches (1-16 per column): CanopyFluxes, "
+                       "transpiration sink, SoilTemperature, SoilWater; no snow, 20 layers",
+           "column_steps_per_s": st.dims.ncol * args.canopy_steps / wall,
+           "patch_steps_per_s": st.dims.npatch * args.canopy_steps / wall,
+           "step_ms": step_ms, "steps": args.canopy_steps,
+           "canopy_kernel_ms": can_ms,
+           "mean_canopy_iterations": float(nit.mean()),
+           "canopy_roofline": {"kernel": "k_canopy", "alg_bytes_per_launch": b,
+                               "achieved_GBps": b / (can_ms * 1e-3) / 1e9,
+                               "frac": b / (can_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+    ctx.free(filt)
+    ds.free()
+    return out
 
 
 def main():
@@ -91,8 +216,12 @@ def main():
     ap.add_argument("--single-kernel", action="store_true", help="force the one-kernel fused step")
     ap.add_argument("--order", choices=["bins", "substeps"], default="substeps",
                     help="column order: (bedrock, snow) bins, or also by the sub-steps of the warm-up step")
-    ap.add_argument("--breakdown", action="store_true",
-                    help="also time SoilTemperature and SoilWater launched separately")
+    ap.add_argument("--kernel-reps", type=int, default=8,
+                    help="launches per kernel in the per-kernel timing (roofline) phase")
+    ap.add_argument("--canopy-ncol", type=int, default=262144,
+                    help="columns of the full-column (CanopyFluxes) measurement; 0 skips it")
+    ap.add_argument("--canopy-steps", type=int, default=6)
+    ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="run a streaming copy kernel of this size (MB) after the steps, "
                          "to calibrate the HBM byte counters of a profiler run")
@@ -163,21 +292,15 @@ def main():
         res["plain_column_steps_per_s"] = args.ncol * args.steps / wall_p
         res["fused_over_plain"] = value / res["plain_column_steps_per_s"]
 
-    if args.breakdown:
-        ds = DeviceState(st, ctx)
-        for name, fn in (("soiltemperature_fused_kernel_ms", lambda: ctx.soiltemperature_dev(ds, filt)),
-                         ("soilwater_fused_kernel_ms", lambda: ctx.soilwater_dev(ds, filt))):
-            fn()
-            ctx.sync()
-            e0 = ctx.record()
-            for _ in range(3):
-                fn()
-            e1 = ctx.record()
-            ctx.sync()
-            res[name] = ctx.elapsed_ms(e0, e1) / 3
-            ctx.destroy_event(e0)
-            ctx.destroy_event(e1)
-        ds.free()
+    # per-kernel launch times on the same state (events on the library stream):
+    # heat solve, water fractions, Richards -- the step's work launched separately
+    ds = DeviceState(st, ctx)
+    t_heat, t_frac, t_rich = time_events(ctx, [lambda: ctx.soiltemperature_dev(ds, filt),
+                                               lambda: ctx.soilwater_fractions_dev(ds, filt),
+                                               lambda: ctx.soilwater_dev(ds, filt)], args.kernel_reps)
+    ds.free()
+    res["kernel_ms"] = {"soiltemp (k_soiltemp)": t_heat, "fractions (k_soilwater_fractions)": t_frac,
+                        "richards (k_soilwater_g)": t_rich}
     if args.calib_copy_mb > 0:
         nb = args.calib_copy_mb << 20
         a, b = ctx.alloc(nb), ctx.alloc(nb)
@@ -191,8 +314,17 @@ def main():
         ctx.free(a)
         ctx.free(b)
 
-    bytes_alg = algorithmic_bytes(st)
-    achieved = bytes_alg / (step_ms * 1e-3) / 1e9
+    b_rich, b_heat = richards_bytes(st), soiltemp_bytes(st)
+    lanes = ctx.get_tuning()["richards_lanes"]
+    kern = {"richards": (t_rich, b_rich, "k_soilwater_g<%d>" % lanes if lanes > 1 else "k_soilwater"),
+            "soiltemp": (t_heat, b_heat, "k_soiltemp")}
+    dom = max(kern, key=lambda k: kern[k][0])
+    kms, kbytes, kname = kern[dom]
+    achieved = kbytes / (kms * 1e-3) / 1e9
+    res["roofline_other"] = {k: {"kernel": v[2], "ms": v[0], "alg_bytes_per_launch": v[1],
+                                 "achieved_GBps": v[1] / (v[0] * 1e-3) / 1e9} for k, v in kern.items() if k != dom}
+    if args.canopy_ncol > 0:
+        res["full_column_step"] = full_column(ctx, args)
     out = {
         "metric": "column-steps/sec (FP64, 1 GPU)",
         "value": value,
@@ -210,13 +342,13 @@ def main():
                                "SoilTemperature + SoilWater, dtime 1800 s, 48-step day",
                    "ncol": args.ncol, "binned": not args.no_sort},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_soil_step", "alg_bytes_per_launch": bytes_alg},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get(kname),
+                     "kernel": kname, "launch_ms": kms, "alg_bytes_per_launch": kbytes},
         "detail": res,
     }
     if not args.no_cpu:
-        v, sample = cpu_baseline(st, args.cpu_seconds)
-        out["cpu_baseline"] = {"value": v, "unit": "column-steps/s", "cores": 1, "kind": "port",
+        v, threads, sample = cpu_baseline(st, args.cpu_seconds, args.cpu_threads or None)
+        out["cpu_baseline"] = {"value": v, "unit": "column-steps/s", "cores": threads, "kind": "port",
                                "sample": sample}
     print(json.dumps(out), flush=True)
     ctx.close()

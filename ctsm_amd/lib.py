@@ -280,6 +280,13 @@ class Context:
         self._check(self.L.ctsm_soilwater_dev(self.h, C.byref(ds.layout()), C.byref(cs), n, fp, dtime,
                                               int(plain)), "SoilWater")
 
+    def soilwater_fractions_dev(self, ds, filter_dev=None, n=None):
+        cs, _ = ds.structs()
+        n = ds.dims.ncol if n is None else n
+        fp = C.c_void_p(filter_dev) if filter_dev is not None else None
+        self._check(self.L.ctsm_soilwater_fractions_dev(self.h, C.byref(ds.layout()), C.byref(cs), n, fp),
+                    "SetSoilWaterFractions")
+
     def soiltemperature_dev(self, ds, filter_dev=None, n=None, dtime=1800.0, plain=False):
         cs, ps = ds.structs()
         n = ds.dims.ncol if n is None else n
