@@ -1250,22 +1250,29 @@ __global__ void __launch_bounds__(256) k_root_sink(CView V, ctsm_col_state col, 
         report_error(err, c, CTSM_E_ARG);
         return;
     }
+    // the p2c sum and the sink's normaliser are the same sum in the same order
     double tcol = 0.0;
-    for (int p = pi; p <= pf; ++p)
-        if (S.active[V.p1(p)]) tcol = tcol + S.qflx_tran_veg[V.p1(p)] * S.wtcol[V.p1(p)];
+    double r[MAX_NLEVSOI];
+#pragma unroll
+    for (int j = 0; j < MAX_NLEVSOI; ++j) r[j] = 0.0;
+    for (int p = pi; p <= pf; ++p) {
+        const long q = V.p1(p);
+        if (!S.active[q]) continue;
+        const double qt = S.qflx_tran_veg[q], w = S.wtcol[q];
+        tcol = tcol + qt * w;
+        // per level the patches are summed in ascending order, as the reference's j-outer loop does
+#pragma unroll
+        for (int j = 1; j <= MAX_NLEVSOI; ++j)
+            if (j <= V.nlevsoi) r[j - 1] = r[j - 1] + S.rootr[V.soip(p, j)] * qt * w;
+    }
     S.qflx_tran_veg_col[cc] = tcol;
-    double temp = 0.0;
-    for (int p = pi; p <= pf; ++p)
-        if (S.active[V.p1(p)]) temp = temp + S.qflx_tran_veg[V.p1(p)] * S.wtcol[V.p1(p)];
-    for (int j = 1; j <= V.nlevsoi; ++j) {
-        double r = 0.0;
-        for (int p = pi; p <= pf; ++p) {
-            const long q = V.p1(p);
-            if (S.active[q]) r = r + S.rootr[V.soip(p, j)] * S.qflx_tran_veg[q] * S.wtcol[q];
-        }
-        if (temp != 0.0) r = r / temp;
-        S.rootr_col[V.soi(c, j)] = r;
-        S.qflx_rootsoi_col[V.soi(c, j)] = r * tcol;
+#pragma unroll
+    for (int j = 1; j <= MAX_NLEVSOI; ++j) {
+        if (j > V.nlevsoi) continue;
+        double rc = r[j - 1];
+        if (tcol != 0.0) rc = rc / tcol;
+        S.rootr_col[V.soi(c, j)] = rc;
+        S.qflx_rootsoi_col[V.soi(c, j)] = rc * tcol;
     }
 }
 
