@@ -217,3 +217,24 @@ def test_forcing_height_below_canopy_is_an_error():
     rc, bad = O.canopyfluxes(st, cfg, S.canopy_params(), O.PftStruct(S.pft_table()))
     assert rc != 0
     assert st.pch["column"][bad - 1] == 4
+
+
+def test_land_step_invariant_under_column_permutation():
+    """Storing the batch in another column order (patches renumbered with their
+    columns) changes no result: the property the binned device layout relies on."""
+    st = S.canopy_workload("canopy_uneven", ncol=96)
+    cfg = O.canopy_config()
+    pft = O.PftStruct(S.pft_table())
+    perm = np.random.default_rng(7).permutation(96)
+    sp = st.subset(perm)
+    for s_ in (st, sp):
+        for _ in range(2):
+            rc, bad = O.land_step(s_, cfg, S.canopy_params(), pft)
+            assert rc == 0, bad
+    back = st.subset(perm)
+    for grp in ("col", "pch", "can"):
+        a, b = getattr(back, grp), getattr(sp, grp)
+        for k in b:
+            if k in ("column", "gridcell", "patchi", "patchf"):
+                continue
+            np.testing.assert_array_equal(a[k], b[k], err_msg=f"{grp}.{k}")
