@@ -143,6 +143,28 @@ def cpu_baseline(st, seconds=15.0, threads=None):
                                  f"of the same workload ({el:.1f} s)")
 
 
+def served_ensemble(ctx, args):
+    """Perturbed-parameter members arriving as a Poisson stream, run by the
+    continuous-batching server in one resident pool; latency = completion -
+    arrival (wall clock)."""
+    from ctsm_amd import serve, synthetic
+    ctx.set_canopy_params(synthetic.canopy_params(), synthetic.pft_table())
+    reqs = serve.make_requests(args.serve_requests, args.serve_rate, seed=0xC75D0004,
+                               day_scale=args.serve_day_scale)
+    srv, done, wall = serve.serve(ctx, reqs, args.serve_pool_cols, args.serve_pool_cols * 17)
+    lat = np.array([q.done - q.arrival for q in done])
+    out = {"workload": f"{len(reqs)} members, Poisson {args.serve_rate}/s, 1-20,000 columns (log-uniform) with "
+                       f"1-16 patches, 30 parameters x U(0.5,1.5), 1-365 days (log-uniform) x 48 steps x "
+                       f"{args.serve_day_scale}; land step with CanopyFluxes",
+           "pool_columns": args.serve_pool_cols, "pool_bytes_per_column": srv.bytes_per_column(),
+           "completed": len(done), "wall_s": wall,
+           "latency_p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
+           "latency_p99_s": float(np.percentile(lat, 99)) if len(lat) else None,
+           "column_steps_per_s": srv.column_steps / wall}
+    srv.close()
+    return out
+
+
 def time_events(ctx, fns, reps):
     """Mean ms of each callable launched `reps` times, events on the library stream."""
     acc = [0.0] * len(fns)
@@ -232,6 +254,12 @@ def main():
                     help="columns of the full-column (CanopyFluxes) measurement; 0 skips it")
     ap.add_argument("--canopy-steps", type=int, default=6)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--serve-requests", type=int, default=80,
+                    help="served-ensemble measurement: number of requests (0 skips it)")
+    ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-day-scale", type=float, default=0.1,
+                    help="run lengths are log-uniform 1-365 days x 48 steps x this factor")
+    ap.add_argument("--serve-pool-cols", type=int, default=131072)
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="run a streaming copy kernel of this size (MB) after the steps, "
                          "to calibrate the HBM byte counters of a profiler run")
@@ -352,6 +380,8 @@ def main():
                                  "achieved_GBps": v[1] / (v[0] * 1e-3) / 1e9} for k, v in kern.items() if k != dom}
     if args.canopy_ncol > 0:
         res["full_column_step"] = full_column(ctx, args)
+    if args.serve_requests > 0:
+        res["served_ensemble"] = served_ensemble(ctx, args)
     out = {
         "metric": "column-steps/sec (FP64, 1 GPU)",
         "value": value,

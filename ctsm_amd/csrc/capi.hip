@@ -823,6 +823,26 @@ int ctsm_memcpy(ctsm_ctx *ctx, void *dst, const void *src, uint64_t bytes, int32
     return e == hipSuccess ? CTSM_OK : hip_fail(ctx, e);
 }
 
+int ctsm_memcpy2d(ctsm_ctx *ctx, void *dst, uint64_t dpitch, const void *src, uint64_t spitch, uint64_t width,
+                  uint64_t height, int32_t kind)
+{
+    if (!ctx || (!dst && width * height) || (!src && width * height) || width > dpitch || width > spitch)
+        return CTSM_E_ARG;
+    if (width == 0 || height == 0) return CTSM_OK;
+    hipMemcpyKind k;
+    switch (kind) {
+    case 1: k = hipMemcpyHostToDevice; break;
+    case 2: k = hipMemcpyDeviceToHost; break;
+    case 3: k = hipMemcpyDeviceToDevice; break;
+    default: return CTSM_E_ARG;
+    }
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = hipMemcpy2DAsync(dst, dpitch, src, spitch, width, height, k, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? CTSM_OK : hip_fail(ctx, e);
+}
+
 int ctsm_dev_copy(ctsm_ctx *ctx, void *dst, const void *src, uint64_t bytes)
 {
     if (!ctx || (!dst && bytes) || (!src && bytes)) return CTSM_E_ARG;

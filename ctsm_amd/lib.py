@@ -30,7 +30,7 @@ EXPORTS = [
     "ctsm_soil_step_fused_dev", "ctsm_dev_alloc", "ctsm_dev_free", "ctsm_memcpy", "ctsm_dev_copy",
     "ctsm_event_record", "ctsm_event_elapsed", "ctsm_event_destroy", "ctsm_sync", "ctsm_stream",
     "ctsm_set_canopy_params", "ctsm_canopyfluxes", "ctsm_root_sink", "ctsm_canopyfluxes_dev",
-    "ctsm_root_sink_dev", "ctsm_land_step_dev",
+    "ctsm_root_sink_dev", "ctsm_land_step_dev", "ctsm_memcpy2d",
 ]
 
 
@@ -72,6 +72,8 @@ def load():
     L.ctsm_dev_free.argtypes = [C.c_void_p, C.c_void_p]
     L.ctsm_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_int32]
     L.ctsm_dev_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]
+    L.ctsm_memcpy2d.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64,
+                                C.c_int32]
     L.ctsm_event_record.argtypes = [C.c_void_p, P(C.c_void_p)]
     L.ctsm_event_elapsed.argtypes = [C.c_void_p, C.c_void_p, P(C.c_float)]
     L.ctsm_event_destroy.argtypes = [C.c_void_p]
@@ -227,6 +229,21 @@ class Context:
         assert arr.flags.c_contiguous
         self._check(self.L.ctsm_memcpy(self.h, arr.ctypes.data_as(C.c_void_p), C.c_void_p(dptr), arr.nbytes, 2),
                     "ctsm_memcpy")
+
+    def h2d_2d(self, dptr, dpitch, arr):
+        """Copy a 2-D numpy array (rows = levels) into a device region with row pitch `dpitch` bytes."""
+        a = np.ascontiguousarray(arr)
+        rows = a.shape[0] if a.ndim == 2 else 1
+        w = a.nbytes // rows
+        self._check(self.L.ctsm_memcpy2d(self.h, C.c_void_p(dptr), int(dpitch), a.ctypes.data_as(C.c_void_p), w, w,
+                                         rows, 1), "ctsm_memcpy2d")
+
+    def d2h_2d(self, arr, dptr, spitch):
+        assert arr.flags.c_contiguous
+        rows = arr.shape[0] if arr.ndim == 2 else 1
+        w = arr.nbytes // rows
+        self._check(self.L.ctsm_memcpy2d(self.h, arr.ctypes.data_as(C.c_void_p), w, C.c_void_p(dptr), int(spitch), w,
+                                         rows, 2), "ctsm_memcpy2d")
 
     def copy(self, dst, src, nbytes):
         """Kernel copy device -> device on the library stream (enqueues)."""

@@ -500,6 +500,10 @@ int ctsm_dev_alloc(ctsm_ctx *ctx, uint64_t bytes, void **out);
 int ctsm_dev_free(ctsm_ctx *ctx, void *ptr);
 /* kind: 1 = host->device, 2 = device->host, 3 = device->device.  Synchronous. */
 int ctsm_memcpy(ctsm_ctx *ctx, void *dst, const void *src, uint64_t bytes, int32_t kind);
+/* 2-D copy (rows of `width` bytes, `height` rows, pitches in bytes), e.g. one
+ * member's column range of a level-major field; kind as ctsm_memcpy.  Synchronous. */
+int ctsm_memcpy2d(ctsm_ctx *ctx, void *dst, uint64_t dpitch, const void *src, uint64_t spitch,
+                  uint64_t width, uint64_t height, int32_t kind);
 /* Device-to-device copy by a streaming kernel on the context's stream
  * (enqueues only; used for state snapshots). */
 int ctsm_dev_copy(ctsm_ctx *ctx, void *dst, const void *src, uint64_t bytes);
