@@ -10,21 +10,22 @@
  * the GPU and the CPU check must agree on the elementary functions.
  *
  * Method (textbook argument reduction with double-double intermediates):
- *   log(x): x = 2^e m, m in [sqrt(1/2), sqrt(2)); f = (m-1)/(m+1) in
- *           double-double; log m = 2 atanh f = 2f + 2f^3/3 + f^5 sum_k
- *           2 f^(2k)/(2k+5), the first two terms in double-double, the tail
- *           (|f|^5 <= 1.5e-4) by Horner in double.  Relative error ~2^-64.
- *   exp(z): z = k ln2 + r (Cody-Waite split of ln2 into 32 + 53 bits),
- *           |r| <= ln2/2, e^r - 1 by a degree-14 Taylor polynomial, scaled by
- *           2^k.  The low word of a double-double z is folded into r.
+ *   log(x): x = 2^e m, m in [sqrt(1/2), sqrt(2)); m = F (1 + u) with F the
+ *           nearest k/128 (table of log F in double-double), u in
+ *           double-double from an fma remainder (no division), log(1+u) by
+ *           its series to u^9.  Relative error ~2^-64.
+ *   exp(z): z = (64 m + j) ln2/64 + r (Cody-Waite split of ln2/64 into
+ *           32 + 53 bits), |r| <= ln2/128, 2^(j/64) from a double-double
+ *           table, e^r - 1 by its Taylor polynomial to r^6, scaled by 2^m.
+ *           The low word of a double-double z is folded into r.
  *   pow(x,y) = exp(y * log x) with y*log x formed in double-double.
  *   atan(x):  table of atan(k/8) plus the odd series of a reduced argument
  *           (see cm_atan).
  * Results are within about 1 ulp of the true value (checked against the host
  * libm in tests/test_math.py); they are not claimed correctly rounded.
- * Constants are ln 2, 1/ln 2, 1/ln 10, 2/3, pi/2, atan(k/8), 1/n! and
- * 1/(2k+1) to binary64 (computed at 60 digits with Python's decimal and
- * mpmath modules).
+ * Constants and tables (ln 2, 1/ln 10, log(k/128), 2^(j/64), pi/2,
+ * atan(k/8), 1/n, 1/n!) are the binary64 / double-double roundings of
+ * 60-digit values computed with Python's mpmath.
  */
 #ifndef CTSM_MATH_H
 #define CTSM_MATH_H
@@ -43,11 +44,8 @@
 /* ln 2 = CM_LN2_HI32 (32 significant bits, so k*CM_LN2_HI32 is exact) + CM_LN2_LO32 */
 #define CM_LN2_HI32 0x1.62e42fee00000p-1
 #define CM_LN2_LO32 0x1.a39ef35793c76p-33
-#define CM_INV_LN2 0x1.71547652b82fep+0
 #define CM_INV_LN10_H 0x1.bcb7b1526e50ep-2
 #define CM_INV_LN10_L 0x1.95355baaafad3p-57
-#define CM_TWO_THIRDS_H 0x1.5555555555555p-1
-#define CM_TWO_THIRDS_L 0x1.5555555555555p-55
 #define CM_SQRT_HALF 0x1.6a09e667f3bcdp-1
 
 /* error-free transformations */
@@ -81,87 +79,234 @@ CM_FN void cm_dd_mul(double ah, double al, double bh, double bl, double *h, doub
     cm_fast_two_sum(p, e, h, l);
 }
 
-/* natural log of x > 0 (finite, normal or subnormal) as a double-double */
+/* natural log of x > 0 (finite, normal or subnormal) as a double-double.
+ * x = 2^e m, m in [sqrt(1/2), sqrt(2)); F = k/128 nearest m (table of log F
+ * in double-double and 1/F), u = (m - F)/F in double-double without a
+ * division (f = m - F is exact, the remainder of f * (1/F) comes from an fma),
+ * log(1 + u) = u - u^2/2 + ... + u^9/9 (|u| <= 2^-7.5, truncation < 2^-66 u). */
+#define CM_LOGT_K0 91
 CM_FN void cm_log_dd(double x, double *lh, double *ll)
 {
+    static const double CM_LOGT[91][3] = {
+        {-0x1.5d5bddf595f30p-2, 0x1.6541148cbb8a2p-56, 0x1.6816816816817p+0},
+        {-0x1.522ae0738a3d8p-2, 0x1.8f7e9b38a6979p-57, 0x1.642c8590b2164p+0},
+        {-0x1.4718dc271c41bp-2, -0x1.8fb4c14c56eefp-60, 0x1.6058160581606p+0},
+        {-0x1.3c25277333184p-2, 0x1.2ad27e50a8ec6p-56, 0x1.5c9882b931057p+0},
+        {-0x1.314f1e1d35ce4p-2, 0x1.3d69909e5c3dcp-56, 0x1.58ed2308158edp+0},
+        {-0x1.269621134db92p-2, -0x1.e0efadd9db02bp-56, 0x1.5555555555555p+0},
+        {-0x1.1bf99635a6b95p-2, 0x1.12aeb84249223p-57, 0x1.51d07eae2f815p+0},
+        {-0x1.1178e8227e47cp-2, 0x1.0e63a5f01c691p-57, 0x1.4e5e0a72f0539p+0},
+        {-0x1.07138604d5862p-2, -0x1.cdb16ed4e9138p-56, 0x1.4afd6a052bf5bp+0},
+        {-0x1.f991c6cb3b379p-3, -0x1.f665066f980a2p-57, 0x1.47ae147ae147bp+0},
+        {-0x1.e530effe71012p-3, -0x1.2276041f43042p-59, 0x1.446f86562d9fbp+0},
+        {-0x1.d1037f2655e7bp-3, -0x1.60629242471a2p-57, 0x1.4141414141414p+0},
+        {-0x1.bd087383bd8adp-3, -0x1.dd355f6a516d7p-60, 0x1.3e22cbce4a902p+0},
+        {-0x1.a93ed3c8ad9e3p-3, -0x1.bcafa9de97203p-57, 0x1.3b13b13b13b14p+0},
+        {-0x1.95a5adcf7017fp-3, -0x1.142c507fb7a3dp-58, 0x1.3813813813814p+0},
+        {-0x1.823c16551a3c2p-3, 0x1.1232ce70be781p-57, 0x1.3521cfb2b78c1p+0},
+        {-0x1.6f0128b756abcp-3, 0x1.8de59c21e166cp-57, 0x1.323e34a2b10bfp+0},
+        {-0x1.5bf406b543db2p-3, 0x1.1f5b44c0df7e7p-61, 0x1.2f684bda12f68p+0},
+        {-0x1.4913d8333b561p-3, 0x1.0d5604930f135p-58, 0x1.2c9fb4d812ca0p+0},
+        {-0x1.365fcb0159016p-3, -0x1.7d411a5b944adp-58, 0x1.29e4129e4129ep+0},
+        {-0x1.23d712a49c202p-3, 0x1.6e38161051d69p-57, 0x1.27350b8812735p+0},
+        {-0x1.1178e8227e47cp-3, 0x1.0e63a5f01c691p-58, 0x1.2492492492492p+0},
+        {-0x1.fe89139dbd566p-4, 0x1.ac9f4215f9393p-58, 0x1.21fb78121fb78p+0},
+        {-0x1.da727638446a2p-4, -0x1.401fa71733019p-58, 0x1.1f7047dc11f70p+0},
+        {-0x1.b6ac88dad5b1cp-4, 0x1.0057eed1ca59fp-59, 0x1.1cf06ada2811dp+0},
+        {-0x1.9335e5d594989p-4, 0x1.478a85704ccb7p-58, 0x1.1a7b9611a7b96p+0},
+        {-0x1.700d30aeac0e1p-4, 0x1.72566212cdd05p-61, 0x1.1811811811812p+0},
+        {-0x1.4d3115d207eacp-4, -0x1.769f42c7842ccp-58, 0x1.15b1e5f75270dp+0},
+        {-0x1.2aa04a44717a5p-4, 0x1.d15d38d2fa3f7p-58, 0x1.135c81135c811p+0},
+        {-0x1.08598b59e3a07p-4, 0x1.dd7009902bf32p-58, 0x1.1111111111111p+0},
+        {-0x1.ccb73cdddb2ccp-5, 0x1.e48fb0500efd4p-59, 0x1.0ecf56be69c90p+0},
+        {-0x1.894aa149fb343p-5, -0x1.a8be97660a23dp-60, 0x1.0c9714fbcda3bp+0},
+        {-0x1.466aed42de3eap-5, 0x1.cdd6f7f4a137ep-59, 0x1.0a6810a6810a7p+0},
+        {-0x1.0415d89e74444p-5, -0x1.c05cf1d753622p-59, 0x1.0842108421084p+0},
+        {-0x1.8492528c8cabfp-6, 0x1.d192d0619fa67p-60, 0x1.0624dd2f1a9fcp+0},
+        {-0x1.0205658935847p-6, -0x1.27c8e8416e71fp-60, 0x1.0410410410410p+0},
+        {-0x1.010157588de71p-7, -0x1.46662d417ced0p-62, 0x1.0204081020408p+0},
+        {0.0, 0.0, 0x1.0000000000000p+0},
+        {0x1.fe02a6b106789p-8, -0x1.e44b7e3711ebfp-67, 0x1.fc07f01fc07f0p-1},
+        {0x1.fc0a8b0fc03e4p-7, -0x1.83092c59642a1p-62, 0x1.f81f81f81f820p-1},
+        {0x1.7b91b07d5b11bp-6, -0x1.5b602ace3a510p-60, 0x1.f44659e4a4271p-1},
+        {0x1.f829b0e783300p-6, 0x1.33e3f04f1ef23p-60, 0x1.f07c1f07c1f08p-1},
+        {0x1.39e87b9febd60p-5, -0x1.5bfa937f551bbp-59, 0x1.ecc07b301ecc0p-1},
+        {0x1.77458f632dcfcp-5, 0x1.18d3ca87b9296p-59, 0x1.e9131abf0b767p-1},
+        {0x1.b42dd711971bfp-5, -0x1.eb9759c130499p-60, 0x1.e573ac901e574p-1},
+        {0x1.f0a30c01162a6p-5, 0x1.85f325c5bbacdp-59, 0x1.e1e1e1e1e1e1ep-1},
+        {0x1.16536eea37ae1p-4, -0x1.79da3e8c22cdap-60, 0x1.de5d6e3f8868ap-1},
+        {0x1.341d7961bd1d1p-4, -0x1.b599f227becbbp-58, 0x1.dae6076b981dbp-1},
+        {0x1.51b073f06183fp-4, 0x1.a49e39a1a8be4p-58, 0x1.d77b654b82c34p-1},
+        {0x1.6f0d28ae56b4cp-4, -0x1.906d99184b992p-58, 0x1.d41d41d41d41dp-1},
+        {0x1.8c345d6319b21p-4, -0x1.4a697ab3424a9p-61, 0x1.d0cb58f6ec074p-1},
+        {0x1.a926d3a4ad563p-4, 0x1.942f48aa70ea9p-58, 0x1.cd85689039b0bp-1},
+        {0x1.c5e548f5bc743p-4, 0x1.5d617ef8161b1p-60, 0x1.ca4b3055ee191p-1},
+        {0x1.e27076e2af2e6p-4, -0x1.61578001e0162p-60, 0x1.c71c71c71c71cp-1},
+        {0x1.fec9131dbeabbp-4, -0x1.5746b9981b36cp-58, 0x1.c3f8f01c3f8f0p-1},
+        {0x1.0d77e7cd08e59p-3, 0x1.9a5dc5e9030acp-57, 0x1.c0e070381c0e0p-1},
+        {0x1.1b72ad52f67a0p-3, 0x1.483023472cd74p-58, 0x1.bdd2b899406f7p-1},
+        {0x1.29552f81ff523p-3, 0x1.301771c407dbfp-57, 0x1.bacf914c1bad0p-1},
+        {0x1.371fc201e8f74p-3, 0x1.de6cb62af18a0p-58, 0x1.b7d6c3dda338bp-1},
+        {0x1.44d2b6ccb7d1ep-3, 0x1.9f4f6543e1f88p-57, 0x1.b4e81b4e81b4fp-1},
+        {0x1.526e5e3a1b438p-3, -0x1.746ff8a470d3ap-57, 0x1.b2036406c80d9p-1},
+        {0x1.5ff3070a793d4p-3, -0x1.bc60efafc6f6ep-58, 0x1.af286bca1af28p-1},
+        {0x1.6d60fe719d21dp-3, -0x1.caae268ecd179p-57, 0x1.ac5701ac5701bp-1},
+        {0x1.7ab890210d909p-3, 0x1.be36b2d6a0608p-59, 0x1.a98ef606a63bep-1},
+        {0x1.87fa06520c911p-3, -0x1.bf7fdbfa08d9ap-57, 0x1.a6d01a6d01a6dp-1},
+        {0x1.9525a9cf456b4p-3, 0x1.d904c1d4e2e26p-57, 0x1.a41a41a41a41ap-1},
+        {0x1.a23bc1fe2b563p-3, 0x1.93711b07a998cp-59, 0x1.a16d3f97a4b02p-1},
+        {0x1.af3c94e80bff3p-3, -0x1.398cff3641985p-58, 0x1.9ec8e951033d9p-1},
+        {0x1.bc286742d8cd6p-3, 0x1.4fce744870f55p-58, 0x1.9c2d14ee4a102p-1},
+        {0x1.c8ff7c79a9a22p-3, -0x1.4f689f8434012p-57, 0x1.999999999999ap-1},
+        {0x1.d5c216b4fbb91p-3, 0x1.6e443597e4d40p-57, 0x1.970e4f80cb872p-1},
+        {0x1.e27076e2af2e6p-3, -0x1.61578001e0162p-59, 0x1.948b0fcd6e9e0p-1},
+        {0x1.ef0adcbdc5936p-3, 0x1.48637950dc20dp-57, 0x1.920fb49d0e229p-1},
+        {0x1.fb9186d5e3e2bp-3, -0x1.caaae64f21acbp-57, 0x1.8f9c18f9c18fap-1},
+        {0x1.0402594b4d041p-2, -0x1.28ec217a5022dp-57, 0x1.8d3018d3018d3p-1},
+        {0x1.0a324e27390e3p-2, 0x1.7dcfde8061c03p-56, 0x1.8acb90f6bf3aap-1},
+        {0x1.1058bf9ae4ad5p-2, 0x1.89fa0ab4cb31dp-58, 0x1.886e5f0abb04ap-1},
+        {0x1.1675cababa60ep-2, 0x1.ce63eab883717p-61, 0x1.8618618618618p-1},
+        {0x1.1c898c16999fbp-2, -0x1.0e5c62aff1c44p-60, 0x1.83c977ab2beddp-1},
+        {0x1.22941fbcf7966p-2, -0x1.76f5eb09628afp-56, 0x1.8181818181818p-1},
+        {0x1.2895a13de86a3p-2, 0x1.7ad24c13f040ep-56, 0x1.7f405fd017f40p-1},
+        {0x1.2e8e2bae11d31p-2, -0x1.8f4cdb95ebdf9p-56, 0x1.7d05f417d05f4p-1},
+        {0x1.347dd9a987d55p-2, -0x1.4dd4c580919f8p-57, 0x1.7ad2208e0ecc3p-1},
+        {0x1.3a64c556945eap-2, -0x1.c68651945f97cp-57, 0x1.78a4c8178a4c8p-1},
+        {0x1.404308686a7e4p-2, -0x1.0bcfb6082ce6dp-56, 0x1.767dce434a9b1p-1},
+        {0x1.4618bc21c5ec2p-2, 0x1.f42decdeccf1dp-56, 0x1.745d1745d1746p-1},
+        {0x1.4be5f957778a1p-2, -0x1.259b35b04813dp-57, 0x1.724287f46debcp-1},
+        {0x1.51aad872df82dp-2, 0x1.3927ac19f55e3p-59, 0x1.702e05c0b8170p-1},
+        {0x1.5767717455a6cp-2, 0x1.526adb283660cp-56, 0x1.6e1f76b4337c7p-1},
+        {0x1.5d1bdbf5809cap-2, 0x1.4236383dc7fe1p-56, 0x1.6c16c16c16c17p-1},
+        {0x1.62c82f2b9c795p-2, 0x1.7b7af915300e5p-57, 0x1.6a13cd1537290p-1},
+    };
     int e;
     double m = frexp(x, &e);                 /* m in [0.5, 1) */
     if (m < CM_SQRT_HALF) { m = m * 2.0; e = e - 1; }   /* m in [sqrt(1/2), sqrt(2)) */
-    const double num = m - 1.0;              /* exact (Sterbenz) */
-    double dh, dl;
-    cm_two_sum(m, 1.0, &dh, &dl);            /* m + 1 exactly */
-    /* f = num / (dh + dl) in double-double */
-    const double q1 = num / dh;
-    double r = fma(-q1, dh, num);            /* exact remainder */
-    r = r - q1 * dl;
-    const double q2 = r / dh;
-    double fh, fl;
-    cm_fast_two_sum(q1, q2, &fh, &fl);
-    /* s = f^2, t = f^3 in double-double */
-    double sh, sl, th, tl;
-    cm_dd_mul(fh, fl, fh, fl, &sh, &sl);
-    cm_dd_mul(fh, fl, sh, sl, &th, &tl);
-    /* u = (2/3) f^3 */
-    double uh, ul;
-    cm_dd_mul(CM_TWO_THIRDS_H, CM_TWO_THIRDS_L, th, tl, &uh, &ul);
-    /* tail: f^5 * sum_{k>=0} 2 s^k / (2k+5), |s| <= 0.0295 */
-    const double s = sh;
-    double p = 0x1.0842108421084p-4;         /* 2/31 */
-    p = 0x1.1a7b9611a7b96p-4 + s * p;        /* 2/29 */
-    p = 0x1.2f684bda12f68p-4 + s * p;        /* 2/27 */
-    p = 0x1.47ae147ae147bp-4 + s * p;        /* 2/25 */
-    p = 0x1.642c8590b2164p-4 + s * p;        /* 2/23 */
-    p = 0x1.8618618618618p-4 + s * p;        /* 2/21 */
-    p = 0x1.af286bca1af28p-4 + s * p;        /* 2/19 */
-    p = 0x1.e1e1e1e1e1e1ep-4 + s * p;        /* 2/17 */
-    p = 0x1.1111111111111p-3 + s * p;        /* 2/15 */
-    p = 0x1.3b13b13b13b14p-3 + s * p;        /* 2/13 */
-    p = 0x1.745d1745d1746p-3 + s * p;        /* 2/11 */
-    p = 0x1.c71c71c71c71cp-3 + s * p;        /* 2/9  */
-    p = 0x1.2492492492492p-2 + s * p;        /* 2/7  */
-    p = 0x1.999999999999ap-2 + s * p;        /* 2/5  */
-    const double v = (th * sh) * p;          /* f^5 * tail */
-    /* log m = 2f + u + v */
-    double ah, al;
-    cm_dd_add(2.0 * fh, 2.0 * fl, uh, ul, &ah, &al);
-    cm_dd_add(ah, al, v, 0.0, &ah, &al);
-    /* + e ln 2 */
-    double kh, kl;
-    cm_dd_mul((double)e, 0.0, CM_LN2_H, CM_LN2_L, &kh, &kl);
-    cm_dd_add(kh, kl, ah, al, lh, ll);
+    const int k = (int)rint(m * 128.0);      /* 91..181 */
+    const double F = (double)k * 0.0078125;  /* exact */
+    const double f = m - F;                  /* exact (Sterbenz), |f| <= 2^-8 */
+    const double *t = CM_LOGT[k - CM_LOGT_K0];
+    const double q1 = f * t[2];
+    const double r = fma(-q1, F, f);         /* f - q1 F exactly */
+    const double q2 = r * t[2];
+    const double u = q1;
+    double p = 0x1.c71c71c71c71cp-4;         /*  1/9 */
+    p = -0x1.0000000000000p-3 + u * p;       /* -1/8 */
+    p = 0x1.2492492492492p-3 + u * p;        /*  1/7 */
+    p = -0x1.5555555555555p-3 + u * p;       /* -1/6 */
+    p = 0x1.999999999999ap-3 + u * p;        /*  1/5 */
+    p = -0x1.0000000000000p-2 + u * p;       /* -1/4 */
+    p = 0x1.5555555555555p-2 + u * p;        /*  1/3 */
+    p = -0.5 + u * p;                        /* -1/2 */
+    p = (u * u) * p;                         /* log(1+u) - u */
+    const double ed = (double)e;
+    double s1, e1, s2, e2;
+    cm_two_sum(ed * CM_LN2_HI32, t[0], &s1, &e1);   /* e ln2 (high part exact) + log F */
+    cm_two_sum(s1, q1, &s2, &e2);
+    const double lo = ((e1 + e2) + (ed * CM_LN2_LO32 + t[1])) + ((q2 - q1 * q2) + p);
+    cm_fast_two_sum(s2, lo, lh, ll);
 }
 
-/* exp of the double-double zh + zl */
+/* ln 2 / 64 = CM_L64_HI (32 significant bits) + CM_L64_LO; 64 / ln 2 */
+#define CM_L64_HI 0x1.62e42fee00000p-7
+#define CM_L64_LO 0x1.a39ef35793c76p-39
+#define CM_INV_L64 0x1.71547652b82fep+6
+
+/* exp of the double-double zh + zl: z = (64 m + j) ln2/64 + r (Cody-Waite),
+ * |r| <= ln2/128, e^z = 2^m 2^(j/64) (1 + p(r)) with 2^(j/64) from a
+ * double-double table and p the Taylor polynomial to r^6 (truncation < 2^-64). */
 CM_FN double cm_exp_dd(double zh, double zl)
 {
+    static const double CM_EXP2T[64][2] = {
+        {0x1.0000000000000p+0, 0.0},
+        {0x1.02c9a3e778061p+0, -0x1.19083535b085dp-56},
+        {0x1.059b0d3158574p+0, 0x1.d73e2a475b465p-55},
+        {0x1.0874518759bc8p+0, 0x1.186be4bb284ffp-57},
+        {0x1.0b5586cf9890fp+0, 0x1.8a62e4adc610bp-54},
+        {0x1.0e3ec32d3d1a2p+0, 0x1.03a1727c57b53p-59},
+        {0x1.11301d0125b51p+0, -0x1.6c51039449b3ap-54},
+        {0x1.1429aaea92de0p+0, -0x1.32fbf9af1369ep-54},
+        {0x1.172b83c7d517bp+0, -0x1.19041b9d78a76p-55},
+        {0x1.1a35beb6fcb75p+0, 0x1.e5b4c7b4968e4p-55},
+        {0x1.1d4873168b9aap+0, 0x1.e016e00a2643cp-54},
+        {0x1.2063b88628cd6p+0, 0x1.dc775814a8495p-55},
+        {0x1.2387a6e756238p+0, 0x1.9b07eb6c70573p-54},
+        {0x1.26b4565e27cddp+0, 0x1.2bd339940e9d9p-55},
+        {0x1.29e9df51fdee1p+0, 0x1.612e8afad1255p-55},
+        {0x1.2d285a6e4030bp+0, 0x1.0024754db41d5p-54},
+        {0x1.306fe0a31b715p+0, 0x1.6f46ad23182e4p-55},
+        {0x1.33c08b26416ffp+0, 0x1.32721843659a6p-54},
+        {0x1.371a7373aa9cbp+0, -0x1.63aeabf42eae2p-54},
+        {0x1.3a7db34e59ff7p+0, -0x1.5e436d661f5e3p-56},
+        {0x1.3dea64c123422p+0, 0x1.ada0911f09ebcp-55},
+        {0x1.4160a21f72e2ap+0, -0x1.ef3691c309278p-58},
+        {0x1.44e086061892dp+0, 0x1.89b7a04ef80d0p-59},
+        {0x1.486a2b5c13cd0p+0, 0x1.3c1a3b69062f0p-56},
+        {0x1.4bfdad5362a27p+0, 0x1.d4397afec42e2p-56},
+        {0x1.4f9b2769d2ca7p+0, -0x1.4b309d25957e3p-54},
+        {0x1.5342b569d4f82p+0, -0x1.07abe1db13cadp-55},
+        {0x1.56f4736b527dap+0, 0x1.9bb2c011d93adp-54},
+        {0x1.5ab07dd485429p+0, 0x1.6324c054647adp-54},
+        {0x1.5e76f15ad2148p+0, 0x1.ba6f93080e65ep-54},
+        {0x1.6247eb03a5585p+0, -0x1.383c17e40b497p-54},
+        {0x1.6623882552225p+0, -0x1.bb60987591c34p-54},
+        {0x1.6a09e667f3bcdp+0, -0x1.bdd3413b26456p-54},
+        {0x1.6dfb23c651a2fp+0, -0x1.bbe3a683c88abp-57},
+        {0x1.71f75e8ec5f74p+0, -0x1.16e4786887a99p-55},
+        {0x1.75feb564267c9p+0, -0x1.0245957316dd3p-54},
+        {0x1.7a11473eb0187p+0, -0x1.41577ee04992fp-55},
+        {0x1.7e2f336cf4e62p+0, 0x1.05d02ba15797ep-56},
+        {0x1.82589994cce13p+0, -0x1.d4c1dd41532d8p-54},
+        {0x1.868d99b4492edp+0, -0x1.fc6f89bd4f6bap-54},
+        {0x1.8ace5422aa0dbp+0, 0x1.6e9f156864b27p-54},
+        {0x1.8f1ae99157736p+0, 0x1.5cc13a2e3976cp-55},
+        {0x1.93737b0cdc5e5p+0, -0x1.75fc781b57ebcp-57},
+        {0x1.97d829fde4e50p+0, -0x1.d185b7c1b85d1p-54},
+        {0x1.9c49182a3f090p+0, 0x1.c7c46b071f2bep-56},
+        {0x1.a0c667b5de565p+0, -0x1.359495d1cd533p-54},
+        {0x1.a5503b23e255dp+0, -0x1.d2f6edb8d41e1p-54},
+        {0x1.a9e6b5579fdbfp+0, 0x1.0fac90ef7fd31p-54},
+        {0x1.ae89f995ad3adp+0, 0x1.7a1cd345dcc81p-54},
+        {0x1.b33a2b84f15fbp+0, -0x1.2805e3084d708p-57},
+        {0x1.b7f76f2fb5e47p+0, -0x1.5584f7e54ac3bp-56},
+        {0x1.bcc1e904bc1d2p+0, 0x1.23dd07a2d9e84p-55},
+        {0x1.c199bdd85529cp+0, 0x1.11065895048ddp-55},
+        {0x1.c67f12e57d14bp+0, 0x1.2884dff483cadp-54},
+        {0x1.cb720dcef9069p+0, 0x1.503cbd1e949dbp-56},
+        {0x1.d072d4a07897cp+0, -0x1.cbc3743797a9cp-54},
+        {0x1.d5818dcfba487p+0, 0x1.2ed02d75b3707p-55},
+        {0x1.da9e603db3285p+0, 0x1.c2300696db532p-54},
+        {0x1.dfc97337b9b5fp+0, -0x1.1a5cd4f184b5cp-54},
+        {0x1.e502ee78b3ff6p+0, 0x1.39e8980a9cc8fp-55},
+        {0x1.ea4afa2a490dap+0, -0x1.e9c23179c2893p-54},
+        {0x1.efa1bee615a27p+0, 0x1.dc7f486a4b6b0p-54},
+        {0x1.f50765b6e4540p+0, 0x1.9d3e12dd8a18bp-54},
+        {0x1.fa7c1819e90d8p+0, 0x1.74853f3a5931ep-55},
+    };
     if (zh != zh) return zh;
     if (zh > 709.8) return HUGE_VAL;
     if (zh < -745.2) return 0.0;
-    const double k = rint(zh * CM_INV_LN2);
-    double r = (zh - k * CM_LN2_HI32) - k * CM_LN2_LO32;
+    const double kd = rint(zh * CM_INV_L64);
+    const int k = (int)kd;
+    const int j = k & 63;
+    const int m = (k - j) / 64;
+    double r = (zh - kd * CM_L64_HI) - kd * CM_L64_LO;
     r = r + zl;
-    /* e^r - 1, Taylor to r^14 (|r| <= 0.35) */
-    double p = 0x1.93974a8c07c9dp-37;        /* 1/14! */
-    p = 0x1.6124613a86d09p-33 + r * p;       /* 1/13! */
-    p = 0x1.1eed8eff8d898p-29 + r * p;       /* 1/12! */
-    p = 0x1.ae64567f544e4p-26 + r * p;       /* 1/11! */
-    p = 0x1.27e4fb7789f5cp-22 + r * p;       /* 1/10! */
-    p = 0x1.71de3a556c734p-19 + r * p;       /* 1/9!  */
-    p = 0x1.a01a01a01a01ap-16 + r * p;       /* 1/8!  */
-    p = 0x1.a01a01a01a01ap-13 + r * p;       /* 1/7!  */
-    p = 0x1.6c16c16c16c17p-10 + r * p;       /* 1/6!  */
-    p = 0x1.1111111111111p-7 + r * p;        /* 1/5!  */
-    p = 0x1.5555555555555p-5 + r * p;        /* 1/4!  */
-    p = 0x1.5555555555555p-3 + r * p;        /* 1/3!  */
-    p = 0.5 + r * p;                         /* 1/2!  */
-    const double em1 = r + (r * r) * p;
-    const int ki = (int)k;
-    /* 2^k * (1 + em1); split the scaling so large |k| cannot overflow early */
-    if (ki > 1000 || ki < -1000) {
-        const int k1 = ki / 2;
-        return ldexp(ldexp(1.0 + em1, k1), ki - k1);
+    double p = 0x1.6c16c16c16c17p-10;        /* 1/720 */
+    p = 0x1.1111111111111p-7 + r * p;        /* 1/120 */
+    p = 0x1.5555555555555p-5 + r * p;        /* 1/24  */
+    p = 0x1.5555555555555p-3 + r * p;        /* 1/6   */
+    p = 0.5 + r * p;                         /* 1/2   */
+    p = r + (r * r) * p;                     /* e^r - 1 */
+    const double th = CM_EXP2T[j][0], tl = CM_EXP2T[j][1];
+    const double res = th + (tl + th * p);
+    /* 2^m * res; split the scaling so large |m| cannot overflow early */
+    if (m > 1000 || m < -1000) {
+        const int m1 = m / 2;
+        return ldexp(ldexp(res, m1), m - m1);
     }
-    return ldexp(1.0 + em1, ki);
+    return ldexp(res, m);
 }
 
 CM_FN double cm_exp(double x) { return cm_exp_dd(x, 0.0); }

@@ -129,3 +129,30 @@ def test_atan_within_one_ulp():
     assert worst < 1.0, worst
     assert np.all(np.abs(got - np.arctan(x)) <= np.spacing(np.abs(np.arctan(x))))
     assert _vec("atan", np.array([1.0]))[0] == math.atan(1.0)
+
+
+def test_pow_and_log_hard_cases():
+    """x near 1 with large exponents (log cancellation), results near the
+    overflow / underflow thresholds, and table-boundary arguments."""
+    mp = pytest.importorskip("mpmath")
+    mp.mp.dps = 50
+    r = _rng(5)
+    x = np.concatenate([1.0 + r.uniform(-1e-6, 1e-6, 200), 1.0 + r.uniform(-1e-12, 1e-12, 100),
+                        r.uniform(1.5, 3.0, 200),
+                        np.array([(k + 0.5) / 128.0 * 2.0 ** e for k in range(91, 181, 7) for e in (-3, 0, 5)])])
+    y = np.concatenate([r.uniform(-5e4, 5e4, 300), r.uniform(600.0, 1000.0, 200) * np.sign(r.uniform(-1, 1, 200)),
+                        r.uniform(-30.0, 30.0, len(x) - 500)])
+    got = _vec("pow", x, y)
+    worst = 0.0
+    for xi, yi, gi in zip(x, y, got):
+        ex = mp.power(mp.mpf(xi), mp.mpf(yi))
+        if ex > mp.mpf("1.7e308") or ex < mp.mpf("2.3e-308"):
+            continue
+        worst = max(worst, float(abs(mp.mpf(gi) - ex) / mp.mpf(math.ulp(float(ex)))))
+    assert worst < 1.0, worst
+    lx = _vec("log", x)
+    for xi, gi in zip(x, lx):
+        if xi == 1.0:
+            continue
+        ex = mp.log(mp.mpf(xi))
+        assert float(abs(mp.mpf(gi) - ex) / mp.mpf(math.ulp(float(ex)))) < 1.0
