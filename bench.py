@@ -203,6 +203,7 @@ def full_column(ctx, args):
     for _ in range(args.warmup):
         ctx.land_step_dev(ds, filterc_dev=filt, structs=structs)
     ctx.sync()
+    split, _, split_table = AT.autotune_canopy(ctx, ds)
     e0 = ctx.record()
     t0 = time.perf_counter()
     for _ in range(args.canopy_steps):
@@ -213,7 +214,7 @@ def full_column(ctx, args):
     step_ms = ctx.elapsed_ms(e0, e1) / args.canopy_steps
     ctx.destroy_event(e0)
     ctx.destroy_event(e1)
-    can_ms, = time_events(ctx, [lambda: ctx.canopyfluxes_dev(ds)], 3)
+    can_ms, = time_events(ctx, [lambda: ctx.canopyfluxes_dev(ds, structs=structs)], 3)
     b = canopy_bytes(st)
     nit = np.empty(st.dims.npatch)
     ctx.d2h(nit, ds.can["num_iter"])
@@ -224,7 +225,9 @@ def full_column(ctx, args):
            "step_ms": step_ms, "steps": args.canopy_steps,
            "canopy_kernel_ms": can_ms,
            "mean_canopy_iterations": float(nit.mean()),
-           "canopy_roofline": {"kernel": "k_canopy", "alg_bytes_per_launch": b,
+           "canopy_rounds": {"first_iterations": split[0], "later_iterations": split[1],
+                             "candidates_ms": {f"{a}/{c}": round(ms, 3) for (a, c), ms in split_table}},
+           "canopy_roofline": {"kernel": "k_canopy_round (all rounds of one call)", "alg_bytes_per_launch": b,
                                "achieved_GBps": b / (can_ms * 1e-3) / 1e9,
                                "frac": b / (can_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
     ds.free()

@@ -43,6 +43,8 @@ class Tuning(C.Structure):
         ("richards_block", C.c_int32),
         ("soiltemp_block", C.c_int32),
         ("fused_single_kernel", C.c_int32),
+        ("canopy_first_iterations", C.c_int32),
+        ("canopy_round_iterations", C.c_int32),
     ]
 
 

@@ -93,3 +93,38 @@ def autotune(ctx, ds, filter_dev=None, n=None, steps=2, candidates=None, dtime=1
     free_snapshot(ctx, snap)
     ctx.set_tuning(**best)
     return best, best_ms, table
+
+
+# (first-round iterations, later-round iterations) of the CanopyFluxes
+# leaf-temperature iteration; first > itmax is the one-pass kernel.
+CANOPY_CANDIDATES = [(1001, 1), (4, 2), (4, 4), (5, 3), (6, 3), (6, 6), (8, 4), (10, 5)]
+
+
+def autotune_canopy(ctx, ds, filter_dev=None, n=None, reps=2, candidates=None, dtime=1800.0, verbose=False):
+    """Pick the CanopyFluxes round split that runs this state fastest (timed
+    on a snapshot of the resident state, which is left unchanged)."""
+    candidates = candidates or CANOPY_CANDIDATES
+    snap = snapshot(ctx, ds)
+    structs = (*ds.structs(), ds.canopy_struct())
+    best, best_ms, table = None, float("inf"), []
+    for first, later in candidates:
+        ctx.set_tuning(canopy_first_iterations=first, canopy_round_iterations=later)
+        ms = 0.0
+        for _ in range(reps):
+            restore(ctx, ds, snap)
+            e0 = ctx.record()
+            ctx.canopyfluxes_dev(ds, filterp_dev=filter_dev, n=n, dtime=dtime, structs=structs)
+            e1 = ctx.record()
+            ctx.sync()
+            ms += ctx.elapsed_ms(e0, e1) / reps
+            ctx.destroy_event(e0)
+            ctx.destroy_event(e1)
+        table.append(((first, later), ms))
+        if verbose:
+            print(f"autotune canopy first={first} later={later} -> {ms:.3f} ms", flush=True)
+        if ms < best_ms:
+            best, best_ms = (first, later), ms
+    restore(ctx, ds, snap)
+    free_snapshot(ctx, snap)
+    ctx.set_tuning(canopy_first_iterations=best[0], canopy_round_iterations=best[1])
+    return best, best_ms, table

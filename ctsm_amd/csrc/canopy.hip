@@ -638,13 +638,26 @@ __device__ void temp_factors(const ctsm_canopy_params &P, const PsnConst &K, dou
 
 #define PF(f) pft[(f) * CTSM_MAX_PFT + ivt]
 
-__global__ void __launch_bounds__(64)
-k_canopy(CView V, CSwitch sw, ctsm_canopy_params P, const double *__restrict__ pft, ctsm_col_state col,
-         ctsm_canopy_state S, int n, const int32_t *__restrict__ filter, double dtime, ErrWord *err)
+// Iteration state of a patch carried between rounds, indexed by the patch's
+// position in the filter (structure of arrays).
+struct CanopyWork {
+    double *t_veg, *del, *efeb, *obuold, *nmozsgn, *um, *obu, *fm, *qsatl, *el, *qsatldT, *taf, *qaf, *delq,
+        *itlef, *gs_mol;
+};
+constexpr int N_WORK_FIELDS = 16;
+static_assert(sizeof(CanopyWork) == N_WORK_FIELDS * sizeof(double *), "one pointer per saved value");
+
+// One patch of CanopyFluxes for at most `budget` leaf-temperature
+// iterations.  first: the time step's set-up runs (root stress, roughness,
+// Monin-Obukhov initialisation); otherwise the iteration state is restored
+// from W.  Returns true when the patch is still iterating (its state saved in
+// W[i0]); false when it finished (outputs written) or failed.
+__device__ __forceinline__ bool canopy_patch(const CView &V, const CSwitch &sw, const ctsm_canopy_params &P,
+                             const double *__restrict__ pft, const ctsm_col_state &col, const ctsm_canopy_state &S,
+                             int i0, const int32_t *__restrict__ filter, double dtime, ErrWord *err, bool first,
+                             int budget, const CanopyWork &W)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int p = filter ? filter[i] : V.begp + i;
+    const int p = filter ? filter[i0] : V.begp + i0;
     const long pp = V.p1(p);
     const int c = S.column[pp];
     const int g = S.gridcell[pp];
@@ -652,7 +665,7 @@ k_canopy(CView V, CSwitch sw, ctsm_canopy_params P, const double *__restrict__ p
     // indices that address other arrays must lie inside their allocations
     if (ivt < 0 || ivt >= CTSM_MAX_PFT || c < V.begc || c >= V.begc + V.ldc || g < V.begg || g >= V.begg + V.ldg) {
         report_error(err, p, CTSM_E_ARG);
-        return;
+        return false;
     }
     const long cc = V.c1(c), gg = V.g1(g);
     const double btran0 = 0.0, zii = 1000.0, beta = 1.0, delmax = 1.0, dlemin = 0.1, dtmin = 0.01;
@@ -712,8 +725,10 @@ k_canopy(CView V, CSwitch sw, ctsm_canopy_params P, const double *__restrict__ p
 
     // ---- effective porosity, volumetric liquid water, root moisture stress
     double btran = btran0;
-    double rr[MAX_NLEVGRND];
-    {
+    if (!first) {
+        btran = S.btran[pp];
+    } else {
+        double rr[MAX_NLEVGRND];
         const double smpsc = PF(F_SMPSC), smpso = PF(F_SMPSO);
 #pragma unroll
         for (int j = 1; j <= MAX_NLEVGRND; ++j) {
@@ -743,11 +758,14 @@ k_canopy(CView V, CSwitch sw, ctsm_canopy_params P, const double *__restrict__ p
             if (j > V.nlevgrnd) continue;
             S.rootr[V.soip(p, j)] = (btran > btran0) ? rr[j - 1] / btran : 0.0;
         }
+        S.btran[pp] = btran;
     }
 
     // ---- aerodynamic parameters (CanopyFluxesMod.F90:897-943)
     double displa = S.displa[pp], z0mv = S.z0mv[pp];
-    if (sw.z0param == CTSM_Z0_ZENGWANG2007) {
+    if (!first) {
+        // already updated in place by the first round
+    } else if (sw.z0param == CTSM_Z0_ZENGWANG2007) {
         const double lt = fmin90(elai + esai, TLSAI_CRIT);
         const double egvf = (1.0 - ALPHA_AERO * cm_exp(-lt)) / (1.0 - ALPHA_AERO * cm_exp(-TLSAI_CRIT));
         displa = egvf * displa;
@@ -775,56 +793,69 @@ k_canopy(CView V, CSwitch sw, ctsm_canopy_params P, const double *__restrict__ p
     const double hu = S.forc_hgt_u[gg] + z0mv + displa;
     const double ht = S.forc_hgt_t[gg] + z0hv + displa;
     const double hq = S.forc_hgt_q[gg] + z0qv + displa;
-    S.displa[pp] = displa;
-    S.z0mv[pp] = z0mv;
-    S.z0hv[pp] = z0hv;
-    S.z0qv[pp] = z0qv;
-    S.forc_hgt_u_patch[pp] = hu;
-    S.forc_hgt_t_patch[pp] = ht;
-    S.forc_hgt_q_patch[pp] = hq;
+    if (first) {
+        S.displa[pp] = displa;
+        S.z0mv[pp] = z0mv;
+        S.z0hv[pp] = z0hv;
+        S.z0qv[pp] = z0qv;
+        S.forc_hgt_u_patch[pp] = hu;
+        S.forc_hgt_t_patch[pp] = ht;
+        S.forc_hgt_q_patch[pp] = hq;
+    }
 
     // ---- radiation terms, leaf humidity, initial profiles (:945-1002)
     const double forc_lwrad = col.forc_lwrad[cc];
     const double air = emv * (1.0 + (1.0 - emv) * (1.0 - emg)) * forc_lwrad;
     const double bir = -(2.0 - emv * (1.0 - emg)) * emv * SB;
     const double cir = emv * emg * SB;
-    double t_veg = S.t_veg[pp];
-    double qsatl, el, qsatldT;
-    qsat(t_veg, forc_pbot, qsatl, el, qsatldT);
+    double t_veg, qsatl, el, qsatldT, taf, qaf, dth = 0.0, dqh = 0.0, delq, um, obu;
+    int nmozsgn;
     const double co2 = S.forc_pco2[gg], o2 = S.forc_po2[gg];
-    int nmozsgn = 0;
-    double taf = (t_grnd + thm) / 2.0;
-    double qaf = (forc_q + qg) / 2.0;
     const double fu = S.forc_u[gg], fvv = S.forc_v[gg];
     const double ur = fmax90(P.wind_min, sqrt(fu * fu + fvv * fvv));
-    double dth = thm - taf;
-    double dqh = forc_q - qaf;
-    double delq = qg - qaf;
-    const double dthv = dth * (1.0 + 0.61 * forc_q) + 0.61 * forc_th * dqh;
     const double zldis = hu - displa;
-    if (zldis < 0.0) {
-        report_error(err, p, ENDRUN);
-        return;
-    }
-    double um, obu;
-    {
+    if (first) {
+        t_veg = S.t_veg[pp];
+        qsat(t_veg, forc_pbot, qsatl, el, qsatldT);
+        nmozsgn = 0;
+        taf = (t_grnd + thm) / 2.0;
+        qaf = (forc_q + qg) / 2.0;
+        dth = thm - taf;
+        dqh = forc_q - qaf;
+        delq = qg - qaf;
+        const double dthv = dth * (1.0 + 0.61 * forc_q) + 0.61 * forc_th * dqh;
+        if (zldis < 0.0) {
+            report_error(err, p, ENDRUN);
+            return false;
+        }
         // FrictionVelocityMod.F90:1158-1209
         const double wc = 0.5;
         if (dthv >= 0.0) um = fmax90(ur, 0.1);
         else um = sqrt(ur * ur + wc * wc);
         const double rib = GRAV * zldis * dthv / (thv * um * um);
-        double zeta;
+        double zeta0;
         if (rib >= 0.0) {
-            zeta = rib * cm_log(zldis / z0mv) / (1.0 - 5.0 * fmin90(rib, 0.19));
-            zeta = fmin90(P.zetamaxstable, fmax90(zeta, 0.01));
+            zeta0 = rib * cm_log(zldis / z0mv) / (1.0 - 5.0 * fmin90(rib, 0.19));
+            zeta0 = fmin90(P.zetamaxstable, fmax90(zeta0, 0.01));
         } else {
-            zeta = rib * cm_log(zldis / z0mv);
-            zeta = fmax90(-100.0, fmin90(zeta, -0.01));
+            zeta0 = rib * cm_log(zldis / z0mv);
+            zeta0 = fmax90(-100.0, fmin90(zeta0, -0.01));
         }
-        obu = zldis / zeta;
+        obu = zldis / zeta0;
+    } else {
+        t_veg = W.t_veg[i0];
+        qsatl = W.qsatl[i0];
+        el = W.el[i0];
+        qsatldT = W.qsatldT[i0];
+        taf = W.taf[i0];
+        qaf = W.qaf[i0];
+        delq = W.delq[i0];
+        um = W.um[i0];
+        obu = W.obu[i0];
+        nmozsgn = (int)W.nmozsgn[i0];
     }
     double num_iter = 0.0;
-    const double tl_ini = t_veg;
+    const double tl_ini = S.t_veg[pp];
     double t_stem = S.t_stem[pp];
     const double ts_ini = t_stem;
 
@@ -837,7 +868,7 @@ k_canopy(CView V, CSwitch sw, ctsm_canopy_params P, const double *__restrict__ p
     psn_constants(sw, P, pft, ivt, forc_pbot, o2, S.t_a10[pp], btran, dayl_factor, K, bad);
     Phase sun, sha;
     Shared sh;
-    sh.gs_mol = S.gs_mol[pp];
+    sh.gs_mol = first ? S.gs_mol[pp] : W.gs_mol[i0];
     sh.rh_leaf = S.rh_leaf[pp];
     sh.vpd_can = S.vpd_can[pp];
     sun.gs_ln = S.gs_mol_sun_ln[pp];
@@ -854,7 +885,8 @@ k_canopy(CView V, CSwitch sw, ctsm_canopy_params P, const double *__restrict__ p
     TFac T;
 
     // ---- iteration state
-    double del = 0.0, del2 = 0.0, efeb = 0.0, efe = 0.0, obuold = 0.0;
+    double del = first ? 0.0 : W.del[i0], del2 = 0.0, efeb = first ? 0.0 : W.efeb[i0], efe = 0.0;
+    double obuold = first ? 0.0 : W.obuold[i0];
     double wtlq0 = 0.0, wtalq = 0.0, wtgq = 0.0, wtaq0 = 0.0;
     double tlbef = t_veg, dt_veg = 0.0, errp = 0.0;
     double ustar = 0.0, temp1 = 0.0, temp2 = 0.0, temp12m = 0.0, temp22m = 0.0;
@@ -864,7 +896,7 @@ k_canopy(CView V, CSwitch sw, ctsm_canopy_params P, const double *__restrict__ p
     double qflx_tran_veg = S.qflx_tran_veg[pp], qflx_evap_veg = 0.0, zeta = 0.0, svpts = 0.0, eah = 0.0;
     double rh_af = 0.0, vpd = 0.0;
     FV fv;
-    fv.fm = 0.0;
+    fv.fm = first ? 0.0 : W.fm[i0];
     const double laisun = S.laisun[pp], laisha = S.laisha[pp], fdry = S.fdry[pp], fwet = S.fwet[pp];
     const double liqcan0 = S.liqcan[pp], snocan0 = S.snocan[pp];
     const double h2ocan = liqcan0 + snocan0;
@@ -875,7 +907,7 @@ k_canopy(CView V, CSwitch sw, ctsm_canopy_params P, const double *__restrict__ p
     const int snl = col.snl[cc];
     if (snl < -V.nlevsno || snl > 0) {
         report_error(err, p, CTSM_E_ARG);
-        return;
+        return false;
     }
     const double frac_sno = col.frac_sno_eff[cc], frac_h2osfc = col.frac_h2osfc[cc];
     const double t_top = col.t_soisno[V.sno(c, snl + 1)], t_soi1 = col.t_soisno[V.sno(c, 1)];
@@ -884,7 +916,9 @@ k_canopy(CView V, CSwitch sw, ctsm_canopy_params P, const double *__restrict__ p
                             + frac_h2osfc * p4(t_h2osfc));
     const double lai_dl_eff = P.lai_dl * (1.0 - fmin90(col.snow_depth[cc] / P.z_dl, 1.0));
 
-    for (int itlef = 0; itlef <= sw.itmax;) {
+    int itlef = first ? 0 : (int)W.itlef[i0];
+    bool finished = false;
+    for (int it = 0; it < budget && itlef <= sw.itmax; ++it) {
         friction_velocity(hu, ht, hq, displa, z0mv, z0hv, z0qv, obu, itlef + 1, ur, um, fv);
         ustar = fv.ustar;
         temp1 = fv.temp1;
@@ -1040,8 +1074,32 @@ k_canopy(CView V, CSwitch sw, ctsm_canopy_params P, const double *__restrict__ p
             efeb = efe;
             const double det = fmax90(del, del2);
             num_iter = (double)itlef;
-            if (det < dtmin && dele < dlemin) break;
+            if (det < dtmin && dele < dlemin) {
+                finished = true;
+                break;
+            }
         }
+    }
+    if (!finished && itlef <= sw.itmax) {
+        // still iterating: keep the state for the next round
+        W.t_veg[i0] = t_veg;
+        W.del[i0] = del;
+        W.efeb[i0] = efeb;
+        W.obuold[i0] = obuold;
+        W.nmozsgn[i0] = (double)nmozsgn;
+        W.um[i0] = um;
+        W.obu[i0] = obu;
+        W.fm[i0] = fv.fm;
+        W.qsatl[i0] = qsatl;
+        W.el[i0] = el;
+        W.qsatldT[i0] = qsatldT;
+        W.taf[i0] = taf;
+        W.qaf[i0] = qaf;
+        W.delq[i0] = delq;
+        W.itlef[i0] = (double)itlef;
+        W.gs_mol[i0] = sh.gs_mol;
+        if (bad) report_error(err, p, ENDRUN);
+        return true;
     }
 
     // ---- fluxes after the iteration (:1467-1615)
@@ -1209,6 +1267,42 @@ k_canopy(CView V, CSwitch sw, ctsm_canopy_params P, const double *__restrict__ p
     S.gs_mol_sun_ln[pp] = sun.gs_ln;
     S.gs_mol_sha_ln[pp] = sha.gs_ln;
     if (bad) report_error(err, p, ENDRUN);
+    return false;
+}
+
+// One round of CanopyFluxes over the patches still iterating (round 0: the
+// whole filter), compacting those that continue into list_out with one atomic
+// per wavefront.  Blocks are one wavefront; the grid strides over the count.
+__global__ void __launch_bounds__(64)
+k_canopy_round(CView V, CSwitch sw, ctsm_canopy_params P, const double *__restrict__ pft, ctsm_col_state col,
+               ctsm_canopy_state S, int n, const int32_t *__restrict__ filter, double dtime, ErrWord *err,
+               int round, int budget, const int32_t *__restrict__ list_in, const int32_t *__restrict__ cnt_in,
+               int32_t *__restrict__ list_out, int32_t *__restrict__ cnt_out, CanopyWork W)
+{
+    const int count = (round == 0) ? n : *cnt_in;
+    const int lane = threadIdx.x;
+    for (int base = blockIdx.x * 64; base < count; base += gridDim.x * 64) {
+        const int idx = base + lane;
+        bool cont = false;
+        int i0 = 0;
+        if (idx < count) {
+            i0 = (round == 0) ? idx : list_in[idx];
+            cont = canopy_patch(V, sw, P, pft, col, S, i0, filter, dtime, err, round == 0, budget, W);
+        }
+        const unsigned long long mask = __ballot(cont);
+        if (mask) {
+            const int leader = __ffsll((unsigned long long)mask) - 1;
+            int pos = 0;
+            if (lane == leader) pos = atomicAdd(cnt_out, __popcll(mask));
+            pos = __shfl(pos, leader);
+            if (cont) list_out[pos + __popcll(mask & ((1ull << lane) - 1ull))] = i0;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(64) k_zero_counts(int32_t *cnt, int n)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) cnt[i] = 0;
 }
 #undef PF
 
@@ -1278,9 +1372,29 @@ __global__ void __launch_bounds__(256) k_root_sink(CView V, ctsm_col_state col, 
 
 }  // namespace canopy
 
+// Rounds after the first one: enough to take a patch from first_its
+// iterations to itmax + 1.
+static int canopy_later_rounds(int itmax, int first_its, int round_its)
+{
+    const int left = itmax + 1 - first_its;
+    return left > 0 ? (left + round_its - 1) / round_its : 0;
+}
+
+// Round buffer: N_WORK_FIELDS doubles per patch of the filter, two patch
+// lists, one counter per round.  0 when the iteration runs in one pass.
+size_t canopy_work_bytes(int n, int itmax, int first_its, int round_its)
+{
+    using namespace canopy;
+    const int R = canopy_later_rounds(itmax, first_its, round_its);
+    if (R == 0 || n <= 0) return 0;
+    return (size_t)N_WORK_FIELDS * n * sizeof(double) + 2 * (size_t)n * sizeof(int32_t) +
+           (size_t)(R + 1) * sizeof(int32_t);
+}
+
 void launch_canopy(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg, const ctsm_canopy_params &prm,
                    const double *pft_dev, const ctsm_col_state &col, const ctsm_canopy_state &can, int n,
-                   const int32_t *filter, double dtime, ErrWord *err, int block, bool zero_cgrnd)
+                   const int32_t *filter, double dtime, ErrWord *err, void *work, int first_its, int round_its,
+                   bool zero_cgrnd)
 {
     using namespace canopy;
     CView V{L.begc_alloc, L.ldc, L.begp_alloc, L.ldp, L.begg_alloc, L.ldg, cfg.nlevsno, cfg.nlevsoi, cfg.nlevgrnd};
@@ -1291,9 +1405,32 @@ void launch_canopy(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg,
     if (n <= 0) return;
     if (zero_cgrnd)
         hipLaunchKernelGGL(k_zero_cgrnd, dim3((n + 255) / 256), dim3(256), 0, st, n, filter, L.begp_alloc, can);
-    (void)block;
-    hipLaunchKernelGGL(k_canopy, dim3((n + 63) / 64), dim3(64), 0, st, V, sw, prm, pft_dev, col, can, n, filter,
-                       dtime, err);
+    const int R = work ? canopy_later_rounds(cfg.itmax_canopy_fluxes, first_its, round_its) : 0;
+    const int blocks = (n + 63) / 64;
+    if (R == 0) {
+        // one pass: every patch finishes (converged or itmax reached) in round 0
+        hipLaunchKernelGGL(k_canopy_round, dim3(blocks), dim3(64), 0, st, V, sw, prm, pft_dev, col, can, n, filter,
+                           dtime, err, 0, cfg.itmax_canopy_fluxes + 1, (const int32_t *)nullptr,
+                           (const int32_t *)nullptr, (int32_t *)nullptr, (int32_t *)nullptr, CanopyWork{});
+        return;
+    }
+    double *w = (double *)work;
+    auto field = [&](int k) { return w + (size_t)k * n; };
+    const CanopyWork W{field(0), field(1), field(2),  field(3),  field(4),  field(5),  field(6),  field(7),
+                       field(8), field(9), field(10), field(11), field(12), field(13), field(14), field(15)};
+    int32_t *list[2] = {(int32_t *)(w + (size_t)N_WORK_FIELDS * n), nullptr};
+    list[1] = list[0] + n;
+    int32_t *cnt = list[1] + n;
+    hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, st, cnt, R + 1);
+    // later rounds stride over the patches still iterating with a grid that
+    // keeps every wave slot of the chip busy
+    const int later_blocks = blocks < 8192 ? blocks : 8192;
+    for (int r = 0; r <= R; ++r) {
+        const int budget = r == 0 ? first_its : round_its;
+        hipLaunchKernelGGL(k_canopy_round, dim3(r == 0 ? blocks : later_blocks), dim3(64), 0, st, V, sw, prm, pft_dev,
+                           col, can, n, filter, dtime, err, r, budget, (const int32_t *)list[(r + 1) & 1],
+                           (const int32_t *)(r ? cnt + r - 1 : nullptr), list[r & 1], cnt + r, W);
+    }
 }
 
 void launch_root_sink(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg, const ctsm_col_state &col,

@@ -33,9 +33,10 @@ size_t soiltemp_plain_workspace_doubles(int n);
 size_t soilwater_plain_workspace_doubles(int n);
 void launch_soilwater_plain(hipStream_t, const View &, const Params &, const ctsm_col_state &, int, const int32_t *,
                             double, double *, long, ErrWord *);
+size_t canopy_work_bytes(int n, int itmax, int first_its, int round_its);
 void launch_canopy(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_canopy_params &, const double *,
-                   const ctsm_col_state &, const ctsm_canopy_state &, int, const int32_t *, double, ErrWord *, int,
-                   bool);
+                   const ctsm_col_state &, const ctsm_canopy_state &, int, const int32_t *, double, ErrWord *,
+                   void *, int, int, bool);
 void launch_root_sink(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_col_state &,
                       const ctsm_canopy_state &, int, const int32_t *, ErrWord *);
 }  // namespace ctsm
@@ -224,7 +225,7 @@ struct ctsm_ctx {
     int device = 0;
     ctsm_config cfg{};
     Params prm{};
-    ctsm_tuning tun{4, 128, 128, 0};
+    ctsm_tuning tun{4, 128, 128, 0, 6, 3};
     hipStream_t stream = nullptr;
     ErrWord *err = nullptr;            // device
     DevBuf work;                       // heat-solve workspace
@@ -233,6 +234,7 @@ struct ctsm_ctx {
     std::vector<DevBuf> canp, canc, cang; // canopy drop-in mirrors
     ctsm_canopy_params cprm{};         // canopy param-file scalars
     DevBuf pft;                        // PFT table (N_PFT_FIELDS x CTSM_MAX_PFT doubles)
+    DevBuf cwork;                      // CanopyFluxes round state, patch lists and counters
     bool have_canopy = false;
     int32_t last_code = 0, last_index = 0;
     std::mutex mu;
@@ -506,6 +508,11 @@ int ctsm_destroy(ctsm_ctx *ctx)
     ctx->filt.release();
     for (auto &b : ctx->colmir) b.release();
     for (auto &b : ctx->pchmir) b.release();
+    for (auto &b : ctx->canp) b.release();
+    for (auto &b : ctx->canc) b.release();
+    for (auto &b : ctx->cang) b.release();
+    ctx->pft.release();
+    ctx->cwork.release();
     if (ctx->err) hipFree(ctx->err);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -521,6 +528,9 @@ int ctsm_set_tuning(ctsm_ctx *ctx, const ctsm_tuning *t)
     /* the one-lane Richards kernel is built for 64-lane blocks (register budget of one wave per SIMD) */
     if (t->richards_lanes == 1 && t->richards_block > 64) return CTSM_E_ARG;
     if (t->soiltemp_block < 0 || t->soiltemp_block > 128 || (t->soiltemp_block % 64) != 0) return CTSM_E_ARG;
+    if (t->canopy_first_iterations < 1 || t->canopy_first_iterations > 1001 || t->canopy_round_iterations < 1 ||
+        t->canopy_round_iterations > 1001)
+        return CTSM_E_ARG;
     ctx->tun = *t;
     return CTSM_OK;
 }
@@ -670,13 +680,28 @@ static int check_canopy_layout(const ctsm_ctx *ctx, const ctsm_layout *L)
     return CTSM_OK;
 }
 
+// Grows the canopy round buffer to n patches and launches CanopyFluxes.
+static int run_canopy(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state &col, const ctsm_canopy_state &can,
+                      int32_t n, const int32_t *filter_dev, double dtime, bool zero_cgrnd)
+{
+    const int k0 = ctx->tun.canopy_first_iterations, k = ctx->tun.canopy_round_iterations;
+    const size_t wb = canopy_work_bytes(n, ctx->cfg.itmax_canopy_fluxes, k0, k);
+    if (wb) {
+        hipError_t he = ctx->cwork.ensure(wb);
+        if (he != hipSuccess) return hip_fail(ctx, he);
+    }
+    launch_canopy(ctx->stream, *lay, ctx->cfg, ctx->cprm, (const double *)ctx->pft.p, col, can, n, filter_dev, dtime,
+                  ctx->err, wb ? ctx->cwork.p : nullptr, k0, k, zero_cgrnd);
+    return CTSM_OK;
+}
+
 int ctsm_canopyfluxes_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
                           const ctsm_canopy_state *can, int32_t n, const int32_t *filterp_dev, double dtime)
 {
     if (!ctx || !col || !can || n < 0 || check_canopy_layout(ctx, lay) || (!filterp_dev && n > lay->ldp))
         return CTSM_E_ARG;
-    launch_canopy(ctx->stream, *lay, ctx->cfg, ctx->cprm, (const double *)ctx->pft.p, *col, *can, n, filterp_dev,
-                  dtime, ctx->err, 64, false);
+    int rc = run_canopy(ctx, lay, *col, *can, n, filterp_dev, dtime, false);
+    if (rc) return rc;
     return finish_launch(ctx);
 }
 
@@ -702,10 +727,10 @@ int ctsm_land_step_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_sta
         pch->qflx_ev_snow != can->qflx_ev_snow || pch->qflx_ev_soil != can->qflx_ev_soil ||
         pch->qflx_ev_h2osfc != can->qflx_ev_h2osfc || col->qflx_rootsoi != can->qflx_rootsoi_col)
         return CTSM_E_ARG;
-    launch_canopy(ctx->stream, *lay, ctx->cfg, ctx->cprm, (const double *)ctx->pft.p, *col, *can, np, filterp_dev,
-                  dtime, ctx->err, 64, true);
+    int rc = run_canopy(ctx, lay, *col, *can, np, filterp_dev, dtime, true);
+    if (rc) return rc;
     launch_root_sink(ctx->stream, *lay, ctx->cfg, *col, *can, nc, filterc_dev, ctx->err);
-    int rc = finish_launch(ctx);
+    rc = finish_launch(ctx);
     if (rc) return rc;
     return ctsm_soil_step_fused_dev(ctx, lay, col, pch, nc, filterc_dev, dtime);
 }
@@ -754,8 +779,7 @@ int ctsm_canopyfluxes(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_stat
     if ((rc = canopy_mirror(ctx, lay, can, &dcan, true))) return rc;
     const int32_t *dfilt;
     if ((rc = upload_filter(ctx, n, filterp, &dfilt))) return rc;
-    launch_canopy(ctx->stream, *lay, ctx->cfg, ctx->cprm, (const double *)ctx->pft.p, dcol, dcan, n, dfilt, dtime,
-                  ctx->err, 64, false);
+    if ((rc = run_canopy(ctx, lay, dcol, dcan, n, dfilt, dtime, false))) return rc;
     if ((rc = finish_launch(ctx))) return rc;
     rc = mirror(ctx, kColFields, (int)(sizeof(kColFields) / sizeof(kColFields[0])), col, &dcol, ctx->colmir,
                 lay->begc_alloc, lay->ldc, cr, false);

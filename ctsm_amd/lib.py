@@ -189,9 +189,12 @@ class Context:
         self._check(self.L.ctsm_root_sink(self.h, C.byref(st.layout()), C.byref(cs), C.byref(can), n, fp),
                     "root sink")
 
-    def canopyfluxes_dev(self, ds, filterp_dev=None, n=None, dtime=1800.0):
-        cs, _ = ds.structs()
-        can = ds.canopy_struct()
+    def canopyfluxes_dev(self, ds, filterp_dev=None, n=None, dtime=1800.0, structs=None):
+        if structs is None:
+            cs, _ = ds.structs()
+            can = ds.canopy_struct()
+        else:
+            cs, _, can = structs
         n = ds.dims.npatch if n is None else n
         fp = C.c_void_p(filterp_dev) if filterp_dev is not None else None
         self._check(self.L.ctsm_canopyfluxes_dev(self.h, C.byref(ds.layout()), C.byref(cs), C.byref(can), n, fp,

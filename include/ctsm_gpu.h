@@ -369,12 +369,21 @@ typedef struct ctsm_canopy_state {
  * richards_lanes: lanes cooperating on one column in the Richards solve
  *   (1, 2, 4 or 8); the heat solve always runs one column per lane.
  * fused_single_kernel: 1 = the whole soil step in one kernel (one lane per
- *   column), 0 = heat solve kernel + Richards kernel (2 launches per step). */
+ *   column), 0 = heat solve kernel + Richards kernel (2 launches per step).
+ * canopy_first_iterations, canopy_round_iterations: CanopyFluxes runs its
+ *   leaf-temperature iteration in rounds; the first round takes every patch of
+ *   the filter for at most canopy_first_iterations iterations, each later round
+ *   only the patches still iterating (compacted) for at most
+ *   canopy_round_iterations more.  canopy_first_iterations > itmax_canopy_fluxes
+ *   runs the whole iteration in one pass.  Both >= 1; results do not depend on
+ *   them. */
 typedef struct ctsm_tuning {
     int32_t richards_lanes;
     int32_t richards_block;
     int32_t soiltemp_block;
     int32_t fused_single_kernel;
+    int32_t canopy_first_iterations;
+    int32_t canopy_round_iterations;
 } ctsm_tuning;
 
 typedef struct ctsm_ctx ctsm_ctx;

@@ -44,7 +44,9 @@ def _assert_same(g, o, groups=("col", "pch", "can")):
 
 
 def _oracle_cfg(ctx):
-    return O.canopy_config(**{k: getattr(ctx.cfg, k) for k in CANOPY_CFG_KEYS})
+    cfg = O.canopy_config(**{k: getattr(ctx.cfg, k) for k in CANOPY_CFG_KEYS})
+    cfg.itmax_canopy_fluxes = ctx.cfg.itmax_canopy_fluxes
+    return cfg
 
 
 @pytest.mark.parametrize("kw", [
@@ -117,6 +119,37 @@ def test_land_step_bitwise(cctx, nsteps, workload):
         rc, bad = O.land_step(o, cfg, S.canopy_params(), pft)
         assert rc == 0, bad
     _assert_same(g, o)
+
+
+@pytest.mark.parametrize("first,later,itmax", [
+    (41, 3, 40),     # the whole iteration in one pass
+    (6, 3, 40),      # default rounds
+    (1, 1, 40),      # one iteration per round: the most rounds
+    (3, 2, 40),
+    (2, 5, 6),       # itmax reached inside the rounds
+])
+def test_canopy_rounds_bitwise(cctx, first, later, itmax):
+    """Results do not depend on how the iteration is cut into rounds."""
+    ctx = cctx(itmax_canopy_fluxes=itmax)
+    before = ctx.get_tuning()
+    st = S.canopy_workload("canopy_uneven", ncol=512)
+    filt = np.arange(st.dims.npatch, 0, -2, dtype=np.int32)       # every other patch, descending
+    print(f"case: CanopyFluxes rounds first={first} later={later} itmax={itmax}, "
+          f"{len(filt)} of {st.dims.npatch} patches", flush=True)
+    ctx.set_tuning(canopy_first_iterations=first, canopy_round_iterations=later)
+    try:
+        g = st.copy()
+        ctx.canopyfluxes(g, filter_p=filt)
+    finally:
+        ctx.set_tuning(**before)
+    o = st.copy()
+    cfg = _oracle_cfg(ctx)
+    cfg.itmax_canopy_fluxes = itmax
+    rc, bad = O.canopyfluxes(o, cfg, S.canopy_params(), O.PftStruct(S.pft_table()), filter_p=filt)
+    assert rc == 0, bad
+    _assert_same(g, o)
+    if itmax < 40:
+        assert g.can["num_iter"].max() == itmax + 1
 
 
 def test_canopy_error_reports_patch(cctx):
