@@ -137,16 +137,17 @@ __device__ __forceinline__ double stab2(double zeta)
     return 2.0 * cm_log((1.0 + chik2) * 0.5);
 }
 
-// the temperature / humidity profile block repeated in FrictionVelocityMod.F90:916-1060
-__device__ double temp_profile(double zldis, double z0, double obu)
+// the temperature / humidity profile block repeated in FrictionVelocityMod.F90:916-1060;
+// log_z = log(zldis / z0), which does not change over the canopy iteration
+__device__ double temp_profile(double zldis, double z0, double obu, double log_z)
 {
     const double zetat = 0.465;
     const double zeta = zldis / obu;
     if (zeta < -zetat)
         return VKC / (cm_log(-zetat * obu / z0) - stab2(-zetat) + stab2(z0 / obu)
                       + 0.8 * (cm_pow(zetat, -0.333) - cm_pow(-zeta, -0.333)));
-    if (zeta < 0.0) return VKC / (cm_log(zldis / z0) - stab2(zeta) + stab2(z0 / obu));
-    if (zeta <= 1.0) return VKC / (cm_log(zldis / z0) + 5.0 * zeta - 5.0 * z0 / obu);
+    if (zeta < 0.0) return VKC / (log_z - stab2(zeta) + stab2(z0 / obu));
+    if (zeta <= 1.0) return VKC / (log_z + 5.0 * zeta - 5.0 * z0 / obu);
     return VKC / (cm_log(obu / z0) + 5.0 - 5.0 * z0 / obu + (5.0 * cm_log(zeta) + zeta - 1.0));
 }
 
@@ -163,22 +164,49 @@ struct FV {
     double ustar, temp1, temp2, temp12m, temp22m, fm, u10, u10_clm, va, fv, vds;
 };
 
-// FrictionVelocityMod.F90:754-1117 (patch form)
-__device__ void friction_velocity(double hu, double ht, double hq, double displa, double z0m, double z0h,
-                                  double z0q, double obu, int iter, double ur, double um, FV &o)
+// FrictionVelocityMod.F90:754-1117 (patch form), the part the canopy
+// iteration consumes: ustar, the heat / moisture profile factors temp1 and
+// temp2, and the running mean of the dust-model fm.  The logarithms of
+// height over roughness length (lm: momentum, lh: heat, lq: moisture) are the
+// step's constants.
+struct FVLogs {
+    double lm, lh, lq;
+};
+__device__ __forceinline__ void friction_velocity(double hu, double ht, double hq, double displa, double z0m,
+                                                  double z0h, double z0q, const FVLogs &L, double obu, int iter,
+                                                  double um, FV &o)
 {
     const double zetam = 1.574;
-    double zldis = hu - displa;
-    double zeta = zldis / obu;
+    const double zldis = hu - displa;
+    const double zeta = zldis / obu;
     if (zeta < -zetam)
         o.ustar = VKC * um / (cm_log(-zetam * obu / z0m) - stab1(-zetam) + stab1(z0m / obu)
                               + 1.14 * (cm_pow(-zeta, 0.333) - cm_pow(zetam, 0.333)));
     else if (zeta < 0.0)
-        o.ustar = VKC * um / (cm_log(zldis / z0m) - stab1(zeta) + stab1(z0m / obu));
+        o.ustar = VKC * um / (L.lm - stab1(zeta) + stab1(z0m / obu));
     else if (zeta <= 1.0)
-        o.ustar = VKC * um / (cm_log(zldis / z0m) + 5.0 * zeta - 5.0 * z0m / obu);
+        o.ustar = VKC * um / (L.lm + 5.0 * zeta - 5.0 * z0m / obu);
     else
         o.ustar = VKC * um / (cm_log(obu / z0m) + 5.0 - 5.0 * z0m / obu + (5.0 * cm_log(zeta) + zeta - 1.0));
+    o.temp1 = temp_profile(ht - displa, z0h, obu, L.lh);
+    if (hq == ht && z0q == z0h) o.temp2 = o.temp1;
+    else o.temp2 = temp_profile(hq - displa, z0q, obu, L.lq);
+    const double zm1 = fmin90(zeta, 1.0);
+    const double fmnew = (zm1 < 0.0) ? fm_profile(zm1) : -5.0 * zm1;
+    o.fm = (iter == 1) ? fmnew : 0.5 * (o.fm + fmnew);
+}
+
+// The diagnostics FrictionVelocity computes beside them (deposition
+// velocity, 10-m winds, 2-m profile factors).  Nothing in the iteration reads
+// them, so they are evaluated once, from the inputs (obu, um) and the ustar
+// and fm of the last iteration: the same values the reference keeps from its
+// last call.
+__device__ void friction_velocity_diag(double hu, double displa, double z0m, double z0h, double z0q, double obu,
+                                       double ur, double um, FV &o)
+{
+    const double zetam = 1.574;
+    const double zldis = hu - displa;
+    const double zeta = zldis / obu;
     if (zeta < 0.0) o.vds = 2.e-3 * o.ustar * (1.0 + cm_pow(300.0 / (-obu), 0.666));
     else o.vds = 2.e-3 * o.ustar;
     if (zldis - z0m <= 10.0) {
@@ -197,16 +225,9 @@ __device__ void friction_velocity(double hu, double ht, double hq, double displa
                                                + (5.0 * cm_log(zeta) + zeta - 1.0)));
     }
     o.va = um;
-    o.temp1 = temp_profile(ht - displa, z0h, obu);
-    if (hq == ht && z0q == z0h) o.temp2 = o.temp1;
-    else o.temp2 = temp_profile(hq - displa, z0q, obu);
-    o.temp12m = temp_profile(2.0 + z0h, z0h, obu);
+    o.temp12m = temp_profile(2.0 + z0h, z0h, obu, cm_log((2.0 + z0h) / z0h));
     if (z0q == z0h) o.temp22m = o.temp12m;
-    else o.temp22m = temp_profile(2.0 + z0q, z0q, obu);
-    zeta = zldis / obu;   // zldis is hu - displa again
-    const double zm1 = fmin90(zeta, 1.0);
-    const double fmnew = (zm1 < 0.0) ? fm_profile(zm1) : -5.0 * zm1;
-    o.fm = (iter == 1) ? fmnew : 0.5 * (o.fm + fmnew);
+    else o.temp22m = temp_profile(2.0 + z0q, z0q, obu, cm_log((2.0 + z0q) / z0q));
     double zeta10 = fmin90(10.0 / obu, 1.0);
     if (zeta == 0.0) zeta10 = 0.0;
     const double fm10 = (zeta10 < 0.0) ? fm_profile(zeta10) : -5.0 * zeta10;
@@ -897,6 +918,7 @@ __device__ __forceinline__ bool canopy_patch(const CView &V, const CSwitch &sw, 
     double rh_af = 0.0, vpd = 0.0;
     FV fv;
     fv.fm = first ? 0.0 : W.fm[i0];
+    double obu_fv = obu, um_fv = um;      // the inputs of the last FrictionVelocity call
     const double laisun = S.laisun[pp], laisha = S.laisha[pp], fdry = S.fdry[pp], fwet = S.fwet[pp];
     const double liqcan0 = S.liqcan[pp], snocan0 = S.snocan[pp];
     const double h2ocan = liqcan0 + snocan0;
@@ -915,16 +937,23 @@ __device__ __forceinline__ bool canopy_patch(const CView &V, const CSwitch &sw, 
     const double lw_grnd = (frac_sno * p4(t_top) + (1.0 - frac_sno - frac_h2osfc) * p4(t_soi1)
                             + frac_h2osfc * p4(t_h2osfc));
     const double lai_dl_eff = P.lai_dl * (1.0 - fmin90(col.snow_depth[cc] / P.z_dl, 1.0));
+    // step constants of the iteration
+    const double w_veg = cm_exp(-(elai + esai));
+    const double dl_ext = 1.0 - cm_exp(-lai_dl_eff);
+    FVLogs fvlog;
+    fvlog.lm = cm_log(zldis / z0mv);
+    fvlog.lh = cm_log((ht - displa) / z0hv);
+    fvlog.lq = cm_log((hq - displa) / z0qv);
 
     int itlef = first ? 0 : (int)W.itlef[i0];
     bool finished = false;
     for (int it = 0; it < budget && itlef <= sw.itmax; ++it) {
-        friction_velocity(hu, ht, hq, displa, z0mv, z0hv, z0qv, obu, itlef + 1, ur, um, fv);
+        friction_velocity(hu, ht, hq, displa, z0mv, z0hv, z0qv, fvlog, obu, itlef + 1, um, fv);
+        obu_fv = obu;
+        um_fv = um;
         ustar = fv.ustar;
         temp1 = fv.temp1;
         temp2 = fv.temp2;
-        temp12m = fv.temp12m;
-        temp22m = fv.temp22m;
         tlbef = t_veg;
         del2 = del;
         ram1 = 1.0 / (ustar * ustar / um);
@@ -934,7 +963,7 @@ __device__ __forceinline__ bool canopy_patch(const CView &V, const CSwitch &sw, 
         const double uuc = fmin90(0.4, (0.03 * um / ustar));
         const double cf = P.cv / (sqrt(uaf) * sqrt(dleaf));
         rb = 1.0 / (cf * uaf);
-        const double w = cm_exp(-(elai + esai));
+        const double w = w_veg;
         const double csoilb = VKC / (P.a_coef * cm_pow(z0mg * uaf / NU, P.a_exp));
         const double ri = (GRAV * htop * (taf - t_grnd)) / (taf * (uaf * uaf));
         double csoilcn;
@@ -992,7 +1021,7 @@ __device__ __forceinline__ bool canopy_patch(const CView &V, const CSwitch &sw, 
         }
         const double wtaq = fvn / raw1;
         const double wtlq = fvn * (elai + esai) / rb * rpp;
-        const double rdl = (1.0 - cm_exp(-lai_dl_eff)) / (0.004 * uaf);
+        const double rdl = dl_ext / (0.004 * uaf);
         if (delq < 0.0) {
             wtgq = fvn / (raw2 + rdl);
         } else {
@@ -1101,6 +1130,10 @@ __device__ __forceinline__ bool canopy_patch(const CView &V, const CSwitch &sw, 
         if (bad) report_error(err, p, ENDRUN);
         return true;
     }
+
+    friction_velocity_diag(hu, displa, z0mv, z0hv, z0qv, obu_fv, ur, um_fv, fv);
+    temp12m = fv.temp12m;
+    temp22m = fv.temp22m;
 
     // ---- fluxes after the iteration (:1467-1615)
     const double fr = frac_rad_abs_by_stem;
