@@ -247,12 +247,12 @@ def main():
     ap.add_argument("--filter-sort", action="store_true",
                     help="bin through a sorted column filter instead of storing the batch in binned order")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--no-graph", action="store_true", help="launch each step eagerly instead of replaying "
+                                                            "the captured step")
     ap.add_argument("--lanes", type=int, default=0, help="force lanes per column in the Richards solve")
     ap.add_argument("--single-kernel", action="store_true", help="force the one-kernel fused step")
     ap.add_argument("--order", choices=["bins", "substeps"], default="substeps",
                     help="column order: (bedrock, snow) bins, or also by the sub-steps of the warm-up step")
-    ap.add_argument("--kernel-reps", type=int, default=8,
-                    help="launches per kernel in the per-kernel timing (roofline) phase")
     ap.add_argument("--canopy-ncol", type=int, default=262144,
                     help="columns of the full-column (CanopyFluxes) measurement; 0 skips it")
     ap.add_argument("--canopy-steps", type=int, default=6)
@@ -321,18 +321,28 @@ def main():
         for _ in range(args.warmup):
             ctx.step_dev(ds, filter_dev=filt, plain=plain, structs=structs)
         ctx.sync()
+        # the fused step is captured once and replayed: one graph launch per step
+        graph = None
+        if not plain and not args.no_graph:
+            graph = ctx.graph_capture(ds, nsteps=1, canopy=False, filterc_dev=filt,
+                                      structs=(*structs, None))
         # events on the library's own stream, around each step's launch(es)
         starts, stops = [], []
         t0 = time.perf_counter()
         for k in range(args.steps):
             starts.append(ctx.record())
-            ctx.step_dev(ds, filter_dev=filt, plain=plain, structs=structs)
+            if graph is not None:
+                ctx.graph_launch(graph)
+            else:
+                ctx.step_dev(ds, filter_dev=filt, plain=plain, structs=structs)
             stops.append(ctx.record())
         ctx.sync()
         wall = time.perf_counter() - t0
         per = [ctx.elapsed_ms(a, b) for a, b in zip(starts, stops)]
         for e in starts + stops:
             ctx.destroy_event(e)
+        if graph is not None:
+            ctx.graph_destroy(graph)
         return wall, float(np.mean(per)), ds
 
     wall, step_ms, ds = run(plain=False)
@@ -348,14 +358,15 @@ def main():
         res["plain_column_steps_per_s"] = args.ncol * args.steps / wall_p
         res["fused_over_plain"] = value / res["plain_column_steps_per_s"]
 
-    # per-kernel launch times on the same state (events on the library stream):
-    # heat solve, water fractions, Richards -- the step's work launched separately
+    # per-kernel launch times over the same trajectory as the timed steps
+    # (events on the library stream): heat solve, water fractions, Richards --
+    # the step's work launched as separate kernels, averaged over the steps
     ds = DeviceState(st, ctx)
-    for _ in range(max(args.warmup, 2)):      # past the first step's extra Richards sub-steps
+    for _ in range(args.warmup):
         ctx.step_dev(ds, filter_dev=filt)
     t_heat, t_frac, t_rich = time_events(ctx, [lambda: ctx.soiltemperature_dev(ds, filt),
                                                lambda: ctx.soilwater_fractions_dev(ds, filt),
-                                               lambda: ctx.soilwater_dev(ds, filt)], args.kernel_reps)
+                                               lambda: ctx.soilwater_dev(ds, filt)], args.steps)
     ds.free()
     res["kernel_ms"] = {"soiltemp (k_soiltemp)": t_heat, "fractions (k_soilwater_fractions)": t_frac,
                         "richards (k_soilwater_g)": t_rich}
@@ -400,7 +411,8 @@ def main():
         "data": "synthetic (seeded generator, inputs resident in HBM)",
         "config": {"workload": "uneven columns: 65,536 soil columns, snow 0-12 layers, bedrock 5-20, "
                                "SoilTemperature + SoilWater, dtime 1800 s, 48-step day",
-                   "ncol": args.ncol, "binning": binning},
+                   "ncol": args.ncol, "binning": binning,
+                   "launch": "eager" if args.no_graph else "captured step replayed as one graph launch"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get(kname),
                      "kernel": kname, "launch_ms": kms, "alg_bytes_per_launch": kbytes},

@@ -735,6 +735,89 @@ int ctsm_land_step_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_sta
     return ctsm_soil_step_fused_dev(ctx, lay, col, pch, nc, filterc_dev, dtime);
 }
 
+// ---- captured step sequences ---------------------------------------------
+
+struct ctsm_graph {
+    ctsm_ctx *ctx = nullptr;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ex = nullptr;
+    DevBuf work, cwork;   // workspaces the captured kernels use
+};
+
+int ctsm_graph_destroy(ctsm_graph *gr)
+{
+    if (!gr) return CTSM_OK;
+    if (gr->ctx) {
+        hipSetDevice(gr->ctx->device);
+        hipStreamSynchronize(gr->ctx->stream);
+    }
+    if (gr->ex) hipGraphExecDestroy(gr->ex);
+    if (gr->g) hipGraphDestroy(gr->g);
+    gr->work.release();
+    gr->cwork.release();
+    delete gr;
+    return CTSM_OK;
+}
+
+int ctsm_graph_capture(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                       const ctsm_patch_state *pch, const ctsm_canopy_state *can, int32_t nc,
+                       const int32_t *filterc_dev, int32_t np, const int32_t *filterp_dev, double dtime,
+                       int32_t nsteps, ctsm_graph **out)
+{
+    if (!out) return CTSM_E_ARG;
+    *out = nullptr;
+    if (!ctx || !col || !pch || nc < 0 || np < 0 || nsteps < 1 || check_layout(lay) ||
+        (!filterc_dev && nc > lay->ldc) || (can && (check_canopy_layout(ctx, lay) || (!filterp_dev && np > lay->ldp))))
+        return CTSM_E_ARG;
+    hipError_t he = hipSetDevice(ctx->device);
+    if (he != hipSuccess) return hip_fail(ctx, he);
+    ctsm_graph *gr = new ctsm_graph;
+    gr->ctx = ctx;
+    // size the graph's own workspaces first: nothing may allocate under capture
+    he = gr->work.ensure(std::max<size_t>(soiltemp_workspace_doubles(nc), 1) * sizeof(double));
+    const size_t cwb = can ? canopy_work_bytes(np, ctx->cfg.itmax_canopy_fluxes, ctx->tun.canopy_first_iterations,
+                                               ctx->tun.canopy_round_iterations)
+                           : 0;
+    if (he == hipSuccess && cwb) he = gr->cwork.ensure(cwb);
+    if (he != hipSuccess) {
+        ctsm_graph_destroy(gr);
+        return hip_fail(ctx, he);
+    }
+    std::swap(ctx->work, gr->work);
+    std::swap(ctx->cwork, gr->cwork);
+    int rc = CTSM_OK;
+    he = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    if (he == hipSuccess) {
+        for (int s = 0; s < nsteps && rc == CTSM_OK; ++s)
+            rc = can ? ctsm_land_step_dev(ctx, lay, col, pch, can, nc, filterc_dev, np, filterp_dev, dtime)
+                     : ctsm_soil_step_fused_dev(ctx, lay, col, pch, nc, filterc_dev, dtime);
+        hipError_t he2 = hipStreamEndCapture(ctx->stream, &gr->g);
+        if (he2 != hipSuccess && rc == CTSM_OK) rc = hip_fail(ctx, he2);
+    } else {
+        rc = hip_fail(ctx, he);
+    }
+    std::swap(ctx->work, gr->work);
+    std::swap(ctx->cwork, gr->cwork);
+    if (rc == CTSM_OK) {
+        he = hipGraphInstantiate(&gr->ex, gr->g, nullptr, nullptr, 0);
+        if (he != hipSuccess) rc = hip_fail(ctx, he);
+    }
+    if (rc != CTSM_OK) {
+        gr->ctx = nullptr;      // nothing of it was launched
+        ctsm_graph_destroy(gr);
+        return rc;
+    }
+    *out = gr;
+    return CTSM_OK;
+}
+
+int ctsm_graph_launch(ctsm_ctx *ctx, ctsm_graph *gr)
+{
+    if (!ctx || !gr || gr->ctx != ctx || !gr->ex) return CTSM_E_ARG;
+    hipError_t he = hipGraphLaunch(gr->ex, ctx->stream);
+    return he == hipSuccess ? CTSM_OK : hip_fail(ctx, he);
+}
+
 static int canopy_mirror(ctsm_ctx *ctx, const ctsm_layout *L, const ctsm_canopy_state *can, ctsm_canopy_state *dcan,
                          bool to_device)
 {

@@ -31,6 +31,7 @@ EXPORTS = [
     "ctsm_event_record", "ctsm_event_elapsed", "ctsm_event_destroy", "ctsm_sync", "ctsm_stream",
     "ctsm_set_canopy_params", "ctsm_canopyfluxes", "ctsm_root_sink", "ctsm_canopyfluxes_dev",
     "ctsm_root_sink_dev", "ctsm_land_step_dev", "ctsm_memcpy2d",
+    "ctsm_graph_capture", "ctsm_graph_launch", "ctsm_graph_destroy",
 ]
 
 
@@ -65,6 +66,10 @@ def load():
     L.ctsm_soilwater_fractions_dev.argtypes = col + [C.c_int32, C.c_void_p]
     L.ctsm_soiltemperature_dev.argtypes = col + [P(abi.PatchState), C.c_int32, C.c_void_p, C.c_double, C.c_int32]
     L.ctsm_soil_step_fused_dev.argtypes = col + [P(abi.PatchState), C.c_int32, C.c_void_p, C.c_double]
+    L.ctsm_graph_capture.argtypes = col + [P(abi.PatchState), P(abi.CanopyState), C.c_int32, C.c_void_p, C.c_int32,
+                                           C.c_void_p, C.c_double, C.c_int32, P(C.c_void_p)]
+    L.ctsm_graph_launch.argtypes = [C.c_void_p, C.c_void_p]
+    L.ctsm_graph_destroy.argtypes = [C.c_void_p]
     L.ctsm_sync.argtypes = [C.c_void_p]
     L.ctsm_stream.argtypes = [C.c_void_p]
     L.ctsm_stream.restype = C.c_void_p
@@ -213,6 +218,32 @@ class Context:
         fp = C.c_void_p(filterp_dev) if filterp_dev is not None else None
         self._check(self.L.ctsm_land_step_dev(self.h, C.byref(ds.layout()), C.byref(cs), C.byref(ps), C.byref(can),
                                               nc, fc, np_, fp, dtime), "land step")
+
+    def graph_capture(self, ds, nsteps=1, canopy=True, filterc_dev=None, nc=None, filterp_dev=None, np_=None,
+                      dtime=1800.0, structs=None):
+        """Capture `nsteps` land steps (canopy=True) or soil steps into one
+        replayable graph over the DeviceState; returns the graph handle."""
+        if structs is None:
+            cs, ps = ds.structs()
+            can = ds.canopy_struct() if canopy else None
+        else:
+            cs, ps, can = structs
+            can = can if canopy else None
+        nc = ds.dims.ncol if nc is None else nc
+        np_ = (ds.dims.npatch if np_ is None else np_) if canopy else 0
+        fc = C.c_void_p(filterc_dev) if filterc_dev is not None else None
+        fp = C.c_void_p(filterp_dev) if filterp_dev is not None else None
+        g = C.c_void_p()
+        self._check(self.L.ctsm_graph_capture(self.h, C.byref(ds.layout()), C.byref(cs), C.byref(ps),
+                                              C.byref(can) if can is not None else None, nc, fc, np_, fp, dtime,
+                                              nsteps, C.byref(g)), "ctsm_graph_capture")
+        return g.value
+
+    def graph_launch(self, g):
+        self._check(self.L.ctsm_graph_launch(self.h, C.c_void_p(g)), "ctsm_graph_launch")
+
+    def graph_destroy(self, g):
+        self.L.ctsm_graph_destroy(C.c_void_p(g))
 
     # ---- device memory / events ------------------------------------------
     def alloc(self, nbytes):

@@ -502,6 +502,26 @@ int ctsm_land_step_dev(ctsm_ctx *ctx, const ctsm_layout *lay,
                        int32_t num_filterc, const int32_t *filterc_dev,
                        int32_t num_filterp, const int32_t *filterp_dev, double dtime);
 
+/* ---- captured step sequences (no counterpart in the reference: a batched
+ * run replays the same launches every step, so they are captured once into a
+ * HIP graph and replayed as one launch) -----------------------------------------
+ * ctsm_graph_capture records `nsteps` consecutive steps over device-resident
+ * state, with the context's current tuning: ctsm_land_step_dev when `can` is
+ * not NULL, else ctsm_soil_step_fused_dev (pch is then the soil step's patch
+ * state).  The graph holds the state pointers, filters and dtime given here
+ * and its own workspace; nothing is launched by the capture.  Each
+ * ctsm_graph_launch enqueues the steps on the context's stream, exactly the
+ * kernels the eager calls launch; failures are reported by the next ctsm_sync
+ * as for eager steps.  Destroy graphs before their context. */
+typedef struct ctsm_graph ctsm_graph;
+int ctsm_graph_capture(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                       const ctsm_patch_state *pch, const ctsm_canopy_state *can,
+                       int32_t num_filterc, const int32_t *filterc_dev,
+                       int32_t num_filterp, const int32_t *filterp_dev, double dtime,
+                       int32_t nsteps, ctsm_graph **out);
+int ctsm_graph_launch(ctsm_ctx *ctx, ctsm_graph *graph);
+int ctsm_graph_destroy(ctsm_graph *graph);
+
 /* ---- device memory and timing helpers --------------------------------------
  * Hosts need no other GPU runtime: device-resident state is allocated, filled
  * and read back through these (all on the context's device and stream). */
