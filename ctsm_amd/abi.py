@@ -321,6 +321,26 @@ CANOPY_FIELDS = [
     ("canopy_err", "f64", "p"),
 ]
 
+# ctsm_soilflux_state, in declaration order (kind: "p" patch, "c" column, "sno" snow+soil levels)
+SOILFLUX_FIELDS = [
+    ("active", "i32", "p"),
+    ("t_ssbef", "f64", "sno"), ("t_h2osfc_bef", "f64", "c"),
+    ("eflx_sh_grnd", "f64", "p"), ("qflx_evap_soi", "f64", "p"), ("qflx_ev_snow", "f64", "p"),
+    ("qflx_ev_soil", "f64", "p"), ("qflx_ev_h2osfc", "f64", "p"),
+    ("cgrnds", "f64", "p"), ("cgrndl", "f64", "p"), ("ulrad", "f64", "p"), ("eflx_sh_veg", "f64", "p"),
+    ("eflx_sh_stem", "f64", "p"), ("qflx_evap_veg", "f64", "p"), ("qflx_tran_veg", "f64", "p"),
+    ("eflx_soil_grnd", "f64", "p"), ("eflx_soil_grnd_r", "f64", "p"), ("eflx_sh_tot", "f64", "p"),
+    ("eflx_sh_tot_r", "f64", "p"),
+    ("qflx_evap_tot", "f64", "p"), ("eflx_lh_tot", "f64", "p"), ("eflx_lh_tot_r", "f64", "p"),
+    ("qflx_evap_can", "f64", "p"), ("eflx_lh_vege", "f64", "p"), ("eflx_lh_vegt", "f64", "p"),
+    ("eflx_lh_grnd", "f64", "p"),
+    ("eflx_lwrad_out", "f64", "p"), ("eflx_lwrad_out_r", "f64", "p"), ("eflx_lwrad_net", "f64", "p"),
+    ("eflx_lwrad_net_r", "f64", "p"), ("t_skin", "f64", "p"),
+    ("qflx_liqevap_from_top_layer", "f64", "p"), ("qflx_solidevap_from_top_layer", "f64", "p"),
+    ("qflx_liqdew_to_top_layer", "f64", "p"), ("qflx_soliddew_to_top_layer", "f64", "p"),
+    ("errsoi_patch", "f64", "p"), ("errsoi_col", "f64", "c"),
+]
+
 CANOPY_PARAM_NAMES = [
     "cv", "a_coef", "a_exp", "csoilc", "z_dl", "lai_dl", "wind_min", "zetamaxstable",
     "kc25_coef", "ko25_coef", "cp25_yr2000", "kcha", "koha", "cpha",
@@ -352,6 +372,10 @@ class PftParams(C.Structure):
 
 class CanopyState(C.Structure):
     _fields_ = [(n, _CT[t]) for n, t, _ in CANOPY_FIELDS]
+
+
+class SoilFluxState(C.Structure):
+    _fields_ = [(n, _CT[t]) for n, t, _ in SOILFLUX_FIELDS]
 
 
 class ColState(C.Structure):

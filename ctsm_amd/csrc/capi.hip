@@ -39,6 +39,13 @@ void launch_canopy(hipStream_t, const ctsm_layout &, const ctsm_config &, const 
                    void *, int, int, bool);
 void launch_root_sink(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_col_state &,
                       const ctsm_canopy_state &, int, const int32_t *, ErrWord *);
+void launch_save_before(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_col_state &,
+                        const ctsm_soilflux_state &, int, const int32_t *);
+void launch_soilfluxes(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_col_state &,
+                       const ctsm_patch_state &, const ctsm_soilflux_state &, int, const int32_t *, int,
+                       const int32_t *, double, ErrWord *);
+void launch_balance(hipStream_t, const ctsm_layout &, const ctsm_config &, const double *, int, const int32_t *,
+                    unsigned long long *);
 }  // namespace ctsm
 
 using namespace ctsm;
@@ -155,6 +162,31 @@ const FieldMeta kCanopyGrid[] = {
     CN(near_local_noon, K_G, 4, false),
 };
 #undef CN
+// SoilFluxes state, split by index space
+#define SFP(name, elem, wr) {offsetof(ctsm_soilflux_state, name), K_P, elem, wr}
+const FieldMeta kSfPatch[] = {
+    SFP(active, 4, false),
+    SFP(eflx_sh_grnd, 8, true), SFP(qflx_evap_soi, 8, true), SFP(qflx_ev_snow, 8, true), SFP(qflx_ev_soil, 8, true),
+    SFP(qflx_ev_h2osfc, 8, true), SFP(cgrnds, 8, false), SFP(cgrndl, 8, false), SFP(ulrad, 8, false),
+    SFP(eflx_sh_veg, 8, false), SFP(eflx_sh_stem, 8, false), SFP(qflx_evap_veg, 8, false), SFP(qflx_tran_veg, 8, false),
+    SFP(eflx_soil_grnd, 8, true), SFP(eflx_soil_grnd_r, 8, true), SFP(eflx_sh_tot, 8, true),
+    SFP(eflx_sh_tot_r, 8, true), SFP(qflx_evap_tot, 8, true), SFP(eflx_lh_tot, 8, true), SFP(eflx_lh_tot_r, 8, true),
+    SFP(qflx_evap_can, 8, true), SFP(eflx_lh_vege, 8, true), SFP(eflx_lh_vegt, 8, true), SFP(eflx_lh_grnd, 8, true),
+    SFP(eflx_lwrad_out, 8, true), SFP(eflx_lwrad_out_r, 8, true), SFP(eflx_lwrad_net, 8, true),
+    SFP(eflx_lwrad_net_r, 8, true), SFP(t_skin, 8, true), SFP(qflx_liqevap_from_top_layer, 8, true),
+    SFP(qflx_solidevap_from_top_layer, 8, true), SFP(qflx_liqdew_to_top_layer, 8, true),
+    SFP(qflx_soliddew_to_top_layer, 8, true), SFP(errsoi_patch, 8, true),
+};
+#undef SFP
+const FieldMeta kSfCol[] = {
+    {offsetof(ctsm_soilflux_state, t_ssbef), K_SNO, 8, false},
+    {offsetof(ctsm_soilflux_state, t_h2osfc_bef), K_C, 8, false},
+    {offsetof(ctsm_soilflux_state, errsoi_col), K_C, 8, true},
+};
+constexpr int kNSfP = sizeof(kSfPatch) / sizeof(kSfPatch[0]);
+constexpr int kNSfC = sizeof(kSfCol) / sizeof(kSfCol[0]);
+static_assert((kNSfP + kNSfC) * sizeof(void *) == sizeof(ctsm_soilflux_state),
+              "every ctsm_soilflux_state member has a mirror entry");
 constexpr int kNCanP = sizeof(kCanopyPatch) / sizeof(kCanopyPatch[0]);
 constexpr int kNCanC = sizeof(kCanopyCol) / sizeof(kCanopyCol[0]);
 constexpr int kNCanG = sizeof(kCanopyGrid) / sizeof(kCanopyGrid[0]);
@@ -235,6 +267,9 @@ struct ctsm_ctx {
     ctsm_canopy_params cprm{};         // canopy param-file scalars
     DevBuf pft;                        // PFT table (N_PFT_FIELDS x CTSM_MAX_PFT doubles)
     DevBuf cwork;                      // CanopyFluxes round state, patch lists and counters
+    DevBuf filt2;                      // second filter (drop-in calls taking column and patch filters)
+    DevBuf bal;                        // balance-check reduction word
+    std::vector<DevBuf> sfp, sfc;      // SoilFluxes drop-in mirrors
     bool have_canopy = false;
     int32_t last_code = 0, last_index = 0;
     std::mutex mu;
@@ -513,6 +548,10 @@ int ctsm_destroy(ctsm_ctx *ctx)
     for (auto &b : ctx->cang) b.release();
     ctx->pft.release();
     ctx->cwork.release();
+    ctx->filt2.release();
+    ctx->bal.release();
+    for (auto &b : ctx->sfp) b.release();
+    for (auto &b : ctx->sfc) b.release();
     if (ctx->err) hipFree(ctx->err);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -748,11 +787,11 @@ int ctsm_graph_destroy(ctsm_graph *gr)
 {
     if (!gr) return CTSM_OK;
     if (gr->ctx) {
-        hipSetDevice(gr->ctx->device);
-        hipStreamSynchronize(gr->ctx->stream);
+        (void)hipSetDevice(gr->ctx->device);
+        (void)hipStreamSynchronize(gr->ctx->stream);
     }
-    if (gr->ex) hipGraphExecDestroy(gr->ex);
-    if (gr->g) hipGraphDestroy(gr->g);
+    if (gr->ex) (void)hipGraphExecDestroy(gr->ex);
+    if (gr->g) (void)hipGraphDestroy(gr->g);
     gr->work.release();
     gr->cwork.release();
     delete gr;
@@ -841,6 +880,168 @@ static int upload_filter(ctsm_ctx *ctx, int32_t n, const int32_t *filter, const 
     if (he != hipSuccess) return hip_fail(ctx, he);
     *dfilt = (const int32_t *)ctx->filt.p;
     return CTSM_OK;
+}
+
+// ---- SoilFluxes and the soil energy balance check ---------------------------
+
+static bool sf_complete(const ctsm_soilflux_state *sf)
+{
+    for (int f = 0; f < kNSfP; ++f)
+        if (!*(void *const *)((const char *)sf + kSfPatch[f].offset)) return false;
+    for (int f = 0; f < kNSfC; ++f)
+        if (!*(void *const *)((const char *)sf + kSfCol[f].offset)) return false;
+    return true;
+}
+
+int ctsm_soilfluxes_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                        const ctsm_patch_state *pch, const ctsm_soilflux_state *sf, int32_t nc,
+                        const int32_t *filterc_dev, int32_t np, const int32_t *filterp_dev, double dtime)
+{
+    if (!ctx || !col || !pch || !sf || nc < 0 || np < 0 || check_layout(lay) || lay->ldp <= 0 ||
+        (!filterc_dev && nc > lay->ldc) || (!filterp_dev && np > lay->ldp) || !sf_complete(sf))
+        return CTSM_E_ARG;
+    launch_soilfluxes(ctx->stream, *lay, ctx->cfg, *col, *pch, *sf, nc, filterc_dev, np, filterp_dev, dtime, ctx->err);
+    return finish_launch(ctx);
+}
+
+int ctsm_soil_balance_check_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_soilflux_state *sf, int32_t n,
+                                const int32_t *filter_dev, double *errsoi_max, int32_t *index)
+{
+    if (!ctx || !sf || !sf->errsoi_col || !errsoi_max || !index || n < 0 || check_layout(lay) ||
+        (!filter_dev && n > lay->ldc))
+        return CTSM_E_ARG;
+    hipError_t he = ctx->bal.ensure(2 * sizeof(unsigned long long));
+    if (he != hipSuccess) return hip_fail(ctx, he);
+    launch_balance(ctx->stream, *lay, ctx->cfg, sf->errsoi_col, n, filter_dev, (unsigned long long *)ctx->bal.p);
+    int rc = finish_launch(ctx);
+    if (rc) return rc;
+    unsigned long long w[2] = {0ull, 0ull};
+    he = hipMemcpyAsync(w, ctx->bal.p, sizeof(w), hipMemcpyDeviceToHost, ctx->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+    if (he != hipSuccess) return hip_fail(ctx, he);
+    double m;
+    std::memcpy(&m, &w[0], sizeof(m));
+    *errsoi_max = m;
+    *index = (n > 0 && w[1] != 0x7fffffffull) ? (int32_t)w[1] : 0;
+    return CTSM_OK;
+}
+
+int ctsm_soilfluxes(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col, const ctsm_patch_state *pch,
+                    const ctsm_soilflux_state *sf, int32_t nc, const int32_t *filterc, int32_t np,
+                    const int32_t *filterp, double dtime)
+{
+    if (!ctx || !col || !pch || !sf || nc < 0 || np < 0 || check_layout(lay) || lay->ldp <= 0 ||
+        (!filterc && nc > lay->ldc) || (!filterp && np > lay->ldp) || !sf_complete(sf))
+        return CTSM_E_ARG;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipError_t he = hipSetDevice(ctx->device);
+    if (he != hipSuccess) return hip_fail(ctx, he);
+    for (int i = 0; filterc && i < nc; ++i)
+        if (filterc[i] < lay->begc_alloc || filterc[i] >= lay->begc_alloc + lay->ldc) return CTSM_E_ARG;
+    for (int i = 0; filterp && i < np; ++i)
+        if (filterp[i] < lay->begp_alloc || filterp[i] >= lay->begp_alloc + lay->ldp) return CTSM_E_ARG;
+    // only the members SoilFluxes reads are mirrored
+    ctsm_col_state hc{};
+    hc.snl = col->snl; hc.patchi = col->patchi; hc.patchf = col->patchf;
+    hc.frac_sno_eff = col->frac_sno_eff; hc.frac_h2osfc = col->frac_h2osfc;
+    hc.h2osoi_liq = col->h2osoi_liq; hc.h2osoi_ice = col->h2osoi_ice; hc.t_soisno = col->t_soisno;
+    hc.t_h2osfc = col->t_h2osfc; hc.t_grnd = col->t_grnd; hc.emg = col->emg; hc.forc_lwrad = col->forc_lwrad;
+    hc.htvp = col->htvp; hc.xmf = col->xmf; hc.xmf_h2osfc = col->xmf_h2osfc; hc.fact = col->fact;
+    hc.c_h2osfc = col->c_h2osfc; hc.eflx_h2osfc_to_snow = col->eflx_h2osfc_to_snow;
+    if (!hc.snl || !hc.patchi || !hc.patchf || !hc.frac_sno_eff || !hc.frac_h2osfc || !hc.h2osoi_liq ||
+        !hc.h2osoi_ice || !hc.t_soisno || !hc.t_h2osfc || !hc.t_grnd || !hc.emg || !hc.forc_lwrad || !hc.htvp ||
+        !hc.xmf || !hc.xmf_h2osfc || !hc.fact || !hc.c_h2osfc || !hc.eflx_h2osfc_to_snow)
+        return CTSM_E_ARG;
+    ctsm_patch_state hp{};
+    hp.column = pch->column; hp.wtcol = pch->wtcol; hp.frac_veg_nosno = pch->frac_veg_nosno;
+    hp.sabg_soil = pch->sabg_soil; hp.sabg_snow = pch->sabg_snow; hp.dlrad = pch->dlrad;
+    if (!hp.column || !hp.wtcol || !hp.frac_veg_nosno || !hp.sabg_soil || !hp.sabg_snow || !hp.dlrad)
+        return CTSM_E_ARG;
+    const Range cr{lay->begc_alloc, lay->begc_alloc + lay->ldc - 1};
+    const Range pr{lay->begp_alloc, lay->begp_alloc + lay->ldp - 1};
+    ctsm_col_state dcol{};
+    ctsm_patch_state dpch{};
+    ctsm_soilflux_state dsf{};
+    const int ncf = (int)(sizeof(kColFields) / sizeof(kColFields[0]));
+    const int npf = (int)(sizeof(kPatchFields) / sizeof(kPatchFields[0]));
+    int rc = mirror(ctx, kColFields, ncf, &hc, &dcol, ctx->colmir, lay->begc_alloc, lay->ldc, cr, true);
+    if (!rc) rc = mirror(ctx, kPatchFields, npf, &hp, &dpch, ctx->pchmir, lay->begp_alloc, lay->ldp, pr, true);
+    if (!rc) rc = mirror(ctx, kSfPatch, kNSfP, sf, &dsf, ctx->sfp, lay->begp_alloc, lay->ldp, pr, true);
+    if (!rc) rc = mirror(ctx, kSfCol, kNSfC, sf, &dsf, ctx->sfc, lay->begc_alloc, lay->ldc, cr, true);
+    if (rc) return rc;
+    const int32_t *dfc = nullptr, *dfp = nullptr;
+    if ((rc = upload_filter(ctx, nc, filterc, &dfc))) return rc;
+    if (filterp && np > 0) {
+        he = ctx->filt2.ensure((size_t)np * sizeof(int32_t));
+        if (he == hipSuccess)
+            he = hipMemcpyAsync(ctx->filt2.p, filterp, (size_t)np * sizeof(int32_t), hipMemcpyHostToDevice,
+                                ctx->stream);
+        if (he != hipSuccess) return hip_fail(ctx, he);
+        dfp = (const int32_t *)ctx->filt2.p;
+    }
+    launch_soilfluxes(ctx->stream, *lay, ctx->cfg, dcol, dpch, dsf, nc, dfc, np, dfp, dtime, ctx->err);
+    if ((rc = finish_launch(ctx))) return rc;
+    rc = mirror(ctx, kSfPatch, kNSfP, sf, &dsf, ctx->sfp, lay->begp_alloc, lay->ldp, pr, false);
+    if (!rc) rc = mirror(ctx, kSfCol, kNSfC, sf, &dsf, ctx->sfc, lay->begc_alloc, lay->ldc, cr, false);
+    if (rc) return rc;
+    return sync_and_check(ctx);
+}
+
+int ctsm_soil_balance_check(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_soilflux_state *sf, int32_t n,
+                            const int32_t *filterc, double *errsoi_max, int32_t *index)
+{
+    if (!ctx || !sf || !sf->errsoi_col || !errsoi_max || !index || n < 0 || check_layout(lay) ||
+        (!filterc && n > lay->ldc))
+        return CTSM_E_ARG;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipError_t he = hipSetDevice(ctx->device);
+    if (he != hipSuccess) return hip_fail(ctx, he);
+    for (int i = 0; filterc && i < n; ++i)
+        if (filterc[i] < lay->begc_alloc || filterc[i] >= lay->begc_alloc + lay->ldc) return CTSM_E_ARG;
+    ctsm_soilflux_state hs{};
+    hs.errsoi_col = sf->errsoi_col;
+    ctsm_soilflux_state dsf{};
+    const Range cr{lay->begc_alloc, lay->begc_alloc + lay->ldc - 1};
+    int rc = mirror(ctx, kSfCol, kNSfC, &hs, &dsf, ctx->sfc, lay->begc_alloc, lay->ldc, cr, true);
+    if (rc) return rc;
+    const int32_t *dfc = nullptr;
+    if ((rc = upload_filter(ctx, n, filterc, &dfc))) return rc;
+    return ctsm_soil_balance_check_dev(ctx, lay, &dsf, n, dfc, errsoi_max, index);
+}
+
+int ctsm_land_step_full_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                            const ctsm_patch_state *pch, const ctsm_canopy_state *can, const ctsm_soilflux_state *sf,
+                            int32_t nc, const int32_t *filterc_dev, int32_t np, const int32_t *filterp_dev,
+                            double dtime)
+{
+    if (!ctx || !col || !pch || !can || !sf || nc < 0 || np < 0 || check_canopy_layout(ctx, lay) ||
+        (!filterc_dev && nc > lay->ldc) || (!filterp_dev && np > lay->ldp) || !sf_complete(sf))
+        return CTSM_E_ARG;
+    // the arrays one routine writes and the next reads must coincide
+    if (pch->dlrad != can->dlrad || pch->cgrnd != can->cgrnd || pch->eflx_sh_grnd != can->eflx_sh_grnd ||
+        pch->eflx_sh_snow != can->eflx_sh_snow || pch->eflx_sh_soil != can->eflx_sh_soil ||
+        pch->eflx_sh_h2osfc != can->eflx_sh_h2osfc || pch->qflx_evap_soi != can->qflx_evap_soi ||
+        pch->qflx_ev_snow != can->qflx_ev_snow || pch->qflx_ev_soil != can->qflx_ev_soil ||
+        pch->qflx_ev_h2osfc != can->qflx_ev_h2osfc || col->qflx_rootsoi != can->qflx_rootsoi_col ||
+        sf->eflx_sh_grnd != can->eflx_sh_grnd || sf->qflx_evap_soi != can->qflx_evap_soi ||
+        sf->qflx_ev_snow != can->qflx_ev_snow || sf->qflx_ev_soil != can->qflx_ev_soil ||
+        sf->qflx_ev_h2osfc != can->qflx_ev_h2osfc || sf->cgrnds != can->cgrnds || sf->cgrndl != can->cgrndl ||
+        sf->ulrad != can->ulrad || sf->eflx_sh_veg != can->eflx_sh_veg || sf->eflx_sh_stem != can->eflx_sh_stem ||
+        sf->qflx_evap_veg != can->qflx_evap_veg || sf->qflx_tran_veg != can->qflx_tran_veg ||
+        sf->t_skin != can->t_skin || sf->active != can->active)
+        return CTSM_E_ARG;
+    int rc;
+    if ((rc = ensure_work(ctx, soiltemp_workspace_doubles(nc)))) return rc;
+    launch_save_before(ctx->stream, *lay, ctx->cfg, *col, *sf, nc, filterc_dev);
+    if ((rc = run_canopy(ctx, lay, *col, *can, np, filterp_dev, dtime, true))) return rc;
+    const View V = make_view(ctx, *lay);
+    launch_soiltemp_fractions(ctx->stream, V, ctx->prm, *col, *pch, nc, filterc_dev, dtime, (double *)ctx->work.p,
+                              nc, ctx->err, ctx->tun.soiltemp_block);
+    launch_soilfluxes(ctx->stream, *lay, ctx->cfg, *col, *pch, *sf, nc, filterc_dev, np, filterp_dev, dtime, ctx->err);
+    launch_root_sink(ctx->stream, *lay, ctx->cfg, *col, *can, nc, filterc_dev, ctx->err);
+    launch_soilwater(ctx->stream, V, ctx->prm, *col, nc, filterc_dev, dtime, ctx->err, ctx->tun.richards_lanes,
+                     ctx->tun.richards_block);
+    return finish_launch(ctx);
 }
 
 int ctsm_canopyfluxes(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,

@@ -1,0 +1,257 @@
+// soilfluxes.hip -- SoilFluxes and the soil energy balance check on gfx950.
+//
+// Replaces SoilFluxes (SoilFluxesMod.F90:38-518) for soil / crop patches and
+// the soil part of EnergyBalanceCheck (BalanceCheckMod.F90:1101-1119), and
+// saves the before-step temperatures of BiogeophysPreFluxCalcsMod.F90:302-331.
+// All of it is a streaming per-patch / per-column update (HBM-bound, no
+// solve), so each kernel is one lane per patch or column with the arrays
+// read and written once, column-fastest as the reference lays them out:
+//
+//   k_save_before     per column: t_ssbef(c, :) = t_soisno(c, :), t_h2osfc_bef
+//   k_soilfluxes      per patch: the reference's patch loops :277-570 fused in
+//                     their order (a patch's values depend only on its own
+//                     column), ending with errsoi_patch
+//   k_errsoi_p2c      per column: p2c_1d_filter of errsoi (subgridAveMod.F90:312-318)
+//   k_balance_max /   max |errsoi_col| over the filter (64-bit atomic max on the
+//   k_balance_loc     bits of a non-negative double, which order as the values)
+//                     and its lowest column index (atomic min), as maxloc
+//
+// Every expression keeps the reference's operand order; integer powers by
+// squaring as the oracle writes them.
+#include "common.h"
+
+namespace ctsm {
+namespace sflux {
+
+constexpr double HVAP = 2.501e6;
+
+struct SView {
+    int begc, ldc, begp, ldp, nlevsno, nlevgrnd;
+    __device__ __forceinline__ long c1(int c) const { return (long)(c - begc); }
+    __device__ __forceinline__ long sno(int c, int j) const { return (long)(c - begc) + (long)(j + nlevsno - 1) * ldc; }
+    __device__ __forceinline__ long p1(int p) const { return (long)(p - begp); }
+};
+
+__device__ __forceinline__ double p4(double x)
+{
+    const double x2 = x * x;
+    return x2 * x2;
+}
+__device__ __forceinline__ double p3(double x) { return (x * x) * x; }
+
+__global__ void __launch_bounds__(256) k_save_before(SView V, ctsm_col_state S, ctsm_soilflux_state F, int n,
+                                                     const int32_t *__restrict__ filter)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = filter ? filter[i] : V.begc + i;
+    if (c < V.begc || c >= V.begc + V.ldc) return;
+    for (int j = -V.nlevsno + 1; j <= V.nlevgrnd; ++j) F.t_ssbef[V.sno(c, j)] = S.t_soisno[V.sno(c, j)];
+    F.t_h2osfc_bef[V.c1(c)] = S.t_h2osfc[V.c1(c)];
+}
+
+// SoilFluxesMod.F90:257-570 for one patch.
+__global__ void __launch_bounds__(256) k_soilfluxes(SView V, ctsm_col_state S, ctsm_patch_state P,
+                                                    ctsm_soilflux_state F, int n, const int32_t *__restrict__ filter,
+                                                    double dtime, ErrWord *err)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int p = filter ? filter[i] : V.begp + i;
+    if (p < V.begp || p >= V.begp + V.ldp) { report_error(err, p, CTSM_E_ARG); return; }
+    const long pp = V.p1(p);
+    const int c = P.column[pp];
+    if (c < V.begc || c >= V.begc + V.ldc) { report_error(err, p, CTSM_E_ARG); return; }
+    const long cc = V.c1(c);
+    const int snl = S.snl[cc];
+    if (snl < -V.nlevsno || snl > 0) { report_error(err, p, CTSM_E_ARG); return; }
+    const int jt = snl + 1;
+    const double fs = S.frac_sno_eff[cc], fh = S.frac_h2osfc[cc];
+    const double tb_top = F.t_ssbef[V.sno(c, jt)], tb1 = F.t_ssbef[V.sno(c, 1)], thb = F.t_h2osfc_bef[cc];
+
+    // :257-273 previous ground temperature and its increment
+    double t_grnd0;
+    if (snl < 0) t_grnd0 = fs * tb_top + (1 - fs - fh) * tb1 + fh * thb;
+    else t_grnd0 = (1 - fh) * tb1 + fh * thb;
+    const double tinc = S.t_grnd[cc] - t_grnd0;
+
+    // :277-294 correction to the present ground temperature
+    const double cgs = F.cgrnds[pp], cgl = F.cgrndl[pp];
+    double sh_grnd = F.eflx_sh_grnd[pp] + tinc * cgs;
+    double evap_soi = F.qflx_evap_soi[pp] + tinc * cgl;
+    double ev_snow = F.qflx_ev_snow[pp] + tinc * cgl;
+    F.qflx_ev_soil[pp] = F.qflx_ev_soil[pp] + tinc * cgl;
+    F.qflx_ev_h2osfc[pp] = F.qflx_ev_h2osfc[pp] + tinc * cgl;
+
+    // :297-358 partition into liquid / solid evaporation and dew
+    const double liq = S.h2osoi_liq[V.sno(c, jt)], ice = S.h2osoi_ice[V.sno(c, jt)];
+    double liqevap = 0.0, solidevap = 0.0, soliddew = 0.0, liqdew = 0.0;
+    if (ev_snow >= 0.0) {
+        if ((liq + ice) > 0.0) liqevap = fmax90(ev_snow * (liq / (liq + ice)), 0.0);
+        else liqevap = 0.0;
+        solidevap = ev_snow - liqevap;
+    } else {
+        if (S.t_grnd[cc] < TFRZ) soliddew = fabs(ev_snow);
+        else liqdew = fabs(ev_snow);
+    }
+
+    // :360-410 limits from the water of the top layer
+    const double htvp = S.htvp[cc];
+    if (jt < 1) {
+        const double lim = (ice + liq) / (fs * dtime);
+        if (ev_snow > lim) {
+            const double dem = ev_snow;
+            ev_snow = lim;
+            evap_soi = evap_soi - fs * (dem - lim);
+            liqevap = fmax90(liq / (fs * dtime), 0.0);
+            solidevap = fmax90(ice / (fs * dtime), 0.0);
+            sh_grnd = sh_grnd + fs * (dem - lim) * htvp;
+        }
+    }
+    if (jt == 1 && fh < 1.0) {
+        const double lim = ice / (dtime * (1.0 - fh));
+        if (solidevap >= lim) {
+            const double dem = solidevap;
+            solidevap = lim;
+            liqevap = liqevap + (dem - lim);
+        }
+    }
+    F.eflx_sh_grnd[pp] = sh_grnd;
+    F.qflx_evap_soi[pp] = evap_soi;
+    F.qflx_ev_snow[pp] = ev_snow;
+    F.qflx_liqevap_from_top_layer[pp] = liqevap;
+    F.qflx_solidevap_from_top_layer[pp] = solidevap;
+    F.qflx_soliddew_to_top_layer[pp] = soliddew;
+    F.qflx_liqdew_to_top_layer[pp] = liqdew;
+
+    // :412-471 ground heat flux, totals, history terms
+    const double emg = S.emg[cc], forc_lwrad = S.forc_lwrad[cc];
+    const double lw_grnd = (fs * p4(tb_top) + (1.0 - fs - fh) * p4(tb1) + fh * p4(thb));
+    const double fveg = (double)(1 - P.frac_veg_nosno[pp]);
+    const double soil_grnd = ((1.0 - fs) * P.sabg_soil[pp] + fs * P.sabg_snow[pp]) + P.dlrad[pp]
+                             + fveg * emg * forc_lwrad - emg * SB * lw_grnd - emg * SB * p3(t_grnd0) * (4.0 * tinc)
+                             - (sh_grnd + evap_soi * htvp);
+    F.eflx_soil_grnd[pp] = soil_grnd;
+    F.eflx_soil_grnd_r[pp] = soil_grnd;
+    double sh_tot = F.eflx_sh_veg[pp] + sh_grnd;
+    sh_tot = sh_tot + F.eflx_sh_stem[pp];
+    F.eflx_sh_tot[pp] = sh_tot;
+    F.eflx_sh_tot_r[pp] = sh_tot;
+    const double evap_veg = F.qflx_evap_veg[pp], tran_veg = F.qflx_tran_veg[pp];
+    F.qflx_evap_tot[pp] = evap_veg + evap_soi;
+    const double lh_tot = HVAP * evap_veg + htvp * evap_soi;
+    F.eflx_lh_tot[pp] = lh_tot;
+    F.eflx_lh_tot_r[pp] = lh_tot;
+    F.qflx_evap_can[pp] = evap_veg - tran_veg;
+    F.eflx_lh_vege[pp] = (evap_veg - tran_veg) * HVAP;
+    F.eflx_lh_vegt[pp] = tran_veg * HVAP;
+    F.eflx_lh_grnd[pp] = evap_soi * htvp;
+
+    // :476-499 soil energy conservation error
+    double e = soil_grnd - S.xmf[cc] - S.xmf_h2osfc[cc] - fh * (S.t_h2osfc[cc] - thb) * (S.c_h2osfc[cc] / dtime);
+    e = e + S.eflx_h2osfc_to_snow[cc];
+    for (int j = jt; j <= 0; ++j)
+        e = e - fs * (S.t_soisno[V.sno(c, j)] - F.t_ssbef[V.sno(c, j)]) / S.fact[V.sno(c, j)];
+    for (int j = 1; j <= V.nlevgrnd; ++j)
+        e = e - (S.t_soisno[V.sno(c, j)] - F.t_ssbef[V.sno(c, j)]) / S.fact[V.sno(c, j)];
+    F.errsoi_patch[pp] = e;
+
+    // :526-570 outgoing longwave, skin temperature of bare ground
+    const double lw_out = F.ulrad[pp] + fveg * (1.0 - emg) * forc_lwrad + fveg * emg * SB * lw_grnd
+                          + 4.0 * emg * SB * p3(t_grnd0) * tinc;
+    F.eflx_lwrad_out[pp] = lw_out;
+    if (P.frac_veg_nosno[pp] == 0) F.t_skin[pp] = sqrt(sqrt(lw_grnd));
+    F.eflx_lwrad_net[pp] = lw_out - forc_lwrad;
+    F.eflx_lwrad_net_r[pp] = lw_out - forc_lwrad;
+    F.eflx_lwrad_out_r[pp] = lw_out;
+}
+
+// :574-576, p2c_1d_filter (subgridAveMod.F90:312-318)
+__global__ void __launch_bounds__(256) k_errsoi_p2c(SView V, ctsm_col_state S, ctsm_patch_state P,
+                                                    ctsm_soilflux_state F, int n, const int32_t *__restrict__ filter,
+                                                    ErrWord *err)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = filter ? filter[i] : V.begc + i;
+    if (c < V.begc || c >= V.begc + V.ldc) { report_error(err, c, CTSM_E_ARG); return; }
+    const long cc = V.c1(c);
+    const int pi = S.patchi[cc], pf = S.patchf[cc];
+    if (pf >= pi && (pi < V.begp || pf >= V.begp + V.ldp)) { report_error(err, c, CTSM_E_ARG); return; }
+    double s = 0.0;
+    for (int p = pi; p <= pf; ++p) {
+        const long pp = V.p1(p);
+        if (F.active[pp]) s = s + F.errsoi_patch[pp] * P.wtcol[pp];
+    }
+    F.errsoi_col[cc] = s;
+}
+
+// Balance reduction word: [0] bits of the max |errsoi_col|, [1] its column.
+__global__ void k_balance_init(unsigned long long *w)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        w[0] = 0ull;
+        w[1] = 0x7fffffffull;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_balance_max(SView V, const double *__restrict__ errsoi_col, int n,
+                                                     const int32_t *__restrict__ filter, unsigned long long *w)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = filter ? filter[i] : V.begc + i;
+    if (c < V.begc || c >= V.begc + V.ldc) return;
+    const double a = fabs(errsoi_col[V.c1(c)]);
+    atomicMax(&w[0], (unsigned long long)__double_as_longlong(a));
+}
+
+__global__ void __launch_bounds__(256) k_balance_loc(SView V, const double *__restrict__ errsoi_col, int n,
+                                                     const int32_t *__restrict__ filter, unsigned long long *w)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = filter ? filter[i] : V.begc + i;
+    if (c < V.begc || c >= V.begc + V.ldc) return;
+    const double a = fabs(errsoi_col[V.c1(c)]);
+    if ((unsigned long long)__double_as_longlong(a) == w[0]) atomicMin(&w[1], (unsigned long long)c);
+}
+
+}  // namespace sflux
+
+static sflux::SView sview(const ctsm_layout &L, const ctsm_config &cfg)
+{
+    return sflux::SView{L.begc_alloc, L.ldc, L.begp_alloc, L.ldp, cfg.nlevsno, cfg.nlevgrnd};
+}
+
+void launch_save_before(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg, const ctsm_col_state &S,
+                        const ctsm_soilflux_state &F, int n, const int32_t *filter)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(sflux::k_save_before, dim3((n + 255) / 256), dim3(256), 0, st, sview(L, cfg), S, F, n, filter);
+}
+
+void launch_soilfluxes(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg, const ctsm_col_state &S,
+                       const ctsm_patch_state &P, const ctsm_soilflux_state &F, int nc, const int32_t *filterc,
+                       int np, const int32_t *filterp, double dtime, ErrWord *err)
+{
+    const sflux::SView V = sview(L, cfg);
+    if (np > 0)
+        hipLaunchKernelGGL(sflux::k_soilfluxes, dim3((np + 255) / 256), dim3(256), 0, st, V, S, P, F, np, filterp,
+                           dtime, err);
+    if (nc > 0)
+        hipLaunchKernelGGL(sflux::k_errsoi_p2c, dim3((nc + 255) / 256), dim3(256), 0, st, V, S, P, F, nc, filterc,
+                           err);
+}
+
+void launch_balance(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg, const double *errsoi_col, int n,
+                    const int32_t *filter, unsigned long long *w)
+{
+    const sflux::SView V = sview(L, cfg);
+    hipLaunchKernelGGL(sflux::k_balance_init, dim3(1), dim3(64), 0, st, w);
+    if (n <= 0) return;
+    hipLaunchKernelGGL(sflux::k_balance_max, dim3((n + 255) / 256), dim3(256), 0, st, V, errsoi_col, n, filter, w);
+    hipLaunchKernelGGL(sflux::k_balance_loc, dim3((n + 255) / 256), dim3(256), 0, st, V, errsoi_col, n, filter, w);
+}
+
+}  // namespace ctsm

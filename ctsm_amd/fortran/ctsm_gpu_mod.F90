@@ -141,9 +141,27 @@ module ctsm_gpu_mod
      type(c_ptr) :: lnca = c_null_ptr, c3flag = c_null_ptr, canopy_err = c_null_ptr
   end type ctsm_canopy_state_t
 
+  ! SoilFluxes state (member order and meaning as ctsm_soilflux_state)
+  type, bind(C), public :: ctsm_soilflux_state_t
+     type(c_ptr) :: active = c_null_ptr, t_ssbef = c_null_ptr, t_h2osfc_bef = c_null_ptr
+     type(c_ptr) :: eflx_sh_grnd = c_null_ptr, qflx_evap_soi = c_null_ptr, qflx_ev_snow = c_null_ptr
+     type(c_ptr) :: qflx_ev_soil = c_null_ptr, qflx_ev_h2osfc = c_null_ptr, cgrnds = c_null_ptr
+     type(c_ptr) :: cgrndl = c_null_ptr, ulrad = c_null_ptr, eflx_sh_veg = c_null_ptr
+     type(c_ptr) :: eflx_sh_stem = c_null_ptr, qflx_evap_veg = c_null_ptr, qflx_tran_veg = c_null_ptr
+     type(c_ptr) :: eflx_soil_grnd = c_null_ptr, eflx_soil_grnd_r = c_null_ptr, eflx_sh_tot = c_null_ptr
+     type(c_ptr) :: eflx_sh_tot_r = c_null_ptr, qflx_evap_tot = c_null_ptr, eflx_lh_tot = c_null_ptr
+     type(c_ptr) :: eflx_lh_tot_r = c_null_ptr, qflx_evap_can = c_null_ptr, eflx_lh_vege = c_null_ptr
+     type(c_ptr) :: eflx_lh_vegt = c_null_ptr, eflx_lh_grnd = c_null_ptr, eflx_lwrad_out = c_null_ptr
+     type(c_ptr) :: eflx_lwrad_out_r = c_null_ptr, eflx_lwrad_net = c_null_ptr, eflx_lwrad_net_r = c_null_ptr
+     type(c_ptr) :: t_skin = c_null_ptr, qflx_liqevap_from_top_layer = c_null_ptr, qflx_solidevap_from_top_layer = c_null_ptr
+     type(c_ptr) :: qflx_liqdew_to_top_layer = c_null_ptr, qflx_soliddew_to_top_layer = c_null_ptr, errsoi_patch = c_null_ptr
+     type(c_ptr) :: errsoi_col = c_null_ptr
+  end type ctsm_soilflux_state_t
+
   public :: ctsm_config_default, ctsm_create, ctsm_destroy, ctsm_last_error
   public :: ctsm_soilwater, ctsm_soiltemperature, ctsm_soilwater_fractions
   public :: ctsm_set_canopy_params, ctsm_canopyfluxes, ctsm_root_sink
+  public :: ctsm_soilfluxes, ctsm_soil_balance_check
 
   interface
      subroutine ctsm_config_default(cfg) bind(C, name='ctsm_config_default')
@@ -230,6 +248,34 @@ module ctsm_gpu_mod
        integer(c_int32_t), value :: num_filterc
        integer(c_int32_t), intent(in) :: filterc(*)
      end function ctsm_root_sink
+
+     integer(c_int) function ctsm_soilfluxes(ctx, lay, col, pch, sf, num_nolakec, filter_nolakec, &
+          num_nolakep, filter_nolakep, dtime) bind(C, name='ctsm_soilfluxes')
+       import :: c_int, c_int32_t, c_double, c_ptr, ctsm_layout_t, ctsm_col_state_t, ctsm_patch_state_t, &
+            ctsm_soilflux_state_t
+       type(c_ptr), value :: ctx
+       type(ctsm_layout_t), intent(in) :: lay
+       type(ctsm_col_state_t), intent(in) :: col
+       type(ctsm_patch_state_t), intent(in) :: pch
+       type(ctsm_soilflux_state_t), intent(in) :: sf
+       integer(c_int32_t), value :: num_nolakec
+       integer(c_int32_t), intent(in) :: filter_nolakec(*)
+       integer(c_int32_t), value :: num_nolakep
+       integer(c_int32_t), intent(in) :: filter_nolakep(*)
+       real(c_double), value :: dtime
+     end function ctsm_soilfluxes
+
+     integer(c_int) function ctsm_soil_balance_check(ctx, lay, sf, num_filterc, filterc, errsoi_max, idx) &
+          bind(C, name='ctsm_soil_balance_check')
+       import :: c_int, c_int32_t, c_double, c_ptr, ctsm_layout_t, ctsm_soilflux_state_t
+       type(c_ptr), value :: ctx
+       type(ctsm_layout_t), intent(in) :: lay
+       type(ctsm_soilflux_state_t), intent(in) :: sf
+       integer(c_int32_t), value :: num_filterc
+       integer(c_int32_t), intent(in) :: filterc(*)
+       real(c_double), intent(out) :: errsoi_max
+       integer(c_int32_t), intent(out) :: idx
+     end function ctsm_soil_balance_check
   end interface
 
 end module ctsm_gpu_mod

@@ -32,6 +32,8 @@ EXPORTS = [
     "ctsm_set_canopy_params", "ctsm_canopyfluxes", "ctsm_root_sink", "ctsm_canopyfluxes_dev",
     "ctsm_root_sink_dev", "ctsm_land_step_dev", "ctsm_memcpy2d",
     "ctsm_graph_capture", "ctsm_graph_launch", "ctsm_graph_destroy",
+    "ctsm_soilfluxes", "ctsm_soil_balance_check", "ctsm_soilfluxes_dev", "ctsm_soil_balance_check_dev",
+    "ctsm_land_step_full_dev",
 ]
 
 
@@ -68,6 +70,14 @@ def load():
     L.ctsm_soil_step_fused_dev.argtypes = col + [P(abi.PatchState), C.c_int32, C.c_void_p, C.c_double]
     L.ctsm_graph_capture.argtypes = col + [P(abi.PatchState), P(abi.CanopyState), C.c_int32, C.c_void_p, C.c_int32,
                                            C.c_void_p, C.c_double, C.c_int32, P(C.c_void_p)]
+    sfa = col + [P(abi.PatchState), P(abi.SoilFluxState), C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_double]
+    L.ctsm_soilfluxes.argtypes = sfa
+    L.ctsm_soilfluxes_dev.argtypes = sfa
+    bal = [C.c_void_p, P(abi.Layout), P(abi.SoilFluxState), C.c_int32, C.c_void_p, P(C.c_double), P(C.c_int32)]
+    L.ctsm_soil_balance_check.argtypes = bal
+    L.ctsm_soil_balance_check_dev.argtypes = bal
+    L.ctsm_land_step_full_dev.argtypes = col + [P(abi.PatchState), P(abi.CanopyState), P(abi.SoilFluxState),
+                                                C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_double]
     L.ctsm_graph_launch.argtypes = [C.c_void_p, C.c_void_p]
     L.ctsm_graph_destroy.argtypes = [C.c_void_p]
     L.ctsm_sync.argtypes = [C.c_void_p]
@@ -218,6 +228,60 @@ class Context:
         fp = C.c_void_p(filterp_dev) if filterp_dev is not None else None
         self._check(self.L.ctsm_land_step_dev(self.h, C.byref(ds.layout()), C.byref(cs), C.byref(ps), C.byref(can),
                                               nc, fc, np_, fp, dtime), "land step")
+
+    # ---- SoilFluxes and the soil energy balance check ---------------------
+    def soilfluxes(self, st, filter_c=None, filter_p=None, dtime=1800.0):
+        """Drop-in SoilFluxes on a host state (after SoilTemperature)."""
+        cs, ps = st.structs()
+        sf = st.soilflux_struct()
+        fc, fcp, nc = _filter(filter_c, st.dims.ncol)
+        fp, fpp, np_ = _filter(filter_p, st.dims.npatch)
+        self._check(self.L.ctsm_soilfluxes(self.h, C.byref(st.layout()), C.byref(cs), C.byref(ps), C.byref(sf),
+                                           nc, fcp, np_, fpp, dtime), "SoilFluxes")
+
+    def soil_balance_check(self, st, filter_c=None):
+        """(max |errsoi_col|, its column) over host arrays."""
+        sf = st.soilflux_struct()
+        fc, fcp, nc = _filter(filter_c, st.dims.ncol)
+        m, idx = C.c_double(), C.c_int32()
+        self._check(self.L.ctsm_soil_balance_check(self.h, C.byref(st.layout()), C.byref(sf), nc, fcp, C.byref(m),
+                                                   C.byref(idx)), "soil balance check")
+        return m.value, idx.value
+
+    def soilfluxes_dev(self, ds, filterc_dev=None, nc=None, filterp_dev=None, np_=None, dtime=1800.0):
+        cs, ps = ds.structs()
+        sf = ds.soilflux_struct()
+        nc = ds.dims.ncol if nc is None else nc
+        np_ = ds.dims.npatch if np_ is None else np_
+        fc = C.c_void_p(filterc_dev) if filterc_dev is not None else None
+        fp = C.c_void_p(filterp_dev) if filterp_dev is not None else None
+        self._check(self.L.ctsm_soilfluxes_dev(self.h, C.byref(ds.layout()), C.byref(cs), C.byref(ps), C.byref(sf),
+                                               nc, fc, np_, fp, dtime), "SoilFluxes")
+
+    def soil_balance_check_dev(self, ds, filterc_dev=None, nc=None):
+        sf = ds.soilflux_struct()
+        nc = ds.dims.ncol if nc is None else nc
+        fc = C.c_void_p(filterc_dev) if filterc_dev is not None else None
+        m, idx = C.c_double(), C.c_int32()
+        self._check(self.L.ctsm_soil_balance_check_dev(self.h, C.byref(ds.layout()), C.byref(sf), nc, fc,
+                                                       C.byref(m), C.byref(idx)), "soil balance check")
+        return m.value, idx.value
+
+    def land_step_full_dev(self, ds, filterc_dev=None, nc=None, filterp_dev=None, np_=None, dtime=1800.0,
+                           structs=None):
+        """Land step with the before-step save and SoilFluxes (enqueues only)."""
+        if structs is None:
+            cs, ps = ds.structs()
+            can, sf = ds.canopy_struct(), ds.soilflux_struct()
+        else:
+            cs, ps, can, sf = structs
+        nc = ds.dims.ncol if nc is None else nc
+        np_ = ds.dims.npatch if np_ is None else np_
+        fc = C.c_void_p(filterc_dev) if filterc_dev is not None else None
+        fp = C.c_void_p(filterp_dev) if filterp_dev is not None else None
+        self._check(self.L.ctsm_land_step_full_dev(self.h, C.byref(ds.layout()), C.byref(cs), C.byref(ps),
+                                                   C.byref(can), C.byref(sf), nc, fc, np_, fp, dtime),
+                    "land step (full)")
 
     def graph_capture(self, ds, nsteps=1, canopy=True, filterc_dev=None, nc=None, filterp_dev=None, np_=None,
                       dtime=1800.0, structs=None):

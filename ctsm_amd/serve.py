@@ -25,6 +25,7 @@ from . import abi, synthetic
 from .state import Dims, DeviceState, HostState
 
 _CAN_KIND = {n: k for n, _, k in abi.CANOPY_FIELDS}
+_SF_KIND = {n: k for n, _, k in abi.SOILFLUX_FIELDS}
 
 # Parameters scaled per member by U(0.5, 1.5), with the bounds that keep them
 # physical: (group, field, lower, upper) -- group "c" column, "can" canopy.
@@ -141,10 +142,11 @@ class EnsembleServer:
     def _copy(self, req, to_pool):
         st, d = req.state, self.dims
         for group, host, dev in (("c", st.col, self.pool.col), ("p", st.pch, self.pool.pch),
-                                 ("g", st.can, self.pool.can)):
+                                 ("g", st.can, self.pool.can), ("s", st.sf, self.pool.sf)):
             for k, a in host.items():
                 shape, dt = self.pool._shapes[(group, k)]
-                isp = group == "p" or (group == "g" and _CAN_KIND[k] in ("p", "soip"))
+                isp = (group == "p" or (group == "g" and _CAN_KIND[k] in ("p", "soip"))
+                       or (group == "s" and _SF_KIND[k] == "p"))
                 off = req.pat0 if isp else req.col0
                 cap = d.npatch if isp else d.ncol
                 esz = np.dtype(dt).itemsize

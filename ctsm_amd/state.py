@@ -26,6 +26,18 @@ CANOPY_ALIAS = {
 }
 
 
+# SoilFluxes fields that are the same arrays as a column, patch or canopy field
+# (the rest -- the saved temperatures, errsoi and the flux outputs -- are its own).
+SOILFLUX_ALIAS = {
+    "active": ("g", "active"), "eflx_sh_grnd": ("p", "eflx_sh_grnd"), "qflx_evap_soi": ("p", "qflx_evap_soi"),
+    "qflx_ev_snow": ("p", "qflx_ev_snow"), "qflx_ev_soil": ("p", "qflx_ev_soil"),
+    "qflx_ev_h2osfc": ("p", "qflx_ev_h2osfc"), "cgrnds": ("g", "cgrnds"), "cgrndl": ("g", "cgrndl"),
+    "ulrad": ("g", "ulrad"), "eflx_sh_veg": ("g", "eflx_sh_veg"), "eflx_sh_stem": ("g", "eflx_sh_stem"),
+    "qflx_evap_veg": ("g", "qflx_evap_veg"), "qflx_tran_veg": ("g", "qflx_tran_veg"), "t_skin": ("g", "t_skin"),
+}
+SOILFLUX_OWN = [(n, t, k) for n, t, k in abi.SOILFLUX_FIELDS if n not in SOILFLUX_ALIAS]
+
+
 class Dims:
     def __init__(self, ncol, npatch, nlevsno=12, nlevsoi=20, nlevgrnd=25, ngrid=0):
         self.ncol, self.npatch, self.ngrid = int(ncol), int(npatch), int(ngrid)
@@ -55,6 +67,7 @@ class HostState:
                 a[...] = np.nan
             self.pch[name] = a
         self.can = {}
+        self.sf = {}
         if dims.ngrid > 0:
             for name, t, kind in abi.CANOPY_FIELDS:
                 if name in CANOPY_ALIAS:
@@ -63,6 +76,11 @@ class HostState:
                 if fill_nan and t == "f64":
                     a[...] = np.nan
                 self.can[name] = a
+            for name, t, kind in SOILFLUX_OWN:
+                a = np.zeros(dims.shape(kind), dtype=_NP[t])
+                if fill_nan and t == "f64":
+                    a[...] = np.nan
+                self.sf[name] = a
 
     def copy(self):
         o = HostState.__new__(HostState)
@@ -70,6 +88,7 @@ class HostState:
         o.col = {k: v.copy() for k, v in self.col.items()}
         o.pch = {k: v.copy() for k, v in self.pch.items()}
         o.can = {k: v.copy() for k, v in self.can.items()}
+        o.sf = {k: v.copy() for k, v in getattr(self, "sf", {}).items()}
         return o
 
     def canopy_array(self, name):
@@ -89,6 +108,25 @@ class HostState:
             assert a.flags.c_contiguous
             setattr(cs, name, a.ctypes.data_as(abi._CT[t]))
         return cs
+
+    def soilflux_array(self, name):
+        """The array behind SoilFluxes field `name` (aliases resolve to col/pch/can)."""
+        if name in SOILFLUX_ALIAS:
+            g, k = SOILFLUX_ALIAS[name]
+            if g == "p":
+                return self.pch[k]
+            if g == "c":
+                return self.col[k]
+            return self.canopy_array(k)
+        return self.sf[name]
+
+    def soilflux_struct(self):
+        ss = abi.SoilFluxState()
+        for name, t, _ in abi.SOILFLUX_FIELDS:
+            a = self.soilflux_array(name)
+            assert a.flags.c_contiguous
+            setattr(ss, name, a.ctypes.data_as(abi._CT[t]))
+        return ss
 
     def structs(self):
         """ctypes ColState / PatchState pointing at these numpy buffers."""
@@ -136,6 +174,10 @@ class HostState:
                 idx = {"p": pidx, "soip": pidx, "g": cols}.get(kind, cols)
                 o.can[name] = np.ascontiguousarray(src[..., idx])
             o.can["gridcell"] = o.pch["column"].copy()
+            o.sf = {}
+            for name, t, kind in SOILFLUX_OWN:
+                idx = pidx if kind == "p" else cols
+                o.sf[name] = np.ascontiguousarray(self.sf[name][..., idx])
         return o
 
 
@@ -147,9 +189,9 @@ class DeviceState:
         self.ctx = ctx
         self.dims = host.dims
         self._shapes = {}
-        self.col, self.pch, self.can = {}, {}, {}
+        self.col, self.pch, self.can, self.sf = {}, {}, {}, {}
         for group, src, dst in (("c", host.col, self.col), ("p", host.pch, self.pch),
-                                ("g", getattr(host, "can", {}), self.can)):
+                                ("g", getattr(host, "can", {}), self.can), ("s", getattr(host, "sf", {}), self.sf)):
             for k, v in src.items():
                 a = np.ascontiguousarray(v)
                 ptr = ctx.alloc(max(a.nbytes, 8))
@@ -171,6 +213,19 @@ class DeviceState:
             setattr(cs, name, C.cast(C.c_void_p(ptr), abi._CT[t]))
         return cs
 
+    def soilflux_struct(self):
+        ss = abi.SoilFluxState()
+        for name, t, _ in abi.SOILFLUX_FIELDS:
+            if name in SOILFLUX_ALIAS:
+                g, k = SOILFLUX_ALIAS[name]
+                if g == "g" and k in CANOPY_ALIAS:
+                    g, k = CANOPY_ALIAS[k]
+                ptr = {"p": self.pch, "c": self.col, "g": self.can}[g][k]
+            else:
+                ptr = self.sf[name]
+            setattr(ss, name, C.cast(C.c_void_p(ptr), abi._CT[t]))
+        return ss
+
     def structs(self):
         cs = abi.ColState()
         for name, t, _ in abi.COL_FIELDS:
@@ -183,8 +238,9 @@ class DeviceState:
     def to_host(self):
         o = HostState.__new__(HostState)
         o.dims = self.dims
-        o.col, o.pch, o.can = {}, {}, {}
-        for group, src, dst in (("c", self.col, o.col), ("p", self.pch, o.pch), ("g", self.can, o.can)):
+        o.col, o.pch, o.can, o.sf = {}, {}, {}, {}
+        for group, src, dst in (("c", self.col, o.col), ("p", self.pch, o.pch), ("g", self.can, o.can),
+                                ("s", self.sf, o.sf)):
             for k, ptr in src.items():
                 shape, dt = self._shapes[(group, k)]
                 a = np.empty(shape, dtype=dt)
@@ -195,7 +251,7 @@ class DeviceState:
     def upload(self, host):
         """Overwrite the device copy with a host state of the same dims."""
         for group, src, dst in (("c", host.col, self.col), ("p", host.pch, self.pch),
-                                ("g", getattr(host, "can", {}), self.can)):
+                                ("g", getattr(host, "can", {}), self.can), ("s", getattr(host, "sf", {}), self.sf)):
             for k, v in src.items():
                 self.ctx.h2d(dst[k], np.ascontiguousarray(v))
 
@@ -203,7 +259,7 @@ class DeviceState:
         return sum(int(np.prod(s)) * np.dtype(d).itemsize for s, d in self._shapes.values())
 
     def free(self):
-        for d in (self.col, self.pch, self.can):
+        for d in (self.col, self.pch, self.can, self.sf):
             for k, ptr in d.items():
                 self.ctx.free(ptr)
             d.clear()

@@ -415,7 +415,9 @@ def with_canopy(soil, seed, pmax=16, pmin=1, col0=0):
     qh2o, _ = _qsat_np(c["t_h2osfc"], pbot)
     beta = 0.2 + 0.8 * uniform(seed, S_BETA, gid)
     g["qg_soil"][:] = qsoil * beta
-    g["qg_snow"][:] = qsnow
+    # snow-free columns take the soil's humidity for the snow surface, so the
+    # heat solve's top-layer flux equals the soil one (SurfaceHumidityMod.F90:198-203)
+    g["qg_snow"][:] = np.where(c["snl"] < 0, qsnow, g["qg_soil"])
     g["qg_h2osfc"][:] = qh2o
     fs, fh = c["frac_sno_eff"], c["frac_h2osfc"]
     g["qg"][:] = fs * qsnow + (1.0 - fs - fh) * g["qg_soil"] + fh * qh2o
@@ -496,6 +498,11 @@ def with_canopy(soil, seed, pmax=16, pmin=1, col0=0):
               "qflx_evap_soi", "qflx_ev_snow", "qflx_ev_soil", "qflx_ev_h2osfc", "eflx_gnet", "dgnetdT",
               "sabg_chk"):
         p[k][:] = 0.0
+    # SoilFluxes: temperatures saved before the step, outputs cleared
+    for k in st.sf:
+        st.sf[k][...] = 0.0
+    st.sf["t_ssbef"][...] = c["t_soisno"]
+    st.sf["t_h2osfc_bef"][:] = c["t_h2osfc"]
     return st
 
 

@@ -365,6 +365,34 @@ typedef struct ctsm_canopy_state {
     double        *canopy_err;
 } ctsm_canopy_state;
 
+/* ---- SoilFluxes state (SoilFluxesMod.F90:38-518) --------------------------
+ * The patch and column arrays SoilFluxes reads and writes beyond
+ * ctsm_col_state (column fields: snl, patchi/patchf, frac_sno_eff,
+ * frac_h2osfc, h2osoi_liq/ice, t_soisno, t_h2osfc, t_grnd, emg, forc_lwrad,
+ * htvp, xmf, xmf_h2osfc, fact, c_h2osfc, eflx_h2osfc_to_snow) and
+ * ctsm_patch_state (column, wtcol, frac_veg_nosno, sabg_soil, sabg_snow,
+ * dlrad).  Patch arrays are (begp:endp), column arrays (begc:endc), t_ssbef
+ * is a [SNO] array like t_soisno.  Non-urban soil / crop landunits. */
+typedef struct ctsm_soilflux_state {
+    const int32_t *active;        /* (p) in  patch%active (the p2c of errsoi)            */
+    /* TemperatureType: saved at the start of the step (BiogeophysPreFluxCalcsMod.F90:302-331) */
+    double        *t_ssbef;       /* [SNO] in (written by the land step's save)          */
+    double        *t_h2osfc_bef;  /* (c)   in (written by the land step's save)          */
+    /* corrected to the new ground temperature (inout) */
+    double        *eflx_sh_grnd, *qflx_evap_soi, *qflx_ev_snow, *qflx_ev_soil, *qflx_ev_h2osfc;
+    /* CanopyFluxes outputs read here */
+    const double  *cgrnds, *cgrndl, *ulrad, *eflx_sh_veg, *eflx_sh_stem, *qflx_evap_veg, *qflx_tran_veg;
+    /* outputs */
+    double        *eflx_soil_grnd, *eflx_soil_grnd_r, *eflx_sh_tot, *eflx_sh_tot_r;
+    double        *qflx_evap_tot, *eflx_lh_tot, *eflx_lh_tot_r;
+    double        *qflx_evap_can, *eflx_lh_vege, *eflx_lh_vegt, *eflx_lh_grnd;
+    double        *eflx_lwrad_out, *eflx_lwrad_out_r, *eflx_lwrad_net, *eflx_lwrad_net_r, *t_skin;
+    double        *qflx_liqevap_from_top_layer, *qflx_solidevap_from_top_layer;
+    double        *qflx_liqdew_to_top_layer, *qflx_soliddew_to_top_layer;
+    double        *errsoi_patch;  /* (p) soil energy conservation error (W/m2)           */
+    double        *errsoi_col;    /* (c) p2c of errsoi_patch (W/m2)                      */
+} ctsm_soilflux_state;
+
 /* ---- launch shapes (no counterpart in the reference: set by the autotuner) --
  * richards_lanes: lanes cooperating on one column in the Richards solve
  *   (1, 2, 4 or 8); the heat solve always runs one column per lane.
@@ -459,6 +487,25 @@ int ctsm_root_sink(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *
                    const ctsm_canopy_state *can,
                    int32_t num_filterc, const int32_t *filterc);
 
+/* Replaces SoilFluxes (SoilFluxesMod.F90:38-518, called at clm_driver.F90:921)
+ * for soil / crop patches: fluxes corrected to the new ground temperature,
+ * evaporation partitioned and limited, ground heat flux, total and history
+ * fluxes, outgoing longwave, and the soil energy error errsoi (patch, then
+ * p2c to column).  sf->t_ssbef and sf->t_h2osfc_bef hold the values saved
+ * before the step. */
+int ctsm_soilfluxes(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                    const ctsm_patch_state *pch, const ctsm_soilflux_state *sf,
+                    int32_t num_nolakec, const int32_t *filter_nolakec,
+                    int32_t num_nolakep, const int32_t *filter_nolakep, double dtime);
+
+/* The soil energy part of EnergyBalanceCheck (BalanceCheckMod.F90:1101-1119):
+ * max |errsoi_col| over the filtered columns and the lowest column index
+ * holding it (maxloc).  The caller applies the reference's thresholds (warn
+ * above 1e-5 W/m2, endrun above 1e-4 W/m2 after the skip steps). */
+int ctsm_soil_balance_check(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_soilflux_state *sf,
+                            int32_t num_filterc, const int32_t *filterc,
+                            double *errsoi_max, int32_t *index);
+
 /* ---- device-resident entry points -----------------------------------------
  * Same arguments, but every pointer (fields and filter) is device memory and
  * the call only enqueues work on the context's stream.  filter == NULL means
@@ -490,6 +537,16 @@ int ctsm_root_sink_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_sta
                        const ctsm_canopy_state *can,
                        int32_t num_filterc, const int32_t *filterc_dev);
 
+int ctsm_soilfluxes_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                        const ctsm_patch_state *pch, const ctsm_soilflux_state *sf,
+                        int32_t num_filterc, const int32_t *filterc_dev,
+                        int32_t num_filterp, const int32_t *filterp_dev, double dtime);
+/* Device form of ctsm_soil_balance_check: waits for the stream and returns the
+ * maximum and its column on the host. */
+int ctsm_soil_balance_check_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_soilflux_state *sf,
+                                int32_t num_filterc, const int32_t *filterc_dev,
+                                double *errsoi_max, int32_t *index);
+
 /* One land step with canopy: CanopyFluxes over the patch filter, then per
  * column SoilTemperature, SetSoilWaterFractions, the transpiration sink and
  * SoilWater (clm_driver.F90:766, 900, 936; HydrologyNoDrainageMod.F90:302-346).
@@ -501,6 +558,19 @@ int ctsm_land_step_dev(ctsm_ctx *ctx, const ctsm_layout *lay,
                        const ctsm_canopy_state *can,
                        int32_t num_filterc, const int32_t *filterc_dev,
                        int32_t num_filterp, const int32_t *filterp_dev, double dtime);
+
+/* The land step with the reference's bookkeeping around it: t_ssbef and
+ * t_h2osfc_bef saved first (BiogeophysPreFluxCalcsMod.F90:302-331), then
+ * CanopyFluxes, SoilTemperature and SetSoilWaterFractions, SoilFluxes
+ * (clm_driver.F90:921), the transpiration sink, SoilWater.  Arrays shared by
+ * two structs must be the same device arrays, as for ctsm_land_step_dev, and
+ * the CanopyFluxes outputs SoilFluxes reads (sf->cgrnds, ...) must be the
+ * canopy state's. */
+int ctsm_land_step_full_dev(ctsm_ctx *ctx, const ctsm_layout *lay,
+                            const ctsm_col_state *col, const ctsm_patch_state *pch,
+                            const ctsm_canopy_state *can, const ctsm_soilflux_state *sf,
+                            int32_t num_filterc, const int32_t *filterc_dev,
+                            int32_t num_filterp, const int32_t *filterp_dev, double dtime);
 
 /* ---- captured step sequences (no counterpart in the reference: a batched
  * run replays the same launches every step, so they are captured once into a

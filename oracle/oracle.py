@@ -227,6 +227,74 @@ def land_step(st, cfg, prm, pft, dtime=1800.0):
     return soilwater(st, cfg, dtime)
 
 
+# ---- SoilFluxes and the soil energy balance check --------------------------------
+
+def _sf_lib():
+    L = lib()
+    if not getattr(L, "_sf_ready", False):
+        P = C.POINTER
+        L.oracle_save_before.argtypes = [P(abi.Config), P(abi.Layout), P(abi.ColState), P(abi.SoilFluxState),
+                                         C.c_int, abi.i32p]
+        L.oracle_soilfluxes.argtypes = [P(abi.Config), P(abi.Layout), P(abi.ColState), P(abi.PatchState),
+                                        P(abi.SoilFluxState), C.c_int, abi.i32p, C.c_int, abi.i32p, C.c_double]
+        L.oracle_soil_balance.argtypes = [P(abi.Layout), P(abi.SoilFluxState), C.c_int, abi.i32p, abi.f64p,
+                                          P(C.c_int)]
+        L._sf_ready = True
+    return L
+
+
+def save_before(st, cfg=None, filter_c=None):
+    """t_ssbef / t_h2osfc_bef <- t_soisno / t_h2osfc (BiogeophysPreFluxCalcsMod.F90:302-331)."""
+    L = _sf_lib()
+    cs, _ = st.structs()
+    sf = st.soilflux_struct()
+    f, fp = _filt(filter_c if filter_c is not None else np.arange(1, st.dims.ncol + 1))
+    L.oracle_save_before(C.byref(cfg or canopy_config()), C.byref(st.layout()), C.byref(cs), C.byref(sf),
+                         len(f), fp)
+
+
+def soilfluxes(st, cfg=None, dtime=1800.0, filter_c=None, filter_p=None):
+    """SoilFluxes (SoilFluxesMod.F90:38-518) on a state after SoilTemperature."""
+    L = _sf_lib()
+    cs, ps = st.structs()
+    sf = st.soilflux_struct()
+    fc, fcp = _filt(filter_c if filter_c is not None else np.arange(1, st.dims.ncol + 1))
+    fpp, fpp_p = _filt(filter_p if filter_p is not None else np.arange(1, st.dims.npatch + 1))
+    return L.oracle_soilfluxes(C.byref(cfg or canopy_config()), C.byref(st.layout()), C.byref(cs), C.byref(ps),
+                               C.byref(sf), len(fc), fcp, len(fpp), fpp_p, C.c_double(dtime))
+
+
+def soil_balance(st, filter_c=None):
+    """(max |errsoi_col|, column index) as BalanceCheckMod.F90:1101-1119 finds them."""
+    L = _sf_lib()
+    sf = st.soilflux_struct()
+    f, fp = _filt(filter_c if filter_c is not None else np.arange(1, st.dims.ncol + 1))
+    m = C.c_double(0.0)
+    idx = C.c_int(0)
+    L.oracle_soil_balance(C.byref(st.layout()), C.byref(sf), len(f), fp, C.byref(m), C.byref(idx))
+    return m.value, idx.value
+
+
+def land_step_full(st, cfg, prm, pft, dtime=1800.0):
+    """land_step with the reference's bookkeeping: t_ssbef / t_h2osfc_bef
+    saved first, SoilFluxes between SoilTemperature and the hydrology
+    (clm_driver.F90:921)."""
+    save_before(st, cfg)
+    st.pch["cgrnd"][:] = 0.0
+    st.can["cgrnds"][:] = 0.0
+    st.can["cgrndl"][:] = 0.0
+    rc, bad = canopyfluxes(st, cfg, prm, pft, dtime=dtime)
+    if rc:
+        return rc, bad
+    rc, bad = soiltemperature(st, cfg, dtime)
+    if rc:
+        return rc, bad
+    soilwater_fractions(st, cfg)
+    soilfluxes(st, cfg, dtime)
+    root_sink(st, cfg)
+    return soilwater(st, cfg, dtime)
+
+
 def qsat(T, p):
     L = _canopy_lib()
     T = np.ascontiguousarray(T, dtype=np.float64)
