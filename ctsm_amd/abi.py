@@ -45,6 +45,7 @@ class Tuning(C.Structure):
         ("fused_single_kernel", C.c_int32),
         ("canopy_first_iterations", C.c_int32),
         ("canopy_round_iterations", C.c_int32),
+        ("step_streams", C.c_int32),
     ]
 
 

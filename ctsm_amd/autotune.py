@@ -9,7 +9,8 @@ shape against the oracle):
     are ordered by (bedrock depth, sub-steps taken in the previous step,
     snow layers) -- the previous step is a good predictor of the next;
   * launch shape: lanes per column in the Richards solve (1, 2, 4, 8), block
-    sizes, and one fused kernel vs. heat-solve kernel + Richards kernel.
+    sizes, one fused kernel vs. heat-solve kernel + Richards kernel, and the
+    number of concurrent streams the step's columns are split across.
 
 autotune() times each candidate shape for a few steps on a snapshot of the
 resident state (restored between candidates with device copies) and keeps
@@ -18,24 +19,34 @@ the fastest on the context.
 import numpy as np
 
 DEFAULT_CANDIDATES = [
-    dict(richards_lanes=8, richards_block=128, soiltemp_block=128, fused_single_kernel=0),
-    dict(richards_lanes=8, richards_block=256, soiltemp_block=64, fused_single_kernel=0),
-    dict(richards_lanes=4, richards_block=128, soiltemp_block=128, fused_single_kernel=0),
-    dict(richards_lanes=2, richards_block=128, soiltemp_block=128, fused_single_kernel=0),
-    dict(richards_lanes=1, richards_block=64, soiltemp_block=128, fused_single_kernel=0),
-    dict(richards_lanes=1, richards_block=64, soiltemp_block=64, fused_single_kernel=1),
+    dict(richards_lanes=8, richards_block=128, soiltemp_block=128, fused_single_kernel=0, step_streams=4),
+    dict(richards_lanes=8, richards_block=128, soiltemp_block=128, fused_single_kernel=0, step_streams=2),
+    dict(richards_lanes=8, richards_block=128, soiltemp_block=128, fused_single_kernel=0, step_streams=1),
+    dict(richards_lanes=8, richards_block=256, soiltemp_block=64, fused_single_kernel=0, step_streams=4),
+    dict(richards_lanes=4, richards_block=128, soiltemp_block=128, fused_single_kernel=0, step_streams=4),
+    dict(richards_lanes=2, richards_block=128, soiltemp_block=128, fused_single_kernel=0, step_streams=4),
+    dict(richards_lanes=1, richards_block=64, soiltemp_block=128, fused_single_kernel=0, step_streams=4),
+    dict(richards_lanes=1, richards_block=64, soiltemp_block=64, fused_single_kernel=1, step_streams=1),
 ]
 
 
-def column_order(snl, nbedrock, prev_substeps=None):
-    """1-based column order that groups similar work into wavefronts."""
+def column_order(snl, nbedrock, prev_substeps=None, parts=1):
+    """1-based column order that groups similar work into wavefronts.
+
+    parts > 1 deals the sorted columns over that many contiguous parts
+    (part k takes every parts-th column of the sorted order), so a step split
+    into contiguous parts across streams gives every part the same work mix
+    while each part stays sorted."""
     snl = np.asarray(snl)
     nb = np.asarray(nbedrock)
     if prev_substeps is None:
         keys = (snl, nb)
     else:
         keys = (snl, np.asarray(prev_substeps), nb)
-    return (np.lexsort(keys) + 1).astype(np.int32)
+    o = np.lexsort(keys)
+    if parts > 1:
+        o = np.concatenate([o[k::parts] for k in range(parts)])
+    return (o + 1).astype(np.int32)
 
 
 def snapshot(ctx, ds):
@@ -64,8 +75,9 @@ def free_snapshot(ctx, snap):
         ctx.free(q)
 
 
-def autotune(ctx, ds, filter_dev=None, n=None, steps=2, candidates=None, dtime=1800.0, verbose=False):
-    """Pick the fastest launch shape for this state.  Leaves `ds` unchanged."""
+def autotune(ctx, ds, filter_dev=None, n=None, steps=4, candidates=None, dtime=1800.0, verbose=False):
+    """Pick the fastest launch shape for this state, timing `steps` steps in
+    one multi-step call (as a batched run issues them).  Leaves `ds` unchanged."""
     candidates = candidates or DEFAULT_CANDIDATES
     snap = snapshot(ctx, ds)
     structs = ds.structs()
@@ -77,8 +89,7 @@ def autotune(ctx, ds, filter_dev=None, n=None, steps=2, candidates=None, dtime=1
         ctx.sync()
         restore(ctx, ds, snap)
         e0 = ctx.record()
-        for _ in range(steps):
-            ctx.step_dev(ds, filter_dev=filter_dev, n=n, dtime=dtime, structs=structs)
+        ctx.steps_dev(ds, steps, filter_dev=filter_dev, n=n, dtime=dtime, structs=structs)
         e1 = ctx.record()
         ctx.sync()
         ms = ctx.elapsed_ms(e0, e1) / steps

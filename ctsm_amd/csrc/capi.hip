@@ -243,6 +243,14 @@ __global__ void k_copy1(char *__restrict__ dst, const char *__restrict__ src, si
         dst[i] = src[i];
 }
 
+// Column indices begc .. begc + n - 1: the filter of "every column", so a
+// step split into contiguous parts can address part k as out + k * len.
+__global__ void __launch_bounds__(256) k_iota(int begc, int n, int32_t *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = begc + i;
+}
+
 __global__ void k_reset_err(ErrWord *e)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -257,7 +265,11 @@ struct ctsm_ctx {
     int device = 0;
     ctsm_config cfg{};
     Params prm{};
-    ctsm_tuning tun{4, 128, 128, 0, 6, 3};
+    ctsm_tuning tun{4, 128, 128, 0, 6, 3, 4};
+    static constexpr int kMaxStreams = 4;
+    hipStream_t side[kMaxStreams - 1] = {nullptr, nullptr, nullptr};   // concurrent parts of a step
+    hipEvent_t ev_fork = nullptr, ev_join[kMaxStreams - 1] = {nullptr, nullptr, nullptr};
+    DevBuf partf;                      // per-part column filters of a split step
     hipStream_t stream = nullptr;
     ErrWord *err = nullptr;            // device
     DevBuf work;                       // heat-solve workspace
@@ -520,8 +532,13 @@ int ctsm_create(const ctsm_config *cfg, int device, ctsm_ctx **out)
     ctx->device = device;
     ctx->cfg = *cfg;
     ctx->prm = make_params(*cfg);
-    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc((void **)&ctx->err, sizeof(ErrWord)) != hipSuccess) {
+    bool ok = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipMalloc((void **)&ctx->err, sizeof(ErrWord)) == hipSuccess &&
+              hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) == hipSuccess;
+    for (int k = 0; ok && k < ctsm_ctx::kMaxStreams - 1; ++k)
+        ok = hipStreamCreateWithFlags(&ctx->side[k], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&ctx->ev_join[k], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
         ctsm_destroy(ctx);
         return CTSM_E_HIP;
     }
@@ -552,6 +569,12 @@ int ctsm_destroy(ctsm_ctx *ctx)
     ctx->bal.release();
     for (auto &b : ctx->sfp) b.release();
     for (auto &b : ctx->sfc) b.release();
+    ctx->partf.release();
+    for (int k = 0; k < ctsm_ctx::kMaxStreams - 1; ++k) {
+        if (ctx->side[k]) { (void)hipStreamSynchronize(ctx->side[k]); (void)hipStreamDestroy(ctx->side[k]); }
+        if (ctx->ev_join[k]) (void)hipEventDestroy(ctx->ev_join[k]);
+    }
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
     if (ctx->err) hipFree(ctx->err);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -570,6 +593,7 @@ int ctsm_set_tuning(ctsm_ctx *ctx, const ctsm_tuning *t)
     if (t->canopy_first_iterations < 1 || t->canopy_first_iterations > 1001 || t->canopy_round_iterations < 1 ||
         t->canopy_round_iterations > 1001)
         return CTSM_E_ARG;
+    if (t->step_streams < 1 || t->step_streams > ctsm_ctx::kMaxStreams) return CTSM_E_ARG;
     ctx->tun = *t;
     return CTSM_OK;
 }
@@ -655,24 +679,98 @@ int ctsm_soilwater_fractions_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ct
     return finish_launch(ctx);
 }
 
+// Streams a fused soil step over n columns is split across (1 = not split).
+static int step_split(const ctsm_ctx *ctx, int n)
+{
+    const int S = ctx->tun.step_streams;
+    return (!ctx->tun.fused_single_kernel && S > 1 && n >= 64 * S) ? S : 1;
+}
+// Workspace doubles and per-part filter bytes of a fused soil step.
+static size_t step_work_doubles(const ctsm_ctx *ctx, int n)
+{
+    const int S = step_split(ctx, n);
+    if (S == 1) return soiltemp_workspace_doubles(n);
+    return (size_t)S * soiltemp_workspace_doubles((n + S - 1) / S);
+}
+static size_t step_partf_bytes(const ctsm_ctx *ctx, int n)
+{
+    return step_split(ctx, n) == 1 ? 0 : (size_t)n * sizeof(int32_t);
+}
+
+// nsteps fused soil steps.  Split over S streams, each part runs all its
+// steps on its own stream between one fork and one join, so the parts drift
+// apart and the heat solve of one part (one wave per SIMD, latency-bound)
+// overlaps the Richards solve of another; results are per column, so
+// identical to the unsplit steps.  Parts are contiguous slices of the filter
+// so a wave's columns stay adjacent in memory; a caller that stores its batch
+// binned by work keeps the parts balanced by dealing the bins over them.
+static int soil_steps(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col, const ctsm_patch_state *pch,
+                      int32_t n, const int32_t *filter_dev, double dtime, int nsteps)
+{
+    if (!ctx || check_layout(lay) || !col || !pch || n < 0 || nsteps < 1 || (!filter_dev && n > lay->ldc))
+        return CTSM_E_ARG;
+    if (n == 0) return CTSM_OK;
+    int rc;
+    if ((rc = ensure_work(ctx, step_work_doubles(ctx, n)))) return rc;
+    const View V = make_view(ctx, *lay);
+    const int S = step_split(ctx, n);
+    if (ctx->tun.fused_single_kernel) {
+        for (int s = 0; s < nsteps; ++s)
+            launch_soil_step(ctx->stream, V, ctx->prm, *col, *pch, n, filter_dev, dtime, (double *)ctx->work.p, n,
+                             ctx->err);
+    } else if (S > 1) {
+        const long len = (n + S - 1) / S;
+        hipError_t he = hipSuccess;
+        const int32_t *pf = filter_dev;
+        if (!pf) {
+            he = ctx->partf.ensure(step_partf_bytes(ctx, n));
+            if (he != hipSuccess) return hip_fail(ctx, he);
+            hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, lay->begc_alloc, n,
+                               (int32_t *)ctx->partf.p);
+            pf = (const int32_t *)ctx->partf.p;
+        }
+        he = hipEventRecord(ctx->ev_fork, ctx->stream);
+        for (int k = 1; he == hipSuccess && k < S; ++k) he = hipStreamWaitEvent(ctx->side[k - 1], ctx->ev_fork, 0);
+        if (he != hipSuccess) return hip_fail(ctx, he);
+        for (int k = 0; k < S; ++k) {
+            hipStream_t st = k == 0 ? ctx->stream : ctx->side[k - 1];
+            const int cnt = (int)std::min<long>(len, (long)n - k * len);
+            if (cnt <= 0) continue;
+            double *ws = (double *)ctx->work.p + (size_t)k * soiltemp_workspace_doubles((int)len);
+            for (int s = 0; s < nsteps; ++s) {
+                launch_soiltemp_fractions(st, V, ctx->prm, *col, *pch, cnt, pf + k * len, dtime, ws, len, ctx->err,
+                                          ctx->tun.soiltemp_block);
+                launch_soilwater(st, V, ctx->prm, *col, cnt, pf + k * len, dtime, ctx->err, ctx->tun.richards_lanes,
+                                 ctx->tun.richards_block);
+            }
+        }
+        for (int k = 1; he == hipSuccess && k < S; ++k) {
+            he = hipEventRecord(ctx->ev_join[k - 1], ctx->side[k - 1]);
+            if (he == hipSuccess) he = hipStreamWaitEvent(ctx->stream, ctx->ev_join[k - 1], 0);
+        }
+        if (he != hipSuccess) return hip_fail(ctx, he);
+    } else {
+        for (int s = 0; s < nsteps; ++s) {
+            launch_soiltemp_fractions(ctx->stream, V, ctx->prm, *col, *pch, n, filter_dev, dtime,
+                                      (double *)ctx->work.p, n, ctx->err, ctx->tun.soiltemp_block);
+            launch_soilwater(ctx->stream, V, ctx->prm, *col, n, filter_dev, dtime, ctx->err,
+                             ctx->tun.richards_lanes, ctx->tun.richards_block);
+        }
+    }
+    return finish_launch(ctx);
+}
+
 int ctsm_soil_step_fused_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
                              const ctsm_patch_state *pch, int32_t n, const int32_t *filter_dev, double dtime)
 {
-    if (!ctx || check_layout(lay) || !col || !pch || n < 0 || (!filter_dev && n > lay->ldc)) return CTSM_E_ARG;
-    if (n == 0) return CTSM_OK;
-    int rc;
-    if ((rc = ensure_work(ctx, soiltemp_workspace_doubles(n)))) return rc;
-    const View V = make_view(ctx, *lay);
-    if (ctx->tun.fused_single_kernel) {
-        launch_soil_step(ctx->stream, V, ctx->prm, *col, *pch, n, filter_dev, dtime, (double *)ctx->work.p, n,
-                         ctx->err);
-    } else {
-        launch_soiltemp_fractions(ctx->stream, V, ctx->prm, *col, *pch, n, filter_dev, dtime, (double *)ctx->work.p,
-                                  n, ctx->err, ctx->tun.soiltemp_block);
-        launch_soilwater(ctx->stream, V, ctx->prm, *col, n, filter_dev, dtime, ctx->err, ctx->tun.richards_lanes,
-                         ctx->tun.richards_block);
-    }
-    return finish_launch(ctx);
+    return soil_steps(ctx, lay, col, pch, n, filter_dev, dtime, 1);
+}
+
+int ctsm_soil_steps_fused_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                              const ctsm_patch_state *pch, int32_t n, const int32_t *filter_dev, double dtime,
+                              int32_t nsteps)
+{
+    return soil_steps(ctx, lay, col, pch, n, filter_dev, dtime, nsteps);
 }
 
 // ---- canopy ----------------------------------------------------------------
@@ -780,7 +878,7 @@ struct ctsm_graph {
     ctsm_ctx *ctx = nullptr;
     hipGraph_t g = nullptr;
     hipGraphExec_t ex = nullptr;
-    DevBuf work, cwork;   // workspaces the captured kernels use
+    DevBuf work, cwork, partf;   // workspaces the captured kernels use
 };
 
 int ctsm_graph_destroy(ctsm_graph *gr)
@@ -794,6 +892,7 @@ int ctsm_graph_destroy(ctsm_graph *gr)
     if (gr->g) (void)hipGraphDestroy(gr->g);
     gr->work.release();
     gr->cwork.release();
+    gr->partf.release();
     delete gr;
     return CTSM_OK;
 }
@@ -813,7 +912,8 @@ int ctsm_graph_capture(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_sta
     ctsm_graph *gr = new ctsm_graph;
     gr->ctx = ctx;
     // size the graph's own workspaces first: nothing may allocate under capture
-    he = gr->work.ensure(std::max<size_t>(soiltemp_workspace_doubles(nc), 1) * sizeof(double));
+    he = gr->work.ensure(std::max<size_t>(step_work_doubles(ctx, nc), 1) * sizeof(double));
+    if (he == hipSuccess && step_partf_bytes(ctx, nc)) he = gr->partf.ensure(step_partf_bytes(ctx, nc));
     const size_t cwb = can ? canopy_work_bytes(np, ctx->cfg.itmax_canopy_fluxes, ctx->tun.canopy_first_iterations,
                                                ctx->tun.canopy_round_iterations)
                            : 0;
@@ -824,12 +924,16 @@ int ctsm_graph_capture(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_sta
     }
     std::swap(ctx->work, gr->work);
     std::swap(ctx->cwork, gr->cwork);
+    std::swap(ctx->partf, gr->partf);
     int rc = CTSM_OK;
     he = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
     if (he == hipSuccess) {
-        for (int s = 0; s < nsteps && rc == CTSM_OK; ++s)
-            rc = can ? ctsm_land_step_dev(ctx, lay, col, pch, can, nc, filterc_dev, np, filterp_dev, dtime)
-                     : ctsm_soil_step_fused_dev(ctx, lay, col, pch, nc, filterc_dev, dtime);
+        if (can) {
+            for (int s = 0; s < nsteps && rc == CTSM_OK; ++s)
+                rc = ctsm_land_step_dev(ctx, lay, col, pch, can, nc, filterc_dev, np, filterp_dev, dtime);
+        } else {
+            rc = soil_steps(ctx, lay, col, pch, nc, filterc_dev, dtime, nsteps);
+        }
         hipError_t he2 = hipStreamEndCapture(ctx->stream, &gr->g);
         if (he2 != hipSuccess && rc == CTSM_OK) rc = hip_fail(ctx, he2);
     } else {
@@ -837,6 +941,7 @@ int ctsm_graph_capture(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_sta
     }
     std::swap(ctx->work, gr->work);
     std::swap(ctx->cwork, gr->cwork);
+    std::swap(ctx->partf, gr->partf);
     if (rc == CTSM_OK) {
         he = hipGraphInstantiate(&gr->ex, gr->g, nullptr, nullptr, 0);
         if (he != hipSuccess) rc = hip_fail(ctx, he);

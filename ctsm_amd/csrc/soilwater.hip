@@ -54,9 +54,16 @@ k_soilwater_fractions(View V, ctsm_col_state S, int n, const int32_t *__restrict
 }
 
 // G lanes per column (richards_group.h).  Groups are aligned to G lanes, so a
-// column's lanes retire together.
+// column's lanes retire together.  The solve is latency-bound (a trial is a
+// serial elimination chain), so for G = 8 the register budget is capped at 128
+// VGPRs for four waves per SIMD: the few values that then live in scratch cost
+// less than the latency the extra waves hide (measured: 1.89 -> 1.66 ms on
+// 65,536 uneven columns; 3, 5, 6 and 8 waves were slower).
+template <int G> struct RichardsWaves { static constexpr int value = 1; };
+template <> struct RichardsWaves<8> { static constexpr int value = 4; };
+
 template <int G>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RichardsWaves<G>::value)))
 k_soilwater_g(View V, Params P, ctsm_col_state S, int n, const int32_t *__restrict__ filter, double dtime,
               ErrWord *err)
 {

@@ -33,7 +33,7 @@ EXPORTS = [
     "ctsm_root_sink_dev", "ctsm_land_step_dev", "ctsm_memcpy2d",
     "ctsm_graph_capture", "ctsm_graph_launch", "ctsm_graph_destroy",
     "ctsm_soilfluxes", "ctsm_soil_balance_check", "ctsm_soilfluxes_dev", "ctsm_soil_balance_check_dev",
-    "ctsm_land_step_full_dev",
+    "ctsm_land_step_full_dev", "ctsm_soil_steps_fused_dev",
 ]
 
 
@@ -68,6 +68,7 @@ def load():
     L.ctsm_soilwater_fractions_dev.argtypes = col + [C.c_int32, C.c_void_p]
     L.ctsm_soiltemperature_dev.argtypes = col + [P(abi.PatchState), C.c_int32, C.c_void_p, C.c_double, C.c_int32]
     L.ctsm_soil_step_fused_dev.argtypes = col + [P(abi.PatchState), C.c_int32, C.c_void_p, C.c_double]
+    L.ctsm_soil_steps_fused_dev.argtypes = col + [P(abi.PatchState), C.c_int32, C.c_void_p, C.c_double, C.c_int32]
     L.ctsm_graph_capture.argtypes = col + [P(abi.PatchState), P(abi.CanopyState), C.c_int32, C.c_void_p, C.c_int32,
                                            C.c_void_p, C.c_double, C.c_int32, P(C.c_void_p)]
     sfa = col + [P(abi.PatchState), P(abi.SoilFluxState), C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_double]
@@ -387,6 +388,14 @@ class Context:
         else:
             self._check(self.L.ctsm_soil_step_fused_dev(self.h, C.byref(lay), C.byref(cs), C.byref(ps), n, fp,
                                                         dtime), "soil step")
+
+    def steps_dev(self, ds, nsteps, filter_dev=None, n=None, dtime=1800.0, structs=None):
+        """nsteps fused soil steps in one call (enqueues only)."""
+        cs, ps = structs if structs is not None else ds.structs()
+        n = ds.dims.ncol if n is None else n
+        fp = C.c_void_p(filter_dev) if filter_dev is not None else None
+        self._check(self.L.ctsm_soil_steps_fused_dev(self.h, C.byref(ds.layout()), C.byref(cs), C.byref(ps), n, fp,
+                                                     dtime, nsteps), "soil steps")
 
     def soilwater_dev(self, ds, filter_dev=None, n=None, dtime=1800.0, plain=False):
         cs, _ = ds.structs()

@@ -404,7 +404,11 @@ typedef struct ctsm_soilflux_state {
  *   only the patches still iterating (compacted) for at most
  *   canopy_round_iterations more.  canopy_first_iterations > itmax_canopy_fluxes
  *   runs the whole iteration in one pass.  Both >= 1; results do not depend on
- *   them. */
+ *   them.
+ * step_streams: the fused soil step splits its filter into this many
+ *   contiguous parts run on as many streams of the context at once, joined
+ *   before the call's later work (1..4).  A batch stored in work-binned
+ *   order keeps the parts balanced by dealing the bins over the parts. */
 typedef struct ctsm_tuning {
     int32_t richards_lanes;
     int32_t richards_block;
@@ -412,6 +416,7 @@ typedef struct ctsm_tuning {
     int32_t fused_single_kernel;
     int32_t canopy_first_iterations;
     int32_t canopy_round_iterations;
+    int32_t step_streams;
 } ctsm_tuning;
 
 typedef struct ctsm_ctx ctsm_ctx;
@@ -529,6 +534,13 @@ int ctsm_soilwater_fractions_dev(ctsm_ctx *ctx, const ctsm_layout *lay,
 int ctsm_soil_step_fused_dev(ctsm_ctx *ctx, const ctsm_layout *lay,
                              const ctsm_col_state *col, const ctsm_patch_state *pch,
                              int32_t num_filter, const int32_t *filter_dev, double dtime);
+/* nsteps such steps in one call.  With step_streams > 1 each part of the
+ * columns runs all nsteps on its own stream between one fork and one join
+ * (the parts need not wait for each other between steps). */
+int ctsm_soil_steps_fused_dev(ctsm_ctx *ctx, const ctsm_layout *lay,
+                              const ctsm_col_state *col, const ctsm_patch_state *pch,
+                              int32_t num_filter, const int32_t *filter_dev, double dtime,
+                              int32_t nsteps);
 
 int ctsm_canopyfluxes_dev(ctsm_ctx *ctx, const ctsm_layout *lay,
                           const ctsm_col_state *col, const ctsm_canopy_state *can,

@@ -176,6 +176,29 @@ def test_bad_column_reports_index(ctx):
     assert ei.value.index == 42
 
 
+@pytest.mark.parametrize("streams", [1, 2, 4])
+def test_multi_step_call_matches_oracle_bitwise(ctx, streams):
+    """Several steps in one call: with the columns split across streams each
+    part runs all its steps without waiting for the others."""
+    from ctsm_amd.state import DeviceState
+    st = synthetic.workload("uneven", ncol=2000)
+    print(f"case: 4 steps in one call, {streams} streams, 2,000 uneven columns", flush=True)
+    before = ctx.get_tuning()
+    ctx.set_tuning(richards_lanes=8, richards_block=128, fused_single_kernel=0, step_streams=streams)
+    ds = DeviceState(st, ctx)
+    try:
+        ctx.steps_dev(ds, 4)
+        ctx.sync()
+        g = ds.to_host()
+    finally:
+        ds.free()
+        ctx.set_tuning(**before)
+    o = st.copy()
+    for _ in range(4):
+        assert O.soil_step(o)[0] == 0
+    compare_states(g, o, 0.0, ST_FIELDS + ["qcharge", "num_substeps"])
+
+
 def test_tuning_rejects_oversized_one_lane_block(ctx):
     """The one-lane Richards kernel is built for 64-lane blocks."""
     before = ctx.get_tuning()
@@ -191,6 +214,10 @@ def test_tuning_rejects_oversized_one_lane_block(ctx):
     dict(richards_lanes=4, fused_single_kernel=0),
     dict(richards_lanes=8, fused_single_kernel=0),
     dict(richards_lanes=8, richards_block=256, soiltemp_block=64, fused_single_kernel=0),
+    dict(richards_lanes=8, fused_single_kernel=0, step_streams=1),
+    dict(richards_lanes=8, fused_single_kernel=0, step_streams=2),
+    dict(richards_lanes=4, fused_single_kernel=0, step_streams=3),
+    dict(richards_lanes=8, fused_single_kernel=0, step_streams=4),
 ])
 def test_launch_shapes_match_oracle_bitwise(ctx, shape):
     """Every launch shape the autotuner may pick gives the oracle's bits."""
@@ -201,7 +228,8 @@ def test_launch_shapes_match_oracle_bitwise(ctx, shape):
         perm = np.lexsort((st.col["snl"], st.col["nbedrock"])) + 1
         g = _run_device_steps(ctx, st, 3, filter_perm=perm)
     finally:
-        ctx.set_tuning(richards_lanes=8, richards_block=128, soiltemp_block=128, fused_single_kernel=0)
+        ctx.set_tuning(richards_lanes=8, richards_block=128, soiltemp_block=128, fused_single_kernel=0,
+                       step_streams=4)
     o = st.copy()
     for _ in range(3):
         assert O.soil_step(o)[0] == 0
