@@ -3,12 +3,14 @@
 A "step" is one time step (dtime = 1800 s) of the soil physics over every
 column of the batch: SoilTemperature (snow + soil heat solve with phase
 change) with SetSoilWaterFractions, then the moisture-form Richards solve
-(the library's ctsm_soil_step_fused_dev; launch shape chosen by the
-autotuner).  The headline workload is the uneven-column batch: 65,536
-columns, snow layers uniform 0..12, bedrock index uniform 5..20, one patch per
-column, FP64, synthetic seeded inputs resident in HBM.  The same steps are
-also timed in the plain form (one launch per reference routine,
-intermediates through HBM) for the fused/plain ratio.
+(the library's fused soil step; launch shape -- lanes per column, blocks,
+streams the columns are split across -- chosen by the autotuner).  The timed
+steps are captured once as one HIP graph and launched once.  The headline
+workload is the uneven-column batch: 65,536 columns, snow layers uniform
+0..12, bedrock index uniform 5..20, one patch per column, FP64, synthetic
+seeded inputs resident in HBM.  The same steps are also timed in the plain
+form (one launch per reference routine, intermediates through HBM) for the
+fused/plain ratio.
 
 Prints ONE JSON line with the metric, the roofline of the dominant kernel
 (algorithmic bytes per launch / its measured launch time vs 8 TB/s), a CPU
@@ -29,10 +31,16 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per
-# dispatch), each scaled by the same counter's reading on a streaming copy of
-# known size; absent until measured for the kernel.
-TRAFFIC = {}
+# HBM bytes per launch of the full-batch kernel from rocprofv3 --pmc
+# FETCH_SIZE / WRITE_SIZE (KiB per dispatch), each scaled by the same
+# counter's reading on a streaming copy of 1 GiB (FETCH x 2.0, WRITE x 1.0):
+# k_soilwater_g<8> on the 65,536-column batch reads 151,589 KiB x 2 and
+# writes 276,934 KiB (profiles/richards_heat_solve_uneven65536.md).
+TRAFFIC = {"k_soilwater_g<8>": (151589.2 * 2.0 + 276934.4) * 1024.0}
+# FP64 VALU utilisation of the same launch from rocprofv3 --pmc SQ_INSTS_VALU_*_F64
+# (same profile): 64 lanes x (ADD + MUL + 2 FMA + TRANS) instructions per launch
+# over the launch time, against the 78.6 TFLOP/s FP64 vector peak.
+FP64_VALU_FRAC = {"k_soilwater_g<8>": 64 * (62.838e6 + 65.686e6 + 2 * 92.273e6 + 15.013e6) / 1.6863e-3 / 78.6e12}
 
 
 def richards_bytes(st):
@@ -123,7 +131,9 @@ def cpu_baseline(st, seconds=15.0, threads=None):
     threads = threads or min(16, os.cpu_count() or 1)
     per = 1024
     ncol = st.dims.ncol
-    subs = [st.subset((np.arange(per) + i * per) % ncol) for i in range(threads)]
+    # a seeded sample of the whole batch (its storage order is binned by work)
+    pick = np.random.default_rng(12345).permutation(ncol)
+    subs = [st.subset(np.sort(pick[(np.arange(per) + i * per) % ncol])) for i in range(threads)]
 
     def work(sub):
         n = 0
@@ -131,7 +141,7 @@ def cpu_baseline(st, seconds=15.0, threads=None):
         while True:
             O.soil_step(sub)
             n += 1
-            if time.perf_counter() - t >= seconds or n >= 48:
+            if time.perf_counter() - t >= seconds or n >= 48 * 40:
                 return n
 
     t0 = time.perf_counter()
@@ -139,8 +149,8 @@ def cpu_baseline(st, seconds=15.0, threads=None):
         steps = list(ex.map(work, subs))
     el = time.perf_counter() - t0
     total = per * sum(steps)
-    return total / el, threads, (f"{threads} threads x {per} columns x {min(steps)}-{max(steps)} steps "
-                                 f"of the same workload ({el:.1f} s)")
+    return total / el, threads, (f"{threads} threads x {per} columns (a seeded random sample of the batch) x "
+                                 f"{min(steps)}-{max(steps)} steps ({el:.1f} s)")
 
 
 def served_ensemble(ctx, args):
@@ -231,7 +241,49 @@ def full_column(ctx, args):
                                "achieved_GBps": b / (can_ms * 1e-3) / 1e9,
                                "frac": b / (can_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
     ds.free()
+    if args.canopy_parts > 1:
+        out["concurrent_parts"] = _full_column_parts(ctx, args, st)
     return out
+
+
+def _full_column_parts(ctx, args, st):
+    """The same batch as concurrent sub-batches: contiguous parts of the
+    binned order (balanced, as the bins are dealt over the parts), each
+    resident in its own context and stream and stepped without waiting for
+    the others, so one part's long canopy rounds overlap another's work."""
+    from ctsm_amd import synthetic
+    from ctsm_amd.lib import Context
+    from ctsm_amd.state import DeviceState
+    P = args.canopy_parts
+    n = st.dims.ncol
+    bounds = [n * k // P for k in range(P + 1)]
+    tun = dict(ctx.get_tuning(), step_streams=1)
+    ctxs = [ctx] + [Context() for _ in range(P - 1)]
+    before = ctx.get_tuning()
+    for c in ctxs:
+        c.set_canopy_params(synthetic.canopy_params(), synthetic.pft_table())
+        c.set_tuning(**tun)
+    dss = [DeviceState(st.subset(np.arange(bounds[k], bounds[k + 1])), c) for k, c in enumerate(ctxs)]
+    sts = [(*d.structs(), d.canopy_struct()) for d in dss]
+    for c, d, s in zip(ctxs, dss, sts):
+        c.land_step_dev(d, structs=s)
+    for c in ctxs:
+        c.sync()
+    t0 = time.perf_counter()
+    for c, d, s in zip(ctxs, dss, sts):
+        for _ in range(args.canopy_steps):
+            c.land_step_dev(d, structs=s)
+    for c in ctxs:
+        c.sync()
+    wall = time.perf_counter() - t0
+    for d in dss:
+        d.free()
+    for c in ctxs[1:]:
+        c.close()
+    ctx.set_tuning(**before)
+    return {"parts": P, "column_steps_per_s": n * args.canopy_steps / wall,
+            "patch_steps_per_s": st.dims.npatch * args.canopy_steps / wall,
+            "step_ms": wall / args.canopy_steps * 1e3}
 
 
 def main():
@@ -258,6 +310,8 @@ def main():
     ap.add_argument("--canopy-ncol", type=int, default=262144,
                     help="columns of the full-column (CanopyFluxes) measurement; 0 skips it")
     ap.add_argument("--canopy-steps", type=int, default=6)
+    ap.add_argument("--canopy-parts", type=int, default=1,
+                    help="also run the full-column batch as this many concurrent sub-batches (1 = no)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--serve-requests", type=int, default=80,
                     help="served-ensemble measurement: number of requests (0 skips it)")
@@ -422,7 +476,8 @@ def main():
                              "the timed steps captured as one graph, launched once"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get(kname),
-                     "kernel": kname, "launch_ms": kms, "alg_bytes_per_launch": kbytes},
+                     "kernel": kname, "launch_ms": kms, "alg_bytes_per_launch": kbytes,
+                     "fp64_valu_frac": FP64_VALU_FRAC.get(kname)},
         "detail": res,
     }
     if not args.no_cpu:
