@@ -368,7 +368,7 @@ def main():
                        step_streams=args.streams or (1 if args.single_kernel else 4))
     elif not args.no_autotune:
         ds0 = DeviceState(st, ctx)
-        tuning, _, table = AT.autotune(ctx, ds0, filter_dev=filt, steps=4)
+        tuning, _, table = AT.autotune(ctx, ds0, filter_dev=filt, steps=8, reps=2)
         ds0.free()
 
     def run(plain):

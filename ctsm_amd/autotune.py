@@ -75,9 +75,10 @@ def free_snapshot(ctx, snap):
         ctx.free(q)
 
 
-def autotune(ctx, ds, filter_dev=None, n=None, steps=4, candidates=None, dtime=1800.0, verbose=False):
+def autotune(ctx, ds, filter_dev=None, n=None, steps=8, reps=2, candidates=None, dtime=1800.0, verbose=False):
     """Pick the fastest launch shape for this state, timing `steps` steps in
-    one multi-step call (as a batched run issues them).  Leaves `ds` unchanged."""
+    one multi-step call (as a batched run issues them), best of `reps` runs
+    from the same snapshot.  Leaves `ds` unchanged."""
     candidates = candidates or DEFAULT_CANDIDATES
     snap = snapshot(ctx, ds)
     structs = ds.structs()
@@ -87,14 +88,16 @@ def autotune(ctx, ds, filter_dev=None, n=None, steps=4, candidates=None, dtime=1
         restore(ctx, ds, snap)
         ctx.step_dev(ds, filter_dev=filter_dev, n=n, dtime=dtime, structs=structs)   # warm
         ctx.sync()
-        restore(ctx, ds, snap)
-        e0 = ctx.record()
-        ctx.steps_dev(ds, steps, filter_dev=filter_dev, n=n, dtime=dtime, structs=structs)
-        e1 = ctx.record()
-        ctx.sync()
-        ms = ctx.elapsed_ms(e0, e1) / steps
-        ctx.destroy_event(e0)
-        ctx.destroy_event(e1)
+        ms = float("inf")
+        for _ in range(reps):
+            restore(ctx, ds, snap)
+            e0 = ctx.record()
+            ctx.steps_dev(ds, steps, filter_dev=filter_dev, n=n, dtime=dtime, structs=structs)
+            e1 = ctx.record()
+            ctx.sync()
+            ms = min(ms, ctx.elapsed_ms(e0, e1) / steps)
+            ctx.destroy_event(e0)
+            ctx.destroy_event(e1)
         table.append((dict(cand), ms))
         if verbose:
             print(f"autotune {cand} -> {ms:.3f} ms/step", flush=True)
