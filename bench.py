@@ -175,17 +175,24 @@ def served_ensemble(ctx, args):
     return out
 
 
-def time_events(ctx, fns, reps):
-    """Mean ms of each callable launched `reps` times, events on the library stream."""
+def time_events(ctx, fns, reps, prefill=None, inner=1):
+    """Mean ms of each callable launched `reps` x `inner` times, events on the
+    library stream.  For kernels shorter than the host's launch overhead,
+    `prefill` enqueues long work first so that the GPU is still busy while the
+    host enqueues the timed launches back to back: the events then time the
+    kernels, not the host."""
     acc = [0.0] * len(fns)
     for _ in range(reps):
+        if prefill is not None:
+            prefill()
         evs = [ctx.record()]
         for fn in fns:
-            fn()
+            for _ in range(inner):
+                fn()
             evs.append(ctx.record())
         ctx.sync()
         for i in range(len(fns)):
-            acc[i] += ctx.elapsed_ms(evs[i], evs[i + 1])
+            acc[i] += ctx.elapsed_ms(evs[i], evs[i + 1]) / inner
         for e in evs:
             ctx.destroy_event(e)
     return [a / reps for a in acc]
@@ -240,10 +247,57 @@ def full_column(ctx, args):
            "canopy_roofline": {"kernel": "k_canopy_round (all rounds of one call)", "alg_bytes_per_launch": b,
                                "achieved_GBps": b / (can_ms * 1e-3) / 1e9,
                                "frac": b / (can_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+    out["with_soilfluxes_and_surface_water"] = _full_land_step(ctx, args, ds, structs)
     ds.free()
     if args.canopy_parts > 1:
         out["concurrent_parts"] = _full_column_parts(ctx, args, st)
     return out
+
+
+def surface_water_bytes(st):
+    """Algorithmic bytes of one surface water launch: per column 22 FP64 inputs
+    (14 of the chain, frac_sno_eff, frac_h2osfc, icefrac and hksat of layers
+    1-3), snl, and 16 FP64 outputs (h2osfc included)."""
+    return st.dims.ncol * (22 * 8 + 4 + 16 * 8)
+
+
+def _full_land_step(ctx, args, ds, structs):
+    """The land step with the reference's bookkeeping (before-step save,
+    SoilFluxes, the surface water input chain feeding SoilWater's
+    infiltration) on the same resident batch, and the launch times of the
+    SoilFluxes and surface water kernels."""
+    from ctsm_amd import synthetic
+    hydro = synthetic.hydro_params()
+    full = (*structs, ds.soilflux_struct())
+    for _ in range(2):
+        ctx.land_step_full_dev(ds, structs=full, hydro=hydro)
+    ctx.sync()
+    e0 = ctx.record()
+    t0 = time.perf_counter()
+    for _ in range(args.canopy_steps):
+        ctx.land_step_full_dev(ds, structs=full, hydro=hydro)
+    e1 = ctx.record()
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    step_ms = ctx.elapsed_ms(e0, e1) / args.canopy_steps
+    ctx.destroy_event(e0)
+    ctx.destroy_event(e1)
+    worst, _ = ctx.soil_balance_check_dev(ds)
+    err = np.empty(ds.dims.ncol)
+    ctx.d2h(err, ds.sf["errsoi_col"])
+    liq = np.empty((ds.dims.nlevsno + ds.dims.nlevgrnd, ds.dims.ncol))
+    ctx.d2h(liq, ds.col["h2osoi_liq"])
+    sw_ms, sf_ms = time_events(ctx, [lambda: ctx.surface_water_dev(ds, hydro), lambda: ctx.soilfluxes_dev(ds)], 3,
+                               prefill=lambda: ctx.canopyfluxes_dev(ds, structs=structs), inner=10)
+    b = surface_water_bytes(ds)
+    return {"column_steps_per_s": ds.dims.ncol * args.canopy_steps / wall, "step_ms": step_ms,
+            "max_abs_errsoi_Wm2": worst,
+            "columns_errsoi_over_1e-5_Wm2": int(np.sum(~(np.abs(err) <= 1e-5))),
+            "columns_negative_soil_liquid": int(np.sum((liq[ds.dims.nlevsno:] < 0).any(axis=0))),
+            "surface_water_kernel_ms": sw_ms, "soilfluxes_kernels_ms": sf_ms,
+            "surface_water_roofline": {"kernel": "k_surface_water", "alg_bytes_per_launch": b,
+                                       "achieved_GBps": b / (sw_ms * 1e-3) / 1e9,
+                                       "frac": b / (sw_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
 
 
 def _full_column_parts(ctx, args, st):

@@ -342,6 +342,22 @@ SOILFLUX_FIELDS = [
     ("errsoi_patch", "f64", "p"), ("errsoi_col", "f64", "c"),
 ]
 
+# ctsm_surface_water_state, in declaration order (all column arrays)
+SURFWATER_FIELDS = [(n, "f64", "c") for n in (
+    "qflx_rain_plus_snomelt", "qflx_snow_h2osfc", "qflx_floodc",
+    "qflx_liqevap_from_top_layer", "qflx_ev_soil", "qflx_ev_h2osfc",
+    "zwt", "zwt_perched", "frost_table", "wtfact", "topo_slope", "h2osfc_thresh", "frac_h2osfc_nosnow",
+    "h2osfc",
+    "fsat", "fcov", "qflx_sat_excess_surf", "qflx_top_soil", "qflx_in_soil", "qflx_top_soil_to_h2osfc",
+    "qinmax", "qflx_infl_excess", "qflx_in_soil_limited", "qflx_in_h2osfc", "qflx_infl_excess_surf",
+    "qflx_h2osfc_surf", "qflx_h2osfc_drain", "qflx_infl", "qflx_surf")]
+
+
+class HydroParams(C.Structure):
+    _fields_ = [("fff", C.c_double), ("pc", C.c_double), ("mu", C.c_double), ("h2osfcflag", C.c_int32),
+                ("pad", C.c_int32)]
+
+
 CANOPY_PARAM_NAMES = [
     "cv", "a_coef", "a_exp", "csoilc", "z_dl", "lai_dl", "wind_min", "zetamaxstable",
     "kc25_coef", "ko25_coef", "cp25_yr2000", "kcha", "koha", "cpha",
@@ -373,6 +389,10 @@ class PftParams(C.Structure):
 
 class CanopyState(C.Structure):
     _fields_ = [(n, _CT[t]) for n, t, _ in CANOPY_FIELDS]
+
+
+class SurfaceWaterState(C.Structure):
+    _fields_ = [(n, _CT[t]) for n, t, _ in SURFWATER_FIELDS]
 
 
 class SoilFluxState(C.Structure):

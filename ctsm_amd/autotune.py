@@ -52,7 +52,8 @@ def column_order(snl, nbedrock, prev_substeps=None, parts=1):
 def snapshot(ctx, ds):
     """Device copy of every field of a DeviceState (returns dict of pointers)."""
     snap = {}
-    for group, d in (("c", ds.col), ("p", ds.pch), ("g", ds.can), ("s", getattr(ds, "sf", {}))):
+    for group, d in (("c", ds.col), ("p", ds.pch), ("g", ds.can), ("s", getattr(ds, "sf", {})),
+                     ("y", getattr(ds, "hy", {}))):
         for k, ptr in d.items():
             shape, dt = ds._shapes[(group, k)]
             nb = int(np.prod(shape)) * np.dtype(dt).itemsize
@@ -65,7 +66,7 @@ def snapshot(ctx, ds):
 
 def restore(ctx, ds, snap):
     for (group, k), (q, nb) in snap.items():
-        dst = {"c": ds.col, "p": ds.pch, "g": ds.can, "s": getattr(ds, "sf", {})}[group][k]
+        dst = {"c": ds.col, "p": ds.pch, "g": ds.can, "s": getattr(ds, "sf", {}), "y": getattr(ds, "hy", {})}[group][k]
         ctx.copy(dst, q, nb)
 
 

@@ -46,6 +46,11 @@ void launch_soilfluxes(hipStream_t, const ctsm_layout &, const ctsm_config &, co
                        const int32_t *, double, ErrWord *);
 void launch_balance(hipStream_t, const ctsm_layout &, const ctsm_config &, const double *, int, const int32_t *,
                     unsigned long long *);
+void launch_surface_water(hipStream_t, const ctsm_layout &, double, const ctsm_hydro_params &, const ctsm_col_state &,
+                          const ctsm_surface_water_state &, int, const int32_t *, double, ErrWord *);
+void launch_patch2col3(hipStream_t, const ctsm_layout &, const ctsm_col_state &, const int32_t *, const double *,
+                       const double *, const double *, const double *, double *, double *, double *, int,
+                       const int32_t *, ErrWord *);
 }  // namespace ctsm
 
 using namespace ctsm;
@@ -183,6 +188,22 @@ const FieldMeta kSfCol[] = {
     {offsetof(ctsm_soilflux_state, t_h2osfc_bef), K_C, 8, false},
     {offsetof(ctsm_soilflux_state, errsoi_col), K_C, 8, true},
 };
+// surface water input chain: column members only
+#define SW(name, wr) {offsetof(ctsm_surface_water_state, name), K_C, 8, wr}
+const FieldMeta kSw[] = {
+    SW(qflx_rain_plus_snomelt, false), SW(qflx_snow_h2osfc, false), SW(qflx_floodc, false),
+    SW(qflx_liqevap_from_top_layer, false), SW(qflx_ev_soil, false), SW(qflx_ev_h2osfc, false),
+    SW(zwt, false), SW(zwt_perched, false), SW(frost_table, false), SW(wtfact, false),
+    SW(topo_slope, false), SW(h2osfc_thresh, false), SW(frac_h2osfc_nosnow, false), SW(h2osfc, true),
+    SW(fsat, true), SW(fcov, true), SW(qflx_sat_excess_surf, true), SW(qflx_top_soil, true), SW(qflx_in_soil, true),
+    SW(qflx_top_soil_to_h2osfc, true), SW(qinmax, true), SW(qflx_infl_excess, true), SW(qflx_in_soil_limited, true),
+    SW(qflx_in_h2osfc, true), SW(qflx_infl_excess_surf, true), SW(qflx_h2osfc_surf, true),
+    SW(qflx_h2osfc_drain, true), SW(qflx_infl, true), SW(qflx_surf, true),
+};
+#undef SW
+constexpr int kNSw = sizeof(kSw) / sizeof(kSw[0]);
+static_assert(kNSw * sizeof(void *) == sizeof(ctsm_surface_water_state),
+              "every ctsm_surface_water_state member has a mirror entry");
 constexpr int kNSfP = sizeof(kSfPatch) / sizeof(kSfPatch[0]);
 constexpr int kNSfC = sizeof(kSfCol) / sizeof(kSfCol[0]);
 static_assert((kNSfP + kNSfC) * sizeof(void *) == sizeof(ctsm_soilflux_state),
@@ -282,6 +303,7 @@ struct ctsm_ctx {
     DevBuf filt2;                      // second filter (drop-in calls taking column and patch filters)
     DevBuf bal;                        // balance-check reduction word
     std::vector<DevBuf> sfp, sfc;      // SoilFluxes drop-in mirrors
+    std::vector<DevBuf> swm;           // surface-water drop-in mirrors
     bool have_canopy = false;
     int32_t last_code = 0, last_index = 0;
     std::mutex mu;
@@ -569,6 +591,7 @@ int ctsm_destroy(ctsm_ctx *ctx)
     ctx->bal.release();
     for (auto &b : ctx->sfp) b.release();
     for (auto &b : ctx->sfc) b.release();
+    for (auto &b : ctx->swm) b.release();
     ctx->partf.release();
     for (int k = 0; k < ctsm_ctx::kMaxStreams - 1; ++k) {
         if (ctx->side[k]) { (void)hipStreamSynchronize(ctx->side[k]); (void)hipStreamDestroy(ctx->side[k]); }
@@ -1114,13 +1137,63 @@ int ctsm_soil_balance_check(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_so
     return ctsm_soil_balance_check_dev(ctx, lay, &dsf, n, dfc, errsoi_max, index);
 }
 
+static bool sw_complete(const ctsm_surface_water_state *sw)
+{
+    for (int f = 0; f < kNSw; ++f)
+        if (!*(void *const *)((const char *)sw + kSw[f].offset)) return false;
+    return true;
+}
+
+int ctsm_surface_water_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                           const ctsm_surface_water_state *sw, const ctsm_hydro_params *prm, int32_t n,
+                           const int32_t *filter_dev, double dtime)
+{
+    if (!ctx || !col || !sw || !prm || n < 0 || check_layout(lay) || (!filter_dev && n > lay->ldc) ||
+        !sw_complete(sw) || !col->snl || !col->frac_sno_eff || !col->frac_h2osfc || !col->icefrac || !col->hksat)
+        return CTSM_E_ARG;
+    launch_surface_water(ctx->stream, *lay, ctx->cfg.e_ice, *prm, *col, *sw, n, filter_dev, dtime, ctx->err);
+    return finish_launch(ctx);
+}
+
+int ctsm_surface_water(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                       const ctsm_surface_water_state *sw, const ctsm_hydro_params *prm, int32_t n,
+                       const int32_t *filter, double dtime)
+{
+    if (!ctx || !col || !sw || !prm || n < 0 || check_layout(lay) || (!filter && n > lay->ldc) || !sw_complete(sw) ||
+        !col->snl || !col->frac_sno_eff || !col->frac_h2osfc || !col->icefrac || !col->hksat)
+        return CTSM_E_ARG;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipError_t he = hipSetDevice(ctx->device);
+    if (he != hipSuccess) return hip_fail(ctx, he);
+    for (int i = 0; filter && i < n; ++i)
+        if (filter[i] < lay->begc_alloc || filter[i] >= lay->begc_alloc + lay->ldc) return CTSM_E_ARG;
+    ctsm_col_state hc{};
+    hc.snl = col->snl; hc.frac_sno_eff = col->frac_sno_eff; hc.frac_h2osfc = col->frac_h2osfc;
+    hc.icefrac = col->icefrac; hc.hksat = col->hksat;
+    const Range cr{lay->begc_alloc, lay->begc_alloc + lay->ldc - 1};
+    ctsm_col_state dcol{};
+    ctsm_surface_water_state dsw{};
+    const int ncf = (int)(sizeof(kColFields) / sizeof(kColFields[0]));
+    int rc = mirror(ctx, kColFields, ncf, &hc, &dcol, ctx->colmir, lay->begc_alloc, lay->ldc, cr, true);
+    if (!rc) rc = mirror(ctx, kSw, kNSw, sw, &dsw, ctx->swm, lay->begc_alloc, lay->ldc, cr, true);
+    if (rc) return rc;
+    const int32_t *dfc = nullptr;
+    if ((rc = upload_filter(ctx, n, filter, &dfc))) return rc;
+    launch_surface_water(ctx->stream, *lay, ctx->cfg.e_ice, *prm, dcol, dsw, n, dfc, dtime, ctx->err);
+    if ((rc = finish_launch(ctx))) return rc;
+    if ((rc = mirror(ctx, kSw, kNSw, sw, &dsw, ctx->swm, lay->begc_alloc, lay->ldc, cr, false))) return rc;
+    return sync_and_check(ctx);
+}
+
 int ctsm_land_step_full_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
                             const ctsm_patch_state *pch, const ctsm_canopy_state *can, const ctsm_soilflux_state *sf,
-                            int32_t nc, const int32_t *filterc_dev, int32_t np, const int32_t *filterp_dev,
-                            double dtime)
+                            const ctsm_surface_water_state *sw, const ctsm_hydro_params *hprm, int32_t nc,
+                            const int32_t *filterc_dev, int32_t np, const int32_t *filterp_dev, double dtime)
 {
     if (!ctx || !col || !pch || !can || !sf || nc < 0 || np < 0 || check_canopy_layout(ctx, lay) ||
         (!filterc_dev && nc > lay->ldc) || (!filterp_dev && np > lay->ldp) || !sf_complete(sf))
+        return CTSM_E_ARG;
+    if (sw && (!hprm || !sw_complete(sw) || sw->qflx_infl != col->qflx_infl || sw->h2osfc != col->h2osfc))
         return CTSM_E_ARG;
     // the arrays one routine writes and the next reads must coincide
     if (pch->dlrad != can->dlrad || pch->cgrnd != can->cgrnd || pch->eflx_sh_grnd != can->eflx_sh_grnd ||
@@ -1144,6 +1217,12 @@ int ctsm_land_step_full_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_co
                               nc, ctx->err, ctx->tun.soiltemp_block);
     launch_soilfluxes(ctx->stream, *lay, ctx->cfg, *col, *pch, *sf, nc, filterc_dev, np, filterp_dev, dtime, ctx->err);
     launch_root_sink(ctx->stream, *lay, ctx->cfg, *col, *can, nc, filterc_dev, ctx->err);
+    if (sw) {
+        launch_patch2col3(ctx->stream, *lay, *col, can->active, can->wtcol, sf->qflx_liqevap_from_top_layer,
+                          can->qflx_ev_soil, can->qflx_ev_h2osfc, sw->qflx_liqevap_from_top_layer, sw->qflx_ev_soil,
+                          sw->qflx_ev_h2osfc, nc, filterc_dev, ctx->err);
+        launch_surface_water(ctx->stream, *lay, ctx->cfg.e_ice, *hprm, *col, *sw, nc, filterc_dev, dtime, ctx->err);
+    }
     launch_soilwater(ctx->stream, V, ctx->prm, *col, nc, filterc_dev, dtime, ctx->err, ctx->tun.richards_lanes,
                      ctx->tun.richards_block);
     return finish_launch(ctx);

@@ -1,5 +1,5 @@
 /*
- * ctsm_math.h -- deterministic binary64 log / exp / pow / log10 / atan.
+ * ctsm_math.h -- deterministic binary64 log / exp / pow / log10 / atan / sin.
  *
  * One implementation, compiled for the gfx950 kernels and for host code, so
  * both produce the same bits for the same arguments: only IEEE operations
@@ -21,6 +21,7 @@
  *   pow(x,y) = exp(y * log x) with y*log x formed in double-double.
  *   atan(x):  table of atan(k/8) plus the odd series of a reduced argument
  *           (see cm_atan).
+ *   sin(x):   three-part reduction by pi/2, Taylor polynomials (see cm_sin).
  * Results are within about 1 ulp of the true value (checked against the host
  * libm in tests/test_math.py); they are not claimed correctly rounded.
  * Constants and tables (ln 2, 1/ln 10, log(k/128), 2^(j/64), pi/2,
@@ -406,6 +407,47 @@ CM_FN double cm_pow(double x, double y)
     const double zh = y * lh;
     const double zl = fma(y, lh, -zh) + y * ll;
     return cm_exp_dd(zh, zl);
+}
+
+/* sin(x) for |x| < 2^20 (the physics takes it of slopes in radians, 0..pi/2).
+ * x = k pi/2 + r with k the nearest integer of x 2/pi and pi/2 split in three
+ * parts (33, 33 and 53 significant bits, so k P1 and k P2 are exact), r kept
+ * as r + r_lo, |r| <= pi/4; then sin r = r + r^3 S(r^2) or
+ * cos r = 1 - r^2/2 + r^4 C(r^2), the Taylor series to r^17 / r^18
+ * (truncation below 1e-19 for |r| <= pi/4), with the tail r_lo and the
+ * rounding of 1 - r^2/2 carried as in fdlibm's kernel_sin / kernel_cos
+ * (Sun Microsystems, 1993). */
+#define CM_PIO2_1 0x1.921fb54400000p+0
+#define CM_PIO2_2 0x1.0b4611a600000p-34
+#define CM_PIO2_3 0x1.3198a2e037073p-69
+#define CM_2_PI 0x1.45f306dc9c883p-1
+CM_FN double cm_sin(double x)
+{
+    if (x != x || fabs(x) == HUGE_VAL) return x - x;
+    if (x == 0.0) return x;
+    if (!(fabs(x) < 0x1.0p+20)) return sin(x);
+    const double kd = rint(x * CM_2_PI);
+    const double t = (x - kd * CM_PIO2_1) - kd * CM_PIO2_2;
+    const double p3 = kd * CM_PIO2_3;
+    const double r = t - p3;
+    const double rl = (t - r) - p3;
+    const double z = r * r;
+    const int q = ((int)kd) & 3;
+    double v;
+    if ((q & 1) == 0) {
+        const double S = -0x1.5555555555555p-3 + z * (0x1.1111111111111p-7 + z * (-0x1.a01a01a01a01ap-13
+                          + z * (0x1.71de3a556c734p-19 + z * (-0x1.ae64567f544e4p-26 + z * (0x1.6124613a86d09p-33
+                          + z * (-0x1.ae7f3e733b81fp-41 + z * 0x1.952c77030ad4ap-49))))));
+        v = r + (r * (z * S) + rl);
+    } else {
+        const double C4 = 0x1.5555555555555p-5 + z * (-0x1.6c16c16c16c17p-10 + z * (0x1.a01a01a01a01ap-16
+                           + z * (-0x1.27e4fb7789f5cp-22 + z * (0x1.1eed8eff8d898p-29 + z * (-0x1.93974a8c07c9dp-37
+                           + z * (0x1.ae7f3e733b81fp-45 + z * -0x1.6827863b97d97p-53))))));
+        const double hz = 0.5 * z;
+        const double w = 1.0 - hz;
+        v = w + (((1.0 - w) - hz) + (z * (z * C4) - r * rl));
+    }
+    return (q & 2) ? -v : v;
 }
 
 #endif /* CTSM_MATH_H */

@@ -1,0 +1,186 @@
+// hydrology.hip -- the surface water input chain of HydrologyNoDrainage on gfx950.
+//
+// Replaces HydrologyNoDrainageMod.F90:306-336 for soil / crop columns:
+// SaturatedExcessRunoff (TOPMODEL fsat, SaturatedExcessRunoffMod.F90:203-360),
+// SetQflxInputs (SoilHydrologyMod.F90:298-366), InfiltrationExcessRunoff with
+// qinmax from hksat (InfiltrationExcessRunoffMod.F90), RouteInfiltrationExcess
+// (SoilHydrologyMod.F90:369-423), UpdateH2osfc with QflxH2osfcSurf and
+// QflxH2osfcDrain (SurfaceWaterMod.F90:345-560), Infiltration (:426-457) and
+// TotalSurfaceRunoff (:460-550).  Every routine is a per-column update, so the
+// chain is one lane per column reading each input once and writing each
+// output once (HBM-bound, ~30 FP64 per column); the routines' order inside a
+// column is the reference's, so the values are the reference's.
+//
+// k_patch2col: the column values of qflx_liqevap_from_top_layer,
+// qflx_ev_soil and qflx_ev_h2osfc that SetQflxInputs reads, as clm_drv_patch2col
+// forms them (p2c_1d_filter, subgridAveMod.F90:312-318), for the land step.
+#include "common.h"
+
+namespace ctsm {
+namespace hydro {
+
+constexpr double RPI = 3.14159265358979323846;
+
+struct HView {
+    int begc, ldc, begp, ldp;
+    __device__ __forceinline__ long c1(int c) const { return (long)(c - begc); }
+    __device__ __forceinline__ long soi(int c, int j) const { return (long)(c - begc) + (long)(j - 1) * ldc; }
+    __device__ __forceinline__ long p1(int p) const { return (long)(p - begp); }
+};
+
+__global__ void __launch_bounds__(256) k_surface_water(HView V, double e_ice, ctsm_hydro_params P, ctsm_col_state S,
+                                                       ctsm_surface_water_state W, int n,
+                                                       const int32_t *__restrict__ filter, double dtime,
+                                                       ErrWord *err)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = filter ? filter[i] : V.begc + i;
+    if (c < V.begc || c >= V.begc + V.ldc) { report_error(err, c, CTSM_E_ARG); return; }
+    const long cc = V.c1(c);
+
+    // SaturatedExcessRunoff, ComputeFsatTopmodel
+    const double zwt = W.zwt[cc], zwtp = W.zwt_perched[cc], ft = W.frost_table[cc];
+    double fsat;
+    if (ft > zwtp && ft <= zwt) fsat = W.wtfact[cc] * cm_exp(-0.5 * P.fff * zwtp);
+    else fsat = W.wtfact[cc] * cm_exp(-0.5 * P.fff * zwt);
+    const double rain = W.qflx_rain_plus_snomelt[cc];
+    const double sat_excess = fsat * rain;
+    W.fsat[cc] = fsat;
+    W.qflx_sat_excess_surf[cc] = sat_excess;
+    W.fcov[cc] = fsat;
+
+    // SetQflxInputs
+    const double top = rain + W.qflx_snow_h2osfc[cc] + W.qflx_floodc[cc];
+    double fsno, qevap;
+    if (S.snl[cc] >= 0) {
+        fsno = 0.0;
+        qevap = W.qflx_liqevap_from_top_layer[cc];
+    } else {
+        fsno = S.frac_sno_eff[cc];
+        qevap = W.qflx_ev_soil[cc];
+    }
+    const double fh = S.frac_h2osfc[cc];
+    double in_soil = (1.0 - fh) * (top - sat_excess);
+    double to_h2osfc = fh * (top - sat_excess);
+    in_soil = in_soil - (1.0 - fsno - fh) * qevap;
+    to_h2osfc = to_h2osfc - fh * W.qflx_ev_h2osfc[cc];
+    W.qflx_top_soil[cc] = top;
+    W.qflx_in_soil[cc] = in_soil;
+    W.qflx_top_soil_to_h2osfc[cc] = to_h2osfc;
+
+    // InfiltrationExcessRunoff, ComputeQinmaxHksat (minval over layers 1..3)
+    double qmin = 0.0;
+#pragma unroll
+    for (int j = 1; j <= 3; ++j) {
+        const double v = cm_pow(10.0, -e_ice * S.icefrac[V.soi(c, j)]) * S.hksat[V.soi(c, j)];
+        if (j == 1 || v < qmin) qmin = v;
+    }
+    const double qinmax = (1.0 - fsat) * qmin;
+    const double infl_excess = fmax90(0.0, (in_soil - (1.0 - fh) * qinmax));
+    W.qinmax[cc] = qinmax;
+    W.qflx_infl_excess[cc] = infl_excess;
+
+    // RouteInfiltrationExcess (soil / crop)
+    const double in_soil_limited = in_soil - infl_excess;
+    double in_h2osfc, infl_excess_surf;
+    if (P.h2osfcflag != 0) {
+        in_h2osfc = to_h2osfc + infl_excess;
+        infl_excess_surf = 0.0;
+    } else {
+        in_h2osfc = to_h2osfc;
+        infl_excess_surf = infl_excess;
+    }
+    W.qflx_in_soil_limited[cc] = in_soil_limited;
+    W.qflx_in_h2osfc[cc] = in_h2osfc;
+    W.qflx_infl_excess_surf[cc] = infl_excess_surf;
+
+    // UpdateH2osfc: QflxH2osfcSurf
+    double frac_infclust = 0.0;
+    if (P.h2osfcflag == 1) {
+        const double fn = W.frac_h2osfc_nosnow[cc];
+        if (fn <= P.pc) frac_infclust = 0.0;
+        else frac_infclust = cm_pow(fn - P.pc, P.mu);
+    }
+    const double h0 = W.h2osfc[cc], th = W.h2osfc_thresh[cc];
+    double surf;
+    if (h0 > th && P.h2osfcflag != 0) {
+        const double k_wet = 1.0e-4 * cm_sin((RPI / 180.0) * W.topo_slope[cc]);
+        surf = k_wet * frac_infclust * (h0 - th);
+        surf = fmin90(surf, (h0 - th) / dtime);
+    } else {
+        surf = 0.0;
+    }
+    if (surf < (double)1.0e-8f) surf = 0.0;      // default-REAL literal in SurfaceWaterMod.F90:497
+    W.qflx_h2osfc_surf[cc] = surf;
+    double hp = h0 + (in_h2osfc - surf) * dtime;
+    if (fabs(hp) < 1.e-13 * fabs(h0)) hp = 0.0;
+    // QflxH2osfcDrain
+    double drain;
+    if (hp < 0.0) {
+        drain = hp / dtime;
+    } else {
+        drain = fmin90(fh * qinmax, hp / dtime);
+        if (P.h2osfcflag == 0) drain = fmax90(0.0, hp / dtime);
+    }
+    W.qflx_h2osfc_drain[cc] = drain;
+    double h1 = hp - drain * dtime;
+    if (fabs(h1) < 1.e-13 * fabs(hp)) h1 = 0.0;
+    W.h2osfc[cc] = h1;
+
+    // Infiltration, TotalSurfaceRunoff
+    W.qflx_infl[cc] = in_soil_limited + drain;
+    W.qflx_surf[cc] = sat_excess + infl_excess_surf + surf;
+}
+
+// p2c_1d_filter of three patch fluxes (clm_drv_patch2col, subgridAveMod.F90:312-318)
+__global__ void __launch_bounds__(256) k_patch2col(HView V, ctsm_col_state S, const int32_t *__restrict__ active,
+                                                   const double *__restrict__ wtcol, const double *__restrict__ pa,
+                                                   const double *__restrict__ pb, const double *__restrict__ pc,
+                                                   double *ca, double *cb, double *ccol, int n,
+                                                   const int32_t *__restrict__ filter, ErrWord *err)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = filter ? filter[i] : V.begc + i;
+    if (c < V.begc || c >= V.begc + V.ldc) { report_error(err, c, CTSM_E_ARG); return; }
+    const long cc = V.c1(c);
+    const int pi = S.patchi[cc], pf = S.patchf[cc];
+    if (pf >= pi && (pi < V.begp || pf >= V.begp + V.ldp)) { report_error(err, c, CTSM_E_ARG); return; }
+    double a = 0.0, b = 0.0, d = 0.0;
+    for (int p = pi; p <= pf; ++p) {
+        const long pp = V.p1(p);
+        if (!active[pp]) continue;
+        const double w = wtcol[pp];
+        a = a + pa[pp] * w;
+        b = b + pb[pp] * w;
+        d = d + pc[pp] * w;
+    }
+    ca[cc] = a;
+    cb[cc] = b;
+    ccol[cc] = d;
+}
+
+}  // namespace hydro
+
+void launch_surface_water(hipStream_t st, const ctsm_layout &L, double e_ice, const ctsm_hydro_params &P,
+                          const ctsm_col_state &S, const ctsm_surface_water_state &W, int n, const int32_t *filter,
+                          double dtime, ErrWord *err)
+{
+    if (n <= 0) return;
+    const hydro::HView V{L.begc_alloc, L.ldc, L.begp_alloc, L.ldp};
+    hipLaunchKernelGGL(hydro::k_surface_water, dim3((n + 255) / 256), dim3(256), 0, st, V, e_ice, P, S, W, n, filter,
+                       dtime, err);
+}
+
+void launch_patch2col3(hipStream_t st, const ctsm_layout &L, const ctsm_col_state &S, const int32_t *active,
+                       const double *wtcol, const double *pa, const double *pb, const double *pc, double *ca,
+                       double *cb, double *ccol, int n, const int32_t *filter, ErrWord *err)
+{
+    if (n <= 0) return;
+    const hydro::HView V{L.begc_alloc, L.ldc, L.begp_alloc, L.ldp};
+    hipLaunchKernelGGL(hydro::k_patch2col, dim3((n + 255) / 256), dim3(256), 0, st, V, S, active, wtcol, pa, pb, pc,
+                       ca, cb, ccol, n, filter, err);
+}
+
+}  // namespace ctsm

@@ -158,10 +158,32 @@ module ctsm_gpu_mod
      type(c_ptr) :: errsoi_col = c_null_ptr
   end type ctsm_soilflux_state_t
 
+  ! Surface water input chain state (member order and meaning as ctsm_surface_water_state)
+  type, bind(C), public :: ctsm_surface_water_state_t
+     type(c_ptr) :: qflx_rain_plus_snomelt = c_null_ptr, qflx_snow_h2osfc = c_null_ptr, qflx_floodc = c_null_ptr
+     type(c_ptr) :: qflx_liqevap_from_top_layer = c_null_ptr, qflx_ev_soil = c_null_ptr, qflx_ev_h2osfc = c_null_ptr
+     type(c_ptr) :: zwt = c_null_ptr, zwt_perched = c_null_ptr, frost_table = c_null_ptr, wtfact = c_null_ptr
+     type(c_ptr) :: topo_slope = c_null_ptr, h2osfc_thresh = c_null_ptr, frac_h2osfc_nosnow = c_null_ptr
+     type(c_ptr) :: h2osfc = c_null_ptr
+     type(c_ptr) :: fsat = c_null_ptr, fcov = c_null_ptr, qflx_sat_excess_surf = c_null_ptr
+     type(c_ptr) :: qflx_top_soil = c_null_ptr, qflx_in_soil = c_null_ptr, qflx_top_soil_to_h2osfc = c_null_ptr
+     type(c_ptr) :: qinmax = c_null_ptr, qflx_infl_excess = c_null_ptr, qflx_in_soil_limited = c_null_ptr
+     type(c_ptr) :: qflx_in_h2osfc = c_null_ptr, qflx_infl_excess_surf = c_null_ptr
+     type(c_ptr) :: qflx_h2osfc_surf = c_null_ptr, qflx_h2osfc_drain = c_null_ptr, qflx_infl = c_null_ptr
+     type(c_ptr) :: qflx_surf = c_null_ptr
+  end type ctsm_surface_water_state_t
+
+  ! fff, pc, mu (param file) and h2osfcflag, as ctsm_hydro_params
+  type, bind(C), public :: ctsm_hydro_params_t
+     real(c_double) :: fff, pc, mu
+     integer(c_int32_t) :: h2osfcflag
+     integer(c_int32_t) :: pad = 0
+  end type ctsm_hydro_params_t
+
   public :: ctsm_config_default, ctsm_create, ctsm_destroy, ctsm_last_error
   public :: ctsm_soilwater, ctsm_soiltemperature, ctsm_soilwater_fractions
   public :: ctsm_set_canopy_params, ctsm_canopyfluxes, ctsm_root_sink
-  public :: ctsm_soilfluxes, ctsm_soil_balance_check
+  public :: ctsm_soilfluxes, ctsm_soil_balance_check, ctsm_surface_water
 
   interface
      subroutine ctsm_config_default(cfg) bind(C, name='ctsm_config_default')
@@ -276,6 +298,20 @@ module ctsm_gpu_mod
        real(c_double), intent(out) :: errsoi_max
        integer(c_int32_t), intent(out) :: idx
      end function ctsm_soil_balance_check
+
+     integer(c_int) function ctsm_surface_water(ctx, lay, col, sw, prm, num_hydrologyc, filter_hydrologyc, &
+          dtime) bind(C, name='ctsm_surface_water')
+       import :: c_int, c_int32_t, c_double, c_ptr, ctsm_layout_t, ctsm_col_state_t, &
+            ctsm_surface_water_state_t, ctsm_hydro_params_t
+       type(c_ptr), value :: ctx
+       type(ctsm_layout_t), intent(in) :: lay
+       type(ctsm_col_state_t), intent(in) :: col
+       type(ctsm_surface_water_state_t), intent(in) :: sw
+       type(ctsm_hydro_params_t), intent(in) :: prm
+       integer(c_int32_t), value :: num_hydrologyc
+       integer(c_int32_t), intent(in) :: filter_hydrologyc(*)
+       real(c_double), value :: dtime
+     end function ctsm_surface_water
   end interface
 
 end module ctsm_gpu_mod

@@ -142,7 +142,8 @@ class EnsembleServer:
     def _copy(self, req, to_pool):
         st, d = req.state, self.dims
         for group, host, dev in (("c", st.col, self.pool.col), ("p", st.pch, self.pool.pch),
-                                 ("g", st.can, self.pool.can), ("s", st.sf, self.pool.sf)):
+                                 ("g", st.can, self.pool.can), ("s", st.sf, self.pool.sf),
+                                 ("y", st.hy, self.pool.hy)):
             for k, a in host.items():
                 shape, dt = self.pool._shapes[(group, k)]
                 isp = (group == "p" or (group == "g" and _CAN_KIND[k] in ("p", "soip"))

@@ -37,6 +37,10 @@ SOILFLUX_ALIAS = {
 }
 SOILFLUX_OWN = [(n, t, k) for n, t, k in abi.SOILFLUX_FIELDS if n not in SOILFLUX_ALIAS]
 
+# surface-water fields that are column-state arrays; the rest are its own
+SURFWATER_ALIAS = {"h2osfc": ("c", "h2osfc"), "qflx_infl": ("c", "qflx_infl")}
+SURFWATER_OWN = [(n, t, k) for n, t, k in abi.SURFWATER_FIELDS if n not in SURFWATER_ALIAS]
+
 
 class Dims:
     def __init__(self, ncol, npatch, nlevsno=12, nlevsoi=20, nlevgrnd=25, ngrid=0):
@@ -81,6 +85,12 @@ class HostState:
                 if fill_nan and t == "f64":
                     a[...] = np.nan
                 self.sf[name] = a
+        self.hy = {}
+        for name, t, kind in SURFWATER_OWN:
+            a = np.zeros(dims.shape(kind), dtype=_NP[t])
+            if fill_nan and t == "f64":
+                a[...] = np.nan
+            self.hy[name] = a
 
     def copy(self):
         o = HostState.__new__(HostState)
@@ -89,6 +99,7 @@ class HostState:
         o.pch = {k: v.copy() for k, v in self.pch.items()}
         o.can = {k: v.copy() for k, v in self.can.items()}
         o.sf = {k: v.copy() for k, v in getattr(self, "sf", {}).items()}
+        o.hy = {k: v.copy() for k, v in getattr(self, "hy", {}).items()}
         return o
 
     def canopy_array(self, name):
@@ -127,6 +138,14 @@ class HostState:
             assert a.flags.c_contiguous
             setattr(ss, name, a.ctypes.data_as(abi._CT[t]))
         return ss
+
+    def surfwater_struct(self):
+        sw = abi.SurfaceWaterState()
+        for name, t, _ in abi.SURFWATER_FIELDS:
+            a = self.col[SURFWATER_ALIAS[name][1]] if name in SURFWATER_ALIAS else self.hy[name]
+            assert a.flags.c_contiguous
+            setattr(sw, name, a.ctypes.data_as(abi._CT[t]))
+        return sw
 
     def structs(self):
         """ctypes ColState / PatchState pointing at these numpy buffers."""
@@ -178,6 +197,7 @@ class HostState:
             for name, t, kind in SOILFLUX_OWN:
                 idx = pidx if kind == "p" else cols
                 o.sf[name] = np.ascontiguousarray(self.sf[name][..., idx])
+        o.hy = {k: np.ascontiguousarray(v[..., cols]) for k, v in getattr(self, "hy", {}).items()}
         return o
 
 
@@ -189,9 +209,10 @@ class DeviceState:
         self.ctx = ctx
         self.dims = host.dims
         self._shapes = {}
-        self.col, self.pch, self.can, self.sf = {}, {}, {}, {}
+        self.col, self.pch, self.can, self.sf, self.hy = {}, {}, {}, {}, {}
         for group, src, dst in (("c", host.col, self.col), ("p", host.pch, self.pch),
-                                ("g", getattr(host, "can", {}), self.can), ("s", getattr(host, "sf", {}), self.sf)):
+                                ("g", getattr(host, "can", {}), self.can), ("s", getattr(host, "sf", {}), self.sf),
+                                ("y", getattr(host, "hy", {}), self.hy)):
             for k, v in src.items():
                 a = np.ascontiguousarray(v)
                 ptr = ctx.alloc(max(a.nbytes, 8))
@@ -226,6 +247,13 @@ class DeviceState:
             setattr(ss, name, C.cast(C.c_void_p(ptr), abi._CT[t]))
         return ss
 
+    def surfwater_struct(self):
+        sw = abi.SurfaceWaterState()
+        for name, t, _ in abi.SURFWATER_FIELDS:
+            ptr = self.col[SURFWATER_ALIAS[name][1]] if name in SURFWATER_ALIAS else self.hy[name]
+            setattr(sw, name, C.cast(C.c_void_p(ptr), abi._CT[t]))
+        return sw
+
     def structs(self):
         cs = abi.ColState()
         for name, t, _ in abi.COL_FIELDS:
@@ -238,9 +266,9 @@ class DeviceState:
     def to_host(self):
         o = HostState.__new__(HostState)
         o.dims = self.dims
-        o.col, o.pch, o.can, o.sf = {}, {}, {}, {}
+        o.col, o.pch, o.can, o.sf, o.hy = {}, {}, {}, {}, {}
         for group, src, dst in (("c", self.col, o.col), ("p", self.pch, o.pch), ("g", self.can, o.can),
-                                ("s", self.sf, o.sf)):
+                                ("s", self.sf, o.sf), ("y", self.hy, o.hy)):
             for k, ptr in src.items():
                 shape, dt = self._shapes[(group, k)]
                 a = np.empty(shape, dtype=dt)
@@ -251,7 +279,8 @@ class DeviceState:
     def upload(self, host):
         """Overwrite the device copy with a host state of the same dims."""
         for group, src, dst in (("c", host.col, self.col), ("p", host.pch, self.pch),
-                                ("g", getattr(host, "can", {}), self.can), ("s", getattr(host, "sf", {}), self.sf)):
+                                ("g", getattr(host, "can", {}), self.can), ("s", getattr(host, "sf", {}), self.sf),
+                                ("y", getattr(host, "hy", {}), self.hy)):
             for k, v in src.items():
                 self.ctx.h2d(dst[k], np.ascontiguousarray(v))
 
@@ -259,7 +288,7 @@ class DeviceState:
         return sum(int(np.prod(s)) * np.dtype(d).itemsize for s, d in self._shapes.values())
 
     def free(self):
-        for d in (self.col, self.pch, self.can, self.sf):
+        for d in (self.col, self.pch, self.can, self.sf, self.hy):
             for k, ptr in d.items():
                 self.ctx.free(ptr)
             d.clear()

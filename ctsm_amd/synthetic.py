@@ -297,6 +297,13 @@ def canopy_params():
                 jmax25top_sf=1.0, leaf_mr_vcm=0.015)
 
 
+def hydro_params(fff=0.5, pc=0.4, mu=0.14, h2osfcflag=1):
+    """Representative param-file scalars of the surface water chain (fff of
+    SaturatedExcessRunoffMod, pc and mu of SurfaceWaterMod) and h2osfcflag."""
+    from .abi import HydroParams
+    return HydroParams(fff, pc, mu, h2osfcflag, 0)
+
+
 def pft_table():
     """Rows 0..16 (bare soil, 14 natural PFTs, two crops) with representative
     pftcon values; row 14 is the C4 grass."""
@@ -503,6 +510,33 @@ def with_canopy(soil, seed, pmax=16, pmin=1, col0=0):
         st.sf[k][...] = 0.0
     st.sf["t_ssbef"][...] = c["t_soisno"]
     st.sf["t_h2osfc_bef"][:] = c["t_h2osfc"]
+    with_hydrology(st, seed, col0=col0)
+    return st
+
+
+S_RAIN, S_ZWT, S_ZWTP, S_FROST, S_WTF, S_SLOPE, S_THRESH, S_EVAP = 70, 71, 72, 73, 74, 75, 76, 77
+
+
+def with_hydrology(st, seed, col0=0):
+    """Inputs of the surface water input chain (rain and snow melt, water
+    tables, frost table, slope, micro-topography threshold, column
+    evaporation fluxes), keyed by global column id; outputs cleared.  The
+    column's frac_h2osfc / h2osfc are left as they are."""
+    c, h = st.col, st.hy
+    cols = np.arange(st.dims.ncol, dtype=np.int64) + col0
+    for k in h:
+        h[k][...] = 0.0
+    h["qflx_rain_plus_snomelt"][:] = 5e-4 * uniform(seed, S_RAIN, cols)
+    zwt = 0.2 + 7.8 * uniform(seed, S_ZWT, cols)
+    h["zwt"][:] = zwt
+    h["zwt_perched"][:] = zwt * (0.2 + 0.8 * uniform(seed, S_ZWTP, cols))
+    h["frost_table"][:] = 10.0 * uniform(seed, S_FROST, cols)
+    h["wtfact"][:] = 0.1 + 0.4 * uniform(seed, S_WTF, cols)
+    h["topo_slope"][:] = 30.0 * uniform(seed, S_SLOPE, cols)
+    h["h2osfc_thresh"][:] = 5.0 + 45.0 * uniform(seed, S_THRESH, cols)
+    h["frac_h2osfc_nosnow"][:] = c["frac_h2osfc"]
+    for j, k in enumerate(("qflx_liqevap_from_top_layer", "qflx_ev_soil", "qflx_ev_h2osfc")):
+        h[k][:] = -1e-6 + 5.1e-5 * uniform(seed, S_EVAP, cols, j)
     return st
 
 

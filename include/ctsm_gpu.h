@@ -393,6 +393,38 @@ typedef struct ctsm_soilflux_state {
     double        *errsoi_col;    /* (c) p2c of errsoi_patch (W/m2)                      */
 } ctsm_soilflux_state;
 
+/* ---- surface water inputs of the soil (HydrologyNoDrainageMod.F90:306-336) --
+ * Column arrays (begc:endc) of SaturatedExcessRunoff (TOPMODEL fsat),
+ * SetQflxInputs, InfiltrationExcessRunoff (qinmax from hksat), RouteInfiltrationExcess,
+ * UpdateH2osfc, Infiltration and TotalSurfaceRunoff for soil / crop columns;
+ * snl, frac_sno_eff, frac_h2osfc, icefrac and hksat come from ctsm_col_state.
+ * qflx_infl is the SoilWater input (ctsm_col_state.qflx_infl) and h2osfc the
+ * surface water (ctsm_col_state.h2osfc): pass the same arrays. */
+typedef struct ctsm_surface_water_state {
+    /* inputs */
+    const double  *qflx_rain_plus_snomelt, *qflx_snow_h2osfc, *qflx_floodc;
+    /* column values of the patch fluxes (p2c); ctsm_land_step_full_dev forms
+     * them from the patches, as clm_drv_patch2col does */
+    double        *qflx_liqevap_from_top_layer, *qflx_ev_soil, *qflx_ev_h2osfc;
+    const double  *zwt, *zwt_perched, *frost_table, *wtfact;                        /* SoilHydrologyType, SoilStateType */
+    const double  *topo_slope, *h2osfc_thresh, *frac_h2osfc_nosnow;
+    /* inout */
+    double        *h2osfc;
+    /* outputs */
+    double        *fsat, *fcov, *qflx_sat_excess_surf, *qflx_top_soil, *qflx_in_soil, *qflx_top_soil_to_h2osfc;
+    double        *qinmax, *qflx_infl_excess, *qflx_in_soil_limited, *qflx_in_h2osfc, *qflx_infl_excess_surf;
+    double        *qflx_h2osfc_surf, *qflx_h2osfc_drain, *qflx_infl, *qflx_surf;
+} ctsm_surface_water_state;
+
+/* Param-file / namelist scalars of that chain: fff (SaturatedExcessRunoffMod
+ * readParams), pc and mu (SurfaceWaterMod readParams), h2osfcflag
+ * (SoilHydrologyType, 1 = surface water active). */
+typedef struct ctsm_hydro_params {
+    double  fff, pc, mu;
+    int32_t h2osfcflag;
+    int32_t pad;
+} ctsm_hydro_params;
+
 /* ---- launch shapes (no counterpart in the reference: set by the autotuner) --
  * richards_lanes: lanes cooperating on one column in the Richards solve
  *   (1, 2, 4 or 8); the heat solve always runs one column per lane.
@@ -511,6 +543,16 @@ int ctsm_soil_balance_check(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_so
                             int32_t num_filterc, const int32_t *filterc,
                             double *errsoi_max, int32_t *index);
 
+/* Replaces the surface water input chain of HydrologyNoDrainage
+ * (HydrologyNoDrainageMod.F90:306-336: SaturatedExcessRunoff with the TOPMODEL
+ * fsat, SetQflxInputs, InfiltrationExcessRunoff with qinmax from hksat,
+ * RouteInfiltrationExcess, UpdateH2osfc, Infiltration, TotalSurfaceRunoff) for
+ * soil / crop columns; produces qflx_infl for SoilWater.  SetFloodc is the
+ * caller's (qflx_floodc is an input). */
+int ctsm_surface_water(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                       const ctsm_surface_water_state *sw, const ctsm_hydro_params *prm,
+                       int32_t num_hydrologyc, const int32_t *filter_hydrologyc, double dtime);
+
 /* ---- device-resident entry points -----------------------------------------
  * Same arguments, but every pointer (fields and filter) is device memory and
  * the call only enqueues work on the context's stream.  filter == NULL means
@@ -549,6 +591,9 @@ int ctsm_root_sink_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_sta
                        const ctsm_canopy_state *can,
                        int32_t num_filterc, const int32_t *filterc_dev);
 
+int ctsm_surface_water_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                           const ctsm_surface_water_state *sw, const ctsm_hydro_params *prm,
+                           int32_t num_filterc, const int32_t *filterc_dev, double dtime);
 int ctsm_soilfluxes_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
                         const ctsm_patch_state *pch, const ctsm_soilflux_state *sf,
                         int32_t num_filterc, const int32_t *filterc_dev,
@@ -574,13 +619,17 @@ int ctsm_land_step_dev(ctsm_ctx *ctx, const ctsm_layout *lay,
 /* The land step with the reference's bookkeeping around it: t_ssbef and
  * t_h2osfc_bef saved first (BiogeophysPreFluxCalcsMod.F90:302-331), then
  * CanopyFluxes, SoilTemperature and SetSoilWaterFractions, SoilFluxes
- * (clm_driver.F90:921), the transpiration sink, SoilWater.  Arrays shared by
- * two structs must be the same device arrays, as for ctsm_land_step_dev, and
- * the CanopyFluxes outputs SoilFluxes reads (sf->cgrnds, ...) must be the
- * canopy state's. */
+ * (clm_driver.F90:921), the transpiration sink and, when sw is not NULL, the
+ * column evaporation fluxes (clm_drv_patch2col) and the surface water input
+ * chain that sets qflx_infl (HydrologyNoDrainageMod.F90:306-336), then
+ * SoilWater.  Arrays shared by two structs must be the same device arrays, as
+ * for ctsm_land_step_dev, the CanopyFluxes outputs SoilFluxes reads
+ * (sf->cgrnds, ...) must be the canopy state's, and sw->qflx_infl /
+ * sw->h2osfc must be col->qflx_infl / col->h2osfc. */
 int ctsm_land_step_full_dev(ctsm_ctx *ctx, const ctsm_layout *lay,
                             const ctsm_col_state *col, const ctsm_patch_state *pch,
                             const ctsm_canopy_state *can, const ctsm_soilflux_state *sf,
+                            const ctsm_surface_water_state *sw, const ctsm_hydro_params *hprm,
                             int32_t num_filterc, const int32_t *filterc_dev,
                             int32_t num_filterp, const int32_t *filterp_dev, double dtime);
 

@@ -275,10 +275,50 @@ def soil_balance(st, filter_c=None):
     return m.value, idx.value
 
 
-def land_step_full(st, cfg, prm, pft, dtime=1800.0):
+def _hy_lib():
+    L = lib()
+    if not getattr(L, "_hy_ready", False):
+        P = C.POINTER
+        L.oracle_surface_water.argtypes = [P(abi.Config), P(abi.Layout), P(abi.ColState), P(abi.SurfaceWaterState),
+                                           P(abi.HydroParams), C.c_int, abi.i32p, C.c_double]
+        L.oracle_patch2col3.argtypes = [P(abi.Layout), P(abi.ColState), abi.i32p, abi.f64p] + [abi.f64p] * 6 + [
+            C.c_int, abi.i32p]
+        L._hy_ready = True
+    return L
+
+
+def hydro_params(fff=0.5, pc=0.4, mu=0.14, h2osfcflag=1):
+    return abi.HydroParams(fff, pc, mu, h2osfcflag, 0)
+
+
+def surface_water(st, hydro, cfg=None, dtime=1800.0, filter_c=None):
+    """The surface water input chain (HydrologyNoDrainageMod.F90:306-336)."""
+    L = _hy_lib()
+    cs, _ = st.structs()
+    sw = st.surfwater_struct()
+    f, fp = _filt(filter_c if filter_c is not None else np.arange(1, st.dims.ncol + 1))
+    return L.oracle_surface_water(C.byref(cfg or canopy_config()), C.byref(st.layout()), C.byref(cs), C.byref(sw),
+                                  C.byref(hydro), len(f), fp, C.c_double(dtime))
+
+
+def patch2col3(st, filter_c=None):
+    """Column values of the three evaporation fluxes SetQflxInputs reads."""
+    L = _hy_lib()
+    cs, _ = st.structs()
+    f, fp = _filt(filter_c if filter_c is not None else np.arange(1, st.dims.ncol + 1))
+    g, hy = st.can, st.hy
+    arrs = [st.canopy_array("active"), st.canopy_array("wtcol"), st.sf["qflx_liqevap_from_top_layer"],
+            st.canopy_array("qflx_ev_soil"), st.canopy_array("qflx_ev_h2osfc"),
+            hy["qflx_liqevap_from_top_layer"], hy["qflx_ev_soil"], hy["qflx_ev_h2osfc"]]
+    ptrs = [arrs[0].ctypes.data_as(abi.i32p)] + [a.ctypes.data_as(abi.f64p) for a in arrs[1:]]
+    L.oracle_patch2col3(C.byref(st.layout()), C.byref(cs), *ptrs, len(f), fp)
+
+
+def land_step_full(st, cfg, prm, pft, dtime=1800.0, hydro=None):
     """land_step with the reference's bookkeeping: t_ssbef / t_h2osfc_bef
     saved first, SoilFluxes between SoilTemperature and the hydrology
-    (clm_driver.F90:921)."""
+    (clm_driver.F90:921) and, given hydro, the column evaporation fluxes and
+    the surface water input chain before SoilWater."""
     save_before(st, cfg)
     st.pch["cgrnd"][:] = 0.0
     st.can["cgrnds"][:] = 0.0
@@ -292,6 +332,9 @@ def land_step_full(st, cfg, prm, pft, dtime=1800.0):
     soilwater_fractions(st, cfg)
     soilfluxes(st, cfg, dtime)
     root_sink(st, cfg)
+    if hydro is not None:
+        patch2col3(st)
+        surface_water(st, hydro, cfg, dtime)
     return soilwater(st, cfg, dtime)
 
 

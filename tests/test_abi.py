@@ -42,7 +42,8 @@ def test_library_exports_every_declared_symbol():
 STRUCTS = (("ctsm_config", abi.Config), ("ctsm_layout", abi.Layout), ("ctsm_tuning", abi.Tuning),
            ("ctsm_col_state", abi.ColState), ("ctsm_patch_state", abi.PatchState),
            ("ctsm_canopy_params", abi.CanopyParams), ("ctsm_pft_params", abi.PftParams),
-           ("ctsm_canopy_state", abi.CanopyState), ("ctsm_soilflux_state", abi.SoilFluxState))
+           ("ctsm_canopy_state", abi.CanopyState), ("ctsm_soilflux_state", abi.SoilFluxState),
+           ("ctsm_surface_water_state", abi.SurfaceWaterState), ("ctsm_hydro_params", abi.HydroParams))
 
 
 def _probe(tmp_path):
@@ -88,8 +89,10 @@ program sizes
   type(ctsm_pft_params_t) :: f
   type(ctsm_canopy_state_t) :: g
   type(ctsm_soilflux_state_t) :: h
-  print '(8I8)', c_sizeof(a), c_sizeof(b), c_sizeof(c), c_sizeof(d), c_sizeof(e), c_sizeof(f), c_sizeof(g), &
-       c_sizeof(h)
+  type(ctsm_surface_water_state_t) :: i
+  type(ctsm_hydro_params_t) :: j
+  print '(10I8)', c_sizeof(a), c_sizeof(b), c_sizeof(c), c_sizeof(d), c_sizeof(e), c_sizeof(f), c_sizeof(g), &
+       c_sizeof(h), c_sizeof(i), c_sizeof(j)
 end program sizes
 """)
     mod = os.path.join(ROOT, "ctsm_amd", "fortran", "ctsm_gpu_mod.F90")
@@ -98,7 +101,7 @@ end program sizes
     sizes = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
     assert sizes == [C.sizeof(abi.Config), C.sizeof(abi.Layout), C.sizeof(abi.ColState), C.sizeof(abi.PatchState),
                      C.sizeof(abi.CanopyParams), C.sizeof(abi.PftParams), C.sizeof(abi.CanopyState),
-                     C.sizeof(abi.SoilFluxState)]
+                     C.sizeof(abi.SoilFluxState), C.sizeof(abi.SurfaceWaterState), C.sizeof(abi.HydroParams)]
 
 
 def test_config_rejected_before_device():

@@ -131,6 +131,26 @@ def test_atan_within_one_ulp():
     assert _vec("atan", np.array([1.0]))[0] == math.atan(1.0)
 
 
+def test_sin_within_one_ulp():
+    mp = pytest.importorskip("mpmath")
+    mp.mp.dps = 50
+    r = _rng(6)
+    deg = r.uniform(0.0, 90.0, 600)
+    x = np.concatenate([(math.pi / 180.0) * deg, r.uniform(-10, 10, 400), 10.0 ** r.uniform(-9, 0, 200),
+                        r.uniform(-1000, 1000, 100), np.array([math.pi / 2, math.pi, 1e-300, -0.5, 0.0])])
+    got = _vec("sin", x)
+    worst = 0.0
+    for xi, gi in zip(x, got):
+        ex = mp.sin(mp.mpf(xi))
+        e = float(ex)
+        if e == 0.0:
+            assert gi == 0.0
+            continue
+        worst = max(worst, float(abs(mp.mpf(gi) - ex) / mp.mpf(math.ulp(e))))
+    assert worst < 1.0, worst
+    assert np.all(np.abs(got - np.sin(x)) <= 2 * np.spacing(np.abs(np.sin(x))))
+
+
 def test_pow_and_log_hard_cases():
     """x near 1 with large exponents (log cancellation), results near the
     overflow / underflow thresholds, and table-boundary arguments."""

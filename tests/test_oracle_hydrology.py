@@ -1,0 +1,218 @@
+"""Pins of the surface water oracle (oracle/hydrology_oracle.c) on the CPU.
+
+The surface water input chain (HydrologyNoDrainageMod.F90:306-336) has no
+golden vectors in the reference's tests; it is pinned by what the physics and
+the routines' closed forms fix, each evaluated independently with numpy:
+  * water balance of the surface: what reaches the ground (qflx_top_soil less
+    the evaporation SetQflxInputs removes) = qflx_infl + qflx_surf + the
+    change of h2osfc over the step, column by column, for both h2osfcflag
+    values — a dropped or mis-routed flux breaks it;
+  * h2osfc never negative after UpdateH2osfc, and with h2osfcflag = 0 the
+    pool is emptied into the soil every step;
+  * TOPMODEL fsat (both water-table branches), qinmax (minimum over layers
+    1..3 with ice impedance), the SetQflxInputs partition (snow and snow-free
+    branches), the overflow of the pool (QflxH2osfcSurf, including the
+    per-step cap and the 1e-8 cut) and the drainage limit, against numpy;
+  * special cases: no surface water fraction, frac_h2osfc_nosnow <= pc,
+    nothing falling and nothing evaporating;
+  * p2c of the three evaporation fluxes against a numpy weighted sum.
+"""
+import numpy as np
+import pytest
+
+from ctsm_amd import synthetic as S
+from oracle import oracle as O
+
+DT = 1800.0
+
+
+def _state(ncol=512, seed=0x5EED0011, snow=True, fh=True, deep=True):
+    """Soil columns with a surface water fraction and standing water that
+    overflows its threshold in about half of the columns."""
+    cols = np.arange(ncol, dtype=np.int64)
+    frac = 0.6 * S.uniform(seed, 90, cols) if fh else None
+    st = S.soil_only(ncol, seed, snow=snow, bedrock=True, frac_h2osfc=frac)
+    S.with_hydrology(st, seed)
+    if deep:
+        st.col["h2osfc"][:] = 100.0 * S.uniform(seed, 91, cols)
+    for j in range(3):      # some ice in the top layers, so the impedance term matters
+        st.col["icefrac"][j] = np.where(S.uniform(seed, 92, cols, j) < 0.3, S.uniform(seed, 93, cols, j), 0.0)
+    return st
+
+
+def _hy(st):
+    """All surface water fields; qflx_infl and h2osfc live in the column state."""
+    return dict(st.hy, qflx_infl=st.col["qflx_infl"], h2osfc=st.col["h2osfc"])
+
+
+def _run(st, hydro, dtime=DT):
+    h0 = st.col["h2osfc"].copy()
+    assert O.surface_water(st, hydro, dtime=dtime) == 0
+    return h0
+
+
+def _evap_removed(st):
+    c, h = st.col, st.hy
+    snow = c["snl"] < 0
+    fsno = np.where(snow, c["frac_sno_eff"], 0.0)
+    qevap = np.where(snow, h["qflx_ev_soil"], h["qflx_liqevap_from_top_layer"])
+    fh = c["frac_h2osfc"]
+    return (1.0 - fsno - fh) * qevap + fh * h["qflx_ev_h2osfc"]
+
+
+def _fsat(st, fff):
+    h = st.hy
+    perched = (h["frost_table"] > h["zwt_perched"]) & (h["frost_table"] <= h["zwt"])
+    z = np.where(perched, h["zwt_perched"], h["zwt"])
+    return h["wtfact"] * np.exp(-0.5 * fff * z), perched
+
+
+@pytest.mark.parametrize("flag", [1, 0])
+def test_surface_water_balance(flag):
+    st = _state()
+    hydro = O.hydro_params(h2osfcflag=flag)
+    h0 = _run(st, hydro)
+    h = _hy(st)
+    lhs = h["qflx_top_soil"] - _evap_removed(st)
+    rhs = h["qflx_infl"] + h["qflx_surf"] + (st.col["h2osfc"] - h0) / DT
+    scale = np.abs(h["qflx_top_soil"]) + np.abs(h0) / DT + 1e-12
+    assert np.max(np.abs(lhs - rhs) / scale) < 1e-12
+    assert np.all(st.col["h2osfc"] >= 0.0)
+    # the routes out of the surface add up
+    assert np.array_equal(h["qflx_infl"], h["qflx_in_soil_limited"] + h["qflx_h2osfc_drain"])
+    assert np.array_equal(h["qflx_surf"],
+                          h["qflx_sat_excess_surf"] + h["qflx_infl_excess_surf"] + h["qflx_h2osfc_surf"])
+
+
+def test_no_surface_water_pool_drains_everything():
+    st = _state()
+    _run(st, O.hydro_params(h2osfcflag=0))
+    h = _hy(st)
+    assert np.all(st.col["h2osfc"] == 0.0)
+    assert np.all(h["qflx_h2osfc_surf"] == 0.0)
+    assert np.array_equal(h["qflx_infl_excess_surf"], h["qflx_infl_excess"])
+    assert np.array_equal(h["qflx_in_h2osfc"], h["qflx_top_soil_to_h2osfc"])
+
+
+def test_fsat_and_qinmax_closed_forms():
+    st = _state()
+    hydro = O.hydro_params(fff=0.5)
+    _run(st, hydro)
+    h, c = _hy(st), st.col
+    fsat, perched = _fsat(st, 0.5)
+    assert perched.any() and (~perched).any()
+    assert np.allclose(h["fsat"], fsat, rtol=1e-15, atol=0)
+    assert np.array_equal(h["fcov"], h["fsat"])
+    assert np.allclose(h["qflx_sat_excess_surf"], fsat * h["qflx_rain_plus_snomelt"], rtol=1e-15, atol=0)
+    e_ice = O.canopy_config().e_ice
+    imp = 10.0 ** (-e_ice * c["icefrac"][:3]) * c["hksat"][:3]
+    assert (c["icefrac"][:3] > 0).any()
+    qinmax = (1.0 - fsat) * imp.min(axis=0)
+    assert np.allclose(h["qinmax"], qinmax, rtol=1e-14, atol=0)
+    excess = np.maximum(0.0, h["qflx_in_soil"] - (1.0 - c["frac_h2osfc"]) * qinmax)
+    assert np.allclose(h["qflx_infl_excess"], excess, rtol=1e-12, atol=1e-20)
+    assert (h["qflx_infl_excess"] > 0).any() and (h["qflx_infl_excess"] == 0).any()
+
+
+def test_input_partition_closed_form():
+    st = _state()
+    _run(st, O.hydro_params())
+    h, c = _hy(st), st.col
+    snow = c["snl"] < 0
+    assert snow.any() and (~snow).any()
+    top = h["qflx_rain_plus_snomelt"] + h["qflx_snow_h2osfc"] + h["qflx_floodc"]
+    assert np.array_equal(h["qflx_top_soil"], top)
+    fh = c["frac_h2osfc"]
+    fsno = np.where(snow, c["frac_sno_eff"], 0.0)
+    qevap = np.where(snow, h["qflx_ev_soil"], h["qflx_liqevap_from_top_layer"])
+    net = top - h["qflx_sat_excess_surf"]
+    assert np.allclose(h["qflx_in_soil"], (1.0 - fh) * net - (1.0 - fsno - fh) * qevap, rtol=1e-13, atol=1e-22)
+    assert np.allclose(h["qflx_top_soil_to_h2osfc"], fh * net - fh * h["qflx_ev_h2osfc"], rtol=1e-13, atol=1e-22)
+
+
+@pytest.mark.parametrize("dtime", [DT, 1.0e5])
+def test_overflow_and_drainage_closed_forms(dtime):
+    st = _state()
+    hydro = O.hydro_params(pc=0.4, mu=0.14)
+    h0 = _run(st, hydro, dtime=dtime)
+    h, c = _hy(st), st.col
+    th = h["h2osfc_thresh"]
+    fn = h["frac_h2osfc_nosnow"]
+    clust = np.where(fn > 0.4, np.maximum(fn - 0.4, 0.0) ** 0.14, 0.0)
+    kwet = 1.0e-4 * np.sin(np.deg2rad(h["topo_slope"]))
+    over = h0 > th
+    q = np.where(over, np.minimum(kwet * clust * (h0 - th), (h0 - th) / dtime), 0.0)
+    q = np.where(q < np.float64(np.float32(1.0e-8)), 0.0, q)
+    assert np.allclose(h["qflx_h2osfc_surf"], q, rtol=1e-13, atol=0)
+    assert (h["qflx_h2osfc_surf"] > 0).any()
+    if dtime > 1e4:     # the per-step cap binds for some columns
+        assert np.any((q > 0) & (q == (h0 - th) / dtime))
+    hp = h0 + (h["qflx_in_h2osfc"] - h["qflx_h2osfc_surf"]) * dtime
+    cap = np.where(hp < 0, hp / dtime, np.minimum(c["frac_h2osfc"] * h["qinmax"], hp / dtime))
+    assert np.allclose(h["qflx_h2osfc_drain"], cap, rtol=1e-12, atol=1e-18)
+    assert np.all(st.col["h2osfc"] >= 0.0)
+
+
+def test_special_cases():
+    # no surface water fraction: rain goes to the soil side, and only the
+    # infiltration excess ponds (it cannot drain through a zero fraction)
+    st = _state(fh=False, deep=False)
+    st.hy["frac_h2osfc_nosnow"][:] = 0.0
+    _run(st, O.hydro_params())
+    h = _hy(st)
+    assert np.all(h["qflx_top_soil_to_h2osfc"] == 0.0)
+    assert np.all(h["qflx_h2osfc_surf"] == 0.0)
+    assert np.all(h["qflx_h2osfc_drain"] == 0.0)
+    assert (h["qflx_infl_excess"] > 0).any()
+    assert np.allclose(st.col["h2osfc"], h["qflx_infl_excess"] * DT, rtol=1e-15, atol=0)
+    assert np.array_equal(h["qflx_infl"], h["qflx_in_soil"] - h["qflx_infl_excess"])
+    # a pool fraction at or below pc does not overflow, however deep
+    st = _state()
+    st.hy["frac_h2osfc_nosnow"][:] = np.minimum(st.hy["frac_h2osfc_nosnow"], 0.4)
+    _run(st, O.hydro_params(pc=0.4))
+    assert np.all(st.hy["qflx_h2osfc_surf"] == 0.0)
+    # nothing falling, nothing evaporating, no standing water: every flux is zero
+    st = _state(deep=False)
+    st.col["h2osfc"][:] = 0.0
+    for k in ("qflx_rain_plus_snomelt", "qflx_liqevap_from_top_layer", "qflx_ev_soil", "qflx_ev_h2osfc"):
+        st.hy[k][:] = 0.0
+    _run(st, O.hydro_params())
+    for k in ("qflx_infl", "qflx_surf", "qflx_in_soil", "qflx_h2osfc_drain", "qflx_infl_excess"):
+        assert np.all(_hy(st)[k] == 0.0), k
+    assert np.all(st.col["h2osfc"] == 0.0)
+
+
+def test_filter_touches_only_listed_columns():
+    st = _state(ncol=256)
+    ref = st.copy()
+    filt = np.arange(1, 257, 3, dtype=np.int32)
+    assert O.surface_water(st, O.hydro_params(), filter_c=filt) == 0
+    assert O.surface_water(ref, O.hydro_params()) == 0
+    k = filt - 1
+    rest = np.setdiff1d(np.arange(256), k)
+    assert np.array_equal(st.col["qflx_infl"][k], ref.col["qflx_infl"][k])
+    assert np.array_equal(st.col["h2osfc"][k], ref.col["h2osfc"][k])
+    assert np.all(st.hy["fsat"][rest] == 0.0)
+
+
+def test_patch2col_of_evaporation_fluxes():
+    st = S.canopy_workload("canopy_uneven", ncol=128)
+    rng = np.random.default_rng(7)
+    for a in (st.sf["qflx_liqevap_from_top_layer"], st.canopy_array("qflx_ev_soil"),
+              st.canopy_array("qflx_ev_h2osfc")):
+        a[:] = rng.uniform(-1e-5, 1e-4, a.shape)
+    active = st.canopy_array("active").copy()
+    active[::5] = 0
+    st.canopy_array("active")[:] = active
+    O.patch2col3(st)
+    col = st.pch["column"] - 1 if "column" in st.pch else None
+    if col is None:
+        pi, pf = st.col["patchi"], st.col["patchf"]
+        col = np.repeat(np.arange(st.dims.ncol), pf - pi + 1)
+    w = st.canopy_array("wtcol") * (active != 0)
+    for src, dst in ((st.sf["qflx_liqevap_from_top_layer"], "qflx_liqevap_from_top_layer"),
+                     (st.canopy_array("qflx_ev_soil"), "qflx_ev_soil"),
+                     (st.canopy_array("qflx_ev_h2osfc"), "qflx_ev_h2osfc")):
+        ref = np.zeros(st.dims.ncol)
+        np.add.at(ref, col, src * w)
+        assert np.allclose(st.hy[dst], ref, rtol=1e-13, atol=1e-20), dst
