@@ -353,6 +353,11 @@ SURFWATER_FIELDS = [(n, "f64", "c") for n in (
     "qflx_h2osfc_surf", "qflx_h2osfc_drain", "qflx_infl", "qflx_surf")]
 
 
+WATERTABLE_FIELDS = [("zwt", "f64", "c"), ("zwt_perched", "f64", "c"), ("frost_table", "f64", "c"),
+                     ("h2osoi_vol", "f64", "soi"), ("qflx_liqdew_to_top_layer", "f64", "c"),
+                     ("qflx_soliddew_to_top_layer", "f64", "c"), ("qflx_solidevap_from_top_layer", "f64", "c")]
+
+
 class HydroParams(C.Structure):
     _fields_ = [("fff", C.c_double), ("pc", C.c_double), ("mu", C.c_double), ("h2osfcflag", C.c_int32),
                 ("pad", C.c_int32)]
@@ -393,6 +398,10 @@ class CanopyState(C.Structure):
 
 class SurfaceWaterState(C.Structure):
     _fields_ = [(n, _CT[t]) for n, t, _ in SURFWATER_FIELDS]
+
+
+class WaterTableState(C.Structure):
+    _fields_ = [(n, _CT[t]) for n, t, _ in WATERTABLE_FIELDS]
 
 
 class SoilFluxState(C.Structure):

@@ -51,6 +51,8 @@ void launch_surface_water(hipStream_t, const ctsm_layout &, double, const ctsm_h
 void launch_patch2col3(hipStream_t, const ctsm_layout &, const ctsm_col_state &, const int32_t *, const double *,
                        const double *, const double *, const double *, double *, double *, double *, int,
                        const int32_t *, ErrWord *);
+void launch_water_table(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_col_state &,
+                        const ctsm_water_table_state &, int, const int32_t *, double, ErrWord *);
 }  // namespace ctsm
 
 using namespace ctsm;
@@ -204,6 +206,19 @@ const FieldMeta kSw[] = {
 constexpr int kNSw = sizeof(kSw) / sizeof(kSw[0]);
 static_assert(kNSw * sizeof(void *) == sizeof(ctsm_surface_water_state),
               "every ctsm_surface_water_state member has a mirror entry");
+// water tables and condensation
+const FieldMeta kWt[] = {
+    {offsetof(ctsm_water_table_state, zwt), K_C, 8, true},
+    {offsetof(ctsm_water_table_state, zwt_perched), K_C, 8, true},
+    {offsetof(ctsm_water_table_state, frost_table), K_C, 8, true},
+    {offsetof(ctsm_water_table_state, h2osoi_vol), K_SOI, 8, true},
+    {offsetof(ctsm_water_table_state, qflx_liqdew_to_top_layer), K_C, 8, false},
+    {offsetof(ctsm_water_table_state, qflx_soliddew_to_top_layer), K_C, 8, false},
+    {offsetof(ctsm_water_table_state, qflx_solidevap_from_top_layer), K_C, 8, false},
+};
+constexpr int kNWt = sizeof(kWt) / sizeof(kWt[0]);
+static_assert(kNWt * sizeof(void *) == sizeof(ctsm_water_table_state),
+              "every ctsm_water_table_state member has a mirror entry");
 constexpr int kNSfP = sizeof(kSfPatch) / sizeof(kSfPatch[0]);
 constexpr int kNSfC = sizeof(kSfCol) / sizeof(kSfCol[0]);
 static_assert((kNSfP + kNSfC) * sizeof(void *) == sizeof(ctsm_soilflux_state),
@@ -304,6 +319,7 @@ struct ctsm_ctx {
     DevBuf bal;                        // balance-check reduction word
     std::vector<DevBuf> sfp, sfc;      // SoilFluxes drop-in mirrors
     std::vector<DevBuf> swm;           // surface-water drop-in mirrors
+    std::vector<DevBuf> wtm;           // water-table drop-in mirrors
     bool have_canopy = false;
     int32_t last_code = 0, last_index = 0;
     std::mutex mu;
@@ -592,6 +608,7 @@ int ctsm_destroy(ctsm_ctx *ctx)
     for (auto &b : ctx->sfp) b.release();
     for (auto &b : ctx->sfc) b.release();
     for (auto &b : ctx->swm) b.release();
+    for (auto &b : ctx->wtm) b.release();
     ctx->partf.release();
     for (int k = 0; k < ctsm_ctx::kMaxStreams - 1; ++k) {
         if (ctx->side[k]) { (void)hipStreamSynchronize(ctx->side[k]); (void)hipStreamDestroy(ctx->side[k]); }
@@ -1185,15 +1202,73 @@ int ctsm_surface_water(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_sta
     return sync_and_check(ctx);
 }
 
+// the members every water-table call needs (h2osoi_vol may be NULL)
+static bool wt_complete(const ctsm_water_table_state *wt)
+{
+    return wt->zwt && wt->zwt_perched && wt->frost_table && wt->qflx_liqdew_to_top_layer &&
+           wt->qflx_soliddew_to_top_layer && wt->qflx_solidevap_from_top_layer;
+}
+
+static bool wt_col_complete(const ctsm_col_state *col)
+{
+    return col->t_soisno && col->dz && col->z && col->zi && col->watsat && col->nbedrock && col->snl &&
+           col->frac_h2osfc && col->h2osoi_liq && col->h2osoi_ice;
+}
+
+int ctsm_water_table_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                         const ctsm_water_table_state *wt, int32_t n, const int32_t *filter_dev, double dtime)
+{
+    if (!ctx || !col || !wt || n < 0 || check_layout(lay) || (!filter_dev && n > lay->ldc) || !wt_complete(wt) ||
+        !wt_col_complete(col))
+        return CTSM_E_ARG;
+    launch_water_table(ctx->stream, *lay, ctx->cfg, *col, *wt, n, filter_dev, dtime, ctx->err);
+    return finish_launch(ctx);
+}
+
+int ctsm_water_table(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                     const ctsm_water_table_state *wt, int32_t n, const int32_t *filter, double dtime)
+{
+    if (!ctx || !col || !wt || n < 0 || check_layout(lay) || (!filter && n > lay->ldc) || !wt_complete(wt) ||
+        !wt_col_complete(col))
+        return CTSM_E_ARG;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipError_t he = hipSetDevice(ctx->device);
+    if (he != hipSuccess) return hip_fail(ctx, he);
+    for (int i = 0; filter && i < n; ++i)
+        if (filter[i] < lay->begc_alloc || filter[i] >= lay->begc_alloc + lay->ldc) return CTSM_E_ARG;
+    ctsm_col_state hc{};
+    hc.t_soisno = col->t_soisno; hc.dz = col->dz; hc.z = col->z; hc.zi = col->zi; hc.watsat = col->watsat;
+    hc.nbedrock = col->nbedrock; hc.snl = col->snl; hc.frac_h2osfc = col->frac_h2osfc;
+    hc.h2osoi_liq = col->h2osoi_liq; hc.h2osoi_ice = col->h2osoi_ice;
+    const Range cr{lay->begc_alloc, lay->begc_alloc + lay->ldc - 1};
+    ctsm_col_state dcol{};
+    ctsm_water_table_state dwt{};
+    const int ncf = (int)(sizeof(kColFields) / sizeof(kColFields[0]));
+    int rc = mirror(ctx, kColFields, ncf, &hc, &dcol, ctx->colmir, lay->begc_alloc, lay->ldc, cr, true);
+    if (!rc) rc = mirror(ctx, kWt, kNWt, wt, &dwt, ctx->wtm, lay->begc_alloc, lay->ldc, cr, true);
+    if (rc) return rc;
+    const int32_t *dfc = nullptr;
+    if ((rc = upload_filter(ctx, n, filter, &dfc))) return rc;
+    launch_water_table(ctx->stream, *lay, ctx->cfg, dcol, dwt, n, dfc, dtime, ctx->err);
+    if ((rc = finish_launch(ctx))) return rc;
+    if ((rc = mirror(ctx, kColFields, ncf, &hc, &dcol, ctx->colmir, lay->begc_alloc, lay->ldc, cr, false))) return rc;
+    if ((rc = mirror(ctx, kWt, kNWt, wt, &dwt, ctx->wtm, lay->begc_alloc, lay->ldc, cr, false))) return rc;
+    return sync_and_check(ctx);
+}
+
 int ctsm_land_step_full_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
                             const ctsm_patch_state *pch, const ctsm_canopy_state *can, const ctsm_soilflux_state *sf,
-                            const ctsm_surface_water_state *sw, const ctsm_hydro_params *hprm, int32_t nc,
+                            const ctsm_surface_water_state *sw, const ctsm_hydro_params *hprm,
+                            const ctsm_water_table_state *wt, int32_t nc,
                             const int32_t *filterc_dev, int32_t np, const int32_t *filterp_dev, double dtime)
 {
     if (!ctx || !col || !pch || !can || !sf || nc < 0 || np < 0 || check_canopy_layout(ctx, lay) ||
         (!filterc_dev && nc > lay->ldc) || (!filterp_dev && np > lay->ldp) || !sf_complete(sf))
         return CTSM_E_ARG;
     if (sw && (!hprm || !sw_complete(sw) || sw->qflx_infl != col->qflx_infl || sw->h2osfc != col->h2osfc))
+        return CTSM_E_ARG;
+    if (wt && (!wt_complete(wt) || (sw && (sw->zwt != wt->zwt || sw->zwt_perched != wt->zwt_perched ||
+                                           sw->frost_table != wt->frost_table))))
         return CTSM_E_ARG;
     // the arrays one routine writes and the next reads must coincide
     if (pch->dlrad != can->dlrad || pch->cgrnd != can->cgrnd || pch->eflx_sh_grnd != can->eflx_sh_grnd ||
@@ -1225,6 +1300,13 @@ int ctsm_land_step_full_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_co
     }
     launch_soilwater(ctx->stream, V, ctx->prm, *col, nc, filterc_dev, dtime, ctx->err, ctx->tun.richards_lanes,
                      ctx->tun.richards_block);
+    if (wt) {
+        launch_patch2col3(ctx->stream, *lay, *col, can->active, can->wtcol, sf->qflx_liqdew_to_top_layer,
+                          sf->qflx_soliddew_to_top_layer, sf->qflx_solidevap_from_top_layer,
+                          wt->qflx_liqdew_to_top_layer, wt->qflx_soliddew_to_top_layer,
+                          wt->qflx_solidevap_from_top_layer, nc, filterc_dev, ctx->err);
+        launch_water_table(ctx->stream, *lay, ctx->cfg, *col, *wt, nc, filterc_dev, dtime, ctx->err);
+    }
     return finish_launch(ctx);
 }
 

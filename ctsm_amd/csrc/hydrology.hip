@@ -11,9 +11,15 @@
 // output once (HBM-bound, ~30 FP64 per column); the routines' order inside a
 // column is the reference's, so the values are the reference's.
 //
-// k_patch2col: the column values of qflx_liqevap_from_top_layer,
-// qflx_ev_soil and qflx_ev_h2osfc that SetQflxInputs reads, as clm_drv_patch2col
-// forms them (p2c_1d_filter, subgridAveMod.F90:312-318), for the land step.
+// k_water_table: the calls after SoilWater (HydrologyNoDrainageMod.F90:355-373,
+// no aquifer layer): PerchedWaterTable (SoilHydrologyMod.F90:1525-1641),
+// ThetaBasedWaterTable (:1933-2025) and RenewCondensation (:2569-2678), one
+// lane per column, each a search over the column's layers.
+//
+// k_patch2col: column values of three patch fluxes as clm_drv_patch2col forms
+// them (p2c_1d_filter, subgridAveMod.F90:312-318), for the land step: the
+// evaporation fluxes SetQflxInputs reads and the dew / sublimation fluxes
+// RenewCondensation reads.
 #include "common.h"
 
 namespace ctsm {
@@ -22,9 +28,11 @@ namespace hydro {
 constexpr double RPI = 3.14159265358979323846;
 
 struct HView {
-    int begc, ldc, begp, ldp;
+    int begc, ldc, begp, ldp, nlevsno, nlevsoi;
     __device__ __forceinline__ long c1(int c) const { return (long)(c - begc); }
     __device__ __forceinline__ long soi(int c, int j) const { return (long)(c - begc) + (long)(j - 1) * ldc; }
+    __device__ __forceinline__ long sno(int c, int j) const { return (long)(c - begc) + (long)(j + nlevsno - 1) * ldc; }
+    __device__ __forceinline__ long zi(int c, int j) const { return (long)(c - begc) + (long)(j + nlevsno) * ldc; }
     __device__ __forceinline__ long p1(int p) const { return (long)(p - begp); }
 };
 
@@ -133,6 +141,112 @@ __global__ void __launch_bounds__(256) k_surface_water(HView V, double e_ice, ct
     W.qflx_surf[cc] = sat_excess + infl_excess_surf + surf;
 }
 
+// sat_lev of both water-table searches: `0.9` without a kind, a single-precision
+// value (SoilHydrologyMod.F90:1553, 1983)
+constexpr double SAT_LEV = (double)0.9f;
+// tolerance of RenewCondensation's truncation (SoilHydrologyMod.F90:71)
+constexpr double RENEW_TOLERANCE = 1.e-12;
+
+// volumetric water of soil layer k
+__device__ __forceinline__ double layer_vol(const HView &V, const ctsm_col_state &S, int c, int k)
+{
+    const long i = V.sno(c, k);
+    return S.h2osoi_liq[i] / (S.dz[i] * DENH2O) + S.h2osoi_ice[i] / (S.dz[i] * DENICE);
+}
+
+// water table from the interpolated depth where s crosses sat_lev between layers k and k + 1
+__device__ __forceinline__ double crossing(const HView &V, const ctsm_col_state &S, int c, int k, double s1, double s2)
+{
+    const double z0 = S.z[V.sno(c, k)], z1 = S.z[V.sno(c, k + 1)];
+    const double m = (z1 - z0) / (s2 - s1);
+    const double b = z1 - m * s2;
+    return fmax90(0.0, m * SAT_LEV + b);
+}
+
+__global__ void __launch_bounds__(256) k_water_table(HView V, ctsm_col_state S, ctsm_water_table_state W, int n,
+                                                     const int32_t *__restrict__ filter, double dtime, ErrWord *err)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = filter ? filter[i] : V.begc + i;
+    if (c < V.begc || c >= V.begc + V.ldc) { report_error(err, c, CTSM_E_ARG); return; }
+    const long cc = V.c1(c);
+    const int nlevsoi = V.nlevsoi;
+
+    // PerchedWaterTable: the frost table is the top of the first frozen layer
+    // under an unfrozen one
+    int k_frz = (S.t_soisno[V.sno(c, 1)] > TFRZ) ? nlevsoi : 1;
+    for (int k = 2; k <= nlevsoi; ++k) {
+        if (S.t_soisno[V.sno(c, k - 1)] > TFRZ && S.t_soisno[V.sno(c, k)] <= TFRZ) {
+            k_frz = k;
+            break;
+        }
+    }
+    const double frost = S.zi[V.zi(c, k_frz - 1)];
+    double zwt_perched = frost;
+    const double t_frz = S.t_soisno[V.sno(c, k_frz)];
+    if (W.zwt[cc] < frost && t_frz <= TFRZ) {
+        // water table above the frost table: nothing to do
+    } else if (k_frz > 1) {
+        int k_perch = 1;
+        for (int k = k_frz; k >= 1; --k) {
+            const double vol = layer_vol(V, S, c, k);
+            if (W.h2osoi_vol) W.h2osoi_vol[V.soi(c, k)] = vol;
+            if (vol / S.watsat[V.soi(c, k)] <= SAT_LEV) {
+                k_perch = k;
+                break;
+            }
+        }
+        if (t_frz > TFRZ) k_perch = k_frz;
+        if (k_frz > k_perch) {
+            const double s1 = layer_vol(V, S, c, k_perch) / S.watsat[V.soi(c, k_perch)];
+            const double s2 = layer_vol(V, S, c, k_perch + 1) / S.watsat[V.soi(c, k_perch + 1)];
+            zwt_perched = (s1 > s2) ? S.zi[V.zi(c, k_perch - 1)] : crossing(V, S, c, k_perch, s1, s2);
+        }
+    }
+    W.frost_table[cc] = frost;
+    W.zwt_perched[cc] = zwt_perched;
+
+    // ThetaBasedWaterTable: first layer from the bedrock up below sat_lev
+    const int nb = S.nbedrock[cc];
+    int k_zwt = nb;
+    bool all_sat = true;
+    for (int k = nb; k >= 1; --k) {
+        const double vol = layer_vol(V, S, c, k);
+        if (W.h2osoi_vol) W.h2osoi_vol[V.soi(c, k)] = vol;
+        if (vol / S.watsat[V.soi(c, k)] <= SAT_LEV) {
+            k_zwt = k;
+            all_sat = false;
+            break;
+        }
+    }
+    if (all_sat) k_zwt = 1;
+    double zwt;
+    if (k_zwt == 1) {
+        zwt = S.zi[V.zi(c, 1)];
+    } else if (k_zwt < nb) {
+        const double s1 = layer_vol(V, S, c, k_zwt) / S.watsat[V.soi(c, k_zwt)];
+        const double s2 = layer_vol(V, S, c, k_zwt + 1) / S.watsat[V.soi(c, k_zwt + 1)];
+        zwt = crossing(V, S, c, k_zwt, s1, s2);
+    } else {
+        zwt = S.zi[V.zi(c, nb)];
+    }
+    W.zwt[cc] = zwt;
+
+    // RenewCondensation: dew, frost and sublimation of a snow-free top layer
+    if (S.snl[cc] + 1 >= 1) {
+        const long t = V.sno(c, 1);
+        const double wf = 1.0 - S.frac_h2osfc[cc];
+        S.h2osoi_liq[t] = S.h2osoi_liq[t] + wf * W.qflx_liqdew_to_top_layer[cc] * dtime;
+        double ice = S.h2osoi_ice[t] + wf * W.qflx_soliddew_to_top_layer[cc] * dtime;
+        const double before = ice;
+        ice = ice - wf * W.qflx_solidevap_from_top_layer[cc] * dtime;
+        if (fabs(ice) < RENEW_TOLERANCE * fabs(before)) ice = 0.0;
+        S.h2osoi_ice[t] = ice;
+        if (ice < 0.0) report_error(err, c, CTSM_E_ENDRUN);
+    }
+}
+
 // p2c_1d_filter of three patch fluxes (clm_drv_patch2col, subgridAveMod.F90:312-318)
 __global__ void __launch_bounds__(256) k_patch2col(HView V, ctsm_col_state S, const int32_t *__restrict__ active,
                                                    const double *__restrict__ wtcol, const double *__restrict__ pa,
@@ -168,7 +282,7 @@ void launch_surface_water(hipStream_t st, const ctsm_layout &L, double e_ice, co
                           double dtime, ErrWord *err)
 {
     if (n <= 0) return;
-    const hydro::HView V{L.begc_alloc, L.ldc, L.begp_alloc, L.ldp};
+    const hydro::HView V{L.begc_alloc, L.ldc, L.begp_alloc, L.ldp, 0, 0};
     hipLaunchKernelGGL(hydro::k_surface_water, dim3((n + 255) / 256), dim3(256), 0, st, V, e_ice, P, S, W, n, filter,
                        dtime, err);
 }
@@ -178,9 +292,17 @@ void launch_patch2col3(hipStream_t st, const ctsm_layout &L, const ctsm_col_stat
                        double *cb, double *ccol, int n, const int32_t *filter, ErrWord *err)
 {
     if (n <= 0) return;
-    const hydro::HView V{L.begc_alloc, L.ldc, L.begp_alloc, L.ldp};
+    const hydro::HView V{L.begc_alloc, L.ldc, L.begp_alloc, L.ldp, 0, 0};
     hipLaunchKernelGGL(hydro::k_patch2col, dim3((n + 255) / 256), dim3(256), 0, st, V, S, active, wtcol, pa, pb, pc,
                        ca, cb, ccol, n, filter, err);
+}
+
+void launch_water_table(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg, const ctsm_col_state &S,
+                        const ctsm_water_table_state &W, int n, const int32_t *filter, double dtime, ErrWord *err)
+{
+    if (n <= 0) return;
+    const hydro::HView V{L.begc_alloc, L.ldc, L.begp_alloc, L.ldp, cfg.nlevsno, cfg.nlevsoi};
+    hipLaunchKernelGGL(hydro::k_water_table, dim3((n + 255) / 256), dim3(256), 0, st, V, S, W, n, filter, dtime, err);
 }
 
 }  // namespace ctsm

@@ -173,6 +173,14 @@ module ctsm_gpu_mod
      type(c_ptr) :: qflx_surf = c_null_ptr
   end type ctsm_surface_water_state_t
 
+  ! Water tables and condensation state (member order and meaning as ctsm_water_table_state)
+  type, bind(C), public :: ctsm_water_table_state_t
+     type(c_ptr) :: zwt = c_null_ptr, zwt_perched = c_null_ptr, frost_table = c_null_ptr
+     type(c_ptr) :: h2osoi_vol = c_null_ptr
+     type(c_ptr) :: qflx_liqdew_to_top_layer = c_null_ptr, qflx_soliddew_to_top_layer = c_null_ptr
+     type(c_ptr) :: qflx_solidevap_from_top_layer = c_null_ptr
+  end type ctsm_water_table_state_t
+
   ! fff, pc, mu (param file) and h2osfcflag, as ctsm_hydro_params
   type, bind(C), public :: ctsm_hydro_params_t
      real(c_double) :: fff, pc, mu
@@ -183,7 +191,7 @@ module ctsm_gpu_mod
   public :: ctsm_config_default, ctsm_create, ctsm_destroy, ctsm_last_error
   public :: ctsm_soilwater, ctsm_soiltemperature, ctsm_soilwater_fractions
   public :: ctsm_set_canopy_params, ctsm_canopyfluxes, ctsm_root_sink
-  public :: ctsm_soilfluxes, ctsm_soil_balance_check, ctsm_surface_water
+  public :: ctsm_soilfluxes, ctsm_soil_balance_check, ctsm_surface_water, ctsm_water_table
 
   interface
      subroutine ctsm_config_default(cfg) bind(C, name='ctsm_config_default')
@@ -312,6 +320,18 @@ module ctsm_gpu_mod
        integer(c_int32_t), intent(in) :: filter_hydrologyc(*)
        real(c_double), value :: dtime
      end function ctsm_surface_water
+
+     integer(c_int) function ctsm_water_table(ctx, lay, col, wt, num_hydrologyc, filter_hydrologyc, dtime) &
+          bind(C, name='ctsm_water_table')
+       import :: c_int, c_int32_t, c_double, c_ptr, ctsm_layout_t, ctsm_col_state_t, ctsm_water_table_state_t
+       type(c_ptr), value :: ctx
+       type(ctsm_layout_t), intent(in) :: lay
+       type(ctsm_col_state_t), intent(in) :: col
+       type(ctsm_water_table_state_t), intent(in) :: wt
+       integer(c_int32_t), value :: num_hydrologyc
+       integer(c_int32_t), intent(in) :: filter_hydrologyc(*)
+       real(c_double), value :: dtime
+     end function ctsm_water_table
   end interface
 
 end module ctsm_gpu_mod

@@ -34,6 +34,7 @@ EXPORTS = [
     "ctsm_graph_capture", "ctsm_graph_launch", "ctsm_graph_destroy",
     "ctsm_soilfluxes", "ctsm_soil_balance_check", "ctsm_soilfluxes_dev", "ctsm_soil_balance_check_dev",
     "ctsm_land_step_full_dev", "ctsm_soil_steps_fused_dev", "ctsm_surface_water", "ctsm_surface_water_dev",
+    "ctsm_water_table", "ctsm_water_table_dev",
 ]
 
 
@@ -79,10 +80,14 @@ def load():
     L.ctsm_soil_balance_check_dev.argtypes = bal
     L.ctsm_land_step_full_dev.argtypes = col + [P(abi.PatchState), P(abi.CanopyState), P(abi.SoilFluxState),
                                                 P(abi.SurfaceWaterState), P(abi.HydroParams),
+                                                P(abi.WaterTableState),
                                                 C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_double]
     swa = col + [P(abi.SurfaceWaterState), P(abi.HydroParams), C.c_int32, C.c_void_p, C.c_double]
     L.ctsm_surface_water.argtypes = swa
     L.ctsm_surface_water_dev.argtypes = swa
+    wta = col + [P(abi.WaterTableState), C.c_int32, C.c_void_p, C.c_double]
+    L.ctsm_water_table.argtypes = wta
+    L.ctsm_water_table_dev.argtypes = wta
     L.ctsm_graph_launch.argtypes = [C.c_void_p, C.c_void_p]
     L.ctsm_graph_destroy.argtypes = [C.c_void_p]
     L.ctsm_sync.argtypes = [C.c_void_p]
@@ -273,15 +278,17 @@ class Context:
         return m.value, idx.value
 
     def land_step_full_dev(self, ds, filterc_dev=None, nc=None, filterp_dev=None, np_=None, dtime=1800.0,
-                           structs=None, hydro=None):
+                           structs=None, hydro=None, water_table=False):
         """Land step with the before-step save, SoilFluxes and, given hydro
-        (HydroParams), the surface water input chain (enqueues only)."""
+        (HydroParams), the surface water input chain; with water_table, the
+        water tables and RenewCondensation after SoilWater (enqueues only)."""
         if structs is None:
             cs, ps = ds.structs()
             can, sf = ds.canopy_struct(), ds.soilflux_struct()
         else:
             cs, ps, can, sf = structs
         sw = ds.surfwater_struct() if hydro is not None else None
+        wt = ds.watertable_struct() if water_table else None
         nc = ds.dims.ncol if nc is None else nc
         np_ = ds.dims.npatch if np_ is None else np_
         fc = C.c_void_p(filterc_dev) if filterc_dev is not None else None
@@ -290,6 +297,7 @@ class Context:
                                                    C.byref(can), C.byref(sf),
                                                    C.byref(sw) if sw is not None else None,
                                                    C.byref(hydro) if hydro is not None else None,
+                                                   C.byref(wt) if wt is not None else None,
                                                    nc, fc, np_, fp, dtime), "land step (full)")
 
     def surface_water(self, st, hydro, filter_c=None, dtime=1800.0):
@@ -307,6 +315,22 @@ class Context:
         fp = C.c_void_p(filter_dev) if filter_dev is not None else None
         self._check(self.L.ctsm_surface_water_dev(self.h, C.byref(ds.layout()), C.byref(cs), C.byref(sw),
                                                   C.byref(hydro), n, fp, dtime), "surface water")
+
+    def water_table(self, st, filter_c=None, dtime=1800.0):
+        """Drop-in water tables and RenewCondensation on a host state."""
+        cs, _ = st.structs()
+        wt = st.watertable_struct()
+        f, fp, n = _filter(filter_c, st.dims.ncol)
+        self._check(self.L.ctsm_water_table(self.h, C.byref(st.layout()), C.byref(cs), C.byref(wt), n, fp, dtime),
+                    "water table")
+
+    def water_table_dev(self, ds, filter_dev=None, n=None, dtime=1800.0):
+        cs, _ = ds.structs()
+        wt = ds.watertable_struct()
+        n = ds.dims.ncol if n is None else n
+        fp = C.c_void_p(filter_dev) if filter_dev is not None else None
+        self._check(self.L.ctsm_water_table_dev(self.h, C.byref(ds.layout()), C.byref(cs), C.byref(wt), n, fp,
+                                                dtime), "water table")
 
     def graph_capture(self, ds, nsteps=1, canopy=True, filterc_dev=None, nc=None, filterp_dev=None, np_=None,
                       dtime=1800.0, structs=None):

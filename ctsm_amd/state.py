@@ -40,6 +40,10 @@ SOILFLUX_OWN = [(n, t, k) for n, t, k in abi.SOILFLUX_FIELDS if n not in SOILFLU
 # surface-water fields that are column-state arrays; the rest are its own
 SURFWATER_ALIAS = {"h2osfc": ("c", "h2osfc"), "qflx_infl": ("c", "qflx_infl")}
 SURFWATER_OWN = [(n, t, k) for n, t, k in abi.SURFWATER_FIELDS if n not in SURFWATER_ALIAS]
+# water-table fields: zwt, zwt_perched and frost_table are the surface water's
+# (kept in the same group); h2osoi_vol and the column dew fluxes are its own
+WATERTABLE_SHARED = ("zwt", "zwt_perched", "frost_table")
+WATERTABLE_OWN = [(n, t, k) for n, t, k in abi.WATERTABLE_FIELDS if n not in WATERTABLE_SHARED]
 
 
 class Dims:
@@ -86,7 +90,7 @@ class HostState:
                     a[...] = np.nan
                 self.sf[name] = a
         self.hy = {}
-        for name, t, kind in SURFWATER_OWN:
+        for name, t, kind in SURFWATER_OWN + WATERTABLE_OWN:
             a = np.zeros(dims.shape(kind), dtype=_NP[t])
             if fill_nan and t == "f64":
                 a[...] = np.nan
@@ -146,6 +150,14 @@ class HostState:
             assert a.flags.c_contiguous
             setattr(sw, name, a.ctypes.data_as(abi._CT[t]))
         return sw
+
+    def watertable_struct(self):
+        wt = abi.WaterTableState()
+        for name, t, _ in abi.WATERTABLE_FIELDS:
+            a = self.hy[name]
+            assert a.flags.c_contiguous
+            setattr(wt, name, a.ctypes.data_as(abi._CT[t]))
+        return wt
 
     def structs(self):
         """ctypes ColState / PatchState pointing at these numpy buffers."""
@@ -253,6 +265,12 @@ class DeviceState:
             ptr = self.col[SURFWATER_ALIAS[name][1]] if name in SURFWATER_ALIAS else self.hy[name]
             setattr(sw, name, C.cast(C.c_void_p(ptr), abi._CT[t]))
         return sw
+
+    def watertable_struct(self):
+        wt = abi.WaterTableState()
+        for name, t, _ in abi.WATERTABLE_FIELDS:
+            setattr(wt, name, C.cast(C.c_void_p(self.hy[name]), abi._CT[t]))
+        return wt
 
     def structs(self):
         cs = abi.ColState()

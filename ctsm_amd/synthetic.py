@@ -514,7 +514,7 @@ def with_canopy(soil, seed, pmax=16, pmin=1, col0=0):
     return st
 
 
-S_RAIN, S_ZWT, S_ZWTP, S_FROST, S_WTF, S_SLOPE, S_THRESH, S_EVAP = 70, 71, 72, 73, 74, 75, 76, 77
+S_RAIN, S_ZWT, S_ZWTP, S_FROST, S_WTF, S_SLOPE, S_THRESH, S_EVAP, S_DEW = 70, 71, 72, 73, 74, 75, 76, 77, 78
 
 
 def with_hydrology(st, seed, col0=0):
@@ -537,6 +537,14 @@ def with_hydrology(st, seed, col0=0):
     h["frac_h2osfc_nosnow"][:] = c["frac_h2osfc"]
     for j, k in enumerate(("qflx_liqevap_from_top_layer", "qflx_ev_soil", "qflx_ev_h2osfc")):
         h[k][:] = -1e-6 + 5.1e-5 * uniform(seed, S_EVAP, cols, j)
+    # dew, frost and sublimation of the top layer (RenewCondensation); the
+    # sublimation never exceeds the ice there over an 1800 s step, as
+    # SoilFluxes' limit guarantees in the model
+    h["qflx_liqdew_to_top_layer"][:] = 1e-5 * uniform(seed, S_DEW, cols, 0)
+    h["qflx_soliddew_to_top_layer"][:] = 1e-5 * uniform(seed, S_DEW, cols, 1)
+    ice1 = c["h2osoi_ice"][st.dims.nlevsno]
+    h["qflx_solidevap_from_top_layer"][:] = uniform(seed, S_DEW, cols, 2) * (
+        h["qflx_soliddew_to_top_layer"] + ice1 / 1800.0)
     return st
 
 

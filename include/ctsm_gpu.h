@@ -416,6 +416,23 @@ typedef struct ctsm_surface_water_state {
     double        *qflx_h2osfc_surf, *qflx_h2osfc_drain, *qflx_infl, *qflx_surf;
 } ctsm_surface_water_state;
 
+/* ---- water tables and condensation after SoilWater (HydrologyNoDrainageMod.F90:
+ * 355-373, the CLM5/6 path without the aquifer layer) -------------------------
+ * Column arrays of PerchedWaterTable, ThetaBasedWaterTable and RenewCondensation
+ * for soil / crop columns; t_soisno, dz, z, zi, watsat, nbedrock, snl,
+ * frac_h2osfc, h2osoi_liq and h2osoi_ice come from ctsm_col_state.  zwt,
+ * zwt_perched and frost_table are the surface water chain's inputs of the next
+ * step: pass the same arrays as in ctsm_surface_water_state. */
+typedef struct ctsm_water_table_state {
+    double        *zwt;                          /* (c)   inout water table depth (m)           */
+    double        *zwt_perched, *frost_table;    /* (c)   out   (m)                             */
+    double        *h2osoi_vol;                   /* [SOI] out   for the layers the two searches
+                                                    visit, as the reference; may be NULL        */
+    /* column values of the patch fluxes (p2c, mm H2O/s); ctsm_land_step_full_dev
+     * forms them from the patches, as clm_drv_patch2col does */
+    double        *qflx_liqdew_to_top_layer, *qflx_soliddew_to_top_layer, *qflx_solidevap_from_top_layer;
+} ctsm_water_table_state;
+
 /* Param-file / namelist scalars of that chain: fff (SaturatedExcessRunoffMod
  * readParams), pc and mu (SurfaceWaterMod readParams), h2osfcflag
  * (SoilHydrologyType, 1 = surface water active). */
@@ -553,6 +570,17 @@ int ctsm_surface_water(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_sta
                        const ctsm_surface_water_state *sw, const ctsm_hydro_params *prm,
                        int32_t num_hydrologyc, const int32_t *filter_hydrologyc, double dtime);
 
+/* Replaces the calls after SoilWater in HydrologyNoDrainage
+ * (HydrologyNoDrainageMod.F90:355-373 without the aquifer layer):
+ * PerchedWaterTable (frost table, perched water table), ThetaBasedWaterTable
+ * (water table) and RenewCondensation (dew, frost and sublimation of the top
+ * soil layer of snow-free columns) for soil / crop columns.  Returns
+ * CTSM_E_ENDRUN with the column, as the reference's endrun, when sublimation
+ * leaves the top layer's ice significantly negative. */
+int ctsm_water_table(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                     const ctsm_water_table_state *wt, int32_t num_hydrologyc,
+                     const int32_t *filter_hydrologyc, double dtime);
+
 /* ---- device-resident entry points -----------------------------------------
  * Same arguments, but every pointer (fields and filter) is device memory and
  * the call only enqueues work on the context's stream.  filter == NULL means
@@ -594,6 +622,9 @@ int ctsm_root_sink_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_sta
 int ctsm_surface_water_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
                            const ctsm_surface_water_state *sw, const ctsm_hydro_params *prm,
                            int32_t num_filterc, const int32_t *filterc_dev, double dtime);
+int ctsm_water_table_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                         const ctsm_water_table_state *wt, int32_t num_filterc, const int32_t *filterc_dev,
+                         double dtime);
 int ctsm_soilfluxes_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
                         const ctsm_patch_state *pch, const ctsm_soilflux_state *sf,
                         int32_t num_filterc, const int32_t *filterc_dev,
@@ -622,14 +653,18 @@ int ctsm_land_step_dev(ctsm_ctx *ctx, const ctsm_layout *lay,
  * (clm_driver.F90:921), the transpiration sink and, when sw is not NULL, the
  * column evaporation fluxes (clm_drv_patch2col) and the surface water input
  * chain that sets qflx_infl (HydrologyNoDrainageMod.F90:306-336), then
- * SoilWater.  Arrays shared by two structs must be the same device arrays, as
- * for ctsm_land_step_dev, the CanopyFluxes outputs SoilFluxes reads
- * (sf->cgrnds, ...) must be the canopy state's, and sw->qflx_infl /
- * sw->h2osfc must be col->qflx_infl / col->h2osfc. */
+ * SoilWater and, when wt is not NULL, the column dew / sublimation fluxes
+ * (clm_drv_patch2col), the water tables and RenewCondensation
+ * (HydrologyNoDrainageMod.F90:355-373).  Arrays shared by two structs must be
+ * the same device arrays, as for ctsm_land_step_dev, the CanopyFluxes outputs
+ * SoilFluxes reads (sf->cgrnds, ...) must be the canopy state's,
+ * sw->qflx_infl / sw->h2osfc must be col->qflx_infl / col->h2osfc, and
+ * wt->zwt / zwt_perched / frost_table must be sw's. */
 int ctsm_land_step_full_dev(ctsm_ctx *ctx, const ctsm_layout *lay,
                             const ctsm_col_state *col, const ctsm_patch_state *pch,
                             const ctsm_canopy_state *can, const ctsm_soilflux_state *sf,
                             const ctsm_surface_water_state *sw, const ctsm_hydro_params *hprm,
+                            const ctsm_water_table_state *wt,
                             int32_t num_filterc, const int32_t *filterc_dev,
                             int32_t num_filterp, const int32_t *filterp_dev, double dtime);
 

@@ -23,11 +23,22 @@
  *   Infiltration             SoilHydrologyMod.F90:426-457
  *   TotalSurfaceRunoff       SoilHydrologyMod.F90:460-550 (hydrologic columns)
  *
+ * and the calls after SoilWater (HydrologyNoDrainageMod.F90:355-373, no
+ * aquifer layer) in oracle_water_table:
+ *
+ *   PerchedWaterTable        SoilHydrologyMod.F90:1525-1641
+ *   ThetaBasedWaterTable     SoilHydrologyMod.F90:1933-2025
+ *   RenewCondensation        SoilHydrologyMod.F90:2569-2678 (soil / crop columns),
+ *                            truncate_small_values with tolerance 1e-12 (:71)
+ *
  * Pins (tests/test_oracle_hydrology.py): the water balance of the surface
  * (qflx_top_soil - evaporation terms = qflx_infl + qflx_surf + dh2osfc/dt over
  * the column), h2osfc never negative, qflx_infl never above the soil's
  * infiltration capacity plus h2osfc drainage, closed forms of fsat and qinmax,
- * the h2osfcflag = 0 path (no surface water pool), and cm_sin against mpmath.
+ * the h2osfcflag = 0 path (no surface water pool), and cm_sin against mpmath;
+ * for the water tables: saturated / dry / layered columns whose table depth is
+ * fixed by construction, the interpolation's closed form, the frost-table
+ * layer against a numpy search, and the water mass RenewCondensation adds.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -37,10 +48,12 @@
 #define H_PI 3.14159265358979323846
 
 typedef struct {
-    int begc, ldc;
+    int begc, ldc, nlevsno;
 } h_lay_t;
 #define HC(a, c)      ((a)[(c) - Lo.begc])
 #define HSOI(a, c, j) ((a)[((c) - Lo.begc) + (long)((j) - 1) * Lo.ldc])
+#define HSNO(a, c, j) ((a)[((c) - Lo.begc) + (long)((j) + Lo.nlevsno - 1) * Lo.ldc])
+#define HZI(a, c, j)  ((a)[((c) - Lo.begc) + (long)((j) + Lo.nlevsno) * Lo.ldc])
 
 static double h_max(double a, double b) { return (b > a) ? b : a; }
 static double h_min(double a, double b) { return (b < a) ? b : a; }
@@ -49,7 +62,7 @@ int oracle_surface_water(const ctsm_config *cfg, const ctsm_layout *L, const cts
                          ctsm_surface_water_state *W, const ctsm_hydro_params *P, int num_c, const int *filter,
                          double dtime)
 {
-    h_lay_t Lo = { L->begc_alloc, L->ldc };
+    h_lay_t Lo = { L->begc_alloc, L->ldc, cfg->nlevsno };
     double *h2osfc_partial = (double *)calloc((size_t)(L->ldc > 0 ? L->ldc : 1), sizeof(double));
     int fc, j;
     if (!h2osfc_partial) return 1;
@@ -204,4 +217,125 @@ void oracle_patch2col3(const ctsm_layout *L, const ctsm_col_state *S, const int3
         cb[k] = b;
         cc[k] = d;
     }
+}
+
+/* ---- water tables and condensation after SoilWater ---------------------- */
+
+#define H_DENH2O 1000.0
+#define H_DENICE 917.0
+#define H_TFRZ 273.15
+
+int oracle_water_table(const ctsm_config *cfg, const ctsm_layout *L, const ctsm_col_state *S,
+                       ctsm_water_table_state *W, int num_c, const int *filter, double dtime)
+{
+    h_lay_t Lo = { L->begc_alloc, L->ldc, cfg->nlevsno };
+    const int nlevsoi = cfg->nlevsoi;
+    int fc, k, rc = 0;
+    double *ice_before = (double *)calloc((size_t)(L->ldc > 0 ? L->ldc : 1), sizeof(double));
+    int *modified = (int *)calloc((size_t)(num_c > 0 ? num_c : 1), sizeof(int));
+    int num_modified = 0;
+    if (!ice_before || !modified) { free(ice_before); free(modified); return 1; }
+#define COL(fc) (filter ? filter[fc] : L->begc_alloc + (fc))
+#define VOL(c, k) (HSNO(S->h2osoi_liq, c, k) / (HSNO(S->dz, c, k) * H_DENH2O) \
+                   + HSNO(S->h2osoi_ice, c, k) / (HSNO(S->dz, c, k) * H_DENICE))
+
+    /* PerchedWaterTable */
+    for (fc = 0; fc < num_c; ++fc) {
+        const int c = COL(fc);
+        const double sat_lev = (double)0.9f;
+        int k_frz, k_perch;
+        if (HSNO(S->t_soisno, c, 1) > H_TFRZ) k_frz = nlevsoi;
+        else k_frz = 1;
+        for (k = 2; k <= nlevsoi; ++k) {
+            if (HSNO(S->t_soisno, c, k - 1) > H_TFRZ && HSNO(S->t_soisno, c, k) <= H_TFRZ) {
+                k_frz = k;
+                break;
+            }
+        }
+        HC(W->frost_table, c) = HZI(S->zi, c, k_frz - 1);
+        HC(W->zwt_perched, c) = HC(W->frost_table, c);
+        if (HC(W->zwt, c) < HC(W->frost_table, c) && HSNO(S->t_soisno, c, k_frz) <= H_TFRZ) {
+            /* water table above the frost table */
+        } else if (k_frz > 1) {
+            k_perch = 1;
+            for (k = k_frz; k >= 1; --k) {
+                const double vol = VOL(c, k);
+                if (W->h2osoi_vol) HSOI(W->h2osoi_vol, c, k) = vol;
+                if (vol / HSOI(S->watsat, c, k) <= sat_lev) {
+                    k_perch = k;
+                    break;
+                }
+            }
+            if (HSNO(S->t_soisno, c, k_frz) > H_TFRZ) k_perch = k_frz;
+            if (k_frz > k_perch) {
+                const double s1 = VOL(c, k_perch) / HSOI(S->watsat, c, k_perch);
+                const double s2 = VOL(c, k_perch + 1) / HSOI(S->watsat, c, k_perch + 1);
+                if (s1 > s2) {
+                    HC(W->zwt_perched, c) = HZI(S->zi, c, k_perch - 1);
+                } else {
+                    const double m = (HSNO(S->z, c, k_perch + 1) - HSNO(S->z, c, k_perch)) / (s2 - s1);
+                    const double b = HSNO(S->z, c, k_perch + 1) - m * s2;
+                    HC(W->zwt_perched, c) = h_max(0.0, m * sat_lev + b);
+                }
+            }
+        }
+    }
+
+    /* ThetaBasedWaterTable */
+    for (fc = 0; fc < num_c; ++fc) {
+        const int c = COL(fc);
+        const int nb = HC(S->nbedrock, c);
+        const double sat_lev = (double)0.9f;
+        int k_zwt = nb, sat_flag = 1;
+        HC(W->zwt, c) = HZI(S->zi, c, nlevsoi);
+        for (k = nb; k >= 1; --k) {
+            const double vol = VOL(c, k);
+            if (W->h2osoi_vol) HSOI(W->h2osoi_vol, c, k) = vol;
+            if (vol / HSOI(S->watsat, c, k) <= sat_lev) {
+                k_zwt = k;
+                sat_flag = 0;
+                break;
+            }
+        }
+        if (sat_flag == 1) k_zwt = 1;
+        if (k_zwt == 1) {
+            HC(W->zwt, c) = HZI(S->zi, c, 1);
+        } else if (k_zwt < nb) {
+            const double s1 = VOL(c, k_zwt) / HSOI(S->watsat, c, k_zwt);
+            const double s2 = VOL(c, k_zwt + 1) / HSOI(S->watsat, c, k_zwt + 1);
+            const double m = (HSNO(S->z, c, k_zwt + 1) - HSNO(S->z, c, k_zwt)) / (s2 - s1);
+            const double b = HSNO(S->z, c, k_zwt + 1) - m * s2;
+            HC(W->zwt, c) = h_max(0.0, m * sat_lev + b);
+        } else {
+            HC(W->zwt, c) = HZI(S->zi, c, nb);
+        }
+    }
+
+    /* RenewCondensation */
+    for (fc = 0; fc < num_c; ++fc) {
+        const int c = COL(fc);
+        if (HC(S->snl, c) + 1 >= 1) {
+            modified[num_modified++] = c;
+            HSNO(S->h2osoi_liq, c, 1) = HSNO(S->h2osoi_liq, c, 1)
+                                        + (1.0 - HC(S->frac_h2osfc, c)) * HC(W->qflx_liqdew_to_top_layer, c) * dtime;
+            HSNO(S->h2osoi_ice, c, 1) = HSNO(S->h2osoi_ice, c, 1)
+                                        + (1.0 - HC(S->frac_h2osfc, c)) * HC(W->qflx_soliddew_to_top_layer, c) * dtime;
+            ice_before[c - Lo.begc] = HSNO(S->h2osoi_ice, c, 1);
+            HSNO(S->h2osoi_ice, c, 1) = HSNO(S->h2osoi_ice, c, 1)
+                                        - (1.0 - HC(S->frac_h2osfc, c)) * HC(W->qflx_solidevap_from_top_layer, c) * dtime;
+        }
+    }
+    for (fc = 0; fc < num_modified; ++fc) {
+        const int c = modified[fc];
+        if (fabs(HSNO(S->h2osoi_ice, c, 1)) < 1.e-12 * fabs(ice_before[c - Lo.begc])) HSNO(S->h2osoi_ice, c, 1) = 0.0;
+    }
+    for (fc = 0; fc < num_modified; ++fc) {
+        const int c = modified[fc];
+        if (HSNO(S->h2osoi_ice, c, 1) < 0.0) { rc = c; break; }
+    }
+#undef VOL
+#undef COL
+    free(ice_before);
+    free(modified);
+    return rc;
 }

@@ -283,6 +283,8 @@ def _hy_lib():
                                            P(abi.HydroParams), C.c_int, abi.i32p, C.c_double]
         L.oracle_patch2col3.argtypes = [P(abi.Layout), P(abi.ColState), abi.i32p, abi.f64p] + [abi.f64p] * 6 + [
             C.c_int, abi.i32p]
+        L.oracle_water_table.argtypes = [P(abi.Config), P(abi.Layout), P(abi.ColState), P(abi.WaterTableState),
+                                         C.c_int, abi.i32p, C.c_double]
         L._hy_ready = True
     return L
 
@@ -301,24 +303,43 @@ def surface_water(st, hydro, cfg=None, dtime=1800.0, filter_c=None):
                                   C.byref(hydro), len(f), fp, C.c_double(dtime))
 
 
-def patch2col3(st, filter_c=None):
-    """Column values of the three evaporation fluxes SetQflxInputs reads."""
+def patch2col3(st, filter_c=None, dew=False):
+    """Column values of the three evaporation fluxes SetQflxInputs reads, or
+    (dew=True) of the dew / sublimation fluxes RenewCondensation reads."""
     L = _hy_lib()
     cs, _ = st.structs()
     f, fp = _filt(filter_c if filter_c is not None else np.arange(1, st.dims.ncol + 1))
     g, hy = st.can, st.hy
-    arrs = [st.canopy_array("active"), st.canopy_array("wtcol"), st.sf["qflx_liqevap_from_top_layer"],
-            st.canopy_array("qflx_ev_soil"), st.canopy_array("qflx_ev_h2osfc"),
-            hy["qflx_liqevap_from_top_layer"], hy["qflx_ev_soil"], hy["qflx_ev_h2osfc"]]
+    if dew:
+        names = ("qflx_liqdew_to_top_layer", "qflx_soliddew_to_top_layer", "qflx_solidevap_from_top_layer")
+        arrs = [st.canopy_array("active"), st.canopy_array("wtcol")] + [st.sf[k] for k in names] + [
+            hy[k] for k in names]
+    else:
+        arrs = [st.canopy_array("active"), st.canopy_array("wtcol"), st.sf["qflx_liqevap_from_top_layer"],
+                st.canopy_array("qflx_ev_soil"), st.canopy_array("qflx_ev_h2osfc"),
+                hy["qflx_liqevap_from_top_layer"], hy["qflx_ev_soil"], hy["qflx_ev_h2osfc"]]
     ptrs = [arrs[0].ctypes.data_as(abi.i32p)] + [a.ctypes.data_as(abi.f64p) for a in arrs[1:]]
     L.oracle_patch2col3(C.byref(st.layout()), C.byref(cs), *ptrs, len(f), fp)
 
 
-def land_step_full(st, cfg, prm, pft, dtime=1800.0, hydro=None):
+def water_table(st, cfg=None, dtime=1800.0, filter_c=None):
+    """PerchedWaterTable, ThetaBasedWaterTable and RenewCondensation
+    (HydrologyNoDrainageMod.F90:355-373).  Returns 0, or the first column
+    whose top-layer ice sublimation leaves it negative (the endrun)."""
+    L = _hy_lib()
+    cs, _ = st.structs()
+    wt = st.watertable_struct()
+    f, fp = _filt(filter_c if filter_c is not None else np.arange(1, st.dims.ncol + 1))
+    return L.oracle_water_table(C.byref(cfg or canopy_config()), C.byref(st.layout()), C.byref(cs), C.byref(wt),
+                                len(f), fp, C.c_double(dtime))
+
+
+def land_step_full(st, cfg, prm, pft, dtime=1800.0, hydro=None, water_table_after=False):
     """land_step with the reference's bookkeeping: t_ssbef / t_h2osfc_bef
     saved first, SoilFluxes between SoilTemperature and the hydrology
-    (clm_driver.F90:921) and, given hydro, the column evaporation fluxes and
-    the surface water input chain before SoilWater."""
+    (clm_driver.F90:921), given hydro the column evaporation fluxes and the
+    surface water input chain before SoilWater, and with water_table_after
+    the column dew fluxes, the water tables and RenewCondensation after it."""
     save_before(st, cfg)
     st.pch["cgrnd"][:] = 0.0
     st.can["cgrnds"][:] = 0.0
@@ -335,7 +356,12 @@ def land_step_full(st, cfg, prm, pft, dtime=1800.0, hydro=None):
     if hydro is not None:
         patch2col3(st)
         surface_water(st, hydro, cfg, dtime)
-    return soilwater(st, cfg, dtime)
+    rc, bad = soilwater(st, cfg, dtime)
+    if rc or not water_table_after:
+        return rc, bad
+    patch2col3(st, dew=True)
+    c = water_table(st, cfg, dtime)
+    return (0, None) if c == 0 else (1, c)
 
 
 def qsat(T, p):

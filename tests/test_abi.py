@@ -43,7 +43,8 @@ STRUCTS = (("ctsm_config", abi.Config), ("ctsm_layout", abi.Layout), ("ctsm_tuni
            ("ctsm_col_state", abi.ColState), ("ctsm_patch_state", abi.PatchState),
            ("ctsm_canopy_params", abi.CanopyParams), ("ctsm_pft_params", abi.PftParams),
            ("ctsm_canopy_state", abi.CanopyState), ("ctsm_soilflux_state", abi.SoilFluxState),
-           ("ctsm_surface_water_state", abi.SurfaceWaterState), ("ctsm_hydro_params", abi.HydroParams))
+           ("ctsm_surface_water_state", abi.SurfaceWaterState), ("ctsm_hydro_params", abi.HydroParams),
+           ("ctsm_water_table_state", abi.WaterTableState))
 
 
 def _probe(tmp_path):
@@ -91,8 +92,9 @@ program sizes
   type(ctsm_soilflux_state_t) :: h
   type(ctsm_surface_water_state_t) :: i
   type(ctsm_hydro_params_t) :: j
-  print '(10I8)', c_sizeof(a), c_sizeof(b), c_sizeof(c), c_sizeof(d), c_sizeof(e), c_sizeof(f), c_sizeof(g), &
-       c_sizeof(h), c_sizeof(i), c_sizeof(j)
+  type(ctsm_water_table_state_t) :: k
+  print '(11I8)', c_sizeof(a), c_sizeof(b), c_sizeof(c), c_sizeof(d), c_sizeof(e), c_sizeof(f), c_sizeof(g), &
+       c_sizeof(h), c_sizeof(i), c_sizeof(j), c_sizeof(k)
 end program sizes
 """)
     mod = os.path.join(ROOT, "ctsm_amd", "fortran", "ctsm_gpu_mod.F90")
@@ -101,7 +103,8 @@ end program sizes
     sizes = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
     assert sizes == [C.sizeof(abi.Config), C.sizeof(abi.Layout), C.sizeof(abi.ColState), C.sizeof(abi.PatchState),
                      C.sizeof(abi.CanopyParams), C.sizeof(abi.PftParams), C.sizeof(abi.CanopyState),
-                     C.sizeof(abi.SoilFluxState), C.sizeof(abi.SurfaceWaterState), C.sizeof(abi.HydroParams)]
+                     C.sizeof(abi.SoilFluxState), C.sizeof(abi.SurfaceWaterState), C.sizeof(abi.HydroParams),
+                     C.sizeof(abi.WaterTableState)]
 
 
 def test_config_rejected_before_device():

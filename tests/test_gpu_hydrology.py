@@ -110,3 +110,100 @@ def test_land_step_full_with_surface_water_bitwise(hctx, nsteps, name):
     assert worst < 1e-7, worst
     # the chain fed SoilWater: infiltration is no longer the synthetic input
     assert np.any(o.col["qflx_infl"] != st.col["qflx_infl"])
+
+
+# ---- water tables and RenewCondensation ----------------------------------------
+
+def _wt_state(ncol, seed=0x5EED0031):
+    """Columns with random freezing profiles, saturations and water tables, so
+    every branch of both searches is taken."""
+    cols = np.arange(ncol, dtype=np.int64)
+    st = S.soil_only(ncol, seed, snow=True, bedrock=True, frac_h2osfc=0.4 * S.uniform(seed, 94, cols))
+    S.with_hydrology(st, seed)
+    ns, nl = st.dims.nlevsno, st.dims.nlevsoi
+    rng = np.random.default_rng(seed & 0xFFFF)
+    kf = rng.integers(1, nl + 2, ncol)                   # nl + 1: unfrozen column
+    for k in range(1, st.dims.nlevgrnd + 1):
+        st.col["t_soisno"][ns + k - 1] = np.where(k < kf, 275.0 + 10 * rng.random(ncol), 273.15 - 8 * rng.random(ncol))
+        if k <= nl:
+            s = np.where(rng.random(ncol) < 0.5, rng.uniform(0.2, 0.9, ncol), rng.uniform(0.9, 1.0, ncol))
+            w = s * st.col["watsat"][k - 1] * st.col["dz"][ns + k - 1] * 1000.0
+            f = np.where(k >= kf, rng.random(ncol), 0.0)
+            st.col["h2osoi_liq"][ns + k - 1] = (1 - f) * w
+            st.col["h2osoi_ice"][ns + k - 1] = f * w * 0.917
+    st.hy["zwt"][:] = rng.uniform(0.0, 10.0, ncol)
+    ice1 = st.col["h2osoi_ice"][ns]
+    st.hy["qflx_solidevap_from_top_layer"][:] = rng.random(ncol) * (st.hy["qflx_soliddew_to_top_layer"] + ice1 / DT)
+    return st
+
+
+def test_water_table_dropin_bitwise(hctx):
+    st = _wt_state(2000)
+    print("case: water tables + RenewCondensation drop-in, 2000 columns", flush=True)
+    g, o = st.copy(), st.copy()
+    hctx.water_table(g)
+    assert O.water_table(o) == 0
+    _same(g, o)
+    # every branch ran: perched tables below the frost table, interpolated and bedrock tables
+    assert np.any(o.hy["zwt_perched"] != o.hy["frost_table"])
+    assert len(np.unique(o.hy["zwt"])) > 500
+
+
+def test_water_table_dev_bitwise(hctx):
+    st = _wt_state(4096, seed=0x5EED0032)
+    perm = (np.random.default_rng(5).permutation(st.dims.ncol) + 1).astype(np.int32)[:3000]
+    print("case: water tables device entry, 3000 of 4096 columns, permuted filter", flush=True)
+    ds = DeviceState(st, hctx)
+    f = hctx.device_array(perm)
+    try:
+        hctx.water_table_dev(ds, filter_dev=f, n=len(perm))
+        hctx.sync()
+        g = ds.to_host()
+    finally:
+        hctx.free(f)
+        ds.free()
+    o = st.copy()
+    assert O.water_table(o, filter_c=perm) == 0
+    _same(g, o)
+
+
+@pytest.mark.parametrize("nsteps,name", [(1, "canopy"), (3, "canopy_uneven")])
+def test_land_step_full_with_water_tables_bitwise(hctx, nsteps, name):
+    st = S.canopy_workload(name, ncol=320, seed=0xC75D0031)
+    print(f"case: full land step + surface water + water tables x{nsteps}, {name}, 320 columns / "
+          f"{st.dims.npatch} patches", flush=True)
+    hydro = O.hydro_params()
+    ds = DeviceState(st, hctx)
+    try:
+        for _ in range(nsteps):
+            hctx.land_step_full_dev(ds, hydro=hydro, water_table=True)
+        hctx.sync()
+        g = ds.to_host()
+    finally:
+        ds.free()
+    o = st.copy()
+    cfg = O.canopy_config()
+    pft = O.PftStruct(S.pft_table())
+    for _ in range(nsteps):
+        rc, bad = O.land_step_full(o, cfg, S.canopy_params(), pft, hydro=hydro, water_table_after=True)
+        assert rc == 0, bad
+    _same(g, o, groups=("col", "pch", "can", "sf", "hy"))
+    # the loop is closed: the water tables the next step's fsat reads were recomputed
+    assert np.any(o.hy["zwt"] != st.hy["zwt"]) and np.any(o.hy["frost_table"] != st.hy["frost_table"])
+
+
+def test_water_table_endrun_reports_column():
+    from ctsm_amd.lib import Context, CtsmError
+    st = _wt_state(64, seed=0x5EED0033)
+    bare = np.nonzero(st.col["snl"] == 0)[0]
+    k = int(bare[3])
+    st.hy["qflx_solidevap_from_top_layer"][k] = 1.0     # far more than the top layer's ice
+    print(f"case: RenewCondensation sublimation beyond the ice, column {k + 1} of 64", flush=True)
+    assert O.water_table(st.copy()) == k + 1
+    c = Context()
+    try:
+        with pytest.raises(CtsmError) as ei:
+            c.water_table(st)
+        assert ei.value.index == k + 1
+    finally:
+        c.close()

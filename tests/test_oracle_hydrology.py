@@ -216,3 +216,162 @@ def test_patch2col_of_evaporation_fluxes():
         ref = np.zeros(st.dims.ncol)
         np.add.at(ref, col, src * w)
         assert np.allclose(st.hy[dst], ref, rtol=1e-13, atol=1e-20), dst
+
+
+# ---- water tables and RenewCondensation (HydrologyNoDrainageMod.F90:355-373) ----
+
+SAT_LEV = float(np.float32(0.9))      # `0.9` without a kind: the single-precision value
+TFRZ = 273.15
+
+
+def _wt_state(ncol=256, seed=0x5EED0021, snow=False):
+    frac = 0.4 * S.uniform(seed, 94, np.arange(ncol, dtype=np.int64))
+    st = S.soil_only(ncol, seed, snow=snow, bedrock=True, frac_h2osfc=frac)
+    S.with_hydrology(st, seed)
+    return st
+
+
+def _lev(st):
+    ns = st.dims.nlevsno
+    c = st.col
+    return dict(
+        t=lambda k: c["t_soisno"][ns + k - 1], z=lambda k: c["z"][ns + k - 1], dz=lambda k: c["dz"][ns + k - 1],
+        zi=lambda j: c["zi"][ns + j], ws=lambda k: c["watsat"][k - 1],
+        liq=lambda k: c["h2osoi_liq"][ns + k - 1], ice=lambda k: c["h2osoi_ice"][ns + k - 1])
+
+
+def _set_saturation(st, s_of_layer):
+    """Liquid water for relative saturation s_of_layer(k) (array over columns), no ice."""
+    ns = st.dims.nlevsno
+    c = st.col
+    for k in range(1, st.dims.nlevgrnd + 1):
+        c["h2osoi_ice"][ns + k - 1] = 0.0
+        if k <= st.dims.nlevsoi:
+            ws = c["watsat"][k - 1]
+            c["h2osoi_liq"][ns + k - 1] = s_of_layer(k) * ws * c["dz"][ns + k - 1] * 1000.0
+
+
+def test_water_table_saturated_dry_and_interpolated():
+    st = _wt_state()
+    n = st.dims.ncol
+    L = _lev(st)
+    nb = st.col["nbedrock"].astype(int)
+    cols = np.arange(n)
+    kind = cols % 3                      # 0 saturated, 1 dry, 2 dry at layer K over saturated layers
+    K = np.maximum(1, nb // 2)
+    rng = np.random.default_rng(11)
+    s_dry = rng.uniform(0.3, 0.8, n)
+    s_wet = rng.uniform(0.95, 1.0, n)
+    _set_saturation(st, lambda k: np.where(kind == 0, s_wet, np.where(kind == 1, s_dry,
+                                                                       np.where(k <= K, s_dry, s_wet))))
+    st.col["t_soisno"][...] = 285.0      # unfrozen: no perched table
+    assert O.water_table(st) == 0
+    zwt = st.hy["zwt"]
+    zi = st.col["zi"]
+    ns = st.dims.nlevsno
+    sat = kind == 0
+    assert np.array_equal(zwt[sat], zi[ns + 1, sat])                       # zi(1)
+    dry = kind == 1
+    assert np.array_equal(zwt[dry], zi[ns + nb[dry], cols[dry]])           # zi(nbedrock)
+    mid = (kind == 2) & (K < nb) & (K > 1)
+    assert mid.sum() > 20
+    k = K[mid]
+    c = cols[mid]
+    s1 = (st.col["h2osoi_liq"][ns + k - 1, c] / (st.col["dz"][ns + k - 1, c] * 1000.0)) / st.col["watsat"][k - 1, c]
+    s2 = (st.col["h2osoi_liq"][ns + k, c] / (st.col["dz"][ns + k, c] * 1000.0)) / st.col["watsat"][k, c]
+    z0, z1 = st.col["z"][ns + k - 1, c], st.col["z"][ns + k, c]
+    expect = np.maximum(0.0, z0 + (SAT_LEV - s1) * (z1 - z0) / (s2 - s1))
+    assert np.allclose(zwt[mid], expect, rtol=1e-12, atol=1e-14)
+    assert np.all((zwt[mid] >= z0 - 1e-12) & (zwt[mid] <= z1 + 1e-12))
+    # volumetric water of the visited layers
+    vol = st.hy["h2osoi_vol"]
+    assert np.allclose(vol[nb[dry] - 1, cols[dry]],
+                       st.col["h2osoi_liq"][ns + nb[dry] - 1, cols[dry]] / (st.col["dz"][ns + nb[dry] - 1, cols[dry]] * 1000.0),
+                       rtol=1e-15)
+    # unfrozen columns: no perched table, the frost table at zi(nlevsoi - 1)
+    nl = st.dims.nlevsoi
+    assert np.array_equal(st.hy["frost_table"], zi[ns + nl - 1])
+    assert np.array_equal(st.hy["zwt_perched"], st.hy["frost_table"])
+
+
+def test_frost_table_and_perched_water():
+    st = _wt_state(seed=0x5EED0022)
+    n = st.dims.ncol
+    ns = st.dims.nlevsno
+    nl = st.dims.nlevsoi
+    cols = np.arange(n)
+    rng = np.random.default_rng(12)
+    kf = rng.integers(3, nl, n)                         # first frozen layer under thawed ones
+    kp = np.maximum(1, kf - rng.integers(1, 3, n))      # dry layer above the perched water
+    for k in range(1, st.dims.nlevgrnd + 1):
+        st.col["t_soisno"][ns + k - 1] = np.where(k < kf, 280.0, 265.0)
+    s_dry = rng.uniform(0.3, 0.8, n)
+    s_wet = rng.uniform(0.95, 1.0, n)
+    _set_saturation(st, lambda k: np.where(k <= kp, s_dry, s_wet))
+    st.hy["zwt"][:] = 50.0                              # deep water table: the perched search runs
+    liq = st.col["h2osoi_liq"].copy()                   # the searches see the water before condensation
+    assert O.water_table(st) == 0
+    zi, z = st.col["zi"], st.col["z"]
+    assert np.array_equal(st.hy["frost_table"], zi[ns + kf - 1, cols])
+    per = kf > kp
+    k, c = kp[per], cols[per]
+    assert (k == 1).any() and (k > 1).any()
+    s1 = (liq[ns + k - 1, c] / (st.col["dz"][ns + k - 1, c] * 1000.0)) / st.col["watsat"][k - 1, c]
+    s2 = (liq[ns + k, c] / (st.col["dz"][ns + k, c] * 1000.0)) / st.col["watsat"][k, c]
+    z0, z1 = z[ns + k - 1, c], z[ns + k, c]
+    expect = np.maximum(0.0, z0 + (SAT_LEV - s1) * (z1 - z0) / (s2 - s1))
+    assert np.allclose(st.hy["zwt_perched"][per], expect, rtol=1e-12, atol=1e-14)
+    # a water table above the frost table leaves the perched table at the frost table
+    st2 = _wt_state(seed=0x5EED0022)
+    st2.col["t_soisno"][...] = st.col["t_soisno"]
+    st2.col["h2osoi_liq"][...] = liq
+    st2.col["h2osoi_ice"][...] = 0.0
+    st2.hy["zwt"][:] = 0.0
+    assert O.water_table(st2) == 0
+    assert np.array_equal(st2.hy["zwt_perched"], st2.hy["frost_table"])
+    # frozen from the top: the frost table is the surface, nothing perched
+    st3 = _wt_state(seed=0x5EED0023)
+    st3.col["t_soisno"][...] = 260.0
+    assert O.water_table(st3) == 0
+    assert np.all(st3.hy["frost_table"] == zi[ns]) and np.all(st3.hy["zwt_perched"] == zi[ns])
+
+
+def test_renew_condensation_mass_and_truncation():
+    st = _wt_state(seed=0x5EED0024, snow=True)
+    ns = st.dims.nlevsno
+    cols = np.arange(st.dims.ncol)
+    liq0 = st.col["h2osoi_liq"][ns].copy()
+    ice0 = st.col["h2osoi_ice"][ns].copy()
+    h = st.hy
+    # one column whose sublimation removes all its ice up to a rounding residue
+    bare = np.nonzero((st.col["snl"] == 0) & (st.col["frac_h2osfc"] > 0.1))[0]
+    j = bare[0]
+    wf_j = 1.0 - st.col["frac_h2osfc"][j]
+    for ice0_j in np.linspace(0.1, 3.0, 400):
+        q = ice0_j / (wf_j * DT)
+        left = ice0_j - wf_j * q * DT
+        if left != 0.0 and abs(left) < 1e-12 * ice0_j:
+            break
+    else:
+        raise AssertionError("no rounding residue found")
+    st.col["h2osoi_ice"][ns, j] = ice0_j
+    ice0[j] = ice0_j
+    h["qflx_soliddew_to_top_layer"][j] = 0.0
+    h["qflx_solidevap_from_top_layer"][j] = q
+    assert O.water_table(st) == 0
+    snow = st.col["snl"] < 0
+    assert np.array_equal(st.col["h2osoi_liq"][ns, snow], liq0[snow])     # snow columns untouched
+    assert np.array_equal(st.col["h2osoi_ice"][ns, snow], ice0[snow])
+    ok = ~snow
+    wf = 1.0 - st.col["frac_h2osfc"]
+    assert np.allclose(st.col["h2osoi_liq"][ns, ok] - liq0[ok], wf[ok] * h["qflx_liqdew_to_top_layer"][ok] * DT,
+                       rtol=1e-9, atol=1e-12)
+    dice = wf * (h["qflx_soliddew_to_top_layer"] - h["qflx_solidevap_from_top_layer"]) * DT
+    assert np.allclose(st.col["h2osoi_ice"][ns, ok] - ice0[ok], dice[ok], rtol=1e-9, atol=1e-12)
+    assert st.col["h2osoi_ice"][ns, j] == 0.0                                # the residue truncated
+    assert np.all(st.col["h2osoi_ice"][ns, ok] >= 0.0)
+    # sublimation beyond the ice: the reference's endrun, reported with the column
+    st2 = _wt_state(seed=0x5EED0024, snow=True)
+    k = bare[1]
+    st2.hy["qflx_solidevap_from_top_layer"][k] = 1.0
+    assert O.water_table(st2) == k + 1
