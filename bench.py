@@ -293,7 +293,8 @@ def _full_land_step(ctx, args, ds, structs):
     sw_ms, sf_ms = time_events(ctx, [lambda: ctx.surface_water_dev(ds, hydro), lambda: ctx.soilfluxes_dev(ds)], 3,
                                prefill=lambda: ctx.canopyfluxes_dev(ds, structs=structs), inner=10)
     b = surface_water_bytes(ds)
-    return {"column_steps_per_s": ds.dims.ncol * args.canopy_steps / wall, "step_ms": step_ms,
+    out = {"history": _history_and_checkpoint(ctx, ds, structs)}
+    return {**out, "column_steps_per_s": ds.dims.ncol * args.canopy_steps / wall, "step_ms": step_ms,
             "max_abs_errsoi_Wm2": worst,
             "columns_errsoi_over_1e-5_Wm2": int(np.sum(~(np.abs(err) <= 1e-5))),
             "columns_negative_soil_liquid": int(np.sum((liq[ds.dims.nlevsno:] < 0).any(axis=0))),
@@ -301,6 +302,49 @@ def _full_land_step(ctx, args, ds, structs):
             "surface_water_roofline": {"kernel": "k_surface_water", "alg_bytes_per_launch": b,
                                        "achieved_GBps": b / (sw_ms * 1e-3) / 1e9,
                                        "frac": b / (sw_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+
+
+def _history_and_checkpoint(ctx, ds, structs):
+    """The default history tape's per-step update (one launch over every field,
+    buffers in HBM) with its HBM roofline, and a checkpoint of the whole
+    resident state written and read back through the library."""
+    import os
+    import shutil
+    import tempfile
+    from ctsm_amd import restart
+    from ctsm_amd.history import HistoryTape
+    tape = HistoryTape(ctx, ds)
+    elems = sum(f.n1d * f.num2d for f in tape.desc)
+    upd_ms, = time_events(ctx, [tape.update], 3, prefill=lambda: ctx.canopyfluxes_dev(ds, structs=structs), inner=10)
+    tape.free()
+    hb = elems * (8 + 8 + 4 + 8 + 4)        # field read, hbuf and nacs read and written
+    out = {"fields": len(tape.specs), "elements": elems, "update_kernel_ms": upd_ms,
+           "update_roofline": {"kernel": "k_hist_update", "alg_bytes_per_launch": hb,
+                               "achieved_GBps": hb / (upd_ms * 1e-3) / 1e9,
+                               "frac": hb / (upd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+    tmp = tempfile.gettempdir()
+    need = ds.nbytes()
+    if shutil.disk_usage(tmp).free < 3 * need:
+        out["checkpoint"] = f"skipped: {need / 1e9:.1f} GB state, too little space in {tmp}"
+        return out
+    fd, path = tempfile.mkstemp(suffix=".ckpt")
+    os.close(fd)
+    try:
+        t0 = time.perf_counter()
+        restart.checkpoint(ctx, ds, path)
+        t_write = time.perf_counter() - t0
+        nbytes = os.path.getsize(path)
+        t0 = time.perf_counter()
+        ds2, _ = restart.restore(ctx, path)
+        t_read = time.perf_counter() - t0
+        ds2.free()
+        out.update({"checkpoint_bytes": nbytes, "checkpoint_write_GBps": nbytes / t_write / 1e9,
+                    "checkpoint_restore_GBps": nbytes / t_read / 1e9})
+    except Exception as e:      # a full or slow file system must not end the benchmark
+        out["checkpoint"] = f"failed: {e}"
+    finally:
+        os.remove(path)
+    return out
 
 
 def _full_column_parts(ctx, args, st):

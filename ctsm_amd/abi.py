@@ -404,6 +404,16 @@ class WaterTableState(C.Structure):
     _fields_ = [(n, _CT[t]) for n, t, _ in WATERTABLE_FIELDS]
 
 
+class HistField(C.Structure):
+    _fields_ = [("field", C.c_void_p), ("active", C.c_void_p), ("hbuf", C.c_void_p), ("nacs", C.c_void_p),
+                ("ld", C.c_int64), ("n1d", C.c_int32), ("num2d", C.c_int32), ("avgflag", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+class CkptField(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("ptr", C.c_void_p), ("bytes", C.c_uint64)]
+
+
 class SoilFluxState(C.Structure):
     _fields_ = [(n, _CT[t]) for n, t, _ in SOILFLUX_FIELDS]
 

@@ -53,6 +53,9 @@ void launch_patch2col3(hipStream_t, const ctsm_layout &, const ctsm_col_state &,
                        const int32_t *, ErrWord *);
 void launch_water_table(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_col_state &,
                         const ctsm_water_table_state &, int, const int32_t *, double, ErrWord *);
+void launch_hist(hipStream_t, const ctsm_hist_field *, int, int);
+int ckpt_write(hipStream_t, const char *, const ctsm_ckpt_field *, int, hipError_t *);
+int ckpt_read(hipStream_t, const char *, const ctsm_ckpt_field *, int, hipError_t *);
 }  // namespace ctsm
 
 using namespace ctsm;
@@ -1469,3 +1472,54 @@ int ctsm_sync(ctsm_ctx *ctx)
 void *ctsm_stream(ctsm_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
 }  // extern "C"
+
+// ---- history buffers ---------------------------------------------------------
+
+static int hist_call(ctsm_ctx *ctx, const ctsm_hist_field *fields, int32_t n, int op)
+{
+    if (!ctx || n < 0 || (n > 0 && !fields)) return CTSM_E_ARG;
+    for (int32_t i = 0; i < n; ++i) {
+        const ctsm_hist_field &f = fields[i];
+        const int a = f.avgflag;
+        if (f.n1d < 0 || f.num2d < 0 || (long long)f.n1d * f.num2d > INT32_MAX ||
+            (a != 'A' && a != 'S' && a != 'I' && a != 'X' && a != 'M'))
+            return CTSM_E_ARG;
+        if ((long long)f.n1d * f.num2d == 0) continue;
+        if (!f.hbuf || !f.nacs || (op == 0 && !f.field) || (f.num2d > 1 && f.ld < f.n1d)) return CTSM_E_ARG;
+    }
+    launch_hist(ctx->stream, fields, n, op);
+    return finish_launch(ctx);
+}
+
+int ctsm_hist_update_dev(ctsm_ctx *ctx, const ctsm_hist_field *fields, int32_t n) { return hist_call(ctx, fields, n, 0); }
+int ctsm_hist_normalize_dev(ctsm_ctx *ctx, const ctsm_hist_field *fields, int32_t n) { return hist_call(ctx, fields, n, 1); }
+int ctsm_hist_zero_dev(ctsm_ctx *ctx, const ctsm_hist_field *fields, int32_t n) { return hist_call(ctx, fields, n, 2); }
+
+// ---- checkpoints ---------------------------------------------------------------
+
+static int ckpt_call(ctsm_ctx *ctx, const char *path, const ctsm_ckpt_field *fields, int32_t n, bool write)
+{
+    if (!ctx || !path || n < 0 || (n > 0 && !fields)) return CTSM_E_ARG;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipError_t he = hipSetDevice(ctx->device);
+    if (he != hipSuccess) return hip_fail(ctx, he);
+    // the state a checkpoint holds is the one after every enqueued step; a
+    // failed step is reported instead of written
+    int rc = sync_and_check(ctx);
+    if (rc) return rc;
+    rc = write ? ckpt_write(ctx->stream, path, fields, n, &he) : ckpt_read(ctx->stream, path, fields, n, &he);
+    if (rc == CTSM_E_HIP) return hip_fail(ctx, he);
+    ctx->last_code = rc;
+    ctx->last_index = 0;
+    return rc;
+}
+
+int ctsm_checkpoint_write(ctsm_ctx *ctx, const char *path, const ctsm_ckpt_field *fields, int32_t n)
+{
+    return ckpt_call(ctx, path, fields, n, true);
+}
+
+int ctsm_checkpoint_read(ctsm_ctx *ctx, const char *path, const ctsm_ckpt_field *fields, int32_t n)
+{
+    return ckpt_call(ctx, path, fields, n, false);
+}

@@ -18,9 +18,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libctsm_mi355x.so")
 _LIB = None
 
-CTSM_OK, CTSM_E_ARG, CTSM_E_CONFIG, CTSM_E_HIP, CTSM_E_SOLVER, CTSM_E_NODEVICE, CTSM_E_ENDRUN = range(7)
+CTSM_OK, CTSM_E_ARG, CTSM_E_CONFIG, CTSM_E_HIP, CTSM_E_SOLVER, CTSM_E_NODEVICE, CTSM_E_ENDRUN, CTSM_E_IO = range(8)
 _NAMES = {1: "CTSM_E_ARG", 2: "CTSM_E_CONFIG", 3: "CTSM_E_HIP", 4: "CTSM_E_SOLVER", 5: "CTSM_E_NODEVICE",
-          6: "CTSM_E_ENDRUN"}
+          6: "CTSM_E_ENDRUN", 7: "CTSM_E_IO"}
 
 # every symbol declared in include/ctsm_gpu.h
 EXPORTS = [
@@ -35,6 +35,8 @@ EXPORTS = [
     "ctsm_soilfluxes", "ctsm_soil_balance_check", "ctsm_soilfluxes_dev", "ctsm_soil_balance_check_dev",
     "ctsm_land_step_full_dev", "ctsm_soil_steps_fused_dev", "ctsm_surface_water", "ctsm_surface_water_dev",
     "ctsm_water_table", "ctsm_water_table_dev",
+    "ctsm_hist_update_dev", "ctsm_hist_normalize_dev", "ctsm_hist_zero_dev",
+    "ctsm_checkpoint_write", "ctsm_checkpoint_read",
 ]
 
 
@@ -88,6 +90,10 @@ def load():
     wta = col + [P(abi.WaterTableState), C.c_int32, C.c_void_p, C.c_double]
     L.ctsm_water_table.argtypes = wta
     L.ctsm_water_table_dev.argtypes = wta
+    for fn in ("ctsm_hist_update_dev", "ctsm_hist_normalize_dev", "ctsm_hist_zero_dev"):
+        getattr(L, fn).argtypes = [C.c_void_p, P(abi.HistField), C.c_int32]
+    for fn in ("ctsm_checkpoint_write", "ctsm_checkpoint_read"):
+        getattr(L, fn).argtypes = [C.c_void_p, C.c_char_p, P(abi.CkptField), C.c_int32]
     L.ctsm_graph_launch.argtypes = [C.c_void_p, C.c_void_p]
     L.ctsm_graph_destroy.argtypes = [C.c_void_p]
     L.ctsm_sync.argtypes = [C.c_void_p]
@@ -331,6 +337,25 @@ class Context:
         fp = C.c_void_p(filter_dev) if filter_dev is not None else None
         self._check(self.L.ctsm_water_table_dev(self.h, C.byref(ds.layout()), C.byref(cs), C.byref(wt), n, fp,
                                                 dtime), "water table")
+
+    def hist_update(self, fields):
+        """hist_update_hbuf over an array of abi.HistField (enqueues only)."""
+        self._check(self.L.ctsm_hist_update_dev(self.h, fields, len(fields)), "history update")
+
+    def hist_normalize(self, fields):
+        self._check(self.L.ctsm_hist_normalize_dev(self.h, fields, len(fields)), "history normalize")
+
+    def hist_zero(self, fields):
+        self._check(self.L.ctsm_hist_zero_dev(self.h, fields, len(fields)), "history zero")
+
+    def checkpoint_write(self, path, fields):
+        """fields: list of (name, pointer, bytes); waits for the stream first."""
+        arr = (abi.CkptField * len(fields))(*[abi.CkptField(n.encode(), p, b) for n, p, b in fields])
+        self._check(self.L.ctsm_checkpoint_write(self.h, str(path).encode(), arr, len(fields)), "checkpoint write")
+
+    def checkpoint_read(self, path, fields):
+        arr = (abi.CkptField * len(fields))(*[abi.CkptField(n.encode(), p, b) for n, p, b in fields])
+        self._check(self.L.ctsm_checkpoint_read(self.h, str(path).encode(), arr, len(fields)), "checkpoint read")
 
     def graph_capture(self, ds, nsteps=1, canopy=True, filterc_dev=None, nc=None, filterp_dev=None, np_=None,
                       dtime=1800.0, structs=None):

@@ -57,6 +57,8 @@ extern "C" {
 #define CTSM_E_SOLVER      4   /* LAPACK-style info != 0 in a column solve        */
 #define CTSM_E_NODEVICE    5   /* no GPU visible                                  */
 #define CTSM_E_ENDRUN      6   /* a check where the reference calls endrun fails   */
+#define CTSM_E_IO          7   /* checkpoint file cannot be written / read, or does
+                                  not hold the fields asked for (name, size, checksum) */
                                /* (canopy: quadratic, brent bracket, gs < 0, ...)  */
 
 /* soil water lower / upper boundary conditions (SoilWaterMovementMod.F90:55-59) */
@@ -687,6 +689,57 @@ int ctsm_graph_capture(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_sta
                        int32_t nsteps, ctsm_graph **out);
 int ctsm_graph_launch(ctsm_ctx *ctx, ctsm_graph *graph);
 int ctsm_graph_destroy(ctsm_graph *graph);
+
+/* ---- history buffers on device (histFileMod.F90:1362-2169 hist_update_hbuf,
+ * :2302-2358 hfields_normalize, :2361-2381 hfields_zero) --------------------------
+ * A history field kept at the resolution of its source, as the reference's
+ * branch that does not average up to the gridcell: n1d points x num2d levels of
+ * `field`, level j (0-based) at field + j * ld.  `active` (n1d flags; NULL: every
+ * point counts) is the patch / column `active` flag the reference checks.  hbuf
+ * and nacs are num2d x n1d, point fastest, in device memory.  avgflag is the
+ * flag's character: 'A' time average, 'S' sum, 'I' instantaneous, 'X' maximum,
+ * 'M' minimum ('L', local solar time, is not taken).  Values equal to 1e36
+ * (spval) are missing, as in the reference.  The descriptors are passed by
+ * value to the kernels, so the calls may be captured in a graph. */
+typedef struct ctsm_hist_field {
+    const double  *field;
+    const int32_t *active;
+    double        *hbuf;
+    int32_t       *nacs;
+    int64_t        ld;
+    int32_t        n1d;
+    int32_t        num2d;
+    int32_t        avgflag;
+    int32_t        pad;
+} ctsm_hist_field;
+
+/* Accumulates every field into its buffer (hist_update_hbuf, one step). */
+int ctsm_hist_update_dev(ctsm_ctx *ctx, const ctsm_hist_field *fields, int32_t nfields);
+/* Divides the 'A' buffers by their counts (hfields_normalize, before a write). */
+int ctsm_hist_normalize_dev(ctsm_ctx *ctx, const ctsm_hist_field *fields, int32_t nfields);
+/* Clears buffers and counts (hfields_zero, after a write). */
+int ctsm_hist_zero_dev(ctsm_ctx *ctx, const ctsm_hist_field *fields, int32_t nfields);
+
+/* ---- checkpoint / resume of device-resident state --------------------------
+ * The drop-in path leaves restart files to the Fortran host (restFileMod.F90:
+ * 74-156 writes the host arrays); a device-resident run writes its state here.
+ * A checkpoint file holds named byte ranges: an 8-byte magic "CTSMCKP1", the
+ * field count and, per field, a 64-byte name, its size, its file offset and a
+ * 64-bit FNV-1a checksum of its 8-byte words, then the data, each range
+ * aligned to 4 KiB.  Writing
+ * waits for the context's stream, then streams the ranges to the file through
+ * two pinned staging buffers (the next chunk's copy overlaps the previous
+ * chunk's write).  Reading fills each named range from the file the same way
+ * and returns CTSM_E_IO when a name is missing, a size differs or a checksum
+ * does not match.  `ptr` may be device or host memory. */
+typedef struct ctsm_ckpt_field {
+    const char *name;     /* NUL-terminated, at most 63 characters */
+    void       *ptr;
+    uint64_t    bytes;
+} ctsm_ckpt_field;
+
+int ctsm_checkpoint_write(ctsm_ctx *ctx, const char *path, const ctsm_ckpt_field *fields, int32_t nfields);
+int ctsm_checkpoint_read(ctsm_ctx *ctx, const char *path, const ctsm_ckpt_field *fields, int32_t nfields);
 
 /* ---- device memory and timing helpers --------------------------------------
  * Hosts need no other GPU runtime: device-resident state is allocated, filled
