@@ -264,19 +264,20 @@ def surface_water_bytes(st):
 def _full_land_step(ctx, args, ds, structs):
     """The land step with the reference's bookkeeping (before-step save,
     SoilFluxes, the surface water input chain feeding SoilWater's
-    infiltration, the water tables and RenewCondensation after it) on the
+    infiltration, the water tables and RenewCondensation after it, the
+    drainage of HydrologyDrainage) on the
     same resident batch, and the launch times of the SoilFluxes and surface
     water kernels."""
     from ctsm_amd import synthetic
     hydro = synthetic.hydro_params()
     full = (*structs, ds.soilflux_struct())
     for _ in range(2):
-        ctx.land_step_full_dev(ds, structs=full, hydro=hydro, water_table=True)
+        ctx.land_step_full_dev(ds, structs=full, hydro=hydro, water_table=True, drainage=True)
     ctx.sync()
     e0 = ctx.record()
     t0 = time.perf_counter()
     for _ in range(args.canopy_steps):
-        ctx.land_step_full_dev(ds, structs=full, hydro=hydro, water_table=True)
+        ctx.land_step_full_dev(ds, structs=full, hydro=hydro, water_table=True, drainage=True)
     e1 = ctx.record()
     ctx.sync()
     wall = time.perf_counter() - t0

@@ -358,9 +358,17 @@ WATERTABLE_FIELDS = [("zwt", "f64", "c"), ("zwt_perched", "f64", "c"), ("frost_t
                      ("qflx_soliddew_to_top_layer", "f64", "c"), ("qflx_solidevap_from_top_layer", "f64", "c")]
 
 
+DRAINAGE_FIELDS = [(n, "f64", "soi" if n == "h2osoi_vol" else "c") for n in (
+    "topo_slope", "zwt_perched", "frost_table", "qflx_surf", "qflx_snwcp_liq", "zwt", "h2osoi_vol",
+    "qflx_drain_perched", "qflx_latflow_out", "qflx_latflow_in", "qflx_rsub_sat", "qflx_ice_runoff_xs",
+    "qflx_drain", "qflx_qrgwl", "qflx_runoff")]
+
+
 class HydroParams(C.Structure):
-    _fields_ = [("fff", C.c_double), ("pc", C.c_double), ("mu", C.c_double), ("h2osfcflag", C.c_int32),
-                ("pad", C.c_int32)]
+    _fields_ = [("fff", C.c_double), ("pc", C.c_double), ("mu", C.c_double), ("aq_sp_yield_min", C.c_double),
+                ("n_baseflow", C.c_double), ("baseflow_scalar", C.c_double), ("perched_baseflow_scalar", C.c_double),
+                ("h2osfcflag", C.c_int32), ("pad", C.c_int32)]
+
 
 
 CANOPY_PARAM_NAMES = [
@@ -398,6 +406,10 @@ class CanopyState(C.Structure):
 
 class SurfaceWaterState(C.Structure):
     _fields_ = [(n, _CT[t]) for n, t, _ in SURFWATER_FIELDS]
+
+
+class DrainageState(C.Structure):
+    _fields_ = [(n, _CT[t]) for n, t, _ in DRAINAGE_FIELDS]
 
 
 class WaterTableState(C.Structure):

@@ -54,6 +54,8 @@ void launch_patch2col3(hipStream_t, const ctsm_layout &, const ctsm_col_state &,
 void launch_water_table(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_col_state &,
                         const ctsm_water_table_state &, int, const int32_t *, double, ErrWord *);
 void launch_hist(hipStream_t, const ctsm_hist_field *, int, int);
+void launch_drainage(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_hydro_params &,
+                     const ctsm_col_state &, const ctsm_drainage_state &, int, const int32_t *, double, ErrWord *);
 int ckpt_write(hipStream_t, const char *, const ctsm_ckpt_field *, int, hipError_t *);
 int ckpt_read(hipStream_t, const char *, const ctsm_ckpt_field *, int, hipError_t *);
 }  // namespace ctsm
@@ -222,6 +224,19 @@ const FieldMeta kWt[] = {
 constexpr int kNWt = sizeof(kWt) / sizeof(kWt[0]);
 static_assert(kNWt * sizeof(void *) == sizeof(ctsm_water_table_state),
               "every ctsm_water_table_state member has a mirror entry");
+// drainage
+#define DR(name, kind, wr) {offsetof(ctsm_drainage_state, name), kind, 8, wr}
+const FieldMeta kDr[] = {
+    DR(topo_slope, K_C, false), DR(zwt_perched, K_C, false), DR(frost_table, K_C, false), DR(qflx_surf, K_C, false),
+    DR(qflx_snwcp_liq, K_C, false), DR(zwt, K_C, true), DR(h2osoi_vol, K_SOI, true),
+    DR(qflx_drain_perched, K_C, true), DR(qflx_latflow_out, K_C, true), DR(qflx_latflow_in, K_C, true),
+    DR(qflx_rsub_sat, K_C, true), DR(qflx_ice_runoff_xs, K_C, true), DR(qflx_drain, K_C, true),
+    DR(qflx_qrgwl, K_C, true), DR(qflx_runoff, K_C, true),
+};
+#undef DR
+constexpr int kNDr = sizeof(kDr) / sizeof(kDr[0]);
+static_assert(kNDr * sizeof(void *) == sizeof(ctsm_drainage_state),
+              "every ctsm_drainage_state member has a mirror entry");
 constexpr int kNSfP = sizeof(kSfPatch) / sizeof(kSfPatch[0]);
 constexpr int kNSfC = sizeof(kSfCol) / sizeof(kSfCol[0]);
 static_assert((kNSfP + kNSfC) * sizeof(void *) == sizeof(ctsm_soilflux_state),
@@ -323,6 +338,7 @@ struct ctsm_ctx {
     std::vector<DevBuf> sfp, sfc;      // SoilFluxes drop-in mirrors
     std::vector<DevBuf> swm;           // surface-water drop-in mirrors
     std::vector<DevBuf> wtm;           // water-table drop-in mirrors
+    std::vector<DevBuf> drm;           // drainage drop-in mirrors
     bool have_canopy = false;
     int32_t last_code = 0, last_index = 0;
     std::mutex mu;
@@ -612,6 +628,7 @@ int ctsm_destroy(ctsm_ctx *ctx)
     for (auto &b : ctx->sfc) b.release();
     for (auto &b : ctx->swm) b.release();
     for (auto &b : ctx->wtm) b.release();
+    for (auto &b : ctx->drm) b.release();
     ctx->partf.release();
     for (int k = 0; k < ctsm_ctx::kMaxStreams - 1; ++k) {
         if (ctx->side[k]) { (void)hipStreamSynchronize(ctx->side[k]); (void)hipStreamDestroy(ctx->side[k]); }
@@ -1259,10 +1276,69 @@ int ctsm_water_table(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state
     return sync_and_check(ctx);
 }
 
+// the members every drainage call needs (h2osoi_vol may be NULL)
+static bool dr_complete(const ctsm_drainage_state *dr)
+{
+    for (int f = 0; f < kNDr; ++f) {
+        if (kDr[f].offset == offsetof(ctsm_drainage_state, h2osoi_vol)) continue;
+        if (!*(void *const *)((const char *)dr + kDr[f].offset)) return false;
+    }
+    return true;
+}
+
+static bool dr_col_complete(const ctsm_col_state *col)
+{
+    return col->dz && col->z && col->zi && col->watsat && col->bsw && col->sucsat && col->hksat &&
+           col->eff_porosity && col->nbedrock && col->h2osoi_liq && col->h2osoi_ice && col->h2osfc && col->icefrac;
+}
+
+int ctsm_drainage_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                      const ctsm_drainage_state *dr, const ctsm_hydro_params *prm, int32_t n,
+                      const int32_t *filter_dev, double dtime)
+{
+    if (!ctx || !col || !dr || !prm || n < 0 || check_layout(lay) || (!filter_dev && n > lay->ldc) ||
+        !dr_complete(dr) || !dr_col_complete(col))
+        return CTSM_E_ARG;
+    launch_drainage(ctx->stream, *lay, ctx->cfg, *prm, *col, *dr, n, filter_dev, dtime, ctx->err);
+    return finish_launch(ctx);
+}
+
+int ctsm_drainage(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col, const ctsm_drainage_state *dr,
+                  const ctsm_hydro_params *prm, int32_t n, const int32_t *filter, double dtime)
+{
+    if (!ctx || !col || !dr || !prm || n < 0 || check_layout(lay) || (!filter && n > lay->ldc) ||
+        !dr_complete(dr) || !dr_col_complete(col))
+        return CTSM_E_ARG;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipError_t he = hipSetDevice(ctx->device);
+    if (he != hipSuccess) return hip_fail(ctx, he);
+    for (int i = 0; filter && i < n; ++i)
+        if (filter[i] < lay->begc_alloc || filter[i] >= lay->begc_alloc + lay->ldc) return CTSM_E_ARG;
+    ctsm_col_state hc{};
+    hc.dz = col->dz; hc.z = col->z; hc.zi = col->zi; hc.watsat = col->watsat; hc.bsw = col->bsw;
+    hc.sucsat = col->sucsat; hc.hksat = col->hksat; hc.eff_porosity = col->eff_porosity; hc.nbedrock = col->nbedrock;
+    hc.h2osoi_liq = col->h2osoi_liq; hc.h2osoi_ice = col->h2osoi_ice; hc.h2osfc = col->h2osfc;
+    hc.icefrac = col->icefrac;
+    const Range cr{lay->begc_alloc, lay->begc_alloc + lay->ldc - 1};
+    ctsm_col_state dcol{};
+    ctsm_drainage_state ddr{};
+    const int ncf = (int)(sizeof(kColFields) / sizeof(kColFields[0]));
+    int rc = mirror(ctx, kColFields, ncf, &hc, &dcol, ctx->colmir, lay->begc_alloc, lay->ldc, cr, true);
+    if (!rc) rc = mirror(ctx, kDr, kNDr, dr, &ddr, ctx->drm, lay->begc_alloc, lay->ldc, cr, true);
+    if (rc) return rc;
+    const int32_t *dfc = nullptr;
+    if ((rc = upload_filter(ctx, n, filter, &dfc))) return rc;
+    launch_drainage(ctx->stream, *lay, ctx->cfg, *prm, dcol, ddr, n, dfc, dtime, ctx->err);
+    if ((rc = finish_launch(ctx))) return rc;
+    if ((rc = mirror(ctx, kColFields, ncf, &hc, &dcol, ctx->colmir, lay->begc_alloc, lay->ldc, cr, false))) return rc;
+    if ((rc = mirror(ctx, kDr, kNDr, dr, &ddr, ctx->drm, lay->begc_alloc, lay->ldc, cr, false))) return rc;
+    return sync_and_check(ctx);
+}
+
 int ctsm_land_step_full_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
                             const ctsm_patch_state *pch, const ctsm_canopy_state *can, const ctsm_soilflux_state *sf,
                             const ctsm_surface_water_state *sw, const ctsm_hydro_params *hprm,
-                            const ctsm_water_table_state *wt, int32_t nc,
+                            const ctsm_water_table_state *wt, const ctsm_drainage_state *dr, int32_t nc,
                             const int32_t *filterc_dev, int32_t np, const int32_t *filterp_dev, double dtime)
 {
     if (!ctx || !col || !pch || !can || !sf || nc < 0 || np < 0 || check_canopy_layout(ctx, lay) ||
@@ -1272,6 +1348,10 @@ int ctsm_land_step_full_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_co
         return CTSM_E_ARG;
     if (wt && (!wt_complete(wt) || (sw && (sw->zwt != wt->zwt || sw->zwt_perched != wt->zwt_perched ||
                                            sw->frost_table != wt->frost_table))))
+        return CTSM_E_ARG;
+    if (dr && (!hprm || !dr_complete(dr) || (wt && (dr->zwt != wt->zwt || dr->zwt_perched != wt->zwt_perched ||
+                                                    dr->frost_table != wt->frost_table)) ||
+               (sw && (dr->qflx_surf != sw->qflx_surf || dr->topo_slope != sw->topo_slope))))
         return CTSM_E_ARG;
     // the arrays one routine writes and the next reads must coincide
     if (pch->dlrad != can->dlrad || pch->cgrnd != can->cgrnd || pch->eflx_sh_grnd != can->eflx_sh_grnd ||
@@ -1310,6 +1390,7 @@ int ctsm_land_step_full_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_co
                           wt->qflx_solidevap_from_top_layer, nc, filterc_dev, ctx->err);
         launch_water_table(ctx->stream, *lay, ctx->cfg, *col, *wt, nc, filterc_dev, dtime, ctx->err);
     }
+    if (dr) launch_drainage(ctx->stream, *lay, ctx->cfg, *hprm, *col, *dr, nc, filterc_dev, dtime, ctx->err);
     return finish_launch(ctx);
 }
 

@@ -153,8 +153,6 @@ int ckpt_write(hipStream_t st, const char *path, const ctsm_ckpt_field *fields, 
         memcpy(e + kNameLen + 16, &sum[i], 8);
     }
     if (!seek(fp.f, 0) || fwrite(hdr.data(), 1, hdr.size(), fp.f) != hdr.size()) return CTSM_E_IO;
-    // the file spans to the end of the last range
-    if (pos > hdr.size() && (!seek(fp.f, pos - 1) || fputc(0, fp.f) == EOF)) return CTSM_E_IO;
     if (fflush(fp.f) != 0) return CTSM_E_IO;
     return CTSM_OK;
 }

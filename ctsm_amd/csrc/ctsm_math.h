@@ -396,10 +396,12 @@ CM_FN double cm_atan(double x)
     return x < 0.0 ? -res : res;
 }
 
-/* x**y for x > 0 (the only case the physics uses); x == 0 and x < 0 follow C. */
+/* x**y for x > 0 (the only case the physics uses); x == 0 and x < 0 follow C;
+ * x**0 = 1 and x**1 = x exactly, as a C pow gives them. */
 CM_FN double cm_pow(double x, double y)
 {
     if (y == 0.0) return 1.0;
+    if (y == 1.0) return x;
     if (!(x > 0.0) || x == HUGE_VAL || y != y) return pow(x, y);
     if (x == 1.0) return 1.0;
     double lh, ll;
@@ -421,33 +423,65 @@ CM_FN double cm_pow(double x, double y)
 #define CM_PIO2_2 0x1.0b4611a600000p-34
 #define CM_PIO2_3 0x1.3198a2e037073p-69
 #define CM_2_PI 0x1.45f306dc9c883p-1
+/* x = k pi/2 + r + rl; returns k mod 4 */
+CM_FN int cm__reduce_pio2(double x, double *r, double *rl)
+{
+    const double kd = rint(x * CM_2_PI);
+    const double t = (x - kd * CM_PIO2_1) - kd * CM_PIO2_2;
+    const double p3 = kd * CM_PIO2_3;
+    *r = t - p3;
+    *rl = (t - *r) - p3;
+    return ((int)kd) & 3;
+}
+
+/* sin(r + rl) and cos(r + rl) for |r| <= pi/4 */
+CM_FN double cm__sin_r(double r, double rl)
+{
+    const double z = r * r;
+    const double S = -0x1.5555555555555p-3 + z * (0x1.1111111111111p-7 + z * (-0x1.a01a01a01a01ap-13
+                      + z * (0x1.71de3a556c734p-19 + z * (-0x1.ae64567f544e4p-26 + z * (0x1.6124613a86d09p-33
+                      + z * (-0x1.ae7f3e733b81fp-41 + z * 0x1.952c77030ad4ap-49))))));
+    return r + (r * (z * S) + rl);
+}
+
+CM_FN double cm__cos_r(double r, double rl)
+{
+    const double z = r * r;
+    const double C4 = 0x1.5555555555555p-5 + z * (-0x1.6c16c16c16c17p-10 + z * (0x1.a01a01a01a01ap-16
+                       + z * (-0x1.27e4fb7789f5cp-22 + z * (0x1.1eed8eff8d898p-29 + z * (-0x1.93974a8c07c9dp-37
+                       + z * (0x1.ae7f3e733b81fp-45 + z * -0x1.6827863b97d97p-53))))));
+    const double hz = 0.5 * z;
+    const double w = 1.0 - hz;
+    return w + (((1.0 - w) - hz) + (z * (z * C4) - r * rl));
+}
+
 CM_FN double cm_sin(double x)
 {
     if (x != x || fabs(x) == HUGE_VAL) return x - x;
     if (x == 0.0) return x;
     if (!(fabs(x) < 0x1.0p+20)) return sin(x);
-    const double kd = rint(x * CM_2_PI);
-    const double t = (x - kd * CM_PIO2_1) - kd * CM_PIO2_2;
-    const double p3 = kd * CM_PIO2_3;
-    const double r = t - p3;
-    const double rl = (t - r) - p3;
-    const double z = r * r;
-    const int q = ((int)kd) & 3;
-    double v;
-    if ((q & 1) == 0) {
-        const double S = -0x1.5555555555555p-3 + z * (0x1.1111111111111p-7 + z * (-0x1.a01a01a01a01ap-13
-                          + z * (0x1.71de3a556c734p-19 + z * (-0x1.ae64567f544e4p-26 + z * (0x1.6124613a86d09p-33
-                          + z * (-0x1.ae7f3e733b81fp-41 + z * 0x1.952c77030ad4ap-49))))));
-        v = r + (r * (z * S) + rl);
-    } else {
-        const double C4 = 0x1.5555555555555p-5 + z * (-0x1.6c16c16c16c17p-10 + z * (0x1.a01a01a01a01ap-16
-                           + z * (-0x1.27e4fb7789f5cp-22 + z * (0x1.1eed8eff8d898p-29 + z * (-0x1.93974a8c07c9dp-37
-                           + z * (0x1.ae7f3e733b81fp-45 + z * -0x1.6827863b97d97p-53))))));
-        const double hz = 0.5 * z;
-        const double w = 1.0 - hz;
-        v = w + (((1.0 - w) - hz) + (z * (z * C4) - r * rl));
-    }
+    double r, rl;
+    const int q = cm__reduce_pio2(x, &r, &rl);
+    const double v = (q & 1) == 0 ? cm__sin_r(r, rl) : cm__cos_r(r, rl);
     return (q & 2) ? -v : v;
+}
+
+/* tan(x) for |x| < 2^20 (the drainage takes it of slopes in radians): the
+ * quotient of the reduced sine and cosine above, sin r / cos r or
+ * -cos r / sin r by quadrant, the division's rounding corrected once (within
+ * 2.5 ulp: the reduced sine and cosine each carry up to half an ulp). */
+CM_FN double cm_tan(double x)
+{
+    if (x != x || fabs(x) == HUGE_VAL) return x - x;
+    if (x == 0.0) return x;
+    if (!(fabs(x) < 0x1.0p+20)) return tan(x);
+    double r, rl;
+    const int q = cm__reduce_pio2(x, &r, &rl);
+    const double s = cm__sin_r(r, rl), c = cm__cos_r(r, rl);
+    const double num = (q & 1) == 0 ? s : c, den = (q & 1) == 0 ? c : s;
+    double t = num / den;
+    t = t + fma(-t, den, num) / den;    /* the quotient's own rounding corrected */
+    return (q & 1) == 0 ? t : -t;
 }
 
 #endif /* CTSM_MATH_H */

@@ -16,6 +16,12 @@
 // ThetaBasedWaterTable (:1933-2025) and RenewCondensation (:2569-2678), one
 // lane per column, each a search over the column's layers.
 //
+// k_drainage: the soil-column part of HydrologyDrainage without the aquifer
+// layer or hillslopes (HydrologyDrainageMod.F90:137-149, 160-245):
+// PerchedLateralFlow (SoilHydrologyMod.F90:1645-1929), SubsurfaceLateralFlow
+// (:2029-2565), the volumetric water and the runoff sum; one lane per column,
+// the routines' column loops run in order inside the lane.
+//
 // k_patch2col: column values of three patch fluxes as clm_drv_patch2col forms
 // them (p2c_1d_filter, subgridAveMod.F90:312-318), for the land step: the
 // evaporation fluxes SetQflxInputs reads and the dew / sublimation fluxes
@@ -28,7 +34,7 @@ namespace hydro {
 constexpr double RPI = 3.14159265358979323846;
 
 struct HView {
-    int begc, ldc, begp, ldp, nlevsno, nlevsoi;
+    int begc, ldc, begp, ldp, nlevsno, nlevsoi, nlevgrnd;
     __device__ __forceinline__ long c1(int c) const { return (long)(c - begc); }
     __device__ __forceinline__ long soi(int c, int j) const { return (long)(c - begc) + (long)(j - 1) * ldc; }
     __device__ __forceinline__ long sno(int c, int j) const { return (long)(c - begc) + (long)(j + nlevsno - 1) * ldc; }
@@ -247,6 +253,199 @@ __global__ void __launch_bounds__(256) k_water_table(HView V, ctsm_col_state S, 
     }
 }
 
+constexpr double WATMIN = 0.01;    // clm_varcon.F90:81
+constexpr double PONDMX = 0.0;     // clm_varcon.F90:116
+
+// specific yield of layer k at water table depth zw (SoilHydrologyMod.F90:1904-1906, 2459-2461)
+__device__ __forceinline__ double spec_yield(const HView &V, const ctsm_col_state &S, const ctsm_hydro_params &P,
+                                             int c, int k, double zw)
+{
+    const long s = V.soi(c, k);
+    const double sy = S.watsat[s] * (1. - cm_pow(1. + 1.e3 * zw / S.sucsat[s], -(1. / S.bsw[s])));
+    return fmax90(sy, P.aq_sp_yield_min);
+}
+
+__global__ void __launch_bounds__(256) k_drainage(HView V, double e_ice, ctsm_hydro_params P, ctsm_col_state S,
+                                                  ctsm_drainage_state D, int n, const int32_t *__restrict__ filter,
+                                                  double dtime, ErrWord *err)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = filter ? filter[i] : V.begc + i;
+    if (c < V.begc || c >= V.begc + V.ldc) { report_error(err, c, CTSM_E_ARG); return; }
+    const long cc = V.c1(c);
+    const int nlevsoi = V.nlevsoi;
+    const int nb = S.nbedrock[cc];
+    double *liq = S.h2osoi_liq, *ice = S.h2osoi_ice;
+    auto L = [&](int k) -> double & { return liq[V.sno(c, k)]; };
+    auto Z = [&](int k) { return S.zi[V.zi(c, k)]; };
+    auto DZ = [&](int k) { return S.dz[V.sno(c, k)]; };
+
+    // ---- PerchedLateralFlow (no hillslope)
+    const double frost = D.frost_table[cc], zwtp = D.zwt_perched[cc];
+    int k_frost = nb, k_perch = nb;
+    for (int k = 1; k <= nb; ++k)
+        if (frost >= Z(k - 1) && frost < Z(k)) { k_frost = k; break; }
+    for (int k = 1; k <= nb; ++k)
+        if (zwtp >= Z(k - 1) && zwtp < Z(k)) { k_perch = k; break; }
+    double perched_out = 0.0;
+    if (frost > zwtp) {
+        const double q_perch_max = P.perched_baseflow_scalar * cm_sin(D.topo_slope[cc] * (RPI / 180.0));
+        double wtsub = 0.0, q_perch = 0.0;
+        for (int k = k_perch; k <= k_frost - 1; ++k) {
+            q_perch = q_perch + S.hksat[V.soi(c, k)] * DZ(k);
+            wtsub = wtsub + DZ(k);
+        }
+        if (wtsub > 0.0) q_perch = q_perch / wtsub;
+        perched_out = q_perch_max * q_perch * (frost - zwtp);
+    }
+    double drain_perched = 0.0 + perched_out;
+    double tot = drain_perched * dtime;
+    for (int k = k_perch; k <= k_frost - 1; ++k) {
+        const double s_y = spec_yield(V, S, P, c, k, zwtp);
+        double dl = (k == k_perch) ? fmin90(tot, (s_y * (Z(k) - zwtp) * 1.e3)) : fmin90(tot, (s_y * DZ(k) * 1.e3));
+        dl = fmax90(dl, 0.0);
+        tot = tot - dl;
+        L(k) = L(k) - dl;
+    }
+    drain_perched = drain_perched - tot / dtime;
+
+    // ---- SubsurfaceLateralFlow (no hillslope, lateral flow on)
+    double dzsum = 0.0, icefracsum = 0.0;
+    double zwt = D.zwt[cc];
+    int jwt = nlevsoi;
+    for (int j = 1; j <= nlevsoi; ++j) {
+        if (zwt <= Z(j)) { jwt = j - 1; break; }
+    }
+    for (int j = 1; j <= nlevsoi; ++j) {
+        const long s = V.soi(c, j);
+        const double vol_ice = fmin90(S.watsat[s], ice[V.sno(c, j)] / (DZ(j) * DENICE));
+        const double f = fmin90(1.0, vol_ice / S.watsat[s]);
+        S.icefrac[s] = f;
+        if (j >= (jwt > 1 ? jwt : 1)) {
+            const double dzmm = DZ(j) * 1.e3;
+            dzsum = dzsum + dzmm;
+            icefracsum = icefracsum + f * dzmm;
+        }
+    }
+    const double ice_imped_col = cm_pow(10.0, -e_ice * (icefracsum / dzsum));
+    const double zinb = Z(nb);
+    double latflow_out = 0.0;
+    if (zwt <= zinb)
+        latflow_out = ice_imped_col * P.baseflow_scalar * cm_tan(RPI / 180.0 * D.topo_slope[cc]) *
+                      cm_pow(zinb - zwt, P.n_baseflow);
+    const double latflow_in = 0.0;
+    const double net = latflow_out - latflow_in;
+    double drainage = (zwt <= zinb) ? net : 0.0;
+    tot = -drainage * dtime;
+    double h2osfc = S.h2osfc[cc];
+    if (tot > 0.) {          // rising water table
+        for (int j = jwt + 1; j >= 1; --j) {
+            if (Z(j) < frost) {
+                const double s_y = spec_yield(V, S, P, c, j, zwt);
+                double dl = fmin90(tot, (s_y * DZ(j) * 1.e3));
+                dl = fmax90(dl, 0.0);
+                L(j) = L(j) + dl;
+                tot = tot - dl;
+                if (tot <= 0.) {
+                    zwt = zwt - dl / s_y / 1000.0;
+                    break;
+                } else {
+                    zwt = Z(j - 1);
+                }
+            }
+        }
+        h2osfc = h2osfc + tot;
+    } else {                 // deepening water table
+        for (int j = jwt + 1; j <= nb; ++j) {
+            const double s_y = spec_yield(V, S, P, c, j, zwt);
+            double dl = fmax90(tot, -(s_y * (Z(j) - zwt) * 1.e3));
+            dl = fmin90(dl, 0.0);
+            L(j) = L(j) + dl;
+            tot = tot - dl;
+            if (tot >= 0.) {
+                zwt = zwt - dl / s_y / 1000.0;
+                break;
+            } else {
+                zwt = Z(j);
+            }
+        }
+        drainage = drainage + tot / dtime;
+    }
+    zwt = fmax90(0.0, zwt);
+    zwt = fmin90(80.0, zwt);
+
+    // saturation excess pushed up, layer by layer
+    for (int j = nlevsoi; j >= 2; --j) {
+        const double cap = S.eff_porosity[V.soi(c, j)] * (DZ(j) * 1.e3);
+        const double xsi = fmax90(L(j) - cap, 0.0);
+        L(j) = fmin90(cap, L(j));
+        L(j - 1) = L(j - 1) + xsi;
+    }
+    // the top layer's excess liquid to the surface water, excess ice to runoff
+    const double room = PONDMX + S.watsat[V.soi(c, 1)] * (DZ(1) * 1.e3);
+    double &ice1 = ice[V.sno(c, 1)];
+    double xs1 = fmax90(fmax90(L(1) - WATMIN, 0.0) - fmax90(0.0, (room - ice1 - WATMIN)), 0.0);
+    L(1) = L(1) - xs1;
+    h2osfc = h2osfc + xs1;
+    double rsub_sat = 0.0;
+    xs1 = fmax90(fmax90(ice1, 0.0) - fmax90(0.0, (room - L(1))), 0.0);
+    ice1 = fmin90(fmax90(0.0, room - L(1)), ice1);
+    const double ice_runoff_xs = xs1 / dtime;
+    // no layer below the minimum water: the deficit comes from the layer below
+    for (int j = 1; j <= nlevsoi - 1; ++j) {
+        double xs;
+        if (L(j) < WATMIN) {
+            xs = WATMIN - L(j);
+            if (j == jwt) zwt = zwt + xs / S.eff_porosity[V.soi(c, j)] / 1000.0;
+        } else {
+            xs = 0.0;
+        }
+        L(j) = L(j) + xs;
+        L(j + 1) = L(j + 1) - xs;
+    }
+    // ... and the bottom layer's from the layers above, the rest from outside
+    {
+        const int j = nlevsoi;
+        double xs = 0.0;
+        if (L(j) < WATMIN) {
+            xs = WATMIN - L(j);
+            for (int k = nlevsoi - 1; k >= 1; --k) {
+                const double avail = fmax90(L(k) - WATMIN - xs, 0.0);
+                if (avail >= xs) {
+                    L(j) = L(j) + xs;
+                    L(k) = L(k) - xs;
+                    xs = 0.0;
+                    break;
+                } else {
+                    L(j) = L(j) + avail;
+                    L(k) = L(k) - avail;
+                    xs = xs - avail;
+                }
+            }
+        }
+        L(j) = L(j) + xs;
+        rsub_sat = rsub_sat - xs / dtime;
+    }
+    const double qdrain = rsub_sat + drainage;
+    const double qrgwl = D.qflx_snwcp_liq[cc];
+
+    // HydrologyDrainage: volumetric water, runoff
+    if (D.h2osoi_vol)
+        for (int j = 1; j <= V.nlevgrnd; ++j)
+            D.h2osoi_vol[V.soi(c, j)] = L(j) / (DZ(j) * DENH2O) + ice[V.sno(c, j)] / (DZ(j) * DENICE);
+    S.h2osfc[cc] = h2osfc;
+    D.zwt[cc] = zwt;
+    D.qflx_drain_perched[cc] = drain_perched;
+    D.qflx_latflow_out[cc] = latflow_out;
+    D.qflx_latflow_in[cc] = latflow_in;
+    D.qflx_rsub_sat[cc] = rsub_sat;
+    D.qflx_ice_runoff_xs[cc] = ice_runoff_xs;
+    D.qflx_drain[cc] = qdrain;
+    D.qflx_qrgwl[cc] = qrgwl;
+    D.qflx_runoff[cc] = qdrain + D.qflx_surf[cc] + qrgwl + drain_perched;
+}
+
 // p2c_1d_filter of three patch fluxes (clm_drv_patch2col, subgridAveMod.F90:312-318)
 __global__ void __launch_bounds__(256) k_patch2col(HView V, ctsm_col_state S, const int32_t *__restrict__ active,
                                                    const double *__restrict__ wtcol, const double *__restrict__ pa,
@@ -282,7 +481,7 @@ void launch_surface_water(hipStream_t st, const ctsm_layout &L, double e_ice, co
                           double dtime, ErrWord *err)
 {
     if (n <= 0) return;
-    const hydro::HView V{L.begc_alloc, L.ldc, L.begp_alloc, L.ldp, 0, 0};
+    const hydro::HView V{L.begc_alloc, L.ldc, L.begp_alloc, L.ldp, 0, 0, 0};
     hipLaunchKernelGGL(hydro::k_surface_water, dim3((n + 255) / 256), dim3(256), 0, st, V, e_ice, P, S, W, n, filter,
                        dtime, err);
 }
@@ -292,7 +491,7 @@ void launch_patch2col3(hipStream_t st, const ctsm_layout &L, const ctsm_col_stat
                        double *cb, double *ccol, int n, const int32_t *filter, ErrWord *err)
 {
     if (n <= 0) return;
-    const hydro::HView V{L.begc_alloc, L.ldc, L.begp_alloc, L.ldp, 0, 0};
+    const hydro::HView V{L.begc_alloc, L.ldc, L.begp_alloc, L.ldp, 0, 0, 0};
     hipLaunchKernelGGL(hydro::k_patch2col, dim3((n + 255) / 256), dim3(256), 0, st, V, S, active, wtcol, pa, pb, pc,
                        ca, cb, ccol, n, filter, err);
 }
@@ -301,8 +500,18 @@ void launch_water_table(hipStream_t st, const ctsm_layout &L, const ctsm_config 
                         const ctsm_water_table_state &W, int n, const int32_t *filter, double dtime, ErrWord *err)
 {
     if (n <= 0) return;
-    const hydro::HView V{L.begc_alloc, L.ldc, L.begp_alloc, L.ldp, cfg.nlevsno, cfg.nlevsoi};
+    const hydro::HView V{L.begc_alloc, L.ldc, L.begp_alloc, L.ldp, cfg.nlevsno, cfg.nlevsoi, cfg.nlevgrnd};
     hipLaunchKernelGGL(hydro::k_water_table, dim3((n + 255) / 256), dim3(256), 0, st, V, S, W, n, filter, dtime, err);
+}
+
+void launch_drainage(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg, const ctsm_hydro_params &P,
+                     const ctsm_col_state &S, const ctsm_drainage_state &D, int n, const int32_t *filter,
+                     double dtime, ErrWord *err)
+{
+    if (n <= 0) return;
+    const hydro::HView V{L.begc_alloc, L.ldc, L.begp_alloc, L.ldp, cfg.nlevsno, cfg.nlevsoi, cfg.nlevgrnd};
+    hipLaunchKernelGGL(hydro::k_drainage, dim3((n + 255) / 256), dim3(256), 0, st, V, cfg.e_ice, P, S, D, n, filter,
+                       dtime, err);
 }
 
 }  // namespace ctsm

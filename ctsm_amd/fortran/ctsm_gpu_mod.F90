@@ -181,9 +181,20 @@ module ctsm_gpu_mod
      type(c_ptr) :: qflx_solidevap_from_top_layer = c_null_ptr
   end type ctsm_water_table_state_t
 
-  ! fff, pc, mu (param file) and h2osfcflag, as ctsm_hydro_params
+  ! Drainage state (member order and meaning as ctsm_drainage_state)
+  type, bind(C), public :: ctsm_drainage_state_t
+     type(c_ptr) :: topo_slope = c_null_ptr, zwt_perched = c_null_ptr, frost_table = c_null_ptr
+     type(c_ptr) :: qflx_surf = c_null_ptr, qflx_snwcp_liq = c_null_ptr, zwt = c_null_ptr
+     type(c_ptr) :: h2osoi_vol = c_null_ptr
+     type(c_ptr) :: qflx_drain_perched = c_null_ptr, qflx_latflow_out = c_null_ptr, qflx_latflow_in = c_null_ptr
+     type(c_ptr) :: qflx_rsub_sat = c_null_ptr, qflx_ice_runoff_xs = c_null_ptr, qflx_drain = c_null_ptr
+     type(c_ptr) :: qflx_qrgwl = c_null_ptr, qflx_runoff = c_null_ptr
+  end type ctsm_drainage_state_t
+
+  ! fff, pc, mu, the drainage parameters (param file) and h2osfcflag, as ctsm_hydro_params
   type, bind(C), public :: ctsm_hydro_params_t
      real(c_double) :: fff, pc, mu
+     real(c_double) :: aq_sp_yield_min, n_baseflow, baseflow_scalar, perched_baseflow_scalar
      integer(c_int32_t) :: h2osfcflag
      integer(c_int32_t) :: pad = 0
   end type ctsm_hydro_params_t
@@ -191,7 +202,7 @@ module ctsm_gpu_mod
   public :: ctsm_config_default, ctsm_create, ctsm_destroy, ctsm_last_error
   public :: ctsm_soilwater, ctsm_soiltemperature, ctsm_soilwater_fractions
   public :: ctsm_set_canopy_params, ctsm_canopyfluxes, ctsm_root_sink
-  public :: ctsm_soilfluxes, ctsm_soil_balance_check, ctsm_surface_water, ctsm_water_table
+  public :: ctsm_soilfluxes, ctsm_soil_balance_check, ctsm_surface_water, ctsm_water_table, ctsm_drainage
 
   interface
      subroutine ctsm_config_default(cfg) bind(C, name='ctsm_config_default')
@@ -332,6 +343,20 @@ module ctsm_gpu_mod
        integer(c_int32_t), intent(in) :: filter_hydrologyc(*)
        real(c_double), value :: dtime
      end function ctsm_water_table
+
+     integer(c_int) function ctsm_drainage(ctx, lay, col, dr, prm, num_hydrologyc, filter_hydrologyc, dtime) &
+          bind(C, name='ctsm_drainage')
+       import :: c_int, c_int32_t, c_double, c_ptr, ctsm_layout_t, ctsm_col_state_t, ctsm_drainage_state_t, &
+            ctsm_hydro_params_t
+       type(c_ptr), value :: ctx
+       type(ctsm_layout_t), intent(in) :: lay
+       type(ctsm_col_state_t), intent(in) :: col
+       type(ctsm_drainage_state_t), intent(in) :: dr
+       type(ctsm_hydro_params_t), intent(in) :: prm
+       integer(c_int32_t), value :: num_hydrologyc
+       integer(c_int32_t), intent(in) :: filter_hydrologyc(*)
+       real(c_double), value :: dtime
+     end function ctsm_drainage
   end interface
 
 end module ctsm_gpu_mod

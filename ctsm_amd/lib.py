@@ -36,7 +36,7 @@ EXPORTS = [
     "ctsm_land_step_full_dev", "ctsm_soil_steps_fused_dev", "ctsm_surface_water", "ctsm_surface_water_dev",
     "ctsm_water_table", "ctsm_water_table_dev",
     "ctsm_hist_update_dev", "ctsm_hist_normalize_dev", "ctsm_hist_zero_dev",
-    "ctsm_checkpoint_write", "ctsm_checkpoint_read",
+    "ctsm_checkpoint_write", "ctsm_checkpoint_read", "ctsm_drainage", "ctsm_drainage_dev",
 ]
 
 
@@ -82,7 +82,7 @@ def load():
     L.ctsm_soil_balance_check_dev.argtypes = bal
     L.ctsm_land_step_full_dev.argtypes = col + [P(abi.PatchState), P(abi.CanopyState), P(abi.SoilFluxState),
                                                 P(abi.SurfaceWaterState), P(abi.HydroParams),
-                                                P(abi.WaterTableState),
+                                                P(abi.WaterTableState), P(abi.DrainageState),
                                                 C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_double]
     swa = col + [P(abi.SurfaceWaterState), P(abi.HydroParams), C.c_int32, C.c_void_p, C.c_double]
     L.ctsm_surface_water.argtypes = swa
@@ -90,6 +90,9 @@ def load():
     wta = col + [P(abi.WaterTableState), C.c_int32, C.c_void_p, C.c_double]
     L.ctsm_water_table.argtypes = wta
     L.ctsm_water_table_dev.argtypes = wta
+    dra = col + [P(abi.DrainageState), P(abi.HydroParams), C.c_int32, C.c_void_p, C.c_double]
+    L.ctsm_drainage.argtypes = dra
+    L.ctsm_drainage_dev.argtypes = dra
     for fn in ("ctsm_hist_update_dev", "ctsm_hist_normalize_dev", "ctsm_hist_zero_dev"):
         getattr(L, fn).argtypes = [C.c_void_p, P(abi.HistField), C.c_int32]
     for fn in ("ctsm_checkpoint_write", "ctsm_checkpoint_read"):
@@ -284,7 +287,7 @@ class Context:
         return m.value, idx.value
 
     def land_step_full_dev(self, ds, filterc_dev=None, nc=None, filterp_dev=None, np_=None, dtime=1800.0,
-                           structs=None, hydro=None, water_table=False):
+                           structs=None, hydro=None, water_table=False, drainage=False):
         """Land step with the before-step save, SoilFluxes and, given hydro
         (HydroParams), the surface water input chain; with water_table, the
         water tables and RenewCondensation after SoilWater (enqueues only)."""
@@ -295,6 +298,7 @@ class Context:
             cs, ps, can, sf = structs
         sw = ds.surfwater_struct() if hydro is not None else None
         wt = ds.watertable_struct() if water_table else None
+        dr = ds.drainage_struct() if drainage else None
         nc = ds.dims.ncol if nc is None else nc
         np_ = ds.dims.npatch if np_ is None else np_
         fc = C.c_void_p(filterc_dev) if filterc_dev is not None else None
@@ -304,6 +308,7 @@ class Context:
                                                    C.byref(sw) if sw is not None else None,
                                                    C.byref(hydro) if hydro is not None else None,
                                                    C.byref(wt) if wt is not None else None,
+                                                   C.byref(dr) if dr is not None else None,
                                                    nc, fc, np_, fp, dtime), "land step (full)")
 
     def surface_water(self, st, hydro, filter_c=None, dtime=1800.0):
@@ -337,6 +342,22 @@ class Context:
         fp = C.c_void_p(filter_dev) if filter_dev is not None else None
         self._check(self.L.ctsm_water_table_dev(self.h, C.byref(ds.layout()), C.byref(cs), C.byref(wt), n, fp,
                                                 dtime), "water table")
+
+    def drainage(self, st, hydro, filter_c=None, dtime=1800.0):
+        """Drop-in drainage of HydrologyDrainage on a host state."""
+        cs, _ = st.structs()
+        dr = st.drainage_struct()
+        f, fp, n = _filter(filter_c, st.dims.ncol)
+        self._check(self.L.ctsm_drainage(self.h, C.byref(st.layout()), C.byref(cs), C.byref(dr), C.byref(hydro),
+                                         n, fp, dtime), "drainage")
+
+    def drainage_dev(self, ds, hydro, filter_dev=None, n=None, dtime=1800.0):
+        cs, _ = ds.structs()
+        dr = ds.drainage_struct()
+        n = ds.dims.ncol if n is None else n
+        fp = C.c_void_p(filter_dev) if filter_dev is not None else None
+        self._check(self.L.ctsm_drainage_dev(self.h, C.byref(ds.layout()), C.byref(cs), C.byref(dr),
+                                             C.byref(hydro), n, fp, dtime), "drainage")
 
     def hist_update(self, fields):
         """hist_update_hbuf over an array of abi.HistField (enqueues only)."""

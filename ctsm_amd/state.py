@@ -44,6 +44,9 @@ SURFWATER_OWN = [(n, t, k) for n, t, k in abi.SURFWATER_FIELDS if n not in SURFW
 # (kept in the same group); h2osoi_vol and the column dew fluxes are its own
 WATERTABLE_SHARED = ("zwt", "zwt_perched", "frost_table")
 WATERTABLE_OWN = [(n, t, k) for n, t, k in abi.WATERTABLE_FIELDS if n not in WATERTABLE_SHARED]
+# drainage fields: the surface water's and water table's are shared (same group)
+DRAINAGE_OWN = [(n, t, k) for n, t, k in abi.DRAINAGE_FIELDS
+                if n not in {f[0] for f in abi.SURFWATER_FIELDS + abi.WATERTABLE_FIELDS}]
 
 
 class Dims:
@@ -90,7 +93,7 @@ class HostState:
                     a[...] = np.nan
                 self.sf[name] = a
         self.hy = {}
-        for name, t, kind in SURFWATER_OWN + WATERTABLE_OWN:
+        for name, t, kind in SURFWATER_OWN + WATERTABLE_OWN + DRAINAGE_OWN:
             a = np.zeros(dims.shape(kind), dtype=_NP[t])
             if fill_nan and t == "f64":
                 a[...] = np.nan
@@ -158,6 +161,14 @@ class HostState:
             assert a.flags.c_contiguous
             setattr(wt, name, a.ctypes.data_as(abi._CT[t]))
         return wt
+
+    def drainage_struct(self):
+        dr = abi.DrainageState()
+        for name, t, _ in abi.DRAINAGE_FIELDS:
+            a = self.col[SURFWATER_ALIAS[name][1]] if name in SURFWATER_ALIAS else self.hy[name]
+            assert a.flags.c_contiguous
+            setattr(dr, name, a.ctypes.data_as(abi._CT[t]))
+        return dr
 
     def structs(self):
         """ctypes ColState / PatchState pointing at these numpy buffers."""
@@ -271,6 +282,13 @@ class DeviceState:
         for name, t, _ in abi.WATERTABLE_FIELDS:
             setattr(wt, name, C.cast(C.c_void_p(self.hy[name]), abi._CT[t]))
         return wt
+
+    def drainage_struct(self):
+        dr = abi.DrainageState()
+        for name, t, _ in abi.DRAINAGE_FIELDS:
+            ptr = self.col[SURFWATER_ALIAS[name][1]] if name in SURFWATER_ALIAS else self.hy[name]
+            setattr(dr, name, C.cast(C.c_void_p(ptr), abi._CT[t]))
+        return dr
 
     def structs(self):
         cs = abi.ColState()

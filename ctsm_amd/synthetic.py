@@ -297,11 +297,16 @@ def canopy_params():
                 jmax25top_sf=1.0, leaf_mr_vcm=0.015)
 
 
-def hydro_params(fff=0.5, pc=0.4, mu=0.14, h2osfcflag=1):
-    """Representative param-file scalars of the surface water chain (fff of
-    SaturatedExcessRunoffMod, pc and mu of SurfaceWaterMod) and h2osfcflag."""
+def hydro_params(fff=0.5, pc=0.4, mu=0.14, h2osfcflag=1, aq_sp_yield_min=0.01, n_baseflow=1.0,
+                 baseflow_scalar=5.5e-3, perched_baseflow_scalar=1.0e-5):
+    """Representative param-file scalars of the hydrology (fff of
+    SaturatedExcessRunoffMod, pc and mu of SurfaceWaterMod, the drainage
+    parameters of SoilHydrologyMod; the param file itself is not available
+    offline) and h2osfcflag."""
     from .abi import HydroParams
-    return HydroParams(fff, pc, mu, h2osfcflag, 0)
+    return HydroParams(fff=fff, pc=pc, mu=mu, aq_sp_yield_min=aq_sp_yield_min, n_baseflow=n_baseflow,
+                       baseflow_scalar=baseflow_scalar, perched_baseflow_scalar=perched_baseflow_scalar,
+                       h2osfcflag=h2osfcflag, pad=0)
 
 
 def pft_table():

@@ -435,11 +435,31 @@ typedef struct ctsm_water_table_state {
     double        *qflx_liqdew_to_top_layer, *qflx_soliddew_to_top_layer, *qflx_solidevap_from_top_layer;
 } ctsm_water_table_state;
 
-/* Param-file / namelist scalars of that chain: fff (SaturatedExcessRunoffMod
- * readParams), pc and mu (SurfaceWaterMod readParams), h2osfcflag
- * (SoilHydrologyType, 1 = surface water active). */
+/* ---- drainage (HydrologyDrainageMod.F90:127-245 without the aquifer layer) --
+ * Column arrays of PerchedLateralFlow (SoilHydrologyMod.F90:1645-1929) and
+ * SubsurfaceLateralFlow (:2029-2565) for soil / crop columns off hillslopes, and
+ * the runoff sum of HydrologyDrainage; dz, z, zi, watsat, bsw, sucsat, hksat,
+ * eff_porosity, nbedrock, h2osoi_liq, h2osoi_ice, h2osfc and icefrac (rewritten
+ * from the ice, as the reference does) come from ctsm_col_state.  zwt,
+ * zwt_perched, frost_table, topo_slope and qflx_surf are the same arrays as in
+ * the surface water and water-table states. */
+typedef struct ctsm_drainage_state {
+    const double  *topo_slope, *zwt_perched, *frost_table;   /* (c) in  */
+    const double  *qflx_surf, *qflx_snwcp_liq;               /* (c) in  (mm H2O/s) */
+    double        *zwt;                                      /* (c) inout (m) */
+    double        *h2osoi_vol;                               /* [SOI] out, may be NULL */
+    /* outputs (mm H2O/s) */
+    double        *qflx_drain_perched, *qflx_latflow_out, *qflx_latflow_in, *qflx_rsub_sat;
+    double        *qflx_ice_runoff_xs, *qflx_drain, *qflx_qrgwl, *qflx_runoff;
+} ctsm_drainage_state;
+
+/* Param-file / namelist scalars of the hydrology: fff (SaturatedExcessRunoffMod
+ * readParams), pc and mu (SurfaceWaterMod readParams), aq_sp_yield_min,
+ * n_baseflow, baseflow_scalar and perched_baseflow_scalar (SoilHydrologyMod
+ * readParams), h2osfcflag (SoilHydrologyType, 1 = surface water active). */
 typedef struct ctsm_hydro_params {
     double  fff, pc, mu;
+    double  aq_sp_yield_min, n_baseflow, baseflow_scalar, perched_baseflow_scalar;
     int32_t h2osfcflag;
     int32_t pad;
 } ctsm_hydro_params;
@@ -583,6 +603,15 @@ int ctsm_water_table(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state
                      const ctsm_water_table_state *wt, int32_t num_hydrologyc,
                      const int32_t *filter_hydrologyc, double dtime);
 
+/* Replaces the soil-column part of HydrologyDrainage without the aquifer layer
+ * or hillslopes (HydrologyDrainageMod.F90:137-149, 160-245): PerchedLateralFlow,
+ * SubsurfaceLateralFlow (baseflow from the water table, the water table's rise
+ * or fall, saturation excess pushed up, the minimum-water corrections), the
+ * volumetric water and qflx_runoff. */
+int ctsm_drainage(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                  const ctsm_drainage_state *dr, const ctsm_hydro_params *prm,
+                  int32_t num_hydrologyc, const int32_t *filter_hydrologyc, double dtime);
+
 /* ---- device-resident entry points -----------------------------------------
  * Same arguments, but every pointer (fields and filter) is device memory and
  * the call only enqueues work on the context's stream.  filter == NULL means
@@ -627,6 +656,9 @@ int ctsm_surface_water_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col
 int ctsm_water_table_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
                          const ctsm_water_table_state *wt, int32_t num_filterc, const int32_t *filterc_dev,
                          double dtime);
+int ctsm_drainage_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                      const ctsm_drainage_state *dr, const ctsm_hydro_params *prm,
+                      int32_t num_filterc, const int32_t *filterc_dev, double dtime);
 int ctsm_soilfluxes_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
                         const ctsm_patch_state *pch, const ctsm_soilflux_state *sf,
                         int32_t num_filterc, const int32_t *filterc_dev,
@@ -657,7 +689,8 @@ int ctsm_land_step_dev(ctsm_ctx *ctx, const ctsm_layout *lay,
  * chain that sets qflx_infl (HydrologyNoDrainageMod.F90:306-336), then
  * SoilWater and, when wt is not NULL, the column dew / sublimation fluxes
  * (clm_drv_patch2col), the water tables and RenewCondensation
- * (HydrologyNoDrainageMod.F90:355-373).  Arrays shared by two structs must be
+ * (HydrologyNoDrainageMod.F90:355-373) and, when dr is not NULL (it needs
+ * hprm), the drainage of HydrologyDrainage (clm_driver.F90:1148).  Arrays shared by two structs must be
  * the same device arrays, as for ctsm_land_step_dev, the CanopyFluxes outputs
  * SoilFluxes reads (sf->cgrnds, ...) must be the canopy state's,
  * sw->qflx_infl / sw->h2osfc must be col->qflx_infl / col->h2osfc, and
@@ -666,7 +699,7 @@ int ctsm_land_step_full_dev(ctsm_ctx *ctx, const ctsm_layout *lay,
                             const ctsm_col_state *col, const ctsm_patch_state *pch,
                             const ctsm_canopy_state *can, const ctsm_soilflux_state *sf,
                             const ctsm_surface_water_state *sw, const ctsm_hydro_params *hprm,
-                            const ctsm_water_table_state *wt,
+                            const ctsm_water_table_state *wt, const ctsm_drainage_state *dr,
                             int32_t num_filterc, const int32_t *filterc_dev,
                             int32_t num_filterp, const int32_t *filterp_dev, double dtime);
 

@@ -1114,3 +1114,5 @@ void oracle_cm_atan(int n, const double *x, double *out)
 { int i; for (i = 0; i < n; ++i) out[i] = cm_atan(x[i]); }
 void oracle_cm_sin(int n, const double *x, double *out)
 { int i; for (i = 0; i < n; ++i) out[i] = cm_sin(x[i]); }
+void oracle_cm_tan(int n, const double *x, double *out)
+{ int i; for (i = 0; i < n; ++i) out[i] = cm_tan(x[i]); }

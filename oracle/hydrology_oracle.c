@@ -31,6 +31,13 @@
  *   RenewCondensation        SoilHydrologyMod.F90:2569-2678 (soil / crop columns),
  *                            truncate_small_values with tolerance 1e-12 (:71)
  *
+ * and the soil-column drainage of HydrologyDrainage (HydrologyDrainageMod.F90:
+ * 137-149, 160-245, no aquifer layer, no hillslopes) in oracle_drainage:
+ *
+ *   PerchedLateralFlow       SoilHydrologyMod.F90:1645-1929
+ *   SubsurfaceLateralFlow    SoilHydrologyMod.F90:2029-2565 (lateral flow on)
+ *   h2osoi_vol, qflx_runoff  HydrologyDrainageMod.F90:160-168, 236-244
+ *
  * Pins (tests/test_oracle_hydrology.py): the water balance of the surface
  * (qflx_top_soil - evaporation terms = qflx_infl + qflx_surf + dh2osfc/dt over
  * the column), h2osfc never negative, qflx_infl never above the soil's
@@ -338,4 +345,258 @@ int oracle_water_table(const ctsm_config *cfg, const ctsm_layout *L, const ctsm_
     free(ice_before);
     free(modified);
     return rc;
+}
+
+/* ---- drainage (HydrologyDrainage, soil columns) ---------------------------- */
+
+#define H_WATMIN 0.01
+#define H_PONDMX 0.0
+
+static double h_spec_yield(const h_lay_t *Lp, const ctsm_col_state *S, const ctsm_hydro_params *P, int c, int k,
+                           double zw)
+{
+    const h_lay_t Lo = *Lp;
+    double s_y = HSOI(S->watsat, c, k) * (1. - cm_pow(1. + 1.e3 * zw / HSOI(S->sucsat, c, k), -(1. / HSOI(S->bsw, c, k))));
+    return h_max(s_y, P->aq_sp_yield_min);
+}
+
+int oracle_drainage(const ctsm_config *cfg, const ctsm_layout *L, const ctsm_col_state *S, ctsm_drainage_state *D,
+                    const ctsm_hydro_params *P, int num_c, const int *filter, double dtime)
+{
+    h_lay_t Lo = { L->begc_alloc, L->ldc, cfg->nlevsno };
+    const int nlevsoi = cfg->nlevsoi, nlevgrnd = cfg->nlevgrnd;
+    const int ncol = L->ldc > 0 ? L->ldc : 1;
+    int *k_frost = (int *)calloc((size_t)ncol, sizeof(int));
+    int *k_perch = (int *)calloc((size_t)ncol, sizeof(int));
+    int *jwt = (int *)calloc((size_t)ncol, sizeof(int));
+    double *out = (double *)calloc((size_t)ncol, sizeof(double));
+    double *icef = (double *)calloc((size_t)ncol * (size_t)nlevsoi, sizeof(double));
+    double *imped_col = (double *)calloc((size_t)ncol, sizeof(double));
+    double *drainage = (double *)calloc((size_t)ncol, sizeof(double));
+    int fc, k, j, i;
+    if (!k_frost || !k_perch || !jwt || !out || !icef || !imped_col || !drainage) return 1;
+#define COL(fc) (filter ? filter[fc] : L->begc_alloc + (fc))
+#define IX(c) ((c) - Lo.begc)
+#define LIQ(c, k) HSNO(S->h2osoi_liq, c, k)
+#define ICE(c, k) HSNO(S->h2osoi_ice, c, k)
+#define DZ(c, k) HSNO(S->dz, c, k)
+#define ZI(c, k) HZI(S->zi, c, k)
+
+    /* ---- PerchedLateralFlow */
+    for (fc = 0; fc < num_c; ++fc) {
+        const int c = COL(fc), nb = HC(S->nbedrock, c);
+        k_frost[IX(c)] = nb;
+        k_perch[IX(c)] = nb;
+        for (k = 1; k <= nb; ++k) {
+            if (HC(D->frost_table, c) >= ZI(c, k - 1) && HC(D->frost_table, c) < ZI(c, k)) { k_frost[IX(c)] = k; break; }
+        }
+        for (k = 1; k <= nb; ++k) {
+            if (HC(D->zwt_perched, c) >= ZI(c, k - 1) && HC(D->zwt_perched, c) < ZI(c, k)) { k_perch[IX(c)] = k; break; }
+        }
+    }
+    for (fc = 0; fc < num_c; ++fc) {
+        const int c = COL(fc);
+        HC(D->qflx_drain_perched, c) = 0.0;
+        out[IX(c)] = 0.0;
+        if (HC(D->frost_table, c) > HC(D->zwt_perched, c)) {
+            const double q_perch_max = P->perched_baseflow_scalar * cm_sin(HC(D->topo_slope, c) * (H_PI / 180.0));
+            double wtsub = 0.0, q_perch = 0.0;
+            for (k = k_perch[IX(c)]; k <= k_frost[IX(c)] - 1; ++k) {
+                q_perch = q_perch + HSOI(S->hksat, c, k) * DZ(c, k);
+                wtsub = wtsub + DZ(c, k);
+            }
+            if (wtsub > 0.0) q_perch = q_perch / wtsub;
+            out[IX(c)] = q_perch_max * q_perch * (HC(D->frost_table, c) - HC(D->zwt_perched, c));
+        }
+    }
+    for (fc = 0; fc < num_c; ++fc) {
+        const int c = COL(fc);
+        HC(D->qflx_drain_perched, c) = HC(D->qflx_drain_perched, c) + out[IX(c)];
+    }
+    for (fc = 0; fc < num_c; ++fc) {
+        const int c = COL(fc);
+        double tot = HC(D->qflx_drain_perched, c) * dtime;
+        for (k = k_perch[IX(c)]; k <= k_frost[IX(c)] - 1; ++k) {
+            const double s_y = h_spec_yield(&Lo, S, P, c, k, HC(D->zwt_perched, c));
+            double dl;
+            if (k == k_perch[IX(c)]) dl = h_min(tot, (s_y * (ZI(c, k) - HC(D->zwt_perched, c)) * 1.e3));
+            else dl = h_min(tot, (s_y * (DZ(c, k)) * 1.e3));
+            dl = h_max(dl, 0.0);
+            tot = tot - dl;
+            LIQ(c, k) = LIQ(c, k) - dl;
+        }
+        HC(D->qflx_drain_perched, c) = HC(D->qflx_drain_perched, c) - tot / dtime;
+    }
+
+    /* ---- SubsurfaceLateralFlow */
+    for (j = 1; j <= nlevsoi; ++j) {
+        for (fc = 0; fc < num_c; ++fc) {
+            const int c = COL(fc);
+            const double vol_ice = h_min(HSOI(S->watsat, c, j), ICE(c, j) / (DZ(c, j) * H_DENICE));
+            icef[(size_t)IX(c) * nlevsoi + (j - 1)] = h_min(1.0, vol_ice / HSOI(S->watsat, c, j));
+            HSOI(S->icefrac, c, j) = icef[(size_t)IX(c) * nlevsoi + (j - 1)];
+        }
+    }
+    for (fc = 0; fc < num_c; ++fc) {
+        const int c = COL(fc);
+        HC(D->qflx_drain, c) = 0.0;
+        HC(D->qflx_rsub_sat, c) = 0.0;
+        drainage[IX(c)] = 0.0;
+        HC(D->qflx_latflow_in, c) = 0.0;
+        HC(D->qflx_latflow_out, c) = 0.0;
+    }
+    for (fc = 0; fc < num_c; ++fc) {
+        const int c = COL(fc);
+        jwt[IX(c)] = nlevsoi;
+        for (j = 1; j <= nlevsoi; ++j) {
+            if (HC(D->zwt, c) <= ZI(c, j)) { jwt[IX(c)] = j - 1; break; }
+        }
+    }
+    for (fc = 0; fc < num_c; ++fc) {
+        const int c = COL(fc);
+        double dzsum = 0.0, icefracsum = 0.0;
+        for (j = (jwt[IX(c)] > 1 ? jwt[IX(c)] : 1); j <= nlevsoi; ++j) {
+            const double dzmm = DZ(c, j) * 1.e3;
+            dzsum = dzsum + dzmm;
+            icefracsum = icefracsum + icef[(size_t)IX(c) * nlevsoi + (j - 1)] * dzmm;
+        }
+        imped_col[IX(c)] = cm_pow(10.0, -cfg->e_ice * (icefracsum / dzsum));
+    }
+    for (fc = 0; fc < num_c; ++fc) {
+        const int c = COL(fc), nb = HC(S->nbedrock, c);
+        if (HC(D->zwt, c) <= ZI(c, nb))
+            HC(D->qflx_latflow_out, c) = imped_col[IX(c)] * P->baseflow_scalar * cm_tan(H_PI / 180.0 * HC(D->topo_slope, c))
+                                         * cm_pow(ZI(c, nb) - HC(D->zwt, c), P->n_baseflow);
+    }
+    for (fc = 0; fc < num_c; ++fc) {
+        const int c = COL(fc), nb = HC(S->nbedrock, c);
+        const double net = HC(D->qflx_latflow_out, c) - HC(D->qflx_latflow_in, c);
+        double tot;
+        if (HC(D->zwt, c) <= ZI(c, nb)) drainage[IX(c)] = net;
+        else drainage[IX(c)] = 0.0;
+        tot = -drainage[IX(c)] * dtime;
+        if (tot > 0.) {
+            for (j = jwt[IX(c)] + 1; j >= 1; --j) {
+                if (ZI(c, j) < HC(D->frost_table, c)) {
+                    const double s_y = h_spec_yield(&Lo, S, P, c, j, HC(D->zwt, c));
+                    double dl = h_min(tot, (s_y * DZ(c, j) * 1.e3));
+                    dl = h_max(dl, 0.0);
+                    LIQ(c, j) = LIQ(c, j) + dl;
+                    tot = tot - dl;
+                    if (tot <= 0.) {
+                        HC(D->zwt, c) = HC(D->zwt, c) - dl / s_y / 1000.0;
+                        break;
+                    } else {
+                        HC(D->zwt, c) = ZI(c, j - 1);
+                    }
+                }
+            }
+            HC(S->h2osfc, c) = HC(S->h2osfc, c) + tot;
+        } else {
+            for (j = jwt[IX(c)] + 1; j <= nb; ++j) {
+                const double s_y = h_spec_yield(&Lo, S, P, c, j, HC(D->zwt, c));
+                double dl = h_max(tot, -(s_y * (ZI(c, j) - HC(D->zwt, c)) * 1.e3));
+                dl = h_min(dl, 0.0);
+                LIQ(c, j) = LIQ(c, j) + dl;
+                tot = tot - dl;
+                if (tot >= 0.) {
+                    HC(D->zwt, c) = HC(D->zwt, c) - dl / s_y / 1000.0;
+                    break;
+                } else {
+                    HC(D->zwt, c) = ZI(c, j);
+                }
+            }
+            drainage[IX(c)] = drainage[IX(c)] + tot / dtime;
+        }
+        HC(D->zwt, c) = h_max(0.0, HC(D->zwt, c));
+        HC(D->zwt, c) = h_min(80.0, HC(D->zwt, c));
+    }
+    for (j = nlevsoi; j >= 2; --j) {
+        for (fc = 0; fc < num_c; ++fc) {
+            const int c = COL(fc);
+            const double dzmm = DZ(c, j) * 1.e3;
+            const double xsi = h_max(LIQ(c, j) - HSOI(S->eff_porosity, c, j) * dzmm, 0.0);
+            LIQ(c, j) = h_min(HSOI(S->eff_porosity, c, j) * dzmm, LIQ(c, j));
+            LIQ(c, j - 1) = LIQ(c, j - 1) + xsi;
+        }
+    }
+    for (fc = 0; fc < num_c; ++fc) {
+        const int c = COL(fc);
+        const double dzmm1 = DZ(c, 1) * 1.e3;
+        double xs1 = h_max(h_max(LIQ(c, 1) - H_WATMIN, 0.0)
+                           - h_max(0.0, (H_PONDMX + HSOI(S->watsat, c, 1) * dzmm1 - ICE(c, 1) - H_WATMIN)), 0.0);
+        LIQ(c, 1) = LIQ(c, 1) - xs1;
+        HC(S->h2osfc, c) = HC(S->h2osfc, c) + xs1;
+        HC(D->qflx_rsub_sat, c) = 0.0;
+        xs1 = h_max(h_max(ICE(c, 1), 0.0) - h_max(0.0, (H_PONDMX + HSOI(S->watsat, c, 1) * dzmm1 - LIQ(c, 1))), 0.0);
+        ICE(c, 1) = h_min(h_max(0.0, H_PONDMX + HSOI(S->watsat, c, 1) * dzmm1 - LIQ(c, 1)), ICE(c, 1));
+        HC(D->qflx_ice_runoff_xs, c) = xs1 / dtime;
+    }
+    for (j = 1; j <= nlevsoi - 1; ++j) {
+        for (fc = 0; fc < num_c; ++fc) {
+            const int c = COL(fc);
+            double xs;
+            if (LIQ(c, j) < H_WATMIN) {
+                xs = H_WATMIN - LIQ(c, j);
+                if (j == jwt[IX(c)]) HC(D->zwt, c) = HC(D->zwt, c) + xs / HSOI(S->eff_porosity, c, j) / 1000.0;
+            } else {
+                xs = 0.0;
+            }
+            LIQ(c, j) = LIQ(c, j) + xs;
+            LIQ(c, j + 1) = LIQ(c, j + 1) - xs;
+        }
+    }
+    j = nlevsoi;
+    for (fc = 0; fc < num_c; ++fc) {
+        const int c = COL(fc);
+        double xs;
+        if (LIQ(c, j) < H_WATMIN) {
+            xs = H_WATMIN - LIQ(c, j);
+            for (i = nlevsoi - 1; i >= 1; --i) {
+                const double avail = h_max(LIQ(c, i) - H_WATMIN - xs, 0.0);
+                if (avail >= xs) {
+                    LIQ(c, j) = LIQ(c, j) + xs;
+                    LIQ(c, i) = LIQ(c, i) - xs;
+                    xs = 0.0;
+                    break;
+                } else {
+                    LIQ(c, j) = LIQ(c, j) + avail;
+                    LIQ(c, i) = LIQ(c, i) - avail;
+                    xs = xs - avail;
+                }
+            }
+        } else {
+            xs = 0.0;
+        }
+        LIQ(c, j) = LIQ(c, j) + xs;
+        HC(D->qflx_rsub_sat, c) = HC(D->qflx_rsub_sat, c) - xs / dtime;
+    }
+    for (fc = 0; fc < num_c; ++fc) {
+        const int c = COL(fc);
+        HC(D->qflx_drain, c) = HC(D->qflx_rsub_sat, c) + drainage[IX(c)];
+        HC(D->qflx_qrgwl, c) = HC(D->qflx_snwcp_liq, c);
+    }
+
+    /* ---- HydrologyDrainage: volumetric water, runoff */
+    if (D->h2osoi_vol) {
+        for (j = 1; j <= nlevgrnd; ++j) {
+            for (fc = 0; fc < num_c; ++fc) {
+                const int c = COL(fc);
+                HSOI(D->h2osoi_vol, c, j) = LIQ(c, j) / (DZ(c, j) * H_DENH2O) + ICE(c, j) / (DZ(c, j) * H_DENICE);
+            }
+        }
+    }
+    for (fc = 0; fc < num_c; ++fc) {
+        const int c = COL(fc);
+        HC(D->qflx_runoff, c) = HC(D->qflx_drain, c) + HC(D->qflx_surf, c) + HC(D->qflx_qrgwl, c)
+                                + HC(D->qflx_drain_perched, c);
+    }
+#undef COL
+#undef IX
+#undef LIQ
+#undef ICE
+#undef DZ
+#undef ZI
+    free(k_frost); free(k_perch); free(jwt); free(out); free(icef); free(imped_col); free(drainage);
+    return 0;
 }

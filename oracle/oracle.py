@@ -285,12 +285,17 @@ def _hy_lib():
             C.c_int, abi.i32p]
         L.oracle_water_table.argtypes = [P(abi.Config), P(abi.Layout), P(abi.ColState), P(abi.WaterTableState),
                                          C.c_int, abi.i32p, C.c_double]
+        L.oracle_drainage.argtypes = [P(abi.Config), P(abi.Layout), P(abi.ColState), P(abi.DrainageState),
+                                      P(abi.HydroParams), C.c_int, abi.i32p, C.c_double]
         L._hy_ready = True
     return L
 
 
-def hydro_params(fff=0.5, pc=0.4, mu=0.14, h2osfcflag=1):
-    return abi.HydroParams(fff, pc, mu, h2osfcflag, 0)
+def hydro_params(fff=0.5, pc=0.4, mu=0.14, h2osfcflag=1, aq_sp_yield_min=0.01, n_baseflow=1.0,
+                 baseflow_scalar=5.5e-3, perched_baseflow_scalar=1.0e-5):
+    return abi.HydroParams(fff=fff, pc=pc, mu=mu, aq_sp_yield_min=aq_sp_yield_min, n_baseflow=n_baseflow,
+                           baseflow_scalar=baseflow_scalar, perched_baseflow_scalar=perched_baseflow_scalar,
+                           h2osfcflag=h2osfcflag, pad=0)
 
 
 def surface_water(st, hydro, cfg=None, dtime=1800.0, filter_c=None):
@@ -334,7 +339,18 @@ def water_table(st, cfg=None, dtime=1800.0, filter_c=None):
                                 len(f), fp, C.c_double(dtime))
 
 
-def land_step_full(st, cfg, prm, pft, dtime=1800.0, hydro=None, water_table_after=False):
+def drainage(st, hydro, cfg=None, dtime=1800.0, filter_c=None):
+    """PerchedLateralFlow, SubsurfaceLateralFlow and the runoff sum of
+    HydrologyDrainage (HydrologyDrainageMod.F90:137-245, no aquifer layer)."""
+    L = _hy_lib()
+    cs, _ = st.structs()
+    dr = st.drainage_struct()
+    f, fp = _filt(filter_c if filter_c is not None else np.arange(1, st.dims.ncol + 1))
+    return L.oracle_drainage(C.byref(cfg or canopy_config()), C.byref(st.layout()), C.byref(cs), C.byref(dr),
+                             C.byref(hydro), len(f), fp, C.c_double(dtime))
+
+
+def land_step_full(st, cfg, prm, pft, dtime=1800.0, hydro=None, water_table_after=False, drainage_after=False):
     """land_step with the reference's bookkeeping: t_ssbef / t_h2osfc_bef
     saved first, SoilFluxes between SoilTemperature and the hydrology
     (clm_driver.F90:921), given hydro the column evaporation fluxes and the
@@ -357,11 +373,16 @@ def land_step_full(st, cfg, prm, pft, dtime=1800.0, hydro=None, water_table_afte
         patch2col3(st)
         surface_water(st, hydro, cfg, dtime)
     rc, bad = soilwater(st, cfg, dtime)
-    if rc or not water_table_after:
+    if rc:
         return rc, bad
-    patch2col3(st, dew=True)
-    c = water_table(st, cfg, dtime)
-    return (0, None) if c == 0 else (1, c)
+    if water_table_after:
+        patch2col3(st, dew=True)
+        c = water_table(st, cfg, dtime)
+        if c:
+            return 1, c
+    if drainage_after:
+        drainage(st, hydro, cfg, dtime)
+    return 0, None
 
 
 def qsat(T, p):

@@ -45,7 +45,7 @@ STRUCTS = (("ctsm_config", abi.Config), ("ctsm_layout", abi.Layout), ("ctsm_tuni
            ("ctsm_canopy_state", abi.CanopyState), ("ctsm_soilflux_state", abi.SoilFluxState),
            ("ctsm_surface_water_state", abi.SurfaceWaterState), ("ctsm_hydro_params", abi.HydroParams),
            ("ctsm_water_table_state", abi.WaterTableState), ("ctsm_hist_field", abi.HistField),
-           ("ctsm_ckpt_field", abi.CkptField))
+           ("ctsm_ckpt_field", abi.CkptField), ("ctsm_drainage_state", abi.DrainageState))
 
 
 def _probe(tmp_path):
@@ -94,8 +94,9 @@ program sizes
   type(ctsm_surface_water_state_t) :: i
   type(ctsm_hydro_params_t) :: j
   type(ctsm_water_table_state_t) :: k
-  print '(11I8)', c_sizeof(a), c_sizeof(b), c_sizeof(c), c_sizeof(d), c_sizeof(e), c_sizeof(f), c_sizeof(g), &
-       c_sizeof(h), c_sizeof(i), c_sizeof(j), c_sizeof(k)
+  type(ctsm_drainage_state_t) :: l
+  print '(12I8)', c_sizeof(a), c_sizeof(b), c_sizeof(c), c_sizeof(d), c_sizeof(e), c_sizeof(f), c_sizeof(g), &
+       c_sizeof(h), c_sizeof(i), c_sizeof(j), c_sizeof(k), c_sizeof(l)
 end program sizes
 """)
     mod = os.path.join(ROOT, "ctsm_amd", "fortran", "ctsm_gpu_mod.F90")
@@ -105,7 +106,7 @@ end program sizes
     assert sizes == [C.sizeof(abi.Config), C.sizeof(abi.Layout), C.sizeof(abi.ColState), C.sizeof(abi.PatchState),
                      C.sizeof(abi.CanopyParams), C.sizeof(abi.PftParams), C.sizeof(abi.CanopyState),
                      C.sizeof(abi.SoilFluxState), C.sizeof(abi.SurfaceWaterState), C.sizeof(abi.HydroParams),
-                     C.sizeof(abi.WaterTableState)]
+                     C.sizeof(abi.WaterTableState), C.sizeof(abi.DrainageState)]
 
 
 def test_config_rejected_before_device():

@@ -207,3 +207,70 @@ def test_water_table_endrun_reports_column():
         assert ei.value.index == k + 1
     finally:
         c.close()
+
+
+# ---- drainage --------------------------------------------------------------------
+
+def _dr_state(ncol, seed=0x5EED0061):
+    from tests.test_oracle_hydrology import _dr_state as make
+    return make(ncol, seed=seed)
+
+
+def test_drainage_dropin_bitwise(hctx):
+    st = _dr_state(1500)
+    print("case: drainage drop-in, 1500 columns", flush=True)
+    hydro = O.hydro_params()
+    g, o = st.copy(), st.copy()
+    hctx.drainage(g, hydro)
+    assert O.drainage(o, hydro) == 0
+    _same(g, o)
+    assert (o.hy["qflx_drain_perched"] > 0).any() and (o.hy["qflx_latflow_out"] > 0).any()
+
+
+def test_drainage_dev_bitwise(hctx):
+    st = _dr_state(4096, seed=0x5EED0062)
+    perm = (np.random.default_rng(9).permutation(st.dims.ncol) + 1).astype(np.int32)[:3500]
+    print("case: drainage device entry, 3500 of 4096 columns, permuted filter", flush=True)
+    hydro = O.hydro_params()
+    ds = DeviceState(st, hctx)
+    f = hctx.device_array(perm)
+    try:
+        hctx.drainage_dev(ds, hydro, filter_dev=f, n=len(perm))
+        hctx.sync()
+        g = ds.to_host()
+    finally:
+        hctx.free(f)
+        ds.free()
+    o = st.copy()
+    assert O.drainage(o, hydro, filter_c=perm) == 0
+    _same(g, o)
+
+
+@pytest.mark.parametrize("nsteps,name", [(1, "canopy"), (4, "canopy_uneven")])
+def test_land_step_full_with_drainage_bitwise(hctx, nsteps, name):
+    st = S.canopy_workload(name, ncol=320, seed=0xC75D0061)
+    print(f"case: full land step + surface water + water tables + drainage x{nsteps}, {name}, 320 columns / "
+          f"{st.dims.npatch} patches", flush=True)
+    hydro = O.hydro_params()
+    ds = DeviceState(st, hctx)
+    worst = 0.0
+    try:
+        for _ in range(nsteps):
+            hctx.land_step_full_dev(ds, hydro=hydro, water_table=True, drainage=True)
+            m, _ = hctx.soil_balance_check_dev(ds)
+            worst = max(worst, m)
+        hctx.sync()
+        g = ds.to_host()
+    finally:
+        ds.free()
+    o = st.copy()
+    cfg = O.canopy_config()
+    pft = O.PftStruct(S.pft_table())
+    for _ in range(nsteps):
+        rc, bad = O.land_step_full(o, cfg, S.canopy_params(), pft, hydro=hydro, water_table_after=True,
+                                   drainage_after=True)
+        assert rc == 0, bad
+    _same(g, o, groups=("col", "pch", "can", "sf", "hy"))
+    assert worst < 1e-7, worst
+    ns, nl = st.dims.nlevsno, st.dims.nlevsoi
+    assert np.all(o.col["h2osoi_liq"][ns:ns + nl] >= 0.01 - 1e-12)     # the minimum-water corrections ran

@@ -97,3 +97,27 @@ def test_damaged_or_mismatched_checkpoint_is_refused(rctx, tmp_path):
         rctx.checkpoint_read(path, [("c/t_soisno", ptr, nbytes)])
     finally:
         ds.free()
+
+
+def test_checkpoint_of_raw_ranges(rctx, tmp_path):
+    """Ranges larger than one staging chunk and ranges whose size is a multiple
+    of the 4 KiB alignment (the last one ending the file) come back intact."""
+    rng = np.random.default_rng(17)
+    sizes = [40 << 20, 12345, 4096, 3 << 20]
+    print(f"case: checkpoint of raw device ranges {sizes}", flush=True)
+    host = [rng.integers(0, 256, n, dtype=np.uint8) for n in sizes]
+    dev = [rctx.alloc(n) for n in sizes]
+    back = [rctx.alloc(n) for n in sizes]
+    try:
+        for p, h in zip(dev, host):
+            rctx.h2d(p, h)
+        path = tmp_path / "raw.ckpt"
+        rctx.checkpoint_write(path, [(f"r{i}", p, n) for i, (p, n) in enumerate(zip(dev, sizes))])
+        rctx.checkpoint_read(path, [(f"r{i}", p, n) for i, (p, n) in enumerate(zip(back, sizes))])
+        for p, h in zip(back, host):
+            g = np.empty_like(h)
+            rctx.d2h(g, p)
+            assert np.array_equal(g, h)
+    finally:
+        for p in dev + back:
+            rctx.free(p)

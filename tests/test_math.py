@@ -151,6 +151,26 @@ def test_sin_within_one_ulp():
     assert np.all(np.abs(got - np.sin(x)) <= 2 * np.spacing(np.abs(np.sin(x))))
 
 
+def test_tan_within_two_and_a_half_ulp():
+    mp = pytest.importorskip("mpmath")
+    mp.mp.dps = 50
+    r = _rng(7)
+    deg = r.uniform(0.0, 89.0, 600)
+    x = np.concatenate([(math.pi / 180.0) * deg, r.uniform(-10, 10, 400), 10.0 ** r.uniform(-9, 0, 200),
+                        np.array([1e-300, -0.5, 0.0, 1.0])])
+    got = _vec("tan", x)
+    worst = 0.0
+    for xi, gi in zip(x, got):
+        ex = mp.tan(mp.mpf(xi))
+        e = float(ex)
+        if e == 0.0:
+            assert gi == 0.0
+            continue
+        worst = max(worst, float(abs(mp.mpf(gi) - ex) / mp.mpf(math.ulp(e))))
+    assert worst < 2.5, worst
+    assert np.all(np.abs(got - np.tan(x)) <= 3 * np.spacing(np.abs(np.tan(x))))
+
+
 def test_pow_and_log_hard_cases():
     """x near 1 with large exponents (log cancellation), results near the
     overflow / underflow thresholds, and table-boundary arguments."""

@@ -375,3 +375,136 @@ def test_renew_condensation_mass_and_truncation():
     k = bare[1]
     st2.hy["qflx_solidevap_from_top_layer"][k] = 1.0
     assert O.water_table(st2) == k + 1
+
+
+# ---- drainage (HydrologyDrainageMod.F90:137-245, no aquifer layer) ----------------
+
+WATMIN = 0.01
+
+
+def _dr_state(ncol=512, seed=0x5EED0041, wet=False):
+    """Soil columns whose water tables, perched tables and frost tables come
+    from the water-table oracle, with ice in some layers and a few columns
+    drained below the minimum water (as a Richards solve can leave them)."""
+    st = _wt_state(ncol, seed=seed, snow=True)
+    ns, nl = st.dims.nlevsno, st.dims.nlevsoi
+    rng = np.random.default_rng(seed & 0xFFFF)
+    cols = np.arange(ncol)
+    kf = rng.integers(2, nl + 3, ncol)
+    for k in range(1, st.dims.nlevgrnd + 1):
+        st.col["t_soisno"][ns + k - 1] = np.where(k < kf, 280.0, 268.0)
+        if k <= nl:
+            lo, hi = (0.85, 1.0) if wet else (0.1, 1.0)
+            s = rng.uniform(lo, hi, ncol)
+            w = s * st.col["watsat"][k - 1] * st.col["dz"][ns + k - 1] * 1000.0
+            f = np.where(k >= kf, 0.5 * rng.random(ncol), 0.0)
+            st.col["h2osoi_liq"][ns + k - 1] = (1 - f) * w
+            st.col["h2osoi_ice"][ns + k - 1] = f * w * 0.917
+    dry = rng.random(ncol) < 0.1            # a negative layer, as left by the Richards solve
+    kneg = rng.integers(1, nl + 1, ncol)
+    st.col["h2osoi_liq"][ns + kneg[dry] - 1, cols[dry]] = -rng.uniform(0.0, 3.0, dry.sum())
+    vol_ice = np.minimum(st.col["watsat"][:st.dims.nlevgrnd], st.col["h2osoi_ice"][ns:] / (st.col["dz"][ns:] * 917.0))
+    st.col["eff_porosity"][...] = np.maximum(0.01, st.col["watsat"] - vol_ice)
+    st.hy["zwt"][:] = 50.0
+    for k in ("qflx_liqdew_to_top_layer", "qflx_soliddew_to_top_layer", "qflx_solidevap_from_top_layer"):
+        st.hy[k][:] = 0.0
+    assert O.water_table(st) == 0           # zwt, zwt_perched, frost_table from the state
+    st.hy["qflx_surf"][:] = rng.uniform(0.0, 1e-4, ncol)
+    return st
+
+
+def _water(st):
+    ns, nl = st.dims.nlevsno, st.dims.nlevsoi
+    return (st.col["h2osoi_liq"][ns:ns + nl].sum(0) + st.col["h2osoi_ice"][ns:ns + nl].sum(0)
+            + st.col["h2osfc"])
+
+
+@pytest.mark.parametrize("wet", [False, True])
+def test_drainage_water_balance_and_limits(wet):
+    st = _dr_state(wet=wet)
+    if not wet:                             # a column too dry to refill its bottom layer
+        ns, nl = st.dims.nlevsno, st.dims.nlevsoi
+        st.col["h2osoi_liq"][ns:ns + nl, 3] = 0.0
+        st.col["h2osoi_liq"][ns + nl - 1, 3] = -0.5
+        st.col["h2osoi_ice"][ns:ns + nl, 3] = 0.0
+    ns, nl = st.dims.nlevsno, st.dims.nlevsoi
+    dzmm = st.col["dz"][ns:ns + nl] * 1e3
+    # a quiet column: every layer between the minimum water and saturation, no
+    # ice, the water table below the bedrock, no perched water -- unchanged
+    q = 5
+    st.col["h2osoi_ice"][ns:ns + nl, q] = 0.0
+    st.col["eff_porosity"][:nl, q] = st.col["watsat"][:nl, q]
+    st.col["h2osoi_liq"][ns:ns + nl, q] = 0.5 * st.col["watsat"][:nl, q] * dzmm[:, q]
+    st.hy["zwt"][q] = 100.0                 # also beyond the 80 m cap
+    st.hy["frost_table"][q] = st.hy["zwt_perched"][q] = 0.5
+    liq_q = st.col["h2osoi_liq"][ns:ns + nl, q].copy()
+    # a top layer holding more ice than its pore space: the excess runs off
+    e = 7
+    st.col["h2osoi_ice"][ns, e] = 2.0 * st.col["watsat"][0, e] * dzmm[0, e]
+    w0 = _water(st)
+    zwt0 = st.hy["zwt"].copy()
+    assert O.drainage(st, O.hydro_params()) == 0
+    h = st.hy
+    assert np.array_equal(st.col["h2osoi_liq"][ns:ns + nl, q], liq_q)
+    assert h["zwt"][q] == 80.0 and h["qflx_drain"][q] == 0.0 and h["qflx_drain_perched"][q] == 0.0
+    assert h["qflx_ice_runoff_xs"][e] > 0.0
+    w1 = _water(st)
+    out = (h["qflx_drain"] + h["qflx_drain_perched"] + h["qflx_ice_runoff_xs"]) * DT
+    assert np.max(np.abs((w1 - w0) + out)) < 1e-10 * np.max(np.abs(w0))
+    ns, nl = st.dims.nlevsno, st.dims.nlevsoi
+    assert np.all(st.col["h2osoi_liq"][ns:ns + nl] >= WATMIN - 1e-12)
+    assert np.all((h["zwt"] >= 0.0) & (h["zwt"] <= 80.0))
+    assert np.array_equal(h["qflx_runoff"], h["qflx_drain"] + h["qflx_surf"] + h["qflx_qrgwl"]
+                          + h["qflx_drain_perched"])
+    assert (h["qflx_latflow_out"] > 0).any() and np.all(h["qflx_latflow_in"] == 0.0)
+    if not wet:
+        assert (h["qflx_rsub_sat"] < 0).any()          # the bottom layer's deficit came from outside
+        assert np.any(zwt0 != h["zwt"])
+
+
+def test_drainage_closed_forms():
+    st = _dr_state(seed=0x5EED0042, wet=True)
+    p = O.hydro_params()
+    ns, nl = st.dims.nlevsno, st.dims.nlevsoi
+    cols = np.arange(st.dims.ncol)
+    nb = st.col["nbedrock"].astype(int)
+    zi = st.col["zi"]
+    zwt0, zwtp, frost = st.hy["zwt"].copy(), st.hy["zwt_perched"].copy(), st.hy["frost_table"].copy()
+    ice = st.col["h2osoi_ice"][ns:ns + nl]
+    dz = st.col["dz"][ns:ns + nl]
+    ws = st.col["watsat"][:nl]
+    assert O.drainage(st, p) == 0
+    # icefrac rewritten from the ice
+    icefrac = np.minimum(1.0, np.minimum(ws, ice / (dz * 917.0)) / ws)
+    assert np.allclose(st.col["icefrac"][:nl], icefrac, rtol=1e-15, atol=0)
+    # baseflow from the water table, with the column's ice impedance below it
+    zinb = zi[ns + nb, cols]
+    jwt = np.array([next((j - 1 for j in range(1, nl + 1) if zwt0[c] <= zi[ns + j, c]), nl) for c in cols])
+    imped = np.empty(len(cols))
+    for c in cols:
+        j0 = max(jwt[c], 1)
+        dzmm = dz[j0 - 1:, c] * 1e3
+        imped[c] = 10.0 ** (-6.0 * np.sum(icefrac[j0 - 1:, c] * dzmm) / np.sum(dzmm))
+    slope = st.hy["topo_slope"]
+    q = np.where(zwt0 <= zinb, imped * p.baseflow_scalar * np.tan(np.deg2rad(slope))
+                 * np.maximum(zinb - zwt0, 0.0) ** p.n_baseflow, 0.0)
+    assert np.allclose(st.hy["qflx_latflow_out"], q, rtol=1e-12, atol=0)
+    assert (q > 0).any() and (q == 0).any()
+    # perched drainage: the transmissivity-weighted rate, never more than the layers hold
+    perched = frost > zwtp
+    assert perched.any()
+    hk = st.col["hksat"][:nl]
+    k_fr = np.array([next((k for k in range(1, nb[c] + 1) if zi[ns + k - 1, c] <= frost[c] < zi[ns + k, c]), nb[c])
+                     for c in cols])
+    k_pe = np.array([next((k for k in range(1, nb[c] + 1) if zi[ns + k - 1, c] <= zwtp[c] < zi[ns + k, c]), nb[c])
+                     for c in cols])
+    expect = np.zeros(len(cols))
+    for c in np.nonzero(perched)[0]:
+        ks = slice(k_pe[c] - 1, k_fr[c] - 1)
+        w = dz[ks, c].sum()
+        qp = (hk[ks, c] * dz[ks, c]).sum() / w if w > 0 else 0.0
+        expect[c] = p.perched_baseflow_scalar * np.sin(np.deg2rad(slope[c])) * qp * (frost[c] - zwtp[c])
+    got = st.hy["qflx_drain_perched"]
+    assert np.all(got <= expect * (1 + 1e-12) + 1e-30)
+    unlimited = np.isclose(got[perched], expect[perched], rtol=1e-12, atol=0)
+    assert unlimited.mean() > 0.5, unlimited.mean()     # most columns are not limited by the layers' water
