@@ -19,6 +19,8 @@
 // (ft, fth, 2**x) depend on t_veg only, so they are evaluated once per
 // iteration and shared by the sun and shade phases (the same values the
 // reference computes twice).
+#include <climits>
+
 #include "common.h"
 
 namespace ctsm {
@@ -1365,12 +1367,10 @@ __global__ void __launch_bounds__(256) k_zero_cgrnd(int n, const int32_t *__rest
 
 // p2c_1d_filter of qflx_tran_veg (clm_driver.F90:1714-1716) and
 // Compute_EffecRootFrac_And_VertTranSink_Default (SoilWaterPlantSinkMod.F90:332-424)
-__global__ void __launch_bounds__(256) k_root_sink(CView V, ctsm_col_state col, ctsm_canopy_state S, int n,
-                                                   const int32_t *__restrict__ filter, ErrWord *err)
+// for one column, reading its patches straight from HBM
+__device__ __forceinline__ void root_sink_column(const CView &V, const ctsm_col_state &col,
+                                                 const ctsm_canopy_state &S, int c, ErrWord *err)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int c = filter ? filter[i] : V.begc + i;
     const long cc = V.c1(c);
     const int pi = col.patchi[cc], pf = col.patchf[cc];
     if (pf >= pi && (pi < V.begp || pf >= V.begp + V.ldp)) {   // patches outside the allocation
@@ -1397,6 +1397,91 @@ __global__ void __launch_bounds__(256) k_root_sink(CView V, ctsm_col_state col, 
     for (int j = 1; j <= MAX_NLEVSOI; ++j) {
         if (j > V.nlevsoi) continue;
         double rc = r[j - 1];
+        if (tcol != 0.0) rc = rc / tcol;
+        S.rootr_col[V.soi(c, j)] = rc;
+        S.qflx_rootsoi_col[V.soi(c, j)] = rc * tcol;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_root_sink(CView V, ctsm_col_state col, ctsm_canopy_state S, int n,
+                                                   const int32_t *__restrict__ filter, ErrWord *err)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    root_sink_column(V, col, S, filter ? filter[i] : V.begc + i, err);
+}
+
+// The same for consecutive columns (no filter) whose patches are consecutive:
+// a block stages the patch range of RS_COLS columns in LDS (weights and
+// activity once, then rootr RS_GROUPS levels at a time, every element read once
+// and coalesced); thread group g sums level j0 + g of its column over the
+// column's patches in ascending order from LDS -- the operations and their
+// order are those of root_sink_column.  A block whose patches exceed the
+// staging space takes root_sink_column.
+constexpr int RS_COLS = 64;
+constexpr int RS_GROUPS = 4;
+constexpr int RS_CAP = 16 * RS_COLS;
+
+__global__ void __launch_bounds__(RS_COLS *RS_GROUPS) k_root_sink_staged(CView V, ctsm_col_state col,
+                                                                         ctsm_canopy_state S, int n, ErrWord *err)
+{
+    __shared__ double s_qt[RS_CAP], s_w[RS_CAP], s_row[RS_GROUPS][RS_CAP];
+    __shared__ int32_t s_act[RS_CAP];
+    __shared__ int s_lo, s_hi, s_bad;
+    const int t = threadIdx.x;
+    const int lane = t % RS_COLS, g = t / RS_COLS;
+    const int i = blockIdx.x * RS_COLS + lane;
+    const int c = V.begc + i;
+    const long cc = V.c1(c);
+    const bool live = i < n;
+    int pi = 0, pf = -1;
+    if (live) {
+        pi = col.patchi[cc];
+        pf = col.patchf[cc];
+    }
+    if (t == 0) { s_lo = INT_MAX; s_hi = INT_MIN; s_bad = 0; }
+    __syncthreads();
+    if (g == 0 && live && pf >= pi) {
+        if (pi < V.begp || pf >= V.begp + V.ldp) atomicOr(&s_bad, 1);
+        atomicMin(&s_lo, pi);
+        atomicMax(&s_hi, pf);
+    }
+    __syncthreads();
+    const int lo = s_lo, hi = s_hi;
+    const int np = hi >= lo ? hi - lo + 1 : 0;
+    if (s_bad || np > RS_CAP) {              // block-uniform
+        if (g == 0 && live) root_sink_column(V, col, S, c, err);
+        return;
+    }
+    for (int k = t; k < np; k += RS_COLS * RS_GROUPS) {
+        const long q = V.p1(lo + k);
+        s_qt[k] = S.qflx_tran_veg[q];
+        s_w[k] = S.wtcol[q];
+        s_act[k] = S.active[q];
+    }
+    __syncthreads();
+    // every group forms its column's total (the same operations, so the same bits)
+    double tcol = 0.0;
+    for (int p = pi; p <= pf; ++p) {
+        const int k = p - lo;
+        if (!s_act[k]) continue;
+        tcol = tcol + s_qt[k] * s_w[k];
+    }
+    if (g == 0 && live) S.qflx_tran_veg_col[cc] = tcol;
+    for (int j0 = 1; j0 <= V.nlevsoi; j0 += RS_GROUPS) {
+        __syncthreads();                     // the previous levels have been summed
+        for (int r = 0; r < RS_GROUPS && j0 + r <= V.nlevsoi; ++r)
+            for (int k = t; k < np; k += RS_COLS * RS_GROUPS) s_row[r][k] = S.rootr[V.soip(lo + k, j0 + r)];
+        __syncthreads();
+        const int j = j0 + g;
+        if (!live || j > V.nlevsoi) continue;
+        double r = 0.0;
+        for (int p = pi; p <= pf; ++p) {
+            const int k = p - lo;
+            if (!s_act[k]) continue;
+            r = r + s_row[g][k] * s_qt[k] * s_w[k];
+        }
+        double rc = r;
         if (tcol != 0.0) rc = rc / tcol;
         S.rootr_col[V.soi(c, j)] = rc;
         S.qflx_rootsoi_col[V.soi(c, j)] = rc * tcol;
@@ -1472,7 +1557,11 @@ void launch_root_sink(hipStream_t st, const ctsm_layout &L, const ctsm_config &c
     using namespace canopy;
     if (n <= 0) return;
     CView V{L.begc_alloc, L.ldc, L.begp_alloc, L.ldp, L.begg_alloc, L.ldg, cfg.nlevsno, cfg.nlevsoi, cfg.nlevgrnd};
-    hipLaunchKernelGGL(k_root_sink, dim3((n + 255) / 256), dim3(256), 0, st, V, col, can, n, filter, err);
+    if (filter)
+        hipLaunchKernelGGL(k_root_sink, dim3((n + 255) / 256), dim3(256), 0, st, V, col, can, n, filter, err);
+    else
+        hipLaunchKernelGGL(k_root_sink_staged, dim3((n + RS_COLS - 1) / RS_COLS), dim3(RS_COLS * RS_GROUPS), 0, st, V,
+                           col, can, n, err);
 }
 
 }  // namespace ctsm

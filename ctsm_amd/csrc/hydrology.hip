@@ -26,6 +26,8 @@
 // them (p2c_1d_filter, subgridAveMod.F90:312-318), for the land step: the
 // evaporation fluxes SetQflxInputs reads and the dew / sublimation fluxes
 // RenewCondensation reads.
+#include <climits>
+
 #include "common.h"
 
 namespace ctsm {
@@ -447,31 +449,101 @@ __global__ void __launch_bounds__(256) k_drainage(HView V, double e_ice, ctsm_hy
 }
 
 // p2c_1d_filter of three patch fluxes (clm_drv_patch2col, subgridAveMod.F90:312-318)
-__global__ void __launch_bounds__(256) k_patch2col(HView V, ctsm_col_state S, const int32_t *__restrict__ active,
-                                                   const double *__restrict__ wtcol, const double *__restrict__ pa,
-                                                   const double *__restrict__ pb, const double *__restrict__ pc,
-                                                   double *ca, double *cb, double *ccol, int n,
-                                                   const int32_t *__restrict__ filter, ErrWord *err)
+struct P2C {
+    const int32_t *active;
+    const double *wtcol, *pa, *pb, *pc;
+    double *ca, *cb, *cc;
+};
+
+__device__ __forceinline__ void patch2col_column(const HView &V, const ctsm_col_state &S, const P2C &F, int c,
+                                                 ErrWord *err)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int c = filter ? filter[i] : V.begc + i;
-    if (c < V.begc || c >= V.begc + V.ldc) { report_error(err, c, CTSM_E_ARG); return; }
     const long cc = V.c1(c);
     const int pi = S.patchi[cc], pf = S.patchf[cc];
     if (pf >= pi && (pi < V.begp || pf >= V.begp + V.ldp)) { report_error(err, c, CTSM_E_ARG); return; }
     double a = 0.0, b = 0.0, d = 0.0;
     for (int p = pi; p <= pf; ++p) {
         const long pp = V.p1(p);
-        if (!active[pp]) continue;
-        const double w = wtcol[pp];
-        a = a + pa[pp] * w;
-        b = b + pb[pp] * w;
-        d = d + pc[pp] * w;
+        if (!F.active[pp]) continue;
+        const double w = F.wtcol[pp];
+        a = a + F.pa[pp] * w;
+        b = b + F.pb[pp] * w;
+        d = d + F.pc[pp] * w;
     }
-    ca[cc] = a;
-    cb[cc] = b;
-    ccol[cc] = d;
+    F.ca[cc] = a;
+    F.cb[cc] = b;
+    F.cc[cc] = d;
+}
+
+__global__ void __launch_bounds__(256) k_patch2col(HView V, ctsm_col_state S, P2C F, int n,
+                                                   const int32_t *__restrict__ filter, ErrWord *err)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = filter ? filter[i] : V.begc + i;
+    if (c < V.begc || c >= V.begc + V.ldc) { report_error(err, c, CTSM_E_ARG); return; }
+    patch2col_column(V, S, F, c, err);
+}
+
+// The same for consecutive columns (no filter): a block of P2C_COLS columns
+// stages its consecutive patch range in LDS with coalesced reads, then each
+// lane sums its own patches from LDS in ascending order (the operations of
+// patch2col_column, in its order).  A block whose patches exceed the staging
+// space takes patch2col_column.
+constexpr int P2C_COLS = 64;
+constexpr int P2C_CAP = 16 * P2C_COLS;
+
+__global__ void __launch_bounds__(P2C_COLS) k_patch2col_staged(HView V, ctsm_col_state S, P2C F, int n, ErrWord *err)
+{
+    __shared__ double s_w[P2C_CAP], s_a[P2C_CAP], s_b[P2C_CAP], s_c[P2C_CAP];
+    __shared__ int32_t s_act[P2C_CAP];
+    __shared__ int s_lo, s_hi, s_bad;
+    const int t = threadIdx.x;
+    const int i = blockIdx.x * P2C_COLS + t;
+    const int c = V.begc + i;
+    const long cc = V.c1(c);
+    const bool live = i < n;
+    int pi = 0, pf = -1;
+    if (live) {
+        pi = S.patchi[cc];
+        pf = S.patchf[cc];
+    }
+    if (t == 0) { s_lo = INT_MAX; s_hi = INT_MIN; s_bad = 0; }
+    __syncthreads();
+    if (live && pf >= pi) {
+        if (pi < V.begp || pf >= V.begp + V.ldp) atomicOr(&s_bad, 1);
+        atomicMin(&s_lo, pi);
+        atomicMax(&s_hi, pf);
+    }
+    __syncthreads();
+    const int lo = s_lo, hi = s_hi;
+    const int np = hi >= lo ? hi - lo + 1 : 0;
+    if (s_bad || np > P2C_CAP) {             // block-uniform
+        if (live) patch2col_column(V, S, F, c, err);
+        return;
+    }
+    for (int k = t; k < np; k += P2C_COLS) {
+        const long q = V.p1(lo + k);
+        s_act[k] = F.active[q];
+        s_w[k] = F.wtcol[q];
+        s_a[k] = F.pa[q];
+        s_b[k] = F.pb[q];
+        s_c[k] = F.pc[q];
+    }
+    __syncthreads();
+    if (!live) return;
+    double a = 0.0, b = 0.0, d = 0.0;
+    for (int p = pi; p <= pf; ++p) {
+        const int k = p - lo;
+        if (!s_act[k]) continue;
+        const double w = s_w[k];
+        a = a + s_a[k] * w;
+        b = b + s_b[k] * w;
+        d = d + s_c[k] * w;
+    }
+    F.ca[cc] = a;
+    F.cb[cc] = b;
+    F.cc[cc] = d;
 }
 
 }  // namespace hydro
@@ -492,8 +564,12 @@ void launch_patch2col3(hipStream_t st, const ctsm_layout &L, const ctsm_col_stat
 {
     if (n <= 0) return;
     const hydro::HView V{L.begc_alloc, L.ldc, L.begp_alloc, L.ldp, 0, 0, 0};
-    hipLaunchKernelGGL(hydro::k_patch2col, dim3((n + 255) / 256), dim3(256), 0, st, V, S, active, wtcol, pa, pb, pc,
-                       ca, cb, ccol, n, filter, err);
+    const hydro::P2C F{active, wtcol, pa, pb, pc, ca, cb, ccol};
+    if (filter)
+        hipLaunchKernelGGL(hydro::k_patch2col, dim3((n + 255) / 256), dim3(256), 0, st, V, S, F, n, filter, err);
+    else
+        hipLaunchKernelGGL(hydro::k_patch2col_staged, dim3((n + hydro::P2C_COLS - 1) / hydro::P2C_COLS),
+                           dim3(hydro::P2C_COLS), 0, st, V, S, F, n, err);
 }
 
 void launch_water_table(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg, const ctsm_col_state &S,
