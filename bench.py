@@ -165,12 +165,14 @@ def served_ensemble(ctx, args):
     lat = np.array([q.done - q.arrival for q in done])
     out = {"workload": f"{len(reqs)} members, Poisson {args.serve_rate}/s, 1-20,000 columns (log-uniform) with "
                        f"1-16 patches, 30 parameters x U(0.5,1.5), 1-365 days (log-uniform) x 48 steps x "
-                       f"{args.serve_day_scale}; land step with CanopyFluxes",
+                       f"{args.serve_day_scale}; full land step (CanopyFluxes, SoilTemperature, SoilFluxes, "
+                       f"surface water, SoilWater, water tables, drainage) and the soil balance check every step",
            "pool_columns": args.serve_pool_cols, "pool_bytes_per_column": srv.bytes_per_column(),
            "completed": len(done), "wall_s": wall,
            "latency_p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
            "latency_p99_s": float(np.percentile(lat, 99)) if len(lat) else None,
-           "column_steps_per_s": srv.column_steps / wall}
+           "column_steps_per_s": srv.column_steps / wall,
+           "max_abs_errsoi_Wm2": srv.worst_errsoi}
     srv.close()
     return out
 

@@ -5,10 +5,16 @@ generated like the full-column workload, with hydrology / energy parameters
 scaled, to be run for a number of time steps.  Members share one
 device-resident pool (a `DeviceState` whose column / patch / gridcell ranges
 are handed out as slots); admission is by the pool's capacity, i.e. the HBM
-budget fixed when the pool is created.  Every tick runs one land step
-(`ctsm_land_step_dev`: CanopyFluxes, transpiration sink, SoilTemperature,
-SoilWater) over the columns of all resident members together; a member
-leaves the pool, with its final state copied back, when its steps are done.
+budget fixed when the pool is created.  Every tick runs land steps over the
+columns of all resident members together -- the full step
+(`ctsm_land_step_full_dev`: the before-step save, CanopyFluxes,
+SoilTemperature, SoilFluxes, the transpiration sink, the surface water chain,
+SoilWater, the water tables and RenewCondensation, the drainage) followed by
+the soil energy balance check, or with full=False the bare
+`ctsm_land_step_dev`; a member leaves the pool, with its final state copied
+back, when its steps are done.  `checkpoint` writes the pool and the resident
+members' bookkeeping to a file and `EnsembleServer.resume` continues from it,
+bit for bit, for long served runs.
 
 Columns are independent, so a member's results do not depend on which other
 members share its ticks or on its slot position (tests/test_gpu_serve.py
@@ -21,7 +27,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import abi, synthetic
+from . import abi, restart, synthetic
 from .state import Dims, DeviceState, HostState
 
 _CAN_KIND = {n: k for n, _, k in abi.CANOPY_FIELDS}
@@ -103,6 +109,14 @@ class RangeAlloc:
                 return s
         return -1
 
+    def take(self, s, n):
+        """Mark [s, s + n) as used (it must lie inside one free range)."""
+        for i, (a, l) in enumerate(self.free):
+            if a <= s and s + n <= a + l:
+                self.free[i:i + 1] = [r for r in ((a, s - a), (s + n, a + l - s - n)) if r[1] > 0]
+                return
+        raise ValueError(f"range [{s}, {s + n}) is not free")
+
     def release(self, s, n):
         i = bisect.bisect(self.free, (s, 0))
         self.free.insert(i, (s, n))
@@ -118,13 +132,21 @@ class RangeAlloc:
 class EnsembleServer:
     """A resident pool of `cap_cols` columns / `cap_patches` patches."""
 
-    def __init__(self, ctx, cap_cols, cap_patches, nlevsno=12, nlevsoi=20, nlevgrnd=25, max_steps_per_tick=8):
+    def __init__(self, ctx, cap_cols, cap_patches, nlevsno=12, nlevsoi=20, nlevgrnd=25, max_steps_per_tick=8,
+                 full=True, hydro=None, pool=None):
         self.ctx = ctx
         self.dims = Dims(cap_cols, cap_patches, nlevsno, nlevsoi, nlevgrnd, ngrid=cap_cols)
-        tmpl = HostState(self.dims, fill_nan=False)
-        self.pool = DeviceState(tmpl, ctx)
-        del tmpl
+        if pool is None:
+            tmpl = HostState(self.dims, fill_nan=False)
+            pool = DeviceState(tmpl, ctx)
+            del tmpl
+        self.pool = pool
+        self.full = full
+        self.hydro = hydro if hydro is not None else synthetic.hydro_params()
         self.structs = (*self.pool.structs(), self.pool.canopy_struct())
+        if full:
+            self.structs = (*self.structs, self.pool.soilflux_struct())
+        self.worst_errsoi = 0.0
         self.cols = RangeAlloc(cap_cols)
         self.pats = RangeAlloc(cap_patches)
         self.active = []
@@ -215,13 +237,49 @@ class EnsembleServer:
         self._filters()
         k = min(self.max_steps_per_tick, min(r.left for r in self.active))
         for _ in range(k):
-            self.ctx.land_step_dev(self.pool, filterc_dev=self.fc, nc=self.nc, filterp_dev=self.fp, np_=self.np,
-                                   dtime=dtime, structs=self.structs)
+            if self.full:
+                self.ctx.land_step_full_dev(self.pool, filterc_dev=self.fc, nc=self.nc, filterp_dev=self.fp,
+                                            np_=self.np, dtime=dtime, structs=self.structs, hydro=self.hydro,
+                                            water_table=True, drainage=True)
+                m, _ = self.ctx.soil_balance_check_dev(self.pool, filterc_dev=self.fc, nc=self.nc)
+                self.worst_errsoi = max(self.worst_errsoi, m)
+            else:
+                self.ctx.land_step_dev(self.pool, filterc_dev=self.fc, nc=self.nc, filterp_dev=self.fp, np_=self.np,
+                                       dtime=dtime, structs=self.structs)
         self.ctx.sync()
         for r in self.active:
             r.left -= k
         self.column_steps += k * self.nc
         return k
+
+    def checkpoint(self, path):
+        """Write the pool (after the enqueued steps) and the resident members'
+        slots and remaining steps to `path`."""
+        extra = {"active": [[r.rid, r.col0, r.pat0, r.left, r.state.dims.ncol, r.state.dims.npatch]
+                            for r in self.active],
+                 "column_steps": self.column_steps, "worst_errsoi": self.worst_errsoi,
+                 "max_steps_per_tick": self.max_steps_per_tick, "full": self.full}
+        restart.checkpoint(self.ctx, self.pool, path, extra=extra)
+
+    @classmethod
+    def resume(cls, ctx, path, requests, hydro=None):
+        """A server continuing from a checkpoint.  `requests` maps request ids
+        to Request objects whose `state` holds arrays of the member's shape
+        (they receive the results when the member retires)."""
+        pool, extra = restart.restore(ctx, path)
+        d = pool.dims
+        srv = cls(ctx, d.ncol, d.npatch, d.nlevsno, d.nlevsoi, d.nlevgrnd, extra["max_steps_per_tick"],
+                  full=extra["full"], hydro=hydro, pool=pool)
+        for rid, col0, pat0, left, ncol, npatch in extra["active"]:
+            r = requests[rid]
+            assert r.state.dims.ncol == ncol and r.state.dims.npatch == npatch, rid
+            srv.cols.take(col0, ncol)
+            srv.pats.take(pat0, npatch)
+            r.col0, r.pat0, r.left = col0, pat0, left
+            srv.active.append(r)
+        srv.column_steps = extra["column_steps"]
+        srv.worst_errsoi = extra["worst_errsoi"]
+        return srv
 
     def close(self):
         self.ctx.free(self.fc)
@@ -248,9 +306,9 @@ def build_member(req, pmax=16):
     return st
 
 
-def serve(ctx, requests, cap_cols, cap_patches, wall_limit=None, max_steps_per_tick=8):
+def serve(ctx, requests, cap_cols, cap_patches, wall_limit=None, max_steps_per_tick=8, full=True):
     """Run the request stream in real time; returns (server, finished requests, wall seconds)."""
-    srv = EnsembleServer(ctx, cap_cols, cap_patches, max_steps_per_tick=max_steps_per_tick)
+    srv = EnsembleServer(ctx, cap_cols, cap_patches, max_steps_per_tick=max_steps_per_tick, full=full)
     pending = sorted(requests, key=lambda q: q.arrival)
     queue, finished = [], []
     for q in pending:

@@ -27,14 +27,62 @@ def test_members_in_the_mix_equal_members_alone():
     finally:
         ctx.close()
     assert len(done) == len(reqs)
+    assert srv.worst_errsoi < 1e-7, srv.worst_errsoi
     cfg = O.canopy_config()
     pft = O.PftStruct(S.pft_table())
+    hydro = O.hydro_params()
     for q in done:
         o = alone[q.rid]
         for _ in range(q.steps):
-            rc, bad = O.land_step(o, cfg, S.canopy_params(), pft)
+            rc, bad = O.land_step_full(o, cfg, S.canopy_params(), pft, hydro=hydro, water_table_after=True,
+                                       drainage_after=True)
             assert rc == 0, bad
-        for grp in ("col", "pch", "can"):
+        for grp in ("col", "pch", "can", "sf", "hy"):
             a, b = getattr(q.state, grp), getattr(o, grp)
             for k in b:
                 np.testing.assert_array_equal(a[k], b[k], err_msg=f"member {q.rid} {grp}.{k}")
+
+
+def _drain(srv):
+    out = {}
+    while srv.active:
+        srv.tick()
+        for q in [q for q in srv.active if q.left <= 0]:
+            srv.retire(q)
+            out[q.rid] = q.state
+    return out
+
+
+def test_served_pool_resumes_from_a_checkpoint(tmp_path):
+    from ctsm_amd.lib import Context
+    ctx = Context()
+    ctx.set_canopy_params(S.canopy_params(), S.pft_table())
+    reqs = serve.make_requests(4, rate=1e6, seed=23, max_cols=200, max_days=2, steps_per_day=4)
+    print("case: served pool checkpointed after 2 ticks and resumed, 4 members of up to 200 columns", flush=True)
+    path = tmp_path / "pool.ckpt"
+    try:
+        srv = serve.EnsembleServer(ctx, 1024, 1024 * 17, max_steps_per_tick=1)
+        for q in reqs:
+            serve.build_member(q)
+            assert srv.admit(q)
+        for _ in range(2):
+            srv.tick()
+        srv.checkpoint(path)
+        a = _drain(srv)
+        srv.close()
+        again = {}
+        for q in serve.make_requests(4, rate=1e6, seed=23, max_cols=200, max_days=2, steps_per_day=4):
+            serve.build_member(q)
+            again[q.rid] = q
+        srv2 = serve.EnsembleServer.resume(ctx, path, again)
+        b = _drain(srv2)
+        assert srv2.worst_errsoi < 1e-7
+        srv2.close()
+    finally:
+        ctx.close()
+    assert set(a) == set(b) == {q.rid for q in reqs}
+    for rid in a:
+        for grp in ("col", "pch", "can", "sf", "hy"):
+            x, y = getattr(a[rid], grp), getattr(b[rid], grp)
+            for k in x:
+                np.testing.assert_array_equal(x[k], y[k], err_msg=f"member {rid} {grp}.{k}")
