@@ -286,6 +286,20 @@ def _full_land_step(ctx, args, ds, structs):
     step_ms = ctx.elapsed_ms(e0, e1) / args.canopy_steps
     ctx.destroy_event(e0)
     ctx.destroy_event(e1)
+    # the same steps captured once and launched as one graph
+    g = ctx.graph_capture_full(ds, nsteps=args.canopy_steps, hydro=hydro, water_table=True, drainage=True)
+    ctx.graph_launch(g)                      # first launch uploads the graph
+    ctx.sync()
+    e0 = ctx.record()
+    t0 = time.perf_counter()
+    ctx.graph_launch(g)
+    e1 = ctx.record()
+    ctx.sync()
+    graph_wall = time.perf_counter() - t0
+    graph_step_ms = ctx.elapsed_ms(e0, e1) / args.canopy_steps
+    ctx.destroy_event(e0)
+    ctx.destroy_event(e1)
+    ctx.graph_destroy(g)
     worst, _ = ctx.soil_balance_check_dev(ds)
     err = np.empty(ds.dims.ncol)
     ctx.d2h(err, ds.sf["errsoi_col"])
@@ -298,6 +312,8 @@ def _full_land_step(ctx, args, ds, structs):
     b = surface_water_bytes(ds)
     out = {"history": _history_and_checkpoint(ctx, ds, structs)}
     return {**out, "column_steps_per_s": ds.dims.ncol * args.canopy_steps / wall, "step_ms": step_ms,
+            "graph_column_steps_per_s": ds.dims.ncol * args.canopy_steps / graph_wall,
+            "graph_step_ms": graph_step_ms,
             "max_abs_errsoi_Wm2": worst,
             "columns_errsoi_over_1e-5_Wm2": int(np.sum(~(np.abs(err) <= 1e-5))),
             "columns_negative_soil_liquid": int(np.sum((liq[ds.dims.nlevsno:] < 0).any(axis=0))),

@@ -957,26 +957,21 @@ int ctsm_graph_destroy(ctsm_graph *gr)
     return CTSM_OK;
 }
 
-int ctsm_graph_capture(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
-                       const ctsm_patch_state *pch, const ctsm_canopy_state *can, int32_t nc,
-                       const int32_t *filterc_dev, int32_t np, const int32_t *filterp_dev, double dtime,
-                       int32_t nsteps, ctsm_graph **out)
+// Captures what `steps` enqueues into a graph with its own workspaces (sized
+// before the capture: nothing may allocate under it).
+extern "C++" {
+template <class Steps>
+static int capture_graph(ctsm_ctx *ctx, int32_t nc, int32_t np, bool canopy, ctsm_graph **out, Steps steps)
 {
-    if (!out) return CTSM_E_ARG;
-    *out = nullptr;
-    if (!ctx || !col || !pch || nc < 0 || np < 0 || nsteps < 1 || check_layout(lay) ||
-        (!filterc_dev && nc > lay->ldc) || (can && (check_canopy_layout(ctx, lay) || (!filterp_dev && np > lay->ldp))))
-        return CTSM_E_ARG;
     hipError_t he = hipSetDevice(ctx->device);
     if (he != hipSuccess) return hip_fail(ctx, he);
     ctsm_graph *gr = new ctsm_graph;
     gr->ctx = ctx;
-    // size the graph's own workspaces first: nothing may allocate under capture
     he = gr->work.ensure(std::max<size_t>(step_work_doubles(ctx, nc), 1) * sizeof(double));
     if (he == hipSuccess && step_partf_bytes(ctx, nc)) he = gr->partf.ensure(step_partf_bytes(ctx, nc));
-    const size_t cwb = can ? canopy_work_bytes(np, ctx->cfg.itmax_canopy_fluxes, ctx->tun.canopy_first_iterations,
-                                               ctx->tun.canopy_round_iterations)
-                           : 0;
+    const size_t cwb = canopy ? canopy_work_bytes(np, ctx->cfg.itmax_canopy_fluxes, ctx->tun.canopy_first_iterations,
+                                                  ctx->tun.canopy_round_iterations)
+                              : 0;
     if (he == hipSuccess && cwb) he = gr->cwork.ensure(cwb);
     if (he != hipSuccess) {
         ctsm_graph_destroy(gr);
@@ -988,12 +983,7 @@ int ctsm_graph_capture(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_sta
     int rc = CTSM_OK;
     he = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
     if (he == hipSuccess) {
-        if (can) {
-            for (int s = 0; s < nsteps && rc == CTSM_OK; ++s)
-                rc = ctsm_land_step_dev(ctx, lay, col, pch, can, nc, filterc_dev, np, filterp_dev, dtime);
-        } else {
-            rc = soil_steps(ctx, lay, col, pch, nc, filterc_dev, dtime, nsteps);
-        }
+        rc = steps();
         hipError_t he2 = hipStreamEndCapture(ctx->stream, &gr->g);
         if (he2 != hipSuccess && rc == CTSM_OK) rc = hip_fail(ctx, he2);
     } else {
@@ -1013,6 +1003,47 @@ int ctsm_graph_capture(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_sta
     }
     *out = gr;
     return CTSM_OK;
+}
+}  // extern "C++"
+
+int ctsm_graph_capture(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                       const ctsm_patch_state *pch, const ctsm_canopy_state *can, int32_t nc,
+                       const int32_t *filterc_dev, int32_t np, const int32_t *filterp_dev, double dtime,
+                       int32_t nsteps, ctsm_graph **out)
+{
+    if (!out) return CTSM_E_ARG;
+    *out = nullptr;
+    if (!ctx || !col || !pch || nc < 0 || np < 0 || nsteps < 1 || check_layout(lay) ||
+        (!filterc_dev && nc > lay->ldc) || (can && (check_canopy_layout(ctx, lay) || (!filterp_dev && np > lay->ldp))))
+        return CTSM_E_ARG;
+    return capture_graph(ctx, nc, np, can != nullptr, out, [&]() {
+        if (!can) return soil_steps(ctx, lay, col, pch, nc, filterc_dev, dtime, nsteps);
+        int rc = CTSM_OK;
+        for (int s = 0; s < nsteps && rc == CTSM_OK; ++s)
+            rc = ctsm_land_step_dev(ctx, lay, col, pch, can, nc, filterc_dev, np, filterp_dev, dtime);
+        return rc;
+    });
+}
+
+int ctsm_graph_capture_full(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                            const ctsm_patch_state *pch, const ctsm_canopy_state *can,
+                            const ctsm_soilflux_state *sf, const ctsm_surface_water_state *sw,
+                            const ctsm_hydro_params *hprm, const ctsm_water_table_state *wt,
+                            const ctsm_drainage_state *dr, int32_t nc, const int32_t *filterc_dev, int32_t np,
+                            const int32_t *filterp_dev, double dtime, int32_t nsteps, ctsm_graph **out)
+{
+    if (!out) return CTSM_E_ARG;
+    *out = nullptr;
+    if (!ctx || !col || !pch || !can || !sf || nc < 0 || np < 0 || nsteps < 1 || check_canopy_layout(ctx, lay) ||
+        (!filterc_dev && nc > lay->ldc) || (!filterp_dev && np > lay->ldp))
+        return CTSM_E_ARG;
+    return capture_graph(ctx, nc, np, true, out, [&]() {
+        int rc = CTSM_OK;
+        for (int s = 0; s < nsteps && rc == CTSM_OK; ++s)
+            rc = ctsm_land_step_full_dev(ctx, lay, col, pch, can, sf, sw, hprm, wt, dr, nc, filterc_dev, np,
+                                         filterp_dev, dtime);
+        return rc;
+    });
 }
 
 int ctsm_graph_launch(ctsm_ctx *ctx, ctsm_graph *gr)

@@ -37,6 +37,7 @@ EXPORTS = [
     "ctsm_water_table", "ctsm_water_table_dev",
     "ctsm_hist_update_dev", "ctsm_hist_normalize_dev", "ctsm_hist_zero_dev",
     "ctsm_checkpoint_write", "ctsm_checkpoint_read", "ctsm_drainage", "ctsm_drainage_dev",
+    "ctsm_graph_capture_full",
 ]
 
 
@@ -97,6 +98,11 @@ def load():
         getattr(L, fn).argtypes = [C.c_void_p, P(abi.HistField), C.c_int32]
     for fn in ("ctsm_checkpoint_write", "ctsm_checkpoint_read"):
         getattr(L, fn).argtypes = [C.c_void_p, C.c_char_p, P(abi.CkptField), C.c_int32]
+    L.ctsm_graph_capture_full.argtypes = col + [P(abi.PatchState), P(abi.CanopyState), P(abi.SoilFluxState),
+                                                P(abi.SurfaceWaterState), P(abi.HydroParams),
+                                                P(abi.WaterTableState), P(abi.DrainageState),
+                                                C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_double, C.c_int32,
+                                                P(C.c_void_p)]
     L.ctsm_graph_launch.argtypes = [C.c_void_p, C.c_void_p]
     L.ctsm_graph_destroy.argtypes = [C.c_void_p]
     L.ctsm_sync.argtypes = [C.c_void_p]
@@ -396,6 +402,27 @@ class Context:
         self._check(self.L.ctsm_graph_capture(self.h, C.byref(ds.layout()), C.byref(cs), C.byref(ps),
                                               C.byref(can) if can is not None else None, nc, fc, np_, fp, dtime,
                                               nsteps, C.byref(g)), "ctsm_graph_capture")
+        return g.value
+
+    def graph_capture_full(self, ds, nsteps=1, hydro=None, water_table=False, drainage=False, filterc_dev=None,
+                           nc=None, filterp_dev=None, np_=None, dtime=1800.0):
+        """Capture `nsteps` full land steps (land_step_full_dev's arguments)
+        into one replayable graph; returns the graph handle."""
+        cs, ps = ds.structs()
+        can, sf = ds.canopy_struct(), ds.soilflux_struct()
+        sw = ds.surfwater_struct() if hydro is not None else None
+        wt = ds.watertable_struct() if water_table else None
+        dr = ds.drainage_struct() if drainage else None
+        nc = ds.dims.ncol if nc is None else nc
+        np_ = ds.dims.npatch if np_ is None else np_
+        fc = C.c_void_p(filterc_dev) if filterc_dev is not None else None
+        fp = C.c_void_p(filterp_dev) if filterp_dev is not None else None
+        ref = lambda x: C.byref(x) if x is not None else None
+        g = C.c_void_p()
+        self._check(self.L.ctsm_graph_capture_full(self.h, C.byref(ds.layout()), C.byref(cs), C.byref(ps),
+                                                   C.byref(can), C.byref(sf), ref(sw), ref(hydro), ref(wt), ref(dr),
+                                                   nc, fc, np_, fp, dtime, nsteps, C.byref(g)),
+                    "ctsm_graph_capture_full")
         return g.value
 
     def graph_launch(self, g):

@@ -720,6 +720,17 @@ int ctsm_graph_capture(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_sta
                        int32_t num_filterc, const int32_t *filterc_dev,
                        int32_t num_filterp, const int32_t *filterp_dev, double dtime,
                        int32_t nsteps, ctsm_graph **out);
+/* The same for `nsteps` full land steps (ctsm_land_step_full_dev with the
+ * given SoilFluxes, surface water, water-table and drainage states; sw, hprm,
+ * wt and dr may be NULL as there).  The balance check is not part of it: it
+ * waits for the stream. */
+int ctsm_graph_capture_full(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                            const ctsm_patch_state *pch, const ctsm_canopy_state *can,
+                            const ctsm_soilflux_state *sf, const ctsm_surface_water_state *sw,
+                            const ctsm_hydro_params *hprm, const ctsm_water_table_state *wt,
+                            const ctsm_drainage_state *dr, int32_t num_filterc, const int32_t *filterc_dev,
+                            int32_t num_filterp, const int32_t *filterp_dev, double dtime,
+                            int32_t nsteps, ctsm_graph **out);
 int ctsm_graph_launch(ctsm_ctx *ctx, ctsm_graph *graph);
 int ctsm_graph_destroy(ctsm_graph *graph);
 

@@ -75,3 +75,32 @@ def test_graph_capture_rejects_bad_arguments(ctx):
             ctx.graph_capture(ds, nsteps=1, canopy=False, nc=ds.dims.ncol + 1)
     finally:
         ds.free()
+
+
+def test_full_land_step_graph_replays_match_eager():
+    """The full land step (SoilFluxes, surface water, water tables, drainage)
+    captured for 2 steps; each replay equals 2 eager steps."""
+    from ctsm_amd.lib import Context
+    from oracle import oracle as O
+    c = Context()
+    c.set_canopy_params(S.canopy_params(), S.pft_table())
+    st = S.canopy_workload("canopy_uneven", ncol=512, seed=0xC75D0081)
+    print(f"case: full land-step graph of 2 steps, 512 columns / {st.dims.npatch} patches, 3 replays", flush=True)
+    hydro = O.hydro_params()
+    de, dg = DeviceState(st, c), DeviceState(st, c)
+    g = None
+    try:
+        g = c.graph_capture_full(dg, nsteps=2, hydro=hydro, water_table=True, drainage=True)
+        for r in range(3):
+            for _ in range(2):
+                c.land_step_full_dev(de, hydro=hydro, water_table=True, drainage=True)
+            c.sync()
+            c.graph_launch(g)
+            c.sync()
+            _same(dg.to_host(), de.to_host(), ("col", "pch", "can", "sf", "hy"), f"replay {r}")
+    finally:
+        if g is not None:
+            c.graph_destroy(g)
+        de.free()
+        dg.free()
+        c.close()
