@@ -1,0 +1,88 @@
+"""The checkpoint file format, on the CPU: the writer and reader of
+ctsm_amd/csrc/checkpoint.hip built by g++ against a host-only stand-in of the
+HIP calls (tests/host_hip), the file parsed independently here -- magic, field
+table, 4 KiB-aligned ranges, FNV-1a checksums over 64-bit words -- with fields
+larger than one 32 MiB staging chunk, empty fields and a range ending exactly on
+the alignment; a flipped byte or a wrong size is refused (CTSM_E_IO)."""
+import os
+import shutil
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SIZES = [100, (70 << 20) + 5, 0, 8, 12345, 3 << 20]     # the last ends on the 4 KiB alignment
+CTSM_E_IO = 7
+
+
+@pytest.fixture(scope="module")
+def tool(tmp_path_factory):
+    if not shutil.which("g++"):
+        pytest.skip("g++ not available")
+    d = tmp_path_factory.mktemp("ckpt")
+    exe = d / "ckpt"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-I", os.path.join(ROOT, "tests", "host_hip"),
+                    "-I", os.path.join(ROOT, "include"), "-x", "c++",
+                    os.path.join(ROOT, "ctsm_amd", "csrc", "checkpoint.hip"), "-x", "c++",
+                    os.path.join(ROOT, "tests", "host_hip", "ckpt_main.cpp"), "-o", str(exe)], check=True)
+    return exe, d
+
+
+def _run(exe, mode, path, sizes):
+    out = subprocess.run([str(exe), mode, str(path)] + [str(s) for s in sizes], capture_output=True, text=True,
+                         check=True).stdout.split()
+    return int(out[-1])
+
+
+def _fnv1a_words(b):
+    h = 0xcbf29ce484222325
+    m = (1 << 64) - 1
+    n8 = len(b) // 8
+    for w in np.frombuffer(b[:8 * n8], dtype="<u8").tolist():
+        h = ((h ^ w) * 0x100000001b3) & m
+    for x in b[8 * n8:]:
+        h = ((h ^ x) * 0x100000001b3) & m
+    return h
+
+
+def _pattern(i, n):
+    k = np.arange(n, dtype=np.uint64)
+    return ((k * 131 + i * 7) & 0xff).astype(np.uint8).tobytes()
+
+
+def test_file_layout_and_checksums(tool):
+    exe, d = tool
+    path = d / "a.ckpt"
+    assert _run(exe, "w", path, SIZES) == 0
+    raw = path.read_bytes()
+    assert raw[:8] == b"CTSMCKP1"
+    (n,) = struct.unpack("<I", raw[8:12])
+    assert n == len(SIZES)
+    ends = []
+    for i in range(n):
+        e = raw[16 + 88 * i:16 + 88 * (i + 1)]
+        name = e[:64].split(b"\0", 1)[0].decode()
+        size, off, csum = struct.unpack("<QQQ", e[64:])
+        assert name == f"field{i}" and size == SIZES[i] and off % 4096 == 0
+        data = raw[off:off + size]
+        assert data == _pattern(i, size), name
+        assert csum == _fnv1a_words(data), name
+        ends.append(off + size)
+    assert len(raw) >= max(ends)
+    assert _run(exe, "r", path, SIZES) == 0
+
+
+def test_damaged_or_mismatched_file_is_refused(tool):
+    exe, d = tool
+    path = d / "b.ckpt"
+    assert _run(exe, "w", path, SIZES) == 0
+    raw = bytearray(path.read_bytes())
+    e = raw[16 + 88 * 5:16 + 88 * 6]
+    size, off, _ = struct.unpack("<QQQ", bytes(e[64:]))
+    raw[off + size - 1] ^= 0x40                       # the last byte of the range that ends the file
+    bad = d / "bad.ckpt"
+    bad.write_bytes(bytes(raw))
+    assert _run(exe, "r", bad, SIZES) == CTSM_E_IO
+    assert _run(exe, "r", path, SIZES[:-1] + [SIZES[-1] - 8]) == CTSM_E_IO
