@@ -274,3 +274,35 @@ def test_land_step_full_with_drainage_bitwise(hctx, nsteps, name):
     assert worst < 1e-7, worst
     ns, nl = st.dims.nlevsno, st.dims.nlevsoi
     assert np.all(o.col["h2osoi_liq"][ns:ns + nl] >= 0.01 - 1e-12)     # the minimum-water corrections ran
+
+
+def test_full_land_step_48_steps_bitwise(hctx):
+    """One day (48 steps) of the full land step with every hydrology piece, on
+    snow and bedrock mixes: the GPU stays bit for bit on the oracle, the energy
+    balance closes every step and no soil layer falls below the minimum water."""
+    st = S.canopy_workload("canopy_uneven", ncol=96, seed=0xC75D0091)
+    print(f"case: 48 full land steps with hydrology and drainage, 96 columns / {st.dims.npatch} patches",
+          flush=True)
+    hydro = O.hydro_params()
+    ds = DeviceState(st, hctx)
+    worst = 0.0
+    try:
+        for _ in range(48):
+            hctx.land_step_full_dev(ds, hydro=hydro, water_table=True, drainage=True)
+            m, _ = hctx.soil_balance_check_dev(ds)
+            worst = max(worst, m)
+        hctx.sync()
+        g = ds.to_host()
+    finally:
+        ds.free()
+    o = st.copy()
+    cfg = O.canopy_config()
+    pft = O.PftStruct(S.pft_table())
+    for _ in range(48):
+        rc, bad = O.land_step_full(o, cfg, S.canopy_params(), pft, hydro=hydro, water_table_after=True,
+                                   drainage_after=True)
+        assert rc == 0, bad
+    _same(g, o, groups=("col", "pch", "can", "sf", "hy"))
+    assert worst < 1e-7, worst
+    ns, nl = st.dims.nlevsno, st.dims.nlevsoi
+    assert np.all(o.col["h2osoi_liq"][ns:ns + nl] >= 0.01 - 1e-12)
