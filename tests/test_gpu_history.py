@@ -136,3 +136,20 @@ def test_history_tape_over_land_steps(tmp_path):
         tape.free()
         ds.free()
         c.close()
+
+
+def test_hist_rejects_bad_descriptors(ctx):
+    from ctsm_amd.lib import CTSM_E_ARG, CtsmError
+    print("case: history descriptors with a bad flag / a short level stride", flush=True)
+    hb, na, src = ctx.alloc(8 * 20), ctx.alloc(4 * 20), ctx.alloc(8 * 20)
+    try:
+        for d in (abi.HistField(src, None, hb, na, 10, 10, 2, ord("L"), 0),
+                  abi.HistField(src, None, hb, na, 5, 10, 2, ord("A"), 0),
+                  abi.HistField(None, None, hb, na, 10, 10, 2, ord("A"), 0)):
+            arr = (abi.HistField * 1)(d)
+            with pytest.raises(CtsmError) as ei:
+                ctx.hist_update(arr)
+            assert ei.value.code == CTSM_E_ARG
+    finally:
+        for p in (hb, na, src):
+            ctx.free(p)

@@ -306,3 +306,31 @@ def test_full_land_step_48_steps_bitwise(hctx):
     assert worst < 1e-7, worst
     ns, nl = st.dims.nlevsno, st.dims.nlevsoi
     assert np.all(o.col["h2osoi_liq"][ns:ns + nl] >= 0.01 - 1e-12)
+
+
+def test_hydrology_entries_reject_bad_arguments(hctx):
+    """Filters outside the allocation and incomplete states are refused with
+    CTSM_E_ARG before anything runs; the context stays usable."""
+    from ctsm_amd.lib import CTSM_E_ARG, CtsmError
+    st = _dr_state(64, seed=0x5EED0063)
+    print("case: hydrology entries with bad arguments, 64 columns", flush=True)
+    hydro = O.hydro_params()
+    bad = np.array([1, 65], dtype=np.int32)
+    for call in (lambda: hctx.surface_water(st, hydro, filter_c=bad),
+                 lambda: hctx.water_table(st, filter_c=bad),
+                 lambda: hctx.drainage(st, hydro, filter_c=bad)):
+        with pytest.raises(CtsmError) as ei:
+            call()
+        assert ei.value.code == CTSM_E_ARG
+    # a drainage state without its outputs
+    import ctypes as C
+    cs, _ = st.structs()
+    dr = st.drainage_struct()
+    dr.qflx_runoff = None
+    rc = hctx.L.ctsm_drainage(hctx.h, C.byref(st.layout()), C.byref(cs), C.byref(dr), C.byref(hydro),
+                              st.dims.ncol, None, 1800.0)
+    assert rc == CTSM_E_ARG
+    g, o = st.copy(), st.copy()
+    hctx.drainage(g, hydro)
+    assert O.drainage(o, hydro) == 0
+    _same(g, o)
