@@ -38,5 +38,37 @@ def build(verbose=False, extra=()):
     return LIB
 
 
+FORTRAN_DRIVER = os.path.join(HERE, "fortran", "drive_soilwater")
+
+
+def flang():
+    for p in (os.environ.get("FLANG"), "/opt/rocm/lib/llvm/bin/flang"):
+        if p and os.path.exists(p):
+            return p
+    return None
+
+
+def build_fortran_driver(verbose=False):
+    """The Fortran SoilWater driver (ctsm_gpu_mod + drive_soilwater.F90, linked
+    to the in-tree library); returns its path, or None without flang."""
+    fc = flang()
+    if fc is None:
+        return None
+    import tempfile
+    fdir = os.path.join(HERE, "fortran")
+    with tempfile.TemporaryDirectory() as mod_dir:
+        cmd = [fc, "-O2", "-J", mod_dir, os.path.join(fdir, "ctsm_gpu_mod.F90"), os.path.join(fdir, "drive_soilwater.F90"),
+               "-L", HERE, "-lctsm_mi355x", "-Wl,-rpath,$ORIGIN/..", "-o", FORTRAN_DRIVER + ".tmp"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("flang failed")
+    os.replace(FORTRAN_DRIVER + ".tmp", FORTRAN_DRIVER)
+    return FORTRAN_DRIVER
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_fortran_driver(verbose=True))

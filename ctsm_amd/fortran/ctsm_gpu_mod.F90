@@ -18,7 +18,7 @@ module ctsm_gpu_mod
   private
 
   integer(c_int), parameter, public :: CTSM_OK = 0, CTSM_E_ARG = 1, CTSM_E_CONFIG = 2, &
-       CTSM_E_HIP = 3, CTSM_E_SOLVER = 4, CTSM_E_NODEVICE = 5, CTSM_E_ENDRUN = 6
+       CTSM_E_HIP = 3, CTSM_E_SOLVER = 4, CTSM_E_NODEVICE = 5, CTSM_E_ENDRUN = 6, CTSM_E_IO = 7
   integer(c_int), parameter, public :: CTSM_Z0_ZENGWANG2007 = 0, CTSM_Z0_MEIER2022 = 1
   integer(c_int), parameter, public :: CTSM_STOMATA_BB1987 = 1, CTSM_STOMATA_MEDLYN2011 = 2
   integer(c_int), parameter, public :: CTSM_SOILRESIS_LEEPIELKE1992 = 0, CTSM_SOILRESIS_SL14 = 1
