@@ -374,7 +374,10 @@ int oracle_drainage(const ctsm_config *cfg, const ctsm_layout *L, const ctsm_col
     double *imped_col = (double *)calloc((size_t)ncol, sizeof(double));
     double *drainage = (double *)calloc((size_t)ncol, sizeof(double));
     int fc, k, j, i;
-    if (!k_frost || !k_perch || !jwt || !out || !icef || !imped_col || !drainage) return 1;
+    if (!k_frost || !k_perch || !jwt || !out || !icef || !imped_col || !drainage) {
+        free(k_frost); free(k_perch); free(jwt); free(out); free(icef); free(imped_col); free(drainage);
+        return 1;
+    }
 #define COL(fc) (filter ? filter[fc] : L->begc_alloc + (fc))
 #define IX(c) ((c) - Lo.begc)
 #define LIQ(c, k) HSNO(S->h2osoi_liq, c, k)
