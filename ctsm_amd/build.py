@@ -39,6 +39,7 @@ def build(verbose=False, extra=()):
 
 
 FORTRAN_DRIVER = os.path.join(HERE, "fortran", "drive_soilwater")
+FORTRAN_STEP_DRIVER = os.path.join(HERE, "fortran", "drive_step")
 
 
 def flang():
@@ -49,24 +50,26 @@ def flang():
 
 
 def build_fortran_driver(verbose=False):
-    """The Fortran SoilWater driver (ctsm_gpu_mod + drive_soilwater.F90, linked
-    to the in-tree library); returns its path, or None without flang."""
+    """The Fortran drivers (ctsm_gpu_mod with drive_soilwater.F90 and with
+    drive_step.F90, linked to the in-tree library); returns their paths, or
+    None without flang."""
     fc = flang()
     if fc is None:
         return None
     import tempfile
     fdir = os.path.join(HERE, "fortran")
-    with tempfile.TemporaryDirectory() as mod_dir:
-        cmd = [fc, "-O2", "-J", mod_dir, os.path.join(fdir, "ctsm_gpu_mod.F90"), os.path.join(fdir, "drive_soilwater.F90"),
-               "-L", HERE, "-lctsm_mi355x", "-Wl,-rpath,$ORIGIN/..", "-o", FORTRAN_DRIVER + ".tmp"]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("flang failed")
-    os.replace(FORTRAN_DRIVER + ".tmp", FORTRAN_DRIVER)
-    return FORTRAN_DRIVER
+    for prog, exe in (("drive_soilwater.F90", FORTRAN_DRIVER), ("drive_step.F90", FORTRAN_STEP_DRIVER)):
+        with tempfile.TemporaryDirectory() as mod_dir:
+            cmd = [fc, "-O2", "-J", mod_dir, os.path.join(fdir, "ctsm_gpu_mod.F90"), os.path.join(fdir, prog),
+                   "-L", HERE, "-lctsm_mi355x", "-Wl,-rpath,$ORIGIN/..", "-o", exe + ".tmp"]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("flang failed")
+        os.replace(exe + ".tmp", exe)
+    return FORTRAN_DRIVER, FORTRAN_STEP_DRIVER
 
 
 if __name__ == "__main__":

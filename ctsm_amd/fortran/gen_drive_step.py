@@ -1,0 +1,155 @@
+"""Writes drive_step.F90, a Fortran program that calls SoilWater,
+SoilTemperature or CanopyFluxes through ctsm_gpu_mod on a state held the way
+CTSM holds it (one allocatable array per member with the reference's bounds,
+bound by c_loc; members shared between structs point at the same array).
+
+The member lists come from ctsm_amd/abi.py, so the program follows the
+header; run this script again after a state struct changes:
+
+    python ctsm_amd/fortran/gen_drive_step.py
+"""
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from ctsm_amd import abi  # noqa: E402
+from ctsm_amd.state import CANOPY_ALIAS  # noqa: E402
+
+BOUNDS = {"c": "(begc:endc)", "p": "(begp:endp)", "g": "(begg:endg)",
+          "sno": "(begc:endc, -nlevsno+1:nlevgrnd)", "zi": "(begc:endc, -nlevsno:nlevgrnd)",
+          "soi": "(begc:endc, 1:nlevgrnd)", "snow": "(begc:endc, -nlevsno+1:0)",
+          "lyr": "(begp:endp, -nlevsno+1:1)", "soip": "(begp:endp, 1:nlevgrnd)"}
+FIRST = {"c": "(begc)", "p": "(begp)", "g": "(begg)", "sno": "(begc, -nlevsno+1)", "zi": "(begc, -nlevsno)",
+         "soi": "(begc, 1)", "snow": "(begc, -nlevsno+1)", "lyr": "(begp, -nlevsno+1)", "soip": "(begp, 1)"}
+
+
+def members():
+    """(struct, prefix, name, type, kind) of every array the program holds, in file order."""
+    out = [("col", "c_", n, t, k) for n, t, k in abi.COL_FIELDS]
+    out += [("pch", "p_", n, t, k) for n, t, k in abi.PATCH_FIELDS]
+    out += [("can", "g_", n, t, k) for n, t, k in abi.CANOPY_FIELDS if n not in CANOPY_ALIAS]
+    return out
+
+
+def generate():
+    decl, alloc, rd, bind, wr = [], [], [], [], []
+    for struct, pre, n, t, k in members():
+        typ = "integer(c_int32_t)" if t == "i32" else "real(c_double)"
+        rank = "(:)" if k in ("c", "p", "g") else "(:,:)"
+        v = pre + n
+        decl.append(f"  {typ}, allocatable, target :: {v}{rank}")
+        alloc.append(f"  allocate({v}{BOUNDS[k]})")
+        rd.append(f"  read(u) {v}")
+        bind.append(f"  {struct}%{n} = c_loc({v}{FIRST[k]})")
+        wr.append(f"  write(u) {v}")
+    for n, (grp, k) in CANOPY_ALIAS.items():
+        bind.append(f"  can%{n} = {'pch' if grp == 'p' else 'col'}%{k}")
+    pft_real = [n for n, t in abi.PFT_FIELDS if t == "f64"]
+    pft_int = [n for n, t in abi.PFT_FIELDS if t == "i32"]
+    pft_bind = [f"  pft%{n} = c_loc(pftbuf(1, {i + 1}))" for i, n in enumerate(pft_real)]
+    pft_bind += [f"  pft%{n} = c_loc(pftibuf(1, {i + 1}))" for i, n in enumerate(pft_int)]
+    nl = "\n"
+    return f'''! drive_step.F90 -- written by gen_drive_step.py from the member lists of
+! ctsm_amd/abi.py; do not edit by hand.
+!
+! One of SoilWater, SoilTemperature or CanopyFluxes from Fortran through
+! ctsm_gpu_mod, on a state held the way CTSM holds it: one allocatable array per
+! member with the reference's bounds, bound by c_loc of its first element;
+! canopy members that are column or patch arrays point at those arrays.
+!
+! Usage: drive_step IN OUT.  IN holds, as int32, ncol, npatch, ngrid, nlevsno,
+! nlevsoi, nlevgrnd, mode (1 SoilWater, 2 SoilTemperature, 3 CanopyFluxes) and
+! npft; the ctsm_config bytes; the canopy parameters (doubles) and the PFT
+! table (npft values per field: the real fields, then the integer ones, each in
+! member order); then every column, patch and canopy array in member order.
+! OUT receives the arrays after the call.  Prints the return code.
+program drive_step
+  use iso_c_binding
+  use ctsm_gpu_mod
+  implicit none
+  integer(c_int32_t) :: ncol, npatch, ngrid, nlevsno, nlevsoi, nlevgrnd, mode, npft
+  integer :: begc, endc, begp, endp, begg, endg, u, i, rc
+  character(len=4096) :: fin, fout
+  type(ctsm_config_t) :: cfg
+  type(ctsm_layout_t) :: lay
+  type(ctsm_col_state_t) :: col
+  type(ctsm_patch_state_t) :: pch
+  type(ctsm_canopy_state_t) :: can
+  type(ctsm_canopy_params_t) :: prm
+  type(ctsm_pft_params_t) :: pft
+  type(c_ptr) :: ctx
+  integer(c_int32_t), allocatable :: filter(:)
+  real(c_double), allocatable, target :: pftbuf(:,:)
+  integer(c_int32_t), allocatable, target :: pftibuf(:,:)
+{nl.join(decl)}
+
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+  open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
+  read(u) ncol, npatch, ngrid, nlevsno, nlevsoi, nlevgrnd, mode, npft
+  read(u) cfg
+  read(u) prm
+  allocate(pftbuf(npft, {len(pft_real)}), pftibuf(npft, {len(pft_int)}))
+  read(u) pftbuf
+  read(u) pftibuf
+  begc = 1
+  endc = ncol
+  begp = 1
+  endp = npatch
+  begg = 1
+  endg = ngrid
+{nl.join(alloc)}
+{nl.join(rd)}
+  close(u)
+
+{nl.join(bind)}
+  pft%npft = npft
+{nl.join(pft_bind)}
+
+  rc = ctsm_create(cfg, 0, ctx)
+  if (rc /= CTSM_OK) then
+     print '(I0)', rc
+     stop 1
+  end if
+  lay%begc_alloc = begc
+  lay%ldc = endc - begc + 1
+  lay%begp_alloc = begp
+  lay%ldp = endp - begp + 1
+  lay%begg_alloc = begg
+  lay%ldg = endg - begg + 1
+  select case (mode)
+  case (1)
+     allocate(filter(ncol))
+     filter = [(i, i = begc, endc)]
+     rc = ctsm_soilwater(ctx, lay, col, ncol, filter, 1800.0_c_double)
+  case (2)
+     allocate(filter(ncol))
+     filter = [(i, i = begc, endc)]
+     rc = ctsm_soiltemperature(ctx, lay, col, pch, ncol, filter, 1800.0_c_double)
+  case (3)
+     rc = ctsm_set_canopy_params(ctx, prm, pft)
+     if (rc == CTSM_OK) then
+        allocate(filter(npatch))
+        filter = [(i, i = begp, endp)]
+        rc = ctsm_canopyfluxes(ctx, lay, col, can, npatch, filter, 1800.0_c_double)
+     end if
+  case default
+     rc = CTSM_E_ARG
+  end select
+  print '(I0)', rc
+  i = ctsm_destroy(ctx)
+
+  open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
+{nl.join(wr)}
+  close(u)
+end program drive_step
+'''
+
+
+if __name__ == "__main__":
+    path = os.path.join(HERE, "drive_step.F90")
+    with open(path, "w") as f:
+        f.write(generate())
+    print(path)

@@ -44,3 +44,102 @@ def test_soilwater_from_fortran_bitwise(tmp_path, name, ncol):
     for n in o.col:
         np.testing.assert_array_equal(got[n], o.col[n], err_msg=n)
     assert np.any(o.col["h2osoi_liq"] != st.col["h2osoi_liq"])
+
+
+def _write_step_input(path, st, mode, cfg, prm=None, pft=None):
+    from ctsm_amd.state import CANOPY_ALIAS
+    d = st.dims
+    npft = len(pft["dleaf"]) if pft is not None else 1
+    with open(path, "wb") as f:
+        f.write(np.array([d.ncol, d.npatch, max(d.ngrid, 1), d.nlevsno, d.nlevsoi, d.nlevgrnd, mode, npft],
+                         dtype=np.int32).tobytes())
+        f.write(bytes(cfg))
+        f.write(bytes(prm) if prm is not None else bytes(abi.CanopyParams()))
+        for typ, dt in (("f64", np.float64), ("i32", np.int32)):
+            names = [n for n, t in abi.PFT_FIELDS if t == typ]
+            tab = np.zeros((len(names), npft), dtype=dt)
+            for i, n in enumerate(names):
+                if pft is not None:
+                    tab[i] = pft[n]
+            f.write(np.ascontiguousarray(tab).tobytes())     # (npft, field) in Fortran order
+        arrays = [st.col[n] for n, _, _ in abi.COL_FIELDS] + [st.pch[n] for n, _, _ in abi.PATCH_FIELDS]
+        if st.can:
+            arrays += [st.can[n] for n, _, _ in abi.CANOPY_FIELDS if n not in CANOPY_ALIAS]
+        else:
+            arrays += [np.zeros(_shape(st, k), dtype=np.int32 if t == "i32" else np.float64)
+                       for n, t, k in abi.CANOPY_FIELDS if n not in CANOPY_ALIAS]
+        for a in arrays:
+            f.write(np.ascontiguousarray(a).tobytes())
+    return arrays
+
+
+def _shape(st, kind):
+    d = st.dims
+    n = {"p": d.npatch, "soip": d.npatch, "lyr": d.npatch, "g": max(d.ngrid, 1)}.get(kind, d.ncol)
+    nl = abi.nlev_of(kind, d.nlevsno, d.nlevgrnd)
+    return (n,) if kind in ("c", "p", "g") else (nl, n)
+
+
+def _run_step(tmp_path, st, mode, cfg, prm=None, pft=None):
+    exe = build.FORTRAN_STEP_DRIVER
+    if not os.path.exists(exe):
+        pytest.skip("Fortran driver not built (no flang)")
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    arrays = _write_step_input(fin, st, mode, cfg, prm, pft)
+    r = subprocess.run([exe, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert int(r.stdout.split()[0]) == 0, r.stdout
+    raw = fout.read_bytes()
+    out, pos = [], 0
+    for a in arrays:
+        out.append(np.frombuffer(raw[pos:pos + a.nbytes], dtype=a.dtype).reshape(a.shape))
+        pos += a.nbytes
+    assert pos == len(raw)
+    return out
+
+
+def _compare(st, got, o):
+    from ctsm_amd.state import CANOPY_ALIAS
+    k = 0
+    for n, _, _ in abi.COL_FIELDS:
+        np.testing.assert_array_equal(got[k], o.col[n], err_msg="col." + n)
+        k += 1
+    for n, _, _ in abi.PATCH_FIELDS:
+        np.testing.assert_array_equal(got[k], o.pch[n], err_msg="pch." + n)
+        k += 1
+    if st.can:
+        for n, _, _ in abi.CANOPY_FIELDS:
+            if n in CANOPY_ALIAS:
+                continue
+            np.testing.assert_array_equal(got[k], o.can[n], err_msg="can." + n)
+            k += 1
+
+
+def test_soiltemperature_from_fortran_bitwise(tmp_path):
+    st = S.workload("uneven", ncol=256)
+    print("case: SoilTemperature from Fortran, uneven, 256 columns", flush=True)
+    cfg = O.canopy_config()          # the library's defaults (the soil path reads only its soil members)
+    got = _run_step(tmp_path, st, 2, cfg)
+    o = st.copy()
+    rc, bad = O.soiltemperature(o, cfg)
+    assert rc == 0, bad
+    _compare(st, got, o)
+    assert np.any(o.col["t_soisno"] != st.col["t_soisno"])
+
+
+def test_canopyfluxes_from_fortran_bitwise(tmp_path):
+    st = S.canopy_workload("canopy_uneven", ncol=128)
+    print(f"case: CanopyFluxes from Fortran, 128 columns / {st.dims.npatch} patches", flush=True)
+    cfg = O.canopy_config()
+    prm = O.canopy_params_struct(S.canopy_params())
+    pft = S.pft_table()
+    o = st.copy()
+    o.pch["cgrnd"][:] = 0.0         # as the land step zeroes them before CanopyFluxes
+    o.can["cgrnds"][:] = 0.0
+    o.can["cgrndl"][:] = 0.0
+    st2 = o.copy()
+    got = _run_step(tmp_path, st2, 3, cfg, prm, pft)
+    rc, bad = O.canopyfluxes(o, cfg, S.canopy_params(), O.PftStruct(pft))
+    assert rc == 0, bad
+    _compare(st2, got, o)
+    assert np.any(o.can["t_veg"] != st.can["t_veg"])
