@@ -7,10 +7,12 @@
 ! canopy members that are column or patch arrays point at those arrays.
 !
 ! Usage: drive_step IN OUT.  IN holds, as int32, ncol, npatch, ngrid, nlevsno,
-! nlevsoi, nlevgrnd, mode (1 SoilWater, 2 SoilTemperature, 3 CanopyFluxes) and
-! npft; the ctsm_config bytes; the canopy parameters (doubles) and the PFT
-! table (npft values per field: the real fields, then the integer ones, each in
-! member order); then every column, patch and canopy array in member order.
+! nlevsoi, nlevgrnd, mode (1 SoilWater, 2 SoilTemperature, 3 CanopyFluxes,
+! 4 surface water, 5 water tables, 6 drainage) and npft; the ctsm_config bytes;
+! the canopy parameters (doubles), the PFT table (npft values per field: the
+! real fields, then the integer ones, each in member order) and the hydrology
+! parameters; then every column, patch, canopy and hydrology array in member
+! order.
 ! OUT receives the arrays after the call.  Prints the return code.
 program drive_step
   use iso_c_binding
@@ -26,6 +28,10 @@ program drive_step
   type(ctsm_canopy_state_t) :: can
   type(ctsm_canopy_params_t) :: prm
   type(ctsm_pft_params_t) :: pft
+  type(ctsm_surface_water_state_t) :: sw
+  type(ctsm_water_table_state_t) :: wt
+  type(ctsm_drainage_state_t) :: dr
+  type(ctsm_hydro_params_t) :: hprm
   type(c_ptr) :: ctx
   integer(c_int32_t), allocatable :: filter(:)
   real(c_double), allocatable, target :: pftbuf(:,:)
@@ -264,6 +270,46 @@ program drive_step
   real(c_double), allocatable, target :: g_lnca(:)
   integer(c_int32_t), allocatable, target :: g_c3flag(:)
   real(c_double), allocatable, target :: g_canopy_err(:)
+  real(c_double), allocatable, target :: y_qflx_rain_plus_snomelt(:)
+  real(c_double), allocatable, target :: y_qflx_snow_h2osfc(:)
+  real(c_double), allocatable, target :: y_qflx_floodc(:)
+  real(c_double), allocatable, target :: y_qflx_liqevap_from_top_layer(:)
+  real(c_double), allocatable, target :: y_qflx_ev_soil(:)
+  real(c_double), allocatable, target :: y_qflx_ev_h2osfc(:)
+  real(c_double), allocatable, target :: y_zwt(:)
+  real(c_double), allocatable, target :: y_zwt_perched(:)
+  real(c_double), allocatable, target :: y_frost_table(:)
+  real(c_double), allocatable, target :: y_wtfact(:)
+  real(c_double), allocatable, target :: y_topo_slope(:)
+  real(c_double), allocatable, target :: y_h2osfc_thresh(:)
+  real(c_double), allocatable, target :: y_frac_h2osfc_nosnow(:)
+  real(c_double), allocatable, target :: y_fsat(:)
+  real(c_double), allocatable, target :: y_fcov(:)
+  real(c_double), allocatable, target :: y_qflx_sat_excess_surf(:)
+  real(c_double), allocatable, target :: y_qflx_top_soil(:)
+  real(c_double), allocatable, target :: y_qflx_in_soil(:)
+  real(c_double), allocatable, target :: y_qflx_top_soil_to_h2osfc(:)
+  real(c_double), allocatable, target :: y_qinmax(:)
+  real(c_double), allocatable, target :: y_qflx_infl_excess(:)
+  real(c_double), allocatable, target :: y_qflx_in_soil_limited(:)
+  real(c_double), allocatable, target :: y_qflx_in_h2osfc(:)
+  real(c_double), allocatable, target :: y_qflx_infl_excess_surf(:)
+  real(c_double), allocatable, target :: y_qflx_h2osfc_surf(:)
+  real(c_double), allocatable, target :: y_qflx_h2osfc_drain(:)
+  real(c_double), allocatable, target :: y_qflx_surf(:)
+  real(c_double), allocatable, target :: y_h2osoi_vol(:,:)
+  real(c_double), allocatable, target :: y_qflx_liqdew_to_top_layer(:)
+  real(c_double), allocatable, target :: y_qflx_soliddew_to_top_layer(:)
+  real(c_double), allocatable, target :: y_qflx_solidevap_from_top_layer(:)
+  real(c_double), allocatable, target :: y_qflx_snwcp_liq(:)
+  real(c_double), allocatable, target :: y_qflx_drain_perched(:)
+  real(c_double), allocatable, target :: y_qflx_latflow_out(:)
+  real(c_double), allocatable, target :: y_qflx_latflow_in(:)
+  real(c_double), allocatable, target :: y_qflx_rsub_sat(:)
+  real(c_double), allocatable, target :: y_qflx_ice_runoff_xs(:)
+  real(c_double), allocatable, target :: y_qflx_drain(:)
+  real(c_double), allocatable, target :: y_qflx_qrgwl(:)
+  real(c_double), allocatable, target :: y_qflx_runoff(:)
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
@@ -274,6 +320,7 @@ program drive_step
   allocate(pftbuf(npft, 22), pftibuf(npft, 2))
   read(u) pftbuf
   read(u) pftibuf
+  read(u) hprm
   begc = 1
   endc = ncol
   begp = 1
@@ -514,6 +561,46 @@ program drive_step
   allocate(g_lnca(begp:endp))
   allocate(g_c3flag(begp:endp))
   allocate(g_canopy_err(begp:endp))
+  allocate(y_qflx_rain_plus_snomelt(begc:endc))
+  allocate(y_qflx_snow_h2osfc(begc:endc))
+  allocate(y_qflx_floodc(begc:endc))
+  allocate(y_qflx_liqevap_from_top_layer(begc:endc))
+  allocate(y_qflx_ev_soil(begc:endc))
+  allocate(y_qflx_ev_h2osfc(begc:endc))
+  allocate(y_zwt(begc:endc))
+  allocate(y_zwt_perched(begc:endc))
+  allocate(y_frost_table(begc:endc))
+  allocate(y_wtfact(begc:endc))
+  allocate(y_topo_slope(begc:endc))
+  allocate(y_h2osfc_thresh(begc:endc))
+  allocate(y_frac_h2osfc_nosnow(begc:endc))
+  allocate(y_fsat(begc:endc))
+  allocate(y_fcov(begc:endc))
+  allocate(y_qflx_sat_excess_surf(begc:endc))
+  allocate(y_qflx_top_soil(begc:endc))
+  allocate(y_qflx_in_soil(begc:endc))
+  allocate(y_qflx_top_soil_to_h2osfc(begc:endc))
+  allocate(y_qinmax(begc:endc))
+  allocate(y_qflx_infl_excess(begc:endc))
+  allocate(y_qflx_in_soil_limited(begc:endc))
+  allocate(y_qflx_in_h2osfc(begc:endc))
+  allocate(y_qflx_infl_excess_surf(begc:endc))
+  allocate(y_qflx_h2osfc_surf(begc:endc))
+  allocate(y_qflx_h2osfc_drain(begc:endc))
+  allocate(y_qflx_surf(begc:endc))
+  allocate(y_h2osoi_vol(begc:endc, 1:nlevgrnd))
+  allocate(y_qflx_liqdew_to_top_layer(begc:endc))
+  allocate(y_qflx_soliddew_to_top_layer(begc:endc))
+  allocate(y_qflx_solidevap_from_top_layer(begc:endc))
+  allocate(y_qflx_snwcp_liq(begc:endc))
+  allocate(y_qflx_drain_perched(begc:endc))
+  allocate(y_qflx_latflow_out(begc:endc))
+  allocate(y_qflx_latflow_in(begc:endc))
+  allocate(y_qflx_rsub_sat(begc:endc))
+  allocate(y_qflx_ice_runoff_xs(begc:endc))
+  allocate(y_qflx_drain(begc:endc))
+  allocate(y_qflx_qrgwl(begc:endc))
+  allocate(y_qflx_runoff(begc:endc))
   read(u) c_snl
   read(u) c_nbedrock
   read(u) c_patchi
@@ -748,6 +835,46 @@ program drive_step
   read(u) g_lnca
   read(u) g_c3flag
   read(u) g_canopy_err
+  read(u) y_qflx_rain_plus_snomelt
+  read(u) y_qflx_snow_h2osfc
+  read(u) y_qflx_floodc
+  read(u) y_qflx_liqevap_from_top_layer
+  read(u) y_qflx_ev_soil
+  read(u) y_qflx_ev_h2osfc
+  read(u) y_zwt
+  read(u) y_zwt_perched
+  read(u) y_frost_table
+  read(u) y_wtfact
+  read(u) y_topo_slope
+  read(u) y_h2osfc_thresh
+  read(u) y_frac_h2osfc_nosnow
+  read(u) y_fsat
+  read(u) y_fcov
+  read(u) y_qflx_sat_excess_surf
+  read(u) y_qflx_top_soil
+  read(u) y_qflx_in_soil
+  read(u) y_qflx_top_soil_to_h2osfc
+  read(u) y_qinmax
+  read(u) y_qflx_infl_excess
+  read(u) y_qflx_in_soil_limited
+  read(u) y_qflx_in_h2osfc
+  read(u) y_qflx_infl_excess_surf
+  read(u) y_qflx_h2osfc_surf
+  read(u) y_qflx_h2osfc_drain
+  read(u) y_qflx_surf
+  read(u) y_h2osoi_vol
+  read(u) y_qflx_liqdew_to_top_layer
+  read(u) y_qflx_soliddew_to_top_layer
+  read(u) y_qflx_solidevap_from_top_layer
+  read(u) y_qflx_snwcp_liq
+  read(u) y_qflx_drain_perched
+  read(u) y_qflx_latflow_out
+  read(u) y_qflx_latflow_in
+  read(u) y_qflx_rsub_sat
+  read(u) y_qflx_ice_runoff_xs
+  read(u) y_qflx_drain
+  read(u) y_qflx_qrgwl
+  read(u) y_qflx_runoff
   close(u)
 
   col%snl = c_loc(c_snl(begc))
@@ -998,6 +1125,57 @@ program drive_step
   can%qflx_ev_soil = pch%qflx_ev_soil
   can%qflx_ev_h2osfc = pch%qflx_ev_h2osfc
   can%qflx_rootsoi_col = col%qflx_rootsoi
+  sw%qflx_rain_plus_snomelt = c_loc(y_qflx_rain_plus_snomelt(begc))
+  sw%qflx_snow_h2osfc = c_loc(y_qflx_snow_h2osfc(begc))
+  sw%qflx_floodc = c_loc(y_qflx_floodc(begc))
+  sw%qflx_liqevap_from_top_layer = c_loc(y_qflx_liqevap_from_top_layer(begc))
+  sw%qflx_ev_soil = c_loc(y_qflx_ev_soil(begc))
+  sw%qflx_ev_h2osfc = c_loc(y_qflx_ev_h2osfc(begc))
+  sw%zwt = c_loc(y_zwt(begc))
+  sw%zwt_perched = c_loc(y_zwt_perched(begc))
+  sw%frost_table = c_loc(y_frost_table(begc))
+  sw%wtfact = c_loc(y_wtfact(begc))
+  sw%topo_slope = c_loc(y_topo_slope(begc))
+  sw%h2osfc_thresh = c_loc(y_h2osfc_thresh(begc))
+  sw%frac_h2osfc_nosnow = c_loc(y_frac_h2osfc_nosnow(begc))
+  sw%h2osfc = col%h2osfc
+  sw%fsat = c_loc(y_fsat(begc))
+  sw%fcov = c_loc(y_fcov(begc))
+  sw%qflx_sat_excess_surf = c_loc(y_qflx_sat_excess_surf(begc))
+  sw%qflx_top_soil = c_loc(y_qflx_top_soil(begc))
+  sw%qflx_in_soil = c_loc(y_qflx_in_soil(begc))
+  sw%qflx_top_soil_to_h2osfc = c_loc(y_qflx_top_soil_to_h2osfc(begc))
+  sw%qinmax = c_loc(y_qinmax(begc))
+  sw%qflx_infl_excess = c_loc(y_qflx_infl_excess(begc))
+  sw%qflx_in_soil_limited = c_loc(y_qflx_in_soil_limited(begc))
+  sw%qflx_in_h2osfc = c_loc(y_qflx_in_h2osfc(begc))
+  sw%qflx_infl_excess_surf = c_loc(y_qflx_infl_excess_surf(begc))
+  sw%qflx_h2osfc_surf = c_loc(y_qflx_h2osfc_surf(begc))
+  sw%qflx_h2osfc_drain = c_loc(y_qflx_h2osfc_drain(begc))
+  sw%qflx_infl = col%qflx_infl
+  sw%qflx_surf = c_loc(y_qflx_surf(begc))
+  wt%zwt = c_loc(y_zwt(begc))
+  wt%zwt_perched = c_loc(y_zwt_perched(begc))
+  wt%frost_table = c_loc(y_frost_table(begc))
+  wt%h2osoi_vol = c_loc(y_h2osoi_vol(begc, 1))
+  wt%qflx_liqdew_to_top_layer = c_loc(y_qflx_liqdew_to_top_layer(begc))
+  wt%qflx_soliddew_to_top_layer = c_loc(y_qflx_soliddew_to_top_layer(begc))
+  wt%qflx_solidevap_from_top_layer = c_loc(y_qflx_solidevap_from_top_layer(begc))
+  dr%topo_slope = c_loc(y_topo_slope(begc))
+  dr%zwt_perched = c_loc(y_zwt_perched(begc))
+  dr%frost_table = c_loc(y_frost_table(begc))
+  dr%qflx_surf = c_loc(y_qflx_surf(begc))
+  dr%qflx_snwcp_liq = c_loc(y_qflx_snwcp_liq(begc))
+  dr%zwt = c_loc(y_zwt(begc))
+  dr%h2osoi_vol = c_loc(y_h2osoi_vol(begc, 1))
+  dr%qflx_drain_perched = c_loc(y_qflx_drain_perched(begc))
+  dr%qflx_latflow_out = c_loc(y_qflx_latflow_out(begc))
+  dr%qflx_latflow_in = c_loc(y_qflx_latflow_in(begc))
+  dr%qflx_rsub_sat = c_loc(y_qflx_rsub_sat(begc))
+  dr%qflx_ice_runoff_xs = c_loc(y_qflx_ice_runoff_xs(begc))
+  dr%qflx_drain = c_loc(y_qflx_drain(begc))
+  dr%qflx_qrgwl = c_loc(y_qflx_qrgwl(begc))
+  dr%qflx_runoff = c_loc(y_qflx_runoff(begc))
   pft%npft = npft
   pft%dleaf = c_loc(pftbuf(1, 1))
   pft%c3psn = c_loc(pftbuf(1, 2))
@@ -1051,6 +1229,18 @@ program drive_step
         filter = [(i, i = begp, endp)]
         rc = ctsm_canopyfluxes(ctx, lay, col, can, npatch, filter, 1800.0_c_double)
      end if
+  case (4)
+     allocate(filter(ncol))
+     filter = [(i, i = begc, endc)]
+     rc = ctsm_surface_water(ctx, lay, col, sw, hprm, ncol, filter, 1800.0_c_double)
+  case (5)
+     allocate(filter(ncol))
+     filter = [(i, i = begc, endc)]
+     rc = ctsm_water_table(ctx, lay, col, wt, ncol, filter, 1800.0_c_double)
+  case (6)
+     allocate(filter(ncol))
+     filter = [(i, i = begc, endc)]
+     rc = ctsm_drainage(ctx, lay, col, dr, hprm, ncol, filter, 1800.0_c_double)
   case default
      rc = CTSM_E_ARG
   end select
@@ -1292,5 +1482,45 @@ program drive_step
   write(u) g_lnca
   write(u) g_c3flag
   write(u) g_canopy_err
+  write(u) y_qflx_rain_plus_snomelt
+  write(u) y_qflx_snow_h2osfc
+  write(u) y_qflx_floodc
+  write(u) y_qflx_liqevap_from_top_layer
+  write(u) y_qflx_ev_soil
+  write(u) y_qflx_ev_h2osfc
+  write(u) y_zwt
+  write(u) y_zwt_perched
+  write(u) y_frost_table
+  write(u) y_wtfact
+  write(u) y_topo_slope
+  write(u) y_h2osfc_thresh
+  write(u) y_frac_h2osfc_nosnow
+  write(u) y_fsat
+  write(u) y_fcov
+  write(u) y_qflx_sat_excess_surf
+  write(u) y_qflx_top_soil
+  write(u) y_qflx_in_soil
+  write(u) y_qflx_top_soil_to_h2osfc
+  write(u) y_qinmax
+  write(u) y_qflx_infl_excess
+  write(u) y_qflx_in_soil_limited
+  write(u) y_qflx_in_h2osfc
+  write(u) y_qflx_infl_excess_surf
+  write(u) y_qflx_h2osfc_surf
+  write(u) y_qflx_h2osfc_drain
+  write(u) y_qflx_surf
+  write(u) y_h2osoi_vol
+  write(u) y_qflx_liqdew_to_top_layer
+  write(u) y_qflx_soliddew_to_top_layer
+  write(u) y_qflx_solidevap_from_top_layer
+  write(u) y_qflx_snwcp_liq
+  write(u) y_qflx_drain_perched
+  write(u) y_qflx_latflow_out
+  write(u) y_qflx_latflow_in
+  write(u) y_qflx_rsub_sat
+  write(u) y_qflx_ice_runoff_xs
+  write(u) y_qflx_drain
+  write(u) y_qflx_qrgwl
+  write(u) y_qflx_runoff
   close(u)
 end program drive_step

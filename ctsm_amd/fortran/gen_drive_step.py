@@ -1,5 +1,6 @@
 """Writes drive_step.F90, a Fortran program that calls SoilWater,
-SoilTemperature or CanopyFluxes through ctsm_gpu_mod on a state held the way
+SoilTemperature, CanopyFluxes or one of the hydrology entries (surface water,
+water tables, drainage) through ctsm_gpu_mod on a state held the way
 CTSM holds it (one allocatable array per member with the reference's bounds,
 bound by c_loc; members shared between structs point at the same array).
 
@@ -15,7 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
 from ctsm_amd import abi  # noqa: E402
-from ctsm_amd.state import CANOPY_ALIAS  # noqa: E402
+from ctsm_amd.state import (CANOPY_ALIAS, DRAINAGE_OWN, SURFWATER_ALIAS, SURFWATER_OWN,  # noqa: E402
+                            WATERTABLE_OWN)
 
 BOUNDS = {"c": "(begc:endc)", "p": "(begp:endp)", "g": "(begg:endg)",
           "sno": "(begc:endc, -nlevsno+1:nlevgrnd)", "zi": "(begc:endc, -nlevsno:nlevgrnd)",
@@ -30,6 +32,21 @@ def members():
     out = [("col", "c_", n, t, k) for n, t, k in abi.COL_FIELDS]
     out += [("pch", "p_", n, t, k) for n, t, k in abi.PATCH_FIELDS]
     out += [("can", "g_", n, t, k) for n, t, k in abi.CANOPY_FIELDS if n not in CANOPY_ALIAS]
+    out += [(None, "y_", n, t, k) for n, t, k in SURFWATER_OWN + WATERTABLE_OWN + DRAINAGE_OWN]
+    return out
+
+
+def hydro_binds():
+    """Members of the surface water, water-table and drainage states: the
+    column state's arrays where they are the same, else the hydrology arrays."""
+    out = []
+    for struct, fields in (("sw", abi.SURFWATER_FIELDS), ("wt", abi.WATERTABLE_FIELDS),
+                           ("dr", abi.DRAINAGE_FIELDS)):
+        for n, _, k in fields:
+            if n in SURFWATER_ALIAS:
+                out.append(f"  {struct}%{n} = col%{SURFWATER_ALIAS[n][1]}")
+            else:
+                out.append(f"  {struct}%{n} = c_loc(y_{n}{FIRST[k]})")
     return out
 
 
@@ -42,10 +59,12 @@ def generate():
         decl.append(f"  {typ}, allocatable, target :: {v}{rank}")
         alloc.append(f"  allocate({v}{BOUNDS[k]})")
         rd.append(f"  read(u) {v}")
-        bind.append(f"  {struct}%{n} = c_loc({v}{FIRST[k]})")
+        if struct is not None:
+            bind.append(f"  {struct}%{n} = c_loc({v}{FIRST[k]})")
         wr.append(f"  write(u) {v}")
     for n, (grp, k) in CANOPY_ALIAS.items():
         bind.append(f"  can%{n} = {'pch' if grp == 'p' else 'col'}%{k}")
+    bind += hydro_binds()
     pft_real = [n for n, t in abi.PFT_FIELDS if t == "f64"]
     pft_int = [n for n, t in abi.PFT_FIELDS if t == "i32"]
     pft_bind = [f"  pft%{n} = c_loc(pftbuf(1, {i + 1}))" for i, n in enumerate(pft_real)]
@@ -60,10 +79,12 @@ def generate():
 ! canopy members that are column or patch arrays point at those arrays.
 !
 ! Usage: drive_step IN OUT.  IN holds, as int32, ncol, npatch, ngrid, nlevsno,
-! nlevsoi, nlevgrnd, mode (1 SoilWater, 2 SoilTemperature, 3 CanopyFluxes) and
-! npft; the ctsm_config bytes; the canopy parameters (doubles) and the PFT
-! table (npft values per field: the real fields, then the integer ones, each in
-! member order); then every column, patch and canopy array in member order.
+! nlevsoi, nlevgrnd, mode (1 SoilWater, 2 SoilTemperature, 3 CanopyFluxes,
+! 4 surface water, 5 water tables, 6 drainage) and npft; the ctsm_config bytes;
+! the canopy parameters (doubles), the PFT table (npft values per field: the
+! real fields, then the integer ones, each in member order) and the hydrology
+! parameters; then every column, patch, canopy and hydrology array in member
+! order.
 ! OUT receives the arrays after the call.  Prints the return code.
 program drive_step
   use iso_c_binding
@@ -79,6 +100,10 @@ program drive_step
   type(ctsm_canopy_state_t) :: can
   type(ctsm_canopy_params_t) :: prm
   type(ctsm_pft_params_t) :: pft
+  type(ctsm_surface_water_state_t) :: sw
+  type(ctsm_water_table_state_t) :: wt
+  type(ctsm_drainage_state_t) :: dr
+  type(ctsm_hydro_params_t) :: hprm
   type(c_ptr) :: ctx
   integer(c_int32_t), allocatable :: filter(:)
   real(c_double), allocatable, target :: pftbuf(:,:)
@@ -94,6 +119,7 @@ program drive_step
   allocate(pftbuf(npft, {len(pft_real)}), pftibuf(npft, {len(pft_int)}))
   read(u) pftbuf
   read(u) pftibuf
+  read(u) hprm
   begc = 1
   endc = ncol
   begp = 1
@@ -135,6 +161,18 @@ program drive_step
         filter = [(i, i = begp, endp)]
         rc = ctsm_canopyfluxes(ctx, lay, col, can, npatch, filter, 1800.0_c_double)
      end if
+  case (4)
+     allocate(filter(ncol))
+     filter = [(i, i = begc, endc)]
+     rc = ctsm_surface_water(ctx, lay, col, sw, hprm, ncol, filter, 1800.0_c_double)
+  case (5)
+     allocate(filter(ncol))
+     filter = [(i, i = begc, endc)]
+     rc = ctsm_water_table(ctx, lay, col, wt, ncol, filter, 1800.0_c_double)
+  case (6)
+     allocate(filter(ncol))
+     filter = [(i, i = begc, endc)]
+     rc = ctsm_drainage(ctx, lay, col, dr, hprm, ncol, filter, 1800.0_c_double)
   case default
      rc = CTSM_E_ARG
   end select

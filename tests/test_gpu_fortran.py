@@ -46,7 +46,7 @@ def test_soilwater_from_fortran_bitwise(tmp_path, name, ncol):
     assert np.any(o.col["h2osoi_liq"] != st.col["h2osoi_liq"])
 
 
-def _write_step_input(path, st, mode, cfg, prm=None, pft=None):
+def _write_step_input(path, st, mode, cfg, prm=None, pft=None, hydro=None):
     from ctsm_amd.state import CANOPY_ALIAS
     d = st.dims
     npft = len(pft["dleaf"]) if pft is not None else 1
@@ -62,12 +62,14 @@ def _write_step_input(path, st, mode, cfg, prm=None, pft=None):
                 if pft is not None:
                     tab[i] = pft[n]
             f.write(np.ascontiguousarray(tab).tobytes())     # (npft, field) in Fortran order
+        f.write(bytes(hydro if hydro is not None else abi.HydroParams()))
         arrays = [st.col[n] for n, _, _ in abi.COL_FIELDS] + [st.pch[n] for n, _, _ in abi.PATCH_FIELDS]
         if st.can:
             arrays += [st.can[n] for n, _, _ in abi.CANOPY_FIELDS if n not in CANOPY_ALIAS]
         else:
             arrays += [np.zeros(_shape(st, k), dtype=np.int32 if t == "i32" else np.float64)
                        for n, t, k in abi.CANOPY_FIELDS if n not in CANOPY_ALIAS]
+        arrays += list(st.hy.values())
         for a in arrays:
             f.write(np.ascontiguousarray(a).tobytes())
     return arrays
@@ -80,12 +82,12 @@ def _shape(st, kind):
     return (n,) if kind in ("c", "p", "g") else (nl, n)
 
 
-def _run_step(tmp_path, st, mode, cfg, prm=None, pft=None):
+def _run_step(tmp_path, st, mode, cfg, prm=None, pft=None, hydro=None):
     exe = build.FORTRAN_STEP_DRIVER
     if not os.path.exists(exe):
         pytest.skip("Fortran driver not built (no flang)")
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
-    arrays = _write_step_input(fin, st, mode, cfg, prm, pft)
+    arrays = _write_step_input(fin, st, mode, cfg, prm, pft, hydro)
     r = subprocess.run([exe, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert int(r.stdout.split()[0]) == 0, r.stdout
@@ -107,12 +109,15 @@ def _compare(st, got, o):
     for n, _, _ in abi.PATCH_FIELDS:
         np.testing.assert_array_equal(got[k], o.pch[n], err_msg="pch." + n)
         k += 1
-    if st.can:
-        for n, _, _ in abi.CANOPY_FIELDS:
-            if n in CANOPY_ALIAS:
-                continue
+    for n, _, _ in abi.CANOPY_FIELDS:
+        if n in CANOPY_ALIAS:
+            continue
+        if st.can:
             np.testing.assert_array_equal(got[k], o.can[n], err_msg="can." + n)
-            k += 1
+        k += 1
+    for n in o.hy:
+        np.testing.assert_array_equal(got[k], o.hy[n], err_msg="hy." + n)
+        k += 1
 
 
 def test_soiltemperature_from_fortran_bitwise(tmp_path):
@@ -143,3 +148,21 @@ def test_canopyfluxes_from_fortran_bitwise(tmp_path):
     assert rc == 0, bad
     _compare(st2, got, o)
     assert np.any(o.can["t_veg"] != st.can["t_veg"])
+
+
+@pytest.mark.parametrize("mode", [4, 5, 6])
+def test_hydrology_from_fortran_bitwise(tmp_path, mode):
+    from tests.test_gpu_hydrology import _state, _wt_state, _dr_state
+    st = {4: lambda: _state(300), 5: lambda: _wt_state(300), 6: lambda: _dr_state(300)}[mode]()
+    print(f"case: hydrology entry {mode} from Fortran, 300 columns", flush=True)
+    cfg = O.canopy_config()
+    hydro = O.hydro_params()
+    got = _run_step(tmp_path, st, mode, cfg, hydro=hydro)
+    o = st.copy()
+    if mode == 4:
+        assert O.surface_water(o, hydro, cfg) == 0
+    elif mode == 5:
+        assert O.water_table(o, cfg) == 0
+    else:
+        assert O.drainage(o, hydro, cfg) == 0
+    _compare(st, got, o)
