@@ -364,6 +364,18 @@ DRAINAGE_FIELDS = [(n, "f64", "soi" if n == "h2osoi_vol" else "c") for n in (
     "qflx_drain", "qflx_qrgwl", "qflx_runoff")]
 
 
+# plant hydraulic stress, soil side (ctsm_phs_state): patch members, then column members
+PHS_FIELDS = [
+    ("itype", "i32", "p"), ("column", "i32", "p"), ("active", "i32", "p"), ("frac_veg_nosno", "i32", "p"),
+    ("wtcol", "f64", "p"), ("froot_carbon", "f64", "p"), ("tsai", "f64", "p"), ("tlai", "f64", "p"),
+    ("rootfr", "f64", "soip"), ("vegwp", "f64", "veg"), ("k_soil_root", "f64", "soip"),
+    ("root_conductance", "f64", "soip"), ("soil_conductance", "f64", "soip"), ("qflx_hydr_redist", "f64", "p"),
+    ("patchi", "i32", "c"), ("patchf", "i32", "c"), ("dz", "f64", "sno"), ("z", "f64", "sno"),
+    ("hksat", "f64", "soi"), ("hk_l", "f64", "soi"), ("smp_l", "f64", "soi"), ("qflx_rootsoi", "f64", "soi"),
+    ("qflx_phs_neg", "f64", "c"),
+]
+
+
 class HydroParams(C.Structure):
     _fields_ = [("fff", C.c_double), ("pc", C.c_double), ("mu", C.c_double), ("aq_sp_yield_min", C.c_double),
                 ("n_baseflow", C.c_double), ("baseflow_scalar", C.c_double), ("perched_baseflow_scalar", C.c_double),
@@ -410,6 +422,15 @@ class SurfaceWaterState(C.Structure):
 
 class DrainageState(C.Structure):
     _fields_ = [(n, _CT[t]) for n, t, _ in DRAINAGE_FIELDS]
+
+
+class PhsState(C.Structure):
+    _fields_ = [(n, _CT[t]) for n, t, _ in PHS_FIELDS]
+
+
+class PhsParams(C.Structure):
+    _fields_ = [("npft", C.c_int32), ("pad", C.c_int32)] + [
+        (n, f64p) for n in ("krmax", "root_radius", "root_density", "froot_leaf", "psi50", "ck")]
 
 
 class WaterTableState(C.Structure):

@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SRC = ["capi.hip", "soilwater.hip", "soiltemp.hip", "plain.hip", "canopy.hip", "soilfluxes.hip", "hydrology.hip",
-       "history.hip", "checkpoint.hip"]
+       "history.hip", "checkpoint.hip", "phs.hip"]
 LIB = os.path.join(HERE, "libctsm_mi355x.so")
 
 # -ffp-contract=off: no FMA contraction, every expression rounds as the

@@ -56,6 +56,10 @@ void launch_water_table(hipStream_t, const ctsm_layout &, const ctsm_config &, c
 void launch_hist(hipStream_t, const ctsm_hist_field *, int, int);
 void launch_drainage(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_hydro_params &,
                      const ctsm_col_state &, const ctsm_drainage_state &, int, const int32_t *, double, ErrWord *);
+void launch_phs_conductance(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_phs_state &,
+                            const double *, int, int, const int32_t *, ErrWord *);
+void launch_phs_root_sink(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_phs_state &, int,
+                          const int32_t *, ErrWord *);
 int ckpt_write(hipStream_t, const char *, const ctsm_ckpt_field *, int, hipError_t *);
 int ckpt_read(hipStream_t, const char *, const ctsm_ckpt_field *, int, hipError_t *);
 }  // namespace ctsm
@@ -64,7 +68,7 @@ using namespace ctsm;
 
 namespace {
 
-enum Kind { K_C, K_SNO, K_ZI, K_SOI, K_SNOW, K_P, K_LYR, K_G, K_SOIP };
+enum Kind { K_C, K_SNO, K_ZI, K_SOI, K_SNOW, K_P, K_LYR, K_G, K_SOIP, K_VEG };
 
 struct FieldMeta {
     size_t offset;   // offset of the pointer member in the state struct
@@ -237,6 +241,25 @@ const FieldMeta kDr[] = {
 constexpr int kNDr = sizeof(kDr) / sizeof(kDr[0]);
 static_assert(kNDr * sizeof(void *) == sizeof(ctsm_drainage_state),
               "every ctsm_drainage_state member has a mirror entry");
+// plant hydraulic stress, patch members then column members
+#define PH(name, kind, elem, wr) {offsetof(ctsm_phs_state, name), kind, elem, wr}
+const FieldMeta kPhsP[] = {
+    PH(itype, K_P, 4, false), PH(column, K_P, 4, false), PH(active, K_P, 4, false),
+    PH(frac_veg_nosno, K_P, 4, false), PH(wtcol, K_P, 8, false), PH(froot_carbon, K_P, 8, false),
+    PH(tsai, K_P, 8, false), PH(tlai, K_P, 8, false), PH(rootfr, K_SOIP, 8, false), PH(vegwp, K_VEG, 8, false),
+    PH(k_soil_root, K_SOIP, 8, true), PH(root_conductance, K_SOIP, 8, true), PH(soil_conductance, K_SOIP, 8, true),
+    PH(qflx_hydr_redist, K_P, 8, true),
+};
+const FieldMeta kPhsC[] = {
+    PH(patchi, K_C, 4, false), PH(patchf, K_C, 4, false), PH(dz, K_SNO, 8, false), PH(z, K_SNO, 8, false),
+    PH(hksat, K_SOI, 8, false), PH(hk_l, K_SOI, 8, false), PH(smp_l, K_SOI, 8, false),
+    PH(qflx_rootsoi, K_SOI, 8, true), PH(qflx_phs_neg, K_C, 8, true),
+};
+#undef PH
+constexpr int kNPhsP = sizeof(kPhsP) / sizeof(kPhsP[0]);
+constexpr int kNPhsC = sizeof(kPhsC) / sizeof(kPhsC[0]);
+static_assert((kNPhsP + kNPhsC) * sizeof(void *) == sizeof(ctsm_phs_state),
+              "every ctsm_phs_state member has a mirror entry");
 constexpr int kNSfP = sizeof(kSfPatch) / sizeof(kSfPatch[0]);
 constexpr int kNSfC = sizeof(kSfCol) / sizeof(kSfCol[0]);
 static_assert((kNSfP + kNSfC) * sizeof(void *) == sizeof(ctsm_soilflux_state),
@@ -254,6 +277,7 @@ int nlev_of(Kind k, const ctsm_config &cfg)
     switch (k) {
     case K_C: case K_P: case K_G: return 1;
     case K_SOIP: return cfg.nlevgrnd;
+    case K_VEG: return 4;
     case K_SNO: return cfg.nlevsno + cfg.nlevgrnd;
     case K_ZI: return cfg.nlevsno + cfg.nlevgrnd + 1;
     case K_SOI: return cfg.nlevgrnd;
@@ -339,6 +363,9 @@ struct ctsm_ctx {
     std::vector<DevBuf> swm;           // surface-water drop-in mirrors
     std::vector<DevBuf> wtm;           // water-table drop-in mirrors
     std::vector<DevBuf> drm;           // drainage drop-in mirrors
+    std::vector<DevBuf> phsp, phsc;    // plant-hydraulics drop-in mirrors
+    DevBuf phs_tab;                    // PHS parameter table (12 x CTSM_MAX_PFT doubles)
+    int32_t phs_npft = 0;              // 0 until ctsm_set_phs_params
     bool have_canopy = false;
     int32_t last_code = 0, last_index = 0;
     std::mutex mu;
@@ -629,6 +656,9 @@ int ctsm_destroy(ctsm_ctx *ctx)
     for (auto &b : ctx->swm) b.release();
     for (auto &b : ctx->wtm) b.release();
     for (auto &b : ctx->drm) b.release();
+    for (auto &b : ctx->phsp) b.release();
+    for (auto &b : ctx->phsc) b.release();
+    ctx->phs_tab.release();
     ctx->partf.release();
     for (int k = 0; k < ctsm_ctx::kMaxStreams - 1; ++k) {
         if (ctx->side[k]) { (void)hipStreamSynchronize(ctx->side[k]); (void)hipStreamDestroy(ctx->side[k]); }
@@ -1364,6 +1394,136 @@ int ctsm_drainage(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *c
     if ((rc = mirror(ctx, kColFields, ncf, &hc, &dcol, ctx->colmir, lay->begc_alloc, lay->ldc, cr, false))) return rc;
     if ((rc = mirror(ctx, kDr, kNDr, dr, &ddr, ctx->drm, lay->begc_alloc, lay->ldc, cr, false))) return rc;
     return sync_and_check(ctx);
+}
+
+int ctsm_set_phs_params(ctsm_ctx *ctx, const ctsm_phs_params *prm)
+{
+    if (!ctx || !prm || prm->npft < 1 || prm->npft > CTSM_MAX_PFT) return CTSM_E_ARG;
+    const double *rows[] = {prm->krmax, prm->root_radius, prm->root_density, prm->froot_leaf};
+    for (const double *a : rows) if (!a) return CTSM_E_ARG;
+    if (!prm->psi50 || !prm->ck) return CTSM_E_ARG;
+    // rows: krmax, root_radius, root_density, froot_leaf, psi50 of the 4 segments, ck of the 4 segments
+    const int n = prm->npft;
+    std::vector<double> tab((size_t)12 * CTSM_MAX_PFT, 0.0);
+    for (int r = 0; r < 4; ++r)
+        for (int i = 0; i < n; ++i) tab[(size_t)r * CTSM_MAX_PFT + i] = rows[r][i];
+    for (int k = 0; k < 4; ++k)
+        for (int i = 0; i < n; ++i) {
+            tab[(size_t)(4 + k) * CTSM_MAX_PFT + i] = prm->psi50[(size_t)k * n + i];
+            tab[(size_t)(8 + k) * CTSM_MAX_PFT + i] = prm->ck[(size_t)k * n + i];
+        }
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e);
+    if ((e = ctx->phs_tab.ensure(tab.size() * sizeof(double))) != hipSuccess) return hip_fail(ctx, e);
+    if ((e = hipMemcpyAsync(ctx->phs_tab.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice,
+                            ctx->stream)) != hipSuccess)
+        return hip_fail(ctx, e);
+    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return hip_fail(ctx, e);
+    ctx->phs_npft = n;
+    return CTSM_OK;
+}
+
+// The members each PHS call reads or writes; the rest of a caller's struct is
+// neither checked nor copied.
+static ctsm_phs_state phs_pick(const ctsm_phs_state *s, bool conductance)
+{
+    ctsm_phs_state o{};
+    o.z = s->z; o.smp_l = s->smp_l; o.k_soil_root = s->k_soil_root;
+    if (conductance) {
+        o.itype = s->itype; o.column = s->column; o.froot_carbon = s->froot_carbon; o.tsai = s->tsai;
+        o.tlai = s->tlai; o.rootfr = s->rootfr; o.root_conductance = s->root_conductance;
+        o.soil_conductance = s->soil_conductance; o.dz = s->dz; o.hksat = s->hksat; o.hk_l = s->hk_l;
+    } else {
+        o.active = s->active; o.frac_veg_nosno = s->frac_veg_nosno; o.wtcol = s->wtcol; o.vegwp = s->vegwp;
+        o.qflx_hydr_redist = s->qflx_hydr_redist; o.patchi = s->patchi; o.patchf = s->patchf;
+        o.qflx_rootsoi = s->qflx_rootsoi; o.qflx_phs_neg = s->qflx_phs_neg;
+    }
+    return o;
+}
+
+static bool phs_complete(const ctsm_phs_state *s, bool conductance)
+{
+    ctsm_phs_state probe;
+    std::memset(&probe, 0xff, sizeof(probe));
+    const ctsm_phs_state need = phs_pick(&probe, conductance);   // non-null where the call uses the member
+    for (size_t off = 0; off < sizeof(ctsm_phs_state); off += sizeof(void *))
+        if (*(void *const *)((const char *)&need + off) && !*(void *const *)((const char *)s + off)) return false;
+    return true;
+}
+
+static int phs_check(const ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_phs_state *s, int32_t n,
+                     bool conductance, bool has_filter)
+{
+    if (!ctx || !s || n < 0 || check_layout(lay) || lay->ldp <= 0) return CTSM_E_ARG;
+    if (!has_filter && n > (conductance ? lay->ldp : lay->ldc)) return CTSM_E_ARG;
+    if (conductance && ctx->phs_npft == 0) return CTSM_E_ARG;
+    if (ctx->cfg.nlevsoi > MAX_NLEVSOI) return CTSM_E_ARG;
+    return phs_complete(s, conductance) ? CTSM_OK : CTSM_E_ARG;
+}
+
+int ctsm_phs_soil_root_conductance_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_phs_state *phs,
+                                       int32_t n, const int32_t *filter_dev)
+{
+    int rc = phs_check(ctx, lay, phs, n, true, filter_dev != nullptr);
+    if (rc) return rc;
+    launch_phs_conductance(ctx->stream, *lay, ctx->cfg, *phs, (const double *)ctx->phs_tab.p, ctx->phs_npft, n,
+                           filter_dev, ctx->err);
+    return finish_launch(ctx);
+}
+
+int ctsm_phs_root_sink_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_phs_state *phs, int32_t n,
+                           const int32_t *filter_dev)
+{
+    int rc = phs_check(ctx, lay, phs, n, false, filter_dev != nullptr);
+    if (rc) return rc;
+    launch_phs_root_sink(ctx->stream, *lay, ctx->cfg, *phs, n, filter_dev, ctx->err);
+    return finish_launch(ctx);
+}
+
+// Drop-in form of both: mirror the members the call uses over the whole
+// allocation, run, copy the outputs back.
+static int phs_dropin(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_phs_state *phs, int32_t n,
+                      const int32_t *filter, bool conductance)
+{
+    int rc = phs_check(ctx, lay, phs, n, conductance, filter != nullptr);
+    if (rc) return rc;
+    const int beg = conductance ? lay->begp_alloc : lay->begc_alloc, ld = conductance ? lay->ldp : lay->ldc;
+    for (int i = 0; filter && i < n; ++i)
+        if (filter[i] < beg || filter[i] >= beg + ld) return CTSM_E_ARG;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipError_t he = hipSetDevice(ctx->device);
+    if (he != hipSuccess) return hip_fail(ctx, he);
+    const ctsm_phs_state h = phs_pick(phs, conductance);
+    const Range cr{lay->begc_alloc, lay->begc_alloc + lay->ldc - 1};
+    const Range pr{lay->begp_alloc, lay->begp_alloc + lay->ldp - 1};
+    ctsm_phs_state d{};
+    rc = mirror(ctx, kPhsP, kNPhsP, &h, &d, ctx->phsp, lay->begp_alloc, lay->ldp, pr, true);
+    if (!rc) rc = mirror(ctx, kPhsC, kNPhsC, &h, &d, ctx->phsc, lay->begc_alloc, lay->ldc, cr, true);
+    if (rc) return rc;
+    const int32_t *df = nullptr;
+    if ((rc = upload_filter(ctx, n, filter, &df))) return rc;
+    if (conductance)
+        launch_phs_conductance(ctx->stream, *lay, ctx->cfg, d, (const double *)ctx->phs_tab.p, ctx->phs_npft, n, df,
+                               ctx->err);
+    else
+        launch_phs_root_sink(ctx->stream, *lay, ctx->cfg, d, n, df, ctx->err);
+    if ((rc = finish_launch(ctx))) return rc;
+    if ((rc = mirror(ctx, kPhsP, kNPhsP, &h, &d, ctx->phsp, lay->begp_alloc, lay->ldp, pr, false))) return rc;
+    if ((rc = mirror(ctx, kPhsC, kNPhsC, &h, &d, ctx->phsc, lay->begc_alloc, lay->ldc, cr, false))) return rc;
+    return sync_and_check(ctx);
+}
+
+int ctsm_phs_soil_root_conductance(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_phs_state *phs, int32_t n,
+                                   const int32_t *filterp)
+{
+    return phs_dropin(ctx, lay, phs, n, filterp, true);
+}
+
+int ctsm_phs_root_sink(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_phs_state *phs, int32_t n,
+                       const int32_t *filterc)
+{
+    return phs_dropin(ctx, lay, phs, n, filterc, false);
 }
 
 int ctsm_land_step_full_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,

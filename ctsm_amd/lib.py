@@ -37,7 +37,8 @@ EXPORTS = [
     "ctsm_water_table", "ctsm_water_table_dev",
     "ctsm_hist_update_dev", "ctsm_hist_normalize_dev", "ctsm_hist_zero_dev",
     "ctsm_checkpoint_write", "ctsm_checkpoint_read", "ctsm_drainage", "ctsm_drainage_dev",
-    "ctsm_graph_capture_full",
+    "ctsm_graph_capture_full", "ctsm_set_phs_params", "ctsm_phs_soil_root_conductance",
+    "ctsm_phs_soil_root_conductance_dev", "ctsm_phs_root_sink", "ctsm_phs_root_sink_dev",
 ]
 
 
@@ -94,6 +95,10 @@ def load():
     dra = col + [P(abi.DrainageState), P(abi.HydroParams), C.c_int32, C.c_void_p, C.c_double]
     L.ctsm_drainage.argtypes = dra
     L.ctsm_drainage_dev.argtypes = dra
+    L.ctsm_set_phs_params.argtypes = [C.c_void_p, P(abi.PhsParams)]
+    for fn in ("ctsm_phs_soil_root_conductance", "ctsm_phs_soil_root_conductance_dev", "ctsm_phs_root_sink",
+               "ctsm_phs_root_sink_dev"):
+        getattr(L, fn).argtypes = [C.c_void_p, P(abi.Layout), P(abi.PhsState), C.c_int32, C.c_void_p]
     for fn in ("ctsm_hist_update_dev", "ctsm_hist_normalize_dev", "ctsm_hist_zero_dev"):
         getattr(L, fn).argtypes = [C.c_void_p, P(abi.HistField), C.c_int32]
     for fn in ("ctsm_checkpoint_write", "ctsm_checkpoint_read"):
@@ -348,6 +353,25 @@ class Context:
         fp = C.c_void_p(filter_dev) if filter_dev is not None else None
         self._check(self.L.ctsm_water_table_dev(self.h, C.byref(ds.layout()), C.byref(cs), C.byref(wt), n, fp,
                                                 dtime), "water table")
+
+    def set_phs_params(self, prm):
+        """ctsm_set_phs_params with an abi.PhsParams over host arrays."""
+        self._check(self.L.ctsm_set_phs_params(self.h, C.byref(prm)), "set_phs_params")
+
+    def phs_soil_root_conductance(self, lay, phs, n, filt=None, dev=False):
+        """Root-soil conductances of PhotosynthesisHydraulicStress over n patches
+        (filt: host int32 array, or a device pointer with dev=True; None = the
+        first n patches of the layout).  phs holds host arrays, or device arrays
+        with dev=True."""
+        fn = self.L.ctsm_phs_soil_root_conductance_dev if dev else self.L.ctsm_phs_soil_root_conductance
+        fp = C.c_void_p(filt) if dev and filt is not None else (None if filt is None else filt.ctypes.data_as(C.c_void_p))
+        self._check(fn(self.h, C.byref(lay), C.byref(phs), n, fp), "phs soil-root conductance")
+
+    def phs_root_sink(self, lay, phs, n, filt=None, dev=False):
+        """Compute_EffecRootFrac_And_VertTranSink_HydStress over n columns (as above)."""
+        fn = self.L.ctsm_phs_root_sink_dev if dev else self.L.ctsm_phs_root_sink
+        fp = C.c_void_p(filt) if dev and filt is not None else (None if filt is None else filt.ctypes.data_as(C.c_void_p))
+        self._check(fn(self.h, C.byref(lay), C.byref(phs), n, fp), "phs root sink")
 
     def drainage(self, st, hydro, filter_c=None, dtime=1800.0):
         """Drop-in drainage of HydrologyDrainage on a host state."""

@@ -464,6 +464,46 @@ typedef struct ctsm_hydro_params {
     int32_t pad;
 } ctsm_hydro_params;
 
+/* ---- plant hydraulic stress, soil side (use_hydrstress = .true.) -----------
+ * Arrays of the root-soil interface conductance block of
+ * PhotosynthesisHydraulicStress (PhotosynthesisMod.F90:3062-3114) and of
+ * Compute_EffecRootFrac_And_VertTranSink_HydStress (SoilWaterPlantSinkMod.F90:
+ * 236-328).  Members that CTSM keeps in the column and patch types (itype,
+ * column, active, frac_veg_nosno, wtcol, rootfr, patchi/patchf, dz, z, hksat,
+ * hk_l, smp_l, qflx_rootsoi) are the same arrays as in ctsm_col_state /
+ * ctsm_canopy_state.  [VEG] arrays are (begp:endp, 1:4) with the patch
+ * fastest: vegwp(p, k) for the segments sun, sha, xyl, root (k = 1..4). */
+typedef struct ctsm_phs_state {
+    /* patch */
+    const int32_t *itype, *column, *active, *frac_veg_nosno;   /* (p) */
+    const double  *wtcol;                                      /* (p) */
+    const double  *froot_carbon;                               /* (p) fine root carbon (gC/m2) */
+    const double  *tsai, *tlai;                                /* (p) stem / leaf area index, no snow burial */
+    const double  *rootfr;                                     /* [SOIP] root fraction */
+    const double  *vegwp;                                      /* [VEG] vegetation water potential (mm) */
+    double        *k_soil_root;                                /* [SOIP] out soil-root conductance (1/s) */
+    double        *root_conductance, *soil_conductance;        /* [SOIP] out */
+    double        *qflx_hydr_redist;                           /* (p) out negative root-soil flux (mm H2O/s) */
+    /* column */
+    const int32_t *patchi, *patchf;                            /* (c) */
+    const double  *dz, *z;                                     /* [SNO] (m) */
+    const double  *hksat, *hk_l, *smp_l;                       /* [SOI] (mm/s, mm/s, mm) */
+    double        *qflx_rootsoi;                               /* [SOI] out root-soil exchange (mm H2O/s) */
+    double        *qflx_phs_neg;                               /* (c) out sum of negative layer exchanges */
+} ctsm_phs_state;
+
+/* PHS parameters of the param file (PhotosynthesisMod readParams) and pftcon,
+ * host arrays indexed by the vegetation type (0..npft-1): krmax (root
+ * conductance per area per length), root_radius (m), root_density (g biomass /
+ * m3 root), froot_leaf (fine root to leaf ratio); psi50 (mm) and ck of the
+ * Weibull vulnerability curve, segment-major: psi50[(k-1)*npft + ivt]. */
+typedef struct ctsm_phs_params {
+    int32_t       npft;
+    int32_t       pad;
+    const double *krmax, *root_radius, *root_density, *froot_leaf;
+    const double *psi50, *ck;
+} ctsm_phs_params;
+
 /* ---- launch shapes (no counterpart in the reference: set by the autotuner) --
  * richards_lanes: lanes cooperating on one column in the Richards solve
  *   (1, 2, 4 or 8); the heat solve always runs one column per lane.
@@ -612,6 +652,29 @@ int ctsm_drainage(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *c
                   const ctsm_drainage_state *dr, const ctsm_hydro_params *prm,
                   int32_t num_hydrologyc, const int32_t *filter_hydrologyc, double dtime);
 
+/* Stores the PHS parameters in the context (copied to the device; replaces
+ * params_inst%krmax / psi50 / ck and pftcon%root_radius / root_density /
+ * froot_leaf, PhotosynthesisMod.F90:3062-3114 and :5167-5195).  Must precede
+ * the two calls below; CTSM_E_ARG for npft outside 1..CTSM_MAX_PFT or a NULL
+ * table. */
+int ctsm_set_phs_params(ctsm_ctx *ctx, const ctsm_phs_params *prm);
+
+/* Replaces the root-soil interface conductance loop at the head of
+ * PhotosynthesisHydraulicStress (PhotosynthesisMod.F90:3062-3114): per patch of
+ * the filter and soil layer, the Gardner soil conductance, the root conductance
+ * attenuated by the root segment's vulnerability curve plc(smp) (:5167-5195),
+ * and their series conductance k_soil_root (zero in the top layer and where
+ * there are no roots). */
+int ctsm_phs_soil_root_conductance(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_phs_state *phs,
+                                   int32_t num_filterp, const int32_t *filterp);
+
+/* Replaces Compute_EffecRootFrac_And_VertTranSink_HydStress
+ * (SoilWaterPlantSinkMod.F90:236-328): per column of the filter and soil layer
+ * the patch-weighted root-soil exchange k_soil_root (smp - vegwp(root) - z),
+ * its negative part per column and per patch. */
+int ctsm_phs_root_sink(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_phs_state *phs,
+                       int32_t num_filterc, const int32_t *filterc);
+
 /* ---- device-resident entry points -----------------------------------------
  * Same arguments, but every pointer (fields and filter) is device memory and
  * the call only enqueues work on the context's stream.  filter == NULL means
@@ -659,6 +722,10 @@ int ctsm_water_table_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_s
 int ctsm_drainage_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
                       const ctsm_drainage_state *dr, const ctsm_hydro_params *prm,
                       int32_t num_filterc, const int32_t *filterc_dev, double dtime);
+int ctsm_phs_soil_root_conductance_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_phs_state *phs,
+                                       int32_t num_filterp, const int32_t *filterp_dev);
+int ctsm_phs_root_sink_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_phs_state *phs,
+                           int32_t num_filterc, const int32_t *filterc_dev);
 int ctsm_soilfluxes_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
                         const ctsm_patch_state *pch, const ctsm_soilflux_state *sf,
                         int32_t num_filterc, const int32_t *filterc_dev,

@@ -291,6 +291,52 @@ def _hy_lib():
     return L
 
 
+def _phs_lib():
+    L = lib()
+    if not getattr(L, "_phs_ready", False):
+        P = C.POINTER
+        L.oracle_phs_plc.argtypes = [C.c_double] * 3
+        L.oracle_phs_plc.restype = C.c_double
+        L.oracle_phs_d1plc.argtypes = [C.c_double] * 3
+        L.oracle_phs_d1plc.restype = C.c_double
+        L.oracle_phs_soil_root_conductance.argtypes = [P(abi.Config), P(abi.Layout), P(abi.PhsState),
+                                                       P(abi.PhsParams), C.c_int, abi.i32p]
+        L.oracle_phs_root_sink.argtypes = [P(abi.Config), P(abi.Layout), P(abi.PhsState), C.c_int, abi.i32p]
+        L._phs_ready = True
+    return L
+
+
+def phs_plc(x, psi50, ck):
+    """plc, vegetation_weibull (PhotosynthesisMod.F90:5167-5195)."""
+    return _phs_lib().oracle_phs_plc(x, psi50, ck)
+
+
+def phs_d1plc(x, psi50, ck):
+    """d1plc (PhotosynthesisMod.F90:5199-5228)."""
+    return _phs_lib().oracle_phs_d1plc(x, psi50, ck)
+
+
+def _phs_cfg(A):
+    return abi.Config(nlevsno=A.dims["nlevsno"], nlevsoi=A.dims["nlevsoi"], nlevgrnd=A.dims["nlevgrnd"])
+
+
+def phs_soil_root_conductance(A, prm, filter_p=None):
+    """Root-soil conductances (PhotosynthesisMod.F90:3062-3114) on a
+    ctsm_amd.phs.PhsArrays, in place; filter_p defaults to every patch."""
+    f, fp = _filt(filter_p if filter_p is not None else np.arange(A.begp, A.begp + A.npatch))
+    s = A.host_struct()
+    return _phs_lib().oracle_phs_soil_root_conductance(C.byref(_phs_cfg(A)), C.byref(A.layout()), C.byref(s),
+                                                       C.byref(prm), len(f), fp)
+
+
+def phs_root_sink(A, filter_c=None):
+    """Compute_EffecRootFrac_And_VertTranSink_HydStress
+    (SoilWaterPlantSinkMod.F90:236-328) on a PhsArrays, in place."""
+    f, fp = _filt(filter_c if filter_c is not None else np.arange(A.begc, A.begc + A.ncol))
+    s = A.host_struct()
+    return _phs_lib().oracle_phs_root_sink(C.byref(_phs_cfg(A)), C.byref(A.layout()), C.byref(s), len(f), fp)
+
+
 def hydro_params(fff=0.5, pc=0.4, mu=0.14, h2osfcflag=1, aq_sp_yield_min=0.01, n_baseflow=1.0,
                  baseflow_scalar=5.5e-3, perched_baseflow_scalar=1.0e-5):
     return abi.HydroParams(fff=fff, pc=pc, mu=mu, aq_sp_yield_min=aq_sp_yield_min, n_baseflow=n_baseflow,

@@ -45,7 +45,8 @@ STRUCTS = (("ctsm_config", abi.Config), ("ctsm_layout", abi.Layout), ("ctsm_tuni
            ("ctsm_canopy_state", abi.CanopyState), ("ctsm_soilflux_state", abi.SoilFluxState),
            ("ctsm_surface_water_state", abi.SurfaceWaterState), ("ctsm_hydro_params", abi.HydroParams),
            ("ctsm_water_table_state", abi.WaterTableState), ("ctsm_hist_field", abi.HistField),
-           ("ctsm_ckpt_field", abi.CkptField), ("ctsm_drainage_state", abi.DrainageState))
+           ("ctsm_ckpt_field", abi.CkptField), ("ctsm_drainage_state", abi.DrainageState),
+           ("ctsm_phs_state", abi.PhsState), ("ctsm_phs_params", abi.PhsParams))
 
 
 def _probe(tmp_path):

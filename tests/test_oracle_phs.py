@@ -1,0 +1,167 @@
+"""Pins of the plant-hydraulic-stress oracle (oracle/phs_oracle.c).
+
+  * plc / d1plc against the reference's own known answers
+    (src/biogeophys/test/Photosynthesis_test/test_Photosynthesis.pf:65-70 with
+    setParamsForTesting, PhotosynthesisMod.F90:930-931: pft 1, psi50 = -150000
+    mm, ck = 3.95, x = -1000 * nlevgrnd = -25000 mm, tolerance 1e-13), plc(0) = 1,
+    plc(psi50) = 1/2 exactly, the 0.005 cut, d1plc against a central difference;
+  * the root-soil conductances against the Gardner closed form (a root length
+    density of 1/pi gives r_soil = 1 m, so k_soil = min(hksat, hk_l) / 1000), and
+    against a numpy evaluation in a different algebraic form (the series law as
+    k_s k_r / (k_s + k_r)); the top layer and rootless layers at zero, the 1 gC/m2
+    biomass floor, the 1e-16 floors past the vulnerability cut;
+  * the vertical sink against a numpy sum over patches, zero exchange at
+    hydrostatic equilibrium (smp = vegwp(root) + 1000 z, exact in binary64),
+    and inactive, snow-buried and zero-weight patches left out.
+"""
+import numpy as np
+import pytest
+
+from ctsm_amd import phs as PH
+from oracle import oracle as O
+
+PSI50, CK = -150000.0, 3.95
+
+
+def test_plc_known_answers_of_the_reference_test():
+    # test_Photosynthesis.pf:67 and :70
+    assert abs(O.phs_plc(-25000.0, PSI50, CK) - 0.999415208562283) <= 1e-13
+    assert abs(O.phs_d1plc(-25000.0, PSI50, CK) - 9.237002539040872e-8) <= 1e-13
+
+
+def test_plc_special_values_and_cut():
+    assert O.phs_plc(0.0, PSI50, CK) == 1.0
+    assert O.phs_plc(PSI50, PSI50, CK) == 0.5
+    assert O.phs_plc(1.6 * PSI50, PSI50, CK) == pytest.approx(2.0 ** -(1.6 ** CK), rel=1e-14)
+    assert O.phs_plc(1.6 * PSI50, PSI50, CK) > 0.005
+    assert O.phs_plc(1.7 * PSI50, PSI50, CK) == 0.0          # 2**-(1.7**3.95) < 0.005
+
+
+@pytest.mark.parametrize("x", [-2000.0, -60000.0, -150000.0, -210000.0])
+def test_d1plc_is_the_derivative_of_plc(x):
+    h = 1e-4 * abs(x)
+    fd = (O.phs_plc(x + h, PSI50, CK) - O.phs_plc(x - h, PSI50, CK)) / (2 * h)
+    assert O.phs_d1plc(x, PSI50, CK) == pytest.approx(fd, rel=1e-6)
+
+
+def _single(nl=20):
+    A = PH.PhsArrays(1, 1)
+    A["patchi"][:] = 1
+    A["patchf"][:] = 1
+    A["column"][:] = 1
+    A["active"][:] = 1
+    A["frac_veg_nosno"][:] = 1
+    A["wtcol"][:] = 1.0
+    A["dz"][12:, 0] = 0.1
+    A["z"][12:, 0] = 0.05 + 0.1 * np.arange(25)
+    A["hksat"][:] = 1e-2
+    A["hk_l"][:] = 1e-3
+    A["smp_l"][:] = -1000.0
+    A["tsai"][:] = 1.0
+    A["tlai"][:] = 3.0
+    A["rootfr"][:nl, 0] = 1.0 / nl
+    return A
+
+
+def _params(root_density=0.31e6, radius=0.29e-3, krmax=5e-9, froot_leaf=1.5):
+    return PH.phs_params([krmax], [radius], [root_density], [froot_leaf], np.full((4, 1), PSI50),
+                         np.full((4, 1), CK))
+
+
+def test_gardner_closed_form_at_unit_soil_radius():
+    # froot_carbon = 0: the biomass floor 2 g/m3; root_density = 2 / r**2 makes
+    # the root length density 1/pi and r_soil = 1 m
+    A = _single()
+    r = 0.29e-3
+    assert O.phs_soil_root_conductance(A, _params(root_density=2.0 / r ** 2, radius=r)) == 0
+    ks = A["soil_conductance"][:20, 0]
+    assert np.allclose(ks, np.minimum(1e-2, 1e-3) / 1000.0, rtol=1e-14, atol=0)
+    A["hk_l"][:] = 5e-2                                     # hksat the smaller one now
+    O.phs_soil_root_conductance(A, _params(root_density=2.0 / r ** 2, radius=r))
+    assert np.allclose(A["soil_conductance"][:20, 0], 1e-5, rtol=1e-14, atol=0)
+
+
+def _numpy_conductance(A, krmax, radius, density, froot_leaf, psi50, ck):
+    nl = 20
+    p, c = 0, 0
+    fr = A["rootfr"][:nl, p]
+    dz = A["dz"][12:12 + nl, c]
+    z = A["z"][12:12 + nl, c]
+    b = np.maximum(2.0 * A["froot_carbon"][p] * fr / dz, 2.0)
+    L = b / (density * np.pi * radius ** 2)
+    ks = np.maximum(np.minimum(A["hksat"][:nl, c], A["hk_l"][:nl, c]) * np.sqrt(np.pi * L) / 1000.0, 1e-16)
+    plc = 2.0 ** -((A["smp_l"][:nl, c] / psi50) ** ck)
+    plc[plc < 0.005] = 0.0
+    rai = (A["tsai"][p] + A["tlai"][p]) * froot_leaf * fr
+    kr = np.maximum(plc * rai * krmax / (0.25 + z), 1e-16)
+    k = ks * kr / (ks + kr)
+    k[(rai * fr <= 0) | (np.arange(1, nl + 1) == 1)] = 0.0
+    return ks, kr, k
+
+
+@pytest.mark.parametrize("froot", [0.0, 40.0, 900.0])
+def test_conductances_against_numpy(froot):
+    A = _single(nl=14)
+    A["froot_carbon"][:] = froot
+    A["smp_l"][:14, 0] = -np.geomspace(50.0, 6e5, 14)           # the driest rooted layers past the cut
+    A["hk_l"][:, 0] = np.geomspace(1e-9, 5e-2, 25)
+    assert O.phs_soil_root_conductance(A, _params()) == 0
+    ks, kr, k = _numpy_conductance(A, 5e-9, 0.29e-3, 0.31e6, 1.5, PSI50, CK)
+    assert np.allclose(A["soil_conductance"][:20, 0], ks, rtol=1e-12, atol=0)
+    assert np.allclose(A["root_conductance"][:20, 0], kr, rtol=1e-12, atol=0)
+    assert np.allclose(A["k_soil_root"][:20, 0], k, rtol=1e-12, atol=0)
+    assert A["k_soil_root"][0, 0] == 0.0                        # top layer
+    assert np.all(A["k_soil_root"][14:20, 0] == 0.0)            # no roots
+    assert np.all(A["root_conductance"][14:20, 0] == 1e-16)
+    cut = A["smp_l"][:14, 0] < 1.7 * PSI50
+    assert cut.any() and np.all(A["root_conductance"][:14, 0][cut] == 1e-16)
+    assert np.all(A["root_conductance"][:14, 0][~cut] > 1e-16)
+
+
+def test_bad_type_is_reported():
+    A = _single()
+    A["itype"][:] = 3                                           # only one PFT in the table
+    assert O.phs_soil_root_conductance(A, _params()) == 1
+
+
+def _numpy_sink(A):
+    nl = A.dims["nlevsoi"]
+    ns = A.dims["nlevsno"]
+    q = np.zeros((nl, A.ncol))
+    redist = np.zeros(A.npatch)
+    for c in range(A.ncol):
+        for p in range(A["patchi"][c] - 1, A["patchf"][c]):
+            if not (A["active"][p] and A["frac_veg_nosno"][p] > 0 and A["wtcol"][p] > 0):
+                continue
+            flux = A["k_soil_root"][:nl, p] * (A["smp_l"][:nl, c] - A["vegwp"][3, p] - 1000.0 * A["z"][ns:ns + nl, c])
+            redist[p] = flux[flux < 0].sum()
+            q[:, c] += flux * A["wtcol"][p]
+    return q, redist
+
+
+def test_root_sink_against_numpy():
+    A = PH.synthetic(300, seed=4)
+    assert O.phs_root_sink(A) == 0
+    q, redist = _numpy_sink(A)
+    assert np.allclose(A["qflx_rootsoi"][:20], q, rtol=1e-12, atol=1e-30)
+    assert np.allclose(A["qflx_hydr_redist"], redist, rtol=1e-12, atol=1e-30)
+    neg = np.where(q < 0, q, 0.0).sum(0)
+    assert np.allclose(A["qflx_phs_neg"], neg, rtol=1e-12, atol=1e-30)
+    assert np.all(A["qflx_phs_neg"] <= 0)
+    excluded = (A["active"] == 0) | (A["frac_veg_nosno"] == 0) | (A["wtcol"] <= 0)
+    assert excluded.any() and np.all(A["qflx_hydr_redist"][excluded] == 0.0)
+
+
+def test_root_sink_vanishes_at_hydrostatic_equilibrium():
+    A = PH.synthetic(64, seed=5)
+    ns = A.dims["nlevsno"]
+    A["z"][ns:] = np.round(A["z"][ns:] * 8) / 8                 # exact in binary64
+    A["vegwp"][3] = -np.round(np.abs(A["vegwp"][3]))
+    for c in range(A.ncol):
+        ps = list(range(A["patchi"][c] - 1, A["patchf"][c]))
+        psi = A["vegwp"][3, ps[0]] if ps else -5000.0
+        A["vegwp"][3, ps] = psi                                 # one root potential per column
+        A["smp_l"][:, c] = psi + 1000.0 * A["z"][ns:, c]
+    assert O.phs_root_sink(A) == 0
+    assert np.all(A["qflx_rootsoi"][:20] == 0.0)
+    assert np.all(A["qflx_phs_neg"] == 0.0) and np.all(A["qflx_hydr_redist"] == 0.0)
