@@ -251,6 +251,7 @@ def full_column(ctx, args):
                                "frac": b / (can_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
     out["with_soilfluxes_and_surface_water"] = _full_land_step(ctx, args, ds, structs)
     ds.free()
+    out["phs_soil_side"] = _phs_soil_side(ctx, st.dims.ncol)
     if args.canopy_parts > 1:
         out["concurrent_parts"] = _full_column_parts(ctx, args, st)
     return out
@@ -321,6 +322,36 @@ def _full_land_step(ctx, args, ds, structs):
             "surface_water_roofline": {"kernel": "k_surface_water", "alg_bytes_per_launch": b,
                                        "achieved_GBps": b / (sw_ms * 1e-3) / 1e9,
                                        "frac": b / (sw_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+
+
+def _phs_soil_side(ctx, ncol):
+    """The two plant-hydraulic-stress soil kernels (root-soil conductances per
+    patch, the vertical sink per column) on a synthetic case of ncol columns
+    with 0-4 patches each, device-resident, with their HBM rooflines."""
+    from ctsm_amd import phs as PH
+    A = PH.synthetic(ncol, max_patches=4, seed=3)
+    ctx.set_phs_params(PH.default_params())
+    D = PH.DevicePhs(ctx, A)
+    s = D.struct()
+    lay = A.layout()
+    nl = A.dims["nlevsoi"]
+    cond_ms, sink_ms = time_events(ctx, [lambda: ctx.phs_soil_root_conductance(lay, s, A.npatch, None, dev=True),
+                                         lambda: ctx.phs_root_sink(lay, s, A.ncol, None, dev=True)], 3, inner=10)
+    D.free()
+    # unique bytes: per patch its scalars, rootfr and the three outputs per layer;
+    # per column z, dz, hksat, hk_l, smp_l per layer
+    cb = A.npatch * (32 + nl * 32) + A.ncol * nl * 40
+    used = (A["active"] != 0) & (A["frac_veg_nosno"] > 0) & (A["wtcol"] > 0)
+    # per column patch range, z, smp_l and the output per layer, qflx_phs_neg; per
+    # patch the three tests and the output, and for contributing patches
+    # vegwp(root) and k_soil_root per layer
+    sb = A.ncol * (16 + nl * 24) + A.npatch * 24 + int(used.sum()) * (8 + nl * 8)
+    roof = lambda b, ms: {"alg_bytes_per_launch": b, "achieved_GBps": b / (ms * 1e-3) / 1e9,
+                          "frac": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    return {"workload": f"{A.ncol:,} columns, {A.npatch:,} patches (0-4 per column), {nl} soil layers",
+            "conductance_kernel_ms": cond_ms, "root_sink_kernel_ms": sink_ms,
+            "conductance_roofline": dict(kernel="k_phs_conductance", **roof(cb, cond_ms)),
+            "root_sink_roofline": dict(kernel="k_phs_root_sink", **roof(sb, sink_ms))}
 
 
 def _history_and_checkpoint(ctx, ds, structs):

@@ -199,10 +199,32 @@ module ctsm_gpu_mod
      integer(c_int32_t) :: pad = 0
   end type ctsm_hydro_params_t
 
+  ! plant hydraulic stress, soil side, as ctsm_phs_state: patch members, then
+  ! column members; vegwp is vegwp_patch(begp:endp, 1:4)
+  type, bind(C), public :: ctsm_phs_state_t
+     type(c_ptr) :: itype = c_null_ptr, column = c_null_ptr, active = c_null_ptr, frac_veg_nosno = c_null_ptr
+     type(c_ptr) :: wtcol = c_null_ptr, froot_carbon = c_null_ptr, tsai = c_null_ptr, tlai = c_null_ptr
+     type(c_ptr) :: rootfr = c_null_ptr, vegwp = c_null_ptr, k_soil_root = c_null_ptr
+     type(c_ptr) :: root_conductance = c_null_ptr, soil_conductance = c_null_ptr, qflx_hydr_redist = c_null_ptr
+     type(c_ptr) :: patchi = c_null_ptr, patchf = c_null_ptr, dz = c_null_ptr, z = c_null_ptr
+     type(c_ptr) :: hksat = c_null_ptr, hk_l = c_null_ptr, smp_l = c_null_ptr, qflx_rootsoi = c_null_ptr
+     type(c_ptr) :: qflx_phs_neg = c_null_ptr
+  end type ctsm_phs_state_t
+
+  ! PHS tables (params_inst%krmax, psi50, ck; pftcon%root_radius, root_density,
+  ! froot_leaf), psi50 and ck as (0:npft-1, 1:4) arrays
+  type, bind(C), public :: ctsm_phs_params_t
+     integer(c_int32_t) :: npft
+     integer(c_int32_t) :: pad = 0
+     type(c_ptr) :: krmax = c_null_ptr, root_radius = c_null_ptr, root_density = c_null_ptr
+     type(c_ptr) :: froot_leaf = c_null_ptr, psi50 = c_null_ptr, ck = c_null_ptr
+  end type ctsm_phs_params_t
+
   public :: ctsm_config_default, ctsm_create, ctsm_destroy, ctsm_last_error
   public :: ctsm_soilwater, ctsm_soiltemperature, ctsm_soilwater_fractions
   public :: ctsm_set_canopy_params, ctsm_canopyfluxes, ctsm_root_sink
   public :: ctsm_soilfluxes, ctsm_soil_balance_check, ctsm_surface_water, ctsm_water_table, ctsm_drainage
+  public :: ctsm_set_phs_params, ctsm_phs_soil_root_conductance, ctsm_phs_root_sink
 
   interface
      subroutine ctsm_config_default(cfg) bind(C, name='ctsm_config_default')
@@ -357,6 +379,32 @@ module ctsm_gpu_mod
        integer(c_int32_t), intent(in) :: filter_hydrologyc(*)
        real(c_double), value :: dtime
      end function ctsm_drainage
+
+     integer(c_int) function ctsm_set_phs_params(ctx, prm) bind(C, name='ctsm_set_phs_params')
+       import :: c_int, c_ptr, ctsm_phs_params_t
+       type(c_ptr), value :: ctx
+       type(ctsm_phs_params_t), intent(in) :: prm
+     end function ctsm_set_phs_params
+
+     integer(c_int) function ctsm_phs_soil_root_conductance(ctx, lay, phs, num_filterp, filterp) &
+          bind(C, name='ctsm_phs_soil_root_conductance')
+       import :: c_int, c_int32_t, c_ptr, ctsm_layout_t, ctsm_phs_state_t
+       type(c_ptr), value :: ctx
+       type(ctsm_layout_t), intent(in) :: lay
+       type(ctsm_phs_state_t), intent(in) :: phs
+       integer(c_int32_t), value :: num_filterp
+       integer(c_int32_t), intent(in) :: filterp(*)
+     end function ctsm_phs_soil_root_conductance
+
+     integer(c_int) function ctsm_phs_root_sink(ctx, lay, phs, num_filterc, filterc) &
+          bind(C, name='ctsm_phs_root_sink')
+       import :: c_int, c_int32_t, c_ptr, ctsm_layout_t, ctsm_phs_state_t
+       type(c_ptr), value :: ctx
+       type(ctsm_layout_t), intent(in) :: lay
+       type(ctsm_phs_state_t), intent(in) :: phs
+       integer(c_int32_t), value :: num_filterc
+       integer(c_int32_t), intent(in) :: filterc(*)
+     end function ctsm_phs_root_sink
   end interface
 
 end module ctsm_gpu_mod
