@@ -63,9 +63,13 @@ def _single(nl=20):
     return A
 
 
+# the four segments (sun, sha, xyl, root) get distinct curves; the root one is PSI50, CK
+SEG_PSI50 = np.array([[-90000.0], [-120000.0], [-400000.0], [PSI50]])
+SEG_CK = np.array([[2.0], [2.5], [6.0], [CK]])
+
+
 def _params(root_density=0.31e6, radius=0.29e-3, krmax=5e-9, froot_leaf=1.5):
-    return PH.phs_params([krmax], [radius], [root_density], [froot_leaf], np.full((4, 1), PSI50),
-                         np.full((4, 1), CK))
+    return PH.phs_params([krmax], [radius], [root_density], [froot_leaf], SEG_PSI50, SEG_CK)
 
 
 def test_gardner_closed_form_at_unit_soil_radius():
