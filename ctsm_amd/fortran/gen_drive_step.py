@@ -1,10 +1,11 @@
-"""Writes drive_step.F90, a Fortran program that calls SoilWater,
+"""Writes drive_step.F90 and drive_phs.F90.  drive_step.F90 is a Fortran program that calls SoilWater,
 SoilTemperature, CanopyFluxes or one of the hydrology entries (surface water,
 water tables, drainage) through ctsm_gpu_mod on a state held the way
 CTSM holds it (one allocatable array per member with the reference's bounds,
 bound by c_loc; members shared between structs point at the same array).
 
-The member lists come from ctsm_amd/abi.py, so the program follows the
+drive_phs.F90 calls the two plant-hydraulic-stress soil entries the same way.
+The member lists come from ctsm_amd/abi.py, so the programs follow the
 header; run this script again after a state struct changes:
 
     python ctsm_amd/fortran/gen_drive_step.py
@@ -186,8 +187,115 @@ end program drive_step
 '''
 
 
+PHS_BOUNDS = {"p": "(begp:endp)", "c": "(begc:endc)", "soip": "(begp:endp, 1:nlevgrnd)",
+              "veg": "(begp:endp, 1:4)", "sno": "(begc:endc, -nlevsno+1:nlevgrnd)", "soi": "(begc:endc, 1:nlevgrnd)"}
+PHS_FIRST = {"p": "(begp)", "c": "(begc)", "soip": "(begp, 1)", "veg": "(begp, 1)", "sno": "(begc, -nlevsno+1)",
+             "soi": "(begc, 1)"}
+
+
+def generate_phs():
+    decl, alloc, rd, bind, wr = [], [], [], [], []
+    for n, t, k in abi.PHS_FIELDS:
+        typ = "integer(c_int32_t)" if t == "i32" else "real(c_double)"
+        rank = "(:)" if k in ("c", "p") else "(:,:)"
+        decl.append(f"  {typ}, allocatable, target :: a_{n}{rank}")
+        alloc.append(f"  allocate(a_{n}{PHS_BOUNDS[k]})")
+        rd.append(f"  read(u) a_{n}")
+        bind.append(f"  phs%{n} = c_loc(a_{n}{PHS_FIRST[k]})")
+        wr.append(f"  write(u) a_{n}")
+    nl = "\n"
+    return f'''! drive_phs.F90 -- written by gen_drive_step.py from the member list of
+! ctsm_amd/abi.py; do not edit by hand.
+!
+! The plant-hydraulic-stress soil entries from Fortran through ctsm_gpu_mod, on
+! arrays held as CTSM holds them (reference bounds, bound by c_loc).
+!
+! Usage: drive_phs IN OUT.  IN holds, as int32, ncol, npatch, nlevsno, nlevsoi,
+! nlevgrnd, npft and mode (1 conductances over every patch, 2 the sink over
+! every column, 3 both); the PHS tables (krmax, root_radius, root_density,
+! froot_leaf: npft doubles each; psi50 and ck: (npft, 4)); then every
+! ctsm_phs_state member in member order.  OUT receives the members after the
+! calls.  Prints the return code.
+program drive_phs
+  use iso_c_binding
+  use ctsm_gpu_mod
+  implicit none
+  integer(c_int32_t) :: ncol, npatch, nlevsno, nlevsoi, nlevgrnd, npft, mode
+  integer :: begc, endc, begp, endp, u, i, rc
+  character(len=4096) :: fin, fout
+  type(ctsm_config_t) :: cfg
+  type(ctsm_layout_t) :: lay
+  type(ctsm_phs_state_t) :: phs
+  type(ctsm_phs_params_t) :: prm
+  type(c_ptr) :: ctx
+  integer(c_int32_t), allocatable :: filter(:)
+  real(c_double), allocatable, target :: krmax(:), root_radius(:), root_density(:), froot_leaf(:)
+  real(c_double), allocatable, target :: psi50(:,:), ck(:,:)
+{nl.join(decl)}
+
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+  open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
+  read(u) ncol, npatch, nlevsno, nlevsoi, nlevgrnd, npft, mode
+  allocate(krmax(0:npft-1), root_radius(0:npft-1), root_density(0:npft-1), froot_leaf(0:npft-1))
+  allocate(psi50(0:npft-1, 4), ck(0:npft-1, 4))
+  read(u) krmax, root_radius, root_density, froot_leaf, psi50, ck
+  begc = 1
+  endc = ncol
+  begp = 1
+  endp = npatch
+{nl.join(alloc)}
+{nl.join(rd)}
+  close(u)
+{nl.join(bind)}
+  prm%npft = npft
+  prm%krmax = c_loc(krmax(0))
+  prm%root_radius = c_loc(root_radius(0))
+  prm%root_density = c_loc(root_density(0))
+  prm%froot_leaf = c_loc(froot_leaf(0))
+  prm%psi50 = c_loc(psi50(0, 1))
+  prm%ck = c_loc(ck(0, 1))
+
+  call ctsm_config_default(cfg)
+  cfg%nlevsno = nlevsno
+  cfg%nlevsoi = nlevsoi
+  cfg%nlevgrnd = nlevgrnd
+  rc = ctsm_create(cfg, 0, ctx)
+  if (rc /= CTSM_OK) then
+     print '(I0)', rc
+     stop 1
+  end if
+  lay%begc_alloc = begc
+  lay%ldc = endc - begc + 1
+  lay%begp_alloc = begp
+  lay%ldp = endp - begp + 1
+  lay%begg_alloc = 1
+  lay%ldg = 1
+  rc = ctsm_set_phs_params(ctx, prm)
+  if (rc == CTSM_OK .and. iand(mode, 1) /= 0) then
+     allocate(filter(npatch))
+     filter = [(i, i = begp, endp)]
+     rc = ctsm_phs_soil_root_conductance(ctx, lay, phs, npatch, filter)
+     deallocate(filter)
+  end if
+  if (rc == CTSM_OK .and. iand(mode, 2) /= 0) then
+     allocate(filter(ncol))
+     filter = [(i, i = begc, endc)]
+     rc = ctsm_phs_root_sink(ctx, lay, phs, ncol, filter)
+  end if
+  print '(I0)', rc
+  i = ctsm_destroy(ctx)
+
+  open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
+{nl.join(wr)}
+  close(u)
+end program drive_phs
+'''
+
+
 if __name__ == "__main__":
-    path = os.path.join(HERE, "drive_step.F90")
-    with open(path, "w") as f:
-        f.write(generate())
-    print(path)
+    for name, text in (("drive_step.F90", generate()), ("drive_phs.F90", generate_phs())):
+        path = os.path.join(HERE, name)
+        with open(path, "w") as f:
+            f.write(text)
+        print(path)

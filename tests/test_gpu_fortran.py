@@ -166,3 +166,39 @@ def test_hydrology_from_fortran_bitwise(tmp_path, mode):
     else:
         assert O.drainage(o, hydro, cfg) == 0
     _compare(st, got, o)
+
+
+@pytest.mark.parametrize("mode", [1, 2, 3])
+def test_phs_soil_side_from_fortran_bitwise(tmp_path, mode):
+    # drive_phs.F90: the PHS soil entries on arrays with the reference's bounds
+    from ctsm_amd import phs as PH
+    exe = build.FORTRAN_PHS_DRIVER
+    if not os.path.exists(exe):
+        pytest.skip("Fortran driver not built (no flang)")
+    print(f"case: PHS soil side mode {mode} from Fortran, 500 columns", flush=True)
+    A = PH.synthetic(500, seed=40 + mode)
+    prm = PH.default_params()
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    d = A.dims
+    with open(fin, "wb") as f:
+        f.write(np.array([A.ncol, A.npatch, d["nlevsno"], d["nlevsoi"], d["nlevgrnd"], prm.npft, mode],
+                         dtype=np.int32).tobytes())
+        for a in prm._keep:
+            f.write(np.ascontiguousarray(a).tobytes())
+        for n, _, _ in abi.PHS_FIELDS:
+            f.write(A[n].tobytes())
+    r = subprocess.run([exe, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert int(r.stdout.split()[0]) == 0, r.stdout
+    ref = A.copy()
+    if mode & 1:
+        assert O.phs_soil_root_conductance(ref, prm) == 0
+    if mode & 2:
+        assert O.phs_root_sink(ref) == 0
+    raw, pos = fout.read_bytes(), 0
+    for n, _, _ in abi.PHS_FIELDS:
+        a = ref[n]
+        got = np.frombuffer(raw[pos:pos + a.nbytes], dtype=a.dtype).reshape(a.shape)
+        pos += a.nbytes
+        np.testing.assert_array_equal(got, a, err_msg=n)
+    assert pos == len(raw)
