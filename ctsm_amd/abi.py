@@ -430,7 +430,23 @@ class PhsState(C.Structure):
 
 class PhsParams(C.Structure):
     _fields_ = [("npft", C.c_int32), ("pad", C.c_int32)] + [
-        (n, f64p) for n in ("krmax", "root_radius", "root_density", "froot_leaf", "psi50", "ck")]
+        (n, f64p) for n in ("krmax", "root_radius", "root_density", "froot_leaf", "psi50", "ck", "kmax")]
+
+
+# calcstress (ctsm_phs_stress_state): patch members, then column members
+PHS_STRESS_FIELDS = [
+    ("itype", "i32", "p"), ("column", "i32", "p"), ("local_morning", "i32", "p"),
+    ("laisun", "f64", "p"), ("laisha", "f64", "p"), ("elai", "f64", "p"), ("esai", "f64", "p"),
+    ("tsai", "f64", "p"), ("htop", "f64", "p"), ("fdry", "f64", "p"), ("thm", "f64", "p"),
+    ("gb_mol", "f64", "p"), ("gs_mol_sun", "f64", "p"), ("gs_mol_sha", "f64", "p"), ("qsatl", "f64", "p"),
+    ("qaf", "f64", "p"), ("k_soil_root", "f64", "soip"), ("vegwp", "f64", "veg"), ("vegwp_pd", "f64", "veg"),
+    ("bsun", "f64", "p"), ("bsha", "f64", "p"), ("qflx_tran_veg", "f64", "p"),
+    ("forc_rho", "f64", "c"), ("forc_pbot", "f64", "c"), ("z", "f64", "sno"), ("smp_l", "f64", "soi"),
+]
+
+
+class PhsStressState(C.Structure):
+    _fields_ = [(n, _CT[t]) for n, t, _ in PHS_STRESS_FIELDS]
 
 
 class WaterTableState(C.Structure):

@@ -60,6 +60,8 @@ void launch_phs_conductance(hipStream_t, const ctsm_layout &, const ctsm_config 
                             const double *, int, int, const int32_t *, ErrWord *);
 void launch_phs_root_sink(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_phs_state &, int,
                           const int32_t *, ErrWord *);
+void launch_phs_calcstress(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_phs_stress_state &,
+                           const double *, int, int, const int32_t *, ErrWord *);
 int ckpt_write(hipStream_t, const char *, const ctsm_ckpt_field *, int, hipError_t *);
 int ckpt_read(hipStream_t, const char *, const ctsm_ckpt_field *, int, hipError_t *);
 }  // namespace ctsm
@@ -260,6 +262,25 @@ constexpr int kNPhsP = sizeof(kPhsP) / sizeof(kPhsP[0]);
 constexpr int kNPhsC = sizeof(kPhsC) / sizeof(kPhsC[0]);
 static_assert((kNPhsP + kNPhsC) * sizeof(void *) == sizeof(ctsm_phs_state),
               "every ctsm_phs_state member has a mirror entry");
+// calcstress: every member is used
+#define PS(name, kind, elem, wr) {offsetof(ctsm_phs_stress_state, name), kind, elem, wr}
+const FieldMeta kStP[] = {
+    PS(itype, K_P, 4, false), PS(column, K_P, 4, false), PS(local_morning, K_P, 4, false),
+    PS(laisun, K_P, 8, false), PS(laisha, K_P, 8, false), PS(elai, K_P, 8, false), PS(esai, K_P, 8, false),
+    PS(tsai, K_P, 8, false), PS(htop, K_P, 8, false), PS(fdry, K_P, 8, false), PS(thm, K_P, 8, false),
+    PS(gb_mol, K_P, 8, false), PS(gs_mol_sun, K_P, 8, false), PS(gs_mol_sha, K_P, 8, false),
+    PS(qsatl, K_P, 8, false), PS(qaf, K_P, 8, false), PS(k_soil_root, K_SOIP, 8, false),
+    PS(vegwp, K_VEG, 8, true), PS(vegwp_pd, K_VEG, 8, true), PS(bsun, K_P, 8, true), PS(bsha, K_P, 8, true),
+    PS(qflx_tran_veg, K_P, 8, true),
+};
+const FieldMeta kStC[] = {
+    PS(forc_rho, K_C, 8, false), PS(forc_pbot, K_C, 8, false), PS(z, K_SNO, 8, false), PS(smp_l, K_SOI, 8, false),
+};
+#undef PS
+constexpr int kNStP = sizeof(kStP) / sizeof(kStP[0]);
+constexpr int kNStC = sizeof(kStC) / sizeof(kStC[0]);
+static_assert((kNStP + kNStC) * sizeof(void *) == sizeof(ctsm_phs_stress_state),
+              "every ctsm_phs_stress_state member has a mirror entry");
 constexpr int kNSfP = sizeof(kSfPatch) / sizeof(kSfPatch[0]);
 constexpr int kNSfC = sizeof(kSfCol) / sizeof(kSfCol[0]);
 static_assert((kNSfP + kNSfC) * sizeof(void *) == sizeof(ctsm_soilflux_state),
@@ -364,8 +385,10 @@ struct ctsm_ctx {
     std::vector<DevBuf> wtm;           // water-table drop-in mirrors
     std::vector<DevBuf> drm;           // drainage drop-in mirrors
     std::vector<DevBuf> phsp, phsc;    // plant-hydraulics drop-in mirrors
-    DevBuf phs_tab;                    // PHS parameter table (12 x CTSM_MAX_PFT doubles)
+    DevBuf phs_tab;                    // PHS parameter table (16 x CTSM_MAX_PFT doubles)
     int32_t phs_npft = 0;              // 0 until ctsm_set_phs_params
+    bool phs_kmax = false;             // the table holds kmax (calcstress may run)
+    std::vector<DevBuf> stp, stc;      // calcstress drop-in mirrors
     bool have_canopy = false;
     int32_t last_code = 0, last_index = 0;
     std::mutex mu;
@@ -658,6 +681,8 @@ int ctsm_destroy(ctsm_ctx *ctx)
     for (auto &b : ctx->drm) b.release();
     for (auto &b : ctx->phsp) b.release();
     for (auto &b : ctx->phsc) b.release();
+    for (auto &b : ctx->stp) b.release();
+    for (auto &b : ctx->stc) b.release();
     ctx->phs_tab.release();
     ctx->partf.release();
     for (int k = 0; k < ctsm_ctx::kMaxStreams - 1; ++k) {
@@ -1402,15 +1427,16 @@ int ctsm_set_phs_params(ctsm_ctx *ctx, const ctsm_phs_params *prm)
     const double *rows[] = {prm->krmax, prm->root_radius, prm->root_density, prm->froot_leaf};
     for (const double *a : rows) if (!a) return CTSM_E_ARG;
     if (!prm->psi50 || !prm->ck) return CTSM_E_ARG;
-    // rows: krmax, root_radius, root_density, froot_leaf, psi50 of the 4 segments, ck of the 4 segments
+    // rows: krmax, root_radius, root_density, froot_leaf, psi50, ck and kmax of the 4 segments
     const int n = prm->npft;
-    std::vector<double> tab((size_t)12 * CTSM_MAX_PFT, 0.0);
+    std::vector<double> tab((size_t)16 * CTSM_MAX_PFT, 0.0);
     for (int r = 0; r < 4; ++r)
         for (int i = 0; i < n; ++i) tab[(size_t)r * CTSM_MAX_PFT + i] = rows[r][i];
     for (int k = 0; k < 4; ++k)
         for (int i = 0; i < n; ++i) {
             tab[(size_t)(4 + k) * CTSM_MAX_PFT + i] = prm->psi50[(size_t)k * n + i];
             tab[(size_t)(8 + k) * CTSM_MAX_PFT + i] = prm->ck[(size_t)k * n + i];
+            if (prm->kmax) tab[(size_t)(12 + k) * CTSM_MAX_PFT + i] = prm->kmax[(size_t)k * n + i];
         }
     std::lock_guard<std::mutex> g(ctx->mu);
     hipError_t e = hipSetDevice(ctx->device);
@@ -1421,6 +1447,7 @@ int ctsm_set_phs_params(ctsm_ctx *ctx, const ctsm_phs_params *prm)
         return hip_fail(ctx, e);
     if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return hip_fail(ctx, e);
     ctx->phs_npft = n;
+    ctx->phs_kmax = prm->kmax != nullptr;
     return CTSM_OK;
 }
 
@@ -1524,6 +1551,52 @@ int ctsm_phs_root_sink(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_phs_sta
                        const int32_t *filterc)
 {
     return phs_dropin(ctx, lay, phs, n, filterc, false);
+}
+
+static int stress_check(const ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_phs_stress_state *st, int32_t n,
+                        bool has_filter)
+{
+    if (!ctx || !st || n < 0 || check_layout(lay) || lay->ldp <= 0) return CTSM_E_ARG;
+    if ((!has_filter && n > lay->ldp) || ctx->phs_npft == 0 || !ctx->phs_kmax) return CTSM_E_ARG;
+    if (ctx->cfg.nlevsoi > MAX_NLEVSOI) return CTSM_E_ARG;
+    for (size_t off = 0; off < sizeof(ctsm_phs_stress_state); off += sizeof(void *))
+        if (!*(void *const *)((const char *)st + off)) return CTSM_E_ARG;
+    return CTSM_OK;
+}
+
+int ctsm_phs_calcstress_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_phs_stress_state *st, int32_t n,
+                            const int32_t *filter_dev)
+{
+    int rc = stress_check(ctx, lay, st, n, filter_dev != nullptr);
+    if (rc) return rc;
+    launch_phs_calcstress(ctx->stream, *lay, ctx->cfg, *st, (const double *)ctx->phs_tab.p, ctx->phs_npft, n,
+                          filter_dev, ctx->err);
+    return finish_launch(ctx);
+}
+
+int ctsm_phs_calcstress(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_phs_stress_state *st, int32_t n,
+                        const int32_t *filter)
+{
+    int rc = stress_check(ctx, lay, st, n, filter != nullptr);
+    if (rc) return rc;
+    for (int i = 0; filter && i < n; ++i)
+        if (filter[i] < lay->begp_alloc || filter[i] >= lay->begp_alloc + lay->ldp) return CTSM_E_ARG;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipError_t he = hipSetDevice(ctx->device);
+    if (he != hipSuccess) return hip_fail(ctx, he);
+    const Range cr{lay->begc_alloc, lay->begc_alloc + lay->ldc - 1};
+    const Range pr{lay->begp_alloc, lay->begp_alloc + lay->ldp - 1};
+    ctsm_phs_stress_state d{};
+    rc = mirror(ctx, kStP, kNStP, st, &d, ctx->stp, lay->begp_alloc, lay->ldp, pr, true);
+    if (!rc) rc = mirror(ctx, kStC, kNStC, st, &d, ctx->stc, lay->begc_alloc, lay->ldc, cr, true);
+    if (rc) return rc;
+    const int32_t *df = nullptr;
+    if ((rc = upload_filter(ctx, n, filter, &df))) return rc;
+    launch_phs_calcstress(ctx->stream, *lay, ctx->cfg, d, (const double *)ctx->phs_tab.p, ctx->phs_npft, n, df,
+                          ctx->err);
+    if ((rc = finish_launch(ctx))) return rc;
+    if ((rc = mirror(ctx, kStP, kNStP, st, &d, ctx->stp, lay->begp_alloc, lay->ldp, pr, false))) return rc;
+    return sync_and_check(ctx);
 }
 
 int ctsm_land_step_full_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,

@@ -218,13 +218,26 @@ module ctsm_gpu_mod
      integer(c_int32_t) :: pad = 0
      type(c_ptr) :: krmax = c_null_ptr, root_radius = c_null_ptr, root_density = c_null_ptr
      type(c_ptr) :: froot_leaf = c_null_ptr, psi50 = c_null_ptr, ck = c_null_ptr
+     type(c_ptr) :: kmax = c_null_ptr
   end type ctsm_phs_params_t
+
+  ! calcstress arrays, as ctsm_phs_stress_state (vegwp / vegwp_pd (begp:endp, 1:4))
+  type, bind(C), public :: ctsm_phs_stress_state_t
+     type(c_ptr) :: itype = c_null_ptr, column = c_null_ptr, local_morning = c_null_ptr
+     type(c_ptr) :: laisun = c_null_ptr, laisha = c_null_ptr, elai = c_null_ptr, esai = c_null_ptr
+     type(c_ptr) :: tsai = c_null_ptr, htop = c_null_ptr, fdry = c_null_ptr, thm = c_null_ptr
+     type(c_ptr) :: gb_mol = c_null_ptr, gs_mol_sun = c_null_ptr, gs_mol_sha = c_null_ptr
+     type(c_ptr) :: qsatl = c_null_ptr, qaf = c_null_ptr, k_soil_root = c_null_ptr
+     type(c_ptr) :: vegwp = c_null_ptr, vegwp_pd = c_null_ptr, bsun = c_null_ptr, bsha = c_null_ptr
+     type(c_ptr) :: qflx_tran_veg = c_null_ptr, forc_rho = c_null_ptr, forc_pbot = c_null_ptr
+     type(c_ptr) :: z = c_null_ptr, smp_l = c_null_ptr
+  end type ctsm_phs_stress_state_t
 
   public :: ctsm_config_default, ctsm_create, ctsm_destroy, ctsm_last_error
   public :: ctsm_soilwater, ctsm_soiltemperature, ctsm_soilwater_fractions
   public :: ctsm_set_canopy_params, ctsm_canopyfluxes, ctsm_root_sink
   public :: ctsm_soilfluxes, ctsm_soil_balance_check, ctsm_surface_water, ctsm_water_table, ctsm_drainage
-  public :: ctsm_set_phs_params, ctsm_phs_soil_root_conductance, ctsm_phs_root_sink
+  public :: ctsm_set_phs_params, ctsm_phs_soil_root_conductance, ctsm_phs_root_sink, ctsm_phs_calcstress
 
   interface
      subroutine ctsm_config_default(cfg) bind(C, name='ctsm_config_default')
@@ -405,6 +418,16 @@ module ctsm_gpu_mod
        integer(c_int32_t), value :: num_filterc
        integer(c_int32_t), intent(in) :: filterc(*)
      end function ctsm_phs_root_sink
+
+     integer(c_int) function ctsm_phs_calcstress(ctx, lay, st, num_filterp, filterp) &
+          bind(C, name='ctsm_phs_calcstress')
+       import :: c_int, c_int32_t, c_ptr, ctsm_layout_t, ctsm_phs_stress_state_t
+       type(c_ptr), value :: ctx
+       type(ctsm_layout_t), intent(in) :: lay
+       type(ctsm_phs_stress_state_t), intent(in) :: st
+       integer(c_int32_t), value :: num_filterp
+       integer(c_int32_t), intent(in) :: filterp(*)
+     end function ctsm_phs_calcstress
   end interface
 
 end module ctsm_gpu_mod

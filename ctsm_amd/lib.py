@@ -39,6 +39,7 @@ EXPORTS = [
     "ctsm_checkpoint_write", "ctsm_checkpoint_read", "ctsm_drainage", "ctsm_drainage_dev",
     "ctsm_graph_capture_full", "ctsm_set_phs_params", "ctsm_phs_soil_root_conductance",
     "ctsm_phs_soil_root_conductance_dev", "ctsm_phs_root_sink", "ctsm_phs_root_sink_dev",
+    "ctsm_phs_calcstress", "ctsm_phs_calcstress_dev",
 ]
 
 
@@ -99,6 +100,8 @@ def load():
     for fn in ("ctsm_phs_soil_root_conductance", "ctsm_phs_soil_root_conductance_dev", "ctsm_phs_root_sink",
                "ctsm_phs_root_sink_dev"):
         getattr(L, fn).argtypes = [C.c_void_p, P(abi.Layout), P(abi.PhsState), C.c_int32, C.c_void_p]
+    for fn in ("ctsm_phs_calcstress", "ctsm_phs_calcstress_dev"):
+        getattr(L, fn).argtypes = [C.c_void_p, P(abi.Layout), P(abi.PhsStressState), C.c_int32, C.c_void_p]
     for fn in ("ctsm_hist_update_dev", "ctsm_hist_normalize_dev", "ctsm_hist_zero_dev"):
         getattr(L, fn).argtypes = [C.c_void_p, P(abi.HistField), C.c_int32]
     for fn in ("ctsm_checkpoint_write", "ctsm_checkpoint_read"):
@@ -372,6 +375,12 @@ class Context:
         fn = self.L.ctsm_phs_root_sink_dev if dev else self.L.ctsm_phs_root_sink
         fp = C.c_void_p(filt) if dev and filt is not None else (None if filt is None else filt.ctypes.data_as(C.c_void_p))
         self._check(fn(self.h, C.byref(lay), C.byref(phs), n, fp), "phs root sink")
+
+    def phs_calcstress(self, lay, st, n, filt=None, dev=False):
+        """calcstress over n patches of a ctsm_phs_stress_state (as above)."""
+        fn = self.L.ctsm_phs_calcstress_dev if dev else self.L.ctsm_phs_calcstress
+        fp = C.c_void_p(filt) if dev and filt is not None else (None if filt is None else filt.ctypes.data_as(C.c_void_p))
+        self._check(fn(self.h, C.byref(lay), C.byref(st), n, fp), "phs calcstress")
 
     def drainage(self, st, hydro, filter_c=None, dtime=1800.0):
         """Drop-in drainage of HydrologyDrainage on a host state."""

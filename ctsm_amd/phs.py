@@ -22,11 +22,13 @@ _PATCH_KINDS = ("p", "soip", "veg")
 class PhsArrays:
     """Numpy arrays of every ctsm_phs_state member plus the layout numbers."""
 
+    FIELDS, STRUCT = abi.PHS_FIELDS, abi.PhsState
+
     def __init__(self, ncol, npatch, nlevsno=12, nlevsoi=20, nlevgrnd=25, begc=1, begp=1):
         self.ncol, self.npatch, self.begc, self.begp = ncol, npatch, begc, begp
         self.dims = {"nlevsno": nlevsno, "nlevsoi": nlevsoi, "nlevgrnd": nlevgrnd}
         self.a = {}
-        for name, typ, kind in abi.PHS_FIELDS:
+        for name, typ, kind in self.FIELDS:
             n = npatch if kind in _PATCH_KINDS else ncol
             shape = (n,) if kind in ("c", "p") else (_NLEV[kind](self.dims), n)
             self.a[name] = np.zeros(shape, dtype=np.int32 if typ == "i32" else np.float64)
@@ -39,14 +41,14 @@ class PhsArrays:
                           begg_alloc=1, ldg=1)
 
     def copy(self):
-        o = PhsArrays.__new__(PhsArrays)
+        o = type(self).__new__(type(self))
         o.ncol, o.npatch, o.begc, o.begp, o.dims = self.ncol, self.npatch, self.begc, self.begp, dict(self.dims)
         o.a = {k: v.copy() for k, v in self.a.items()}
         return o
 
     def host_struct(self, omit=()):
-        s = abi.PhsState()
-        for name, typ, _ in abi.PHS_FIELDS:
+        s = self.STRUCT()
+        for name, typ, _ in self.FIELDS:
             if name in omit:
                 continue
             setattr(s, name, self.a[name].ctypes.data_as(abi.i32p if typ == "i32" else abi.f64p))
@@ -64,8 +66,8 @@ class DevicePhs:
             self.ptr[name] = p
 
     def struct(self, omit=()):
-        s = abi.PhsState()
-        for name, typ, _ in abi.PHS_FIELDS:
+        s = self.arrs.STRUCT()
+        for name, typ, _ in self.arrs.FIELDS:
             if name not in omit:
                 setattr(s, name, C.cast(C.c_void_p(self.ptr[name]), abi.i32p if typ == "i32" else abi.f64p))
         return s
@@ -84,18 +86,25 @@ class DevicePhs:
         self.ptr = {}
 
 
-def phs_params(krmax, root_radius, root_density, froot_leaf, psi50, ck):
-    """ctsm_phs_params over contiguous copies (kept alive on the returned object)."""
+class StressArrays(PhsArrays):
+    """Numpy arrays of every ctsm_phs_stress_state member (calcstress)."""
+    FIELDS, STRUCT = abi.PHS_STRESS_FIELDS, abi.PhsStressState
+
+
+def phs_params(krmax, root_radius, root_density, froot_leaf, psi50, ck, kmax=None):
+    """ctsm_phs_params over contiguous copies (kept alive on the returned object);
+    psi50, ck and kmax as (4, npft)."""
     arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (krmax, root_radius, root_density, froot_leaf)]
-    p50 = np.ascontiguousarray(psi50, dtype=np.float64).reshape(4, -1)
-    ckk = np.ascontiguousarray(ck, dtype=np.float64).reshape(4, -1)
+    segs = [np.ascontiguousarray(x, dtype=np.float64).reshape(4, -1) for x in (psi50, ck)]
+    if kmax is not None:
+        segs.append(np.ascontiguousarray(kmax, dtype=np.float64).reshape(4, -1))
     npft = arrs[0].size
-    if any(a.size != npft for a in arrs) or p50.shape[1] != npft or ckk.shape[1] != npft:
+    if any(a.size != npft for a in arrs) or any(a.shape[1] != npft for a in segs):
         raise ValueError("PHS tables must all have npft entries")
     prm = abi.PhsParams(npft=npft, pad=0)
-    for name, a in zip(("krmax", "root_radius", "root_density", "froot_leaf", "psi50", "ck"), arrs + [p50, ckk]):
+    for name, a in zip(("krmax", "root_radius", "root_density", "froot_leaf", "psi50", "ck", "kmax"), arrs + segs):
         setattr(prm, name, a.ctypes.data_as(abi.f64p))
-    prm._keep = arrs + [p50, ckk]
+    prm._keep = arrs + segs
     return prm
 
 
@@ -107,7 +116,9 @@ def default_params(npft=17):
     krmax = rng.uniform(2e-9, 1e-8, npft)
     psi50 = -rng.uniform(1.5e5, 5.3e5, (4, npft))
     ck = np.full((4, npft), 3.95)
-    return phs_params(krmax, np.full(npft, 0.29e-3), np.full(npft, 0.31e6), rng.uniform(1.0, 2.0, npft), psi50, ck)
+    kmax = rng.uniform(1e-8, 4e-8, (4, npft))
+    return phs_params(krmax, np.full(npft, 0.29e-3), np.full(npft, 0.31e6), rng.uniform(1.0, 2.0, npft), psi50, ck,
+                      kmax)
 
 
 def synthetic(ncol, max_patches=4, seed=0, npft=17, nlevsno=12, nlevsoi=20, nlevgrnd=25):
@@ -145,4 +156,54 @@ def synthetic(ncol, max_patches=4, seed=0, npft=17, nlevsno=12, nlevsoi=20, nlev
     A["rootfr"][:] = prof / prof.sum(0, keepdims=True)
     A["vegwp"][:] = -rng.uniform(1e3, 3e5, (4, npatch))
     A["k_soil_root"][:] = rng.uniform(0, 1e-9, (nlevgrnd, npatch))
+    return A
+
+
+def synthetic_stress(ncol, max_patches=4, seed=0, npft=17, nlevsno=12, nlevsoi=20, nlevgrnd=25):
+    """A calcstress case: day and night patches (a positive vegwp(sun) marks
+    night), sunlit and shaded leaf area from none to dense, unstressed
+    conductances of a ci solve, soils from wet to dry, root conductances of the
+    magnitudes the soil-side kernel gives."""
+    rng = np.random.default_rng(seed)
+    counts = rng.integers(1, max_patches + 1, ncol)
+    npatch = int(counts.sum())
+    A = StressArrays(ncol, npatch, nlevsno, nlevsoi, nlevgrnd)
+    A["column"][:] = np.repeat(np.arange(A.begc, A.begc + ncol), counts)
+    A["itype"][:] = rng.integers(0, npft, npatch)
+    A["local_morning"][:] = rng.random(npatch) < 0.5
+    lai = rng.uniform(0, 6, npatch) * (rng.random(npatch) < 0.9)
+    fsun = rng.uniform(0, 0.6, npatch)
+    A["laisun"][:] = lai * fsun * (rng.random(npatch) < 0.9)
+    A["laisha"][:] = lai * (1 - fsun) * (rng.random(npatch) < 0.95)
+    A["elai"][:] = A["laisun"] + A["laisha"]
+    A["esai"][:] = rng.uniform(0, 1.5, npatch)
+    A["tsai"][:] = A["esai"] * rng.uniform(1.0, 1.2, npatch)
+    A["htop"][:] = rng.uniform(0.5, 30, npatch)
+    A["fdry"][:] = rng.uniform(0.3, 1.0, npatch)
+    A["thm"][:] = rng.uniform(270, 310, npatch)
+    A["gb_mol"][:] = rng.uniform(2e5, 3e6, npatch)
+    A["gs_mol_sun"][:] = rng.uniform(0, 4e5, npatch) * (rng.random(npatch) < 0.9)
+    A["gs_mol_sha"][:] = rng.uniform(0, 2e5, npatch) * (rng.random(npatch) < 0.9)
+    A["qsatl"][:] = rng.uniform(0.005, 0.03, npatch)
+    A["qaf"][:] = A["qsatl"] - rng.uniform(-0.002, 0.012, npatch)
+    ns = nlevsno
+    dz = np.concatenate([np.full(4, 0.02), np.full(4, 0.04), np.full(6, 0.06), np.full(6, 0.12),
+                         np.full(nlevgrnd - 20, 1.0)])[:nlevgrnd]
+    zi = np.concatenate([[0.0], np.cumsum(dz)])
+    A["z"][ns:, :] = (0.5 * (zi[:-1] + zi[1:]))[:, None]
+    A["smp_l"][:] = -10.0 ** rng.uniform(2, 5.3, (nlevgrnd, ncol))
+    A["forc_rho"][:] = rng.uniform(1.1, 1.3, ncol)
+    A["forc_pbot"][:] = rng.uniform(8.5e4, 1.02e5, ncol)
+    k = 10.0 ** rng.uniform(-11, -8.5, (nlevgrnd, npatch))
+    k[0] = 0.0
+    k[nlevsoi:] = 0.0
+    k[:, rng.random(npatch) < 0.03] = 0.0                     # no roots at all
+    A["k_soil_root"][:] = k
+    base = -rng.uniform(2e4, 2e5, npatch)
+    A["vegwp"][:] = base[None, :] - np.array([3e4, 2.5e4, 1e4, 0.0])[:, None] * rng.uniform(0, 1, (1, npatch))
+    night = rng.random(npatch) < 0.3
+    A["vegwp"][0, night] = -A["vegwp"][0, night]              # the reference's night mark
+    A["bsun"][:] = -1.0
+    A["bsha"][:] = -1.0
+    A["qflx_tran_veg"][:] = -1.0
     return A

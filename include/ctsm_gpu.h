@@ -502,7 +502,37 @@ typedef struct ctsm_phs_params {
     int32_t       pad;
     const double *krmax, *root_radius, *root_density, *froot_leaf;
     const double *psi50, *ck;
+    const double *kmax;   /* params_inst%kmax, segment-major as psi50; may be NULL when
+                             ctsm_phs_calcstress is not used */
 } ctsm_phs_params;
+
+/* Per-patch arrays of calcstress (PhotosynthesisMod.F90:4490-4710), the
+ * vegetation water potential solve of plant hydraulic stress: the four-segment
+ * Newton iteration (spacF / spacA, :4715-4976) from the given vegwp, the
+ * stomatal conductances it allows (getqflx, :5080-5164), getvegwp (:4979-5077)
+ * where Newton stops on a singular Jacobian or there is no transpiration.
+ * gs_mol_sun / gs_mol_sha are the unstressed conductances the caller's ci
+ * solve produced; bsun / bsha the resulting stress factors.  A positive
+ * vegwp(sun) on input marks night (the reference's convention): the night
+ * branch then also writes qflx_tran_veg, and vegwp_pd where local_morning is
+ * set (the caller's get_local_time(londeg) < isecspday/2). */
+typedef struct ctsm_phs_stress_state {
+    /* patch */
+    const int32_t *itype, *column, *local_morning;              /* (p) */
+    const double  *laisun, *laisha, *elai, *esai, *tsai, *htop;  /* (p) */
+    const double  *fdry, *thm;                                  /* (p) */
+    const double  *gb_mol, *gs_mol_sun, *gs_mol_sha;            /* (p) umol H2O/m2/s */
+    const double  *qsatl, *qaf;                                 /* (p) kg/kg */
+    const double  *k_soil_root;                                 /* [SOIP] */
+    double        *vegwp;                                       /* [VEG] inout (mm) */
+    double        *vegwp_pd;                                    /* [VEG] out (mm, spval unless night morning) */
+    double        *bsun, *bsha;                                 /* (p) out */
+    double        *qflx_tran_veg;                               /* (p) out at night (mm H2O/s) */
+    /* column */
+    const double  *forc_rho, *forc_pbot;                        /* (c) */
+    const double  *z;                                           /* [SNO] (m) */
+    const double  *smp_l;                                       /* [SOI] (mm) */
+} ctsm_phs_stress_state;
 
 /* ---- launch shapes (no counterpart in the reference: set by the autotuner) --
  * richards_lanes: lanes cooperating on one column in the Richards solve
@@ -675,6 +705,13 @@ int ctsm_phs_soil_root_conductance(ctsm_ctx *ctx, const ctsm_layout *lay, const 
 int ctsm_phs_root_sink(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_phs_state *phs,
                        int32_t num_filterc, const int32_t *filterc);
 
+/* calcstress (PhotosynthesisMod.F90:4490-4710) for every patch of the filter
+ * at once: the reference calls it patch by patch from ci_func_PHS (:4227-4486);
+ * a caller with the ci solve batched calls this once per evaluation round.
+ * Needs ctsm_set_phs_params with kmax. */
+int ctsm_phs_calcstress(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_phs_stress_state *st,
+                        int32_t num_filterp, const int32_t *filterp);
+
 /* ---- device-resident entry points -----------------------------------------
  * Same arguments, but every pointer (fields and filter) is device memory and
  * the call only enqueues work on the context's stream.  filter == NULL means
@@ -726,6 +763,8 @@ int ctsm_phs_soil_root_conductance_dev(ctsm_ctx *ctx, const ctsm_layout *lay, co
                                        int32_t num_filterp, const int32_t *filterp_dev);
 int ctsm_phs_root_sink_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_phs_state *phs,
                            int32_t num_filterc, const int32_t *filterc_dev);
+int ctsm_phs_calcstress_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_phs_stress_state *st,
+                            int32_t num_filterp, const int32_t *filterp_dev);
 int ctsm_soilfluxes_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
                         const ctsm_patch_state *pch, const ctsm_soilflux_state *sf,
                         int32_t num_filterc, const int32_t *filterc_dev,

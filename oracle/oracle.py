@@ -302,6 +302,10 @@ def _phs_lib():
         L.oracle_phs_soil_root_conductance.argtypes = [P(abi.Config), P(abi.Layout), P(abi.PhsState),
                                                        P(abi.PhsParams), C.c_int, abi.i32p]
         L.oracle_phs_root_sink.argtypes = [P(abi.Config), P(abi.Layout), P(abi.PhsState), C.c_int, abi.i32p]
+        L.oracle_phs_calcstress.argtypes = [P(abi.Config), P(abi.Layout), P(abi.PhsStressState), P(abi.PhsParams),
+                                            C.c_int, abi.i32p]
+        L.oracle_phs_spac.argtypes = [P(abi.Config), P(abi.Layout), P(abi.PhsStressState), P(abi.PhsParams), C.c_int,
+                                      abi.f64p, C.c_double, C.c_double, abi.f64p, abi.f64p]
         L._phs_ready = True
     return L
 
@@ -335,6 +339,27 @@ def phs_root_sink(A, filter_c=None):
     f, fp = _filt(filter_c if filter_c is not None else np.arange(A.begc, A.begc + A.ncol))
     s = A.host_struct()
     return _phs_lib().oracle_phs_root_sink(C.byref(_phs_cfg(A)), C.byref(A.layout()), C.byref(s), len(f), fp)
+
+
+def phs_calcstress(A, prm, filter_p=None):
+    """calcstress (PhotosynthesisMod.F90:4490-4710) for every patch of the
+    filter of a ctsm_amd.phs.StressArrays, in place."""
+    f, fp = _filt(filter_p if filter_p is not None else np.arange(A.begp, A.begp + A.npatch))
+    s = A.host_struct()
+    return _phs_lib().oracle_phs_calcstress(C.byref(_phs_cfg(A)), C.byref(A.layout()), C.byref(s), C.byref(prm),
+                                            len(f), fp)
+
+
+def phs_spac(A, prm, p, x, q_sun, q_sha):
+    """(f, invA, flag) of spacF / spacA (:4715-4976) for patch p at x."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    f = np.zeros(4)
+    ia = np.zeros(16)
+    s = A.host_struct()
+    flag = _phs_lib().oracle_phs_spac(C.byref(_phs_cfg(A)), C.byref(A.layout()), C.byref(s), C.byref(prm), p,
+                                      x.ctypes.data_as(abi.f64p), q_sun, q_sha, f.ctypes.data_as(abi.f64p),
+                                      ia.ctypes.data_as(abi.f64p))
+    return f, ia.reshape(4, 4), flag
 
 
 def hydro_params(fff=0.5, pc=0.4, mu=0.14, h2osfcflag=1, aq_sp_yield_min=0.01, n_baseflow=1.0,

@@ -46,7 +46,8 @@ STRUCTS = (("ctsm_config", abi.Config), ("ctsm_layout", abi.Layout), ("ctsm_tuni
            ("ctsm_surface_water_state", abi.SurfaceWaterState), ("ctsm_hydro_params", abi.HydroParams),
            ("ctsm_water_table_state", abi.WaterTableState), ("ctsm_hist_field", abi.HistField),
            ("ctsm_ckpt_field", abi.CkptField), ("ctsm_drainage_state", abi.DrainageState),
-           ("ctsm_phs_state", abi.PhsState), ("ctsm_phs_params", abi.PhsParams))
+           ("ctsm_phs_state", abi.PhsState), ("ctsm_phs_params", abi.PhsParams),
+           ("ctsm_phs_stress_state", abi.PhsStressState))
 
 
 def _probe(tmp_path):
@@ -98,8 +99,9 @@ program sizes
   type(ctsm_drainage_state_t) :: l
   type(ctsm_phs_state_t) :: m
   type(ctsm_phs_params_t) :: n
-  print '(14I8)', c_sizeof(a), c_sizeof(b), c_sizeof(c), c_sizeof(d), c_sizeof(e), c_sizeof(f), c_sizeof(g), &
-       c_sizeof(h), c_sizeof(i), c_sizeof(j), c_sizeof(k), c_sizeof(l), c_sizeof(m), c_sizeof(n)
+  type(ctsm_phs_stress_state_t) :: o
+  print '(15I8)', c_sizeof(a), c_sizeof(b), c_sizeof(c), c_sizeof(d), c_sizeof(e), c_sizeof(f), c_sizeof(g), &
+       c_sizeof(h), c_sizeof(i), c_sizeof(j), c_sizeof(k), c_sizeof(l), c_sizeof(m), c_sizeof(n), c_sizeof(o)
 end program sizes
 """)
     mod = os.path.join(ROOT, "ctsm_amd", "fortran", "ctsm_gpu_mod.F90")
@@ -110,7 +112,7 @@ end program sizes
                      C.sizeof(abi.CanopyParams), C.sizeof(abi.PftParams), C.sizeof(abi.CanopyState),
                      C.sizeof(abi.SoilFluxState), C.sizeof(abi.SurfaceWaterState), C.sizeof(abi.HydroParams),
                      C.sizeof(abi.WaterTableState), C.sizeof(abi.DrainageState), C.sizeof(abi.PhsState),
-                     C.sizeof(abi.PhsParams)]
+                     C.sizeof(abi.PhsParams), C.sizeof(abi.PhsStressState)]
 
 
 def test_config_rejected_before_device():

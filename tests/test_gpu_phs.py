@@ -1,8 +1,9 @@
-"""The soil side of plant hydraulic stress on the GPU against the oracle, bit
-for bit: the root-soil conductances of PhotosynthesisHydraulicStress and the
-PHS vertical transpiration sink, through the drop-in entries (host arrays) and
-the device-resident ones, with and without filters, chained as CTSM calls them;
-the vulnerability curve against the reference's known answer; error paths."""
+"""Plant hydraulic stress on the GPU against the oracle, bit for bit: the
+root-soil conductances of PhotosynthesisHydraulicStress, the PHS vertical
+transpiration sink and calcstress (the vegetation water potential's Newton
+solve), through the drop-in entries (host arrays) and the device-resident ones,
+with and without filters, chained as CTSM calls them; the vulnerability curve
+against the reference's known answer; error paths."""
 import numpy as np
 import pytest
 
@@ -139,3 +140,59 @@ def test_error_paths(ctx):
     assert O.phs_root_sink(ref) == 0
     ctx.phs_root_sink(A.layout(), A.host_struct(), A.ncol)
     _same(A, ref, SINK)
+
+
+STRESS = ("vegwp", "vegwp_pd", "bsun", "bsha", "qflx_tran_veg")
+
+
+@pytest.mark.parametrize("ncol,seed", [(1, 51), (700, 52), (30000, 53)])
+def test_calcstress_dropin(ctx, ncol, seed):
+    print(f"case: calcstress drop-in, {ncol} columns, seed {seed}", flush=True)
+    A = PH.synthetic_stress(ncol, seed=seed)
+    ref = A.copy()
+    assert O.phs_calcstress(ref, PH.default_params()) == 0
+    ctx.phs_calcstress(A.layout(), A.host_struct(), A.npatch)
+    _same(A, ref, STRESS)
+
+
+def test_calcstress_dev_with_filter(ctx):
+    print("case: calcstress device entry with a patch filter", flush=True)
+    A = PH.synthetic_stress(5000, seed=54)
+    rng = np.random.default_rng(55)
+    filt = np.sort(rng.choice(np.arange(1, A.npatch + 1), A.npatch // 2, replace=False)).astype(np.int32)
+    ref = A.copy()
+    assert O.phs_calcstress(ref, PH.default_params(), filt) == 0
+    D = PH.DevicePhs(ctx, A)
+    fd = ctx.alloc(filt.nbytes)
+    ctx.h2d(fd, filt)
+    ctx.phs_calcstress(A.layout(), D.struct(), len(filt), fd, dev=True)
+    out = D.fetch()
+    _same(out, ref, STRESS)
+    ctx.free(fd)
+    D.free()
+
+
+def test_calcstress_error_paths(ctx):
+    from ctsm_amd.lib import CTSM_E_ARG, Context, CtsmError
+    A = PH.synthetic_stress(8, seed=56)
+    c = Context()
+    try:
+        p = PH.default_params()
+        c.set_phs_params(PH.phs_params(*p._keep[:6]))           # no kmax
+        with pytest.raises(CtsmError) as ei:
+            c.phs_calcstress(A.layout(), A.host_struct(), A.npatch)
+        assert ei.value.code == CTSM_E_ARG
+    finally:
+        c.close()
+    with pytest.raises(CtsmError) as ei:
+        ctx.phs_calcstress(A.layout(), A.host_struct(omit=("qaf",)), A.npatch)
+    assert ei.value.code == CTSM_E_ARG
+    B = A.copy()
+    B["itype"][2] = 99
+    with pytest.raises(CtsmError) as ei:
+        ctx.phs_calcstress(B.layout(), B.host_struct(), B.npatch)
+    assert ei.value.code == CTSM_E_ARG
+    ref = A.copy()
+    assert O.phs_calcstress(ref, PH.default_params()) == 0
+    ctx.phs_calcstress(A.layout(), A.host_struct(), A.npatch)
+    _same(A, ref, STRESS)

@@ -169,3 +169,136 @@ def test_root_sink_vanishes_at_hydrostatic_equilibrium():
     assert O.phs_root_sink(A) == 0
     assert np.all(A["qflx_rootsoi"][:20] == 0.0)
     assert np.all(A["qflx_phs_neg"] == 0.0) and np.all(A["qflx_hydr_redist"] == 0.0)
+
+
+# ---- calcstress (PhotosynthesisMod.F90:4490-4710) -------------------------------
+RGAS = 6.02214e26 * 1.38065e-23
+
+
+def _forward_q(A, p, gs, lai):
+    """Transpiration a stomatal conductance allows (the Penman form getqflx
+    inverts), numpy."""
+    c = A["column"][p] - 1
+    cf = A["forc_pbot"][c] / (RGAS * 1e-3 * A["thm"][p]) * 1e6
+    efpot = A["forc_rho"][c] * (A["elai"][p] + A["esai"][p]) * A["gb_mol"][p] * (A["qsatl"][p] - A["qaf"][p])
+    if not (efpot > 0 and A["elai"][p] > 0 and gs > 0):
+        return 0.0
+    gb = A["gb_mol"][p]
+    return efpot * A["fdry"][p] * lai * gs / (gb * gs + gb * gb) * gb / A["elai"][p] / cf
+
+
+def _plc_np(x, psi50, ck):
+    v = 2.0 ** -((x / psi50) ** ck)
+    return np.where(v < 0.005, 0.0, v)
+
+
+def _stress_case(seed=1, ncol=1500):
+    A = PH.synthetic_stress(ncol, seed=seed)
+    prm = PH.default_params()
+    B = A.copy()
+    assert O.phs_calcstress(B, prm) == 0
+    return A, B, prm
+
+
+def _seg(prm, ivt):
+    p50, ck, km = (k.reshape(4, -1)[:, ivt] for k in prm._keep[4:7])
+    return p50, ck, km
+
+
+def test_calcstress_without_transpiration_is_getvegwp_closed_form():
+    A, B, prm = _stress_case(seed=2)
+    ns = A.dims["nlevsno"]
+    night = A["vegwp"][0] > 0
+    dry = (A["qsatl"] <= A["qaf"]) & ~night                     # efpot <= 0: no transpiration
+    assert dry.sum() > 20
+    for p in np.nonzero(dry)[0]:
+        c = A["column"][p] - 1
+        k = A["k_soil_root"][:20, p]
+        g = 1000.0 * A["z"][ns:ns + 20, c]
+        smp = A["smp_l"][:20, c]
+        x4 = (k * (smp - g)).sum() / k.sum() if k.sum() != 0 else (smp - g).mean()
+        x3 = x4 - 1000.0 * A["htop"][p]
+        np.testing.assert_allclose(B["vegwp"][:, p], [x3, x3, x3, x4], rtol=1e-12)
+        p50, ck, _ = _seg(prm, A["itype"][p])
+        for b, seg in ((B["bsun"][p], 0), (B["bsha"][p], 1)):
+            e = float(_plc_np(x3, p50[seg], ck[seg]))
+            assert b == pytest.approx(0.0 if e < 0.01 else e, rel=1e-12, abs=0)
+
+
+def test_calcstress_newton_balances_uptake_and_transpiration():
+    A, B, prm = _stress_case(seed=3)
+    ns = A.dims["nlevsno"]
+    night = A["vegwp"][0] > 0
+    checked = 0
+    for p in np.nonzero(~night & (A["qsatl"] > A["qaf"]) & (A["elai"] > 0.01))[0]:
+        qs = _forward_q(A, p, A["gs_mol_sun"][p], A["laisun"][p])
+        qh = _forward_q(A, p, A["gs_mol_sha"][p], A["laisha"][p])
+        if qs + qh <= 0:
+            continue
+        x = B["vegwp"][:, p]
+        f, _, _ = O.phs_spac(B, prm, p + 1, x, qs, qh)
+        if np.sqrt((f * f).sum()) >= 1e-6 * (qs + qh):
+            continue                                            # left by the step or iteration limit
+        c = A["column"][p] - 1
+        k = A["k_soil_root"][:20, p]
+        uptake = (k * (A["smp_l"][:20, c] - x[3] - 1000.0 * A["z"][ns:ns + 20, c])).sum()
+        p50, ck, _ = _seg(prm, A["itype"][p])
+        lost = qs * _plc_np(x[0], p50[0], ck[0]) + qh * _plc_np(x[1], p50[1], ck[1])
+        assert uptake == pytest.approx(lost, rel=1e-5, abs=1e-6 * (qs + qh))
+        # the stress factors are the conductances that carry the attenuated fluxes
+        if A["laisun"][p] > 0.001 and A["laisha"][p] > 0.001:
+            for b, gs, lai, q, seg in ((B["bsun"][p], A["gs_mol_sun"][p], A["laisun"][p], qs, 0),
+                                       (B["bsha"][p], A["gs_mol_sha"][p], A["laisha"][p], qh, 1)):
+                if q > 0 and b > 0:
+                    want = q * _plc_np(x[seg], p50[seg], ck[seg])
+                    assert _forward_q(A, p, b * gs, lai) == pytest.approx(want, rel=1e-9)
+        checked += 1
+    assert checked > 100
+
+
+@pytest.mark.parametrize("shaded", [True, False])
+def test_spacA_inverts_the_jacobian_of_spacF(shaded):
+    A, _, prm = _stress_case(seed=4, ncol=300)
+    night = A["vegwp"][0] > 0
+    ok = ~night & (A["laisun"] > 0.01) & (A["tsai"] > 0.1) & (A["k_soil_root"][:20].sum(0) > 0)
+    ok &= (A["laisha"] > 0.01) if shaded else True
+    done = 0
+    for p in np.nonzero(ok)[0][:40]:
+        if not shaded:
+            A["laisha"][p] = 0.0
+        p50, _, _ = _seg(prm, A["itype"][p])
+        x = 0.3 * p50 * np.array([1.3, 1.2, 1.0, 0.8])           # on the smooth part of every curve
+        qs, qh = 2e-5, (1e-5 if shaded else 0.0)
+        _, iA, flag = O.phs_spac(A, prm, p + 1, x, qs, qh)
+        assert flag == 0
+        J = np.zeros((4, 4))
+        for k in range(4):
+            h = 1e-5 * abs(x[k])
+            e = np.zeros(4)
+            e[k] = h
+            fp, _, _ = O.phs_spac(A, prm, p + 1, x + e, qs, qh)
+            fm, _, _ = O.phs_spac(A, prm, p + 1, x - e, qs, qh)
+            J[:, k] = (fp - fm) / (2 * h)
+        # the Newton step is x + invA f, so invA is minus the inverse Jacobian
+        if shaded:
+            np.testing.assert_allclose(iA @ J, -np.eye(4), atol=1e-5)
+        else:
+            # spacF swaps f(sun) and f(sha) when laisha is below the limit (the
+            # sun equation in the sha slot), and spacA inverts the 3x3 block of
+            # slots sha, xyl, root: equations sun, xyl, root against x sun, xyl, root
+            Jb = J[np.ix_([1, 2, 3], [0, 2, 3])]
+            np.testing.assert_allclose(iA[1:, 1:] @ Jb, -np.eye(3), atol=1e-5)
+        done += 1
+    assert done >= 10
+
+
+def test_calcstress_night_outputs_and_ranges():
+    A, B, prm = _stress_case(seed=5)
+    night = A["vegwp"][0] > 0
+    assert night.any() and (~night).any()
+    assert np.all(B["qflx_tran_veg"][night] >= 0) and np.all(B["qflx_tran_veg"][~night] == -1.0)
+    pd = night & (A["local_morning"] != 0)
+    np.testing.assert_array_equal(B["vegwp_pd"][:, pd], B["vegwp"][:, pd])
+    assert np.all(B["vegwp_pd"][:, ~pd] == 1e36)
+    for b in (B["bsun"], B["bsha"]):
+        assert np.all((b == 0) | ((b >= 0.01) & (b <= 1.0 + 1e-9)))
