@@ -229,12 +229,13 @@ def test_calcstress_newton_balances_uptake_and_transpiration():
     A, B, prm = _stress_case(seed=3)
     ns = A.dims["nlevsno"]
     night = A["vegwp"][0] > 0
-    checked = 0
+    checked = candidates = 0
     for p in np.nonzero(~night & (A["qsatl"] > A["qaf"]) & (A["elai"] > 0.01))[0]:
         qs = _forward_q(A, p, A["gs_mol_sun"][p], A["laisun"][p])
         qh = _forward_q(A, p, A["gs_mol_sha"][p], A["laisha"][p])
         if qs + qh <= 0:
             continue
+        candidates += 1
         x = B["vegwp"][:, p]
         f, _, _ = O.phs_spac(B, prm, p + 1, x, qs, qh)
         if np.sqrt((f * f).sum()) >= 1e-6 * (qs + qh):
@@ -254,6 +255,7 @@ def test_calcstress_newton_balances_uptake_and_transpiration():
                     assert _forward_q(A, p, b * gs, lai) == pytest.approx(want, rel=1e-9)
         checked += 1
     assert checked > 100
+    assert checked >= 0.95 * candidates                         # Newton meets tolf = 1e-6 nearly everywhere
 
 
 @pytest.mark.parametrize("shaded", [True, False])
@@ -302,3 +304,40 @@ def test_calcstress_night_outputs_and_ranges():
     assert np.all(B["vegwp_pd"][:, ~pd] == 1e36)
     for b in (B["bsun"], B["bsha"]):
         assert np.all((b == 0) | ((b >= 0.01) & (b <= 1.0 + 1e-9)))
+
+
+def test_calcstress_night_water_potentials_carry_the_flux():
+    # at night getvegwp sets the potentials from the stressed conductances: the
+    # soil uptake equals their transpiration, each segment's potential drop
+    # carries it through that segment's conductance, and without stem area
+    # the xylem sits one canopy height above the root
+    A = PH.synthetic_stress(1500, seed=6)
+    night = A["vegwp"][0] > 0
+    nostem = night & (np.arange(A.npatch) % 5 == 0)
+    A["tsai"][nostem] = 0.0
+    prm = PH.default_params()
+    B = A.copy()
+    assert O.phs_calcstress(B, prm) == 0
+    checked = 0
+    for p in np.nonzero(night)[0]:
+        x = B["vegwp"][:, p]
+        qs = _forward_q(A, p, B["bsun"][p] * A["gs_mol_sun"][p], A["laisun"][p])
+        qh = _forward_q(A, p, B["bsha"][p] * A["gs_mol_sha"][p], A["laisha"][p])
+        q = qs + qh
+        if A["k_soil_root"][:20, p].sum() > 0:                 # without roots nothing is taken up
+            assert B["qflx_tran_veg"][p] == pytest.approx(max(q, 0.0), rel=1e-8, abs=1e-18)
+        else:
+            assert B["qflx_tran_veg"][p] == 0.0
+        p50, ck, km = _seg(prm, A["itype"][p])
+        grav1 = 1000.0 * A["htop"][p]
+        fr = float(_plc_np(x[3], p50[3], ck[3]))
+        if A["tsai"][p] > 0 and fr > 0:
+            assert (x[3] - x[2] - grav1) * fr * km[3] / A["htop"][p] * A["tsai"][p] == pytest.approx(q, rel=1e-8,
+                                                                                                   abs=1e-18)
+        else:
+            assert x[2] == pytest.approx(x[3] - grav1, rel=1e-14)
+        fx = float(_plc_np(x[2], p50[2], ck[2]))
+        if A["laisha"][p] > 0 and fx > 0:
+            assert (x[2] - x[1]) * fx * km[2] * A["laisha"][p] == pytest.approx(qh, rel=1e-8, abs=1e-18)
+        checked += 1
+    assert checked > 300 and nostem.any()
