@@ -348,10 +348,25 @@ def _phs_soil_side(ctx, ncol):
     sb = A.ncol * (16 + nl * 24) + A.npatch * 24 + int(used.sum()) * (8 + nl * 8)
     roof = lambda b, ms: {"alg_bytes_per_launch": b, "achieved_GBps": b / (ms * 1e-3) / 1e9,
                           "frac": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
-    return {"workload": f"{A.ncol:,} columns, {A.npatch:,} patches (0-4 per column), {nl} soil layers",
-            "conductance_kernel_ms": cond_ms, "root_sink_kernel_ms": sink_ms,
-            "conductance_roofline": dict(kernel="k_phs_conductance", **roof(cb, cond_ms)),
-            "root_sink_roofline": dict(kernel="k_phs_root_sink", **roof(sb, sink_ms))}
+    out = {"workload": f"{A.ncol:,} columns, {A.npatch:,} patches (0-4 per column), {nl} soil layers",
+           "conductance_kernel_ms": cond_ms, "root_sink_kernel_ms": sink_ms,
+           "conductance_roofline": dict(kernel="k_phs_conductance", **roof(cb, cond_ms)),
+           "root_sink_roofline": dict(kernel="k_phs_root_sink", **roof(sb, sink_ms))}
+    # calcstress: the Newton solve of the vegetation water potential, every patch
+    # once per call (the input is restored before each timed call)
+    B = PH.synthetic_stress(ncol, max_patches=4, seed=4)
+    D = PH.DevicePhs(ctx, B)
+    s2 = D.struct()
+    lay2 = B.layout()
+    vw = D.ptr["vegwp"]
+    ctx.phs_calcstress(lay2, s2, B.npatch, None, dev=True)
+    stress_ms, = time_events(ctx, [lambda: ctx.phs_calcstress(lay2, s2, B.npatch, None, dev=True)], 5,
+                             prefill=lambda: ctx.h2d(vw, B["vegwp"]))
+    D.free()
+    out["calcstress"] = {"patches": B.npatch, "kernel_ms": stress_ms,
+                         "patch_solves_per_s": B.npatch / (stress_ms * 1e-3),
+                         "bound": "FP64 latency (Newton iterations, 12 pow per iteration)"}
+    return out
 
 
 def _history_and_checkpoint(ctx, ds, structs):
