@@ -41,6 +41,7 @@ def build(verbose=False, extra=()):
 FORTRAN_DRIVER = os.path.join(HERE, "fortran", "drive_soilwater")
 FORTRAN_STEP_DRIVER = os.path.join(HERE, "fortran", "drive_step")
 FORTRAN_PHS_DRIVER = os.path.join(HERE, "fortran", "drive_phs")
+FORTRAN_STRESS_DRIVER = os.path.join(HERE, "fortran", "drive_stress")
 
 
 def flang():
@@ -52,7 +53,7 @@ def flang():
 
 def build_fortran_driver(verbose=False):
     """The Fortran drivers (ctsm_gpu_mod with drive_soilwater.F90, with
-    drive_step.F90 and with drive_phs.F90, linked to the in-tree library);
+    drive_step.F90, with drive_phs.F90 and with drive_stress.F90, linked to the in-tree library);
     returns their paths, or None without flang."""
     fc = flang()
     if fc is None:
@@ -60,7 +61,7 @@ def build_fortran_driver(verbose=False):
     import tempfile
     fdir = os.path.join(HERE, "fortran")
     for prog, exe in (("drive_soilwater.F90", FORTRAN_DRIVER), ("drive_step.F90", FORTRAN_STEP_DRIVER),
-                      ("drive_phs.F90", FORTRAN_PHS_DRIVER)):
+                      ("drive_phs.F90", FORTRAN_PHS_DRIVER), ("drive_stress.F90", FORTRAN_STRESS_DRIVER)):
         with tempfile.TemporaryDirectory() as mod_dir:
             cmd = [fc, "-O2", "-J", mod_dir, os.path.join(fdir, "ctsm_gpu_mod.F90"), os.path.join(fdir, prog),
                    "-L", HERE, "-lctsm_mi355x", "-Wl,-rpath,$ORIGIN/..", "-o", exe + ".tmp"]
@@ -71,7 +72,7 @@ def build_fortran_driver(verbose=False):
             sys.stderr.write(r.stdout + r.stderr)
             raise RuntimeError("flang failed")
         os.replace(exe + ".tmp", exe)
-    return FORTRAN_DRIVER, FORTRAN_STEP_DRIVER, FORTRAN_PHS_DRIVER
+    return FORTRAN_DRIVER, FORTRAN_STEP_DRIVER, FORTRAN_PHS_DRIVER, FORTRAN_STRESS_DRIVER
 
 
 if __name__ == "__main__":

@@ -1,7 +1,7 @@
 ! drive_phs.F90 -- written by gen_drive_step.py from the member list of
 ! ctsm_amd/abi.py; do not edit by hand.
 !
-! The plant-hydraulic-stress soil entries from Fortran through ctsm_gpu_mod, on
+! Plant-hydraulic-stress entries from Fortran through ctsm_gpu_mod, on
 ! arrays held as CTSM holds them (reference bounds, bound by c_loc).
 !
 ! Usage: drive_phs IN OUT.  IN holds, as int32, ncol, npatch, nlevsno, nlevsoi,
@@ -25,6 +25,7 @@ program drive_phs
   integer(c_int32_t), allocatable :: filter(:)
   real(c_double), allocatable, target :: krmax(:), root_radius(:), root_density(:), froot_leaf(:)
   real(c_double), allocatable, target :: psi50(:,:), ck(:,:)
+
   integer(c_int32_t), allocatable, target :: a_itype(:)
   integer(c_int32_t), allocatable, target :: a_column(:)
   integer(c_int32_t), allocatable, target :: a_active(:)
@@ -56,6 +57,7 @@ program drive_phs
   allocate(krmax(0:npft-1), root_radius(0:npft-1), root_density(0:npft-1), froot_leaf(0:npft-1))
   allocate(psi50(0:npft-1, 4), ck(0:npft-1, 4))
   read(u) krmax, root_radius, root_density, froot_leaf, psi50, ck
+
   begc = 1
   endc = ncol
   begp = 1

@@ -4,7 +4,8 @@ water tables, drainage) through ctsm_gpu_mod on a state held the way
 CTSM holds it (one allocatable array per member with the reference's bounds,
 bound by c_loc; members shared between structs point at the same array).
 
-drive_phs.F90 calls the two plant-hydraulic-stress soil entries the same way.
+drive_phs.F90 calls the two plant-hydraulic-stress soil entries the same way,
+drive_stress.F90 the calcstress entry.
 The member lists come from ctsm_amd/abi.py, so the programs follow the
 header; run this script again after a state struct changes:
 
@@ -193,9 +194,11 @@ PHS_FIRST = {"p": "(begp)", "c": "(begc)", "soip": "(begp, 1)", "veg": "(begp, 1
              "soi": "(begc, 1)"}
 
 
-def generate_phs():
+def generate_phs(stress=False):
+    """drive_phs.F90, or with stress drive_stress.F90 (calcstress)."""
+    fields = abi.PHS_STRESS_FIELDS if stress else abi.PHS_FIELDS
     decl, alloc, rd, bind, wr = [], [], [], [], []
-    for n, t, k in abi.PHS_FIELDS:
+    for n, t, k in fields:
         typ = "integer(c_int32_t)" if t == "i32" else "real(c_double)"
         rank = "(:)" if k in ("c", "p") else "(:,:)"
         decl.append(f"  {typ}, allocatable, target :: a_{n}{rank}")
@@ -204,19 +207,47 @@ def generate_phs():
         bind.append(f"  phs%{n} = c_loc(a_{n}{PHS_FIRST[k]})")
         wr.append(f"  write(u) a_{n}")
     nl = "\n"
-    return f'''! drive_phs.F90 -- written by gen_drive_step.py from the member list of
+    prog = "drive_stress" if stress else "drive_phs"
+    stype = "ctsm_phs_stress_state_t" if stress else "ctsm_phs_state_t"
+    if stress:
+        usage = ("! Usage: drive_stress IN OUT.  IN as drive_phs's (mode ignored), the tables\n"
+                 "! followed by kmax (npft, 4), then every ctsm_phs_stress_state member; calls\n"
+                 "! ctsm_phs_calcstress over every patch.")
+        calls = ("  if (rc == CTSM_OK) then\n"
+                 "     allocate(filter(npatch))\n"
+                 "     filter = [(i, i = begp, endp)]\n"
+                 "     rc = ctsm_phs_calcstress(ctx, lay, phs, npatch, filter)\n"
+                 "  end if")
+        kmax_decl = "  real(c_double), allocatable, target :: kmax(:,:)"
+        kmax_read = "  allocate(kmax(0:npft-1, 4))\n  read(u) kmax\n  prm%kmax = c_loc(kmax(0, 1))"
+    else:
+        usage = ("! Usage: drive_phs IN OUT.  IN holds, as int32, ncol, npatch, nlevsno, nlevsoi,\n"
+                 "! nlevgrnd, npft and mode (1 conductances over every patch, 2 the sink over\n"
+                 "! every column, 3 both); the PHS tables (krmax, root_radius, root_density,\n"
+                 "! froot_leaf: npft doubles each; psi50 and ck: (npft, 4)); then every\n"
+                 "! ctsm_phs_state member in member order.  OUT receives the members after the\n"
+                 "! calls.  Prints the return code.")
+        calls = ("  if (rc == CTSM_OK .and. iand(mode, 1) /= 0) then\n"
+                 "     allocate(filter(npatch))\n"
+                 "     filter = [(i, i = begp, endp)]\n"
+                 "     rc = ctsm_phs_soil_root_conductance(ctx, lay, phs, npatch, filter)\n"
+                 "     deallocate(filter)\n"
+                 "  end if\n"
+                 "  if (rc == CTSM_OK .and. iand(mode, 2) /= 0) then\n"
+                 "     allocate(filter(ncol))\n"
+                 "     filter = [(i, i = begc, endc)]\n"
+                 "     rc = ctsm_phs_root_sink(ctx, lay, phs, ncol, filter)\n"
+                 "  end if")
+        kmax_decl = ""
+        kmax_read = ""
+    return f'''! {prog}.F90 -- written by gen_drive_step.py from the member list of
 ! ctsm_amd/abi.py; do not edit by hand.
 !
-! The plant-hydraulic-stress soil entries from Fortran through ctsm_gpu_mod, on
+! Plant-hydraulic-stress entries from Fortran through ctsm_gpu_mod, on
 ! arrays held as CTSM holds them (reference bounds, bound by c_loc).
 !
-! Usage: drive_phs IN OUT.  IN holds, as int32, ncol, npatch, nlevsno, nlevsoi,
-! nlevgrnd, npft and mode (1 conductances over every patch, 2 the sink over
-! every column, 3 both); the PHS tables (krmax, root_radius, root_density,
-! froot_leaf: npft doubles each; psi50 and ck: (npft, 4)); then every
-! ctsm_phs_state member in member order.  OUT receives the members after the
-! calls.  Prints the return code.
-program drive_phs
+{usage}
+program {prog}
   use iso_c_binding
   use ctsm_gpu_mod
   implicit none
@@ -225,12 +256,13 @@ program drive_phs
   character(len=4096) :: fin, fout
   type(ctsm_config_t) :: cfg
   type(ctsm_layout_t) :: lay
-  type(ctsm_phs_state_t) :: phs
+  type({stype}) :: phs
   type(ctsm_phs_params_t) :: prm
   type(c_ptr) :: ctx
   integer(c_int32_t), allocatable :: filter(:)
   real(c_double), allocatable, target :: krmax(:), root_radius(:), root_density(:), froot_leaf(:)
   real(c_double), allocatable, target :: psi50(:,:), ck(:,:)
+{kmax_decl}
 {nl.join(decl)}
 
   call get_command_argument(1, fin)
@@ -240,6 +272,7 @@ program drive_phs
   allocate(krmax(0:npft-1), root_radius(0:npft-1), root_density(0:npft-1), froot_leaf(0:npft-1))
   allocate(psi50(0:npft-1, 4), ck(0:npft-1, 4))
   read(u) krmax, root_radius, root_density, froot_leaf, psi50, ck
+{kmax_read}
   begc = 1
   endc = ncol
   begp = 1
@@ -272,29 +305,20 @@ program drive_phs
   lay%begg_alloc = 1
   lay%ldg = 1
   rc = ctsm_set_phs_params(ctx, prm)
-  if (rc == CTSM_OK .and. iand(mode, 1) /= 0) then
-     allocate(filter(npatch))
-     filter = [(i, i = begp, endp)]
-     rc = ctsm_phs_soil_root_conductance(ctx, lay, phs, npatch, filter)
-     deallocate(filter)
-  end if
-  if (rc == CTSM_OK .and. iand(mode, 2) /= 0) then
-     allocate(filter(ncol))
-     filter = [(i, i = begc, endc)]
-     rc = ctsm_phs_root_sink(ctx, lay, phs, ncol, filter)
-  end if
+{calls}
   print '(I0)', rc
   i = ctsm_destroy(ctx)
 
   open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
 {nl.join(wr)}
   close(u)
-end program drive_phs
+end program {prog}
 '''
 
 
 if __name__ == "__main__":
-    for name, text in (("drive_step.F90", generate()), ("drive_phs.F90", generate_phs())):
+    for name, text in (("drive_step.F90", generate()), ("drive_phs.F90", generate_phs()),
+                       ("drive_stress.F90", generate_phs(stress=True))):
         path = os.path.join(HERE, name)
         with open(path, "w") as f:
             f.write(text)

@@ -202,3 +202,35 @@ def test_phs_soil_side_from_fortran_bitwise(tmp_path, mode):
         pos += a.nbytes
         np.testing.assert_array_equal(got, a, err_msg=n)
     assert pos == len(raw)
+
+
+def test_calcstress_from_fortran_bitwise(tmp_path):
+    # drive_stress.F90: ctsm_phs_calcstress on arrays with the reference's bounds
+    from ctsm_amd import phs as PH
+    exe = build.FORTRAN_STRESS_DRIVER
+    if not os.path.exists(exe):
+        pytest.skip("Fortran driver not built (no flang)")
+    print("case: calcstress from Fortran, 400 columns", flush=True)
+    A = PH.synthetic_stress(400, seed=61)
+    prm = PH.default_params()
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    d = A.dims
+    with open(fin, "wb") as f:
+        f.write(np.array([A.ncol, A.npatch, d["nlevsno"], d["nlevsoi"], d["nlevgrnd"], prm.npft, 0],
+                         dtype=np.int32).tobytes())
+        for a in prm._keep:                                     # the six tables, then kmax
+            f.write(np.ascontiguousarray(a).tobytes())
+        for n, _, _ in abi.PHS_STRESS_FIELDS:
+            f.write(A[n].tobytes())
+    r = subprocess.run([exe, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert int(r.stdout.split()[0]) == 0, r.stdout
+    ref = A.copy()
+    assert O.phs_calcstress(ref, prm) == 0
+    raw, pos = fout.read_bytes(), 0
+    for n, _, _ in abi.PHS_STRESS_FIELDS:
+        a = ref[n]
+        got = np.frombuffer(raw[pos:pos + a.nbytes], dtype=a.dtype).reshape(a.shape)
+        pos += a.nbytes
+        np.testing.assert_array_equal(got, a, err_msg=n)
+    assert pos == len(raw)
