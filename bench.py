@@ -1,22 +1,26 @@
 """Benchmark of the column biogeophysics step on one MI355X.
 
-A "step" is one time step (dtime = 1800 s) of the soil physics over every
-column of the batch: SoilTemperature (snow + soil heat solve with phase
-change) with SetSoilWaterFractions, then the moisture-form Richards solve
-(the library's fused soil step; launch shape -- lanes per column, blocks,
-streams the columns are split across -- chosen by the autotuner).  The timed
-steps are captured once as one HIP graph and launched once.  The headline
-workload is the uneven-column batch: 65,536 columns, snow layers uniform
-0..12, bedrock index uniform 5..20, one patch per column, FP64, synthetic
-seeded inputs resident in HBM.  The same steps are also timed in the plain
-form (one launch per reference routine, intermediates through HBM) for the
-fused/plain ratio.
+A "step" is one full land time step (dtime = 1800 s) over every column of the
+batch, in clm_drv order: the before-step save, CanopyFluxes (per-patch
+leaf-temperature iteration with Photosynthesis, FrictionVelocity, QSat),
+SoilTemperature (snow + soil heat solve with phase change) with
+SetSoilWaterFractions, SoilFluxes, the transpiration sink, the surface water
+input chain, the moisture-form Richards solve (SoilWater), the water tables
+with RenewCondensation, and the soil-column drainage.  The headline workload
+is 1,048,576 columns (snow layers uniform 0..12, bedrock index uniform
+5..20, 1-16 patches per column, FP64, seeded synthetic inputs) resident in
+HBM, generated in binned column order, for 48 steps (one day); the soil
+launch shape and the CanopyFluxes round split are chosen by the autotuner;
+the 48 timed steps are captured once as one HIP graph and launched once.
+The same 48 steps are also run in the plain form (one launch per reference
+routine, CanopyFluxes iterating every patch to convergence in one launch).
 
 Prints ONE JSON line with the metric, the roofline of the dominant kernel
-(algorithmic bytes per launch / its measured launch time vs 8 TB/s), a CPU
-baseline (the C oracle on a bounded sample, one column chunk per thread) and,
-in "detail", the full-column step with CanopyFluxes (262,144 columns, 1-16
-patches each).
+(algorithmic bytes per CanopyFluxes call / its measured time vs 8 TB/s) and
+of the whole step, a CPU baseline (the C oracle's full land step on a
+seeded sample of the same workload, one column chunk per thread) and, in
+"detail", the soil-only uneven batch (65,536 columns) and the served
+ensemble.
 """
 import argparse
 import json
@@ -31,16 +35,13 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per launch of the full-batch kernel from rocprofv3 --pmc
-# FETCH_SIZE / WRITE_SIZE (KiB per dispatch), each scaled by the same
-# counter's reading on a streaming copy of 1 GiB (FETCH x 2.0, WRITE x 1.0):
-# k_soilwater_g<8> on the 65,536-column batch reads 151,589 KiB x 2 and
-# writes 276,934 KiB (profiles/richards_heat_solve_uneven65536.md).
-TRAFFIC = {"k_soilwater_g<8>": (151589.2 * 2.0 + 276934.4) * 1024.0}
-# FP64 VALU utilisation of the same launch from rocprofv3 --pmc SQ_INSTS_VALU_*_F64
-# (same profile): 64 lanes x (ADD + MUL + 2 FMA + TRANS) instructions per launch
-# over the launch time, against the 78.6 TFLOP/s FP64 vector peak.
-FP64_VALU_FRAC = {"k_soilwater_g<8>": 64 * (62.838e6 + 65.686e6 + 2 * 92.273e6 + 15.013e6) / 1.6863e-3 / 78.6e12}
+# Measured HBM traffic (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, KiB per
+# dispatch, scaled by the same counters' readings on a streaming copy of
+# known size) and FP64 VALU share (SQ_INSTS_VALU_*_F64 over the launch time
+# against the 78.6 TFLOP/s vector peak) of a dominant kernel, keyed by
+# (kernel, workload, columns).  The bench reports them only for the run they
+# were measured on; any other run reports null.
+PROFILED = {}
 
 
 def richards_bytes(st):
@@ -81,21 +82,6 @@ def soiltemp_bytes(st):
 _CANOPY_INOUT = {"displa", "z0mv", "t_veg", "t_stem", "snocan", "liqcan", "cgrnds", "cgrndl"}
 
 
-def _canopy_const_fields():
-    import re
-    with open(os.path.join(ROOT, "include", "ctsm_gpu.h")) as f:
-        s = f.read()
-    body = s[s.index("typedef struct ctsm_canopy_state {"):s.index("} ctsm_canopy_state;")]
-    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
-    names = set()
-    for line in body.splitlines():
-        line = line.strip()
-        if line.startswith("const"):
-            for nm in line.split(None, 2)[2].rstrip(";").split(","):
-                names.add(nm.strip().lstrip("*").strip())
-    return names
-
-
 def canopy_bytes(st):
     """Compulsory HBM bytes of one CanopyFluxes launch (k_canopy): every patch
     field of ctsm_canopy_state counted once if read and once if written
@@ -104,7 +90,7 @@ def canopy_bytes(st):
     state (23 FP64 + snl, 7 nlevgrnd profiles read, eff_porosity and
     h2osoi_liqvol written) and per gridcell the forcing (9 FP64 + 1 int32)."""
     from ctsm_amd import abi
-    const = _canopy_const_fields()
+    const = _const_fields("ctsm_canopy_state")
     ins = outs = 0
     for name, t, kind in abi.CANOPY_FIELDS:
         if kind not in ("p", "soip"):
@@ -121,36 +107,6 @@ def canopy_bytes(st):
     per_col = 23 * 8 + 4 + NG * 7 * 8 + NG * 2 * 8
     per_grid = 9 * 8 + 4
     return int(st.dims.npatch * (ins + outs) + st.dims.ncol * (per_col + per_grid))
-
-
-def cpu_baseline(st, seconds=15.0, threads=None):
-    """Time the C oracle on a bounded slice of the same workload, one column
-    chunk per thread (the oracle runs outside the interpreter lock)."""
-    from concurrent.futures import ThreadPoolExecutor
-    from oracle import oracle as O
-    threads = threads or min(16, os.cpu_count() or 1)
-    per = 1024
-    ncol = st.dims.ncol
-    # a seeded sample of the whole batch (its storage order is binned by work)
-    pick = np.random.default_rng(12345).permutation(ncol)
-    subs = [st.subset(np.sort(pick[(np.arange(per) + i * per) % ncol])) for i in range(threads)]
-
-    def work(sub):
-        n = 0
-        t = time.perf_counter()
-        while True:
-            O.soil_step(sub)
-            n += 1
-            if time.perf_counter() - t >= seconds or n >= 48 * 40:
-                return n
-
-    t0 = time.perf_counter()
-    with ThreadPoolExecutor(threads) as ex:
-        steps = list(ex.map(work, subs))
-    el = time.perf_counter() - t0
-    total = per * sum(steps)
-    return total / el, threads, (f"{threads} threads x {per} columns (a seeded random sample of the batch) x "
-                                 f"{min(steps)}-{max(steps)} steps ({el:.1f} s)")
 
 
 def served_ensemble(ctx, args):
@@ -198,130 +154,6 @@ def time_events(ctx, fns, reps, prefill=None, inner=1):
         for e in evs:
             ctx.destroy_event(e)
     return [a / reps for a in acc]
-
-
-def full_column(ctx, args):
-    """Full column step with CanopyFluxes (1-16 patches per column): land
-    step throughput and the canopy kernel's own launch time."""
-    from ctsm_amd import autotune as AT, synthetic
-    from ctsm_amd.state import DeviceState
-    st = synthetic.canopy_workload("canopy", ncol=args.canopy_ncol)
-    ctx.set_canopy_params(synthetic.canopy_params(), synthetic.pft_table())
-    # warm-up steps, then store the batch in binned column order (as the soil bench)
-    ds = DeviceState(st, ctx)
-    for _ in range(max(args.warmup, 1)):
-        ctx.land_step_dev(ds)
-    ctx.sync()
-    stw = ds.to_host()
-    ds.free()
-    order = AT.column_order(stw.col["snl"], stw.col["nbedrock"], stw.col["num_substeps"], parts=4) - 1
-    st = stw.subset(order)
-    ds = DeviceState(st, ctx)
-    filt = None
-    structs = (*ds.structs(), ds.canopy_struct())
-    for _ in range(args.warmup):
-        ctx.land_step_dev(ds, filterc_dev=filt, structs=structs)
-    ctx.sync()
-    split, _, split_table = AT.autotune_canopy(ctx, ds)
-    e0 = ctx.record()
-    t0 = time.perf_counter()
-    for _ in range(args.canopy_steps):
-        ctx.land_step_dev(ds, filterc_dev=filt, structs=structs)
-    e1 = ctx.record()
-    ctx.sync()
-    wall = time.perf_counter() - t0
-    step_ms = ctx.elapsed_ms(e0, e1) / args.canopy_steps
-    ctx.destroy_event(e0)
-    ctx.destroy_event(e1)
-    can_ms, = time_events(ctx, [lambda: ctx.canopyfluxes_dev(ds, structs=structs)], 3)
-    b = canopy_bytes(st)
-    nit = np.empty(st.dims.npatch)
-    ctx.d2h(nit, ds.can["num_iter"])
-    out = {"workload": f"{st.dims.ncol:,} columns, {st.dims.npatch:,} patches (1-16 per column): CanopyFluxes, "
-                       "transpiration sink, SoilTemperature, SoilWater; no snow, 20 layers",
-           "column_steps_per_s": st.dims.ncol * args.canopy_steps / wall,
-           "patch_steps_per_s": st.dims.npatch * args.canopy_steps / wall,
-           "step_ms": step_ms, "steps": args.canopy_steps,
-           "canopy_kernel_ms": can_ms,
-           "mean_canopy_iterations": float(nit.mean()),
-           "canopy_rounds": {"first_iterations": split[0], "later_iterations": split[1],
-                             "candidates_ms": {f"{a}/{c}": round(ms, 3) for (a, c), ms in split_table}},
-           "canopy_roofline": {"kernel": "k_canopy_round (all rounds of one call)", "alg_bytes_per_launch": b,
-                               "achieved_GBps": b / (can_ms * 1e-3) / 1e9,
-                               "frac": b / (can_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
-    out["with_soilfluxes_and_surface_water"] = _full_land_step(ctx, args, ds, structs)
-    ds.free()
-    out["phs_soil_side"] = _phs_soil_side(ctx, st.dims.ncol)
-    if args.canopy_parts > 1:
-        out["concurrent_parts"] = _full_column_parts(ctx, args, st)
-    return out
-
-
-def surface_water_bytes(st):
-    """Algorithmic bytes of one surface water launch: per column 22 FP64 inputs
-    (14 of the chain, frac_sno_eff, frac_h2osfc, icefrac and hksat of layers
-    1-3), snl, and 16 FP64 outputs (h2osfc included)."""
-    return st.dims.ncol * (22 * 8 + 4 + 16 * 8)
-
-
-def _full_land_step(ctx, args, ds, structs):
-    """The land step with the reference's bookkeeping (before-step save,
-    SoilFluxes, the surface water input chain feeding SoilWater's
-    infiltration, the water tables and RenewCondensation after it, the
-    drainage of HydrologyDrainage) on the
-    same resident batch, and the launch times of the SoilFluxes and surface
-    water kernels."""
-    from ctsm_amd import synthetic
-    hydro = synthetic.hydro_params()
-    full = (*structs, ds.soilflux_struct())
-    for _ in range(2):
-        ctx.land_step_full_dev(ds, structs=full, hydro=hydro, water_table=True, drainage=True)
-    ctx.sync()
-    e0 = ctx.record()
-    t0 = time.perf_counter()
-    for _ in range(args.canopy_steps):
-        ctx.land_step_full_dev(ds, structs=full, hydro=hydro, water_table=True, drainage=True)
-    e1 = ctx.record()
-    ctx.sync()
-    wall = time.perf_counter() - t0
-    step_ms = ctx.elapsed_ms(e0, e1) / args.canopy_steps
-    ctx.destroy_event(e0)
-    ctx.destroy_event(e1)
-    # the same steps captured once and launched as one graph
-    g = ctx.graph_capture_full(ds, nsteps=args.canopy_steps, hydro=hydro, water_table=True, drainage=True)
-    ctx.graph_launch(g)                      # first launch uploads the graph
-    ctx.sync()
-    e0 = ctx.record()
-    t0 = time.perf_counter()
-    ctx.graph_launch(g)
-    e1 = ctx.record()
-    ctx.sync()
-    graph_wall = time.perf_counter() - t0
-    graph_step_ms = ctx.elapsed_ms(e0, e1) / args.canopy_steps
-    ctx.destroy_event(e0)
-    ctx.destroy_event(e1)
-    ctx.graph_destroy(g)
-    worst, _ = ctx.soil_balance_check_dev(ds)
-    err = np.empty(ds.dims.ncol)
-    ctx.d2h(err, ds.sf["errsoi_col"])
-    liq = np.empty((ds.dims.nlevsno + ds.dims.nlevgrnd, ds.dims.ncol))
-    ctx.d2h(liq, ds.col["h2osoi_liq"])
-    # (the water-table kernel is not re-run here: RenewCondensation applied
-    # twice would take the step's sublimation twice)
-    sw_ms, sf_ms = time_events(ctx, [lambda: ctx.surface_water_dev(ds, hydro), lambda: ctx.soilfluxes_dev(ds)], 3,
-                               prefill=lambda: ctx.canopyfluxes_dev(ds, structs=structs), inner=10)
-    b = surface_water_bytes(ds)
-    out = {"history": _history_and_checkpoint(ctx, ds, structs)}
-    return {**out, "column_steps_per_s": ds.dims.ncol * args.canopy_steps / wall, "step_ms": step_ms,
-            "graph_column_steps_per_s": ds.dims.ncol * args.canopy_steps / graph_wall,
-            "graph_step_ms": graph_step_ms,
-            "max_abs_errsoi_Wm2": worst,
-            "columns_errsoi_over_1e-5_Wm2": int(np.sum(~(np.abs(err) <= 1e-5))),
-            "columns_negative_soil_liquid": int(np.sum((liq[ds.dims.nlevsno:] < 0).any(axis=0))),
-            "surface_water_kernel_ms": sw_ms, "soilfluxes_kernels_ms": sf_ms,
-            "surface_water_roofline": {"kernel": "k_surface_water", "alg_bytes_per_launch": b,
-                                       "achieved_GBps": b / (sw_ms * 1e-3) / 1e9,
-                                       "frac": b / (sw_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
 
 
 def _phs_soil_side(ctx, ncol):
@@ -412,44 +244,249 @@ def _history_and_checkpoint(ctx, ds, structs):
     return out
 
 
-def _full_column_parts(ctx, args, st):
-    """The same batch as concurrent sub-batches: contiguous parts of the
-    binned order (balanced, as the bins are dealt over the parts), each
-    resident in its own context and stream and stepped without waiting for
-    the others, so one part's long canopy rounds overlap another's work."""
+def extras(ctx, args):
+    """The PHS kernels, the history tape's per-step update and a checkpoint
+    round trip on a 262,144-column full-column state."""
     from ctsm_amd import synthetic
-    from ctsm_amd.lib import Context
     from ctsm_amd.state import DeviceState
-    P = args.canopy_parts
-    n = st.dims.ncol
-    bounds = [n * k // P for k in range(P + 1)]
-    tun = dict(ctx.get_tuning(), step_streams=1)
-    ctxs = [ctx] + [Context() for _ in range(P - 1)]
-    before = ctx.get_tuning()
-    for c in ctxs:
-        c.set_canopy_params(synthetic.canopy_params(), synthetic.pft_table())
-        c.set_tuning(**tun)
-    dss = [DeviceState(st.subset(np.arange(bounds[k], bounds[k + 1])), c) for k, c in enumerate(ctxs)]
-    sts = [(*d.structs(), d.canopy_struct()) for d in dss]
-    for c, d, s in zip(ctxs, dss, sts):
-        c.land_step_dev(d, structs=s)
-    for c in ctxs:
-        c.sync()
+    st = synthetic.canopy_workload("canopy", ncol=262144)
+    ctx.set_canopy_params(synthetic.canopy_params(), synthetic.pft_table())
+    ds = DeviceState(st, ctx)
+    structs = (*ds.structs(), ds.canopy_struct())
+    ctx.land_step_dev(ds, structs=structs)
+    out = {"history": _history_and_checkpoint(ctx, ds, structs)}
+    ds.free()
+    out["phs_soil_side"] = _phs_soil_side(ctx, st.dims.ncol)
+    return out
+
+
+def _const_fields(struct):
+    """Names of the `const` (read-only) members of a struct in ctsm_gpu.h."""
+    import re
+    with open(os.path.join(ROOT, "include", "ctsm_gpu.h")) as f:
+        s = f.read()
+    body = s[s.index("typedef struct %s {" % struct):s.index("} %s;" % struct)]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    names = set()
+    for line in body.splitlines():
+        line = line.strip()
+        if line.startswith("const"):
+            for nm in line.split(None, 2)[2].rstrip(";").split(","):
+                names.add(nm.strip().lstrip("*").strip())
+    return names
+
+
+def surface_water_bytes(st):
+    """Algorithmic bytes of one surface water launch: per column 22 FP64 inputs
+    (14 of the chain, frac_sno_eff, frac_h2osfc, icefrac and hksat of layers
+    1-3), snl, and 16 FP64 outputs (h2osfc included)."""
+    return st.dims.ncol * (22 * 8 + 4 + 16 * 8)
+
+
+def land_step_bytes(st, ncol, npatch):
+    """Algorithmic HBM bytes of one full land step per routine: every field a
+    routine reads counted once and every field it writes counted once (FP64 8
+    B, int32 4 B) over active extents.  The soil solves, CanopyFluxes and
+    the surface water chain use the per-kernel counts above; SoilFluxes
+    counts its patch members (const = read, others read and written) and its
+    column members; the sink reads rootr per patch and writes two profiles
+    per column; the water tables and the drainage read their column members
+    and liq, ice, dz, z, watsat (and bsw, sucsat, hksat, eff_porosity for the
+    drainage) over the soil layers."""
+    from ctsm_amd import abi
+    NG, NS = st.dims.nlevgrnd, st.dims.nlevsno
+    sfc = _const_fields("ctsm_soilflux_state")
+    sf_p = sum((8 if t == "f64" else 4) * (1 if n in sfc else 2) for n, t, k in abi.SOILFLUX_FIELDS if k == "p")
+    S = -st.col["snl"].astype(np.int64)
+    out = {"canopy": canopy_bytes(st), "soiltemp": soiltemp_bytes(st), "richards": richards_bytes(st),
+           "surface_water": surface_water_bytes(st),
+           "soilfluxes": int(npatch * (sf_p + 8 * 2) + (ncol * (8 * 12) + int((S + 1).sum()) * 8 * 3)),
+           "root_sink": int(npatch * (NG * 8 + 8 + 4 + 4) + ncol * (NG * 8 * 2 + 8)),
+           "water_table": int(ncol * (7 * 8 + 20 * 8 * 6)),
+           "drainage": int(ncol * (15 * 8 + 20 * 8 * 10))}
+    return out
+
+
+def land_cpu_baseline(name, ncol, seconds=15.0, threads=None, per=128):
+    """The C oracle's full land step (the same routines as the GPU step) on
+    a seeded random sample of the workload's columns, one chunk of `per`
+    columns per thread, each thread stepping its chunk until `seconds` pass
+    (the oracle runs outside the interpreter lock)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from ctsm_amd import synthetic
+    from oracle import oracle as O
+    threads = threads or min(16, os.cpu_count() or 1)
+    pick = np.random.default_rng(12345).choice(ncol, size=min(ncol, per * threads), replace=False)
+    cfg, prm = O.canopy_config(), synthetic.canopy_params()
+    pft = O.PftStruct(synthetic.pft_table())
+    hydro = synthetic.hydro_params()
+    subs = [synthetic.canopy_workload(name, gids=np.sort(pick[i * per:(i + 1) * per])) for i in range(threads)]
+
+    def work(sub):
+        n = 0
+        t = time.perf_counter()
+        while True:
+            rc, _ = O.land_step_full(sub, cfg, prm, pft, hydro=hydro, water_table_after=True, drainage_after=True)
+            if rc:
+                raise RuntimeError(f"oracle land step failed ({rc})")
+            n += 1
+            if time.perf_counter() - t >= seconds or n >= 48 * 40:
+                return n
+
     t0 = time.perf_counter()
-    for c, d, s in zip(ctxs, dss, sts):
-        for _ in range(args.canopy_steps):
-            c.land_step_dev(d, structs=s)
-    for c in ctxs:
-        c.sync()
+    with ThreadPoolExecutor(threads) as ex:
+        steps = list(ex.map(work, subs))
+    el = time.perf_counter() - t0
+    total = per * sum(steps)
+    return total / el, threads, (f"{threads} threads x {per} columns (a seeded random sample of the "
+                                 f"{ncol:,}-column workload, {sum(s.dims.npatch for s in subs):,} patches in all) x "
+                                 f"{min(steps)}-{max(steps)} full land steps ({el:.1f} s)")
+
+
+def headline(ctx, args):
+    """The full land step on the config-[3] workload: generation in binned
+    order, autotuning, the timed graph replay, per-kernel times of the
+    dominant routines, the plain form over the same steps."""
+    from ctsm_amd import autotune as AT, synthetic
+    from ctsm_amd.state import DeviceState
+    t_gen = time.perf_counter()
+    ids = synthetic.binned_ids(args.workload, args.ncol, parts=1)
+    st = synthetic.canopy_workload(args.workload, gids=ids)
+    t_gen = time.perf_counter() - t_gen
+    ncol, npatch = st.dims.ncol, st.dims.npatch
+    nsno = -st.col["snl"]
+    ppc = st.col["patchf"] - st.col["patchi"] + 1
+    bytes_by = land_step_bytes(st, ncol, npatch)
+    ctx.set_canopy_params(synthetic.canopy_params(), synthetic.pft_table())
+    hydro = synthetic.hydro_params()
+    ds = DeviceState(st, ctx)
+    del st
+    full = (*ds.structs(), ds.canopy_struct(), ds.soilflux_struct())
+    step = lambda: ctx.land_step_full_dev(ds, structs=full, hydro=hydro, water_table=True, drainage=True)
+    for _ in range(args.warmup):
+        step()
+    ctx.sync()
+    res = {"generate_s": round(t_gen, 1), "patches": npatch,
+           "mean_snow_layers": float(nsno.mean()), "mean_patches_per_column": float(ppc.mean())}
+    if not args.no_autotune:
+        split, _, split_table = AT.autotune_canopy(ctx, ds)
+        best, _, table = AT.autotune_land(ctx, ds, hydro, steps=2, reps=2)
+        res["autotune_canopy_rounds_ms"] = {f"{a}/{c}": round(ms, 3) for (a, c), ms in split_table}
+        res["autotune_land_ms_per_step"] = [[c, round(ms, 3)] for c, ms in table]
+    res["tuning"] = ctx.get_tuning()
+    snap = AT.snapshot(ctx, ds)
+    # the timed steps, captured once as one graph; the first launch (which
+    # uploads the graph) is untimed and the state is restored after it
+    graph = ctx.graph_capture_full(ds, nsteps=args.steps, hydro=hydro, water_table=True, drainage=True)
+    ctx.graph_launch(graph)
+    ctx.sync()
+    AT.restore(ctx, ds, snap)
+    ctx.sync()
+    e0 = ctx.record()
+    t0 = time.perf_counter()
+    ctx.graph_launch(graph)
+    e1 = ctx.record()
+    ctx.sync()
     wall = time.perf_counter() - t0
-    for d in dss:
-        d.free()
-    for c in ctxs[1:]:
-        c.close()
+    res["graph_ms_per_step_events"] = ctx.elapsed_ms(e0, e1) / args.steps
+    ctx.destroy_event(e0)
+    ctx.destroy_event(e1)
+    ctx.graph_destroy(graph)
+    worst, worst_col = ctx.soil_balance_check_dev(ds)
+    nit = np.empty(npatch)
+    ctx.d2h(nit, ds.can["num_iter"])
+    res["after_steps"] = {"max_abs_errsoi_Wm2": worst, "mean_canopy_iterations_last_step": float(nit.mean()),
+                          "max_canopy_iterations_last_step": float(nit.max())}
+    # launch times of the step's routines, from the state after the timed day
+    # (events on the library stream; each routine launched alone)
+    AT.restore(ctx, ds, snap)
+    for _ in range(2):
+        step()
+    structs = (*ds.structs(), ds.canopy_struct())
+    k = {}
+    k["canopy"], = time_events(ctx, [lambda: ctx.canopyfluxes_dev(ds, structs=structs)], 3)
+    k["soiltemp"], k["richards"] = time_events(ctx, [lambda: ctx.soiltemperature_dev(ds),
+                                                     lambda: ctx.soilwater_dev(ds)], 3)
+    res["kernel_ms"] = {n: round(v, 4) for n, v in k.items()}
+    res["alg_bytes_per_step"] = bytes_by
+    # the plain form over the same steps from the same state
+    if not args.no_plain:
+        before = ctx.get_tuning()
+        ctx.set_tuning(plain_routines=1)
+        AT.restore(ctx, ds, snap)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        ctx.sync()
+        wall_p = time.perf_counter() - t0
+        ctx.set_tuning(**before)
+        res["plain_ms_per_step"] = wall_p / args.steps * 1e3
+        res["plain_column_steps_per_s"] = ncol * args.steps / wall_p
+        res["batched_over_plain"] = (ncol * args.steps / wall) / res["plain_column_steps_per_s"]
+    AT.free_snapshot(ctx, snap)
+    ds.free()
+    return wall, res, bytes_by, ncol, npatch
+
+
+def soil_uneven(ctx, args):
+    """The soil step alone (SoilTemperature with SetSoilWaterFractions, then
+    SoilWater) on the uneven batch of 65,536 columns, binned by the sub-steps
+    of warm-up steps, autotuned (lanes, blocks, 4 streams), 48 steps captured
+    as one graph; and its plain form."""
+    from ctsm_amd import autotune as AT, synthetic
+    from ctsm_amd.state import DeviceState
+    n = args.soil_ncol
+    st = synthetic.workload("uneven", ncol=n)
+    ds = DeviceState(st, ctx)
+    for _ in range(max(args.warmup, 1)):
+        ctx.step_dev(ds)
+    ctx.sync()
+    stw = ds.to_host()
+    ds.free()
+    order = AT.column_order(stw.col["snl"], stw.col["nbedrock"], stw.col["num_substeps"], parts=4) - 1
+    st = stw.subset(order)
+    before = ctx.get_tuning()
+    ds = DeviceState(st, ctx)
+    tuning, _, table = AT.autotune(ctx, ds, steps=8, reps=2)
+    structs = ds.structs()
+    snap = AT.snapshot(ctx, ds)
+    graph = ctx.graph_capture(ds, nsteps=args.steps, canopy=False, structs=(*structs, None))
+    ctx.graph_launch(graph)
+    ctx.sync()
+    AT.restore(ctx, ds, snap)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.graph_launch(graph)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    ctx.graph_destroy(graph)
+    out = {"workload": f"{n:,} columns, snow 0-12 layers, bedrock 5-20, one patch each; SoilTemperature + "
+                       f"SetSoilWaterFractions + SoilWater, {args.steps} steps captured as one graph",
+           "column_steps_per_s": n * args.steps / wall, "ms_per_step": wall / args.steps * 1e3,
+           "tuning": ctx.get_tuning(), "autotune_ms_per_step": [[c, round(ms, 4)] for c, ms in table]}
+    # per-kernel launch times (full-batch launches) and their rooflines
+    AT.restore(ctx, ds, snap)
+    t_heat, t_rich = time_events(ctx, [lambda: ctx.soiltemperature_dev(ds), lambda: ctx.soilwater_dev(ds)], 8)
+    lanes = ctx.get_tuning()["richards_lanes"]
+    for nm, ms, b, kn in (("richards", t_rich, richards_bytes(st), "k_soilwater_g<%d>" % lanes if lanes > 1
+                           else "k_soilwater"), ("soiltemp", t_heat, soiltemp_bytes(st), "k_soiltemp")):
+        out[nm] = {"kernel": kn, "ms": ms, "alg_bytes_per_launch": b, "achieved_GBps": b / (ms * 1e-3) / 1e9,
+                   "frac": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    if not args.no_plain:
+        AT.restore(ctx, ds, snap)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.step_dev(ds, plain=True, structs=structs)
+        ctx.sync()
+        wp = time.perf_counter() - t0
+        out["plain_column_steps_per_s"] = n * args.steps / wp
+        out["batched_over_plain"] = out["column_steps_per_s"] / out["plain_column_steps_per_s"]
+    AT.free_snapshot(ctx, snap)
+    ds.free()
     ctx.set_tuning(**before)
-    return {"parts": P, "column_steps_per_s": n * args.canopy_steps / wall,
-            "patch_steps_per_s": st.dims.npatch * args.canopy_steps / wall,
-            "step_ms": wall / args.canopy_steps * 1e3}
+    return out
 
 
 def main():
@@ -457,28 +494,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=48)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--ncol", type=int, default=65536)
+    ap.add_argument("--workload", choices=["canopy_uneven", "canopy"], default="canopy_uneven")
+    ap.add_argument("--ncol", type=int, default=1048576)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-sort", action="store_true", help="keep generation order (no binning)")
-    ap.add_argument("--filter-sort", action="store_true",
-                    help="bin through a sorted column filter instead of storing the batch in binned order")
-    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
-    ap.add_argument("--streams", type=int, default=0,
-                    help="with --lanes: streams the step's columns are split across (default 4)")
-    ap.add_argument("--no-graph", action="store_true", help="launch each step eagerly instead of replaying "
-                                                            "the captured step")
-    ap.add_argument("--lanes", type=int, default=0, help="force lanes per column in the Richards solve")
-    ap.add_argument("--single-kernel", action="store_true", help="force the one-kernel fused step")
-    ap.add_argument("--order", choices=["bins", "substeps"], default="substeps",
-                    help="column order: (bedrock, snow) bins, or also by the sub-steps of the warm-up step")
-    ap.add_argument("--canopy-ncol", type=int, default=262144,
-                    help="columns of the full-column (CanopyFluxes) measurement; 0 skips it")
-    ap.add_argument("--canopy-steps", type=int, default=6)
-    ap.add_argument("--canopy-parts", type=int, default=1,
-                    help="also run the full-column batch as this many concurrent sub-batches (1 = no)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--soil-ncol", type=int, default=65536,
+                    help="columns of the soil-only uneven measurement (0 skips it)")
+    ap.add_argument("--extras", action="store_true",
+                    help="also time the PHS kernels, the history tape and a checkpoint of a 262,144-column state")
     ap.add_argument("--serve-requests", type=int, default=80,
                     help="served-ensemble measurement: number of requests (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate (requests/s)")
@@ -494,108 +520,31 @@ def main():
                                    "(columns shard trivially by range, see DESIGN.md)"}))
         sys.exit(2)
 
-    from ctsm_amd import lib, synthetic
-    from ctsm_amd.state import DeviceState
-
-    st = synthetic.workload("uneven", ncol=args.ncol)
+    from ctsm_amd import lib
     ctx = lib.Context(device=0)
-
-    from ctsm_amd import autotune as AT
-
-    # Column binning.  Columns are independent, so the batch may be stored in
-    # any order: the device-resident batch is stored grouped by (snow layers,
-    # Richards sub-steps of the warm-up steps, bedrock depth), which keeps a
-    # wavefront's columns on similar work AND its memory accesses coalesced
-    # (a sorted index filter gives the first but not the second).
-    filt = None
-    binning = "none"
-    if not args.no_sort:
-        ds = DeviceState(st, ctx)
-        for _ in range(max(args.warmup, 1)):
-            ctx.step_dev(ds)
-        ctx.sync()
-        stw = ds.to_host()
-        ds.free()
-        order = AT.column_order(stw.col["snl"], stw.col["nbedrock"],
-                                stw.col["num_substeps"] if args.order == "substeps" else None, parts=4) - 1
-        if args.filter_sort:
-            filt = ctx.device_array(order + 1)
-            st = stw
-            binning = "sorted index filter"
-        else:
-            st = stw.subset(order)
-            binning = "batch stored in binned column order"
-
-    tuning, table = None, []
-    if args.lanes or args.single_kernel:
-        lanes = args.lanes or 1
-        ctx.set_tuning(richards_lanes=lanes, richards_block=64 if lanes == 1 else 128,
-                       fused_single_kernel=int(args.single_kernel),
-                       step_streams=args.streams or (1 if args.single_kernel else 4))
-    elif not args.no_autotune:
-        ds0 = DeviceState(st, ctx)
-        tuning, _, table = AT.autotune(ctx, ds0, filter_dev=filt, steps=8, reps=2)
-        ds0.free()
-
-    def run(plain):
-        nonlocal filt
-        ds = DeviceState(st, ctx)
-        structs = ds.structs()
-        for _ in range(args.warmup):
-            ctx.step_dev(ds, filter_dev=filt, plain=plain, structs=structs)
-        ctx.sync()
-        # the timed steps are captured once as one graph (each part of the
-        # split step runs all steps on its own stream) and launched once
-        graph = None
-        if not plain and not args.no_graph:
-            graph = ctx.graph_capture(ds, nsteps=args.steps, canopy=False, filterc_dev=filt,
-                                      structs=(*structs, None))
-        # events on the library's own stream, around the launch(es)
-        starts, stops = [], []
-        t0 = time.perf_counter()
-        if graph is not None:
-            starts.append(ctx.record())
-            ctx.graph_launch(graph)
-            stops.append(ctx.record())
-        else:
-            for k in range(args.steps):
-                starts.append(ctx.record())
-                ctx.step_dev(ds, filter_dev=filt, plain=plain, structs=structs)
-                stops.append(ctx.record())
-        ctx.sync()
-        wall = time.perf_counter() - t0
-        per = [ctx.elapsed_ms(a, b) / (args.steps if graph is not None else 1) for a, b in zip(starts, stops)]
-        for e in starts + stops:
-            ctx.destroy_event(e)
-        if graph is not None:
-            ctx.graph_destroy(graph)
-        return wall, float(np.mean(per)), ds
-
-    wall, step_ms, ds = run(plain=False)
-    ds.free()
-    value = args.ncol * args.steps / wall
-    res = {"fused_step_ms": step_ms, "tuning": ctx.get_tuning()}
-    if tuning is not None:
-        res["autotune_ms_per_step"] = [[c, round(ms, 4)] for c, ms in table]
-    if not args.no_plain:
-        wall_p, step_ms_p, dsp = run(plain=True)
-        dsp.free()
-        res["plain_step_ms"] = step_ms_p
-        res["plain_column_steps_per_s"] = args.ncol * args.steps / wall_p
-        res["fused_over_plain"] = value / res["plain_column_steps_per_s"]
-
-    # per-kernel launch times over the same trajectory as the timed steps
-    # (events on the library stream): heat solve, water fractions, Richards --
-    # the step's work launched as separate kernels, averaged over the steps
-    ds = DeviceState(st, ctx)
-    for _ in range(args.warmup):
-        ctx.step_dev(ds, filter_dev=filt)
-    t_heat, t_frac, t_rich = time_events(ctx, [lambda: ctx.soiltemperature_dev(ds, filt),
-                                               lambda: ctx.soilwater_fractions_dev(ds, filt),
-                                               lambda: ctx.soilwater_dev(ds, filt)], args.steps)
-    ds.free()
-    res["kernel_ms"] = {"soiltemp (k_soiltemp)": t_heat, "fractions (k_soilwater_fractions)": t_frac,
-                        "richards (k_soilwater_g)": t_rich}
+    print(f"[bench] {args.workload} x {args.ncol:,} columns, {args.steps} steps", file=sys.stderr, flush=True)
+    wall, res, bytes_by, ncol, npatch = headline(ctx, args)
+    value = ncol * args.steps / wall
+    ms_step = wall / args.steps * 1e3
+    kms = res["kernel_ms"]["canopy"]
+    kb = bytes_by["canopy"]
+    achieved = kb / (kms * 1e-3) / 1e9
+    step_bytes = sum(bytes_by.values())
+    res["whole_step_roofline"] = {"alg_bytes_per_step": step_bytes,
+                                  "achieved_GBps": step_bytes / (ms_step * 1e-3) / 1e9,
+                                  "frac": step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    for nm in ("soiltemp", "richards"):
+        ms = res["kernel_ms"][nm]
+        res.setdefault("roofline_other", {})[nm] = {
+            "kernel": "k_soiltemp" if nm == "soiltemp" else "k_soilwater_g",
+            "ms": ms, "alg_bytes_per_launch": bytes_by[nm], "achieved_GBps": bytes_by[nm] / (ms * 1e-3) / 1e9,
+            "frac": bytes_by[nm] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    print(f"[bench] headline {value / 1e6:.2f} M column-steps/s ({ms_step:.2f} ms/step)", file=sys.stderr, flush=True)
+    if args.soil_ncol > 0:
+        res["soil_only_uneven"] = soil_uneven(ctx, args)
+        print("[bench] soil-only batch done", file=sys.stderr, flush=True)
+    if args.extras:
+        res["extras"] = extras(ctx, args)
     if args.calib_copy_mb > 0:
         nb = args.calib_copy_mb << 20
         a, b = ctx.alloc(nb), ctx.alloc(nb)
@@ -608,20 +557,12 @@ def main():
         res["copy_calib"] = {"bytes_read": nb, "bytes_written": nb, "ms": ms, "GBps": 2 * nb / ms / 1e6}
         ctx.free(a)
         ctx.free(b)
-
-    b_rich, b_heat = richards_bytes(st), soiltemp_bytes(st)
-    lanes = ctx.get_tuning()["richards_lanes"]
-    kern = {"richards": (t_rich, b_rich, "k_soilwater_g<%d>" % lanes if lanes > 1 else "k_soilwater"),
-            "soiltemp": (t_heat, b_heat, "k_soiltemp")}
-    dom = max(kern, key=lambda k: kern[k][0])
-    kms, kbytes, kname = kern[dom]
-    achieved = kbytes / (kms * 1e-3) / 1e9
-    res["roofline_other"] = {k: {"kernel": v[2], "ms": v[0], "alg_bytes_per_launch": v[1],
-                                 "achieved_GBps": v[1] / (v[0] * 1e-3) / 1e9} for k, v in kern.items() if k != dom}
-    if args.canopy_ncol > 0:
-        res["full_column_step"] = full_column(ctx, args)
     if args.serve_requests > 0:
         res["served_ensemble"] = served_ensemble(ctx, args)
+        print("[bench] served ensemble done", file=sys.stderr, flush=True)
+    kname = "k_canopy_round"
+    prof = PROFILED.get((kname, args.workload, ncol)) if not args.no_autotune else None
+    lanes = res["tuning"]["richards_lanes"]
     out = {
         "metric": "column-steps/sec (FP64, 1 GPU)",
         "value": value,
@@ -629,25 +570,31 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": ms_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded generator, inputs resident in HBM)",
-        "config": {"workload": "uneven columns: 65,536 soil columns, snow 0-12 layers, bedrock 5-20, "
-                               "SoilTemperature + SoilWater, dtime 1800 s, 48-step day",
-                   "ncol": args.ncol, "binning": binning,
-                   "launch": "eager, one call per step" if args.no_graph else
-                             "the timed steps captured as one graph, launched once"},
+        "config": {"workload": f"{args.workload}: {ncol:,} columns, {npatch:,} patches (1-16 per column), "
+                               + ("snow 0-12 layers, bedrock 5-20, " if args.workload == "canopy_uneven" else
+                                  "no snow, 20 soil layers, ")
+                               + f"full land step (CanopyFluxes, SoilTemperature, SoilFluxes, sink, surface water, "
+                                 f"SoilWater, water tables, drainage), dtime 1800 s, {args.steps} steps",
+                   "ncol": ncol, "npatch": npatch, "steps": args.steps,
+                   "binning": "generated in (snow layers, bedrock) order",
+                   "launch": f"the {args.steps} timed steps captured as one graph, launched once",
+                   "tuning": "autotuned" if not args.no_autotune else "default",
+                   "richards_lanes": lanes},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get(kname),
-                     "kernel": kname, "launch_ms": kms, "alg_bytes_per_launch": kbytes,
-                     "fp64_valu_frac": FP64_VALU_FRAC.get(kname)},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": prof["traffic"] if prof else None,
+                     "kernel": kname + " (all rounds of one CanopyFluxes call)", "launch_ms": kms,
+                     "alg_bytes_per_launch": kb,
+                     "fp64_valu_frac": prof["fp64_valu_frac"] if prof else None},
         "detail": res,
     }
     if not args.no_cpu:
-        v, threads, sample = cpu_baseline(st, args.cpu_seconds, args.cpu_threads or None)
+        v, threads, sample = land_cpu_baseline(args.workload, ncol, args.cpu_seconds, args.cpu_threads or None)
         out["cpu_baseline"] = {"value": v, "unit": "column-steps/s", "cores": threads, "kind": "port",
                                "sample": sample}
     print(json.dumps(out), flush=True)

@@ -143,3 +143,48 @@ def autotune_canopy(ctx, ds, filter_dev=None, n=None, reps=2, candidates=None, d
     free_snapshot(ctx, snap)
     ctx.set_tuning(canopy_first_iterations=best[0], canopy_round_iterations=best[1])
     return best, best_ms, table
+
+
+# Launch shapes of the soil solves inside a full land step (one stream, so
+# step_streams does not apply): lanes per column and block sizes.
+LAND_CANDIDATES = [
+    dict(richards_lanes=8, richards_block=128, soiltemp_block=128),
+    dict(richards_lanes=8, richards_block=256, soiltemp_block=64),
+    dict(richards_lanes=8, richards_block=64, soiltemp_block=64),
+    dict(richards_lanes=4, richards_block=128, soiltemp_block=128),
+    dict(richards_lanes=4, richards_block=256, soiltemp_block=64),
+    dict(richards_lanes=2, richards_block=128, soiltemp_block=128),
+    dict(richards_lanes=1, richards_block=64, soiltemp_block=64),
+]
+
+
+def autotune_land(ctx, ds, hydro, steps=2, reps=2, candidates=None, dtime=1800.0, verbose=False):
+    """Pick the soil launch shape that runs `steps` full land steps (surface
+    water, water tables and drainage included) fastest on this state, best of
+    `reps` runs from the same snapshot; the state is left unchanged."""
+    candidates = candidates or LAND_CANDIDATES
+    snap = snapshot(ctx, ds)
+    full = (*ds.structs(), ds.canopy_struct(), ds.soilflux_struct())
+    best, best_ms, table = None, float("inf"), []
+    for cand in candidates:
+        ctx.set_tuning(**cand)
+        ms = float("inf")
+        for _ in range(reps):
+            restore(ctx, ds, snap)
+            e0 = ctx.record()
+            for _ in range(steps):
+                ctx.land_step_full_dev(ds, structs=full, hydro=hydro, water_table=True, drainage=True, dtime=dtime)
+            e1 = ctx.record()
+            ctx.sync()
+            ms = min(ms, ctx.elapsed_ms(e0, e1) / steps)
+            ctx.destroy_event(e0)
+            ctx.destroy_event(e1)
+        table.append((dict(cand), ms))
+        if verbose:
+            print(f"autotune land {cand} -> {ms:.3f} ms/step", flush=True)
+        if ms < best_ms:
+            best, best_ms = cand, ms
+    restore(ctx, ds, snap)
+    free_snapshot(ctx, snap)
+    ctx.set_tuning(**best)
+    return best, best_ms, table

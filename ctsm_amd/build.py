@@ -24,16 +24,40 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose=False, extra=()):
-    srcs = [os.path.join(HERE, "csrc", s) for s in SRC]
-    cmd = [hipcc()] + FLAGS + list(extra) + ["-I", os.path.join(ROOT, "include"),
-                                             "-I", os.path.join(HERE, "csrc")] + srcs + ["-o", LIB + ".tmp"]
+OBJDIR = os.path.join(ROOT, "_build")
+
+
+def build(verbose=False, extra=(), jobs=None):
+    """Compiles every source to an object in _build/ (in parallel: each file
+    is its own translation unit with its own code object), then links the
+    shared library.  Always recompiles."""
+    from concurrent.futures import ThreadPoolExecutor
+    os.makedirs(OBJDIR, exist_ok=True)
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc")]
+    cflags = [f for f in FLAGS if f != "-shared"] + list(extra)
+    objs = [os.path.join(OBJDIR, os.path.splitext(s)[0] + ".o") for s in SRC]
+
+    def compile_one(i):
+        cmd = [hipcc()] + cflags + inc + ["-c", os.path.join(HERE, "csrc", SRC[i]), "-o", objs[i]]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        return subprocess.run(cmd, capture_output=True, text=True)
+
+    jobs = jobs or min(len(SRC), max(1, min(8, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(jobs) as ex:
+        results = list(ex.map(compile_one, range(len(SRC))))
+    bad = [r for r in results if r.returncode != 0]
+    if bad:
+        for r in bad:
+            sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed")
+    cmd = [hipcc()] + FLAGS + list(extra) + objs + ["-o", LIB + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed")
+        raise RuntimeError("hipcc link failed")
     os.replace(LIB + ".tmp", LIB)
     return LIB
 

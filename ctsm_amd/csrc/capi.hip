@@ -364,7 +364,7 @@ struct ctsm_ctx {
     int device = 0;
     ctsm_config cfg{};
     Params prm{};
-    ctsm_tuning tun{4, 128, 128, 0, 6, 3, 4};
+    ctsm_tuning tun{4, 128, 128, 0, 6, 3, 4, 0};
     static constexpr int kMaxStreams = 4;
     hipStream_t side[kMaxStreams - 1] = {nullptr, nullptr, nullptr};   // concurrent parts of a step
     hipEvent_t ev_fork = nullptr, ev_join[kMaxStreams - 1] = {nullptr, nullptr, nullptr};
@@ -709,6 +709,7 @@ int ctsm_set_tuning(ctsm_ctx *ctx, const ctsm_tuning *t)
         t->canopy_round_iterations > 1001)
         return CTSM_E_ARG;
     if (t->step_streams < 1 || t->step_streams > ctsm_ctx::kMaxStreams) return CTSM_E_ARG;
+    if (t->plain_routines != 0 && t->plain_routines != 1) return CTSM_E_ARG;
     ctx->tun = *t;
     return CTSM_OK;
 }
@@ -806,6 +807,14 @@ static size_t step_work_doubles(const ctsm_ctx *ctx, int n)
     const int S = step_split(ctx, n);
     if (S == 1) return soiltemp_workspace_doubles(n);
     return (size_t)S * soiltemp_workspace_doubles((n + S - 1) / S);
+}
+// Workspace doubles of a full land step (one stream): the heat solve's LU
+// rows, or in the plain form the larger of the plain kernels' intermediates.
+static size_t land_work_doubles(const ctsm_ctx *ctx, int n)
+{
+    if (ctx->tun.plain_routines)
+        return std::max(soiltemp_plain_workspace_doubles(n), soilwater_plain_workspace_doubles(n));
+    return soiltemp_workspace_doubles(n);
 }
 static size_t step_partf_bytes(const ctsm_ctx *ctx, int n)
 {
@@ -936,7 +945,9 @@ static int check_canopy_layout(const ctsm_ctx *ctx, const ctsm_layout *L)
 static int run_canopy(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state &col, const ctsm_canopy_state &can,
                       int32_t n, const int32_t *filter_dev, double dtime, bool zero_cgrnd)
 {
-    const int k0 = ctx->tun.canopy_first_iterations, k = ctx->tun.canopy_round_iterations;
+    const bool one_pass = ctx->tun.plain_routines != 0;
+    const int k0 = one_pass ? 1001 : ctx->tun.canopy_first_iterations;
+    const int k = one_pass ? 1 : ctx->tun.canopy_round_iterations;
     const size_t wb = canopy_work_bytes(n, ctx->cfg.itmax_canopy_fluxes, k0, k);
     if (wb) {
         hipError_t he = ctx->cwork.ensure(wb);
@@ -1022,7 +1033,8 @@ static int capture_graph(ctsm_ctx *ctx, int32_t nc, int32_t np, bool canopy, cts
     if (he != hipSuccess) return hip_fail(ctx, he);
     ctsm_graph *gr = new ctsm_graph;
     gr->ctx = ctx;
-    he = gr->work.ensure(std::max<size_t>(step_work_doubles(ctx, nc), 1) * sizeof(double));
+    he = gr->work.ensure(std::max<size_t>(std::max(step_work_doubles(ctx, nc), land_work_doubles(ctx, nc)), 1) *
+                         sizeof(double));
     if (he == hipSuccess && step_partf_bytes(ctx, nc)) he = gr->partf.ensure(step_partf_bytes(ctx, nc));
     const size_t cwb = canopy ? canopy_work_bytes(np, ctx->cfg.itmax_canopy_fluxes, ctx->tun.canopy_first_iterations,
                                                   ctx->tun.canopy_round_iterations)
@@ -1631,12 +1643,19 @@ int ctsm_land_step_full_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_co
         sf->t_skin != can->t_skin || sf->active != can->active)
         return CTSM_E_ARG;
     int rc;
-    if ((rc = ensure_work(ctx, soiltemp_workspace_doubles(nc)))) return rc;
+    const bool plain = ctx->tun.plain_routines != 0;
+    if ((rc = ensure_work(ctx, land_work_doubles(ctx, nc)))) return rc;
     launch_save_before(ctx->stream, *lay, ctx->cfg, *col, *sf, nc, filterc_dev);
     if ((rc = run_canopy(ctx, lay, *col, *can, np, filterp_dev, dtime, true))) return rc;
     const View V = make_view(ctx, *lay);
-    launch_soiltemp_fractions(ctx->stream, V, ctx->prm, *col, *pch, nc, filterc_dev, dtime, (double *)ctx->work.p,
-                              nc, ctx->err, ctx->tun.soiltemp_block);
+    if (plain) {
+        launch_soiltemp_plain(ctx->stream, V, ctx->prm, *col, *pch, nc, filterc_dev, dtime, (double *)ctx->work.p,
+                              nc, ctx->err);
+        launch_soilwater_fractions(ctx->stream, V, *col, nc, filterc_dev);
+    } else {
+        launch_soiltemp_fractions(ctx->stream, V, ctx->prm, *col, *pch, nc, filterc_dev, dtime,
+                                  (double *)ctx->work.p, nc, ctx->err, ctx->tun.soiltemp_block);
+    }
     launch_soilfluxes(ctx->stream, *lay, ctx->cfg, *col, *pch, *sf, nc, filterc_dev, np, filterp_dev, dtime, ctx->err);
     launch_root_sink(ctx->stream, *lay, ctx->cfg, *col, *can, nc, filterc_dev, ctx->err);
     if (sw) {
@@ -1645,8 +1664,12 @@ int ctsm_land_step_full_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_co
                           sw->qflx_ev_h2osfc, nc, filterc_dev, ctx->err);
         launch_surface_water(ctx->stream, *lay, ctx->cfg.e_ice, *hprm, *col, *sw, nc, filterc_dev, dtime, ctx->err);
     }
-    launch_soilwater(ctx->stream, V, ctx->prm, *col, nc, filterc_dev, dtime, ctx->err, ctx->tun.richards_lanes,
-                     ctx->tun.richards_block);
+    if (plain)
+        launch_soilwater_plain(ctx->stream, V, ctx->prm, *col, nc, filterc_dev, dtime, (double *)ctx->work.p, nc,
+                               ctx->err);
+    else
+        launch_soilwater(ctx->stream, V, ctx->prm, *col, nc, filterc_dev, dtime, ctx->err, ctx->tun.richards_lanes,
+                         ctx->tun.richards_block);
     if (wt) {
         launch_patch2col3(ctx->stream, *lay, *col, can->active, can->wtcol, sf->qflx_liqdew_to_top_layer,
                           sf->qflx_soliddew_to_top_layer, sf->qflx_solidevap_from_top_layer,

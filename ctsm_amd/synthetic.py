@@ -122,15 +122,37 @@ def soil_params(sand, clay, om, zsoi, sf=None):
 S_SAND, S_CLAY, S_OM, S_LIQ, S_T, S_INFL, S_SINKTOT, S_SINKW, S_G, S_NSNO, S_SDZ, S_SRHO, S_ST, S_BED = range(14)
 
 
+def _gids(ncol, col0, gids):
+    """Global ids of the generated columns: col0 .. col0 + ncol - 1, or the
+    given ids (any order: a binned order, a sample) when gids is not None."""
+    if gids is None:
+        return np.arange(col0, col0 + ncol, dtype=np.int64)
+    gids = np.asarray(gids, dtype=np.int64)
+    assert gids.shape == (ncol,)
+    return gids
+
+
+def structure(n, seed, snow=True, bedrock=True, nlevsno=12, nlevsoi=20):
+    """(snow layers, bedrock index) of global columns 0 .. n-1, as soil_only
+    draws them -- enough to bin a batch before generating it."""
+    gid = np.arange(n, dtype=np.int64)
+    nbed = (5 + np.floor(uniform(seed, S_BED, gid) * 16).astype(np.int32)) if bedrock \
+        else np.full(n, nlevsoi, np.int32)
+    nsno = np.floor(uniform(seed, S_NSNO, gid) * (nlevsno + 1)).astype(np.int32) if snow \
+        else np.zeros(n, np.int32)
+    return np.minimum(nsno, nlevsno), np.minimum(nbed, nlevsoi)
+
+
 def soil_only(ncol, seed, snow=False, bedrock=False, col0=0, nlevsno=12, nlevsoi=20, nlevgrnd=25,
-              frac_h2osfc=None):
+              frac_h2osfc=None, gids=None):
     """Columns of the soil-only workloads (one patch per column).
 
-    col0 is the global id of the first column (for regenerating a slice)."""
+    col0 is the global id of the first column (for regenerating a slice);
+    gids, when given, lists the global ids of the columns to generate."""
     d = Dims(ncol, ncol, nlevsno, nlevsoi, nlevgrnd)
     st = HostState(d)
     c, p = st.col, st.pch
-    gid = np.arange(col0, col0 + ncol, dtype=np.int64)
+    gid = _gids(ncol, col0, gids)
     dzs, zis, zs = vertical_grid(nlevsoi, nlevgrnd)
     NS = nlevsno
 
@@ -365,14 +387,14 @@ def _qsat_np(T, p):
     return 0.622 * es / (p - 0.378 * es), es
 
 
-def with_canopy(soil, seed, pmax=16, pmin=1, col0=0):
+def with_canopy(soil, seed, pmax=16, pmin=1, col0=0, gids=None):
     """Expand a soil-only state to 1..pmax vegetated patches per column with the
     canopy forcing and state of the full-column workloads (one gridcell per
     column).  Patch draws are keyed by global patch id col*pmax + k so any
     column regenerates alone."""
     d0 = soil.dims
     ncol = d0.ncol
-    gid = np.arange(col0, col0 + ncol, dtype=np.int64)
+    gid = _gids(ncol, col0, gids)
     npc = pmin + np.floor(uniform(seed, S_NPATCH, gid) * (pmax - pmin + 1)).astype(np.int32)
     npc = np.minimum(npc, pmax)
     npatch = int(npc.sum())
@@ -515,20 +537,20 @@ def with_canopy(soil, seed, pmax=16, pmin=1, col0=0):
         st.sf[k][...] = 0.0
     st.sf["t_ssbef"][...] = c["t_soisno"]
     st.sf["t_h2osfc_bef"][:] = c["t_h2osfc"]
-    with_hydrology(st, seed, col0=col0)
+    with_hydrology(st, seed, col0=col0, gids=gids)
     return st
 
 
 S_RAIN, S_ZWT, S_ZWTP, S_FROST, S_WTF, S_SLOPE, S_THRESH, S_EVAP, S_DEW = 70, 71, 72, 73, 74, 75, 76, 77, 78
 
 
-def with_hydrology(st, seed, col0=0):
+def with_hydrology(st, seed, col0=0, gids=None):
     """Inputs of the surface water input chain (rain and snow melt, water
     tables, frost table, slope, micro-topography threshold, column
     evaporation fluxes), keyed by global column id; outputs cleared.  The
     column's frac_h2osfc / h2osfc are left as they are."""
     c, h = st.col, st.hy
-    cols = np.arange(st.dims.ncol, dtype=np.int64) + col0
+    cols = _gids(st.dims.ncol, col0, gids)
     for k in h:
         h[k][...] = 0.0
     h["qflx_rain_plus_snomelt"][:] = 5e-4 * uniform(seed, S_RAIN, cols)
@@ -553,13 +575,33 @@ def with_hydrology(st, seed, col0=0):
     return st
 
 
-def canopy_workload(name, ncol=None, seed=None):
+CANOPY_SEEDS = {"canopy": 0xC75D0002, "canopy_uneven": 0xC75D0003}
+
+
+def canopy_workload(name, ncol=None, seed=None, gids=None):
     """'canopy' (no snow, 20 layers, 1-16 patches) and 'canopy_uneven' (snow and
-    bedrock mixes of the uneven workload, 1-16 patches)."""
+    bedrock mixes of the uneven workload, 1-16 patches).  gids: generate these
+    global columns (in this order) instead of 0 .. ncol-1."""
+    if gids is not None:
+        ncol = len(gids)
     if name == "canopy":
-        s = 0xC75D0002 if seed is None else seed
-        return with_canopy(soil_only(ncol or 262144, s), s)
+        s = CANOPY_SEEDS[name] if seed is None else seed
+        return with_canopy(soil_only(ncol or 262144, s, gids=gids), s, gids=gids)
     if name == "canopy_uneven":
-        s = 0xC75D0003 if seed is None else seed
-        return with_canopy(soil_only(ncol or 1048576, s, snow=True, bedrock=True), s)
+        s = CANOPY_SEEDS[name] if seed is None else seed
+        return with_canopy(soil_only(ncol or 1048576, s, snow=True, bedrock=True, gids=gids), s, gids=gids)
     raise KeyError(name)
+
+
+def binned_ids(name, ncol, parts=4, seed=None):
+    """Global ids 0 .. ncol-1 of a canopy workload in binned order: sorted by
+    (snow layers, bedrock index) -- the work of a column's heat and Richards
+    solves -- then dealt over `parts` contiguous parts (part k takes every
+    parts-th column of the sorted order), so each part has the same mix."""
+    s = CANOPY_SEEDS[name] if seed is None else seed
+    uneven = name == "canopy_uneven"
+    nsno, nbed = structure(ncol, s, snow=uneven, bedrock=uneven)
+    o = np.lexsort((nbed, nsno))
+    if parts > 1:
+        o = np.concatenate([o[k::parts] for k in range(parts)])
+    return o.astype(np.int64)

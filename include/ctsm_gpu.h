@@ -549,7 +549,13 @@ typedef struct ctsm_phs_stress_state {
  * step_streams: the fused soil step splits its filter into this many
  *   contiguous parts run on as many streams of the context at once, joined
  *   before the call's later work (1..4).  A batch stored in work-binned
- *   order keeps the parts balanced by dealing the bins over the parts. */
+ *   order keeps the parts balanced by dealing the bins over the parts.
+ * plain_routines: 1 = the land steps (ctsm_land_step_full_dev and its
+ *   captures) run the plain form: SoilTemperature and SoilWater as one launch
+ *   per reference routine with intermediates through HBM (the plain kernels of
+ *   the *_dev entries' `plain` argument), CanopyFluxes as one launch that
+ *   iterates every patch to convergence (no compacted rounds).  The comparison
+ *   baseline of the batched step; results are the same. */
 typedef struct ctsm_tuning {
     int32_t richards_lanes;
     int32_t richards_block;
@@ -558,6 +564,7 @@ typedef struct ctsm_tuning {
     int32_t canopy_first_iterations;
     int32_t canopy_round_iterations;
     int32_t step_streams;
+    int32_t plain_routines;
 } ctsm_tuning;
 
 typedef struct ctsm_ctx ctsm_ctx;

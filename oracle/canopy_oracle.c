@@ -747,6 +747,37 @@ static int photosynthesis(const ctx_t *X, int p, int sun, double esat_tv, double
     return rc;
 }
 
+/* Test access: Photosynthesis for patch p (1-based global index) and one
+ * phase (sun = 1, shade = 0) at the leaf temperature held in t_veg(p), with
+ * the canopy-air arguments the iteration would pass; writes the same patch
+ * outputs.  Used only by the pins of the temperature kinetics and the
+ * co-limitation. */
+int oracle_photosynthesis_patch(const ctsm_config *cfg, const ctsm_canopy_params *prm, const ctsm_pft_params *pft,
+                                const ctsm_layout *lay, const ctsm_col_state *col, const ctsm_canopy_state *S,
+                                int p, int sun, double esat_tv, double eair, double oair, double cair, double rb,
+                                double btran, double dayl_factor)
+{
+    ctx_t X;
+    X.L.begc = lay->begc_alloc;
+    X.L.ldc = lay->ldc;
+    X.L.begp = lay->begp_alloc;
+    X.L.ldp = lay->ldp;
+    X.L.begg = lay->begg_alloc;
+    X.L.nlevsno = cfg->nlevsno;
+    X.L.nlevgrnd = cfg->nlevgrnd;
+    X.cfg = cfg;
+    X.prm = prm;
+    X.pft = pft;
+    X.col = col;
+    X.cs = S;
+    return photosynthesis(&X, p, sun, esat_tv, eair, oair, cair, rb, btran, dayl_factor);
+}
+
+/* Test access to the temperature functions (PhotosynthesisMod.F90:2488-2561). */
+double oracle_ft(double tl, double ha) { return ft(tl, ha); }
+double oracle_fth(double tl, double hd, double se, double c) { return fth(tl, hd, se, c); }
+double oracle_fth25(double hd, double se) { return fth25(hd, se); }
+
 /* ---------------------------------------------------------------------- */
 /* CanopyFluxesMod.F90:191-1765.  Returns 0, or ORC_ENDRUN with *bad_patch
  * set where the reference calls endrun. */

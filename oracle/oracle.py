@@ -17,7 +17,9 @@ _LIB = None
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        # CTSM_ORACLE_LIB points the checker at another build of the same
+        # sources (the planted-mistake builds of tests/test_oracle_mutants.py)
+        path = os.environ.get("CTSM_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
         L = C.CDLL(path)
@@ -155,6 +157,13 @@ def _canopy_lib():
         L.oracle_quadratic.argtypes = [C.c_double, C.c_double, C.c_double, abi.f64p, abi.f64p]
         L.oracle_frictionvelocity_vec.argtypes = [C.c_int] + [abi.f64p] * 11
         L.oracle_moninobukini.argtypes = [C.c_double] * 6 + [abi.f64p, abi.f64p]
+        L.oracle_photosynthesis_patch.argtypes = [C.POINTER(abi.Config), C.POINTER(abi.CanopyParams),
+                                                  C.POINTER(abi.PftParams), C.POINTER(abi.Layout),
+                                                  C.POINTER(abi.ColState), C.POINTER(abi.CanopyState),
+                                                  C.c_int, C.c_int] + [C.c_double] * 7
+        for fn, na in (("oracle_ft", 2), ("oracle_fth", 4), ("oracle_fth25", 2)):
+            getattr(L, fn).argtypes = [C.c_double] * na
+            getattr(L, fn).restype = C.c_double
         L._canopy_ready = True
     return L
 
@@ -199,6 +208,30 @@ def canopyfluxes(st, cfg, prm, pft, filter_p=None, dtime=1800.0):
                                C.byref(st.layout()), C.byref(cs), C.byref(can), len(f), fp, C.c_double(dtime),
                                C.byref(bad))
     return rc, bad.value
+
+
+def photosynthesis_patch(st, cfg, prm, pft, p, sun, esat_tv, eair, oair, cair, rb, btran, dayl_factor=1.0):
+    """Photosynthesis for patch p (1-based) and one phase at the leaf
+    temperature held in st.can['t_veg'] (test access to the oracle's
+    routine; outputs are written into the state)."""
+    L = _canopy_lib()
+    cs, _ = st.structs()
+    can = st.canopy_struct()
+    return L.oracle_photosynthesis_patch(C.byref(cfg), C.byref(canopy_params_struct(prm)), C.byref(pft.s),
+                                         C.byref(st.layout()), C.byref(cs), C.byref(can), int(p), int(sun),
+                                         esat_tv, eair, oair, cair, rb, btran, dayl_factor)
+
+
+def ft(tl, ha):
+    return _canopy_lib().oracle_ft(tl, ha)
+
+
+def fth(tl, hd, se, c):
+    return _canopy_lib().oracle_fth(tl, hd, se, c)
+
+
+def fth25(hd, se):
+    return _canopy_lib().oracle_fth25(hd, se)
 
 
 def root_sink(st, cfg, filter_c=None):

@@ -1,0 +1,131 @@
+"""Planted mistakes: each entry copies the oracle's C sources, plants one
+plausible misreading of the reference (a swapped exponent, a wrong
+activation energy, a dropped term, an off-by-one branch), builds the copy and
+runs the oracle's pins against it through CTSM_ORACLE_LIB.  Every planted
+mistake must make at least one pin fail -- otherwise the pins would let the
+same misreading through in the oracle and, since the GPU is held bit-equal
+to the oracle, in the kernels.
+
+Nothing here touches the GPU or the product library.
+"""
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE = os.path.join(ROOT, "oracle")
+SOURCES = ["ctsm_oracle.c", "canopy_oracle.c", "soilfluxes_oracle.c", "hydrology_oracle.c", "phs_oracle.c"]
+PINS = ["tests/test_oracle_pins.py", "tests/test_oracle_physics.py", "tests/test_oracle_canopy.py"]
+
+# (name, file, text in the oracle, planted text); the text occurs once.
+MUTANTS = [
+    ("thk: liquid and ice exponents swapped (SoilTemperatureMod.F90:720)", "ctsm_oracle.c",
+     "cm_pow(TKWAT, fl * SOI(S->watsat, c, j)) *\n                    cm_pow(TKICE, (1.0 - fl) * SOI(S->watsat, c, j))",
+     "cm_pow(TKWAT, (1.0 - fl) * SOI(S->watsat, c, j)) *\n                    cm_pow(TKICE, fl * SOI(S->watsat, c, j))"),
+    ("thk: Kersten number without the +1", "ctsm_oracle.c",
+     "dke = fmaxf90(0.0, cm_log10(satw) + 1.0);", "dke = fmaxf90(0.0, cm_log10(satw) + 0.9);"),
+    ("thk: frozen Kersten number squared", "ctsm_oracle.c",
+     "else              dke = satw;", "else              dke = satw * satw;"),
+    ("thk: Kersten branch switch strictly above tfrz", "ctsm_oracle.c",
+     "if (T(j) >= TFRZ) dke", "if (T(j) > TFRZ) dke"),
+    ("thk: bedrock from layer nbedrock", "ctsm_oracle.c",
+     "if (j > nbedrock) th = THK_BEDROCK;", "if (j >= nbedrock) th = THK_BEDROCK;"),
+    ("cv: ice and liquid heat capacities swapped", "ctsm_oracle.c",
+     "(ICE(j) * CPICE + LIQ(j) * CPLIQ) + 0.0 * CPICE", "(ICE(j) * CPLIQ + LIQ(j) * CPICE) + 0.0 * CPICE"),
+    ("cv: solids without the (1 - watsat) factor", "ctsm_oracle.c",
+     "A(cv, j) = SOI(S->csol, c, j) * (1.0 - SOI(S->watsat, c, j)) * DZ(j) +",
+     "A(cv, j) = SOI(S->csol, c, j) * DZ(j) +"),
+    ("tk: interface weights swapped", "ctsm_oracle.c",
+     "(A(thk, j) * (Z(j + 1) - ZIv(j)) + A(thk, j + 1) * (ZIv(j) - Z(j)))",
+     "(A(thk, j) * (ZIv(j) - Z(j)) + A(thk, j + 1) * (Z(j + 1) - ZIv(j)))"),
+    ("fact: top layer without capr", "ctsm_oracle.c",
+     "(0.5 * (Z(j) - ZIv(j - 1) + CAPR * (Z(j + 1) - ZIv(j - 1))));",
+     "(0.5 * (Z(j) - ZIv(j - 1) + 1.0 * (Z(j + 1) - ZIv(j - 1))));"),
+    ("supercool: exponent -b for -1/b", "ctsm_oracle.c",
+     "cm_pow(smp / SOI(S->sucsat, c, j), -1.0 / SOI(S->bsw, c, j))",
+     "cm_pow(smp / SOI(S->sucsat, c, j), -SOI(S->bsw, c, j))"),
+    ("vcmax: jmaxha for vcmaxha (PhotosynthesisMod.F90:1804)", "canopy_oracle.c",
+     "vcmax25 * ft(t_veg, P->vcmaxha)", "vcmax25 * ft(t_veg, P->jmaxha)"),
+    ("ft: Arrhenius ratio inverted", "canopy_oracle.c",
+     "(1.0 - (C_TFRZ + 25.0) / tl)", "(1.0 - tl / (C_TFRZ + 25.0))"),
+    ("kc25 from the Ko coefficient", "canopy_oracle.c",
+     "const double kc25 = P->kc25_coef * forc_pbot;", "const double kc25 = P->ko25_coef * forc_pbot;"),
+    ("cp25 without the 0.5", "canopy_oracle.c",
+     "const double cp25 = 0.5 * oair / sco;", "const double cp25 = oair / sco;"),
+    ("lmr: vcmaxha for lmrha", "canopy_oracle.c",
+     "lmr25 * ft(t_veg, P->lmrha)", "lmr25 * ft(t_veg, P->vcmaxha)"),
+    ("vcmax: fth25 normalised with jmaxhd's partner entropy", "canopy_oracle.c",
+     "const double vcmaxc = fth25(P->vcmaxhd, vcmaxse);", "const double vcmaxc = fth25(P->vcmaxhd, vcmaxse * 1.01);"),
+    ("vcmaxse acclimation slope sign", "canopy_oracle.c",
+     "const double vcmaxse = (668.39 - 1.07 * t10c)", "const double vcmaxse = (668.39 + 1.07 * t10c)"),
+    ("C4 low-temperature inhibition sign", "canopy_oracle.c",
+     "(1.0 + cm_exp(0.2 * ((C_TFRZ + 15.0) - t_veg)))", "(1.0 + cm_exp(0.2 * (t_veg - (C_TFRZ + 15.0))))"),
+    ("co-limitation: curvatures swapped", "canopy_oracle.c",
+     "if (oracle_quadratic(s->theta_cj, -(s->ac + s->aj)", "if (oracle_quadratic(s->theta_ip, -(s->ac + s->aj)"),
+    ("co-limitation: larger root", "canopy_oracle.c",
+     "s->ag = fmax90(0.0, fmin90(r1, r2));", "s->ag = fmax90(0.0, fmax90(r1, r2));"),
+    ("ZengWang: z0 weights swapped", "canopy_oracle.c",
+     "cm_exp(egvf * cm_log(PA(S->z0mv, p)) + (1.0 - egvf) * cm_log(CA(S->z0mg, c)))",
+     "cm_exp((1.0 - egvf) * cm_log(PA(S->z0mv, p)) + egvf * cm_log(CA(S->z0mg, c)))"),
+    ("ZengWang: egvf with exp(-2 lt)", "canopy_oracle.c",
+     "(1.0 - C_ALPHA_AERO * cm_exp(-lt))", "(1.0 - C_ALPHA_AERO * cm_exp(-2.0 * lt))"),
+    ("Meier2022: displacement without the square root", "canopy_oracle.c",
+     "cm_exp(-cm_pow(C_CD1 * lt, 0.5))", "cm_exp(-(C_CD1 * lt))"),
+    ("rb: uaf in place of sqrt(uaf)", "canopy_oracle.c",
+     "P->cv / (sqrt(uaf) * sqrt(PA(S->dleaf_patch, p)))", "P->cv / (uaf * sqrt(PA(S->dleaf_patch, p)))"),
+    ("under-canopy weight from elai only", "canopy_oracle.c",
+     "const double w = cm_exp(-(PA(S->elai, p) + PA(S->esai, p)));", "const double w = cm_exp(-(PA(S->elai, p)));"),
+    ("under-canopy wind 0.3 u_m / u*", "canopy_oracle.c",
+     "fmin90(0.4, (0.03 * um / ustar))", "fmin90(0.4, (0.3 * um / ustar))"),
+    ("QSat: a5 digits transposed", "canopy_oracle.c", "0.196237241e-07", "0.196237421e-07"),
+]
+
+CFLAGS = ["-O2", "-mfma", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-std=c99", "-w"]
+
+
+def _build(tmp, name, old, new):
+    for f in SOURCES:
+        shutil.copy(os.path.join(ORACLE, f), tmp)
+    path = os.path.join(tmp, name)
+    with open(path) as fh:
+        s = fh.read()
+    assert s.count(old) == 1, f"{name}: planted text must occur exactly once: {old!r}"
+    with open(path, "w") as fh:
+        fh.write(s.replace(old, new))
+    lib = os.path.join(tmp, "liboracle.so")
+    cmd = ["gcc"] + CFLAGS + ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "ctsm_amd", "csrc"),
+                              "-shared", "-o", lib] + [os.path.join(tmp, f) for f in SOURCES] + ["-lm"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return lib
+
+
+def _caught(mutant):
+    desc, name, old, new = mutant
+    with tempfile.TemporaryDirectory() as tmp:
+        lib = _build(tmp, name, old, new)
+        env = dict(os.environ, CTSM_ORACLE_LIB=lib)
+        r = subprocess.run([sys.executable, "-m", "pytest", "-x", "-q", "-p", "no:cacheprovider", "-m", "not gpu"]
+                           + PINS, cwd=ROOT, env=env, capture_output=True, text=True, timeout=600)
+    return desc, r.returncode != 0, r.stdout[-400:]
+
+
+def test_unmutated_copy_passes():
+    """The harness itself: the unmodified sources built the same way pass."""
+    desc, caught, out = _caught(("identity", "ctsm_oracle.c", "#define TFRZ        273.15",
+                                 "#define TFRZ        273.15"))
+    assert not caught, out
+
+
+@pytest.mark.timeout(900)
+def test_every_planted_mistake_fails_a_pin():
+    assert len(MUTANTS) >= 25
+    with ThreadPoolExecutor(max(1, min(6, (os.cpu_count() or 2) - 1))) as ex:
+        results = list(ex.map(_caught, MUTANTS))
+    survivors = [(d, out) for d, caught, out in results if not caught]
+    assert not survivors, "planted mistakes no pin caught:\n" + "\n".join(d for d, _ in survivors)
