@@ -42,6 +42,7 @@
 #include <string.h>
 #include "ctsm_gpu.h"
 #include "ctsm_math.h"
+#include "oracle_libm.h"   /* -DORACLE_LIBM: the host libm instead */
 
 /* shr_const_mod (tools/mksurfdata_esmf/src/shr_const_mod.F90) and clm_varcon.F90 */
 #define C_TFRZ     273.15

@@ -52,6 +52,7 @@
 #include <string.h>
 #include "ctsm_gpu.h"
 #include "ctsm_math.h"   /* the same elementary functions as the kernels */
+#include "oracle_libm.h"   /* -DORACLE_LIBM: the host libm instead */
 
 /* ---- constants: shr_const_mod (tools/mksurfdata_esmf/src/shr_const_mod.F90:14-52)
  *      and clm_varcon.F90:50-121 ------------------------------------------ */
@@ -72,13 +73,6 @@
 #define CNFAC       0.5
 #define THIN_SFCLAYER 1.0e-6
 #define M_TO_MM     1.e3
-
-#ifdef ORACLE_LIBM
-#undef cm_pow
-#undef cm_log10
-#define cm_pow pow
-#define cm_log10 log10
-#endif
 
 #define NLEVMAX_SOI 64   /* scratch bound for layers */
 #define NBANDMAX    (64 + 64)

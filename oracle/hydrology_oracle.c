@@ -51,6 +51,7 @@
 #include <stdlib.h>
 #include "ctsm_gpu.h"
 #include "ctsm_math.h"
+#include "oracle_libm.h"   /* -DORACLE_LIBM: the host libm instead */
 
 #define H_PI 3.14159265358979323846
 

@@ -44,6 +44,7 @@
 #include <stdlib.h>
 #include "ctsm_gpu.h"
 #include "ctsm_math.h"
+#include "oracle_libm.h"   /* -DORACLE_LIBM: the host libm instead */
 
 #define P_PI 3.14159265358979323846
 #define P_LN2 0.6931471805599453

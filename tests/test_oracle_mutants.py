@@ -98,7 +98,7 @@ def _build(tmp, name, old, new):
     with open(path, "w") as fh:
         fh.write(s.replace(old, new))
     lib = os.path.join(tmp, "liboracle.so")
-    cmd = ["gcc"] + CFLAGS + ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "ctsm_amd", "csrc"),
+    cmd = ["gcc"] + CFLAGS + ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "ctsm_amd", "csrc"), "-I", ORACLE,
                               "-shared", "-o", lib] + [os.path.join(tmp, f) for f in SOURCES] + ["-lm"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
