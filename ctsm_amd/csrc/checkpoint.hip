@@ -11,6 +11,12 @@
 // Data moves through two pinned staging buffers: while the host writes (or
 // reads and hashes) one chunk, the copy engine moves the next one, so a
 // checkpoint runs at the slower of PCIe and the file system.
+// A write goes to "<path>.tmp", is flushed and fsync'ed, and only then
+// renamed over <path>: an interrupted write (a kill, a full disk) leaves the
+// previous checkpoint at <path> whole.  A read first hashes every requested
+// range from the file and compares the checksums; only a file that passes
+// is copied into the caller's memory, so a damaged file leaves the
+// destination untouched.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -18,6 +24,8 @@
 #include <cstring>
 #include <string>
 #include <vector>
+
+#include <unistd.h>
 
 #include "ctsm_gpu.h"
 
@@ -77,6 +85,13 @@ struct File {
     ~File() { if (f) fclose(f); }
 };
 
+// The temporary file of a write: removed unless the write reached the rename.
+struct TmpFile {
+    std::string path;
+    bool keep = false;
+    ~TmpFile() { if (!keep) (void)remove(path.c_str()); }
+};
+
 // Declared after the staging buffers, so it is destroyed first: no copy is
 // still in flight when the buffers are freed, whichever way a call returns.
 struct StreamDrain {
@@ -110,8 +125,9 @@ int ckpt_write(hipStream_t st, const char *path, const ctsm_ckpt_field *fields, 
         off[i] = pos;
         pos = align_up(pos + fields[i].bytes);
     }
+    TmpFile tmp{std::string(path) + ".tmp"};
     File fp;
-    fp.f = fopen(path, "wb");
+    fp.f = fopen(tmp.path.c_str(), "wb");
     if (!fp.f) return CTSM_E_IO;
     Staging S;
     StreamDrain drain{st};
@@ -153,7 +169,11 @@ int ckpt_write(hipStream_t st, const char *path, const ctsm_ckpt_field *fields, 
         memcpy(e + kNameLen + 16, &sum[i], 8);
     }
     if (!seek(fp.f, 0) || fwrite(hdr.data(), 1, hdr.size(), fp.f) != hdr.size()) return CTSM_E_IO;
-    if (fflush(fp.f) != 0) return CTSM_E_IO;
+    if (fflush(fp.f) != 0 || fsync(fileno(fp.f)) != 0) return CTSM_E_IO;
+    const int closed = fclose(fp.f);
+    fp.f = nullptr;
+    if (closed != 0 || rename(tmp.path.c_str(), path) != 0) return CTSM_E_IO;
+    tmp.keep = true;
     return CTSM_OK;
 }
 
@@ -191,6 +211,18 @@ int ckpt_read(hipStream_t st, const char *path, const ctsm_ckpt_field *fields, i
     Staging S;
     StreamDrain drain{st};
     if ((*he = S.init()) != hipSuccess) return CTSM_E_HIP;
+    // first pass: hash every requested range from the file, touching nothing
+    for (int i = 0; i < n; ++i) {
+        uint64_t h = kFnvBasis;
+        for (uint64_t a = 0; a < fields[i].bytes; a += kChunk) {
+            const uint64_t len = std::min<uint64_t>(kChunk, fields[i].bytes - a);
+            if (!seek(fp.f, want[i].off + a) || fread(S.buf[0], 1, len, fp.f) != len) return CTSM_E_IO;
+            h = fnv1a(h, (const unsigned char *)S.buf[0], len);
+        }
+        if (h != want[i].sum) return CTSM_E_IO;
+    }
+    // second pass: the verified ranges into the caller's memory; a range that
+    // reads differently now (the file changed in between) still fails
     bool used[2] = {false, false};
     size_t c = 0;
     bool ok = true;

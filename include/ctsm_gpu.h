@@ -102,14 +102,24 @@ typedef struct ctsm_config {
     double  xTolerLower;        /* 0.01                                             */
     double  e_ice;              /* 6 (ice impedance exponent)                       */
     /* CanopyFluxes / Photosynthesis namelist switches.  use_fates, use_cn,
-     * use_hydrstress, use_luna, use_c13, lnc_opt and perchroot are .false. and
-     * vcmax_opt = 0, nlevcan = 1 on this path. */
+     * use_luna, use_c13, lnc_opt and perchroot are .false. and vcmax_opt = 0,
+     * nlevcan = 1 on this path. */
     int32_t z0param_method;     /* CTSM_Z0_*                        (xml:621-622)    */
     int32_t stomatalcond_method;/* CTSM_STOMATA_*                   (xml:108)        */
     int32_t soil_resis_method;  /* CTSM_SOILRESIS_*                 (xml:270)        */
     int32_t use_biomass_heat_storage;  /* 0/1                       (xml:457)        */
     int32_t use_undercanopy_stability; /* 0/1                       (xml:454)        */
     int32_t light_inhibit;      /* 0/1                              (xml:95)         */
+    /* use_hydrstress (clm_varctl, namelist_defaults_ctsm.xml:726): 1 selects
+     * PhotosynthesisHydraulicStress and the PHS branches of CanopyFluxes
+     * (CanopyFluxesMod.F90:638, 1133, 1157, 1219, 1353).  ctsm_create returns
+     * CTSM_E_CONFIG for 1: the canopy entries run the use_hydrstress = .false.
+     * path only (the PHS soil side and calcstress have entries of their own). */
+    int32_t use_hydrstress;     /* 0                                                 */
+    /* use_excess_ice (clm_varctl; SoilTemperatureMod.F90:287 and the excess-ice
+     * terms of SoilThermProp / Phasechange): ctsm_create returns CTSM_E_CONFIG
+     * for 1. */
+    int32_t use_excess_ice;     /* 0                                                 */
 } ctsm_config;
 
 /* ---- array layout of one call -------------------------------------------- */
@@ -884,11 +894,14 @@ int ctsm_hist_zero_dev(ctsm_ctx *ctx, const ctsm_hist_field *fields, int32_t nfi
  * field count and, per field, a 64-byte name, its size, its file offset and a
  * 64-bit FNV-1a checksum of its 8-byte words, then the data, each range
  * aligned to 4 KiB.  Writing
- * waits for the context's stream, then streams the ranges to the file through
- * two pinned staging buffers (the next chunk's copy overlaps the previous
- * chunk's write).  Reading fills each named range from the file the same way
- * and returns CTSM_E_IO when a name is missing, a size differs or a checksum
- * does not match.  `ptr` may be device or host memory. */
+ * waits for the context's stream, then streams the ranges through two pinned
+ * staging buffers (the next chunk's copy overlaps the previous chunk's write)
+ * into "<path>.tmp", flushes and fsyncs it and renames it over `path`: a
+ * write that fails or is interrupted leaves an existing file at `path` as it
+ * was.  Reading first hashes every named range in the file and returns
+ * CTSM_E_IO, with the destination untouched, when a name is missing, a size
+ * differs or a checksum does not match; only then does it fill the ranges
+ * the same way.  `ptr` may be device or host memory. */
 typedef struct ctsm_ckpt_field {
     const char *name;     /* NUL-terminated, at most 63 characters */
     void       *ptr;

@@ -1,5 +1,7 @@
 // Writes fields of given sizes (filled with a fixed byte pattern) to a
-// checkpoint file and, with "read", reads them back and compares.
+// checkpoint file and, with "read", reads them back and compares.  A read
+// starts from buffers filled with 0x5a; when it fails, every byte must still
+// be 0x5a (a refused file leaves the destination untouched), else exit 8.
 #include "hip/hip_runtime.h"
 #include <cstdint>
 #include <cstdio>
@@ -22,8 +24,7 @@ int main(int argc, char **argv)
     for (int i = 0; i < n; ++i) {
         const size_t sz = strtoull(argv[3 + i], nullptr, 10);
         a[i].resize(sz);
-        if (!rd)
-            for (size_t k = 0; k < sz; ++k) a[i][k] = (unsigned char)((k * 131 + i * 7) & 0xff);
+        for (size_t k = 0; k < sz; ++k) a[i][k] = rd ? 0x5a : (unsigned char)((k * 131 + i * 7) & 0xff);
         names[i] = "field" + std::to_string(i);
         f[i] = {names[i].c_str(), sz ? a[i].data() : nullptr, sz};
     }
@@ -34,6 +35,10 @@ int main(int argc, char **argv)
         for (int i = 0; i < n; ++i)
             for (size_t k = 0; k < a[i].size(); ++k)
                 if (a[i][k] != (unsigned char)((k * 131 + i * 7) & 0xff)) { printf("mismatch %d %zu\n", i, k); return 9; }
+    if (rd && rc != 0)
+        for (int i = 0; i < n; ++i)
+            for (size_t k = 0; k < a[i].size(); ++k)
+                if (a[i][k] != 0x5a) { printf("dirty %d %zu\n", i, k); return 8; }
     printf("%d\n", rc);
     return 0;
 }

@@ -3,9 +3,13 @@ ctsm_amd/csrc/checkpoint.hip built by g++ against a host-only stand-in of the
 HIP calls (tests/host_hip), the file parsed independently here -- magic, field
 table, 4 KiB-aligned ranges, FNV-1a checksums over 64-bit words -- with fields
 larger than one 32 MiB staging chunk, empty fields and a range ending exactly on
-the alignment; a flipped byte or a wrong size is refused (CTSM_E_IO)."""
+the alignment; a flipped byte or a wrong size is refused (CTSM_E_IO) with the
+destination untouched; a write that is killed or fails partway leaves the
+previous file whole."""
 import os
+import resource
 import shutil
+import signal
 import struct
 import subprocess
 
@@ -86,3 +90,45 @@ def test_damaged_or_mismatched_file_is_refused(tool):
     bad.write_bytes(bytes(raw))
     assert _run(exe, "r", bad, SIZES) == CTSM_E_IO
     assert _run(exe, "r", path, SIZES[:-1] + [SIZES[-1] - 8]) == CTSM_E_IO
+
+
+def _limited(limit, ignore_xfsz):
+    """preexec_fn: files this process writes stop at `limit` bytes; past it
+    the write is killed by SIGXFSZ (a crash mid-write) or, with the signal
+    ignored, fails with EFBIG (a full disk)."""
+    def f():
+        resource.setrlimit(resource.RLIMIT_FSIZE, (limit, limit))
+        signal.signal(signal.SIGXFSZ, signal.SIG_IGN if ignore_xfsz else signal.SIG_DFL)
+    return f
+
+
+@pytest.mark.parametrize("ignore_xfsz", [False, True], ids=["killed", "write-fails"])
+def test_interrupted_write_keeps_the_previous_file(tool, ignore_xfsz):
+    exe, d = tool
+    path = d / f"keep_{int(ignore_xfsz)}.ckpt"
+    assert _run(exe, "w", path, SIZES) == 0
+    before = path.read_bytes()
+    big = [80 << 20, 3 << 20]                       # a larger checkpoint, cut off at 16 MiB
+    r = subprocess.run([str(exe), "w", str(path)] + [str(s) for s in big], capture_output=True, text=True,
+                       preexec_fn=_limited(16 << 20, ignore_xfsz))
+    if ignore_xfsz:
+        assert r.returncode == 0 and int(r.stdout.split()[-1]) == CTSM_E_IO, r.stdout + r.stderr
+        assert not os.path.exists(str(path) + ".tmp")     # the failed write cleans up after itself
+    else:
+        assert r.returncode == -signal.SIGXFSZ, (r.returncode, r.stderr)
+    assert path.read_bytes() == before
+    assert _run(exe, "r", path, SIZES) == 0
+
+
+def test_refused_file_leaves_the_destination_untouched(tool):
+    exe, d = tool
+    path = d / "c.ckpt"
+    assert _run(exe, "w", path, SIZES) == 0
+    raw = bytearray(path.read_bytes())
+    e = raw[16 + 88 * 5:16 + 88 * 6]                 # damage the last field: every earlier one checks out
+    size, off, _ = struct.unpack("<QQQ", bytes(e[64:]))
+    raw[off + size // 2] ^= 0x01
+    bad = d / "c_bad.ckpt"
+    bad.write_bytes(bytes(raw))
+    # exit 0 and CTSM_E_IO: ckpt_main exits 8 if any destination byte changed
+    assert _run(exe, "r", bad, SIZES) == CTSM_E_IO
