@@ -34,6 +34,8 @@ class Config(C.Structure):
         ("use_biomass_heat_storage", C.c_int32),
         ("use_undercanopy_stability", C.c_int32),
         ("light_inhibit", C.c_int32),
+        ("use_hydrstress", C.c_int32),
+        ("use_excess_ice", C.c_int32),
     ]
 
 

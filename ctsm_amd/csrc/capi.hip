@@ -611,6 +611,8 @@ void ctsm_config_default(ctsm_config *cfg)
     cfg->use_biomass_heat_storage = 1;
     cfg->use_undercanopy_stability = 0;
     cfg->light_inhibit = 1;
+    cfg->use_hydrstress = 0;
+    cfg->use_excess_ice = 0;
 }
 
 int ctsm_create(const ctsm_config *cfg, int device, ctsm_ctx **out)
@@ -631,6 +633,7 @@ int ctsm_create(const ctsm_config *cfg, int device, ctsm_ctx **out)
         (unsigned)cfg->use_biomass_heat_storage > 1u || (unsigned)cfg->use_undercanopy_stability > 1u ||
         (unsigned)cfg->light_inhibit > 1u || cfg->itmax_canopy_fluxes < 0 || cfg->itmax_canopy_fluxes > 1000)
         return CTSM_E_CONFIG;
+    if (cfg->use_hydrstress != 0 || cfg->use_excess_ice != 0) return CTSM_E_CONFIG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return CTSM_E_NODEVICE;
     if (device < 0 || device >= ndev) return CTSM_E_ARG;

@@ -34,6 +34,7 @@ module ctsm_gpu_mod
      real(c_double)     :: dtmin, verySmall, xTolerUpper, xTolerLower, e_ice
      integer(c_int32_t) :: z0param_method, stomatalcond_method, soil_resis_method
      integer(c_int32_t) :: use_biomass_heat_storage, use_undercanopy_stability, light_inhibit
+     integer(c_int32_t) :: use_hydrstress, use_excess_ice
   end type ctsm_config_t
 
   type, bind(C), public :: ctsm_layout_t

@@ -121,7 +121,8 @@ def test_config_rejected_before_device():
     for bad in (dict(soilwater_movement_method=0), dict(lower_boundary_condition=4),
                 dict(upper_boundary_condition=0), dict(nlevsno=13), dict(flux_calculation=42),
                 dict(z0param_method=2), dict(stomatalcond_method=0), dict(soil_resis_method=3),
-                dict(use_biomass_heat_storage=2), dict(itmax_canopy_fluxes=-1)):
+                dict(use_biomass_heat_storage=2), dict(itmax_canopy_fluxes=-1),
+                dict(use_hydrstress=1), dict(use_excess_ice=1)):
         cfg = lib.default_config(**bad)
         assert L.ctsm_create(C.byref(cfg), 0, C.byref(h)) == lib.CTSM_E_CONFIG, bad
 
