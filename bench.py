@@ -371,8 +371,10 @@ def headline(ctx, args):
     if not args.no_autotune:
         split, _, split_table = AT.autotune_canopy(ctx, ds)
         best, _, table = AT.autotune_land(ctx, ds, hydro, steps=2, reps=2)
-        res["autotune_canopy_rounds_ms"] = {f"{a}/{c}": round(ms, 3) for (a, c), ms in split_table}
+        res["autotune_canopy_ms"] = {f"{a}/{c}/w{w}": round(ms, 3) for (a, c, w), ms in split_table}
         res["autotune_land_ms_per_step"] = [[c, round(ms, 3)] for c, ms in table]
+    if args.set:
+        ctx.set_tuning(**{k: int(v) for k, v in (a.split("=", 1) for a in args.set)})
     res["tuning"] = ctx.get_tuning()
     snap = AT.snapshot(ctx, ds)
     # the timed steps, captured once as one graph; the first launch (which
@@ -501,6 +503,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
+                    help="launch-shape value applied after autotuning (a ctsm_tuning member), e.g. canopy_waves=2")
     ap.add_argument("--soil-ncol", type=int, default=65536,
                     help="columns of the soil-only uneven measurement (0 skips it)")
     ap.add_argument("--extras", action="store_true",
@@ -517,7 +521,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
-                                   "(columns shard trivially by range, see DESIGN.md)"}))
+                                   "(columns and their patches shard trivially by contiguous range)"}))
         sys.exit(2)
 
     from ctsm_amd import lib
@@ -584,7 +588,7 @@ def main():
                    "ncol": ncol, "npatch": npatch, "steps": args.steps,
                    "binning": "generated in (snow layers, bedrock) order",
                    "launch": f"the {args.steps} timed steps captured as one graph, launched once",
-                   "tuning": "autotuned" if not args.no_autotune else "default",
+                   "tuning": ("autotuned" if not args.no_autotune else "default") + "".join(f", {a}" for a in args.set),
                    "richards_lanes": lanes},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": prof["traffic"] if prof else None,

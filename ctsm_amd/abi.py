@@ -49,6 +49,7 @@ class Tuning(C.Structure):
         ("canopy_round_iterations", C.c_int32),
         ("step_streams", C.c_int32),
         ("plain_routines", C.c_int32),
+        ("canopy_waves", C.c_int32),
     ]
 
 

@@ -113,17 +113,25 @@ def autotune(ctx, ds, filter_dev=None, n=None, steps=8, reps=2, candidates=None,
 # (first-round iterations, later-round iterations) of the CanopyFluxes
 # leaf-temperature iteration; first > itmax is the one-pass kernel.
 CANOPY_CANDIDATES = [(1001, 1), (4, 2), (4, 4), (5, 3), (6, 3), (6, 6), (8, 4), (10, 5)]
+# register budgets (waves per SIMD) of the CanopyFluxes kernel
+CANOPY_WAVES = (1, 2, 3)
 
 
-def autotune_canopy(ctx, ds, filter_dev=None, n=None, reps=2, candidates=None, dtime=1800.0, verbose=False):
-    """Pick the CanopyFluxes round split that runs this state fastest (timed
-    on a snapshot of the resident state, which is left unchanged)."""
+def autotune_canopy(ctx, ds, filter_dev=None, n=None, reps=2, candidates=None, waves=None, dtime=1800.0,
+                    verbose=False):
+    """Pick the CanopyFluxes launch shape that runs this state fastest (timed
+    on a snapshot of the resident state, which is left unchanged): the round
+    split over `candidates` at the current register budget, then the budget
+    over `waves` at the best split.  Returns ((first, later, waves), ms,
+    table of ((first, later, waves), ms))."""
     candidates = candidates or CANOPY_CANDIDATES
+    waves = waves or CANOPY_WAVES
     snap = snapshot(ctx, ds)
     structs = (*ds.structs(), ds.canopy_struct())
     best, best_ms, table = None, float("inf"), []
-    for first, later in candidates:
-        ctx.set_tuning(canopy_first_iterations=first, canopy_round_iterations=later)
+
+    def timed(first, later, w):
+        ctx.set_tuning(canopy_first_iterations=first, canopy_round_iterations=later, canopy_waves=w)
         ms = 0.0
         for _ in range(reps):
             restore(ctx, ds, snap)
@@ -134,14 +142,25 @@ def autotune_canopy(ctx, ds, filter_dev=None, n=None, reps=2, candidates=None, d
             ms += ctx.elapsed_ms(e0, e1) / reps
             ctx.destroy_event(e0)
             ctx.destroy_event(e1)
-        table.append(((first, later), ms))
+        table.append(((first, later, w), ms))
         if verbose:
-            print(f"autotune canopy first={first} later={later} -> {ms:.3f} ms", flush=True)
+            print(f"autotune canopy first={first} later={later} waves={w} -> {ms:.3f} ms", flush=True)
+        return ms
+
+    w0 = ctx.get_tuning()["canopy_waves"]
+    for first, later in candidates:
+        ms = timed(first, later, w0)
         if ms < best_ms:
-            best, best_ms = (first, later), ms
+            best, best_ms = (first, later, w0), ms
+    for w in waves:
+        if w == w0:
+            continue
+        ms = timed(best[0], best[1], w)
+        if ms < best_ms:
+            best, best_ms = (best[0], best[1], w), ms
     restore(ctx, ds, snap)
     free_snapshot(ctx, snap)
-    ctx.set_tuning(canopy_first_iterations=best[0], canopy_round_iterations=best[1])
+    ctx.set_tuning(canopy_first_iterations=best[0], canopy_round_iterations=best[1], canopy_waves=best[2])
     return best, best_ms, table
 
 

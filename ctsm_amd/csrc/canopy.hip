@@ -751,12 +751,12 @@ __device__ __forceinline__ bool canopy_patch(const CView &V, const CSwitch &sw, 
     if (!first) {
         btran = S.btran[pp];
     } else {
-        double rr[MAX_NLEVGRND];
+        // rootr holds rootfr * rresis until btran is known, then the
+        // normalised value (SoilMoistStressMod.F90:480-500 writes it the same
+        // way); a run-time level loop keeps the per-level offsets out of the
+        // kernel's uniform registers
         const double smpsc = PF(F_SMPSC), smpso = PF(F_SMPSO);
-#pragma unroll
-        for (int j = 1; j <= MAX_NLEVGRND; ++j) {
-            rr[j - 1] = 0.0;
-            if (j > V.nlevgrnd) continue;
+        for (int j = 1; j <= V.nlevgrnd; ++j) {
             const long cs = V.soi(c, j), cn = V.sno(c, j);
             const double watsat = col.watsat[cs], dz = col.dz[cn];
             const double vol_ice = fmin90(watsat, col.h2osoi_ice[cn] / (DENICE * dz));
@@ -764,22 +764,21 @@ __device__ __forceinline__ bool canopy_patch(const CView &V, const CSwitch &sw, 
             col.eff_porosity[cs] = eff;
             const double liqvol = fmin90(eff, col.h2osoi_liq[cn] / (dz * DENH2O));
             S.h2osoi_liqvol[cs] = liqvol;
-            if (liqvol <= 0.0 || col.t_soisno[cn] <= TFRZ - 2.0) {
-                rr[j - 1] = 0.0;
-            } else {
+            double rr = 0.0;
+            if (!(liqvol <= 0.0 || col.t_soisno[cn] <= TFRZ - 2.0)) {
                 const double s_node = fmax90(liqvol / eff, 0.01);
                 double smp_node = -col.sucsat[cs] * cm_pow(s_node, -col.bsw[cs]);
                 smp_node = fmax90(smpsc, smp_node);
                 const double rres = fmin90((eff / watsat) * (smp_node - smpsc) / (smpso - smpsc), 1.0);
                 S.rresis[V.soip(p, j)] = rres;
-                rr[j - 1] = S.rootfr[V.soip(p, j)] * rres;
-                btran = btran + fmax90(rr[j - 1], 0.0);
+                rr = S.rootfr[V.soip(p, j)] * rres;
+                btran = btran + fmax90(rr, 0.0);
             }
+            S.rootr[V.soip(p, j)] = rr;
         }
-#pragma unroll
-        for (int j = 1; j <= MAX_NLEVGRND; ++j) {
-            if (j > V.nlevgrnd) continue;
-            S.rootr[V.soip(p, j)] = (btran > btran0) ? rr[j - 1] / btran : 0.0;
+        for (int j = 1; j <= V.nlevgrnd; ++j) {
+            const long sp = V.soip(p, j);
+            S.rootr[sp] = (btran > btran0) ? S.rootr[sp] / btran : 0.0;
         }
         S.btran[pp] = btran;
     }
@@ -1307,8 +1306,13 @@ __device__ __forceinline__ bool canopy_patch(const CView &V, const CSwitch &sw, 
 
 // One round of CanopyFluxes over the patches still iterating (round 0: the
 // whole filter), compacting those that continue into list_out with one atomic
-// per wavefront.  Blocks are one wavefront; the grid strides over the count.
-__global__ void __launch_bounds__(64)
+// per wavefront.  Blocks are one wavefront, a patch per lane; the grid covers
+// the whole filter and waves past the round's count return at once.
+// WAVES: the register budget the kernel is compiled for (waves per SIMD);
+// above one wave the compiler spills what does not fit to scratch.  The
+// launch picks the instance (a launch shape the autotuner times).
+template <int WAVES>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES)))
 k_canopy_round(CView V, CSwitch sw, ctsm_canopy_params P, const double *__restrict__ pft, ctsm_col_state col,
                ctsm_canopy_state S, int n, const int32_t *__restrict__ filter, double dtime, ErrWord *err,
                int round, int budget, const int32_t *__restrict__ list_in, const int32_t *__restrict__ cnt_in,
@@ -1316,22 +1320,21 @@ k_canopy_round(CView V, CSwitch sw, ctsm_canopy_params P, const double *__restri
 {
     const int count = (round == 0) ? n : *cnt_in;
     const int lane = threadIdx.x;
-    for (int base = blockIdx.x * 64; base < count; base += gridDim.x * 64) {
-        const int idx = base + lane;
-        bool cont = false;
-        int i0 = 0;
-        if (idx < count) {
-            i0 = (round == 0) ? idx : list_in[idx];
-            cont = canopy_patch(V, sw, P, pft, col, S, i0, filter, dtime, err, round == 0, budget, W);
-        }
-        const unsigned long long mask = __ballot(cont);
-        if (mask) {
-            const int leader = __ffsll((unsigned long long)mask) - 1;
-            int pos = 0;
-            if (lane == leader) pos = atomicAdd(cnt_out, __popcll(mask));
-            pos = __shfl(pos, leader);
-            if (cont) list_out[pos + __popcll(mask & ((1ull << lane) - 1ull))] = i0;
-        }
+    const int idx = blockIdx.x * 64 + lane;
+    if (blockIdx.x * 64 >= count) return;       // whole wave past the list
+    bool cont = false;
+    int i0 = 0;
+    if (idx < count) {
+        i0 = (round == 0) ? idx : list_in[idx];
+        cont = canopy_patch(V, sw, P, pft, col, S, i0, filter, dtime, err, round == 0, budget, W);
+    }
+    const unsigned long long mask = __ballot(cont);
+    if (mask) {
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        int pos = 0;
+        if (lane == leader) pos = atomicAdd(cnt_out, __popcll(mask));
+        pos = __shfl(pos, leader);
+        if (cont) list_out[pos + __popcll(mask & ((1ull << lane) - 1ull))] = i0;
     }
 }
 
@@ -1512,9 +1515,10 @@ size_t canopy_work_bytes(int n, int itmax, int first_its, int round_its)
 void launch_canopy(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg, const ctsm_canopy_params &prm,
                    const double *pft_dev, const ctsm_col_state &col, const ctsm_canopy_state &can, int n,
                    const int32_t *filter, double dtime, ErrWord *err, void *work, int first_its, int round_its,
-                   bool zero_cgrnd)
+                   bool zero_cgrnd, int waves)
 {
     using namespace canopy;
+    auto round_kernel = waves == 3 ? k_canopy_round<3> : waves == 2 ? k_canopy_round<2> : k_canopy_round<1>;
     CView V{L.begc_alloc, L.ldc, L.begp_alloc, L.ldp, L.begg_alloc, L.ldg, cfg.nlevsno, cfg.nlevsoi, cfg.nlevgrnd};
     CSwitch sw{cfg.z0param_method, cfg.stomatalcond_method, cfg.soil_resis_method, cfg.use_biomass_heat_storage,
                cfg.use_undercanopy_stability, cfg.light_inhibit, cfg.itmax_canopy_fluxes};
@@ -1527,7 +1531,7 @@ void launch_canopy(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg,
     const int blocks = (n + 63) / 64;
     if (R == 0) {
         // one pass: every patch finishes (converged or itmax reached) in round 0
-        hipLaunchKernelGGL(k_canopy_round, dim3(blocks), dim3(64), 0, st, V, sw, prm, pft_dev, col, can, n, filter,
+        hipLaunchKernelGGL(round_kernel, dim3(blocks), dim3(64), 0, st, V, sw, prm, pft_dev, col, can, n, filter,
                            dtime, err, 0, cfg.itmax_canopy_fluxes + 1, (const int32_t *)nullptr,
                            (const int32_t *)nullptr, (int32_t *)nullptr, (int32_t *)nullptr, CanopyWork{});
         return;
@@ -1540,12 +1544,9 @@ void launch_canopy(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg,
     list[1] = list[0] + n;
     int32_t *cnt = list[1] + n;
     hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, st, cnt, R + 1);
-    // later rounds stride over the patches still iterating with a grid that
-    // keeps every wave slot of the chip busy
-    const int later_blocks = blocks < 8192 ? blocks : 8192;
     for (int r = 0; r <= R; ++r) {
         const int budget = r == 0 ? first_its : round_its;
-        hipLaunchKernelGGL(k_canopy_round, dim3(r == 0 ? blocks : later_blocks), dim3(64), 0, st, V, sw, prm, pft_dev,
+        hipLaunchKernelGGL(round_kernel, dim3(blocks), dim3(64), 0, st, V, sw, prm, pft_dev,
                            col, can, n, filter, dtime, err, r, budget, (const int32_t *)list[(r + 1) & 1],
                            (const int32_t *)(r ? cnt + r - 1 : nullptr), list[r & 1], cnt + r, W);
     }

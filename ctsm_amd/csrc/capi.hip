@@ -36,7 +36,7 @@ void launch_soilwater_plain(hipStream_t, const View &, const Params &, const cts
 size_t canopy_work_bytes(int n, int itmax, int first_its, int round_its);
 void launch_canopy(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_canopy_params &, const double *,
                    const ctsm_col_state &, const ctsm_canopy_state &, int, const int32_t *, double, ErrWord *,
-                   void *, int, int, bool);
+                   void *, int, int, bool, int);
 void launch_root_sink(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_col_state &,
                       const ctsm_canopy_state &, int, const int32_t *, ErrWord *);
 void launch_save_before(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_col_state &,
@@ -364,7 +364,7 @@ struct ctsm_ctx {
     int device = 0;
     ctsm_config cfg{};
     Params prm{};
-    ctsm_tuning tun{4, 128, 128, 0, 6, 3, 4, 0};
+    ctsm_tuning tun{4, 128, 128, 0, 6, 3, 4, 0, 1};
     static constexpr int kMaxStreams = 4;
     hipStream_t side[kMaxStreams - 1] = {nullptr, nullptr, nullptr};   // concurrent parts of a step
     hipEvent_t ev_fork = nullptr, ev_join[kMaxStreams - 1] = {nullptr, nullptr, nullptr};
@@ -713,6 +713,7 @@ int ctsm_set_tuning(ctsm_ctx *ctx, const ctsm_tuning *t)
         return CTSM_E_ARG;
     if (t->step_streams < 1 || t->step_streams > ctsm_ctx::kMaxStreams) return CTSM_E_ARG;
     if (t->plain_routines != 0 && t->plain_routines != 1) return CTSM_E_ARG;
+    if (t->canopy_waves < 1 || t->canopy_waves > 3) return CTSM_E_ARG;
     ctx->tun = *t;
     return CTSM_OK;
 }
@@ -957,7 +958,7 @@ static int run_canopy(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_stat
         if (he != hipSuccess) return hip_fail(ctx, he);
     }
     launch_canopy(ctx->stream, *lay, ctx->cfg, ctx->cprm, (const double *)ctx->pft.p, col, can, n, filter_dev, dtime,
-                  ctx->err, wb ? ctx->cwork.p : nullptr, k0, k, zero_cgrnd);
+                  ctx->err, wb ? ctx->cwork.p : nullptr, k0, k, zero_cgrnd, ctx->tun.canopy_waves);
     return CTSM_OK;
 }
 

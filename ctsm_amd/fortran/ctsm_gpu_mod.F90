@@ -7,11 +7,11 @@ module ctsm_gpu_mod
   ! the reference's own arrays (e.g. c_loc(waterstatebulk_inst%h2osoi_liq_col)),
   ! so the library reads and writes CTSM's memory in place.
   !
-  ! Drop-in wrappers for SoilWater (SoilWaterMovementMod.F90:240-243) and
-  ! SoilTemperature (SoilTemperatureMod.F90:92-95) built on this module are
-  ! given in INTEGRATION.md; they fill the structs from the *_inst objects,
-  ! convert nothing (filters stay 1-based reference indices) and call endrun
-  ! with the failing column when a call returns CTSM_E_SOLVER.
+  ! A drop-in wrapper for SoilWater (SoilWaterMovementMod.F90:240-243) or
+  ! SoilTemperature (SoilTemperatureMod.F90:92-95) built on this module fills
+  ! the structs from the *_inst objects, converts nothing (filters stay
+  ! 1-based reference indices) and calls endrun with the failing column when
+  ! a call returns CTSM_E_SOLVER.
   !-----------------------------------------------------------------------
   use iso_c_binding
   implicit none

@@ -565,7 +565,10 @@ typedef struct ctsm_phs_stress_state {
  *   per reference routine with intermediates through HBM (the plain kernels of
  *   the *_dev entries' `plain` argument), CanopyFluxes as one launch that
  *   iterates every patch to convergence (no compacted rounds).  The comparison
- *   baseline of the batched step; results are the same. */
+ *   baseline of the batched step; results are the same.
+ * canopy_waves: the register budget of the CanopyFluxes kernel, in waves per
+ *   SIMD (1, 2 or 3: one compiled instance each; a larger budget spills more
+ *   to scratch).  Results do not depend on it. */
 typedef struct ctsm_tuning {
     int32_t richards_lanes;
     int32_t richards_block;
@@ -575,6 +578,7 @@ typedef struct ctsm_tuning {
     int32_t canopy_round_iterations;
     int32_t step_streams;
     int32_t plain_routines;
+    int32_t canopy_waves;
 } ctsm_tuning;
 
 typedef struct ctsm_ctx ctsm_ctx;

@@ -121,22 +121,26 @@ def test_land_step_bitwise(cctx, nsteps, workload):
     _assert_same(g, o)
 
 
-@pytest.mark.parametrize("first,later,itmax", [
-    (41, 3, 40),     # the whole iteration in one pass
-    (6, 3, 40),      # default rounds
-    (1, 1, 40),      # one iteration per round: the most rounds
-    (3, 2, 40),
-    (2, 5, 6),       # itmax reached inside the rounds
+@pytest.mark.parametrize("first,later,itmax,waves", [
+    (41, 3, 40, 1),  # the whole iteration in one pass
+    (6, 3, 40, 1),   # default rounds
+    (1, 1, 40, 1),   # one iteration per round: the most rounds
+    (3, 2, 40, 1),
+    (2, 5, 6, 1),    # itmax reached inside the rounds
+    (6, 3, 40, 2),   # the kernel built for two waves per SIMD
+    (41, 3, 40, 3),  # three waves per SIMD, one pass
+    (2, 5, 6, 3),
 ])
-def test_canopy_rounds_bitwise(cctx, first, later, itmax):
-    """Results do not depend on how the iteration is cut into rounds."""
+def test_canopy_rounds_bitwise(cctx, first, later, itmax, waves):
+    """Results do not depend on how the iteration is cut into rounds or on
+    the register budget the kernel is built for."""
     ctx = cctx(itmax_canopy_fluxes=itmax)
     before = ctx.get_tuning()
     st = S.canopy_workload("canopy_uneven", ncol=512)
     filt = np.arange(st.dims.npatch, 0, -2, dtype=np.int32)       # every other patch, descending
-    print(f"case: CanopyFluxes rounds first={first} later={later} itmax={itmax}, "
+    print(f"case: CanopyFluxes rounds first={first} later={later} itmax={itmax} waves={waves}, "
           f"{len(filt)} of {st.dims.npatch} patches", flush=True)
-    ctx.set_tuning(canopy_first_iterations=first, canopy_round_iterations=later)
+    ctx.set_tuning(canopy_first_iterations=first, canopy_round_iterations=later, canopy_waves=waves)
     try:
         g = st.copy()
         ctx.canopyfluxes(g, filter_p=filt)
