@@ -12,8 +12,15 @@ LIB = os.path.join(HERE, "libctsm_mi355x.so")
 # -ffp-contract=off: no FMA contraction, every expression rounds as the
 #  reference Fortran writes it.
 # -packed-fp32-ops off: no packed FP32 instructions anywhere.
+# -disable-machine-licm: the machine-level loop-invariant pass hoists every
+#  FP64 constant and uniform offset of these long per-column / per-patch
+#  loops in front of them, where they exceed the scalar registers and spill
+#  into vector-register lanes (a v_readlane per use); left in place they are
+#  re-materialised where used.  Measured on the CanopyFluxes kernel: SGPR
+#  spills 550 -> 36, scratch 272 -> 80 bytes per lane.
 FLAGS = ["-O3", "--offload-arch=gfx950", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-mllvm", "-disable-machine-licm",
          "-Wno-unused-command-line-argument"]
 
 

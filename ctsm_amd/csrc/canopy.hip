@@ -297,18 +297,94 @@ __device__ double ci_func(double ci, CiSolve &s)
     return ci - s.cair + s.an * s.forc_pbot * (1.4 * s.gs_mol + 1.6 * s.gb_mol) / (s.gb_mol * s.gs_mol);
 }
 
-// PhotosynthesisMod.F90:2371-2485
-__device__ double brent(double x1, double x2, double f1, double f2, double tol, CiSolve &s)
+// hybrid (PhotosynthesisMod.F90:2251-2368) with its Brent fallback (brent,
+// :2371-2485), as one loop with a single evaluation of ci_func per pass: each
+// pass evaluates ci_func at the point the previous pass chose, then the
+// stage the lane is in (the two starting points, a secant step, the Brent
+// iteration, the re-evaluation at the best point) takes the result and
+// chooses the next point or ends.  Each lane makes exactly the reference's
+// sequence of evaluations and arithmetic; lanes in different stages share the
+// evaluation (only the ci_func side effects matter to the caller).
+__device__ void hybrid(double x0, CiSolve &s)
 {
-    const double eps = 1.e-2;
-    double a = x1, b = x2, fa = f1, fb = f2, c, fc, d = 0.0, e = 0.0;
-    if ((fa > 0.0 && fb > 0.0) || (fa < 0.0 && fb < 0.0)) {
-        s.bad = true;
-        return b;
-    }
-    c = b;
-    fc = fb;
-    for (int iter = 0; iter < 20; ++iter) {
+    const double eps = 1.e-2, eps1 = 1.e-4;
+    enum { F_X0, F_X1, SECANT, BRENT, LAST };
+    int stage = F_X0, iter = 0, biter = 0;
+    double f0 = 0.0, x1 = 0.0, f1 = 0.0, minx = x0, minf = 0.0, tol = 0.0;
+    double a = 0.0, b = 0.0, c = 0.0, fa = 0.0, fb = 0.0, fc = 0.0, d = 0.0, e = 0.0;
+    double xe = x0;
+    for (;;) {
+        const double f = ci_func(xe, s);
+        bool secant_next = false, brent_next = false;
+        if (stage == F_X0) {
+            f0 = f;
+            if (f0 == 0.0) return;
+            minx = x0;
+            minf = f0;
+            x1 = x0 * 0.99;
+            xe = x1;
+            stage = F_X1;
+            continue;
+        } else if (stage == F_X1) {
+            f1 = f;
+            if (f1 == 0.0) return;
+            if (f1 < minf) {
+                minx = x1;
+                minf = f1;
+            }
+            iter = 1;
+            secant_next = true;
+        } else if (stage == SECANT) {
+            f1 = f;
+            if (f1 < minf) {
+                minx = x1;
+                minf = f1;
+            }
+            if (fabs(f1) <= eps1) return;
+            if (f1 * f0 < 0.0) {
+                // brent(x0, x1, f0, f1, tol): the bracket and its first state
+                a = x0;
+                b = x1;
+                fa = f0;
+                fb = f1;
+                if ((fa > 0.0 && fb > 0.0) || (fa < 0.0 && fb < 0.0)) {
+                    s.bad = true;
+                    return;
+                }
+                c = b;
+                fc = fb;
+                biter = 0;
+                brent_next = true;
+            } else if (iter > 40) {
+                xe = minx;
+                stage = LAST;
+                continue;
+            } else {
+                ++iter;
+                secant_next = true;
+            }
+        } else if (stage == BRENT) {
+            fb = f;
+            if (fb == 0.0) return;
+            ++biter;
+            brent_next = true;
+        } else {
+            return;     // LAST: ci_func at the best point has run
+        }
+        if (secant_next) {
+            const double dx = -f1 * (x1 - x0) / (f1 - f0);
+            const double x = x1 + dx;
+            tol = fabs(x) * eps;
+            if (fabs(dx) < tol) return;
+            x0 = x1;
+            f0 = f1;
+            x1 = x;
+            xe = x1;
+            stage = SECANT;
+            continue;
+        }
+        // one Brent iteration up to its evaluation of ci_func
+        if (biter >= 20) return;
         if ((fb > 0.0 && fc > 0.0) || (fb < 0.0 && fc < 0.0)) {
             c = a;
             fc = fa;
@@ -325,7 +401,7 @@ __device__ double brent(double x1, double x2, double f1, double f2, double tol, 
         }
         const double tol1 = 2.0 * eps * fabs(b) + 0.5 * tol;
         const double xm = 0.5 * (c - b);
-        if (fabs(xm) <= tol1 || fb == 0.0) return b;
+        if (fabs(xm) <= tol1 || fb == 0.0) return;
         if (fabs(e) >= tol1 && fabs(fa) > fabs(fb)) {
             const double ss = fb / fa;
             double p, q;
@@ -355,48 +431,8 @@ __device__ double brent(double x1, double x2, double f1, double f2, double tol, 
         fa = fb;
         if (fabs(d) > tol1) b = b + d;
         else b = b + copysign(fabs(tol1), xm);
-        fb = ci_func(b, s);
-        if (fb == 0.0) break;
-    }
-    return b;
-}
-
-// PhotosynthesisMod.F90:2251-2368 (only the ci_func side effects matter to the caller)
-__device__ void hybrid(double x0, CiSolve &s)
-{
-    const double eps = 1.e-2, eps1 = 1.e-4;
-    double f0 = ci_func(x0, s);
-    if (f0 == 0.0) return;
-    double minx = x0, minf = f0;
-    double x1 = x0 * 0.99;
-    double f1 = ci_func(x1, s);
-    if (f1 == 0.0) return;
-    if (f1 < minf) {
-        minx = x1;
-        minf = f1;
-    }
-    for (int iter = 1;; ++iter) {
-        const double dx = -f1 * (x1 - x0) / (f1 - f0);
-        const double x = x1 + dx;
-        const double tol = fabs(x) * eps;
-        if (fabs(dx) < tol) return;
-        x0 = x1;
-        f0 = f1;
-        x1 = x;
-        f1 = ci_func(x1, s);
-        if (f1 < minf) {
-            minx = x1;
-            minf = f1;
-        }
-        if (fabs(f1) <= eps1) return;
-        if (f1 * f0 < 0.0) {
-            brent(x0, x1, f0, f1, tol, s);
-            return;
-        }
-        if (iter > 40) {
-            ci_func(minx, s);
-            return;
-        }
+        xe = b;
+        stage = BRENT;
     }
 }
 
@@ -426,7 +462,8 @@ struct PsnConst {
 };
 
 // PhotosynthesisMod.F90:1243-2062 for one patch and one phase.
-__device__ void photosynthesis(const CSwitch &sw, const ctsm_canopy_params &P, const PsnConst &K, const TFac &T,
+template <class PRM>
+__device__ void photosynthesis(const CSwitch &sw, const PRM &P, const PsnConst &K, const TFac &T,
                                int nrad, double par_z, double lai_z, double vcmaxcint, double o3v, double o3g,
                                bool noon, double forc_pbot, double tgcm, double esat_tv, double eair, double oair,
                                double cair, double rb, double btran, Phase &ph, Shared &sh, bool &bad)
@@ -590,7 +627,8 @@ namespace ctsm {
 namespace canopy {
 
 // constants of the time step for one patch (PhotosynthesisMod.F90:1525-1722)
-__device__ void psn_constants(const CSwitch &sw, const ctsm_canopy_params &P, const double *__restrict__ pft, int ivt,
+template <class PRM>
+__device__ void psn_constants(const CSwitch &sw, const PRM &P, const double *__restrict__ pft, int ivt,
                               double forc_pbot, double oair, double t10, double btran, double dayl_factor,
                               PsnConst &K, bool &bad)
 {
@@ -632,7 +670,8 @@ __device__ void psn_constants(const CSwitch &sw, const ctsm_canopy_params &P, co
     K.mint = pft[F_MEDINT * CTSM_MAX_PFT + ivt];
 }
 
-__device__ void temp_factors(const ctsm_canopy_params &P, const PsnConst &K, double tv, bool anyday, TFac &T)
+template <class PRM>
+__device__ void temp_factors(const PRM &P, const PsnConst &K, double tv, bool anyday, TFac &T)
 {
     T.kc = K.kc25 * ft(tv, P.kcha);
     T.ko = K.ko25 * ft(tv, P.koha);
@@ -675,10 +714,11 @@ static_assert(sizeof(CanopyWork) == N_WORK_FIELDS * sizeof(double *), "one point
 // Monin-Obukhov initialisation); otherwise the iteration state is restored
 // from W.  Returns true when the patch is still iterating (its state saved in
 // W[i0]); false when it finished (outputs written) or failed.
-__device__ __forceinline__ bool canopy_patch(const CView &V, const CSwitch &sw, const ctsm_canopy_params &P,
-                             const double *__restrict__ pft, const ctsm_col_state &col, const ctsm_canopy_state &S,
+template <class PRM, class COL, class CAN, class WRK>
+__device__ __forceinline__ bool canopy_patch(const CView &V, const CSwitch &sw, const PRM &P,
+                             const double *__restrict__ pft, const COL &col, const CAN &S,
                              int i0, const int32_t *__restrict__ filter, double dtime, ErrWord *err, bool first,
-                             int budget, const CanopyWork &W)
+                             int budget, const WRK &W)
 {
     const int p = filter ? filter[i0] : V.begp + i0;
     const long pp = V.p1(p);
@@ -876,7 +916,6 @@ __device__ __forceinline__ bool canopy_patch(const CView &V, const CSwitch &sw, 
         obu = W.obu[i0];
         nmozsgn = (int)W.nmozsgn[i0];
     }
-    double num_iter = 0.0;
     const double tl_ini = S.t_veg[pp];
     double t_stem = S.t_stem[pp];
     const double ts_ini = t_stem;
@@ -886,40 +925,23 @@ __device__ __forceinline__ bool canopy_patch(const CView &V, const CSwitch &sw, 
     const double parsun = S.parsun_z[pp], parsha = S.parsha_z[pp];
     const bool noon = S.near_local_noon[gg] != 0;
     const bool anyday = nrad >= 1 && (parsun > 0.0 || parsha > 0.0);
+    const double lai_z_sun = S.laisun_z[pp], lai_z_sha = S.laisha_z[pp];
+    const double vcint_sun = S.vcmaxcintsun[pp], vcint_sha = S.vcmaxcintsha[pp];
+    const double o3v_sun = S.o3coefvsun[pp], o3v_sha = S.o3coefvsha[pp];
+    const double o3g_sun = S.o3coefgsun[pp], o3g_sha = S.o3coefgsha[pp];
     PsnConst K;
     psn_constants(sw, P, pft, ivt, forc_pbot, o2, S.t_a10[pp], btran, dayl_factor, K, bad);
-    Phase sun, sha;
-    Shared sh;
-    sh.gs_mol = first ? S.gs_mol[pp] : W.gs_mol[i0];
-    sh.rh_leaf = S.rh_leaf[pp];
-    sh.vpd_can = S.vpd_can[pp];
-    sun.gs_ln = S.gs_mol_sun_ln[pp];
-    sha.gs_ln = S.gs_mol_sha_ln[pp];
-    if (nrad < 1) {
-        // nothing below overwrites these: keep the stored values
-        sh.vcmax_z = S.vcmax_z[pp]; sh.tpu_z = S.tpu_z[pp]; sh.kp_z = S.kp_z[pp];
-        sh.ac = S.ac[pp]; sh.aj = S.aj[pp]; sh.ap = S.ap[pp]; sh.ag = S.ag[pp]; sh.an = S.an[pp];
-        sun.rs_z = S.rssun_z[pp]; sun.ci_z = S.cisun_z[pp]; sun.lmr_z = S.lmrsun_z[pp]; sun.psn_z = S.psnsun_z[pp];
-        sha.rs_z = S.rssha_z[pp]; sha.ci_z = S.cisha_z[pp]; sha.lmr_z = S.lmrsha_z[pp]; sha.psn_z = S.psnsha_z[pp];
-        sun.gs_ph = S.gs_mol_sun[pp];
-        sha.gs_ph = S.gs_mol_sha[pp];
-    }
-    TFac T;
-
-    // ---- iteration state
-    double del = first ? 0.0 : W.del[i0], del2 = 0.0, efeb = first ? 0.0 : W.efeb[i0], efe = 0.0;
+    // The iteration carries only its state from one iteration to the next
+    // (and, through W, from round to round); every value the outputs read
+    // after the last iteration is that iteration's own, so nothing else stays
+    // live across the loop.  Photosynthesis writes some of its outputs only
+    // on branches fixed for the step (nrad, par, the stomatal model): those
+    // are stored under the same conditions (otherwise the arrays keep their
+    // values, as in the reference).
+    double gs_mol_c = first ? S.gs_mol[pp] : W.gs_mol[i0];
+    double del = first ? 0.0 : W.del[i0], efeb = first ? 0.0 : W.efeb[i0];
     double obuold = first ? 0.0 : W.obuold[i0];
-    double wtlq0 = 0.0, wtalq = 0.0, wtgq = 0.0, wtaq0 = 0.0;
-    double tlbef = t_veg, dt_veg = 0.0, errp = 0.0;
-    double ustar = 0.0, temp1 = 0.0, temp2 = 0.0, temp12m = 0.0, temp22m = 0.0;
-    double ram1 = 0.0, rb = 0.0, rah1 = 0.0, rah2 = 0.0, raw1 = 0.0, raw2 = 0.0, uaf = 0.0;
-    double wtg = 0.0, wta0 = 0.0, wtl0 = 0.0, wtstem0 = 0.0, wtal = 0.0, wtga = 0.0;
-    double lw_stem = 0.0, lw_leaf = 0.0, eflx_sh_veg = 0.0, eflx_sh_stem = 0.0;
-    double qflx_tran_veg = S.qflx_tran_veg[pp], qflx_evap_veg = 0.0, zeta = 0.0, svpts = 0.0, eah = 0.0;
-    double rh_af = 0.0, vpd = 0.0;
-    FV fv;
-    fv.fm = first ? 0.0 : W.fm[i0];
-    double obu_fv = obu, um_fv = um;      // the inputs of the last FrictionVelocity call
+    double fm_c = first ? 0.0 : W.fm[i0];
     const double laisun = S.laisun[pp], laisha = S.laisha[pp], fdry = S.fdry[pp], fwet = S.fwet[pp];
     const double liqcan0 = S.liqcan[pp], snocan0 = S.snocan[pp];
     const double h2ocan = liqcan0 + snocan0;
@@ -947,10 +969,42 @@ __device__ __forceinline__ bool canopy_patch(const CView &V, const CSwitch &sw, 
     fvlog.lq = cm_log((hq - displa) / z0qv);
 
     int itlef = first ? 0 : (int)W.itlef[i0];
-    bool finished = false;
-    for (int it = 0; it < budget && itlef <= sw.itmax; ++it) {
+    // values of the iteration that the outputs read: the loop leaves for the
+    // outputs only after a whole iteration, so each holds that iteration's value
+    double ustar, temp1, temp2, tlbef, del2, ram1, rah1, raw1, uaf, rb, rah2, raw2, svpts, eah, rh_af, vpd;
+    double wtg, wtl0, wta0, wtstem0, wtga, wtal, lw_stem, lw_leaf, qflx_tran_veg, wtlq0, wtaq0, wtalq;
+    double wtgq, eflx_sh_stem, efe, dt_veg, errp, qflx_evap_veg, eflx_sh_veg, zeta, num_iter;
+    double obu_fv, um_fv;
+    FV fv;
+    TFac T;
+    Phase sun, sha;
+    Shared sh;
+    for (int it = 0;; ++it) {
+        if (it >= budget) {
+            // still iterating: keep the state for the next round
+            W.t_veg[i0] = t_veg;
+            W.del[i0] = del;
+            W.efeb[i0] = efeb;
+            W.obuold[i0] = obuold;
+            W.nmozsgn[i0] = (double)nmozsgn;
+            W.um[i0] = um;
+            W.obu[i0] = obu;
+            W.fm[i0] = fm_c;
+            W.qsatl[i0] = qsatl;
+            W.el[i0] = el;
+            W.qsatldT[i0] = qsatldT;
+            W.taf[i0] = taf;
+            W.qaf[i0] = qaf;
+            W.delq[i0] = delq;
+            W.itlef[i0] = (double)itlef;
+            W.gs_mol[i0] = gs_mol_c;
+            if (bad) report_error(err, p, ENDRUN);
+            return true;
+        }
+        wtgq = 0.0;
+        fv.fm = fm_c;
         friction_velocity(hu, ht, hq, displa, z0mv, z0hv, z0qv, fvlog, obu, itlef + 1, um, fv);
-        obu_fv = obu;
+        obu_fv = obu;       // the inputs of this FrictionVelocity call
         um_fv = um;
         ustar = fv.ustar;
         temp1 = fv.temp1;
@@ -982,11 +1036,22 @@ __device__ __forceinline__ bool canopy_patch(const CView &V, const CSwitch &sw, 
         vpd = fmax90((svpts - eah), 50.0) * 0.001;
 
         // Photosynthesis, sun then shade
+        sun = Phase{};
+        sha = Phase{};
+        sh = Shared{};
+        sh.gs_mol = gs_mol_c;
         temp_factors(P, K, t_veg, anyday, T);
-        photosynthesis(sw, P, K, T, nrad, parsun, S.laisun_z[pp], S.vcmaxcintsun[pp], S.o3coefvsun[pp],
-                       S.o3coefgsun[pp], noon, forc_pbot, thm, svpts, eah, o2, co2, rb, btran, sun, sh, bad);
-        photosynthesis(sw, P, K, T, nrad, parsha, S.laisha_z[pp], S.vcmaxcintsha[pp], S.o3coefvsha[pp],
-                       S.o3coefgsha[pp], noon, forc_pbot, thm, svpts, eah, o2, co2, rb, btran, sha, sh, bad);
+        // one call site for both phases: sun (k = 0), then shade
+#pragma nounroll
+        for (int k = 0; k < 2; ++k) {
+            const bool ks = k == 0;
+            Phase ph{};
+            photosynthesis(sw, P, K, T, nrad, ks ? parsun : parsha, ks ? lai_z_sun : lai_z_sha,
+                           ks ? vcint_sun : vcint_sha, ks ? o3v_sun : o3v_sha, ks ? o3g_sun : o3g_sha, noon,
+                           forc_pbot, thm, svpts, eah, o2, co2, rb, btran, ph, sh, bad);
+            if (ks) sun = ph;
+            else sha = ph;
+        }
 
         // conductances and the leaf-temperature Newton step (:1119-1331)
         const double wta = 1.0 / rah1;
@@ -1098,43 +1163,25 @@ __device__ __forceinline__ bool canopy_patch(const CView &V, const CSwitch &sw, 
         if (nmozsgn >= 4) obu = zldis / (-0.01);
         obuold = obu;
 
+        gs_mol_c = sh.gs_mol;
+        fm_c = fv.fm;
         itlef = itlef + 1;
+        bool converged = false;
+        num_iter = 0.0;
         if (itlef > itmin) {
             const double dele = fabs(efe - efeb);
             efeb = efe;
             const double det = fmax90(del, del2);
             num_iter = (double)itlef;
-            if (det < dtmin && dele < dlemin) {
-                finished = true;
-                break;
-            }
+            converged = det < dtmin && dele < dlemin;
         }
-    }
-    if (!finished && itlef <= sw.itmax) {
-        // still iterating: keep the state for the next round
-        W.t_veg[i0] = t_veg;
-        W.del[i0] = del;
-        W.efeb[i0] = efeb;
-        W.obuold[i0] = obuold;
-        W.nmozsgn[i0] = (double)nmozsgn;
-        W.um[i0] = um;
-        W.obu[i0] = obu;
-        W.fm[i0] = fv.fm;
-        W.qsatl[i0] = qsatl;
-        W.el[i0] = el;
-        W.qsatldT[i0] = qsatldT;
-        W.taf[i0] = taf;
-        W.qaf[i0] = qaf;
-        W.delq[i0] = delq;
-        W.itlef[i0] = (double)itlef;
-        W.gs_mol[i0] = sh.gs_mol;
-        if (bad) report_error(err, p, ENDRUN);
-        return true;
+        if (converged || itlef > sw.itmax) break;
     }
 
+    // ---- the last iteration is done: the fluxes and outputs of the step
     friction_velocity_diag(hu, displa, z0mv, z0hv, z0qv, obu_fv, ur, um_fv, fv);
-    temp12m = fv.temp12m;
-    temp22m = fv.temp22m;
+    const double temp12m = fv.temp12m;
+    const double temp22m = fv.temp22m;
 
     // ---- fluxes after the iteration (:1467-1615)
     const double fr = frac_rad_abs_by_stem;
@@ -1264,77 +1311,117 @@ __device__ __forceinline__ bool canopy_patch(const CView &V, const CSwitch &sw, 
     S.ko[pp] = T.ko;
     S.cp[pp] = T.cp;
     S.lnca[pp] = K.lnc;
-    S.vcmax_z[pp] = sh.vcmax_z;
-    S.tpu_z[pp] = sh.tpu_z;
-    S.kp_z[pp] = sh.kp_z;
     S.gb_mol[pp] = sh.gb_mol;
-    S.ac[pp] = sh.ac;
-    S.aj[pp] = sh.aj;
-    S.ap[pp] = sh.ap;
-    S.ag[pp] = sh.ag;
-    S.an[pp] = sh.an;
     S.gs_mol[pp] = sh.gs_mol;
-    S.rh_leaf[pp] = sh.rh_leaf;
-    S.vpd_can[pp] = sh.vpd_can;
+    const bool medlyn = sw.stomatal == CTSM_STOMATA_MEDLYN2011;
+    const bool lit_sun = nrad >= 1 && parsun > 0.0, lit_sha = nrad >= 1 && parsha > 0.0;
+    if (nrad >= 1) {
+        // written by both phases' canopy layer whether lit or not
+        S.vcmax_z[pp] = sh.vcmax_z;
+        S.tpu_z[pp] = sh.tpu_z;
+        S.kp_z[pp] = sh.kp_z;
+        S.ac[pp] = sh.ac;
+        S.aj[pp] = sh.aj;
+        S.ap[pp] = sh.ap;
+        S.ag[pp] = sh.ag;
+        S.an[pp] = sh.an;
+        S.rssun_z[pp] = sun.rs_z;
+        S.rssha_z[pp] = sha.rs_z;
+        S.cisun_z[pp] = sun.ci_z;
+        S.cisha_z[pp] = sha.ci_z;
+        S.lmrsun_z[pp] = sun.lmr_z;
+        S.lmrsha_z[pp] = sha.lmr_z;
+        S.psnsun_z[pp] = sun.psn_z;
+        S.psnsha_z[pp] = sha.psn_z;
+        S.gs_mol_sun[pp] = sun.gs_ph;
+        S.gs_mol_sha[pp] = sha.gs_ph;
+        // rh_leaf: a dark phase writes 0, a lit one under Ball-Berry its
+        // value; vpd_can: a lit phase under Medlyn (the last writer wins)
+        if (!medlyn || !lit_sun || !lit_sha) S.rh_leaf[pp] = sh.rh_leaf;
+        if (medlyn && (lit_sun || lit_sha)) S.vpd_can[pp] = sh.vpd_can;
+    }
+    if (lit_sun) S.gs_mol_sun_ln[pp] = sun.gs_ln;
+    if (lit_sha) S.gs_mol_sha_ln[pp] = sha.gs_ln;
     S.rssun[pp] = sun.rs;
     S.rssha[pp] = sha.rs;
-    S.rssun_z[pp] = sun.rs_z;
-    S.rssha_z[pp] = sha.rs_z;
-    S.cisun_z[pp] = sun.ci_z;
-    S.cisha_z[pp] = sha.ci_z;
     S.lmrsun[pp] = sun.lmr;
     S.lmrsha[pp] = sha.lmr;
-    S.lmrsun_z[pp] = sun.lmr_z;
-    S.lmrsha_z[pp] = sha.lmr_z;
     S.psnsun[pp] = sun.psn;
     S.psnsha[pp] = sha.psn;
-    S.psnsun_z[pp] = sun.psn_z;
-    S.psnsha_z[pp] = sha.psn_z;
     S.psnsun_wc[pp] = sun.psn_wc;
     S.psnsun_wj[pp] = sun.psn_wj;
     S.psnsun_wp[pp] = sun.psn_wp;
     S.psnsha_wc[pp] = sha.psn_wc;
     S.psnsha_wj[pp] = sha.psn_wj;
     S.psnsha_wp[pp] = sha.psn_wp;
-    S.gs_mol_sun[pp] = sun.gs_ph;
-    S.gs_mol_sha[pp] = sha.gs_ph;
-    S.gs_mol_sun_ln[pp] = sun.gs_ln;
-    S.gs_mol_sha_ln[pp] = sha.gs_ln;
     if (bad) report_error(err, p, ENDRUN);
     return false;
 }
 
+// Everything one launch of k_canopy_round reads: the kernel's only argument.
+struct RoundArgs {
+    CView V;
+    CSwitch sw;
+    ctsm_canopy_params P;
+    const double *pft;
+    ctsm_col_state col;
+    ctsm_canopy_state S;
+    int n;
+    const int32_t *filter;
+    double dtime;
+    ErrWord *err;
+    int round, budget;
+    const int32_t *list_in, *cnt_in;
+    int32_t *list_out, *cnt_out;
+    CanopyWork W;
+};
+#define KARG __attribute__((address_space(4)))
+
 // One round of CanopyFluxes over the patches still iterating (round 0: the
 // whole filter), compacting those that continue into list_out with one atomic
-// per wavefront.  Blocks are one wavefront, a patch per lane; the grid covers
-// the whole filter and waves past the round's count return at once.
+// per wavefront.  Blocks are one wavefront, a patch per lane.  Round 0
+// (STRIDE false) covers the filter with its grid; later rounds (STRIDE true)
+// stride a capped grid over the round's count.
+// The argument is read in place from the kernel-argument segment, each member
+// where it is used: loaded all at once on entry, its few hundred uniform
+// pointers would not fit the scalar registers and spill.  In the striding
+// form the segment pointer is made opaque on every pass, so those loads stay
+// inside the pass instead of being hoisted in front of the loop.
 // WAVES: the register budget the kernel is compiled for (waves per SIMD);
 // above one wave the compiler spills what does not fit to scratch.  The
 // launch picks the instance (a launch shape the autotuner times).
-template <int WAVES>
+template <int WAVES, bool STRIDE>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES)))
-k_canopy_round(CView V, CSwitch sw, ctsm_canopy_params P, const double *__restrict__ pft, ctsm_col_state col,
-               ctsm_canopy_state S, int n, const int32_t *__restrict__ filter, double dtime, ErrWord *err,
-               int round, int budget, const int32_t *__restrict__ list_in, const int32_t *__restrict__ cnt_in,
-               int32_t *__restrict__ list_out, int32_t *__restrict__ cnt_out, CanopyWork W)
+k_canopy_round(RoundArgs args)
 {
-    const int count = (round == 0) ? n : *cnt_in;
+    const KARG RoundArgs *Ap = (const KARG RoundArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+    const int round = Ap->round;
+    const int count = (round == 0) ? Ap->n : *Ap->cnt_in;
     const int lane = threadIdx.x;
-    const int idx = blockIdx.x * 64 + lane;
-    if (blockIdx.x * 64 >= count) return;       // whole wave past the list
-    bool cont = false;
-    int i0 = 0;
-    if (idx < count) {
-        i0 = (round == 0) ? idx : list_in[idx];
-        cont = canopy_patch(V, sw, P, pft, col, S, i0, filter, dtime, err, round == 0, budget, W);
-    }
-    const unsigned long long mask = __ballot(cont);
-    if (mask) {
-        const int leader = __ffsll((unsigned long long)mask) - 1;
-        int pos = 0;
-        if (lane == leader) pos = atomicAdd(cnt_out, __popcll(mask));
-        pos = __shfl(pos, leader);
-        if (cont) list_out[pos + __popcll(mask & ((1ull << lane) - 1ull))] = i0;
+    for (int base = blockIdx.x * 64; base < count; base += gridDim.x * 64) {
+        const KARG RoundArgs *A = Ap;
+        if (STRIDE) asm volatile("" : "+s"(A));
+        const int idx = base + lane;
+        bool cont = false;
+        int i0 = 0;
+        if (idx < count) {
+            i0 = (round == 0) ? idx : A->list_in[idx];
+            const CView V{A->V.begc, A->V.ldc, A->V.begp, A->V.ldp, A->V.begg, A->V.ldg, A->V.nlevsno,
+                          A->V.nlevsoi, A->V.nlevgrnd};
+            const CSwitch sw{A->sw.z0param, A->sw.stomatal, A->sw.soil_resis, A->sw.biomass, A->sw.undercanopy,
+                             A->sw.light_inhibit, A->sw.itmax};
+            cont = canopy_patch(V, sw, A->P, A->pft, A->col, A->S, i0, A->filter, A->dtime, A->err, round == 0,
+                                A->budget, A->W);
+        }
+        const unsigned long long mask = __ballot(cont);
+        if (mask) {
+            const int leader = __ffsll((unsigned long long)mask) - 1;
+            int pos = 0;
+            if (lane == leader) pos = atomicAdd(A->cnt_out, __popcll(mask));
+            pos = __shfl(pos, leader);
+            if (cont) A->list_out[pos + __popcll(mask & ((1ull << lane) - 1ull))] = i0;
+        }
+        if (!STRIDE) break;
     }
 }
 
@@ -1518,7 +1605,10 @@ void launch_canopy(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg,
                    bool zero_cgrnd, int waves)
 {
     using namespace canopy;
-    auto round_kernel = waves == 3 ? k_canopy_round<3> : waves == 2 ? k_canopy_round<2> : k_canopy_round<1>;
+    auto first_kernel = waves == 3 ? k_canopy_round<3, false> : waves == 2 ? k_canopy_round<2, false>
+                                                                          : k_canopy_round<1, false>;
+    auto later_kernel = waves == 3 ? k_canopy_round<3, true> : waves == 2 ? k_canopy_round<2, true>
+                                                                         : k_canopy_round<1, true>;
     CView V{L.begc_alloc, L.ldc, L.begp_alloc, L.ldp, L.begg_alloc, L.ldg, cfg.nlevsno, cfg.nlevsoi, cfg.nlevgrnd};
     CSwitch sw{cfg.z0param_method, cfg.stomatalcond_method, cfg.soil_resis_method, cfg.use_biomass_heat_storage,
                cfg.use_undercanopy_stability, cfg.light_inhibit, cfg.itmax_canopy_fluxes};
@@ -1529,26 +1619,34 @@ void launch_canopy(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg,
         hipLaunchKernelGGL(k_zero_cgrnd, dim3((n + 255) / 256), dim3(256), 0, st, n, filter, L.begp_alloc, can);
     const int R = work ? canopy_later_rounds(cfg.itmax_canopy_fluxes, first_its, round_its) : 0;
     const int blocks = (n + 63) / 64;
+    RoundArgs a{V, sw, prm, pft_dev, col, can, n, filter, dtime, err, 0, 0, nullptr, nullptr, nullptr, nullptr,
+                CanopyWork{}};
     if (R == 0) {
         // one pass: every patch finishes (converged or itmax reached) in round 0
-        hipLaunchKernelGGL(round_kernel, dim3(blocks), dim3(64), 0, st, V, sw, prm, pft_dev, col, can, n, filter,
-                           dtime, err, 0, cfg.itmax_canopy_fluxes + 1, (const int32_t *)nullptr,
-                           (const int32_t *)nullptr, (int32_t *)nullptr, (int32_t *)nullptr, CanopyWork{});
+        a.budget = cfg.itmax_canopy_fluxes + 1;
+        hipLaunchKernelGGL(first_kernel, dim3(blocks), dim3(64), 0, st, a);
         return;
     }
     double *w = (double *)work;
     auto field = [&](int k) { return w + (size_t)k * n; };
-    const CanopyWork W{field(0), field(1), field(2),  field(3),  field(4),  field(5),  field(6),  field(7),
-                       field(8), field(9), field(10), field(11), field(12), field(13), field(14), field(15)};
+    a.W = CanopyWork{field(0), field(1), field(2),  field(3),  field(4),  field(5),  field(6),  field(7),
+                     field(8), field(9), field(10), field(11), field(12), field(13), field(14), field(15)};
     int32_t *list[2] = {(int32_t *)(w + (size_t)N_WORK_FIELDS * n), nullptr};
     list[1] = list[0] + n;
     int32_t *cnt = list[1] + n;
     hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, st, cnt, R + 1);
+    // later rounds stride over the patches still iterating with a grid that
+    // keeps every wave slot of the chip busy
+    const int later_blocks = blocks < 8192 ? blocks : 8192;
     for (int r = 0; r <= R; ++r) {
-        const int budget = r == 0 ? first_its : round_its;
-        hipLaunchKernelGGL(round_kernel, dim3(blocks), dim3(64), 0, st, V, sw, prm, pft_dev,
-                           col, can, n, filter, dtime, err, r, budget, (const int32_t *)list[(r + 1) & 1],
-                           (const int32_t *)(r ? cnt + r - 1 : nullptr), list[r & 1], cnt + r, W);
+        a.round = r;
+        a.budget = r == 0 ? first_its : round_its;
+        a.list_in = list[(r + 1) & 1];
+        a.cnt_in = r ? cnt + r - 1 : nullptr;
+        a.list_out = list[r & 1];
+        a.cnt_out = cnt + r;
+        if (r == 0) hipLaunchKernelGGL(first_kernel, dim3(blocks), dim3(64), 0, st, a);
+        else hipLaunchKernelGGL(later_kernel, dim3(later_blocks), dim3(64), 0, st, a);
     }
 }
 
