@@ -360,8 +360,14 @@ __global__ void k_reset_err(ErrWord *e)
 
 }  // namespace
 
+// Scratch of ctsm_canopyfluxes_dev_noalloc (the compacted rounds' state,
+// lists and counters).  In host code `scratch` is only a handle: kernels get
+// the per-device address ctsm_create resolves with hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[CTSM_NOALLOC_SCRATCH_BYTES];
+
 struct ctsm_ctx {
     int device = 0;
+    void *scratch_dev = nullptr;      // this device's copy of `scratch`
     ctsm_config cfg{};
     Params prm{};
     ctsm_tuning tun{4, 128, 128, 0, 6, 3, 4, 0, 1};
@@ -642,7 +648,8 @@ int ctsm_create(const ctsm_config *cfg, int device, ctsm_ctx **out)
     ctx->device = device;
     ctx->cfg = *cfg;
     ctx->prm = make_params(*cfg);
-    bool ok = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) == hipSuccess &&
+    bool ok = hipGetSymbolAddress(&ctx->scratch_dev, HIP_SYMBOL(scratch)) == hipSuccess && ctx->scratch_dev &&
+              hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) == hipSuccess &&
               hipMalloc((void **)&ctx->err, sizeof(ErrWord)) == hipSuccess &&
               hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) == hipSuccess;
     for (int k = 0; ok && k < ctsm_ctx::kMaxStreams - 1; ++k)
@@ -946,19 +953,31 @@ static int check_canopy_layout(const ctsm_ctx *ctx, const ctsm_layout *L)
 }
 
 // Grows the canopy round buffer to n patches and launches CanopyFluxes.
+// With fixed_work, the rounds use that buffer of fixed_bytes instead and
+// nothing is allocated (CTSM_E_SCRATCH when it is too small).
 static int run_canopy(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state &col, const ctsm_canopy_state &can,
-                      int32_t n, const int32_t *filter_dev, double dtime, bool zero_cgrnd)
+                      int32_t n, const int32_t *filter_dev, double dtime, bool zero_cgrnd,
+                      void *fixed_work = nullptr, size_t fixed_bytes = 0)
 {
     const bool one_pass = ctx->tun.plain_routines != 0;
     const int k0 = one_pass ? 1001 : ctx->tun.canopy_first_iterations;
     const int k = one_pass ? 1 : ctx->tun.canopy_round_iterations;
     const size_t wb = canopy_work_bytes(n, ctx->cfg.itmax_canopy_fluxes, k0, k);
-    if (wb) {
+    void *work = nullptr;
+    if (wb && fixed_work) {
+        if (wb > fixed_bytes) {
+            ctx->last_code = CTSM_E_SCRATCH;
+            ctx->last_index = (int32_t)std::min<size_t>((wb + 1023) / 1024, INT32_MAX);
+            return CTSM_E_SCRATCH;
+        }
+        work = fixed_work;
+    } else if (wb) {
         hipError_t he = ctx->cwork.ensure(wb);
         if (he != hipSuccess) return hip_fail(ctx, he);
+        work = ctx->cwork.p;
     }
     launch_canopy(ctx->stream, *lay, ctx->cfg, ctx->cprm, (const double *)ctx->pft.p, col, can, n, filter_dev, dtime,
-                  ctx->err, wb ? ctx->cwork.p : nullptr, k0, k, zero_cgrnd, ctx->tun.canopy_waves);
+                  ctx->err, work, k0, k, zero_cgrnd, ctx->tun.canopy_waves);
     return CTSM_OK;
 }
 
@@ -968,6 +987,18 @@ int ctsm_canopyfluxes_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_
     if (!ctx || !col || !can || n < 0 || check_canopy_layout(ctx, lay) || (!filterp_dev && n > lay->ldp))
         return CTSM_E_ARG;
     int rc = run_canopy(ctx, lay, *col, *can, n, filterp_dev, dtime, false);
+    if (rc) return rc;
+    return finish_launch(ctx);
+}
+
+int ctsm_canopyfluxes_dev_noalloc(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                                  const ctsm_canopy_state *can, int32_t n, const int32_t *filterp_dev, double dtime)
+{
+    if (!ctx || !col || !can || n < 0 || check_canopy_layout(ctx, lay) || (!filterp_dev && n > lay->ldp))
+        return CTSM_E_ARG;
+    if (!ctx->scratch_dev) return CTSM_E_HIP;
+    int rc = run_canopy(ctx, lay, *col, *can, n, filterp_dev, dtime, false, ctx->scratch_dev,
+                        CTSM_NOALLOC_SCRATCH_BYTES);
     if (rc) return rc;
     return finish_launch(ctx);
 }

@@ -18,9 +18,11 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libctsm_mi355x.so")
 _LIB = None
 
-CTSM_OK, CTSM_E_ARG, CTSM_E_CONFIG, CTSM_E_HIP, CTSM_E_SOLVER, CTSM_E_NODEVICE, CTSM_E_ENDRUN, CTSM_E_IO = range(8)
+(CTSM_OK, CTSM_E_ARG, CTSM_E_CONFIG, CTSM_E_HIP, CTSM_E_SOLVER, CTSM_E_NODEVICE, CTSM_E_ENDRUN, CTSM_E_IO,
+ CTSM_E_SCRATCH) = range(9)
 _NAMES = {1: "CTSM_E_ARG", 2: "CTSM_E_CONFIG", 3: "CTSM_E_HIP", 4: "CTSM_E_SOLVER", 5: "CTSM_E_NODEVICE",
-          6: "CTSM_E_ENDRUN", 7: "CTSM_E_IO"}
+          6: "CTSM_E_ENDRUN", 7: "CTSM_E_IO", 8: "CTSM_E_SCRATCH"}
+NOALLOC_SCRATCH_BYTES = 128 << 20      # CTSM_NOALLOC_SCRATCH_BYTES
 
 # every symbol declared in include/ctsm_gpu.h
 EXPORTS = [
@@ -39,7 +41,7 @@ EXPORTS = [
     "ctsm_checkpoint_write", "ctsm_checkpoint_read", "ctsm_drainage", "ctsm_drainage_dev",
     "ctsm_graph_capture_full", "ctsm_set_phs_params", "ctsm_phs_soil_root_conductance",
     "ctsm_phs_soil_root_conductance_dev", "ctsm_phs_root_sink", "ctsm_phs_root_sink_dev",
-    "ctsm_phs_calcstress", "ctsm_phs_calcstress_dev",
+    "ctsm_phs_calcstress", "ctsm_phs_calcstress_dev", "ctsm_canopyfluxes_dev_noalloc",
 ]
 
 
@@ -130,6 +132,7 @@ def load():
     L.ctsm_canopyfluxes.argtypes = can + [C.c_int32, abi.i32p, C.c_double]
     L.ctsm_root_sink.argtypes = can + [C.c_int32, abi.i32p]
     L.ctsm_canopyfluxes_dev.argtypes = can + [C.c_int32, C.c_void_p, C.c_double]
+    L.ctsm_canopyfluxes_dev_noalloc.argtypes = can + [C.c_int32, C.c_void_p, C.c_double]
     L.ctsm_root_sink_dev.argtypes = can + [C.c_int32, C.c_void_p]
     L.ctsm_land_step_dev.argtypes = col + [P(abi.PatchState), P(abi.CanopyState), C.c_int32, C.c_void_p,
                                            C.c_int32, C.c_void_p, C.c_double]
@@ -247,6 +250,19 @@ class Context:
         fp = C.c_void_p(filterp_dev) if filterp_dev is not None else None
         self._check(self.L.ctsm_canopyfluxes_dev(self.h, C.byref(ds.layout()), C.byref(cs), C.byref(can), n, fp,
                                                  dtime), "CanopyFluxes")
+
+    def canopyfluxes_dev_noalloc(self, ds, filterp_dev=None, n=None, dtime=1800.0, structs=None):
+        """canopyfluxes_dev through the entry that allocates nothing (its
+        scratch is the library's fixed per-device array)."""
+        if structs is None:
+            cs, _ = ds.structs()
+            can = ds.canopy_struct()
+        else:
+            cs, _, can = structs
+        n = ds.dims.npatch if n is None else n
+        fp = C.c_void_p(filterp_dev) if filterp_dev is not None else None
+        self._check(self.L.ctsm_canopyfluxes_dev_noalloc(self.h, C.byref(ds.layout()), C.byref(cs), C.byref(can), n,
+                                                         fp, dtime), "CanopyFluxes (no allocation)")
 
     def land_step_dev(self, ds, filterc_dev=None, nc=None, filterp_dev=None, np_=None, dtime=1800.0, structs=None):
         """One land step with canopy on a DeviceState (enqueues only)."""

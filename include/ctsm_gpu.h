@@ -59,6 +59,8 @@ extern "C" {
 #define CTSM_E_ENDRUN      6   /* a check where the reference calls endrun fails   */
 #define CTSM_E_IO          7   /* checkpoint file cannot be written / read, or does
                                   not hold the fields asked for (name, size, checksum) */
+#define CTSM_E_SCRATCH     8   /* the call needs more than the fixed scratch of a
+                                  *_noalloc entry (ctsm_last_error: the KiB needed) */
                                /* (canopy: quadratic, brent bracket, gs < 0, ...)  */
 
 /* soil water lower / upper boundary conditions (SoilWaterMovementMod.F90:55-59) */
@@ -767,6 +769,23 @@ int ctsm_soil_steps_fused_dev(ctsm_ctx *ctx, const ctsm_layout *lay,
 int ctsm_canopyfluxes_dev(ctsm_ctx *ctx, const ctsm_layout *lay,
                           const ctsm_col_state *col, const ctsm_canopy_state *can,
                           int32_t num_filterp, const int32_t *filterp_dev, double dtime);
+/* ctsm_canopyfluxes_dev without allocating: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, hosts that forbid hipMalloc after set-up).
+ * The compacted rounds' iteration state, patch lists and counters live in a
+ * fixed device array of CTSM_NOALLOC_SCRATCH_BYTES that the library holds on
+ * every GPU it runs on until the process ends, instead of the context's
+ * growable buffer; nothing is allocated or freed on the call path.  Results
+ * equal ctsm_canopyfluxes_dev's.  A call that would need more scratch
+ * (canopy_work_bytes: 136 B per patch of the filter plus 4 B per round) returns
+ * CTSM_E_SCRATCH before launching anything, ctsm_last_error's index then being
+ * the KiB it needs; 128 MiB holds about 986,000 patches.  Every call on one
+ * GPU shares that GPU's array: calls that overlap on one GPU (two contexts,
+ * two streams) race, so serialise them. */
+#define CTSM_NOALLOC_SCRATCH_BYTES (128u << 20)
+int ctsm_canopyfluxes_dev_noalloc(ctsm_ctx *ctx, const ctsm_layout *lay,
+                                  const ctsm_col_state *col, const ctsm_canopy_state *can,
+                                  int32_t num_filterp, const int32_t *filterp_dev, double dtime);
 int ctsm_root_sink_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
                        const ctsm_canopy_state *can,
                        int32_t num_filterc, const int32_t *filterc_dev);
