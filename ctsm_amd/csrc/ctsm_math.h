@@ -14,18 +14,18 @@
  *           nearest k/128 (table of log F in double-double), u in
  *           double-double from an fma remainder (no division), log(1+u) by
  *           its series to u^9.  Relative error ~2^-64.
- *   exp(z): z = (64 m + j) ln2/64 + r (Cody-Waite split of ln2/64 into
- *           32 + 53 bits), |r| <= ln2/128, 2^(j/64) from a double-double
- *           table, e^r - 1 by its Taylor polynomial to r^6, scaled by 2^m.
- *           The low word of a double-double z is folded into r.
+ *   exp(z): z = m ln2 + r (Cody-Waite split of ln2 into 32 + 53 bits),
+ *           |r| <= ln2/2, e^r - 1 by its Taylor polynomial to r^13 (no
+ *           table), 1 + r carried exactly, scaled by 2^m.  The low word of a
+ *           double-double z is folded into r.
  *   pow(x,y) = exp(y * log x) with y*log x formed in double-double.
  *   atan(x):  table of atan(k/8) plus the odd series of a reduced argument
  *           (see cm_atan).
  *   sin(x):   three-part reduction by pi/2, Taylor polynomials (see cm_sin).
  * Results are within about 1 ulp of the true value (checked against the host
  * libm in tests/test_math.py); they are not claimed correctly rounded.
- * Constants and tables (ln 2, 1/ln 10, log(k/128), 2^(j/64), pi/2,
- * atan(k/8), 1/n, 1/n!) are the binary64 / double-double roundings of
+ * Constants and tables (ln 2, 1/ln 10, log(k/128), pi/2, atan(k/8), 1/n,
+ * 1/n!) are the binary64 / double-double roundings of
  * 60-digit values computed with Python's mpmath.
  */
 #ifndef CTSM_MATH_H
@@ -209,99 +209,41 @@ CM_FN void cm_log_dd(double x, double *lh, double *ll)
     cm_fast_two_sum(s2, lo, lh, ll);
 }
 
-/* ln 2 / 64 = CM_L64_HI (32 significant bits) + CM_L64_LO; 64 / ln 2 */
-#define CM_L64_HI 0x1.62e42fee00000p-7
-#define CM_L64_LO 0x1.a39ef35793c76p-39
-#define CM_INV_L64 0x1.71547652b82fep+6
+/* 1 / ln 2 */
+#define CM_INV_LN2 0x1.71547652b82fep+0
 
-/* exp of the double-double zh + zl: z = (64 m + j) ln2/64 + r (Cody-Waite),
- * |r| <= ln2/128, e^z = 2^m 2^(j/64) (1 + p(r)) with 2^(j/64) from a
- * double-double table and p the Taylor polynomial to r^6 (truncation < 2^-64). */
+/* exp of the double-double zh + zl, without a table (a table lookup is a
+ * memory access per call on the GPU, where the kernels call exp in their
+ * iteration loops): z = m ln2 + r (Cody-Waite, ln 2 split into 32 + 53 bits so
+ * m * CM_LN2_HI32 is exact), |r| <= ln2/2 (+ the low word zl folded in),
+ * e^r = 1 + r + r^2 q(r) with q the Taylor polynomial 1/2 + r/6 + ... + r^11/13!
+ * (truncation below 2^-60 relative), the rounding of 1 + r carried by an
+ * error-free sum, scaled by 2^m. */
 CM_FN double cm_exp_dd(double zh, double zl)
 {
-    static const double CM_EXP2T[64][2] = {
-        {0x1.0000000000000p+0, 0.0},
-        {0x1.02c9a3e778061p+0, -0x1.19083535b085dp-56},
-        {0x1.059b0d3158574p+0, 0x1.d73e2a475b465p-55},
-        {0x1.0874518759bc8p+0, 0x1.186be4bb284ffp-57},
-        {0x1.0b5586cf9890fp+0, 0x1.8a62e4adc610bp-54},
-        {0x1.0e3ec32d3d1a2p+0, 0x1.03a1727c57b53p-59},
-        {0x1.11301d0125b51p+0, -0x1.6c51039449b3ap-54},
-        {0x1.1429aaea92de0p+0, -0x1.32fbf9af1369ep-54},
-        {0x1.172b83c7d517bp+0, -0x1.19041b9d78a76p-55},
-        {0x1.1a35beb6fcb75p+0, 0x1.e5b4c7b4968e4p-55},
-        {0x1.1d4873168b9aap+0, 0x1.e016e00a2643cp-54},
-        {0x1.2063b88628cd6p+0, 0x1.dc775814a8495p-55},
-        {0x1.2387a6e756238p+0, 0x1.9b07eb6c70573p-54},
-        {0x1.26b4565e27cddp+0, 0x1.2bd339940e9d9p-55},
-        {0x1.29e9df51fdee1p+0, 0x1.612e8afad1255p-55},
-        {0x1.2d285a6e4030bp+0, 0x1.0024754db41d5p-54},
-        {0x1.306fe0a31b715p+0, 0x1.6f46ad23182e4p-55},
-        {0x1.33c08b26416ffp+0, 0x1.32721843659a6p-54},
-        {0x1.371a7373aa9cbp+0, -0x1.63aeabf42eae2p-54},
-        {0x1.3a7db34e59ff7p+0, -0x1.5e436d661f5e3p-56},
-        {0x1.3dea64c123422p+0, 0x1.ada0911f09ebcp-55},
-        {0x1.4160a21f72e2ap+0, -0x1.ef3691c309278p-58},
-        {0x1.44e086061892dp+0, 0x1.89b7a04ef80d0p-59},
-        {0x1.486a2b5c13cd0p+0, 0x1.3c1a3b69062f0p-56},
-        {0x1.4bfdad5362a27p+0, 0x1.d4397afec42e2p-56},
-        {0x1.4f9b2769d2ca7p+0, -0x1.4b309d25957e3p-54},
-        {0x1.5342b569d4f82p+0, -0x1.07abe1db13cadp-55},
-        {0x1.56f4736b527dap+0, 0x1.9bb2c011d93adp-54},
-        {0x1.5ab07dd485429p+0, 0x1.6324c054647adp-54},
-        {0x1.5e76f15ad2148p+0, 0x1.ba6f93080e65ep-54},
-        {0x1.6247eb03a5585p+0, -0x1.383c17e40b497p-54},
-        {0x1.6623882552225p+0, -0x1.bb60987591c34p-54},
-        {0x1.6a09e667f3bcdp+0, -0x1.bdd3413b26456p-54},
-        {0x1.6dfb23c651a2fp+0, -0x1.bbe3a683c88abp-57},
-        {0x1.71f75e8ec5f74p+0, -0x1.16e4786887a99p-55},
-        {0x1.75feb564267c9p+0, -0x1.0245957316dd3p-54},
-        {0x1.7a11473eb0187p+0, -0x1.41577ee04992fp-55},
-        {0x1.7e2f336cf4e62p+0, 0x1.05d02ba15797ep-56},
-        {0x1.82589994cce13p+0, -0x1.d4c1dd41532d8p-54},
-        {0x1.868d99b4492edp+0, -0x1.fc6f89bd4f6bap-54},
-        {0x1.8ace5422aa0dbp+0, 0x1.6e9f156864b27p-54},
-        {0x1.8f1ae99157736p+0, 0x1.5cc13a2e3976cp-55},
-        {0x1.93737b0cdc5e5p+0, -0x1.75fc781b57ebcp-57},
-        {0x1.97d829fde4e50p+0, -0x1.d185b7c1b85d1p-54},
-        {0x1.9c49182a3f090p+0, 0x1.c7c46b071f2bep-56},
-        {0x1.a0c667b5de565p+0, -0x1.359495d1cd533p-54},
-        {0x1.a5503b23e255dp+0, -0x1.d2f6edb8d41e1p-54},
-        {0x1.a9e6b5579fdbfp+0, 0x1.0fac90ef7fd31p-54},
-        {0x1.ae89f995ad3adp+0, 0x1.7a1cd345dcc81p-54},
-        {0x1.b33a2b84f15fbp+0, -0x1.2805e3084d708p-57},
-        {0x1.b7f76f2fb5e47p+0, -0x1.5584f7e54ac3bp-56},
-        {0x1.bcc1e904bc1d2p+0, 0x1.23dd07a2d9e84p-55},
-        {0x1.c199bdd85529cp+0, 0x1.11065895048ddp-55},
-        {0x1.c67f12e57d14bp+0, 0x1.2884dff483cadp-54},
-        {0x1.cb720dcef9069p+0, 0x1.503cbd1e949dbp-56},
-        {0x1.d072d4a07897cp+0, -0x1.cbc3743797a9cp-54},
-        {0x1.d5818dcfba487p+0, 0x1.2ed02d75b3707p-55},
-        {0x1.da9e603db3285p+0, 0x1.c2300696db532p-54},
-        {0x1.dfc97337b9b5fp+0, -0x1.1a5cd4f184b5cp-54},
-        {0x1.e502ee78b3ff6p+0, 0x1.39e8980a9cc8fp-55},
-        {0x1.ea4afa2a490dap+0, -0x1.e9c23179c2893p-54},
-        {0x1.efa1bee615a27p+0, 0x1.dc7f486a4b6b0p-54},
-        {0x1.f50765b6e4540p+0, 0x1.9d3e12dd8a18bp-54},
-        {0x1.fa7c1819e90d8p+0, 0x1.74853f3a5931ep-55},
-    };
     if (zh != zh) return zh;
     if (zh > 709.8) return HUGE_VAL;
     if (zh < -745.2) return 0.0;
-    const double kd = rint(zh * CM_INV_L64);
-    const int k = (int)kd;
-    const int j = k & 63;
-    const int m = (k - j) / 64;
-    double r = (zh - kd * CM_L64_HI) - kd * CM_L64_LO;
+    const double kd = rint(zh * CM_INV_LN2);
+    const int m = (int)kd;
+    double r = (zh - kd * CM_LN2_HI32) - kd * CM_LN2_LO32;
     r = r + zl;
-    double p = 0x1.6c16c16c16c17p-10;        /* 1/720 */
-    p = 0x1.1111111111111p-7 + r * p;        /* 1/120 */
-    p = 0x1.5555555555555p-5 + r * p;        /* 1/24  */
-    p = 0x1.5555555555555p-3 + r * p;        /* 1/6   */
-    p = 0.5 + r * p;                         /* 1/2   */
-    p = r + (r * r) * p;                     /* e^r - 1 */
-    const double th = CM_EXP2T[j][0], tl = CM_EXP2T[j][1];
-    const double res = th + (tl + th * p);
+    double q = 0x1.6124613a86d09p-33;        /* 1/13! */
+    q = 0x1.1eed8eff8d898p-29 + r * q;       /* 1/12! */
+    q = 0x1.ae64567f544e4p-26 + r * q;       /* 1/11! */
+    q = 0x1.27e4fb7789f5cp-22 + r * q;       /* 1/10! */
+    q = 0x1.71de3a556c734p-19 + r * q;       /* 1/9!  */
+    q = 0x1.a01a01a01a01ap-16 + r * q;       /* 1/8!  */
+    q = 0x1.a01a01a01a01ap-13 + r * q;       /* 1/7!  */
+    q = 0x1.6c16c16c16c17p-10 + r * q;       /* 1/6!  */
+    q = 0x1.1111111111111p-7 + r * q;        /* 1/5!  */
+    q = 0x1.5555555555555p-5 + r * q;        /* 1/4!  */
+    q = 0x1.5555555555555p-3 + r * q;        /* 1/3!  */
+    q = 0.5 + r * q;                         /* 1/2   */
+    const double p = (r * r) * q;            /* e^r - 1 - r */
+    double s, e;
+    cm_fast_two_sum(1.0, r, &s, &e);         /* 1 + r = s + e exactly (|r| < 1) */
+    const double res = s + (e + p);
     /* 2^m * res; split the scaling so large |m| cannot overflow early */
     if (m > 1000 || m < -1000) {
         const int m1 = m / 2;

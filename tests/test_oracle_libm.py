@@ -12,15 +12,21 @@ column case, a 4,096-column slice of the uneven batch (snow 0-12, bedrock
 snow and bedrock), for 1 and 48 steps, and bounds the relative difference of
 the state north_star names.  It states what a 1e-10 (1 step) / 1e-8 (48
 steps) tolerance against a reference on another libm can and cannot mean:
-the step is well conditioned (differences stay at 1e-13 or below) except on
-columns holding a layer in mixed phase a few kelvin below freezing.  There
-Phasechange sets the liquid to the supercooled-water limit
-watsat (smp / sucsat)**(-1/b), smp = hfus (tfrz - T) / (grav T), whose
-relative sensitivity to T is (1/b) T / (tfrz - T) -- about 40 at 1 K below
-freezing -- and the latent heat it releases or takes feeds back into the
-next step's temperature, so a last-bit difference grows by 10**2 - 10**4
-over a day on those layers.  The reference itself is determined there only
-to that extent: built on another libm, it differs from itself by as much.
+the step is well conditioned (differences stay at 1e-12 or below) except on
+two kinds of column.  (a) Columns whose Richards solve sub-steps: the
+adaptive schedule halves or accepts a sub-step by comparing an error
+estimate with its tolerance, and a last-bit difference that moves the
+estimate across the tolerance changes the schedule, so the two runs differ
+by the time-stepping error itself (up to ~4e-7 relative after one step, on
+the column that takes 35 sub-steps).  (b) Columns holding a layer in mixed
+phase at or a few kelvin below freezing: there Phasechange sets the liquid
+to the supercooled-water limit watsat (smp / sucsat)**(-1/b),
+smp = hfus (tfrz - T) / (grav T), whose relative sensitivity to T is
+(1/b) T / (tfrz - T) -- about 40 at 1 K below freezing -- and the latent heat
+it releases or takes feeds back into the next step's temperature, so a
+last-bit difference grows by 10**2 - 10**4 over a day.  The reference itself
+is determined there only to that extent: built on another libm, it differs
+from itself by as much.
 
 Each build runs in its own process (the loader binds one library per
 process); the parent compares.
@@ -47,11 +53,11 @@ FIELDS = {"col": ("h2osoi_liq", "h2osoi_ice", "t_soisno", "t_grnd"), "pch": (), 
 # Bounds on the largest relative difference between the two builds, and on
 # the number of columns above north_star's bars.  Measured with gcc 11.4 and
 # glibc 2.35 against ctsm_math.h (the report this test prints): after one
-# step at most 4.2e-10 (2 of 4,096 uneven columns above 1e-10, none above
-# 1e-8); after 48 steps at most 2.0e-6 (ice of one 1,024-case column), 3
+# step at most 3.7e-7 (the 1,024-case column with 35 sub-steps; 1 column of
+# 1,024 and 4 of 4,096 above 1e-10); after 48 steps at most 4.1e-7, 3
 # columns of 1,024 and 3 of 4,096 above 1e-8; the full land step's t_veg
 # within 3e-15.  The bounds leave headroom of about 5x.
-BOUND_1 = 5e-9       # after one step
+BOUND_1 = 2e-6       # after one step
 BOUND_48 = 1e-5      # after 48 steps
 MAX_COLS_1 = 8       # columns above 1e-10 after one step (per case)
 MAX_FRAC_48 = 0.01   # fraction of columns above 1e-8 after 48 steps
@@ -83,10 +89,12 @@ def _run(case, out):
                                         drainage_after=True)
     else:
         step = lambda: O.soil_step(st)
+    max_sub = np.zeros(st.dims.ncol)
     for n in range(1, 49):
         rc, bad = step()
         if rc:
             raise SystemExit(f"oracle step {n} failed: rc {rc} at {bad}")
+        max_sub = np.maximum(max_sub, st.col["num_substeps"])
         if n in (1, 48):
             for grp, names in FIELDS.items():
                 for k in names:
@@ -95,6 +103,7 @@ def _run(case, out):
     saved["snl"] = np.array(st.col["snl"])
     saved["nbedrock"] = np.array(st.col["nbedrock"])
     saved["num_substeps"] = np.array(st.col["num_substeps"])
+    saved["max_substeps"] = max_sub
     np.savez(out, **saved)
 
 
@@ -164,22 +173,30 @@ def test_libm_builds_really_differ(runs):
     assert moved > 0
 
 
-def test_sensitive_layers_are_mixed_phase_below_freezing(runs):
+def test_sensitive_columns_substep_most_or_hold_mixed_phase(runs):
     """Every column that moves by more than 1e-9 after 48 steps of a soil
-    case moves most on a layer holding both ice and liquid at freezing or up
-    to 3 K below it (the supercooled-water limit's sensitivity), and such a
-    column exists in each case (so the statement is not vacuous)."""
+    case either moves most on a layer holding both ice and liquid at freezing
+    or up to 3 K below it (the supercooled-water limit's sensitivity), or is
+    among the columns whose Richards solve takes the most sub-steps (at least
+    the batch's 90th percentile of the largest count over the day: the
+    adaptive schedule, decided at its error tolerance, is where a last bit
+    can change the trajectory); mixed-phase columns occur in each case (so
+    the statement is not vacuous)."""
     TFRZ = 273.15
     for case in ("cpu1024", "uneven4096"):
         a, b = runs[case]
         d = np.maximum.reduce([_reldiff(b[f"48:col:{f}"], a[f"48:col:{f}"])
                                for f in ("h2osoi_liq", "h2osoi_ice", "t_soisno")])
         cols = np.nonzero(d.max(axis=0) > 1e-9)[0]
-        assert cols.size > 0, case
+        p90 = np.percentile(a["max_substeps"], 90)
+        n_mixed = 0
         for c in cols:
             lev = int(np.argmax(d[:, c]))
             liq, ice, t = (a[f"48:col:{f}"][lev, c] for f in ("h2osoi_liq", "h2osoi_ice", "t_soisno"))
-            assert liq > 0 and ice > 0 and TFRZ - 3.0 <= t <= TFRZ, (case, c, lev, liq, ice, t)
+            mixed = liq > 0 and ice > 0 and TFRZ - 3.0 <= t <= TFRZ
+            n_mixed += mixed
+            assert mixed or a["max_substeps"][c] >= p90, (case, c, lev, liq, ice, t, a["max_substeps"][c], p90)
+        assert n_mixed > 0, case
 
 
 if __name__ == "__main__":
