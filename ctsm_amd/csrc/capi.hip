@@ -709,7 +709,8 @@ int ctsm_destroy(ctsm_ctx *ctx)
 int ctsm_set_tuning(ctsm_ctx *ctx, const ctsm_tuning *t)
 {
     if (!ctx || !t) return CTSM_E_ARG;
-    if (t->richards_lanes != 1 && t->richards_lanes != 2 && t->richards_lanes != 4 && t->richards_lanes != 8)
+    if (t->richards_lanes != 0 && t->richards_lanes != 1 && t->richards_lanes != 2 && t->richards_lanes != 4 &&
+        t->richards_lanes != 8)
         return CTSM_E_ARG;
     if (t->richards_block < 0 || t->richards_block > 256 || (t->richards_block % 64) != 0) return CTSM_E_ARG;
     /* the one-lane Richards kernel is built for 64-lane blocks (register budget of one wave per SIMD) */

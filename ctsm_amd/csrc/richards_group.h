@@ -15,24 +15,26 @@
 //     group takes the same accept / halve / double decision.
 // A lane keeps ~K*13 doubles instead of 20*13, so nothing spills and several
 // waves share a SIMD.  Values are bit-identical to the one-lane form.
+// G = 0 takes the group size at run time (argument gsz: 2, 4 or 8 lanes of K
+// layers each), so one kernel can give each wavefront the group size its
+// columns' depth needs.
 #pragma once
 #include "common.h"
 
 namespace ctsm {
 
-template <int G>
-__device__ __forceinline__ double gshfl(double v, int src)
+__device__ __forceinline__ double gshfl(double v, int src, int width)
 {
-    return __shfl(v, src, G);
+    return __shfl(v, src, width);
 }
 
-template <int G>
+template <int G, int K = (MAX_NLEVSOI + (G > 0 ? G : 1) - 1) / (G > 0 ? G : 1)>
 __device__ __forceinline__ int richards_group(const View &V, const Params &P, const ctsm_col_state &S, int c,
-                                              double dtime, int g)
+                                              double dtime, int g, int gsz = G)
 {
-    constexpr int K = (MAX_NLEVSOI + G - 1) / G;
+    const int GS = (G > 0) ? G : gsz;       // lanes in the group
     const int nl = S.nbedrock[V.c1(c)];
-    if (nl < 2 || nl > MAX_NLEVSOI) return 1;
+    if (nl < 2 || nl > MAX_NLEVSOI || nl > GS * K) return 1;
 
     double mdz[K], ws[K], bsw[K], hkc[K], suc[K], den[K], sink[K], liq[K];
     double Ud[K], Udu[K], Udl[K], Ub[K], f1[K];
@@ -88,10 +90,10 @@ __device__ __forceinline__ int richards_group(const View &V, const Params &P, co
             smp_0 = -(suc[0] * cm_pow(s2_0, -bsw[0]));
             dsm_0 = (-(bsw[0] * smp_0) / s2_0) / ws[0];
         }
-        const int src_nx = (g + 1 < G) ? g + 1 : g;
-        const double s2_nx = gshfl<G>(s2_0, src_nx);
-        const double smp_nx = gshfl<G>(smp_0, src_nx);
-        const double dsmpdw_nx = gshfl<G>(dsm_0, src_nx);
+        const int src_nx = (g + 1 < GS) ? g + 1 : g;
+        const double s2_nx = gshfl(s2_0, src_nx, GS);
+        const double smp_nx = gshfl(smp_0, src_nx, GS);
+        const double dsmpdw_nx = gshfl(dsm_0, src_nx, GS);
 
         // ---- forward: properties, fluxes and rows of the owned layers
         double qout_last = 0.0, dqodw1_last = 0.0, dqodw2_last = 0.0;
@@ -154,9 +156,9 @@ __device__ __forceinline__ int richards_group(const View &V, const Params &P, co
         }
         // incoming flux of our first layer = outgoing flux of the previous lane's last layer
         const int prv = (g > 0) ? g - 1 : 0;
-        const double qin_p = gshfl<G>(qout_last, prv);
-        const double dqidw0_p = gshfl<G>(dqodw1_last, prv);
-        const double dqidw1_p = gshfl<G>(dqodw2_last, prv);
+        const double qin_p = gshfl(qout_last, prv, GS);
+        const double dqidw0_p = gshfl(dqodw1_last, prv, GS);
+        const double dqidw1_p = gshfl(dqodw2_last, prv, GS);
         // rows (a in Udl, b in Ud, c in Udu, r in Ub) and start-of-step net flux f1
         {
             double qin = (g == 0) ? infl : qin_p;
@@ -180,15 +182,15 @@ __device__ __forceinline__ int richards_group(const View &V, const Params &P, co
             }
         }
         // first row of the next lane (a, b, c, r) for our last elimination step
-        const int nxt = (g + 1 < G) ? g + 1 : g;
-        const double a_nx = gshfl<G>(Udl[0], nxt), b_nx = gshfl<G>(Ud[0], nxt);
-        const double c_nx = gshfl<G>(Udu[0], nxt), r_nx = gshfl<G>(Ub[0], nxt);
+        const int nxt = (g + 1 < GS) ? g + 1 : g;
+        const double a_nx = gshfl(Udl[0], nxt, GS), b_nx = gshfl(Ud[0], nxt, GS);
+        const double c_nx = gshfl(Udu[0], nxt, GS), r_nx = gshfl(Ub[0], nxt, GS);
 
         // ---- dgtsv elimination in G stages
         double cd = 0.0, cdu = 0.0, cb = 0.0;     // current (partly eliminated) row
         bool singular = false;
 #pragma unroll
-        for (int s = 0; s < G; ++s) {
+        for (int s = 0; s < GS; ++s) {
             if (g == s && g * K < nl) {
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
@@ -230,15 +232,15 @@ __device__ __forceinline__ int richards_group(const View &V, const Params &P, co
                     }
                 }
             }
-            cd = gshfl<G>(cd, s);
-            cdu = gshfl<G>(cdu, s);
-            cb = gshfl<G>(cb, s);
+            cd = gshfl(cd, s, GS);
+            cdu = gshfl(cdu, s, GS);
+            cb = gshfl(cb, s, GS);
         }
         // any lane singular -> the whole column fails
         {
             double sg = singular ? 1.0 : 0.0;
 #pragma unroll
-            for (int m = 1; m < G; m <<= 1) sg = fmax90(sg, __shfl_xor(sg, m, G));
+            for (int m = 1; m < GS; m <<= 1) sg = fmax90(sg, __shfl_xor(sg, m, GS));
             if (sg != 0.0) { status = 2; break; }
         }
 
@@ -246,7 +248,7 @@ __device__ __forceinline__ int richards_group(const View &V, const Params &P, co
         double x1 = 0.0, x2 = 0.0, emax = 0.0, dwat_bot = 0.0;
         bool have_e = false;
 #pragma unroll
-        for (int s = G - 1; s >= 0; --s) {
+        for (int s = GS - 1; s >= 0; --s) {
             if (g == s && g * K < nl) {
 #pragma unroll
                 for (int k = K - 1; k >= 0; --k) {
@@ -266,13 +268,13 @@ __device__ __forceinline__ int richards_group(const View &V, const Params &P, co
                     }
                 }
             }
-            x1 = gshfl<G>(x1, s);
-            x2 = gshfl<G>(x2, s);
+            x1 = gshfl(x1, s, GS);
+            x2 = gshfl(x2, s, GS);
         }
         // group max of the error estimate (each owned layer's value; empty lanes hold 0 <= any)
         double errorMax = emax;
 #pragma unroll
-        for (int m = 1; m < G; m <<= 1) errorMax = fmax90(errorMax, __shfl_xor(errorMax, m, G));
+        for (int m = 1; m < GS; m <<= 1) errorMax = fmax90(errorMax, __shfl_xor(errorMax, m, GS));
         if (errorMax > P.xTolerUpper && dtsub > P.dtmin) {
             dtsub = fmax90(dtsub / 2.0, P.dtmin);
             continue;
@@ -280,9 +282,9 @@ __device__ __forceinline__ int richards_group(const View &V, const Params &P, co
 #pragma unroll
         for (int k = 0; k < K; ++k)
             if (g * K + k < nl) liq[k] = liq[k] + Ub[k] * mdz[k];
-        hk_bot = gshfl<G>(hk_bot, bot_lane);
-        dhkdw_bot = gshfl<G>(dhkdw_bot, bot_lane);
-        dwat_bot = gshfl<G>(dwat_bot, bot_lane);
+        hk_bot = gshfl(hk_bot, bot_lane, GS);
+        dhkdw_bot = gshfl(dhkdw_bot, bot_lane, GS);
+        dwat_bot = gshfl(dwat_bot, bot_lane, GS);
         const double qcTemp = zero_flux_bottom ? 0.0 : hk_bot + dhkdw_bot * dwat_bot;
         qcharge = qcharge + qcTemp * (dtsub / dtime);
         dtdone = dtdone + dtsub;
@@ -299,7 +301,7 @@ __device__ __forceinline__ int richards_group(const View &V, const Params &P, co
     // over-saturated layers push water upward (:1405-1409), bottom lane first
     double carry = 0.0;
 #pragma unroll
-    for (int s = G - 1; s >= 0; --s) {
+    for (int s = GS - 1; s >= 0; --s) {
         if (g == s) {
 #pragma unroll
             for (int k = K - 1; k >= 0; --k) {
@@ -316,7 +318,7 @@ __device__ __forceinline__ int richards_group(const View &V, const Params &P, co
                 }
             }
         }
-        carry = gshfl<G>(carry, s);
+        carry = gshfl(carry, s, GS);
     }
 #pragma unroll
     for (int k = 0; k < K; ++k)

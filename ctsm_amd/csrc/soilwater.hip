@@ -76,6 +76,45 @@ k_soilwater_g(View V, Params P, ctsm_col_state S, int n, const int32_t *__restri
     if (rc && g == 0) report_error(err, c, CTSM_E_SOLVER);
 }
 
+// Group size chosen per wavefront (richards_lanes = 0).  A one-wave block
+// takes 32 consecutive columns of the filter and the deepest bedrock index
+// among them; groups of 3 layers per lane then need 2 lanes per column up to
+// 6 layers, 4 up to 12 and 8 up to 20, and the block runs its columns in
+// passes of 64 / lanes.  A batch stored by depth gives most blocks the
+// smallest group its columns allow, so no lane holds only empty layers; the
+// results do not depend on the group size.
+constexpr int ADAPT_SPAN = 32;
+constexpr int ADAPT_K = 3;
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
+k_soilwater_adaptive(View V, Params P, ctsm_col_state S, int n, const int32_t *__restrict__ filter, double dtime,
+                     ErrWord *err)
+{
+    const int lane = threadIdx.x;
+    const int i0 = blockIdx.x * ADAPT_SPAN;
+    const int i1 = min(i0 + ADAPT_SPAN, n);
+    int nl = 0;
+    if (lane < ADAPT_SPAN && i0 + lane < i1) {
+        const int c = filter ? filter[i0 + lane] : V.begc + i0 + lane;
+        if (col_in_range(V, c)) nl = S.nbedrock[V.c1(c)];
+    }
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) nl = max(nl, __shfl_xor(nl, m));
+    const int gsz = (nl <= 2 * ADAPT_K) ? 2 : (nl <= 4 * ADAPT_K) ? 4 : 8;
+    const int per = 64 / gsz;
+    for (int base = i0; base < i1; base += per) {
+        const int i = base + lane / gsz, g = lane % gsz;
+        if (i < i1) {
+            const int c = filter ? filter[i] : V.begc + i;
+            if (!col_in_range(V, c)) {
+                if (g == 0) report_error(err, c, CTSM_E_ARG);
+            } else {
+                const int rc = richards_group<0, ADAPT_K>(V, P, S, c, dtime, g, gsz);
+                if (rc && g == 0) report_error(err, c, CTSM_E_SOLVER);
+            }
+        }
+    }
+}
+
 }  // namespace ctsm
 
 namespace ctsm {
@@ -83,6 +122,11 @@ void launch_soilwater(hipStream_t st, const View &V, const Params &P, const ctsm
                       const int32_t *filter, double dtime, ErrWord *err, int lanes, int block)
 {
     if (n <= 0) return;
+    if (lanes == 0) {
+        hipLaunchKernelGGL(k_soilwater_adaptive, dim3((unsigned)((n + ADAPT_SPAN - 1) / ADAPT_SPAN)), dim3(64), 0, st,
+                           V, P, S, n, filter, dtime, err);
+        return;
+    }
     if (block <= 0 || block > 256) block = (lanes == 1) ? 64 : 128;
     if (lanes != 2 && lanes != 4 && lanes != 8) block = 64;     /* k_soilwater: __launch_bounds__(64) */
     const long threads = (long)n * (lanes > 0 ? lanes : 1);

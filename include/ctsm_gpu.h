@@ -548,7 +548,9 @@ typedef struct ctsm_phs_stress_state {
 
 /* ---- launch shapes (no counterpart in the reference: set by the autotuner) --
  * richards_lanes: lanes cooperating on one column in the Richards solve
- *   (1, 2, 4 or 8); the heat solve always runs one column per lane.
+ *   (1, 2, 4 or 8), or 0: chosen per wavefront from the deepest bedrock index
+ *   among 32 consecutive columns (3 layers per lane: 2 lanes up to 6 layers,
+ *   4 up to 12, 8 up to 20); the heat solve always runs one column per lane.
  * fused_single_kernel: 1 = the whole soil step in one kernel (one lane per
  *   column), 0 = heat solve kernel + Richards kernel (2 launches per step).
  * canopy_first_iterations, canopy_round_iterations: CanopyFluxes runs its

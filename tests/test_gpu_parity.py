@@ -218,6 +218,8 @@ def test_tuning_rejects_oversized_one_lane_block(ctx):
     dict(richards_lanes=8, fused_single_kernel=0, step_streams=2),
     dict(richards_lanes=4, fused_single_kernel=0, step_streams=3),
     dict(richards_lanes=8, fused_single_kernel=0, step_streams=4),
+    dict(richards_lanes=0, fused_single_kernel=0),                   # lanes chosen per wavefront by depth
+    dict(richards_lanes=0, fused_single_kernel=0, step_streams=4),
 ])
 def test_launch_shapes_match_oracle_bitwise(ctx, shape):
     """Every launch shape the autotuner may pick gives the oracle's bits."""
