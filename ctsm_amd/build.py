@@ -58,7 +58,8 @@ def build(verbose=False, extra=(), jobs=None):
         for r in bad:
             sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed")
-    cmd = [hipcc()] + FLAGS + list(extra) + objs + ["-o", LIB + ".tmp"]
+    # roctx: the trace ranges named after the reference's timers
+    cmd = [hipcc()] + FLAGS + list(extra) + objs + ["-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx", "-o", LIB + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -29,6 +29,7 @@
 #include <climits>
 
 #include "common.h"
+#include "trace.h"
 
 namespace ctsm {
 namespace hydro {
@@ -553,6 +554,7 @@ void launch_surface_water(hipStream_t st, const ctsm_layout &L, double e_ice, co
                           double dtime, ErrWord *err)
 {
     if (n <= 0) return;
+    const GptlRange range("hydro_without_drainage");
     const hydro::HView V{L.begc_alloc, L.ldc, L.begp_alloc, L.ldp, 0, 0, 0};
     hipLaunchKernelGGL(hydro::k_surface_water, dim3((n + 255) / 256), dim3(256), 0, st, V, e_ice, P, S, W, n, filter,
                        dtime, err);
@@ -563,6 +565,7 @@ void launch_patch2col3(hipStream_t st, const ctsm_layout &L, const ctsm_col_stat
                        double *cb, double *ccol, int n, const int32_t *filter, ErrWord *err)
 {
     if (n <= 0) return;
+    const GptlRange range("hydro_without_drainage");
     const hydro::HView V{L.begc_alloc, L.ldc, L.begp_alloc, L.ldp, 0, 0, 0};
     const hydro::P2C F{active, wtcol, pa, pb, pc, ca, cb, ccol};
     if (filter)
@@ -576,6 +579,7 @@ void launch_water_table(hipStream_t st, const ctsm_layout &L, const ctsm_config 
                         const ctsm_water_table_state &W, int n, const int32_t *filter, double dtime, ErrWord *err)
 {
     if (n <= 0) return;
+    const GptlRange range("hydro_without_drainage");
     const hydro::HView V{L.begc_alloc, L.ldc, L.begp_alloc, L.ldp, cfg.nlevsno, cfg.nlevsoi, cfg.nlevgrnd};
     hipLaunchKernelGGL(hydro::k_water_table, dim3((n + 255) / 256), dim3(256), 0, st, V, S, W, n, filter, dtime, err);
 }
@@ -585,6 +589,7 @@ void launch_drainage(hipStream_t st, const ctsm_layout &L, const ctsm_config &cf
                      double dtime, ErrWord *err)
 {
     if (n <= 0) return;
+    const GptlRange range("hydro2_drainage");
     const hydro::HView V{L.begc_alloc, L.ldc, L.begp_alloc, L.ldp, cfg.nlevsno, cfg.nlevsoi, cfg.nlevgrnd};
     hipLaunchKernelGGL(hydro::k_drainage, dim3((n + 255) / 256), dim3(256), 0, st, V, cfg.e_ice, P, S, D, n, filter,
                        dtime, err);

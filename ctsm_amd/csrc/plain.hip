@@ -12,6 +12,7 @@
 // (hk, smp, fluxes, amx/bmx/cmx/rmx, dwat) are HBM-resident too.  Same
 // expressions and order as the reference, so results equal the fused path's.
 #include "common.h"
+#include "trace.h"
 #include "soiltemp_dev.h"
 
 namespace ctsm {
@@ -725,14 +726,24 @@ void launch_soiltemp_plain(hipStream_t st, const View &V, const Params &P, const
     const int bs = 128;
     const dim3 g((n + bs - 1) / bs), b(bs);
     PW w{ws, ld};
-    hipLaunchKernelGGL(k_thermprop, g, b, 0, st, V, P, S, n, filter, w);
+    const GptlRange range("soiltemperature");
+    {
+        const GptlRange r("SoilThermProp");
+        hipLaunchKernelGGL(k_thermprop, g, b, 0, st, V, P, S, n, filter, w);
+    }
     hipLaunchKernelGGL(k_groundflux, g, b, 0, st, V, S, Pt, n, filter, w);
     hipLaunchKernelGGL(k_diffflux, g, b, 0, st, V, S, n, filter, dtime, w);
     hipLaunchKernelGGL(k_rhs, g, b, 0, st, V, S, n, filter, dtime, w);
     hipLaunchKernelGGL(k_matrix, g, b, 0, st, V, S, n, filter, dtime, w);
-    hipLaunchKernelGGL(k_banddiag, g, b, 0, st, V, S, n, filter, w, err);
+    {
+        const GptlRange r("SoilTempBandDiag");
+        hipLaunchKernelGGL(k_banddiag, g, b, 0, st, V, S, n, filter, w, err);
+    }
     hipLaunchKernelGGL(k_fn1_h2osfc, g, b, 0, st, V, S, n, filter, dtime, w);
-    hipLaunchKernelGGL(k_phasechange, g, b, 0, st, V, S, n, filter, dtime, w);
+    {
+        const GptlRange r("PhaseChange");
+        hipLaunchKernelGGL(k_phasechange, g, b, 0, st, V, S, n, filter, dtime, w);
+    }
     hipLaunchKernelGGL(k_tail, g, b, 0, st, V, S, n, filter, w);
 }
 
@@ -740,6 +751,7 @@ void launch_soilwater_plain(hipStream_t st, const View &V, const Params &P, cons
                             const int32_t *filter, double dtime, double *ws, long ld, ErrWord *err)
 {
     if (n <= 0) return;
+    const GptlRange range("hydro_without_drainage");
     const int bs = 128;
     PW w{ws, ld};
     hipLaunchKernelGGL(k_soilwater_plain, dim3((n + bs - 1) / bs), dim3(bs), 0, st, V, P, S, n, filter, dtime, w, err);

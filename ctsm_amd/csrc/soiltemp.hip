@@ -19,6 +19,7 @@
 // The patch -> column ground heat flux (ComputeGroundHeatFluxAndDeriv
 // :1543-1796) runs first, looping the column's patches in reference order.
 #include "common.h"
+#include "trace.h"
 #include "soiltemp_dev.h"
 #include "soilwater_dev.h"
 
@@ -648,6 +649,7 @@ void launch_soiltemp(hipStream_t st, const View &V, const Params &P, const ctsm_
                      double *ws, long ld, ErrWord *err)
 {
     if (n <= 0) return;
+    const GptlRange range("soiltemperature");
     const int bs = 64;
     TempWork wk{ws, ld};
     hipLaunchKernelGGL(k_soiltemp, dim3((n + bs - 1) / bs), dim3(bs), 0, st, V, P, S, Pt, n, filter, dtime, wk, err);
@@ -658,6 +660,7 @@ void launch_soil_step(hipStream_t st, const View &V, const Params &P, const ctsm
                       double *ws, long ld, ErrWord *err)
 {
     if (n <= 0) return;
+    const GptlRange range("soiltemperature+hydro_without_drainage");
     const int bs = 64;
     TempWork wk{ws, ld};
     hipLaunchKernelGGL(k_soil_step, dim3((n + bs - 1) / bs), dim3(bs), 0, st, V, P, S, Pt, n, filter, dtime, wk, err);
@@ -668,6 +671,7 @@ void launch_soiltemp_fractions(hipStream_t st, const View &V, const Params &P, c
                                double *ws, long ld, ErrWord *err, int block)
 {
     if (n <= 0) return;
+    const GptlRange range("soiltemperature");
     if (block <= 0 || block > 128) block = 128;
     TempWork wk{ws, ld};
     hipLaunchKernelGGL(k_soiltemp_fractions, dim3((n + block - 1) / block), dim3(block), 0, st, V, P, S, Pt, n,

@@ -19,6 +19,7 @@
 // the column order in the filter (sorted by layer count by the caller) is
 // what keeps a wavefront's lanes doing the same amount of work.
 #include "common.h"
+#include "trace.h"
 #include "soilwater_dev.h"
 #include "richards_group.h"
 
@@ -122,6 +123,7 @@ void launch_soilwater(hipStream_t st, const View &V, const Params &P, const ctsm
                       const int32_t *filter, double dtime, ErrWord *err, int lanes, int block)
 {
     if (n <= 0) return;
+    const GptlRange range("hydro_without_drainage");
     if (lanes == 0) {
         hipLaunchKernelGGL(k_soilwater_adaptive, dim3((unsigned)((n + ADAPT_SPAN - 1) / ADAPT_SPAN)), dim3(64), 0, st,
                            V, P, S, n, filter, dtime, err);

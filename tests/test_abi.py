@@ -115,6 +115,18 @@ end program sizes
                      C.sizeof(abi.PhsParams), C.sizeof(abi.PhsStressState)]
 
 
+def test_trace_ranges_carry_the_reference_timer_names():
+    """The library pushes roctx ranges named after the reference's GPTL
+    timers around its launches (rocprofv3 --marker-trace --kernel-rename
+    reports kernels under them)."""
+    out = subprocess.run(["nm", "-D", lib.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    assert re.search(r"\bU roctxRangePushA\b", out) and re.search(r"\bU roctxRangePop\b", out)
+    data = open(lib.LIB_PATH, "rb").read()
+    for name in (b"can_iter", b"canflux", b"SoilTempBandDiag", b"SoilThermProp", b"PhaseChange",
+                 b"soiltemperature", b"bgp2", b"hydro_without_drainage", b"hydro2_drainage", b"balchk"):
+        assert b"\0" + name + b"\0" in data, name
+
+
 def test_config_rejected_before_device():
     L = lib.load()
     h = C.c_void_p()

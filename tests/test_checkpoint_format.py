@@ -5,7 +5,10 @@ table, 4 KiB-aligned ranges, FNV-1a checksums over 64-bit words -- with fields
 larger than one 32 MiB staging chunk, empty fields and a range ending exactly on
 the alignment; a flipped byte or a wrong size is refused (CTSM_E_IO) with the
 destination untouched; a write that is killed or fails partway leaves the
-previous file whole."""
+previous file whole.  Every case runs on two builds of the tool: -O1, and
+-O1 with AddressSanitizer and UndefinedBehaviorSanitizer, which aborts the
+run (and fails the case) on an out-of-bounds access, a use after free, a leak
+or undefined arithmetic."""
 import os
 import resource
 import shutil
@@ -21,13 +24,17 @@ SIZES = [100, (70 << 20) + 5, 0, 8, 12345, 3 << 20]     # the last ends on the 4
 CTSM_E_IO = 7
 
 
-@pytest.fixture(scope="module")
-def tool(tmp_path_factory):
+SANITIZE = ["-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined", "-fno-sanitize-recover=all"]
+
+
+@pytest.fixture(scope="module", params=["plain", "asan"])
+def tool(request, tmp_path_factory):
     if not shutil.which("g++"):
         pytest.skip("g++ not available")
-    d = tmp_path_factory.mktemp("ckpt")
+    d = tmp_path_factory.mktemp("ckpt_" + request.param)
     exe = d / "ckpt"
-    subprocess.run(["g++", "-O1", "-std=c++17", "-I", os.path.join(ROOT, "tests", "host_hip"),
+    extra = SANITIZE if request.param == "asan" else []
+    subprocess.run(["g++", "-O1", "-std=c++17", *extra, "-I", os.path.join(ROOT, "tests", "host_hip"),
                     "-I", os.path.join(ROOT, "include"), "-x", "c++",
                     os.path.join(ROOT, "ctsm_amd", "csrc", "checkpoint.hip"), "-x", "c++",
                     os.path.join(ROOT, "tests", "host_hip", "ckpt_main.cpp"), "-o", str(exe)], check=True)
