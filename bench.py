@@ -39,8 +39,9 @@ HBM_PEAK_GBS = 8000.0
 # dispatch, scaled by the same counters' readings on a streaming copy of
 # known size) and FP64 VALU share (SQ_INSTS_VALU_*_F64 over the launch time
 # against the 78.6 TFLOP/s vector peak) of a dominant kernel, keyed by
-# (kernel, workload, columns).  The bench reports them only for the run they
-# were measured on; any other run reports null.
+# (kernel, workload, columns), with the launch shape they were measured at.
+# The bench reports them only for a run of that workload, size and launch
+# shape; any other run reports null.
 PROFILED = {}
 
 
@@ -111,22 +112,31 @@ def canopy_bytes(st):
 
 def served_ensemble(ctx, args):
     """Perturbed-parameter members arriving as a Poisson stream, run by the
-    continuous-batching server in one resident pool; latency = completion -
-    arrival (wall clock)."""
+    continuous-batching server (the library's ctsm_serve_* scheduler) in one
+    resident pool; latency = completion - arrival (wall clock), queueing =
+    admission - arrival."""
     from ctsm_amd import serve, synthetic
     ctx.set_canopy_params(synthetic.canopy_params(), synthetic.pft_table())
     reqs = serve.make_requests(args.serve_requests, args.serve_rate, seed=0xC75D0004,
                                day_scale=args.serve_day_scale)
     srv, done, wall = serve.serve(ctx, reqs, args.serve_pool_cols, args.serve_pool_cols * 17)
     lat = np.array([q.done - q.arrival for q in done])
-    out = {"workload": f"{len(reqs)} members, Poisson {args.serve_rate}/s, 1-20,000 columns (log-uniform) with "
-                       f"1-16 patches, 30 parameters x U(0.5,1.5), 1-365 days (log-uniform) x 48 steps x "
-                       f"{args.serve_day_scale}; full land step (CanopyFluxes, SoilTemperature, SoilFluxes, "
-                       f"surface water, SoilWater, water tables, drainage) and the soil balance check every step",
+    wait = np.array([q.admitted - q.arrival for q in done])
+    steps = np.array([q.steps for q in reqs])
+    cols = np.array([q.ncol for q in reqs])
+    out = {"workload": f"{len(reqs)} members, Poisson {args.serve_rate}/s, 1-20,000 columns (log-uniform, "
+                       f"{int(cols.min())}-{int(cols.max())} drawn) with 1-16 patches, 30 parameters x U(0.5,1.5) "
+                       f"(soil Clapp-Hornberger and thermal, surface roughness / emissivity / runoff, 18 PFT "
+                       f"parameters incl. the stomatal slope on the member's own PFT rows), 1-365 days "
+                       f"(log-uniform) x 48 steps x {args.serve_day_scale} ({int(steps.min())}-{int(steps.max())} "
+                       f"steps drawn); full land step (CanopyFluxes, SoilTemperature, SoilFluxes, surface water, "
+                       f"SoilWater, water tables, drainage) and the soil balance check every step",
            "pool_columns": args.serve_pool_cols, "pool_bytes_per_column": srv.bytes_per_column(),
            "completed": len(done), "wall_s": wall,
            "latency_p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
            "latency_p99_s": float(np.percentile(lat, 99)) if len(lat) else None,
+           "queue_wait_p50_s": float(np.percentile(wait, 50)) if len(wait) else None,
+           "queue_wait_p99_s": float(np.percentile(wait, 99)) if len(wait) else None,
            "column_steps_per_s": srv.column_steps / wall,
            "max_abs_errsoi_Wm2": srv.worst_errsoi}
     srv.close()
@@ -509,10 +519,10 @@ def main():
                     help="columns of the soil-only uneven measurement (0 skips it)")
     ap.add_argument("--extras", action="store_true",
                     help="also time the PHS kernels, the history tape and a checkpoint of a 262,144-column state")
-    ap.add_argument("--serve-requests", type=int, default=80,
+    ap.add_argument("--serve-requests", type=int, default=48,
                     help="served-ensemble measurement: number of requests (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate (requests/s)")
-    ap.add_argument("--serve-day-scale", type=float, default=0.1,
+    ap.add_argument("--serve-day-scale", type=float, default=1.0,
                     help="run lengths are log-uniform 1-365 days x 48 steps x this factor")
     ap.add_argument("--serve-pool-cols", type=int, default=131072)
     ap.add_argument("--calib-copy-mb", type=int, default=0,
@@ -565,7 +575,9 @@ def main():
         res["served_ensemble"] = served_ensemble(ctx, args)
         print("[bench] served ensemble done", file=sys.stderr, flush=True)
     kname = "k_canopy_round"
-    prof = PROFILED.get((kname, args.workload, ncol)) if not args.no_autotune else None
+    prof = PROFILED.get((kname, args.workload, ncol))
+    if prof and any(res["tuning"].get(k) != v for k, v in prof["tuning"].items()):
+        prof = None
     lanes = res["tuning"]["richards_lanes"]
     out = {
         "metric": "column-steps/sec (FP64, 1 GPU)",

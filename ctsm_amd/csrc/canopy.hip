@@ -22,6 +22,7 @@
 #include <climits>
 
 #include "common.h"
+#include "trace.h"
 
 namespace ctsm {
 namespace canopy {
@@ -1432,11 +1433,15 @@ __global__ void __launch_bounds__(64) k_zero_counts(int32_t *cnt, int n)
 #undef PF
 
 // photosyns_inst%TimeStepInit (PhotosynthesisMod.F90:1143-1201) and
-// rb1(begp:endp) = 0 (CanopyFluxesMod.F90:830): every patch of the bounds.
-__global__ void __launch_bounds__(256) k_canopy_init(int ldp, ctsm_canopy_state S)
+// rb1(begp:endp) = 0 (CanopyFluxesMod.F90:830): every patch of the bounds,
+// the n patches from begp or, where the bounds are the filter's own patches,
+// the patches filter names.
+__global__ void __launch_bounds__(256) k_canopy_init(int n, const int32_t *__restrict__ filter, int begp,
+                                                     ctsm_canopy_state S)
 {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ldp) return;
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    const long i = filter ? (long)(filter[f] - begp) : (long)f;
     S.psnsun[i] = 0.0; S.psnsun_wc[i] = 0.0; S.psnsun_wj[i] = 0.0; S.psnsun_wp[i] = 0.0;
     S.psnsha[i] = 0.0; S.psnsha_wc[i] = 0.0; S.psnsha_wj[i] = 0.0; S.psnsha_wp[i] = 0.0;
     S.fpsn[i] = 0.0; S.fpsn_wc[i] = 0.0; S.fpsn_wj[i] = 0.0; S.fpsn_wp[i] = 0.0;
@@ -1602,9 +1607,10 @@ size_t canopy_work_bytes(int n, int itmax, int first_its, int round_its)
 void launch_canopy(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg, const ctsm_canopy_params &prm,
                    const double *pft_dev, const ctsm_col_state &col, const ctsm_canopy_state &can, int n,
                    const int32_t *filter, double dtime, ErrWord *err, void *work, int first_its, int round_its,
-                   bool zero_cgrnd, int waves)
+                   bool zero_cgrnd, int waves, bool bounds_are_filter)
 {
     using namespace canopy;
+    const GptlRange range("canflux");
     auto first_kernel = waves == 3 ? k_canopy_round<3, false> : waves == 2 ? k_canopy_round<2, false>
                                                                           : k_canopy_round<1, false>;
     auto later_kernel = waves == 3 ? k_canopy_round<3, true> : waves == 2 ? k_canopy_round<2, true>
@@ -1612,8 +1618,12 @@ void launch_canopy(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg,
     CView V{L.begc_alloc, L.ldc, L.begp_alloc, L.ldp, L.begg_alloc, L.ldg, cfg.nlevsno, cfg.nlevsoi, cfg.nlevgrnd};
     CSwitch sw{cfg.z0param_method, cfg.stomatalcond_method, cfg.soil_resis_method, cfg.use_biomass_heat_storage,
                cfg.use_undercanopy_stability, cfg.light_inhibit, cfg.itmax_canopy_fluxes};
-    if (L.ldp > 0)
-        hipLaunchKernelGGL(k_canopy_init, dim3((L.ldp + 255) / 256), dim3(256), 0, st, L.ldp, can);
+    if (bounds_are_filter && filter) {
+        if (n > 0)
+            hipLaunchKernelGGL(k_canopy_init, dim3((n + 255) / 256), dim3(256), 0, st, n, filter, L.begp_alloc, can);
+    } else if (L.ldp > 0) {
+        hipLaunchKernelGGL(k_canopy_init, dim3((L.ldp + 255) / 256), dim3(256), 0, st, L.ldp, nullptr, 0, can);
+    }
     if (n <= 0) return;
     if (zero_cgrnd)
         hipLaunchKernelGGL(k_zero_cgrnd, dim3((n + 255) / 256), dim3(256), 0, st, n, filter, L.begp_alloc, can);
@@ -1621,6 +1631,8 @@ void launch_canopy(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg,
     const int blocks = (n + 63) / 64;
     RoundArgs a{V, sw, prm, pft_dev, col, can, n, filter, dtime, err, 0, 0, nullptr, nullptr, nullptr, nullptr,
                 CanopyWork{}};
+    // round 0 also runs the part of the routine before the iteration
+    const GptlRange iter_range("can_iter");
     if (R == 0) {
         // one pass: every patch finishes (converged or itmax reached) in round 0
         a.budget = cfg.itmax_canopy_fluxes + 1;
@@ -1653,6 +1665,7 @@ void launch_canopy(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg,
 void launch_root_sink(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg, const ctsm_col_state &col,
                       const ctsm_canopy_state &can, int n, const int32_t *filter, ErrWord *err)
 {
+    const GptlRange range("hydro_without_drainage");
     using namespace canopy;
     if (n <= 0) return;
     CView V{L.begc_alloc, L.ldc, L.begp_alloc, L.ldp, L.begg_alloc, L.ldg, cfg.nlevsno, cfg.nlevsoi, cfg.nlevgrnd};

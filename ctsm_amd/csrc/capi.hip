@@ -10,6 +10,8 @@
 #include <climits>
 #include <cstddef>
 #include <cstring>
+#include <deque>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -36,7 +38,7 @@ void launch_soilwater_plain(hipStream_t, const View &, const Params &, const cts
 size_t canopy_work_bytes(int n, int itmax, int first_its, int round_its);
 void launch_canopy(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_canopy_params &, const double *,
                    const ctsm_col_state &, const ctsm_canopy_state &, int, const int32_t *, double, ErrWord *,
-                   void *, int, int, bool, int);
+                   void *, int, int, bool, int, bool);
 void launch_root_sink(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_col_state &,
                       const ctsm_canopy_state &, int, const int32_t *, ErrWord *);
 void launch_save_before(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_col_state &,
@@ -46,6 +48,9 @@ void launch_soilfluxes(hipStream_t, const ctsm_layout &, const ctsm_config &, co
                        const int32_t *, double, ErrWord *);
 void launch_balance(hipStream_t, const ctsm_layout &, const ctsm_config &, const double *, int, const int32_t *,
                     unsigned long long *);
+void launch_energy_balance(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_col_state &,
+                           const ctsm_patch_state &, const ctsm_canopy_state &, const ctsm_soilflux_state &, int,
+                           const int32_t *, double *, double *, unsigned long long *, ErrWord *);
 void launch_surface_water(hipStream_t, const ctsm_layout &, double, const ctsm_hydro_params &, const ctsm_col_state &,
                           const ctsm_surface_water_state &, int, const int32_t *, double, ErrWord *);
 void launch_patch2col3(hipStream_t, const ctsm_layout &, const ctsm_col_state &, const int32_t *, const double *,
@@ -955,10 +960,14 @@ static int check_canopy_layout(const ctsm_ctx *ctx, const ctsm_layout *L)
 
 // Grows the canopy round buffer to n patches and launches CanopyFluxes.
 // With fixed_work, the rounds use that buffer of fixed_bytes instead and
-// nothing is allocated (CTSM_E_SCRATCH when it is too small).
+// nothing is allocated (CTSM_E_SCRATCH when it is too small).  With
+// bounds_are_filter the per-step resets of the bounds (rb1, the
+// photosynthesis outputs) cover the filter's patches instead of the whole
+// layout: a caller whose layout holds other, independent runs (the ensemble
+// server's pool) must not touch their patches.
 static int run_canopy(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state &col, const ctsm_canopy_state &can,
                       int32_t n, const int32_t *filter_dev, double dtime, bool zero_cgrnd,
-                      void *fixed_work = nullptr, size_t fixed_bytes = 0)
+                      void *fixed_work = nullptr, size_t fixed_bytes = 0, bool bounds_are_filter = false)
 {
     const bool one_pass = ctx->tun.plain_routines != 0;
     const int k0 = one_pass ? 1001 : ctx->tun.canopy_first_iterations;
@@ -978,7 +987,7 @@ static int run_canopy(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_stat
         work = ctx->cwork.p;
     }
     launch_canopy(ctx->stream, *lay, ctx->cfg, ctx->cprm, (const double *)ctx->pft.p, col, can, n, filter_dev, dtime,
-                  ctx->err, work, k0, k, zero_cgrnd, ctx->tun.canopy_waves);
+                  ctx->err, work, k0, k, zero_cgrnd, ctx->tun.canopy_waves, bounds_are_filter);
     return CTSM_OK;
 }
 
@@ -1222,6 +1231,34 @@ int ctsm_soil_balance_check_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const cts
     std::memcpy(&m, &w[0], sizeof(m));
     *errsoi_max = m;
     *index = (n > 0 && w[1] != 0x7fffffffull) ? (int32_t)w[1] : 0;
+    return CTSM_OK;
+}
+
+int ctsm_energy_balance_check_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                                  const ctsm_patch_state *pch, const ctsm_canopy_state *can,
+                                  const ctsm_soilflux_state *sf, int32_t n, const int32_t *filterp_dev, double *errseb,
+                                  double *errlon, double *errseb_max, int32_t *seb_patch, double *errlon_max,
+                                  int32_t *lon_patch)
+{
+    if (!ctx || !col || !pch || !can || !sf || !errseb_max || !seb_patch || !errlon_max || !lon_patch || n < 0 ||
+        check_layout(lay) || lay->ldp <= 0 || (!filterp_dev && n > lay->ldp) || !col->forc_lwrad || !pch->sabg_chk ||
+        !can->sabv || !can->dhsdt_canopy || !can->active || !can->column || !sf->eflx_lwrad_out ||
+        !sf->eflx_lwrad_net || !sf->eflx_sh_tot || !sf->eflx_lh_tot || !sf->eflx_soil_grnd)
+        return CTSM_E_ARG;
+    hipError_t he = ctx->bal.ensure(4 * sizeof(unsigned long long));
+    if (he != hipSuccess) return hip_fail(ctx, he);
+    launch_energy_balance(ctx->stream, *lay, ctx->cfg, *col, *pch, *can, *sf, n, filterp_dev, errseb, errlon,
+                          (unsigned long long *)ctx->bal.p, ctx->err);
+    int rc = finish_launch(ctx);
+    if (rc) return rc;
+    unsigned long long w[4] = {0ull, 0ull, 0ull, 0ull};
+    he = hipMemcpyAsync(w, ctx->bal.p, sizeof(w), hipMemcpyDeviceToHost, ctx->stream);
+    if (he != hipSuccess) return hip_fail(ctx, he);
+    if ((rc = sync_and_check(ctx))) return rc;
+    std::memcpy(errseb_max, &w[0], sizeof(double));
+    std::memcpy(errlon_max, &w[2], sizeof(double));
+    *seb_patch = (n > 0 && w[1] != 0x7fffffffull) ? (int32_t)w[1] : 0;
+    *lon_patch = (n > 0 && w[3] != 0x7fffffffull) ? (int32_t)w[3] : 0;
     return CTSM_OK;
 }
 
@@ -1647,11 +1684,12 @@ int ctsm_phs_calcstress(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_phs_st
     return sync_and_check(ctx);
 }
 
-int ctsm_land_step_full_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
-                            const ctsm_patch_state *pch, const ctsm_canopy_state *can, const ctsm_soilflux_state *sf,
-                            const ctsm_surface_water_state *sw, const ctsm_hydro_params *hprm,
-                            const ctsm_water_table_state *wt, const ctsm_drainage_state *dr, int32_t nc,
-                            const int32_t *filterc_dev, int32_t np, const int32_t *filterp_dev, double dtime)
+static int land_step_full(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                          const ctsm_patch_state *pch, const ctsm_canopy_state *can, const ctsm_soilflux_state *sf,
+                          const ctsm_surface_water_state *sw, const ctsm_hydro_params *hprm,
+                          const ctsm_water_table_state *wt, const ctsm_drainage_state *dr, int32_t nc,
+                          const int32_t *filterc_dev, int32_t np, const int32_t *filterp_dev, double dtime,
+                          bool bounds_are_filter)
 {
     if (!ctx || !col || !pch || !can || !sf || nc < 0 || np < 0 || check_canopy_layout(ctx, lay) ||
         (!filterc_dev && nc > lay->ldc) || (!filterp_dev && np > lay->ldp) || !sf_complete(sf))
@@ -1682,7 +1720,8 @@ int ctsm_land_step_full_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_co
     const bool plain = ctx->tun.plain_routines != 0;
     if ((rc = ensure_work(ctx, land_work_doubles(ctx, nc)))) return rc;
     launch_save_before(ctx->stream, *lay, ctx->cfg, *col, *sf, nc, filterc_dev);
-    if ((rc = run_canopy(ctx, lay, *col, *can, np, filterp_dev, dtime, true))) return rc;
+    if ((rc = run_canopy(ctx, lay, *col, *can, np, filterp_dev, dtime, true, nullptr, 0, bounds_are_filter)))
+        return rc;
     const View V = make_view(ctx, *lay);
     if (plain) {
         launch_soiltemp_plain(ctx->stream, V, ctx->prm, *col, *pch, nc, filterc_dev, dtime, (double *)ctx->work.p,
@@ -1715,6 +1754,16 @@ int ctsm_land_step_full_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_co
     }
     if (dr) launch_drainage(ctx->stream, *lay, ctx->cfg, *hprm, *col, *dr, nc, filterc_dev, dtime, ctx->err);
     return finish_launch(ctx);
+}
+
+int ctsm_land_step_full_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                            const ctsm_patch_state *pch, const ctsm_canopy_state *can, const ctsm_soilflux_state *sf,
+                            const ctsm_surface_water_state *sw, const ctsm_hydro_params *hprm,
+                            const ctsm_water_table_state *wt, const ctsm_drainage_state *dr, int32_t nc,
+                            const int32_t *filterc_dev, int32_t np, const int32_t *filterp_dev, double dtime)
+{
+    return land_step_full(ctx, lay, col, pch, can, sf, sw, hprm, wt, dr, nc, filterc_dev, np, filterp_dev, dtime,
+                          false);
 }
 
 int ctsm_canopyfluxes(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
@@ -1926,4 +1975,474 @@ int ctsm_checkpoint_write(ctsm_ctx *ctx, const char *path, const ctsm_ckpt_field
 int ctsm_checkpoint_read(ctsm_ctx *ctx, const char *path, const ctsm_ckpt_field *fields, int32_t n)
 {
     return ckpt_call(ctx, path, fields, n, false);
+}
+
+// ---- PFT rows of one member ------------------------------------------------------
+
+int ctsm_set_pft_rows(ctsm_ctx *ctx, int32_t first_row, const ctsm_pft_params *pft)
+{
+    if (!ctx || !pft || !ctx->have_canopy || first_row < 0 || pft->npft < 1 ||
+        first_row + pft->npft > CTSM_MAX_PFT)
+        return CTSM_E_ARG;
+    const double *cols[] = {pft->dleaf, pft->c3psn, pft->slatop, pft->leafcn, pft->flnr, pft->fnitr, pft->mbbopt,
+                            pft->medlynslope, pft->medlynintercept, pft->smpso, pft->smpsc, pft->theta_cj,
+                            pft->z0v_Cr, pft->z0v_Cs, pft->z0v_c, pft->z0v_cw, pft->z0v_LAImax};
+    const double *cols2[] = {pft->dbh, pft->fbw, pft->nstem, pft->rstem_per_dbh, pft->wood_density};
+    for (const double *a : cols) if (!a) return CTSM_E_ARG;
+    for (const double *a : cols2) if (!a) return CTSM_E_ARG;
+    if (!pft->is_tree || !pft->is_shrub) return CTSM_E_ARG;
+    const int n = pft->npft;
+    std::vector<double> rows((size_t)24 * n);
+    for (int f = 0; f < 17; ++f) std::memcpy(&rows[(size_t)f * n], cols[f], n * sizeof(double));
+    for (int i = 0; i < n; ++i) {
+        rows[(size_t)17 * n + i] = (double)pft->is_tree[i];
+        rows[(size_t)18 * n + i] = (double)pft->is_shrub[i];
+    }
+    for (int f = 0; f < 5; ++f) std::memcpy(&rows[(size_t)(19 + f) * n], cols2[f], n * sizeof(double));
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e);
+    // one strided copy: field f's rows go to table row f, columns first_row..
+    e = hipMemcpy2DAsync((double *)ctx->pft.p + first_row, CTSM_MAX_PFT * sizeof(double), rows.data(),
+                         n * sizeof(double), n * sizeof(double), 24, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e);
+    return CTSM_OK;
+}
+
+// ---- the ensemble server: continuous batching in a resident pool ---------------
+
+namespace {
+
+// First-fit allocator of index ranges [0, cap).
+struct RangeSet {
+    std::vector<std::pair<int64_t, int64_t>> free;     // (start, length), sorted
+    void init(int64_t cap)
+    {
+        free.clear();
+        if (cap > 0) free.push_back({0, cap});
+    }
+    int64_t alloc(int64_t n)
+    {
+        if (n <= 0) return 0;
+        for (size_t i = 0; i < free.size(); ++i)
+            if (free[i].second >= n) {
+                const int64_t s = free[i].first;
+                if (free[i].second == n) free.erase(free.begin() + i);
+                else free[i] = {s + n, free[i].second - n};
+                return s;
+            }
+        return -1;
+    }
+    bool take(int64_t s, int64_t n)
+    {
+        if (n <= 0) return true;
+        for (size_t i = 0; i < free.size(); ++i) {
+            const int64_t a = free[i].first, l = free[i].second;
+            if (a <= s && s + n <= a + l) {
+                std::vector<std::pair<int64_t, int64_t>> parts;
+                if (s > a) parts.push_back({a, s - a});
+                if (a + l > s + n) parts.push_back({s + n, a + l - s - n});
+                free.erase(free.begin() + i);
+                free.insert(free.begin() + i, parts.begin(), parts.end());
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(int64_t s, int64_t n)
+    {
+        if (n <= 0) return;
+        auto it = std::lower_bound(free.begin(), free.end(), std::make_pair(s, (int64_t)0));
+        free.insert(it, {s, n});
+        std::vector<std::pair<int64_t, int64_t>> m;
+        for (auto &r : free) {
+            if (!m.empty() && m.back().first + m.back().second == r.first) m.back().second += r.second;
+            else m.push_back(r);
+        }
+        free.swap(m);
+    }
+};
+
+struct Member {
+    int32_t ncol = 0, npatch = 0, npft = 0, nsteps = 0, left = 0;
+    int32_t col0 = -1, pat0 = -1, pft0 = -1;
+    int32_t state = CTSM_SERVE_QUEUED;
+    int64_t finish_tick = -1;
+};
+
+constexpr int kRing = 3;          // filter buffers (at most kMaxInflight ticks use them at once)
+constexpr int kMaxInflight = 2;   // ticks enqueued and not yet retired
+
+struct Tick {
+    int64_t id = 0;
+    int slot = 0;                 // balance / error result slot (= ring slot of its filter)
+    int steps = 0;
+    hipEvent_t ev = nullptr;
+};
+
+}  // namespace
+
+struct ctsm_server {
+    ctsm_ctx *ctx = nullptr;
+    ctsm_layout lay{};
+    ctsm_col_state col{};
+    ctsm_patch_state pch{};
+    ctsm_canopy_state can{};
+    ctsm_soilflux_state sf{};
+    ctsm_surface_water_state sw{};
+    ctsm_water_table_state wt{};
+    ctsm_drainage_state dr{};
+    ctsm_hydro_params hprm{};
+    bool have_sw = false, have_wt = false, have_dr = false;
+    double dtime = 1800.0;
+    int32_t max_steps = 8;
+    RangeSet cols, pats, pfts;
+    int32_t pft_first = 0;        // first PFT row members may take
+    std::map<int64_t, Member> members;
+    std::deque<int64_t> queue;
+    int64_t next_ticket = 1, next_tick = 0;
+    // filters: device buffers and pinned host staging, one per ring slot
+    int32_t *fc_dev[kRing] = {}, *fp_dev[kRing] = {}, *fc_host[kRing] = {}, *fp_host[kRing] = {};
+    hipEvent_t ring_ev[kRing] = {};   // the last tick that read the slot's filter
+    bool ring_used[kRing] = {};
+    int cur = -1;
+    int32_t nc = 0, np = 0;
+    bool dirty = true;
+    // per tick slot: max_steps (max, column) balance words and the error word
+    unsigned long long *bal_dev = nullptr, *bal_host = nullptr;
+    ErrWord *err_host = nullptr;
+    std::deque<Tick> inflight;
+    double worst_errsoi = 0.0;
+    int32_t worst_col = 0;
+    int64_t column_steps = 0;
+    int32_t err_code = 0, err_index = 0;
+};
+
+static void serve_free(ctsm_server *s)
+{
+    for (int k = 0; k < kRing; ++k) {
+        if (s->fc_dev[k]) (void)hipFree(s->fc_dev[k]);
+        if (s->fp_dev[k]) (void)hipFree(s->fp_dev[k]);
+        if (s->fc_host[k]) (void)hipHostFree(s->fc_host[k]);
+        if (s->fp_host[k]) (void)hipHostFree(s->fp_host[k]);
+        if (s->ring_ev[k]) (void)hipEventDestroy(s->ring_ev[k]);
+    }
+    for (auto &t : s->inflight)
+        if (t.ev) (void)hipEventDestroy(t.ev);
+    if (s->bal_dev) (void)hipFree(s->bal_dev);
+    if (s->bal_host) (void)hipHostFree(s->bal_host);
+    if (s->err_host) (void)hipHostFree(s->err_host);
+    delete s;
+}
+
+int ctsm_serve_create(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col, const ctsm_patch_state *pch,
+                      const ctsm_canopy_state *can, const ctsm_soilflux_state *sf, const ctsm_surface_water_state *sw,
+                      const ctsm_hydro_params *hprm, const ctsm_water_table_state *wt, const ctsm_drainage_state *dr,
+                      int32_t pft_first, int32_t max_steps_per_tick, double dtime, ctsm_server **out)
+{
+    if (!ctx || !lay || !col || !pch || !can || !sf || !out || check_canopy_layout(ctx, lay) || !sf_complete(sf) ||
+        max_steps_per_tick < 1 || max_steps_per_tick > 64 || !(dtime > 0.0) || pft_first < 0 ||
+        pft_first >= CTSM_MAX_PFT || ((sw || dr) && !hprm))
+        return CTSM_E_ARG;
+    *out = nullptr;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipError_t he = hipSetDevice(ctx->device);
+    if (he != hipSuccess) return hip_fail(ctx, he);
+    ctsm_server *s = new ctsm_server;
+    s->ctx = ctx;
+    s->lay = *lay;
+    s->col = *col;
+    s->pch = *pch;
+    s->can = *can;
+    s->sf = *sf;
+    if (sw) { s->sw = *sw; s->have_sw = true; }
+    if (wt) { s->wt = *wt; s->have_wt = true; }
+    if (dr) { s->dr = *dr; s->have_dr = true; }
+    if (hprm) s->hprm = *hprm;
+    s->dtime = dtime;
+    s->max_steps = max_steps_per_tick;
+    s->cols.init(lay->ldc);
+    s->pats.init(lay->ldp);
+    s->pft_first = pft_first;
+    s->pfts.init(CTSM_MAX_PFT - pft_first);
+    bool ok = true;
+    for (int k = 0; ok && k < kRing; ++k)
+        ok = hipMalloc((void **)&s->fc_dev[k], (size_t)lay->ldc * sizeof(int32_t)) == hipSuccess &&
+             hipMalloc((void **)&s->fp_dev[k], (size_t)lay->ldp * sizeof(int32_t)) == hipSuccess &&
+             hipHostMalloc((void **)&s->fc_host[k], (size_t)lay->ldc * sizeof(int32_t), hipHostMallocDefault) ==
+                 hipSuccess &&
+             hipHostMalloc((void **)&s->fp_host[k], (size_t)lay->ldp * sizeof(int32_t), hipHostMallocDefault) ==
+                 hipSuccess &&
+             hipEventCreateWithFlags(&s->ring_ev[k], hipEventDisableTiming) == hipSuccess;
+    const size_t nbal = (size_t)kRing * max_steps_per_tick * 2;
+    ok = ok && hipMalloc((void **)&s->bal_dev, nbal * sizeof(unsigned long long)) == hipSuccess &&
+         hipHostMalloc((void **)&s->bal_host, nbal * sizeof(unsigned long long), hipHostMallocDefault) == hipSuccess &&
+         hipHostMalloc((void **)&s->err_host, kRing * sizeof(ErrWord), hipHostMallocDefault) == hipSuccess;
+    if (!ok) {
+        serve_free(s);
+        return hip_fail(ctx, hipErrorOutOfMemory);
+    }
+    *out = s;
+    return CTSM_OK;
+}
+
+int ctsm_serve_submit(ctsm_server *s, int32_t ncol, int32_t npatch, int32_t npft, int32_t nsteps, int64_t *ticket)
+{
+    if (!s || !ticket || ncol < 1 || npatch < 0 || npft < 0 || nsteps < 1 || ncol > s->lay.ldc ||
+        npatch > s->lay.ldp || npft > CTSM_MAX_PFT - s->pft_first)
+        return CTSM_E_ARG;
+    Member m;
+    m.ncol = ncol;
+    m.npatch = npatch;
+    m.npft = npft;
+    m.nsteps = m.left = nsteps;
+    const int64_t t = s->next_ticket++;
+    s->members[t] = m;
+    s->queue.push_back(t);
+    *ticket = t;
+    return CTSM_OK;
+}
+
+int ctsm_serve_resume_member(ctsm_server *s, int32_t ncol, int32_t npatch, int32_t npft, int32_t left, int32_t col0,
+                             int32_t pat0, int32_t pft0, int64_t *ticket)
+{
+    if (!s || !ticket || ncol < 1 || npatch < 0 || npft < 0 || left < 1 || col0 < 0 || pat0 < 0 || pft0 < 0)
+        return CTSM_E_ARG;
+    if (!s->cols.take(col0, ncol)) return CTSM_E_ARG;
+    if (!s->pats.take(pat0, npatch)) {
+        s->cols.release(col0, ncol);
+        return CTSM_E_ARG;
+    }
+    if (!s->pfts.take(pft0 - s->pft_first, npft)) {
+        s->cols.release(col0, ncol);
+        s->pats.release(pat0, npatch);
+        return CTSM_E_ARG;
+    }
+    Member m;
+    m.ncol = ncol;
+    m.npatch = npatch;
+    m.npft = npft;
+    m.nsteps = m.left = left;
+    m.col0 = col0;
+    m.pat0 = pat0;
+    m.pft0 = pft0;
+    m.state = CTSM_SERVE_RUNNING;
+    const int64_t t = s->next_ticket++;
+    s->members[t] = m;
+    s->dirty = true;
+    *ticket = t;
+    return CTSM_OK;
+}
+
+int ctsm_serve_start(ctsm_server *s, int64_t ticket)
+{
+    if (!s) return CTSM_E_ARG;
+    auto it = s->members.find(ticket);
+    if (it == s->members.end() || it->second.state != CTSM_SERVE_ADMITTED) return CTSM_E_ARG;
+    it->second.state = CTSM_SERVE_RUNNING;
+    s->dirty = true;
+    return CTSM_OK;
+}
+
+int ctsm_serve_poll(const ctsm_server *s, int64_t ticket, int32_t *state, int32_t *col0, int32_t *pat0, int32_t *pft0,
+                    int32_t *left)
+{
+    if (!s || !state) return CTSM_E_ARG;
+    auto it = s->members.find(ticket);
+    if (it == s->members.end()) return CTSM_E_ARG;
+    const Member &m = it->second;
+    *state = (m.state == CTSM_SERVE_FINISHING) ? CTSM_SERVE_RUNNING : m.state;
+    if (col0) *col0 = m.col0;
+    if (pat0) *pat0 = m.pat0;
+    if (pft0) *pft0 = m.pft0;
+    if (left) *left = m.left;
+    return CTSM_OK;
+}
+
+int ctsm_serve_release(ctsm_server *s, int64_t ticket)
+{
+    if (!s) return CTSM_E_ARG;
+    auto it = s->members.find(ticket);
+    if (it == s->members.end()) return CTSM_E_ARG;
+    Member &m = it->second;
+    if (m.state == CTSM_SERVE_RUNNING || m.state == CTSM_SERVE_FINISHING) return CTSM_E_ARG;
+    if (m.state == CTSM_SERVE_QUEUED) {
+        s->queue.erase(std::find(s->queue.begin(), s->queue.end(), ticket));
+    } else {
+        s->cols.release(m.col0, m.ncol);
+        s->pats.release(m.pat0, m.npatch);
+        s->pfts.release(m.pft0 - s->pft_first, m.npft);
+    }
+    s->members.erase(it);
+    return CTSM_OK;
+}
+
+// Retires the oldest tick once its event has completed (or waits for it).
+static int serve_retire(ctsm_server *s, bool wait)
+{
+    Tick &t = s->inflight.front();
+    hipError_t he = wait ? hipEventSynchronize(t.ev) : hipEventQuery(t.ev);
+    if (he == hipErrorNotReady) return 1;
+    if (he != hipSuccess) return -hip_fail(s->ctx, he);
+    const unsigned long long *w = s->bal_host + (size_t)t.slot * s->max_steps * 2;
+    for (int k = 0; k < t.steps; ++k) {
+        double m;
+        std::memcpy(&m, &w[2 * k], sizeof(m));
+        if (m > s->worst_errsoi) {
+            s->worst_errsoi = m;
+            s->worst_col = (w[2 * k + 1] != 0x7fffffffull) ? (int32_t)w[2 * k + 1] : 0;
+        }
+    }
+    const ErrWord &e = s->err_host[t.slot];
+    if (e.code != 0 && s->err_code == 0) {
+        s->err_code = e.code;
+        s->err_index = e.first_col;
+    }
+    for (auto &kv : s->members)
+        if (kv.second.state == CTSM_SERVE_FINISHING && kv.second.finish_tick == t.id)
+            kv.second.state = CTSM_SERVE_DONE;
+    (void)hipEventDestroy(t.ev);
+    s->inflight.pop_front();
+    return 0;
+}
+
+int ctsm_serve_progress(ctsm_server *s, int32_t wait)
+{
+    if (!s || wait < 0 || wait > 2) return CTSM_E_ARG;
+    ctsm_ctx *ctx = s->ctx;
+    hipError_t he = hipSetDevice(ctx->device);
+    if (he != hipSuccess) return hip_fail(ctx, he);
+    const bool quiesce = wait == 2;
+    // 1. retire completed ticks (wait 1: at least the oldest one; wait 2: all)
+    bool waited = false;
+    while (!s->inflight.empty()) {
+        const int r = serve_retire(s, quiesce || (wait && !waited));
+        if (r < 0) return -r;
+        if (r > 0) break;
+        waited = true;
+    }
+    if (s->err_code) {
+        ctx->last_code = s->err_code;
+        ctx->last_index = s->err_index;
+        return s->err_code;
+    }
+    if (quiesce) return CTSM_OK;       // nothing admitted, nothing enqueued
+    // 2. admit queued members first-come, while the pool has room
+    while (!s->queue.empty()) {
+        Member &m = s->members[s->queue.front()];
+        const int64_t c0 = s->cols.alloc(m.ncol);
+        if (c0 < 0) break;
+        const int64_t p0 = s->pats.alloc(m.npatch);
+        if (p0 < 0) {
+            s->cols.release(c0, m.ncol);
+            break;
+        }
+        const int64_t f0 = s->pfts.alloc(m.npft);
+        if (f0 < 0) {
+            s->cols.release(c0, m.ncol);
+            s->pats.release(p0, m.npatch);
+            break;
+        }
+        m.col0 = (int32_t)c0;
+        m.pat0 = (int32_t)p0;
+        m.pft0 = (int32_t)f0 + s->pft_first;
+        m.state = CTSM_SERVE_ADMITTED;
+        s->queue.pop_front();
+    }
+    // 3. one more tick over the running members, unless enough are in flight
+    if ((int)s->inflight.size() >= kMaxInflight) return CTSM_OK;
+    int32_t k = s->max_steps;
+    bool any = false;
+    for (auto &kv : s->members)
+        if (kv.second.state == CTSM_SERVE_RUNNING) {
+            k = std::min(k, kv.second.left);
+            any = true;
+        }
+    if (!any) return CTSM_OK;
+    if (s->dirty || s->cur < 0) {
+        const int slot = (s->cur + 1) % kRing;
+        // the slot's previous reader must be done before its buffers are refilled
+        if (s->ring_used[slot] && (he = hipEventSynchronize(s->ring_ev[slot])) != hipSuccess)
+            return hip_fail(ctx, he);
+        int32_t nc = 0, np = 0;
+        for (auto &kv : s->members) {
+            const Member &m = kv.second;
+            if (m.state != CTSM_SERVE_RUNNING) continue;
+            for (int32_t i = 0; i < m.ncol; ++i) s->fc_host[slot][nc++] = s->lay.begc_alloc + m.col0 + i;
+            for (int32_t i = 0; i < m.npatch; ++i) s->fp_host[slot][np++] = s->lay.begp_alloc + m.pat0 + i;
+        }
+        if (nc && (he = hipMemcpyAsync(s->fc_dev[slot], s->fc_host[slot], nc * sizeof(int32_t),
+                                       hipMemcpyHostToDevice, ctx->stream)) != hipSuccess)
+            return hip_fail(ctx, he);
+        if (np && (he = hipMemcpyAsync(s->fp_dev[slot], s->fp_host[slot], np * sizeof(int32_t),
+                                       hipMemcpyHostToDevice, ctx->stream)) != hipSuccess)
+            return hip_fail(ctx, he);
+        s->cur = slot;
+        s->nc = nc;
+        s->np = np;
+        s->dirty = false;
+    }
+    const int slot = s->cur;
+    unsigned long long *bal = s->bal_dev + (size_t)slot * s->max_steps * 2;
+    for (int32_t st = 0; st < k; ++st) {
+        // each member's bounds are its own patches: members finished in
+        // an earlier tick and not yet copied out keep their results
+        int rc = land_step_full(ctx, &s->lay, &s->col, &s->pch, &s->can, &s->sf, s->have_sw ? &s->sw : nullptr,
+                                (s->have_sw || s->have_dr) ? &s->hprm : nullptr, s->have_wt ? &s->wt : nullptr,
+                                s->have_dr ? &s->dr : nullptr, s->nc, s->fc_dev[slot], s->np, s->fp_dev[slot],
+                                s->dtime, true);
+        if (rc) return rc;
+        launch_balance(ctx->stream, s->lay, ctx->cfg, s->sf.errsoi_col, s->nc, s->fc_dev[slot], bal + 2 * st);
+    }
+    // the tick's results and error word come back with it; the error word is
+    // then cleared for the next tick
+    if ((he = hipMemcpyAsync(s->bal_host + (size_t)slot * s->max_steps * 2, bal, 2 * k * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
+        (he = hipMemcpyAsync(&s->err_host[slot], ctx->err, sizeof(ErrWord), hipMemcpyDeviceToHost, ctx->stream)) !=
+            hipSuccess)
+        return hip_fail(ctx, he);
+    hipLaunchKernelGGL(k_reset_err, dim3(1), dim3(1), 0, ctx->stream, ctx->err);
+    Tick t;
+    t.id = s->next_tick++;
+    t.slot = slot;
+    t.steps = k;
+    if ((he = hipEventCreateWithFlags(&t.ev, hipEventDisableTiming)) != hipSuccess ||
+        (he = hipEventRecord(t.ev, ctx->stream)) != hipSuccess ||
+        (he = hipEventRecord(s->ring_ev[slot], ctx->stream)) != hipSuccess)
+        return hip_fail(ctx, he);
+    s->ring_used[slot] = true;
+    s->inflight.push_back(t);
+    s->column_steps += (int64_t)k * s->nc;
+    for (auto &kv : s->members) {
+        Member &m = kv.second;
+        if (m.state != CTSM_SERVE_RUNNING) continue;
+        m.left -= k;
+        if (m.left == 0) {
+            m.state = CTSM_SERVE_FINISHING;
+            m.finish_tick = t.id;
+            s->dirty = true;
+        }
+    }
+    return finish_launch(ctx);
+}
+
+int ctsm_serve_stats(const ctsm_server *s, int64_t *column_steps, double *worst_errsoi, int32_t *worst_col,
+                     int32_t *inflight)
+{
+    if (!s) return CTSM_E_ARG;
+    if (column_steps) *column_steps = s->column_steps;
+    if (worst_errsoi) *worst_errsoi = s->worst_errsoi;
+    if (worst_col) *worst_col = s->worst_col;
+    if (inflight) *inflight = (int32_t)s->inflight.size();
+    return CTSM_OK;
+}
+
+int ctsm_serve_destroy(ctsm_server *s)
+{
+    if (!s) return CTSM_OK;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    serve_free(s);
+    return CTSM_OK;
 }

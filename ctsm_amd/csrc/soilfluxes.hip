@@ -19,6 +19,7 @@
 // Every expression keeps the reference's operand order; integer powers by
 // squaring as the oracle writes them.
 #include "common.h"
+#include "trace.h"
 
 namespace ctsm {
 namespace sflux {
@@ -217,6 +218,69 @@ __global__ void __launch_bounds__(256) k_balance_loc(SView V, const double *__re
     if ((unsigned long long)__double_as_longlong(a) == w[0]) atomicMin(&w[1], (unsigned long long)c);
 }
 
+// EnergyBalanceCheck's patch errors (BalanceCheckMod.F90:966-1003), non-urban
+// patches, left to right as the reference writes them; 0 when inactive.
+// w[0..1]: max |errseb| (bits) and its patch, w[2..3]: the same for errlon.
+__device__ __forceinline__ void energy_errors(const SView &V, const ctsm_col_state &S, const ctsm_patch_state &Pt,
+                                              const ctsm_canopy_state &Cn, const ctsm_soilflux_state &F, int p,
+                                              double &seb, double &lon)
+{
+    const long pp = V.p1(p);
+    seb = 0.0;
+    lon = 0.0;
+    if (Cn.active[pp]) {
+        const long cc = V.c1(Cn.column[pp]);
+        seb = Cn.sabv[pp] + Pt.sabg_chk[pp] + S.forc_lwrad[cc] - F.eflx_lwrad_out[pp] - F.eflx_sh_tot[pp] -
+              F.eflx_lh_tot[pp] - F.eflx_soil_grnd[pp] - Cn.dhsdt_canopy[pp];
+        lon = F.eflx_lwrad_out[pp] - F.eflx_lwrad_net[pp] - S.forc_lwrad[cc];
+    }
+}
+
+__global__ void k_energy_init(unsigned long long *w)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        w[0] = 0ull;
+        w[1] = 0x7fffffffull;
+        w[2] = 0ull;
+        w[3] = 0x7fffffffull;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_energy_max(SView V, ctsm_col_state S, ctsm_patch_state Pt,
+                                                    ctsm_canopy_state Cn, ctsm_soilflux_state F, int n,
+                                                    const int32_t *__restrict__ filter, double *errseb,
+                                                    double *errlon, unsigned long long *w, ErrWord *err)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int p = filter ? filter[i] : V.begp + i;
+    if (p < V.begp || p >= V.begp + V.ldp) { report_error(err, p, CTSM_E_ARG); return; }
+    const int c = Cn.column[V.p1(p)];
+    if (Cn.active[V.p1(p)] && (c < V.begc || c >= V.begc + V.ldc)) { report_error(err, p, CTSM_E_ARG); return; }
+    double seb, lon;
+    energy_errors(V, S, Pt, Cn, F, p, seb, lon);
+    if (errseb) errseb[i] = seb;
+    if (errlon) errlon[i] = lon;
+    atomicMax(&w[0], (unsigned long long)__double_as_longlong(fabs(seb)));
+    atomicMax(&w[2], (unsigned long long)__double_as_longlong(fabs(lon)));
+}
+
+__global__ void __launch_bounds__(256) k_energy_loc(SView V, ctsm_col_state S, ctsm_patch_state Pt,
+                                                    ctsm_canopy_state Cn, ctsm_soilflux_state F, int n,
+                                                    const int32_t *__restrict__ filter, unsigned long long *w)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int p = filter ? filter[i] : V.begp + i;
+    if (p < V.begp || p >= V.begp + V.ldp) return;
+    const int c = Cn.column[V.p1(p)];
+    if (Cn.active[V.p1(p)] && (c < V.begc || c >= V.begc + V.ldc)) return;
+    double seb, lon;
+    energy_errors(V, S, Pt, Cn, F, p, seb, lon);
+    if ((unsigned long long)__double_as_longlong(fabs(seb)) == w[0]) atomicMin(&w[1], (unsigned long long)p);
+    if ((unsigned long long)__double_as_longlong(fabs(lon)) == w[2]) atomicMin(&w[3], (unsigned long long)p);
+}
+
 }  // namespace sflux
 
 static sflux::SView sview(const ctsm_layout &L, const ctsm_config &cfg)
@@ -235,6 +299,7 @@ void launch_soilfluxes(hipStream_t st, const ctsm_layout &L, const ctsm_config &
                        const ctsm_patch_state &P, const ctsm_soilflux_state &F, int nc, const int32_t *filterc,
                        int np, const int32_t *filterp, double dtime, ErrWord *err)
 {
+    const GptlRange range("bgp2");
     const sflux::SView V = sview(L, cfg);
     if (np > 0)
         hipLaunchKernelGGL(sflux::k_soilfluxes, dim3((np + 255) / 256), dim3(256), 0, st, V, S, P, F, np, filterp,
@@ -247,11 +312,25 @@ void launch_soilfluxes(hipStream_t st, const ctsm_layout &L, const ctsm_config &
 void launch_balance(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg, const double *errsoi_col, int n,
                     const int32_t *filter, unsigned long long *w)
 {
+    const GptlRange range("balchk");
     const sflux::SView V = sview(L, cfg);
     hipLaunchKernelGGL(sflux::k_balance_init, dim3(1), dim3(64), 0, st, w);
     if (n <= 0) return;
     hipLaunchKernelGGL(sflux::k_balance_max, dim3((n + 255) / 256), dim3(256), 0, st, V, errsoi_col, n, filter, w);
     hipLaunchKernelGGL(sflux::k_balance_loc, dim3((n + 255) / 256), dim3(256), 0, st, V, errsoi_col, n, filter, w);
+}
+
+void launch_energy_balance(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg, const ctsm_col_state &S,
+                           const ctsm_patch_state &P, const ctsm_canopy_state &Cn, const ctsm_soilflux_state &F, int n,
+                           const int32_t *filter, double *errseb, double *errlon, unsigned long long *w, ErrWord *err)
+{
+    const GptlRange range("balchk");
+    const sflux::SView V = sview(L, cfg);
+    hipLaunchKernelGGL(sflux::k_energy_init, dim3(1), dim3(64), 0, st, w);
+    if (n <= 0) return;
+    hipLaunchKernelGGL(sflux::k_energy_max, dim3((n + 255) / 256), dim3(256), 0, st, V, S, P, Cn, F, n, filter,
+                       errseb, errlon, w, err);
+    hipLaunchKernelGGL(sflux::k_energy_loc, dim3((n + 255) / 256), dim3(256), 0, st, V, S, P, Cn, F, n, filter, w);
 }
 
 }  // namespace ctsm

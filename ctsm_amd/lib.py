@@ -42,7 +42,10 @@ EXPORTS = [
     "ctsm_graph_capture_full", "ctsm_set_phs_params", "ctsm_phs_soil_root_conductance",
     "ctsm_phs_soil_root_conductance_dev", "ctsm_phs_root_sink", "ctsm_phs_root_sink_dev",
     "ctsm_phs_calcstress", "ctsm_phs_calcstress_dev", "ctsm_canopyfluxes_dev_noalloc",
+    "ctsm_energy_balance_check_dev", "ctsm_set_pft_rows", "ctsm_serve_create", "ctsm_serve_submit", "ctsm_serve_progress", "ctsm_serve_poll",
+    "ctsm_serve_start", "ctsm_serve_release", "ctsm_serve_resume_member", "ctsm_serve_stats", "ctsm_serve_destroy",
 ]
+SERVE_QUEUED, SERVE_ADMITTED, SERVE_RUNNING, SERVE_DONE = range(4)
 
 
 class CtsmError(RuntimeError):
@@ -133,6 +136,23 @@ def load():
     L.ctsm_root_sink.argtypes = can + [C.c_int32, abi.i32p]
     L.ctsm_canopyfluxes_dev.argtypes = can + [C.c_int32, C.c_void_p, C.c_double]
     L.ctsm_canopyfluxes_dev_noalloc.argtypes = can + [C.c_int32, C.c_void_p, C.c_double]
+    L.ctsm_set_pft_rows.argtypes = [C.c_void_p, C.c_int32, P(abi.PftParams)]
+    L.ctsm_energy_balance_check_dev.argtypes = [C.c_void_p, P(abi.Layout), P(abi.ColState), P(abi.PatchState),
+                                                P(abi.CanopyState), P(abi.SoilFluxState), C.c_int32, C.c_void_p,
+                                                C.c_void_p, C.c_void_p, P(C.c_double), P(C.c_int32),
+                                                P(C.c_double), P(C.c_int32)]
+    L.ctsm_serve_create.argtypes = [C.c_void_p, P(abi.Layout), P(abi.ColState), P(abi.PatchState),
+                                    P(abi.CanopyState), P(abi.SoilFluxState), P(abi.SurfaceWaterState),
+                                    P(abi.HydroParams), P(abi.WaterTableState), P(abi.DrainageState), C.c_int32,
+                                    C.c_int32, C.c_double, P(C.c_void_p)]
+    L.ctsm_serve_submit.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P(C.c_int64)]
+    L.ctsm_serve_progress.argtypes = [C.c_void_p, C.c_int32]
+    L.ctsm_serve_poll.argtypes = [C.c_void_p, C.c_int64] + [P(C.c_int32)] * 5
+    L.ctsm_serve_start.argtypes = [C.c_void_p, C.c_int64]
+    L.ctsm_serve_release.argtypes = [C.c_void_p, C.c_int64]
+    L.ctsm_serve_resume_member.argtypes = [C.c_void_p] + [C.c_int32] * 7 + [P(C.c_int64)]
+    L.ctsm_serve_stats.argtypes = [C.c_void_p, P(C.c_int64), P(C.c_double), P(C.c_int32), P(C.c_int32)]
+    L.ctsm_serve_destroy.argtypes = [C.c_void_p]
     L.ctsm_root_sink_dev.argtypes = can + [C.c_int32, C.c_void_p]
     L.ctsm_land_step_dev.argtypes = col + [P(abi.PatchState), P(abi.CanopyState), C.c_int32, C.c_void_p,
                                            C.c_int32, C.c_void_p, C.c_double]
@@ -225,6 +245,89 @@ class Context:
             setattr(pft, name, a.ctypes.data_as(abi._CT[t]))
             pft.npft = len(a)
         self._check(self.L.ctsm_set_canopy_params(self.h, C.byref(prm), C.byref(pft)), "ctsm_set_canopy_params")
+
+    def energy_balance_check_dev(self, ds, filterp_dev=None, n=None, errseb_dev=None, errlon_dev=None):
+        """EnergyBalanceCheck's patch errors on a DeviceState after a full land
+        step: ((max |errseb|, patch), (max |errlon|, patch)); per-patch values
+        go to errseb_dev / errlon_dev when given."""
+        cs, ps = ds.structs()
+        n = ds.dims.npatch if n is None else n
+        ms, ml = C.c_double(), C.c_double()
+        i_s, i_l = C.c_int32(), C.c_int32()
+        fp = C.c_void_p(filterp_dev) if filterp_dev is not None else None
+        self._check(self.L.ctsm_energy_balance_check_dev(
+            self.h, C.byref(ds.layout()), C.byref(cs), C.byref(ps), C.byref(ds.canopy_struct()),
+            C.byref(ds.soilflux_struct()), n, fp, C.c_void_p(errseb_dev) if errseb_dev else None,
+            C.c_void_p(errlon_dev) if errlon_dev else None, C.byref(ms), C.byref(i_s), C.byref(ml), C.byref(i_l)),
+            "energy balance check")
+        return (ms.value, i_s.value), (ml.value, i_l.value)
+
+    def set_pft_rows(self, first_row, pft_table):
+        """Rows first_row.. of the context's PFT table from pft_table (a
+        perturbed member's own PFT parameters)."""
+        keep = []
+        pft = abi.PftParams()
+        for name, t in abi.PFT_FIELDS:
+            a = np.ascontiguousarray(pft_table[name], dtype=np.int32 if t == "i32" else np.float64)
+            keep.append(a)
+            setattr(pft, name, a.ctypes.data_as(abi._CT[t]))
+            pft.npft = len(a)
+        self._check(self.L.ctsm_set_pft_rows(self.h, int(first_row), C.byref(pft)), "ctsm_set_pft_rows")
+
+    # ---- the ensemble server (continuous batching in a resident pool) -------
+    def serve_create(self, ds, structs, hydro, water_table, drainage, pft_first, max_steps_per_tick, dtime=1800.0):
+        cs, ps, can, sf = structs
+        sw = ds.surfwater_struct()
+        wt = ds.watertable_struct() if water_table else None
+        dr = ds.drainage_struct() if drainage else None
+        hp = hydro
+        out = C.c_void_p()
+        self._keep_serve = (cs, ps, can, sf, sw, wt, dr, hp)
+        self._check(self.L.ctsm_serve_create(self.h, C.byref(ds.layout()), C.byref(cs), C.byref(ps), C.byref(can),
+                                             C.byref(sf), C.byref(sw), C.byref(hp),
+                                             C.byref(wt) if wt is not None else None,
+                                             C.byref(dr) if dr is not None else None, int(pft_first),
+                                             int(max_steps_per_tick), dtime, C.byref(out)), "ctsm_serve_create")
+        return out.value
+
+    def serve_submit(self, srv, ncol, npatch, npft, nsteps):
+        t = C.c_int64()
+        self._check(self.L.ctsm_serve_submit(srv, ncol, npatch, npft, nsteps, C.byref(t)), "ctsm_serve_submit")
+        return t.value
+
+    def serve_progress(self, srv, wait=False, quiesce=False):
+        """One scheduler turn; quiesce waits for every tick in flight and
+        admits and enqueues nothing."""
+        mode = 2 if quiesce else (1 if wait else 0)
+        self._check(self.L.ctsm_serve_progress(srv, mode), "ctsm_serve_progress")
+
+    def serve_poll(self, srv, ticket):
+        """(state, col0, pat0, pft0, steps left) of a ticket."""
+        st, c0, p0, f0, lf = (C.c_int32() for _ in range(5))
+        self._check(self.L.ctsm_serve_poll(srv, ticket, C.byref(st), C.byref(c0), C.byref(p0), C.byref(f0),
+                                           C.byref(lf)), "ctsm_serve_poll")
+        return st.value, c0.value, p0.value, f0.value, lf.value
+
+    def serve_start(self, srv, ticket):
+        self._check(self.L.ctsm_serve_start(srv, ticket), "ctsm_serve_start")
+
+    def serve_release(self, srv, ticket):
+        self._check(self.L.ctsm_serve_release(srv, ticket), "ctsm_serve_release")
+
+    def serve_resume_member(self, srv, ncol, npatch, npft, left, col0, pat0, pft0):
+        t = C.c_int64()
+        self._check(self.L.ctsm_serve_resume_member(srv, ncol, npatch, npft, left, col0, pat0, pft0, C.byref(t)),
+                    "ctsm_serve_resume_member")
+        return t.value
+
+    def serve_stats(self, srv):
+        cs, w, wc, inf = C.c_int64(), C.c_double(), C.c_int32(), C.c_int32()
+        self._check(self.L.ctsm_serve_stats(srv, C.byref(cs), C.byref(w), C.byref(wc), C.byref(inf)),
+                    "ctsm_serve_stats")
+        return cs.value, w.value, wc.value, inf.value
+
+    def serve_destroy(self, srv):
+        self.L.ctsm_serve_destroy(srv)
 
     def canopyfluxes(self, st, filter_p=None, dtime=1800.0):
         cs, _ = st.structs()

@@ -257,8 +257,11 @@ typedef struct ctsm_canopy_params {
 } ctsm_canopy_params;
 
 /* Per-PFT parameters (pftconMod.F90), host arrays of npft rows indexed by
- * patch%itype (0-based PFT number, as in the reference). */
-#define CTSM_MAX_PFT 80
+ * patch%itype (0-based PFT number, as in the reference).  The device table
+ * holds CTSM_MAX_PFT rows: the reference's PFTs take the first ones and an
+ * ensemble server gives each resident perturbed member private rows after
+ * them (ctsm_set_pft_rows, itype renumbered like the column indices). */
+#define CTSM_MAX_PFT 2048
 typedef struct ctsm_pft_params {
     int32_t npft;
     int32_t pad;
@@ -816,6 +819,22 @@ int ctsm_soilfluxes_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_st
 int ctsm_soil_balance_check_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_soilflux_state *sf,
                                 int32_t num_filterc, const int32_t *filterc_dev,
                                 double *errsoi_max, int32_t *index);
+/* The patch part of EnergyBalanceCheck (BalanceCheckMod.F90:966-1003) on the
+ * device-resident state after a full land step, for non-urban patches:
+ *   errseb = sabv + sabg_chk + forc_lwrad - eflx_lwrad_out - eflx_sh_tot
+ *            - eflx_lh_tot - eflx_soil_grnd - dhsdt_canopy
+ *   errlon = eflx_lwrad_out - eflx_lwrad_net - forc_lwrad
+ * (0 for inactive patches), written per filter entry to the device arrays
+ * errseb / errlon when they are not NULL; waits and returns the largest
+ * |errseb| and |errlon| with the lowest patch holding each.  The caller
+ * applies the reference's thresholds (1e-7 W/m2 warning, 1e-5 W/m2 abort
+ * after the skip steps).  errsol checks SurfaceRadiation's fsa / fsr, which
+ * are not on this path. */
+int ctsm_energy_balance_check_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                                  const ctsm_patch_state *pch, const ctsm_canopy_state *can,
+                                  const ctsm_soilflux_state *sf, int32_t num_filterp, const int32_t *filterp_dev,
+                                  double *errseb, double *errlon, double *errseb_max, int32_t *seb_patch,
+                                  double *errlon_max, int32_t *lon_patch);
 
 /* One land step with canopy: CanopyFluxes over the patch filter, then per
  * column SoilTemperature, SetSoilWaterFractions, the transpiration sink and
@@ -962,6 +981,73 @@ int ctsm_sync(ctsm_ctx *ctx);
 /* The context's HIP stream (hipStream_t), for callers that order their own work
  * after the library's, and for timing with events on that stream. */
 void *ctsm_stream(ctsm_ctx *ctx);
+
+/* ---- PFT rows of one member (no counterpart in the reference) -------------
+ * Copies pft's npft rows into rows first_row .. first_row + npft - 1 of the
+ * context's PFT table (after ctsm_set_canopy_params), waiting for the copy:
+ * a perturbed-parameter member runs on its own rows. */
+int ctsm_set_pft_rows(ctsm_ctx *ctx, int32_t first_row, const ctsm_pft_params *pft);
+
+/* ---- ensemble server: continuous batching (no counterpart in the reference)
+ * The reference runs one configuration per job; this serves many small
+ * perturbed-parameter members ("requests") in one resident pool, the
+ * per-clump call of clm_driver.F90:525 repeated over whichever members are
+ * resident.  The pool is device state of ldc columns / ldp patches /
+ * gridcells (one per column) that the caller allocated (ctsm_dev_alloc) and
+ * describes once; the server hands out slots -- contiguous column, patch and
+ * PFT-row ranges, first fit, so admission is by the pool's HBM budget --
+ * and steps every running member together with full land steps
+ * (ctsm_land_step_full_dev over the running columns and patches) followed by
+ * the soil energy balance check.  Data movement stays with the caller, who
+ * knows its arrays:
+ *   submit -> (progress admits it: state ADMITTED, slot in poll's col0 /
+ *   pat0 / pft0) -> the caller copies the member's state into the slot on the
+ *   context's stream, renumbering patchi / patchf / column / gridcell /
+ *   itype by the slot offsets -> start -> (progress runs its steps: RUNNING)
+ *   -> DONE once the last step has completed on the device -> the caller
+ *   copies the state out -> release (frees the slot; also cancels a QUEUED
+ *   or ADMITTED member).
+ * progress never waits for the device unless wait != 0 (wait 1: for the
+ * oldest tick in flight): it retires completed ticks (event queries), admits
+ * queued members first-come while their ranges fit, and enqueues the next tick --
+ * min(max_steps_per_tick, the fewest steps any running member still needs)
+ * steps -- while fewer than two ticks are in flight, so admission and
+ * retirement overlap the device's work.  Columns are independent: a member's
+ * results do not depend on what shares its ticks.  wait 2 quiesces the
+ * server: it waits for every tick in flight and retires them, and admits and
+ * enqueues nothing (the pool then holds whole steps of every running member,
+ * for a checkpoint).  Not thread-safe: one host thread drives a server. */
+#define CTSM_SERVE_QUEUED    0
+#define CTSM_SERVE_ADMITTED  1
+#define CTSM_SERVE_RUNNING   2
+#define CTSM_SERVE_DONE      3
+#define CTSM_SERVE_FINISHING 4   /* internal: last step enqueued (poll reports RUNNING) */
+typedef struct ctsm_server ctsm_server;
+/* sw / hprm / wt / dr as for ctsm_land_step_full_dev (NULL leaves that part
+ * out); members take PFT rows from pft_first on; dtime for every step. */
+int ctsm_serve_create(ctsm_ctx *ctx, const ctsm_layout *pool, const ctsm_col_state *col,
+                      const ctsm_patch_state *pch, const ctsm_canopy_state *can, const ctsm_soilflux_state *sf,
+                      const ctsm_surface_water_state *sw, const ctsm_hydro_params *hprm,
+                      const ctsm_water_table_state *wt, const ctsm_drainage_state *dr, int32_t pft_first,
+                      int32_t max_steps_per_tick, double dtime, ctsm_server **out);
+int ctsm_serve_submit(ctsm_server *s, int32_t ncol, int32_t npatch, int32_t npft, int32_t nsteps, int64_t *ticket);
+int ctsm_serve_progress(ctsm_server *s, int32_t wait);
+/* state (CTSM_SERVE_*), once admitted the slot -- 0-based offsets of the
+ * member's first column / patch in the pool and its first PFT row -- and
+ * the steps not yet enqueued (any pointer but state may be NULL). */
+int ctsm_serve_poll(const ctsm_server *s, int64_t ticket, int32_t *state, int32_t *col0, int32_t *pat0,
+                    int32_t *pft0, int32_t *left);
+int ctsm_serve_start(ctsm_server *s, int64_t ticket);
+int ctsm_serve_release(ctsm_server *s, int64_t ticket);
+/* A RUNNING member at a given slot with `left` steps to go (a pool resumed
+ * from a checkpoint). */
+int ctsm_serve_resume_member(ctsm_server *s, int32_t ncol, int32_t npatch, int32_t npft, int32_t left,
+                             int32_t col0, int32_t pat0, int32_t pft0, int64_t *ticket);
+/* column-steps run, the largest |errsoi| seen (W/m2) and its pool column,
+ * ticks in flight */
+int ctsm_serve_stats(const ctsm_server *s, int64_t *column_steps, double *worst_errsoi, int32_t *worst_col,
+                     int32_t *inflight);
+int ctsm_serve_destroy(ctsm_server *s);
 
 #ifdef __cplusplus
 }

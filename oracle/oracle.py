@@ -272,6 +272,9 @@ def _sf_lib():
                                         P(abi.SoilFluxState), C.c_int, abi.i32p, C.c_int, abi.i32p, C.c_double]
         L.oracle_soil_balance.argtypes = [P(abi.Layout), P(abi.SoilFluxState), C.c_int, abi.i32p, abi.f64p,
                                           P(C.c_int)]
+        L.oracle_energy_balance.argtypes = [P(abi.Layout), P(abi.ColState), P(abi.PatchState), P(abi.CanopyState),
+                                            P(abi.SoilFluxState), C.c_int, abi.i32p, abi.f64p, abi.f64p,
+                                            abi.f64p, P(C.c_int), abi.f64p, P(C.c_int)]
         L._sf_ready = True
     return L
 
@@ -306,6 +309,21 @@ def soil_balance(st, filter_c=None):
     idx = C.c_int(0)
     L.oracle_soil_balance(C.byref(st.layout()), C.byref(sf), len(f), fp, C.byref(m), C.byref(idx))
     return m.value, idx.value
+
+
+def energy_balance(st, filter_p=None):
+    """EnergyBalanceCheck's patch errors (BalanceCheckMod.F90:966-1003):
+    (errseb, errlon per filter entry, (max |errseb|, patch), (max |errlon|, patch))."""
+    L = _sf_lib()
+    cs, ps = st.structs()
+    f, fp = _filt(filter_p if filter_p is not None else np.arange(1, st.dims.npatch + 1))
+    seb, lon = np.zeros(len(f)), np.zeros(len(f))
+    ms, ml = C.c_double(0.0), C.c_double(0.0)
+    i_s, i_l = C.c_int(0), C.c_int(0)
+    L.oracle_energy_balance(C.byref(st.layout()), C.byref(cs), C.byref(ps), C.byref(st.canopy_struct()),
+                            C.byref(st.soilflux_struct()), len(f), fp, seb.ctypes.data_as(abi.f64p),
+                            lon.ctypes.data_as(abi.f64p), C.byref(ms), C.byref(i_s), C.byref(ml), C.byref(i_l))
+    return seb, lon, (ms.value, i_s.value), (ml.value, i_l.value)
 
 
 def _hy_lib():

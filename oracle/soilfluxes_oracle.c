@@ -261,3 +261,37 @@ void oracle_soil_balance(const ctsm_layout *L, const ctsm_soilflux_state *F, int
     *errmax = m < 0.0 ? 0.0 : m;
     *index = idx;
 }
+
+/* EnergyBalanceCheck's patch errors (BalanceCheckMod.F90:966-1003) for
+ * non-urban patches: errseb = sabv + sabg_chk + forc_lwrad - eflx_lwrad_out
+ * - eflx_sh_tot - eflx_lh_tot - eflx_soil_grnd - dhsdt_canopy and errlon =
+ * eflx_lwrad_out - eflx_lwrad_net - forc_lwrad, left to right as written;
+ * 0 for inactive patches.  Writes errseb[fp] / errlon[fp] per filter entry
+ * and the max |err| with the lowest patch index holding it (ties). */
+void oracle_energy_balance(const ctsm_layout *L, const ctsm_col_state *S, const ctsm_patch_state *Pt,
+                           const ctsm_canopy_state *Cn, const ctsm_soilflux_state *F, int num_p, const int *filter_p,
+                           double *errseb, double *errlon, double *seb_max, int *seb_patch, double *lon_max,
+                           int *lon_patch)
+{
+    double ms = -1.0, ml = -1.0;
+    int is = 0, il = 0, fp;
+    for (fp = 0; fp < num_p; ++fp) {
+        const int p = filter_p ? filter_p[fp] : L->begp_alloc + fp;
+        const long pp = p - L->begp_alloc;
+        double seb = 0.0, lon = 0.0;
+        if (Cn->active[pp]) {
+            const long cc = Cn->column[pp] - L->begc_alloc;
+            seb = Cn->sabv[pp] + Pt->sabg_chk[pp] + S->forc_lwrad[cc] - F->eflx_lwrad_out[pp] - F->eflx_sh_tot[pp]
+                  - F->eflx_lh_tot[pp] - F->eflx_soil_grnd[pp] - Cn->dhsdt_canopy[pp];
+            lon = F->eflx_lwrad_out[pp] - F->eflx_lwrad_net[pp] - S->forc_lwrad[cc];
+        }
+        errseb[fp] = seb;
+        errlon[fp] = lon;
+        if (fabs(seb) > ms || (fabs(seb) == ms && p < is)) { ms = fabs(seb); is = p; }
+        if (fabs(lon) > ml || (fabs(lon) == ml && p < il)) { ml = fabs(lon); il = p; }
+    }
+    *seb_max = ms < 0.0 ? 0.0 : ms;
+    *seb_patch = is;
+    *lon_max = ml < 0.0 ? 0.0 : ml;
+    *lon_patch = il;
+}

@@ -14,11 +14,12 @@ def test_members_in_the_mix_equal_members_alone():
     ctx = Context()
     ctx.set_canopy_params(S.canopy_params(), S.pft_table())
     reqs = serve.make_requests(6, rate=1e6, seed=21, max_cols=300, max_days=3, steps_per_day=2)
-    print("case: ensemble server, 6 members of up to 300 columns, pool 2048 columns", flush=True)
+    print("case: ensemble server through the C-ABI scheduler, 6 perturbed members of up to 300 columns, "
+          "pool 2048 columns", flush=True)
     alone = {}
     for q in reqs:
         serve.build_member(q)
-        alone[q.rid] = q.state.copy()
+        alone[q.rid] = (q.state.copy(), q.pft)
     for q in reqs:
         q.state = None
     try:
@@ -29,10 +30,10 @@ def test_members_in_the_mix_equal_members_alone():
     assert len(done) == len(reqs)
     assert srv.worst_errsoi < 1e-7, srv.worst_errsoi
     cfg = O.canopy_config()
-    pft = O.PftStruct(S.pft_table())
     hydro = O.hydro_params()
     for q in done:
-        o = alone[q.rid]
+        o, member_pft = alone[q.rid]
+        pft = O.PftStruct(member_pft)          # the member alone, on its own PFT table
         for _ in range(q.steps):
             rc, bad = O.land_step_full(o, cfg, S.canopy_params(), pft, hydro=hydro, water_table_after=True,
                                        drainage_after=True)
@@ -41,16 +42,6 @@ def test_members_in_the_mix_equal_members_alone():
             a, b = getattr(q.state, grp), getattr(o, grp)
             for k in b:
                 np.testing.assert_array_equal(a[k], b[k], err_msg=f"member {q.rid} {grp}.{k}")
-
-
-def _drain(srv):
-    out = {}
-    while srv.active:
-        srv.tick()
-        for q in [q for q in srv.active if q.left <= 0]:
-            srv.retire(q)
-            out[q.rid] = q.state
-    return out
 
 
 def test_served_pool_resumes_from_a_checkpoint(tmp_path):
@@ -64,22 +55,27 @@ def test_served_pool_resumes_from_a_checkpoint(tmp_path):
         srv = serve.EnsembleServer(ctx, 1024, 1024 * 17, max_steps_per_tick=1)
         for q in reqs:
             serve.build_member(q)
-            assert srv.admit(q)
-        for _ in range(2):
-            srv.tick()
+            srv.submit(q)
+        for _ in range(3):
+            srv.pump(wait=True)
         srv.checkpoint(path)
-        a = _drain(srv)
+        srv.drain()
+        a = {q.rid: q.state for q in srv.finished}
         srv.close()
         again = {}
         for q in serve.make_requests(4, rate=1e6, seed=23, max_cols=200, max_days=2, steps_per_day=4):
             serve.build_member(q)
             again[q.rid] = q
         srv2 = serve.EnsembleServer.resume(ctx, path, again)
-        b = _drain(srv2)
+        srv2.drain()
+        b = {q.rid: q.state for q in srv2.finished}
         assert srv2.worst_errsoi < 1e-7
         srv2.close()
     finally:
         ctx.close()
+    # every member was still running at the checkpoint (4 to 8 steps each, at
+    # most 2 enqueued): the resumed pool finishes all of them as the
+    # uninterrupted one does
     assert set(a) == set(b) == {q.rid for q in reqs}
     for rid in a:
         for grp in ("col", "pch", "can", "sf", "hy"):

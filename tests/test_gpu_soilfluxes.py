@@ -105,3 +105,38 @@ def test_land_step_full_bitwise(sctx, nsteps, name):
         assert rc == 0, bad
     _same(g, o)
     assert worst < 1e-7, worst             # the reference warns at 1e-5 W/m2
+
+
+def test_energy_balance_check_dev(sctx):
+    """EnergyBalanceCheck's patch errors on the device after full land steps:
+    per-patch errseb / errlon bit for bit with the oracle's on the same
+    state, through a descending patch filter with inactive patches, the max
+    and its patch as the oracle finds them, and both closed below 1e-9 W/m2."""
+    st = S.canopy_workload("canopy_uneven", ncol=700)
+    st.can["active"][::9] = 0
+    print(f"case: energy balance check after 2 full land steps, 700 columns / {st.dims.npatch} patches", flush=True)
+    hydro = S.hydro_params()
+    ds = DeviceState(st, sctx)
+    filt = np.arange(st.dims.npatch, 0, -1, dtype=np.int32)
+    fdev = sctx.alloc(filt.nbytes)
+    seb_dev = sctx.alloc(8 * len(filt))
+    lon_dev = sctx.alloc(8 * len(filt))
+    try:
+        for _ in range(2):
+            sctx.land_step_full_dev(ds, hydro=hydro, water_table=True, drainage=True)
+        sctx.h2d(fdev, filt)
+        (ms, ps), (ml, pl) = sctx.energy_balance_check_dev(ds, filterp_dev=fdev, n=len(filt), errseb_dev=seb_dev,
+                                                           errlon_dev=lon_dev)
+        seb, lon = np.empty(len(filt)), np.empty(len(filt))
+        sctx.d2h(seb, seb_dev)
+        sctx.d2h(lon, lon_dev)
+        g = ds.to_host()
+    finally:
+        for p in (fdev, seb_dev, lon_dev):
+            sctx.free(p)
+        ds.free()
+    o_seb, o_lon, (oms, ops), (oml, opl) = O.energy_balance(g, filter_p=filt)
+    np.testing.assert_array_equal(seb, o_seb)
+    np.testing.assert_array_equal(lon, o_lon)
+    assert (ms, ps) == (oms, ops) and (ml, pl) == (oml, opl)
+    assert ms < 1e-9 and ml < 1e-9, (ms, ml)

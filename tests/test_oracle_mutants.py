@@ -83,6 +83,12 @@ MUTANTS = [
     ("under-canopy wind 0.3 u_m / u*", "canopy_oracle.c",
      "fmin90(0.4, (0.03 * um / ustar))", "fmin90(0.4, (0.3 * um / ustar))"),
     ("QSat: a5 digits transposed", "canopy_oracle.c", "0.196237241e-07", "0.196237421e-07"),
+    ("errseb without the canopy heat storage (BalanceCheckMod.F90:989-990)", "soilfluxes_oracle.c",
+     "- F->eflx_lh_tot[pp] - F->eflx_soil_grnd[pp] - Cn->dhsdt_canopy[pp];",
+     "- F->eflx_lh_tot[pp] - F->eflx_soil_grnd[pp];"),
+    ("errseb with the ground heat flux's sign flipped", "soilfluxes_oracle.c",
+     "- F->eflx_lh_tot[pp] - F->eflx_soil_grnd[pp] - Cn->dhsdt_canopy[pp];",
+     "- F->eflx_lh_tot[pp] + F->eflx_soil_grnd[pp] - Cn->dhsdt_canopy[pp];"),
 ]
 
 CFLAGS = ["-O2", "-mfma", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-std=c99", "-w"]

@@ -516,3 +516,31 @@ def test_qsat_flatau_table():
         vp = 1.0 / (p[i] - 0.378 * e)
         assert qs[i] == pytest.approx(0.622 * e * vp, rel=4e-15 + ebound / e), t
         assert dqs[i] == pytest.approx(de * 0.622 * vp * vp * p[i], rel=6e-15 + debound / de + 2 * ebound / e), t
+
+
+# ---- EnergyBalanceCheck's patch errors (BalanceCheckMod.F90:966-1003) --------
+
+@pytest.mark.parametrize("workload", ["canopy", "canopy_uneven"])
+def test_energy_balance_errors_close_after_land_steps(workload):
+    """errseb and errlon are energy residuals of the step: after full land
+    steps (CanopyFluxes, SoilTemperature, SoilFluxes, hydrology) they close
+    to round-off (the reference warns at 1e-7 W/m2), so a dropped term or a
+    wrong sign in the check -- each term is tens to hundreds of W/m2 -- fails
+    here; inactive patches report 0 and the max / maxloc are those of the
+    arrays (lowest patch on ties)."""
+    from ctsm_amd import synthetic as S
+    st = S.canopy_workload(workload, ncol=192)
+    st.can["active"][::7] = 0
+    cfg, prm, pft = O.canopy_config(), S.canopy_params(), O.PftStruct(S.pft_table())
+    for _ in range(2):
+        rc, bad = O.land_step_full(st, cfg, prm, pft, hydro=S.hydro_params(), water_table_after=True,
+                                   drainage_after=True)
+        assert rc == 0, bad
+    seb, lon, (ms, ps), (ml, pl) = O.energy_balance(st)
+    act = st.can["active"] != 0
+    assert np.all(seb[~act] == 0.0) and np.all(lon[~act] == 0.0)
+    assert np.max(np.abs(seb)) < 1e-9 and np.max(np.abs(lon)) < 1e-11
+    assert ms == np.max(np.abs(seb)) and ps == int(np.argmax(np.abs(seb))) + 1
+    assert ml == np.max(np.abs(lon)) and pl == int(np.argmax(np.abs(lon))) + 1
+    # the terms are not small: the closure is a real check
+    assert np.max(np.abs(st.sf["eflx_soil_grnd"][act])) > 10.0 and np.max(np.abs(st.can["dhsdt_canopy"][act])) > 1e-3
