@@ -42,7 +42,15 @@ HBM_PEAK_GBS = 8000.0
 # (kernel, workload, columns), with the launch shape they were measured at.
 # The bench reports them only for a run of that workload, size and launch
 # shape; any other run reports null.
-PROFILED = {}
+PROFILED = {
+    # profiles/canopy_rounds_canopy_uneven1048576.md: FETCH 9,303,183 KiB x 2.0 + WRITE 19,185,725 KiB
+    # x 1.0 per call; 535.6 GFLOP FP64 in 53.50 ms
+    ("k_canopy_round", "canopy_uneven", 1048576): {
+        "traffic": (9303183 * 2.0 + 19185725 * 1.0) * 1024,
+        "fp64_valu_frac": 535.6e9 / 53.50e-3 / 78.6e12,
+        "tuning": {"canopy_first_iterations": 6, "canopy_round_iterations": 6, "canopy_waves": 1},
+    },
+}
 
 
 def richards_bytes(st):
@@ -130,7 +138,7 @@ def served_ensemble(ctx, args):
                        f"parameters incl. the stomatal slope on the member's own PFT rows), 1-365 days "
                        f"(log-uniform) x 48 steps x {args.serve_day_scale} ({int(steps.min())}-{int(steps.max())} "
                        f"steps drawn); full land step (CanopyFluxes, SoilTemperature, SoilFluxes, surface water, "
-                       f"SoilWater, water tables, drainage) and the soil balance check every step",
+                       f"SoilWater, water tables, drainage) and the energy balance checks (errsoi, errseb, errlon) every step",
            "pool_columns": args.serve_pool_cols, "pool_bytes_per_column": srv.bytes_per_column(),
            "completed": len(done), "wall_s": wall,
            "latency_p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
@@ -138,7 +146,9 @@ def served_ensemble(ctx, args):
            "queue_wait_p50_s": float(np.percentile(wait, 50)) if len(wait) else None,
            "queue_wait_p99_s": float(np.percentile(wait, 99)) if len(wait) else None,
            "column_steps_per_s": srv.column_steps / wall,
-           "max_abs_errsoi_Wm2": srv.worst_errsoi}
+           "max_abs_errsoi_Wm2": srv.worst_errsoi,
+           "max_abs_errseb_Wm2": srv.worst_energy_errors[0],
+           "max_abs_errlon_Wm2": srv.worst_energy_errors[1]}
     srv.close()
     return out
 

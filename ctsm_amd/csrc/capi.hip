@@ -2073,6 +2073,9 @@ struct Member {
 
 constexpr int kRing = 3;          // filter buffers (at most kMaxInflight ticks use them at once)
 constexpr int kMaxInflight = 2;   // ticks enqueued and not yet retired
+// balance words per step: |errsoi| max and its column, |errseb| max and its
+// patch, |errlon| max and its patch
+constexpr int kBalWords = 6;
 
 struct Tick {
     int64_t id = 0;
@@ -2095,6 +2098,7 @@ struct ctsm_server {
     ctsm_drainage_state dr{};
     ctsm_hydro_params hprm{};
     bool have_sw = false, have_wt = false, have_dr = false;
+    bool check_energy = false;        // the state holds what errseb / errlon read
     double dtime = 1800.0;
     int32_t max_steps = 8;
     RangeSet cols, pats, pfts;
@@ -2109,12 +2113,14 @@ struct ctsm_server {
     int cur = -1;
     int32_t nc = 0, np = 0;
     bool dirty = true;
-    // per tick slot: max_steps (max, column) balance words and the error word
+    // per tick slot: max_steps x kBalWords balance words and the error word
     unsigned long long *bal_dev = nullptr, *bal_host = nullptr;
     ErrWord *err_host = nullptr;
     std::deque<Tick> inflight;
     double worst_errsoi = 0.0;
     int32_t worst_col = 0;
+    double worst_errseb = 0.0, worst_errlon = 0.0;
+    int32_t worst_seb_patch = 0, worst_lon_patch = 0;
     int64_t column_steps = 0;
     int32_t err_code = 0, err_index = 0;
 };
@@ -2160,6 +2166,9 @@ int ctsm_serve_create(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_stat
     if (wt) { s->wt = *wt; s->have_wt = true; }
     if (dr) { s->dr = *dr; s->have_dr = true; }
     if (hprm) s->hprm = *hprm;
+    s->check_energy = col->forc_lwrad && pch->sabg_chk && can->sabv && can->dhsdt_canopy && can->active &&
+                      can->column && sf->eflx_lwrad_out && sf->eflx_lwrad_net && sf->eflx_sh_tot &&
+                      sf->eflx_lh_tot && sf->eflx_soil_grnd;
     s->dtime = dtime;
     s->max_steps = max_steps_per_tick;
     s->cols.init(lay->ldc);
@@ -2175,7 +2184,7 @@ int ctsm_serve_create(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_stat
              hipHostMalloc((void **)&s->fp_host[k], (size_t)lay->ldp * sizeof(int32_t), hipHostMallocDefault) ==
                  hipSuccess &&
              hipEventCreateWithFlags(&s->ring_ev[k], hipEventDisableTiming) == hipSuccess;
-    const size_t nbal = (size_t)kRing * max_steps_per_tick * 2;
+    const size_t nbal = (size_t)kRing * max_steps_per_tick * kBalWords;
     ok = ok && hipMalloc((void **)&s->bal_dev, nbal * sizeof(unsigned long long)) == hipSuccess &&
          hipHostMalloc((void **)&s->bal_host, nbal * sizeof(unsigned long long), hipHostMallocDefault) == hipSuccess &&
          hipHostMalloc((void **)&s->err_host, kRing * sizeof(ErrWord), hipHostMallocDefault) == hipSuccess;
@@ -2285,13 +2294,21 @@ static int serve_retire(ctsm_server *s, bool wait)
     hipError_t he = wait ? hipEventSynchronize(t.ev) : hipEventQuery(t.ev);
     if (he == hipErrorNotReady) return 1;
     if (he != hipSuccess) return -hip_fail(s->ctx, he);
-    const unsigned long long *w = s->bal_host + (size_t)t.slot * s->max_steps * 2;
-    for (int k = 0; k < t.steps; ++k) {
+    const unsigned long long *w = s->bal_host + (size_t)t.slot * s->max_steps * kBalWords;
+    auto worse = [](const unsigned long long *x, double &worst, int32_t &where) {
         double m;
-        std::memcpy(&m, &w[2 * k], sizeof(m));
-        if (m > s->worst_errsoi) {
-            s->worst_errsoi = m;
-            s->worst_col = (w[2 * k + 1] != 0x7fffffffull) ? (int32_t)w[2 * k + 1] : 0;
+        std::memcpy(&m, &x[0], sizeof(m));
+        if (m > worst) {
+            worst = m;
+            where = (x[1] != 0x7fffffffull) ? (int32_t)x[1] : 0;
+        }
+    };
+    for (int k = 0; k < t.steps; ++k) {
+        const unsigned long long *x = w + (size_t)k * kBalWords;
+        worse(x, s->worst_errsoi, s->worst_col);
+        if (s->check_energy) {
+            worse(x + 2, s->worst_errseb, s->worst_seb_patch);
+            worse(x + 4, s->worst_errlon, s->worst_lon_patch);
         }
     }
     const ErrWord &e = s->err_host[t.slot];
@@ -2384,7 +2401,7 @@ int ctsm_serve_progress(ctsm_server *s, int32_t wait)
         s->dirty = false;
     }
     const int slot = s->cur;
-    unsigned long long *bal = s->bal_dev + (size_t)slot * s->max_steps * 2;
+    unsigned long long *bal = s->bal_dev + (size_t)slot * s->max_steps * kBalWords;
     for (int32_t st = 0; st < k; ++st) {
         // each member's bounds are its own patches: members finished in
         // an earlier tick and not yet copied out keep their results
@@ -2393,11 +2410,17 @@ int ctsm_serve_progress(ctsm_server *s, int32_t wait)
                                 s->have_dr ? &s->dr : nullptr, s->nc, s->fc_dev[slot], s->np, s->fp_dev[slot],
                                 s->dtime, true);
         if (rc) return rc;
-        launch_balance(ctx->stream, s->lay, ctx->cfg, s->sf.errsoi_col, s->nc, s->fc_dev[slot], bal + 2 * st);
+        // EnergyBalanceCheck after every step (BalanceCheckMod.F90:966-1003, 1101-1119)
+        unsigned long long *b = bal + (size_t)st * kBalWords;
+        launch_balance(ctx->stream, s->lay, ctx->cfg, s->sf.errsoi_col, s->nc, s->fc_dev[slot], b);
+        if (s->check_energy)
+            launch_energy_balance(ctx->stream, s->lay, ctx->cfg, s->col, s->pch, s->can, s->sf, s->np,
+                                  s->fp_dev[slot], nullptr, nullptr, b + 2, ctx->err);
     }
     // the tick's results and error word come back with it; the error word is
     // then cleared for the next tick
-    if ((he = hipMemcpyAsync(s->bal_host + (size_t)slot * s->max_steps * 2, bal, 2 * k * sizeof(unsigned long long),
+    if ((he = hipMemcpyAsync(s->bal_host + (size_t)slot * s->max_steps * kBalWords, bal,
+                             (size_t)kBalWords * k * sizeof(unsigned long long),
                              hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
         (he = hipMemcpyAsync(&s->err_host[slot], ctx->err, sizeof(ErrWord), hipMemcpyDeviceToHost, ctx->stream)) !=
             hipSuccess)
@@ -2428,9 +2451,14 @@ int ctsm_serve_progress(ctsm_server *s, int32_t wait)
 }
 
 int ctsm_serve_stats(const ctsm_server *s, int64_t *column_steps, double *worst_errsoi, int32_t *worst_col,
+                     double *worst_errseb, int32_t *seb_patch, double *worst_errlon, int32_t *lon_patch,
                      int32_t *inflight)
 {
     if (!s) return CTSM_E_ARG;
+    if (worst_errseb) *worst_errseb = s->worst_errseb;
+    if (seb_patch) *seb_patch = s->worst_seb_patch;
+    if (worst_errlon) *worst_errlon = s->worst_errlon;
+    if (lon_patch) *lon_patch = s->worst_lon_patch;
     if (column_steps) *column_steps = s->column_steps;
     if (worst_errsoi) *worst_errsoi = s->worst_errsoi;
     if (worst_col) *worst_col = s->worst_col;

@@ -151,7 +151,8 @@ def load():
     L.ctsm_serve_start.argtypes = [C.c_void_p, C.c_int64]
     L.ctsm_serve_release.argtypes = [C.c_void_p, C.c_int64]
     L.ctsm_serve_resume_member.argtypes = [C.c_void_p] + [C.c_int32] * 7 + [P(C.c_int64)]
-    L.ctsm_serve_stats.argtypes = [C.c_void_p, P(C.c_int64), P(C.c_double), P(C.c_int32), P(C.c_int32)]
+    L.ctsm_serve_stats.argtypes = [C.c_void_p, P(C.c_int64), P(C.c_double), P(C.c_int32), P(C.c_double),
+                                   P(C.c_int32), P(C.c_double), P(C.c_int32), P(C.c_int32)]
     L.ctsm_serve_destroy.argtypes = [C.c_void_p]
     L.ctsm_root_sink_dev.argtypes = can + [C.c_int32, C.c_void_p]
     L.ctsm_land_step_dev.argtypes = col + [P(abi.PatchState), P(abi.CanopyState), C.c_int32, C.c_void_p,
@@ -321,10 +322,13 @@ class Context:
         return t.value
 
     def serve_stats(self, srv):
+        """(column-steps, worst |errsoi|, its pool column, ticks in flight,
+        worst |errseb|, its pool patch, worst |errlon|, its pool patch)."""
         cs, w, wc, inf = C.c_int64(), C.c_double(), C.c_int32(), C.c_int32()
-        self._check(self.L.ctsm_serve_stats(srv, C.byref(cs), C.byref(w), C.byref(wc), C.byref(inf)),
-                    "ctsm_serve_stats")
-        return cs.value, w.value, wc.value, inf.value
+        seb, sp, lon, lp = C.c_double(), C.c_int32(), C.c_double(), C.c_int32()
+        self._check(self.L.ctsm_serve_stats(srv, C.byref(cs), C.byref(w), C.byref(wc), C.byref(seb), C.byref(sp),
+                                            C.byref(lon), C.byref(lp), C.byref(inf)), "ctsm_serve_stats")
+        return cs.value, w.value, wc.value, inf.value, seb.value, sp.value, lon.value, lp.value
 
     def serve_destroy(self, srv):
         self.L.ctsm_serve_destroy(srv)

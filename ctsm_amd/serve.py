@@ -129,13 +129,23 @@ class EnsembleServer:
     def bytes_per_column(self):
         return self.pool.nbytes() / self.dims.ncol
 
+    def _stats(self):
+        """The scheduler's counters (kept from the moment of close())."""
+        return self.ctx.serve_stats(self.h) if self.h is not None else self._final
+
     @property
     def worst_errsoi(self):
-        return self.ctx.serve_stats(self.h)[1]
+        return self._stats()[1]
 
     @property
     def column_steps(self):
-        return self.ctx.serve_stats(self.h)[0]
+        return self._stats()[0]
+
+    @property
+    def worst_energy_errors(self):
+        """Largest |errseb| and |errlon| (W/m2) over every served step."""
+        s = self._stats()
+        return s[4], s[6]
 
     @property
     def active(self):
@@ -262,7 +272,13 @@ class EnsembleServer:
         return srv
 
     def close(self):
+        """Destroy the scheduler and free the pool; the counters stay
+        readable."""
+        if self.h is None:
+            return
+        self._final = self.ctx.serve_stats(self.h)
         self.ctx.serve_destroy(self.h)
+        self.h = None
         self.pool.free()
 
 

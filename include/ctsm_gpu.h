@@ -997,8 +997,9 @@ int ctsm_set_pft_rows(ctsm_ctx *ctx, int32_t first_row, const ctsm_pft_params *p
  * describes once; the server hands out slots -- contiguous column, patch and
  * PFT-row ranges, first fit, so admission is by the pool's HBM budget --
  * and steps every running member together with full land steps
- * (ctsm_land_step_full_dev over the running columns and patches) followed by
- * the soil energy balance check.  Data movement stays with the caller, who
+ * (ctsm_land_step_full_dev over the running columns and patches) each
+ * followed by the energy balance checks (errsoi per column, errseb / errlon
+ * per patch).  Data movement stays with the caller, who
  * knows its arrays:
  *   submit -> (progress admits it: state ADMITTED, slot in poll's col0 /
  *   pat0 / pft0) -> the caller copies the member's state into the slot on the
@@ -1044,8 +1045,11 @@ int ctsm_serve_release(ctsm_server *s, int64_t ticket);
 int ctsm_serve_resume_member(ctsm_server *s, int32_t ncol, int32_t npatch, int32_t npft, int32_t left,
                              int32_t col0, int32_t pat0, int32_t pft0, int64_t *ticket);
 /* column-steps run, the largest |errsoi| seen (W/m2) and its pool column,
- * ticks in flight */
+ * the largest |errseb| and |errlon| seen (W/m2) and their pool patches (0
+ * when the state lacks a field the energy check reads), ticks in flight;
+ * any pointer may be NULL */
 int ctsm_serve_stats(const ctsm_server *s, int64_t *column_steps, double *worst_errsoi, int32_t *worst_col,
+                     double *worst_errseb, int32_t *seb_patch, double *worst_errlon, int32_t *lon_patch,
                      int32_t *inflight);
 int ctsm_serve_destroy(ctsm_server *s);
 

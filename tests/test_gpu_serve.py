@@ -24,11 +24,15 @@ def test_members_in_the_mix_equal_members_alone():
         q.state = None
     try:
         srv, done, wall = serve.serve(ctx, reqs, cap_cols=2048, cap_patches=2048 * 17, max_steps_per_tick=2)
+        energy = srv.worst_energy_errors
         srv.close()
     finally:
         ctx.close()
     assert len(done) == len(reqs)
     assert srv.worst_errsoi < 1e-7, srv.worst_errsoi
+    # errseb / errlon of every patch after every served step, below the reference's warning
+    seb, lon = energy
+    assert 0.0 <= seb < 1e-7 and 0.0 <= lon < 1e-7, (seb, lon)
     cfg = O.canopy_config()
     hydro = O.hydro_params()
     for q in done:
