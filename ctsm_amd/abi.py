@@ -36,6 +36,8 @@ class Config(C.Structure):
         ("light_inhibit", C.c_int32),
         ("use_hydrstress", C.c_int32),
         ("use_excess_ice", C.c_int32),
+        ("modifyphoto_and_lmr_forcrop", C.c_int32),
+        ("pad1", C.c_int32),
     ]
 
 
@@ -434,7 +436,21 @@ class PhsState(C.Structure):
 
 class PhsParams(C.Structure):
     _fields_ = [("npft", C.c_int32), ("pad", C.c_int32)] + [
-        (n, f64p) for n in ("krmax", "root_radius", "root_density", "froot_leaf", "psi50", "ck", "kmax")]
+        (n, f64p) for n in ("krmax", "root_radius", "root_density", "froot_leaf", "psi50", "ck", "kmax", "crop")]
+
+
+# CanopyFluxes with plant hydraulic stress (ctsm_phs_canopy_state); phs2 =
+# (begp:endp, sun / sha)
+PHS_CANOPY_FIELDS = [
+    ("tlai_z", "f64", "p"), ("local_morning", "i32", "p"), ("vegwp", "f64", "veg"), ("vegwp_ln", "f64", "veg"),
+    ("vegwp_pd", "f64", "veg"), ("bsun", "f64", "p"), ("bsha", "f64", "p"), ("an_sun", "f64", "p"),
+    ("an_sha", "f64", "p"), ("vcmax_z_phs", "f64", "phs2"), ("tpu_z_phs", "f64", "phs2"), ("kp_z_phs", "f64", "phs2"),
+    ("ac_phs", "f64", "phs2"), ("aj_phs", "f64", "phs2"), ("ap_phs", "f64", "phs2"), ("ag_phs", "f64", "phs2"),
+]
+
+
+class PhsCanopyState(C.Structure):
+    _fields_ = [(n, _CT[t]) for n, t, _ in PHS_CANOPY_FIELDS]
 
 
 # calcstress (ctsm_phs_stress_state): patch members, then column members

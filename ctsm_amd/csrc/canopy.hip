@@ -20,8 +20,10 @@
 // iteration and shared by the sun and shade phases (the same values the
 // reference computes twice).
 #include <climits>
+#include <type_traits>
 
 #include "common.h"
+#include "phs_dev.h"
 #include "trace.h"
 
 namespace ctsm {
@@ -621,6 +623,8 @@ __device__ void photosynthesis(const CSwitch &sw, const PRM &P, const PsnConst &
     }
 }
 
+#include "canopy_phs.h"
+
 }  // namespace canopy
 }  // namespace ctsm
 
@@ -628,10 +632,13 @@ namespace ctsm {
 namespace canopy {
 
 // constants of the time step for one patch (PhotosynthesisMod.F90:1525-1722)
+// phs: PhotosynthesisHydraulicStress's constants (:3117-3300): the
+// Ball-Berry minimum conductance without btran (the stress enters through
+// bsun / bsha) and the top-leaf nitrogen capped at 10 g/m2
 template <class PRM>
 __device__ void psn_constants(const CSwitch &sw, const PRM &P, const double *__restrict__ pft, int ivt,
                               double forc_pbot, double oair, double t10, double btran, double dayl_factor,
-                              PsnConst &K, bool &bad)
+                              PsnConst &K, bool &bad, bool phs = false)
 {
     const bool medlyn = sw.stomatal == CTSM_STOMATA_MEDLYN2011;
     K.lmrc = fth25(P.lmrhd, P.lmrse);
@@ -644,7 +651,7 @@ __device__ void psn_constants(const CSwitch &sw, const PRM &P, const double *__r
         K.qe = 0.05;
         if (!medlyn) bbbopt = 40000.0;
     }
-    K.bbb = fmax90(bbbopt * btran, 1.0);
+    K.bbb = phs ? bbbopt : fmax90(bbbopt * btran, 1.0);
     K.mbb = pft[F_MBBOPT * CTSM_MAX_PFT + ivt];
     K.kc25 = P.kc25_coef * forc_pbot;
     K.ko25 = P.ko25_coef * forc_pbot;
@@ -653,6 +660,7 @@ __device__ void psn_constants(const CSwitch &sw, const PRM &P, const double *__r
     const double slatop = pft[F_SLATOP * CTSM_MAX_PFT + ivt], leafcn = pft[F_LEAFCN * CTSM_MAX_PFT + ivt];
     if (slatop * leafcn <= 0.0) bad = true;
     K.lnc = 1.0 / (slatop * leafcn);
+    if (phs) K.lnc = fmin90(K.lnc, 10.0);
     K.vcmax25top = K.lnc * pft[F_FLNR * CTSM_MAX_PFT + ivt] * P.fnr * P.act25 * dayl_factor;
     K.vcmax25top = K.vcmax25top * pft[F_FNITR * CTSM_MAX_PFT + ivt];
     const double t10c = fmin90(fmax90(t10 - TFRZ, 11.0), 35.0);
@@ -715,11 +723,14 @@ static_assert(sizeof(CanopyWork) == N_WORK_FIELDS * sizeof(double *), "one point
 // Monin-Obukhov initialisation); otherwise the iteration state is restored
 // from W.  Returns true when the patch is still iterating (its state saved in
 // W[i0]); false when it finished (outputs written) or failed.
-template <class PRM, class COL, class CAN, class WRK>
+// PHS: CanopyFluxes with plant hydraulic stress, H its PhsArgs (unused
+// otherwise): PhotosynthesisHydraulicStress in place of the two Photosynthesis
+// calls and the branches of CanopyFluxesMod.F90:1219-1230, 1353-1356.
+template <bool PHS, class PRM, class COL, class CAN, class WRK, class PHA>
 __device__ __forceinline__ bool canopy_patch(const CView &V, const CSwitch &sw, const PRM &P,
                              const double *__restrict__ pft, const COL &col, const CAN &S,
                              int i0, const int32_t *__restrict__ filter, double dtime, ErrWord *err, bool first,
-                             int budget, const WRK &W)
+                             int budget, const WRK &W, const PHA &H)
 {
     const int p = filter ? filter[i0] : V.begp + i0;
     const long pp = V.p1(p);
@@ -931,7 +942,7 @@ __device__ __forceinline__ bool canopy_patch(const CView &V, const CSwitch &sw, 
     const double o3v_sun = S.o3coefvsun[pp], o3v_sha = S.o3coefvsha[pp];
     const double o3g_sun = S.o3coefgsun[pp], o3g_sha = S.o3coefgsha[pp];
     PsnConst K;
-    psn_constants(sw, P, pft, ivt, forc_pbot, o2, S.t_a10[pp], btran, dayl_factor, K, bad);
+    psn_constants(sw, P, pft, ivt, forc_pbot, o2, S.t_a10[pp], btran, dayl_factor, K, bad, PHS);
     // The iteration carries only its state from one iteration to the next
     // (and, through W, from round to round); every value the outputs read
     // after the last iteration is that iteration's own, so nothing else stays
@@ -1042,16 +1053,24 @@ __device__ __forceinline__ bool canopy_patch(const CView &V, const CSwitch &sw, 
         sh = Shared{};
         sh.gs_mol = gs_mol_c;
         temp_factors(P, K, t_veg, anyday, T);
-        // one call site for both phases: sun (k = 0), then shade
+        if constexpr (PHS) {
+            // sun and shade together; btran and the transpiration come from the hydraulics
+            btran = photosynthesis_phs(V, sw, P, K, T, H, S, p, c, ivt, nrad, noon, forc_pbot, thm, svpts, eah, o2,
+                                       co2, rb, qsatl, qaf, sun, sha, sh, bad);
+            S.btran[pp] = btran;
+            qflx_tran_veg = S.qflx_tran_veg[pp];
+        } else {
+            // one call site for both phases: sun (k = 0), then shade
 #pragma nounroll
-        for (int k = 0; k < 2; ++k) {
-            const bool ks = k == 0;
-            Phase ph{};
-            photosynthesis(sw, P, K, T, nrad, ks ? parsun : parsha, ks ? lai_z_sun : lai_z_sha,
-                           ks ? vcint_sun : vcint_sha, ks ? o3v_sun : o3v_sha, ks ? o3g_sun : o3g_sha, noon,
-                           forc_pbot, thm, svpts, eah, o2, co2, rb, btran, ph, sh, bad);
-            if (ks) sun = ph;
-            else sha = ph;
+            for (int k = 0; k < 2; ++k) {
+                const bool ks = k == 0;
+                Phase ph{};
+                photosynthesis(sw, P, K, T, nrad, ks ? parsun : parsha, ks ? lai_z_sun : lai_z_sha,
+                               ks ? vcint_sun : vcint_sha, ks ? o3v_sun : o3v_sha, ks ? o3g_sun : o3g_sha, noon,
+                               forc_pbot, thm, svpts, eah, o2, co2, rb, btran, ph, sh, bad);
+                if (ks) sun = ph;
+                else sha = ph;
+            }
         }
 
         // conductances and the leaf-temperature Newton step (:1119-1331)
@@ -1073,7 +1092,15 @@ __device__ __forceinline__ bool canopy_patch(const CView &V, const CSwitch &sw, 
         else rppdry = 0.0;
         double efpot = forc_rho * ((elai + esai) / rb) * (qsatl - qaf);
         double rpp;
-        if (efpot > 0.0) {
+        if (PHS) {
+            if (efpot > 0.0) {
+                if (btran > btran0) rpp = rppdry + fwet;
+                else rpp = fwet;
+                rpp = fmin90(rpp, (qflx_tran_veg + h2ocan / dtime) / efpot);
+            } else {
+                rpp = 1.0;
+            }
+        } else if (efpot > 0.0) {
             if (btran > btran0) {
                 qflx_tran_veg = efpot * rppdry;
                 rpp = rppdry + fwet;
@@ -1132,8 +1159,10 @@ __device__ __forceinline__ bool canopy_patch(const CView &V, const CSwitch &sw, 
         }
         efpot = forc_rho * ((elai + esai) / rb) * (wtgaq * (qsatl + qsatldT * dt_veg) - wtgq0 * qg - wtaq0 * forc_q);
         qflx_evap_veg = rpp * efpot;
-        if (efpot > 0.0 && btran > btran0) qflx_tran_veg = efpot * rppdry;
-        else qflx_tran_veg = 0.0;
+        if (!PHS) {
+            if (efpot > 0.0 && btran > btran0) qflx_tran_veg = efpot * rppdry;
+            else qflx_tran_veg = 0.0;
+        }
         const double ecidif = fmax90(0.0, qflx_evap_veg - qflx_tran_veg - h2ocan / dtime);
         qflx_evap_veg = fmin90(qflx_evap_veg, qflx_tran_veg + h2ocan / dtime);
         eflx_sh_veg = efsh + dc1 * wtga * dt_veg + errp + erre + HVAP * ecidif;
@@ -1313,19 +1342,22 @@ __device__ __forceinline__ bool canopy_patch(const CView &V, const CSwitch &sw, 
     S.cp[pp] = T.cp;
     S.lnca[pp] = K.lnc;
     S.gb_mol[pp] = sh.gb_mol;
-    S.gs_mol[pp] = sh.gs_mol;
+    if (!PHS) S.gs_mol[pp] = sh.gs_mol;
     const bool medlyn = sw.stomatal == CTSM_STOMATA_MEDLYN2011;
     const bool lit_sun = nrad >= 1 && parsun > 0.0, lit_sha = nrad >= 1 && parsha > 0.0;
     if (nrad >= 1) {
-        // written by both phases' canopy layer whether lit or not
-        S.vcmax_z[pp] = sh.vcmax_z;
-        S.tpu_z[pp] = sh.tpu_z;
-        S.kp_z[pp] = sh.kp_z;
-        S.ac[pp] = sh.ac;
-        S.aj[pp] = sh.aj;
-        S.ap[pp] = sh.ap;
-        S.ag[pp] = sh.ag;
-        S.an[pp] = sh.an;
+        // written by both phases' canopy layer whether lit or not (the PHS
+        // path has arrays of its own for these, written as it goes)
+        if (!PHS) {
+            S.vcmax_z[pp] = sh.vcmax_z;
+            S.tpu_z[pp] = sh.tpu_z;
+            S.kp_z[pp] = sh.kp_z;
+            S.ac[pp] = sh.ac;
+            S.aj[pp] = sh.aj;
+            S.ap[pp] = sh.ap;
+            S.ag[pp] = sh.ag;
+            S.an[pp] = sh.an;
+        }
         S.rssun_z[pp] = sun.rs_z;
         S.rssha_z[pp] = sha.rs_z;
         S.cisun_z[pp] = sun.ci_z;
@@ -1338,11 +1370,11 @@ __device__ __forceinline__ bool canopy_patch(const CView &V, const CSwitch &sw, 
         S.gs_mol_sha[pp] = sha.gs_ph;
         // rh_leaf: a dark phase writes 0, a lit one under Ball-Berry its
         // value; vpd_can: a lit phase under Medlyn (the last writer wins)
-        if (!medlyn || !lit_sun || !lit_sha) S.rh_leaf[pp] = sh.rh_leaf;
-        if (medlyn && (lit_sun || lit_sha)) S.vpd_can[pp] = sh.vpd_can;
+        if (!PHS && (!medlyn || !lit_sun || !lit_sha)) S.rh_leaf[pp] = sh.rh_leaf;
+        if (!PHS && medlyn && (lit_sun || lit_sha)) S.vpd_can[pp] = sh.vpd_can;
     }
-    if (lit_sun) S.gs_mol_sun_ln[pp] = sun.gs_ln;
-    if (lit_sha) S.gs_mol_sha_ln[pp] = sha.gs_ln;
+    if (!PHS && lit_sun) S.gs_mol_sun_ln[pp] = sun.gs_ln;
+    if (!PHS && lit_sha) S.gs_mol_sha_ln[pp] = sha.gs_ln;
     S.rssun[pp] = sun.rs;
     S.rssha[pp] = sha.rs;
     S.lmrsun[pp] = sun.lmr;
@@ -1376,6 +1408,11 @@ struct RoundArgs {
     int32_t *list_out, *cnt_out;
     CanopyWork W;
 };
+// the PHS form's argument: the same, then the PHS arrays
+struct RoundArgsPhs {
+    RoundArgs r;
+    PhsArgs phs;
+};
 #define KARG __attribute__((address_space(4)))
 
 // One round of CanopyFluxes over the patches still iterating (round 0: the
@@ -1391,9 +1428,10 @@ struct RoundArgs {
 // WAVES: the register budget the kernel is compiled for (waves per SIMD);
 // above one wave the compiler spills what does not fit to scratch.  The
 // launch picks the instance (a launch shape the autotuner times).
-template <int WAVES, bool STRIDE>
+// PHS: CanopyFluxes with plant hydraulic stress (argument RoundArgsPhs).
+template <int WAVES, bool STRIDE, bool PHS = false>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES)))
-k_canopy_round(RoundArgs args)
+k_canopy_round(typename std::conditional<PHS, RoundArgsPhs, RoundArgs>::type args)
 {
     const KARG RoundArgs *Ap = (const KARG RoundArgs *)__builtin_amdgcn_kernarg_segment_ptr();
     const int round = Ap->round;
@@ -1411,8 +1449,14 @@ k_canopy_round(RoundArgs args)
                           A->V.nlevsoi, A->V.nlevgrnd};
             const CSwitch sw{A->sw.z0param, A->sw.stomatal, A->sw.soil_resis, A->sw.biomass, A->sw.undercanopy,
                              A->sw.light_inhibit, A->sw.itmax};
-            cont = canopy_patch(V, sw, A->P, A->pft, A->col, A->S, i0, A->filter, A->dtime, A->err, round == 0,
-                                A->budget, A->W);
+            if constexpr (PHS) {
+                const KARG RoundArgsPhs *B = (const KARG RoundArgsPhs *)A;
+                cont = canopy_patch<true>(V, sw, A->P, A->pft, A->col, A->S, i0, A->filter, A->dtime, A->err,
+                                          round == 0, A->budget, A->W, B->phs);
+            } else {
+                cont = canopy_patch<false>(V, sw, A->P, A->pft, A->col, A->S, i0, A->filter, A->dtime, A->err,
+                                           round == 0, A->budget, A->W, *A);
+            }
         }
         const unsigned long long mask = __ballot(cont);
         if (mask) {
@@ -1607,10 +1651,12 @@ size_t canopy_work_bytes(int n, int itmax, int first_its, int round_its)
 void launch_canopy(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg, const ctsm_canopy_params &prm,
                    const double *pft_dev, const ctsm_col_state &col, const ctsm_canopy_state &can, int n,
                    const int32_t *filter, double dtime, ErrWord *err, void *work, int first_its, int round_its,
-                   bool zero_cgrnd, int waves, bool bounds_are_filter)
+                   bool zero_cgrnd, int waves, bool bounds_are_filter, const ctsm_phs_state *phs_q,
+                   const ctsm_phs_canopy_state *phs_h, const double *phs_table)
 {
     using namespace canopy;
     const GptlRange range("canflux");
+    const bool hydrstress = phs_h != nullptr;
     auto first_kernel = waves == 3 ? k_canopy_round<3, false> : waves == 2 ? k_canopy_round<2, false>
                                                                           : k_canopy_round<1, false>;
     auto later_kernel = waves == 3 ? k_canopy_round<3, true> : waves == 2 ? k_canopy_round<2, true>
@@ -1633,10 +1679,24 @@ void launch_canopy(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg,
                 CanopyWork{}};
     // round 0 also runs the part of the routine before the iteration
     const GptlRange iter_range("can_iter");
+    // the PHS form: one instance, built for one wave per SIMD
+    RoundArgsPhs b{};
+    if (hydrstress) b.phs = PhsArgs{*phs_q, *phs_h, phs_table, cfg.nlevsoi, cfg.modifyphoto_and_lmr_forcrop};
+    auto launch = [&](bool later, int grid) {
+        if (hydrstress) {
+            b.r = a;
+            if (later) hipLaunchKernelGGL((k_canopy_round<1, true, true>), dim3(grid), dim3(64), 0, st, b);
+            else hipLaunchKernelGGL((k_canopy_round<1, false, true>), dim3(grid), dim3(64), 0, st, b);
+        } else if (later) {
+            hipLaunchKernelGGL(later_kernel, dim3(grid), dim3(64), 0, st, a);
+        } else {
+            hipLaunchKernelGGL(first_kernel, dim3(grid), dim3(64), 0, st, a);
+        }
+    };
     if (R == 0) {
         // one pass: every patch finishes (converged or itmax reached) in round 0
         a.budget = cfg.itmax_canopy_fluxes + 1;
-        hipLaunchKernelGGL(first_kernel, dim3(blocks), dim3(64), 0, st, a);
+        launch(false, blocks);
         return;
     }
     double *w = (double *)work;
@@ -1657,8 +1717,7 @@ void launch_canopy(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg,
         a.cnt_in = r ? cnt + r - 1 : nullptr;
         a.list_out = list[r & 1];
         a.cnt_out = cnt + r;
-        if (r == 0) hipLaunchKernelGGL(first_kernel, dim3(blocks), dim3(64), 0, st, a);
-        else hipLaunchKernelGGL(later_kernel, dim3(later_blocks), dim3(64), 0, st, a);
+        launch(r > 0, r == 0 ? blocks : later_blocks);
     }
 }
 

@@ -38,7 +38,8 @@ void launch_soilwater_plain(hipStream_t, const View &, const Params &, const cts
 size_t canopy_work_bytes(int n, int itmax, int first_its, int round_its);
 void launch_canopy(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_canopy_params &, const double *,
                    const ctsm_col_state &, const ctsm_canopy_state &, int, const int32_t *, double, ErrWord *,
-                   void *, int, int, bool, int, bool);
+                   void *, int, int, bool, int, bool, const ctsm_phs_state *, const ctsm_phs_canopy_state *,
+                   const double *);
 void launch_root_sink(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_col_state &,
                       const ctsm_canopy_state &, int, const int32_t *, ErrWord *);
 void launch_save_before(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_col_state &,
@@ -396,7 +397,7 @@ struct ctsm_ctx {
     std::vector<DevBuf> wtm;           // water-table drop-in mirrors
     std::vector<DevBuf> drm;           // drainage drop-in mirrors
     std::vector<DevBuf> phsp, phsc;    // plant-hydraulics drop-in mirrors
-    DevBuf phs_tab;                    // PHS parameter table (16 x CTSM_MAX_PFT doubles)
+    DevBuf phs_tab;                    // PHS parameter table (17 x CTSM_MAX_PFT doubles)
     int32_t phs_npft = 0;              // 0 until ctsm_set_phs_params
     bool phs_kmax = false;             // the table holds kmax (calcstress may run)
     std::vector<DevBuf> stp, stc;      // calcstress drop-in mirrors
@@ -624,6 +625,8 @@ void ctsm_config_default(ctsm_config *cfg)
     cfg->light_inhibit = 1;
     cfg->use_hydrstress = 0;
     cfg->use_excess_ice = 0;
+    cfg->modifyphoto_and_lmr_forcrop = 1;
+    cfg->pad1 = 0;
 }
 
 int ctsm_create(const ctsm_config *cfg, int device, ctsm_ctx **out)
@@ -644,7 +647,9 @@ int ctsm_create(const ctsm_config *cfg, int device, ctsm_ctx **out)
         (unsigned)cfg->use_biomass_heat_storage > 1u || (unsigned)cfg->use_undercanopy_stability > 1u ||
         (unsigned)cfg->light_inhibit > 1u || cfg->itmax_canopy_fluxes < 0 || cfg->itmax_canopy_fluxes > 1000)
         return CTSM_E_CONFIG;
-    if (cfg->use_hydrstress != 0 || cfg->use_excess_ice != 0) return CTSM_E_CONFIG;
+    if ((unsigned)cfg->use_hydrstress > 1u || cfg->use_excess_ice != 0 ||
+        (unsigned)cfg->modifyphoto_and_lmr_forcrop > 1u)
+        return CTSM_E_CONFIG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return CTSM_E_NODEVICE;
     if (device < 0 || device >= ndev) return CTSM_E_ARG;
@@ -967,8 +972,11 @@ static int check_canopy_layout(const ctsm_ctx *ctx, const ctsm_layout *L)
 // server's pool) must not touch their patches.
 static int run_canopy(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state &col, const ctsm_canopy_state &can,
                       int32_t n, const int32_t *filter_dev, double dtime, bool zero_cgrnd,
-                      void *fixed_work = nullptr, size_t fixed_bytes = 0, bool bounds_are_filter = false)
+                      void *fixed_work = nullptr, size_t fixed_bytes = 0, bool bounds_are_filter = false,
+                      const ctsm_phs_state *phs_q = nullptr, const ctsm_phs_canopy_state *phs_h = nullptr)
 {
+    // the configuration fixes the form: the PHS one only through ctsm_canopyfluxes_phs_dev
+    if ((ctx->cfg.use_hydrstress != 0) != (phs_h != nullptr)) return CTSM_E_CONFIG;
     const bool one_pass = ctx->tun.plain_routines != 0;
     const int k0 = one_pass ? 1001 : ctx->tun.canopy_first_iterations;
     const int k = one_pass ? 1 : ctx->tun.canopy_round_iterations;
@@ -987,7 +995,8 @@ static int run_canopy(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_stat
         work = ctx->cwork.p;
     }
     launch_canopy(ctx->stream, *lay, ctx->cfg, ctx->cprm, (const double *)ctx->pft.p, col, can, n, filter_dev, dtime,
-                  ctx->err, work, k0, k, zero_cgrnd, ctx->tun.canopy_waves, bounds_are_filter);
+                  ctx->err, work, k0, k, zero_cgrnd, ctx->tun.canopy_waves, bounds_are_filter, phs_q, phs_h,
+                  (const double *)ctx->phs_tab.p);
     return CTSM_OK;
 }
 
@@ -1512,9 +1521,9 @@ int ctsm_set_phs_params(ctsm_ctx *ctx, const ctsm_phs_params *prm)
     const double *rows[] = {prm->krmax, prm->root_radius, prm->root_density, prm->froot_leaf};
     for (const double *a : rows) if (!a) return CTSM_E_ARG;
     if (!prm->psi50 || !prm->ck) return CTSM_E_ARG;
-    // rows: krmax, root_radius, root_density, froot_leaf, psi50, ck and kmax of the 4 segments
+    // rows: krmax, root_radius, root_density, froot_leaf, psi50, ck and kmax of the 4 segments, crop
     const int n = prm->npft;
-    std::vector<double> tab((size_t)16 * CTSM_MAX_PFT, 0.0);
+    std::vector<double> tab((size_t)17 * CTSM_MAX_PFT, 0.0);
     for (int r = 0; r < 4; ++r)
         for (int i = 0; i < n; ++i) tab[(size_t)r * CTSM_MAX_PFT + i] = rows[r][i];
     for (int k = 0; k < 4; ++k)
@@ -1523,6 +1532,8 @@ int ctsm_set_phs_params(ctsm_ctx *ctx, const ctsm_phs_params *prm)
             tab[(size_t)(8 + k) * CTSM_MAX_PFT + i] = prm->ck[(size_t)k * n + i];
             if (prm->kmax) tab[(size_t)(12 + k) * CTSM_MAX_PFT + i] = prm->kmax[(size_t)k * n + i];
         }
+    if (prm->crop)
+        for (int i = 0; i < n; ++i) tab[(size_t)16 * CTSM_MAX_PFT + i] = prm->crop[i];
     std::lock_guard<std::mutex> g(ctx->mu);
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(ctx, e);
@@ -1581,6 +1592,26 @@ int ctsm_phs_soil_root_conductance_dev(ctsm_ctx *ctx, const ctsm_layout *lay, co
     if (rc) return rc;
     launch_phs_conductance(ctx->stream, *lay, ctx->cfg, *phs, (const double *)ctx->phs_tab.p, ctx->phs_npft, n,
                            filter_dev, ctx->err);
+    return finish_launch(ctx);
+}
+
+int ctsm_canopyfluxes_phs_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                              const ctsm_canopy_state *can, const ctsm_phs_state *phs,
+                              const ctsm_phs_canopy_state *phc, int32_t n, const int32_t *filterp_dev, double dtime)
+{
+    if (!ctx || !col || !can || !phc || n < 0 || check_canopy_layout(ctx, lay) || (!filterp_dev && n > lay->ldp))
+        return CTSM_E_ARG;
+    if (!ctx->cfg.use_hydrstress) return CTSM_E_CONFIG;
+    int rc = phs_check(ctx, lay, phs, n, true, filterp_dev != nullptr);
+    if (rc) return rc;
+    if (!ctx->phs_kmax) return CTSM_E_ARG;   // calcstress needs kmax
+    for (size_t off = 0; off < sizeof(ctsm_phs_canopy_state); off += sizeof(void *))
+        if (!*(void *const *)((const char *)phc + off)) return CTSM_E_ARG;
+    // the conductance loop at the head of PhotosynthesisHydraulicStress, then the iteration
+    launch_phs_conductance(ctx->stream, *lay, ctx->cfg, *phs, (const double *)ctx->phs_tab.p, ctx->phs_npft, n,
+                           filterp_dev, ctx->err);
+    rc = run_canopy(ctx, lay, *col, *can, n, filterp_dev, dtime, false, nullptr, 0, false, phs, phc);
+    if (rc) return rc;
     return finish_launch(ctx);
 }
 

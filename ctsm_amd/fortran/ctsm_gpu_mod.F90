@@ -35,6 +35,7 @@ module ctsm_gpu_mod
      integer(c_int32_t) :: z0param_method, stomatalcond_method, soil_resis_method
      integer(c_int32_t) :: use_biomass_heat_storage, use_undercanopy_stability, light_inhibit
      integer(c_int32_t) :: use_hydrstress, use_excess_ice
+     integer(c_int32_t) :: modifyphoto_and_lmr_forcrop, pad1
   end type ctsm_config_t
 
   type, bind(C), public :: ctsm_layout_t
@@ -219,8 +220,18 @@ module ctsm_gpu_mod
      integer(c_int32_t) :: pad = 0
      type(c_ptr) :: krmax = c_null_ptr, root_radius = c_null_ptr, root_density = c_null_ptr
      type(c_ptr) :: froot_leaf = c_null_ptr, psi50 = c_null_ptr, ck = c_null_ptr
-     type(c_ptr) :: kmax = c_null_ptr
+     type(c_ptr) :: kmax = c_null_ptr, crop = c_null_ptr
   end type ctsm_phs_params_t
+
+  ! CanopyFluxes with plant hydraulic stress, as ctsm_phs_canopy_state
+  ! ([VEG] (begp:endp, 1:4), [PHS2] (begp:endp, 1:2))
+  type, bind(C), public :: ctsm_phs_canopy_state_t
+     type(c_ptr) :: tlai_z = c_null_ptr, local_morning = c_null_ptr
+     type(c_ptr) :: vegwp = c_null_ptr, vegwp_ln = c_null_ptr, vegwp_pd = c_null_ptr
+     type(c_ptr) :: bsun = c_null_ptr, bsha = c_null_ptr, an_sun = c_null_ptr, an_sha = c_null_ptr
+     type(c_ptr) :: vcmax_z_phs = c_null_ptr, tpu_z_phs = c_null_ptr, kp_z_phs = c_null_ptr
+     type(c_ptr) :: ac_phs = c_null_ptr, aj_phs = c_null_ptr, ap_phs = c_null_ptr, ag_phs = c_null_ptr
+  end type ctsm_phs_canopy_state_t
 
   ! calcstress arrays, as ctsm_phs_stress_state (vegwp / vegwp_pd (begp:endp, 1:4))
   type, bind(C), public :: ctsm_phs_stress_state_t

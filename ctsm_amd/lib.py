@@ -32,6 +32,7 @@ EXPORTS = [
     "ctsm_soil_step_fused_dev", "ctsm_dev_alloc", "ctsm_dev_free", "ctsm_memcpy", "ctsm_dev_copy",
     "ctsm_event_record", "ctsm_event_elapsed", "ctsm_event_destroy", "ctsm_sync", "ctsm_stream",
     "ctsm_set_canopy_params", "ctsm_canopyfluxes", "ctsm_root_sink", "ctsm_canopyfluxes_dev",
+    "ctsm_canopyfluxes_phs_dev",
     "ctsm_root_sink_dev", "ctsm_land_step_dev", "ctsm_memcpy2d",
     "ctsm_graph_capture", "ctsm_graph_launch", "ctsm_graph_destroy",
     "ctsm_soilfluxes", "ctsm_soil_balance_check", "ctsm_soilfluxes_dev", "ctsm_soil_balance_check_dev",
@@ -136,6 +137,8 @@ def load():
     L.ctsm_root_sink.argtypes = can + [C.c_int32, abi.i32p]
     L.ctsm_canopyfluxes_dev.argtypes = can + [C.c_int32, C.c_void_p, C.c_double]
     L.ctsm_canopyfluxes_dev_noalloc.argtypes = can + [C.c_int32, C.c_void_p, C.c_double]
+    L.ctsm_canopyfluxes_phs_dev.argtypes = can + [P(abi.PhsState), P(abi.PhsCanopyState), C.c_int32, C.c_void_p,
+                                                  C.c_double]
     L.ctsm_set_pft_rows.argtypes = [C.c_void_p, C.c_int32, P(abi.PftParams)]
     L.ctsm_energy_balance_check_dev.argtypes = [C.c_void_p, P(abi.Layout), P(abi.ColState), P(abi.PatchState),
                                                 P(abi.CanopyState), P(abi.SoilFluxState), C.c_int32, C.c_void_p,
@@ -357,6 +360,17 @@ class Context:
         fp = C.c_void_p(filterp_dev) if filterp_dev is not None else None
         self._check(self.L.ctsm_canopyfluxes_dev(self.h, C.byref(ds.layout()), C.byref(cs), C.byref(can), n, fp,
                                                  dtime), "CanopyFluxes")
+
+    def canopyfluxes_phs_dev(self, ds, dphs, filterp_dev=None, n=None, dtime=1800.0):
+        """CanopyFluxes with plant hydraulic stress on device state ds and its
+        phs.DeviceCanopyPhs arrays (the context's config has use_hydrstress = 1)."""
+        cs, _ = ds.structs()
+        can = ds.canopy_struct()
+        phs, phc = dphs.structs()
+        n = ds.dims.npatch if n is None else n
+        fp = C.c_void_p(filterp_dev) if filterp_dev is not None else None
+        self._check(self.L.ctsm_canopyfluxes_phs_dev(self.h, C.byref(ds.layout()), C.byref(cs), C.byref(can),
+                                                     C.byref(phs), C.byref(phc), n, fp, dtime), "CanopyFluxes (PHS)")
 
     def canopyfluxes_dev_noalloc(self, ds, filterp_dev=None, n=None, dtime=1800.0, structs=None):
         """canopyfluxes_dev through the entry that allocates nothing (its

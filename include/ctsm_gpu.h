@@ -114,14 +114,19 @@ typedef struct ctsm_config {
     int32_t light_inhibit;      /* 0/1                              (xml:95)         */
     /* use_hydrstress (clm_varctl, namelist_defaults_ctsm.xml:726): 1 selects
      * PhotosynthesisHydraulicStress and the PHS branches of CanopyFluxes
-     * (CanopyFluxesMod.F90:638, 1133, 1157, 1219, 1353).  ctsm_create returns
-     * CTSM_E_CONFIG for 1: the canopy entries run the use_hydrstress = .false.
-     * path only (the PHS soil side and calcstress have entries of their own). */
+     * (CanopyFluxesMod.F90:638, 1133, 1157, 1219, 1353).  With 1, CanopyFluxes
+     * runs through ctsm_canopyfluxes_phs_dev (the other canopy and land-step
+     * entries return CTSM_E_CONFIG); with 0, ctsm_canopyfluxes_phs_dev does. */
     int32_t use_hydrstress;     /* 0                                                 */
     /* use_excess_ice (clm_varctl; SoilTemperatureMod.F90:287 and the excess-ice
      * terms of SoilThermProp / Phasechange): ctsm_create returns CTSM_E_CONFIG
      * for 1. */
     int32_t use_excess_ice;     /* 0                                                 */
+    /* modifyphoto_and_lmr_forcrop (photosyns_inst, namelist_defaults_ctsm.xml:
+     * 104): 0/1, read by PhotosynthesisHydraulicStress only
+     * (PhotosynthesisMod.F90:3516, 3533, 3739, 3774). */
+    int32_t modifyphoto_and_lmr_forcrop;   /* 1                                      */
+    int32_t pad1;
 } ctsm_config;
 
 /* ---- array layout of one call -------------------------------------------- */
@@ -518,7 +523,9 @@ typedef struct ctsm_phs_params {
     const double *krmax, *root_radius, *root_density, *froot_leaf;
     const double *psi50, *ck;
     const double *kmax;   /* params_inst%kmax, segment-major as psi50; may be NULL when
-                             ctsm_phs_calcstress is not used */
+                             neither ctsm_phs_calcstress nor the PHS canopy path is used */
+    const double *crop;   /* pftcon%crop (0 or 1; PhotosynthesisMod.F90:3516, 3739); may be
+                             NULL: no crop types */
 } ctsm_phs_params;
 
 /* Per-patch arrays of calcstress (PhotosynthesisMod.F90:4490-4710), the
@@ -548,6 +555,32 @@ typedef struct ctsm_phs_stress_state {
     const double  *z;                                           /* [SNO] (m) */
     const double  *smp_l;                                       /* [SOI] (mm) */
 } ctsm_phs_stress_state;
+
+/* CanopyFluxes with plant hydraulic stress (use_hydrstress = 1):
+ * PhotosynthesisHydraulicStress (PhotosynthesisMod.F90:2704-3811) with the
+ * coupled sun / shade ci solve hybrid_PHS, brent_PHS, ci_func_PHS
+ * (:3815-4486) and calcstress inside it, in every iteration of the leaf
+ * temperature, and the PHS branches of CanopyFluxes (CanopyFluxesMod.F90:
+ * 1133-1166, 1219-1230, 1353-1356).  The arrays of that path beyond
+ * ctsm_canopy_state and ctsm_phs_state: [VEG] arrays (begp:endp, 1:4) with
+ * the patch fastest (sun, sha, xyl, root); [PHS2] arrays (begp:endp, 1:2)
+ * with the patch fastest (sun, sha) for the single canopy layer (nlevcan =
+ * 1).  Photosynthesis outputs both paths share (psnsun ..., rssun, rssha,
+ * cisun_z, gs_mol_sun / _sha and their noon values, gb_mol, kc, ko, cp, qe,
+ * bbb, mbb, lnca, vpd_can, btran, qflx_tran_veg) stay in ctsm_canopy_state;
+ * the PHS path leaves the non-PHS arrays vcmax_z, tpu_z, kp_z, ac, aj, ap,
+ * ag, an, gs_mol and rh_leaf untouched, as the reference does. */
+typedef struct ctsm_phs_canopy_state {
+    /* patch */
+    const double  *tlai_z;             /* (p) tlai_z(p, 1), the layer's leaf area */
+    const int32_t *local_morning;      /* (p) nonzero where a night calcstress writes vegwp_pd */
+    double        *vegwp;              /* [VEG] inout (mm) */
+    double        *vegwp_ln, *vegwp_pd;  /* [VEG] out (mm; spval away from noon / morning) */
+    double        *bsun, *bsha;        /* (p) inout (no canopy layer: btran = the old bsun) */
+    double        *an_sun, *an_sha;    /* (p) out net photosynthesis (umol CO2/m2/s) */
+    double        *vcmax_z_phs, *tpu_z_phs, *kp_z_phs;   /* [PHS2] out */
+    double        *ac_phs, *aj_phs, *ap_phs, *ag_phs;    /* [PHS2] out */
+} ctsm_phs_canopy_state;
 
 /* ---- launch shapes (no counterpart in the reference: set by the autotuner) --
  * richards_lanes: lanes cooperating on one column in the Richards solve
@@ -808,6 +841,21 @@ int ctsm_phs_soil_root_conductance_dev(ctsm_ctx *ctx, const ctsm_layout *lay, co
                                        int32_t num_filterp, const int32_t *filterp_dev);
 int ctsm_phs_root_sink_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_phs_state *phs,
                            int32_t num_filterc, const int32_t *filterc_dev);
+/* CanopyFluxes with plant hydraulic stress (CanopyFluxesMod.F90:191-1765 with
+ * use_hydrstress = .true.; the context's config has use_hydrstress = 1 and
+ * ctsm_set_phs_params was given kmax): the root-soil conductances of the
+ * filter's patches into phs (k_soil_root, root_conductance,
+ * soil_conductance: the loop at the head of PhotosynthesisHydraulicStress,
+ * :3062-3114, whose inputs do not change during the iteration), then the
+ * leaf-temperature iteration with PhotosynthesisHydraulicStress in it.  phs
+ * supplies the conductance members and z / smp_l; col, can as for
+ * ctsm_canopyfluxes_dev.  Errors as ctsm_canopyfluxes_dev, and
+ * CTSM_E_ENDRUN for a brent_PHS root not bracketed or a negative stomatal
+ * conductance (the reference's endrun). */
+int ctsm_canopyfluxes_phs_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                              const ctsm_canopy_state *can, const ctsm_phs_state *phs,
+                              const ctsm_phs_canopy_state *phc, int32_t num_filterp, const int32_t *filterp_dev,
+                              double dtime);
 int ctsm_phs_calcstress_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_phs_stress_state *st,
                             int32_t num_filterp, const int32_t *filterp_dev);
 int ctsm_soilfluxes_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,

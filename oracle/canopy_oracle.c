@@ -43,6 +43,7 @@
 #include "ctsm_gpu.h"
 #include "ctsm_math.h"
 #include "oracle_libm.h"   /* -DORACLE_LIBM: the host libm instead */
+#include "phs_oracle.h"
 
 /* shr_const_mod (tools/mksurfdata_esmf/src/shr_const_mod.F90) and clm_varcon.F90 */
 #define C_TFRZ     273.15
@@ -474,6 +475,11 @@ typedef struct {
     const ctsm_pft_params *pft;
     const ctsm_col_state *col;
     const ctsm_canopy_state *cs;
+    /* use_hydrstress: the PHS arrays and parameters (NULL otherwise) */
+    const ctsm_phs_state *phs;
+    const ctsm_phs_canopy_state *phc;
+    const ctsm_phs_params *pp;
+    int nlevsoi;
 } ctx_t;
 
 static double ft(double tl, double ha)
@@ -748,6 +754,645 @@ static int photosynthesis(const ctx_t *X, int p, int sun, double esat_tv, double
     return rc;
 }
 
+/* ---- plant hydraulic stress: PhotosynthesisHydraulicStress -------------- */
+
+#define SUN 0
+#define SHA 1
+#define VEGA(a, p, k)  ((a)[((p) - L.begp) + (long)(k) * L.ldp])    /* k = 0 sun .. 3 root */
+#define PHS2(a, p, k)  ((a)[((p) - L.begp) + (long)(k) * L.ldp])    /* k = 0 sun, 1 sha */
+
+typedef struct {
+    /* constants of the patch's call */
+    int c3, stomatal;
+    double forc_pbot, cair, oair, gb_mol, rh_can, theta_cj, theta_ip, qe, bbb, mbb, medlynslope, medlynintercept;
+    double cp, kc, ko;
+    double vcmax_z[2], tpu_z[2], kp_z[2], je[2], lmr_z[2], par_z[2];
+    double qsatl, qaf;
+    p_stress_t leaf;       /* calcstress's view of the patch */
+    int morning;           /* get_local_time < isecspday / 2 */
+    /* photosyns_inst values ci_func_PHS writes */
+    double ac[2], aj[2], ap[2], ag[2], an[2];
+    /* the arrays calcstress writes: its last call's values */
+    int stressed;          /* a calcstress call happened */
+    double pd[4];          /* vegwp_pd: x where night and morning, else spval */
+    int tran_set;          /* a night call (or hybrid_PHS) set qflx_tran_veg */
+    double tran;
+    int err;
+} phs_t;
+
+static void phs_calcstress(phs_t *s, double *x, double *bsun, double *bsha, double gs_sun, double gs_sha)
+{
+    int night;
+    double tran = 0.0;
+    oracle_phs_calcstress_one(&s->leaf, x, gs_sun, gs_sha, s->qsatl, s->qaf, bsun, bsha, &night, &tran);
+    s->stressed = 1;
+    for (int k = 0; k < 4; ++k) s->pd[k] = (night && s->morning) ? x[k] : C_SPVAL;
+    if (night) {
+        s->tran_set = 1;
+        s->tran = tran;
+    }
+}
+
+/* ci_func_PHS (PhotosynthesisMod.F90:4227-4486) */
+static void ci_func_phs(phs_t *s, double *x, double cisun, double cisha, double *fsun, double *fsha, double *bsun,
+                        double *bsha, int bflag, double gs0sun, double gs0sha, double *gs_sun, double *gs_sha)
+{
+    double r1, r2, ai;
+    if (bflag) phs_calcstress(s, x, bsun, bsha, gs0sun, gs0sha);
+    const double ci[2] = {cisun, cisha};
+    const double b[2] = {*bsun, *bsha};
+    if (s->c3) {
+        for (int k = 0; k < 2; ++k)
+            s->ac[k] = b[k] * s->vcmax_z[k] * fmax90(ci[k] - s->cp, 0.0) / (ci[k] + s->kc * (1.0 + s->oair / s->ko));
+        for (int k = 0; k < 2; ++k) s->aj[k] = s->je[k] * fmax90(ci[k] - s->cp, 0.0) / (4.0 * ci[k] + 8.0 * s->cp);
+        for (int k = 0; k < 2; ++k) s->ap[k] = 3.0 * s->tpu_z[k];
+    } else {
+        for (int k = 0; k < 2; ++k) s->ac[k] = b[k] * s->vcmax_z[k];
+        for (int k = 0; k < 2; ++k) s->aj[k] = s->qe * s->par_z[k] * 4.6;
+        for (int k = 0; k < 2; ++k) s->ap[k] = s->kp_z[k] * fmax90(ci[k], 0.0) / s->forc_pbot;
+    }
+    for (int k = 0; k < 2; ++k) {
+        if (oracle_quadratic(s->theta_cj, -(s->ac[k] + s->aj[k]), s->ac[k] * s->aj[k], &r1, &r2)) s->err = ORC_ENDRUN;
+        ai = fmin90(r1, r2);
+        if (oracle_quadratic(s->theta_ip, -(ai + s->ap[k]), ai * s->ap[k], &r1, &r2)) s->err = ORC_ENDRUN;
+        s->ag[k] = fmax90(0.0, fmin90(r1, r2));
+    }
+    s->an[SUN] = s->ag[SUN] - *bsun * s->lmr_z[SUN];
+    s->an[SHA] = s->ag[SHA] - *bsha * s->lmr_z[SHA];
+    const double gsmin = (s->stomatal == CTSM_STOMATA_MEDLYN2011) ? s->medlynintercept : s->bbb;
+    if (s->an[SUN] < 0.0) {
+        *gs_sun = fmax90(*bsun * gsmin, 1.0);
+        *fsun = 0.0;
+    }
+    if (s->an[SHA] < 0.0) {
+        *gs_sha = fmax90(*bsha * gsmin, 1.0);
+        *fsha = 0.0;
+    }
+    if (s->an[SUN] < 0.0 && s->an[SHA] < 0.0) return;
+    double cs[2] = {0.0, 0.0};
+    double *gs[2] = {gs_sun, gs_sha};
+    for (int k = 0; k < 2; ++k) {
+        if (s->an[k] < 0.0) continue;
+        cs[k] = s->cair - 1.4 / s->gb_mol * s->an[k] * s->forc_pbot;
+        cs[k] = fmax90(cs[k], MAX_CS);
+        if (s->stomatal == CTSM_STOMATA_MEDLYN2011) {
+            const double term = 1.6 * s->an[k] / (cs[k] / s->forc_pbot * 1.e06);
+            const double aq = 1.0;
+            const double bq = -(2.0 * (s->medlynintercept * 1.e-06 + term)
+                                + (s->medlynslope * term) * (s->medlynslope * term) / (s->gb_mol * 1.e-06 * s->rh_can));
+            const double cq = s->medlynintercept * s->medlynintercept * 1.e-12
+                              + (2.0 * s->medlynintercept * 1.e-06
+                                 + term * (1.0 - s->medlynslope * s->medlynslope / s->rh_can)) * term;
+            if (oracle_quadratic(aq, bq, cq, &r1, &r2)) s->err = ORC_ENDRUN;
+            *gs[k] = fmax90(r1, r2) * 1.e06;
+        } else {
+            const double bk = k == SUN ? *bsun : *bsha;
+            const double aq = cs[k];
+            const double bq = cs[k] * (s->gb_mol - fmax90(bk * s->bbb, 1.0)) - s->mbb * s->an[k] * s->forc_pbot;
+            const double cq = -s->gb_mol * (cs[k] * fmax90(bk * s->bbb, 1.0)
+                                            + s->mbb * s->an[k] * s->forc_pbot * s->rh_can);
+            if (oracle_quadratic(aq, bq, cq, &r1, &r2)) s->err = ORC_ENDRUN;
+            *gs[k] = fmax90(r1, r2);
+        }
+    }
+    double *fv[2] = {fsun, fsha};
+    for (int k = 0; k < 2; ++k) {
+        if (s->an[k] < 0.0) continue;
+        if (*gs[k] > 0.0)
+            *fv[k] = ci[k] - s->cair + s->an[k] * s->forc_pbot * (1.4 * *gs[k] + 1.6 * s->gb_mol) / (s->gb_mol * *gs[k]);
+        else
+            *fv[k] = ci[k] - s->cair;
+    }
+}
+
+/* brent_PHS (:4068-4223): both phases' brackets shrink together; tol is the
+ * sunlit tolerance for both */
+static void brent_phs(phs_t *s, double *xsun, double x1sun, double x2sun, double f1sun, double f2sun, double *xsha,
+                      double x1sha, double x2sha, double f1sha, double f2sha, double tol, double *gs_sun,
+                      double *gs_sha, double *bsun, double *bsha)
+{
+    const int itmax = 20;
+    const double eps = 1.e-4;
+    double a[2] = {x1sun, x1sha}, b[2] = {x2sun, x2sha}, fa[2] = {f1sun, f1sha}, fb[2] = {f2sun, f2sha};
+    double c[2], fc[2], d[2] = {0.0, 0.0}, e[2] = {0.0, 0.0}, tol1[2], xm[2];
+    double dummy[4] = {0.0, 0.0, 0.0, 0.0};   /* x of the calls: calcstress is not called (bflag false) */
+    for (int k = 0; k < 2; ++k)
+        if ((fa[k] > 0.0 && fb[k] > 0.0) || (fa[k] < 0.0 && fb[k] < 0.0)) {
+            s->err = ORC_ENDRUN;
+            *xsun = b[SUN];
+            *xsha = b[SHA];
+            return;
+        }
+    c[0] = b[0]; c[1] = b[1];
+    fc[0] = fb[0]; fc[1] = fb[1];
+    int iter = 0;
+    for (;;) {
+        if (iter == itmax) break;
+        iter = iter + 1;
+        for (int k = 0; k < 2; ++k) {
+            if ((fb[k] > 0.0 && fc[k] > 0.0) || (fb[k] < 0.0 && fc[k] < 0.0)) {
+                c[k] = a[k];
+                fc[k] = fa[k];
+                d[k] = b[k] - a[k];
+                e[k] = d[k];
+            }
+            if (fabs(fc[k]) < fabs(fb[k])) {
+                a[k] = b[k];
+                b[k] = c[k];
+                c[k] = a[k];
+                fa[k] = fb[k];
+                fb[k] = fc[k];
+                fc[k] = fa[k];
+            }
+        }
+        for (int k = 0; k < 2; ++k) tol1[k] = 2.0 * eps * fabs(b[k]) + 0.5 * tol;
+        for (int k = 0; k < 2; ++k) xm[k] = 0.5 * (c[k] - b[k]);
+        if ((fabs(xm[SUN]) <= tol1[SUN] || fb[SUN] == 0.0) && (fabs(xm[SHA]) <= tol1[SHA] || fb[SHA] == 0.0)) {
+            *xsun = b[SUN];
+            *xsha = b[SHA];
+            return;
+        }
+        for (int k = 0; k < 2; ++k) {
+            if (fabs(e[k]) >= tol1[k] && fabs(fa[k]) > fabs(fb[k])) {
+                const double ss = fb[k] / fa[k];
+                double p, q, r;
+                if (a[k] == c[k]) {
+                    p = 2.0 * xm[k] * ss;
+                    q = 1.0 - ss;
+                } else {
+                    q = fa[k] / fc[k];
+                    r = fb[k] / fc[k];
+                    p = ss * (2.0 * xm[k] * q * (q - r) - (b[k] - a[k]) * (r - 1.0));
+                    q = (q - 1.0) * (r - 1.0) * (ss - 1.0);
+                }
+                if (p > 0.0) q = -q;
+                p = fabs(p);
+                if (2.0 * p < fmin90(3.0 * xm[k] * q - fabs(tol1[k] * q), fabs(e[k] * q))) {
+                    e[k] = d[k];
+                    d[k] = p / q;
+                } else {
+                    d[k] = xm[k];
+                    e[k] = d[k];
+                }
+            } else {
+                d[k] = xm[k];
+                e[k] = d[k];
+            }
+            a[k] = b[k];
+            fa[k] = fb[k];
+            if (fabs(d[k]) > tol1[k]) b[k] = b[k] + d[k];
+            else b[k] = b[k] + copysign(fabs(tol1[k]), xm[k]);
+        }
+        const double gs0sun = *gs_sun, gs0sha = *gs_sha;
+        ci_func_phs(s, dummy, b[SUN], b[SHA], &fb[SUN], &fb[SHA], bsun, bsha, 0, gs0sun, gs0sha, gs_sun, gs_sha);
+        if (fb[SUN] == 0.0 && fb[SHA] == 0.0) break;
+    }
+    *xsun = b[SUN];
+    *xsha = b[SHA];
+}
+
+/* hybrid_PHS (:3815-4064): the outer loop updates bsun / bsha through
+ * calcstress, the inner one is the secant / Brent search for both ci;
+ * vegwp (4) inout; returns soilflux (clipped at 0) */
+static double hybrid_phs(phs_t *s, double *x0sun, double *x0sha, double *vegwp, double *bsun, double *bsha,
+                         double *gs_sun, double *gs_sha)
+{
+    const double toldb = 1.e-2, eps = 1.e-2, eps1 = 1.e-4;
+    const int itmax = 3;
+    double x[4];
+    double x1sun = *x0sun, x1sha = *x0sha, f0sun = 0.0, f0sha = 0.0, f1sun = 0.0, f1sha = 0.0;
+    double xsun, xsha, dxsun, dxsha, tolsun, tolsha, minf = 0.0, minxsun = 0.0, minxsha = 0.0;
+    int bflag = 0;
+    double b0sun = -1.0, b0sha = -1.0, gs0sun = 0.0, gs0sha = 0.0, dbsun, dbsha;
+    *bsun = 1.0;
+    *bsha = 1.0;
+    int iter1 = 0;
+    for (;;) {
+        for (int k = 0; k < 4; ++k) x[k] = vegwp[k];
+        iter1 = iter1 + 1;
+        int iter2 = 0;
+        *x0sun = fmax90(0.1, x1sun);
+        x1sun = 0.99 * x1sun;
+        *x0sha = fmax90(0.1, x1sha);
+        x1sha = 0.99 * x1sha;
+        tolsun = fabs(x1sun) * eps;
+        tolsha = fabs(x1sha) * eps;
+        ci_func_phs(s, x, *x0sun, *x0sha, &f0sun, &f0sha, bsun, bsha, bflag, gs0sun, gs0sha, gs_sun, gs_sha);
+        dbsun = b0sun - *bsun;
+        dbsha = b0sha - *bsha;
+        b0sun = *bsun;
+        b0sha = *bsha;
+        bflag = 0;
+        ci_func_phs(s, x, x1sun, x1sha, &f1sun, &f1sha, bsun, bsha, bflag, gs0sun, gs0sha, gs_sun, gs_sha);
+        for (;;) {
+            if (fabs(f0sun) < eps1 && fabs(f0sha) < eps1) {
+                x1sun = *x0sun;
+                x1sha = *x0sha;
+                break;
+            }
+            if (fabs(f1sun) < eps1 && fabs(f1sha) < eps1) break;
+            iter2 = iter2 + 1;
+            if ((f1sun - f0sun) == 0.0) dxsun = 0.5 * (x1sun + *x0sun) - x1sun;
+            else dxsun = -f1sun * (x1sun - *x0sun) / (f1sun - f0sun);
+            if ((f1sha - f0sha) == 0.0) dxsha = 0.5 * (x1sha + *x0sha) - x1sha;
+            else dxsha = -f1sha * (x1sha - *x0sha) / (f1sha - f0sha);
+            *x0sun = x1sun;
+            x1sun = x1sun + dxsun;
+            *x0sha = x1sha;
+            x1sha = x1sha + dxsha;
+            ci_func_phs(s, x, x1sun, x1sha, &f1sun, &f1sha, bsun, bsha, bflag, gs0sun, gs0sha, gs_sun, gs_sha);
+            if (fabs(dxsun) < tolsun && fabs(dxsha) < tolsha) {
+                *x0sun = x1sun;
+                *x0sha = x1sha;
+                break;
+            }
+            if (iter2 == 1) {
+                minf = fabs(f1sun + f1sha);
+                minxsun = x1sun;
+                minxsha = x1sha;
+            } else if (fabs(f1sun + f1sha) < minf) {
+                minf = fabs(f1sun + f1sha);
+                minxsun = x1sun;
+                minxsha = x1sha;
+            }
+            if (fabs(f1sun) < eps1 && fabs(f1sha) < eps1) break;
+            if (f1sun * f0sun < 0.0 && f1sha * f0sha < 0.0) {
+                brent_phs(s, &xsun, *x0sun, x1sun, f0sun, f1sun, &xsha, *x0sha, x1sha, f0sha, f1sha, tolsun, gs_sun,
+                          gs_sha, bsun, bsha);
+                *x0sun = xsun;
+                *x0sha = xsha;
+                break;
+            }
+            if (iter2 > itmax) {
+                x1sun = minxsun;
+                x1sha = minxsha;
+                ci_func_phs(s, x, x1sun, x1sha, &f1sun, &f1sha, bsun, bsha, bflag, gs0sun, gs0sha, gs_sun, gs_sha);
+                break;
+            }
+        }
+        if (*bsun > 0.01) gs0sun = *gs_sun / *bsun;
+        if (*bsha > 0.01) gs0sha = *gs_sha / *bsha;
+        bflag = 1;
+        if (fabs(dbsun) < toldb && fabs(dbsha) < toldb) break;
+        if (iter1 > itmax) break;
+    }
+    *x0sun = x1sun;
+    *x0sha = x1sha;
+    double soilflux = oracle_phs_getvegwp_one(&s->leaf, x, *gs_sun, *gs_sha, s->qsatl, s->qaf);
+    for (int k = 0; k < 4; ++k) vegwp[k] = x[k];
+    if (soilflux < 0.0) soilflux = 0.0;
+    return soilflux;
+}
+
+/* PhotosynthesisHydraulicStress (:2704-3811) for patch p, the root-soil
+ * conductances already in phs->k_soil_root (their loop, :3062-3114, reads
+ * nothing the iteration changes).  Writes the patch's outputs and btran,
+ * qflx_tran_veg where the reference does. */
+static int photosynthesis_phs(const ctx_t *X, int p, double esat_tv, double eair, double oair, double cair, double rb,
+                              double dayl_factor, double qsatl, double qaf)
+{
+    const cl_t L = X->L;
+    const ctsm_canopy_state *S = X->cs;
+    const ctsm_phs_canopy_state *H = X->phc;
+    const ctsm_canopy_params *P = X->prm;
+    const ctsm_pft_params *F = X->pft;
+    const ctsm_phs_params *PP = X->pp;
+    const int stomatal = X->cfg->stomatalcond_method;
+    const int c = PA(S->column, p);
+    const int g = PA(S->gridcell, p);
+    const int ivt = PA(S->itype, p);
+    const double forc_pbot = CA(S->forc_pbot, c);
+    const double t_veg = PA(S->t_veg, p);
+    const double t10 = PA(S->t_a10, p);
+    const double tgcm = PA(S->thm, p);
+    const int nrad = PA(S->nrad, p);
+    const double par_z[2] = {PA(S->parsun_z, p), PA(S->parsha_z, p)};
+    const double lai_z[2] = {PA(S->laisun_z, p), PA(S->laisha_z, p)};
+    const double vcmaxcint[2] = {PA(S->vcmaxcintsun, p), PA(S->vcmaxcintsha, p)};
+    const double o3coefv[2] = {PA(S->o3coefvsun, p), PA(S->o3coefvsha, p)};
+    const double o3coefg[2] = {PA(S->o3coefgsun, p), PA(S->o3coefgsha, p)};
+    double *ci_z[2] = {&PA(S->cisun_z, p), &PA(S->cisha_z, p)};
+    double *rs[2] = {&PA(S->rssun, p), &PA(S->rssha, p)};
+    double *rs_z[2] = {&PA(S->rssun_z, p), &PA(S->rssha_z, p)};
+    double *lmr[2] = {&PA(S->lmrsun, p), &PA(S->lmrsha, p)};
+    double *lmr_z[2] = {&PA(S->lmrsun_z, p), &PA(S->lmrsha_z, p)};
+    double *psn[2] = {&PA(S->psnsun, p), &PA(S->psnsha, p)};
+    double *psn_z[2] = {&PA(S->psnsun_z, p), &PA(S->psnsha_z, p)};
+    double *psn_wc[2] = {&PA(S->psnsun_wc, p), &PA(S->psnsha_wc, p)};
+    double *psn_wj[2] = {&PA(S->psnsun_wj, p), &PA(S->psnsha_wj, p)};
+    double *psn_wp[2] = {&PA(S->psnsun_wp, p), &PA(S->psnsha_wp, p)};
+    double *gs_mol[2] = {&PA(S->gs_mol_sun, p), &PA(S->gs_mol_sha, p)};
+    double *gs_mol_ln[2] = {&PA(S->gs_mol_sun_ln, p), &PA(S->gs_mol_sha_ln, p)};
+    double *an[2] = {&PA(H->an_sun, p), &PA(H->an_sha, p)};
+    double *bsun = &PA(H->bsun, p), *bsha = &PA(H->bsha, p);
+    double *bb[2] = {bsun, bsha};
+    const int np = PP->npft;
+    const int crop = PP->crop ? (PP->crop[ivt] != 0.0) : 0;
+    const int modify = X->cfg->modifyphoto_and_lmr_forcrop != 0;
+    int rc = ORC_OK;
+    phs_t s;
+    memset(&s, 0, sizeof(s));
+
+    /* misc parameters (:3117-3160) */
+    const double lmrc = fth25(P->lmrhd, P->lmrse);
+    const int c3 = ((int)lrint(F->c3psn[ivt]) == 1);
+    PA(S->c3flag, p) = c3;
+    double bbbopt = 0.0;
+    if (c3) {
+        PA(S->qe, p) = 0.0;
+        if (stomatal == CTSM_STOMATA_BB1987) bbbopt = 10000.0;
+    } else {
+        PA(S->qe, p) = 0.05;
+        if (stomatal == CTSM_STOMATA_BB1987) bbbopt = 40000.0;
+    }
+    if (stomatal == CTSM_STOMATA_BB1987) {
+        PA(S->bbb, p) = bbbopt;
+        PA(S->mbb, p) = F->mbbopt[ivt];
+    }
+    const double kc25 = P->kc25_coef * forc_pbot;
+    const double ko25 = P->ko25_coef * forc_pbot;
+    const double sco = 0.5 * 0.209 / P->cp25_yr2000;
+    const double cp25 = 0.5 * oair / sco;
+    PA(S->kc, p) = kc25 * ft(t_veg, P->kcha);
+    PA(S->ko, p) = ko25 * ft(t_veg, P->koha);
+    PA(S->cp, p) = cp25 * ft(t_veg, P->cpha);
+
+    /* canopy-top capacities and the layer loop (:3165-3480) */
+    if (F->slatop[ivt] * F->leafcn[ivt] <= 0.0) return ORC_ENDRUN;
+    PA(S->lnca, p) = 1.0 / (F->slatop[ivt] * F->leafcn[ivt]);
+    PA(S->lnca, p) = fmin90(PA(S->lnca, p), 10.0);
+    const double lnc = PA(S->lnca, p);
+    double vcmax25top = lnc * F->flnr[ivt] * P->fnr * P->act25 * dayl_factor;
+    vcmax25top = vcmax25top * F->fnitr[ivt];
+    const double t10c = fmin90(fmax90(t10 - C_TFRZ, 11.0), 35.0);
+    const double jmax25top = ((2.59 - 0.035 * t10c) * vcmax25top) * P->jmax25top_sf;
+    const double tpu25top = P->tpu25ratio * vcmax25top;
+    const double kp25top = P->kp25ratio * vcmax25top;
+    double lmr25top;
+    if (c3) lmr25top = vcmax25top * P->leaf_mr_vcm;
+    else lmr25top = vcmax25top * 0.025;
+    double vcmax_z[2] = {0.0, 0.0}, jmax_z[2] = {0.0, 0.0}, tpu_z[2] = {0.0, 0.0}, kp_z[2] = {0.0, 0.0};
+    if (nrad >= 1) {
+        for (int k = 0; k < 2; ++k) {
+            const double lmr25 = lmr25top * vcmaxcint[k];
+            if (c3) {
+                *lmr_z[k] = lmr25 * ft(t_veg, P->lmrha) * fth(t_veg, P->lmrhd, P->lmrse, lmrc);
+            } else {
+                *lmr_z[k] = lmr25 * cm_pow(2.0, (t_veg - (C_TFRZ + 25.0)) / 10.0);
+                *lmr_z[k] = *lmr_z[k] / (1.0 + cm_exp(1.3 * (t_veg - (C_TFRZ + 55.0))));
+            }
+        }
+        for (int k = 0; k < 2; ++k)
+            *lmr_z[k] = *lmr_z[k] * fmin90((0.2 * cm_exp(3.218 * PA(H->tlai_z, p))), 1.0);
+        if (par_z[SUN] > 0.0) {
+            const double vcmaxse = (668.39 - 1.07 * t10c) * P->vcmaxse_sf;
+            const double jmaxse = (659.70 - 0.75 * t10c) * P->jmaxse_sf;
+            const double tpuse = (668.39 - 1.07 * t10c) * P->tpuse_sf;
+            const double vcmaxc = fth25(P->vcmaxhd, vcmaxse);
+            const double jmaxc = fth25(P->jmaxhd, jmaxse);
+            const double tpuc = fth25(P->tpuhd, tpuse);
+            for (int k = 0; k < 2; ++k) {
+                const double vcmax25 = vcmax25top * vcmaxcint[k];
+                const double jmax25 = jmax25top * vcmaxcint[k];
+                const double tpu25 = tpu25top * vcmaxcint[k];
+                vcmax_z[k] = vcmax25 * ft(t_veg, P->vcmaxha) * fth(t_veg, P->vcmaxhd, vcmaxse, vcmaxc);
+                jmax_z[k] = jmax25 * ft(t_veg, P->jmaxha) * fth(t_veg, P->jmaxhd, jmaxse, jmaxc);
+                tpu_z[k] = tpu25 * ft(t_veg, P->tpuha) * fth(t_veg, P->tpuhd, tpuse, tpuc);
+            }
+            if (!c3) {
+                for (int k = 0; k < 2; ++k) {
+                    vcmax_z[k] = vcmax25top * vcmaxcint[k] * cm_pow(2.0, (t_veg - (C_TFRZ + 25.0)) / 10.0);
+                    vcmax_z[k] = vcmax_z[k] / (1.0 + cm_exp(0.2 * ((C_TFRZ + 15.0) - t_veg)));
+                    vcmax_z[k] = vcmax_z[k] / (1.0 + cm_exp(0.3 * (t_veg - (C_TFRZ + 40.0))));
+                }
+            }
+            for (int k = 0; k < 2; ++k)
+                kp_z[k] = kp25top * vcmaxcint[k] * cm_pow(2.0, (t_veg - (C_TFRZ + 25.0)) / 10.0);
+        }
+        for (int k = 0; k < 2; ++k)
+            if (X->cfg->light_inhibit && par_z[k] > 0.0) *lmr_z[k] = *lmr_z[k] * 0.67;
+        for (int k = 0; k < 2; ++k) {
+            PHS2(H->vcmax_z_phs, p, k) = vcmax_z[k];
+            PHS2(H->tpu_z_phs, p, k) = tpu_z[k];
+            PHS2(H->kp_z_phs, p, k) = kp_z[k];
+        }
+    }
+
+    /* leaf-level photosynthesis and conductance (:3485-3700) */
+    const double rsmax0 = 2.e4;
+    const double cf = forc_pbot / (C_RGAS * 1.e-3 * tgcm) * 1.e06;
+    const double gb = 1.0 / rb;
+    PA(S->gb_mol, p) = gb * cf;
+    const double gb_mol = PA(S->gb_mol, p);
+    double psn_wc_z[2] = {0.0, 0.0}, psn_wj_z[2] = {0.0, 0.0}, psn_wp_z[2] = {0.0, 0.0};
+    if (nrad >= 1) {
+        /* calcstress's view of the patch */
+        const ctsm_phs_state *Q = X->phs;
+        s.leaf.laisun = PA(S->laisun, p); s.leaf.laisha = PA(S->laisha, p); s.leaf.elai = PA(S->elai, p);
+        s.leaf.esai = PA(S->esai, p); s.leaf.tsai = PA(Q->tsai, p); s.leaf.htop = PA(S->htop, p);
+        s.leaf.fdry = PA(S->fdry, p); s.leaf.thm = tgcm; s.leaf.rho = CA(S->forc_rho, c); s.leaf.pbot = forc_pbot;
+        s.leaf.gb = gb_mol;
+        for (int k = 0; k < 4; ++k) {
+            s.leaf.kmax[k] = PP->kmax[k * np + ivt];
+            s.leaf.psi50[k] = PP->psi50[k * np + ivt];
+            s.leaf.ck[k] = PP->ck[k * np + ivt];
+        }
+        s.leaf.k = &SOIP(Q->k_soil_root, p, 1);
+        s.leaf.smp = &SOIC(Q->smp_l, c, 1);
+        s.leaf.z = &SNOC(Q->z, c, 1);
+        s.leaf.sk = L.ldp;
+        s.leaf.ss = L.ldc;
+        s.leaf.n = X->nlevsoi;
+        s.qsatl = qsatl;
+        s.qaf = qaf;
+        s.morning = PA(H->local_morning, p) != 0;
+        const double gsmin = (stomatal == CTSM_STOMATA_MEDLYN2011) ? F->medlynintercept[ivt] : PA(S->bbb, p);
+        double vegwp[4];
+        for (int k = 0; k < 4; ++k) vegwp[k] = VEGA(H->vegwp, p, k);
+        if (par_z[SUN] <= 0.0) {
+            /* night: the stress factors at the minimum conductance (:3500-3548) */
+            vegwp[SUN] = 1.0;
+            phs_calcstress(&s, vegwp, bsun, bsha, gsmin, gsmin);
+            for (int k = 0; k < 2; ++k) {
+                PHS2(H->ac_phs, p, k) = 0.0;
+                PHS2(H->aj_phs, p, k) = 0.0;
+                PHS2(H->ap_phs, p, k) = 0.0;
+                PHS2(H->ag_phs, p, k) = 0.0;
+                if (!crop || !modify) *an[k] = PHS2(H->ag_phs, p, k) - *bb[k] * *lmr_z[k];
+                else *an[k] = PHS2(H->ag_phs, p, k) - *lmr_z[k];
+                *psn_z[k] = 0.0;
+                psn_wc_z[k] = 0.0;
+                psn_wj_z[k] = 0.0;
+                psn_wp_z[k] = 0.0;
+                *rs_z[k] = fmin90(rsmax0, 1.0 / (fmax90(*bb[k] * gsmin, 1.0)) * cf);
+                *ci_z[k] = 0.0;
+            }
+            for (int k = 0; k < 2; ++k) *gs_mol[k] = cf / *rs_z[k];
+        } else {
+            /* day (:3550-3697) */
+            const double ceair = fmin90(eair, esat_tv);
+            double rh_can;
+            if (stomatal == CTSM_STOMATA_BB1987) {
+                rh_can = ceair / esat_tv;
+            } else {
+                rh_can = fmax90((esat_tv - ceair), 50.0) * 0.001;
+                PA(S->vpd_can, p) = rh_can;
+            }
+            for (int k = 0; k < 2; ++k) {
+                const double qabs = 0.5 * (1.0 - P->fnps) * par_z[k] * 4.6;
+                double r1, r2;
+                if (oracle_quadratic(P->theta_psii, -(qabs + jmax_z[k]), qabs * jmax_z[k], &r1, &r2)) rc = ORC_ENDRUN;
+                s.je[k] = fmin90(r1, r2);
+            }
+            for (int k = 0; k < 2; ++k) *ci_z[k] = c3 ? 0.7 * cair : 0.4 * cair;
+            s.c3 = c3;
+            s.stomatal = stomatal;
+            s.forc_pbot = forc_pbot;
+            s.cair = cair;
+            s.oair = oair;
+            s.gb_mol = gb_mol;
+            s.rh_can = rh_can;
+            s.theta_cj = F->theta_cj[ivt];
+            s.theta_ip = P->theta_ip;
+            s.qe = PA(S->qe, p);
+            s.bbb = PA(S->bbb, p);
+            s.mbb = PA(S->mbb, p);
+            s.medlynslope = F->medlynslope[ivt];
+            s.medlynintercept = F->medlynintercept[ivt];
+            s.cp = PA(S->cp, p);
+            s.kc = PA(S->kc, p);
+            s.ko = PA(S->ko, p);
+            for (int k = 0; k < 2; ++k) {
+                s.vcmax_z[k] = vcmax_z[k];
+                s.tpu_z[k] = tpu_z[k];
+                s.kp_z[k] = kp_z[k];
+                s.lmr_z[k] = *lmr_z[k];
+                s.par_z[k] = par_z[k];
+                s.ac[k] = PHS2(H->ac_phs, p, k);
+                s.aj[k] = PHS2(H->aj_phs, p, k);
+                s.ap[k] = PHS2(H->ap_phs, p, k);
+                s.ag[k] = PHS2(H->ag_phs, p, k);
+                s.an[k] = *an[k];
+            }
+            const double soilflux = hybrid_phs(&s, ci_z[SUN], ci_z[SHA], vegwp, bsun, bsha, gs_mol[SUN], gs_mol[SHA]);
+            s.tran_set = 1;
+            s.tran = soilflux;
+            if (s.err) rc = ORC_ENDRUN;
+            for (int k = 0; k < 2; ++k) {
+                PHS2(H->ac_phs, p, k) = s.ac[k];
+                PHS2(H->aj_phs, p, k) = s.aj[k];
+                PHS2(H->ap_phs, p, k) = s.ap[k];
+                PHS2(H->ag_phs, p, k) = s.ag[k];
+                *an[k] = s.an[k];
+            }
+            for (int k = 0; k < 4; ++k)
+                VEGA(H->vegwp_ln, p, k) = GA(S->near_local_noon, g) ? vegwp[k] : C_SPVAL;
+            const double gs_slope = (stomatal == CTSM_STOMATA_MEDLYN2011) ? F->medlynslope[ivt] : PA(S->mbb, p);
+            (void)gs_slope;   /* the reference's Ball-Berry check only logs */
+            for (int k = 0; k < 2; ++k)
+                if (*an[k] < 0.0) *gs_mol[k] = fmax90(*bb[k] * gsmin, 1.0);
+            for (int k = 0; k < 2; ++k) *gs_mol_ln[k] = GA(S->near_local_noon, g) ? *gs_mol[k] : C_SPVAL;
+            for (int k = 0; k < 2; ++k) {
+                double cs = cair - 1.4 / gb_mol * *an[k] * forc_pbot;
+                cs = fmax90(cs, MAX_CS);
+                (void)cs;
+                *ci_z[k] = cair - *an[k] * forc_pbot * (1.4 * *gs_mol[k] + 1.6 * gb_mol) / (gb_mol * *gs_mol[k]);
+                *ci_z[k] = fmax90(*ci_z[k], 1.e-06);
+            }
+            for (int k = 0; k < 2; ++k) {
+                const double gs = *gs_mol[k] / cf;
+                *rs_z[k] = fmin90(1.0 / gs, rsmax0);
+                *rs_z[k] = *rs_z[k] / o3coefg[k];
+            }
+            for (int k = 0; k < 2; ++k) {
+                *psn_z[k] = PHS2(H->ag_phs, p, k);
+                *psn_z[k] = *psn_z[k] * o3coefv[k];
+                psn_wc_z[k] = 0.0;
+                psn_wj_z[k] = 0.0;
+                psn_wp_z[k] = 0.0;
+                const double ac = PHS2(H->ac_phs, p, k), aj = PHS2(H->aj_phs, p, k), ap = PHS2(H->ap_phs, p, k);
+                if (ac <= aj && ac <= ap) psn_wc_z[k] = *psn_z[k];
+                else if (aj < ac && aj <= ap) psn_wj_z[k] = *psn_z[k];
+                else if (ap < ac && ap < aj) psn_wp_z[k] = *psn_z[k];
+            }
+            /* the leaf-surface humidity and the Ball-Berry check (:3677-3695) only log */
+            if (*gs_mol[SUN] < 0.0 || *gs_mol[SHA] < 0.0) rc = ORC_ENDRUN;
+        }
+        for (int k = 0; k < 4; ++k) VEGA(H->vegwp, p, k) = vegwp[k];
+        if (s.stressed)
+            for (int k = 0; k < 4; ++k) VEGA(H->vegwp_pd, p, k) = s.pd[k];
+        if (s.tran_set) PA(S->qflx_tran_veg, p) = s.tran;
+    }
+
+    /* canopy sums (:3705-3800) and btran */
+    double laican[2] = {0.0, 0.0};
+    for (int k = 0; k < 2; ++k) {
+        double psncan = 0.0, psncan_wc = 0.0, psncan_wj = 0.0, psncan_wp = 0.0, lmrcan = 0.0, gscan = 0.0;
+        if (nrad >= 1) {
+            psncan = psncan + *psn_z[k] * lai_z[k];
+            psncan_wc = psncan_wc + psn_wc_z[k] * lai_z[k];
+            psncan_wj = psncan_wj + psn_wj_z[k] * lai_z[k];
+            psncan_wp = psncan_wp + psn_wp_z[k] * lai_z[k];
+            if (!crop && modify) lmrcan = lmrcan + *lmr_z[k] * lai_z[k] * *bb[k];
+            else lmrcan = lmrcan + *lmr_z[k] * lai_z[k];
+            gscan = gscan + lai_z[k] / (rb + *rs_z[k]);
+            laican[k] = laican[k] + lai_z[k];
+        }
+        if (laican[k] > 0.0) {
+            *psn[k] = psncan / laican[k];
+            *psn_wc[k] = psncan_wc / laican[k];
+            *psn_wj[k] = psncan_wj / laican[k];
+            *psn_wp[k] = psncan_wp / laican[k];
+            *lmr[k] = lmrcan / laican[k];
+            *rs[k] = laican[k] / gscan - rb;
+        } else {
+            *psn[k] = 0.0;
+            *psn_wc[k] = 0.0;
+            *psn_wj[k] = 0.0;
+            *psn_wp[k] = 0.0;
+            *lmr[k] = 0.0;
+            *rs[k] = 0.0;
+        }
+    }
+    if (laican[SHA] + laican[SUN] > 0.0)
+        PA(S->btran, p) = *bsun * (laican[SUN] / (laican[SUN] + laican[SHA]))
+                          + *bsha * (laican[SHA] / (laican[SUN] + laican[SHA]));
+    else
+        PA(S->btran, p) = *bsun;
+    return rc;
+}
+
+/* Test access: PhotosynthesisHydraulicStress for patch p (1-based global
+ * index) at the leaf temperature held in t_veg(p), with the canopy-air
+ * arguments the iteration would pass; the conductances must already be in
+ * phs->k_soil_root.  Writes the patch's outputs as in the iteration. */
+int oracle_photosynthesis_phs_patch(const ctsm_config *cfg, const ctsm_canopy_params *prm, const ctsm_pft_params *pft,
+                                    const ctsm_phs_params *pp, const ctsm_layout *lay, const ctsm_col_state *col,
+                                    const ctsm_canopy_state *S, const ctsm_phs_state *phs,
+                                    const ctsm_phs_canopy_state *phc, int p, double esat_tv, double eair,
+                                    double oair, double cair, double rb, double dayl_factor, double qsatl,
+                                    double qaf)
+{
+    ctx_t X;
+    X.L.begc = lay->begc_alloc;
+    X.L.ldc = lay->ldc;
+    X.L.begp = lay->begp_alloc;
+    X.L.ldp = lay->ldp;
+    X.L.begg = lay->begg_alloc;
+    X.L.nlevsno = cfg->nlevsno;
+    X.L.nlevgrnd = cfg->nlevgrnd;
+    X.cfg = cfg;
+    X.prm = prm;
+    X.pft = pft;
+    X.col = col;
+    X.cs = S;
+    X.phs = phs;
+    X.phc = phc;
+    X.pp = pp;
+    X.nlevsoi = cfg->nlevsoi;
+    return photosynthesis_phs(&X, p, esat_tv, eair, oair, cair, rb, dayl_factor, qsatl, qaf);
+}
+
 /* Test access: Photosynthesis for patch p (1-based global index) and one
  * phase (sun = 1, shade = 0) at the leaf temperature held in t_veg(p), with
  * the canopy-air arguments the iteration would pass; writes the same patch
@@ -782,8 +1427,9 @@ double oracle_fth25(double hd, double se) { return fth25(hd, se); }
 /* ---------------------------------------------------------------------- */
 /* CanopyFluxesMod.F90:191-1765.  Returns 0, or ORC_ENDRUN with *bad_patch
  * set where the reference calls endrun. */
-int oracle_canopyfluxes(const ctsm_config *cfg, const ctsm_canopy_params *prm, const ctsm_pft_params *pft,
+static int canopyfluxes(const ctsm_config *cfg, const ctsm_canopy_params *prm, const ctsm_pft_params *pft,
                         const ctsm_layout *lay, const ctsm_col_state *col, const ctsm_canopy_state *S,
+                        const ctsm_phs_params *pp, const ctsm_phs_state *phs, const ctsm_phs_canopy_state *phc,
                         int num_exposedvegp, const int *filter_exposedvegp, double dtime, int *bad_patch)
 {
     ctx_t X;
@@ -801,6 +1447,11 @@ int oracle_canopyfluxes(const ctsm_config *cfg, const ctsm_canopy_params *prm, c
     X.pft = pft;
     X.col = col;
     X.cs = S;
+    X.phs = phs;
+    X.phc = phc;
+    X.pp = pp;
+    X.nlevsoi = cfg->nlevsoi;
+    const int hydrstress = phc != NULL;
     const ctsm_canopy_params *P = prm;
     const ctsm_pft_params *F = pft;
     const int nlevgrnd = cfg->nlevgrnd;
@@ -1085,20 +1736,32 @@ int oracle_canopyfluxes(const ctsm_config *cfg, const ctsm_canopy_params *prm, c
             PA(S->raw2, p) = PA(raw2, p);
             PA(S->vpd, p) = fmax90((PA(svpts, p) - PA(eah, p)), 50.0) * 0.001;
         }
-        for (f = 0; f < fn; ++f) {
-            p = filterp[f];
-            if (photosynthesis(&X, p, 1, PA(svpts, p), PA(eah, p), PA(o2, p), PA(co2, p), PA(rb, p), PA(S->btran, p),
-                               PA(dayl_factor, p))) {
-                rc = ORC_ENDRUN;
-                if (!*bad_patch) *bad_patch = p;
+        if (hydrstress) {
+            /* PhotosynthesisHydraulicStress, sun and shade together (:1133-1141) */
+            for (f = 0; f < fn; ++f) {
+                p = filterp[f];
+                if (photosynthesis_phs(&X, p, PA(svpts, p), PA(eah, p), PA(o2, p), PA(co2, p), PA(rb, p),
+                                       PA(dayl_factor, p), PA(qsatl, p), PA(S->qaf, p))) {
+                    rc = ORC_ENDRUN;
+                    if (!*bad_patch) *bad_patch = p;
+                }
             }
-        }
-        for (f = 0; f < fn; ++f) {
-            p = filterp[f];
-            if (photosynthesis(&X, p, 0, PA(svpts, p), PA(eah, p), PA(o2, p), PA(co2, p), PA(rb, p), PA(S->btran, p),
-                               PA(dayl_factor, p))) {
-                rc = ORC_ENDRUN;
-                if (!*bad_patch) *bad_patch = p;
+        } else {
+            for (f = 0; f < fn; ++f) {
+                p = filterp[f];
+                if (photosynthesis(&X, p, 1, PA(svpts, p), PA(eah, p), PA(o2, p), PA(co2, p), PA(rb, p),
+                                   PA(S->btran, p), PA(dayl_factor, p))) {
+                    rc = ORC_ENDRUN;
+                    if (!*bad_patch) *bad_patch = p;
+                }
+            }
+            for (f = 0; f < fn; ++f) {
+                p = filterp[f];
+                if (photosynthesis(&X, p, 0, PA(svpts, p), PA(eah, p), PA(o2, p), PA(co2, p), PA(rb, p),
+                                   PA(S->btran, p), PA(dayl_factor, p))) {
+                    rc = ORC_ENDRUN;
+                    if (!*bad_patch) *bad_patch = p;
+                }
             }
         }
         for (f = 0; f < fn; ++f) {
@@ -1130,7 +1793,16 @@ int oracle_canopyfluxes(const ctsm_config *cfg, const ctsm_canopy_params *prm, c
             double efpot = rho * ((PA(S->elai, p) + PA(S->esai, p)) / rbp) * (PA(qsatl, p) - PA(S->qaf, p));
             const double h2ocan = PA(S->liqcan, p) + PA(S->snocan, p);
             double rpp;
-            if (efpot > 0.0) {
+            if (hydrstress) {
+                /* transpiration comes from the hydraulics, not from rppdry (:1219-1230) */
+                if (efpot > 0.0) {
+                    if (PA(S->btran, p) > btran0) rpp = rppdry + PA(S->fwet, p);
+                    else rpp = PA(S->fwet, p);
+                    rpp = fmin90(rpp, (PA(S->qflx_tran_veg, p) + h2ocan / dtime) / efpot);
+                } else {
+                    rpp = 1.0;
+                }
+            } else if (efpot > 0.0) {
                 if (PA(S->btran, p) > btran0) {
                     PA(S->qflx_tran_veg, p) = efpot * rppdry;
                     rpp = rppdry + PA(S->fwet, p);
@@ -1209,8 +1881,10 @@ int oracle_canopyfluxes(const ctsm_config *cfg, const ctsm_canopy_params *prm, c
                     * (wtgaq * (PA(qsatl, p) + PA(qsatldT, p) * dtv) - wtgq0 * CA(S->qg, c) - PA(wtaq0, p) * CA(S->forc_q, c));
             PA(S->qflx_evap_veg, p) = rpp * efpot;
             double ecidif = 0.0;
-            if (efpot > 0.0 && PA(S->btran, p) > btran0) PA(S->qflx_tran_veg, p) = efpot * rppdry;
-            else PA(S->qflx_tran_veg, p) = 0.0;
+            if (!hydrstress) {   /* (:1353-1367) */
+                if (efpot > 0.0 && PA(S->btran, p) > btran0) PA(S->qflx_tran_veg, p) = efpot * rppdry;
+                else PA(S->qflx_tran_veg, p) = 0.0;
+            }
             ecidif = fmax90(0.0, PA(S->qflx_evap_veg, p) - PA(S->qflx_tran_veg, p) - h2ocan / dtime);
             PA(S->qflx_evap_veg, p) = fmin90(PA(S->qflx_evap_veg, p), PA(S->qflx_tran_veg, p) + h2ocan / dtime);
             PA(S->eflx_sh_veg, p) = efsh + dc1 * PA(wtga, p) * dtv + PA(err, p) + erre + C_HVAP * ecidif;
@@ -1398,6 +2072,32 @@ done:
     free(frac_rad_abs_by_stem); free(lw_stem); free(lw_leaf); free(sa_stem); free(sa_leaf); free(sa_internal);
     free(uuc); free(snocan_baseline); free(nmozsgn); free(filterp); free(fporig);
     return rc;
+}
+
+/* CanopyFluxesMod.F90:191-1765 with use_hydrstress = .false. */
+int oracle_canopyfluxes(const ctsm_config *cfg, const ctsm_canopy_params *prm, const ctsm_pft_params *pft,
+                        const ctsm_layout *lay, const ctsm_col_state *col, const ctsm_canopy_state *S,
+                        int num_exposedvegp, const int *filter_exposedvegp, double dtime, int *bad_patch)
+{
+    return canopyfluxes(cfg, prm, pft, lay, col, S, NULL, NULL, NULL, num_exposedvegp, filter_exposedvegp, dtime,
+                        bad_patch);
+}
+
+/* CanopyFluxesMod.F90:191-1765 with use_hydrstress = .true.: the root-soil
+ * conductances of the filter's patches (PhotosynthesisMod.F90:3062-3114, the
+ * loop PhotosynthesisHydraulicStress runs on every call over inputs the
+ * iteration leaves alone) once, then the iteration with
+ * PhotosynthesisHydraulicStress.  Returns 1 (bad_patch 0) for a patch whose
+ * column or type is out of range in the conductance loop. */
+int oracle_canopyfluxes_phs(const ctsm_config *cfg, const ctsm_canopy_params *prm, const ctsm_pft_params *pft,
+                            const ctsm_phs_params *pp, const ctsm_layout *lay, const ctsm_col_state *col,
+                            const ctsm_canopy_state *S, const ctsm_phs_state *phs, const ctsm_phs_canopy_state *phc,
+                            int num_exposedvegp, const int *filter_exposedvegp, double dtime, int *bad_patch)
+{
+    *bad_patch = 0;
+    if (oracle_phs_soil_root_conductance(cfg, lay, phs, pp, num_exposedvegp, filter_exposedvegp)) return 1;
+    return canopyfluxes(cfg, prm, pft, lay, col, S, pp, phs, phc, num_exposedvegp, filter_exposedvegp, dtime,
+                        bad_patch);
 }
 
 /* clm_driver.F90:1714-1716 (p2c_1d_filter of qflx_tran_veg) followed by

@@ -1050,6 +1050,7 @@ void oracle_config_default(ctsm_config *cfg)
     cfg->itmax_canopy_fluxes = 40;
     cfg->dtmin = 60.0; cfg->verySmall = 1.e-8; cfg->xTolerUpper = 1.e-1; cfg->xTolerLower = 1.e-2;
     cfg->e_ice = 6.0;
+    cfg->modifyphoto_and_lmr_forcrop = 1;
 }
 
 /* ========================================================================= */

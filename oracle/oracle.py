@@ -151,6 +151,17 @@ def _canopy_lib():
                                           C.POINTER(abi.PftParams), C.POINTER(abi.Layout),
                                           C.POINTER(abi.ColState), C.POINTER(abi.CanopyState), C.c_int, abi.i32p,
                                           C.c_double, abi.i32p]
+        L.oracle_canopyfluxes_phs.argtypes = [C.POINTER(abi.Config), C.POINTER(abi.CanopyParams),
+                                              C.POINTER(abi.PftParams), C.POINTER(abi.PhsParams),
+                                              C.POINTER(abi.Layout), C.POINTER(abi.ColState),
+                                              C.POINTER(abi.CanopyState), C.POINTER(abi.PhsState),
+                                              C.POINTER(abi.PhsCanopyState), C.c_int, abi.i32p, C.c_double,
+                                              abi.i32p]
+        L.oracle_photosynthesis_phs_patch.argtypes = [C.POINTER(abi.Config), C.POINTER(abi.CanopyParams),
+                                                      C.POINTER(abi.PftParams), C.POINTER(abi.PhsParams),
+                                                      C.POINTER(abi.Layout), C.POINTER(abi.ColState),
+                                                      C.POINTER(abi.CanopyState), C.POINTER(abi.PhsState),
+                                                      C.POINTER(abi.PhsCanopyState), C.c_int] + [C.c_double] * 8
         L.oracle_root_sink.argtypes = [C.POINTER(abi.Config), C.POINTER(abi.Layout), C.POINTER(abi.ColState),
                                        C.POINTER(abi.CanopyState), C.c_int, abi.i32p]
         L.oracle_qsat_vec.argtypes = [C.c_int, abi.f64p, abi.f64p, abi.f64p, abi.f64p, abi.f64p]
@@ -208,6 +219,45 @@ def canopyfluxes(st, cfg, prm, pft, filter_p=None, dtime=1800.0):
                                C.byref(st.layout()), C.byref(cs), C.byref(can), len(f), fp, C.c_double(dtime),
                                C.byref(bad))
     return rc, bad.value
+
+
+def canopyfluxes_phs(st, cphs, cfg, prm, pft, pp, filter_p=None, dtime=1800.0):
+    """CanopyFluxes with plant hydraulic stress (oracle_canopyfluxes_phs) on
+    the land state st and its CanopyPhs arrays cphs; cfg.use_hydrstress is
+    not read (the PHS path is this call)."""
+    L = _canopy_lib()
+    cs, _ = st.structs()
+    can = st.canopy_struct()
+    phs, phc = cphs.host_structs(st)
+    f, fp = _filt(filter_p if filter_p is not None else np.arange(1, st.dims.npatch + 1))
+    bad = C.c_int32(0)
+    rc = L.oracle_canopyfluxes_phs(C.byref(cfg), C.byref(canopy_params_struct(prm)), C.byref(pft.s), C.byref(pp),
+                                   C.byref(st.layout()), C.byref(cs), C.byref(can), C.byref(phs), C.byref(phc),
+                                   len(f), fp, C.c_double(dtime), C.byref(bad))
+    return rc, bad.value
+
+
+def photosynthesis_phs_patch(st, cphs, cfg, prm, pft, pp, p, esat_tv, eair, oair, cair, rb, dayl_factor, qsatl, qaf):
+    """PhotosynthesisHydraulicStress for patch p (1-based) at the leaf
+    temperature in st.can['t_veg'] (test access; the conductances must be in
+    cphs['k_soil_root']); outputs are written into st and cphs."""
+    L = _canopy_lib()
+    cs, _ = st.structs()
+    can = st.canopy_struct()
+    phs, phc = cphs.host_structs(st)
+    return L.oracle_photosynthesis_phs_patch(C.byref(cfg), C.byref(canopy_params_struct(prm)), C.byref(pft.s),
+                                             C.byref(pp), C.byref(st.layout()), C.byref(cs), C.byref(can),
+                                             C.byref(phs), C.byref(phc), int(p), esat_tv, eair, oair, cair, rb,
+                                             dayl_factor, qsatl, qaf)
+
+
+def phs_conductance_for(st, cphs, pp, cfg=None, filter_p=None):
+    """The root-soil conductances of a land state's patches into cphs."""
+    L = _phs_lib()
+    phs, _ = cphs.host_structs(st)
+    f, fp = _filt(filter_p if filter_p is not None else np.arange(1, st.dims.npatch + 1))
+    return L.oracle_phs_soil_root_conductance(C.byref(cfg or config()), C.byref(st.layout()), C.byref(phs),
+                                              C.byref(pp), len(f), fp)
 
 
 def photosynthesis_patch(st, cfg, prm, pft, p, sun, esat_tv, eair, oair, cair, rb, btran, dayl_factor=1.0):

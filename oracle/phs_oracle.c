@@ -45,6 +45,7 @@
 #include "ctsm_gpu.h"
 #include "ctsm_math.h"
 #include "oracle_libm.h"   /* -DORACLE_LIBM: the host libm instead */
+#include "phs_oracle.h"
 
 #define P_PI 3.14159265358979323846
 #define P_LN2 0.6931471805599453
@@ -151,16 +152,6 @@ int oracle_phs_root_sink(const ctsm_config *cfg, const ctsm_layout *L, const cts
 #define P_RGAS (6.02214e26 * 1.38065e-23)
 #define P_SPVAL 1.e36
 #define P_TOL_LAI 0.001
-
-typedef struct {
-    double laisun, laisha, elai, esai, tsai, htop, fdry, thm, rho, pbot, gb;
-    double kmax[4], psi50[4], ck[4];
-    const double *k;     /* k_soil_root(p, 1..n), stride ldp */
-    const double *smp;   /* smp_l(c, 1..n), stride ldc */
-    const double *z;     /* z(c, 1..n), stride ldc */
-    long sk, ss;
-    int n;
-} p_stress_t;
 
 static double p_plc(const p_stress_t *P, int seg, double x) { return oracle_phs_plc(x, P->psi50[seg], P->ck[seg]); }
 static double p_d1plc(const p_stress_t *P, int seg, double x) { return oracle_phs_d1plc(x, P->psi50[seg], P->ck[seg]); }
@@ -418,6 +409,18 @@ static void p_calcstress(const p_stress_t *P, double *x, double gs_sun_in, doubl
         if (soilflux < 0.0) soilflux = 0.0;
         *qflx_tran_veg = soilflux;
     }
+}
+
+void oracle_phs_calcstress_one(const p_stress_t *P, double *x, double gs_sun_in, double gs_sha_in, double qsatl,
+                               double qaf, double *bsun, double *bsha, int *night, double *qflx_tran_veg)
+{
+    p_calcstress(P, x, gs_sun_in, gs_sha_in, qsatl, qaf, bsun, bsha, night, qflx_tran_veg);
+}
+
+double oracle_phs_getvegwp_one(const p_stress_t *P, double *x, double gs_sun, double gs_sha, double qsatl,
+                               double qaf)
+{
+    return p_getvegwp(P, x, gs_sun, gs_sha, qsatl, qaf);
 }
 
 /* returns 0, or 1 when a patch's column or type is out of range */

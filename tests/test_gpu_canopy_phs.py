@@ -1,0 +1,175 @@
+"""CanopyFluxes with plant hydraulic stress on the GPU against the oracle, bit
+for bit (ctsm_canopyfluxes_phs_dev: the root-soil conductances, then the
+canopy iteration with PhotosynthesisHydraulicStress in place of
+Photosynthesis; CanopyFluxesMod.F90:1043-1061, PhotosynthesisMod.F90:
+2704-4486): Medlyn and Ball-Berry, crop types with and without the crop
+photosynthesis / respiration modification, day and night patches, the
+iteration cut into rounds differently, a patch filter, and the gating of the
+two CanopyFluxes entries on the context's use_hydrstress.
+
+Each test prints its case before the GPU work starts.
+"""
+import numpy as np
+import pytest
+
+from ctsm_amd import phs as PH
+from ctsm_amd import synthetic as S
+from ctsm_amd.state import DeviceState
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+NPFT = len(S.pft_table()["dleaf"])
+CROP = (np.arange(NPFT) >= 15).astype(np.float64)       # the two crop types of the 17-type table
+
+
+@pytest.fixture(scope="module")
+def pctx():
+    """A PHS context per (stomatal method, crop modification)."""
+    from ctsm_amd.lib import Context, default_config
+    made = {}
+
+    def get(stomatal=2, modify=1, itmax=40):
+        key = (stomatal, modify, itmax)
+        if key not in made:
+            ctx = Context(default_config(use_hydrstress=1, stomatalcond_method=stomatal,
+                                         modifyphoto_and_lmr_forcrop=modify, itmax_canopy_fluxes=itmax))
+            ctx.set_canopy_params(S.canopy_params(), S.pft_table())
+            ctx.set_phs_params(PH.default_params(NPFT, CROP))
+            made[key] = ctx
+        return made[key]
+
+    yield get
+    for c in made.values():
+        c.close()
+
+
+def _oracle_cfg(ctx):
+    cfg = O.canopy_config(stomatalcond_method=ctx.cfg.stomatalcond_method)
+    cfg.itmax_canopy_fluxes = ctx.cfg.itmax_canopy_fluxes
+    cfg.modifyphoto_and_lmr_forcrop = ctx.cfg.modifyphoto_and_lmr_forcrop
+    cfg.use_hydrstress = 1
+    return cfg
+
+
+def _workload(ncol, seed, night_frac=0.3):
+    st = S.canopy_workload("canopy_uneven", ncol=ncol)
+    A = PH.canopy_phs(st, seed, night_frac=night_frac)
+    # crop types on a share of the patches, so both crop branches run
+    it = st.can["itype"]
+    rng = np.random.default_rng(seed + 100)
+    it[rng.random(it.size) < 0.15] = 15
+    return st, A
+
+
+def _run_gpu(ctx, st, A, filt=None):
+    ds = DeviceState(st, ctx)
+    dphs = PH.DeviceCanopyPhs(ctx, A, ds)
+    fd = None
+    try:
+        if filt is not None:
+            fd = ctx.alloc(filt.nbytes)
+            ctx.h2d(fd, filt)
+        ctx.canopyfluxes_phs_dev(ds, dphs, fd, None if filt is None else len(filt))
+        ctx.sync()
+        return ds.to_host(), dphs.fetch()
+    finally:
+        if fd is not None:
+            ctx.free(fd)
+        dphs.free()
+        ds.free()
+
+
+def _run_oracle(ctx, st, A, filt=None):
+    o, oa = st.copy(), A.copy()
+    rc, bad = O.canopyfluxes_phs(o, oa, _oracle_cfg(ctx), S.canopy_params(), O.PftStruct(S.pft_table()),
+                                 PH.default_params(NPFT, CROP), filt)
+    assert rc == 0, bad
+    return o, oa
+
+
+def _assert_same(g, ga, o, oa):
+    for grp in ("col", "pch", "can"):
+        a, b = getattr(g, grp), getattr(o, grp)
+        for k in b:
+            np.testing.assert_array_equal(a[k], b[k], err_msg=f"{grp}.{k}")
+    for k in oa.a:
+        np.testing.assert_array_equal(ga[k], oa[k], err_msg=f"phs.{k}")
+
+
+@pytest.mark.parametrize("stomatal,modify", [(2, 1), (1, 1), (2, 0), (1, 0)],
+                         ids=["medlyn", "ballberry", "medlyn-nomodify", "ballberry-nomodify"])
+def test_canopyfluxes_phs_bitwise(pctx, stomatal, modify):
+    ctx = pctx(stomatal, modify)
+    st, A = _workload(384, 21 + stomatal + 2 * modify)
+    print(f"case: CanopyFluxes PHS stomatal={stomatal} modify={modify}, 384 columns / {st.dims.npatch} patches",
+          flush=True)
+    g, ga = _run_gpu(ctx, st, A)
+    o, oa = _run_oracle(ctx, st, A)
+    _assert_same(g, ga, o, oa)
+    # the case exercises what it is meant to: stressed and unstressed
+    # leaves, crops, day and night
+    nr = o.can["nrad"] >= 1
+    assert np.any(oa["bsun"][nr] < 0.9) and np.any(oa["bsun"][nr] > 0.95)
+    assert np.any(o.can["itype"] == 15)
+    assert np.any(o.can["parsun_z"][nr] == 0.0) and np.any(o.can["parsun_z"][nr] > 0.0)
+
+
+@pytest.mark.parametrize("first,later,itmax", [
+    (41, 3, 40),   # the whole iteration in one pass
+    (1, 1, 40),    # one iteration per round: the most rounds
+    (2, 5, 6),     # itmax reached inside the rounds
+])
+def test_canopyfluxes_phs_rounds_and_filter(pctx, first, later, itmax):
+    """Results do not depend on how the iteration is cut into rounds; patches
+    outside the filter are left as they were."""
+    ctx = pctx(2, 1, itmax)
+    st, A = _workload(256, 40 + first)
+    filt = np.arange(st.dims.npatch, 0, -2, dtype=np.int32)
+    print(f"case: CanopyFluxes PHS rounds first={first} later={later} itmax={itmax}, "
+          f"{len(filt)} of {st.dims.npatch} patches", flush=True)
+    before = ctx.get_tuning()
+    ctx.set_tuning(canopy_first_iterations=first, canopy_round_iterations=later)
+    try:
+        g, ga = _run_gpu(ctx, st, A, filt)
+    finally:
+        ctx.set_tuning(**before)
+    o, oa = _run_oracle(ctx, st, A, filt)
+    _assert_same(g, ga, o, oa)
+    rest = np.setdiff1d(np.arange(st.dims.npatch), filt - 1)
+    np.testing.assert_array_equal(ga["bsun"][rest], A["bsun"][rest])
+    if itmax < 40:
+        assert o.can["num_iter"].max() == itmax + 1
+
+
+def test_canopyfluxes_entries_gated_on_use_hydrstress(pctx):
+    from ctsm_amd.lib import CTSM_E_CONFIG, Context, CtsmError, default_config
+    st, A = _workload(8, 77)
+    print("case: CanopyFluxes entry gating, 8 columns", flush=True)
+    ctx = pctx()
+    ds = DeviceState(st, ctx)
+    try:
+        with pytest.raises(CtsmError) as ei:              # the plain entry on a PHS context
+            ctx.canopyfluxes_dev(ds)
+        assert ei.value.code == CTSM_E_CONFIG
+    finally:
+        ds.free()
+    plain = Context(default_config())
+    try:
+        plain.set_canopy_params(S.canopy_params(), S.pft_table())
+        plain.set_phs_params(PH.default_params(NPFT, CROP))
+        ds = DeviceState(st, plain)
+        dphs = PH.DeviceCanopyPhs(plain, A, ds)
+        try:
+            with pytest.raises(CtsmError) as ei:          # the PHS entry on a context without it
+                plain.canopyfluxes_phs_dev(ds, dphs)
+            assert ei.value.code == CTSM_E_CONFIG
+        finally:
+            dphs.free()
+            ds.free()
+    finally:
+        plain.close()
+    # the PHS context still works after the reported error
+    g, ga = _run_gpu(ctx, st, A)
+    o, oa = _run_oracle(ctx, st, A)
+    _assert_same(g, ga, o, oa)

@@ -20,9 +20,13 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE = os.path.join(ROOT, "oracle")
 SOURCES = ["ctsm_oracle.c", "canopy_oracle.c", "soilfluxes_oracle.c", "hydrology_oracle.c", "phs_oracle.c"]
-PINS = ["tests/test_oracle_pins.py", "tests/test_oracle_physics.py", "tests/test_oracle_canopy.py"]
+PINS = ["tests/test_oracle_pins.py", "tests/test_oracle_physics.py", "tests/test_oracle_canopy.py",
+        "tests/test_oracle_phs_canopy.py"]
 
-# (name, file, text in the oracle, planted text); the text occurs once.
+# (name, file, text in the oracle, planted text[, occurrences]); the text occurs
+# once unless a count is given: the hydraulic-stress photosynthesis restates
+# the same kinetics as the plain one (PhotosynthesisMod.F90 has them twice),
+# so a misreading of those lines is planted in both.
 MUTANTS = [
     ("thk: liquid and ice exponents swapped (SoilTemperatureMod.F90:720)", "ctsm_oracle.c",
      "cm_pow(TKWAT, fl * SOI(S->watsat, c, j)) *\n                    cm_pow(TKICE, (1.0 - fl) * SOI(S->watsat, c, j))",
@@ -50,21 +54,21 @@ MUTANTS = [
      "cm_pow(smp / SOI(S->sucsat, c, j), -1.0 / SOI(S->bsw, c, j))",
      "cm_pow(smp / SOI(S->sucsat, c, j), -SOI(S->bsw, c, j))"),
     ("vcmax: jmaxha for vcmaxha (PhotosynthesisMod.F90:1804)", "canopy_oracle.c",
-     "vcmax25 * ft(t_veg, P->vcmaxha)", "vcmax25 * ft(t_veg, P->jmaxha)"),
+     "PA(S->vcmax_z, p) = vcmax25 * ft(t_veg, P->vcmaxha)", "PA(S->vcmax_z, p) = vcmax25 * ft(t_veg, P->jmaxha)"),
     ("ft: Arrhenius ratio inverted", "canopy_oracle.c",
      "(1.0 - (C_TFRZ + 25.0) / tl)", "(1.0 - tl / (C_TFRZ + 25.0))"),
     ("kc25 from the Ko coefficient", "canopy_oracle.c",
-     "const double kc25 = P->kc25_coef * forc_pbot;", "const double kc25 = P->ko25_coef * forc_pbot;"),
+     "const double kc25 = P->kc25_coef * forc_pbot;", "const double kc25 = P->ko25_coef * forc_pbot;", 2),
     ("cp25 without the 0.5", "canopy_oracle.c",
-     "const double cp25 = 0.5 * oair / sco;", "const double cp25 = oair / sco;"),
+     "const double cp25 = 0.5 * oair / sco;", "const double cp25 = oair / sco;", 2),
     ("lmr: vcmaxha for lmrha", "canopy_oracle.c",
-     "lmr25 * ft(t_veg, P->lmrha)", "lmr25 * ft(t_veg, P->vcmaxha)"),
+     "lmr25 * ft(t_veg, P->lmrha)", "lmr25 * ft(t_veg, P->vcmaxha)", 2),
     ("vcmax: fth25 normalised with jmaxhd's partner entropy", "canopy_oracle.c",
-     "const double vcmaxc = fth25(P->vcmaxhd, vcmaxse);", "const double vcmaxc = fth25(P->vcmaxhd, vcmaxse * 1.01);"),
+     "const double vcmaxc = fth25(P->vcmaxhd, vcmaxse);", "const double vcmaxc = fth25(P->vcmaxhd, vcmaxse * 1.01);", 2),
     ("vcmaxse acclimation slope sign", "canopy_oracle.c",
-     "const double vcmaxse = (668.39 - 1.07 * t10c)", "const double vcmaxse = (668.39 + 1.07 * t10c)"),
+     "const double vcmaxse = (668.39 - 1.07 * t10c)", "const double vcmaxse = (668.39 + 1.07 * t10c)", 2),
     ("C4 low-temperature inhibition sign", "canopy_oracle.c",
-     "(1.0 + cm_exp(0.2 * ((C_TFRZ + 15.0) - t_veg)))", "(1.0 + cm_exp(0.2 * (t_veg - (C_TFRZ + 15.0))))"),
+     "(1.0 + cm_exp(0.2 * ((C_TFRZ + 15.0) - t_veg)))", "(1.0 + cm_exp(0.2 * (t_veg - (C_TFRZ + 15.0))))", 2),
     ("co-limitation: curvatures swapped", "canopy_oracle.c",
      "if (oracle_quadratic(s->theta_cj, -(s->ac + s->aj)", "if (oracle_quadratic(s->theta_ip, -(s->ac + s->aj)"),
     ("co-limitation: larger root", "canopy_oracle.c",
@@ -86,6 +90,18 @@ MUTANTS = [
     ("errseb without the canopy heat storage (BalanceCheckMod.F90:989-990)", "soilfluxes_oracle.c",
      "- F->eflx_lh_tot[pp] - F->eflx_soil_grnd[pp] - Cn->dhsdt_canopy[pp];",
      "- F->eflx_lh_tot[pp] - F->eflx_soil_grnd[pp];"),
+    ("PHS: getvegwp from the unstressed conductances (PhotosynthesisMod.F90:4051)", "canopy_oracle.c",
+     "oracle_phs_getvegwp_one(&s->leaf, x, *gs_sun, *gs_sha, s->qsatl, s->qaf);",
+     "oracle_phs_getvegwp_one(&s->leaf, x, gs0sun, gs0sha, s->qsatl, s->qaf);"),
+    ("PHS: Ball-Berry minimum conductance without the stress factor (:4431-4432)", "canopy_oracle.c",
+     "const double bq = cs[k] * (s->gb_mol - fmax90(bk * s->bbb, 1.0)) - s->mbb * s->an[k] * s->forc_pbot;",
+     "const double bq = cs[k] * (s->gb_mol - fmax90(s->bbb, 1.0)) - s->mbb * s->an[k] * s->forc_pbot;"),
+    ("PHS: night respiration without bsun (:3516-3520)", "canopy_oracle.c",
+     "if (!crop || !modify) *an[k] = PHS2(H->ag_phs, p, k) - *bb[k] * *lmr_z[k];",
+     "if (!crop || !modify) *an[k] = PHS2(H->ag_phs, p, k) - *lmr_z[k];"),
+    ("PHS: btran weights swapped (:3789-3790)", "canopy_oracle.c",
+     "PA(S->btran, p) = *bsun * (laican[SUN] / (laican[SUN] + laican[SHA]))",
+     "PA(S->btran, p) = *bsun * (laican[SHA] / (laican[SUN] + laican[SHA]))"),
     ("errseb with the ground heat flux's sign flipped", "soilfluxes_oracle.c",
      "- F->eflx_lh_tot[pp] - F->eflx_soil_grnd[pp] - Cn->dhsdt_canopy[pp];",
      "- F->eflx_lh_tot[pp] + F->eflx_soil_grnd[pp] - Cn->dhsdt_canopy[pp];"),
@@ -94,13 +110,13 @@ MUTANTS = [
 CFLAGS = ["-O2", "-mfma", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-std=c99", "-w"]
 
 
-def _build(tmp, name, old, new):
+def _build(tmp, name, old, new, times=1):
     for f in SOURCES:
         shutil.copy(os.path.join(ORACLE, f), tmp)
     path = os.path.join(tmp, name)
     with open(path) as fh:
         s = fh.read()
-    assert s.count(old) == 1, f"{name}: planted text must occur exactly once: {old!r}"
+    assert s.count(old) == times, f"{name}: planted text must occur exactly {times}x: {old!r}"
     with open(path, "w") as fh:
         fh.write(s.replace(old, new))
     lib = os.path.join(tmp, "liboracle.so")
@@ -112,9 +128,9 @@ def _build(tmp, name, old, new):
 
 
 def _caught(mutant):
-    desc, name, old, new = mutant
+    desc, name, old, new, *times = mutant
     with tempfile.TemporaryDirectory() as tmp:
-        lib = _build(tmp, name, old, new)
+        lib = _build(tmp, name, old, new, *times)
         env = dict(os.environ, CTSM_ORACLE_LIB=lib)
         r = subprocess.run([sys.executable, "-m", "pytest", "-x", "-q", "-p", "no:cacheprovider", "-m", "not gpu"]
                            + PINS, cwd=ROOT, env=env, capture_output=True, text=True, timeout=600)
