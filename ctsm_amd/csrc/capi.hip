@@ -76,7 +76,7 @@ using namespace ctsm;
 
 namespace {
 
-enum Kind { K_C, K_SNO, K_ZI, K_SOI, K_SNOW, K_P, K_LYR, K_G, K_SOIP, K_VEG };
+enum Kind { K_C, K_SNO, K_ZI, K_SOI, K_SNOW, K_P, K_LYR, K_G, K_SOIP, K_VEG, K_PHS2 };
 
 struct FieldMeta {
     size_t offset;   // offset of the pointer member in the state struct
@@ -283,6 +283,19 @@ const FieldMeta kStC[] = {
     PS(forc_rho, K_C, 8, false), PS(forc_pbot, K_C, 8, false), PS(z, K_SNO, 8, false), PS(smp_l, K_SOI, 8, false),
 };
 #undef PS
+// CanopyFluxes with plant hydraulic stress: every ctsm_phs_canopy_state member
+#define PC(name, kind, elem, wr) {offsetof(ctsm_phs_canopy_state, name), kind, elem, wr}
+const FieldMeta kPhc[] = {
+    PC(tlai_z, K_P, 8, false), PC(local_morning, K_P, 4, false), PC(vegwp, K_VEG, 8, true),
+    PC(vegwp_ln, K_VEG, 8, true), PC(vegwp_pd, K_VEG, 8, true), PC(bsun, K_P, 8, true), PC(bsha, K_P, 8, true),
+    PC(an_sun, K_P, 8, true), PC(an_sha, K_P, 8, true), PC(vcmax_z_phs, K_PHS2, 8, true),
+    PC(tpu_z_phs, K_PHS2, 8, true), PC(kp_z_phs, K_PHS2, 8, true), PC(ac_phs, K_PHS2, 8, true),
+    PC(aj_phs, K_PHS2, 8, true), PC(ap_phs, K_PHS2, 8, true), PC(ag_phs, K_PHS2, 8, true),
+};
+#undef PC
+constexpr int kNPhc = sizeof(kPhc) / sizeof(kPhc[0]);
+static_assert(kNPhc * sizeof(void *) == sizeof(ctsm_phs_canopy_state),
+              "every ctsm_phs_canopy_state member has a mirror entry");
 constexpr int kNStP = sizeof(kStP) / sizeof(kStP[0]);
 constexpr int kNStC = sizeof(kStC) / sizeof(kStC[0]);
 static_assert((kNStP + kNStC) * sizeof(void *) == sizeof(ctsm_phs_stress_state),
@@ -305,6 +318,7 @@ int nlev_of(Kind k, const ctsm_config &cfg)
     case K_C: case K_P: case K_G: return 1;
     case K_SOIP: return cfg.nlevgrnd;
     case K_VEG: return 4;
+    case K_PHS2: return 2;
     case K_SNO: return cfg.nlevsno + cfg.nlevgrnd;
     case K_ZI: return cfg.nlevsno + cfg.nlevgrnd + 1;
     case K_SOI: return cfg.nlevgrnd;
@@ -401,6 +415,7 @@ struct ctsm_ctx {
     int32_t phs_npft = 0;              // 0 until ctsm_set_phs_params
     bool phs_kmax = false;             // the table holds kmax (calcstress may run)
     std::vector<DevBuf> stp, stc;      // calcstress drop-in mirrors
+    std::vector<DevBuf> phcm;          // PHS CanopyFluxes drop-in mirrors
     bool have_canopy = false;
     int32_t last_code = 0, last_index = 0;
     std::mutex mu;
@@ -703,6 +718,7 @@ int ctsm_destroy(ctsm_ctx *ctx)
     for (auto &b : ctx->phsc) b.release();
     for (auto &b : ctx->stp) b.release();
     for (auto &b : ctx->stc) b.release();
+    for (auto &b : ctx->phcm) b.release();
     ctx->phs_tab.release();
     ctx->partf.release();
     for (int k = 0; k < ctsm_ctx::kMaxStreams - 1; ++k) {
@@ -975,7 +991,7 @@ static int run_canopy(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_stat
                       void *fixed_work = nullptr, size_t fixed_bytes = 0, bool bounds_are_filter = false,
                       const ctsm_phs_state *phs_q = nullptr, const ctsm_phs_canopy_state *phs_h = nullptr)
 {
-    // the configuration fixes the form: the PHS one only through ctsm_canopyfluxes_phs_dev
+    // the configuration fixes the form: the PHS one only through the ctsm_canopyfluxes_phs entries
     if ((ctx->cfg.use_hydrstress != 0) != (phs_h != nullptr)) return CTSM_E_CONFIG;
     const bool one_pass = ctx->tun.plain_routines != 0;
     const int k0 = one_pass ? 1001 : ctx->tun.canopy_first_iterations;
@@ -1595,18 +1611,27 @@ int ctsm_phs_soil_root_conductance_dev(ctsm_ctx *ctx, const ctsm_layout *lay, co
     return finish_launch(ctx);
 }
 
-int ctsm_canopyfluxes_phs_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
-                              const ctsm_canopy_state *can, const ctsm_phs_state *phs,
-                              const ctsm_phs_canopy_state *phc, int32_t n, const int32_t *filterp_dev, double dtime)
+static int canopy_phs_check(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                            const ctsm_canopy_state *can, const ctsm_phs_state *phs,
+                            const ctsm_phs_canopy_state *phc, int32_t n, bool has_filter)
 {
-    if (!ctx || !col || !can || !phc || n < 0 || check_canopy_layout(ctx, lay) || (!filterp_dev && n > lay->ldp))
+    if (!ctx || !col || !can || !phc || n < 0 || check_canopy_layout(ctx, lay) || (!has_filter && n > lay->ldp))
         return CTSM_E_ARG;
     if (!ctx->cfg.use_hydrstress) return CTSM_E_CONFIG;
-    int rc = phs_check(ctx, lay, phs, n, true, filterp_dev != nullptr);
+    int rc = phs_check(ctx, lay, phs, n, true, has_filter);
     if (rc) return rc;
     if (!ctx->phs_kmax) return CTSM_E_ARG;   // calcstress needs kmax
     for (size_t off = 0; off < sizeof(ctsm_phs_canopy_state); off += sizeof(void *))
         if (!*(void *const *)((const char *)phc + off)) return CTSM_E_ARG;
+    return CTSM_OK;
+}
+
+int ctsm_canopyfluxes_phs_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                              const ctsm_canopy_state *can, const ctsm_phs_state *phs,
+                              const ctsm_phs_canopy_state *phc, int32_t n, const int32_t *filterp_dev, double dtime)
+{
+    int rc = canopy_phs_check(ctx, lay, col, can, phs, phc, n, filterp_dev != nullptr);
+    if (rc) return rc;
     // the conductance loop at the head of PhotosynthesisHydraulicStress, then the iteration
     launch_phs_conductance(ctx->stream, *lay, ctx->cfg, *phs, (const double *)ctx->phs_tab.p, ctx->phs_npft, n,
                            filterp_dev, ctx->err);
@@ -1667,6 +1692,48 @@ int ctsm_phs_root_sink(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_phs_sta
                        const int32_t *filterc)
 {
     return phs_dropin(ctx, lay, phs, n, filterc, false);
+}
+
+// Drop-in form of ctsm_canopyfluxes_phs_dev: the column, canopy, conductance
+// and PHS canopy members mirrored over the whole allocation, the writable ones
+// copied back.  Arrays two structs share (vegwp, the aliased inputs) get one
+// mirror each; only one of them is writable, so the copy back is unambiguous.
+int ctsm_canopyfluxes_phs(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                          const ctsm_canopy_state *can, const ctsm_phs_state *phs, const ctsm_phs_canopy_state *phc,
+                          int32_t n, const int32_t *filterp, double dtime)
+{
+    int rc = canopy_phs_check(ctx, lay, col, can, phs, phc, n, filterp != nullptr);
+    if (rc) return rc;
+    for (int i = 0; filterp && i < n; ++i)
+        if (filterp[i] < lay->begp_alloc || filterp[i] >= lay->begp_alloc + lay->ldp) return CTSM_E_ARG;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipError_t he = hipSetDevice(ctx->device);
+    if (he != hipSuccess) return hip_fail(ctx, he);
+    const Range cr{lay->begc_alloc, lay->begc_alloc + lay->ldc - 1};
+    const Range pr{lay->begp_alloc, lay->begp_alloc + lay->ldp - 1};
+    const ctsm_phs_state h = phs_pick(phs, true);
+    ctsm_col_state dcol{};
+    ctsm_canopy_state dcan{};
+    ctsm_phs_state dphs{};
+    ctsm_phs_canopy_state dphc{};
+    const int ncolf = (int)(sizeof(kColFields) / sizeof(kColFields[0]));
+    auto move = [&](bool to_device) {
+        int r = mirror(ctx, kColFields, ncolf, col, &dcol, ctx->colmir, lay->begc_alloc, lay->ldc, cr, to_device);
+        if (!r) r = canopy_mirror(ctx, lay, can, &dcan, to_device);
+        if (!r) r = mirror(ctx, kPhsP, kNPhsP, &h, &dphs, ctx->phsp, lay->begp_alloc, lay->ldp, pr, to_device);
+        if (!r) r = mirror(ctx, kPhsC, kNPhsC, &h, &dphs, ctx->phsc, lay->begc_alloc, lay->ldc, cr, to_device);
+        if (!r) r = mirror(ctx, kPhc, kNPhc, phc, &dphc, ctx->phcm, lay->begp_alloc, lay->ldp, pr, to_device);
+        return r;
+    };
+    if ((rc = move(true))) return rc;
+    const int32_t *dfilt;
+    if ((rc = upload_filter(ctx, n, filterp, &dfilt))) return rc;
+    launch_phs_conductance(ctx->stream, *lay, ctx->cfg, dphs, (const double *)ctx->phs_tab.p, ctx->phs_npft, n, dfilt,
+                           ctx->err);
+    if ((rc = run_canopy(ctx, lay, dcol, dcan, n, dfilt, dtime, false, nullptr, 0, false, &dphs, &dphc))) return rc;
+    if ((rc = finish_launch(ctx))) return rc;
+    if ((rc = move(false))) return rc;
+    return sync_and_check(ctx);
 }
 
 static int stress_check(const ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_phs_stress_state *st, int32_t n,
@@ -1802,6 +1869,7 @@ int ctsm_canopyfluxes(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_stat
 {
     if (!ctx || !col || !can || n < 0 || check_canopy_layout(ctx, lay) || (!filterp && n > lay->ldp))
         return CTSM_E_ARG;
+    if (ctx->cfg.use_hydrstress) return CTSM_E_CONFIG;   // before any copy: the PHS entries serve this context
     std::lock_guard<std::mutex> g(ctx->mu);
     hipError_t he = hipSetDevice(ctx->device);
     if (he != hipSuccess) return hip_fail(ctx, he);

@@ -440,6 +440,23 @@ module ctsm_gpu_mod
        integer(c_int32_t), value :: num_filterp
        integer(c_int32_t), intent(in) :: filterp(*)
      end function ctsm_phs_calcstress
+
+     ! CanopyFluxes with use_hydrstress = .true. (the context's config has
+     ! use_hydrstress = 1): in place of the call at clm_driver.F90:766
+     integer(c_int) function ctsm_canopyfluxes_phs(ctx, lay, col, can, phs, phc, num_exposedvegp, &
+          filter_exposedvegp, dtime) bind(C, name='ctsm_canopyfluxes_phs')
+       import :: c_int, c_int32_t, c_double, c_ptr, ctsm_layout_t, ctsm_col_state_t, ctsm_canopy_state_t, &
+            ctsm_phs_state_t, ctsm_phs_canopy_state_t
+       type(c_ptr), value :: ctx
+       type(ctsm_layout_t), intent(in) :: lay
+       type(ctsm_col_state_t), intent(in) :: col
+       type(ctsm_canopy_state_t), intent(in) :: can
+       type(ctsm_phs_state_t), intent(in) :: phs
+       type(ctsm_phs_canopy_state_t), intent(in) :: phc
+       integer(c_int32_t), value :: num_exposedvegp
+       integer(c_int32_t), intent(in) :: filter_exposedvegp(*)
+       real(c_double), value :: dtime
+     end function ctsm_canopyfluxes_phs
   end interface
 
 end module ctsm_gpu_mod

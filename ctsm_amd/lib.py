@@ -32,7 +32,7 @@ EXPORTS = [
     "ctsm_soil_step_fused_dev", "ctsm_dev_alloc", "ctsm_dev_free", "ctsm_memcpy", "ctsm_dev_copy",
     "ctsm_event_record", "ctsm_event_elapsed", "ctsm_event_destroy", "ctsm_sync", "ctsm_stream",
     "ctsm_set_canopy_params", "ctsm_canopyfluxes", "ctsm_root_sink", "ctsm_canopyfluxes_dev",
-    "ctsm_canopyfluxes_phs_dev",
+    "ctsm_canopyfluxes_phs_dev", "ctsm_canopyfluxes_phs",
     "ctsm_root_sink_dev", "ctsm_land_step_dev", "ctsm_memcpy2d",
     "ctsm_graph_capture", "ctsm_graph_launch", "ctsm_graph_destroy",
     "ctsm_soilfluxes", "ctsm_soil_balance_check", "ctsm_soilfluxes_dev", "ctsm_soil_balance_check_dev",
@@ -139,6 +139,7 @@ def load():
     L.ctsm_canopyfluxes_dev_noalloc.argtypes = can + [C.c_int32, C.c_void_p, C.c_double]
     L.ctsm_canopyfluxes_phs_dev.argtypes = can + [P(abi.PhsState), P(abi.PhsCanopyState), C.c_int32, C.c_void_p,
                                                   C.c_double]
+    L.ctsm_canopyfluxes_phs.argtypes = can + [P(abi.PhsState), P(abi.PhsCanopyState), C.c_int32, abi.i32p, C.c_double]
     L.ctsm_set_pft_rows.argtypes = [C.c_void_p, C.c_int32, P(abi.PftParams)]
     L.ctsm_energy_balance_check_dev.argtypes = [C.c_void_p, P(abi.Layout), P(abi.ColState), P(abi.PatchState),
                                                 P(abi.CanopyState), P(abi.SoilFluxState), C.c_int32, C.c_void_p,
@@ -342,6 +343,16 @@ class Context:
         f, fp, n = _filter(filter_p, st.dims.npatch)
         self._check(self.L.ctsm_canopyfluxes(self.h, C.byref(st.layout()), C.byref(cs), C.byref(can), n, fp, dtime),
                     "CanopyFluxes")
+
+    def canopyfluxes_phs(self, st, cphs, filter_p=None, dtime=1800.0):
+        """Drop-in CanopyFluxes with plant hydraulic stress on the host state st
+        and its phs.CanopyPhs arrays (ctsm_canopyfluxes_phs)."""
+        cs, _ = st.structs()
+        can = st.canopy_struct()
+        phs, phc = cphs.host_structs(st)
+        f, fp, n = _filter(filter_p, st.dims.npatch)
+        self._check(self.L.ctsm_canopyfluxes_phs(self.h, C.byref(st.layout()), C.byref(cs), C.byref(can),
+                                                 C.byref(phs), C.byref(phc), n, fp, dtime), "CanopyFluxes (PHS)")
 
     def root_sink(self, st, filter_c=None):
         cs, _ = st.structs()

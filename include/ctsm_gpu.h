@@ -856,6 +856,15 @@ int ctsm_canopyfluxes_phs_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_
                               const ctsm_canopy_state *can, const ctsm_phs_state *phs,
                               const ctsm_phs_canopy_state *phc, int32_t num_filterp, const int32_t *filterp_dev,
                               double dtime);
+/* Drop-in form (host arrays, as ctsm_canopyfluxes): replaces the call of
+ * CanopyFluxes at clm_driver.F90:766 when use_hydrstress = .true.  Copies the
+ * members in, runs ctsm_canopyfluxes_phs_dev's work, copies the outputs
+ * (col, can, the three conductances of phs, every phc member but tlai_z and
+ * local_morning) back and waits. */
+int ctsm_canopyfluxes_phs(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                          const ctsm_canopy_state *can, const ctsm_phs_state *phs,
+                          const ctsm_phs_canopy_state *phc, int32_t num_exposedvegp, const int32_t *filter_exposedvegp,
+                          double dtime);
 int ctsm_phs_calcstress_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_phs_stress_state *st,
                             int32_t num_filterp, const int32_t *filterp_dev);
 int ctsm_soilfluxes_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,

@@ -4,8 +4,9 @@ canopy iteration with PhotosynthesisHydraulicStress in place of
 Photosynthesis; CanopyFluxesMod.F90:1043-1061, PhotosynthesisMod.F90:
 2704-4486): Medlyn and Ball-Berry, crop types with and without the crop
 photosynthesis / respiration modification, day and night patches, the
-iteration cut into rounds differently, a patch filter, and the gating of the
-two CanopyFluxes entries on the context's use_hydrstress.
+iteration cut into rounds differently, a patch filter, the host-array drop-in
+(ctsm_canopyfluxes_phs), and the gating of the plain and PHS CanopyFluxes
+entries on the context's use_hydrstress.
 
 Each test prints its case before the GPU work starts.
 """
@@ -142,6 +143,24 @@ def test_canopyfluxes_phs_rounds_and_filter(pctx, first, later, itmax):
         assert o.can["num_iter"].max() == itmax + 1
 
 
+@pytest.mark.parametrize("stomatal", [2, 1], ids=["medlyn", "ballberry"])
+def test_canopyfluxes_phs_dropin(pctx, stomatal):
+    """The host-array entry (ctsm_canopyfluxes_phs) on a random third of the
+    patches: the same bits as the oracle, the rest untouched."""
+    ctx = pctx(stomatal, 1)
+    st, A = _workload(200, 60 + stomatal)
+    rng = np.random.default_rng(61)
+    filt = np.sort(rng.choice(np.arange(1, st.dims.npatch + 1), st.dims.npatch // 3, replace=False)).astype(np.int32)
+    print(f"case: CanopyFluxes PHS drop-in stomatal={stomatal}, {len(filt)} of {st.dims.npatch} patches", flush=True)
+    g, ga = st.copy(), A.copy()
+    ctx.canopyfluxes_phs(g, ga, filter_p=filt)
+    o, oa = _run_oracle(ctx, st, A, filt)
+    _assert_same(g, ga, o, oa)
+    rest = np.setdiff1d(np.arange(st.dims.npatch), filt - 1)
+    for k in ("bsun", "k_soil_root", "vegwp"):
+        np.testing.assert_array_equal(ga[k][..., rest], A[k][..., rest], err_msg=k)
+
+
 def test_canopyfluxes_entries_gated_on_use_hydrstress(pctx):
     from ctsm_amd.lib import CTSM_E_CONFIG, Context, CtsmError, default_config
     st, A = _workload(8, 77)
@@ -151,6 +170,12 @@ def test_canopyfluxes_entries_gated_on_use_hydrstress(pctx):
     try:
         with pytest.raises(CtsmError) as ei:              # the plain entry on a PHS context
             ctx.canopyfluxes_dev(ds)
+        assert ei.value.code == CTSM_E_CONFIG
+        with pytest.raises(CtsmError) as ei:              # and the land step, which calls it first
+            ctx.land_step_dev(ds)
+        assert ei.value.code == CTSM_E_CONFIG
+        with pytest.raises(CtsmError) as ei:              # and the plain drop-in
+            ctx.canopyfluxes(st.copy())
         assert ei.value.code == CTSM_E_CONFIG
     finally:
         ds.free()
@@ -163,6 +188,9 @@ def test_canopyfluxes_entries_gated_on_use_hydrstress(pctx):
         try:
             with pytest.raises(CtsmError) as ei:          # the PHS entry on a context without it
                 plain.canopyfluxes_phs_dev(ds, dphs)
+            assert ei.value.code == CTSM_E_CONFIG
+            with pytest.raises(CtsmError) as ei:          # and its drop-in
+                plain.canopyfluxes_phs(st.copy(), A.copy())
             assert ei.value.code == CTSM_E_CONFIG
         finally:
             dphs.free()
