@@ -726,11 +726,75 @@ static_assert(sizeof(CanopyWork) == N_WORK_FIELDS * sizeof(double *), "one point
 // PHS: CanopyFluxes with plant hydraulic stress, H its PhsArgs (unused
 // otherwise): PhotosynthesisHydraulicStress in place of the two Photosynthesis
 // calls and the branches of CanopyFluxesMod.F90:1219-1230, 1353-1356.
+// The column-level half of calc_effective_soilporosity, calc_volumetric_h2oliq
+// and calc_root_moist_stress (SoilMoistStressMod.F90:70-126, 190-240,
+// 420-514): effective porosity, volumetric liquid water and, where the level
+// takes up water, the soil matric potential of CH78 before the PFT's floor
+// and eff / watsat.  The reference computes them once per patch of the filter
+// (CanopyFluxesMod.F90:833-891 builds a column filter with one entry per
+// patch); here the 64 lanes of a wave share the work of the runs of lanes
+// whose patches share a column, one (run, level) item per lane at a time, and
+// each lane's patch then reads its column's terms from LDS.  Same operations on
+// the same inputs as the per-patch form, so the same bits.  kNoUptake marks a
+// level that takes up nothing (no liquid, or frozen below -2 C): a signalling
+// NaN, which no arithmetic produces.
+constexpr long long kNoUptake = 0x7ff0dead00000001LL;
+
+struct SoilTerms {
+    double smp;     // -sucsat s^-bsw, or kNoUptake
+    double ratio;   // eff_porosity / watsat
+};
+
+// Returns the index of the lane's run (-1 for a lane without a valid patch);
+// every lane of the wave calls it (it synchronises the block, one wave).
+template <class COL, class CAN>
+__device__ int soil_terms_of_wave(const CView &V, const COL &col, const CAN &S, const int32_t *__restrict__ filter,
+                                  int i0, bool active, SoilTerms *terms, int *run_col)
+{
+    const int lane = threadIdx.x;
+    int c = -1;
+    if (active) {
+        // the patches canopy_patch accepts (it reports the others)
+        const long pp = V.p1(filter ? filter[i0] : V.begp + i0);
+        const int cp = S.column[pp], g = S.gridcell[pp], ivt = S.itype[pp];
+        if (!(ivt < 0 || ivt >= CTSM_MAX_PFT || cp < V.begc || cp >= V.begc + V.ldc || g < V.begg ||
+              g >= V.begg + V.ldg))
+            c = cp;
+    }
+    const int prev = __shfl_up(c, 1);
+    const bool lead = c >= 0 && (lane == 0 || prev != c);
+    const unsigned long long leaders = __ballot(lead);
+    if (lead) run_col[__popcll(leaders & ((1ull << lane) - 1ull))] = c;
+    __syncthreads();
+    const int nl = V.nlevgrnd;
+    const int items = __popcll(leaders) * nl;
+    for (int it = lane; it < items; it += 64) {
+        const int r = it / nl, j = it - r * nl + 1;
+        const int rc = run_col[r];
+        const long cs = V.soi(rc, j), cn = V.sno(rc, j);
+        const double watsat = col.watsat[cs], dz = col.dz[cn];
+        const double vol_ice = fmin90(watsat, col.h2osoi_ice[cn] / (DENICE * dz));
+        const double eff = watsat - vol_ice;
+        col.eff_porosity[cs] = eff;
+        const double liqvol = fmin90(eff, col.h2osoi_liq[cn] / (dz * DENH2O));
+        S.h2osoi_liqvol[cs] = liqvol;
+        SoilTerms t{__longlong_as_double(kNoUptake), 0.0};
+        if (!(liqvol <= 0.0 || col.t_soisno[cn] <= TFRZ - 2.0)) {
+            const double s_node = fmax90(liqvol / eff, 0.01);
+            t.smp = -col.sucsat[cs] * cm_pow(s_node, -col.bsw[cs]);
+            t.ratio = eff / watsat;
+        }
+        terms[r * MAX_NLEVGRND + j - 1] = t;
+    }
+    __syncthreads();
+    return c >= 0 ? __popcll(leaders & ((2ull << lane) - 1ull)) - 1 : -1;
+}
+
 template <bool PHS, class PRM, class COL, class CAN, class WRK, class PHA>
 __device__ __forceinline__ bool canopy_patch(const CView &V, const CSwitch &sw, const PRM &P,
                              const double *__restrict__ pft, const COL &col, const CAN &S,
                              int i0, const int32_t *__restrict__ filter, double dtime, ErrWord *err, bool first,
-                             int budget, const WRK &W, const PHA &H)
+                             int budget, const WRK &W, const PHA &H, const SoilTerms *soil, double *rr_lane)
 {
     const int p = filter ? filter[i0] : V.begp + i0;
     const long pp = V.p1(p);
@@ -803,34 +867,29 @@ __device__ __forceinline__ bool canopy_patch(const CView &V, const CSwitch &sw, 
     if (!first) {
         btran = S.btran[pp];
     } else {
-        // rootr holds rootfr * rresis until btran is known, then the
-        // normalised value (SoilMoistStressMod.F90:480-500 writes it the same
-        // way); a run-time level loop keeps the per-level offsets out of the
-        // kernel's uniform registers
+        // rootfr * rresis waits in the lane's LDS column (rr_lane, stride 64)
+        // until btran is known; rootr is then written once, normalised
+        // (SoilMoistStressMod.F90:480-500).  A run-time level loop keeps the
+        // per-level offsets out of the kernel's uniform registers.  The
+        // column's terms (effective porosity, liquid water, the matric
+        // potential) are the wave's, in soil[] (soil_terms_of_wave).
         const double smpsc = PF(F_SMPSC), smpso = PF(F_SMPSO);
         for (int j = 1; j <= V.nlevgrnd; ++j) {
-            const long cs = V.soi(c, j), cn = V.sno(c, j);
-            const double watsat = col.watsat[cs], dz = col.dz[cn];
-            const double vol_ice = fmin90(watsat, col.h2osoi_ice[cn] / (DENICE * dz));
-            const double eff = watsat - vol_ice;
-            col.eff_porosity[cs] = eff;
-            const double liqvol = fmin90(eff, col.h2osoi_liq[cn] / (dz * DENH2O));
-            S.h2osoi_liqvol[cs] = liqvol;
+            const SoilTerms t = soil[j - 1];
             double rr = 0.0;
-            if (!(liqvol <= 0.0 || col.t_soisno[cn] <= TFRZ - 2.0)) {
-                const double s_node = fmax90(liqvol / eff, 0.01);
-                double smp_node = -col.sucsat[cs] * cm_pow(s_node, -col.bsw[cs]);
+            if (__double_as_longlong(t.smp) != kNoUptake) {
+                double smp_node = t.smp;
                 smp_node = fmax90(smpsc, smp_node);
-                const double rres = fmin90((eff / watsat) * (smp_node - smpsc) / (smpso - smpsc), 1.0);
+                const double rres = fmin90(t.ratio * (smp_node - smpsc) / (smpso - smpsc), 1.0);
                 S.rresis[V.soip(p, j)] = rres;
                 rr = S.rootfr[V.soip(p, j)] * rres;
                 btran = btran + fmax90(rr, 0.0);
             }
-            S.rootr[V.soip(p, j)] = rr;
+            rr_lane[(j - 1) * 64] = rr;
         }
         for (int j = 1; j <= V.nlevgrnd; ++j) {
             const long sp = V.soip(p, j);
-            S.rootr[sp] = (btran > btran0) ? S.rootr[sp] / btran : 0.0;
+            S.rootr[sp] = (btran > btran0) ? rr_lane[(j - 1) * 64] / btran : 0.0;
         }
         S.btran[pp] = btran;
     }
@@ -1437,25 +1496,39 @@ k_canopy_round(typename std::conditional<PHS, RoundArgsPhs, RoundArgs>::type arg
     const int round = Ap->round;
     const int count = (round == 0) ? Ap->n : *Ap->cnt_in;
     const int lane = threadIdx.x;
+    // round 0 runs the step's set-up; it is always launched with STRIDE false
+    // (the instance that holds the LDS of the soil terms)
+    const bool first = round == 0;
     for (int base = blockIdx.x * 64; base < count; base += gridDim.x * 64) {
         const KARG RoundArgs *A = Ap;
         if (STRIDE) asm volatile("" : "+s"(A));
         const int idx = base + lane;
         bool cont = false;
-        int i0 = 0;
+        int i0 = (idx < count) ? ((round == 0) ? idx : A->list_in[idx]) : 0;
+        const CView V{A->V.begc, A->V.ldc, A->V.begp, A->V.ldp, A->V.begg, A->V.ldg, A->V.nlevsno,
+                      A->V.nlevsoi, A->V.nlevgrnd};
+        const SoilTerms *soil = nullptr;
+        double *rr_lane = nullptr;
+        if constexpr (!STRIDE) {
+            if (first) {
+                __shared__ SoilTerms terms[64 * MAX_NLEVGRND];
+                __shared__ double rr[MAX_NLEVGRND * 64];
+                __shared__ int run_col[64];
+                const int run = soil_terms_of_wave(V, A->col, A->S, A->filter, i0, idx < count, terms, run_col);
+                soil = terms + (run < 0 ? 0 : run) * MAX_NLEVGRND;
+                rr_lane = rr + lane;
+            }
+        }
         if (idx < count) {
-            i0 = (round == 0) ? idx : A->list_in[idx];
-            const CView V{A->V.begc, A->V.ldc, A->V.begp, A->V.ldp, A->V.begg, A->V.ldg, A->V.nlevsno,
-                          A->V.nlevsoi, A->V.nlevgrnd};
             const CSwitch sw{A->sw.z0param, A->sw.stomatal, A->sw.soil_resis, A->sw.biomass, A->sw.undercanopy,
                              A->sw.light_inhibit, A->sw.itmax};
             if constexpr (PHS) {
                 const KARG RoundArgsPhs *B = (const KARG RoundArgsPhs *)A;
                 cont = canopy_patch<true>(V, sw, A->P, A->pft, A->col, A->S, i0, A->filter, A->dtime, A->err,
-                                          round == 0, A->budget, A->W, B->phs);
+                                          first, A->budget, A->W, B->phs, soil, rr_lane);
             } else {
                 cont = canopy_patch<false>(V, sw, A->P, A->pft, A->col, A->S, i0, A->filter, A->dtime, A->err,
-                                           round == 0, A->budget, A->W, *A);
+                                           first, A->budget, A->W, *A, soil, rr_lane);
             }
         }
         const unsigned long long mask = __ballot(cont);
