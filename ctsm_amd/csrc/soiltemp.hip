@@ -29,7 +29,11 @@ namespace ctsm {
 constexpr int UFIELDS = 6;
 // Per-level saved values for sweep 3: fn (old flux), tk.
 constexpr int LFIELDS = 2;
-constexpr int WS_DOUBLES_PER_COL = MAX_ROWS * UFIELDS + (MAX_NLEVSNO + MAX_NLEVGRND + 1) * LFIELDS;
+constexpr int LEVELS = MAX_NLEVSNO + MAX_NLEVGRND + 1;
+// The column's absorbed solar radiation of snow levels ltop..0 and of level 1
+// (the patch sum of sabg_lyr * wtcol), index j + nlevsno - 1.
+constexpr int SLEVELS = MAX_NLEVSNO + 1;
+constexpr int WS_DOUBLES_PER_COL = MAX_ROWS * UFIELDS + LEVELS * LFIELDS + SLEVELS;
 
 struct TempWork {
     double *ws;     // workspace base
@@ -38,7 +42,152 @@ struct TempWork {
     __device__ __forceinline__ double &Lv(int lane, int lev_ix, int f) const {
         return ws[((long)MAX_ROWS * UFIELDS + (long)lev_ix * LFIELDS + f) * ld + lane];
     }
+    __device__ __forceinline__ double &Sab(int lane, int jj) const {
+        return ws[((long)MAX_ROWS * UFIELDS + (long)LEVELS * LFIELDS + jj) * ld + lane];
+    }
 };
+
+// ---- ComputeGroundHeatFluxAndDeriv (SoilTemperatureMod.F90:1543-1796) ----
+// The patch -> column part: per patch the net ground heat flux and its
+// derivative (eflx_gnet, dgnetdT, sabg_chk written), summed into the column
+// in patch order (hs_soil, hs_h2osfc, dhsdT, then hs_top_snow, and the
+// absorbed solar radiation of each snow level).  With one column per lane the
+// lanes would walk their patches in step, each load touching a cache line per
+// lane; instead the 64 lanes of a wave take the wave's patches in order, 64 at
+// a time (coalesced), and each column's lane then adds its patches' terms from
+// LDS in patch order: the same products summed in the same order, so the same
+// bits as the per-column loops.
+struct GroundFlux {
+    double hs_soil, hs_h2osfc, dhsdT, hs_top_snow;
+};
+
+// LDS of one wave: the columns' first patch, offset in the wave's patch list,
+// top level and constants (emg, forc_lwrad, htvp, frac_sno and the emitted
+// longwave terms), and the terms of the 64 patches in flight.
+struct GfWave {
+    int pi[64], off[64], ltop[64];
+    double cc[9][64];
+    double x[4][64];
+    double y[SLEVELS][64];
+};
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Every lane of the wave calls it (valid: the lane has a column in range);
+// the sums of a lane whose column soiltemp_column rejects are not used.  The
+// snow levels' sums go to wk.Sab(i, .).
+__device__ void ground_flux_of_wave(const View &V, const ctsm_col_state &S, const ctsm_patch_state &Pt, int c,
+                                    bool valid, GfWave &L, const TempWork &wk, int i, GroundFlux &g)
+{
+    const int lane = threadIdx.x & 63;
+    const int NS = V.nlevsno;
+    int n_own = 0, pi = 0, ltop = 1;
+    if (valid) {
+        const long cc = V.c1(c);
+        const int snl = S.snl[cc];
+        const int p0 = S.patchi[cc], p1 = S.patchf[cc];
+        if (!(snl < -NS || snl > 0) && !(p0 < V.begp || p1 >= V.begp + V.ldp)) {
+            ltop = snl + 1;
+            pi = p0;
+            n_own = (p1 >= p0) ? p1 - p0 + 1 : 0;
+            const double emg = S.emg[cc];
+            const double tg = S.t_grnd[cc], ts = S.t_soisno[V.sno(c, ltop)];
+            const double t1 = S.t_soisno[V.sno(c, 1)], th = S.t_h2osfc[cc];
+            const double tg2 = tg * tg, ts2 = ts * ts, t12 = t1 * t1, th2 = th * th;
+            L.cc[0][lane] = emg;
+            L.cc[1][lane] = S.forc_lwrad[cc];
+            L.cc[2][lane] = S.htvp[cc];
+            L.cc[3][lane] = S.frac_sno_eff[cc];
+            L.cc[4][lane] = emg * SB * (tg2 * tg2);          // lwrad_emit
+            L.cc[5][lane] = 4.0 * emg * SB * (tg2 * tg);     // dlwrad_emit
+            L.cc[6][lane] = emg * SB * (ts2 * ts2);          // lwrad_emit_snow
+            L.cc[7][lane] = emg * SB * (t12 * t12);          // lwrad_emit_soil
+            L.cc[8][lane] = emg * SB * (th2 * th2);          // lwrad_emit_h2osfc
+        }
+    }
+    int incl = n_own;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(incl, d);
+        if (lane >= d) incl += t;
+    }
+    const int off = incl - n_own;
+    const int total = __shfl(incl, 63);
+    L.pi[lane] = pi;
+    L.off[lane] = off;
+    L.ltop[lane] = ltop;
+    double dhsdT = 0.0, hs_soil = 0.0, hs_h2osfc = 0.0, hs_top_snow = 0.0;
+    double sab[SLEVELS];
+#pragma unroll
+    for (int jj = 0; jj < SLEVELS; ++jj) sab[jj] = 0.0;
+    const int jj_top = ltop + NS - 1;
+    wave_sync();
+    for (int k0 = 0; k0 < total; k0 += 64) {
+        const int k = k0 + lane;
+        if (k < total) {
+            int lo = 0, hi = 63;       // the column: the last lane whose list starts at or before k
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (L.off[mid] <= k) lo = mid;
+                else hi = mid - 1;
+            }
+            const int o = lo;
+            const int p = L.pi[o] + (k - L.off[o]);
+            const long pp = V.p1(p);
+            const double emg = L.cc[0][o], forc_lwrad = L.cc[1][o], htvp = L.cc[2][o], frac_sno = L.cc[3][o];
+            const double fveg = (double)Pt.frac_veg_nosno[pp], wt = Pt.wtcol[pp];
+            const double lwin = (1.0 - fveg) * emg * forc_lwrad;
+            const double dl = Pt.dlrad[pp];
+            const double eg = Pt.sabg[pp] + dl + lwin - L.cc[4][o] -
+                              (Pt.eflx_sh_grnd[pp] + Pt.qflx_evap_soi[pp] * htvp);
+            Pt.eflx_gnet[pp] = eg;
+            Pt.sabg_chk[pp] = frac_sno * Pt.sabg_snow[pp] + (1.0 - frac_sno) * Pt.sabg_soil[pp];
+            const double egsoil = Pt.sabg_soil[pp] + dl + lwin - L.cc[7][o] -
+                                  (Pt.eflx_sh_soil[pp] + Pt.qflx_ev_soil[pp] * htvp);
+            const double egh2o = Pt.sabg_soil[pp] + dl + lwin - L.cc[8][o] -
+                                 (Pt.eflx_sh_h2osfc[pp] + Pt.qflx_ev_h2osfc[pp] * htvp);
+            const double dg = -Pt.cgrnd[pp] - L.cc[5][o];
+            Pt.dgnetdT[pp] = dg;
+            const int lt = L.ltop[o];
+            const double sl = Pt.sabg_lyr[V.lyr(p, lt)];
+            const double egsnow = sl + dl + lwin - L.cc[6][o] - (Pt.eflx_sh_snow[pp] + Pt.qflx_ev_snow[pp] * htvp);
+            L.x[0][lane] = dg * wt;
+            L.x[1][lane] = egsoil * wt;
+            L.x[2][lane] = egh2o * wt;
+            L.x[3][lane] = egsnow * wt;
+            for (int j = lt; j <= 1; ++j) L.y[j + NS - 1][lane] = Pt.sabg_lyr[V.lyr(p, j)] * wt;
+        }
+        wave_sync();
+        if (n_own > 0) {
+            const int a = (off > k0 ? off : k0) - k0;
+            const int e = off + n_own, b = (e < k0 + 64 ? e : k0 + 64) - k0;
+            for (int s = a; s < b; ++s) {
+                dhsdT = dhsdT + L.x[0][s];
+                hs_soil = hs_soil + L.x[1][s];
+                hs_h2osfc = hs_h2osfc + L.x[2][s];
+                hs_top_snow = hs_top_snow + L.x[3][s];
+#pragma unroll
+                for (int jj = 0; jj < SLEVELS; ++jj)
+                    if (jj >= jj_top && jj <= NS) sab[jj] = sab[jj] + L.y[jj][s];
+            }
+        }
+        wave_sync();
+    }
+    g.hs_soil = hs_soil;
+    g.hs_h2osfc = hs_h2osfc;
+    g.dhsdT = dhsdT;
+    g.hs_top_snow = hs_top_snow;
+    if (valid) {
+#pragma unroll
+        for (int jj = 0; jj < SLEVELS; ++jj)
+            if (jj >= jj_top && jj <= NS) wk.Sab(i, jj) = sab[jj];
+    }
+}
 
 // The 3-row elimination window: W[a][m] = A(k+a, k+m), m = 0..4; B[a] = rhs.
 struct Window {
@@ -109,7 +258,7 @@ __device__ __forceinline__ bool eliminate(Window &w, int km)
 // Full column.  `lane` indexes the workspace.  Returns 0 or an error code.
 __device__ int soiltemp_column(const View &V, const Params &P, const ctsm_col_state &S,
                                const ctsm_patch_state &Pt, int c, double dtime,
-                               const TempWork &wk, int lane)
+                               const TempWork &wk, int lane, const GroundFlux &gf)
 {
     const int nlevgrnd = V.nlevgrnd;
     const int snl = S.snl[V.c1(c)], nbedrock = S.nbedrock[V.c1(c)];
@@ -121,52 +270,12 @@ __device__ int soiltemp_column(const View &V, const Params &P, const ctsm_col_st
     double t_h2osfc = S.t_h2osfc[V.c1(c)];
     const int ltop = snl + 1;     // top level
 
-    // ---- ComputeGroundHeatFluxAndDeriv
-    double hs_soil = 0.0, hs_h2osfc = 0.0, dhsdT = 0.0, hs_top_snow = 0.0;
+    // ---- ComputeGroundHeatFluxAndDeriv: summed over the wave (ground_flux_of_wave)
+    const double hs_soil = gf.hs_soil, hs_h2osfc = gf.hs_h2osfc, dhsdT = gf.dhsdT, hs_top_snow = gf.hs_top_snow;
     const int pi = S.patchi[V.c1(c)], pf = S.patchf[V.c1(c)];
     if (pi < V.begp || pf >= V.begp + V.ldp) return 1;
-    {
-        const double tg = S.t_grnd[V.c1(c)], ts = S.t_soisno[V.sno(c, ltop)];
-        const double t1 = S.t_soisno[V.sno(c, 1)], th = t_h2osfc;
-        const double tg2 = tg * tg, ts2 = ts * ts, t12 = t1 * t1, th2 = th * th;
-        const double lwrad_emit = emg * SB * (tg2 * tg2);
-        const double dlwrad_emit = 4.0 * emg * SB * (tg2 * tg);
-        const double lwrad_emit_snow = emg * SB * (ts2 * ts2);
-        const double lwrad_emit_soil = emg * SB * (t12 * t12);
-        const double lwrad_emit_h2osfc = emg * SB * (th2 * th2);
-        for (int p = pi; p <= pf; ++p) {
-            const double fveg = (double)Pt.frac_veg_nosno[V.p1(p)], wt = Pt.wtcol[V.p1(p)];
-            const double lwin = (1.0 - fveg) * emg * forc_lwrad;
-            const double dl = Pt.dlrad[V.p1(p)];
-            const double eg = Pt.sabg[V.p1(p)] + dl + lwin - lwrad_emit -
-                              (Pt.eflx_sh_grnd[V.p1(p)] + Pt.qflx_evap_soi[V.p1(p)] * htvp);
-            Pt.eflx_gnet[V.p1(p)] = eg;
-            Pt.sabg_chk[V.p1(p)] = frac_sno * Pt.sabg_snow[V.p1(p)] + (1.0 - frac_sno) * Pt.sabg_soil[V.p1(p)];
-            const double egsoil = Pt.sabg_soil[V.p1(p)] + dl + lwin - lwrad_emit_soil -
-                                  (Pt.eflx_sh_soil[V.p1(p)] + Pt.qflx_ev_soil[V.p1(p)] * htvp);
-            const double egh2o = Pt.sabg_soil[V.p1(p)] + dl + lwin - lwrad_emit_h2osfc -
-                                 (Pt.eflx_sh_h2osfc[V.p1(p)] + Pt.qflx_ev_h2osfc[V.p1(p)] * htvp);
-            const double dg = -Pt.cgrnd[V.p1(p)] - dlwrad_emit;
-            Pt.dgnetdT[V.p1(p)] = dg;
-            dhsdT = dhsdT + dg * wt;
-            hs_soil = hs_soil + egsoil * wt;
-            hs_h2osfc = hs_h2osfc + egh2o * wt;
-        }
-        for (int p = pi; p <= pf; ++p) {
-            const double fveg = (double)Pt.frac_veg_nosno[V.p1(p)], wt = Pt.wtcol[V.p1(p)];
-            const double lwin = (1.0 - fveg) * emg * forc_lwrad;
-            const double sl = Pt.sabg_lyr[V.lyr(p, ltop)];
-            const double egsnow = sl + Pt.dlrad[V.p1(p)] + lwin - lwrad_emit_snow -
-                                  (Pt.eflx_sh_snow[V.p1(p)] + Pt.qflx_ev_snow[V.p1(p)] * htvp);
-            hs_top_snow = hs_top_snow + egsnow * wt;
-        }
-    }
     // column absorbed solar of level j in [ltop, 1], summed over patches in order
-    auto sabg_col = [&](int j) -> double {
-        double s = 0.0;
-        for (int p = pi; p <= pf; ++p) s = s + Pt.sabg_lyr[V.lyr(p, j)] * Pt.wtcol[V.p1(p)];
-        return s;
-    };
+    auto sabg_col = [&](int j) -> double { return wk.Sab(lane, j + V.nlevsno - 1); };
 
     // ---- standing water properties (:257-266, :816-821)
     double c_h2osfc, dz_h2osfc;
@@ -598,15 +707,34 @@ __device__ int soiltemp_column(const View &V, const Params &P, const ctsm_col_st
     return 0;
 }
 
+// The lane's column (filter entry i) and its ground heat flux sums; every
+// lane of the block takes part (the sums are the wave's work).  Returns false
+// for a lane with nothing to solve (past the filter, or a column out of range:
+// reported).
+template <int BLOCK>
+__device__ __forceinline__ bool column_and_ground_flux(const View &V, const ctsm_col_state &S,
+                                                       const ctsm_patch_state &Pt, int n,
+                                                       const int32_t *__restrict__ filter, const TempWork &wk,
+                                                       ErrWord *err, int i, int &c, GroundFlux &gf)
+{
+    __shared__ GfWave gfl[BLOCK / 64];
+    const bool active = i < n;
+    c = active ? (filter ? filter[i] : V.begc + i) : V.begc;
+    const bool in_range = active && col_in_range(V, c);
+    ground_flux_of_wave(V, S, Pt, c, in_range, gfl[threadIdx.x >> 6], wk, i, gf);
+    if (active && !in_range) report_error(err, c, CTSM_E_ARG);
+    return in_range;
+}
+
 __global__ void __launch_bounds__(64)
 k_soiltemp(View V, Params P, ctsm_col_state S, ctsm_patch_state Pt, int n,
            const int32_t *__restrict__ filter, double dtime, TempWork wk, ErrWord *err)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int c = filter ? filter[i] : V.begc + i;
-    if (!col_in_range(V, c)) { report_error(err, c, CTSM_E_ARG); return; }
-    const int rc = soiltemp_column(V, P, S, Pt, c, dtime, wk, i);
+    int c;
+    GroundFlux gf;
+    if (!column_and_ground_flux<64>(V, S, Pt, n, filter, wk, err, i, c, gf)) return;
+    const int rc = soiltemp_column(V, P, S, Pt, c, dtime, wk, i, gf);
     if (rc) report_error(err, c, CTSM_E_SOLVER);
 }
 
@@ -619,10 +747,10 @@ k_soiltemp_fractions(View V, Params P, ctsm_col_state S, ctsm_patch_state Pt, in
                      const int32_t *__restrict__ filter, double dtime, TempWork wk, ErrWord *err)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int c = filter ? filter[i] : V.begc + i;
-    if (!col_in_range(V, c)) { report_error(err, c, CTSM_E_ARG); return; }
-    int rc = soiltemp_column(V, P, S, Pt, c, dtime, wk, i);
+    int c;
+    GroundFlux gf;
+    if (!column_and_ground_flux<128>(V, S, Pt, n, filter, wk, err, i, c, gf)) return;
+    int rc = soiltemp_column(V, P, S, Pt, c, dtime, wk, i, gf);
     if (rc == 0) rc = soilwater_fractions_column(V, S, c);
     if (rc) report_error(err, c, CTSM_E_SOLVER);
 }
@@ -633,10 +761,10 @@ k_soil_step(View V, Params P, ctsm_col_state S, ctsm_patch_state Pt, int n,
             const int32_t *__restrict__ filter, double dtime, TempWork wk, ErrWord *err)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int c = filter ? filter[i] : V.begc + i;
-    if (!col_in_range(V, c)) { report_error(err, c, CTSM_E_ARG); return; }
-    int rc = soiltemp_column(V, P, S, Pt, c, dtime, wk, i);
+    int c;
+    GroundFlux gf;
+    if (!column_and_ground_flux<64>(V, S, Pt, n, filter, wk, err, i, c, gf)) return;
+    int rc = soiltemp_column(V, P, S, Pt, c, dtime, wk, i, gf);
     if (rc == 0) rc = soilwater_fractions_column(V, S, c);
     if (rc == 0) rc = soilwater_column(V, P, S, c, dtime);
     if (rc) report_error(err, c, CTSM_E_SOLVER);
