@@ -19,10 +19,11 @@ the fastest on the context.
 import numpy as np
 
 DEFAULT_CANDIDATES = [
+    dict(richards_lanes=8, richards_block=128, soiltemp_block=64, fused_single_kernel=0, step_streams=4),
     dict(richards_lanes=8, richards_block=128, soiltemp_block=128, fused_single_kernel=0, step_streams=4),
-    dict(richards_lanes=0, richards_block=128, soiltemp_block=128, fused_single_kernel=0, step_streams=4),
-    dict(richards_lanes=8, richards_block=128, soiltemp_block=128, fused_single_kernel=0, step_streams=2),
-    dict(richards_lanes=8, richards_block=128, soiltemp_block=128, fused_single_kernel=0, step_streams=1),
+    dict(richards_lanes=0, richards_block=128, soiltemp_block=64, fused_single_kernel=0, step_streams=4),
+    dict(richards_lanes=8, richards_block=128, soiltemp_block=64, fused_single_kernel=0, step_streams=2),
+    dict(richards_lanes=8, richards_block=128, soiltemp_block=64, fused_single_kernel=0, step_streams=1),
     dict(richards_lanes=8, richards_block=256, soiltemp_block=64, fused_single_kernel=0, step_streams=4),
     dict(richards_lanes=4, richards_block=128, soiltemp_block=128, fused_single_kernel=0, step_streams=4),
     dict(richards_lanes=2, richards_block=128, soiltemp_block=128, fused_single_kernel=0, step_streams=4),
@@ -168,14 +169,14 @@ def autotune_canopy(ctx, ds, filter_dev=None, n=None, reps=2, candidates=None, w
 # Launch shapes of the soil solves inside a full land step (one stream, so
 # step_streams does not apply): lanes per column and block sizes.
 LAND_CANDIDATES = [
-    dict(richards_lanes=0, richards_block=128, soiltemp_block=128),
     dict(richards_lanes=0, richards_block=128, soiltemp_block=64),
+    dict(richards_lanes=8, richards_block=128, soiltemp_block=64),
     dict(richards_lanes=8, richards_block=128, soiltemp_block=128),
     dict(richards_lanes=8, richards_block=256, soiltemp_block=64),
     dict(richards_lanes=8, richards_block=64, soiltemp_block=64),
-    dict(richards_lanes=4, richards_block=128, soiltemp_block=128),
+    dict(richards_lanes=4, richards_block=128, soiltemp_block=64),
     dict(richards_lanes=4, richards_block=256, soiltemp_block=64),
-    dict(richards_lanes=2, richards_block=128, soiltemp_block=128),
+    dict(richards_lanes=2, richards_block=128, soiltemp_block=64),
     dict(richards_lanes=1, richards_block=64, soiltemp_block=64),
 ]
 
