@@ -746,7 +746,7 @@ struct SoilTerms {
 };
 
 // Returns the index of the lane's run (-1 for a lane without a valid patch);
-// every lane of the wave calls it (it synchronises the block, one wave).
+// every lane of the wave calls it.
 template <class COL, class CAN>
 __device__ int soil_terms_of_wave(const CView &V, const COL &col, const CAN &S, const int32_t *__restrict__ filter,
                                   int i0, bool active, SoilTerms *terms, int *run_col)
@@ -765,7 +765,7 @@ __device__ int soil_terms_of_wave(const CView &V, const COL &col, const CAN &S, 
     const bool lead = c >= 0 && (lane == 0 || prev != c);
     const unsigned long long leaders = __ballot(lead);
     if (lead) run_col[__popcll(leaders & ((1ull << lane) - 1ull))] = c;
-    __syncthreads();
+    wave_sync();
     const int nl = V.nlevgrnd;
     const int items = __popcll(leaders) * nl;
     for (int it = lane; it < items; it += 64) {
@@ -786,7 +786,7 @@ __device__ int soil_terms_of_wave(const CView &V, const COL &col, const CAN &S, 
         }
         terms[r * MAX_NLEVGRND + j - 1] = t;
     }
-    __syncthreads();
+    wave_sync();
     return c >= 0 ? __popcll(leaders & ((2ull << lane) - 1ull)) - 1 : -1;
 }
 

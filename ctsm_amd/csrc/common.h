@@ -83,4 +83,14 @@ __device__ inline void report_error(ErrWord *e, int c, int code)
     }
 }
 
+// Orders the LDS accesses of one wavefront's lanes around it (the lanes
+// exchange values through LDS; a wavefront's LDS operations complete in
+// issue order).  No workgroup barrier: the exchanges here stay inside a wave.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 }  // namespace ctsm

@@ -71,13 +71,6 @@ struct GfWave {
     double y[SLEVELS][64];
 };
 
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
-
 // Every lane of the wave calls it (valid: the lane has a column in range);
 // the sums of a lane whose column soiltemp_column rejects are not used.  The
 // snow levels' sums go to wk.Sab(i, .).
