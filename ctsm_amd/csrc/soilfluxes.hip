@@ -167,24 +167,74 @@ __global__ void __launch_bounds__(256) k_soilfluxes(SView V, ctsm_col_state S, c
     F.eflx_lwrad_out_r[pp] = lw_out;
 }
 
-// :574-576, p2c_1d_filter (subgridAveMod.F90:312-318)
+// :574-576, p2c_1d_filter (subgridAveMod.F90:312-318): a column per lane.
+// The lanes of a wave take the wave's patches 64 at a time (coalesced loads),
+// and each column's lane adds its patches' weighted errors from LDS in patch
+// order (inactive patches skipped), the same sum as a loop over the column's
+// patches.
 __global__ void __launch_bounds__(256) k_errsoi_p2c(SView V, ctsm_col_state S, ctsm_patch_state P,
                                                     ctsm_soilflux_state F, int n, const int32_t *__restrict__ filter,
                                                     ErrWord *err)
 {
+    __shared__ int s_pi[4][64], s_off[4][64];
+    __shared__ double s_x[4][64];
+    __shared__ int s_on[4][64];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int c = filter ? filter[i] : V.begc + i;
-    if (c < V.begc || c >= V.begc + V.ldc) { report_error(err, c, CTSM_E_ARG); return; }
-    const long cc = V.c1(c);
-    const int pi = S.patchi[cc], pf = S.patchf[cc];
-    if (pf >= pi && (pi < V.begp || pf >= V.begp + V.ldp)) { report_error(err, c, CTSM_E_ARG); return; }
-    double s = 0.0;
-    for (int p = pi; p <= pf; ++p) {
-        const long pp = V.p1(p);
-        if (F.active[pp]) s = s + F.errsoi_patch[pp] * P.wtcol[pp];
+    int c = V.begc, n_own = 0, pi = 0;
+    bool ok = false;
+    if (i < n) {
+        c = filter ? filter[i] : V.begc + i;
+        if (c < V.begc || c >= V.begc + V.ldc) {
+            report_error(err, c, CTSM_E_ARG);
+        } else {
+            const long cc = V.c1(c);
+            const int p0 = S.patchi[cc], p1 = S.patchf[cc];
+            if (p1 >= p0 && (p0 < V.begp || p1 >= V.begp + V.ldp)) {
+                report_error(err, c, CTSM_E_ARG);
+            } else {
+                ok = true;
+                pi = p0;
+                n_own = p1 >= p0 ? p1 - p0 + 1 : 0;
+            }
+        }
     }
-    F.errsoi_col[cc] = s;
+    int incl = n_own;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(incl, d);
+        if (lane >= d) incl += t;
+    }
+    const int off = incl - n_own;
+    const int total = __shfl(incl, 63);
+    s_pi[w][lane] = pi;
+    s_off[w][lane] = off;
+    double s = 0.0;
+    wave_sync();
+    for (int k0 = 0; k0 < total; k0 += 64) {
+        const int k = k0 + lane;
+        if (k < total) {
+            int lo = 0, hi = 63;       // the column: the last lane whose list starts at or before k
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_off[w][mid] <= k) lo = mid;
+                else hi = mid - 1;
+            }
+            const long pp = V.p1(s_pi[w][lo] + (k - s_off[w][lo]));
+            const int on = F.active[pp];
+            s_on[w][lane] = on;
+            s_x[w][lane] = on ? F.errsoi_patch[pp] * P.wtcol[pp] : 0.0;
+        }
+        wave_sync();
+        if (n_own > 0) {
+            const int a = (off > k0 ? off : k0) - k0;
+            const int e = off + n_own, b = (e < k0 + 64 ? e : k0 + 64) - k0;
+            for (int t = a; t < b; ++t)
+                if (s_on[w][t]) s = s + s_x[w][t];
+        }
+        wave_sync();
+    }
+    if (ok) F.errsoi_col[V.c1(c)] = s;
 }
 
 // Balance reduction word: [0] bits of the max |errsoi_col|, [1] its column.
