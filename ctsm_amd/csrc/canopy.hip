@@ -143,17 +143,33 @@ __device__ __forceinline__ double stab2(double zeta)
 }
 
 // the temperature / humidity profile block repeated in FrictionVelocityMod.F90:916-1060;
-// log_z = log(zldis / z0), which does not change over the canopy iteration
+// log_z = log(zldis / z0), which does not change over the canopy iteration.
+// The lanes of a wave sit in different stability regimes; each elementary
+// function a regime needs is evaluated once for all the lanes that need it
+// (the very unstable and the very stable regime share the logarithm of
+// their height ratio, the two unstable ones stab2(z0 / obu)) rather than once
+// per regime branch.  Every lane computes its own regime's expression with
+// the same operations in the same order.
 __device__ double temp_profile(double zldis, double z0, double obu, double log_z)
 {
     const double zetat = 0.465;
     const double zeta = zldis / obu;
-    if (zeta < -zetat)
-        return VKC / (cm_log(-zetat * obu / z0) - stab2(-zetat) + stab2(z0 / obu)
-                      + 0.8 * (cm_pow(zetat, -0.333) - cm_pow(-zeta, -0.333)));
-    if (zeta < 0.0) return VKC / (log_z - stab2(zeta) + stab2(z0 / obu));
-    if (zeta <= 1.0) return VKC / (log_z + 5.0 * zeta - 5.0 * z0 / obu);
-    return VKC / (cm_log(obu / z0) + 5.0 - 5.0 * z0 / obu + (5.0 * cm_log(zeta) + zeta - 1.0));
+    const bool ra = zeta < -zetat, rb = !ra && zeta < 0.0, rc = !ra && !rb && zeta <= 1.0, rd = !(ra || rb || rc);
+    double lg = 0.0, sa = 0.0, sb = 0.0, pw = 0.0, lz = 0.0;
+    if (ra || rd) lg = cm_log(ra ? -zetat * obu / z0 : obu / z0);
+    if (ra || rb) sb = stab2(z0 / obu);
+    if (ra) {
+        sa = stab2(-zetat);
+        pw = cm_pow(zetat, -0.333) - cm_pow(-zeta, -0.333);
+    }
+    if (rb) sa = stab2(zeta);
+    if (rd) lz = cm_log(zeta);
+    double den;
+    if (ra) den = lg - sa + sb + 0.8 * pw;
+    else if (rb) den = log_z - sa + sb;
+    else if (rc) den = log_z + 5.0 * zeta - 5.0 * z0 / obu;
+    else den = lg + 5.0 - 5.0 * z0 / obu + (5.0 * lz + zeta - 1.0);
+    return VKC / den;
 }
 
 __device__ __forceinline__ double fm_profile(double z)
@@ -184,15 +200,24 @@ __device__ __forceinline__ void friction_velocity(double hu, double ht, double h
     const double zetam = 1.574;
     const double zldis = hu - displa;
     const double zeta = zldis / obu;
-    if (zeta < -zetam)
-        o.ustar = VKC * um / (cm_log(-zetam * obu / z0m) - stab1(-zetam) + stab1(z0m / obu)
-                              + 1.14 * (cm_pow(-zeta, 0.333) - cm_pow(zetam, 0.333)));
-    else if (zeta < 0.0)
-        o.ustar = VKC * um / (L.lm - stab1(zeta) + stab1(z0m / obu));
-    else if (zeta <= 1.0)
-        o.ustar = VKC * um / (L.lm + 5.0 * zeta - 5.0 * z0m / obu);
-    else
-        o.ustar = VKC * um / (cm_log(obu / z0m) + 5.0 - 5.0 * z0m / obu + (5.0 * cm_log(zeta) + zeta - 1.0));
+    // each elementary function once for the lanes whose regime needs it (as
+    // in temp_profile)
+    const bool ra = zeta < -zetam, rb = !ra && zeta < 0.0, rc = !ra && !rb && zeta <= 1.0, rd = !(ra || rb || rc);
+    double lg = 0.0, sa = 0.0, sb = 0.0, pw = 0.0, lz = 0.0;
+    if (ra || rd) lg = cm_log(ra ? -zetam * obu / z0m : obu / z0m);
+    if (ra || rb) sb = stab1(z0m / obu);
+    if (ra) {
+        sa = stab1(-zetam);
+        pw = cm_pow(-zeta, 0.333) - cm_pow(zetam, 0.333);
+    }
+    if (rb) sa = stab1(zeta);
+    if (rd) lz = cm_log(zeta);
+    double den;
+    if (ra) den = lg - sa + sb + 1.14 * pw;
+    else if (rb) den = L.lm - sa + sb;
+    else if (rc) den = L.lm + 5.0 * zeta - 5.0 * z0m / obu;
+    else den = lg + 5.0 - 5.0 * z0m / obu + (5.0 * lz + zeta - 1.0);
+    o.ustar = VKC * um / den;
     o.temp1 = temp_profile(ht - displa, z0h, obu, L.lh);
     if (hq == ht && z0q == z0h) o.temp2 = o.temp1;
     else o.temp2 = temp_profile(hq - displa, z0q, obu, L.lq);
