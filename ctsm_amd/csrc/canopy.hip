@@ -243,16 +243,25 @@ __device__ void friction_velocity_diag(double hu, double displa, double z0m, dou
         o.u10_clm = um;
     } else {
         const double z10 = 10.0 + z0m;
-        if (zeta < -zetam)
-            o.u10_clm = um - (o.ustar / VKC * (cm_log(-zetam * obu / z10) - stab1(-zetam) + stab1(z10 / obu)
-                                               + 1.14 * (cm_pow(-zeta, 0.333) - cm_pow(zetam, 0.333))));
-        else if (zeta < 0.0)
-            o.u10_clm = um - (o.ustar / VKC * (cm_log(zldis / z10) - stab1(zeta) + stab1(z10 / obu)));
-        else if (zeta <= 1.0)
-            o.u10_clm = um - (o.ustar / VKC * (cm_log(zldis / z10) + 5.0 * zeta - 5.0 * z10 / obu));
-        else
-            o.u10_clm = um - (o.ustar / VKC * (cm_log(obu / z10) + 5.0 - 5.0 * z10 / obu
-                                               + (5.0 * cm_log(zeta) + zeta - 1.0)));
+        // one evaluation of each elementary function for the lanes whose
+        // regime needs it (as in friction_velocity)
+        const bool ra = zeta < -zetam, rb = !ra && zeta < 0.0, rc = !ra && !rb && zeta <= 1.0;
+        const bool rd = !(ra || rb || rc);
+        const double lg = cm_log(ra ? -zetam * obu / z10 : (rd ? obu / z10 : zldis / z10));
+        double sa = 0.0, sb = 0.0, pw = 0.0, lz = 0.0;
+        if (ra || rb) sb = stab1(z10 / obu);
+        if (ra) {
+            sa = stab1(-zetam);
+            pw = cm_pow(-zeta, 0.333) - cm_pow(zetam, 0.333);
+        }
+        if (rb) sa = stab1(zeta);
+        if (rd) lz = cm_log(zeta);
+        double t;
+        if (ra) t = lg - sa + sb + 1.14 * pw;
+        else if (rb) t = lg - sa + sb;
+        else if (rc) t = lg + 5.0 * zeta - 5.0 * z10 / obu;
+        else t = lg + 5.0 - 5.0 * z10 / obu + (5.0 * lz + zeta - 1.0);
+        o.u10_clm = um - (o.ustar / VKC * t);
     }
     o.va = um;
     o.temp12m = temp_profile(2.0 + z0h, z0h, obu, cm_log((2.0 + z0h) / z0h));
@@ -266,13 +275,19 @@ __device__ void friction_velocity_diag(double hu, double displa, double z0m, dou
     o.fv = o.ustar;
 }
 
-__device__ __forceinline__ double ft(double tl, double ha)
+// the exponents of ft and fth (PhotosynthesisMod.F90:2488-2561)
+__device__ __forceinline__ double ft_arg(double tl, double ha)
 {
-    return cm_exp(ha / (RGAS * 1.e-3 * (TFRZ + 25.0)) * (1.0 - (TFRZ + 25.0) / tl));
+    return ha / (RGAS * 1.e-3 * (TFRZ + 25.0)) * (1.0 - (TFRZ + 25.0) / tl);
 }
+__device__ __forceinline__ double fth_arg(double tl, double hd, double se)
+{
+    return (-hd + se * tl) / (RGAS * 1.e-3 * tl);
+}
+__device__ __forceinline__ double ft(double tl, double ha) { return cm_exp(ft_arg(tl, ha)); }
 __device__ __forceinline__ double fth(double tl, double hd, double se, double scaleFactor)
 {
-    return scaleFactor / (1.0 + cm_exp((-hd + se * tl) / (RGAS * 1.e-3 * tl)));
+    return scaleFactor / (1.0 + cm_exp(fth_arg(tl, hd, se)));
 }
 __device__ __forceinline__ double fth25(double hd, double se)
 {
@@ -711,19 +726,24 @@ __device__ void temp_factors(const PRM &P, const PsnConst &K, double tv, bool an
     T.ko = K.ko25 * ft(tv, P.koha);
     T.cp = K.cp25 * ft(tv, P.cpha);
     T.pow2 = cm_pow(2.0, (tv - (TFRZ + 25.0)) / 10.0);
+    // C3 and C4 lanes share a wave: the exponentials of the two pathways'
+    // factors are evaluated once, each lane on its own pathway's argument
+    const double eL = cm_exp(K.c3 ? ft_arg(tv, P.lmrha) : 1.3 * (tv - (TFRZ + 55.0)));
     if (K.c3) {
-        T.ftL = ft(tv, P.lmrha);
+        T.ftL = eL;
         T.fthL = fth(tv, P.lmrhd, P.lmrse, K.lmrc);
     } else {
-        T.dL4 = 1.0 + cm_exp(1.3 * (tv - (TFRZ + 55.0)));
+        T.dL4 = 1.0 + eL;
     }
     if (anyday) {
+        const double eV1 = cm_exp(K.c3 ? ft_arg(tv, P.vcmaxha) : 0.2 * ((TFRZ + 15.0) - tv));
+        const double eV2 = cm_exp(K.c3 ? fth_arg(tv, P.vcmaxhd, K.vcmaxse) : 0.3 * (tv - (TFRZ + 40.0)));
         if (K.c3) {
-            T.ftV = ft(tv, P.vcmaxha);
-            T.fthV = fth(tv, P.vcmaxhd, K.vcmaxse, K.vcmaxc);
+            T.ftV = eV1;
+            T.fthV = K.vcmaxc / (1.0 + eV2);
         } else {
-            T.dV1 = 1.0 + cm_exp(0.2 * ((TFRZ + 15.0) - tv));
-            T.dV2 = 1.0 + cm_exp(0.3 * (tv - (TFRZ + 40.0)));
+            T.dV1 = 1.0 + eV1;
+            T.dV2 = 1.0 + eV2;
         }
         T.ftJ = ft(tv, P.jmaxha);
         T.fthJ = fth(tv, P.jmaxhd, K.jmaxse, K.jmaxc);
