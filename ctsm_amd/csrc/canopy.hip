@@ -307,14 +307,19 @@ struct CiSolve {
 __device__ double ci_func(double ci, CiSolve &s)
 {
     double r1, r2;
+    // the C3 Rubisco rate and the C4 PEP carboxylase rate are the one divide
+    // of each pathway: a wave of both kinds takes it once, each lane on its
+    // own pathway's operands
+    const double q = (s.c3 ? s.vcmax_z * fmax90(ci - s.cp, 0.0) : s.kp_z * fmax90(ci, 0.0))
+                     / (s.c3 ? ci + s.kc * (1.0 + s.oair / s.ko) : s.forc_pbot);
     if (s.c3) {
-        s.ac = s.vcmax_z * fmax90(ci - s.cp, 0.0) / (ci + s.kc * (1.0 + s.oair / s.ko));
+        s.ac = q;
         s.aj = s.je * fmax90(ci - s.cp, 0.0) / (4.0 * ci + 8.0 * s.cp);
         s.ap = 3.0 * s.tpu_z;
     } else {
         s.ac = s.vcmax_z;
         s.aj = s.qe * s.par_z * 4.6;
-        s.ap = s.kp_z * fmax90(ci, 0.0) / s.forc_pbot;
+        s.ap = q;
     }
     if (!quadratic(s.theta_cj, -(s.ac + s.aj), s.ac * s.aj, r1, r2)) s.bad = true;
     const double ai = fmin90(r1, r2);
@@ -1220,11 +1225,15 @@ __device__ __forceinline__ bool canopy_patch(const CView &V, const CSwitch &sw, 
         const double wtaq = fvn / raw1;
         const double wtlq = fvn * (elai + esai) / rb * rpp;
         const double rdl = dl_ext / (0.004 * uaf);
-        if (delq < 0.0) {
-            wtgq = fvn / (raw2 + rdl);
-        } else {
-            if (sw.soil_resis == CTSM_SOILRESIS_LEEPIELKE1992) wtgq = S.soilbeta[cc] * fvn / (raw2 + rdl);
-            if (sw.soil_resis == CTSM_SOILRESIS_SL14) wtgq = fvn / (raw2 + S.soilresis[cc]);
+        // the branches' divides share one evaluation (operands per lane)
+        {
+            const bool lee = sw.soil_resis == CTSM_SOILRESIS_LEEPIELKE1992, sl14 = sw.soil_resis == CTSM_SOILRESIS_SL14;
+            if (delq < 0.0 || lee || sl14) {
+                const bool wet = delq < 0.0;
+                const double num = (wet || !lee) ? fvn : S.soilbeta[cc] * fvn;
+                const double den = (wet || lee) ? raw2 + rdl : raw2 + S.soilresis[cc];
+                wtgq = num / den;
+            }
         }
         const double wtsqi = 1.0 / (wtaq + wtlq + wtgq);
         const double wtgq0 = wtgq * wtsqi;
