@@ -43,11 +43,11 @@ HBM_PEAK_GBS = 8000.0
 # The bench reports them only for a run of that workload, size and launch
 # shape; any other run reports null.
 PROFILED = {
-    # profiles/canopy_rounds_canopy_uneven1048576.md: FETCH 8,386,124 KiB x 2.0 + WRITE 18,181,632 KiB
-    # x 1.0 per call; 513.1 GFLOP FP64 in 46.48 ms
+    # profiles/canopy_rounds_canopy_uneven1048576.md: FETCH 8,222,014 KiB x 2.0 + WRITE 17,863,759 KiB
+    # x 1.0 per call; 470.3 GFLOP FP64 in 43.50 ms
     ("k_canopy_round", "canopy_uneven", 1048576): {
-        "traffic": (8386124 * 2.0 + 18181632 * 1.0) * 1024,
-        "fp64_valu_frac": 513.1e9 / 46.48e-3 / 78.6e12,
+        "traffic": (8222014 * 2.0 + 17863759 * 1.0) * 1024,
+        "fp64_valu_frac": 470.3e9 / 43.50e-3 / 78.6e12,
         "tuning": {"canopy_first_iterations": 6, "canopy_round_iterations": 6, "canopy_waves": 1},
     },
 }
