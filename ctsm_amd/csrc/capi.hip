@@ -31,6 +31,8 @@ void launch_soil_step(hipStream_t, const View &, const Params &, const ctsm_col_
 void launch_soiltemp_plain(hipStream_t, const View &, const Params &, const ctsm_col_state &, const ctsm_patch_state &,
                            int, const int32_t *, double, double *, long, ErrWord *);
 size_t soiltemp_workspace_doubles(int n);
+size_t richards_order_bytes(int n);
+const int32_t *launch_richards_order(hipStream_t, const View &, const ctsm_col_state &, int, const int32_t *, void *);
 size_t soiltemp_plain_workspace_doubles(int n);
 size_t soilwater_plain_workspace_doubles(int n);
 void launch_soilwater_plain(hipStream_t, const View &, const Params &, const ctsm_col_state &, int, const int32_t *,
@@ -799,8 +801,15 @@ int ctsm_soilwater_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_sta
         launch_soilwater_plain(ctx->stream, make_view(ctx, *lay), ctx->prm, *col, n, filter_dev, dtime,
                                (double *)ctx->work.p, n, ctx->err);
     } else {
-        launch_soilwater(ctx->stream, make_view(ctx, *lay), ctx->prm, *col, n, filter_dev, dtime, ctx->err,
-                         ctx->tun.richards_lanes, ctx->tun.richards_block);
+        const View V = make_view(ctx, *lay);
+        const int32_t *rf = filter_dev;
+        if (ctx->tun.richards_lanes != 0) {      // columns sorted by their previous sub-step count
+            int rc;
+            if ((rc = ensure_work(ctx, (richards_order_bytes(n) + 7) / 8))) return rc;
+            rf = launch_richards_order(ctx->stream, V, *col, n, filter_dev, ctx->work.p);
+        }
+        launch_soilwater(ctx->stream, V, ctx->prm, *col, n, rf, dtime, ctx->err, ctx->tun.richards_lanes,
+                         ctx->tun.richards_block);
     }
     return finish_launch(ctx);
 }
@@ -842,9 +851,10 @@ static int step_split(const ctsm_ctx *ctx, int n)
 // Workspace doubles and per-part filter bytes of a fused soil step.
 static size_t step_work_doubles(const ctsm_ctx *ctx, int n)
 {
+    // per part the heat solve's rows, reused for the Richards order after it
     const int S = step_split(ctx, n);
-    if (S == 1) return soiltemp_workspace_doubles(n);
-    return (size_t)S * soiltemp_workspace_doubles((n + S - 1) / S);
+    const int len = (n + S - 1) / S;
+    return (size_t)S * std::max(soiltemp_workspace_doubles(len), (richards_order_bytes(len) + 7) / 8);
 }
 // Workspace doubles of a full land step (one stream): the heat solve's LU
 // rows, or in the plain form the larger of the plain kernels' intermediates.
@@ -852,7 +862,9 @@ static size_t land_work_doubles(const ctsm_ctx *ctx, int n)
 {
     if (ctx->tun.plain_routines)
         return std::max(soiltemp_plain_workspace_doubles(n), soilwater_plain_workspace_doubles(n));
-    return soiltemp_workspace_doubles(n);
+    // the heat solve's rows, then (reused once the heat solve is done) the
+    // Richards solve's column order
+    return std::max(soiltemp_workspace_doubles(n), (richards_order_bytes(n) + 7) / 8);
 }
 static size_t step_partf_bytes(const ctsm_ctx *ctx, int n)
 {
@@ -898,11 +910,13 @@ static int soil_steps(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_stat
             hipStream_t st = k == 0 ? ctx->stream : ctx->side[k - 1];
             const int cnt = (int)std::min<long>(len, (long)n - k * len);
             if (cnt <= 0) continue;
-            double *ws = (double *)ctx->work.p + (size_t)k * soiltemp_workspace_doubles((int)len);
+            double *ws = (double *)ctx->work.p + (size_t)k * (step_work_doubles(ctx, n) / S);
             for (int s = 0; s < nsteps; ++s) {
                 launch_soiltemp_fractions(st, V, ctx->prm, *col, *pch, cnt, pf + k * len, dtime, ws, len, ctx->err,
                                           ctx->tun.soiltemp_block);
-                launch_soilwater(st, V, ctx->prm, *col, cnt, pf + k * len, dtime, ctx->err, ctx->tun.richards_lanes,
+                const int32_t *rf = pf + k * len;
+                if (ctx->tun.richards_lanes != 0) rf = launch_richards_order(st, V, *col, cnt, rf, ws);
+                launch_soilwater(st, V, ctx->prm, *col, cnt, rf, dtime, ctx->err, ctx->tun.richards_lanes,
                                  ctx->tun.richards_block);
             }
         }
@@ -915,8 +929,10 @@ static int soil_steps(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_stat
         for (int s = 0; s < nsteps; ++s) {
             launch_soiltemp_fractions(ctx->stream, V, ctx->prm, *col, *pch, n, filter_dev, dtime,
                                       (double *)ctx->work.p, n, ctx->err, ctx->tun.soiltemp_block);
-            launch_soilwater(ctx->stream, V, ctx->prm, *col, n, filter_dev, dtime, ctx->err,
-                             ctx->tun.richards_lanes, ctx->tun.richards_block);
+            const int32_t *rf = filter_dev;
+            if (ctx->tun.richards_lanes != 0) rf = launch_richards_order(ctx->stream, V, *col, n, rf, ctx->work.p);
+            launch_soilwater(ctx->stream, V, ctx->prm, *col, n, rf, dtime, ctx->err, ctx->tun.richards_lanes,
+                             ctx->tun.richards_block);
         }
     }
     return finish_launch(ctx);
@@ -1837,12 +1853,17 @@ static int land_step_full(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_
                           sw->qflx_ev_h2osfc, nc, filterc_dev, ctx->err);
         launch_surface_water(ctx->stream, *lay, ctx->cfg.e_ice, *hprm, *col, *sw, nc, filterc_dev, dtime, ctx->err);
     }
-    if (plain)
+    if (plain) {
         launch_soilwater_plain(ctx->stream, V, ctx->prm, *col, nc, filterc_dev, dtime, (double *)ctx->work.p, nc,
                                ctx->err);
-    else
-        launch_soilwater(ctx->stream, V, ctx->prm, *col, nc, filterc_dev, dtime, ctx->err, ctx->tun.richards_lanes,
+    } else {
+        // columns sorted by their previous sub-step count, so a wave's columns
+        // sub-step alike (the adaptive group size keeps the filter's depth order)
+        const int32_t *rf = filterc_dev;
+        if (ctx->tun.richards_lanes != 0) rf = launch_richards_order(ctx->stream, V, *col, nc, filterc_dev, ctx->work.p);
+        launch_soilwater(ctx->stream, V, ctx->prm, *col, nc, rf, dtime, ctx->err, ctx->tun.richards_lanes,
                          ctx->tun.richards_block);
+    }
     if (wt) {
         launch_patch2col3(ctx->stream, *lay, *col, can->active, can->wtcol, sf->qflx_liqdew_to_top_layer,
                           sf->qflx_soliddew_to_top_layer, sf->qflx_solidevap_from_top_layer,
