@@ -19,6 +19,7 @@
 // the column order in the filter (sorted by layer count by the caller) is
 // what keeps a wavefront's lanes doing the same amount of work.
 #include "common.h"
+#include "bucket_order.h"
 #include "trace.h"
 #include "soilwater_dev.h"
 #include "richards_group.h"
@@ -144,103 +145,34 @@ void launch_soilwater(hipStream_t st, const View &V, const Params &P, const ctsm
 // The solve sub-steps adaptively, and the lanes of a wave iterate until the
 // last of its columns is done: columns that need few sub-steps wait for a
 // neighbour that needs many.  The previous step's count predicts the next,
-// so a step can solve its columns sorted by it: a stable counting sort of the
-// filter on num_substeps (SUB_KEYS buckets, the last one open-ended) into a
-// column list the solve then takes as its filter.  Each column's result does
-// not depend on the order its wave was formed in.
-constexpr int SUB_KEYS = 64;
-constexpr int SUB_BLOCK = 256;
-
-__device__ __forceinline__ int sub_key(const View &V, const ctsm_col_state &S, int c)
-{
-    if (!col_in_range(V, c)) return 0;
-    const double ns = S.num_substeps[V.c1(c)];
-    if (!(ns > 0.0)) return 0;
-    return ns >= (double)(SUB_KEYS - 1) ? SUB_KEYS - 1 : (int)ns;
-}
-
-// per block of SUB_BLOCK filter entries, the count of each key: hist[key * nblk + block]
-__global__ void __launch_bounds__(SUB_BLOCK) k_sub_count(View V, ctsm_col_state S, int n,
-                                                         const int32_t *__restrict__ filter, int32_t *hist)
-{
-    __shared__ int cnt[SUB_KEYS];
-    if (threadIdx.x < SUB_KEYS) cnt[threadIdx.x] = 0;
-    __syncthreads();
-    const int i = blockIdx.x * SUB_BLOCK + threadIdx.x;
-    if (i < n) atomicAdd(&cnt[sub_key(V, S, filter ? filter[i] : V.begc + i)], 1);
-    __syncthreads();
-    if (threadIdx.x < SUB_KEYS) hist[(long)threadIdx.x * gridDim.x + blockIdx.x] = cnt[threadIdx.x];
-}
-
-// exclusive prefix sum of hist[0 .. m) in one block (key-major, so the blocks of
-// one key are consecutive and each key follows the smaller ones)
-__global__ void __launch_bounds__(1024) k_sub_scan(int32_t *hist, int m)
-{
-    __shared__ int part[1024];
-    const int t = threadIdx.x;
-    const int per = (m + 1023) / 1024;
-    const int lo = t * per, hi = min(lo + per, m);
-    int s = 0;
-    for (int k = lo; k < hi; ++k) s += hist[k];
-    part[t] = s;
-    __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {
-        const int v = t >= d ? part[t - d] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
+// so a step can solve its columns sorted by it (bucket_order.h, BO_KEYS
+// buckets, the last one open-ended) and pass the sorted column list to the
+// solve as its filter.  Each column's result does not depend on the order its
+// wave was formed in.
+struct SubstepKey {
+    View V;
+    ctsm_col_state S;
+    const int32_t *filter;
+    __device__ int32_t value(int i) const { return filter ? filter[i] : V.begc + i; }
+    __device__ int key(int i) const
+    {
+        const int c = value(i);
+        if (!col_in_range(V, c)) return 0;
+        const double ns = S.num_substeps[V.c1(c)];
+        if (!(ns > 0.0)) return 0;
+        return ns >= (double)(BO_KEYS - 1) ? BO_KEYS - 1 : (int)ns;
     }
-    int run = part[t] - s;
-    for (int k = lo; k < hi; ++k) {
-        const int v = hist[k];
-        hist[k] = run;
-        run += v;
-    }
-}
+};
 
-// each entry to its key's range, entries of one key in filter order
-__global__ void __launch_bounds__(SUB_BLOCK) k_sub_scatter(View V, ctsm_col_state S, int n,
-                                                           const int32_t *__restrict__ filter,
-                                                           const int32_t *__restrict__ start, int32_t *order)
-{
-    __shared__ int next[SUB_KEYS];
-    if (threadIdx.x < SUB_KEYS) next[threadIdx.x] = start[(long)threadIdx.x * gridDim.x + blockIdx.x];
-    const int i = blockIdx.x * SUB_BLOCK + threadIdx.x;
-    const int c = (i < n) ? (filter ? filter[i] : V.begc + i) : 0;
-    const int key = (i < n) ? sub_key(V, S, c) : -1;
-    const int lane = threadIdx.x & 63;
-    const unsigned long long below = (1ull << lane) - 1ull;
-    for (int w = 0; w < SUB_BLOCK / 64; ++w) {
-        __syncthreads();
-        if ((int)(threadIdx.x >> 6) != w) continue;
-        // the wave's keys one at a time (ascending lanes = filter order)
-        unsigned long long todo = __ballot(key >= 0);
-        while (todo) {
-            const int k = __shfl(key, __ffsll((unsigned long long)todo) - 1);
-            const unsigned long long same = __ballot(key == k);
-            if (key == k) order[next[k] + __popcll(same & below)] = c;
-            if (lane == __ffsll((unsigned long long)todo) - 1) next[k] += __popcll(same);
-            todo &= ~same;
-        }
-    }
-}
-
-size_t richards_order_bytes(int n)
-{
-    const long nblk = (n + SUB_BLOCK - 1) / SUB_BLOCK;
-    return (size_t)n * sizeof(int32_t) + (size_t)SUB_KEYS * nblk * sizeof(int32_t);
-}
+size_t richards_order_bytes(int n) { return (size_t)n * sizeof(int32_t) + bucket_order_table_bytes(n); }
 
 // The filter entries sorted by num_substeps into ws (richards_order_bytes(n));
 // returns the sorted column list (inside ws).
 const int32_t *launch_richards_order(hipStream_t st, const View &V, const ctsm_col_state &S, int n,
                                      const int32_t *filter, void *ws)
 {
-    const int nblk = (n + SUB_BLOCK - 1) / SUB_BLOCK;
-    int32_t *order = (int32_t *)ws, *hist = order + n;
-    hipLaunchKernelGGL(k_sub_count, dim3(nblk), dim3(SUB_BLOCK), 0, st, V, S, n, filter, hist);
-    hipLaunchKernelGGL(k_sub_scan, dim3(1), dim3(1024), 0, st, hist, SUB_KEYS * nblk);
-    hipLaunchKernelGGL(k_sub_scatter, dim3(nblk), dim3(SUB_BLOCK), 0, st, V, S, n, filter, hist, order);
+    int32_t *order = (int32_t *)ws;
+    launch_bucket_order(st, SubstepKey{V, S, filter}, n, nullptr, order + n, order);
     return order;
 }
 
