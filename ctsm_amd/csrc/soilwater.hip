@@ -57,12 +57,13 @@ k_soilwater_fractions(View V, ctsm_col_state S, int n, const int32_t *__restrict
 
 // G lanes per column (richards_group.h).  Groups are aligned to G lanes, so a
 // column's lanes retire together.  The solve is latency-bound (a trial is a
-// serial elimination chain), so for G = 8 the register budget is capped at 128
-// VGPRs for four waves per SIMD: the few values that then live in scratch cost
-// less than the latency the extra waves hide (measured: 1.89 -> 1.66 ms on
-// 65,536 uneven columns; 3, 5, 6 and 8 waves were slower).
+// serial elimination chain), so for G = 8 the register budget is capped at 168
+// VGPRs for three waves per SIMD: the few values that then live in scratch
+// (10 VGPRs) cost less than the latency the extra wave hides.  Measured on the
+// columns ordered by their previous sub-step count, 1,048,576 uneven columns:
+// 5.07 ms at four waves (77 VGPRs spilled), 4.37 at three, 5.17 at two.
 template <int G> struct RichardsWaves { static constexpr int value = 1; };
-template <> struct RichardsWaves<8> { static constexpr int value = 4; };
+template <> struct RichardsWaves<8> { static constexpr int value = 3; };
 
 template <int G>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RichardsWaves<G>::value)))
