@@ -370,6 +370,15 @@ DRAINAGE_FIELDS = [(n, "f64", "soi" if n == "h2osoi_vol" else "c") for n in (
     "qflx_drain", "qflx_qrgwl", "qflx_runoff")]
 
 
+# column water stores and fluxes of the water balance check (ctsm_water_balance_state)
+WATERBAL_FIELDS = [
+    ("h2osno_no_layers", "f64", "c"), ("total_plant_stored_h2o", "f64", "c"), ("wa", "f64", "c"),
+    ("excess_ice", "f64", "soi"), ("dynbal_baseline_liq", "f64", "c"), ("dynbal_baseline_ice", "f64", "c"),
+    ("hydrologically_active", "i32", "c"), ("col_active", "i32", "c")] + [(n, "f64", "c") for n in (
+        "forc_rain", "forc_snow", "qflx_floodc", "qflx_sfc_irrig", "qflx_glcice_dyn_water_flux", "qflx_evap_tot",
+        "qflx_surf", "qflx_qrgwl", "qflx_drain", "qflx_drain_perched", "qflx_ice_runoff",
+        "qflx_snwcp_discarded_liq", "qflx_snwcp_discarded_ice", "begwb", "endwb", "errh2o")]
+
 # plant hydraulic stress, soil side (ctsm_phs_state): patch members, then column members
 PHS_FIELDS = [
     ("itype", "i32", "p"), ("column", "i32", "p"), ("active", "i32", "p"), ("frac_veg_nosno", "i32", "p"),
@@ -428,6 +437,10 @@ class SurfaceWaterState(C.Structure):
 
 class DrainageState(C.Structure):
     _fields_ = [(n, _CT[t]) for n, t, _ in DRAINAGE_FIELDS]
+
+
+class WaterBalanceState(C.Structure):
+    _fields_ = [(n, _CT[t]) for n, t, _ in WATERBAL_FIELDS]
 
 
 class PhsState(C.Structure):

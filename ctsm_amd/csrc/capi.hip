@@ -54,6 +54,14 @@ void launch_balance(hipStream_t, const ctsm_layout &, const ctsm_config &, const
 void launch_energy_balance(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_col_state &,
                            const ctsm_patch_state &, const ctsm_canopy_state &, const ctsm_soilflux_state &, int,
                            const int32_t *, double *, double *, unsigned long long *, ErrWord *);
+void launch_water_mass(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_col_state &,
+                       const ctsm_canopy_state &, const ctsm_water_balance_state &, int, const int32_t *, double, int,
+                       double *, ErrWord *);
+void launch_water_balance(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_water_balance_state &, int,
+                          const int32_t *, double, unsigned long long *, ErrWord *);
+void launch_solar_balance(hipStream_t, const ctsm_layout &, const ctsm_config &, const ctsm_canopy_state &, int,
+                          const int32_t *, const double *, const double *, const double *, const double *, double *,
+                          unsigned long long *, ErrWord *);
 void launch_surface_water(hipStream_t, const ctsm_layout &, double, const ctsm_hydro_params &, const ctsm_col_state &,
                           const ctsm_surface_water_state &, int, const int32_t *, double, ErrWord *);
 void launch_patch2col3(hipStream_t, const ctsm_layout &, const ctsm_col_state &, const int32_t *, const double *,
@@ -1300,6 +1308,71 @@ int ctsm_energy_balance_check_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const c
     std::memcpy(errlon_max, &w[2], sizeof(double));
     *seb_patch = (n > 0 && w[1] != 0x7fffffffull) ? (int32_t)w[1] : 0;
     *lon_patch = (n > 0 && w[3] != 0x7fffffffull) ? (int32_t)w[3] : 0;
+    return CTSM_OK;
+}
+
+int ctsm_solar_balance_check_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_canopy_state *can, int32_t n,
+                                 const int32_t *filterp_dev, const double *fsa, const double *fsr,
+                                 const double *forc_solad_col, const double *forc_solai, double *errsol,
+                                 double *errsol_max, int32_t *sol_patch)
+{
+    if (!ctx || !can || !fsa || !fsr || !forc_solad_col || !forc_solai || !errsol_max || !sol_patch || n < 0 ||
+        check_layout(lay) || lay->ldp <= 0 || lay->ldg <= 0 || (!filterp_dev && n > lay->ldp) || !can->active ||
+        !can->column || !can->gridcell)
+        return CTSM_E_ARG;
+    hipError_t he = ctx->bal.ensure(2 * sizeof(unsigned long long));
+    if (he != hipSuccess) return hip_fail(ctx, he);
+    launch_solar_balance(ctx->stream, *lay, ctx->cfg, *can, n, filterp_dev, fsa, fsr, forc_solad_col, forc_solai,
+                         errsol, (unsigned long long *)ctx->bal.p, ctx->err);
+    int rc = finish_launch(ctx);
+    if (rc) return rc;
+    unsigned long long w[2] = {0ull, 0ull};
+    he = hipMemcpyAsync(w, ctx->bal.p, sizeof(w), hipMemcpyDeviceToHost, ctx->stream);
+    if (he != hipSuccess) return hip_fail(ctx, he);
+    if ((rc = sync_and_check(ctx))) return rc;
+    std::memcpy(errsol_max, &w[0], sizeof(double));
+    *sol_patch = (n > 0 && w[1] != 0x7fffffffull) ? (int32_t)w[1] : 0;
+    return CTSM_OK;
+}
+
+int ctsm_water_mass_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                        const ctsm_canopy_state *can, const ctsm_water_balance_state *wb, int32_t n,
+                        const int32_t *filter_dev, double aquifer_water_baseline, int32_t subtract_dynbal_baselines,
+                        double *water_mass)
+{
+    if (!ctx || !col || !can || !wb || !water_mass || n < 0 || check_layout(lay) || lay->ldp <= 0 ||
+        (!filter_dev && n > lay->ldc) || !col->snl || !col->patchi || !col->patchf || !col->h2osoi_liq ||
+        !col->h2osoi_ice || !col->h2osfc || !can->active || !can->wtcol || !can->liqcan || !can->snocan ||
+        !wb->h2osno_no_layers || !wb->total_plant_stored_h2o || !wb->wa || !wb->hydrologically_active ||
+        (subtract_dynbal_baselines && (!wb->dynbal_baseline_liq || !wb->dynbal_baseline_ice)))
+        return CTSM_E_ARG;
+    launch_water_mass(ctx->stream, *lay, ctx->cfg, *col, *can, *wb, n, filter_dev, aquifer_water_baseline,
+                      subtract_dynbal_baselines ? 1 : 0, water_mass, ctx->err);
+    return finish_launch(ctx);
+}
+
+int ctsm_water_balance_check_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_water_balance_state *wb,
+                                 int32_t n, const int32_t *filterc_dev, double dtime, double *errh2o_max,
+                                 int32_t *errh2o_col)
+{
+    if (!ctx || !wb || !errh2o_max || !errh2o_col || n < 0 || check_layout(lay) ||
+        (!filterc_dev && n > lay->ldc) || !wb->col_active || !wb->begwb || !wb->endwb || !wb->forc_rain ||
+        !wb->forc_snow || !wb->qflx_floodc || !wb->qflx_sfc_irrig || !wb->qflx_glcice_dyn_water_flux ||
+        !wb->qflx_evap_tot || !wb->qflx_surf || !wb->qflx_qrgwl || !wb->qflx_drain || !wb->qflx_drain_perched ||
+        !wb->qflx_ice_runoff || !wb->qflx_snwcp_discarded_liq || !wb->qflx_snwcp_discarded_ice)
+        return CTSM_E_ARG;
+    hipError_t he = ctx->bal.ensure(2 * sizeof(unsigned long long));
+    if (he != hipSuccess) return hip_fail(ctx, he);
+    launch_water_balance(ctx->stream, *lay, ctx->cfg, *wb, n, filterc_dev, dtime, (unsigned long long *)ctx->bal.p,
+                         ctx->err);
+    int rc = finish_launch(ctx);
+    if (rc) return rc;
+    unsigned long long w[2] = {0ull, 0ull};
+    he = hipMemcpyAsync(w, ctx->bal.p, sizeof(w), hipMemcpyDeviceToHost, ctx->stream);
+    if (he != hipSuccess) return hip_fail(ctx, he);
+    if ((rc = sync_and_check(ctx))) return rc;
+    std::memcpy(errh2o_max, &w[0], sizeof(double));
+    *errh2o_col = (n > 0 && w[1] != 0x7fffffffull) ? (int32_t)w[1] : 0;
     return CTSM_OK;
 }
 

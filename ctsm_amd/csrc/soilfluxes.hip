@@ -331,6 +331,145 @@ __global__ void __launch_bounds__(256) k_energy_loc(SView V, ctsm_col_state S, c
     if ((unsigned long long)__double_as_longlong(fabs(lon)) == w[2]) atomicMin(&w[3], (unsigned long long)p);
 }
 
+
+// ComputeLiqIceMassNonLake (TotalWaterAndHeatMod.F90:200-393) for a soil or
+// crop column, sums in the reference's order: the canopy water's
+// p2c_1d_filter (subgridAveMod.F90:312-318), plant storage, unlayered snow,
+// the snow layers above snl, the aquifer, surface water, the ground layers
+// with excess ice, then the dynbal baselines when subtracted.
+__global__ void __launch_bounds__(256) k_water_mass(SView V, ctsm_col_state S, ctsm_canopy_state Cn,
+                                                    ctsm_water_balance_state W, int n,
+                                                    const int32_t *__restrict__ filter, double baseline,
+                                                    int subtract, double *__restrict__ mass, ErrWord *err)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = filter ? filter[i] : V.begc + i;
+    if (c < V.begc || c >= V.begc + V.ldc) { report_error(err, c, CTSM_E_ARG); return; }
+    const long cc = V.c1(c);
+    const int pi = S.patchi[cc], pf = S.patchf[cc], snl = S.snl[cc];
+    if ((pf >= pi && (pi < V.begp || pf >= V.begp + V.ldp)) || snl < -V.nlevsno || snl > 0) {
+        report_error(err, c, CTSM_E_ARG);
+        return;
+    }
+    double lc = 0.0, sc = 0.0;
+    for (int p = pi; p <= pf; ++p) {
+        const long pp = V.p1(p);
+        if (Cn.active[pp]) {
+            lc = lc + Cn.liqcan[pp] * Cn.wtcol[pp];
+            sc = sc + Cn.snocan[pp] * Cn.wtcol[pp];
+        }
+    }
+    double liquid = 0.0, ice = 0.0;
+    liquid = liquid + lc + W.total_plant_stored_h2o[cc];
+    ice = ice + sc;
+    ice = ice + W.h2osno_no_layers[cc];
+    for (int j = snl + 1; j <= 0; ++j) {
+        liquid = liquid + S.h2osoi_liq[V.sno(c, j)];
+        ice = ice + S.h2osoi_ice[V.sno(c, j)];
+    }
+    if (W.hydrologically_active[cc]) liquid = liquid + (W.wa[cc] - baseline);
+    liquid = liquid + S.h2osfc[cc];
+    for (int j = 1; j <= V.nlevgrnd; ++j) {
+        liquid = liquid + S.h2osoi_liq[V.sno(c, j)];
+        const double xi = W.excess_ice ? W.excess_ice[cc + (long)(j - 1) * V.ldc] : 0.0;
+        ice = ice + S.h2osoi_ice[V.sno(c, j)] + xi;
+    }
+    if (subtract) {
+        liquid = liquid - W.dynbal_baseline_liq[cc];
+        ice = ice - W.dynbal_baseline_ice[cc];
+    }
+    mass[cc] = liquid + ice;
+}
+
+// errh2o_col (BalanceCheckMod.F90:588-610), left to right as written; 0 for
+// inactive columns.
+__device__ __forceinline__ double water_error(const ctsm_water_balance_state &W, long cc, double dtime)
+{
+    if (!W.col_active[cc]) return 0.0;
+    return W.endwb[cc] - W.begwb[cc] -
+           (W.forc_rain[cc] + W.forc_snow[cc] + W.qflx_floodc[cc] + W.qflx_sfc_irrig[cc] +
+            W.qflx_glcice_dyn_water_flux[cc] - W.qflx_evap_tot[cc] - W.qflx_surf[cc] - W.qflx_qrgwl[cc] -
+            W.qflx_drain[cc] - W.qflx_drain_perched[cc] - W.qflx_ice_runoff[cc] - W.qflx_snwcp_discarded_liq[cc] -
+            W.qflx_snwcp_discarded_ice[cc]) *
+               dtime;
+}
+
+// w[0..1] as k_balance_max / k_balance_loc: bits of the max |errh2o|, its column.
+__global__ void __launch_bounds__(256) k_water_balance_max(SView V, ctsm_water_balance_state W, int n,
+                                                           const int32_t *__restrict__ filter, double dtime,
+                                                           unsigned long long *w, ErrWord *err)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = filter ? filter[i] : V.begc + i;
+    if (c < V.begc || c >= V.begc + V.ldc) { report_error(err, c, CTSM_E_ARG); return; }
+    const long cc = V.c1(c);
+    const double e = water_error(W, cc, dtime);
+    if (W.errh2o) W.errh2o[cc] = e;
+    atomicMax(&w[0], (unsigned long long)__double_as_longlong(fabs(e)));
+}
+
+__global__ void __launch_bounds__(256) k_water_balance_loc(SView V, ctsm_water_balance_state W, int n,
+                                                           const int32_t *__restrict__ filter, double dtime,
+                                                           unsigned long long *w)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = filter ? filter[i] : V.begc + i;
+    if (c < V.begc || c >= V.begc + V.ldc) return;
+    const double e = water_error(W, V.c1(c), dtime);
+    if ((unsigned long long)__double_as_longlong(fabs(e)) == w[0]) atomicMin(&w[1], (unsigned long long)c);
+}
+
+// errsol (BalanceCheckMod.F90:964-970) for non-urban patches; 0 when inactive.
+struct SolarIn {
+    const double *fsa, *fsr, *solad, *solai;
+    int begg, ldg;
+};
+
+__device__ __forceinline__ double solar_error(const SView &V, const ctsm_canopy_state &Cn, const SolarIn &R, int p)
+{
+    const long pp = V.p1(p);
+    if (!Cn.active[pp]) return 0.0;
+    const long cc = V.c1(Cn.column[pp]);
+    const long gg = (long)(Cn.gridcell[pp] - R.begg);
+    return R.fsa[pp] + R.fsr[pp] - (R.solad[cc] + R.solad[cc + V.ldc] + R.solai[gg] + R.solai[gg + R.ldg]);
+}
+
+__device__ __forceinline__ bool solar_patch_ok(const SView &V, const ctsm_canopy_state &Cn, const SolarIn &R, int p)
+{
+    if (p < V.begp || p >= V.begp + V.ldp) return false;
+    const long pp = V.p1(p);
+    if (!Cn.active[pp]) return true;
+    const int c = Cn.column[pp], g = Cn.gridcell[pp];
+    return c >= V.begc && c < V.begc + V.ldc && g >= R.begg && g < R.begg + R.ldg;
+}
+
+__global__ void __launch_bounds__(256) k_solar_max(SView V, ctsm_canopy_state Cn, SolarIn R, int n,
+                                                   const int32_t *__restrict__ filter, double *errsol,
+                                                   unsigned long long *w, ErrWord *err)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int p = filter ? filter[i] : V.begp + i;
+    if (!solar_patch_ok(V, Cn, R, p)) { report_error(err, p, CTSM_E_ARG); return; }
+    const double e = solar_error(V, Cn, R, p);
+    if (errsol) errsol[i] = e;
+    atomicMax(&w[0], (unsigned long long)__double_as_longlong(fabs(e)));
+}
+
+__global__ void __launch_bounds__(256) k_solar_loc(SView V, ctsm_canopy_state Cn, SolarIn R, int n,
+                                                   const int32_t *__restrict__ filter, unsigned long long *w)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int p = filter ? filter[i] : V.begp + i;
+    if (!solar_patch_ok(V, Cn, R, p)) return;
+    const double e = solar_error(V, Cn, R, p);
+    if ((unsigned long long)__double_as_longlong(fabs(e)) == w[0]) atomicMin(&w[1], (unsigned long long)p);
+}
+
 }  // namespace sflux
 
 static sflux::SView sview(const ctsm_layout &L, const ctsm_config &cfg)
@@ -381,6 +520,44 @@ void launch_energy_balance(hipStream_t st, const ctsm_layout &L, const ctsm_conf
     hipLaunchKernelGGL(sflux::k_energy_max, dim3((n + 255) / 256), dim3(256), 0, st, V, S, P, Cn, F, n, filter,
                        errseb, errlon, w, err);
     hipLaunchKernelGGL(sflux::k_energy_loc, dim3((n + 255) / 256), dim3(256), 0, st, V, S, P, Cn, F, n, filter, w);
+}
+
+void launch_water_mass(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg, const ctsm_col_state &S,
+                       const ctsm_canopy_state &Cn, const ctsm_water_balance_state &W, int n, const int32_t *filter,
+                       double baseline, int subtract, double *mass, ErrWord *err)
+{
+    const GptlRange range("balchk");
+    if (n <= 0) return;
+    hipLaunchKernelGGL(sflux::k_water_mass, dim3((n + 255) / 256), dim3(256), 0, st, sview(L, cfg), S, Cn, W, n,
+                       filter, baseline, subtract, mass, err);
+}
+
+void launch_water_balance(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg,
+                          const ctsm_water_balance_state &W, int n, const int32_t *filter, double dtime,
+                          unsigned long long *w, ErrWord *err)
+{
+    const GptlRange range("balchk");
+    const sflux::SView V = sview(L, cfg);
+    hipLaunchKernelGGL(sflux::k_balance_init, dim3(1), dim3(64), 0, st, w);
+    if (n <= 0) return;
+    hipLaunchKernelGGL(sflux::k_water_balance_max, dim3((n + 255) / 256), dim3(256), 0, st, V, W, n, filter, dtime,
+                       w, err);
+    hipLaunchKernelGGL(sflux::k_water_balance_loc, dim3((n + 255) / 256), dim3(256), 0, st, V, W, n, filter, dtime,
+                       w);
+}
+
+void launch_solar_balance(hipStream_t st, const ctsm_layout &L, const ctsm_config &cfg, const ctsm_canopy_state &Cn,
+                          int n, const int32_t *filter, const double *fsa, const double *fsr, const double *solad,
+                          const double *solai, double *errsol, unsigned long long *w, ErrWord *err)
+{
+    const GptlRange range("balchk");
+    const sflux::SView V = sview(L, cfg);
+    const sflux::SolarIn R{fsa, fsr, solad, solai, L.begg_alloc, L.ldg};
+    hipLaunchKernelGGL(sflux::k_balance_init, dim3(1), dim3(64), 0, st, w);
+    if (n <= 0) return;
+    hipLaunchKernelGGL(sflux::k_solar_max, dim3((n + 255) / 256), dim3(256), 0, st, V, Cn, R, n, filter, errsol, w,
+                       err);
+    hipLaunchKernelGGL(sflux::k_solar_loc, dim3((n + 255) / 256), dim3(256), 0, st, V, Cn, R, n, filter, w);
 }
 
 }  // namespace ctsm

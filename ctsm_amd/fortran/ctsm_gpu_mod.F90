@@ -193,6 +193,20 @@ module ctsm_gpu_mod
      type(c_ptr) :: qflx_qrgwl = c_null_ptr, qflx_runoff = c_null_ptr
   end type ctsm_drainage_state_t
 
+  ! column water stores and fluxes of the water balance check (member order and
+  ! meaning as ctsm_water_balance_state); device arrays
+  type, bind(C), public :: ctsm_water_balance_state_t
+     type(c_ptr) :: h2osno_no_layers = c_null_ptr, total_plant_stored_h2o = c_null_ptr, wa = c_null_ptr
+     type(c_ptr) :: excess_ice = c_null_ptr, dynbal_baseline_liq = c_null_ptr, dynbal_baseline_ice = c_null_ptr
+     type(c_ptr) :: hydrologically_active = c_null_ptr, col_active = c_null_ptr
+     type(c_ptr) :: forc_rain = c_null_ptr, forc_snow = c_null_ptr, qflx_floodc = c_null_ptr
+     type(c_ptr) :: qflx_sfc_irrig = c_null_ptr, qflx_glcice_dyn_water_flux = c_null_ptr
+     type(c_ptr) :: qflx_evap_tot = c_null_ptr, qflx_surf = c_null_ptr, qflx_qrgwl = c_null_ptr
+     type(c_ptr) :: qflx_drain = c_null_ptr, qflx_drain_perched = c_null_ptr, qflx_ice_runoff = c_null_ptr
+     type(c_ptr) :: qflx_snwcp_discarded_liq = c_null_ptr, qflx_snwcp_discarded_ice = c_null_ptr
+     type(c_ptr) :: begwb = c_null_ptr, endwb = c_null_ptr, errh2o = c_null_ptr
+  end type ctsm_water_balance_state_t
+
   ! fff, pc, mu, the drainage parameters (param file) and h2osfcflag, as ctsm_hydro_params
   type, bind(C), public :: ctsm_hydro_params_t
      real(c_double) :: fff, pc, mu
@@ -457,6 +471,51 @@ module ctsm_gpu_mod
        integer(c_int32_t), intent(in) :: filter_exposedvegp(*)
        real(c_double), value :: dtime
      end function ctsm_canopyfluxes_phs
+
+     ! ComputeWaterMassNonLake for begwb / endwb on device-resident state
+     ! (TotalWaterAndHeatMod.F90:92-393); water_mass is a device (c) array
+     integer(c_int) function ctsm_water_mass_dev(ctx, lay, col, can, wb, num_nolakec, filter_nolakec_dev, &
+          aquifer_water_baseline, subtract_dynbal_baselines, water_mass) bind(C, name='ctsm_water_mass_dev')
+       import :: c_int, c_int32_t, c_double, c_ptr, ctsm_layout_t, ctsm_col_state_t, ctsm_canopy_state_t, &
+            ctsm_water_balance_state_t
+       type(c_ptr), value :: ctx
+       type(ctsm_layout_t), intent(in) :: lay
+       type(ctsm_col_state_t), intent(in) :: col
+       type(ctsm_canopy_state_t), intent(in) :: can
+       type(ctsm_water_balance_state_t), intent(in) :: wb
+       integer(c_int32_t), value :: num_nolakec
+       type(c_ptr), value :: filter_nolakec_dev
+       real(c_double), value :: aquifer_water_baseline
+       integer(c_int32_t), value :: subtract_dynbal_baselines
+       type(c_ptr), value :: water_mass
+     end function ctsm_water_mass_dev
+
+     ! the column water balance check of BalanceCheck (BalanceCheckMod.F90:588-630)
+     integer(c_int) function ctsm_water_balance_check_dev(ctx, lay, wb, num_filterc, filterc_dev, dtime, &
+          errh2o_max, errh2o_col) bind(C, name='ctsm_water_balance_check_dev')
+       import :: c_int, c_int32_t, c_double, c_ptr, ctsm_layout_t, ctsm_water_balance_state_t
+       type(c_ptr), value :: ctx
+       type(ctsm_layout_t), intent(in) :: lay
+       type(ctsm_water_balance_state_t), intent(in) :: wb
+       integer(c_int32_t), value :: num_filterc
+       type(c_ptr), value :: filterc_dev
+       real(c_double), value :: dtime
+       real(c_double), intent(out) :: errh2o_max
+       integer(c_int32_t), intent(out) :: errh2o_col
+     end function ctsm_water_balance_check_dev
+
+     ! the solar radiation balance of EnergyBalanceCheck (BalanceCheckMod.F90:964-970, 1008-1012)
+     integer(c_int) function ctsm_solar_balance_check_dev(ctx, lay, can, num_filterp, filterp_dev, fsa, fsr, &
+          forc_solad_col, forc_solai, errsol, errsol_max, sol_patch) bind(C, name='ctsm_solar_balance_check_dev')
+       import :: c_int, c_int32_t, c_double, c_ptr, ctsm_layout_t, ctsm_canopy_state_t
+       type(c_ptr), value :: ctx
+       type(ctsm_layout_t), intent(in) :: lay
+       type(ctsm_canopy_state_t), intent(in) :: can
+       integer(c_int32_t), value :: num_filterp
+       type(c_ptr), value :: filterp_dev, fsa, fsr, forc_solad_col, forc_solai, errsol
+       real(c_double), intent(out) :: errsol_max
+       integer(c_int32_t), intent(out) :: sol_patch
+     end function ctsm_solar_balance_check_dev
   end interface
 
 end module ctsm_gpu_mod

@@ -45,6 +45,7 @@ EXPORTS = [
     "ctsm_phs_calcstress", "ctsm_phs_calcstress_dev", "ctsm_canopyfluxes_dev_noalloc",
     "ctsm_energy_balance_check_dev", "ctsm_set_pft_rows", "ctsm_serve_create", "ctsm_serve_submit", "ctsm_serve_progress", "ctsm_serve_poll",
     "ctsm_serve_start", "ctsm_serve_release", "ctsm_serve_resume_member", "ctsm_serve_stats", "ctsm_serve_destroy",
+    "ctsm_solar_balance_check_dev", "ctsm_water_mass_dev", "ctsm_water_balance_check_dev",
 ]
 SERVE_QUEUED, SERVE_ADMITTED, SERVE_RUNNING, SERVE_DONE = range(4)
 
@@ -145,6 +146,14 @@ def load():
                                                 P(abi.CanopyState), P(abi.SoilFluxState), C.c_int32, C.c_void_p,
                                                 C.c_void_p, C.c_void_p, P(C.c_double), P(C.c_int32),
                                                 P(C.c_double), P(C.c_int32)]
+    L.ctsm_solar_balance_check_dev.argtypes = [C.c_void_p, P(abi.Layout), P(abi.CanopyState), C.c_int32, C.c_void_p,
+                                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                               P(C.c_double), P(C.c_int32)]
+    L.ctsm_water_mass_dev.argtypes = [C.c_void_p, P(abi.Layout), P(abi.ColState), P(abi.CanopyState),
+                                      P(abi.WaterBalanceState), C.c_int32, C.c_void_p, C.c_double, C.c_int32,
+                                      C.c_void_p]
+    L.ctsm_water_balance_check_dev.argtypes = [C.c_void_p, P(abi.Layout), P(abi.WaterBalanceState), C.c_int32,
+                                               C.c_void_p, C.c_double, P(C.c_double), P(C.c_int32)]
     L.ctsm_serve_create.argtypes = [C.c_void_p, P(abi.Layout), P(abi.ColState), P(abi.PatchState),
                                     P(abi.CanopyState), P(abi.SoilFluxState), P(abi.SurfaceWaterState),
                                     P(abi.HydroParams), P(abi.WaterTableState), P(abi.DrainageState), C.c_int32,
@@ -266,6 +275,38 @@ class Context:
             C.c_void_p(errlon_dev) if errlon_dev else None, C.byref(ms), C.byref(i_s), C.byref(ml), C.byref(i_l)),
             "energy balance check")
         return (ms.value, i_s.value), (ml.value, i_l.value)
+
+    def solar_balance_check_dev(self, ds, fsa_dev, fsr_dev, solad_dev, solai_dev, filterp_dev=None, n=None,
+                                errsol_dev=None):
+        """errsol of EnergyBalanceCheck over device arrays fsa / fsr (p),
+        forc_solad_col (c, 2) and forc_solai (g, 2): (max |errsol|, patch)."""
+        n = ds.dims.npatch if n is None else n
+        m, i = C.c_double(), C.c_int32()
+        vp = lambda x: C.c_void_p(x) if x else None
+        self._check(self.L.ctsm_solar_balance_check_dev(
+            self.h, C.byref(ds.layout()), C.byref(ds.canopy_struct()), n, vp(filterp_dev), vp(fsa_dev), vp(fsr_dev),
+            vp(solad_dev), vp(solai_dev), vp(errsol_dev), C.byref(m), C.byref(i)), "solar balance check")
+        return m.value, i.value
+
+    def water_mass_dev(self, ds, dwb, out_dev, filter_dev=None, n=None, baseline=5000.0, subtract=False):
+        """ComputeWaterMassNonLake on device: the column water mass of each
+        filter column into the device (c) array out_dev (enqueued)."""
+        cs, _ = ds.structs()
+        n = ds.dims.ncol if n is None else n
+        self._check(self.L.ctsm_water_mass_dev(
+            self.h, C.byref(ds.layout()), C.byref(cs), C.byref(ds.canopy_struct()), C.byref(dwb.struct()), n,
+            C.c_void_p(filter_dev) if filter_dev else None, baseline, int(bool(subtract)), C.c_void_p(out_dev)),
+            "water mass")
+
+    def water_balance_check_dev(self, ds, dwb, filter_dev=None, n=None, dtime=1800.0):
+        """errh2o per column into the DeviceWaterBalance's errh2o (when it has
+        one): (max |errh2o|, column)."""
+        n = ds.dims.ncol if n is None else n
+        m, i = C.c_double(), C.c_int32()
+        self._check(self.L.ctsm_water_balance_check_dev(
+            self.h, C.byref(ds.layout()), C.byref(dwb.struct()), n, C.c_void_p(filter_dev) if filter_dev else None,
+            dtime, C.byref(m), C.byref(i)), "water balance check")
+        return m.value, i.value
 
     def set_pft_rows(self, first_row, pft_table):
         """Rows first_row.. of the context's PFT table from pft_table (a

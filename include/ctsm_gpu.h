@@ -885,13 +885,78 @@ int ctsm_soil_balance_check_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const cts
  * errseb / errlon when they are not NULL; waits and returns the largest
  * |errseb| and |errlon| with the lowest patch holding each.  The caller
  * applies the reference's thresholds (1e-7 W/m2 warning, 1e-5 W/m2 abort
- * after the skip steps).  errsol checks SurfaceRadiation's fsa / fsr, which
- * are not on this path. */
+ * after the skip steps).  errsol is ctsm_solar_balance_check_dev. */
 int ctsm_energy_balance_check_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
                                   const ctsm_patch_state *pch, const ctsm_canopy_state *can,
                                   const ctsm_soilflux_state *sf, int32_t num_filterp, const int32_t *filterp_dev,
                                   double *errseb, double *errlon, double *errseb_max, int32_t *seb_patch,
                                   double *errlon_max, int32_t *lon_patch);
+
+/* The solar part of EnergyBalanceCheck (BalanceCheckMod.F90:964-970, 1008-
+ * 1012) for non-urban patches:
+ *   errsol = fsa + fsr - (forc_solad_col(c,1) + forc_solad_col(c,2)
+ *                         + forc_solai(g,1) + forc_solai(g,2))
+ * with c = can->column(p), g = can->gridcell(p); 0 for inactive patches.
+ * fsa / fsr are (p) arrays (SolarAbsorbedType / SurfaceAlbedoType), forc_solad_col
+ * is (begc:endc, 1:2) and forc_solai (begg:endg, 1:2), all device arrays.
+ * Writes errsol per filter entry when not NULL; waits and returns the largest
+ * |errsol| and the lowest patch holding it (0 with no patches). */
+int ctsm_solar_balance_check_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_canopy_state *can,
+                                 int32_t num_filterp, const int32_t *filterp_dev, const double *fsa,
+                                 const double *fsr, const double *forc_solad_col, const double *forc_solai,
+                                 double *errsol, double *errsol_max, int32_t *sol_patch);
+
+/* Column water stores and fluxes of the water balance check.  The stores the
+ * path itself holds come from ctsm_col_state (snl, patchi / patchf,
+ * h2osoi_liq / h2osoi_ice [SNO], h2osfc) and ctsm_canopy_state (active, wtcol,
+ * liqcan, snocan); these are the rest.  All (c) arrays, device memory. */
+typedef struct ctsm_water_balance_state {
+    const double  *h2osno_no_layers;       /* waterstate%h2osno_no_layers_col (mm)             */
+    const double  *total_plant_stored_h2o; /* waterdiagnostic%total_plant_stored_h2o_col (mm)  */
+    const double  *wa;                     /* waterstate%wa_col: unconfined aquifer (mm)       */
+    const double  *excess_ice;             /* (c, SOI) waterstate%excess_ice_col; NULL: zero   */
+    const double  *dynbal_baseline_liq;    /* read when the baselines are subtracted           */
+    const double  *dynbal_baseline_ice;
+    const int32_t *hydrologically_active;  /* col%hydrologically_active                        */
+    const int32_t *col_active;             /* col%active                                       */
+    /* the step's fluxes (mm/s), BalanceCheckMod.F90:592-605 */
+    const double  *forc_rain, *forc_snow, *qflx_floodc, *qflx_sfc_irrig, *qflx_glcice_dyn_water_flux;
+    const double  *qflx_evap_tot, *qflx_surf, *qflx_qrgwl, *qflx_drain, *qflx_drain_perched;
+    const double  *qflx_ice_runoff, *qflx_snwcp_discarded_liq, *qflx_snwcp_discarded_ice;
+    const double  *begwb, *endwb;          /* column water mass at the step's start / end (mm)  */
+    double        *errh2o;                 /* out: errh2o_col (mm); NULL: not written           */
+} ctsm_water_balance_state;
+
+/* ComputeWaterMassNonLake (TotalWaterAndHeatMod.F90:92-141, 200-393) for soil
+ * and crop columns: per filter column, liquid = p2c(liqcan) +
+ * total_plant_stored_h2o + the snow layers' liquid + (wa -
+ * aquifer_water_baseline when hydrologically active) + h2osfc + the soil
+ * layers' liquid (1..nlevgrnd); ice = p2c(snocan) + h2osno_no_layers + the
+ * snow layers' ice + the soil layers' ice and excess ice; less the dynbal
+ * baselines when subtract_dynbal_baselines != 0; water_mass = liquid + ice,
+ * summed in the reference's order.  p2c is p2c_1d_filter (subgridAveMod.F90:
+ * 292-320: patches patchi..patchf, active ones weighted by wtcol).  The
+ * reference calls it for begwb (BalanceCheckMod.F90:426, baselines not
+ * subtracted) and endwb (HydrologyDrainageMod.F90:182).  Enqueued on the
+ * context's stream; water_mass is a device (c) array. */
+int ctsm_water_mass_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
+                        const ctsm_canopy_state *can, const ctsm_water_balance_state *wb, int32_t num_nolakec,
+                        const int32_t *filter_nolakec_dev, double aquifer_water_baseline,
+                        int32_t subtract_dynbal_baselines, double *water_mass);
+
+/* The column water balance check of BalanceCheck (BalanceCheckMod.F90:588-
+ * 630): per filter column
+ *   errh2o = endwb - begwb - (forc_rain + forc_snow + qflx_floodc + qflx_sfc_irrig
+ *            + qflx_glcice_dyn_water_flux - qflx_evap_tot - qflx_surf - qflx_qrgwl
+ *            - qflx_drain - qflx_drain_perched - qflx_ice_runoff
+ *            - qflx_snwcp_discarded_liq - qflx_snwcp_discarded_ice) * dtime
+ * (0 for inactive columns), written to wb->errh2o when not NULL; waits and
+ * returns the largest |errh2o| and the lowest column holding it.  The caller
+ * applies the reference's thresholds (h2o_warning_thresh, 1e-5 mm abort after
+ * the skip steps). */
+int ctsm_water_balance_check_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_water_balance_state *wb,
+                                 int32_t num_filterc, const int32_t *filterc_dev, double dtime,
+                                 double *errh2o_max, int32_t *errh2o_col);
 
 /* One land step with canopy: CanopyFluxes over the patch filter, then per
  * column SoilTemperature, SetSoilWaterFractions, the transpiration sink and

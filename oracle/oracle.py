@@ -376,6 +376,61 @@ def energy_balance(st, filter_p=None):
     return seb, lon, (ms.value, i_s.value), (ml.value, i_l.value)
 
 
+# ---- the column water mass and balance, the solar balance ------------------------
+
+def _bal_lib():
+    L = lib()
+    if not getattr(L, "_bal_ready", False):
+        P = C.POINTER
+        L.oracle_water_mass.argtypes = [P(abi.Config), P(abi.Layout), P(abi.ColState), P(abi.CanopyState),
+                                        P(abi.WaterBalanceState), C.c_int, abi.i32p, C.c_double, C.c_int, abi.f64p]
+        L.oracle_water_balance.argtypes = [P(abi.Layout), P(abi.WaterBalanceState), C.c_int, abi.i32p, C.c_double,
+                                           abi.f64p, P(C.c_int)]
+        L.oracle_solar_balance.argtypes = [P(abi.Layout), P(abi.CanopyState), C.c_int, abi.i32p, abi.f64p, abi.f64p,
+                                           abi.f64p, abi.f64p, abi.f64p, abi.f64p, P(C.c_int)]
+        L._bal_ready = True
+    return L
+
+
+def water_mass(st, wb, cfg=None, filter_c=None, baseline=5000.0, subtract=False):
+    """ComputeWaterMassNonLake (TotalWaterAndHeatMod.F90:92-141, 200-393): the
+    column water mass (mm) per column of st (0 outside the filter); wb is a
+    ctsm_amd.balance.WaterBalance."""
+    L = _bal_lib()
+    cs, _ = st.structs()
+    out = np.zeros(st.dims.ncol)
+    f, fp = _filt(filter_c if filter_c is not None else np.arange(1, st.dims.ncol + 1))
+    L.oracle_water_mass(C.byref(cfg or canopy_config()), C.byref(st.layout()), C.byref(cs),
+                        C.byref(st.canopy_struct()), C.byref(wb.struct()), len(f), fp, C.c_double(baseline),
+                        int(bool(subtract)), out.ctypes.data_as(abi.f64p))
+    return out
+
+
+def water_balance(st, wb, dtime=1800.0, filter_c=None):
+    """errh2o per column into wb["errh2o"] (when present) and (max |errh2o|,
+    column) as BalanceCheckMod.F90:588-620 finds them."""
+    L = _bal_lib()
+    f, fp = _filt(filter_c if filter_c is not None else np.arange(1, st.dims.ncol + 1))
+    m, idx = C.c_double(0.0), C.c_int(0)
+    L.oracle_water_balance(C.byref(st.layout()), C.byref(wb.struct()), len(f), fp, C.c_double(dtime), C.byref(m),
+                           C.byref(idx))
+    return m.value, idx.value
+
+
+def solar_balance(st, fsa, fsr, forc_solad_col, forc_solai, filter_p=None):
+    """errsol per filter entry and (max |errsol|, patch) (BalanceCheckMod.F90:
+    964-970, 1008-1012); forc_solad_col is (2, ncol), forc_solai (2, ngrid)."""
+    L = _bal_lib()
+    f, fp = _filt(filter_p if filter_p is not None else np.arange(1, st.dims.npatch + 1))
+    err = np.zeros(len(f))
+    m, idx = C.c_double(0.0), C.c_int(0)
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (fsa, fsr, forc_solad_col, forc_solai)]
+    L.oracle_solar_balance(C.byref(st.layout()), C.byref(st.canopy_struct()), len(f), fp,
+                           *[a.ctypes.data_as(abi.f64p) for a in arrs], err.ctypes.data_as(abi.f64p), C.byref(m),
+                           C.byref(idx))
+    return err, (m.value, idx.value)
+
+
 def _hy_lib():
     L = lib()
     if not getattr(L, "_hy_ready", False):

@@ -47,7 +47,8 @@ STRUCTS = (("ctsm_config", abi.Config), ("ctsm_layout", abi.Layout), ("ctsm_tuni
            ("ctsm_water_table_state", abi.WaterTableState), ("ctsm_hist_field", abi.HistField),
            ("ctsm_ckpt_field", abi.CkptField), ("ctsm_drainage_state", abi.DrainageState),
            ("ctsm_phs_state", abi.PhsState), ("ctsm_phs_params", abi.PhsParams),
-           ("ctsm_phs_stress_state", abi.PhsStressState), ("ctsm_phs_canopy_state", abi.PhsCanopyState))
+           ("ctsm_phs_stress_state", abi.PhsStressState), ("ctsm_phs_canopy_state", abi.PhsCanopyState),
+           ("ctsm_water_balance_state", abi.WaterBalanceState))
 
 
 def _probe(tmp_path):
@@ -101,9 +102,10 @@ program sizes
   type(ctsm_phs_params_t) :: n
   type(ctsm_phs_stress_state_t) :: o
   type(ctsm_phs_canopy_state_t) :: q
-  print '(16I8)', c_sizeof(a), c_sizeof(b), c_sizeof(c), c_sizeof(d), c_sizeof(e), c_sizeof(f), c_sizeof(g), &
+  type(ctsm_water_balance_state_t) :: r
+  print '(17I8)', c_sizeof(a), c_sizeof(b), c_sizeof(c), c_sizeof(d), c_sizeof(e), c_sizeof(f), c_sizeof(g), &
        c_sizeof(h), c_sizeof(i), c_sizeof(j), c_sizeof(k), c_sizeof(l), c_sizeof(m), c_sizeof(n), c_sizeof(o), &
-       c_sizeof(q)
+       c_sizeof(q), c_sizeof(r)
 end program sizes
 """)
     mod = os.path.join(ROOT, "ctsm_amd", "fortran", "ctsm_gpu_mod.F90")
@@ -114,7 +116,8 @@ end program sizes
                      C.sizeof(abi.CanopyParams), C.sizeof(abi.PftParams), C.sizeof(abi.CanopyState),
                      C.sizeof(abi.SoilFluxState), C.sizeof(abi.SurfaceWaterState), C.sizeof(abi.HydroParams),
                      C.sizeof(abi.WaterTableState), C.sizeof(abi.DrainageState), C.sizeof(abi.PhsState),
-                     C.sizeof(abi.PhsParams), C.sizeof(abi.PhsStressState), C.sizeof(abi.PhsCanopyState)]
+                     C.sizeof(abi.PhsParams), C.sizeof(abi.PhsStressState), C.sizeof(abi.PhsCanopyState),
+                     C.sizeof(abi.WaterBalanceState)]
 
 
 def test_trace_ranges_carry_the_reference_timer_names():

@@ -1,0 +1,247 @@
+"""Pins of the oracle's column water mass, column water balance and solar
+balance (ComputeWaterMassNonLake, TotalWaterAndHeatMod.F90:92-393;
+BalanceCheckMod.F90:588-620, 964-970, 1008-1012).
+
+The water mass is pinned store by store: water added to any store the
+reference counts raises the mass by exactly that amount, water added where
+the reference does not look (an inactive patch's canopy water, a snow layer
+below the column's top snow layer, the aquifer of a hydrologically inactive
+column, the dynbal baselines unless they are subtracted) leaves it alone,
+and the whole mass agrees with an exactly rounded sum over the stores.  The
+balance errors are pinned by closure (a column whose water changed by its net
+input has errh2o = 0; absorbed plus reflected equal to the incoming gives
+errsol = 0) and by the sign of each flux, and max / maxloc against numpy.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from ctsm_amd import balance as B
+from ctsm_amd import synthetic as S
+from oracle import oracle as O
+
+NCOL = 48
+
+
+@pytest.fixture(scope="module")
+def base():
+    st = S.canopy_workload("canopy_uneven", ncol=NCOL)
+    rng = np.random.default_rng(5)
+    st.can["snocan"][:] = rng.uniform(0.0, 0.3, st.dims.npatch) * (rng.random(st.dims.npatch) < 0.5)
+    st.can["active"][rng.random(st.dims.npatch) < 0.1] = 0
+    wb = B.synthetic(st, 6)
+    return st, wb
+
+
+def _cfg(st):
+    cfg = O.canopy_config()
+    assert cfg.nlevsno == st.dims.nlevsno and cfg.nlevgrnd == st.dims.nlevgrnd
+    return cfg
+
+
+def _mass(st, wb, **kw):
+    return O.water_mass(st, wb, _cfg(st), **kw)
+
+
+def _lo(st):
+    return 1 - st.dims.nlevsno
+
+
+def _stores(st, wb, c, subtract=False, baseline=B.AQUIFER_WATER_BASELINE):
+    """Every term the reference adds for column c (0-based), as a list."""
+    col, can = st.col, st.can
+    lo = _lo(st)
+    out = []
+    pi, pf = col["patchi"][c] - 1, col["patchf"][c] - 1
+    for p in range(pi, pf + 1):
+        if can["active"][p]:
+            w = st.pch["wtcol"][p]
+            out += [can["liqcan"][p] * w, can["snocan"][p] * w]
+    out += [wb["total_plant_stored_h2o"][c], wb["h2osno_no_layers"][c]]
+    for j in range(col["snl"][c] + 1, 1):
+        out += [col["h2osoi_liq"][j - lo, c], col["h2osoi_ice"][j - lo, c]]
+    if wb["hydrologically_active"][c]:
+        out += [wb["wa"][c], -baseline]
+    out.append(col["h2osfc"][c])
+    for j in range(1, st.dims.nlevgrnd + 1):
+        out += [col["h2osoi_liq"][j - lo, c], col["h2osoi_ice"][j - lo, c], wb["excess_ice"][j - 1, c]]
+    if subtract:
+        out += [-wb["dynbal_baseline_liq"][c], -wb["dynbal_baseline_ice"][c]]
+    return out
+
+
+@pytest.mark.parametrize("subtract", [False, True])
+def test_water_mass_is_the_sum_of_the_stores(base, subtract):
+    st, wb = base
+    m = _mass(st, wb, subtract=subtract)
+    for c in range(NCOL):
+        ref = math.fsum(_stores(st, wb, c, subtract))
+        assert abs(m[c] - ref) <= 1e-12 * max(1.0, abs(ref)), c
+    # and the stores are not trivially zero: snow layers, canopy, aquifer, excess ice all present
+    assert np.any(st.col["snl"] < 0) and np.any(st.can["liqcan"] > 0) and np.any(st.can["snocan"] > 0)
+    assert np.any(wb["hydrologically_active"] == 0) and np.any(wb["excess_ice"] > 0)
+
+
+def _bump_cases(st, wb):
+    """(description, mutate(st, wb, delta) -> column index, counted?) per store."""
+    col, can = st.col, st.can
+    lo = _lo(st)
+    snowy = int(np.flatnonzero((col["snl"] <= -2) & (col["snl"] > -st.dims.nlevsno))[0])
+    act_p = [p for p in range(st.dims.npatch) if can["active"][p] and st.pch["wtcol"][p] > 0]
+    inact_p = [p for p in range(st.dims.npatch) if not can["active"][p]]
+    dry = int(np.flatnonzero(wb["hydrologically_active"] == 0)[0])
+    wet = int(np.flatnonzero(wb["hydrologically_active"] == 1)[0])
+    cases = []
+
+    def patch_store(name, p, counted):
+        def f(st, wb, d):
+            w = st.pch["wtcol"][p]
+            st.can[name][p] += d / w if w > 0 else d
+            return st.pch["column"][p] - 1
+        return (f"{name} of patch {p}", f, counted)
+
+    cases += [patch_store("liqcan", act_p[3], True), patch_store("snocan", act_p[5], True),
+              patch_store("liqcan", inact_p[0], False), patch_store("snocan", inact_p[1], False)]
+
+    def col_store(name, c, counted, grp="wb"):
+        def f(st, wb, d):
+            (wb.a if grp == "wb" else st.col)[name][c] += d
+            return c
+        return (f"{name} of column {c}", f, counted)
+
+    cases += [col_store("total_plant_stored_h2o", 7, True), col_store("h2osno_no_layers", 9, True),
+              col_store("wa", wet, True), col_store("wa", dry, False), col_store("h2osfc", 11, True, "col"),
+              col_store("dynbal_baseline_liq", 12, False), col_store("dynbal_baseline_ice", 13, False)]
+
+    def layer(name, c, j, counted, grp="col", lo_=lo):
+        def f(st, wb, d):
+            (st.col if grp == "col" else wb.a)[name][j - lo_, c] += d
+            return c
+        return (f"{name} of column {c} layer {j}", f, counted)
+
+    snl = int(col["snl"][snowy])
+    cases += [layer("h2osoi_liq", snowy, snl + 1, True), layer("h2osoi_ice", snowy, 0, True),
+              layer("h2osoi_liq", snowy, snl, False), layer("h2osoi_ice", snowy, snl, False),
+              layer("h2osoi_liq", 3, 1, True), layer("h2osoi_ice", 4, st.dims.nlevgrnd, True),
+              layer("h2osoi_liq", 5, st.dims.nlevgrnd, True),
+              layer("excess_ice", 6, 1, True, "wb", 1), layer("excess_ice", 8, st.dims.nlevgrnd, True, "wb", 1)]
+    return cases
+
+
+def test_each_store_counts_once_and_only_where_the_reference_looks(base):
+    st0, wb0 = base
+    m0 = _mass(st0, wb0)
+    delta = 0.125
+    for desc, mutate, counted in _bump_cases(st0, wb0):
+        st, wb = st0.copy(), wb0.copy()
+        c = mutate(st, wb, delta)
+        m = _mass(st, wb)
+        others = np.delete(np.arange(NCOL), c)
+        np.testing.assert_array_equal(m[others], m0[others], err_msg=desc)
+        expect = delta if counted else 0.0
+        assert abs((m[c] - m0[c]) - expect) <= 1e-9 * max(1.0, abs(m0[c])), (desc, m[c] - m0[c])
+
+
+def test_dynbal_baselines_subtracted_when_asked(base):
+    st, wb0 = base
+    wb = wb0.copy()
+    m0 = _mass(st, wb, subtract=True)
+    wb["dynbal_baseline_liq"][2] += 1.0
+    wb["dynbal_baseline_ice"][3] += 2.0
+    m = _mass(st, wb, subtract=True)
+    assert abs((m[2] - m0[2]) + 1.0) < 1e-9 and abs((m[3] - m0[3]) + 2.0) < 1e-9
+
+
+def test_water_mass_filter_and_aquifer_baseline(base):
+    st, wb = base
+    full = _mass(st, wb)
+    filt = np.arange(NCOL, 0, -3, dtype=np.int32)
+    part = _mass(st, wb, filter_c=filt)
+    np.testing.assert_array_equal(part[filt - 1], full[filt - 1])
+    rest = np.setdiff1d(np.arange(NCOL), filt - 1)
+    assert np.all(part[rest] == 0.0)
+    shifted = _mass(st, wb, baseline=B.AQUIFER_WATER_BASELINE + 10.0)
+    act = wb["hydrologically_active"] == 1
+    np.testing.assert_allclose(full[act] - shifted[act], 10.0, rtol=0, atol=1e-9)
+    np.testing.assert_array_equal(full[~act], shifted[~act])
+
+
+def _net(wb, c):
+    gains = sum(wb[f][c] for f in B.GAINS)
+    losses = sum(wb[f][c] for f in B.FLUXES if f not in B.GAINS)
+    return gains - losses
+
+
+def test_errh2o_closes_and_each_flux_has_its_sign(base):
+    st, wb0 = base
+    wb = wb0.copy()
+    dt = 1800.0
+    for c in range(NCOL):
+        wb["endwb"][c] = wb["begwb"][c] + _net(wb, c) * dt
+    O.water_balance(st, wb, dt)
+    e0 = wb["errh2o"].copy()
+    act = wb["col_active"] == 1
+    assert np.all(np.abs(e0[act]) < 1e-9)
+    assert np.all(e0[~act] == 0.0) and np.any(~act)
+    for f in B.FLUXES:
+        w = wb.copy()
+        c = int(np.flatnonzero(act)[2])
+        w[f][c] += 1e-4
+        O.water_balance(st, w, dt)
+        change = w["errh2o"][c] - e0[c]
+        sign = -1.0 if f in B.GAINS else 1.0          # more input with the same end water: a loss unaccounted
+        assert abs(change - sign * 1e-4 * dt) < 1e-9, f
+    w = wb.copy()
+    w["endwb"][5] += 3.0
+    O.water_balance(st, w, dt)
+    assert abs(w["errh2o"][5] - e0[5] - (3.0 if act[5] else 0.0)) < 1e-9
+
+
+def test_errh2o_max_and_maxloc(base):
+    st, wb0 = base
+    wb = wb0.copy()
+    dt = 1800.0
+    m, c = O.water_balance(st, wb, dt)
+    e = wb["errh2o"]
+    assert m == np.max(np.abs(e)) and c == int(np.argmax(np.abs(e))) + 1
+    # ties: the lowest column
+    wb["endwb"][:] = wb["begwb"]
+    for f in B.FLUXES:
+        wb[f][:] = 0.0
+    wb["endwb"][[9, 4, 30]] += 2.0
+    m, c = O.water_balance(st, wb, dt)
+    assert m == 2.0 and c == (5 if wb["col_active"][4] else 10)
+    filt = np.array([31, 10], dtype=np.int32)
+    m, c = O.water_balance(st, wb, dt, filter_c=filt)
+    assert m == 2.0 and c == (10 if wb["col_active"][9] else 31)
+
+
+def test_errsol_closes_and_each_term_has_its_sign(base):
+    st, _ = base
+    rng = np.random.default_rng(9)
+    n, npch = st.dims.ncol, st.dims.npatch
+    ng = st.dims.ngrid or n
+    solad = rng.uniform(0.0, 400.0, (2, n))
+    solai = rng.uniform(0.0, 150.0, (2, ng))
+    col = st.pch["column"] - 1
+    g = st.can["gridcell"] - 1
+    incoming = solad[0, col] + solad[1, col] + solai[0, g] + solai[1, g]
+    fsr = incoming * rng.uniform(0.05, 0.3, npch)
+    fsa = incoming - fsr
+    err, (m, p) = O.solar_balance(st, fsa, fsr, solad, solai)
+    act = st.can["active"] == 1
+    assert np.all(np.abs(err[act]) < 1e-10) and np.all(err[~act] == 0.0)
+    q = int(np.flatnonzero(act)[7])
+    for arr, idx, sign in ((fsa, (q,), 1.0), (fsr, (q,), 1.0), (solad, (0, col[q]), -1.0), (solad, (1, col[q]), -1.0),
+                           (solai, (0, g[q]), -1.0), (solai, (1, g[q]), -1.0)):
+        a2 = arr.copy()
+        a2[idx] += 2.0
+        args = [a2 if arr is x else x for x in (fsa, fsr, solad, solai)]
+        e2, _ = O.solar_balance(st, *args)
+        assert abs(e2[q] - err[q] - sign * 2.0) < 1e-9
+    fsa2 = fsa.copy()
+    fsa2[[40, 12]] += 5.0
+    err, (m, p) = O.solar_balance(st, fsa2, fsr, solad, solai)
+    a = np.abs(err)
+    assert m == a.max() and p == int(np.argmax(a)) + 1

@@ -19,8 +19,9 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE = os.path.join(ROOT, "oracle")
-SOURCES = ["ctsm_oracle.c", "canopy_oracle.c", "soilfluxes_oracle.c", "hydrology_oracle.c", "phs_oracle.c"]
-PINS = ["tests/test_oracle_pins.py", "tests/test_oracle_physics.py", "tests/test_oracle_canopy.py",
+SOURCES = ["ctsm_oracle.c", "canopy_oracle.c", "soilfluxes_oracle.c", "hydrology_oracle.c", "phs_oracle.c",
+           "balance_oracle.c"]
+PINS = ["tests/test_oracle_balance.py", "tests/test_oracle_pins.py", "tests/test_oracle_physics.py", "tests/test_oracle_canopy.py",
         "tests/test_oracle_phs_canopy.py"]
 
 # (name, file, text in the oracle, planted text[, occurrences]); the text occurs
@@ -105,6 +106,24 @@ MUTANTS = [
     ("errseb with the ground heat flux's sign flipped", "soilfluxes_oracle.c",
      "- F->eflx_lh_tot[pp] - F->eflx_soil_grnd[pp] - Cn->dhsdt_canopy[pp];",
      "- F->eflx_lh_tot[pp] + F->eflx_soil_grnd[pp] - Cn->dhsdt_canopy[pp];"),
+    ("water mass: top snow layer (j = 0) left out", "balance_oracle.c",
+     "for (j = S->snl[cc] + 1; j <= 0; ++j) {", "for (j = S->snl[cc] + 1; j < 0; ++j) {"),
+    ("water mass: aquifer counted in hydrologically inactive columns", "balance_oracle.c",
+     "if (W->hydrologically_active[cc]) liquid", "if (1) liquid"),
+    ("water mass: excess ice left out", "balance_oracle.c",
+     "+ (W->excess_ice ? W->excess_ice[cj(L, c, j, 1)] : 0.0);", "+ 0.0;"),
+    ("water mass: canopy water of inactive patches counted", "balance_oracle.c",
+     "if (Cn->active[pp]) s = s + x[pp] * Cn->wtcol[pp];", "s = s + x[pp] * Cn->wtcol[pp];"),
+    ("water mass: canopy water not weighted by wtcol", "balance_oracle.c",
+     "if (Cn->active[pp]) s = s + x[pp] * Cn->wtcol[pp];", "if (Cn->active[pp]) s = s + x[pp];"),
+    ("water mass: the deepest ground layer left out", "balance_oracle.c",
+     "for (j = 1; j <= nlevgrnd; ++j) {", "for (j = 1; j < nlevgrnd; ++j) {"),
+    ("errh2o: perched drainage counted as a gain", "balance_oracle.c",
+     "- W->qflx_drain[cc] - W->qflx_drain_perched[cc]", "- W->qflx_drain[cc] + W->qflx_drain_perched[cc]"),
+    ("errh2o: begwb and endwb swapped", "balance_oracle.c",
+     "e = W->endwb[cc] - W->begwb[cc]", "e = W->begwb[cc] - W->endwb[cc]"),
+    ("errsol: the diffuse near-infrared band read as the visible one", "balance_oracle.c",
+     "+ forc_solai[gg + L->ldg]);", "+ forc_solai[gg]);"),
 ]
 
 CFLAGS = ["-O2", "-mfma", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-std=c99", "-w"]
