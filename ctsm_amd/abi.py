@@ -128,6 +128,8 @@ COL_FIELDS = [
     ("eflx_snomelt", "f64", "c"),
     ("eflx_h2osfc_to_snow", "f64", "c"),
     ("forc_lwrad", "f64", "c"),
+    ("zwt", "f64", "c"),
+    ("smpmin", "f64", "c"),
 ]
 
 PATCH_FIELDS = [
@@ -372,7 +374,7 @@ DRAINAGE_FIELDS = [(n, "f64", "soi" if n == "h2osoi_vol" else "c") for n in (
 
 # column water stores and fluxes of the water balance check (ctsm_water_balance_state)
 WATERBAL_FIELDS = [
-    ("h2osno_no_layers", "f64", "c"), ("total_plant_stored_h2o", "f64", "c"), ("wa", "f64", "c"),
+    ("total_plant_stored_h2o", "f64", "c"), ("wa", "f64", "c"),
     ("excess_ice", "f64", "soi"), ("dynbal_baseline_liq", "f64", "c"), ("dynbal_baseline_ice", "f64", "c"),
     ("hydrologically_active", "i32", "c"), ("col_active", "i32", "c")] + [(n, "f64", "c") for n in (
         "forc_rain", "forc_snow", "qflx_floodc", "qflx_sfc_irrig", "qflx_glcice_dyn_water_flux", "qflx_evap_tot",

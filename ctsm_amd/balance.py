@@ -88,16 +88,17 @@ class DeviceWaterBalance:
 
 def synthetic(st, seed, excess_ice=True):
     """Seeded stores and fluxes for the columns of a state `st` (ctsm_amd.state
-    host state): unlayered snow only where the column has no snow layers,
-    an aquifer near its baseline, a tenth of the columns hydrologically
-    inactive and a twentieth inactive, fluxes of a wet day (mm/s), begwb /
-    endwb of the size of a soil column's water."""
+    host state): an aquifer near its baseline, a tenth of the columns
+    hydrologically inactive and a twentieth inactive, fluxes of a wet day
+    (mm/s), begwb / endwb of the size of a soil column's water.  Also sets
+    the state's unlayered snow (st.col["h2osno_no_layers"]) on columns
+    without snow layers."""
     d = st.dims
     n = d.ncol
     rng = np.random.default_rng(seed)
     snl = st.col["snl"]
+    st.col["h2osno_no_layers"][:] = np.where(snl == 0, rng.uniform(0.0, 8.0, n) * (rng.random(n) < 0.4), 0.0)
     a = {
-        "h2osno_no_layers": np.where(snl == 0, rng.uniform(0.0, 8.0, n) * (rng.random(n) < 0.4), 0.0),
         "total_plant_stored_h2o": rng.uniform(0.0, 0.5, n),
         "wa": AQUIFER_WATER_BASELINE + rng.normal(0.0, 200.0, n),
         "excess_ice": rng.uniform(0.0, 30.0, (d.nlevgrnd, n)) * (rng.random((d.nlevgrnd, n)) < 0.2)

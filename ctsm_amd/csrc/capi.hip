@@ -115,7 +115,7 @@ const FieldMeta kColFields[] = {
     CF(xmf, K_C, 8, true), CF(xmf_h2osfc, K_C, 8, true), CF(c_h2osfc, K_C, 8, true), CF(imelt, K_SNO, 4, true),
     CF(emg, K_C, 8, false), CF(htvp, K_C, 8, false), CF(eflx_bot, K_C, 8, false), CF(eflx_fgr12, K_C, 8, true),
     CF(eflx_fgr, K_SOI, 8, true), CF(eflx_snomelt, K_C, 8, true), CF(eflx_h2osfc_to_snow, K_C, 8, true),
-    CF(forc_lwrad, K_C, 8, false),
+    CF(forc_lwrad, K_C, 8, false), CF(zwt, K_C, 8, false), CF(smpmin, K_C, 8, false),
 };
 #undef CF
 #define PF(name, kind, elem, wr) {offsetof(ctsm_patch_state, name), kind, elem, wr}
@@ -659,7 +659,8 @@ int ctsm_create(const ctsm_config *cfg, int device, ctsm_ctx **out)
     if (!cfg || !out) return CTSM_E_ARG;
     *out = nullptr;
     if (cfg->soilwater_movement_method != 1 || cfg->upper_boundary_condition != CTSM_BC_FLUX ||
-        (cfg->lower_boundary_condition != CTSM_BC_FLUX && cfg->lower_boundary_condition != CTSM_BC_ZERO_FLUX) ||
+        (cfg->lower_boundary_condition != CTSM_BC_FLUX && cfg->lower_boundary_condition != CTSM_BC_ZERO_FLUX &&
+         cfg->lower_boundary_condition != CTSM_BC_WATERTABLE) ||
         cfg->flux_calculation != 1 ||
         (cfg->snow_thermal_cond_method != CTSM_SNOWTHK_JORDAN1991 &&
          cfg->snow_thermal_cond_method != CTSM_SNOWTHK_STURM1997) ||
@@ -1342,8 +1343,8 @@ int ctsm_water_mass_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_st
 {
     if (!ctx || !col || !can || !wb || !water_mass || n < 0 || check_layout(lay) || lay->ldp <= 0 ||
         (!filter_dev && n > lay->ldc) || !col->snl || !col->patchi || !col->patchf || !col->h2osoi_liq ||
-        !col->h2osoi_ice || !col->h2osfc || !can->active || !can->wtcol || !can->liqcan || !can->snocan ||
-        !wb->h2osno_no_layers || !wb->total_plant_stored_h2o || !wb->wa || !wb->hydrologically_active ||
+        !col->h2osoi_ice || !col->h2osfc || !col->h2osno_no_layers || !can->active || !can->wtcol || !can->liqcan ||
+        !can->snocan || !wb->total_plant_stored_h2o || !wb->wa || !wb->hydrologically_active ||
         (subtract_dynbal_baselines && (!wb->dynbal_baseline_liq || !wb->dynbal_baseline_ice)))
         return CTSM_E_ARG;
     launch_water_mass(ctx->stream, *lay, ctx->cfg, *col, *can, *wb, n, filter_dev, aquifer_water_baseline,
@@ -1526,6 +1527,7 @@ int ctsm_water_table_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_s
     if (!ctx || !col || !wt || n < 0 || check_layout(lay) || (!filter_dev && n > lay->ldc) || !wt_complete(wt) ||
         !wt_col_complete(col))
         return CTSM_E_ARG;
+    if (ctx->cfg.lower_boundary_condition == CTSM_BC_WATERTABLE) return CTSM_E_CONFIG;
     launch_water_table(ctx->stream, *lay, ctx->cfg, *col, *wt, n, filter_dev, dtime, ctx->err);
     return finish_launch(ctx);
 }
@@ -1536,6 +1538,7 @@ int ctsm_water_table(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state
     if (!ctx || !col || !wt || n < 0 || check_layout(lay) || (!filter && n > lay->ldc) || !wt_complete(wt) ||
         !wt_col_complete(col))
         return CTSM_E_ARG;
+    if (ctx->cfg.lower_boundary_condition == CTSM_BC_WATERTABLE) return CTSM_E_CONFIG;
     std::lock_guard<std::mutex> g(ctx->mu);
     hipError_t he = hipSetDevice(ctx->device);
     if (he != hipSuccess) return hip_fail(ctx, he);
@@ -1584,6 +1587,7 @@ int ctsm_drainage_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_stat
     if (!ctx || !col || !dr || !prm || n < 0 || check_layout(lay) || (!filter_dev && n > lay->ldc) ||
         !dr_complete(dr) || !dr_col_complete(col))
         return CTSM_E_ARG;
+    if (ctx->cfg.lower_boundary_condition == CTSM_BC_WATERTABLE) return CTSM_E_CONFIG;
     launch_drainage(ctx->stream, *lay, ctx->cfg, *prm, *col, *dr, n, filter_dev, dtime, ctx->err);
     return finish_launch(ctx);
 }
@@ -1594,6 +1598,7 @@ int ctsm_drainage(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *c
     if (!ctx || !col || !dr || !prm || n < 0 || check_layout(lay) || (!filter && n > lay->ldc) ||
         !dr_complete(dr) || !dr_col_complete(col))
         return CTSM_E_ARG;
+    if (ctx->cfg.lower_boundary_condition == CTSM_BC_WATERTABLE) return CTSM_E_CONFIG;
     std::lock_guard<std::mutex> g(ctx->mu);
     hipError_t he = hipSetDevice(ctx->device);
     if (he != hipSuccess) return hip_fail(ctx, he);
@@ -1883,6 +1888,7 @@ static int land_step_full(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_
         return CTSM_E_ARG;
     if (sw && (!hprm || !sw_complete(sw) || sw->qflx_infl != col->qflx_infl || sw->h2osfc != col->h2osfc))
         return CTSM_E_ARG;
+    if ((wt || dr) && ctx->cfg.lower_boundary_condition == CTSM_BC_WATERTABLE) return CTSM_E_CONFIG;
     if (wt && (!wt_complete(wt) || (sw && (sw->zwt != wt->zwt || sw->zwt_perched != wt->zwt_perched ||
                                            sw->frost_table != wt->frost_table))))
         return CTSM_E_ARG;
@@ -2344,6 +2350,7 @@ int ctsm_serve_create(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_stat
         max_steps_per_tick < 1 || max_steps_per_tick > 64 || !(dtime > 0.0) || pft_first < 0 ||
         pft_first >= CTSM_MAX_PFT || ((sw || dr) && !hprm))
         return CTSM_E_ARG;
+    if ((wt || dr) && ctx->cfg.lower_boundary_condition == CTSM_BC_WATERTABLE) return CTSM_E_CONFIG;
     *out = nullptr;
     std::lock_guard<std::mutex> g(ctx->mu);
     hipError_t he = hipSetDevice(ctx->device);

@@ -14,6 +14,7 @@
 #include "common.h"
 #include "trace.h"
 #include "soiltemp_dev.h"
+#include "watertable_bc.h"
 
 namespace ctsm {
 
@@ -558,6 +559,9 @@ __global__ void k_soilwater_plain(View V, Params P, ctsm_col_state S, int n, con
     if (!col_in_range(V, c)) { report_error(err, c, CTSM_E_ARG); return; }
     const int nl = S.nbedrock[V.c1(c)];
     if (nl < 2 || nl > MAX_NLEVSOI) { report_error(err, c, CTSM_E_SOLVER); return; }
+    const bool wt_bottom = (P.lower_bc == CTSM_BC_WATERTABLE);
+    if (wt_bottom && (nl != V.nlevsoi || !S.zwt || !S.smpmin)) { report_error(err, c, CTSM_E_ARG); return; }
+    const int jwt = wt_bottom ? watertable_layer(V, S, c) : 0;
 #define A(f, j) w.at(i, (f) * MAX_NLEVSOI + (j) - 1)
 #define LIQ(j) S.h2osoi_liq[V.sno(c, j)]
 #define DZ(j) S.dz[V.sno(c, j)]
@@ -624,6 +628,12 @@ __global__ void k_soilwater_plain(View V, Params P, ctsm_col_state S, int n, con
         if (P.lower_bc == CTSM_BC_FLUX) {
             A(R_QOUT, nl) = A(R_HK, nl);
             A(R_DQODW1, nl) = A(R_DHKDW, nl) / WS(nl);
+        } else if (wt_bottom && jwt >= V.nlevsoi) {
+            double qo, dq;
+            watertable_bottom_flux(A(R_HK, nl), A(R_DHKDW, nl), A(R_SMP, nl), A(R_DSMPDW, nl), WS(nl),
+                                   S.zwt[V.c1(c)], S.z[V.sno(c, nl)], qo, dq);
+            A(R_QOUT, nl) = qo;
+            A(R_DQODW1, nl) = dq;
         } else {
             A(R_QOUT, nl) = 0.0;
             A(R_DQODW1, nl) = 0.0;
@@ -688,12 +698,19 @@ __global__ void k_soilwater_plain(View V, Params P, ctsm_col_state S, int n, con
             continue;
         }
         for (int j = 1; j <= nl; ++j) LIQ(j) = LIQ(j) + A(R_DWAT, j) * (M_TO_MM * DZ(j));
-        const double qcTemp = (P.lower_bc == CTSM_BC_FLUX) ? A(R_HK, nl) + A(R_DHKDW, nl) * A(R_DWAT, nl) : 0.0;
+        const double qcTemp = (P.lower_bc == CTSM_BC_FLUX) ? A(R_HK, nl) + A(R_DHKDW, nl) * A(R_DWAT, nl)
+                              : wt_bottom ? A(R_QOUT, nl) + A(R_DQODW1, nl) * A(R_DWAT, nl) : 0.0;
         qcharge = qcharge + qcTemp * (dtsub / dtime);
         dtdone = dtdone + dtsub;
         if (fabs(dtime - dtdone) < P.verySmall) break;
         if (errorMax < P.xTolerLower) dtsub = dtsub * 2.0;
         dtsub = fmin90(dtsub, dtime - dtdone);
+    }
+    if (wt_bottom && jwt < V.nlevsoi) {
+        // compute_qcharge on the last trial's vwc_liq, imped and smp
+        const double ka = watertable_ka(A(R_VWC, jwt + 1), WS(jwt + 1), A(R_IMPED, jwt + 1) * S.hksat[V.soi(c, jwt + 1)],
+                                        S.bsw[V.soi(c, jwt + 1)]);
+        qcharge = watertable_qcharge(V, S, c, jwt, ka, A(R_SMP, jwt > 1 ? jwt : 1), dtime);
     }
     S.num_substeps[V.c1(c)] = (double)nsub;
     S.qcharge[V.c1(c)] = qcharge;

@@ -20,6 +20,7 @@
 // columns' depth needs.
 #pragma once
 #include "common.h"
+#include "watertable_bc.h"
 
 namespace ctsm {
 
@@ -28,7 +29,9 @@ __device__ __forceinline__ double gshfl(double v, int src, int width)
     return __shfl(v, src, width);
 }
 
-template <int G, int K = (MAX_NLEVSOI + (G > 0 ? G : 1) - 1) / (G > 0 ? G : 1)>
+// WT: the water-table bottom boundary (watertable_bc.h), a separate
+// instantiation so the flux / zero-flux kernels keep their registers.
+template <int G, bool WT = false, int K = (MAX_NLEVSOI + (G > 0 ? G : 1) - 1) / (G > 0 ? G : 1)>
 __device__ __forceinline__ int richards_group(const View &V, const Params &P, const ctsm_col_state &S, int c,
                                               double dtime, int g, int gsz = G)
 {
@@ -70,6 +73,15 @@ __device__ __forceinline__ int richards_group(const View &V, const Params &P, co
     }
 
     const bool zero_flux_bottom = (P.lower_bc == CTSM_BC_ZERO_FLUX);
+    constexpr bool wt_bottom = WT;
+    int jwt = 0;
+    double zwt = 0.0, z_bot = 0.0, q_ka = 0.0, q_smp = 0.0;
+    if (wt_bottom) {
+        if (nl != V.nlevsoi || !S.zwt || !S.smpmin) return RICHARDS_BAD_COLUMN;
+        jwt = watertable_layer(V, S, c);
+        zwt = S.zwt[V.c1(c)];
+        z_bot = S.z[V.sno(c, nl)];
+    }
     const int bot_lane = (nl - 1) / K;
     int nsub = 0;
     double dtsub = dtime, dtdone = 0.0, qcharge = 0.0;
@@ -137,8 +149,20 @@ __device__ __forceinline__ int richards_group(const View &V, const Params &P, co
                         hk_bot = hk;
                         dhkdw_bot = dhkdw;
                         if (zero_flux_bottom) { qout = 0.0; dqodw1 = 0.0; }
+                        else if (wt_bottom) {
+                            if (jwt < V.nlevsoi) { qout = 0.0; dqodw1 = 0.0; }
+                            else watertable_bottom_flux(hk, dhkdw, smp, dsmpdw, ws[k], zwt, z_bot, qout, dqodw1);
+                            // the sub-step's drainage is qout + dqodw1 * dwat
+                            hk_bot = qout;
+                            dhkdw_bot = dqodw1;
+                        }
                         else { qout = hk; dqodw1 = dhkdw / ws[k]; }
                         dqodw2 = 0.0;
+                    }
+                    if (candidate && wt_bottom) {
+                        // what compute_qcharge reads of the last trial
+                        if (j == jwt) q_ka = watertable_ka(fmax90(liq[k], 1.0e-6) / mdz[k], ws[k], hkc[k], bsw[k]);
+                        if (j == (jwt > 1 ? jwt : 1) - 1) q_smp = smp;
                     }
                     if (candidate) {
                         S.smp_l[V.soi(c, j + 1)] = smp;
@@ -294,6 +318,11 @@ __device__ __forceinline__ int richards_group(const View &V, const Params &P, co
     }
     if (status) return status;
 
+    if (wt_bottom && jwt < V.nlevsoi) {
+        const double ka = gshfl(q_ka, jwt / K, GS);
+        const double smp_w = gshfl(q_smp, ((jwt > 1 ? jwt : 1) - 1) / K, GS);
+        qcharge = watertable_qcharge(V, S, c, jwt, ka, smp_w, dtime);
+    }
     if (g == 0) {
         S.num_substeps[V.c1(c)] = (double)nsub;
         S.qcharge[V.c1(c)] = qcharge;

@@ -363,7 +363,7 @@ __global__ void __launch_bounds__(256) k_water_mass(SView V, ctsm_col_state S, c
     double liquid = 0.0, ice = 0.0;
     liquid = liquid + lc + W.total_plant_stored_h2o[cc];
     ice = ice + sc;
-    ice = ice + W.h2osno_no_layers[cc];
+    ice = ice + S.h2osno_no_layers[cc];
     for (int j = snl + 1; j <= 0; ++j) {
         liquid = liquid + S.h2osoi_liq[V.sno(c, j)];
         ice = ice + S.h2osoi_ice[V.sno(c, j)];

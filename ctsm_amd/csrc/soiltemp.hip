@@ -754,6 +754,7 @@ k_soiltemp_fractions(View V, Params P, ctsm_col_state S, ctsm_patch_state Pt, in
 }
 
 // The fused soil step in one kernel: SoilTemperature -> SetSoilWaterFractions -> SoilWater.
+template <bool WT>
 __global__ void __launch_bounds__(64)
 k_soil_step(View V, Params P, ctsm_col_state S, ctsm_patch_state Pt, int n,
             const int32_t *__restrict__ filter, double dtime, TempWork wk, ErrWord *err)
@@ -764,8 +765,8 @@ k_soil_step(View V, Params P, ctsm_col_state S, ctsm_patch_state Pt, int n,
     if (!column_and_ground_flux<64>(V, S, Pt, n, filter, wk, err, i, c, gf)) return;
     int rc = soiltemp_column(V, P, S, Pt, c, dtime, wk, i, gf);
     if (rc == 0) rc = soilwater_fractions_column(V, S, c);
-    if (rc == 0) rc = soilwater_column(V, P, S, c, dtime);
-    if (rc) report_error(err, c, CTSM_E_SOLVER);
+    if (rc == 0) rc = richards_column<MAX_NLEVSOI, WT>(V, P, S, c, dtime);
+    if (rc) report_error(err, c, richards_error_code(rc));
 }
 
 size_t soiltemp_workspace_doubles(int n) { return (size_t)WS_DOUBLES_PER_COL * (size_t)n; }
@@ -789,7 +790,12 @@ void launch_soil_step(hipStream_t st, const View &V, const Params &P, const ctsm
     const GptlRange range("soiltemperature+hydro_without_drainage");
     const int bs = 64;
     TempWork wk{ws, ld};
-    hipLaunchKernelGGL(k_soil_step, dim3((n + bs - 1) / bs), dim3(bs), 0, st, V, P, S, Pt, n, filter, dtime, wk, err);
+    if (P.lower_bc == CTSM_BC_WATERTABLE)
+        hipLaunchKernelGGL(k_soil_step<true>, dim3((n + bs - 1) / bs), dim3(bs), 0, st, V, P, S, Pt, n, filter, dtime, wk,
+                           err);
+    else
+        hipLaunchKernelGGL(k_soil_step<false>, dim3((n + bs - 1) / bs), dim3(bs), 0, st, V, P, S, Pt, n, filter, dtime,
+                           wk, err);
 }
 
 void launch_soiltemp_fractions(hipStream_t st, const View &V, const Params &P, const ctsm_col_state &S,

@@ -26,6 +26,7 @@
 
 namespace ctsm {
 
+template <bool WT>
 __global__ void __launch_bounds__(64)
 k_soilwater(View V, Params P, ctsm_col_state S, int n, const int32_t *__restrict__ filter,
             double dtime, ErrWord *err)
@@ -34,8 +35,8 @@ k_soilwater(View V, Params P, ctsm_col_state S, int n, const int32_t *__restrict
     if (i >= n) return;
     const int c = filter ? filter[i] : V.begc + i;
     if (!col_in_range(V, c)) { report_error(err, c, CTSM_E_ARG); return; }
-    const int rc = richards_column<MAX_NLEVSOI>(V, P, S, c, dtime);
-    if (rc) report_error(err, c, CTSM_E_SOLVER);
+    const int rc = richards_column<MAX_NLEVSOI, WT>(V, P, S, c, dtime);
+    if (rc) report_error(err, c, richards_error_code(rc));
 }
 
 // SetSoilWaterFractions (SoilHydrologyMod.F90:240-252), excess_ice = 0.
@@ -65,7 +66,7 @@ k_soilwater_fractions(View V, ctsm_col_state S, int n, const int32_t *__restrict
 template <int G> struct RichardsWaves { static constexpr int value = 1; };
 template <> struct RichardsWaves<8> { static constexpr int value = 3; };
 
-template <int G>
+template <int G, bool WT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RichardsWaves<G>::value)))
 k_soilwater_g(View V, Params P, ctsm_col_state S, int n, const int32_t *__restrict__ filter, double dtime,
               ErrWord *err)
@@ -75,8 +76,8 @@ k_soilwater_g(View V, Params P, ctsm_col_state S, int n, const int32_t *__restri
     if (i >= n) return;
     const int c = filter ? filter[i] : V.begc + i;
     if (!col_in_range(V, c)) { if (g == 0) report_error(err, c, CTSM_E_ARG); return; }
-    const int rc = richards_group<G>(V, P, S, c, dtime, g);
-    if (rc && g == 0) report_error(err, c, CTSM_E_SOLVER);
+    const int rc = richards_group<G, WT>(V, P, S, c, dtime, g);
+    if (rc && g == 0) report_error(err, c, richards_error_code(rc));
 }
 
 // Group size chosen per wavefront (richards_lanes = 0).  A one-wave block
@@ -88,6 +89,7 @@ k_soilwater_g(View V, Params P, ctsm_col_state S, int n, const int32_t *__restri
 // results do not depend on the group size.
 constexpr int ADAPT_SPAN = 32;
 constexpr int ADAPT_K = 3;
+template <bool WT>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
 k_soilwater_adaptive(View V, Params P, ctsm_col_state S, int n, const int32_t *__restrict__ filter, double dtime,
                      ErrWord *err)
@@ -111,8 +113,8 @@ k_soilwater_adaptive(View V, Params P, ctsm_col_state S, int n, const int32_t *_
             if (!col_in_range(V, c)) {
                 if (g == 0) report_error(err, c, CTSM_E_ARG);
             } else {
-                const int rc = richards_group<0, ADAPT_K>(V, P, S, c, dtime, g, gsz);
-                if (rc && g == 0) report_error(err, c, CTSM_E_SOLVER);
+                const int rc = richards_group<0, WT, ADAPT_K>(V, P, S, c, dtime, g, gsz);
+                if (rc && g == 0) report_error(err, c, richards_error_code(rc));
             }
         }
     }
@@ -121,14 +123,26 @@ k_soilwater_adaptive(View V, Params P, ctsm_col_state S, int n, const int32_t *_
 }  // namespace ctsm
 
 namespace ctsm {
+template <bool WT>
+void launch_soilwater_k(hipStream_t st, const View &V, const Params &P, const ctsm_col_state &S, int n,
+                        const int32_t *filter, double dtime, ErrWord *err, int lanes, int block);
+
 void launch_soilwater(hipStream_t st, const View &V, const Params &P, const ctsm_col_state &S, int n,
                       const int32_t *filter, double dtime, ErrWord *err, int lanes, int block)
 {
     if (n <= 0) return;
     const GptlRange range("hydro_without_drainage");
+    if (P.lower_bc == CTSM_BC_WATERTABLE) launch_soilwater_k<true>(st, V, P, S, n, filter, dtime, err, lanes, block);
+    else launch_soilwater_k<false>(st, V, P, S, n, filter, dtime, err, lanes, block);
+}
+
+template <bool WT>
+void launch_soilwater_k(hipStream_t st, const View &V, const Params &P, const ctsm_col_state &S, int n,
+                        const int32_t *filter, double dtime, ErrWord *err, int lanes, int block)
+{
     if (lanes == 0) {
-        hipLaunchKernelGGL(k_soilwater_adaptive, dim3((unsigned)((n + ADAPT_SPAN - 1) / ADAPT_SPAN)), dim3(64), 0, st,
-                           V, P, S, n, filter, dtime, err);
+        hipLaunchKernelGGL(k_soilwater_adaptive<WT>, dim3((unsigned)((n + ADAPT_SPAN - 1) / ADAPT_SPAN)), dim3(64), 0,
+                           st, V, P, S, n, filter, dtime, err);
         return;
     }
     if (block <= 0 || block > 256) block = (lanes == 1) ? 64 : 128;
@@ -136,10 +150,10 @@ void launch_soilwater(hipStream_t st, const View &V, const Params &P, const ctsm
     const long threads = (long)n * (lanes > 0 ? lanes : 1);
     const dim3 grid((unsigned)((threads + block - 1) / block)), blk(block);
     switch (lanes) {
-    case 2: hipLaunchKernelGGL(k_soilwater_g<2>, grid, blk, 0, st, V, P, S, n, filter, dtime, err); break;
-    case 4: hipLaunchKernelGGL(k_soilwater_g<4>, grid, blk, 0, st, V, P, S, n, filter, dtime, err); break;
-    case 8: hipLaunchKernelGGL(k_soilwater_g<8>, grid, blk, 0, st, V, P, S, n, filter, dtime, err); break;
-    default: hipLaunchKernelGGL(k_soilwater, grid, blk, 0, st, V, P, S, n, filter, dtime, err); break;
+    case 2: hipLaunchKernelGGL((k_soilwater_g<2, WT>), grid, blk, 0, st, V, P, S, n, filter, dtime, err); break;
+    case 4: hipLaunchKernelGGL((k_soilwater_g<4, WT>), grid, blk, 0, st, V, P, S, n, filter, dtime, err); break;
+    case 8: hipLaunchKernelGGL((k_soilwater_g<8, WT>), grid, blk, 0, st, V, P, S, n, filter, dtime, err); break;
+    default: hipLaunchKernelGGL(k_soilwater<WT>, grid, blk, 0, st, V, P, S, n, filter, dtime, err); break;
     }
 }
 // ---- the Richards solve's column order ----------------------------------

@@ -2,6 +2,7 @@
 // the soil-water kernel and the fused soil-step kernel.  See soilwater.hip.
 #pragma once
 #include "common.h"
+#include "watertable_bc.h"
 
 namespace ctsm {
 
@@ -16,7 +17,9 @@ struct RichardsStatic {
     double sink;    // qflx_rootsoi_col(j)
 };
 
-template <int ML>
+// WT: the water-table bottom boundary (watertable_bc.h), a separate
+// instantiation so the flux / zero-flux kernels keep their registers.
+template <int ML, bool WT = false>
 __device__ __forceinline__ int richards_column(const View &V, const Params &P, const ctsm_col_state &S,
                                                int c, double dtime)
 {
@@ -54,6 +57,15 @@ __device__ __forceinline__ int richards_column(const View &V, const Params &P, c
     }
 
     const bool zero_flux_bottom = (P.lower_bc == CTSM_BC_ZERO_FLUX);
+    constexpr bool wt_bottom = WT;
+    int jwt = 0;
+    double zwt = 0.0, z_bot = 0.0, q_ka = 0.0, q_smp = 0.0;
+    if (wt_bottom) {
+        if (nl != V.nlevsoi || !S.zwt || !S.smpmin) return RICHARDS_BAD_COLUMN;
+        jwt = watertable_layer(V, S, c);
+        zwt = S.zwt[V.c1(c)];
+        z_bot = S.z[V.sno(c, nl)];
+    }
     int nsub = 0;
     double dtsub = dtime, dtdone = 0.0, qcharge = 0.0;
     int status = 0;
@@ -121,11 +133,22 @@ __device__ __forceinline__ int richards_column(const View &V, const Params &P, c
                     if (zero_flux_bottom) {
                         qout = 0.0;
                         dqodw1 = 0.0;
+                    } else if (wt_bottom) {
+                        if (jwt < V.nlevsoi) { qout = 0.0; dqodw1 = 0.0; }
+                        else watertable_bottom_flux(hk, dhkdw, smp, dsmpdw, ws[j], zwt, z_bot, qout, dqodw1);
+                        // the sub-step's drainage is qout + dqodw1 * dwat
+                        hk_bot = qout;
+                        dhkdw_bot = dqodw1;
                     } else {
                         qout = hk;
                         dqodw1 = dhkdw / ws[j];
                     }
                     dqodw2 = 0.0;
+                }
+                if (candidate && wt_bottom) {
+                    // what compute_qcharge reads of the last trial
+                    if (j == jwt) q_ka = watertable_ka(fmax90(liq[j], 1.0e-6) / mdz[j], ws[j], hkc[j], bsw[j]);
+                    if (j == (jwt > 1 ? jwt : 1) - 1) q_smp = smp;
                 }
                 if (candidate) {
                     S.qin[V.soi(c, j + 1)] = qin;
@@ -223,6 +246,7 @@ __device__ __forceinline__ int richards_column(const View &V, const Params &P, c
     }
     if (status) return status;
 
+    if (wt_bottom && jwt < V.nlevsoi) qcharge = watertable_qcharge(V, S, c, jwt, q_ka, q_smp, dtime);
     S.num_substeps[V.c1(c)] = (double)nsub;
     S.qcharge[V.c1(c)] = qcharge;
     // over-saturated layers push water upward (:1405-1409)
@@ -258,11 +282,6 @@ __device__ __forceinline__ int soilwater_fractions_column(const View &V, const c
         S.icefrac[V.soi(c, j)] = fmin90(1.0, vol_ice / ws);
     }
     return 0;
-}
-
-__device__ __forceinline__ int soilwater_column(const View &V, const Params &P, const ctsm_col_state &S, int c, double dtime)
-{
-    return richards_column<MAX_NLEVSOI>(V, P, S, c, dtime);
 }
 
 }  // namespace ctsm

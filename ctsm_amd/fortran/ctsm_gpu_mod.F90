@@ -64,6 +64,7 @@ module ctsm_gpu_mod
      type(c_ptr) :: emg = c_null_ptr, htvp = c_null_ptr, eflx_bot = c_null_ptr, eflx_fgr12 = c_null_ptr
      type(c_ptr) :: eflx_fgr = c_null_ptr, eflx_snomelt = c_null_ptr, eflx_h2osfc_to_snow = c_null_ptr
      type(c_ptr) :: forc_lwrad = c_null_ptr
+     type(c_ptr) :: zwt = c_null_ptr, smpmin = c_null_ptr
   end type ctsm_col_state_t
 
   type, bind(C), public :: ctsm_patch_state_t
@@ -196,7 +197,7 @@ module ctsm_gpu_mod
   ! column water stores and fluxes of the water balance check (member order and
   ! meaning as ctsm_water_balance_state); device arrays
   type, bind(C), public :: ctsm_water_balance_state_t
-     type(c_ptr) :: h2osno_no_layers = c_null_ptr, total_plant_stored_h2o = c_null_ptr, wa = c_null_ptr
+     type(c_ptr) :: total_plant_stored_h2o = c_null_ptr, wa = c_null_ptr
      type(c_ptr) :: excess_ice = c_null_ptr, dynbal_baseline_liq = c_null_ptr, dynbal_baseline_ice = c_null_ptr
      type(c_ptr) :: hydrologically_active = c_null_ptr, col_active = c_null_ptr
      type(c_ptr) :: forc_rain = c_null_ptr, forc_snow = c_null_ptr, qflx_floodc = c_null_ptr

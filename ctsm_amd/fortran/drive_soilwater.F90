@@ -79,6 +79,8 @@ program drive_soilwater
   real(c_double), allocatable, target :: eflx_snomelt(:)
   real(c_double), allocatable, target :: eflx_h2osfc_to_snow(:)
   real(c_double), allocatable, target :: forc_lwrad(:)
+  real(c_double), allocatable, target :: zwt(:)
+  real(c_double), allocatable, target :: smpmin(:)
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
@@ -143,6 +145,8 @@ program drive_soilwater
   allocate(eflx_snomelt(begc:endc))
   allocate(eflx_h2osfc_to_snow(begc:endc))
   allocate(forc_lwrad(begc:endc))
+  allocate(zwt(begc:endc))
+  allocate(smpmin(begc:endc))
   read(u) snl
   read(u) nbedrock
   read(u) patchi
@@ -200,6 +204,8 @@ program drive_soilwater
   read(u) eflx_snomelt
   read(u) eflx_h2osfc_to_snow
   read(u) forc_lwrad
+  read(u) zwt
+  read(u) smpmin
   close(u)
 
   col%snl = c_loc(snl(begc))
@@ -259,6 +265,8 @@ program drive_soilwater
   col%eflx_snomelt = c_loc(eflx_snomelt(begc))
   col%eflx_h2osfc_to_snow = c_loc(eflx_h2osfc_to_snow(begc))
   col%forc_lwrad = c_loc(forc_lwrad(begc))
+  col%zwt = c_loc(zwt(begc))
+  col%smpmin = c_loc(smpmin(begc))
 
   call ctsm_config_default(cfg)
   cfg%nlevsno = nlevsno
@@ -337,5 +345,7 @@ program drive_soilwater
   write(u) eflx_snomelt
   write(u) eflx_h2osfc_to_snow
   write(u) forc_lwrad
+  write(u) zwt
+  write(u) smpmin
   close(u)
 end program drive_soilwater

@@ -93,6 +93,8 @@ program drive_step
   real(c_double), allocatable, target :: c_eflx_snomelt(:)
   real(c_double), allocatable, target :: c_eflx_h2osfc_to_snow(:)
   real(c_double), allocatable, target :: c_forc_lwrad(:)
+  real(c_double), allocatable, target :: c_zwt(:)
+  real(c_double), allocatable, target :: c_smpmin(:)
   integer(c_int32_t), allocatable, target :: p_column(:)
   real(c_double), allocatable, target :: p_wtcol(:)
   integer(c_int32_t), allocatable, target :: p_frac_veg_nosno(:)
@@ -384,6 +386,8 @@ program drive_step
   allocate(c_eflx_snomelt(begc:endc))
   allocate(c_eflx_h2osfc_to_snow(begc:endc))
   allocate(c_forc_lwrad(begc:endc))
+  allocate(c_zwt(begc:endc))
+  allocate(c_smpmin(begc:endc))
   allocate(p_column(begp:endp))
   allocate(p_wtcol(begp:endp))
   allocate(p_frac_veg_nosno(begp:endp))
@@ -658,6 +662,8 @@ program drive_step
   read(u) c_eflx_snomelt
   read(u) c_eflx_h2osfc_to_snow
   read(u) c_forc_lwrad
+  read(u) c_zwt
+  read(u) c_smpmin
   read(u) p_column
   read(u) p_wtcol
   read(u) p_frac_veg_nosno
@@ -934,6 +940,8 @@ program drive_step
   col%eflx_snomelt = c_loc(c_eflx_snomelt(begc))
   col%eflx_h2osfc_to_snow = c_loc(c_eflx_h2osfc_to_snow(begc))
   col%forc_lwrad = c_loc(c_forc_lwrad(begc))
+  col%zwt = c_loc(c_zwt(begc))
+  col%smpmin = c_loc(c_smpmin(begc))
   pch%column = c_loc(p_column(begp))
   pch%wtcol = c_loc(p_wtcol(begp))
   pch%frac_veg_nosno = c_loc(p_frac_veg_nosno(begp))
@@ -1305,6 +1313,8 @@ program drive_step
   write(u) c_eflx_snomelt
   write(u) c_eflx_h2osfc_to_snow
   write(u) c_forc_lwrad
+  write(u) c_zwt
+  write(u) c_smpmin
   write(u) p_column
   write(u) p_wtcol
   write(u) p_frac_veg_nosno

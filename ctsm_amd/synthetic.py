@@ -119,7 +119,7 @@ def soil_params(sand, clay, om, zsoi, sf=None):
 
 
 # RNG stream ids (one per drawn quantity)
-S_SAND, S_CLAY, S_OM, S_LIQ, S_T, S_INFL, S_SINKTOT, S_SINKW, S_G, S_NSNO, S_SDZ, S_SRHO, S_ST, S_BED = range(14)
+S_SAND, S_CLAY, S_OM, S_LIQ, S_T, S_INFL, S_SINKTOT, S_SINKW, S_G, S_NSNO, S_SDZ, S_SRHO, S_ST, S_BED, S_ZWTC = range(15)
 
 
 def _gids(ncol, col0, gids):
@@ -251,6 +251,9 @@ def soil_only(ncol, seed, snow=False, bedrock=False, col0=0, nlevsno=12, nlevsoi
     c["htvp"][:] = HVAP
     c["eflx_bot"][:] = 0.0
     c["forc_lwrad"][:] = SB * top_t ** 4
+    # read by SoilWater only with the water-table bottom boundary
+    c["zwt"][:] = 0.2 + 7.8 * uniform(seed, S_ZWTC, gid)
+    c["smpmin"][:] = -1.e8                          # SoilStateType's smpmin (mm)
 
     # soil-water fractions consistent with the initial ice (SoilHydrologyMod.F90:240-252)
     vol_ice = np.minimum(c["watsat"][:nlevgrnd], ice[NS:] / (c["dz"][NS:] * 917.0))

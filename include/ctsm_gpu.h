@@ -93,7 +93,13 @@ typedef struct ctsm_config {
     int32_t nlevgrnd;           /* 25                                               */
     int32_t soilwater_movement_method; /* 1 = moisture form (only form supported) */
     int32_t upper_boundary_condition;  /* CTSM_BC_FLUX                            */
-    int32_t lower_boundary_condition;  /* CTSM_BC_ZERO_FLUX or CTSM_BC_FLUX       */
+    int32_t lower_boundary_condition;  /* CTSM_BC_ZERO_FLUX, CTSM_BC_FLUX or
+     * CTSM_BC_WATERTABLE (SoilWaterMovementMod.F90:1376-1377, 1767-1808 and
+     * compute_qcharge :2016-2135): every column must then have nbedrock =
+     * nlevsoi (the reference requires use_bedrock = .false., :181-183; a
+     * column that does not fails with CTSM_E_ARG) and col->zwt / col->smpmin;
+     * the water-table and drainage entries, which follow the path without the
+     * aquifer layer, return CTSM_E_CONFIG on such a context               */
     int32_t flux_calculation;   /* 1 = inexpensive, 42 = expensive                  */
     int32_t snow_thermal_cond_method;  /* CTSM_SNOWTHK_*                          */
     int32_t itmax_canopy_fluxes;/* 40                                               */
@@ -209,6 +215,10 @@ typedef struct ctsm_col_state {
     double        *eflx_h2osfc_to_snow; /* (c) out                                        */
     /* atm2lnd (downscaled to column) */
     const double  *forc_lwrad;    /* (c)      in                                          */
+    /* SoilHydrologyType / SoilStateType, read by SoilWater only with the
+     * water-table bottom boundary (lower_boundary_condition = CTSM_BC_WATERTABLE) */
+    const double  *zwt;           /* (c)      in   water table depth (m)                  */
+    const double  *smpmin;        /* (c)      in   minimum soil potential (mm)            */
 } ctsm_col_state;
 
 /* ---- patch state (PatchType / SolarAbsorbedType / EnergyFluxType) -------- */
@@ -908,10 +918,9 @@ int ctsm_solar_balance_check_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ct
 
 /* Column water stores and fluxes of the water balance check.  The stores the
  * path itself holds come from ctsm_col_state (snl, patchi / patchf,
- * h2osoi_liq / h2osoi_ice [SNO], h2osfc) and ctsm_canopy_state (active, wtcol,
+ * h2osoi_liq / h2osoi_ice [SNO], h2osfc, h2osno_no_layers) and ctsm_canopy_state (active, wtcol,
  * liqcan, snocan); these are the rest.  All (c) arrays, device memory. */
 typedef struct ctsm_water_balance_state {
-    const double  *h2osno_no_layers;       /* waterstate%h2osno_no_layers_col (mm)             */
     const double  *total_plant_stored_h2o; /* waterdiagnostic%total_plant_stored_h2o_col (mm)  */
     const double  *wa;                     /* waterstate%wa_col: unconfined aquifer (mm)       */
     const double  *excess_ice;             /* (c, SOI) waterstate%excess_ice_col; NULL: zero   */

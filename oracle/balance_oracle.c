@@ -65,7 +65,7 @@ void oracle_water_mass(const ctsm_config *cfg, const ctsm_layout *L, const ctsm_
         const double snocan_col = p2c_col(L, S, Cn, Cn->snocan, c);
         liquid = liquid + liqcan_col + W->total_plant_stored_h2o[cc];
         ice = ice + snocan_col;
-        ice = ice + W->h2osno_no_layers[cc];
+        ice = ice + S->h2osno_no_layers[cc];
         for (j = S->snl[cc] + 1; j <= 0; ++j) {
             liquid = liquid + S->h2osoi_liq[cj(L, c, j, lo)];
             ice = ice + S->h2osoi_ice[cj(L, c, j, lo)];

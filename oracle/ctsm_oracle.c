@@ -255,6 +255,17 @@ typedef struct {
     double cmx[NLEVMAX_SOI + 1], rmx[NLEVMAX_SOI + 1];
 } richards_sys;
 
+/* jwt of the water-table boundary (SoilWaterMovementMod.F90:1769-1776 and
+ * :2077-2084): the first layer j with zwt <= zi(j) gives j - 1, nlevsoi when
+ * the water table is below the soil column. */
+static int watertable_layer(const ctsm_config *cfg, lay_t Lo, const ctsm_col_state *S, int c)
+{
+    int j;
+    for (j = 1; j <= cfg->nlevsoi; ++j)
+        if (C1(S->zwt, c) <= ZI(S->zi, c, j)) return j - 1;
+    return cfg->nlevsoi;
+}
+
 static void richards_assemble(const ctsm_config *cfg, lay_t Lo, const ctsm_col_state *S, int c,
                               int nlayers, double dtsub, const double *liq, richards_sys *R)
 {
@@ -314,6 +325,21 @@ static void richards_assemble(const ctsm_config *cfg, lay_t Lo, const ctsm_col_s
     if (cfg->lower_boundary_condition == CTSM_BC_FLUX) {
         R->qout[j] = R->hk[j];
         R->dqodw1[j] = R->dhkdw[j] / SOI(S->watsat, c, j);
+    } else if (cfg->lower_boundary_condition == CTSM_BC_WATERTABLE) {
+        /* :1767-1808 with zdflag = 0; the columns have nlayers = nlevsoi */
+        const int jwt = watertable_layer(cfg, Lo, S, c);
+        if (j > jwt) {
+            R->qout[j] = 0.0;
+            R->dqodw1[j] = 0.0;
+            R->dqodw2[j] = 0.0;
+        } else {
+            const double dhkds1 = R->dhkdw[j] / SOI(S->watsat, c, j);
+            const double num = -R->smp[j];
+            const double den = M_TO_MM * (C1(S->zwt, c) - SNO(S->z, c, j));
+            R->qout[j] = -(R->hk[j] * num / den) + R->hk[j];
+            R->dqodw1[j] = (R->hk[j] * R->dsmpdw[j] - dhkds1 * num) / den + dhkds1;
+            R->dqodw2[j] = 0.0;
+        }
     } else { /* bc_zero_flux */
         R->qout[j] = 0.0;
         R->dqodw1[j] = 0.0;
@@ -349,6 +375,8 @@ static int soilwater_col(const ctsm_config *cfg, lay_t Lo, const ctsm_col_state 
     double dtsub = dtime, dtdone = 0.0, errorMax, qcTemp;
 
     if (nlayers < 2 || nlayers > cfg->nlevsoi || nlayers > NLEVMAX_SOI) return -1;
+    /* the water-table boundary requires use_bedrock = .false. (:181-183) */
+    if (cfg->lower_boundary_condition == CTSM_BC_WATERTABLE && nlayers != cfg->nlevsoi) return -3;
     C1(S->qcharge, c) = 0.0;
 
     for (;;) {
@@ -394,6 +422,8 @@ static int soilwater_col(const ctsm_config *cfg, lay_t Lo, const ctsm_col_state 
             SNO(S->h2osoi_liq, c, j) = SNO(S->h2osoi_liq, c, j) + dwat[j] * (M_TO_MM * SNO(S->dz, c, j));
         if (cfg->lower_boundary_condition == CTSM_BC_FLUX)
             qcTemp = R.hk[nlayers] + R.dhkdw[nlayers] * dwat[nlayers];
+        else if (cfg->lower_boundary_condition == CTSM_BC_WATERTABLE)
+            qcTemp = R.qout[nlayers] + R.dqodw1[nlayers] * dwat[nlayers];
         else
             qcTemp = 0.0;
         C1(S->qcharge, c) = C1(S->qcharge, c) + qcTemp * (dtsub / dtime);
@@ -403,6 +433,28 @@ static int soilwater_col(const ctsm_config *cfg, lay_t Lo, const ctsm_col_state 
         dtsub = fminf90(dtsub, dtime - dtdone);
     }
     C1(S->num_substeps, c) = (double)nsubstep;
+    if (cfg->lower_boundary_condition == CTSM_BC_WATERTABLE) {
+        /* compute_qcharge (:2016-2135) on the last trial's vwc_liq, smp, imped */
+        const int jwt = watertable_layer(cfg, Lo, S, c);
+        if (jwt < cfg->nlevsoi) {
+            const double zwt = C1(S->zwt, c);
+            const double wh_zwt = -SOI(S->sucsat, c, jwt + 1) - zwt * M_TO_MM;
+            const int jw = jwt > 1 ? jwt : 1;
+            double s1, ka, smp1, wh, q;
+            s1 = fmaxf90(R.vwc_liq[jwt + 1] / SOI(S->watsat, c, jwt + 1), 0.01);
+            s1 = fminf90(1.0, s1);
+            ka = R.imped[jwt + 1] * SOI(S->hksat, c, jwt + 1) * cm_pow(s1, 2.0 * SOI(S->bsw, c, jwt + 1) + 3.0);
+            smp1 = fmaxf90(C1(S->smpmin, c), R.smp[jw]);
+            wh = smp1 - SNO(S->z, c, jw) * M_TO_MM;
+            if (jwt == 0)  /* 1.e-3 is a default-real literal */
+                q = -(ka * (wh_zwt - wh) / ((zwt + (double)1.e-3f) * M_TO_MM));
+            else
+                q = -(ka * (wh_zwt - wh) / ((zwt - SNO(S->z, c, jwt)) * M_TO_MM * 2.0));
+            q = fmaxf90(-10.0 / dtime, q);
+            q = fminf90(10.0 / dtime, q);
+            C1(S->qcharge, c) = q;
+        }
+    }
     /* over-saturation push-up :1405-1409 */
     for (j = nlayers; j >= 2; --j) {
         double cap = SOI(S->eff_porosity, c, j) * M_TO_MM * SNO(S->dz, c, j);

@@ -1,0 +1,159 @@
+"""SoilWater with the water-table bottom boundary (lower_boundary_condition =
+CTSM_BC_WATERTABLE: SoilWaterMovementMod.F90:1376-1377, 1767-1808,
+compute_qcharge :2016-2135) on the GPU against the oracle, bit for bit, for
+every launch shape of the Richards solve (lanes per column 0 / 1 / 2 / 4 / 8,
+the plain kernel, the fused soil step in one and in two kernels), the
+host-array drop-in, a permuted filter; tables inside the column (jwt = 0 and
+deeper), just below it and far below; the column and configuration checks.
+
+Each test prints its case before the GPU work starts.
+"""
+import numpy as np
+import pytest
+
+from ctsm_amd import synthetic as S
+from ctsm_amd.state import DeviceState
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+BC_WATERTABLE = 3
+NS = 12
+FIELDS = ("h2osoi_liq", "qcharge", "smp_l", "hk_l", "qin", "qout", "num_substeps")
+
+
+@pytest.fixture(scope="module")
+def wctx():
+    from ctsm_amd.lib import Context, default_config
+    ctx = Context(default_config(lower_boundary_condition=BC_WATERTABLE))
+    yield ctx
+    ctx.close()
+
+
+def _ocfg():
+    cfg = O.config()
+    cfg.lower_boundary_condition = BC_WATERTABLE
+    return cfg
+
+
+def _workload(ncol, seed):
+    st = S.soil_only(ncol, seed, snow=True, bedrock=False)
+    rng = np.random.default_rng(seed)
+    c = np.arange(ncol)
+    zi = st.col["zi"][NS:NS + 21]                          # zi(0..20)
+    kind = rng.integers(0, 4, ncol)
+    jwt = rng.integers(1, 20, ncol)
+    inside = zi[jwt, c] + rng.uniform(0.05, 0.95, ncol) * (zi[jwt + 1, c] - zi[jwt, c])
+    top = zi[1, c] * rng.uniform(0.05, 0.95, ncol)
+    below = zi[20, c] + rng.uniform(0.01, 3.0, ncol)
+    far = zi[20, c] + rng.uniform(10.0, 80.0, ncol)
+    st.col["zwt"][:] = np.choose(kind, [inside, top, below, far])
+    return st
+
+
+def _compare(g, o):
+    for k in FIELDS:
+        np.testing.assert_array_equal(g.col[k], o.col[k], err_msg=k)
+    np.testing.assert_array_equal(g.col["h2osoi_ice"], o.col["h2osoi_ice"])
+
+
+def _oracle_soilwater(st):
+    o = st.copy()
+    rc, bad = O.soilwater(o, _ocfg())
+    assert rc == 0, bad
+    return o
+
+
+@pytest.mark.parametrize("lanes,plain", [(8, False), (4, False), (2, False), (1, False), (0, False), (1, True)],
+                         ids=["g8", "g4", "g2", "one-lane", "adaptive", "plain"])
+def test_soilwater_watertable_bitwise(wctx, lanes, plain):
+    ncol = 3000
+    st = _workload(ncol, 301 + lanes)
+    print(f"case: SoilWater water-table boundary, lanes={lanes} plain={plain}, {ncol} columns", flush=True)
+    before = wctx.get_tuning()
+    wctx.set_tuning(richards_lanes=lanes, richards_block=64)      # 64: the one-lane kernel's block
+    ds = DeviceState(st, wctx)
+    try:
+        wctx.soilwater_dev(ds, plain=plain)
+        wctx.sync()
+        g = ds.to_host()
+    finally:
+        ds.free()
+        wctx.set_tuning(**before)
+    o = _oracle_soilwater(st)
+    _compare(g, o)
+    # every regime present: recharge from inside, capillary rise and drainage below
+    assert np.any(o.col["qcharge"] < 0.0) and np.any(o.col["qcharge"] > 0.0)
+    assert np.any(o.col["qout"][19] < 0.0)
+
+
+def test_soilwater_watertable_dropin_and_filter(wctx):
+    ncol = 1500
+    st = _workload(ncol, 311)
+    filt = np.random.default_rng(312).permutation(np.arange(1, ncol + 1))[: ncol // 2].astype(np.int32)
+    print(f"case: SoilWater water-table boundary drop-in, {len(filt)} of {ncol} columns", flush=True)
+    g = st.copy()
+    wctx.soilwater(g, filter_=filt)
+    o = st.copy()
+    rc, bad = O.soilwater(o, _ocfg(), 1800.0, filt)
+    assert rc == 0, bad
+    _compare(g, o)
+    rest = np.setdiff1d(np.arange(ncol), filt - 1)
+    np.testing.assert_array_equal(g.col["h2osoi_liq"][:, rest], st.col["h2osoi_liq"][:, rest])
+
+
+@pytest.mark.parametrize("fused", [1, 0], ids=["one-kernel", "two-kernels"])
+def test_soil_step_watertable_bitwise(wctx, fused):
+    ncol = 2000
+    st = _workload(ncol, 321 + fused)
+    print(f"case: soil step with the water-table boundary, fused_single_kernel={fused}, {ncol} columns", flush=True)
+    before = wctx.get_tuning()
+    wctx.set_tuning(fused_single_kernel=fused, richards_lanes=8)
+    ds = DeviceState(st, wctx)
+    try:
+        for _ in range(3):
+            wctx.step_dev(ds)
+        wctx.sync()
+        g = ds.to_host()
+    finally:
+        ds.free()
+        wctx.set_tuning(**before)
+    o = st.copy()
+    for _ in range(3):
+        rc, bad = O.soil_step(o, _ocfg())
+        assert rc == 0, bad
+    _compare(g, o)
+    np.testing.assert_array_equal(g.col["t_soisno"], o.col["t_soisno"])
+
+
+def test_watertable_checks(wctx):
+    from ctsm_amd.lib import CTSM_E_ARG, CTSM_E_CONFIG, CtsmError
+    st = S.soil_only(64, 331, snow=False, bedrock=True)
+    bad = np.flatnonzero(st.col["nbedrock"] < 20)
+    assert bad.size
+    print("case: water-table boundary checks, 64 columns with bedrock", flush=True)
+    ds = DeviceState(st, wctx)
+    try:
+        with pytest.raises(CtsmError) as ei:                 # a column with bedrock above nlevsoi
+            wctx.soilwater_dev(ds)
+            wctx.sync()
+        assert ei.value.code == CTSM_E_ARG and ei.value.index == bad[0] + 1
+        with pytest.raises(CtsmError) as ei:
+            wctx.soilwater_dev(ds, plain=True)
+            wctx.sync()
+        assert ei.value.code == CTSM_E_ARG
+    finally:
+        ds.free()
+    # the water tables and the drainage of the path without the aquifer layer
+    hst = S.with_hydrology(S.soil_only(16, 332, bedrock=False), 333)
+    with pytest.raises(CtsmError) as ei:
+        wctx.water_table(hst.copy())
+    assert ei.value.code == CTSM_E_CONFIG
+    with pytest.raises(CtsmError) as ei:
+        wctx.drainage(hst.copy(), S.hydro_params())
+    assert ei.value.code == CTSM_E_CONFIG
+    # the context still runs the solve afterwards
+    st2 = _workload(128, 334)
+    g = st2.copy()
+    wctx.soilwater(g)
+    _compare(g, _oracle_soilwater(st2))

@@ -58,7 +58,7 @@ def _stores(st, wb, c, subtract=False, baseline=B.AQUIFER_WATER_BASELINE):
         if can["active"][p]:
             w = st.pch["wtcol"][p]
             out += [can["liqcan"][p] * w, can["snocan"][p] * w]
-    out += [wb["total_plant_stored_h2o"][c], wb["h2osno_no_layers"][c]]
+    out += [wb["total_plant_stored_h2o"][c], col["h2osno_no_layers"][c]]
     for j in range(col["snl"][c] + 1, 1):
         out += [col["h2osoi_liq"][j - lo, c], col["h2osoi_ice"][j - lo, c]]
     if wb["hydrologically_active"][c]:
@@ -110,7 +110,7 @@ def _bump_cases(st, wb):
             return c
         return (f"{name} of column {c}", f, counted)
 
-    cases += [col_store("total_plant_stored_h2o", 7, True), col_store("h2osno_no_layers", 9, True),
+    cases += [col_store("total_plant_stored_h2o", 7, True), col_store("h2osno_no_layers", 9, True, "col"),
               col_store("wa", wet, True), col_store("wa", dry, False), col_store("h2osfc", 11, True, "col"),
               col_store("dynbal_baseline_liq", 12, False), col_store("dynbal_baseline_ice", 13, False)]
 

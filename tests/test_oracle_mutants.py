@@ -21,7 +21,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE = os.path.join(ROOT, "oracle")
 SOURCES = ["ctsm_oracle.c", "canopy_oracle.c", "soilfluxes_oracle.c", "hydrology_oracle.c", "phs_oracle.c",
            "balance_oracle.c"]
-PINS = ["tests/test_oracle_balance.py", "tests/test_oracle_pins.py", "tests/test_oracle_physics.py", "tests/test_oracle_canopy.py",
+PINS = ["tests/test_oracle_balance.py", "tests/test_oracle_watertable.py", "tests/test_oracle_pins.py", "tests/test_oracle_physics.py", "tests/test_oracle_canopy.py",
         "tests/test_oracle_phs_canopy.py"]
 
 # (name, file, text in the oracle, planted text[, occurrences]); the text occurs
@@ -124,6 +124,27 @@ MUTANTS = [
      "e = W->endwb[cc] - W->begwb[cc]", "e = W->begwb[cc] - W->endwb[cc]"),
     ("errsol: the diffuse near-infrared band read as the visible one", "balance_oracle.c",
      "+ forc_solai[gg + L->ldg]);", "+ forc_solai[gg]);"),
+    ("water table: jwt one layer too deep (:1772)", "ctsm_oracle.c",
+     "if (C1(S->zwt, c) <= ZI(S->zi, c, j)) return j - 1;", "if (C1(S->zwt, c) <= ZI(S->zi, c, j)) return j;"),
+    ("water table: suction gradient toward the table with the wrong sign (:1801)", "ctsm_oracle.c",
+     "const double num = -R->smp[j];", "const double num = R->smp[j];"),
+    ("water table: distance to the table from the bottom interface, not the node (:1802)", "ctsm_oracle.c",
+     "M_TO_MM * (C1(S->zwt, c) - SNO(S->z, c, j));", "M_TO_MM * (C1(S->zwt, c) - ZI(S->zi, c, j));"),
+    ("water table: drainage from the free-drainage expression (:1376-1377)", "ctsm_oracle.c",
+     "qcTemp = R.qout[nlayers] + R.dqodw1[nlayers] * dwat[nlayers];",
+     "qcTemp = R.hk[nlayers] + R.dhkdw[nlayers] * dwat[nlayers];"),
+    ("compute_qcharge: 1.e-3 taken as a double (:2117)", "ctsm_oracle.c",
+     "((zwt + (double)1.e-3f) * M_TO_MM)", "((zwt + 1.e-3) * M_TO_MM)"),
+    ("compute_qcharge: without the half distance (:2121)", "ctsm_oracle.c",
+     "* M_TO_MM * 2.0));", "* M_TO_MM));"),
+    ("compute_qcharge: conductivity of layer jwt, not jwt+1 (:2096-2100)", "ctsm_oracle.c",
+     "s1 = fmaxf90(R.vwc_liq[jwt + 1] / SOI(S->watsat, c, jwt + 1), 0.01);",
+     "s1 = fmaxf90(R.vwc_liq[jw] / SOI(S->watsat, c, jw), 0.01);"),
+    ("compute_qcharge: suction of layer jwt+1, not max(1, jwt) (:2103)", "ctsm_oracle.c",
+     "smp1 = fmaxf90(C1(S->smpmin, c), R.smp[jw]);", "smp1 = fmaxf90(C1(S->smpmin, c), R.smp[jwt + 1]);"),
+    ("compute_qcharge: limit of 1 mm per step (:2125-2126)", "ctsm_oracle.c",
+     "q = fmaxf90(-10.0 / dtime, q);\n            q = fminf90(10.0 / dtime, q);",
+     "q = fmaxf90(-1.0 / dtime, q);\n            q = fminf90(1.0 / dtime, q);"),
 ]
 
 CFLAGS = ["-O2", "-mfma", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-std=c99", "-w"]
