@@ -55,7 +55,7 @@ PROFILED = {
     # (heat solve) per call
     ("k_soilwater_g", "canopy_uneven", 1048576): {
         "traffic": (3121768 * 2.0 + 2415621 * 1.0) * 1024,
-        "tuning": {"richards_lanes": 8, "richards_block": 128},
+        "tuning": {"richards_lanes": 8},          # measured in 128-lane blocks; the block size moves no bytes
     },
     ("k_soiltemp", "canopy_uneven", 1048576): {
         "traffic": (4832447 * 2.0 + 4272971 * 1.0) * 1024,
