@@ -947,7 +947,9 @@ typedef struct ctsm_water_balance_state {
  * 292-320: patches patchi..patchf, active ones weighted by wtcol).  The
  * reference calls it for begwb (BalanceCheckMod.F90:426, baselines not
  * subtracted) and endwb (HydrologyDrainageMod.F90:182).  Enqueued on the
- * context's stream; water_mass is a device (c) array. */
+ * context's stream; water_mass is a device (c) array.  A filter entry outside
+ * the layout, a patch range outside it or snl outside [-nlevsno, 0] is
+ * reported (CTSM_E_ARG with the column) by the next call that waits. */
 int ctsm_water_mass_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_state *col,
                         const ctsm_canopy_state *can, const ctsm_water_balance_state *wb, int32_t num_nolakec,
                         const int32_t *filter_nolakec_dev, double aquifer_water_baseline,
