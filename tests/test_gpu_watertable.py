@@ -2,7 +2,8 @@
 CTSM_BC_WATERTABLE: SoilWaterMovementMod.F90:1376-1377, 1767-1808,
 compute_qcharge :2016-2135) on the GPU against the oracle, bit for bit, for
 every launch shape of the Richards solve (lanes per column 0 / 1 / 2 / 4 / 8,
-the plain kernel, the fused soil step in one and in two kernels), the
+the plain kernel, the fused soil step in one and in two kernels, a captured
+graph of soil steps replayed), the
 host-array drop-in, a permuted filter; tables inside the column (jwt = 0 and
 deeper), just below it and far below; the column and configuration checks.
 
@@ -157,3 +158,28 @@ def test_watertable_checks(wctx):
     g = st2.copy()
     wctx.soilwater(g)
     _compare(g, _oracle_soilwater(st2))
+
+
+def test_soil_steps_graph_watertable_bitwise(wctx):
+    """Three soil steps captured as one graph and replayed twice, with the
+    program's own synchronisations between the replays: six steps, equal to
+    the oracle's six after each replay."""
+    ncol = 1024
+    st = _workload(ncol, 341)
+    print(f"case: soil-step graph with the water-table boundary, {ncol} columns, 2 x 3 steps", flush=True)
+    ds = DeviceState(st, wctx)
+    g = None
+    try:
+        g = wctx.graph_capture(ds, nsteps=3, canopy=False)
+        o = st.copy()
+        for rep in range(2):
+            wctx.graph_launch(g)
+            wctx.sync()
+            for _ in range(3):
+                rc, bad = O.soil_step(o, _ocfg())
+                assert rc == 0, bad
+            _compare(ds.to_host(), o)
+    finally:
+        if g is not None:
+            wctx.graph_destroy(g)
+        ds.free()
