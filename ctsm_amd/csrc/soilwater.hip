@@ -86,11 +86,15 @@ k_soilwater_g(View V, Params P, ctsm_col_state S, int n, const int32_t *__restri
 // 6 layers, 4 up to 12 and 8 up to 20, and the block runs its columns in
 // passes of 64 / lanes.  A batch stored by depth gives most blocks the
 // smallest group its columns allow, so no lane holds only empty layers; the
-// results do not depend on the group size.
+// results do not depend on the group size.  Built for three waves per SIMD
+// (168 VGPRs): at four (128 VGPRs, 120 spilled) the solve of 1,048,576
+// uneven columns took 8.3 ms, at three 7.4, at two 8.1.  The columns keep
+// their depth-binned order here (no sub-step order), which is why the 8-lane
+// kernel with that order (4.4 ms) is the default.
 constexpr int ADAPT_SPAN = 32;
 constexpr int ADAPT_K = 3;
 template <bool WT>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
 k_soilwater_adaptive(View V, Params P, ctsm_col_state S, int n, const int32_t *__restrict__ filter, double dtime,
                      ErrWord *err)
 {
