@@ -32,7 +32,8 @@ void launch_soiltemp_plain(hipStream_t, const View &, const Params &, const ctsm
                            int, const int32_t *, double, double *, long, ErrWord *);
 size_t soiltemp_workspace_doubles(int n);
 size_t richards_order_bytes(int n);
-const int32_t *launch_richards_order(hipStream_t, const View &, const ctsm_col_state &, int, const int32_t *, void *);
+const int32_t *launch_richards_order(hipStream_t, const View &, const ctsm_col_state &, int, const int32_t *, void *,
+                                     int);
 size_t soiltemp_plain_workspace_doubles(int n);
 size_t soilwater_plain_workspace_doubles(int n);
 void launch_soilwater_plain(hipStream_t, const View &, const Params &, const ctsm_col_state &, int, const int32_t *,
@@ -812,10 +813,10 @@ int ctsm_soilwater_dev(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_sta
     } else {
         const View V = make_view(ctx, *lay);
         const int32_t *rf = filter_dev;
-        if (ctx->tun.richards_lanes != 0) {      // columns sorted by their previous sub-step count
+        {                                        // columns sorted by their previous sub-step count
             int rc;
             if ((rc = ensure_work(ctx, (richards_order_bytes(n) + 7) / 8))) return rc;
-            rf = launch_richards_order(ctx->stream, V, *col, n, filter_dev, ctx->work.p);
+            rf = launch_richards_order(ctx->stream, V, *col, n, filter_dev, ctx->work.p, ctx->tun.richards_lanes);
         }
         launch_soilwater(ctx->stream, V, ctx->prm, *col, n, rf, dtime, ctx->err, ctx->tun.richards_lanes,
                          ctx->tun.richards_block);
@@ -924,7 +925,7 @@ static int soil_steps(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_stat
                 launch_soiltemp_fractions(st, V, ctx->prm, *col, *pch, cnt, pf + k * len, dtime, ws, len, ctx->err,
                                           ctx->tun.soiltemp_block);
                 const int32_t *rf = pf + k * len;
-                if (ctx->tun.richards_lanes != 0) rf = launch_richards_order(st, V, *col, cnt, rf, ws);
+                rf = launch_richards_order(st, V, *col, cnt, rf, ws, ctx->tun.richards_lanes);
                 launch_soilwater(st, V, ctx->prm, *col, cnt, rf, dtime, ctx->err, ctx->tun.richards_lanes,
                                  ctx->tun.richards_block);
             }
@@ -939,7 +940,7 @@ static int soil_steps(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_stat
             launch_soiltemp_fractions(ctx->stream, V, ctx->prm, *col, *pch, n, filter_dev, dtime,
                                       (double *)ctx->work.p, n, ctx->err, ctx->tun.soiltemp_block);
             const int32_t *rf = filter_dev;
-            if (ctx->tun.richards_lanes != 0) rf = launch_richards_order(ctx->stream, V, *col, n, rf, ctx->work.p);
+            rf = launch_richards_order(ctx->stream, V, *col, n, rf, ctx->work.p, ctx->tun.richards_lanes);
             launch_soilwater(ctx->stream, V, ctx->prm, *col, n, rf, dtime, ctx->err, ctx->tun.richards_lanes,
                              ctx->tun.richards_block);
         }
@@ -1939,7 +1940,7 @@ static int land_step_full(ctsm_ctx *ctx, const ctsm_layout *lay, const ctsm_col_
         // columns sorted by their previous sub-step count, so a wave's columns
         // sub-step alike (the adaptive group size keeps the filter's depth order)
         const int32_t *rf = filterc_dev;
-        if (ctx->tun.richards_lanes != 0) rf = launch_richards_order(ctx->stream, V, *col, nc, filterc_dev, ctx->work.p);
+        rf = launch_richards_order(ctx->stream, V, *col, nc, filterc_dev, ctx->work.p, ctx->tun.richards_lanes);
         launch_soilwater(ctx->stream, V, ctx->prm, *col, nc, rf, dtime, ctx->err, ctx->tun.richards_lanes,
                          ctx->tun.richards_block);
     }

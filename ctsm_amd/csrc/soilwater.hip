@@ -88,9 +88,10 @@ k_soilwater_g(View V, Params P, ctsm_col_state S, int n, const int32_t *__restri
 // smallest group its columns allow, so no lane holds only empty layers; the
 // results do not depend on the group size.  Built for three waves per SIMD
 // (168 VGPRs): at four (128 VGPRs, 120 spilled) the solve of 1,048,576
-// uneven columns took 8.3 ms, at three 7.4, at two 8.1.  The columns keep
-// their depth-binned order here (no sub-step order), which is why the 8-lane
-// kernel with that order (4.4 ms) is the default.
+// uneven columns took 8.3 ms, at three 7.4, at two 8.1; with the columns
+// ordered by sub-step count and depth class (SubstepDepthKey) 6.3-7.0 ms.
+// A deep block runs its 32 columns in four passes of 8, so the 8-lane kernel
+// (4.4 ms with its order) stays the default.
 constexpr int ADAPT_SPAN = 32;
 constexpr int ADAPT_K = 3;
 template <bool WT>
@@ -183,15 +184,35 @@ struct SubstepKey {
     }
 };
 
+// For the adaptive-group kernel: the sub-step count (up to 20) and, within
+// it, the group size the column's depth needs (2, 4 or 8 lanes), so the 32
+// consecutive columns a wave takes share both.
+struct SubstepDepthKey {
+    SubstepKey k;
+    __device__ int32_t value(int i) const { return k.value(i); }
+    __device__ int key(int i) const
+    {
+        const int c = value(i);
+        if (!col_in_range(k.V, c)) return 0;
+        const int ns = k.key(i) < 20 ? k.key(i) : 20;
+        const int nl = k.S.nbedrock[k.V.c1(c)];
+        return ns * 3 + (nl <= 2 * ADAPT_K ? 0 : nl <= 4 * ADAPT_K ? 1 : 2);
+    }
+};
+
 size_t richards_order_bytes(int n) { return (size_t)n * sizeof(int32_t) + bucket_order_table_bytes(n); }
 
 // The filter entries sorted by num_substeps into ws (richards_order_bytes(n));
-// returns the sorted column list (inside ws).
+// returns the sorted column list (inside ws).  lanes: the Richards kernel's
+// lanes per column (0: the adaptive-group kernel, which also sorts by depth).
 const int32_t *launch_richards_order(hipStream_t st, const View &V, const ctsm_col_state &S, int n,
-                                     const int32_t *filter, void *ws)
+                                     const int32_t *filter, void *ws, int lanes)
 {
     int32_t *order = (int32_t *)ws;
-    launch_bucket_order(st, SubstepKey{V, S, filter}, n, nullptr, order + n, order);
+    if (lanes == 0)
+        launch_bucket_order(st, SubstepDepthKey{SubstepKey{V, S, filter}}, n, nullptr, order + n, order);
+    else
+        launch_bucket_order(st, SubstepKey{V, S, filter}, n, nullptr, order + n, order);
     return order;
 }
 
