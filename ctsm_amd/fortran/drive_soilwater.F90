@@ -17,6 +17,7 @@ program drive_soilwater
   integer(c_int32_t) :: ncol, nlevsno, nlevsoi, nlevgrnd
   integer :: begc, endc, u, i, rc
   character(len=4096) :: fin, fout
+  character(len=16) :: bcarg
   type(ctsm_config_t) :: cfg
   type(ctsm_layout_t) :: lay
   type(ctsm_col_state_t) :: col
@@ -272,6 +273,11 @@ program drive_soilwater
   cfg%nlevsno = nlevsno
   cfg%nlevsoi = nlevsoi
   cfg%nlevgrnd = nlevgrnd
+  ! optional third argument: soilwater_movement's lower_boundary_condition
+  if (command_argument_count() >= 3) then
+     call get_command_argument(3, bcarg)
+     read(bcarg, *) cfg%lower_boundary_condition
+  end if
   rc = ctsm_create(cfg, 0, ctx)
   if (rc /= CTSM_OK) then
      print '(I0)', rc

@@ -15,12 +15,20 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name,ncol", [("cpu1024", 256), ("uneven", 200)])
-def test_soilwater_from_fortran_bitwise(tmp_path, name, ncol):
+@pytest.mark.parametrize("name,ncol,bc", [("cpu1024", 256, None), ("uneven", 200, None), ("watertable", 300, 3)])
+def test_soilwater_from_fortran_bitwise(tmp_path, name, ncol, bc):
+    """bc 3: the water-table bottom boundary (SoilWaterMovementMod.F90:1767-1808,
+    compute_qcharge), zwt and smpmin bound from Fortran."""
     exe = build.FORTRAN_DRIVER
     if not os.path.exists(exe):
         pytest.skip("Fortran driver not built (no flang)")
-    st = S.workload(name, ncol=ncol)
+    if bc is None:
+        st = S.workload(name, ncol=ncol)
+    else:
+        st = S.soil_only(ncol, 351, snow=True, bedrock=False)
+        zi = st.col["zi"][st.dims.nlevsno:st.dims.nlevsno + 21]
+        c = np.arange(ncol)
+        st.col["zwt"][:] = np.where(c % 2 == 0, 0.5 * (zi[5, c] + zi[6, c]), zi[20, c] + 1.0)
     print(f"case: SoilWater from Fortran, {name}, {ncol} columns", flush=True)
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     d = st.dims
@@ -28,7 +36,8 @@ def test_soilwater_from_fortran_bitwise(tmp_path, name, ncol):
         f.write(np.array([d.ncol, d.nlevsno, d.nlevsoi, d.nlevgrnd], dtype=np.int32).tobytes())
         for n, _, _ in abi.COL_FIELDS:
             f.write(np.ascontiguousarray(st.col[n]).tobytes())
-    r = subprocess.run([exe, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([exe, str(fin), str(fout)] + ([str(bc)] if bc else []), capture_output=True, text=True,
+                       timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert int(r.stdout.split()[0]) == 0, r.stdout
     raw = fout.read_bytes()
@@ -39,7 +48,10 @@ def test_soilwater_from_fortran_bitwise(tmp_path, name, ncol):
         pos += a.nbytes
     assert pos == len(raw)
     o = st.copy()
-    rc, bad = O.soilwater(o)
+    cfg = O.config()
+    if bc:
+        cfg.lower_boundary_condition = bc
+    rc, bad = O.soilwater(o, cfg)
     assert rc == 0, bad
     for n in o.col:
         np.testing.assert_array_equal(got[n], o.col[n], err_msg=n)
